@@ -1,0 +1,437 @@
+/*
+  C++ API of the MI355X implementation of TACS's assembly / block-CSR /
+  GMRES / adjoint path.  The class names, constructor argument orders, method
+  signatures, reference-counting rules (objects start at refcount 0; callers
+  incref/decref; arguments are borrowed) and int/stderr error convention
+  follow the reference headers cited on each class, so a C++ driver written
+  against the reference's subset below compiles against this header.
+
+  Everything lives on one GPU: vectors and matrices are device resident;
+  TACSBVec::getArray returns a host mirror that is synchronised on demand.
+  The communicator argument of the reference constructors is kept in place as
+  an opaque TACSComm (this build is serial; pass TACS_COMM_SELF).
+*/
+#ifndef TACS_HIP_CXX_H
+#define TACS_HIP_CXX_H
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "tacs_hip.h"
+
+typedef double TacsScalar;
+typedef struct tacs_comm_opaque* TACSComm;
+#define TACS_COMM_SELF ((TACSComm)0)
+
+enum MatrixOrientation { TACS_MAT_NORMAL, TACS_MAT_TRANSPOSE };
+enum ElementStrainType { TACS_LINEAR_STRAIN, TACS_NONLINEAR_STRAIN };
+enum OrthoType { CLASSICAL_GRAM_SCHMIDT, MODIFIED_GRAM_SCHMIDT };
+
+// src/TACSObject.h:112-140
+class TACSObject {
+ public:
+  TACSObject();
+  virtual ~TACSObject();
+  void incref();
+  void decref();
+  int refcount() const { return ref_count; }
+  virtual const char* getObjectName();
+
+ private:
+  int ref_count;
+};
+
+// src/constitutive/TACSMaterialProperties.h (isotropic constructor)
+class TACSMaterialProperties : public TACSObject {
+ public:
+  TACSMaterialProperties(TacsScalar rho, TacsScalar specific_heat, TacsScalar E,
+                         TacsScalar nu, TacsScalar ys, TacsScalar alpha = 0.0,
+                         TacsScalar kappa = 0.0);
+  TacsScalar getDensity() const { return rho; }
+  void evalTangentStiffness3D(TacsScalar C[]);
+  TacsScalar rho, specific_heat, E, nu, G, ys, alpha, kappa;
+};
+
+class TACSConstitutive : public TACSObject {};
+
+// src/constitutive/TACSSolidConstitutive.h
+class TACSSolidConstitutive : public TACSConstitutive {
+ public:
+  TACSSolidConstitutive(TACSMaterialProperties* props, TacsScalar t = 1.0,
+                        int tNum = -1, TacsScalar tlb = 0.0, TacsScalar tub = 1.0);
+  ~TACSSolidConstitutive();
+  TACSMaterialProperties* getMaterialProperties() { return props; }
+  TACSMaterialProperties* props;
+  TacsScalar t, tlb, tub;
+  int tNum;
+};
+
+// src/elements/basis/TACSHexaBasis.h
+class TACSElementBasis : public TACSObject {
+ public:
+  virtual int getOrder() = 0;  // nodes per direction (2: 8-node, 3: 27-node)
+  int getNumNodes() {
+    int o = getOrder();
+    return o * o * o;
+  }
+};
+class TACSLinearHexaBasis : public TACSElementBasis {
+ public:
+  int getOrder() { return 2; }
+};
+class TACSQuadraticHexaBasis : public TACSElementBasis {
+ public:
+  int getOrder() { return 3; }
+};
+
+class TACSElementModel : public TACSObject {
+ public:
+  virtual int getVarsPerNode() = 0;
+};
+
+// src/elements/TACSLinearElasticity.h:805 (TACSLinearElasticity3D)
+class TACSLinearElasticity3D : public TACSElementModel {
+ public:
+  TACSLinearElasticity3D(TACSSolidConstitutive* stiff,
+                         ElementStrainType strain_type = TACS_LINEAR_STRAIN);
+  ~TACSLinearElasticity3D();
+  int getVarsPerNode() { return 3; }
+  TACSSolidConstitutive* stiff;
+  ElementStrainType strain_type;
+};
+
+// src/elements/TACSElement.h:37
+class TACSElement : public TACSObject {
+ public:
+  virtual int getVarsPerNode() = 0;
+  virtual int getNumNodes() = 0;
+  int getNumVariables() { return getVarsPerNode() * getNumNodes(); }
+};
+
+// src/elements/TACSElement3D.h
+class TACSElement3D : public TACSElement {
+ public:
+  TACSElement3D(TACSElementModel* model, TACSElementBasis* basis);
+  ~TACSElement3D();
+  int getVarsPerNode() { return model->getVarsPerNode(); }
+  int getNumNodes() { return basis->getNumNodes(); }
+  TACSElementModel* model;
+  TACSElementBasis* basis;
+};
+
+// src/elements/TACSTraction3D.h:22 (constant traction, coordinate components)
+class TACSTraction3D : public TACSElement {
+ public:
+  TACSTraction3D(int varsPerNode, int faceIndex, TACSElementBasis* basis,
+                 const TacsScalar trac[], int tractionCoordinateComponent = 1);
+  ~TACSTraction3D();
+  int getVarsPerNode() { return varsPerNode; }
+  int getNumNodes() { return basis->getNumNodes(); }
+  int varsPerNode, faceIndex, coordComponent;
+  TACSElementBasis* basis;
+  TacsScalar trac[24];
+};
+
+// src/TACSAuxElements.h:52-101
+class TACSAuxElements : public TACSObject {
+ public:
+  TACSAuxElements(int size = 100);
+  ~TACSAuxElements();
+  void addElement(int num, TACSElement* elem);
+  int getNumAux() const { return n; }
+  void clear();
+  int n, cap, version;
+  int* nums;
+  TACSElement** elems;
+};
+
+// src/bpmat/KSM.h:392 (TACSBcMap)
+class TACSBcMap : public TACSObject {
+ public:
+  TACSBcMap(int bsize, int num_bcs);
+  ~TACSBcMap();
+  void addBC(int node, int nvals, const int* bc_vars = NULL,
+             const TacsScalar* bc_vals = NULL);
+  int getBCs(const int** nodes, const int** vars, TacsScalar** values);
+  int bsize, nbcs, max_size;
+  int *nodes, *vars;
+  TacsScalar* values;
+};
+
+// src/bpmat/KSM.h:43 (TACSVec) and src/bpmat/TACSBVec.h:67 (TACSBVec)
+class TACSVec : public TACSObject {
+ public:
+  virtual TacsScalar norm() = 0;
+  virtual void scale(TacsScalar alpha) = 0;
+  virtual TacsScalar dot(TACSVec* x) = 0;
+  virtual void axpy(TacsScalar alpha, TACSVec* x) = 0;
+  virtual void copyValues(TACSVec* x) = 0;
+  virtual void axpby(TacsScalar alpha, TacsScalar beta, TACSVec* x) = 0;
+  virtual void zeroEntries() = 0;
+  virtual void setRand(double lower = -1.0, double upper = 1.0) = 0;
+};
+
+class TACSBVec : public TACSVec {
+ public:
+  TACSBVec(int nnodes, int bsize);
+  ~TACSBVec();
+  void getSize(int* size);
+  int getBlockSize() { return bsize; }
+  int getArray(TacsScalar** array);  // host mirror; host becomes current
+  TacsScalar norm();
+  void scale(TacsScalar alpha);
+  TacsScalar dot(TACSVec* x);
+  void axpy(TacsScalar alpha, TACSVec* x);
+  void copyValues(TACSVec* x);
+  void axpby(TacsScalar alpha, TacsScalar beta, TACSVec* x);
+  void zeroEntries();
+  void set(TacsScalar val);
+  void setRand(double lower = -1.0, double upper = 1.0);
+  void applyBCs(TACSBcMap* map, TACSVec* vec = NULL, TacsScalar lambda = 1.0);
+  void setBCs(TACSBcMap* map, TacsScalar lambda = 1.0);
+  // Device access: devRead uploads a newer host copy; devWrite also marks
+  // the device copy as the current one.
+  const double* devRead();
+  double* devWrite();
+  size_t length() const { return n; }
+  int nnodes, bsize;
+  size_t n;
+  double* d_x;
+  double* h_x;
+  int state;  // 0 both current, 1 device newer, 2 host newer
+};
+
+// src/bpmat/KSM.h:91 (TACSMat), src/bpmat/TACSParallelMat.h:52 (one rank:
+// the [Aloc] block-CSR matrix of BCSRMat, src/bpmat/BCSRMat.h:34)
+class TACSMat : public TACSObject {
+ public:
+  virtual TACSVec* createVec() = 0;
+  virtual void mult(TACSVec* x, TACSVec* y) = 0;
+  virtual void zeroEntries() = 0;
+  virtual void applyBCs(TACSBcMap* bcmap) = 0;
+};
+
+class TACSParallelMat : public TACSMat {
+ public:
+  TACSParallelMat(int bsize, int nrows, const int* rowp, const int* cols);
+  ~TACSParallelMat();
+  TACSVec* createVec();
+  void mult(TACSVec* x, TACSVec* y);
+  void zeroEntries();
+  void applyBCs(TACSBcMap* bcmap);
+  int getSize(int* nr, int* nc);
+  // Host copies of the block-CSR arrays (BCSRMat::getArrays BCSRMat.h:82)
+  void getArrays(int* bsize, int* nrows, const int** rowp, const int** cols,
+                 TacsScalar** A);
+  // FP32 shadow of the values for the mixed-precision solver
+  float* getValues32(int refresh);
+  int bsize, nrows;
+  long nnzb;
+  int *d_rowp, *d_cols;
+  double* d_A;
+  float* d_A32;
+  int A32_current;
+  int* h_rowp;
+  int* h_cols;
+  double* h_A;
+};
+
+// src/bpmat/KSM.h:147 (TACSPc)
+class TACSPc : public TACSObject {
+ public:
+  virtual void factor() = 0;
+  virtual void applyFactor(TACSVec* x, TACSVec* y) = 0;
+};
+
+// Block-Jacobi: inverse diagonal blocks of the assembled matrix (the quantity
+// BCSRMat::factorDiag src/bpmat/BCSRMat.cpp:1527 computes), in FP64 and FP32.
+class TACSBlockJacobiPc : public TACSPc {
+ public:
+  TACSBlockJacobiPc(TACSParallelMat* mat);
+  ~TACSBlockJacobiPc();
+  void factor();
+  void applyFactor(TACSVec* x, TACSVec* y);
+  void applyFactor32(const float* x, float* y);
+  TACSParallelMat* mat;
+  int* d_diag;
+  double* d_Dinv;
+  float* d_Dinv32;
+};
+
+// src/bpmat/KSM.h:200 (TACSKsm), :266 (GMRES)
+class TACSKsm : public TACSObject {
+ public:
+  virtual int solve(TACSVec* b, TACSVec* x, int zero_guess = 1) = 0;
+  virtual void setTolerances(double rtol, double atol) = 0;
+  int getIterCount() { return iterCount; }
+  TacsScalar getResidualNorm() { return resNorm; }
+  int iterCount = 0;
+  TacsScalar resNorm = 0.0;
+};
+
+class GMRES : public TACSKsm {
+ public:
+  GMRES(TACSMat* mat, TACSPc* pc, int m, int nrestart, int isFlexible = 0);
+  ~GMRES();
+  int solve(TACSVec* b, TACSVec* x, int zero_guess = 1);
+  void setTolerances(double rtol, double atol);
+  void setOrthoType(enum OrthoType otype);
+  double rtol, atol;
+  int msub, nrestart, ortho;
+  TACSParallelMat* mat;
+  TACSBlockJacobiPc* pc;
+  double* d_W;      // (msub+1) basis vectors, contiguous
+  double* d_work;
+  double* d_scal;   // H, res, Qc, Qs, status scalars
+  void* d_red;
+  double* h_pin;    // pinned host scalars
+};
+
+// FP32 GMRES (FP32 matrix, block-Jacobi and Krylov basis; Gram-Schmidt dot
+// products and the Hessenberg in FP64) inside FP64 iterative refinement
+// (outer residual b - A x in FP64).  No counterpart in the reference.
+class TACSMixedGMRES : public TACSKsm {
+ public:
+  TACSMixedGMRES(TACSParallelMat* mat, TACSBlockJacobiPc* pc, int m, int nrestart,
+                 double inner_rtol = 1e-4, int max_outer = 20);
+  ~TACSMixedGMRES();
+  int solve(TACSVec* b, TACSVec* x, int zero_guess = 1);
+  void setTolerances(double rtol, double atol);
+  int innerSolve(const float* r, float* d);
+  double rtol, atol, inner_rtol;
+  int msub, nrestart, max_outer, outerCount;
+  TACSParallelMat* mat;
+  TACSBlockJacobiPc* pc;
+  float* d_W;
+  float* d_work32;
+  float* d_r32;
+  float* d_d32;
+  double* d_r;
+  double* d_scal;
+  void* d_red;
+  double* h_pin;
+};
+
+class TACSAssembler;
+
+// src/functions/TACSFunction.h:107
+class TACSFunction : public TACSObject {
+ public:
+  TACSFunction(TACSAssembler* assembler);
+  ~TACSFunction();
+  virtual TacsScalar getFunctionValue() = 0;
+  TACSAssembler* getAssembler() { return assembler; }
+  TACSAssembler* assembler;
+};
+
+// src/functions/TACSCompliance.h
+class TACSCompliance : public TACSFunction {
+ public:
+  TACSCompliance(TACSAssembler* assembler);
+  TacsScalar getFunctionValue() { return compliance; }
+  TacsScalar compliance = 0.0;
+};
+
+// src/functions/TACSKSFailure.h (KS_CONTINUOUS aggregation)
+class TACSKSFailure : public TACSFunction {
+ public:
+  TACSKSFailure(TACSAssembler* assembler, double ksWeight = 80.0,
+                double alpha = 1.0, double safetyFactor = 1.0);
+  TacsScalar getFunctionValue();
+  TacsScalar getMaximumFailure() { return maxFail; }
+  double ksWeight, alpha, safetyFactor;
+  TacsScalar maxFail = -1e20, ksFailSum = 0.0;
+};
+
+// src/TACSAssembler.h:61-325 (single rank, one GPU)
+class TACSAssembler : public TACSObject {
+ public:
+  TACSAssembler(TACSComm comm, int varsPerNode, int numOwnedNodes,
+                int numElements, int numDependentNodes = 0);
+  ~TACSAssembler();
+
+  int setElementConnectivity(const int* ptr, const int* conn);
+  int setElements(TACSElement** elements);
+  int addBCs(int nnodes, const int* nodes, int nbcs = -1, const int* vars = NULL,
+             const TacsScalar* vals = NULL);
+  int initialize();
+  void setAuxElements(TACSAuxElements* aux);
+  TACSAuxElements* getAuxElements() { return auxElements; }
+
+  int getVarsPerNode() { return varsPerNode; }
+  int getNumNodes() { return numNodes; }
+  int getNumOwnedNodes() { return numNodes; }
+  int getNumElements() { return numElements; }
+  int getNumDesignVars() { return numDesignVars; }
+  TACSBcMap* getBcMap() { return bcMap; }
+  void setNumThreads(int t) {}
+
+  TACSBVec* createVec();
+  TACSBVec* createNodeVec();
+  TACSBVec* createDesignVec();
+  TACSParallelMat* createMat();
+
+  void setNodes(TACSBVec* X);
+  void getNodes(TACSBVec* X);
+  void setVariables(TACSBVec* q, TACSBVec* qdot = NULL, TACSBVec* qddot = NULL);
+  void getVariables(TACSBVec* q, TACSBVec* qdot = NULL, TACSBVec* qddot = NULL);
+  void zeroVariables();
+  void setDesignVars(TACSBVec* dvs);
+  void getDesignVars(TACSBVec* dvs);
+  void applyBCs(TACSVec* vec);
+  void applyBCs(TACSMat* mat);
+
+  void assembleRes(TACSBVec* residual, const TacsScalar lambda = 1.0,
+                   const bool applyBCs = true);
+  void assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScalar gamma,
+                        TACSBVec* residual, TACSMat* A,
+                        MatrixOrientation matOr = TACS_MAT_NORMAL,
+                        const TacsScalar lambda = 1.0, const bool applyBCs = true);
+  void evalFunctions(int numFuncs, TACSFunction** funcs, TacsScalar* funcVals);
+  void addDVSens(TacsScalar coef, int numFuncs, TACSFunction** funcs, TACSBVec** dfdx);
+  void addSVSens(TacsScalar alpha, TacsScalar beta, TacsScalar gamma, int numFuncs,
+                 TACSFunction** funcs, TACSBVec** dfdu);
+  void addAdjointResProducts(TacsScalar scale, int numAdjoints, TACSBVec** adjoint,
+                             TACSBVec** dfdx, const TacsScalar lambda = 1.0);
+
+  // Device view of the element-level state used by the kernels
+  const double* devNodes() { return d_X; }
+  const double* devVars() { return d_u; }
+  int getElementOrder() { return order; }
+  double lastJacobianMs() { return jac_ms; }
+  int status();  // reads and clears the kernel status word
+
+  // internals
+  int elementVectorsToNodes(const double* ev, double* out, int accumulate);
+  int scatterElementScalarsToDVs(const double* es, double* dfdx);
+  int uploadAux();
+
+  int varsPerNode, numNodes, numElements, npe, order, initialized;
+  int numDesignVars;
+  int* h_ptr;
+  int* h_conn;
+  TACSElement** elements;
+  TACSBcMap* bcMap;
+  TACSAuxElements* auxElements;
+  int aux_version;
+  SolidMaterial mat;
+  // device data
+  int *d_conn, *d_n2e_ptr, *d_n2e, *d_bcvars, *d_bcnodes, *d_bcmask;
+  double* d_bcvals;
+  int nbc_nodes;
+  int *d_dvnum, *d_dv2e_ptr, *d_dv2e;
+  double *d_tfixed, *d_telem;
+  double *d_X, *d_u;
+  double *d_ev, *d_es;
+  int *d_aux_elem, *d_aux_face, *d_n2a_ptr, *d_n2a;
+  double *d_aux_trac, *d_aux_ev;
+  int naux;
+  int* d_status;
+  void* d_red;
+  double* d_scalar;
+  int *h_rowp, *h_cols;
+  long nnzb;
+  double jac_ms;
+};
+
+#endif

@@ -1,0 +1,294 @@
+/*
+  C ABI of the MI355X implementation of TACS's FE assembly / block-CSR /
+  GMRES / adjoint path (library: tacs_amd/lib/libtacs_hip.so).
+
+  Two layers, both plain C (pointers, sizes, status ints; no C++ or torch
+  types):
+
+  1. Kernel entry points (tacs_hip_*): stateless launchers over raw device
+     pointers on an explicit HIP stream.  Each replaces one CPU loop of the
+     reference, cited on its declaration.  They return 0 or a hipError_t /
+     TACS_HIP_ERR_* code for launch problems; in-kernel problems are reported
+     through the device status word passed to them.
+
+  2. Object entry points (tacs_*): opaque, reference-counted handles to the
+     C++ classes of TACSHip.h.  They mirror, one to one, the methods the
+     reference's Cython layer binds (tacs/cpp_headers/TACS.pxd:321-428,
+     tacs/TACS.pyx, tacs/elements.pyx, tacs/constitutive.pyx,
+     tacs/functions.pyx).  Handles start with refcount 0 like TACSObject
+     (src/TACSObject.h:112-140); tacs_incref/tacs_decref manage them.
+     Functions returning int return 0 on success and non-zero on failure, with
+     a message in tacs_last_error() (the reference prints to stderr and
+     returns int codes, e.g. TACSAssembler::initialize src/TACSAssembler.cpp:2519).
+*/
+#ifndef TACS_HIP_C_H
+#define TACS_HIP_C_H
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Packed solid material: the reference's C[21] tangent stiffness
+   (TACSMaterialProperties::evalTangentStiffness3D
+   src/constitutive/TACSMaterialProperties.cpp:270), density and yield
+   stress, plus the Lame constants used when the material is isotropic. */
+typedef struct SolidMaterial {
+  double C[21];
+  double rho;
+  double ys;
+  int isotropic;
+  double lam, mu;
+} SolidMaterial;
+
+/* ------------------------------------------------------------------------
+   1. Kernel entry points
+   ------------------------------------------------------------------------ */
+
+/* Jacobian of the solid element family written straight into block-CSR
+   (b = 3), Dirichlet rows applied when bcvars != NULL.
+   Replaces the element loop of TACSAssembler::assembleJacobian
+   (src/TACSAssembler.cpp:4352-4383) -> TACSElement3D::addJacobian
+   (src/elements/TACSElement3D.cpp:176) -> TACSAssembler::addMatValues
+   (src/TACSAssembler.h:545) -> BCSRMat::addRowValues
+   (src/bpmat/BCSRMat.cpp:1800) and TACSParallelMat::applyBCs
+   (src/bpmat/TACSParallelMat.cpp:343).
+   order: 2 (8-node) or 3 (27-node) hexahedra; n2e: node->element CSR with
+   entries (element << 5 | local node); A (FP64) and/or A32 (FP32) outputs. */
+int tacs_hip_solid_jacobian(int order, int nnodes, const int* n2e_ptr, const int* n2e,
+                            const int* conn, const double* X, const double* telem,
+                            const SolidMaterial* mat, double alpha, double gamma,
+                            const int* rowp, const int* cols, const int* bcvars,
+                            double* A, float* A32, int* status, hipStream_t stream);
+
+/* Element vectors ev[nelem * 3*npe]:
+   mode 0 residual (TACSElement3D::addResidual src/elements/TACSElement3D.cpp:139),
+   mode 1 compliance d/du (TACSCompliance::getElementSVSens
+          src/functions/TACSCompliance.cpp:95), coef = alpha,
+   mode 2 KS-failure d/du (TACSKSFailure::getElementSVSens
+          src/functions/TACSKSFailure.cpp:221). */
+int tacs_hip_solid_elem_vectors(int order, int mode, int nelem, const int* conn,
+                                const double* X, const double* u, const double* telem,
+                                const SolidMaterial* mat, double coef, double ks_weight,
+                                double ks_max, double ks_sum, double safety, double* ev,
+                                hipStream_t stream);
+
+/* Traction aux-element vectors (TACSTraction3D::addResidual
+   src/elements/TACSTraction3D.cpp:87, coordinate-component tractions). */
+int tacs_hip_solid_traction_vectors(int order, int naux, const int* aux_elem,
+                                    const int* aux_face, const double* aux_trac,
+                                    const int* conn, const double* X, double lambda,
+                                    double* ev, hipStream_t stream);
+
+/* Node gather of element vectors in ascending element order: the reverse of
+   TACSBVec::setValues ADD (src/bpmat/TACSBVec.cpp:757). nv = 24 or 81. */
+int tacs_hip_gather_elem_vectors(int nv, int nnodes, const int* n2e_ptr, const int* n2e,
+                                 const double* ev, int accumulate, double* out,
+                                 hipStream_t stream);
+
+/* Element scalars out[nelem]:
+   mode 0 compliance integrand (TACSCompliance::elementWiseEval :69),
+   mode 1 KS max, mode 2 KS sum (TACSKSFailure::elementWiseEval :125),
+   mode 3 dC/dt (TACSCompliance::addElementDVSens :162),
+   mode 4 psi^T dR/dt (TACSElement3D::addAdjResProduct
+          src/elements/TACSElement3D.cpp:227). */
+int tacs_hip_solid_elem_scalars(int order, int mode, int nelem, const int* conn,
+                                const double* X, const double* u, const double* psi,
+                                const double* telem, const SolidMaterial* mat, double scale,
+                                double ks_weight, double ks_max, double safety, double* out,
+                                hipStream_t stream);
+
+/* t_e = dv[tNum_e] or the element's fixed t (TACSSolidConstitutive::
+   setDesignVars src/constitutive/TACSSolidConstitutive.cpp:70 applied to
+   every element, as TACSAssembler::setDesignVars src/TACSAssembler.cpp:3242). */
+int tacs_hip_gather_design_vars(int nelem, const int* dvnum, const double* tfixed,
+                                const double* dv, double* telem, hipStream_t stream);
+/* dfdx[d] += sum of es[e] over the elements e of design variable d, in
+   ascending element order (dfdx->setValues ADD in TACSAssembler::addDVSens
+   src/TACSAssembler.cpp:4767 and addAdjointResProducts :5033). */
+int tacs_hip_scatter_dv_sens(int ndv, const int* dv2e_ptr, const int* dv2e, const double* es,
+                             double* dfdx, hipStream_t stream);
+/* Dirichlet values on a vector (TACSBVec::applyBCs src/bpmat/TACSBVec.cpp:546). */
+int tacs_hip_apply_bc_vec(int b, int nbc, const int* nodes, const int* masks, const double* vals,
+                          const double* u, double lambda, double* x, hipStream_t stream);
+
+/* y (+)= A x, b x b block-CSR, b = 3 or 6 (BCSRMat::mult
+   src/bpmat/BCSRMat.cpp:1226 -> BCSRMatVecMult3 src/bpmat/BCSRMatMult3.cpp:28,
+   BCSRMatVecMult6 src/bpmat/BCSRMatMult6.cpp:82). */
+int tacs_hip_bsr_spmv(int b, int nrows, const int* rowp, const int* cols, const double* A,
+                      const double* x, double* y, int add, hipStream_t stream);
+/* FP32 variant of tacs_hip_bsr_spmv (mixed-precision inner solve). */
+int tacs_hip_bsr_spmv_f32(int b, int nrows, const int* rowp, const int* cols, const float* A,
+                          const float* x, float* y, int add, hipStream_t stream);
+/* Y = A X for nrhs right-hand sides stored node-major
+   X[(3*node + c)*nrhs + j]; the matrix is read once for all of them
+   (the reference multiplies one vector at a time, BCSRMatVecMult3). */
+int tacs_hip_bsr3_spmm(int nrows, int nrhs, const int* rowp, const int* cols, const double* A,
+                       const double* X, double* Y, hipStream_t stream);
+/* Diagonal block positions; inverse diagonal blocks (BCSRMat::factorDiag
+   src/bpmat/BCSRMat.cpp:1527); y = D^{-1} x. */
+int tacs_hip_bsr_find_diag(int nrows, const int* rowp, const int* cols, int* diag,
+                           hipStream_t stream);
+int tacs_hip_bsr_diag_inverse(int b, int nrows, const int* diag, const double* A, double* Dinv,
+                              float* Dinv32, int* status, hipStream_t stream);
+int tacs_hip_bsr_diag_apply(int b, int nrows, const double* Dinv, const double* x, double* y,
+                            hipStream_t stream);
+int tacs_hip_bsr_diag_apply_f32(int b, int nrows, const float* Dinv, const float* x, float* y,
+                                hipStream_t stream);
+/* Dirichlet rows of an assembled matrix (BCSRMat::zeroRow
+   src/bpmat/BCSRMat.cpp:2027). */
+int tacs_hip_bsr_zero_rows(int b, int nbc, const int* nodes, const int* vars, const int* rowp,
+                           const int* cols, double* A, int ident, hipStream_t stream);
+int tacs_hip_convert_f64_f32(size_t n, const double* x, float* y, hipStream_t stream);
+
+/* Vector kernels of TACSBVec (src/bpmat/TACSBVec.cpp: norm :196, dot :237,
+   axpy :330, scale :228) and GMRES (src/bpmat/KSM.cpp:785; MGS :526,
+   CGS :515).  prec is 64 (double) or 32 (float); reductions are
+   deterministic and write their result to device memory; scratch is a
+   device buffer of tacs_hip_red_scratch_bytes() bytes, zeroed once. */
+size_t tacs_hip_red_scratch_bytes(void);
+int tacs_hip_dot(int prec, size_t n, const void* wprev, const double* hprev, void* q,
+                 const void* w, void* scratch, double* out, int take_sqrt, double* out_inv,
+                 hipStream_t stream);
+int tacs_hip_mdot(int prec, size_t n, int k, const void* W, size_t ldw, const void* q,
+                  void* scratch, double* h, hipStream_t stream);
+int tacs_hip_cgs_update(int prec, size_t n, int k, const void* W, size_t ldw, const double* c,
+                        const double* c2, void* q, hipStream_t stream);
+int tacs_hip_lincomb(int prec_w, int prec_y, size_t n, int k, const void* W, size_t ldw,
+                     const double* c, void* y, int add, hipStream_t stream);
+int tacs_hip_scale(int prec, size_t n, void* x, const double* sp, double sv, int invert,
+                   hipStream_t stream);
+int tacs_hip_axpby(int px, int py, size_t n, double a, const void* x, double b, void* y,
+                   hipStream_t stream);
+int tacs_hip_fill(int prec, size_t n, void* x, double v, hipStream_t stream);
+int tacs_hip_reduce(size_t n, const double* x, int op, void* scratch, double* out,
+                    hipStream_t stream);
+int tacs_hip_givens(int i, int m, double* H, double* Qc, double* Qs, double* res, double* st,
+                    hipStream_t stream);
+int tacs_hip_backsolve(int niters, int m, const double* H, double* res, hipStream_t stream);
+int tacs_hip_scalar_op(const double* src, double s, double* dst, hipStream_t stream);
+
+/* ------------------------------------------------------------------------
+   2. Object entry points (mirror tacs/cpp_headers/TACS.pxd and the Cython
+      wrappers; all handles are TACSObject*)
+   ------------------------------------------------------------------------ */
+typedef void* tacs_obj;
+
+const char* tacs_last_error(void);
+void tacs_incref(tacs_obj o);
+void tacs_decref(tacs_obj o);
+int tacs_refcount(tacs_obj o);
+/* Stream used by every object operation (default: the null stream). */
+void tacs_set_stream(hipStream_t stream);
+hipStream_t tacs_get_stream(void);
+int tacs_device_synchronize(void);
+
+/* constitutive.pyx: MaterialProperties :89, SolidConstitutive :875 */
+tacs_obj tacs_material_create(double rho, double specific_heat, double E, double nu,
+                              double ys, double alpha, double kappa);
+tacs_obj tacs_solid_constitutive_create(tacs_obj props, double t, int tNum, double tlb,
+                                        double tub);
+/* elements.pyx: LinearHexaBasis :297, QuadraticHexaBasis :308,
+   LinearElasticity3D :498, Element3D :564; TACSTraction3D
+   (src/elements/TACSTraction3D.h:22) */
+tacs_obj tacs_linear_hexa_basis_create(void);
+tacs_obj tacs_quadratic_hexa_basis_create(void);
+tacs_obj tacs_linear_elasticity3d_create(tacs_obj stiff, int strain_type);
+tacs_obj tacs_element3d_create(tacs_obj model, tacs_obj basis);
+tacs_obj tacs_traction3d_create(int vars_per_node, int face, tacs_obj basis,
+                                const double* trac, int coord_component);
+/* TACSAuxElements (src/TACSAuxElements.h:52) */
+tacs_obj tacs_aux_elements_create(int size);
+int tacs_aux_elements_add(tacs_obj aux, int elem_index, tacs_obj element);
+
+/* TACSAssembler (TACS.pxd:321-428; TACS.pyx Assembler :1595) */
+tacs_obj tacs_assembler_create(int vars_per_node, int num_owned_nodes, int num_elements,
+                               int num_dependent_nodes);
+int tacs_assembler_set_element_connectivity(tacs_obj a, const int* ptr, const int* conn);
+int tacs_assembler_set_elements(tacs_obj a, tacs_obj* elements);
+int tacs_assembler_add_bcs(tacs_obj a, int nnodes, const int* nodes, int nbcs, const int* vars,
+                           const double* vals);
+int tacs_assembler_initialize(tacs_obj a);
+int tacs_assembler_set_aux_elements(tacs_obj a, tacs_obj aux);
+int tacs_assembler_get_info(tacs_obj a, int* vars_per_node, int* num_nodes,
+                            int* num_elements, int* num_design_vars);
+tacs_obj tacs_assembler_create_vec(tacs_obj a);
+tacs_obj tacs_assembler_create_node_vec(tacs_obj a);
+tacs_obj tacs_assembler_create_design_vec(tacs_obj a);
+tacs_obj tacs_assembler_create_mat(tacs_obj a);
+int tacs_assembler_set_nodes(tacs_obj a, tacs_obj X);
+int tacs_assembler_get_nodes(tacs_obj a, tacs_obj X);
+int tacs_assembler_set_variables(tacs_obj a, tacs_obj q);
+int tacs_assembler_get_variables(tacs_obj a, tacs_obj q);
+int tacs_assembler_zero_variables(tacs_obj a);
+int tacs_assembler_set_design_vars(tacs_obj a, tacs_obj dvs);
+int tacs_assembler_get_design_vars(tacs_obj a, tacs_obj dvs);
+int tacs_assembler_apply_bcs_vec(tacs_obj a, tacs_obj vec);
+int tacs_assembler_apply_bcs_mat(tacs_obj a, tacs_obj mat);
+/* TACSAssembler::assembleRes src/TACSAssembler.cpp:4133 */
+int tacs_assembler_assemble_res(tacs_obj a, tacs_obj res, double lambda, int apply_bcs);
+/* TACSAssembler::assembleJacobian src/TACSAssembler.cpp:4291 */
+int tacs_assembler_assemble_jacobian(tacs_obj a, double alpha, double beta, double gamma,
+                                     tacs_obj res, tacs_obj mat, int mat_or, double lambda,
+                                     int apply_bcs);
+/* evalFunctions :4617, addDVSens :4767, addSVSens :4941,
+   addAdjointResProducts :5033 */
+int tacs_assembler_eval_functions(tacs_obj a, int n, tacs_obj* funcs, double* vals);
+int tacs_assembler_add_dv_sens(tacs_obj a, double coef, int n, tacs_obj* funcs,
+                               tacs_obj* dfdx);
+int tacs_assembler_add_sv_sens(tacs_obj a, double alpha, double beta, double gamma, int n,
+                               tacs_obj* funcs, tacs_obj* dfdu);
+int tacs_assembler_add_adjoint_res_products(tacs_obj a, double scale, int n, tacs_obj* adjoint,
+                                            tacs_obj* dfdx, double lambda);
+
+/* TACSBVec (TACS.pyx Vec :594; src/bpmat/TACSBVec.h:67) */
+int tacs_vec_get_size(tacs_obj v, int* size, int* bsize);
+double* tacs_vec_get_array(tacs_obj v);     /* host mirror, host becomes current */
+double* tacs_vec_device_array(tacs_obj v);  /* device array, device becomes current */
+double tacs_vec_norm(tacs_obj v);
+double tacs_vec_dot(tacs_obj v, tacs_obj x);
+int tacs_vec_axpy(tacs_obj v, double alpha, tacs_obj x);
+int tacs_vec_axpby(tacs_obj v, double alpha, double beta, tacs_obj x);
+int tacs_vec_scale(tacs_obj v, double alpha);
+int tacs_vec_copy_values(tacs_obj v, tacs_obj x);
+int tacs_vec_zero_entries(tacs_obj v);
+int tacs_vec_set(tacs_obj v, double val);
+int tacs_vec_set_rand(tacs_obj v, double lower, double upper);
+
+/* TACSParallelMat (TACS.pyx Mat :960) */
+int tacs_mat_mult(tacs_obj m, tacs_obj x, tacs_obj y);
+int tacs_mat_zero_entries(tacs_obj m);
+int tacs_mat_get_info(tacs_obj m, int* bsize, int* nrows, long* nnzb);
+/* Host copies of the block-CSR arrays (BCSRMat::getArrays BCSRMat.h:82). */
+int tacs_mat_get_arrays(tacs_obj m, const int** rowp, const int** cols, const double** A);
+/* Device arrays of the matrix (valid until the matrix is destroyed). */
+int tacs_mat_device_arrays(tacs_obj m, const int** rowp, const int** cols, double** A);
+
+/* Preconditioner / Krylov (TACS.pyx Pc :1106, KSM :1256) */
+tacs_obj tacs_block_jacobi_pc_create(tacs_obj mat);
+int tacs_pc_factor(tacs_obj pc);
+int tacs_pc_apply(tacs_obj pc, tacs_obj x, tacs_obj y);
+tacs_obj tacs_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int is_flexible);
+int tacs_gmres_set_ortho_type(tacs_obj ksm, int ortho);
+tacs_obj tacs_mixed_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart,
+                                 double inner_rtol, int max_outer);
+int tacs_ksm_set_tolerances(tacs_obj ksm, double rtol, double atol);
+int tacs_ksm_solve(tacs_obj ksm, tacs_obj b, tacs_obj x, int zero_guess);
+int tacs_ksm_get_iter_count(tacs_obj ksm);
+double tacs_ksm_get_res_norm(tacs_obj ksm);
+int tacs_mixed_gmres_get_outer_count(tacs_obj ksm);
+
+/* Functions (functions.pyx Compliance :172, KSFailure :358) */
+tacs_obj tacs_compliance_create(tacs_obj assembler);
+tacs_obj tacs_ksfailure_create(tacs_obj assembler, double ks_weight, double alpha,
+                               double safety_factor);
+double tacs_function_get_value(tacs_obj f);
+double tacs_ksfailure_get_max_failure(tacs_obj f);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

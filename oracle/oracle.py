@@ -1,0 +1,233 @@
+"""ctypes front end of the oracle library (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  It is the CPU restatement of the reference's
+algorithms that the HIP path is checked against; the product never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        L.orc_hex_quadrature.restype = ctypes.c_double
+        L.orc_hex_face_quadrature.restype = ctypes.c_double
+        L.orc_node_csr.restype = ctypes.c_long
+        L.orc_assemble_solid.restype = ctypes.c_long
+        L.orc_gmres.restype = ctypes.c_int
+        L.orc_bsr_diag_inv.restype = ctypes.c_int
+        L.orc_compliance.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a, dt=np.float64):
+    if a is None:
+        return None
+    assert a.dtype == dt and a.flags.c_contiguous, (a.dtype, dt)
+    return a.ctypes.data_as(_d if dt == np.float64 else _i)
+
+
+def _dd(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ii(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def iso_C21(E, nu):
+    C = np.zeros(21)
+    lib().orc_iso_C21(ctypes.c_double(E), ctypes.c_double(nu), _p(C))
+    return C
+
+
+def hex_quadrature(order, n):
+    pt = np.zeros(3)
+    w = lib().orc_hex_quadrature(order, n, _p(pt))
+    return w, pt
+
+
+def hex_basis(order, pt):
+    npe = order ** 3
+    N = np.zeros(npe)
+    Nxi = np.zeros(3 * npe)
+    lib().orc_hex_basis(order, _p(_dd(pt)), _p(N), _p(Nxi))
+    return N, Nxi.reshape(npe, 3)
+
+
+def solid_element(order, X, u, C21, t=1.0, rho=0.0, alpha=1.0, gamma=0.0,
+                  want_res=True, want_mat=True):
+    nv = 3 * order ** 3
+    res = np.zeros(nv) if want_res else None
+    mat = np.zeros((nv, nv)) if want_mat else None
+    lib().orc_solid_element(order, _p(_dd(X)), _p(_dd(u)), _p(_dd(C21)),
+                            ctypes.c_double(t), ctypes.c_double(rho),
+                            ctypes.c_double(alpha), ctypes.c_double(gamma),
+                            _p(res), _p(mat))
+    return res, mat
+
+
+def node_csr(nnodes, conn, npe):
+    conn = _ii(conn).reshape(-1)
+    nelem = conn.size // npe
+    rowp = np.zeros(nnodes + 1, np.int32)
+    nnz = lib().orc_node_csr(nnodes, nelem, npe, _p(conn, np.int32),
+                             _p(rowp, np.int32), None, ctypes.c_long(0))
+    cols = np.zeros(nnz, np.int32)
+    got = lib().orc_node_csr(nnodes, nelem, npe, _p(conn, np.int32),
+                             _p(rowp, np.int32), _p(cols, np.int32),
+                             ctypes.c_long(nnz))
+    assert got == nnz
+    return rowp, cols
+
+
+def assemble_solid(order, conn, X, u, t_elem, C21, rowp, cols, rho=0.0,
+                   alpha=1.0, gamma=0.0, want_mat=True, want_res=True):
+    npe = order ** 3
+    conn = _ii(conn).reshape(-1)
+    nelem = conn.size // npe
+    nnodes = rowp.size - 1
+    A = np.zeros(cols.size * 9) if want_mat else None
+    res = np.zeros(3 * nnodes) if want_res else None
+    missing = lib().orc_assemble_solid(
+        order, nelem, _p(conn, np.int32), _p(_dd(X)),
+        _p(_dd(u)) if u is not None else None, _p(_dd(t_elem)), _p(_dd(C21)),
+        ctypes.c_double(rho), ctypes.c_double(alpha), ctypes.c_double(gamma),
+        _p(_ii(rowp), np.int32), _p(_ii(cols), np.int32), _p(A), _p(res))
+    assert missing == 0
+    return A, res
+
+
+def add_tractions(order, conn, X, aux_elem, aux_face, aux_trac, res):
+    aux_elem = _ii(aux_elem)
+    lib().orc_add_tractions(order, aux_elem.size, _p(aux_elem, np.int32),
+                            _p(_ii(aux_face), np.int32), _p(_dd(aux_trac)),
+                            _p(_ii(conn).reshape(-1), np.int32), _p(_dd(X)),
+                            _p(res))
+    return res
+
+
+def apply_bc_mat(b, bc_nodes, bc_vars, rowp, cols, A):
+    bc_nodes = _ii(bc_nodes)
+    lib().orc_apply_bc_mat(b, bc_nodes.size, _p(bc_nodes, np.int32),
+                           _p(_ii(bc_vars), np.int32), _p(_ii(rowp), np.int32),
+                           _p(_ii(cols), np.int32), _p(A))
+
+
+def apply_bc_vec(b, bc_nodes, bc_vars, x, bc_vals=None, u=None, lam=1.0):
+    bc_nodes = _ii(bc_nodes)
+    lib().orc_apply_bc_vec(b, bc_nodes.size, _p(bc_nodes, np.int32),
+                           _p(_ii(bc_vars), np.int32),
+                           _p(_dd(bc_vals)) if bc_vals is not None else None,
+                           _p(_dd(u)) if u is not None else None,
+                           ctypes.c_double(lam), _p(x))
+
+
+def bsr_mult(b, rowp, cols, A, x):
+    n = rowp.size - 1
+    y = np.zeros(b * n)
+    lib().orc_bsr_mult(b, n, _p(_ii(rowp), np.int32), _p(_ii(cols), np.int32),
+                       _p(_dd(A)), _p(_dd(x)), _p(y))
+    return y
+
+
+def bsr_diag_inv(b, rowp, cols, A):
+    n = rowp.size - 1
+    D = np.zeros(n * b * b)
+    bad = lib().orc_bsr_diag_inv(b, n, _p(_ii(rowp), np.int32),
+                                 _p(_ii(cols), np.int32), _p(_dd(A)), _p(D))
+    assert bad == 0
+    return D
+
+
+def gmres(b, rowp, cols, A, rhs, Dinv=None, m=30, nrestart=10, rtol=1e-8,
+          atol=1e-30, x0=None):
+    n = rowp.size - 1
+    x = np.zeros(b * n) if x0 is None else _dd(x0).copy()
+    iters = ctypes.c_int(0)
+    rn = ctypes.c_double(0.0)
+    flag = lib().orc_gmres(b, n, _p(_ii(rowp), np.int32), _p(_ii(cols), np.int32),
+                           _p(_dd(A)), _p(_dd(Dinv)) if Dinv is not None else None,
+                           _p(_dd(rhs)), _p(x), m, nrestart,
+                           ctypes.c_double(rtol), ctypes.c_double(atol),
+                           1 if x0 is None else 0, ctypes.byref(iters),
+                           ctypes.byref(rn))
+    return x, flag, iters.value, rn.value
+
+
+def compliance(order, conn, X, u, t_elem, C21):
+    conn = _ii(conn).reshape(-1)
+    return lib().orc_compliance(order, conn.size // order ** 3, _p(conn, np.int32),
+                                _p(_dd(X)), _p(_dd(u)), _p(_dd(t_elem)),
+                                _p(_dd(C21)))
+
+
+def compliance_svsens(order, conn, X, u, t_elem, C21, alpha=1.0):
+    conn = _ii(conn).reshape(-1)
+    dfdu = np.zeros(np.asarray(u).size)
+    lib().orc_compliance_svsens(order, conn.size // order ** 3,
+                                _p(conn, np.int32), _p(_dd(X)), _p(_dd(u)),
+                                _p(_dd(t_elem)), _p(_dd(C21)),
+                                ctypes.c_double(alpha), _p(dfdu))
+    return dfdu
+
+
+def compliance_dvsens(order, conn, X, u, C21, scale=1.0):
+    conn = _ii(conn).reshape(-1)
+    nelem = conn.size // order ** 3
+    out = np.zeros(nelem)
+    lib().orc_compliance_dvsens(order, nelem, _p(conn, np.int32), _p(_dd(X)),
+                                _p(_dd(u)), _p(_dd(C21)), ctypes.c_double(scale),
+                                _p(out))
+    return out
+
+
+def adj_res_product(order, conn, X, u, psi, C21, scale=1.0):
+    conn = _ii(conn).reshape(-1)
+    nelem = conn.size // order ** 3
+    out = np.zeros(nelem)
+    lib().orc_adj_res_product(order, nelem, _p(conn, np.int32), _p(_dd(X)),
+                              _p(_dd(u)), _p(_dd(psi)), _p(_dd(C21)),
+                              ctypes.c_double(scale), _p(out))
+    return out
+
+
+def ks_failure(order, conn, X, u, C21, ys, ks_weight, alpha=1.0, safety=1.0):
+    conn = _ii(conn).reshape(-1)
+    out = np.zeros(3)
+    lib().orc_ks_failure(order, conn.size // order ** 3, _p(conn, np.int32),
+                         _p(_dd(X)), _p(_dd(u)), _p(_dd(C21)),
+                         ctypes.c_double(ys), ctypes.c_double(ks_weight),
+                         ctypes.c_double(alpha), ctypes.c_double(safety), _p(out))
+    return out
+
+
+def ks_svsens(order, conn, X, u, C21, ys, ks_weight, max_fail, ks_sum,
+              alpha=1.0, safety=1.0):
+    conn = _ii(conn).reshape(-1)
+    dfdu = np.zeros(np.asarray(u).size)
+    lib().orc_ks_svsens(order, conn.size // order ** 3, _p(conn, np.int32),
+                        _p(_dd(X)), _p(_dd(u)), _p(_dd(C21)), ctypes.c_double(ys),
+                        ctypes.c_double(ks_weight), ctypes.c_double(safety),
+                        ctypes.c_double(max_fail), ctypes.c_double(ks_sum),
+                        ctypes.c_double(alpha), _p(dfdu))
+    return dfdu

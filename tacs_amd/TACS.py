@@ -1,0 +1,315 @@
+"""Mirror of tacs.TACS for the assembly / solve / adjoint path (tacs/TACS.pyx:
+Vec :594, Mat :960, Pc :1106, KSM :1256, Assembler :1595, AuxElements :912).
+
+Same class and method names, argument orders and defaults as the reference's
+Cython wrappers; the objects live on one GPU.  Differences, by design:
+``Pc(mat)`` builds the block-Jacobi preconditioner (the reference builds a
+Schur-complement direct factorisation), and ``Assembler.create`` ignores its
+communicator argument (single rank).
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import lib, check
+from ._obj import TACSHandle
+
+dtype = np.float64
+NORMAL = 0
+TRANSPOSE = 1
+CLASSICAL_GRAM_SCHMIDT = 0
+MODIFIED_GRAM_SCHMIDT = 1
+
+
+def _ptr_array(objs):
+    arr = (ctypes.c_void_p * len(objs))(*[o.ptr for o in objs])
+    return arr
+
+
+class Vec(TACSHandle):
+    def getSize(self):
+        n, b = ctypes.c_int(), ctypes.c_int()
+        lib().tacs_vec_get_size(self.ptr, ctypes.byref(n), ctypes.byref(b))
+        return n.value
+
+    def getBlockSize(self):
+        n, b = ctypes.c_int(), ctypes.c_int()
+        lib().tacs_vec_get_size(self.ptr, ctypes.byref(n), ctypes.byref(b))
+        return b.value
+
+    def getArray(self):
+        """Host view of the vector; modifications are uploaded before the next
+        device operation (TACSBVec::getArray semantics)."""
+        n = self.getSize()
+        p = lib().tacs_vec_get_array(self.ptr)
+        if n == 0:
+            return np.zeros(0)
+        return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_double)), shape=(n,))
+
+    def getDevicePointer(self):
+        """Raw device address of the values (device copy becomes current)."""
+        return lib().tacs_vec_device_array(self.ptr)
+
+    def norm(self):
+        return lib().tacs_vec_norm(self.ptr)
+
+    def dot(self, vec):
+        return lib().tacs_vec_dot(self.ptr, vec.ptr)
+
+    def copyValues(self, vec):
+        check(lib().tacs_vec_copy_values(self.ptr, vec.ptr), "Vec.copyValues")
+
+    def scale(self, alpha):
+        lib().tacs_vec_scale(self.ptr, alpha)
+
+    def axpy(self, alpha, vec):
+        lib().tacs_vec_axpy(self.ptr, alpha, vec.ptr)
+
+    def axpby(self, alpha, beta, vec):
+        lib().tacs_vec_axpby(self.ptr, alpha, beta, vec.ptr)
+
+    def zeroEntries(self):
+        lib().tacs_vec_zero_entries(self.ptr)
+
+    def set(self, val):
+        lib().tacs_vec_set(self.ptr, val)
+
+    def setRand(self, lower=-1.0, upper=1.0):
+        lib().tacs_vec_set_rand(self.ptr, lower, upper)
+
+
+class Mat(TACSHandle):
+    def mult(self, x, y):
+        check(lib().tacs_mat_mult(self.ptr, x.ptr, y.ptr), "Mat.mult")
+
+    def zeroEntries(self):
+        lib().tacs_mat_zero_entries(self.ptr)
+
+    def getInfo(self):
+        b, n, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
+        lib().tacs_mat_get_info(self.ptr, ctypes.byref(b), ctypes.byref(n), ctypes.byref(nnz))
+        return b.value, n.value, nnz.value
+
+    def getArrays(self):
+        """Host copies (rowp, cols, A[nnzb, b, b]) of the block-CSR matrix
+        (BCSRMat::getArrays)."""
+        b, n, nnz = self.getInfo()
+        rp, cp, ap = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        lib().tacs_mat_get_arrays(self.ptr, ctypes.byref(rp), ctypes.byref(cp), ctypes.byref(ap))
+        I = ctypes.POINTER(ctypes.c_int)
+        D = ctypes.POINTER(ctypes.c_double)
+        rowp = np.ctypeslib.as_array(ctypes.cast(rp, I), shape=(n + 1,)).copy()
+        cols = np.ctypeslib.as_array(ctypes.cast(cp, I), shape=(nnz,)).copy()
+        A = np.ctypeslib.as_array(ctypes.cast(ap, D), shape=(nnz * b * b,)).copy()
+        return rowp, cols, A.reshape(nnz, b, b)
+
+    def getDeviceArrays(self):
+        rp, cp, ap = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        lib().tacs_mat_device_arrays(self.ptr, ctypes.byref(rp), ctypes.byref(cp),
+                                     ctypes.byref(ap))
+        return rp.value, cp.value, ap.value
+
+
+class Pc(TACSHandle):
+    """Block-Jacobi preconditioner of an assembled matrix."""
+
+    def __init__(self, mat):
+        self.mat = mat
+        super().__init__(lib().tacs_block_jacobi_pc_create(mat.ptr), keep=(mat,))
+
+    def factor(self):
+        check(lib().tacs_pc_factor(self.ptr), "Pc.factor")
+
+    def applyFactor(self, x, y):
+        lib().tacs_pc_apply(self.ptr, x.ptr, y.ptr)
+
+
+class KSM(TACSHandle):
+    """GMRES(m) with right preconditioning (GMRES::solve src/bpmat/KSM.cpp:785)."""
+
+    def __init__(self, mat, pc, m, restarts=0, isFlexible=0):
+        super().__init__(lib().tacs_gmres_create(mat.ptr, pc.ptr if pc else None, m, restarts,
+                                                 isFlexible), keep=(mat, pc))
+
+    def solve(self, b, x, zero_guess=1):
+        return lib().tacs_ksm_solve(self.ptr, b.ptr, x.ptr, zero_guess)
+
+    def setTolerances(self, rtol, atol):
+        lib().tacs_ksm_set_tolerances(self.ptr, rtol, atol)
+
+    def setOrthoType(self, ortho):
+        check(lib().tacs_gmres_set_ortho_type(self.ptr, ortho), "KSM.setOrthoType")
+
+    def getIterCount(self):
+        return lib().tacs_ksm_get_iter_count(self.ptr)
+
+    def getResidualNorm(self):
+        return lib().tacs_ksm_get_res_norm(self.ptr)
+
+
+class MixedKSM(KSM):
+    """FP32 GMRES(m) + block-Jacobi inside FP64 iterative refinement: the
+    outer residual b - A x, the Gram-Schmidt dot products and the Hessenberg
+    stay in FP64.  No counterpart in the reference."""
+
+    def __init__(self, mat, pc, m=30, restarts=50, inner_rtol=1e-4, max_outer=20):
+        TACSHandle.__init__(self, lib().tacs_mixed_gmres_create(mat.ptr, pc.ptr, m, restarts,
+                                                                inner_rtol, max_outer),
+                            keep=(mat, pc))
+
+    def getOuterCount(self):
+        return lib().tacs_mixed_gmres_get_outer_count(self.ptr)
+
+
+class AuxElements(TACSHandle):
+    def __init__(self, size=100):
+        super().__init__(lib().tacs_aux_elements_create(size))
+        self._elems = []
+
+    def addElement(self, num, elem):
+        self._elems.append(elem)
+        check(lib().tacs_aux_elements_add(self.ptr, int(num), elem.ptr), "AuxElements.addElement")
+
+
+class Assembler(TACSHandle):
+    @staticmethod
+    def create(comm, varsPerNode, numOwnedNodes, numElements, numDependentNodes=0):
+        return Assembler(varsPerNode, numOwnedNodes, numElements, numDependentNodes)
+
+    def __init__(self, varsPerNode, numOwnedNodes, numElements, numDependentNodes=0):
+        super().__init__(lib().tacs_assembler_create(varsPerNode, numOwnedNodes, numElements,
+                                                     numDependentNodes))
+        self._elements = None
+        self._aux = None
+
+    # -- mesh definition -------------------------------------------------
+    def setElementConnectivity(self, ptr, conn):
+        ptr = np.ascontiguousarray(ptr, dtype=np.intc)
+        conn = np.ascontiguousarray(conn, dtype=np.intc)
+        if ptr.size - 1 != self.getNumElements():
+            raise ValueError("Connectivity must match number of elements")
+        check(lib().tacs_assembler_set_element_connectivity(self.ptr, ptr.ctypes.data,
+                                                            conn.ctypes.data),
+              "setElementConnectivity")
+
+    def setElements(self, elements):
+        if len(elements) != self.getNumElements():
+            raise ValueError("Element list must match number of elements")
+        self._elements = list(elements)
+        check(lib().tacs_assembler_set_elements(self.ptr, _ptr_array(self._elements)),
+              "setElements")
+
+    def addBCs(self, nodes, _vars=None, values=None):
+        nodes = np.ascontiguousarray(nodes, dtype=np.intc)
+        nb = -1
+        vp = vv = None
+        if _vars is not None:
+            _vars = np.ascontiguousarray(_vars, dtype=np.intc)
+            nb = _vars.size
+            vp = _vars.ctypes.data
+        if values is not None:
+            values = np.ascontiguousarray(values, dtype=np.float64)
+            vv = values.ctypes.data
+        check(lib().tacs_assembler_add_bcs(self.ptr, nodes.size, nodes.ctypes.data, nb, vp, vv),
+              "addBCs")
+
+    def initialize(self):
+        check(lib().tacs_assembler_initialize(self.ptr), "initialize")
+
+    def setAuxElements(self, elems=None):
+        self._aux = elems
+        check(lib().tacs_assembler_set_aux_elements(self.ptr, elems.ptr if elems else None),
+              "setAuxElements")
+
+    # -- sizes -------------------------------------------------------------
+    def _info(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        lib().tacs_assembler_get_info(self.ptr, *[ctypes.byref(x) for x in v])
+        return [x.value for x in v]
+
+    def getVarsPerNode(self):
+        return self._info()[0]
+
+    def getNumOwnedNodes(self):
+        return self._info()[1]
+
+    def getNumNodes(self):
+        return self._info()[1]
+
+    def getNumElements(self):
+        return self._info()[2]
+
+    def getNumDesignVars(self):
+        return self._info()[3]
+
+    # -- vectors and matrices ---------------------------------------------
+    def createVec(self):
+        return Vec(lib().tacs_assembler_create_vec(self.ptr))
+
+    def createNodeVec(self):
+        return Vec(lib().tacs_assembler_create_node_vec(self.ptr))
+
+    def createDesignVec(self):
+        return Vec(lib().tacs_assembler_create_design_vec(self.ptr))
+
+    def createMat(self):
+        return Mat(lib().tacs_assembler_create_mat(self.ptr))
+
+    def setNodes(self, X):
+        check(lib().tacs_assembler_set_nodes(self.ptr, X.ptr), "setNodes")
+
+    def getNodes(self, X):
+        check(lib().tacs_assembler_get_nodes(self.ptr, X.ptr), "getNodes")
+
+    def setVariables(self, vec, dvec=None, ddvec=None):
+        check(lib().tacs_assembler_set_variables(self.ptr, vec.ptr), "setVariables")
+
+    def getVariables(self, vec, dvec=None, ddvec=None):
+        check(lib().tacs_assembler_get_variables(self.ptr, vec.ptr), "getVariables")
+
+    def zeroVariables(self):
+        lib().tacs_assembler_zero_variables(self.ptr)
+
+    def setDesignVars(self, vec):
+        check(lib().tacs_assembler_set_design_vars(self.ptr, vec.ptr), "setDesignVars")
+
+    def getDesignVars(self, vec):
+        check(lib().tacs_assembler_get_design_vars(self.ptr, vec.ptr), "getDesignVars")
+
+    def applyBCs(self, obj):
+        if isinstance(obj, Mat):
+            lib().tacs_assembler_apply_bcs_mat(self.ptr, obj.ptr)
+        else:
+            lib().tacs_assembler_apply_bcs_vec(self.ptr, obj.ptr)
+
+    # -- the hot path -------------------------------------------------------
+    def assembleRes(self, residual, loadScale=1.0, applyBCs=True):
+        check(lib().tacs_assembler_assemble_res(self.ptr, residual.ptr, loadScale,
+                                                1 if applyBCs else 0), "assembleRes")
+
+    def assembleJacobian(self, alpha, beta, gamma, residual, A, matOr=NORMAL, loadScale=1.0,
+                         applyBCs=True):
+        check(lib().tacs_assembler_assemble_jacobian(
+            self.ptr, alpha, beta, gamma, residual.ptr if residual is not None else None, A.ptr,
+            matOr, loadScale, 1 if applyBCs else 0), "assembleJacobian")
+
+    def evalFunctions(self, funclist):
+        vals = np.zeros(len(funclist))
+        check(lib().tacs_assembler_eval_functions(self.ptr, len(funclist), _ptr_array(funclist),
+                                                  vals.ctypes.data), "evalFunctions")
+        return vals
+
+    def addDVSens(self, funclist, dfdxlist, alpha=1.0):
+        check(lib().tacs_assembler_add_dv_sens(self.ptr, alpha, len(funclist),
+                                               _ptr_array(funclist), _ptr_array(dfdxlist)),
+              "addDVSens")
+
+    def addSVSens(self, funclist, dfdulist, alpha=1.0, beta=0.0, gamma=0.0):
+        check(lib().tacs_assembler_add_sv_sens(self.ptr, alpha, beta, gamma, len(funclist),
+                                               _ptr_array(funclist), _ptr_array(dfdulist)),
+              "addSVSens")
+
+    def addAdjointResProducts(self, adjlist, dfdxlist, alpha=1.0, loadScale=1.0):
+        check(lib().tacs_assembler_add_adjoint_res_products(
+            self.ptr, alpha, len(adjlist), _ptr_array(adjlist), _ptr_array(dfdxlist), loadScale),
+            "addAdjointResProducts")

@@ -1,0 +1,373 @@
+// gfx950 kernels for TACS's block-CSR matrix (BCSRMat, row-major b x b
+// blocks, src/bpmat/BCSRMat.h:34): SpMV / multi-RHS SpMM, the block-Jacobi
+// preconditioner (inverse diagonal blocks, as BCSRMat::factorDiag
+// src/bpmat/BCSRMat.cpp:1527 computes them), Dirichlet rows
+// (BCSRMat::zeroRow :2027) and precision conversion.
+#include "tacs_hip_common.h"
+
+#define WAVE_SYNC()                                        \
+  do {                                                     \
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); \
+    __builtin_amdgcn_wave_barrier();                       \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// 3x3 SpMV: one wavefront per block row.  Lane l = 9*bo + 3*r + c (bo < 7)
+// owns entry (r, c) of blocks bo, bo+7, bo+14, ... of the row, so every load
+// instruction of the wave reads 63 consecutive values of A.  Partial sums are
+// reduced over c and bo with cross-lane shuffles.  y = A x (+ y if beta).
+// ---------------------------------------------------------------------------
+template <typename T, typename ACC>
+__global__ __launch_bounds__(256) void bsr3_spmv(int nrows, const int* __restrict__ rowp,
+                                                 const int* __restrict__ cols,
+                                                 const T* __restrict__ A,
+                                                 const T* __restrict__ x, T* __restrict__ y,
+                                                 int add) {
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int bo = lane / 9;
+  const int rc = lane - 9 * bo;
+  const int c = rc % 3;
+  const bool ok = lane < 63;
+  const int nw = gridDim.x * 4;
+  for (int row = blockIdx.x * 4 + wid; row < nrows; row += nw) {
+    const int rs = rowp[row];
+    const int nb = rowp[row + 1] - rs;
+    ACC acc = 0;
+    if (ok) {
+      const T* Ar = A + (size_t)rs * 9 + rc;
+      const int* cr = cols + rs;
+      for (int s = bo; s < nb; s += 7) {
+        int col = cr[s];
+        acc += (ACC)Ar[(size_t)s * 9] * (ACC)x[3 * col + c];
+      }
+    }
+    // reduce over c (lanes l, l+1, l+2 with c = 0)
+    ACC a1 = __shfl(acc, (lane + 1) & 63, 64);
+    ACC a2 = __shfl(acc, (lane + 2) & 63, 64);
+    ACC s3 = acc + a1 + a2;
+    // reduce over bo: lane r (< 3) gathers lanes 9*b + 3*r
+    ACC tot = 0;
+#pragma unroll
+    for (int b = 0; b < 7; b++) tot += __shfl(s3, (9 * b + 3 * lane) & 63, 64);
+    if (lane < 3) {
+      size_t o = 3 * (size_t)row + lane;
+      y[o] = add ? (T)((ACC)y[o] + tot) : (T)tot;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Generic b x b SpMV (b = 6 for shells): one wavefront per block row, lanes
+// stride over the row's contiguous values.
+// ---------------------------------------------------------------------------
+template <int B, typename T, typename ACC>
+__global__ __launch_bounds__(256) void bsr_spmv_generic(int nrows, const int* __restrict__ rowp,
+                                                        const int* __restrict__ cols,
+                                                        const T* __restrict__ A,
+                                                        const T* __restrict__ x,
+                                                        T* __restrict__ y, int add) {
+  constexpr int B2 = B * B;
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nw = gridDim.x * 4;
+  for (int row = blockIdx.x * 4 + wid; row < nrows; row += nw) {
+    const int rs = rowp[row];
+    const int nval = (rowp[row + 1] - rs) * B2;
+    ACC acc[B];
+#pragma unroll
+    for (int r = 0; r < B; r++) acc[r] = 0;
+    const T* Ar = A + (size_t)rs * B2;
+    for (int k = lane; k < nval; k += 64) {
+      int s = k / B2, rc = k - s * B2, r = rc / B, c = rc - r * B;
+      ACC v = (ACC)Ar[k] * (ACC)x[(size_t)cols[rs + s] * B + c];
+#pragma unroll
+      for (int rr = 0; rr < B; rr++) acc[rr] += (rr == r) ? v : (ACC)0;
+    }
+#pragma unroll
+    for (int r = 0; r < B; r++) {
+      ACC t = acc[r];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+      acc[r] = t;
+    }
+    if (lane < B) {
+      ACC t = 0;
+#pragma unroll
+      for (int r = 0; r < B; r++) t = (r == lane) ? acc[r] : t;
+      size_t o = (size_t)B * row + lane;
+      y[o] = add ? (T)((ACC)y[o] + t) : (T)t;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-RHS 3x3 SpMM: Y[:, j] = A X[:, j] for nrhs right-hand sides stored
+// node-major, X[(3*node + c)*nrhs + j].  The matrix is read once for all
+// right-hand sides.  One wavefront per block row; lane = (rhs j, row r).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void bsr3_spmm(int nrows, int nrhs, const int* __restrict__ rowp,
+                                                 const int* __restrict__ cols,
+                                                 const double* __restrict__ A,
+                                                 const double* __restrict__ X,
+                                                 double* __restrict__ Y) {
+  __shared__ double blk[4][2][9 * 8];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nw = gridDim.x * 4;
+  for (int row = blockIdx.x * 4 + wid; row < nrows; row += nw) {
+    const int rs = rowp[row];
+    const int nb = rowp[row + 1] - rs;
+    for (int j0 = 0; j0 < nrhs; j0 += 21) {
+      // lane -> (rhs jj < 21, r < 3)
+      const int jj = lane / 3, r = lane - 3 * jj;
+      const int j = j0 + jj;
+      const bool ok = (lane < 63) && (j < nrhs);
+      double acc = 0.0;
+      for (int s0 = 0; s0 < nb; s0 += 8) {
+        const int nbs = min(8, nb - s0);
+        // stage up to 8 blocks in LDS with coalesced loads
+        for (int k = lane; k < nbs * 9; k += 64) blk[wid][0][k] = A[(size_t)(rs + s0) * 9 + k];
+        WAVE_SYNC();
+        if (ok) {
+          for (int s = 0; s < nbs; s++) {
+            int col = cols[rs + s0 + s];
+            const double* a = &blk[wid][0][9 * s + 3 * r];
+            const double* xx = &X[(size_t)3 * col * nrhs + j];
+            acc += a[0] * xx[0] + a[1] * xx[nrhs] + a[2] * xx[2 * nrhs];
+          }
+        }
+        WAVE_SYNC();
+      }
+      if (ok) Y[(size_t)(3 * row + r) * nrhs + j] = acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal-block positions, block-Jacobi inverse and application.
+// ---------------------------------------------------------------------------
+__global__ void bsr_find_diag(int nrows, const int* __restrict__ rowp,
+                              const int* __restrict__ cols, int* __restrict__ diag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  int lo = rowp[i], hi = rowp[i + 1] - 1, d = -1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    int v = cols[mid];
+    if (v == i) {
+      d = mid;
+      break;
+    }
+    if (v < i) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  diag[i] = d;
+}
+
+template <int B>
+__global__ void bsr_diag_inverse(int nrows, const int* __restrict__ diag,
+                                 const double* __restrict__ A, double* __restrict__ Dinv,
+                                 float* __restrict__ Dinv32, int* __restrict__ status) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  double m[B * B], r[B * B];
+  int d = diag[i];
+  if (d < 0) {
+    atomicExch(status, TACS_HIP_ERR_MISSING_ENTRY);
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < B * B; k++) {
+    m[k] = A[(size_t)d * B * B + k];
+    r[k] = (k % (B + 1) == 0) ? 1.0 : 0.0;
+  }
+  // Gauss-Jordan with partial pivoting
+  for (int c = 0; c < B; c++) {
+    int p = c;
+    for (int ii = c + 1; ii < B; ii++)
+      if (fabs(m[B * ii + c]) > fabs(m[B * p + c])) p = ii;
+    if (m[B * p + c] == 0.0) {
+      atomicExch(status, TACS_HIP_ERR_SINGULAR);
+      return;
+    }
+    if (p != c) {
+      for (int k = 0; k < B; k++) {
+        double t = m[B * c + k];
+        m[B * c + k] = m[B * p + k];
+        m[B * p + k] = t;
+        t = r[B * c + k];
+        r[B * c + k] = r[B * p + k];
+        r[B * p + k] = t;
+      }
+    }
+    double dd = 1.0 / m[B * c + c];
+    for (int k = 0; k < B; k++) {
+      m[B * c + k] *= dd;
+      r[B * c + k] *= dd;
+    }
+    for (int ii = 0; ii < B; ii++) {
+      if (ii != c) {
+        double f = m[B * ii + c];
+        for (int k = 0; k < B; k++) {
+          m[B * ii + k] -= f * m[B * c + k];
+          r[B * ii + k] -= f * r[B * c + k];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < B * B; k++) {
+    if (Dinv) Dinv[(size_t)i * B * B + k] = r[k];
+    if (Dinv32) Dinv32[(size_t)i * B * B + k] = (float)r[k];
+  }
+}
+
+// y = D^{-1} x per block; one thread per block row
+template <int B, typename T>
+__global__ void bsr_diag_apply(int nrows, const T* __restrict__ Dinv, const T* __restrict__ x,
+                               T* __restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  T xx[B];
+#pragma unroll
+  for (int c = 0; c < B; c++) xx[c] = x[(size_t)B * i + c];
+  const T* d = Dinv + (size_t)B * B * i;
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    T v = 0;
+#pragma unroll
+    for (int c = 0; c < B; c++) v += d[B * r + c] * xx[c];
+    y[(size_t)B * i + r] = v;
+  }
+}
+
+// Dirichlet rows on an assembled matrix: one wavefront per BC node.
+__global__ void bsr_zero_rows(int b, int nbc, const int* __restrict__ nodes,
+                              const int* __restrict__ vars, const int* __restrict__ rowp,
+                              const int* __restrict__ cols, double* __restrict__ A,
+                              int ident) {
+  int q = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (q >= nbc) return;
+  int row = nodes[q], mask = vars[q];
+  int rs = rowp[row], b2 = b * b;
+  int nval = (rowp[row + 1] - rs) * b2;
+  for (int k = lane; k < nval; k += 64) {
+    int s = k / b2, rc = k - s * b2, r = rc / b, c = rc - r * b;
+    if (mask & (1 << r)) {
+      A[(size_t)rs * b2 + k] = (ident && cols[rs + s] == row && r == c) ? 1.0 : 0.0;
+    }
+  }
+}
+
+__global__ void convert_f64_to_f32(size_t n, const double* __restrict__ x, float* __restrict__ y) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) y[i] = (float)x[i];
+}
+
+extern "C" {
+
+int tacs_hip_bsr_spmv(int b, int nrows, const int* rowp, const int* cols, const double* A,
+                      const double* x, double* y, int add, hipStream_t stream) {
+  if (nrows <= 0) return 0;
+  int grid = tacs_grid_for(nrows, 4, 65536);
+  if (b == 3)
+    hipLaunchKernelGGL((bsr3_spmv<double, double>), dim3(grid), dim3(256), 0, stream, nrows, rowp,
+                       cols, A, x, y, add);
+  else if (b == 6)
+    hipLaunchKernelGGL((bsr_spmv_generic<6, double, double>), dim3(grid), dim3(256), 0, stream,
+                       nrows, rowp, cols, A, x, y, add);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_bsr_spmv_f32(int b, int nrows, const int* rowp, const int* cols, const float* A,
+                          const float* x, float* y, int add, hipStream_t stream) {
+  if (nrows <= 0) return 0;
+  int grid = tacs_grid_for(nrows, 4, 65536);
+  if (b == 3)
+    hipLaunchKernelGGL((bsr3_spmv<float, float>), dim3(grid), dim3(256), 0, stream, nrows, rowp,
+                       cols, A, x, y, add);
+  else if (b == 6)
+    hipLaunchKernelGGL((bsr_spmv_generic<6, float, float>), dim3(grid), dim3(256), 0, stream,
+                       nrows, rowp, cols, A, x, y, add);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_bsr3_spmm(int nrows, int nrhs, const int* rowp, const int* cols, const double* A,
+                       const double* X, double* Y, hipStream_t stream) {
+  if (nrows <= 0 || nrhs <= 0) return 0;
+  int grid = tacs_grid_for(nrows, 4, 65536);
+  hipLaunchKernelGGL(bsr3_spmm, dim3(grid), dim3(256), 0, stream, nrows, nrhs, rowp, cols, A, X,
+                     Y);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_bsr_find_diag(int nrows, const int* rowp, const int* cols, int* diag,
+                           hipStream_t stream) {
+  if (nrows <= 0) return 0;
+  hipLaunchKernelGGL(bsr_find_diag, dim3((nrows + 255) / 256), dim3(256), 0, stream, nrows, rowp,
+                     cols, diag);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_bsr_diag_inverse(int b, int nrows, const int* diag, const double* A, double* Dinv,
+                              float* Dinv32, int* status, hipStream_t stream) {
+  if (nrows <= 0) return 0;
+  dim3 g((nrows + 127) / 128), t(128);
+  if (b == 3)
+    hipLaunchKernelGGL(bsr_diag_inverse<3>, g, t, 0, stream, nrows, diag, A, Dinv, Dinv32, status);
+  else if (b == 6)
+    hipLaunchKernelGGL(bsr_diag_inverse<6>, g, t, 0, stream, nrows, diag, A, Dinv, Dinv32, status);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_bsr_diag_apply(int b, int nrows, const double* Dinv, const double* x, double* y,
+                            hipStream_t stream) {
+  if (nrows <= 0) return 0;
+  dim3 g((nrows + 255) / 256), t(256);
+  if (b == 3)
+    hipLaunchKernelGGL((bsr_diag_apply<3, double>), g, t, 0, stream, nrows, Dinv, x, y);
+  else if (b == 6)
+    hipLaunchKernelGGL((bsr_diag_apply<6, double>), g, t, 0, stream, nrows, Dinv, x, y);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_bsr_diag_apply_f32(int b, int nrows, const float* Dinv, const float* x, float* y,
+                                hipStream_t stream) {
+  if (nrows <= 0) return 0;
+  dim3 g((nrows + 255) / 256), t(256);
+  if (b == 3)
+    hipLaunchKernelGGL((bsr_diag_apply<3, float>), g, t, 0, stream, nrows, Dinv, x, y);
+  else if (b == 6)
+    hipLaunchKernelGGL((bsr_diag_apply<6, float>), g, t, 0, stream, nrows, Dinv, x, y);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_bsr_zero_rows(int b, int nbc, const int* nodes, const int* vars, const int* rowp,
+                           const int* cols, double* A, int ident, hipStream_t stream) {
+  if (nbc <= 0) return 0;
+  hipLaunchKernelGGL(bsr_zero_rows, dim3((nbc + 3) / 4), dim3(256), 0, stream, b, nbc, nodes, vars,
+                     rowp, cols, A, ident);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_convert_f64_f32(size_t n, const double* x, float* y, hipStream_t stream) {
+  if (n == 0) return 0;
+  int grid = (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+  hipLaunchKernelGGL(convert_f64_to_f32, dim3(grid), dim3(256), 0, stream, n, x, y);
+  return (int)hipGetLastError();
+}
+
+}  // extern "C"

@@ -1,0 +1,430 @@
+// Object-level C ABI over the classes of include/TACSHip.h (declared in
+// include/tacs_hip.h).  Every function mirrors one method bound by the
+// reference's Cython layer (tacs/cpp_headers/TACS.pxd, tacs/*.pyx).
+#include <stdio.h>
+
+#include "../../include/TACSHip.h"
+
+int tacs_set_error(const char* fmt, ...);
+const char* tacs_get_error();
+void tacs_set_stream_impl(hipStream_t s);
+hipStream_t tacs_stream();
+
+#define OBJ(T, h) (dynamic_cast<T*>((TACSObject*)(h)))
+#define REQUIRE(T, v, h)                                           \
+  T* v = OBJ(T, h);                                                \
+  if (!v) return tacs_set_error("%s: expected a %s handle", __func__, #T)
+#define REQUIRE_NULL(T, v, h)                                      \
+  T* v = OBJ(T, h);                                                \
+  if (!v) {                                                        \
+    tacs_set_error("%s: expected a %s handle", __func__, #T);      \
+    return NULL;                                                   \
+  }
+
+static int last_error_code() {
+  const char* e = tacs_get_error();
+  return (e && e[0]) ? 1 : 0;
+}
+void tacs_clear_error();
+static void clear_error() { tacs_clear_error(); }
+
+extern "C" {
+
+const char* tacs_last_error(void) { return tacs_get_error(); }
+void tacs_incref(tacs_obj o) {
+  if (o) ((TACSObject*)o)->incref();
+}
+void tacs_decref(tacs_obj o) {
+  if (o) ((TACSObject*)o)->decref();
+}
+int tacs_refcount(tacs_obj o) { return o ? ((TACSObject*)o)->refcount() : 0; }
+void tacs_set_stream(hipStream_t s) { tacs_set_stream_impl(s); }
+hipStream_t tacs_get_stream(void) { return tacs_stream(); }
+int tacs_device_synchronize(void) { return (int)hipStreamSynchronize(tacs_stream()); }
+
+tacs_obj tacs_material_create(double rho, double cp, double E, double nu, double ys, double alpha,
+                              double kappa) {
+  return (TACSObject*)new TACSMaterialProperties(rho, cp, E, nu, ys, alpha, kappa);
+}
+tacs_obj tacs_solid_constitutive_create(tacs_obj props, double t, int tNum, double tlb,
+                                        double tub) {
+  REQUIRE_NULL(TACSMaterialProperties, p, props);
+  return (TACSObject*)new TACSSolidConstitutive(p, t, tNum, tlb, tub);
+}
+tacs_obj tacs_linear_hexa_basis_create(void) { return (TACSObject*)new TACSLinearHexaBasis(); }
+tacs_obj tacs_quadratic_hexa_basis_create(void) {
+  return (TACSObject*)new TACSQuadraticHexaBasis();
+}
+tacs_obj tacs_linear_elasticity3d_create(tacs_obj stiff, int strain_type) {
+  REQUIRE_NULL(TACSSolidConstitutive, s, stiff);
+  return (TACSObject*)new TACSLinearElasticity3D(s, (ElementStrainType)strain_type);
+}
+tacs_obj tacs_element3d_create(tacs_obj model, tacs_obj basis) {
+  REQUIRE_NULL(TACSElementModel, m, model);
+  REQUIRE_NULL(TACSElementBasis, b, basis);
+  return (TACSObject*)new TACSElement3D(m, b);
+}
+tacs_obj tacs_traction3d_create(int vpn, int face, tacs_obj basis, const double* trac, int coord) {
+  REQUIRE_NULL(TACSElementBasis, b, basis);
+  if (face < 0 || face > 5) {
+    tacs_set_error("tacs_traction3d_create: face %d out of range", face);
+    return NULL;
+  }
+  return (TACSObject*)new TACSTraction3D(vpn, face, b, trac, coord);
+}
+tacs_obj tacs_aux_elements_create(int size) { return (TACSObject*)new TACSAuxElements(size); }
+int tacs_aux_elements_add(tacs_obj aux, int elem, tacs_obj element) {
+  REQUIRE(TACSAuxElements, a, aux);
+  REQUIRE(TACSElement, e, element);
+  a->addElement(elem, e);
+  return 0;
+}
+
+tacs_obj tacs_assembler_create(int vpn, int nowned, int nelems, int ndep) {
+  clear_error();
+  TACSAssembler* a = new TACSAssembler(TACS_COMM_SELF, vpn, nowned, nelems, ndep);
+  return (TACSObject*)a;
+}
+int tacs_assembler_set_element_connectivity(tacs_obj h, const int* ptr, const int* conn) {
+  REQUIRE(TACSAssembler, a, h);
+  return a->setElementConnectivity(ptr, conn);
+}
+int tacs_assembler_set_elements(tacs_obj h, tacs_obj* elems) {
+  REQUIRE(TACSAssembler, a, h);
+  TACSElement** e = new TACSElement*[a->numElements];
+  for (int i = 0; i < a->numElements; i++) {
+    e[i] = OBJ(TACSElement, elems[i]);
+    if (!e[i]) {
+      delete[] e;
+      return tacs_set_error("setElements: entry %d is not an element", i);
+    }
+  }
+  int r = a->setElements(e);
+  delete[] e;
+  return r;
+}
+int tacs_assembler_add_bcs(tacs_obj h, int nnodes, const int* nodes, int nbcs, const int* vars,
+                           const double* vals) {
+  REQUIRE(TACSAssembler, a, h);
+  return a->addBCs(nnodes, nodes, nbcs, vars, vals);
+}
+int tacs_assembler_initialize(tacs_obj h) {
+  REQUIRE(TACSAssembler, a, h);
+  clear_error();
+  return a->initialize();
+}
+int tacs_assembler_set_aux_elements(tacs_obj h, tacs_obj aux) {
+  REQUIRE(TACSAssembler, a, h);
+  a->setAuxElements(aux ? OBJ(TACSAuxElements, aux) : NULL);
+  return 0;
+}
+int tacs_assembler_get_info(tacs_obj h, int* vpn, int* nn, int* ne, int* ndv) {
+  REQUIRE(TACSAssembler, a, h);
+  if (vpn) *vpn = a->varsPerNode;
+  if (nn) *nn = a->numNodes;
+  if (ne) *ne = a->numElements;
+  if (ndv) *ndv = a->numDesignVars;
+  return 0;
+}
+tacs_obj tacs_assembler_create_vec(tacs_obj h) {
+  REQUIRE_NULL(TACSAssembler, a, h);
+  return (TACSObject*)a->createVec();
+}
+tacs_obj tacs_assembler_create_node_vec(tacs_obj h) {
+  REQUIRE_NULL(TACSAssembler, a, h);
+  return (TACSObject*)a->createNodeVec();
+}
+tacs_obj tacs_assembler_create_design_vec(tacs_obj h) {
+  REQUIRE_NULL(TACSAssembler, a, h);
+  return (TACSObject*)a->createDesignVec();
+}
+tacs_obj tacs_assembler_create_mat(tacs_obj h) {
+  REQUIRE_NULL(TACSAssembler, a, h);
+  return (TACSObject*)a->createMat();
+}
+
+#define ASM_VEC_OP(name, call)                      \
+  int name(tacs_obj h, tacs_obj v) {                \
+    REQUIRE(TACSAssembler, a, h);                   \
+    REQUIRE(TACSBVec, x, v);                        \
+    clear_error();                                  \
+    a->call(x);                                     \
+    return last_error_code();                       \
+  }
+ASM_VEC_OP(tacs_assembler_set_nodes, setNodes)
+ASM_VEC_OP(tacs_assembler_get_nodes, getNodes)
+ASM_VEC_OP(tacs_assembler_set_variables, setVariables)
+ASM_VEC_OP(tacs_assembler_get_variables, getVariables)
+ASM_VEC_OP(tacs_assembler_set_design_vars, setDesignVars)
+ASM_VEC_OP(tacs_assembler_get_design_vars, getDesignVars)
+
+int tacs_assembler_zero_variables(tacs_obj h) {
+  REQUIRE(TACSAssembler, a, h);
+  a->zeroVariables();
+  return 0;
+}
+int tacs_assembler_apply_bcs_vec(tacs_obj h, tacs_obj v) {
+  REQUIRE(TACSAssembler, a, h);
+  REQUIRE(TACSBVec, x, v);
+  a->applyBCs((TACSVec*)x);
+  return 0;
+}
+int tacs_assembler_apply_bcs_mat(tacs_obj h, tacs_obj m) {
+  REQUIRE(TACSAssembler, a, h);
+  REQUIRE(TACSParallelMat, x, m);
+  a->applyBCs((TACSMat*)x);
+  return 0;
+}
+int tacs_assembler_assemble_res(tacs_obj h, tacs_obj res, double lambda, int bcs) {
+  REQUIRE(TACSAssembler, a, h);
+  REQUIRE(TACSBVec, r, res);
+  clear_error();
+  a->assembleRes(r, lambda, bcs != 0);
+  return last_error_code();
+}
+int tacs_assembler_assemble_jacobian(tacs_obj h, double alpha, double beta, double gamma,
+                                     tacs_obj res, tacs_obj mat, int mat_or, double lambda,
+                                     int bcs) {
+  REQUIRE(TACSAssembler, a, h);
+  REQUIRE(TACSParallelMat, m, mat);
+  TACSBVec* r = res ? OBJ(TACSBVec, res) : NULL;
+  clear_error();
+  a->assembleJacobian(alpha, beta, gamma, r, m, (MatrixOrientation)mat_or, lambda, bcs != 0);
+  return last_error_code();
+}
+int tacs_assembler_eval_functions(tacs_obj h, int n, tacs_obj* funcs, double* vals) {
+  REQUIRE(TACSAssembler, a, h);
+  TACSFunction* f[64];
+  if (n > 64) return tacs_set_error("evalFunctions: at most 64 functions");
+  for (int i = 0; i < n; i++) f[i] = OBJ(TACSFunction, funcs[i]);
+  clear_error();
+  a->evalFunctions(n, f, vals);
+  return last_error_code();
+}
+int tacs_assembler_add_dv_sens(tacs_obj h, double coef, int n, tacs_obj* funcs, tacs_obj* dfdx) {
+  REQUIRE(TACSAssembler, a, h);
+  TACSFunction* f[64];
+  TACSBVec* d[64];
+  if (n > 64) return tacs_set_error("addDVSens: at most 64 functions");
+  for (int i = 0; i < n; i++) {
+    f[i] = OBJ(TACSFunction, funcs[i]);
+    d[i] = OBJ(TACSBVec, dfdx[i]);
+  }
+  clear_error();
+  a->addDVSens(coef, n, f, d);
+  return last_error_code();
+}
+int tacs_assembler_add_sv_sens(tacs_obj h, double alpha, double beta, double gamma, int n,
+                               tacs_obj* funcs, tacs_obj* dfdu) {
+  REQUIRE(TACSAssembler, a, h);
+  TACSFunction* f[64];
+  TACSBVec* d[64];
+  if (n > 64) return tacs_set_error("addSVSens: at most 64 functions");
+  for (int i = 0; i < n; i++) {
+    f[i] = OBJ(TACSFunction, funcs[i]);
+    d[i] = OBJ(TACSBVec, dfdu[i]);
+  }
+  clear_error();
+  a->addSVSens(alpha, beta, gamma, n, f, d);
+  return last_error_code();
+}
+int tacs_assembler_add_adjoint_res_products(tacs_obj h, double scale, int n, tacs_obj* adjoint,
+                                            tacs_obj* dfdx, double lambda) {
+  REQUIRE(TACSAssembler, a, h);
+  TACSBVec* ad[64];
+  TACSBVec* d[64];
+  if (n > 64) return tacs_set_error("addAdjointResProducts: at most 64 adjoints");
+  for (int i = 0; i < n; i++) {
+    ad[i] = OBJ(TACSBVec, adjoint[i]);
+    d[i] = OBJ(TACSBVec, dfdx[i]);
+  }
+  clear_error();
+  a->addAdjointResProducts(scale, n, ad, d, lambda);
+  return last_error_code();
+}
+
+int tacs_vec_get_size(tacs_obj v, int* size, int* bsize) {
+  REQUIRE(TACSBVec, x, v);
+  if (size) *size = (int)x->n;
+  if (bsize) *bsize = x->bsize;
+  return 0;
+}
+double* tacs_vec_get_array(tacs_obj v) {
+  REQUIRE_NULL(TACSBVec, x, v);
+  double* h = NULL;
+  x->getArray(&h);
+  return h;
+}
+double* tacs_vec_device_array(tacs_obj v) {
+  REQUIRE_NULL(TACSBVec, x, v);
+  return x->devWrite();
+}
+double tacs_vec_norm(tacs_obj v) {
+  TACSBVec* x = OBJ(TACSBVec, v);
+  return x ? x->norm() : 0.0;
+}
+double tacs_vec_dot(tacs_obj v, tacs_obj w) {
+  TACSBVec* x = OBJ(TACSBVec, v);
+  TACSBVec* y = OBJ(TACSBVec, w);
+  return (x && y) ? x->dot(y) : 0.0;
+}
+int tacs_vec_axpy(tacs_obj v, double alpha, tacs_obj w) {
+  REQUIRE(TACSBVec, y, v);
+  REQUIRE(TACSBVec, x, w);
+  y->axpy(alpha, x);
+  return 0;
+}
+int tacs_vec_axpby(tacs_obj v, double alpha, double beta, tacs_obj w) {
+  REQUIRE(TACSBVec, y, v);
+  REQUIRE(TACSBVec, x, w);
+  y->axpby(alpha, beta, x);
+  return 0;
+}
+int tacs_vec_scale(tacs_obj v, double alpha) {
+  REQUIRE(TACSBVec, y, v);
+  y->scale(alpha);
+  return 0;
+}
+int tacs_vec_copy_values(tacs_obj v, tacs_obj w) {
+  REQUIRE(TACSBVec, y, v);
+  REQUIRE(TACSBVec, x, w);
+  if (x->n != y->n) return tacs_set_error("copyValues: size mismatch");
+  y->copyValues(x);
+  return 0;
+}
+int tacs_vec_zero_entries(tacs_obj v) {
+  REQUIRE(TACSBVec, y, v);
+  y->zeroEntries();
+  return 0;
+}
+int tacs_vec_set(tacs_obj v, double val) {
+  REQUIRE(TACSBVec, y, v);
+  y->set(val);
+  return 0;
+}
+int tacs_vec_set_rand(tacs_obj v, double lo, double hi) {
+  REQUIRE(TACSBVec, y, v);
+  y->setRand(lo, hi);
+  return 0;
+}
+
+int tacs_mat_mult(tacs_obj m, tacs_obj x, tacs_obj y) {
+  REQUIRE(TACSParallelMat, A, m);
+  REQUIRE(TACSBVec, xv, x);
+  REQUIRE(TACSBVec, yv, y);
+  clear_error();
+  A->mult(xv, yv);
+  return last_error_code();
+}
+int tacs_mat_zero_entries(tacs_obj m) {
+  REQUIRE(TACSParallelMat, A, m);
+  A->zeroEntries();
+  return 0;
+}
+int tacs_mat_get_info(tacs_obj m, int* bsize, int* nrows, long* nnzb) {
+  REQUIRE(TACSParallelMat, A, m);
+  if (bsize) *bsize = A->bsize;
+  if (nrows) *nrows = A->nrows;
+  if (nnzb) *nnzb = A->nnzb;
+  return 0;
+}
+int tacs_mat_get_arrays(tacs_obj m, const int** rowp, const int** cols, const double** Av) {
+  REQUIRE(TACSParallelMat, A, m);
+  double* h = NULL;
+  A->getArrays(NULL, NULL, rowp, cols, &h);
+  if (Av) *Av = h;
+  return 0;
+}
+int tacs_mat_device_arrays(tacs_obj m, const int** rowp, const int** cols, double** Av) {
+  REQUIRE(TACSParallelMat, A, m);
+  if (rowp) *rowp = A->d_rowp;
+  if (cols) *cols = A->d_cols;
+  if (Av) *Av = A->d_A;
+  return 0;
+}
+
+tacs_obj tacs_block_jacobi_pc_create(tacs_obj mat) {
+  REQUIRE_NULL(TACSParallelMat, A, mat);
+  return (TACSObject*)new TACSBlockJacobiPc(A);
+}
+int tacs_pc_factor(tacs_obj pc) {
+  REQUIRE(TACSPc, p, pc);
+  clear_error();
+  p->factor();
+  return last_error_code();
+}
+int tacs_pc_apply(tacs_obj pc, tacs_obj x, tacs_obj y) {
+  REQUIRE(TACSPc, p, pc);
+  REQUIRE(TACSBVec, xv, x);
+  REQUIRE(TACSBVec, yv, y);
+  p->applyFactor(xv, yv);
+  return 0;
+}
+tacs_obj tacs_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int flexible) {
+  REQUIRE_NULL(TACSParallelMat, A, mat);
+  TACSPc* p = pc ? OBJ(TACSPc, pc) : NULL;
+  if (m < 1 || m > 128) {
+    tacs_set_error("GMRES: subspace size must be in [1, 128]");
+    return NULL;
+  }
+  return (TACSObject*)new GMRES(A, p, m, nrestart, flexible);
+}
+int tacs_gmres_set_ortho_type(tacs_obj ksm, int ortho) {
+  REQUIRE(GMRES, g, ksm);
+  if (ortho == CLASSICAL_GRAM_SCHMIDT && g->msub > 31)
+    return tacs_set_error("classical Gram-Schmidt requires a subspace of at most 31");
+  g->setOrthoType((enum OrthoType)ortho);
+  return 0;
+}
+tacs_obj tacs_mixed_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, double inner_rtol,
+                                 int max_outer) {
+  REQUIRE_NULL(TACSParallelMat, A, mat);
+  REQUIRE_NULL(TACSBlockJacobiPc, p, pc);
+  return (TACSObject*)new TACSMixedGMRES(A, p, m, nrestart, inner_rtol, max_outer);
+}
+int tacs_ksm_set_tolerances(tacs_obj ksm, double rtol, double atol) {
+  REQUIRE(TACSKsm, k, ksm);
+  k->setTolerances(rtol, atol);
+  return 0;
+}
+int tacs_ksm_solve(tacs_obj ksm, tacs_obj b, tacs_obj x, int zero_guess) {
+  TACSKsm* k = OBJ(TACSKsm, ksm);
+  TACSBVec* bv = OBJ(TACSBVec, b);
+  TACSBVec* xv = OBJ(TACSBVec, x);
+  if (!k || !bv || !xv) {
+    tacs_set_error("tacs_ksm_solve: bad handle");
+    return -1;
+  }
+  return k->solve(bv, xv, zero_guess);
+}
+int tacs_ksm_get_iter_count(tacs_obj ksm) {
+  TACSKsm* k = OBJ(TACSKsm, ksm);
+  return k ? k->getIterCount() : -1;
+}
+double tacs_ksm_get_res_norm(tacs_obj ksm) {
+  TACSKsm* k = OBJ(TACSKsm, ksm);
+  return k ? k->getResidualNorm() : -1.0;
+}
+int tacs_mixed_gmres_get_outer_count(tacs_obj ksm) {
+  TACSMixedGMRES* k = OBJ(TACSMixedGMRES, ksm);
+  return k ? k->outerCount : -1;
+}
+
+tacs_obj tacs_compliance_create(tacs_obj assembler) {
+  REQUIRE_NULL(TACSAssembler, a, assembler);
+  return (TACSObject*)new TACSCompliance(a);
+}
+tacs_obj tacs_ksfailure_create(tacs_obj assembler, double w, double alpha, double sf) {
+  REQUIRE_NULL(TACSAssembler, a, assembler);
+  return (TACSObject*)new TACSKSFailure(a, w, alpha, sf);
+}
+double tacs_function_get_value(tacs_obj f) {
+  TACSFunction* fn = OBJ(TACSFunction, f);
+  return fn ? fn->getFunctionValue() : 0.0;
+}
+double tacs_ksfailure_get_max_failure(tacs_obj f) {
+  TACSKSFailure* k = OBJ(TACSKSFailure, f);
+  return k ? k->maxFail : 0.0;
+}
+
+}  // extern "C"

@@ -1,0 +1,1307 @@
+// Host side of the MI355X TACS path: the C++ classes of include/TACSHip.h.
+// They keep the reference's public semantics (src/TACSAssembler.h,
+// src/bpmat/TACSBVec.h, src/bpmat/TACSParallelMat.h, src/bpmat/KSM.h,
+// src/functions/*.h) while all data lives on the GPU and every loop runs as a
+// kernel of tacs_amd/csrc/*.hip.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/TACSHip.h"
+
+// ---------------------------------------------------------------------------
+// Library state: stream and last error
+// ---------------------------------------------------------------------------
+static hipStream_t g_stream = 0;
+static thread_local std::string g_err;
+
+int tacs_set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  fprintf(stderr, "TACS error: %s\n", buf);
+  return 1;
+}
+const char* tacs_get_error() { return g_err.c_str(); }
+void tacs_clear_error() { g_err.clear(); }
+void tacs_set_stream_impl(hipStream_t s) { g_stream = s; }
+hipStream_t tacs_stream() { return g_stream; }
+
+#define HCHECK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      tacs_set_error("%s failed at %s:%d: %s", #expr, __FILE__, __LINE__,                \
+                     hipGetErrorString(_e));                                             \
+  } while (0)
+#define KCHECK(expr)                                                                     \
+  do {                                                                                   \
+    int _e = (expr);                                                                     \
+    if (_e != 0) tacs_set_error("%s failed at %s:%d: code %d", #expr, __FILE__, __LINE__, _e); \
+  } while (0)
+
+template <typename T>
+static T* dalloc(size_t n) {
+  T* p = NULL;
+  if (n == 0) n = 1;
+  HCHECK(hipMalloc((void**)&p, n * sizeof(T)));
+  return p;
+}
+static void dfree(void* p) {
+  if (p) HCHECK(hipFree(p));
+}
+template <typename T>
+static void h2d(T* d, const T* h, size_t n) {
+  if (n) HCHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, g_stream));
+}
+template <typename T>
+static void d2h(T* h, const T* d, size_t n) {
+  if (n) HCHECK(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, g_stream));
+  HCHECK(hipStreamSynchronize(g_stream));
+}
+static double read_scalar(const double* d) {
+  double v = 0.0;
+  d2h(&v, d, 1);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// TACSObject
+// ---------------------------------------------------------------------------
+TACSObject::TACSObject() : ref_count(0) {}
+TACSObject::~TACSObject() {}
+void TACSObject::incref() { ref_count++; }
+void TACSObject::decref() {
+  ref_count--;
+  if (ref_count == 0) delete this;
+}
+const char* TACSObject::getObjectName() { return "TACSObject"; }
+
+// ---------------------------------------------------------------------------
+// Materials and elements (descriptors; the element family is compiled into
+// the kernels)
+// ---------------------------------------------------------------------------
+TACSMaterialProperties::TACSMaterialProperties(TacsScalar _rho, TacsScalar _cp, TacsScalar _E,
+                                               TacsScalar _nu, TacsScalar _ys, TacsScalar _alpha,
+                                               TacsScalar _kappa)
+    : rho(_rho), specific_heat(_cp), E(_E), nu(_nu), ys(_ys), alpha(_alpha), kappa(_kappa) {
+  G = 0.5 * E / (1.0 + nu);  // TACSMaterialProperties.cpp:31
+}
+
+// Isotropic C[21] (TACSMaterialProperties::evalTangentStiffness3D :270)
+void TACSMaterialProperties::evalTangentStiffness3D(TacsScalar C[]) {
+  TacsScalar D = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  for (int i = 0; i < 21; i++) C[i] = 0.0;
+  C[0] = C[6] = C[11] = (1.0 - nu) * D;
+  C[1] = C[2] = C[7] = nu * D;
+  C[15] = C[18] = C[20] = G;
+}
+
+TACSSolidConstitutive::TACSSolidConstitutive(TACSMaterialProperties* p, TacsScalar _t, int _tNum,
+                                             TacsScalar _tlb, TacsScalar _tub)
+    : props(p), t(_t), tlb(_tlb), tub(_tub), tNum(_tNum) {
+  if (props) props->incref();
+}
+TACSSolidConstitutive::~TACSSolidConstitutive() {
+  if (props) props->decref();
+}
+
+TACSLinearElasticity3D::TACSLinearElasticity3D(TACSSolidConstitutive* s, ElementStrainType st)
+    : stiff(s), strain_type(st) {
+  stiff->incref();
+}
+TACSLinearElasticity3D::~TACSLinearElasticity3D() { stiff->decref(); }
+
+TACSElement3D::TACSElement3D(TACSElementModel* m, TACSElementBasis* b) : model(m), basis(b) {
+  model->incref();
+  basis->incref();
+}
+TACSElement3D::~TACSElement3D() {
+  model->decref();
+  basis->decref();
+}
+
+TACSTraction3D::TACSTraction3D(int vpn, int face, TACSElementBasis* b, const TacsScalar t[],
+                               int coord)
+    : varsPerNode(vpn), faceIndex(face), coordComponent(coord), basis(b) {
+  basis->incref();
+  int n = coord ? vpn : 3 * vpn;
+  for (int i = 0; i < 24; i++) trac[i] = (i < n) ? t[i] : 0.0;
+}
+TACSTraction3D::~TACSTraction3D() { basis->decref(); }
+
+TACSAuxElements::TACSAuxElements(int size) : n(0), cap(size > 0 ? size : 16), version(0) {
+  nums = new int[cap];
+  elems = new TACSElement*[cap];
+}
+TACSAuxElements::~TACSAuxElements() {
+  for (int i = 0; i < n; i++) elems[i]->decref();
+  delete[] nums;
+  delete[] elems;
+}
+void TACSAuxElements::addElement(int num, TACSElement* e) {
+  if (n >= cap) {
+    int nc = 2 * cap;
+    int* nn = new int[nc];
+    TACSElement** ne = new TACSElement*[nc];
+    memcpy(nn, nums, n * sizeof(int));
+    memcpy(ne, elems, n * sizeof(TACSElement*));
+    delete[] nums;
+    delete[] elems;
+    nums = nn;
+    elems = ne;
+    cap = nc;
+  }
+  e->incref();
+  nums[n] = num;
+  elems[n] = e;
+  n++;
+  version++;
+}
+void TACSAuxElements::clear() {
+  for (int i = 0; i < n; i++) elems[i]->decref();
+  n = 0;
+  version++;
+}
+
+// ---------------------------------------------------------------------------
+// TACSBcMap (KSM.cpp:76)
+// ---------------------------------------------------------------------------
+TACSBcMap::TACSBcMap(int _bsize, int num) : bsize(_bsize), nbcs(0) {
+  max_size = num > 0 ? num : 32;
+  nodes = new int[max_size];
+  vars = new int[max_size];
+  values = new TacsScalar[bsize * max_size];
+}
+TACSBcMap::~TACSBcMap() {
+  delete[] nodes;
+  delete[] vars;
+  delete[] values;
+}
+void TACSBcMap::addBC(int node, int nvals, const int* bc_vars, const TacsScalar* bc_vals) {
+  if (nbcs >= max_size) {
+    int ms = 2 * max_size;
+    int* tn = new int[ms];
+    int* tv = new int[ms];
+    TacsScalar* tval = new TacsScalar[bsize * ms];
+    memcpy(tn, nodes, nbcs * sizeof(int));
+    memcpy(tv, vars, nbcs * sizeof(int));
+    memcpy(tval, values, bsize * nbcs * sizeof(TacsScalar));
+    delete[] nodes;
+    delete[] vars;
+    delete[] values;
+    nodes = tn;
+    vars = tv;
+    values = tval;
+    max_size = ms;
+  }
+  nodes[nbcs] = node;
+  vars[nbcs] = 0;
+  for (int k = 0; k < bsize; k++) values[bsize * nbcs + k] = 0.0;
+  if (bc_vars && bc_vals) {
+    for (int i = 0; i < nvals; i++) {
+      if (bc_vars[i] >= 0 && bc_vars[i] < bsize) {
+        vars[nbcs] |= (1 << bc_vars[i]);
+        values[bsize * nbcs + bc_vars[i]] = bc_vals[i];
+      }
+    }
+  } else if (bc_vars) {
+    for (int i = 0; i < nvals; i++)
+      if (bc_vars[i] >= 0 && bc_vars[i] < bsize) vars[nbcs] |= (1 << bc_vars[i]);
+  } else {
+    for (int i = 0; i < nvals && i < bsize; i++) vars[nbcs] |= (1 << i);
+  }
+  nbcs++;
+}
+int TACSBcMap::getBCs(const int** n, const int** v, TacsScalar** vals) {
+  if (n) *n = nodes;
+  if (v) *v = vars;
+  if (vals) *vals = values;
+  return nbcs;
+}
+
+// ---------------------------------------------------------------------------
+// TACSBVec: device-resident block vector with a lazily synchronised host copy
+// ---------------------------------------------------------------------------
+enum { VEC_DEVICE = 0, VEC_BOTH = 1, VEC_HOST = 2 };
+
+static void* g_vec_scratch = NULL;  // reduction scratch shared by vector ops
+static double* g_vec_scalar = NULL;
+static void* vec_scratch() {
+  if (!g_vec_scratch) {
+    size_t b = tacs_hip_red_scratch_bytes();
+    HCHECK(hipMalloc(&g_vec_scratch, b));
+    HCHECK(hipMemset(g_vec_scratch, 0, b));
+    g_vec_scalar = dalloc<double>(8);
+  }
+  return g_vec_scratch;
+}
+
+TACSBVec::TACSBVec(int _nnodes, int _bsize) : nnodes(_nnodes), bsize(_bsize) {
+  n = (size_t)nnodes * bsize;
+  d_x = dalloc<double>(n);
+  HCHECK(hipMemsetAsync(d_x, 0, (n ? n : 1) * sizeof(double), g_stream));
+  h_x = NULL;
+  state = VEC_DEVICE;
+}
+TACSBVec::~TACSBVec() {
+  dfree(d_x);
+  if (h_x) HCHECK(hipHostFree(h_x));
+}
+void TACSBVec::getSize(int* size) { *size = (int)n; }
+int TACSBVec::getArray(TacsScalar** array) {
+  if (!h_x) {
+    HCHECK(hipHostMalloc((void**)&h_x, (n ? n : 1) * sizeof(double), 0));
+    if (state == VEC_BOTH) state = VEC_DEVICE;
+  }
+  if (state == VEC_DEVICE) d2h(h_x, d_x, n);
+  state = VEC_HOST;
+  if (array) *array = h_x;
+  return (int)n;
+}
+const double* TACSBVec::devRead() {
+  if (state == VEC_HOST) {
+    h2d(d_x, h_x, n);
+    state = VEC_BOTH;
+  }
+  return d_x;
+}
+double* TACSBVec::devWrite() {
+  if (state == VEC_HOST) h2d(d_x, h_x, n);
+  state = VEC_DEVICE;
+  return d_x;
+}
+TacsScalar TACSBVec::norm() {
+  void* rs = vec_scratch();
+  const double* x = devRead();
+  KCHECK(tacs_hip_dot(64, n, NULL, NULL, (void*)x, x, rs, g_vec_scalar, 1, NULL, g_stream));
+  return read_scalar(g_vec_scalar);
+}
+void TACSBVec::scale(TacsScalar alpha) {
+  KCHECK(tacs_hip_scale(64, n, devWrite(), NULL, alpha, 0, g_stream));
+}
+TacsScalar TACSBVec::dot(TACSVec* tx) {
+  TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
+  void* rs = vec_scratch();
+  const double* a = devRead();
+  const double* b = x->devRead();
+  KCHECK(tacs_hip_dot(64, n, NULL, NULL, (void*)a, b, rs, g_vec_scalar, 0, NULL, g_stream));
+  return read_scalar(g_vec_scalar);
+}
+void TACSBVec::axpy(TacsScalar alpha, TACSVec* tx) {
+  TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
+  const double* xd = x->devRead();
+  KCHECK(tacs_hip_axpby(64, 64, n, alpha, xd, 1.0, devWrite(), g_stream));
+}
+void TACSBVec::axpby(TacsScalar alpha, TacsScalar beta, TACSVec* tx) {
+  TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
+  const double* xd = x->devRead();
+  KCHECK(tacs_hip_axpby(64, 64, n, alpha, xd, beta, devWrite(), g_stream));
+}
+void TACSBVec::copyValues(TACSVec* tx) {
+  TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
+  const double* xd = x->devRead();
+  if (n) HCHECK(hipMemcpyAsync(devWrite(), xd, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
+}
+void TACSBVec::zeroEntries() { KCHECK(tacs_hip_fill(64, n, devWrite(), 0.0, g_stream)); }
+void TACSBVec::set(TacsScalar v) { KCHECK(tacs_hip_fill(64, n, devWrite(), v, g_stream)); }
+void TACSBVec::setRand(double lower, double upper) {
+  double* h;
+  getArray(&h);
+  for (size_t i = 0; i < n; i++) h[i] = lower + (upper - lower) * (rand() / (double)RAND_MAX);
+}
+void TACSBVec::applyBCs(TACSBcMap* map, TACSVec* tvec, TacsScalar lambda) {
+  // TACSBVec::applyBCs src/bpmat/TACSBVec.cpp:546 (entries applied in order)
+  const int *nodes, *vars;
+  TacsScalar* vals;
+  int nbc = map->getBCs(&nodes, &vars, &vals);
+  if (nbc == 0) return;
+  TACSBVec* u = tvec ? dynamic_cast<TACSBVec*>(tvec) : NULL;
+  double* h;
+  double* hu = NULL;
+  if (u) {
+    u->getArray(&hu);
+  }
+  getArray(&h);
+  for (int i = 0; i < nbc; i++) {
+    for (int k = 0; k < bsize; k++) {
+      if (vars[i] & (1 << k)) {
+        size_t idx = (size_t)bsize * nodes[i] + k;
+        h[idx] = hu ? hu[idx] - lambda * vals[bsize * i + k] : 0.0;
+      }
+    }
+  }
+}
+void TACSBVec::setBCs(TACSBcMap* map, TacsScalar lambda) {
+  const int *nodes, *vars;
+  TacsScalar* vals;
+  int nbc = map->getBCs(&nodes, &vars, &vals);
+  if (nbc == 0) return;
+  double* h;
+  getArray(&h);
+  for (int i = 0; i < nbc; i++)
+    for (int k = 0; k < bsize; k++)
+      if (vars[i] & (1 << k)) h[(size_t)bsize * nodes[i] + k] = lambda * vals[bsize * i + k];
+}
+
+// ---------------------------------------------------------------------------
+// TACSParallelMat (one rank = the [Aloc] block-CSR matrix)
+// ---------------------------------------------------------------------------
+TACSParallelMat::TACSParallelMat(int _bsize, int _nrows, const int* rowp, const int* cols)
+    : bsize(_bsize), nrows(_nrows) {
+  nnzb = rowp[nrows];
+  h_rowp = new int[nrows + 1];
+  h_cols = new int[nnzb > 0 ? nnzb : 1];
+  memcpy(h_rowp, rowp, (nrows + 1) * sizeof(int));
+  memcpy(h_cols, cols, nnzb * sizeof(int));
+  d_rowp = dalloc<int>(nrows + 1);
+  d_cols = dalloc<int>(nnzb);
+  h2d(d_rowp, h_rowp, nrows + 1);
+  h2d(d_cols, h_cols, nnzb);
+  size_t nv = (size_t)nnzb * bsize * bsize;
+  d_A = dalloc<double>(nv);
+  HCHECK(hipMemsetAsync(d_A, 0, (nv ? nv : 1) * sizeof(double), g_stream));
+  d_A32 = NULL;
+  A32_current = 0;
+  h_A = NULL;
+}
+TACSParallelMat::~TACSParallelMat() {
+  dfree(d_rowp);
+  dfree(d_cols);
+  dfree(d_A);
+  dfree(d_A32);
+  delete[] h_rowp;
+  delete[] h_cols;
+  if (h_A) HCHECK(hipHostFree(h_A));
+}
+TACSVec* TACSParallelMat::createVec() { return new TACSBVec(nrows, bsize); }
+void TACSParallelMat::mult(TACSVec* tx, TACSVec* ty) {
+  TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
+  TACSBVec* y = dynamic_cast<TACSBVec*>(ty);
+  const double* xd = x->devRead();
+  KCHECK(tacs_hip_bsr_spmv(bsize, nrows, d_rowp, d_cols, d_A, xd, y->devWrite(), 0, g_stream));
+}
+void TACSParallelMat::zeroEntries() {
+  KCHECK(tacs_hip_fill(64, (size_t)nnzb * bsize * bsize, d_A, 0.0, g_stream));
+  A32_current = 0;
+}
+void TACSParallelMat::applyBCs(TACSBcMap* bcmap) {
+  const int *nodes, *vars;
+  int nbc = bcmap->getBCs(&nodes, &vars, NULL);
+  if (nbc == 0) return;
+  int* dn = dalloc<int>(nbc);
+  int* dv = dalloc<int>(nbc);
+  h2d(dn, nodes, nbc);
+  h2d(dv, vars, nbc);
+  KCHECK(tacs_hip_bsr_zero_rows(bsize, nbc, dn, dv, d_rowp, d_cols, d_A, 1, g_stream));
+  HCHECK(hipStreamSynchronize(g_stream));
+  dfree(dn);
+  dfree(dv);
+  A32_current = 0;
+}
+int TACSParallelMat::getSize(int* nr, int* nc) {
+  *nr = nrows * bsize;
+  *nc = nrows * bsize;
+  return 0;
+}
+void TACSParallelMat::getArrays(int* b, int* nr, const int** rowp, const int** cols,
+                                TacsScalar** A) {
+  size_t nv = (size_t)nnzb * bsize * bsize;
+  if (!h_A) HCHECK(hipHostMalloc((void**)&h_A, (nv ? nv : 1) * sizeof(double), 0));
+  d2h(h_A, d_A, nv);
+  if (b) *b = bsize;
+  if (nr) *nr = nrows;
+  if (rowp) *rowp = h_rowp;
+  if (cols) *cols = h_cols;
+  if (A) *A = h_A;
+}
+float* TACSParallelMat::getValues32(int refresh) {
+  size_t nv = (size_t)nnzb * bsize * bsize;
+  if (!d_A32) {
+    d_A32 = dalloc<float>(nv);
+    A32_current = 0;
+  }
+  if (refresh && !A32_current) {
+    KCHECK(tacs_hip_convert_f64_f32(nv, d_A, d_A32, g_stream));
+    A32_current = 1;
+  }
+  return d_A32;
+}
+
+// ---------------------------------------------------------------------------
+// Block-Jacobi preconditioner
+// ---------------------------------------------------------------------------
+TACSBlockJacobiPc::TACSBlockJacobiPc(TACSParallelMat* m) : mat(m) {
+  mat->incref();
+  d_diag = dalloc<int>(mat->nrows);
+  KCHECK(tacs_hip_bsr_find_diag(mat->nrows, mat->d_rowp, mat->d_cols, d_diag, g_stream));
+  size_t nb = (size_t)mat->nrows * mat->bsize * mat->bsize;
+  d_Dinv = dalloc<double>(nb);
+  d_Dinv32 = dalloc<float>(nb);
+}
+TACSBlockJacobiPc::~TACSBlockJacobiPc() {
+  dfree(d_diag);
+  dfree(d_Dinv);
+  dfree(d_Dinv32);
+  mat->decref();
+}
+void TACSBlockJacobiPc::factor() {
+  int* st = dalloc<int>(1);
+  HCHECK(hipMemsetAsync(st, 0, sizeof(int), g_stream));
+  KCHECK(tacs_hip_bsr_diag_inverse(mat->bsize, mat->nrows, d_diag, mat->d_A, d_Dinv, d_Dinv32, st,
+                                   g_stream));
+  int hs = 0;
+  d2h(&hs, st, 1);
+  dfree(st);
+  if (hs) tacs_set_error("TACSBlockJacobiPc::factor: singular or missing diagonal block (%d)", hs);
+}
+void TACSBlockJacobiPc::applyFactor(TACSVec* tx, TACSVec* ty) {
+  TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
+  TACSBVec* y = dynamic_cast<TACSBVec*>(ty);
+  const double* xd = x->devRead();
+  KCHECK(tacs_hip_bsr_diag_apply(mat->bsize, mat->nrows, d_Dinv, xd, y->devWrite(), g_stream));
+}
+void TACSBlockJacobiPc::applyFactor32(const float* x, float* y) {
+  KCHECK(tacs_hip_bsr_diag_apply_f32(mat->bsize, mat->nrows, d_Dinv32, x, y, g_stream));
+}
+
+// ---------------------------------------------------------------------------
+// GMRES (FP64), restating GMRES::solve src/bpmat/KSM.cpp:785-958 with the
+// Arnoldi scalars kept on the device.
+// ---------------------------------------------------------------------------
+// scalar layout inside d_scal for subspace m
+struct GmresScal {
+  size_t H, res, Qc, Qs, misc, tmp, total;
+  GmresScal(int m) {
+    H = 0;
+    res = H + (size_t)(m + 1) * m;
+    Qc = res + (m + 1);
+    Qs = Qc + m;
+    misc = Qs + m;
+    tmp = misc + 16;
+    total = tmp + (m + 1);
+  }
+};
+enum { MISC_ST = 0, MISC_NRM = 1, MISC_INV = 2, MISC_NINV = 3, MISC_ONE = 4 };
+
+GMRES::GMRES(TACSMat* m, TACSPc* p, int _m, int _nrestart, int isFlexible)
+    : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart >= 0 ? _nrestart : 0),
+      ortho(MODIFIED_GRAM_SCHMIDT) {
+  mat = dynamic_cast<TACSParallelMat*>(m);
+  pc = p ? dynamic_cast<TACSBlockJacobiPc*>(p) : NULL;
+  mat->incref();
+  if (pc) pc->incref();
+  size_t n = (size_t)mat->nrows * mat->bsize;
+  d_W = dalloc<double>((size_t)(msub + 1) * n);
+  d_work = dalloc<double>(n);
+  GmresScal L(msub);
+  d_scal = dalloc<double>(L.total);
+  HCHECK(hipMemsetAsync(d_scal, 0, L.total * sizeof(double), g_stream));
+  size_t rb = tacs_hip_red_scratch_bytes();
+  HCHECK(hipMalloc(&d_red, rb));
+  HCHECK(hipMemsetAsync(d_red, 0, rb, g_stream));
+  HCHECK(hipHostMalloc((void**)&h_pin, 64 * sizeof(double), 0));
+}
+GMRES::~GMRES() {
+  dfree(d_W);
+  dfree(d_work);
+  dfree(d_scal);
+  dfree(d_red);
+  HCHECK(hipHostFree(h_pin));
+  mat->decref();
+  if (pc) pc->decref();
+}
+void GMRES::setTolerances(double r, double a) {
+  rtol = r;
+  atol = a;
+}
+void GMRES::setOrthoType(enum OrthoType o) { ortho = o; }
+
+static double pin_read(double* h_pin, const double* d) {
+  HCHECK(hipMemcpyAsync(h_pin, d, sizeof(double), hipMemcpyDeviceToHost, g_stream));
+  HCHECK(hipStreamSynchronize(g_stream));
+  return h_pin[0];
+}
+
+int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
+  TACSBVec* b = dynamic_cast<TACSBVec*>(tb);
+  TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
+  const size_t n = (size_t)mat->nrows * mat->bsize;
+  const int bs = mat->bsize, nr = mat->nrows;
+  GmresScal L(msub);
+  double* H = d_scal + L.H;
+  double* res = d_scal + L.res;
+  double* Qc = d_scal + L.Qc;
+  double* Qs = d_scal + L.Qs;
+  double* misc = d_scal + L.misc;
+  double* tmp = d_scal + L.tmp;
+  auto W = [&](int i) { return d_W + (size_t)i * n; };
+  const double* bd = b->devRead();
+  double* xd = x->devWrite();
+  double rhs_norm = 0.0;
+  int solve_flag = 0;
+  iterCount = 0;
+
+  for (int count = 0; count < nrestart + 1; count++) {
+    if (zero_guess && count == 0) {
+      KCHECK(tacs_hip_fill(64, n, xd, 0.0, g_stream));
+      HCHECK(hipMemcpyAsync(W(0), bd, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
+      KCHECK(tacs_hip_dot(64, n, NULL, NULL, W(0), W(0), d_red, res, 1, misc + MISC_INV, g_stream));
+      KCHECK(tacs_hip_scale(64, n, W(0), misc + MISC_INV, 0.0, 0, g_stream));
+    } else {
+      KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, W(0), 0, g_stream));
+      KCHECK(tacs_hip_axpby(64, 64, n, -1.0, bd, 1.0, W(0), g_stream));
+      KCHECK(tacs_hip_dot(64, n, NULL, NULL, W(0), W(0), d_red, res, 1, misc + MISC_INV, g_stream));
+      KCHECK(tacs_hip_scalar_op(misc + MISC_INV, -1.0, misc + MISC_NINV, g_stream));
+      KCHECK(tacs_hip_scale(64, n, W(0), misc + MISC_NINV, 0.0, 0, g_stream));
+    }
+    double r0 = pin_read(h_pin, res);
+    if (count == 0) {
+      rhs_norm = r0;
+      resNorm = rhs_norm;
+    }
+    int niters = 0;
+    if (r0 < atol) {
+      solve_flag = 1;
+      break;
+    }
+    for (int i = 0; i < msub; i++) {
+      double* q = W(i + 1);
+      if (pc) {
+        KCHECK(tacs_hip_bsr_diag_apply(bs, nr, pc->d_Dinv, W(i), d_work, g_stream));
+        KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, d_work, q, 0, g_stream));
+      } else {
+        KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, W(i), q, 0, g_stream));
+      }
+      double* h = H + (size_t)i * (msub + 1);
+      if (ortho == MODIFIED_GRAM_SCHMIDT) {
+        // h_j = W_j . q, q -= h_j W_j for j = 0..i (ModifiedGramSchmidt
+        // KSM.cpp:526); each subtraction is fused into the next dot.
+        KCHECK(tacs_hip_dot(64, n, NULL, NULL, q, W(0), d_red, &h[0], 0, NULL, g_stream));
+        for (int j = 1; j <= i; j++)
+          KCHECK(tacs_hip_dot(64, n, W(j - 1), &h[j - 1], q, W(j), d_red, &h[j], 0, NULL, g_stream));
+        KCHECK(tacs_hip_dot(64, n, W(i), &h[i], q, q, d_red, &h[i + 1], 1, misc + MISC_INV, g_stream));
+      } else {
+        // classical Gram-Schmidt, twice (ClassicalGramSchmidt KSM.cpp:515)
+        int k = i + 1;
+        KCHECK(tacs_hip_fill(64, k, h, 0.0, g_stream));
+        KCHECK(tacs_hip_mdot(64, n, k, d_W, n, q, d_red, h, g_stream));
+        KCHECK(tacs_hip_cgs_update(64, n, k, d_W, n, h, NULL, q, g_stream));
+        KCHECK(tacs_hip_fill(64, k, tmp, 0.0, g_stream));
+        KCHECK(tacs_hip_mdot(64, n, k, d_W, n, q, d_red, tmp, g_stream));
+        KCHECK(tacs_hip_cgs_update(64, n, k, d_W, n, tmp, NULL, q, g_stream));
+        KCHECK(tacs_hip_axpby(64, 64, k, 1.0, tmp, 1.0, h, g_stream));
+        KCHECK(tacs_hip_dot(64, n, NULL, NULL, q, q, d_red, &h[i + 1], 1, misc + MISC_INV, g_stream));
+      }
+      KCHECK(tacs_hip_scale(64, n, q, misc + MISC_INV, 0.0, 0, g_stream));
+      KCHECK(tacs_hip_givens(i, msub, H, Qc, Qs, res, misc + MISC_ST, g_stream));
+      niters++;
+      resNorm = pin_read(h_pin, misc + MISC_ST);
+      if (resNorm < atol || resNorm < rtol * rhs_norm || !std::isfinite(resNorm)) {
+        solve_flag = std::isfinite(resNorm) ? 1 : 0;
+        break;
+      }
+    }
+    iterCount += niters;
+    if (niters > 0) {
+      KCHECK(tacs_hip_backsolve(niters, msub, H, res, g_stream));
+      if (pc) {
+        KCHECK(tacs_hip_lincomb(64, 64, n, niters, d_W, n, res, d_work, 0, g_stream));
+        KCHECK(tacs_hip_bsr_diag_apply(bs, nr, pc->d_Dinv, d_work, W(0), g_stream));
+        KCHECK(tacs_hip_axpby(64, 64, n, 1.0, W(0), 1.0, xd, g_stream));
+      } else {
+        KCHECK(tacs_hip_lincomb(64, 64, n, niters, d_W, n, res, xd, 1, g_stream));
+      }
+    }
+    if (solve_flag || !std::isfinite(resNorm)) break;
+  }
+  HCHECK(hipStreamSynchronize(g_stream));
+  return solve_flag;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: FP32 GMRES (CGS2, FP64 Hessenberg) inside FP64 iterative
+// refinement.
+// ---------------------------------------------------------------------------
+TACSMixedGMRES::TACSMixedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int _m, int _nrestart,
+                               double _inner_rtol, int _max_outer)
+    : rtol(1e-8), atol(1e-30), inner_rtol(_inner_rtol), msub(_m), nrestart(_nrestart),
+      max_outer(_max_outer), outerCount(0), mat(m), pc(p) {
+  mat->incref();
+  if (pc) pc->incref();
+  if (msub > 31) msub = 31;  // CGS multi-dot width
+  size_t n = (size_t)mat->nrows * mat->bsize;
+  d_W = dalloc<float>((size_t)(msub + 1) * n);
+  d_work32 = dalloc<float>(n);
+  d_r32 = dalloc<float>(n);
+  d_d32 = dalloc<float>(n);
+  d_r = dalloc<double>(n);
+  GmresScal L(msub);
+  d_scal = dalloc<double>(L.total);
+  HCHECK(hipMemsetAsync(d_scal, 0, L.total * sizeof(double), g_stream));
+  size_t rb = tacs_hip_red_scratch_bytes();
+  HCHECK(hipMalloc(&d_red, rb));
+  HCHECK(hipMemsetAsync(d_red, 0, rb, g_stream));
+  HCHECK(hipHostMalloc((void**)&h_pin, 64 * sizeof(double), 0));
+}
+TACSMixedGMRES::~TACSMixedGMRES() {
+  dfree(d_W);
+  dfree(d_work32);
+  dfree(d_r32);
+  dfree(d_d32);
+  dfree(d_r);
+  dfree(d_scal);
+  dfree(d_red);
+  HCHECK(hipHostFree(h_pin));
+  mat->decref();
+  if (pc) pc->decref();
+}
+void TACSMixedGMRES::setTolerances(double r, double a) {
+  rtol = r;
+  atol = a;
+}
+
+// Solve A32 d = r (r of unit norm) to inner_rtol; returns Arnoldi steps.
+int TACSMixedGMRES::innerSolve(const float* r, float* d) {
+  const size_t n = (size_t)mat->nrows * mat->bsize;
+  const int bs = mat->bsize, nr = mat->nrows;
+  float* A32 = mat->getValues32(1);
+  GmresScal L(msub);
+  double* H = d_scal + L.H;
+  double* res = d_scal + L.res;
+  double* Qc = d_scal + L.Qc;
+  double* Qs = d_scal + L.Qs;
+  double* misc = d_scal + L.misc;
+  double* tmp = d_scal + L.tmp;
+  auto W = [&](int i) { return d_W + (size_t)i * n; };
+  int total = 0;
+  double r0norm = 0.0;
+  KCHECK(tacs_hip_fill(32, n, d, 0.0, g_stream));
+  for (int count = 0; count < nrestart + 1; count++) {
+    if (count == 0) {
+      HCHECK(hipMemcpyAsync(W(0), r, n * sizeof(float), hipMemcpyDeviceToDevice, g_stream));
+    } else {
+      // W0 = r - A32 d
+      KCHECK(tacs_hip_bsr_spmv_f32(bs, nr, mat->d_rowp, mat->d_cols, A32, d, W(0), 0, g_stream));
+      KCHECK(tacs_hip_axpby(32, 32, n, 1.0, r, -1.0, W(0), g_stream));
+    }
+    KCHECK(tacs_hip_dot(32, n, NULL, NULL, W(0), W(0), d_red, res, 1, misc + MISC_INV, g_stream));
+    KCHECK(tacs_hip_scale(32, n, W(0), misc + MISC_INV, 0.0, 0, g_stream));
+    double rn = pin_read(h_pin, res);
+    if (count == 0) r0norm = rn;
+    if (rn <= inner_rtol * r0norm || rn == 0.0) break;
+    int niters = 0, done = 0;
+    for (int i = 0; i < msub; i++) {
+      float* q = W(i + 1);
+      KCHECK(tacs_hip_bsr_diag_apply_f32(bs, nr, pc->d_Dinv32, W(i), d_work32, g_stream));
+      KCHECK(tacs_hip_bsr_spmv_f32(bs, nr, mat->d_rowp, mat->d_cols, A32, d_work32, q, 0, g_stream));
+      double* h = H + (size_t)i * (msub + 1);
+      int k = i + 1;
+      KCHECK(tacs_hip_fill(64, k, h, 0.0, g_stream));
+      KCHECK(tacs_hip_mdot(32, n, k, d_W, n, q, d_red, h, g_stream));
+      KCHECK(tacs_hip_cgs_update(32, n, k, d_W, n, h, NULL, q, g_stream));
+      KCHECK(tacs_hip_fill(64, k, tmp, 0.0, g_stream));
+      KCHECK(tacs_hip_mdot(32, n, k, d_W, n, q, d_red, tmp, g_stream));
+      KCHECK(tacs_hip_cgs_update(32, n, k, d_W, n, tmp, NULL, q, g_stream));
+      KCHECK(tacs_hip_axpby(64, 64, k, 1.0, tmp, 1.0, h, g_stream));
+      KCHECK(tacs_hip_dot(32, n, NULL, NULL, q, q, d_red, &h[i + 1], 1, misc + MISC_INV, g_stream));
+      KCHECK(tacs_hip_scale(32, n, q, misc + MISC_INV, 0.0, 0, g_stream));
+      KCHECK(tacs_hip_givens(i, msub, H, Qc, Qs, res, misc + MISC_ST, g_stream));
+      niters++;
+      double st = pin_read(h_pin, misc + MISC_ST);
+      if (st <= inner_rtol * r0norm || !std::isfinite(st)) {
+        done = 1;
+        break;
+      }
+    }
+    total += niters;
+    KCHECK(tacs_hip_backsolve(niters, msub, H, res, g_stream));
+    KCHECK(tacs_hip_lincomb(32, 32, n, niters, d_W, n, res, d_work32, 0, g_stream));
+    KCHECK(tacs_hip_bsr_diag_apply_f32(bs, nr, pc->d_Dinv32, d_work32, W(0), g_stream));
+    KCHECK(tacs_hip_axpby(32, 32, n, 1.0, W(0), 1.0, d, g_stream));
+    if (done) break;
+  }
+  return total;
+}
+
+int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
+  TACSBVec* b = dynamic_cast<TACSBVec*>(tb);
+  TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
+  const size_t n = (size_t)mat->nrows * mat->bsize;
+  const int bs = mat->bsize, nr = mat->nrows;
+  GmresScal L(msub);
+  double* misc = d_scal + L.misc;
+  const double* bd = b->devRead();
+  double* xd = x->devWrite();
+  iterCount = 0;
+  outerCount = 0;
+  if (zero_guess) {
+    KCHECK(tacs_hip_fill(64, n, xd, 0.0, g_stream));
+    HCHECK(hipMemcpyAsync(d_r, bd, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
+  } else {
+    KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
+    KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
+  }
+  double r0 = 0.0;
+  int flag = 0;
+  for (int k = 0; k <= max_outer; k++) {
+    KCHECK(tacs_hip_dot(64, n, NULL, NULL, d_r, d_r, d_red, misc + MISC_NRM, 1, misc + MISC_INV,
+                        g_stream));
+    double rn = pin_read(h_pin, misc + MISC_NRM);
+    if (k == 0) r0 = rn;
+    resNorm = rn;
+    if (rn <= rtol * r0 || rn < atol) {
+      flag = 1;
+      break;
+    }
+    if (!std::isfinite(rn) || k == max_outer) break;
+    // r32 = r / |r|
+    KCHECK(tacs_hip_axpby(64, 32, n, 1.0 / rn, d_r, 0.0, d_r32, g_stream));
+    iterCount += innerSolve(d_r32, d_d32);
+    outerCount++;
+    // x += |r| d ; r = b - A x (FP64)
+    KCHECK(tacs_hip_axpby(32, 64, n, rn, d_d32, 1.0, xd, g_stream));
+    KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
+    KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
+  }
+  HCHECK(hipStreamSynchronize(g_stream));
+  return flag;
+}
+
+// ---------------------------------------------------------------------------
+// Functions
+// ---------------------------------------------------------------------------
+TACSFunction::TACSFunction(TACSAssembler* a) : assembler(a) { assembler->incref(); }
+TACSFunction::~TACSFunction() { assembler->decref(); }
+TACSCompliance::TACSCompliance(TACSAssembler* a) : TACSFunction(a) {}
+TACSKSFailure::TACSKSFailure(TACSAssembler* a, double w, double al, double sf)
+    : TACSFunction(a), ksWeight(w), alpha(al), safetyFactor(sf) {}
+// TACSKSFailure::getFunctionValue src/functions/TACSKSFailure.cpp:74
+TacsScalar TACSKSFailure::getFunctionValue() {
+  return maxFail + std::log(ksFailSum / alpha) / ksWeight;
+}
+
+// ---------------------------------------------------------------------------
+// TACSAssembler
+// ---------------------------------------------------------------------------
+TACSAssembler::TACSAssembler(TACSComm comm, int vpn, int nowned, int nelems, int ndep)
+    : varsPerNode(vpn), numNodes(nowned), numElements(nelems), npe(0), order(0),
+      initialized(0), numDesignVars(0), h_ptr(NULL), h_conn(NULL), elements(NULL),
+      auxElements(NULL), aux_version(-1), naux(0), nnzb(0), jac_ms(0.0) {
+  if (ndep != 0) tacs_set_error("TACSAssembler: dependent nodes are not supported");
+  bcMap = new TACSBcMap(varsPerNode, 64);
+  bcMap->incref();
+  memset(&mat, 0, sizeof(mat));
+  d_conn = d_n2e_ptr = d_n2e = d_bcvars = d_bcnodes = d_bcmask = NULL;
+  d_bcvals = NULL;
+  nbc_nodes = 0;
+  d_dvnum = d_dv2e_ptr = d_dv2e = NULL;
+  d_tfixed = d_telem = d_X = d_u = d_ev = d_es = NULL;
+  d_aux_elem = d_aux_face = d_n2a_ptr = d_n2a = NULL;
+  d_aux_trac = d_aux_ev = NULL;
+  d_status = NULL;
+  d_red = NULL;
+  d_scalar = NULL;
+  h_rowp = h_cols = NULL;
+}
+
+TACSAssembler::~TACSAssembler() {
+  if (elements) {
+    for (int i = 0; i < numElements; i++)
+      if (elements[i]) elements[i]->decref();
+    delete[] elements;
+  }
+  delete[] h_ptr;
+  delete[] h_conn;
+  delete[] h_rowp;
+  delete[] h_cols;
+  bcMap->decref();
+  if (auxElements) auxElements->decref();
+  void* ds[] = {d_conn, d_n2e_ptr, d_n2e, d_bcvars, d_bcnodes, d_bcmask, d_bcvals, d_dvnum,
+                d_dv2e_ptr, d_dv2e, d_tfixed, d_telem, d_X, d_u, d_ev, d_es, d_aux_elem,
+                d_aux_face, d_n2a_ptr, d_n2a, d_aux_trac, d_aux_ev, d_status, d_red, d_scalar};
+  for (void* p : ds) dfree(p);
+}
+
+int TACSAssembler::setElementConnectivity(const int* ptr, const int* conn) {
+  if (initialized) return tacs_set_error("setElementConnectivity after initialize()");
+  delete[] h_ptr;
+  delete[] h_conn;
+  h_ptr = new int[numElements + 1];
+  memcpy(h_ptr, ptr, (numElements + 1) * sizeof(int));
+  size_t len = ptr[numElements];
+  h_conn = new int[len > 0 ? len : 1];
+  memcpy(h_conn, conn, len * sizeof(int));
+  for (size_t i = 0; i < len; i++)
+    if (conn[i] < 0 || conn[i] >= numNodes)
+      return tacs_set_error("setElementConnectivity: node %d out of range", conn[i]);
+  return 0;
+}
+
+int TACSAssembler::setElements(TACSElement** elems) {
+  if (initialized) return tacs_set_error("setElements after initialize()");
+  if (!elements) {
+    elements = new TACSElement*[numElements];
+    memset(elements, 0, numElements * sizeof(TACSElement*));
+  }
+  for (int i = 0; i < numElements; i++) {
+    elems[i]->incref();
+    if (elements[i]) elements[i]->decref();
+    elements[i] = elems[i];
+  }
+  return 0;
+}
+
+int TACSAssembler::addBCs(int nnodes, const int* nodes, int nbcs, const int* vars,
+                          const TacsScalar* vals) {
+  // TACSAssembler::addBCs src/TACSAssembler.cpp:801: without vars (or with
+  // nbcs < 0) every variable of the node is constrained
+  if (initialized) return tacs_set_error("Cannot call addBC() after initialize()");
+  if (!vars || nbcs < 0) nbcs = varsPerNode;
+  for (int i = 0; i < nnodes; i++) {
+    if (nodes[i] < 0 || nodes[i] >= numNodes) return tacs_set_error("addBCs: node out of range");
+    bcMap->addBC(nodes[i], nbcs, vars, vals);
+  }
+  return 0;
+}
+
+void TACSAssembler::setAuxElements(TACSAuxElements* aux) {
+  if (aux) aux->incref();
+  if (auxElements) auxElements->decref();
+  auxElements = aux;
+  aux_version = -1;
+}
+
+int TACSAssembler::initialize() {
+  if (initialized) return 0;
+  if (!h_ptr || !h_conn) return tacs_set_error("initialize: connectivity not set");
+  if (!elements) return tacs_set_error("initialize: elements not set");
+  if (varsPerNode != 3) return tacs_set_error("initialize: only 3 variables per node supported");
+  // Classify the element family: TACSElement3D + TACSLinearElasticity3D +
+  // TACSSolidConstitutive on one hexahedral basis and one material.
+  TACSMaterialProperties* props0 = NULL;
+  std::vector<int> dvnum(numElements);
+  std::vector<double> tfixed(numElements);
+  for (int e = 0; e < numElements; e++) {
+    TACSElement3D* el = dynamic_cast<TACSElement3D*>(elements[e]);
+    if (!el) return tacs_set_error("initialize: element %d is not a TACSElement3D", e);
+    TACSLinearElasticity3D* model = dynamic_cast<TACSLinearElasticity3D*>(el->model);
+    if (!model) return tacs_set_error("initialize: element %d model is not TACSLinearElasticity3D", e);
+    if (model->strain_type != TACS_LINEAR_STRAIN)
+      return tacs_set_error("initialize: only TACS_LINEAR_STRAIN is supported");
+    int o = el->basis->getOrder();
+    if (order == 0) order = o;
+    if (o != order) return tacs_set_error("initialize: mixed element bases are not supported");
+    TACSMaterialProperties* p = model->stiff->props;
+    if (!props0) props0 = p;
+    if (p != props0 && (p->E != props0->E || p->nu != props0->nu || p->rho != props0->rho ||
+                        p->ys != props0->ys))
+      return tacs_set_error("initialize: elements must share one material");
+    dvnum[e] = model->stiff->tNum;
+    tfixed[e] = model->stiff->t;
+  }
+  npe = order * order * order;
+  for (int e = 0; e < numElements; e++)
+    if (h_ptr[e + 1] - h_ptr[e] != npe)
+      return tacs_set_error("initialize: element %d has %d nodes, expected %d", e,
+                            h_ptr[e + 1] - h_ptr[e], npe);
+  props0->evalTangentStiffness3D(mat.C);
+  mat.rho = props0->rho;
+  mat.ys = props0->ys;
+  mat.isotropic = 1;
+  mat.mu = props0->G;
+  mat.lam = mat.C[1];  // nu*E/((1+nu)(1-2nu))
+
+  // node -> element CSR (TACSAssembler::computeNodeToElementCSR :1791),
+  // entries (element << 5 | local index), ascending element order
+  std::vector<int> n2e_ptr(numNodes + 1, 0);
+  for (int e = 0; e < numElements; e++)
+    for (int a = 0; a < npe; a++) n2e_ptr[h_conn[e * npe + a] + 1]++;
+  for (int i = 0; i < numNodes; i++) n2e_ptr[i + 1] += n2e_ptr[i];
+  std::vector<int> n2e(n2e_ptr[numNodes]);
+  {
+    std::vector<int> pos(n2e_ptr.begin(), n2e_ptr.end() - 1);
+    for (int e = 0; e < numElements; e++)
+      for (int a = 0; a < npe; a++) n2e[pos[h_conn[e * npe + a]]++] = (e << 5) | a;
+  }
+  // node -> node CSR (computeLocalNodeToNodeCSR :1899): sorted, unique,
+  // diagonal kept
+  h_rowp = new int[numNodes + 1];
+  std::vector<int> cols;
+  cols.reserve((size_t)numNodes * (order == 2 ? 27 : 64));
+  h_rowp[0] = 0;
+  std::vector<int> tmp;
+  for (int i = 0; i < numNodes; i++) {
+    tmp.clear();
+    for (int p = n2e_ptr[i]; p < n2e_ptr[i + 1]; p++) {
+      int e = n2e[p] >> 5;
+      for (int a = 0; a < npe; a++) tmp.push_back(h_conn[e * npe + a]);
+    }
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    cols.insert(cols.end(), tmp.begin(), tmp.end());
+    if (cols.size() > 0x7fffffffL) return tacs_set_error("initialize: too many nonzero blocks");
+    h_rowp[i + 1] = (int)cols.size();
+  }
+  nnzb = (long)cols.size();
+  h_cols = new int[nnzb > 0 ? nnzb : 1];
+  memcpy(h_cols, cols.data(), nnzb * sizeof(int));
+  int maxrow = 0;
+  for (int i = 0; i < numNodes; i++) maxrow = std::max(maxrow, h_rowp[i + 1] - h_rowp[i]);
+  int maxc = (order == 2) ? 64 : 125;
+  if (maxrow > maxc)
+    return tacs_set_error("initialize: a node couples to %d nodes (limit %d)", maxrow, maxc);
+
+  // Boundary conditions: merged per node (mask OR; later values win)
+  {
+    const int *bn, *bv;
+    TacsScalar* bval;
+    int nbc = bcMap->getBCs(&bn, &bv, &bval);
+    std::vector<int> mask(numNodes, 0), order_nodes;
+    std::vector<double> vals((size_t)numNodes * 3, 0.0);
+    for (int q = 0; q < nbc; q++) {
+      int nd = bn[q];
+      if (!mask[nd] && bv[q]) order_nodes.push_back(nd);
+      for (int k = 0; k < 3; k++)
+        if (bv[q] & (1 << k)) vals[3 * nd + k] = bval[3 * q + k];
+      mask[nd] |= bv[q];
+    }
+    std::sort(order_nodes.begin(), order_nodes.end());
+    nbc_nodes = (int)order_nodes.size();
+    std::vector<int> masks(nbc_nodes);
+    std::vector<double> bvals((size_t)nbc_nodes * 3);
+    for (int q = 0; q < nbc_nodes; q++) {
+      masks[q] = mask[order_nodes[q]];
+      for (int k = 0; k < 3; k++) bvals[3 * q + k] = vals[3 * order_nodes[q] + k];
+    }
+    d_bcvars = dalloc<int>(numNodes);
+    h2d(d_bcvars, mask.data(), numNodes);
+    d_bcnodes = dalloc<int>(nbc_nodes);
+    d_bcmask = dalloc<int>(nbc_nodes);
+    d_bcvals = dalloc<double>((size_t)3 * nbc_nodes);
+    h2d(d_bcnodes, order_nodes.data(), nbc_nodes);
+    h2d(d_bcmask, masks.data(), nbc_nodes);
+    h2d(d_bcvals, bvals.data(), (size_t)3 * nbc_nodes);
+    HCHECK(hipStreamSynchronize(g_stream));
+  }
+
+  // Design variables: one per element constitutive (tNum), dv -> elements CSR
+  int maxdv = -1;
+  for (int e = 0; e < numElements; e++) maxdv = std::max(maxdv, dvnum[e]);
+  numDesignVars = maxdv + 1;
+  std::vector<int> dv2e_ptr(numDesignVars + 1, 0), dv2e;
+  for (int e = 0; e < numElements; e++)
+    if (dvnum[e] >= 0) dv2e_ptr[dvnum[e] + 1]++;
+  for (int d = 0; d < numDesignVars; d++) dv2e_ptr[d + 1] += dv2e_ptr[d];
+  dv2e.resize(dv2e_ptr[numDesignVars]);
+  {
+    std::vector<int> pos(dv2e_ptr.begin(), dv2e_ptr.end() - 1);
+    for (int e = 0; e < numElements; e++)
+      if (dvnum[e] >= 0) dv2e[pos[dvnum[e]]++] = e;
+  }
+
+  // Upload
+  size_t clen = (size_t)numElements * npe;
+  d_conn = dalloc<int>(clen);
+  h2d(d_conn, h_conn, clen);
+  d_n2e_ptr = dalloc<int>(numNodes + 1);
+  d_n2e = dalloc<int>(n2e.size());
+  h2d(d_n2e_ptr, n2e_ptr.data(), numNodes + 1);
+  h2d(d_n2e, n2e.data(), n2e.size());
+  d_dvnum = dalloc<int>(numElements);
+  d_tfixed = dalloc<double>(numElements);
+  d_telem = dalloc<double>(numElements);
+  h2d(d_dvnum, dvnum.data(), numElements);
+  h2d(d_tfixed, tfixed.data(), numElements);
+  h2d(d_telem, tfixed.data(), numElements);
+  d_dv2e_ptr = dalloc<int>(numDesignVars + 1);
+  d_dv2e = dalloc<int>(dv2e.size());
+  h2d(d_dv2e_ptr, dv2e_ptr.data(), numDesignVars + 1);
+  h2d(d_dv2e, dv2e.data(), dv2e.size());
+  d_X = dalloc<double>((size_t)3 * numNodes);
+  d_u = dalloc<double>((size_t)3 * numNodes);
+  HCHECK(hipMemsetAsync(d_X, 0, (size_t)3 * numNodes * sizeof(double), g_stream));
+  HCHECK(hipMemsetAsync(d_u, 0, (size_t)3 * numNodes * sizeof(double), g_stream));
+  d_status = dalloc<int>(1);
+  HCHECK(hipMemsetAsync(d_status, 0, sizeof(int), g_stream));
+  size_t rb = tacs_hip_red_scratch_bytes();
+  HCHECK(hipMalloc(&d_red, rb));
+  HCHECK(hipMemsetAsync(d_red, 0, rb, g_stream));
+  d_scalar = dalloc<double>(16);
+  HCHECK(hipStreamSynchronize(g_stream));
+  initialized = 1;
+  return 0;
+}
+
+int TACSAssembler::status() {
+  int s = 0;
+  d2h(&s, d_status, 1);
+  if (s) HCHECK(hipMemsetAsync(d_status, 0, sizeof(int), g_stream));
+  return s;
+}
+
+TACSBVec* TACSAssembler::createVec() { return new TACSBVec(numNodes, varsPerNode); }
+TACSBVec* TACSAssembler::createNodeVec() { return new TACSBVec(numNodes, 3); }
+TACSBVec* TACSAssembler::createDesignVec() { return new TACSBVec(numDesignVars, 1); }
+TACSParallelMat* TACSAssembler::createMat() {
+  if (!initialized) {
+    tacs_set_error("Cannot call createMat() before initialize()");
+    return NULL;
+  }
+  return new TACSParallelMat(varsPerNode, numNodes, h_rowp, h_cols);
+}
+
+void TACSAssembler::setNodes(TACSBVec* X) {
+  HCHECK(hipMemcpyAsync(d_X, X->devRead(), (size_t)3 * numNodes * sizeof(double),
+                        hipMemcpyDeviceToDevice, g_stream));
+}
+void TACSAssembler::getNodes(TACSBVec* X) {
+  HCHECK(hipMemcpyAsync(X->devWrite(), d_X, (size_t)3 * numNodes * sizeof(double),
+                        hipMemcpyDeviceToDevice, g_stream));
+}
+void TACSAssembler::setVariables(TACSBVec* q, TACSBVec*, TACSBVec*) {
+  HCHECK(hipMemcpyAsync(d_u, q->devRead(), (size_t)3 * numNodes * sizeof(double),
+                        hipMemcpyDeviceToDevice, g_stream));
+}
+void TACSAssembler::getVariables(TACSBVec* q, TACSBVec*, TACSBVec*) {
+  HCHECK(hipMemcpyAsync(q->devWrite(), d_u, (size_t)3 * numNodes * sizeof(double),
+                        hipMemcpyDeviceToDevice, g_stream));
+}
+void TACSAssembler::zeroVariables() {
+  KCHECK(tacs_hip_fill(64, (size_t)3 * numNodes, d_u, 0.0, g_stream));
+}
+void TACSAssembler::setDesignVars(TACSBVec* dvs) {
+  KCHECK(tacs_hip_gather_design_vars(numElements, d_dvnum, d_tfixed,
+                                     numDesignVars > 0 ? dvs->devRead() : NULL, d_telem, g_stream));
+}
+void TACSAssembler::getDesignVars(TACSBVec* dvs) {
+  std::vector<double> t(numElements);
+  d2h(t.data(), d_telem, numElements);
+  std::vector<int> dvn(numElements);
+  d2h(dvn.data(), d_dvnum, numElements);
+  double* h;
+  dvs->getArray(&h);
+  for (int e = numElements - 1; e >= 0; e--)
+    if (dvn[e] >= 0) h[dvn[e]] = t[e];
+}
+void TACSAssembler::applyBCs(TACSVec* vec) {
+  TACSBVec* v = dynamic_cast<TACSBVec*>(vec);
+  KCHECK(tacs_hip_apply_bc_vec(3, nbc_nodes, d_bcnodes, d_bcmask, NULL, NULL, 1.0, v->devWrite(),
+                               g_stream));
+}
+void TACSAssembler::applyBCs(TACSMat* m) { m->applyBCs(bcMap); }
+
+int TACSAssembler::uploadAux() {
+  if (!auxElements) {
+    naux = 0;
+    return 0;
+  }
+  if (aux_version == auxElements->version) return 0;
+  // sort aux elements by element index (TACSAuxElements::sort), stable
+  int n = auxElements->getNumAux();
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; i++) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(),
+                   [&](int a, int b) { return auxElements->nums[a] < auxElements->nums[b]; });
+  std::vector<int> ae(n), af(n);
+  std::vector<double> at((size_t)3 * n);
+  for (int q = 0; q < n; q++) {
+    TACSTraction3D* tr = dynamic_cast<TACSTraction3D*>(auxElements->elems[idx[q]]);
+    if (!tr) return tacs_set_error("aux element %d is not a TACSTraction3D", idx[q]);
+    if (!tr->coordComponent)
+      return tacs_set_error("aux element %d: only coordinate-component tractions supported", idx[q]);
+    if (tr->basis->getOrder() != order) return tacs_set_error("aux element basis mismatch");
+    ae[q] = auxElements->nums[idx[q]];
+    af[q] = tr->faceIndex;
+    for (int k = 0; k < 3; k++) at[3 * q + k] = tr->trac[k];
+  }
+  // node -> (aux, local node) CSR, ascending aux order
+  std::vector<int> ptr(numNodes + 1, 0);
+  for (int q = 0; q < n; q++)
+    for (int a = 0; a < npe; a++) ptr[h_conn[(size_t)ae[q] * npe + a] + 1]++;
+  for (int i = 0; i < numNodes; i++) ptr[i + 1] += ptr[i];
+  std::vector<int> list(ptr[numNodes]);
+  std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+  for (int q = 0; q < n; q++)
+    for (int a = 0; a < npe; a++) list[pos[h_conn[(size_t)ae[q] * npe + a]]++] = (q << 5) | a;
+  void* old[] = {d_aux_elem, d_aux_face, d_aux_trac, d_n2a_ptr, d_n2a, d_aux_ev};
+  HCHECK(hipStreamSynchronize(g_stream));
+  for (void* p : old) dfree(p);
+  d_aux_elem = dalloc<int>(n);
+  d_aux_face = dalloc<int>(n);
+  d_aux_trac = dalloc<double>((size_t)3 * n);
+  d_n2a_ptr = dalloc<int>(numNodes + 1);
+  d_n2a = dalloc<int>(list.size());
+  d_aux_ev = dalloc<double>((size_t)n * 3 * npe);
+  h2d(d_aux_elem, ae.data(), n);
+  h2d(d_aux_face, af.data(), n);
+  h2d(d_aux_trac, at.data(), (size_t)3 * n);
+  h2d(d_n2a_ptr, ptr.data(), numNodes + 1);
+  h2d(d_n2a, list.data(), list.size());
+  HCHECK(hipStreamSynchronize(g_stream));
+  naux = n;
+  aux_version = auxElements->version;
+  return 0;
+}
+
+int TACSAssembler::elementVectorsToNodes(const double* ev, double* out, int accumulate) {
+  return tacs_hip_gather_elem_vectors(3 * npe, numNodes, d_n2e_ptr, d_n2e, ev, accumulate, out,
+                                      g_stream);
+}
+
+int TACSAssembler::scatterElementScalarsToDVs(const double* es, double* dfdx) {
+  return tacs_hip_scatter_dv_sens(numDesignVars, d_dv2e_ptr, d_dv2e, es, dfdx, g_stream);
+}
+
+static double* ensure(double*& p, size_t n) {
+  if (!p) p = dalloc<double>(n);
+  return p;
+}
+
+// TACSAssembler::assembleRes src/TACSAssembler.cpp:4133
+void TACSAssembler::assembleRes(TACSBVec* residual, const TacsScalar lambda, const bool bcs) {
+  if (!initialized) {
+    tacs_set_error("assembleRes before initialize()");
+    return;
+  }
+  double* r = residual->devWrite();
+  ensure(d_ev, (size_t)numElements * 3 * npe);
+  KCHECK(tacs_hip_solid_elem_vectors(order, 0, numElements, d_conn, d_X, d_u, d_telem, &mat, 1.0,
+                                     0, 0, 1, 1, d_ev, g_stream));
+  KCHECK(elementVectorsToNodes(d_ev, r, 0));
+  if (uploadAux() == 0 && naux > 0) {
+    KCHECK(tacs_hip_solid_traction_vectors(order, naux, d_aux_elem, d_aux_face, d_aux_trac, d_conn,
+                                           d_X, lambda, d_aux_ev, g_stream));
+    KCHECK(tacs_hip_gather_elem_vectors(3 * npe, numNodes, d_n2a_ptr, d_n2a, d_aux_ev, 1, r,
+                                        g_stream));
+  }
+  if (bcs)
+    KCHECK(tacs_hip_apply_bc_vec(3, nbc_nodes, d_bcnodes, d_bcmask, d_bcvals, d_u, lambda, r,
+                                 g_stream));
+}
+
+// TACSAssembler::assembleJacobian src/TACSAssembler.cpp:4291.  Element
+// matrices are symmetric, so TACS_MAT_TRANSPOSE assembles the same values.
+void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScalar gamma,
+                                     TACSBVec* residual, TACSMat* tA, MatrixOrientation matOr,
+                                     const TacsScalar lambda, const bool bcs) {
+  if (!initialized) {
+    tacs_set_error("assembleJacobian before initialize()");
+    return;
+  }
+  TACSParallelMat* A = dynamic_cast<TACSParallelMat*>(tA);
+  if (!A || A->nrows != numNodes || A->nnzb != nnzb || A->bsize != 3) {
+    tacs_set_error("assembleJacobian: matrix does not match the assembler");
+    return;
+  }
+  if (residual) assembleRes(residual, lambda, bcs);
+  hipEvent_t e0, e1;
+  HCHECK(hipEventCreate(&e0));
+  HCHECK(hipEventCreate(&e1));
+  HCHECK(hipEventRecord(e0, g_stream));
+  KCHECK(tacs_hip_solid_jacobian(order, numNodes, d_n2e_ptr, d_n2e, d_conn, d_X, d_telem, &mat,
+                                 alpha, gamma, A->d_rowp, A->d_cols, bcs ? d_bcvars : NULL, A->d_A,
+                                 A->d_A32, d_status, g_stream));
+  HCHECK(hipEventRecord(e1, g_stream));
+  HCHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HCHECK(hipEventElapsedTime(&ms, e0, e1));
+  jac_ms = ms;
+  HCHECK(hipEventDestroy(e0));
+  HCHECK(hipEventDestroy(e1));
+  A->A32_current = A->d_A32 ? 1 : 0;
+  int s = status();
+  if (s) tacs_set_error("assembleJacobian: kernel status %d", s);
+}
+
+// TACSAssembler::evalFunctions src/TACSAssembler.cpp:4617
+void TACSAssembler::evalFunctions(int numFuncs, TACSFunction** funcs, TacsScalar* vals) {
+  ensure(d_es, numElements);
+  for (int k = 0; k < numFuncs; k++) {
+    vals[k] = 0.0;
+    if (!funcs[k]) continue;
+    if (TACSCompliance* c = dynamic_cast<TACSCompliance*>(funcs[k])) {
+      KCHECK(tacs_hip_solid_elem_scalars(order, 0, numElements, d_conn, d_X, d_u, NULL, d_telem,
+                                         &mat, 1.0, 0, 0, 1, d_es, g_stream));
+      KCHECK(tacs_hip_reduce(numElements, d_es, 0, d_red, d_scalar, g_stream));
+      c->compliance = read_scalar(d_scalar);
+      vals[k] = c->compliance;
+    } else if (TACSKSFailure* f = dynamic_cast<TACSKSFailure*>(funcs[k])) {
+      KCHECK(tacs_hip_solid_elem_scalars(order, 1, numElements, d_conn, d_X, d_u, NULL, d_telem,
+                                         &mat, 1.0, f->ksWeight, 0, f->safetyFactor, d_es,
+                                         g_stream));
+      KCHECK(tacs_hip_reduce(numElements, d_es, 1, d_red, d_scalar, g_stream));
+      f->maxFail = read_scalar(d_scalar);
+      KCHECK(tacs_hip_solid_elem_scalars(order, 2, numElements, d_conn, d_X, d_u, NULL, d_telem,
+                                         &mat, 1.0, f->ksWeight, f->maxFail, f->safetyFactor, d_es,
+                                         g_stream));
+      KCHECK(tacs_hip_reduce(numElements, d_es, 0, d_red, d_scalar, g_stream));
+      f->ksFailSum = read_scalar(d_scalar);
+      vals[k] = f->getFunctionValue();
+    } else {
+      tacs_set_error("evalFunctions: unsupported function type");
+    }
+  }
+}
+
+// TACSAssembler::addSVSens src/TACSAssembler.cpp:4941 (no BCs applied)
+void TACSAssembler::addSVSens(TacsScalar alpha, TacsScalar beta, TacsScalar gamma, int numFuncs,
+                              TACSFunction** funcs, TACSBVec** dfdu) {
+  ensure(d_ev, (size_t)numElements * 3 * npe);
+  for (int k = 0; k < numFuncs; k++) {
+    if (!funcs[k]) continue;
+    if (dynamic_cast<TACSCompliance*>(funcs[k])) {
+      KCHECK(tacs_hip_solid_elem_vectors(order, 1, numElements, d_conn, d_X, d_u, d_telem, &mat,
+                                         alpha, 0, 0, 1, 1, d_ev, g_stream));
+    } else if (TACSKSFailure* f = dynamic_cast<TACSKSFailure*>(funcs[k])) {
+      KCHECK(tacs_hip_solid_elem_vectors(order, 2, numElements, d_conn, d_X, d_u, d_telem, &mat,
+                                         alpha, f->ksWeight, f->maxFail, f->ksFailSum,
+                                         f->safetyFactor, d_ev, g_stream));
+    } else {
+      tacs_set_error("addSVSens: unsupported function type");
+      continue;
+    }
+    KCHECK(elementVectorsToNodes(d_ev, dfdu[k]->devWrite(), 1));
+  }
+}
+
+// TACSAssembler::addDVSens src/TACSAssembler.cpp:4767
+void TACSAssembler::addDVSens(TacsScalar coef, int numFuncs, TACSFunction** funcs,
+                              TACSBVec** dfdx) {
+  ensure(d_es, numElements);
+  for (int k = 0; k < numFuncs; k++) {
+    if (!funcs[k]) continue;
+    if (dynamic_cast<TACSCompliance*>(funcs[k])) {
+      KCHECK(tacs_hip_solid_elem_scalars(order, 3, numElements, d_conn, d_X, d_u, NULL, d_telem,
+                                         &mat, coef, 0, 0, 1, d_es, g_stream));
+      KCHECK(scatterElementScalarsToDVs(d_es, dfdx[k]->devWrite()));
+    } else if (dynamic_cast<TACSKSFailure*>(funcs[k])) {
+      // The solid failure index does not depend on t
+      // (TACSSolidConstitutive::evalFailure :274, no addFailureDVSens).
+    } else {
+      tacs_set_error("addDVSens: unsupported function type");
+    }
+  }
+}
+
+// TACSAssembler::addAdjointResProducts src/TACSAssembler.cpp:5033
+void TACSAssembler::addAdjointResProducts(TacsScalar scale, int numAdjoints, TACSBVec** adjoint,
+                                          TACSBVec** dfdx, const TacsScalar lambda) {
+  ensure(d_es, numElements);
+  for (int k = 0; k < numAdjoints; k++) {
+    KCHECK(tacs_hip_solid_elem_scalars(order, 4, numElements, d_conn, d_X, d_u,
+                                       adjoint[k]->devRead(), d_telem, &mat, scale, 0, 0, 1, d_es,
+                                       g_stream));
+    KCHECK(scatterElementScalarsToDVs(d_es, dfdx[k]->devWrite()));
+  }
+}

@@ -1,0 +1,755 @@
+// gfx950 kernels for TACSElement3D + TACSLinearElasticity3D (linear strain)
+// + TACSSolidConstitutive on 8-node (order 2) and 27-node (order 3) hexahedra.
+//
+// Jacobian assembly is node-centric: one wavefront owns one block row of the
+// BCSR matrix, recomputes the 3 x (3*NPE) row strips of the incident elements
+// at every quadrature point, accumulates them per column block in LDS in
+// ascending element order (the order in which the reference's element loop
+// adds them, src/TACSAssembler.cpp:4352-4383), applies the Dirichlet rows and
+// writes the row once with coalesced stores.  No atomics, no colouring and no
+// zero-fill of the matrix are needed.
+//
+// Vector-valued element quantities (residual, function state sensitivities,
+// traction loads) are computed per element into an element-vector buffer and
+// gathered per node in ascending element order (the reference's setValues ADD
+// order), again without atomics.
+#include "tacs_hip_common.h"
+
+#define WAVE_SYNC()                                    \
+  do {                                                 \
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); \
+    __builtin_amdgcn_wave_barrier();                   \
+  } while (0)
+
+// Gauss points/weights: the reference's literals
+// (src/elements/basis/TACSGaussQuadrature.h:26-30).
+template <int ORD>
+__device__ __forceinline__ double gauss_pt(int i) {
+  if (ORD == 2) return i == 0 ? -0.577350269189626 : 0.577350269189626;
+  return i == 0 ? -0.774596669241483 : (i == 1 ? 0.0 : 0.774596669241483);
+}
+template <int ORD>
+__device__ __forceinline__ double gauss_wt(int i) {
+  if (ORD == 2) return 1.0;
+  return i == 1 ? 8.0 / 9.0 : 5.0 / 9.0;
+}
+
+// Lagrange functions on the Gauss-Lobatto knots (TACSHexaBasis.cpp).
+template <int ORD>
+__device__ __forceinline__ void lag1d(double x, double* n, double* d) {
+  if (ORD == 2) {
+    n[0] = 0.5 * (1.0 - x);
+    n[1] = 0.5 * (1.0 + x);
+    d[0] = -0.5;
+    d[1] = 0.5;
+  } else {
+    n[0] = -0.5 * x * (1.0 - x);
+    n[1] = (1.0 - x) * (1.0 + x);
+    n[2] = 0.5 * (1.0 + x) * x;
+    d[0] = -0.5 + x;
+    d[1] = -2.0 * x;
+    d[2] = 0.5 + x;
+  }
+}
+
+// 1D values at the three coordinates of quadrature point q.
+template <int ORD>
+struct QP1D {
+  double n[3][ORD], d[3][ORD];
+  double w;
+  __device__ __forceinline__ void eval(int q) {
+    int iq = q % ORD, jq = (q / ORD) % ORD, kq = q / (ORD * ORD);
+    lag1d<ORD>(gauss_pt<ORD>(iq), n[0], d[0]);
+    lag1d<ORD>(gauss_pt<ORD>(jq), n[1], d[1]);
+    lag1d<ORD>(gauss_pt<ORD>(kq), n[2], d[2]);
+    w = gauss_wt<ORD>(iq) * gauss_wt<ORD>(jq) * gauss_wt<ORD>(kq);
+  }
+  __device__ __forceinline__ void basis(int a, double& N, double g[3]) const {
+    int ia = a % ORD, ja = (a / ORD) % ORD, ka = a / (ORD * ORD);
+    N = n[0][ia] * n[1][ja] * n[2][ka];
+    g[0] = d[0][ia] * n[1][ja] * n[2][ka];
+    g[1] = n[0][ia] * d[1][ja] * n[2][ka];
+    g[2] = n[0][ia] * n[1][ja] * d[2][ka];
+  }
+};
+
+// inv3x3 of src/elements/TACSElementAlgebra.h:1980; returns det.
+__device__ __forceinline__ double inv3(const double A[9], double B[9]) {
+  double det = A[8] * (A[0] * A[4] - A[3] * A[1]) -
+               A[7] * (A[0] * A[5] - A[3] * A[2]) +
+               A[6] * (A[1] * A[5] - A[2] * A[4]);
+  double r = 1.0 / det;
+  B[0] = (A[4] * A[8] - A[5] * A[7]) * r;
+  B[1] = -(A[1] * A[8] - A[2] * A[7]) * r;
+  B[2] = (A[1] * A[5] - A[2] * A[4]) * r;
+  B[3] = -(A[3] * A[8] - A[5] * A[6]) * r;
+  B[4] = (A[0] * A[8] - A[2] * A[6]) * r;
+  B[5] = -(A[0] * A[5] - A[2] * A[3]) * r;
+  B[6] = (A[3] * A[7] - A[4] * A[6]) * r;
+  B[7] = -(A[0] * A[7] - A[1] * A[6]) * r;
+  B[8] = (A[0] * A[4] - A[1] * A[3]) * r;
+  return det;
+}
+
+__device__ __forceinline__ int voigt(int i, int j) {
+  return i == j ? i : (i + j == 3 ? 3 : (i + j == 2 ? 4 : 5));
+}
+__device__ __forceinline__ double c21(const SolidMaterial& m, int i, int j) {
+  if (i > j) {
+    int t = i;
+    i = j;
+    j = t;
+  }
+  // packed row offsets of the upper triangle: 0,6,11,15,18,20
+  const int off[6] = {0, 6, 11, 15, 18, 20};
+  return m.C[off[i] + (j - i)];
+}
+
+// s = C e (no thickness scaling), TACSSolidConstitutive::evalStress :142.
+__device__ __forceinline__ void stress6(const SolidMaterial& m, const double e[6],
+                                        double s[6]) {
+  if (m.isotropic) {
+    double tr = e[0] + e[1] + e[2];
+    s[0] = m.lam * tr + 2.0 * m.mu * e[0];
+    s[1] = m.lam * tr + 2.0 * m.mu * e[1];
+    s[2] = m.lam * tr + 2.0 * m.mu * e[2];
+    s[3] = m.mu * e[3];
+    s[4] = m.mu * e[4];
+    s[5] = m.mu * e[5];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      double v = 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; j++) v += c21(m, i, j) * e[j];
+      s[i] = v;
+    }
+  }
+}
+
+// Element geometry at one quadrature point: J = Xd^{-1}, returns detXd.
+template <int ORD>
+__device__ __forceinline__ double elem_geometry(const QP1D<ORD>& qp, const int* __restrict__ c,
+                                                const double* __restrict__ X, double J[9]) {
+  constexpr int NPE = ORD * ORD * ORD;
+  double Xd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int b = 0; b < NPE; b++) {
+    double N, g[3];
+    qp.basis(b, N, g);
+    int nb = c[b];
+    double x0 = X[3 * nb], x1 = X[3 * nb + 1], x2 = X[3 * nb + 2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      Xd[k] += x0 * g[k];
+      Xd[3 + k] += x1 * g[k];
+      Xd[6 + k] += x2 * g[k];
+    }
+  }
+  return inv3(Xd, J);
+}
+
+__device__ __forceinline__ void grad_x(const double J[9], const double g[3], double gx[3]) {
+  gx[0] = g[0] * J[0] + g[1] * J[3] + g[2] * J[6];
+  gx[1] = g[0] * J[1] + g[1] * J[4] + g[2] * J[7];
+  gx[2] = g[0] * J[2] + g[1] * J[5] + g[2] * J[8];
+}
+
+// ---------------------------------------------------------------------------
+// Jacobian: one wavefront per block row.
+// ---------------------------------------------------------------------------
+template <int ORD>
+struct JacCfg;
+template <>
+struct JacCfg<2> {
+  static constexpr int MAXC = 64, WPB = 4;
+};
+template <>
+struct JacCfg<3> {
+  static constexpr int MAXC = 125, WPB = 2;
+};
+
+template <int ORD>
+__global__ __launch_bounds__(64 * JacCfg<ORD>::WPB) void solid_jacobian_rows(
+    int nnodes, const int* __restrict__ n2e_ptr, const int* __restrict__ n2e,
+    const int* __restrict__ conn, const double* __restrict__ X,
+    const double* __restrict__ telem, SolidMaterial mat, double alpha,
+    double gamma, const int* __restrict__ rowp, const int* __restrict__ cols,
+    const int* __restrict__ bcvars, double* __restrict__ A,
+    float* __restrict__ A32, int* __restrict__ status) {
+  constexpr int NPE = ORD * ORD * ORD, NQP = NPE;
+  constexpr int MAXC = JacCfg<ORD>::MAXC, WPB = JacCfg<ORD>::WPB;
+  constexpr int ECH = 8;  // elements per chunk
+  __shared__ double geo[WPB][ECH * NQP][10];
+  __shared__ double rowbuf[WPB][MAXC * 9];
+  __shared__ int colbuf[WPB][MAXC];
+
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  double (*G)[10] = geo[wid];
+  double* R = rowbuf[wid];
+  int* CB = colbuf[wid];
+
+  for (int row = blockIdx.x * WPB + wid; row < nnodes; row += gridDim.x * WPB) {
+    const int rs = rowp[row];
+    const int nc = rowp[row + 1] - rs;
+    if (nc > MAXC) {
+      if (lane == 0) atomicExch(status, TACS_HIP_ERR_ROW_TOO_LONG);
+      continue;
+    }
+    for (int k = lane; k < nc; k += 64) CB[k] = cols[rs + k];
+    for (int k = lane; k < nc * 9; k += 64) R[k] = 0.0;
+    const int es = n2e_ptr[row];
+    const int ne = n2e_ptr[row + 1] - es;
+    WAVE_SYNC();
+
+    for (int c0 = 0; c0 < ne; c0 += ECH) {
+      const int nce = min(ECH, ne - c0);
+      // Geometry of every (element, quadrature point) pair of the chunk.
+      for (int p = lane; p < nce * NQP; p += 64) {
+        int e = p / NQP, q = p - e * NQP;
+        int elem = n2e[es + c0 + e] >> 5;
+        QP1D<ORD> qp;
+        qp.eval(q);
+        double J[9];
+        double det = elem_geometry<ORD>(qp, &conn[(size_t)elem * NPE], X, J);
+#pragma unroll
+        for (int k = 0; k < 9; k++) G[p][k] = J[k];
+        G[p][9] = qp.w * det * telem[elem];
+      }
+      WAVE_SYNC();
+      // (element, column node) pairs: accumulate the 3x3 block over qp.
+      for (int p0 = 0; p0 < nce * NPE; p0 += 64) {
+        int p = p0 + lane;
+        bool valid = p < nce * NPE;
+        int e = valid ? p / NPE : 0;
+        int b = valid ? p - e * NPE : 0;
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        int slot = -1;
+        if (valid) {
+          int packed = n2e[es + c0 + e];
+          int elem = packed >> 5, a = packed & 31;
+          int cn = conn[(size_t)elem * NPE + b];
+          int lo = 0, hi = nc - 1;
+          while (lo <= hi) {
+            int mid = (lo + hi) >> 1;
+            int v = CB[mid];
+            if (v == cn) {
+              slot = mid;
+              break;
+            }
+            if (v < cn) lo = mid + 1;
+            else hi = mid - 1;
+          }
+          for (int q = 0; q < NQP; q++) {
+            QP1D<ORD> qp;
+            qp.eval(q);
+            const double* Jq = G[e * NQP + q];
+            double J[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) J[k] = Jq[k];
+            double wdt = Jq[9];
+            double Na, Nb, ga0[3], gb0[3], ga[3], gb[3];
+            qp.basis(a, Na, ga0);
+            qp.basis(b, Nb, gb0);
+            grad_x(J, ga0, ga);
+            grad_x(J, gb0, gb);
+            double f = alpha * wdt;
+            if (mat.isotropic) {
+              double gab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+#pragma unroll
+              for (int j = 0; j < 3; j++) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                  double v = mat.lam * ga[j] * gb[k] + mat.mu * gb[j] * ga[k];
+                  if (j == k) v += mat.mu * gab;
+                  acc[3 * j + k] += f * v;
+                }
+              }
+            } else {
+#pragma unroll
+              for (int j = 0; j < 3; j++) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                  double v = 0.0;
+#pragma unroll
+                  for (int d = 0; d < 3; d++) {
+#pragma unroll
+                    for (int ee = 0; ee < 3; ee++)
+                      v += c21(mat, voigt(j, d), voigt(k, ee)) * ga[d] * gb[ee];
+                  }
+                  acc[3 * j + k] += f * v;
+                }
+              }
+            }
+            if (gamma != 0.0) {
+              double m = gamma * mat.rho * wdt * Na * Nb;
+              acc[0] += m;
+              acc[4] += m;
+              acc[8] += m;
+            }
+          }
+          if (slot < 0) atomicExch(status, TACS_HIP_ERR_MISSING_ENTRY);
+        }
+        // Add the blocks into the row in ascending element order.
+        int efirst = p0 / NPE;
+        int elast = min(nce * NPE, p0 + 64) - 1;
+        elast = elast / NPE;
+        for (int ee = efirst; ee <= elast; ee++) {
+          if (valid && e == ee && slot >= 0) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) R[slot * 9 + k] += acc[k];
+          }
+          WAVE_SYNC();
+        }
+      }
+    }
+    // Dirichlet rows (BCSRMat::zeroRow with ident = 1, BCSRMat.cpp:2027) and
+    // the coalesced write of the row.
+    const int bc = bcvars ? bcvars[row] : 0;
+    double* Ar = A + (size_t)rs * 9;
+    for (int k = lane; k < nc * 9; k += 64) {
+      int s = k / 9, rc = k - 9 * s, r = rc / 3, cc = rc - 3 * r;
+      double v = R[k];
+      if (bc & (1 << r)) v = (CB[s] == row && r == cc) ? 1.0 : 0.0;
+      if (A) Ar[k] = v;
+      if (A32) A32[(size_t)rs * 9 + k] = (float)v;
+    }
+    WAVE_SYNC();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-element vectors: residual, compliance / KS state sensitivities.
+// One thread per element.
+// ---------------------------------------------------------------------------
+enum { EV_RESIDUAL = 0, EV_COMPLIANCE_SV = 1, EV_KS_SV = 2 };
+
+template <int ORD, int MODE>
+__global__ __launch_bounds__(128) void solid_elem_vectors(
+    int nelem, const int* __restrict__ conn, const double* __restrict__ X,
+    const double* __restrict__ u, const double* __restrict__ telem,
+    SolidMaterial mat, double coef, double ks_weight, double ks_max,
+    double ks_sum, double safety, double* __restrict__ out) {
+  constexpr int NPE = ORD * ORD * ORD, NQP = NPE, NV = 3 * NPE;
+  int el = blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= nelem) return;
+  const int* c = &conn[(size_t)el * NPE];
+  double* o = &out[(size_t)el * NV];
+  double r[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) r[k] = 0.0;
+  const double t = telem ? telem[el] : 1.0;
+  for (int q = 0; q < NQP; q++) {
+    QP1D<ORD> qp;
+    qp.eval(q);
+    double J[9];
+    double det = elem_geometry<ORD>(qp, c, X, J);
+    double Ux[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < NPE; b++) {
+      double N, g0[3], g[3];
+      qp.basis(b, N, g0);
+      grad_x(J, g0, g);
+      int nb = c[b];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        double ub = u[3 * nb + j];
+        Ux[3 * j] += ub * g[0];
+        Ux[3 * j + 1] += ub * g[1];
+        Ux[3 * j + 2] += ub * g[2];
+      }
+    }
+    double e[6] = {Ux[0], Ux[4], Ux[8], Ux[5] + Ux[7], Ux[2] + Ux[6], Ux[1] + Ux[3]};
+    double s[6];
+    stress6(mat, e, s);
+    double f;
+    double sv[6];
+    if (MODE == EV_RESIDUAL) {
+      f = coef * qp.w * det * t;  // coef = 1
+#pragma unroll
+      for (int i = 0; i < 6; i++) sv[i] = s[i];
+    } else if (MODE == EV_COMPLIANCE_SV) {
+      f = 2.0 * coef * qp.w * det * t;  // coef = alpha
+#pragma unroll
+      for (int i = 0; i < 6; i++) sv[i] = s[i];
+    } else {
+      // KS failure: d KS/d e = dfdq * C * dvm/ds (TACSSolidConstitutive::
+      // evalFailureStrainSens :301, vonMisesFailure3DStressSens
+      // TACSMaterialProperties.cpp:449)
+      double vm = sqrt(0.5 * ((s[0] - s[1]) * (s[0] - s[1]) + (s[0] - s[2]) * (s[0] - s[2]) +
+                              (s[1] - s[2]) * (s[1] - s[2]) +
+                              6.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5])));
+      double fail = safety * vm / mat.ys;
+      double dfdq = exp(ks_weight * (fail - ks_max)) / ks_sum * qp.w * det * safety;
+      double ds[6] = {0, 0, 0, 0, 0, 0};
+      if (vm != 0.0) {
+        double fact = 0.5 / (mat.ys * vm);
+        ds[0] = fact * (2.0 * s[0] - s[1] - s[2]);
+        ds[1] = fact * (2.0 * s[1] - s[0] - s[2]);
+        ds[2] = fact * (2.0 * s[2] - s[0] - s[1]);
+        ds[3] = 6.0 * fact * s[3];
+        ds[4] = 6.0 * fact * s[4];
+        ds[5] = 6.0 * fact * s[5];
+      }
+      stress6(mat, ds, sv);
+      f = coef * dfdq;
+    }
+#pragma unroll
+    for (int a = 0; a < NPE; a++) {
+      double N, g0[3], g[3];
+      qp.basis(a, N, g0);
+      grad_x(J, g0, g);
+      r[3 * a] += f * (sv[0] * g[0] + sv[5] * g[1] + sv[4] * g[2]);
+      r[3 * a + 1] += f * (sv[5] * g[0] + sv[1] * g[1] + sv[3] * g[2]);
+      r[3 * a + 2] += f * (sv[4] * g[0] + sv[3] * g[1] + sv[2] * g[2]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; k++) o[k] = r[k];
+}
+
+// Traction aux elements (TACSTraction3D::addResidual, TACSTraction3D.cpp:87):
+// one thread per (aux element); element vector out[aux * NV].
+template <int ORD>
+__global__ void solid_traction_vectors(int naux, const int* __restrict__ aux_elem,
+                                       const int* __restrict__ aux_face,
+                                       const double* __restrict__ aux_trac,
+                                       const int* __restrict__ conn,
+                                       const double* __restrict__ X, double lambda,
+                                       double* __restrict__ out) {
+  constexpr int NPE = ORD * ORD * ORD, NV = 3 * NPE;
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= naux) return;
+  const int* c = &conn[(size_t)aux_elem[q] * NPE];
+  const int face = aux_face[q];
+  const double tr[3] = {aux_trac[3 * q], aux_trac[3 * q + 1], aux_trac[3 * q + 2]};
+  double* o = &out[(size_t)q * NV];
+  for (int k = 0; k < NV; k++) o[k] = 0.0;
+  // tangents ordered so that t1 x t2 is the outward normal
+  const double tang[6][6] = {{0, 0, 1, 0, 1, 0}, {0, 1, 0, 0, 0, 1}, {1, 0, 0, 0, 0, 1},
+                             {0, 0, 1, 1, 0, 0}, {0, 1, 0, 1, 0, 0}, {1, 0, 0, 0, 1, 0}};
+  for (int n = 0; n < ORD * ORD; n++) {
+    int ia = n % ORD, ib = n / ORD;
+    double fixed = -1.0 + 2.0 * (face % 2);
+    double pt[3];
+    if (face / 2 == 0) {
+      pt[0] = fixed; pt[1] = gauss_pt<ORD>(ia); pt[2] = gauss_pt<ORD>(ib);
+    } else if (face / 2 == 1) {
+      pt[0] = gauss_pt<ORD>(ia); pt[1] = fixed; pt[2] = gauss_pt<ORD>(ib);
+    } else {
+      pt[0] = gauss_pt<ORD>(ia); pt[1] = gauss_pt<ORD>(ib); pt[2] = fixed;
+    }
+    double w = gauss_wt<ORD>(ia) * gauss_wt<ORD>(ib);
+    QP1D<ORD> qp;
+    lag1d<ORD>(pt[0], qp.n[0], qp.d[0]);
+    lag1d<ORD>(pt[1], qp.n[1], qp.d[1]);
+    lag1d<ORD>(pt[2], qp.n[2], qp.d[2]);
+    double Xd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < NPE; b++) {
+      double N, g[3];
+      qp.basis(b, N, g);
+      for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) Xd[3 * i + k] += X[3 * c[b] + i] * g[k];
+    }
+    const double* tg = tang[face];
+    double t1[3], t2[3];
+    for (int i = 0; i < 3; i++) {
+      t1[i] = Xd[3 * i] * tg[0] + Xd[3 * i + 1] * tg[1] + Xd[3 * i + 2] * tg[2];
+      t2[i] = Xd[3 * i] * tg[3] + Xd[3 * i + 1] * tg[4] + Xd[3 * i + 2] * tg[5];
+    }
+    double nx = t1[1] * t2[2] - t1[2] * t2[1];
+    double ny = t1[2] * t2[0] - t1[0] * t2[2];
+    double nz = t1[0] * t2[1] - t1[1] * t2[0];
+    double wa = lambda * w * sqrt(nx * nx + ny * ny + nz * nz);
+    for (int a = 0; a < NPE; a++) {
+      double N, g[3];
+      qp.basis(a, N, g);
+      for (int k = 0; k < 3; k++) o[3 * a + k] -= wa * N * tr[k];
+    }
+  }
+}
+
+// Node gather of element vectors in ascending element order:
+// out[3*i + j] (+)= sum_{(e,a) in n2e(i)} ev[e*NV + 3a + j].
+template <int NV>
+__global__ void gather_elem_vectors(int nnodes, const int* __restrict__ n2e_ptr,
+                                    const int* __restrict__ n2e,
+                                    const double* __restrict__ ev, int accumulate,
+                                    double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnodes) return;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (accumulate) {
+    s0 = out[3 * i];
+    s1 = out[3 * i + 1];
+    s2 = out[3 * i + 2];
+  }
+  for (int p = n2e_ptr[i]; p < n2e_ptr[i + 1]; p++) {
+    int packed = n2e[p];
+    const double* v = &ev[(size_t)(packed >> 5) * NV + 3 * (packed & 31)];
+    s0 += v[0];
+    s1 += v[1];
+    s2 += v[2];
+  }
+  out[3 * i] = s0;
+  out[3 * i + 1] = s1;
+  out[3 * i + 2] = s2;
+}
+
+// ---------------------------------------------------------------------------
+// Per-element scalars: compliance integrand, KS max / sum, DV sensitivity
+// (e.Ce) and adjoint-residual product (e(u).C e(psi)).
+// ---------------------------------------------------------------------------
+enum { ES_COMPLIANCE = 0, ES_KS_MAX = 1, ES_KS_SUM = 2, ES_DV_COMPLIANCE = 3, ES_ADJ_PRODUCT = 4 };
+
+template <int ORD, int MODE>
+__global__ __launch_bounds__(128) void solid_elem_scalars(
+    int nelem, const int* __restrict__ conn, const double* __restrict__ X,
+    const double* __restrict__ u, const double* __restrict__ psi,
+    const double* __restrict__ telem, SolidMaterial mat, double scale,
+    double ks_weight, double ks_max, double safety, double* __restrict__ out) {
+  constexpr int NPE = ORD * ORD * ORD, NQP = NPE;
+  int el = blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= nelem) return;
+  const int* c = &conn[(size_t)el * NPE];
+  const double t = telem ? telem[el] : 1.0;
+  double acc = (MODE == ES_KS_MAX) ? -1e20 : 0.0;
+  for (int q = 0; q < NQP; q++) {
+    QP1D<ORD> qp;
+    qp.eval(q);
+    double J[9];
+    double det = elem_geometry<ORD>(qp, c, X, J);
+    double Ux[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double Px[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < NPE; b++) {
+      double N, g0[3], g[3];
+      qp.basis(b, N, g0);
+      grad_x(J, g0, g);
+      int nb = c[b];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        double ub = u[3 * nb + j];
+        Ux[3 * j] += ub * g[0];
+        Ux[3 * j + 1] += ub * g[1];
+        Ux[3 * j + 2] += ub * g[2];
+        if (MODE == ES_ADJ_PRODUCT) {
+          double pb = psi[3 * nb + j];
+          Px[3 * j] += pb * g[0];
+          Px[3 * j + 1] += pb * g[1];
+          Px[3 * j + 2] += pb * g[2];
+        }
+      }
+    }
+    double e[6] = {Ux[0], Ux[4], Ux[8], Ux[5] + Ux[7], Ux[2] + Ux[6], Ux[1] + Ux[3]};
+    double s[6];
+    stress6(mat, e, s);
+    double wd = qp.w * det;
+    if (MODE == ES_COMPLIANCE) {
+      acc += wd * t * (e[0] * s[0] + e[1] * s[1] + e[2] * s[2] + e[3] * s[3] + e[4] * s[4] + e[5] * s[5]);
+    } else if (MODE == ES_DV_COMPLIANCE) {
+      acc += scale * wd * (e[0] * s[0] + e[1] * s[1] + e[2] * s[2] + e[3] * s[3] + e[4] * s[4] + e[5] * s[5]);
+    } else if (MODE == ES_ADJ_PRODUCT) {
+      double ph[6] = {Px[0], Px[4], Px[8], Px[5] + Px[7], Px[2] + Px[6], Px[1] + Px[3]};
+      acc += scale * wd * (ph[0] * s[0] + ph[1] * s[1] + ph[2] * s[2] + ph[3] * s[3] + ph[4] * s[4] + ph[5] * s[5]);
+    } else {
+      double vm = sqrt(0.5 * ((s[0] - s[1]) * (s[0] - s[1]) + (s[0] - s[2]) * (s[0] - s[2]) +
+                              (s[1] - s[2]) * (s[1] - s[2]) +
+                              6.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5])));
+      double fail = safety * vm / mat.ys;
+      if (MODE == ES_KS_MAX) acc = fmax(acc, fail);
+      else acc += wd * exp(ks_weight * (fail - ks_max));
+    }
+  }
+  out[el] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Design variables: t_e = dv[tNum_e] (or the fixed t when tNum_e < 0), and
+// the reverse reduction of per-element DV derivatives into the design vector
+// in ascending element order (dfdx->setValues ADD, TACSAssembler.cpp:4811).
+// ---------------------------------------------------------------------------
+__global__ void gather_design_vars(int nelem, const int* __restrict__ dvnum,
+                                   const double* __restrict__ tfixed,
+                                   const double* __restrict__ dv, double* __restrict__ telem) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  int d = dvnum[e];
+  telem[e] = (d >= 0 && dv) ? dv[d] : tfixed[e];
+}
+
+__global__ void scatter_dv_sens(int ndv, const int* __restrict__ dv2e_ptr,
+                                const int* __restrict__ dv2e, const double* __restrict__ es,
+                                double* __restrict__ dfdx) {
+  int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndv) return;
+  double s = 0.0;
+  for (int p = dv2e_ptr[d]; p < dv2e_ptr[d + 1]; p++) s += es[dv2e[p]];
+  dfdx[d] += s;
+}
+
+// Dirichlet values on a vector (TACSBVec::applyBCs src/bpmat/TACSBVec.cpp:546):
+// constrained entries become u - lambda*value (u given) or 0.
+__global__ void apply_bc_vec(int b, int nbc, const int* __restrict__ nodes,
+                             const int* __restrict__ masks, const double* __restrict__ vals,
+                             const double* __restrict__ u, double lambda, double* __restrict__ x) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nbc) return;
+  int node = nodes[q], m = masks[q];
+  for (int k = 0; k < b; k++) {
+    if (m & (1 << k)) {
+      size_t i = (size_t)b * node + k;
+      x[i] = u ? u[i] - lambda * (vals ? vals[b * q + k] : 0.0) : 0.0;
+    }
+  }
+}
+
+// Host-side launchers (C ABI, raw device pointers, explicit stream).
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int tacs_hip_solid_jacobian(int order, int nnodes, const int* n2e_ptr, const int* n2e,
+                            const int* conn, const double* X, const double* telem,
+                            const SolidMaterial* mat, double alpha, double gamma,
+                            const int* rowp, const int* cols, const int* bcvars,
+                            double* A, float* A32, int* status, hipStream_t stream) {
+  if (order == 2) {
+    constexpr int WPB = JacCfg<2>::WPB;
+    int grid = tacs_grid_for(nnodes, WPB, 1 << 20);
+    hipLaunchKernelGGL(solid_jacobian_rows<2>, dim3(grid), dim3(64 * WPB), 0, stream, nnodes,
+                       n2e_ptr, n2e, conn, X, telem, *mat, alpha, gamma, rowp, cols, bcvars, A,
+                       A32, status);
+  } else if (order == 3) {
+    constexpr int WPB = JacCfg<3>::WPB;
+    int grid = tacs_grid_for(nnodes, WPB, 1 << 20);
+    hipLaunchKernelGGL(solid_jacobian_rows<3>, dim3(grid), dim3(64 * WPB), 0, stream, nnodes,
+                       n2e_ptr, n2e, conn, X, telem, *mat, alpha, gamma, rowp, cols, bcvars, A,
+                       A32, status);
+  } else {
+    return TACS_HIP_ERR_ARG;
+  }
+  return (int)hipGetLastError();
+}
+
+// mode: 0 residual (coef = 1), 1 compliance dfdu (coef = alpha), 2 KS dfdu
+int tacs_hip_solid_elem_vectors(int order, int mode, int nelem, const int* conn,
+                                const double* X, const double* u, const double* telem,
+                                const SolidMaterial* mat, double coef, double ks_weight,
+                                double ks_max, double ks_sum, double safety, double* ev,
+                                hipStream_t stream) {
+  int grid = (nelem + 127) / 128;
+  if (nelem <= 0) return 0;
+#define EV_LAUNCH(O, M)                                                                       \
+  hipLaunchKernelGGL((solid_elem_vectors<O, M>), dim3(grid), dim3(128), 0, stream, nelem, conn, \
+                     X, u, telem, *mat, coef, ks_weight, ks_max, ks_sum, safety, ev)
+  if (order == 2) {
+    if (mode == 0) EV_LAUNCH(2, EV_RESIDUAL);
+    else if (mode == 1) EV_LAUNCH(2, EV_COMPLIANCE_SV);
+    else EV_LAUNCH(2, EV_KS_SV);
+  } else if (order == 3) {
+    if (mode == 0) EV_LAUNCH(3, EV_RESIDUAL);
+    else if (mode == 1) EV_LAUNCH(3, EV_COMPLIANCE_SV);
+    else EV_LAUNCH(3, EV_KS_SV);
+  } else {
+    return TACS_HIP_ERR_ARG;
+  }
+#undef EV_LAUNCH
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_solid_traction_vectors(int order, int naux, const int* aux_elem,
+                                    const int* aux_face, const double* aux_trac,
+                                    const int* conn, const double* X, double lambda,
+                                    double* ev, hipStream_t stream) {
+  if (naux <= 0) return 0;
+  int grid = (naux + 63) / 64;
+  if (order == 2)
+    hipLaunchKernelGGL(solid_traction_vectors<2>, dim3(grid), dim3(64), 0, stream, naux,
+                       aux_elem, aux_face, aux_trac, conn, X, lambda, ev);
+  else if (order == 3)
+    hipLaunchKernelGGL(solid_traction_vectors<3>, dim3(grid), dim3(64), 0, stream, naux,
+                       aux_elem, aux_face, aux_trac, conn, X, lambda, ev);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_gather_elem_vectors(int nv, int nnodes, const int* n2e_ptr, const int* n2e,
+                                 const double* ev, int accumulate, double* out,
+                                 hipStream_t stream) {
+  int grid = (nnodes + 255) / 256;
+  if (nnodes <= 0) return 0;
+  if (nv == 24)
+    hipLaunchKernelGGL(gather_elem_vectors<24>, dim3(grid), dim3(256), 0, stream, nnodes,
+                       n2e_ptr, n2e, ev, accumulate, out);
+  else if (nv == 81)
+    hipLaunchKernelGGL(gather_elem_vectors<81>, dim3(grid), dim3(256), 0, stream, nnodes,
+                       n2e_ptr, n2e, ev, accumulate, out);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+// mode: 0 compliance integrand, 1 KS max, 2 KS sum, 3 compliance dC/dt (e.Ce),
+// 4 adjoint-residual product
+int tacs_hip_solid_elem_scalars(int order, int mode, int nelem, const int* conn,
+                                const double* X, const double* u, const double* psi,
+                                const double* telem, const SolidMaterial* mat, double scale,
+                                double ks_weight, double ks_max, double safety, double* out,
+                                hipStream_t stream) {
+  if (nelem <= 0) return 0;
+  int grid = (nelem + 127) / 128;
+#define ES_LAUNCH(O, M)                                                                        \
+  hipLaunchKernelGGL((solid_elem_scalars<O, M>), dim3(grid), dim3(128), 0, stream, nelem, conn, \
+                     X, u, psi, telem, *mat, scale, ks_weight, ks_max, safety, out)
+  if (order == 2) {
+    switch (mode) {
+      case 0: ES_LAUNCH(2, ES_COMPLIANCE); break;
+      case 1: ES_LAUNCH(2, ES_KS_MAX); break;
+      case 2: ES_LAUNCH(2, ES_KS_SUM); break;
+      case 3: ES_LAUNCH(2, ES_DV_COMPLIANCE); break;
+      case 4: ES_LAUNCH(2, ES_ADJ_PRODUCT); break;
+      default: return TACS_HIP_ERR_ARG;
+    }
+  } else if (order == 3) {
+    switch (mode) {
+      case 0: ES_LAUNCH(3, ES_COMPLIANCE); break;
+      case 1: ES_LAUNCH(3, ES_KS_MAX); break;
+      case 2: ES_LAUNCH(3, ES_KS_SUM); break;
+      case 3: ES_LAUNCH(3, ES_DV_COMPLIANCE); break;
+      case 4: ES_LAUNCH(3, ES_ADJ_PRODUCT); break;
+      default: return TACS_HIP_ERR_ARG;
+    }
+  } else {
+    return TACS_HIP_ERR_ARG;
+  }
+#undef ES_LAUNCH
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_gather_design_vars(int nelem, const int* dvnum, const double* tfixed,
+                                const double* dv, double* telem, hipStream_t stream) {
+  if (nelem <= 0) return 0;
+  hipLaunchKernelGGL(gather_design_vars, dim3((nelem + 255) / 256), dim3(256), 0, stream, nelem,
+                     dvnum, tfixed, dv, telem);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_scatter_dv_sens(int ndv, const int* dv2e_ptr, const int* dv2e, const double* es,
+                             double* dfdx, hipStream_t stream) {
+  if (ndv <= 0) return 0;
+  hipLaunchKernelGGL(scatter_dv_sens, dim3((ndv + 255) / 256), dim3(256), 0, stream, ndv,
+                     dv2e_ptr, dv2e, es, dfdx);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_apply_bc_vec(int b, int nbc, const int* nodes, const int* masks, const double* vals,
+                          const double* u, double lambda, double* x, hipStream_t stream) {
+  if (nbc <= 0) return 0;
+  hipLaunchKernelGGL(apply_bc_vec, dim3((nbc + 255) / 256), dim3(256), 0, stream, b, nbc, nodes,
+                     masks, vals, u, lambda, x);
+  return (int)hipGetLastError();
+}
+
+}  // extern "C"

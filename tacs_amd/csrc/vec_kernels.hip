@@ -1,0 +1,390 @@
+// gfx950 vector kernels for TACSBVec and the Krylov solvers: axpy/scale,
+// deterministic reductions whose results stay in device memory (so Arnoldi
+// steps never wait on the host), the fused modified-Gram-Schmidt step, the
+// classical-Gram-Schmidt multi-dot / update pair, and the Givens update of
+// the Hessenberg column (GMRES::solve, src/bpmat/KSM.cpp:785-958).
+#include "tacs_hip_common.h"
+
+#define RED_BLOCKS 512
+#define RED_THREADS 256
+#define RED_MAXK 32
+
+// Reduction scratch: per-block partials and a completion ticket.  The last
+// block to finish sums the partials in block order (deterministic) and
+// resets the ticket, so the scratch can be reused without a memset.
+struct RedScratch {
+  double partial[RED_BLOCKS * RED_MAXK];
+  unsigned int ticket;
+  unsigned int pad[63];
+};
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  const int nw = blockDim.x >> 6;
+  for (int w = 0; w < nw; w++) t += sh[w];
+  return t;
+}
+
+// Returns true in the (single) last block after all partials are visible.
+__device__ __forceinline__ bool last_block(RedScratch* rs) {
+  __shared__ bool amlast;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int t = atomicAdd(&rs->ticket, 1u);
+    amlast = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (amlast) __threadfence();  // acquire: see every block's partial
+  return amlast;
+}
+
+// out[0] = x.y (or sqrt(x.y) if take_sqrt); optional q -= (*hprev) * wprev
+// before the dot (fused MGS step: ModifiedGramSchmidt KSM.cpp:526).
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void dot_fused(size_t n, const T* __restrict__ wprev,
+                                                         const double* __restrict__ hprev,
+                                                         T* q, const T* w, RedScratch* rs,
+                                                         double* __restrict__ out, int take_sqrt,
+                                                         double* __restrict__ out_inv) {
+  __shared__ double sh[RED_THREADS / 64];
+  double acc = 0.0;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  if (wprev) {
+    const double h = *hprev;
+    if (w == q) {  // q -= h*wprev, then |q|^2
+      for (; i < n; i += stride) {
+        T qi = (T)((double)q[i] - h * (double)wprev[i]);
+        q[i] = qi;
+        acc += (double)qi * (double)qi;
+      }
+    } else {
+      for (; i < n; i += stride) {
+        T qi = (T)((double)q[i] - h * (double)wprev[i]);
+        q[i] = qi;
+        acc += (double)w[i] * (double)qi;
+      }
+    }
+  } else {
+    for (; i < n; i += stride) acc += (double)w[i] * (double)q[i];
+  }
+  double bs = block_sum(acc, sh);
+  if (threadIdx.x == 0) rs->partial[blockIdx.x] = bs;
+  if (last_block(rs)) {
+    double t = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) t += rs->partial[b];
+    t = block_sum(t, sh);
+    if (threadIdx.x == 0) {
+      double r = take_sqrt ? sqrt(t) : t;
+      out[0] = r;
+      if (out_inv) out_inv[0] = 1.0 / r;
+      rs->ticket = 0;
+    }
+  }
+}
+
+// Multi-dot: h[j] = W_j . q for j < k, W_j = W + j*ldw (classical GS).
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void mdot_kernel(size_t n, int k, const T* __restrict__ W,
+                                                           size_t ldw, const T* __restrict__ q,
+                                                           RedScratch* rs,
+                                                           double* __restrict__ h) {
+  __shared__ double sh[RED_THREADS / 64];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (int j0 = 0; j0 < k; j0 += 8) {
+    const int kk = min(8, k - j0);
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+      double qi = (double)q[i];
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (j < kk) acc[j] += (double)W[(size_t)(j0 + j) * ldw + i] * qi;
+    }
+    for (int j = 0; j < kk; j++) {
+      double bs = block_sum(acc[j], sh);
+      if (threadIdx.x == 0) rs->partial[(size_t)(j0 + j) * RED_BLOCKS + blockIdx.x] = bs;
+    }
+  }
+  if (last_block(rs)) {
+    for (int j = 0; j < k; j++) {
+      double t = 0.0;
+      for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+        t += rs->partial[(size_t)j * RED_BLOCKS + b];
+      t = block_sum(t, sh);
+      if (threadIdx.x == 0) h[j] += t;
+    }
+    if (threadIdx.x == 0) rs->ticket = 0;
+  }
+}
+
+// q -= sum_j c[j] W_j  (c in device memory, k <= RED_MAXK)
+template <typename T>
+__global__ void cgs_update(size_t n, int k, const T* __restrict__ W, size_t ldw,
+                           const double* __restrict__ c, const double* __restrict__ c2,
+                           T* __restrict__ q) {
+  __shared__ double cs[RED_MAXK];
+  if (threadIdx.x < k) cs[threadIdx.x] = c[threadIdx.x] - (c2 ? c2[threadIdx.x] : 0.0);
+  __syncthreads();
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double s = (double)q[i];
+    for (int j = 0; j < k; j++) s -= cs[j] * (double)W[(size_t)j * ldw + i];
+    q[i] = (T)s;
+  }
+}
+
+// y = sum_j c[j] W_j (+ y if add)
+template <typename T, typename TY>
+__global__ void lincomb(size_t n, int k, const T* __restrict__ W, size_t ldw,
+                        const double* __restrict__ c, TY* __restrict__ y, int add) {
+  __shared__ double cs[128];
+  for (int j = threadIdx.x; j < k; j += blockDim.x) cs[j] = c[j];
+  __syncthreads();
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double s = add ? (double)y[i] : 0.0;
+    for (int j = 0; j < k; j++) s += cs[j] * (double)W[(size_t)j * ldw + i];
+    y[i] = (TY)s;
+  }
+}
+
+// x *= s, s = *sp (or 1/ *sp if invert) or the host value when sp is null
+template <typename T>
+__global__ void scale_kernel(size_t n, T* __restrict__ x, const double* __restrict__ sp,
+                             double sv, int invert) {
+  double s = sp ? (invert ? 1.0 / sp[0] : sp[0]) : sv;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    x[i] = (T)(s * (double)x[i]);
+}
+
+// y = a*x + b*y
+template <typename TX, typename TY>
+__global__ void axpby_kernel(size_t n, double a, const TX* __restrict__ x, double b,
+                             TY* __restrict__ y) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    y[i] = (TY)(a * (double)x[i] + (b == 0.0 ? 0.0 : b * (double)y[i]));
+}
+
+template <typename T>
+__global__ void fill_kernel(size_t n, T* __restrict__ x, T v) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = v;
+}
+
+// out[0] = sum(x) (op 0) or max(x) (op 1), deterministic.
+__global__ __launch_bounds__(RED_THREADS) void reduce_kernel(size_t n, const double* __restrict__ x,
+                                                             int op, RedScratch* rs,
+                                                             double* __restrict__ out) {
+  __shared__ double sh[RED_THREADS / 64];
+  double acc = op ? -1e300 : 0.0;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    acc = op ? fmax(acc, x[i]) : acc + x[i];
+  double bs;
+  if (op) {
+    acc = wave_max(acc);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = acc;
+    __syncthreads();
+    bs = sh[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) bs = fmax(bs, sh[w]);
+  } else {
+    bs = block_sum(acc, sh);
+  }
+  if (threadIdx.x == 0) rs->partial[blockIdx.x] = bs;
+  if (last_block(rs)) {
+    if (threadIdx.x == 0) {
+      double t = rs->partial[0];
+      for (int b = 1; b < (int)gridDim.x; b++) t = op ? fmax(t, rs->partial[b]) : t + rs->partial[b];
+      out[0] = t;
+      rs->ticket = 0;
+    }
+  }
+}
+
+// Givens update of Hessenberg column i (packed column-major, column i at
+// H + i*(m+1)), as GMRES::solve KSM.cpp:864-895.  st[0] = |res[i+1]|.
+__global__ void givens_kernel(int i, int m, double* __restrict__ H, double* __restrict__ Qc,
+                              double* __restrict__ Qs, double* __restrict__ res,
+                              double* __restrict__ st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double* h = H + (size_t)i * (m + 1);
+  for (int k = 0; k < i; k++) {
+    double h1 = h[k], h2 = h[k + 1];
+    h[k] = h1 * Qc[k] + h2 * Qs[k];
+    h[k + 1] = -h1 * Qs[k] + h2 * Qc[k];
+  }
+  double h1 = h[i], h2 = h[i + 1];
+  double sq = sqrt(h1 * h1 + h2 * h2);
+  Qc[i] = h1 / sq;
+  Qs[i] = h2 / sq;
+  h[i] = h1 * Qc[i] + h2 * Qs[i];
+  h[i + 1] = -h1 * Qs[i] + h2 * Qc[i];
+  h1 = res[i];
+  res[i] = h1 * Qc[i];
+  res[i + 1] = -h1 * Qs[i];
+  st[0] = fabs(res[i + 1]);
+}
+
+// Back-substitution of the upper-triangular system H y = res (KSM.cpp:912-918)
+__global__ void backsolve_kernel(int niters, int m, const double* __restrict__ H,
+                                 double* __restrict__ res) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int i = niters - 1; i >= 0; i--) {
+    for (int j = i + 1; j < niters; j++) res[i] -= H[(size_t)j * (m + 1) + i] * res[j];
+    res[i] /= H[(size_t)i * (m + 1) + i];
+  }
+}
+
+// Copy of a device scalar with sign/scale: dst = s * src
+__global__ void scalar_op(const double* __restrict__ src, double s, double* __restrict__ dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = s * src[0];
+}
+
+static inline int vgrid(size_t n) {
+  size_t g = (n + 255) / 256;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+extern "C" {
+
+size_t tacs_hip_red_scratch_bytes() { return sizeof(RedScratch); }
+
+int tacs_hip_dot(int prec, size_t n, const void* wprev, const double* hprev, void* q,
+                 const void* w, void* scratch, double* out, int take_sqrt, double* out_inv,
+                 hipStream_t stream) {
+  RedScratch* rs = (RedScratch*)scratch;
+  if (prec == 64)
+    hipLaunchKernelGGL(dot_fused<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n,
+                       (const double*)wprev, hprev, (double*)q, (const double*)w, rs, out,
+                       take_sqrt, out_inv);
+  else
+    hipLaunchKernelGGL(dot_fused<float>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n,
+                       (const float*)wprev, hprev, (float*)q, (const float*)w, rs, out,
+                       take_sqrt, out_inv);
+  return (int)hipGetLastError();
+}
+
+// h[0..k) += W^T q  (accumulates; zero h first for a plain multi-dot)
+int tacs_hip_mdot(int prec, size_t n, int k, const void* W, size_t ldw, const void* q,
+                  void* scratch, double* h, hipStream_t stream) {
+  if (k > RED_MAXK) return TACS_HIP_ERR_ARG;
+  RedScratch* rs = (RedScratch*)scratch;
+  if (prec == 64)
+    hipLaunchKernelGGL(mdot_kernel<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, k,
+                       (const double*)W, ldw, (const double*)q, rs, h);
+  else
+    hipLaunchKernelGGL(mdot_kernel<float>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, k,
+                       (const float*)W, ldw, (const float*)q, rs, h);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_cgs_update(int prec, size_t n, int k, const void* W, size_t ldw, const double* c,
+                        const double* c2, void* q, hipStream_t stream) {
+  if (k > RED_MAXK) return TACS_HIP_ERR_ARG;
+  if (prec == 64)
+    hipLaunchKernelGGL(cgs_update<double>, dim3(vgrid(n)), dim3(256), 0, stream, n, k,
+                       (const double*)W, ldw, c, c2, (double*)q);
+  else
+    hipLaunchKernelGGL(cgs_update<float>, dim3(vgrid(n)), dim3(256), 0, stream, n, k,
+                       (const float*)W, ldw, c, c2, (float*)q);
+  return (int)hipGetLastError();
+}
+
+// y (+)= sum_j c[j] W_j ; prec_w: basis precision, prec_y: output precision
+int tacs_hip_lincomb(int prec_w, int prec_y, size_t n, int k, const void* W, size_t ldw,
+                     const double* c, void* y, int add, hipStream_t stream) {
+  if (k > 128) return TACS_HIP_ERR_ARG;
+  if (prec_w == 64 && prec_y == 64)
+    hipLaunchKernelGGL((lincomb<double, double>), dim3(vgrid(n)), dim3(256), 0, stream, n, k,
+                       (const double*)W, ldw, c, (double*)y, add);
+  else if (prec_w == 32 && prec_y == 32)
+    hipLaunchKernelGGL((lincomb<float, float>), dim3(vgrid(n)), dim3(256), 0, stream, n, k,
+                       (const float*)W, ldw, c, (float*)y, add);
+  else if (prec_w == 32 && prec_y == 64)
+    hipLaunchKernelGGL((lincomb<float, double>), dim3(vgrid(n)), dim3(256), 0, stream, n, k,
+                       (const float*)W, ldw, c, (double*)y, add);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_scale(int prec, size_t n, void* x, const double* sp, double sv, int invert,
+                   hipStream_t stream) {
+  if (n == 0) return 0;
+  if (prec == 64)
+    hipLaunchKernelGGL(scale_kernel<double>, dim3(vgrid(n)), dim3(256), 0, stream, n, (double*)x,
+                       sp, sv, invert);
+  else
+    hipLaunchKernelGGL(scale_kernel<float>, dim3(vgrid(n)), dim3(256), 0, stream, n, (float*)x,
+                       sp, sv, invert);
+  return (int)hipGetLastError();
+}
+
+// y = a*x + b*y with precisions px (of x) and py (of y)
+int tacs_hip_axpby(int px, int py, size_t n, double a, const void* x, double b, void* y,
+                   hipStream_t stream) {
+  if (n == 0) return 0;
+  if (px == 64 && py == 64)
+    hipLaunchKernelGGL((axpby_kernel<double, double>), dim3(vgrid(n)), dim3(256), 0, stream, n, a,
+                       (const double*)x, b, (double*)y);
+  else if (px == 32 && py == 32)
+    hipLaunchKernelGGL((axpby_kernel<float, float>), dim3(vgrid(n)), dim3(256), 0, stream, n, a,
+                       (const float*)x, b, (float*)y);
+  else if (px == 64 && py == 32)
+    hipLaunchKernelGGL((axpby_kernel<double, float>), dim3(vgrid(n)), dim3(256), 0, stream, n, a,
+                       (const double*)x, b, (float*)y);
+  else
+    hipLaunchKernelGGL((axpby_kernel<float, double>), dim3(vgrid(n)), dim3(256), 0, stream, n, a,
+                       (const float*)x, b, (double*)y);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_fill(int prec, size_t n, void* x, double v, hipStream_t stream) {
+  if (n == 0) return 0;
+  if (prec == 64)
+    hipLaunchKernelGGL(fill_kernel<double>, dim3(vgrid(n)), dim3(256), 0, stream, n, (double*)x,
+                       v);
+  else if (prec == 32)
+    hipLaunchKernelGGL(fill_kernel<float>, dim3(vgrid(n)), dim3(256), 0, stream, n, (float*)x,
+                       (float)v);
+  else
+    hipLaunchKernelGGL(fill_kernel<int>, dim3(vgrid(n)), dim3(256), 0, stream, n, (int*)x, (int)v);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_reduce(size_t n, const double* x, int op, void* scratch, double* out,
+                    hipStream_t stream) {
+  hipLaunchKernelGGL(reduce_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, x, op,
+                     (RedScratch*)scratch, out);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_givens(int i, int m, double* H, double* Qc, double* Qs, double* res, double* st,
+                    hipStream_t stream) {
+  hipLaunchKernelGGL(givens_kernel, dim3(1), dim3(64), 0, stream, i, m, H, Qc, Qs, res, st);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_backsolve(int niters, int m, const double* H, double* res, hipStream_t stream) {
+  hipLaunchKernelGGL(backsolve_kernel, dim3(1), dim3(64), 0, stream, niters, m, H, res);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_scalar_op(const double* src, double s, double* dst, hipStream_t stream) {
+  hipLaunchKernelGGL(scalar_op, dim3(1), dim3(64), 0, stream, src, s, dst);
+  return (int)hipGetLastError();
+}
+
+}  // extern "C"

@@ -1,0 +1,55 @@
+"""Mirror of tacs.elements for the solid path (tacs/elements.pyx:
+LinearHexaBasis :297, QuadraticHexaBasis :308, LinearElasticity3D :498,
+Element3D :564, Traction3D :597)."""
+import ctypes
+
+from ._lib import lib
+from ._obj import TACSHandle
+
+LINEAR = 0
+NONLINEAR = 1
+
+
+class LinearHexaBasis(TACSHandle):
+    order = 2
+
+    def __init__(self):
+        super().__init__(lib().tacs_linear_hexa_basis_create())
+
+
+class QuadraticHexaBasis(TACSHandle):
+    order = 3
+
+    def __init__(self):
+        super().__init__(lib().tacs_quadratic_hexa_basis_create())
+
+
+class LinearElasticity3D(TACSHandle):
+    def __init__(self, con, elem_type=LINEAR):
+        super().__init__(lib().tacs_linear_elasticity3d_create(con.ptr, elem_type), keep=(con,))
+
+    def getVarsPerNode(self):
+        return 3
+
+
+class Element3D(TACSHandle):
+    def __init__(self, model, basis):
+        self.basis = basis
+        super().__init__(lib().tacs_element3d_create(model.ptr, basis.ptr), keep=(model, basis))
+
+    def getVarsPerNode(self):
+        return 3
+
+    def getNumNodes(self):
+        return self.basis.order ** 3
+
+    def createElementTraction(self, faceIndex, trac):
+        return Traction3D(3, faceIndex, self.basis, trac)
+
+
+class Traction3D(TACSHandle):
+    def __init__(self, varsPerNode, faceIndex, basis, traction, coordComp=True):
+        arr = (ctypes.c_double * 24)(*([float(v) for v in traction][:24]
+                                       + [0.0] * max(0, 24 - len(traction))))
+        super().__init__(lib().tacs_traction3d_create(varsPerNode, faceIndex, basis.ptr, arr,
+                                                      1 if coordComp else 0), keep=(basis,))
