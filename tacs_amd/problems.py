@@ -1,0 +1,97 @@
+"""Builders for the structured solid problems of the benchmark and tests, using
+only the reference-style API (TACS.Assembler, elements, constitutive,
+functions), as a pyTACS user script would (cf.
+tests/integration_tests/test_elast_linhexa_element_3d.py:62-120)."""
+import numpy as np
+
+from . import TACS, constitutive, elements, functions
+from .meshes import HexMesh
+
+
+class SolidProblem:
+    """A clamped box of hexahedra with a traction on the x = L face.
+
+    dv: None (fixed t), "shared" (one DV for every element, tNum = 0) or
+    "element" (one DV per element, tNum = element index).
+    traction_patch: None (whole x = L face) or n (the centred n x n patch).
+    """
+
+    def __init__(self, n, order=2, L=1.0, E=70e9, nu=0.3, rho=2700.0, ys=270e6,
+                 traction=(0.0, 0.0, 1e6), t0=1.0, dv=None, traction_patch=None, shape=None):
+        nx, ny, nz = shape if shape is not None else (n, n, n)
+        self.mesh = m = HexMesh(nx, ny, nz, order=order, L=(L * nx / nz, L * ny / nz, L)
+                                if shape is not None else (L, L, L))
+        self.order = order
+        self.props = constitutive.MaterialProperties(rho=rho, E=E, nu=nu, ys=ys)
+        self.basis = elements.LinearHexaBasis() if order == 2 else elements.QuadraticHexaBasis()
+        a = self.assembler = TACS.Assembler.create(None, 3, m.nnodes, m.nelem)
+        ptr = np.arange(0, m.npe * m.nelem + 1, m.npe, dtype=np.intc)
+        a.setElementConnectivity(ptr, m.conn.reshape(-1))
+        if dv == "element":
+            elems = []
+            for e in range(m.nelem):
+                con = constitutive.SolidConstitutive(self.props, t=t0, tNum=e)
+                elems.append(elements.Element3D(elements.LinearElasticity3D(con), self.basis))
+        else:
+            con = constitutive.SolidConstitutive(self.props, t=t0, tNum=0 if dv == "shared" else -1)
+            elem = elements.Element3D(elements.LinearElasticity3D(con), self.basis)
+            elems = [elem] * m.nelem
+        a.setElements(elems)
+        self.bc_nodes = m.nodes_on_x(0)
+        a.addBCs(self.bc_nodes)
+        a.initialize()
+        self.X = a.createNodeVec()
+        self.X.getArray()[:] = m.X.reshape(-1)
+        a.setNodes(self.X)
+        # traction aux elements on the x = L face (face index 1 = +x)
+        if traction_patch is None:
+            self.aux_elems, self.aux_faces = m.elems_on_x(1)
+        else:
+            self.aux_elems, self.aux_faces = m.elems_on_x1_patch(traction_patch)
+        self.traction = np.asarray(traction, dtype=float)
+        self.aux = TACS.AuxElements()
+        tr = elements.Traction3D(3, 1, self.basis, list(self.traction))
+        for e in self.aux_elems:
+            self.aux.addElement(int(e), tr)
+        a.setAuxElements(self.aux)
+        self.res = a.createVec()
+        self.u = a.createVec()
+        self.mat = a.createMat()
+
+    def set_thickness(self, t):
+        """Design vector values (one per DV)."""
+        x = self.assembler.createDesignVec()
+        x.getArray()[:] = t
+        self.assembler.setDesignVars(x)
+        return x
+
+    def assemble(self):
+        a = self.assembler
+        a.zeroVariables()
+        a.assembleJacobian(1.0, 0.0, 0.0, self.res, self.mat)
+
+    def solve_fp64(self, m=30, restarts=500, rtol=1e-8, ortho=TACS.MODIFIED_GRAM_SCHMIDT):
+        pc = TACS.Pc(self.mat)
+        pc.factor()
+        ksm = TACS.KSM(self.mat, pc, m, restarts)
+        ksm.setTolerances(rtol, 1e-30)
+        if ortho != TACS.MODIFIED_GRAM_SCHMIDT:
+            ksm.setOrthoType(ortho)
+        flag = ksm.solve(self.res, self.u)
+        self.u.scale(-1.0)
+        self.assembler.setVariables(self.u)
+        return flag, ksm.getIterCount(), pc, ksm
+
+    def solve_mixed(self, m=30, restarts=100, rtol=1e-8, inner_rtol=1e-4, max_outer=20):
+        pc = TACS.Pc(self.mat)
+        pc.factor()
+        ksm = TACS.MixedKSM(self.mat, pc, m, restarts, inner_rtol, max_outer)
+        ksm.setTolerances(rtol, 1e-30)
+        flag = ksm.solve(self.res, self.u)
+        self.u.scale(-1.0)
+        self.assembler.setVariables(self.u)
+        return flag, ksm.getIterCount(), pc, ksm
+
+    def compliance(self):
+        f = functions.Compliance(self.assembler)
+        return self.assembler.evalFunctions([f])[0], f

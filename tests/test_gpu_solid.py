@@ -1,0 +1,164 @@
+"""GPU parity of the solid assembly / solve / adjoint path against the oracle
+(CPU restatement of the reference) on the same inputs."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tacs_amd import TACS, functions
+from tacs_amd.problems import SolidProblem
+
+pytestmark = pytest.mark.gpu
+
+# FP64 parity bar: values within 1e-10 relative (normwise where entries
+# cancel), sparsity pattern and BC rows bit-exact.
+RTOL = 1e-10
+
+
+def _oracle_system(p, t_elem=None, apply_bcs=True):
+    m = p.mesh
+    C = O.iso_C21(p.props.props["E"], p.props.props["nu"])
+    rowp, cols = O.node_csr(m.nnodes, m.conn, m.npe)
+    t = np.ones(m.nelem) if t_elem is None else t_elem
+    A, res = O.assemble_solid(p.order, m.conn, m.X, np.zeros(3 * m.nnodes), t, C, rowp, cols)
+    trac = np.tile(p.traction, (p.aux_elems.size, 1))
+    O.add_tractions(p.order, m.conn, m.X, p.aux_elems, p.aux_faces, trac, res)
+    bcv = np.full(p.bc_nodes.size, 7, np.int32)
+    if apply_bcs:
+        O.apply_bc_mat(3, p.bc_nodes, bcv, rowp, cols, A)
+        O.apply_bc_vec(3, p.bc_nodes, bcv, res, u=np.zeros(3 * m.nnodes))
+    return C, t, rowp, cols, A, res
+
+
+def _dense(rowp, cols, A, b=3):
+    n = rowp.size - 1
+    K = np.zeros((b * n, b * n))
+    Ab = A.reshape(-1, b, b)
+    for i in range(n):
+        for j in range(rowp[i], rowp[i + 1]):
+            K[b * i:b * i + b, b * cols[j]:b * cols[j] + b] = Ab[j]
+    return K
+
+
+@pytest.mark.parametrize("n,order", [(4, 2), (2, 3)])
+def test_jacobian_and_residual_match_oracle(n, order):
+    p = SolidProblem(n, order=order)
+    p.assemble()
+    rowp, cols, A = p.mat.getArrays()
+    C, t, orp, ocols, oA, ores = _oracle_system(p)
+    assert np.array_equal(rowp, orp) and np.array_equal(cols, ocols)
+    scale = np.abs(oA).max()
+    assert np.abs(A.reshape(-1) - oA).max() <= RTOL * scale
+    # BC rows are exact: zero rows with a unit diagonal
+    bc = np.zeros(rowp.size - 1, bool)
+    bc[p.bc_nodes] = True
+    Ab = A.reshape(-1, 3, 3)
+    for i in np.nonzero(bc)[0][:10]:
+        for j in range(rowp[i], rowp[i + 1]):
+            expect = np.eye(3) if cols[j] == i else np.zeros((3, 3))
+            assert np.array_equal(Ab[j], expect)
+    r = p.res.getArray()
+    assert np.abs(r - ores).max() <= RTOL * np.abs(ores).max()
+
+
+def test_spmv_matches_oracle():
+    p = SolidProblem(6)
+    p.assemble()
+    rowp, cols, A = p.mat.getArrays()
+    x = p.assembler.createVec()
+    rng = np.random.default_rng(0)
+    xv = rng.standard_normal(x.getSize())
+    x.getArray()[:] = xv
+    y = p.assembler.createVec()
+    p.mat.mult(x, y)
+    yo = O.bsr_mult(3, rowp, cols, A.reshape(-1), xv)
+    assert np.abs(y.getArray() - yo).max() <= 1e-13 * np.abs(A).max() * np.abs(xv).sum() / xv.size * 30
+
+
+def test_golden_cube_compliance_and_ks():
+    """tests/integration_tests/test_elast_linhexa_element_3d.py:11-27: 3x3x3
+    cube of side 10, rho=2570, E=70e9, nu=0.3, ys=350e6, x=0 clamped,
+    nodal forces 1e8 on every dof; compliance and KS (ksWeight 10)."""
+    p = SolidProblem(3, L=10.0, E=70e9, nu=0.3, rho=2570.0, ys=350e6, traction=(0, 0, 0))
+    a = p.assembler
+    a.zeroVariables()
+    a.assembleJacobian(1.0, 0.0, 0.0, p.res, p.mat)
+    f = a.createVec()
+    f.getArray()[:] = 1e8
+    a.applyBCs(f)
+    p.res.axpy(-1.0, f)
+    pc = TACS.Pc(p.mat)
+    pc.factor()
+    ksm = TACS.KSM(p.mat, pc, 100, 2)
+    ksm.setTolerances(1e-12, 1e-30)
+    assert ksm.solve(p.res, p.u) == 1
+    p.u.scale(-1.0)
+    a.setVariables(p.u)
+    funcs = [functions.KSFailure(a, ksWeight=10.0), functions.Compliance(a)]
+    vals = a.evalFunctions(funcs)
+    assert abs(vals[1] - 178995399.11928475) <= 1e-9 * 178995399.11928475
+    assert abs(vals[0] - 1.1074017881777185) <= 1e-9
+
+
+@pytest.mark.parametrize("solver", ["fp64", "fp64_cgs", "mixed"])
+def test_solve_functions_and_adjoint(solver):
+    n = 6
+    p = SolidProblem(n, dv="element")
+    rng = np.random.default_rng(1)
+    t = rng.uniform(0.1, 1.0, p.mesh.nelem)
+    p.set_thickness(t)
+    p.assemble()
+    if solver == "mixed":
+        flag, its, pc, ksm = p.solve_mixed(rtol=1e-10)
+    else:
+        ortho = TACS.CLASSICAL_GRAM_SCHMIDT if solver == "fp64_cgs" else TACS.MODIFIED_GRAM_SCHMIDT
+        flag, its, pc, ksm = p.solve_fp64(m=30, rtol=1e-10, ortho=ortho)
+    assert flag == 1
+    C, tt, rowp, cols, A, res = _oracle_system(p, t)
+    K = _dense(rowp, cols, A)
+    u = p.u.getArray().copy()
+    # the solver's own stopping test, evaluated with the oracle matrix in FP64
+    assert np.linalg.norm(K @ u + res) <= 1.05e-10 * np.linalg.norm(res)
+    # reference solution: dense solve of the system with the Dirichlet dofs
+    # removed (the unit BC rows make the full matrix badly scaled)
+    free = np.ones(u.size, bool)
+    for nd in p.bc_nodes:
+        free[3 * nd:3 * nd + 3] = False
+    u_ref = np.zeros_like(u)
+    u_ref[free] = -np.linalg.solve(K[np.ix_(free, free)], res[free])
+    assert np.all(u[~free] == 0.0)
+    assert np.abs(u - u_ref).max() <= 1e-6 * np.abs(u_ref).max()
+    a = p.assembler
+    comp = functions.Compliance(a)
+    ks = functions.KSFailure(a, ksWeight=100.0)
+    vals = a.evalFunctions([comp, ks])
+    # function kernels at the same u: FP64 parity
+    c_gpu_u = O.compliance(2, p.mesh.conn, p.mesh.X, u, t, C)
+    ks_gpu_u = O.ks_failure(2, p.mesh.conn, p.mesh.X, u, C, 270e6, 100.0)
+    assert abs(vals[0] - c_gpu_u) <= RTOL * abs(c_gpu_u)
+    assert abs(vals[1] - ks_gpu_u[0]) <= RTOL * abs(ks_gpu_u[0])
+    # and at the reference solution, to the solve's accuracy
+    c_ref = O.compliance(2, p.mesh.conn, p.mesh.X, u_ref, t, C)
+    assert abs(vals[0] - c_ref) <= 1e-8 * abs(c_ref)
+    # state sensitivities at the GPU solution vs the oracle at the same u
+    dfdu = [a.createVec(), a.createVec()]
+    a.addSVSens([comp, ks], dfdu)
+    g0 = O.compliance_svsens(2, p.mesh.conn, p.mesh.X, u, t, C)
+    kso = O.ks_failure(2, p.mesh.conn, p.mesh.X, u, C, 270e6, 100.0)
+    g1 = O.ks_svsens(2, p.mesh.conn, p.mesh.X, u, C, 270e6, 100.0, kso[1], kso[2])
+    assert np.abs(dfdu[0].getArray() - g0).max() <= RTOL * np.abs(g0).max()
+    assert np.abs(dfdu[1].getArray() - g1).max() <= 1e-9 * np.abs(g1).max()
+    # adjoint of the compliance: K psi = dC/du, total dC/dt = dC/dt - psi^T dR/dt
+    a.applyBCs(dfdu[0])
+    psi = a.createVec()
+    ksm.solve(dfdu[0], psi)
+    dfdx = a.createDesignVec()
+    a.addDVSens([comp], [dfdx], 1.0)
+    a.addAdjointResProducts([psi], [dfdx], -1.0)
+    grad = dfdx.getArray().copy()
+    psi_v = psi.getArray().copy()
+    g_ref = O.compliance_dvsens(2, p.mesh.conn, p.mesh.X, u, C) - \
+        O.adj_res_product(2, p.mesh.conn, p.mesh.X, u, psi_v, C)
+    assert np.abs(grad - g_ref).max() <= 1e-9 * np.abs(g_ref).max()
+    # closed form: dC/dt_e = -u_e^T K0_e u_e  (< 0)
+    closed = -O.compliance_dvsens(2, p.mesh.conn, p.mesh.X, u_ref, C)
+    assert np.abs(grad - closed).max() <= 1e-6 * np.abs(closed).max()
