@@ -1,0 +1,320 @@
+"""Benchmark of the FE assembly / solve / adjoint path on one MI355X.
+
+Workload (the headline configuration): a 128 x 128 x 128 cube of 8-node
+linear hexahedra (2,097,152 elements, 2,146,689 nodes, 6,440,067 DOF, 3x3
+block-CSR), E = 70e9, nu = 0.3, rho = 2700, x = 0 clamped, traction
+(0, 0, 1e6) Pa on x = 1, one thickness design variable per element.
+
+One step = the whole hot path on that input: assembleJacobian (FP64 values
+plus the FP32 copy) + assembleRes, the mixed-precision GMRES solve to 1e-8
+relative residual (FP32 Arnoldi cycles with block-Jacobi, FP64 restart
+residuals), compliance, addSVSens, the adjoint solve (same solver), addDVSens
+and addAdjointResProducts.  value = elements / step time (Melem/s).
+
+The plain FP64 GPU path (GMRES(m) with modified Gram-Schmidt, the reference's
+default, block-Jacobi, all FP64) runs the same step once outside the timed
+region for the mixed-vs-FP64 comparison and the gradient error.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """HIP events on the library's stream (the null stream)."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float),
+                                                 ctypes.c_void_p, ctypes.c_void_p]
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e):
+        assert self.hip.hipEventRecord(e, ctypes.c_void_p(0)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b) == 0
+        return ms.value
+
+
+def spmv_bytes(b, nrows, nnzb, word):
+    """Algorithmic bytes of y = A x: values, column indices, row pointers, x
+    and y once each."""
+    return word * b * b * nnzb + 4 * nnzb + 4 * (nrows + 1) + 2 * word * b * nrows
+
+
+def time_spmv(L, ev, p, reps=50):
+    """Average duration of the FP64 and FP32 block-SpMV kernels (HIP events)."""
+    rp, cp, ap = p.mat.getDeviceArrays()
+    b, nr, nnz = p.mat.getInfo()
+    x = p.assembler.createVec()
+    x.set(1.0)
+    y = p.assembler.createVec()
+    xd, yd = x.getDevicePointer(), y.getDevicePointer()
+    res = {}
+    for _ in range(5):
+        L.tacs_hip_bsr_spmv(3, nr, rp, cp, ap, xd, yd, 0, None)
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0)
+    for _ in range(reps):
+        L.tacs_hip_bsr_spmv(3, nr, rp, cp, ap, xd, yd, 0, None)
+    ev.record(e1)
+    res["fp64_ms"] = ev.elapsed_ms(e0, e1) / reps
+    # FP32 copy of the values, FP32 vectors
+    import tacs_amd.TACS as T  # noqa
+    n = 3 * nr
+    hip = ev.hip
+    a32 = ctypes.c_void_p()
+    x32 = ctypes.c_void_p()
+    y32 = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a32), ctypes.c_size_t(4 * 9 * nnz)) == 0
+    assert hip.hipMalloc(ctypes.byref(x32), ctypes.c_size_t(4 * n)) == 0
+    assert hip.hipMalloc(ctypes.byref(y32), ctypes.c_size_t(4 * n)) == 0
+    L.tacs_hip_convert_f64_f32(9 * nnz, ap, a32, None)
+    L.tacs_hip_convert_f64_f32(n, xd, x32, None)
+    for _ in range(5):
+        L.tacs_hip_bsr_spmv_f32(3, nr, rp, cp, a32, x32, y32, 0, None)
+    ev.record(e0)
+    for _ in range(reps):
+        L.tacs_hip_bsr_spmv_f32(3, nr, rp, cp, a32, x32, y32, 0, None)
+    ev.record(e1)
+    res["fp32_ms"] = ev.elapsed_ms(e0, e1) / reps
+    for q in (a32, x32, y32):
+        hip.hipFree(q)
+    res["fp64_bytes"] = spmv_bytes(3, nr, nnz, 8)
+    res["fp32_bytes"] = spmv_bytes(3, nr, nnz, 4)
+    return res
+
+
+def make_solver(p, solver, m, rtol):
+    from tacs_amd import TACS
+    pc = TACS.Pc(p.mat)
+    if solver == "mixed":
+        ksm = TACS.MixedKSM(p.mat, pc, m, 100000)
+    else:
+        ksm = TACS.KSM(p.mat, pc, m, 100000)
+    ksm.setTolerances(rtol, 1e-30)
+    return pc, ksm
+
+
+def run_step(p, pc, ksm, want_grad=False):
+    """One pass of the hot path; returns timings and results."""
+    from tacs_amd import functions, lib
+    L = lib()
+    a = p.assembler
+    t = {}
+    t0 = time.perf_counter()
+    a.zeroVariables()
+    a.assembleJacobian(1.0, 0.0, 0.0, p.res, p.mat)
+    L.tacs_device_synchronize()
+    t["assembly_s"] = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    pc.factor()
+    flag_f = ksm.solve(p.res, p.u)
+    its_f = ksm.getIterCount()
+    p.u.scale(-1.0)
+    a.setVariables(p.u)
+    comp = functions.Compliance(a)
+    c = a.evalFunctions([comp])[0]
+    dfdu = a.createVec()
+    a.addSVSens([comp], [dfdu])
+    a.applyBCs(dfdu)
+    psi = a.createVec()
+    flag_a = ksm.solve(dfdu, psi)
+    its_a = ksm.getIterCount()
+    dfdx = a.createDesignVec()
+    a.addDVSens([comp], [dfdx], 1.0)
+    a.addAdjointResProducts([psi], [dfdx], -1.0)
+    L.tacs_device_synchronize()
+    t["solve_adjoint_s"] = time.perf_counter() - t1
+    t["step_s"] = time.perf_counter() - t0
+    out = dict(t=t, compliance=c, its=(its_f, its_a), flags=(flag_f, flag_a))
+    if want_grad:
+        out["grad"] = dfdx.getArray().copy()
+        out["u"] = p.u.getArray().copy()
+    return out
+
+
+def cpu_baseline(p, gpu_its, budget_s=25.0):
+    """The oracle (CPU restatement of the reference path, one core) timed on a
+    bounded sample of the same 128^3 workload: the assembly of a slice of the
+    elements and the first Arnoldi steps of GMRES(30) on the full matrix;
+    scaled to the whole step with the GPU FP64 run's iteration count."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    m = p.mesh
+    C = O.iso_C21(70e9, 0.3)
+    t0 = time.perf_counter()
+    rowp, cols, A = p.mat.getArrays()
+    t_setup = time.perf_counter() - t0
+    nsample = max(1, m.nelem // 64)
+    conn_s = np.ascontiguousarray(m.conn[:nsample])
+    Xs = m.X.reshape(-1)
+    t0 = time.perf_counter()
+    O.assemble_solid(2, conn_s, Xs, np.zeros(3 * m.nnodes), np.ones(nsample), C, rowp, cols)
+    t_asm = (time.perf_counter() - t0) * (m.nelem / nsample)
+    Aflat = A.reshape(-1)
+    D = O.bsr_diag_inv(3, rowp, cols, Aflat)
+    rhs = -p.res.getArray().copy()
+    kits = 8
+    t0 = time.perf_counter()
+    O.gmres(3, rowp, cols, Aflat, rhs, D, m=30, nrestart=0, rtol=1e-30, max_its=kits)
+    t_it = (time.perf_counter() - t0) / kits
+    its = sum(gpu_its)
+    step = t_asm + its * t_it
+    return {"value": m.nelem / step / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle (C restatement, 1 core) on the 128^3 workload: assembly of "
+                       f"{nsample} of {m.nelem} elements ({t_asm:.1f} s scaled), first {kits} "
+                       f"Arnoldi steps of FP64 GMRES(30)+block-Jacobi on the full matrix "
+                       f"({t_it:.2f} s/step), scaled to the {its} steps the GPU FP64 path took "
+                       f"(later steps orthogonalise against more vectors, so this "
+                       f"underestimates the CPU time)"),
+            "step_s_estimate": step}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--m", type=int, default=30)
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="assemble and time the SpMV kernels only (for profiler runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
+              "described in DESIGN.md and not built", file=sys.stderr)
+        sys.exit(2)
+
+    from tacs_amd import lib
+    from tacs_amd.problems import SolidProblem
+    L = lib()
+    ev = HipEvents()
+    t0 = time.perf_counter()
+    p = SolidProblem(args.n, dv="element")
+    log(f"setup {time.perf_counter() - t0:.1f}s: {p.mesh.nelem} elements, {3 * p.mesh.nnodes} DOF")
+
+    if args.kernels_only:
+        p.assemble()
+        ev = HipEvents()
+        sp = time_spmv(L, ev, p)
+        e0, e1 = ev.event(), ev.event()
+        ev.record(e0)
+        for _ in range(3):
+            p.assembler.assembleJacobian(1.0, 0.0, 0.0, None, p.mat)
+        ev.record(e1)
+        sp["assembly_ms"] = ev.elapsed_ms(e0, e1) / 3
+        sp["fp64_GBps"] = sp["fp64_bytes"] / sp["fp64_ms"] / 1e6
+        sp["fp32_GBps"] = sp["fp32_bytes"] / sp["fp32_ms"] / 1e6
+        print(json.dumps(sp))
+        return
+    pc, ksm = make_solver(p, "mixed", args.m, args.rtol)
+    for w in range(args.warmup):
+        r = run_step(p, pc, ksm)
+        log(f"warmup {w}: {r['t']} its={r['its']}")
+    L.tacs_device_synchronize()
+    t0 = time.perf_counter()
+    steps = []
+    for k in range(args.steps):
+        steps.append(run_step(p, pc, ksm, want_grad=(k == args.steps - 1)))
+        log(f"step {k}: {steps[-1]['t']} its={steps[-1]['its']} flags={steps[-1]['flags']}")
+    L.tacs_device_synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    mixed = steps[-1]
+
+    # assembly kernel timing and SpMV roofline (HIP events)
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0)
+    for _ in range(3):
+        p.assembler.assembleJacobian(1.0, 0.0, 0.0, None, p.mat)
+    ev.record(e1)
+    asm_ms = ev.elapsed_ms(e0, e1) / 3
+    sp = time_spmv(L, ev, p)
+    peak_bw = 8000.0
+    bw64 = sp["fp64_bytes"] / sp["fp64_ms"] / 1e6
+    bw32 = sp["fp32_bytes"] / sp["fp32_ms"] / 1e6
+    jac_flops = 69120.0 * p.mesh.nelem  # dense B^T C B count per 8-node element
+    jac_tf = jac_flops / (asm_ms * 1e-3) / 1e12
+
+    fp64 = None
+    if not args.no_fp64:
+        del pc, ksm
+        pc64, ksm64 = make_solver(p, "fp64", args.m, args.rtol)
+        fp64 = run_step(p, pc64, ksm64, want_grad=True)
+        log(f"fp64 step: {fp64['t']} its={fp64['its']} flags={fp64['flags']}")
+
+    out = {
+        "metric": "assembly Melem/s + solve+adjoint time; mixed/batched vs FP64 GPU @1e-8 resid",
+        "value": p.mesh.nelem / (ms_per_step * 1e-3) / 1e6,
+        "unit": "Melem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 assembly + f32/f64 mixed solve",
+        "data": "synthetic structured mesh (seeded generator, no external data)",
+        "config": {"workload": f"linear hex cube {args.n}^3, E=70e9 nu=0.3, x=0 clamped, "
+                               f"traction 1e6 Pa on x=1, per-element thickness DVs; assembly + "
+                               f"mixed GMRES({args.m})+block-Jacobi solve + compliance adjoint "
+                               f"to {args.rtol:g} relative residual",
+                   "elements": p.mesh.nelem, "dof": 3 * p.mesh.nnodes,
+                   "nnz_blocks": p.mat.getInfo()[2]},
+        "assembly_Melem_s": p.mesh.nelem / (asm_ms * 1e-3) / 1e6,
+        "assembly_ms": asm_ms,
+        "solve_adjoint_s_mixed": mixed["t"]["solve_adjoint_s"],
+        "iterations_mixed": list(mixed["its"]),
+        "roofline": {"bound": "hbm", "achieved": bw64, "peak": peak_bw, "unit": "GB/s",
+                     "frac": bw64 / peak_bw, "traffic": None,
+                     "kernel": "bsr3_spmv<double> (FP64 block-SpMV, plain path)",
+                     "avg_ms": sp["fp64_ms"], "algorithmic_bytes": sp["fp64_bytes"]},
+        "roofline_fp32_spmv": {"bound": "hbm", "achieved": bw32, "peak": peak_bw,
+                               "unit": "GB/s", "frac": bw32 / peak_bw, "avg_ms": sp["fp32_ms"],
+                               "algorithmic_bytes": sp["fp32_bytes"]},
+        "roofline_assembly": {"bound": "mfma", "achieved": jac_tf, "peak": 78.6,
+                              "unit": "TFLOP/s", "frac": jac_tf / 78.6,
+                              "note": "69,120 FLOP/element dense B^T C B count; FP64 vector peak"},
+    }
+    if fp64 is not None:
+        g, g64 = mixed["grad"], fp64["grad"]
+        out["solve_adjoint_s_fp64"] = fp64["t"]["solve_adjoint_s"]
+        out["iterations_fp64"] = list(fp64["its"])
+        out["speedup_mixed_vs_fp64"] = fp64["t"]["solve_adjoint_s"] / mixed["t"]["solve_adjoint_s"]
+        out["grad_rel_err_vs_fp64"] = float(np.abs(g - g64).max() / np.abs(g64).max())
+        out["u_rel_err_vs_fp64"] = float(np.abs(mixed["u"] - fp64["u"]).max()
+                                         / np.abs(fp64["u"]).max())
+        out["compliance_rel_err_vs_fp64"] = abs(mixed["compliance"] - fp64["compliance"]) / abs(
+            fp64["compliance"])
+    if not args.no_cpu:
+        its = fp64["its"] if fp64 is not None else mixed["its"]
+        out["cpu_baseline"] = cpu_baseline(p, its)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -287,24 +287,23 @@ class GMRES : public TACSKsm {
   double* h_pin;    // pinned host scalars
 };
 
-// FP32 GMRES (FP32 matrix, block-Jacobi and Krylov basis; Gram-Schmidt dot
-// products and the Hessenberg in FP64) inside FP64 iterative refinement
-// (outer residual b - A x in FP64).  No counterpart in the reference.
+// Mixed-precision restarted GMRES(m): each cycle starts from the FP64
+// residual b - A x and runs its Arnoldi steps on the FP32 matrix, FP32
+// block-Jacobi and an FP32 basis (Gram-Schmidt dot products, Hessenberg and
+// Givens rotations in FP64); the correction is accumulated in FP64.  No
+// counterpart in the reference.
 class TACSMixedGMRES : public TACSKsm {
  public:
-  TACSMixedGMRES(TACSParallelMat* mat, TACSBlockJacobiPc* pc, int m, int nrestart,
-                 double inner_rtol = 1e-4, int max_outer = 20);
+  TACSMixedGMRES(TACSParallelMat* mat, TACSBlockJacobiPc* pc, int m, int nrestart);
   ~TACSMixedGMRES();
   int solve(TACSVec* b, TACSVec* x, int zero_guess = 1);
   void setTolerances(double rtol, double atol);
-  int innerSolve(const float* r, float* d);
-  double rtol, atol, inner_rtol;
-  int msub, nrestart, max_outer, outerCount;
+  double rtol, atol;
+  int msub, nrestart, outerCount;
   TACSParallelMat* mat;
   TACSBlockJacobiPc* pc;
   float* d_W;
   float* d_work32;
-  float* d_r32;
   float* d_d32;
   double* d_r;
   double* d_scal;

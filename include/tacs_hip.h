@@ -272,8 +272,9 @@ int tacs_pc_factor(tacs_obj pc);
 int tacs_pc_apply(tacs_obj pc, tacs_obj x, tacs_obj y);
 tacs_obj tacs_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int is_flexible);
 int tacs_gmres_set_ortho_type(tacs_obj ksm, int ortho);
-tacs_obj tacs_mixed_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart,
-                                 double inner_rtol, int max_outer);
+/* Mixed-precision restarted GMRES(m) (FP32 Arnoldi cycles, FP64 restart
+   residual and solution); no counterpart in the reference. */
+tacs_obj tacs_mixed_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart);
 int tacs_ksm_set_tolerances(tacs_obj ksm, double rtol, double atol);
 int tacs_ksm_solve(tacs_obj ksm, tacs_obj b, tacs_obj x, int zero_guess);
 int tacs_ksm_get_iter_count(tacs_obj ksm);

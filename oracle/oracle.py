@@ -33,6 +33,7 @@ def lib():
         L.orc_node_csr.restype = ctypes.c_long
         L.orc_assemble_solid.restype = ctypes.c_long
         L.orc_gmres.restype = ctypes.c_int
+        L.orc_gmres_capped.restype = ctypes.c_int
         L.orc_bsr_diag_inv.restype = ctypes.c_int
         L.orc_compliance.restype = ctypes.c_double
         _lib = L
@@ -160,17 +161,17 @@ def bsr_diag_inv(b, rowp, cols, A):
 
 
 def gmres(b, rowp, cols, A, rhs, Dinv=None, m=30, nrestart=10, rtol=1e-8,
-          atol=1e-30, x0=None):
+          atol=1e-30, x0=None, max_its=0):
     n = rowp.size - 1
     x = np.zeros(b * n) if x0 is None else _dd(x0).copy()
     iters = ctypes.c_int(0)
     rn = ctypes.c_double(0.0)
-    flag = lib().orc_gmres(b, n, _p(_ii(rowp), np.int32), _p(_ii(cols), np.int32),
-                           _p(_dd(A)), _p(_dd(Dinv)) if Dinv is not None else None,
-                           _p(_dd(rhs)), _p(x), m, nrestart,
-                           ctypes.c_double(rtol), ctypes.c_double(atol),
-                           1 if x0 is None else 0, ctypes.byref(iters),
-                           ctypes.byref(rn))
+    flag = lib().orc_gmres_capped(b, n, _p(_ii(rowp), np.int32), _p(_ii(cols), np.int32),
+                                  _p(_dd(A)), _p(_dd(Dinv)) if Dinv is not None else None,
+                                  _p(_dd(rhs)), _p(x), m, nrestart,
+                                  ctypes.c_double(rtol), ctypes.c_double(atol),
+                                  1 if x0 is None else 0, max_its, ctypes.byref(iters),
+                                  ctypes.byref(rn))
     return x, flag, iters.value, rn.value
 
 

@@ -592,6 +592,12 @@ static void apply_pc(int b, int nrows, const double *Dinv, const double *x,
   }
 }
 
+int orc_gmres_capped(int b, int nrows, const int *rowp, const int *cols,
+                     const double *A, const double *Dinv, const double *rhs,
+                     double *x, int m, int nrestart, double rtol, double atol,
+                     int zero_guess, int max_its, int *iters,
+                     double *resnorm_out);
+
 static double vdot(size_t n, const double *x, const double *y) {
   double s = 0.0;
   for (size_t i = 0; i < n; i++) s += x[i] * y[i];
@@ -616,6 +622,17 @@ int orc_gmres(int b, int nrows, const int *rowp, const int *cols,
               const double *A, const double *Dinv, const double *rhs,
               double *x, int m, int nrestart, double rtol, double atol,
               int zero_guess, int *iters, double *resnorm_out) {
+  return orc_gmres_capped(b, nrows, rowp, cols, A, Dinv, rhs, x, m, nrestart,
+                          rtol, atol, zero_guess, 0, iters, resnorm_out);
+}
+
+/* Same, stopping after max_its Arnoldi steps when max_its > 0 (used only to
+   time a bounded sample of iterations). */
+int orc_gmres_capped(int b, int nrows, const int *rowp, const int *cols,
+                     const double *A, const double *Dinv, const double *rhs,
+                     double *x, int m, int nrestart, double rtol, double atol,
+                     int zero_guess, int max_its, int *iters,
+                     double *resnorm_out) {
   size_t n = (size_t)b * nrows;
   double **W = (double **)malloc((m + 1) * sizeof(double *));
   for (int i = 0; i <= m; i++) W[i] = (double *)malloc(n * sizeof(double));
@@ -678,6 +695,7 @@ int orc_gmres(int b, int nrows, const int *rowp, const int *cols,
         solve_flag = 1;
         break;
       }
+      if (max_its > 0 && total + niters >= max_its) break;
     }
     total += niters;
     for (int i = niters - 1; i >= 0; i--) {
@@ -688,7 +706,7 @@ int orc_gmres(int b, int nrows, const int *rowp, const int *cols,
     for (int i = 0; i < niters; i++) vaxpy(n, res[i], W[i], work);
     apply_pc(b, nrows, Dinv, work, W[0]);
     vaxpy(n, 1.0, W[0], x);
-    if (solve_flag) break;
+    if (solve_flag || (max_its > 0 && total >= max_its)) break;
   }
   if (iters) *iters = total;
   if (resnorm_out) *resnorm_out = resNorm;

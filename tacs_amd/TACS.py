@@ -148,13 +148,14 @@ class KSM(TACSHandle):
 
 
 class MixedKSM(KSM):
-    """FP32 GMRES(m) + block-Jacobi inside FP64 iterative refinement: the
-    outer residual b - A x, the Gram-Schmidt dot products and the Hessenberg
-    stay in FP64.  No counterpart in the reference."""
+    """Mixed-precision restarted GMRES(m) with block-Jacobi: every cycle
+    restarts from the FP64 residual b - A x; its Arnoldi steps use the FP32
+    matrix, preconditioner and basis with FP64 Gram-Schmidt dot products and
+    Hessenberg; the correction is accumulated in FP64.  No counterpart in the
+    reference."""
 
-    def __init__(self, mat, pc, m=30, restarts=50, inner_rtol=1e-4, max_outer=20):
-        TACSHandle.__init__(self, lib().tacs_mixed_gmres_create(mat.ptr, pc.ptr, m, restarts,
-                                                                inner_rtol, max_outer),
+    def __init__(self, mat, pc, m=30, restarts=1000):
+        TACSHandle.__init__(self, lib().tacs_mixed_gmres_create(mat.ptr, pc.ptr, m, restarts),
                             keep=(mat, pc))
 
     def getOuterCount(self):

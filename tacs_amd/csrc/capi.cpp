@@ -376,11 +376,14 @@ int tacs_gmres_set_ortho_type(tacs_obj ksm, int ortho) {
   g->setOrthoType((enum OrthoType)ortho);
   return 0;
 }
-tacs_obj tacs_mixed_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, double inner_rtol,
-                                 int max_outer) {
+tacs_obj tacs_mixed_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart) {
   REQUIRE_NULL(TACSParallelMat, A, mat);
   REQUIRE_NULL(TACSBlockJacobiPc, p, pc);
-  return (TACSObject*)new TACSMixedGMRES(A, p, m, nrestart, inner_rtol, max_outer);
+  if (m < 1 || m > 31) {
+    tacs_set_error("mixed GMRES: subspace size must be in [1, 31]");
+    return NULL;
+  }
+  return (TACSObject*)new TACSMixedGMRES(A, p, m, nrestart);
 }
 int tacs_ksm_set_tolerances(tacs_obj ksm, double rtol, double atol) {
   REQUIRE(TACSKsm, k, ksm);

@@ -630,20 +630,22 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
 }
 
 // ---------------------------------------------------------------------------
-// Mixed precision: FP32 GMRES (CGS2, FP64 Hessenberg) inside FP64 iterative
-// refinement.
+// Mixed precision restarted GMRES: every cycle starts from the FP64 residual
+// r = b - A x (FP64 matrix), runs up to m Arnoldi steps in FP32 (FP32 matrix,
+// block-Jacobi and basis; classical Gram-Schmidt twice with FP64 dot
+// products; FP64 Hessenberg and Givens), and adds the FP32 correction to the
+// FP64 solution.  This is iterative refinement with one GMRES(m) cycle as the
+// inner solver, so the convergence follows FP64 GMRES(m) while the
+// bandwidth-bound work runs on FP32 data.
 // ---------------------------------------------------------------------------
-TACSMixedGMRES::TACSMixedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int _m, int _nrestart,
-                               double _inner_rtol, int _max_outer)
-    : rtol(1e-8), atol(1e-30), inner_rtol(_inner_rtol), msub(_m), nrestart(_nrestart),
-      max_outer(_max_outer), outerCount(0), mat(m), pc(p) {
+TACSMixedGMRES::TACSMixedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int _m, int _nrestart)
+    : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart), outerCount(0), mat(m), pc(p) {
   mat->incref();
   if (pc) pc->incref();
   if (msub > 31) msub = 31;  // CGS multi-dot width
   size_t n = (size_t)mat->nrows * mat->bsize;
   d_W = dalloc<float>((size_t)(msub + 1) * n);
   d_work32 = dalloc<float>(n);
-  d_r32 = dalloc<float>(n);
   d_d32 = dalloc<float>(n);
   d_r = dalloc<double>(n);
   GmresScal L(msub);
@@ -657,7 +659,6 @@ TACSMixedGMRES::TACSMixedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int _m,
 TACSMixedGMRES::~TACSMixedGMRES() {
   dfree(d_W);
   dfree(d_work32);
-  dfree(d_r32);
   dfree(d_d32);
   dfree(d_r);
   dfree(d_scal);
@@ -671,8 +672,9 @@ void TACSMixedGMRES::setTolerances(double r, double a) {
   atol = a;
 }
 
-// Solve A32 d = r (r of unit norm) to inner_rtol; returns Arnoldi steps.
-int TACSMixedGMRES::innerSolve(const float* r, float* d) {
+int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
+  TACSBVec* b = dynamic_cast<TACSBVec*>(tb);
+  TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
   const size_t n = (size_t)mat->nrows * mat->bsize;
   const int bs = mat->bsize, nr = mat->nrows;
   float* A32 = mat->getValues32(1);
@@ -684,23 +686,35 @@ int TACSMixedGMRES::innerSolve(const float* r, float* d) {
   double* misc = d_scal + L.misc;
   double* tmp = d_scal + L.tmp;
   auto W = [&](int i) { return d_W + (size_t)i * n; };
-  int total = 0;
-  double r0norm = 0.0;
-  KCHECK(tacs_hip_fill(32, n, d, 0.0, g_stream));
-  for (int count = 0; count < nrestart + 1; count++) {
-    if (count == 0) {
-      HCHECK(hipMemcpyAsync(W(0), r, n * sizeof(float), hipMemcpyDeviceToDevice, g_stream));
-    } else {
-      // W0 = r - A32 d
-      KCHECK(tacs_hip_bsr_spmv_f32(bs, nr, mat->d_rowp, mat->d_cols, A32, d, W(0), 0, g_stream));
-      KCHECK(tacs_hip_axpby(32, 32, n, 1.0, r, -1.0, W(0), g_stream));
+  const double* bd = b->devRead();
+  double* xd = x->devWrite();
+  iterCount = 0;
+  outerCount = 0;
+  if (zero_guess) {
+    KCHECK(tacs_hip_fill(64, n, xd, 0.0, g_stream));
+    HCHECK(hipMemcpyAsync(d_r, bd, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
+  } else {
+    KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
+    KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
+  }
+  double r0 = 0.0;
+  int flag = 0;
+  for (int count = 0; count <= nrestart; count++) {
+    // FP64 residual norm; W0 = r/|r| in FP32 (res[0] = 1)
+    KCHECK(tacs_hip_dot(64, n, NULL, NULL, d_r, d_r, d_red, misc + MISC_NRM, 1, misc + MISC_INV,
+                        g_stream));
+    double rn = pin_read(h_pin, misc + MISC_NRM);
+    if (count == 0) r0 = rn;
+    resNorm = rn;
+    if (rn <= rtol * r0 || rn < atol) {
+      flag = 1;
+      break;
     }
-    KCHECK(tacs_hip_dot(32, n, NULL, NULL, W(0), W(0), d_red, res, 1, misc + MISC_INV, g_stream));
-    KCHECK(tacs_hip_scale(32, n, W(0), misc + MISC_INV, 0.0, 0, g_stream));
-    double rn = pin_read(h_pin, res);
-    if (count == 0) r0norm = rn;
-    if (rn <= inner_rtol * r0norm || rn == 0.0) break;
-    int niters = 0, done = 0;
+    if (!std::isfinite(rn)) break;
+    KCHECK(tacs_hip_axpby(64, 32, n, 1.0 / rn, d_r, 0.0, W(0), g_stream));
+    KCHECK(tacs_hip_fill(64, 1, res, 1.0, g_stream));
+    const double target = std::max(rtol * r0, atol) / rn;  // in units of |r|
+    int niters = 0;
     for (int i = 0; i < msub; i++) {
       float* q = W(i + 1);
       KCHECK(tacs_hip_bsr_diag_apply_f32(bs, nr, pc->d_Dinv32, W(i), d_work32, g_stream));
@@ -719,57 +733,14 @@ int TACSMixedGMRES::innerSolve(const float* r, float* d) {
       KCHECK(tacs_hip_givens(i, msub, H, Qc, Qs, res, misc + MISC_ST, g_stream));
       niters++;
       double st = pin_read(h_pin, misc + MISC_ST);
-      if (st <= inner_rtol * r0norm || !std::isfinite(st)) {
-        done = 1;
-        break;
-      }
+      if (st <= target || !std::isfinite(st)) break;
     }
-    total += niters;
+    iterCount += niters;
+    outerCount++;
+    // x += |r| M^{-1} (W y);  r = b - A x in FP64
     KCHECK(tacs_hip_backsolve(niters, msub, H, res, g_stream));
     KCHECK(tacs_hip_lincomb(32, 32, n, niters, d_W, n, res, d_work32, 0, g_stream));
-    KCHECK(tacs_hip_bsr_diag_apply_f32(bs, nr, pc->d_Dinv32, d_work32, W(0), g_stream));
-    KCHECK(tacs_hip_axpby(32, 32, n, 1.0, W(0), 1.0, d, g_stream));
-    if (done) break;
-  }
-  return total;
-}
-
-int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
-  TACSBVec* b = dynamic_cast<TACSBVec*>(tb);
-  TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
-  const size_t n = (size_t)mat->nrows * mat->bsize;
-  const int bs = mat->bsize, nr = mat->nrows;
-  GmresScal L(msub);
-  double* misc = d_scal + L.misc;
-  const double* bd = b->devRead();
-  double* xd = x->devWrite();
-  iterCount = 0;
-  outerCount = 0;
-  if (zero_guess) {
-    KCHECK(tacs_hip_fill(64, n, xd, 0.0, g_stream));
-    HCHECK(hipMemcpyAsync(d_r, bd, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
-  } else {
-    KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
-    KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
-  }
-  double r0 = 0.0;
-  int flag = 0;
-  for (int k = 0; k <= max_outer; k++) {
-    KCHECK(tacs_hip_dot(64, n, NULL, NULL, d_r, d_r, d_red, misc + MISC_NRM, 1, misc + MISC_INV,
-                        g_stream));
-    double rn = pin_read(h_pin, misc + MISC_NRM);
-    if (k == 0) r0 = rn;
-    resNorm = rn;
-    if (rn <= rtol * r0 || rn < atol) {
-      flag = 1;
-      break;
-    }
-    if (!std::isfinite(rn) || k == max_outer) break;
-    // r32 = r / |r|
-    KCHECK(tacs_hip_axpby(64, 32, n, 1.0 / rn, d_r, 0.0, d_r32, g_stream));
-    iterCount += innerSolve(d_r32, d_d32);
-    outerCount++;
-    // x += |r| d ; r = b - A x (FP64)
+    KCHECK(tacs_hip_bsr_diag_apply_f32(bs, nr, pc->d_Dinv32, d_work32, d_d32, g_stream));
     KCHECK(tacs_hip_axpby(32, 64, n, rn, d_d32, 1.0, xd, g_stream));
     KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
     KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
