@@ -153,7 +153,7 @@ int tacs_hip_dot(int prec, size_t n, const void* wprev, const double* hprev, voi
                  const void* w, void* scratch, double* out, int take_sqrt, double* out_inv,
                  hipStream_t stream);
 int tacs_hip_mdot(int prec, size_t n, int k, const void* W, size_t ldw, const void* q,
-                  void* scratch, double* h, hipStream_t stream);
+                  void* scratch, double* h, int accumulate, hipStream_t stream);
 int tacs_hip_cgs_update(int prec, size_t n, int k, const void* W, size_t ldw, const double* c,
                         const double* c2, void* q, hipStream_t stream);
 int tacs_hip_lincomb(int prec_w, int prec_y, size_t n, int k, const void* W, size_t ldw,

@@ -14,10 +14,14 @@
 // ---------------------------------------------------------------------------
 // 3x3 SpMV: one wavefront per block row.  Lane l = 9*bo + 3*r + c (bo < 7)
 // owns entry (r, c) of blocks bo, bo+7, bo+14, ... of the row, so every load
-// instruction of the wave reads 63 consecutive values of A.  Partial sums are
-// reduced over c and bo with cross-lane shuffles.  y = A x (+ y if beta).
+// instruction of the wave reads 63 consecutive values of A.  Rows of up to
+// 7*NT blocks (28 for NT = 4: every row of a structured 8-node hex mesh) are
+// handled with all loads of the row issued before the arithmetic; the
+// values are streamed with non-temporal loads so that x stays in cache.
+// Partial sums are reduced over c and bo with cross-lane shuffles.
+// y = A x (+ y if add).
 // ---------------------------------------------------------------------------
-template <typename T, typename ACC>
+template <typename T, typename ACC, int NT, int R>
 __global__ __launch_bounds__(256) void bsr3_spmv(int nrows, const int* __restrict__ rowp,
                                                  const int* __restrict__ cols,
                                                  const T* __restrict__ A,
@@ -30,29 +34,69 @@ __global__ __launch_bounds__(256) void bsr3_spmv(int nrows, const int* __restric
   const int c = rc % 3;
   const bool ok = lane < 63;
   const int nw = gridDim.x * 4;
-  for (int row = blockIdx.x * 4 + wid; row < nrows; row += nw) {
-    const int rs = rowp[row];
-    const int nb = rowp[row + 1] - rs;
-    ACC acc = 0;
-    if (ok) {
-      const T* Ar = A + (size_t)rs * 9 + rc;
-      const int* cr = cols + rs;
-      for (int s = bo; s < nb; s += 7) {
-        int col = cr[s];
-        acc += (ACC)Ar[(size_t)s * 9] * (ACC)x[3 * col + c];
+  // each wave handles R consecutive rows per pass, with every load of the R
+  // rows in flight together
+  for (int row0 = (blockIdx.x * 4 + wid) * R; row0 < nrows; row0 += nw * R) {
+    int rs[R], nb[R];
+    bool fast = true;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      int row = row0 + r;
+      rs[r] = row < nrows ? rowp[row] : 0;
+      nb[r] = row < nrows ? rowp[row + 1] - rs[r] : 0;
+      fast = fast && nb[r] <= 7 * NT;
+    }
+    ACC acc[R];
+    if (fast) {
+      int col[R][NT];
+      T av[R][NT];
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          int s = bo + 7 * t;
+          col[r][t] = (ok && s < nb[r]) ? cols[rs[r] + s] : -1;
+        }
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          int s = bo + 7 * t;
+          av[r][t] = (ok && s < nb[r])
+                         ? __builtin_nontemporal_load(&A[((size_t)rs[r] + s) * 9 + rc])
+                         : (T)0;
+        }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        acc[r] = 0;
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+          if (col[r][t] >= 0) acc[r] += (ACC)av[r][t] * (ACC)x[3 * col[r][t] + c];
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        acc[r] = 0;
+        if (ok)
+          for (int s = bo; s < nb[r]; s += 7)
+            acc[r] += (ACC)__builtin_nontemporal_load(&A[((size_t)rs[r] + s) * 9 + rc]) *
+                      (ACC)x[3 * cols[rs[r] + s] + c];
       }
     }
-    // reduce over c (lanes l, l+1, l+2 with c = 0)
-    ACC a1 = __shfl(acc, (lane + 1) & 63, 64);
-    ACC a2 = __shfl(acc, (lane + 2) & 63, 64);
-    ACC s3 = acc + a1 + a2;
-    // reduce over bo: lane r (< 3) gathers lanes 9*b + 3*r
-    ACC tot = 0;
 #pragma unroll
-    for (int b = 0; b < 7; b++) tot += __shfl(s3, (9 * b + 3 * lane) & 63, 64);
-    if (lane < 3) {
-      size_t o = 3 * (size_t)row + lane;
-      y[o] = add ? (T)((ACC)y[o] + tot) : (T)tot;
+    for (int r = 0; r < R; r++) {
+      // reduce over c (lanes l, l+1, l+2 with c = 0), then over bo: lane q
+      // (< 3) gathers lanes 9*b + 3*q
+      ACC a1 = __shfl(acc[r], (lane + 1) & 63, 64);
+      ACC a2 = __shfl(acc[r], (lane + 2) & 63, 64);
+      ACC s3 = acc[r] + a1 + a2;
+      ACC tot = 0;
+#pragma unroll
+      for (int b = 0; b < 7; b++) tot += __shfl(s3, (9 * b + 3 * lane) & 63, 64);
+      if (lane < 3 && row0 + r < nrows) {
+        size_t o = 3 * (size_t)(row0 + r) + lane;
+        y[o] = add ? (T)((ACC)y[o] + tot) : (T)tot;
+      }
     }
   }
 }
@@ -272,9 +316,9 @@ extern "C" {
 int tacs_hip_bsr_spmv(int b, int nrows, const int* rowp, const int* cols, const double* A,
                       const double* x, double* y, int add, hipStream_t stream) {
   if (nrows <= 0) return 0;
-  int grid = tacs_grid_for(nrows, 4, 65536);
+  int grid = tacs_grid_for(nrows, 8, 65536);
   if (b == 3)
-    hipLaunchKernelGGL((bsr3_spmv<double, double>), dim3(grid), dim3(256), 0, stream, nrows, rowp,
+    hipLaunchKernelGGL((bsr3_spmv<double, double, 4, 2>), dim3(grid), dim3(256), 0, stream, nrows, rowp,
                        cols, A, x, y, add);
   else if (b == 6)
     hipLaunchKernelGGL((bsr_spmv_generic<6, double, double>), dim3(grid), dim3(256), 0, stream,
@@ -287,9 +331,9 @@ int tacs_hip_bsr_spmv(int b, int nrows, const int* rowp, const int* cols, const 
 int tacs_hip_bsr_spmv_f32(int b, int nrows, const int* rowp, const int* cols, const float* A,
                           const float* x, float* y, int add, hipStream_t stream) {
   if (nrows <= 0) return 0;
-  int grid = tacs_grid_for(nrows, 4, 65536);
+  int grid = tacs_grid_for(nrows, 8, 65536);
   if (b == 3)
-    hipLaunchKernelGGL((bsr3_spmv<float, float>), dim3(grid), dim3(256), 0, stream, nrows, rowp,
+    hipLaunchKernelGGL((bsr3_spmv<float, float, 4, 2>), dim3(grid), dim3(256), 0, stream, nrows, rowp,
                        cols, A, x, y, add);
   else if (b == 6)
     hipLaunchKernelGGL((bsr_spmv_generic<6, float, float>), dim3(grid), dim3(256), 0, stream,

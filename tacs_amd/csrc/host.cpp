@@ -594,11 +594,9 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
       } else {
         // classical Gram-Schmidt, twice (ClassicalGramSchmidt KSM.cpp:515)
         int k = i + 1;
-        KCHECK(tacs_hip_fill(64, k, h, 0.0, g_stream));
-        KCHECK(tacs_hip_mdot(64, n, k, d_W, n, q, d_red, h, g_stream));
+        KCHECK(tacs_hip_mdot(64, n, k, d_W, n, q, d_red, h, 0, g_stream));
         KCHECK(tacs_hip_cgs_update(64, n, k, d_W, n, h, NULL, q, g_stream));
-        KCHECK(tacs_hip_fill(64, k, tmp, 0.0, g_stream));
-        KCHECK(tacs_hip_mdot(64, n, k, d_W, n, q, d_red, tmp, g_stream));
+        KCHECK(tacs_hip_mdot(64, n, k, d_W, n, q, d_red, tmp, 0, g_stream));
         KCHECK(tacs_hip_cgs_update(64, n, k, d_W, n, tmp, NULL, q, g_stream));
         KCHECK(tacs_hip_axpby(64, 64, k, 1.0, tmp, 1.0, h, g_stream));
         KCHECK(tacs_hip_dot(64, n, NULL, NULL, q, q, d_red, &h[i + 1], 1, misc + MISC_INV, g_stream));
@@ -721,11 +719,9 @@ int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
       KCHECK(tacs_hip_bsr_spmv_f32(bs, nr, mat->d_rowp, mat->d_cols, A32, d_work32, q, 0, g_stream));
       double* h = H + (size_t)i * (msub + 1);
       int k = i + 1;
-      KCHECK(tacs_hip_fill(64, k, h, 0.0, g_stream));
-      KCHECK(tacs_hip_mdot(32, n, k, d_W, n, q, d_red, h, g_stream));
+      KCHECK(tacs_hip_mdot(32, n, k, d_W, n, q, d_red, h, 0, g_stream));
       KCHECK(tacs_hip_cgs_update(32, n, k, d_W, n, h, NULL, q, g_stream));
-      KCHECK(tacs_hip_fill(64, k, tmp, 0.0, g_stream));
-      KCHECK(tacs_hip_mdot(32, n, k, d_W, n, q, d_red, tmp, g_stream));
+      KCHECK(tacs_hip_mdot(32, n, k, d_W, n, q, d_red, tmp, 0, g_stream));
       KCHECK(tacs_hip_cgs_update(32, n, k, d_W, n, tmp, NULL, q, g_stream));
       KCHECK(tacs_hip_axpby(64, 64, k, 1.0, tmp, 1.0, h, g_stream));
       KCHECK(tacs_hip_dot(32, n, NULL, NULL, q, q, d_red, &h[i + 1], 1, misc + MISC_INV, g_stream));

@@ -52,6 +52,14 @@ __device__ __forceinline__ void lag1d(double x, double* n, double* d) {
   }
 }
 
+// Element of a short register array selected by a runtime index without
+// dynamic indexing (which would place the array in private memory).
+template <int ORD>
+__device__ __forceinline__ double pick(const double* v, int i) {
+  if (ORD == 2) return i ? v[1] : v[0];
+  return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
+}
+
 // 1D values at the three coordinates of quadrature point q.
 template <int ORD>
 struct QP1D {
@@ -66,10 +74,12 @@ struct QP1D {
   }
   __device__ __forceinline__ void basis(int a, double& N, double g[3]) const {
     int ia = a % ORD, ja = (a / ORD) % ORD, ka = a / (ORD * ORD);
-    N = n[0][ia] * n[1][ja] * n[2][ka];
-    g[0] = d[0][ia] * n[1][ja] * n[2][ka];
-    g[1] = n[0][ia] * d[1][ja] * n[2][ka];
-    g[2] = n[0][ia] * n[1][ja] * d[2][ka];
+    double n0 = pick<ORD>(n[0], ia), n1 = pick<ORD>(n[1], ja), n2 = pick<ORD>(n[2], ka);
+    double d0 = pick<ORD>(d[0], ia), d1 = pick<ORD>(d[1], ja), d2 = pick<ORD>(d[2], ka);
+    N = n0 * n1 * n2;
+    g[0] = d0 * n1 * n2;
+    g[1] = n0 * d1 * n2;
+    g[2] = n0 * n1 * d2;
   }
 };
 

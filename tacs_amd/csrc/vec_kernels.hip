@@ -5,7 +5,7 @@
 // the Hessenberg column (GMRES::solve, src/bpmat/KSM.cpp:785-958).
 #include "tacs_hip_common.h"
 
-#define RED_BLOCKS 512
+#define RED_BLOCKS 2048
 #define RED_THREADS 256
 #define RED_MAXK 32
 
@@ -77,9 +77,12 @@ __global__ __launch_bounds__(RED_THREADS) void dot_fused(size_t n, const T* __re
   double bs = block_sum(acc, sh);
   if (threadIdx.x == 0) rs->partial[blockIdx.x] = bs;
   if (last_block(rs)) {
-    double t = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) t += rs->partial[b];
-    t = block_sum(t, sh);
+    double t = 0.0, t1 = 0.0;
+    int b = threadIdx.x;
+    for (; b + (int)blockDim.x < (int)gridDim.x; b += 2 * blockDim.x)
+      t += rs->partial[b], t1 += rs->partial[b + blockDim.x];
+    if (b < (int)gridDim.x) t += rs->partial[b];
+    t = block_sum(t + t1, sh);
     if (threadIdx.x == 0) {
       double r = take_sqrt ? sqrt(t) : t;
       out[0] = r;
@@ -89,35 +92,56 @@ __global__ __launch_bounds__(RED_THREADS) void dot_fused(size_t n, const T* __re
   }
 }
 
-// Multi-dot: h[j] = W_j . q for j < k, W_j = W + j*ldw (classical GS).
-template <typename T>
+// Multi-dot: h[j] (+)= W_j . q for j < k, W_j = W + j*ldw (classical GS).
+// Each thread keeps k partial sums; the block reduces them in one phase
+// (wave shuffles, then one LDS exchange), and the last block combines the
+// per-block partials in block order.
+template <typename T, int KMAX>
 __global__ __launch_bounds__(RED_THREADS) void mdot_kernel(size_t n, int k, const T* __restrict__ W,
                                                            size_t ldw, const T* __restrict__ q,
-                                                           RedScratch* rs,
-                                                           double* __restrict__ h) {
-  __shared__ double sh[RED_THREADS / 64];
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (int j0 = 0; j0 < k; j0 += 8) {
-    const int kk = min(8, k - j0);
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-      double qi = (double)q[i];
+                                                           RedScratch* rs, double* __restrict__ h,
+                                                           int accumulate) {
+  __shared__ double sh[RED_THREADS / 64][KMAX];
+  double acc[KMAX];
 #pragma unroll
-      for (int j = 0; j < 8; j++)
-        if (j < kk) acc[j] += (double)W[(size_t)(j0 + j) * ldw + i] * qi;
-    }
-    for (int j = 0; j < kk; j++) {
-      double bs = block_sum(acc[j], sh);
-      if (threadIdx.x == 0) rs->partial[(size_t)(j0 + j) * RED_BLOCKS + blockIdx.x] = bs;
+  for (int j = 0; j < KMAX; j++) acc[j] = 0.0;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double qi = (double)q[i];
+#pragma unroll
+    for (int j = 0; j < KMAX; j++)
+      if (j < k) acc[j] += (double)W[(size_t)j * ldw + i] * qi;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < KMAX; j++) {
+    if (j < k) {
+      double v = wave_sum(acc[j]);
+      if (lane == 0) sh[wid][j] = v;
     }
   }
+  __syncthreads();
+  if (threadIdx.x < k) {
+    double t = 0.0;
+    for (int w = 0; w < RED_THREADS / 64; w++) t += sh[w][threadIdx.x];
+    rs->partial[(size_t)threadIdx.x * RED_BLOCKS + blockIdx.x] = t;
+  }
   if (last_block(rs)) {
-    for (int j = 0; j < k; j++) {
-      double t = 0.0;
-      for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-        t += rs->partial[(size_t)j * RED_BLOCKS + b];
-      t = block_sum(t, sh);
-      if (threadIdx.x == 0) h[j] += t;
+    // wave w sums the block partials of vectors w, w + 4, ...
+    const int nb = gridDim.x;
+    for (int j = wid; j < k; j += RED_THREADS / 64) {
+      const double* pj = &rs->partial[(size_t)j * RED_BLOCKS];
+      double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+      int bb = lane;
+      for (; bb + 192 < nb; bb += 256) {
+        t0 += pj[bb];
+        t1 += pj[bb + 64];
+        t2 += pj[bb + 128];
+        t3 += pj[bb + 192];
+      }
+      for (; bb < nb; bb += 64) t0 += pj[bb];
+      double t = wave_sum((t0 + t1) + (t2 + t3));
+      if (lane == 0) h[j] = accumulate ? h[j] + t : t;
     }
     if (threadIdx.x == 0) rs->ticket = 0;
   }
@@ -261,32 +285,48 @@ extern "C" {
 
 size_t tacs_hip_red_scratch_bytes() { return sizeof(RedScratch); }
 
+static inline int red_blocks(size_t n) {
+  size_t b = n / 2048;
+  if (b < 1) b = 1;
+  if (b > RED_BLOCKS) b = RED_BLOCKS;
+  return (int)b;
+}
+
 int tacs_hip_dot(int prec, size_t n, const void* wprev, const double* hprev, void* q,
                  const void* w, void* scratch, double* out, int take_sqrt, double* out_inv,
                  hipStream_t stream) {
   RedScratch* rs = (RedScratch*)scratch;
+  const int nb = red_blocks(n);
   if (prec == 64)
-    hipLaunchKernelGGL(dot_fused<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n,
+    hipLaunchKernelGGL(dot_fused<double>, dim3(nb), dim3(RED_THREADS), 0, stream, n,
                        (const double*)wprev, hprev, (double*)q, (const double*)w, rs, out,
                        take_sqrt, out_inv);
   else
-    hipLaunchKernelGGL(dot_fused<float>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n,
+    hipLaunchKernelGGL(dot_fused<float>, dim3(nb), dim3(RED_THREADS), 0, stream, n,
                        (const float*)wprev, hprev, (float*)q, (const float*)w, rs, out,
                        take_sqrt, out_inv);
   return (int)hipGetLastError();
 }
 
-// h[0..k) += W^T q  (accumulates; zero h first for a plain multi-dot)
+// h[0..k) = W^T q, or h += W^T q when accumulate
 int tacs_hip_mdot(int prec, size_t n, int k, const void* W, size_t ldw, const void* q,
-                  void* scratch, double* h, hipStream_t stream) {
-  if (k > RED_MAXK) return TACS_HIP_ERR_ARG;
+                  void* scratch, double* h, int accumulate, hipStream_t stream) {
+  if (k > RED_MAXK || k < 1) return TACS_HIP_ERR_ARG;
   RedScratch* rs = (RedScratch*)scratch;
-  if (prec == 64)
-    hipLaunchKernelGGL(mdot_kernel<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, k,
-                       (const double*)W, ldw, (const double*)q, rs, h);
-  else
-    hipLaunchKernelGGL(mdot_kernel<float>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, k,
-                       (const float*)W, ldw, (const float*)q, rs, h);
+  const int nb = red_blocks(n);
+#define MD_LAUNCH(T, KM)                                                                       \
+  hipLaunchKernelGGL((mdot_kernel<T, KM>), dim3(nb), dim3(RED_THREADS), 0, stream, n, k,       \
+                     (const T*)W, ldw, (const T*)q, rs, h, accumulate)
+  if (prec == 64) {
+    if (k <= 8) MD_LAUNCH(double, 8);
+    else if (k <= 16) MD_LAUNCH(double, 16);
+    else MD_LAUNCH(double, 32);
+  } else {
+    if (k <= 8) MD_LAUNCH(float, 8);
+    else if (k <= 16) MD_LAUNCH(float, 16);
+    else MD_LAUNCH(float, 32);
+  }
+#undef MD_LAUNCH
   return (int)hipGetLastError();
 }
 

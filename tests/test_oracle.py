@@ -1,0 +1,331 @@
+"""Pins of the oracle (the CPU restatement of the reference path) against the
+reference's own golden values, closed forms, invariants and finite
+differences.  Each check is chosen so that a plausible restatement mistake
+(dropped term, wrong sign, transposed index) fails it."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tacs_amd.meshes import HexMesh
+
+E, NU = 70e9, 0.3
+
+
+def _cube(n, L=1.0, order=2):
+    return HexMesh(n, n, n, order=order, L=(L, L, L))
+
+
+def _system(m, t=None, C=None, bc=True):
+    C = O.iso_C21(E, NU) if C is None else C
+    rowp, cols = O.node_csr(m.nnodes, m.conn, m.npe)
+    t = np.ones(m.nelem) if t is None else t
+    A, res = O.assemble_solid(m.order, m.conn, m.X, np.zeros(3 * m.nnodes), t, C, rowp, cols)
+    bcn = m.nodes_on_x(0)
+    bcv = np.full(bcn.size, 7, np.int32)
+    if bc:
+        O.apply_bc_mat(3, bcn, bcv, rowp, cols, A)
+    return C, rowp, cols, A, bcn, bcv
+
+
+def _dense(rowp, cols, A, b=3):
+    n = rowp.size - 1
+    K = np.zeros((b * n, b * n))
+    Ab = A.reshape(-1, b, b)
+    for i in range(n):
+        for j in range(rowp[i], rowp[i + 1]):
+            K[b * i:b * i + b, b * cols[j]:b * cols[j] + b] = Ab[j]
+    return K
+
+
+def _free_solve(K, f, bcn):
+    free = np.ones(f.size, bool)
+    for nd in bcn:
+        free[3 * nd:3 * nd + 3] = False
+    u = np.zeros_like(f)
+    u[free] = np.linalg.solve(K[np.ix_(free, free)], f[free])
+    return u
+
+
+# ---------------------------------------------------------------- basics
+def test_gauss_literals_and_weights():
+    # literals of src/elements/basis/TACSGaussQuadrature.h:26,29 (not 1/sqrt3)
+    w, pt = O.hex_quadrature(2, 0)
+    assert pt[0] == -0.577350269189626 and w == 1.0
+    w, pt = O.hex_quadrature(3, 0)
+    assert pt[0] == -0.774596669241483 and abs(w - (5 / 9) ** 3) < 1e-16
+    # weights integrate the reference cube volume
+    for order in (2, 3):
+        assert abs(sum(O.hex_quadrature(order, n)[0] for n in range(order ** 3)) - 8.0) < 1e-14
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_basis_partition_of_unity_and_nodal(order):
+    rng = np.random.default_rng(0)
+    for _ in range(5):
+        pt = rng.uniform(-1, 1, 3)
+        N, Nxi = O.hex_basis(order, pt)
+        assert abs(N.sum() - 1.0) < 1e-14
+        assert np.abs(Nxi.sum(axis=0)).max() < 1e-14
+        # derivative check by central differences
+        h = 1e-6
+        for d in range(3):
+            pp, pm = pt.copy(), pt.copy()
+            pp[d] += h
+            pm[d] -= h
+            fd = (O.hex_basis(order, pp)[0] - O.hex_basis(order, pm)[0]) / (2 * h)
+            assert np.abs(fd - Nxi[:, d]).max() < 1e-8
+    # Kronecker property at the nodes, x-fastest numbering
+    knots = [-1.0, 1.0] if order == 2 else [-1.0, 0.0, 1.0]
+    a = 0
+    for k in knots:
+        for j in knots:
+            for i in knots:
+                N, _ = O.hex_basis(order, np.array([i, j, k]))
+                assert abs(N[a] - 1.0) < 1e-14 and abs(N.sum() - 1.0) < 1e-14
+                a += 1
+
+
+def test_isotropic_C21_closed_form():
+    C = O.iso_C21(E, NU)
+    lam = E * NU / ((1 + NU) * (1 - 2 * NU))
+    mu = E / (2 * (1 + NU))
+    assert np.allclose([C[0], C[1], C[2], C[6], C[7], C[11]],
+                       [lam + 2 * mu, lam, lam, lam + 2 * mu, lam, lam + 2 * mu], rtol=1e-15)
+    assert np.allclose([C[15], C[18], C[20]], [mu, mu, mu], rtol=1e-15)
+    assert C[3] == C[4] == C[5] == C[8] == C[9] == C[10] == 0.0
+
+
+# ---------------------------------------------------------------- element
+@pytest.mark.parametrize("order", [2, 3])
+def test_element_symmetry_rigid_modes_and_residual(order):
+    rng = np.random.default_rng(3)
+    m = HexMesh(1, 1, 1, order=order)
+    X = m.X[m.conn[0]].reshape(-1) + 0.05 * rng.standard_normal(3 * m.npe)
+    C = O.iso_C21(E, NU)
+    u = rng.standard_normal(3 * m.npe) * 1e-3
+    res, K = O.solid_element(order, X, u, C, t=0.7)
+    assert np.abs(K - K.T).max() <= 1e-12 * np.abs(K).max()
+    # residual of a linear element equals K u
+    assert np.abs(res - K @ u).max() <= 1e-10 * np.abs(res).max()
+    # six rigid-body modes in the null space
+    Xn = X.reshape(-1, 3)
+    modes = []
+    for d in range(3):
+        v = np.zeros_like(Xn)
+        v[:, d] = 1.0
+        modes.append(v.reshape(-1))
+    for (i, j) in [(0, 1), (1, 2), (0, 2)]:
+        v = np.zeros_like(Xn)
+        v[:, i] = -Xn[:, j]
+        v[:, j] = Xn[:, i]
+        modes.append(v.reshape(-1))
+    for v in modes:
+        assert np.abs(K @ v).max() <= 1e-9 * np.abs(K).max() * np.abs(v).max()
+    # positive semi-definite with exactly 6 zero modes
+    ev = np.linalg.eigvalsh(K)
+    assert (ev > 1e-8 * ev.max()).sum() == 3 * m.npe - 6
+
+
+def test_element_mass_matrix_total_mass():
+    # gamma * consistent mass: sum of one direction's block = rho*t*volume
+    m = HexMesh(1, 1, 1, L=(2.0, 1.0, 0.5))
+    X = m.X[m.conn[0]].reshape(-1)
+    C = O.iso_C21(E, NU)
+    _, M = O.solid_element(2, X, np.zeros(24), C, t=0.5, rho=2700.0, alpha=0.0, gamma=1.0)
+    assert abs(M[0::3, 0::3].sum() - 2700.0 * 0.5 * 1.0) < 1e-9
+
+
+def test_uniaxial_bar_closed_form():
+    # 1 x 1 x N bar, clamped (sliding) so that the stress state is uniaxial:
+    # tip displacement = F L / (E A)
+    n = 6
+    m = HexMesh(n, 1, 1, L=(float(n), 1.0, 1.0))
+    C = O.iso_C21(E, NU)
+    rowp, cols = O.node_csr(m.nnodes, m.conn, 8)
+    A, res = O.assemble_solid(2, m.conn, m.X, np.zeros(3 * m.nnodes), np.ones(m.nelem), C, rowp, cols)
+    ae, af = m.elems_on_x(1)
+    sigma = 1e6
+    O.add_tractions(2, m.conn, m.X, ae, af, np.tile([sigma, 0, 0], (ae.size, 1)), res)
+    # symmetric sliding supports: u_x = 0 on x = 0, u_y = 0 on y = 0, u_z = 0 on z = 0
+    K = _dense(rowp, cols, A)
+    fix = []
+    for i, (x, y, z) in enumerate(m.X):
+        if x == 0.0:
+            fix.append(3 * i)
+        if y == 0.0:
+            fix.append(3 * i + 1)
+        if z == 0.0:
+            fix.append(3 * i + 2)
+    free = np.setdiff1d(np.arange(K.shape[0]), fix)
+    u = np.zeros(K.shape[0])
+    u[free] = np.linalg.solve(K[np.ix_(free, free)], -res[free])
+    tip = u[0::3][m.X[:, 0] == float(n)]
+    assert np.allclose(tip, sigma * n / E, rtol=1e-12)
+    # lateral contraction -nu * eps * width
+    assert np.allclose(u[1::3][(m.X[:, 1] == 1.0) & (m.X[:, 0] == float(n))],
+                       -NU * sigma / E, rtol=1e-10)
+
+
+def test_patch_test_linear_field():
+    # a linear displacement field gives zero residual at interior nodes
+    m = _cube(3)
+    rng = np.random.default_rng(5)
+    Xp = m.X.copy()
+    interior = np.all((m.X > 0) & (m.X < 1), axis=1)
+    Xp[interior] += 0.05 * rng.standard_normal((interior.sum(), 3))
+    G = rng.standard_normal((3, 3)) * 1e-3
+    u = (Xp @ G.T).reshape(-1)
+    rowp, cols = O.node_csr(m.nnodes, m.conn, 8)
+    C = O.iso_C21(E, NU)
+    _, res = O.assemble_solid(2, m.conn, Xp, u, np.ones(m.nelem), C, rowp, cols, want_mat=False)
+    r = res.reshape(-1, 3)
+    assert np.abs(r[interior]).max() <= 1e-9 * np.abs(r).max()
+
+
+# ---------------------------------------------------------------- assembly / pattern
+def test_node_csr_brute_force():
+    m = _cube(2, order=3)
+    rowp, cols = O.node_csr(m.nnodes, m.conn, 27)
+    for i in range(m.nnodes):
+        nb = set()
+        for e in range(m.nelem):
+            if i in m.conn[e]:
+                nb |= set(m.conn[e].tolist())
+        assert cols[rowp[i]:rowp[i + 1]].tolist() == sorted(nb)
+
+
+def test_assembly_equals_dense_sum_and_bc_rows():
+    m = _cube(2)
+    C, rowp, cols, A, bcn, bcv = _system(m, bc=False)
+    K = _dense(rowp, cols, A)
+    Kd = np.zeros_like(K)
+    for e in range(m.nelem):
+        X = m.X[m.conn[e]].reshape(-1)
+        _, Ke = O.solid_element(2, X, np.zeros(24), C)
+        idx = np.repeat(3 * m.conn[e], 3) + np.tile(np.arange(3), 8)
+        Kd[np.ix_(idx, idx)] += Ke
+    assert np.abs(K - Kd).max() <= 1e-12 * np.abs(K).max()
+    O.apply_bc_mat(3, bcn, bcv, rowp, cols, A)
+    K2 = _dense(rowp, cols, A)
+    for nd in bcn:
+        rows = K2[3 * nd:3 * nd + 3]
+        expect = np.zeros_like(rows)
+        expect[:, 3 * nd:3 * nd + 3] = np.eye(3)
+        assert np.array_equal(rows, expect)
+    # columns of constrained dofs are kept (BCSRMat::zeroRow only zeroes rows)
+    assert np.abs(K2[:, 3 * bcn[0]]).max() > 0
+
+
+def test_spmv_against_dense():
+    m = _cube(3)
+    C, rowp, cols, A, bcn, bcv = _system(m)
+    x = np.random.default_rng(0).standard_normal(3 * m.nnodes)
+    y = O.bsr_mult(3, rowp, cols, A, x)
+    assert np.abs(y - _dense(rowp, cols, A) @ x).max() <= 1e-13 * np.abs(y).max()
+
+
+def test_block_jacobi_inverse():
+    m = _cube(2)
+    C, rowp, cols, A, bcn, bcv = _system(m)
+    D = O.bsr_diag_inv(3, rowp, cols, A).reshape(-1, 3, 3)
+    Ab = A.reshape(-1, 3, 3)
+    for i in range(m.nnodes):
+        s = rowp[i] + list(cols[rowp[i]:rowp[i + 1]]).index(i)
+        assert np.abs(D[i] @ Ab[s] - np.eye(3)).max() < 1e-12
+
+
+def test_gmres_matches_dense_solve():
+    m = _cube(4)
+    C, rowp, cols, A, bcn, bcv = _system(m)
+    rhs = np.random.default_rng(2).standard_normal(3 * m.nnodes)
+    O.apply_bc_vec(3, bcn, bcv, rhs)
+    D = O.bsr_diag_inv(3, rowp, cols, A)
+    x, flag, its, rn = O.gmres(3, rowp, cols, A, rhs, D, m=20, nrestart=200, rtol=1e-12)
+    assert flag == 1
+    K = _dense(rowp, cols, A)
+    assert np.linalg.norm(K @ x - rhs) <= 1.01e-12 * np.linalg.norm(rhs)
+    ref = _free_solve(K, rhs, bcn)
+    assert np.abs(x - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+# ---------------------------------------------------------------- golden values
+def test_golden_cube_functions():
+    """tests/integration_tests/test_elast_linhexa_element_3d.py:11-27 (3x3x3 cube,
+    side 10, rho 2570, E 70e9, nu 0.3, ys 350e6, x=0 clamped, 1e8 on every
+    dof): KSFailure(ksWeight=10) 1.1074017881777185, mass 2570000.0,
+    compliance 178995399.11928475."""
+    m = _cube(3, L=10.0)
+    C, rowp, cols, A, bcn, bcv = _system(m)
+    f = np.full(3 * m.nnodes, 1e8)
+    O.apply_bc_vec(3, bcn, bcv, f)
+    u = _free_solve(_dense(rowp, cols, A), f, bcn)
+    c = O.compliance(2, m.conn, m.X, u, np.ones(m.nelem), C)
+    assert abs(c - 178995399.11928475) <= 1e-12 * 178995399.11928475
+    ks = O.ks_failure(2, m.conn, m.X, u, C, 350e6, 10.0)
+    assert abs(ks[0] - 1.1074017881777185) <= 1e-10
+    # mass = rho * volume with the same quadrature
+    vol = sum(O.hex_quadrature(2, q)[0] * (10.0 / 3 / 2) ** 3 for q in range(8)) * m.nelem
+    assert abs(2570.0 * vol - 2570000.0) < 1e-6
+
+
+# ---------------------------------------------------------------- sensitivities
+def test_compliance_equals_work_and_adjoint_identity():
+    m = _cube(3)
+    rng = np.random.default_rng(7)
+    t = rng.uniform(0.2, 1.0, m.nelem)
+    C, rowp, cols, A, bcn, bcv = _system(m, t=t)
+    _, res = O.assemble_solid(2, m.conn, m.X, np.zeros(3 * m.nnodes), t, C, rowp, cols,
+                              want_mat=False)
+    ae, af = m.elems_on_x(1)
+    O.add_tractions(2, m.conn, m.X, ae, af, np.tile([0, 0, 1e6], (ae.size, 1)), res)
+    O.apply_bc_vec(3, bcn, bcv, res)
+    K = _dense(rowp, cols, A)
+    u = -_free_solve(K, res, bcn)
+    c = O.compliance(2, m.conn, m.X, u, t, C)
+    # compliance (no 1/2) = f.u = u^T K u
+    assert abs(c - (-res) @ u) <= 1e-10 * abs(c)
+    dfdu = O.compliance_svsens(2, m.conn, m.X, u, t, C)
+    # dC/du = 2 K u (before BCs)
+    C0, rp0, cl0, A0, _, _ = _system(m, t=t, bc=False)
+    assert np.abs(dfdu - 2 * _dense(rp0, cl0, A0) @ u).max() <= 1e-9 * np.abs(dfdu).max()
+    O.apply_bc_vec(3, bcn, bcv, dfdu)
+    psi = _free_solve(K, dfdu, bcn)
+    g = O.compliance_dvsens(2, m.conn, m.X, u, C) - O.adj_res_product(2, m.conn, m.X, u, psi, C)
+    # closed form dC/dt_e = -u_e^T K0_e u_e
+    assert np.abs(g + O.compliance_dvsens(2, m.conn, m.X, u, C)).max() <= 1e-8 * np.abs(g).max()
+    # central finite differences of the full solve on a few elements
+    for e in [0, 7, 20]:
+        h = 1e-6
+        vals = []
+        for s in (+1, -1):
+            tt = t.copy()
+            tt[e] += s * h
+            _, rp, cl, Ap, _, _ = _system(m, t=tt)
+            up = -_free_solve(_dense(rp, cl, Ap), res, bcn)
+            vals.append(O.compliance(2, m.conn, m.X, up, tt, C))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(fd - g[e]) <= 1e-5 * abs(g[e])
+
+
+def test_ks_state_sensitivity_fd():
+    m = _cube(2)
+    C = O.iso_C21(E, NU)
+    rng = np.random.default_rng(9)
+    u = rng.standard_normal(3 * m.nnodes) * 1e-4
+    out = O.ks_failure(2, m.conn, m.X, u, C, 270e6, 50.0)
+    g = O.ks_svsens(2, m.conn, m.X, u, C, 270e6, 50.0, out[1], out[2])
+    d = rng.standard_normal(u.size)
+    h = 1e-7 * np.abs(u).max()
+    fp = O.ks_failure(2, m.conn, m.X, u + h * d, C, 270e6, 50.0)[0]
+    fm = O.ks_failure(2, m.conn, m.X, u - h * d, C, 270e6, 50.0)[0]
+    assert abs((fp - fm) / (2 * h) - g @ d) <= 1e-6 * abs(g @ d)
+
+
+def test_traction_total_force():
+    # integral of the traction over the loaded face = traction * area
+    for order in (2, 3):
+        m = HexMesh(3, 2, 4, order=order, L=(1.0, 2.0, 0.5))
+        ae, af = m.elems_on_x(1)
+        res = np.zeros(3 * m.nnodes)
+        O.add_tractions(order, m.conn, m.X, ae, af, np.tile([1.0, -2.0, 3.0], (ae.size, 1)), res)
+        assert np.allclose(res.reshape(-1, 3).sum(0), -np.array([1.0, -2.0, 3.0]) * 1.0, rtol=1e-13)
