@@ -363,24 +363,22 @@ int tacs_pc_apply(tacs_obj pc, tacs_obj x, tacs_obj y) {
 tacs_obj tacs_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int flexible) {
   REQUIRE_NULL(TACSParallelMat, A, mat);
   TACSPc* p = pc ? OBJ(TACSPc, pc) : NULL;
-  if (m < 1 || m > 128) {
-    tacs_set_error("GMRES: subspace size must be in [1, 128]");
+  if (m < 1 || m > 512) {
+    tacs_set_error("GMRES: subspace size must be in [1, 512]");
     return NULL;
   }
   return (TACSObject*)new GMRES(A, p, m, nrestart, flexible);
 }
 int tacs_gmres_set_ortho_type(tacs_obj ksm, int ortho) {
   REQUIRE(GMRES, g, ksm);
-  if (ortho == CLASSICAL_GRAM_SCHMIDT && g->msub > 31)
-    return tacs_set_error("classical Gram-Schmidt requires a subspace of at most 31");
   g->setOrthoType((enum OrthoType)ortho);
   return 0;
 }
 tacs_obj tacs_mixed_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart) {
   REQUIRE_NULL(TACSParallelMat, A, mat);
   REQUIRE_NULL(TACSBlockJacobiPc, p, pc);
-  if (m < 1 || m > 31) {
-    tacs_set_error("mixed GMRES: subspace size must be in [1, 31]");
+  if (m < 1 || m > 512) {
+    tacs_set_error("mixed GMRES: subspace size must be in [1, 512]");
     return NULL;
   }
   return (TACSObject*)new TACSMixedGMRES(A, p, m, nrestart);

@@ -527,6 +527,29 @@ void GMRES::setTolerances(double r, double a) {
 }
 void GMRES::setOrthoType(enum OrthoType o) { ortho = o; }
 
+
+// Classical Gram-Schmidt, twice, of q against the k vectors of W (stride n),
+// in chunks of 32 vectors per multi-dot: h[0..k) = W^T q (both passes
+// summed), q -= W h.  prec: 64 or 32 (storage of W and q).
+static void cgs2(int prec, size_t n, int k, const void* W, void* q, double* h, double* tmp,
+                 void* red) {
+  const size_t esz = prec == 64 ? 8 : 4;
+  for (int pass = 0; pass < 2; pass++) {
+    double* c = pass == 0 ? h : tmp;
+    for (int j0 = 0; j0 < k; j0 += 32) {
+      int kk = std::min(32, k - j0);
+      KCHECK(tacs_hip_mdot(prec, n, kk, (const char*)W + esz * n * j0, n, q, red, c + j0, 0,
+                           g_stream));
+    }
+    for (int j0 = 0; j0 < k; j0 += 32) {
+      int kk = std::min(32, k - j0);
+      KCHECK(tacs_hip_cgs_update(prec, n, kk, (const char*)W + esz * n * j0, n, c + j0, NULL, q,
+                                 g_stream));
+    }
+  }
+  KCHECK(tacs_hip_axpby(64, 64, k, 1.0, tmp, 1.0, h, g_stream));
+}
+
 static double pin_read(double* h_pin, const double* d) {
   HCHECK(hipMemcpyAsync(h_pin, d, sizeof(double), hipMemcpyDeviceToHost, g_stream));
   HCHECK(hipStreamSynchronize(g_stream));
@@ -593,12 +616,7 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
         KCHECK(tacs_hip_dot(64, n, W(i), &h[i], q, q, d_red, &h[i + 1], 1, misc + MISC_INV, g_stream));
       } else {
         // classical Gram-Schmidt, twice (ClassicalGramSchmidt KSM.cpp:515)
-        int k = i + 1;
-        KCHECK(tacs_hip_mdot(64, n, k, d_W, n, q, d_red, h, 0, g_stream));
-        KCHECK(tacs_hip_cgs_update(64, n, k, d_W, n, h, NULL, q, g_stream));
-        KCHECK(tacs_hip_mdot(64, n, k, d_W, n, q, d_red, tmp, 0, g_stream));
-        KCHECK(tacs_hip_cgs_update(64, n, k, d_W, n, tmp, NULL, q, g_stream));
-        KCHECK(tacs_hip_axpby(64, 64, k, 1.0, tmp, 1.0, h, g_stream));
+        cgs2(64, n, i + 1, d_W, q, h, tmp, d_red);
         KCHECK(tacs_hip_dot(64, n, NULL, NULL, q, q, d_red, &h[i + 1], 1, misc + MISC_INV, g_stream));
       }
       KCHECK(tacs_hip_scale(64, n, q, misc + MISC_INV, 0.0, 0, g_stream));
@@ -640,7 +658,6 @@ TACSMixedGMRES::TACSMixedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int _m,
     : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart), outerCount(0), mat(m), pc(p) {
   mat->incref();
   if (pc) pc->incref();
-  if (msub > 31) msub = 31;  // CGS multi-dot width
   size_t n = (size_t)mat->nrows * mat->bsize;
   d_W = dalloc<float>((size_t)(msub + 1) * n);
   d_work32 = dalloc<float>(n);
@@ -718,12 +735,7 @@ int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
       KCHECK(tacs_hip_bsr_diag_apply_f32(bs, nr, pc->d_Dinv32, W(i), d_work32, g_stream));
       KCHECK(tacs_hip_bsr_spmv_f32(bs, nr, mat->d_rowp, mat->d_cols, A32, d_work32, q, 0, g_stream));
       double* h = H + (size_t)i * (msub + 1);
-      int k = i + 1;
-      KCHECK(tacs_hip_mdot(32, n, k, d_W, n, q, d_red, h, 0, g_stream));
-      KCHECK(tacs_hip_cgs_update(32, n, k, d_W, n, h, NULL, q, g_stream));
-      KCHECK(tacs_hip_mdot(32, n, k, d_W, n, q, d_red, tmp, 0, g_stream));
-      KCHECK(tacs_hip_cgs_update(32, n, k, d_W, n, tmp, NULL, q, g_stream));
-      KCHECK(tacs_hip_axpby(64, 64, k, 1.0, tmp, 1.0, h, g_stream));
+      cgs2(32, n, i + 1, d_W, q, h, tmp, d_red);
       KCHECK(tacs_hip_dot(32, n, NULL, NULL, q, q, d_red, &h[i + 1], 1, misc + MISC_INV, g_stream));
       KCHECK(tacs_hip_scale(32, n, q, misc + MISC_INV, 0.0, 0, g_stream));
       KCHECK(tacs_hip_givens(i, msub, H, Qc, Qs, res, misc + MISC_ST, g_stream));

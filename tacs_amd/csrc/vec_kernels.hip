@@ -1,6 +1,6 @@
 // gfx950 vector kernels for TACSBVec and the Krylov solvers: axpy/scale,
-// deterministic reductions whose results stay in device memory (so Arnoldi
-// steps never wait on the host), the fused modified-Gram-Schmidt step, the
+// deterministic two-launch reductions whose results stay in device memory
+// (so Arnoldi steps never wait on the host), the fused modified-Gram-Schmidt step, the
 // classical-Gram-Schmidt multi-dot / update pair, and the Givens update of
 // the Hessenberg column (GMRES::solve, src/bpmat/KSM.cpp:785-958).
 #include "tacs_hip_common.h"
@@ -9,13 +9,23 @@
 #define RED_THREADS 256
 #define RED_MAXK 32
 
-// Reduction scratch: per-block partials and a completion ticket.  The last
-// block to finish sums the partials in block order (deterministic) and
-// resets the ticket, so the scratch can be reused without a memset.
+// Blocks of a reduction over n entries: about 8 entries per thread, at most
+// RED_BLOCKS (a function of n only, so results are reproducible).
+static inline int red_blocks(size_t n) {
+  size_t b = n / 2048;
+  if (b < 1) b = 1;
+  if (b > RED_BLOCKS) b = RED_BLOCKS;
+  return (int)b;
+}
+
+// Reduction scratch: per-block partial sums.  Every reduction is two
+// launches: the first writes one partial per block with ordinary stores, the
+// second (one block) combines the partials in block order.  The kernel
+// boundary makes the partials visible; no atomics and no device-scope fences
+// (a device-scope release writes back the L2 of the XCD) are needed, and the
+// result is deterministic.
 struct RedScratch {
   double partial[RED_BLOCKS * RED_MAXK];
-  unsigned int ticket;
-  unsigned int pad[63];
 };
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -30,30 +40,14 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return t;
 }
 
-// Returns true in the (single) last block after all partials are visible.
-__device__ __forceinline__ bool last_block(RedScratch* rs) {
-  __shared__ bool amlast;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned int t = atomicAdd(&rs->ticket, 1u);
-    amlast = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (amlast) __threadfence();  // acquire: see every block's partial
-  return amlast;
-}
-
-// out[0] = x.y (or sqrt(x.y) if take_sqrt); optional q -= (*hprev) * wprev
-// before the dot (fused MGS step: ModifiedGramSchmidt KSM.cpp:526).
+// partial[b] = sum over the block's share of w.q; optionally q -= (*hprev)
+// wprev first (fused MGS step: ModifiedGramSchmidt KSM.cpp:526).
 template <typename T>
-__global__ __launch_bounds__(RED_THREADS) void dot_fused(size_t n, const T* __restrict__ wprev,
-                                                         const double* __restrict__ hprev,
-                                                         T* q, const T* w, RedScratch* rs,
-                                                         double* __restrict__ out, int take_sqrt,
-                                                         double* __restrict__ out_inv) {
+__global__ __launch_bounds__(RED_THREADS) void dot_partial(size_t n, const T* __restrict__ wprev,
+                                                           const double* __restrict__ hprev,
+                                                           T* q, const T* w, RedScratch* rs) {
   __shared__ double sh[RED_THREADS / 64];
-  double acc = 0.0;
+  double acc = 0.0, acc2 = 0.0;
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   if (wprev) {
@@ -72,35 +66,38 @@ __global__ __launch_bounds__(RED_THREADS) void dot_fused(size_t n, const T* __re
       }
     }
   } else {
-    for (; i < n; i += stride) acc += (double)w[i] * (double)q[i];
-  }
-  double bs = block_sum(acc, sh);
-  if (threadIdx.x == 0) rs->partial[blockIdx.x] = bs;
-  if (last_block(rs)) {
-    double t = 0.0, t1 = 0.0;
-    int b = threadIdx.x;
-    for (; b + (int)blockDim.x < (int)gridDim.x; b += 2 * blockDim.x)
-      t += rs->partial[b], t1 += rs->partial[b + blockDim.x];
-    if (b < (int)gridDim.x) t += rs->partial[b];
-    t = block_sum(t + t1, sh);
-    if (threadIdx.x == 0) {
-      double r = take_sqrt ? sqrt(t) : t;
-      out[0] = r;
-      if (out_inv) out_inv[0] = 1.0 / r;
-      rs->ticket = 0;
+    for (; i + stride < n; i += 2 * stride) {
+      acc += (double)w[i] * (double)q[i];
+      acc2 += (double)w[i + stride] * (double)q[i + stride];
     }
+    if (i < n) acc += (double)w[i] * (double)q[i];
+  }
+  double bs = block_sum(acc + acc2, sh);
+  if (threadIdx.x == 0) rs->partial[blockIdx.x] = bs;
+}
+
+// out[0] = sum of nb partials (sqrt if take_sqrt; out_inv = 1/out)
+__global__ __launch_bounds__(RED_THREADS) void dot_finalize(int nb, const RedScratch* rs,
+                                                            double* __restrict__ out,
+                                                            int take_sqrt,
+                                                            double* __restrict__ out_inv) {
+  __shared__ double sh[RED_THREADS / 64];
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) t += rs->partial[b];
+  t = block_sum(t, sh);
+  if (threadIdx.x == 0) {
+    double r = take_sqrt ? sqrt(t) : t;
+    out[0] = r;
+    if (out_inv) out_inv[0] = 1.0 / r;
   }
 }
 
-// Multi-dot: h[j] (+)= W_j . q for j < k, W_j = W + j*ldw (classical GS).
-// Each thread keeps k partial sums; the block reduces them in one phase
-// (wave shuffles, then one LDS exchange), and the last block combines the
-// per-block partials in block order.
+// Multi-dot partials: partial[j][b] = block b's share of W_j . q (classical
+// GS).  Each thread keeps k partial sums; one block reduction phase.
 template <typename T, int KMAX>
-__global__ __launch_bounds__(RED_THREADS) void mdot_kernel(size_t n, int k, const T* __restrict__ W,
-                                                           size_t ldw, const T* __restrict__ q,
-                                                           RedScratch* rs, double* __restrict__ h,
-                                                           int accumulate) {
+__global__ __launch_bounds__(RED_THREADS) void mdot_partial(size_t n, int k, const T* __restrict__ W,
+                                                            size_t ldw, const T* __restrict__ q,
+                                                            RedScratch* rs) {
   __shared__ double sh[RED_THREADS / 64][KMAX];
   double acc[KMAX];
 #pragma unroll
@@ -126,24 +123,26 @@ __global__ __launch_bounds__(RED_THREADS) void mdot_kernel(size_t n, int k, cons
     for (int w = 0; w < RED_THREADS / 64; w++) t += sh[w][threadIdx.x];
     rs->partial[(size_t)threadIdx.x * RED_BLOCKS + blockIdx.x] = t;
   }
-  if (last_block(rs)) {
-    // wave w sums the block partials of vectors w, w + 4, ...
-    const int nb = gridDim.x;
-    for (int j = wid; j < k; j += RED_THREADS / 64) {
-      const double* pj = &rs->partial[(size_t)j * RED_BLOCKS];
-      double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
-      int bb = lane;
-      for (; bb + 192 < nb; bb += 256) {
-        t0 += pj[bb];
-        t1 += pj[bb + 64];
-        t2 += pj[bb + 128];
-        t3 += pj[bb + 192];
-      }
-      for (; bb < nb; bb += 64) t0 += pj[bb];
-      double t = wave_sum((t0 + t1) + (t2 + t3));
-      if (lane == 0) h[j] = accumulate ? h[j] + t : t;
+}
+
+// h[j] (+)= sum_b partial[j][b]; wave w handles vectors w, w + 4, ...
+__global__ __launch_bounds__(RED_THREADS) void mdot_finalize(int nb, int k, const RedScratch* rs,
+                                                             double* __restrict__ h,
+                                                             int accumulate) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int j = wid; j < k; j += RED_THREADS / 64) {
+    const double* pj = &rs->partial[(size_t)j * RED_BLOCKS];
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    int bb = lane;
+    for (; bb + 192 < nb; bb += 256) {
+      t0 += pj[bb];
+      t1 += pj[bb + 64];
+      t2 += pj[bb + 128];
+      t3 += pj[bb + 192];
     }
-    if (threadIdx.x == 0) rs->ticket = 0;
+    for (; bb < nb; bb += 64) t0 += pj[bb];
+    double t = wave_sum((t0 + t1) + (t2 + t3));
+    if (lane == 0) h[j] = accumulate ? h[j] + t : t;
   }
 }
 
@@ -167,7 +166,7 @@ __global__ void cgs_update(size_t n, int k, const T* __restrict__ W, size_t ldw,
 template <typename T, typename TY>
 __global__ void lincomb(size_t n, int k, const T* __restrict__ W, size_t ldw,
                         const double* __restrict__ c, TY* __restrict__ y, int add) {
-  __shared__ double cs[128];
+  __shared__ double cs[512];
   for (int j = threadIdx.x; j < k; j += blockDim.x) cs[j] = c[j];
   __syncthreads();
   const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -203,10 +202,9 @@ __global__ void fill_kernel(size_t n, T* __restrict__ x, T v) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = v;
 }
 
-// out[0] = sum(x) (op 0) or max(x) (op 1), deterministic.
-__global__ __launch_bounds__(RED_THREADS) void reduce_kernel(size_t n, const double* __restrict__ x,
-                                                             int op, RedScratch* rs,
-                                                             double* __restrict__ out) {
+// partial[b] = sum (op 0) or max (op 1) of the block's share of x
+__global__ __launch_bounds__(RED_THREADS) void reduce_partial(size_t n, const double* __restrict__ x,
+                                                              int op, RedScratch* rs) {
   __shared__ double sh[RED_THREADS / 64];
   double acc = op ? -1e300 : 0.0;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -216,7 +214,6 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_kernel(size_t n, const dou
   if (op) {
     acc = wave_max(acc);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    __syncthreads();
     if (lane == 0) sh[wid] = acc;
     __syncthreads();
     bs = sh[0];
@@ -225,13 +222,13 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_kernel(size_t n, const dou
     bs = block_sum(acc, sh);
   }
   if (threadIdx.x == 0) rs->partial[blockIdx.x] = bs;
-  if (last_block(rs)) {
-    if (threadIdx.x == 0) {
-      double t = rs->partial[0];
-      for (int b = 1; b < (int)gridDim.x; b++) t = op ? fmax(t, rs->partial[b]) : t + rs->partial[b];
-      out[0] = t;
-      rs->ticket = 0;
-    }
+}
+
+__global__ void reduce_finalize(int nb, int op, const RedScratch* rs, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double t = rs->partial[0];
+    for (int b = 1; b < nb; b++) t = op ? fmax(t, rs->partial[b]) : t + rs->partial[b];
+    out[0] = t;
   }
 }
 
@@ -285,26 +282,19 @@ extern "C" {
 
 size_t tacs_hip_red_scratch_bytes() { return sizeof(RedScratch); }
 
-static inline int red_blocks(size_t n) {
-  size_t b = n / 2048;
-  if (b < 1) b = 1;
-  if (b > RED_BLOCKS) b = RED_BLOCKS;
-  return (int)b;
-}
-
 int tacs_hip_dot(int prec, size_t n, const void* wprev, const double* hprev, void* q,
                  const void* w, void* scratch, double* out, int take_sqrt, double* out_inv,
                  hipStream_t stream) {
   RedScratch* rs = (RedScratch*)scratch;
   const int nb = red_blocks(n);
   if (prec == 64)
-    hipLaunchKernelGGL(dot_fused<double>, dim3(nb), dim3(RED_THREADS), 0, stream, n,
-                       (const double*)wprev, hprev, (double*)q, (const double*)w, rs, out,
-                       take_sqrt, out_inv);
+    hipLaunchKernelGGL(dot_partial<double>, dim3(nb), dim3(RED_THREADS), 0, stream, n,
+                       (const double*)wprev, hprev, (double*)q, (const double*)w, rs);
   else
-    hipLaunchKernelGGL(dot_fused<float>, dim3(nb), dim3(RED_THREADS), 0, stream, n,
-                       (const float*)wprev, hprev, (float*)q, (const float*)w, rs, out,
-                       take_sqrt, out_inv);
+    hipLaunchKernelGGL(dot_partial<float>, dim3(nb), dim3(RED_THREADS), 0, stream, n,
+                       (const float*)wprev, hprev, (float*)q, (const float*)w, rs);
+  hipLaunchKernelGGL(dot_finalize, dim3(1), dim3(RED_THREADS), 0, stream, nb, rs, out, take_sqrt,
+                     out_inv);
   return (int)hipGetLastError();
 }
 
@@ -315,8 +305,8 @@ int tacs_hip_mdot(int prec, size_t n, int k, const void* W, size_t ldw, const vo
   RedScratch* rs = (RedScratch*)scratch;
   const int nb = red_blocks(n);
 #define MD_LAUNCH(T, KM)                                                                       \
-  hipLaunchKernelGGL((mdot_kernel<T, KM>), dim3(nb), dim3(RED_THREADS), 0, stream, n, k,       \
-                     (const T*)W, ldw, (const T*)q, rs, h, accumulate)
+  hipLaunchKernelGGL((mdot_partial<T, KM>), dim3(nb), dim3(RED_THREADS), 0, stream, n, k,      \
+                     (const T*)W, ldw, (const T*)q, rs)
   if (prec == 64) {
     if (k <= 8) MD_LAUNCH(double, 8);
     else if (k <= 16) MD_LAUNCH(double, 16);
@@ -327,6 +317,8 @@ int tacs_hip_mdot(int prec, size_t n, int k, const void* W, size_t ldw, const vo
     else MD_LAUNCH(float, 32);
   }
 #undef MD_LAUNCH
+  hipLaunchKernelGGL(mdot_finalize, dim3(1), dim3(RED_THREADS), 0, stream, nb, k, rs, h,
+                     accumulate);
   return (int)hipGetLastError();
 }
 
@@ -345,7 +337,7 @@ int tacs_hip_cgs_update(int prec, size_t n, int k, const void* W, size_t ldw, co
 // y (+)= sum_j c[j] W_j ; prec_w: basis precision, prec_y: output precision
 int tacs_hip_lincomb(int prec_w, int prec_y, size_t n, int k, const void* W, size_t ldw,
                      const double* c, void* y, int add, hipStream_t stream) {
-  if (k > 128) return TACS_HIP_ERR_ARG;
+  if (k > 512) return TACS_HIP_ERR_ARG;
   if (prec_w == 64 && prec_y == 64)
     hipLaunchKernelGGL((lincomb<double, double>), dim3(vgrid(n)), dim3(256), 0, stream, n, k,
                        (const double*)W, ldw, c, (double*)y, add);
@@ -406,8 +398,11 @@ int tacs_hip_fill(int prec, size_t n, void* x, double v, hipStream_t stream) {
 
 int tacs_hip_reduce(size_t n, const double* x, int op, void* scratch, double* out,
                     hipStream_t stream) {
-  hipLaunchKernelGGL(reduce_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, x, op,
-                     (RedScratch*)scratch, out);
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(reduce_partial, dim3(nb), dim3(RED_THREADS), 0, stream, n, x, op,
+                     (RedScratch*)scratch);
+  hipLaunchKernelGGL(reduce_finalize, dim3(1), dim3(64), 0, stream, nb, op,
+                     (const RedScratch*)scratch, out);
   return (int)hipGetLastError();
 }
 
