@@ -195,7 +195,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--m", type=int, default=240,
                     help="restart length of the mixed-precision GMRES")

@@ -247,6 +247,10 @@ int tacs_assembler_add_adjoint_res_products(tacs_obj a, double scale, int n, tac
 int tacs_vec_get_size(tacs_obj v, int* size, int* bsize);
 double* tacs_vec_get_array(tacs_obj v);     /* host mirror, host becomes current */
 double* tacs_vec_device_array(tacs_obj v);  /* device array, device becomes current */
+/* Device-to-device copies between a vector and caller memory of n doubles
+   (on the library stream). */
+int tacs_vec_copy_from_device(tacs_obj v, const double* src, size_t n);
+int tacs_vec_copy_to_device(tacs_obj v, double* dst, size_t n);
 double tacs_vec_norm(tacs_obj v);
 double tacs_vec_dot(tacs_obj v, tacs_obj x);
 int tacs_vec_axpy(tacs_obj v, double alpha, tacs_obj x);

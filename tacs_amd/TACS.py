@@ -50,6 +50,14 @@ class Vec(TACSHandle):
         """Raw device address of the values (device copy becomes current)."""
         return lib().tacs_vec_device_array(self.ptr)
 
+    def copyFromDevice(self, ptr, n):
+        """Copy n doubles from device address ptr into the vector."""
+        check(lib().tacs_vec_copy_from_device(self.ptr, ptr, n), "Vec.copyFromDevice")
+
+    def copyToDevice(self, ptr, n):
+        """Copy the vector into n doubles at device address ptr."""
+        check(lib().tacs_vec_copy_to_device(self.ptr, ptr, n), "Vec.copyToDevice")
+
     def norm(self):
         return lib().tacs_vec_norm(self.ptr)
 

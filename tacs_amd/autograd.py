@@ -1,0 +1,98 @@
+"""Compliance of a solid SIMP problem as a differentiable torch function.
+
+Forward: t = (Emin + x^p (E0 - Emin)) / E0 (the reference's solid
+constitutive model is linear in its thickness variable t,
+src/constitutive/TACSSolidConstitutive.cpp:142, so SIMP is applied on top of
+it), assembleJacobian, the solve, and the compliance.  Backward: the adjoint
+solve K psi = dC/du (addSVSens + applyBCs), then
+dC/dt = addDVSens - addAdjointResProducts (TACSAssembler.cpp:4767, :5033) in
+the GPU kernels, which recompute the element strains from u and psi instead
+of storing them, and the chain rule dt/dx = p x^(p-1) (E0 - Emin)/E0.
+"""
+import torch
+
+from . import TACS, functions, lib
+from .meshes import simp_thickness, simp_dthickness
+
+
+def _sync():
+    torch.cuda.synchronize()
+    lib().tacs_device_synchronize()
+
+
+class SolidComplianceProblem:
+    """Holds a SolidProblem (one DV per element, tNum = element index) and its
+    solver for repeated forward / backward passes."""
+
+    def __init__(self, problem, p=3.0, E0=1.0, Emin=1e-9, rtol=1e-8, m=120, mixed=True):
+        self.prob = problem
+        self.p, self.E0, self.Emin, self.rtol = p, E0, Emin, rtol
+        a = problem.assembler
+        self.dv = a.createDesignVec()
+        self.dfdt = a.createDesignVec()
+        self.pc = TACS.Pc(problem.mat)
+        if mixed:
+            self.ksm = TACS.MixedKSM(problem.mat, self.pc, m, 100000)
+        else:
+            self.ksm = TACS.KSM(problem.mat, self.pc, min(m, 512), 100000)
+        self.ksm.setTolerances(rtol, 1e-30)
+        self.comp = functions.Compliance(a)
+        self.iters = []
+
+    def forward(self, x):
+        a = self.prob.assembler
+        t = simp_thickness(x, self.p, self.E0, self.Emin).to(torch.float64).contiguous()
+        _sync()
+        self.dv.copyFromDevice(t.data_ptr(), t.numel())
+        a.setDesignVars(self.dv)
+        a.zeroVariables()
+        a.assembleJacobian(1.0, 0.0, 0.0, self.prob.res, self.prob.mat)
+        self.pc.factor()
+        if self.ksm.solve(self.prob.res, self.prob.u) != 1:
+            raise RuntimeError("forward solve did not converge")
+        self.iters.append(self.ksm.getIterCount())
+        self.prob.u.scale(-1.0)
+        a.setVariables(self.prob.u)
+        return a.evalFunctions([self.comp])[0]
+
+    def gradient_t(self):
+        """dC/dt on the device (torch tensor) at the current state."""
+        a = self.prob.assembler
+        dfdu = a.createVec()
+        a.addSVSens([self.comp], [dfdu])
+        a.applyBCs(dfdu)
+        psi = a.createVec()
+        if self.ksm.solve(dfdu, psi) != 1:
+            raise RuntimeError("adjoint solve did not converge")
+        self.iters.append(self.ksm.getIterCount())
+        self.dfdt.zeroEntries()
+        a.addDVSens([self.comp], [self.dfdt], 1.0)
+        a.addAdjointResProducts([psi], [self.dfdt], -1.0)
+        n = self.dfdt.getSize()
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+        self.dfdt.copyToDevice(out.data_ptr(), n)
+        _sync()
+        return out
+
+
+class SIMPCompliance(torch.autograd.Function):
+    """C(x) for element densities x (1-D float64 CUDA tensor)."""
+
+    @staticmethod
+    def forward(ctx, x, prob):
+        c = prob.forward(x)
+        ctx.prob = prob
+        ctx.save_for_backward(x)
+        return torch.tensor(c, dtype=torch.float64, device=x.device)
+
+    @staticmethod
+    def backward(ctx, grad):
+        (x,) = ctx.saved_tensors
+        prob = ctx.prob
+        dfdt = prob.gradient_t()
+        dtdx = simp_dthickness(x, prob.p, prob.E0, prob.Emin)
+        return grad * dfdt * dtdx, None
+
+
+def simp_compliance(x, prob):
+    return SIMPCompliance.apply(x, prob)

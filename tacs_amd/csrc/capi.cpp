@@ -259,6 +259,20 @@ double* tacs_vec_device_array(tacs_obj v) {
   REQUIRE_NULL(TACSBVec, x, v);
   return x->devWrite();
 }
+int tacs_vec_copy_from_device(tacs_obj v, const double* src, size_t n) {
+  REQUIRE(TACSBVec, x, v);
+  if (n != x->n) return tacs_set_error("copy_from_device: size %zu != %zu", n, x->n);
+  hipError_t e = hipMemcpyAsync(x->devWrite(), src, n * sizeof(double),
+                                hipMemcpyDeviceToDevice, tacs_stream());
+  return e == hipSuccess ? 0 : tacs_set_error("copy_from_device: %s", hipGetErrorString(e));
+}
+int tacs_vec_copy_to_device(tacs_obj v, double* dst, size_t n) {
+  REQUIRE(TACSBVec, x, v);
+  if (n != x->n) return tacs_set_error("copy_to_device: size %zu != %zu", n, x->n);
+  hipError_t e = hipMemcpyAsync(dst, x->devRead(), n * sizeof(double), hipMemcpyDeviceToDevice,
+                                tacs_stream());
+  return e == hipSuccess ? 0 : tacs_set_error("copy_to_device: %s", hipGetErrorString(e));
+}
 double tacs_vec_norm(tacs_obj v) {
   TACSBVec* x = OBJ(TACSBVec, v);
   return x ? x->norm() : 0.0;

@@ -658,6 +658,7 @@ TACSMixedGMRES::TACSMixedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int _m,
     : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart), outerCount(0), mat(m), pc(p) {
   mat->incref();
   if (pc) pc->incref();
+  mat->getValues32(0);  // allocate the FP32 copy now; assembly fills it
   size_t n = (size_t)mat->nrows * mat->bsize;
   d_W = dalloc<float>((size_t)(msub + 1) * n);
   d_work32 = dalloc<float>(n);

@@ -81,8 +81,8 @@ class HexMesh:
 def simp_thickness(x, p=3.0, E0=1.0, Emin=1e-9):
     """SIMP stiffness scale t(x) = (Emin + x^p (E0 - Emin)) / E0 applied on top
     of the reference's linear-in-t solid constitutive model."""
-    return (Emin + np.power(x, p) * (E0 - Emin)) / E0
+    return (Emin + x ** p * (E0 - Emin)) / E0
 
 
 def simp_dthickness(x, p=3.0, E0=1.0, Emin=1e-9):
-    return p * np.power(x, p - 1.0) * (E0 - Emin) / E0
+    return p * x ** (p - 1.0) * (E0 - Emin) / E0
