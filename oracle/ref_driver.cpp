@@ -1,0 +1,118 @@
+// Element-level driver over the reference's own sources (compiled from
+// /root/reference by oracle/build_ref.sh into oracle/_ref/libtacs_ref.so).
+// TEST INFRASTRUCTURE: it only produces reference values for the oracle's
+// checks and the committed fixtures under tests/golden/.
+#include "TACSElement3D.h"
+#include "TACSHexaBasis.h"
+#include "TACSIsoShellConstitutive.h"
+#include "TACSLinearElasticity.h"
+#include "TACSMaterialProperties.h"
+#include "TACSShellElementDefs.h"
+#include "TACSSolidConstitutive.h"
+#include "TACSTraction3D.h"
+
+extern "C" {
+
+// Residual and Jacobian (alpha, beta, gamma) of one 8- or 27-node solid element.
+void ref_solid_element(int order, const double* X, const double* u, double E, double nu,
+                       double rho, double t, double alpha, double gamma, double* res,
+                       double* mat) {
+  TACSMaterialProperties* props = new TACSMaterialProperties(rho, 921.0, E, nu, 270e6, 0.0, 0.0);
+  TACSSolidConstitutive* con = new TACSSolidConstitutive(props, t, 0);
+  TACSLinearElasticity3D* model = new TACSLinearElasticity3D(con, TACS_LINEAR_STRAIN);
+  TACSElementBasis* basis;
+  if (order == 2)
+    basis = new TACSLinearHexaBasis();
+  else
+    basis = new TACSQuadraticHexaBasis();
+  TACSElement3D* elem = new TACSElement3D(model, basis);
+  elem->incref();
+  int nv = elem->getNumVariables();
+  double* zero = new double[nv]();
+  for (int i = 0; i < nv; i++) res[i] = 0.0;
+  for (int i = 0; i < nv * nv; i++) mat[i] = 0.0;
+  elem->addJacobian(0, 0.0, alpha, 0.0, gamma, X, u, zero, zero, res, mat);
+  delete[] zero;
+  elem->decref();
+}
+
+// Traction residual of one solid element face (TACSTraction3D, coordinate
+// components).
+void ref_solid_traction(int order, int face, const double* X, const double* trac, double* res) {
+  TACSElementBasis* basis;
+  if (order == 2)
+    basis = new TACSLinearHexaBasis();
+  else
+    basis = new TACSQuadraticHexaBasis();
+  basis->incref();
+  TACSTraction3D* tr = new TACSTraction3D(3, face, basis, trac, 1);
+  tr->incref();
+  int nv = 3 * basis->getNumNodes();
+  double* zero = new double[nv]();
+  for (int i = 0; i < nv; i++) res[i] = 0.0;
+  tr->addResidual(0, 0.0, X, zero, zero, zero, res);
+  delete[] zero;
+  tr->decref();
+  basis->decref();
+}
+
+static TACSElement* make_quad4(double E, double nu, double ys, double t) {
+  TACSMaterialProperties* props = new TACSMaterialProperties(2700.0, 921.0, E, nu, ys, 0.0, 0.0);
+  TACSIsoShellConstitutive* con = new TACSIsoShellConstitutive(props, t, 0);
+  TACSShellTransform* transform = new TACSShellNaturalTransform();
+  return new TACSQuad4Shell(transform, con);
+}
+
+// Residual and Jacobian of one TACSQuad4Shell (natural transform,
+// TACSIsoShellConstitutive with thickness t).
+void ref_quad4_shell(const double* X, const double* vars, double E, double nu, double ys,
+                     double t, double alpha, double* res, double* mat) {
+  TACSElement* elem = make_quad4(E, nu, ys, t);
+  elem->incref();
+  double zero[24] = {0};
+  for (int i = 0; i < 24; i++) res[i] = 0.0;
+  for (int i = 0; i < 576; i++) mat[i] = 0.0;
+  elem->addJacobian(0, 0.0, alpha, 0.0, 0.0, X, vars, zero, zero, res, mat);
+  elem->decref();
+}
+
+// psi^T dR/dt of one TACSQuad4Shell (thickness DV, number 0).
+double ref_quad4_adj_product(const double* X, const double* vars, const double* psi, double E,
+                             double nu, double ys, double t) {
+  TACSElement* elem = make_quad4(E, nu, ys, t);
+  elem->incref();
+  double zero[24] = {0};
+  double dfdx[1] = {0.0};
+  elem->addAdjResProduct(0, 0.0, 1.0, psi, X, vars, zero, zero, 1, dfdx);
+  elem->decref();
+  return dfdx[0];
+}
+
+// Failure index (KS of the top/bottom von Mises, ksWeight 100 inside the
+// constitutive model) at quadrature point n of one TACSQuad4Shell.
+double ref_quad4_failure(const double* X, const double* vars, double E, double nu, double ys,
+                         double t, int n, double* detXd) {
+  TACSElement* elem = make_quad4(E, nu, ys, t);
+  elem->incref();
+  double zero[24] = {0};
+  double pt[3];
+  elem->getQuadraturePoint(n, pt);
+  double q = 0.0;
+  elem->evalPointQuantity(0, TACS_FAILURE_INDEX, 0.0, n, pt, X, vars, zero, zero, detXd, &q);
+  elem->decref();
+  return q;
+}
+
+// Pressure load residual of one TACSQuad4Shell (TACSShellPressure).
+void ref_quad4_pressure(const double* X, double p, double* res) {
+  TACSElement* elem = make_quad4(70e9, 0.3, 270e6, 0.01);
+  elem->incref();
+  TACSElement* pr = elem->createElementPressure(0, p);
+  pr->incref();
+  double zero[24] = {0};
+  for (int i = 0; i < 24; i++) res[i] = 0.0;
+  pr->addResidual(0, 0.0, X, zero, zero, zero, res);
+  pr->decref();
+  elem->decref();
+}
+}

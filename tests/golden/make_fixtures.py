@@ -1,0 +1,86 @@
+"""Writes tests/golden/elements.npz: element-level reference values computed by
+the reference's own sources (oracle/_ref/libtacs_ref.so, built by
+oracle/build_ref.sh from /root/reference) on seeded inputs.  Run in a
+container that has the reference tree:  python tests/golden/make_fixtures.py
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref", "libtacs_ref.so")
+
+
+def ref_lib():
+    L = ctypes.CDLL(REF, mode=ctypes.RTLD_GLOBAL)
+    L.ref_quad4_adj_product.restype = ctypes.c_double
+    L.ref_quad4_failure.restype = ctypes.c_double
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def hex_nodes(order):
+    k = [-1.0, 1.0] if order == 2 else [-1.0, 0.0, 1.0]
+    return np.array([[x, y, z] for z in k for y in k for x in k])
+
+
+def make(seed=2024):
+    L = ref_lib()
+    rng = np.random.default_rng(seed)
+    c = ctypes.c_double
+    out = {}
+    for order in (2, 3):
+        npe = order ** 3
+        for s in range(2):
+            X = (0.5 * hex_nodes(order) @ np.diag([1.0, 0.8, 1.3])
+                 + 0.06 * rng.standard_normal((npe, 3))).reshape(-1)
+            u = 1e-3 * rng.standard_normal(3 * npe)
+            t = rng.uniform(0.2, 1.0)
+            res = np.zeros(3 * npe)
+            mat = np.zeros(9 * npe * npe)
+            L.ref_solid_element(order, _d(X), _d(u), c(70e9), c(0.3), c(2700.0), c(t), c(1.0),
+                                c(0.25), _d(res), _d(mat))
+            key = f"hex{npe}_{s}"
+            out[key + "_X"], out[key + "_u"], out[key + "_t"] = X, u, np.array([t])
+            out[key + "_res"], out[key + "_mat"] = res, mat
+            trac = rng.standard_normal(3)
+            tr = np.zeros((6, 3 * npe))
+            for face in range(6):
+                L.ref_solid_traction(order, face, _d(X), _d(trac), _d(tr[face]))
+            out[key + "_trac"], out[key + "_trac_res"] = trac, tr
+    for s in range(4):
+        X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], float) * [0.3, 0.2, 1.0]
+        X = (X + 0.02 * rng.standard_normal((4, 3))).reshape(-1)
+        v = 1e-3 * rng.standard_normal(24)
+        psi = rng.standard_normal(24)
+        t = rng.uniform(0.005, 0.02)
+        res = np.zeros(24)
+        mat = np.zeros(576)
+        L.ref_quad4_shell(_d(X), _d(v), c(70e9), c(0.3), c(270e6), c(t), c(1.0), _d(res), _d(mat))
+        adj = L.ref_quad4_adj_product(_d(X), _d(v), _d(psi), c(70e9), c(0.3), c(270e6), c(t))
+        fail = np.zeros(4)
+        det = np.zeros(4)
+        for n in range(4):
+            dd = ctypes.c_double()
+            fail[n] = L.ref_quad4_failure(_d(X), _d(v), c(70e9), c(0.3), c(270e6), c(t), n,
+                                          ctypes.byref(dd))
+            det[n] = dd.value
+        pres = np.zeros(24)
+        L.ref_quad4_pressure(_d(X), c(1e4), _d(pres))
+        key = f"quad4_{s}"
+        out[key + "_X"], out[key + "_v"], out[key + "_psi"] = X, v, psi
+        out[key + "_t"], out[key + "_res"], out[key + "_mat"] = np.array([t]), res, mat
+        out[key + "_adj"], out[key + "_fail"], out[key + "_det"] = np.array([adj]), fail, det
+        out[key + "_pres"] = pres
+    np.savez_compressed(os.path.join(HERE, "elements.npz"), **out)
+    return out
+
+
+if __name__ == "__main__":
+    make()
+    print("wrote", os.path.join(HERE, "elements.npz"))

@@ -1,0 +1,56 @@
+"""The oracle against element-level values computed by the reference's own
+sources (tests/golden/elements.npz, written by tests/golden/make_fixtures.py
+from oracle/_ref), and, where oracle/_ref is built, against fresh inputs."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = np.load(os.path.join(HERE, "golden", "elements.npz"), allow_pickle=False)
+REF = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "libtacs_ref.so")
+
+
+@pytest.mark.parametrize("key", ["hex8_0", "hex8_1", "hex27_0", "hex27_1"])
+def test_solid_element_matches_reference(key):
+    order = 2 if key.startswith("hex8") else 3
+    X, u, t = FIX[key + "_X"], FIX[key + "_u"], FIX[key + "_t"][0]
+    res, mat = O.solid_element(order, X, u, O.iso_C21(70e9, 0.3), t=t, rho=2700.0, alpha=1.0,
+                               gamma=0.25)
+    rr, rm = FIX[key + "_res"], FIX[key + "_mat"]
+    assert np.abs(res - rr).max() <= 1e-13 * np.abs(rr).max()
+    assert np.abs(mat.reshape(-1) - rm).max() <= 1e-13 * np.abs(rm).max()
+    trac = FIX[key + "_trac"]
+    for face in range(6):
+        ro = np.zeros(res.size)
+        O.lib().orc_traction_face(order, face, O._p(np.ascontiguousarray(X)),
+                                  O._p(np.ascontiguousarray(trac)), O._p(ro))
+        ref = FIX[key + "_trac_res"][face]
+        assert np.abs(ro - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built (no reference tree)")
+def test_solid_element_fresh_inputs_against_reference_build():
+    R = ctypes.CDLL(REF, mode=ctypes.RTLD_GLOBAL)
+    rng = np.random.default_rng(99)
+    c = ctypes.c_double
+    for order in (2, 3):
+        npe = order ** 3
+        k = [-1.0, 1.0] if order == 2 else [-1.0, 0.0, 1.0]
+        base = np.array([[x, y, z] for z in k for y in k for x in k])
+        for _ in range(3):
+            X = (base * rng.uniform(0.5, 2.0, 3) + 0.05 * rng.standard_normal((npe, 3))).reshape(-1)
+            u = rng.standard_normal(3 * npe)
+            res = np.zeros(3 * npe)
+            mat = np.zeros(9 * npe * npe)
+            R.ref_solid_element(order, X.ctypes.data_as(ctypes.c_void_p),
+                                u.ctypes.data_as(ctypes.c_void_p), c(200e9), c(0.25), c(7800.0),
+                                c(0.6), c(0.5), c(2.0), res.ctypes.data_as(ctypes.c_void_p),
+                                mat.ctypes.data_as(ctypes.c_void_p))
+            r2, m2 = O.solid_element(order, X, u, O.iso_C21(200e9, 0.25), t=0.6, rho=7800.0,
+                                     alpha=0.5, gamma=2.0)
+            assert np.abs(r2 - res).max() <= 1e-13 * np.abs(res).max()
+            assert np.abs(m2.reshape(-1) - mat).max() <= 1e-13 * np.abs(mat).max()
