@@ -34,10 +34,12 @@ def log(*a):
 
 
 class HipEvents:
-    """HIP events on the library's stream (the null stream)."""
+    """HIP events on the library's stream (the null stream), created through
+    the HIP runtime the library is bound to."""
 
     def __init__(self):
-        self.hip = ctypes.CDLL("libamdhip64.so")
+        from tacs_amd._lib import hip_runtime
+        self.hip = hip_runtime()
         self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float),
                                                  ctypes.c_void_p, ctypes.c_void_p]
 

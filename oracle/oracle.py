@@ -36,6 +36,11 @@ def lib():
         L.orc_gmres_capped.restype = ctypes.c_int
         L.orc_bsr_diag_inv.restype = ctypes.c_int
         L.orc_compliance.restype = ctypes.c_double
+        L.orc_quad4_adj_product.restype = ctypes.c_double
+        L.orc_quad4_failure.restype = ctypes.c_double
+        L.orc_shell_failure.restype = ctypes.c_double
+        L.orc_shell_compliance.restype = ctypes.c_double
+        L.orc_assemble_shell.restype = ctypes.c_long
         _lib = L
     return _lib
 
@@ -232,3 +237,113 @@ def ks_svsens(order, conn, X, u, C21, ys, ks_weight, max_fail, ks_sum,
                         ctypes.c_double(max_fail), ctypes.c_double(ks_sum),
                         ctypes.c_double(alpha), _p(dfdu))
     return dfdu
+
+
+# ---------------------------------------------------------------- shells
+_c = ctypes.c_double
+
+
+def quad4_strains(X, v, drill=True):
+    e = np.zeros((4, 9))
+    detw = np.zeros(4)
+    lib().orc_quad4_strains(_p(_dd(X)), _p(_dd(v)), 1 if drill else 0, _p(e), _p(detw))
+    return e, detw
+
+
+def quad4_element(X, v, E, nu, t, alpha=1.0):
+    res = np.zeros(24)
+    mat = np.zeros(576)
+    lib().orc_quad4_element(_p(_dd(X)), _p(_dd(v)), _c(E), _c(nu), _c(t), _c(alpha),
+                            _p(res), _p(mat))
+    return res, mat.reshape(24, 24)
+
+
+def quad4_adj_product(X, v, psi, E, nu, t):
+    return lib().orc_quad4_adj_product(_p(_dd(X)), _p(_dd(v)), _p(_dd(psi)), _c(E),
+                                       _c(nu), _c(t))
+
+
+def quad4_failure(X, v, E, nu, ys, t, n):
+    det = ctypes.c_double(0.0)
+    f = lib().orc_quad4_failure(_p(_dd(X)), _p(_dd(v)), _c(E), _c(nu), _c(ys), _c(t), n,
+                                ctypes.byref(det))
+    return f, det.value
+
+
+def quad4_pressure(X, p):
+    res = np.zeros(24)
+    lib().orc_quad4_pressure(_p(_dd(X)), _c(p), _p(res))
+    return res
+
+
+def shell_stress(E, nu, t, e):
+    s = np.zeros(9)
+    lib().orc_shell_stress(_c(E), _c(nu), _c(t), _p(_dd(e)), _p(s))
+    return s
+
+
+def shell_failure(E, nu, ys, t, e):
+    de = np.zeros(9)
+    dt = ctypes.c_double(0.0)
+    f = lib().orc_shell_failure(_c(E), _c(nu), _c(ys), _c(t), _p(_dd(e)), _p(de),
+                                ctypes.byref(dt))
+    return f, de, dt.value
+
+
+def assemble_shell(conn, X, u, t_elem, E, nu, rowp, cols):
+    conn = _ii(conn).reshape(-1)
+    nnodes = np.asarray(X).size // 3
+    A = np.zeros(36 * cols.size)
+    res = np.zeros(6 * nnodes)
+    miss = lib().orc_assemble_shell(conn.size // 4, _p(conn, np.int32), _p(_dd(X)),
+                                    _p(_dd(u)) if u is not None else None,
+                                    _p(_dd(t_elem)), _c(E), _c(nu),
+                                    _p(_ii(rowp), np.int32), _p(_ii(cols), np.int32),
+                                    _p(A), _p(res))
+    assert miss == 0
+    return A, res
+
+
+def shell_add_pressure(conn, X, aux_elems, pressures, res):
+    conn = _ii(conn).reshape(-1)
+    aux = _ii(aux_elems)
+    lib().orc_shell_add_pressure(aux.size, _p(aux, np.int32), _p(_dd(pressures)),
+                                 _p(conn, np.int32), _p(_dd(X)), _p(res))
+    return res
+
+
+def shell_compliance(conn, X, u, t_elem, E, nu):
+    conn = _ii(conn).reshape(-1)
+    return lib().orc_shell_compliance(conn.size // 4, _p(conn, np.int32), _p(_dd(X)),
+                                      _p(_dd(u)), _p(_dd(t_elem)), _c(E), _c(nu))
+
+
+def shell_ks_failure(conn, X, u, t_elem, E, nu, ys, ks_weight, alpha=1.0, safety=1.0):
+    conn = _ii(conn).reshape(-1)
+    out = np.zeros(3)
+    lib().orc_shell_ks_failure(conn.size // 4, _p(conn, np.int32), _p(_dd(X)), _p(_dd(u)),
+                               _p(_dd(t_elem)), _c(E), _c(nu), _c(ys), _c(ks_weight),
+                               _c(alpha), _c(safety), _p(out))
+    return out
+
+
+def shell_ks_sens(conn, X, u, t_elem, E, nu, ys, ks_weight, max_fail, ks_sum, safety=1.0):
+    """(dKS/du, dKS/dt_e) of the continuous KS failure function."""
+    conn = _ii(conn).reshape(-1)
+    nelem = conn.size // 4
+    dfdu = np.zeros(np.asarray(u).size)
+    dfdt = np.zeros(nelem)
+    lib().orc_shell_ks_sens(nelem, _p(conn, np.int32), _p(_dd(X)), _p(_dd(u)),
+                            _p(_dd(t_elem)), _c(E), _c(nu), _c(ys), _c(ks_weight),
+                            _c(safety), _c(max_fail), _c(ks_sum), _p(dfdu), _p(dfdt))
+    return dfdu, dfdt
+
+
+def shell_adj_res_product(conn, X, u, psi, t_elem, E, nu, scale=1.0):
+    conn = _ii(conn).reshape(-1)
+    nelem = conn.size // 4
+    out = np.zeros(nelem)
+    lib().orc_shell_adj_res_product(nelem, _p(conn, np.int32), _p(_dd(X)), _p(_dd(u)),
+                                    _p(_dd(psi)), _p(_dd(t_elem)), _c(E), _c(nu),
+                                    _c(scale), _p(out))
+    return out

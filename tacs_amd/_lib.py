@@ -66,6 +66,15 @@ def lib():
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python tacs_amd/build.py` "
                 "(or __graft_entry__.build()); there is no fallback implementation")
+        # PyTorch-ROCm ships its own HIP/ROCr runtime.  Two runtimes in one
+        # process compete for the device (the second one finds no GPU), so
+        # torch, when installed, is loaded first: the library's libamdhip64
+        # dependency then resolves to the runtime already in the process and
+        # torch tensors, the library and its null stream share one runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in parse_header().items():
             f = getattr(L, name)
@@ -73,6 +82,13 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def hip_runtime():
+    """ctypes handle through which the HIP runtime the library is bound to is
+    reached (dlsym on the library's handle searches its dependencies), e.g.
+    hip_runtime().hipEventCreate."""
+    return ctypes.CDLL(LIB_PATH) if lib() is not None else None
 
 
 class TACSError(RuntimeError):

@@ -86,3 +86,47 @@ def simp_thickness(x, p=3.0, E0=1.0, Emin=1e-9):
 
 def simp_dthickness(x, p=3.0, E0=1.0, Emin=1e-9):
     return p * x ** (p - 1.0) * (E0 - Emin) / E0
+
+
+class PlateMesh:
+    """A flat structured plate of 4-node shell elements in the z = 0 plane.
+
+    Node (i, j) (i along x, 0..nx; j along y, 0..ny) has id j*(nx+1) + i;
+    element (i, j) has id j*nx + i and local nodes (i,j), (i+1,j), (i,j+1),
+    (i+1,j+1) -- the tensor-product order of TACSShellQuadBasis<2>
+    (src/elements/shell/TACSShellElementQuadBasis.h:147-150).
+    """
+
+    def __init__(self, nx, ny, Lx=1.0, Ly=1.0, origin=(0.0, 0.0)):
+        self.shape = (nx, ny)
+        x = origin[0] + np.linspace(0.0, Lx, nx + 1)
+        y = origin[1] + np.linspace(0.0, Ly, ny + 1)
+        Y, Xg = np.meshgrid(y, x, indexing="ij")
+        self.X = np.stack([Xg.reshape(-1), Y.reshape(-1), np.zeros(Xg.size)], axis=1)
+        ids = np.arange((nx + 1) * (ny + 1), dtype=np.int64).reshape(ny + 1, nx + 1)
+        self.node_ids = ids
+        ej, ei = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+        ei, ej = ei.reshape(-1), ej.reshape(-1)
+        self.conn = np.stack([ids[ej, ei], ids[ej, ei + 1], ids[ej + 1, ei],
+                              ids[ej + 1, ei + 1]], axis=1).astype(np.int32)
+        self.elem_ij = np.stack([ei, ej], axis=1)
+
+    @property
+    def nnodes(self):
+        return self.X.shape[0]
+
+    @property
+    def nelem(self):
+        return self.conn.shape[0]
+
+    npe = 4
+
+    def edge_nodes(self):
+        """Ids of the nodes on the plate's perimeter, ascending."""
+        ids = self.node_ids
+        edge = np.concatenate([ids[0], ids[-1], ids[:, 0], ids[:, -1]])
+        return np.unique(edge).astype(np.int32)
+
+    def center_node(self):
+        nx, ny = self.shape
+        return int(self.node_ids[ny // 2, nx // 2])

@@ -1,0 +1,165 @@
+"""Pins of the oracle's Quad4 shell restatement (oracle/shell_oracle.c).
+
+- element residual, Jacobian, thickness adjoint product, failure index,
+  detXd and pressure load against values the reference's own element code
+  wrote (tests/golden/elements.npz, made by tests/golden/make_fixtures.py);
+- the 10x10 clamped plate of tests/integration_tests/test_shell_plate_quad.py
+  (pressure 100 kPa and a 10 kN centre load): compliance and KS failure
+  (ksWeight 10) against the golden values held in that test (:32-46);
+- rigid-body null space and symmetry of the element stiffness;
+- central finite differences of the KS failure sensitivities and of the
+  thickness adjoint product.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tacs_amd.meshes import PlateMesh
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+E, NU, YS = 70e9, 0.3, 270e6
+
+
+@pytest.fixture(scope="module")
+def fixtures():
+    return np.load(os.path.join(HERE, "golden", "elements.npz"))
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_quad4_against_reference_element(fixtures, case):
+    k = f"quad4_{case}"
+    X, v, psi = fixtures[k + "_X"], fixtures[k + "_v"], fixtures[k + "_psi"]
+    t = float(fixtures[k + "_t"].reshape(-1)[0])
+    res, mat = O.quad4_element(X, v, E, NU, t)
+    R, M = fixtures[k + "_res"], fixtures[k + "_mat"].reshape(24, 24)
+    assert np.abs(res - R).max() <= 1e-14 * np.abs(R).max()
+    assert np.abs(mat - M).max() <= 1e-14 * np.abs(M).max()
+    adj = O.quad4_adj_product(X, v, psi, E, NU, t)
+    ref_adj = float(fixtures[k + "_adj"].reshape(-1)[0])
+    assert abs(adj - ref_adj) <= 1e-13 * abs(ref_adj)
+    for n in range(4):
+        f, det = O.quad4_failure(X, v, E, NU, YS, t, n)
+        assert abs(f - fixtures[k + "_fail"][n]) <= 1e-13 * abs(fixtures[k + "_fail"][n])
+        assert abs(det - fixtures[k + "_det"][n]) <= 1e-14 * abs(fixtures[k + "_det"][n])
+    pres = O.quad4_pressure(X, 1e4)
+    assert np.abs(pres - fixtures[k + "_pres"]).max() <= 1e-14 * np.abs(fixtures[k + "_pres"]).max()
+
+
+def _warped_quad(rng):
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], float) * 0.3
+    X += rng.uniform(-0.03, 0.03, X.shape)
+    return X.reshape(-1)
+
+
+def test_quad4_rigid_body_and_symmetry():
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        X = _warped_quad(rng)
+        _, K = O.quad4_element(X, np.zeros(24), E, NU, 0.01)
+        assert np.abs(K - K.T).max() <= 1e-14 * np.abs(K).max()
+        Xn = X.reshape(4, 3)
+        for ax in range(3):
+            tr = np.zeros((4, 6))
+            tr[:, ax] = 1.0
+            th = np.zeros(3)
+            th[ax] = 1.0
+            rot = np.zeros((4, 6))
+            rot[:, :3] = np.cross(th, Xn - Xn.mean(0))
+            rot[:, 3:] = th
+            for mode in (tr, rot):
+                r = K @ mode.reshape(-1)
+                assert np.abs(r).max() <= 1e-10 * np.abs(K).max() * np.abs(mode).max()
+
+
+def _plate_dense(m, t):
+    rowp, cols = O.node_csr(m.nnodes, m.conn, 4)
+    A, _ = O.assemble_shell(m.conn, m.X, None, np.full(m.nelem, t), E, NU, rowp, cols)
+    n = m.nnodes
+    K = np.zeros((6 * n, 6 * n))
+    for i in range(n):
+        for j in range(rowp[i], rowp[i + 1]):
+            K[6 * i:6 * i + 6, 6 * cols[j]:6 * cols[j] + 6] = A[36 * j:36 * j + 36].reshape(6, 6)
+    return K
+
+
+@pytest.mark.parametrize("case,comp,ks", [
+    ("pressure", 4679.345460332448, 1.2938623156883402),
+    ("point_load", 683.8571611645635, 0.5757488025912149),
+])
+def test_plate_golden(case, comp, ks):
+    """tests/integration_tests/test_shell_plate_quad.py: 1 m x 1 m plate of
+    10 x 10 Quad4 shells, t = 5 mm, E = 70 GPa, nu = 0.3, ys = 464 MPa,
+    perimeter clamped in all six dofs."""
+    m = PlateMesh(10, 10)
+    t, ys = 0.005, 464e6
+    K = _plate_dense(m, t)
+    n = m.nnodes
+    f = np.zeros(6 * n)
+    if case == "pressure":
+        O.shell_add_pressure(m.conn, m.X, np.arange(m.nelem), np.full(m.nelem, 100e3), f)
+        f = -f
+    else:
+        f[6 * m.center_node() + 2] = 1e4
+    free = np.ones(6 * n, bool)
+    for b in m.edge_nodes():
+        free[6 * b:6 * b + 6] = False
+    u = np.zeros(6 * n)
+    u[free] = np.linalg.solve(K[np.ix_(free, free)], f[free])
+    te = np.full(m.nelem, t)
+    c = O.shell_compliance(m.conn, m.X, u, te, E, NU)
+    k = O.shell_ks_failure(m.conn, m.X, u, te, E, NU, ys, 10.0)[0]
+    # the golden values come from an iterative solve; they agree to ~3e-12
+    assert abs(c - comp) <= 1e-10 * comp
+    assert abs(k - ks) <= 1e-10 * ks
+
+
+def _small_shell(rng, n=3):
+    m = PlateMesh(n, n, 0.4, 0.4)
+    X = m.X.copy()
+    X[:, 2] += rng.uniform(-0.01, 0.01, m.nnodes)
+    t = rng.uniform(0.005, 0.02, m.nelem)
+    u = rng.normal(0.0, 1e-4, 6 * m.nnodes)
+    return m, X.reshape(-1), t, u
+
+
+def test_shell_ks_sensitivities_fd():
+    rng = np.random.default_rng(5)
+    m, X, t, u = _small_shell(rng)
+    w = 30.0
+    val, mx, s = O.shell_ks_failure(m.conn, X, u, t, E, NU, YS, w)
+    dfdu, dfdt = O.shell_ks_sens(m.conn, X, u, t, E, NU, YS, w, mx, s)
+    h = 1e-9
+    for i in rng.choice(u.size, 12, replace=False):
+        up, um = u.copy(), u.copy()
+        up[i] += h
+        um[i] -= h
+        fd = (O.shell_ks_failure(m.conn, X, up, t, E, NU, YS, w)[0]
+              - O.shell_ks_failure(m.conn, X, um, t, E, NU, YS, w)[0]) / (2 * h)
+        assert abs(fd - dfdu[i]) <= 1e-5 * np.abs(dfdu).max()
+    h = 1e-7
+    for e in range(m.nelem):
+        tp, tm = t.copy(), t.copy()
+        tp[e] += h
+        tm[e] -= h
+        fd = (O.shell_ks_failure(m.conn, X, u, tp, E, NU, YS, w)[0]
+              - O.shell_ks_failure(m.conn, X, u, tm, E, NU, YS, w)[0]) / (2 * h)
+        assert abs(fd - dfdt[e]) <= 1e-5 * np.abs(dfdt).max()
+
+
+def test_shell_adjoint_product_fd():
+    rng = np.random.default_rng(7)
+    m, X, t, u = _small_shell(rng)
+    psi = rng.normal(size=u.size)
+    rowp, cols = O.node_csr(m.nnodes, m.conn, 4)
+    prod = O.shell_adj_res_product(m.conn, X, u, psi, t, E, NU)
+    h = 1e-7
+    for e in range(m.nelem):
+        tp, tm = t.copy(), t.copy()
+        tp[e] += h
+        tm[e] -= h
+        rp = O.assemble_shell(m.conn, X, u, tp, E, NU, rowp, cols)[1]
+        rm = O.assemble_shell(m.conn, X, u, tm, E, NU, rowp, cols)[1]
+        fd = psi @ (rp - rm) / (2 * h)
+        assert abs(fd - prod[e]) <= 1e-6 * np.abs(prod).max()
