@@ -66,6 +66,20 @@ class TACSSolidConstitutive : public TACSConstitutive {
   int tNum;
 };
 
+// src/constitutive/TACSShellConstitutive.h, TACSIsoShellConstitutive.h:33
+class TACSShellConstitutive : public TACSConstitutive {};
+class TACSIsoShellConstitutive : public TACSShellConstitutive {
+ public:
+  TACSIsoShellConstitutive(TACSMaterialProperties* props, TacsScalar t = 1.0, int tNum = -1,
+                           TacsScalar tlb = 0.0, TacsScalar tub = 10.0,
+                           TacsScalar tOffset = 0.0, TacsScalar kcorr = 5.0 / 6.0);
+  ~TACSIsoShellConstitutive();
+  TACSMaterialProperties* getMaterialProperties() { return props; }
+  TACSMaterialProperties* props;
+  TacsScalar t, tlb, tub, tOffset, kcorr;
+  int tNum;
+};
+
 // src/elements/basis/TACSHexaBasis.h
 class TACSElementBasis : public TACSObject {
  public:
@@ -130,6 +144,34 @@ class TACSTraction3D : public TACSElement {
   int varsPerNode, faceIndex, coordComponent;
   TACSElementBasis* basis;
   TacsScalar trac[24];
+};
+
+// src/elements/shell/TACSShellElementTransform.h:8-92
+class TACSShellTransform : public TACSObject {};
+class TACSShellNaturalTransform : public TACSShellTransform {};
+
+// TACSQuad4Shell: src/elements/shell/TACSShellElementDefs.h:18-20 over
+// TACSShellElement (src/elements/shell/TACSShellElement.h:39); linear model,
+// linearized rotations, 6 variables per node.
+class TACSQuad4Shell : public TACSElement {
+ public:
+  TACSQuad4Shell(TACSShellTransform* transform, TACSShellConstitutive* con);
+  ~TACSQuad4Shell();
+  int getVarsPerNode() { return 6; }
+  int getNumNodes() { return 4; }
+  // TACSShellElement::createElementPressure (TACSShellElement.h:109)
+  TACSElement* createElementPressure(int faceIndex, TacsScalar p);
+  TACSShellTransform* transform;
+  TACSShellConstitutive* con;
+};
+
+// src/elements/shell/TACSShellPressure.h:9 (uniform pressure)
+class TACSShellPressure : public TACSElement {
+ public:
+  TACSShellPressure(TacsScalar p) : p(p) {}
+  int getVarsPerNode() { return 6; }
+  int getNumNodes() { return 4; }
+  TacsScalar p;
 };
 
 // src/TACSAuxElements.h:52-101
@@ -402,9 +444,14 @@ class TACSAssembler : public TACSObject {
 
   // internals
   int elementVectorsToNodes(const double* ev, double* out, int accumulate);
+  int shellKernel(int prec, int mode, const double* psi, double scale, double ks_weight,
+                  double ks_max, double ks_sum, double safety, void* ke, double* ev, double* es);
+  void addAuxResidual(double* r, TacsScalar lambda);
   int scatterElementScalarsToDVs(const double* es, double* dfdx);
   int uploadAux();
 
+  enum { FAMILY_SOLID = 0, FAMILY_SHELL = 1 };
+  int family;
   int varsPerNode, numNodes, numElements, npe, order, initialized;
   int numDesignVars;
   int* h_ptr;
@@ -414,6 +461,8 @@ class TACSAssembler : public TACSObject {
   TACSAuxElements* auxElements;
   int aux_version;
   SolidMaterial mat;
+  ShellMaterial smat;
+  double* d_ke;  // shell element matrices [nelem][24][24]
   // device data
   int *d_conn, *d_n2e_ptr, *d_n2e, *d_bcvars, *d_bcnodes, *d_bcmask;
   double* d_bcvals;
@@ -423,7 +472,7 @@ class TACSAssembler : public TACSObject {
   double *d_X, *d_u;
   double *d_ev, *d_es;
   int *d_aux_elem, *d_aux_face, *d_n2a_ptr, *d_n2a;
-  double *d_aux_trac, *d_aux_ev;
+  double *d_aux_trac, *d_aux_ev, *d_aux_pres;
   int naux;
   int* d_status;
   void* d_red;

@@ -43,6 +43,20 @@ typedef struct SolidMaterial {
   double lam, mu;
 } SolidMaterial;
 
+/* Isotropic shell material of TACSIsoShellConstitutive
+   (src/constitutive/TACSIsoShellConstitutive.cpp:153-189): plane-stress Q
+   (TACSMaterialProperties::evalTangentStiffness2D
+   src/constitutive/TACSMaterialProperties.cpp:323), shear correction kcorr,
+   drilling regularization (TACSShellConstitutive.cpp:61), yield stress and
+   density.  tOffset = 0. */
+typedef struct ShellMaterial {
+  double Q[6];
+  double kcorr;
+  double drill_reg;
+  double ys;
+  double rho;
+} ShellMaterial;
+
 /* ------------------------------------------------------------------------
    1. Kernel entry points
    ------------------------------------------------------------------------ */
@@ -113,6 +127,56 @@ int tacs_hip_scatter_dv_sens(int ndv, const int* dv2e_ptr, const int* dv2e, cons
 /* Dirichlet values on a vector (TACSBVec::applyBCs src/bpmat/TACSBVec.cpp:546). */
 int tacs_hip_apply_bc_vec(int b, int nbc, const int* nodes, const int* masks, const double* vals,
                           const double* u, double lambda, double* x, hipStream_t stream);
+
+/* Linear MITC4 shell (TACSQuad4Shell = TACSShellElement<TACSQuadLinearQuadrature,
+   TACSShellQuadBasis<2>, TACSLinearizedRotation, TACSShellLinearModel>,
+   src/elements/shell/TACSShellElementDefs.h:18-20, natural transform) over
+   nelem 4-node elements; node-major variables, 6 per node.  mode:
+   0 element stiffness ke[nelem][24][24] (row-major, scaled by `scale`;
+     TACSShellElement::addJacobian TACSShellElement.h:435) and, when ev is
+     given, the element residual K u (addResidual :295); prec 64 or 32
+     (ke in FP64 or FP32; the other modes are FP64 only),
+   1 element residual ev (addResidual :295, static state),
+   2 compliance integrand es (TACSCompliance::elementWiseEval
+     src/functions/TACSCompliance.cpp:69, strain energy density),
+   3 / 4 KS max / KS sum es (TACSKSFailure::elementWiseEval
+     src/functions/TACSKSFailure.cpp:125; failure of
+     TACSIsoShellConstitutive::evalFailure :254),
+   5 KS d/du ev (TACSKSFailure::getElementSVSens :221 ->
+     TACSShellElement::addPointQuantitySVSens :1353), coef = scale,
+   6 KS d/dt es (TACSKSFailure::addElementDVSens -> addPointQuantityDVSens
+     :1193 -> TACSIsoShellConstitutive::addFailureDVSens :381),
+   7 scale * psi^T dR/dt es (TACSShellElement::addAdjResProduct :917 ->
+     TACSIsoShellConstitutive::addStressDVSens :229),
+   8 compliance d/du ev, 9 compliance d/dt es (same call chains as 5, 6). */
+int tacs_hip_shell_elem_kernel(int prec, int mode, int nelem, const int* conn, const double* X,
+                               const double* u, const double* psi, const double* telem,
+                               const ShellMaterial* mat, double scale, double ks_weight,
+                               double ks_max, double ks_sum, double safety, void* ke, double* ev,
+                               double* es, hipStream_t stream);
+
+/* Sum element matrices ke (FP64 when prec = 64, FP32 when 32) into b x b
+   block-CSR rows in ascending element order, Dirichlet rows fused
+   (TACSAssembler::addMatValues src/TACSAssembler.h:545 -> BCSRMat::addRowValues
+   src/bpmat/BCSRMat.cpp:1800; TACSParallelMat::applyBCs
+   src/bpmat/TACSParallelMat.cpp:343).  b = 6, npe = 4. */
+int tacs_hip_gather_elem_matrices(int b, int npe, int prec, int nrows, const int* n2e_ptr,
+                                  const int* n2e, const int* conn, const void* ke, const int* rowp,
+                                  const int* cols, const int* bcvars, double* A, float* A32,
+                                  hipStream_t stream);
+
+/* Node gather of element vectors with b values per node (the reverse of
+   TACSBVec::setValues ADD src/bpmat/TACSBVec.cpp:757): (b, nv) = (6, 24),
+   (3, 24), (3, 81). */
+int tacs_hip_gather_elem_vectors_b(int b, int nv, int nnodes, const int* n2e_ptr, const int* n2e,
+                                   const double* ev, int accumulate, double* out,
+                                   hipStream_t stream);
+
+/* Uniform-pressure shell aux-element vectors ev[naux][24]
+   (TACSShellPressure::addResidual src/elements/shell/TACSShellPressure.h:51). */
+int tacs_hip_shell_pressure_vectors(int naux, const int* aux_elem, const double* pressure,
+                                    const int* conn, const double* X, double lambda, double* ev,
+                                    hipStream_t stream);
 
 /* y (+)= A x, b x b block-CSR, b = 3 or 6 (BCSRMat::mult
    src/bpmat/BCSRMat.cpp:1226 -> BCSRMatVecMult3 src/bpmat/BCSRMatMult3.cpp:28,
@@ -199,6 +263,18 @@ tacs_obj tacs_linear_elasticity3d_create(tacs_obj stiff, int strain_type);
 tacs_obj tacs_element3d_create(tacs_obj model, tacs_obj basis);
 tacs_obj tacs_traction3d_create(int vars_per_node, int face, tacs_obj basis,
                                 const double* trac, int coord_component);
+/* TACSIsoShellConstitutive(props, t, tNum, tlb, tub, tOffset, kcorr)
+   (src/constitutive/TACSIsoShellConstitutive.h:33; tacs/constitutive.pyx:1109) */
+tacs_obj tacs_iso_shell_constitutive_create(tacs_obj props, double t, int tNum, double tlb,
+                                            double tub, double tOffset, double kcorr);
+/* TACSShellNaturalTransform (src/elements/shell/TACSShellElementTransform.h:21) */
+tacs_obj tacs_shell_natural_transform_create(void);
+/* TACSQuad4Shell(transform, con) (src/elements/shell/TACSShellElementDefs.h:18;
+   tacs/elements.pyx:694) */
+tacs_obj tacs_quad4_shell_create(tacs_obj transform, tacs_obj con);
+/* TACSShellElement::createElementPressure(faceIndex, p)
+   (src/elements/shell/TACSShellElement.h:109): a uniform-pressure aux element */
+tacs_obj tacs_element_create_pressure(tacs_obj element, int face, double p);
 /* TACSAuxElements (src/TACSAuxElements.h:52) */
 tacs_obj tacs_aux_elements_create(int size);
 int tacs_aux_elements_add(tacs_obj aux, int elem_index, tacs_obj element);

@@ -347,3 +347,14 @@ def shell_adj_res_product(conn, X, u, psi, t_elem, E, nu, scale=1.0):
                                     _p(_dd(psi)), _p(_dd(t_elem)), _c(E), _c(nu),
                                     _c(scale), _p(out))
     return out
+
+
+def shell_compliance_sens(conn, X, u, t_elem, E, nu):
+    """(dC/du, dC/dt_e at fixed u) of the shell compliance."""
+    conn = _ii(conn).reshape(-1)
+    nelem = conn.size // 4
+    dfdu = np.zeros(np.asarray(u).size)
+    dfdt = np.zeros(nelem)
+    lib().orc_shell_compliance_sens(nelem, _p(conn, np.int32), _p(_dd(X)), _p(_dd(u)),
+                                    _p(_dd(t_elem)), _c(E), _c(nu), _p(dfdu), _p(dfdt))
+    return dfdu, dfdt

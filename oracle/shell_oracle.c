@@ -721,3 +721,42 @@ void orc_shell_adj_res_product(int nelem, const int *conn, const double *X,
     dfdx[el] += scale * orc_quad4_adj_product(Xe, ue, pe, E, nu, t_elem[el]);
   }
 }
+
+/* dC/du of the compliance (TACSShellElement::addPointQuantitySVSens,
+   TACS_STRAIN_ENERGY_DENSITY: 2 s with e[8] = 0 through the transposed
+   strain map) and dC/dt_e at fixed u (addPointQuantityDVSens ->
+   TACSIsoShellConstitutive::addStressDVSens(e, e), :229).  Either output
+   may be NULL; both accumulate. */
+void orc_shell_compliance_sens(int nelem, const int *conn, const double *X,
+                               const double *u, const double *t_elem, double E,
+                               double nu, double *dfdu, double *dfdt) {
+  double Q[6];
+  sh_q(E, nu, Q);
+  for (int el = 0; el < nelem; el++) {
+    const int *c = &conn[4 * el];
+    double Xe[12], ue[24], e[4][9], detw[4], B[4][9][SH_NV];
+    sh_gather(c, X, u, Xe, ue);
+    orc_quad4_strains(Xe, ue, 0, e, detw);
+    if (dfdu) sh_bmat(Xe, 1, B, detw);
+    double t = t_elem[el];
+    for (int q = 0; q < 4; q++) {
+      double s[9];
+      orc_shell_stress(E, nu, t, e[q], s);
+      if (dfdu) {
+        for (int a = 0; a < SH_NV; a++) {
+          double acc = 0.0;
+          for (int k = 0; k < 9; k++) acc += B[q][k][a] * 2.0 * s[k];
+          dfdu[6 * c[a / 6] + a % 6] += detw[q] * acc;
+        }
+      }
+      if (dfdt) {
+        double a3[3], b3[3];
+        sh_symm3(Q, e[q], a3);
+        sh_symm3(Q, &e[q][3], b3);
+        dfdt[el] += detw[q] * (a3[0] * e[q][0] + a3[1] * e[q][1] + a3[2] * e[q][2] +
+                               0.25 * t * t * (b3[0] * e[q][3] + b3[1] * e[q][4] + b3[2] * e[q][5]) +
+                               sh_kcorr * Q[5] * (e[q][6] * e[q][6] + e[q][7] * e[q][7]));
+      }
+    }
+  }
+}

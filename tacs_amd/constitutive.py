@@ -19,3 +19,14 @@ class SolidConstitutive(TACSHandle):
         self.t, self.tNum = t, tNum
         super().__init__(lib().tacs_solid_constitutive_create(props.ptr, t, tNum, tlb, tub),
                          keep=(props,))
+
+
+class IsoShellConstitutive(TACSHandle):
+    """TACSIsoShellConstitutive (tacs/constitutive.pyx:1109): isotropic shell
+    with thickness t (design variable tNum), shear correction kcorr."""
+
+    def __init__(self, props, t=1.0, tNum=-1, tlb=0.0, tub=10.0, tOffset=0.0, kcorr=5.0 / 6.0):
+        self.t, self.tNum = t, tNum
+        super().__init__(lib().tacs_iso_shell_constitutive_create(props.ptr, t, tNum, tlb, tub,
+                                                                  tOffset, kcorr),
+                         keep=(props,))

@@ -72,6 +72,23 @@ tacs_obj tacs_traction3d_create(int vpn, int face, tacs_obj basis, const double*
   }
   return (TACSObject*)new TACSTraction3D(vpn, face, b, trac, coord);
 }
+tacs_obj tacs_iso_shell_constitutive_create(tacs_obj props, double t, int tNum, double tlb,
+                                            double tub, double tOffset, double kcorr) {
+  REQUIRE_NULL(TACSMaterialProperties, p, props);
+  return (TACSObject*)new TACSIsoShellConstitutive(p, t, tNum, tlb, tub, tOffset, kcorr);
+}
+tacs_obj tacs_shell_natural_transform_create(void) {
+  return (TACSObject*)new TACSShellNaturalTransform();
+}
+tacs_obj tacs_quad4_shell_create(tacs_obj transform, tacs_obj con) {
+  REQUIRE_NULL(TACSShellTransform, tr, transform);
+  REQUIRE_NULL(TACSShellConstitutive, c, con);
+  return (TACSObject*)new TACSQuad4Shell(tr, c);
+}
+tacs_obj tacs_element_create_pressure(tacs_obj element, int face, double p) {
+  REQUIRE_NULL(TACSQuad4Shell, e, element);
+  return (TACSObject*)e->createElementPressure(face, p);
+}
 tacs_obj tacs_aux_elements_create(int size) { return (TACSObject*)new TACSAuxElements(size); }
 int tacs_aux_elements_add(tacs_obj aux, int elem, tacs_obj element) {
   REQUIRE(TACSAuxElements, a, aux);

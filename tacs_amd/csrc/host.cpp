@@ -131,6 +131,29 @@ TACSElement3D::~TACSElement3D() {
   basis->decref();
 }
 
+TACSIsoShellConstitutive::TACSIsoShellConstitutive(TACSMaterialProperties* p, TacsScalar _t,
+                                                   int _tNum, TacsScalar _tlb, TacsScalar _tub,
+                                                   TacsScalar _tOffset, TacsScalar _kcorr)
+    : props(p), t(_t), tlb(_tlb), tub(_tub), tOffset(_tOffset), kcorr(_kcorr), tNum(_tNum) {
+  if (props) props->incref();
+}
+TACSIsoShellConstitutive::~TACSIsoShellConstitutive() {
+  if (props) props->decref();
+}
+
+TACSQuad4Shell::TACSQuad4Shell(TACSShellTransform* tr, TACSShellConstitutive* c)
+    : transform(tr), con(c) {
+  transform->incref();
+  con->incref();
+}
+TACSQuad4Shell::~TACSQuad4Shell() {
+  transform->decref();
+  con->decref();
+}
+TACSElement* TACSQuad4Shell::createElementPressure(int, TacsScalar p) {
+  return new TACSShellPressure(p);
+}
+
 TACSTraction3D::TACSTraction3D(int vpn, int face, TACSElementBasis* b, const TacsScalar t[],
                                int coord)
     : varsPerNode(vpn), faceIndex(face), coordComponent(coord), basis(b) {
@@ -782,6 +805,10 @@ TACSAssembler::TACSAssembler(TACSComm comm, int vpn, int nowned, int nelems, int
   bcMap = new TACSBcMap(varsPerNode, 64);
   bcMap->incref();
   memset(&mat, 0, sizeof(mat));
+  memset(&smat, 0, sizeof(smat));
+  family = FAMILY_SOLID;
+  d_ke = NULL;
+  d_aux_pres = NULL;
   d_conn = d_n2e_ptr = d_n2e = d_bcvars = d_bcnodes = d_bcmask = NULL;
   d_bcvals = NULL;
   nbc_nodes = 0;
@@ -809,7 +836,8 @@ TACSAssembler::~TACSAssembler() {
   if (auxElements) auxElements->decref();
   void* ds[] = {d_conn, d_n2e_ptr, d_n2e, d_bcvars, d_bcnodes, d_bcmask, d_bcvals, d_dvnum,
                 d_dv2e_ptr, d_dv2e, d_tfixed, d_telem, d_X, d_u, d_ev, d_es, d_aux_elem,
-                d_aux_face, d_n2a_ptr, d_n2a, d_aux_trac, d_aux_ev, d_status, d_red, d_scalar};
+                d_aux_face, d_n2a_ptr, d_n2a, d_aux_trac, d_aux_ev, d_status, d_red, d_scalar,
+                d_ke, d_aux_pres};
   for (void* p : ds) dfree(p);
 }
 
@@ -866,41 +894,86 @@ int TACSAssembler::initialize() {
   if (initialized) return 0;
   if (!h_ptr || !h_conn) return tacs_set_error("initialize: connectivity not set");
   if (!elements) return tacs_set_error("initialize: elements not set");
-  if (varsPerNode != 3) return tacs_set_error("initialize: only 3 variables per node supported");
-  // Classify the element family: TACSElement3D + TACSLinearElasticity3D +
-  // TACSSolidConstitutive on one hexahedral basis and one material.
-  TACSMaterialProperties* props0 = NULL;
   std::vector<int> dvnum(numElements);
   std::vector<double> tfixed(numElements);
-  for (int e = 0; e < numElements; e++) {
-    TACSElement3D* el = dynamic_cast<TACSElement3D*>(elements[e]);
-    if (!el) return tacs_set_error("initialize: element %d is not a TACSElement3D", e);
-    TACSLinearElasticity3D* model = dynamic_cast<TACSLinearElasticity3D*>(el->model);
-    if (!model) return tacs_set_error("initialize: element %d model is not TACSLinearElasticity3D", e);
-    if (model->strain_type != TACS_LINEAR_STRAIN)
-      return tacs_set_error("initialize: only TACS_LINEAR_STRAIN is supported");
-    int o = el->basis->getOrder();
-    if (order == 0) order = o;
-    if (o != order) return tacs_set_error("initialize: mixed element bases are not supported");
-    TACSMaterialProperties* p = model->stiff->props;
-    if (!props0) props0 = p;
-    if (p != props0 && (p->E != props0->E || p->nu != props0->nu || p->rho != props0->rho ||
-                        p->ys != props0->ys))
-      return tacs_set_error("initialize: elements must share one material");
-    dvnum[e] = model->stiff->tNum;
-    tfixed[e] = model->stiff->t;
+  TACSMaterialProperties* props0 = NULL;
+  if (numElements > 0 && dynamic_cast<TACSQuad4Shell*>(elements[0])) {
+    // Shell family: TACSQuad4Shell + TACSShellNaturalTransform +
+    // TACSIsoShellConstitutive (tOffset = 0), one material.
+    family = FAMILY_SHELL;
+    if (varsPerNode != 6) return tacs_set_error("initialize: shells need 6 variables per node");
+    double kcorr = 0.0;
+    for (int e = 0; e < numElements; e++) {
+      TACSQuad4Shell* el = dynamic_cast<TACSQuad4Shell*>(elements[e]);
+      if (!el) return tacs_set_error("initialize: element %d is not a TACSQuad4Shell", e);
+      if (!dynamic_cast<TACSShellNaturalTransform*>(el->transform))
+        return tacs_set_error("initialize: only TACSShellNaturalTransform is supported");
+      TACSIsoShellConstitutive* con = dynamic_cast<TACSIsoShellConstitutive*>(el->con);
+      if (!con) return tacs_set_error("initialize: element %d constitutive is not isotropic", e);
+      if (con->tOffset != 0.0) return tacs_set_error("initialize: tOffset != 0 is not supported");
+      TACSMaterialProperties* p = con->props;
+      if (!props0) {
+        props0 = p;
+        kcorr = con->kcorr;
+      }
+      if (p != props0 && (p->E != props0->E || p->nu != props0->nu || p->rho != props0->rho ||
+                          p->ys != props0->ys))
+        return tacs_set_error("initialize: elements must share one material");
+      if (con->kcorr != kcorr) return tacs_set_error("initialize: elements must share kcorr");
+      dvnum[e] = con->tNum;
+      tfixed[e] = con->t;
+    }
+    order = 2;
+    npe = 4;
+    // plane-stress Q (TACSMaterialProperties::evalTangentStiffness2D :323)
+    double D = props0->E / (1.0 - props0->nu * props0->nu);
+    smat.Q[0] = D;
+    smat.Q[1] = props0->nu * D;
+    smat.Q[2] = 0.0;
+    smat.Q[3] = D;
+    smat.Q[4] = 0.0;
+    smat.Q[5] = props0->G;
+    smat.kcorr = kcorr;
+    smat.drill_reg = 0.1;  // TACSShellConstitutive::DRILLING_REGULARIZATION
+    smat.ys = props0->ys;
+    smat.rho = props0->rho;
+  } else {
+    // Solid family: TACSElement3D + TACSLinearElasticity3D +
+    // TACSSolidConstitutive on one hexahedral basis and one material.
+    family = FAMILY_SOLID;
+    if (varsPerNode != 3) return tacs_set_error("initialize: solids need 3 variables per node");
+    for (int e = 0; e < numElements; e++) {
+      TACSElement3D* el = dynamic_cast<TACSElement3D*>(elements[e]);
+      if (!el) return tacs_set_error("initialize: element %d is not a TACSElement3D", e);
+      TACSLinearElasticity3D* model = dynamic_cast<TACSLinearElasticity3D*>(el->model);
+      if (!model)
+        return tacs_set_error("initialize: element %d model is not TACSLinearElasticity3D", e);
+      if (model->strain_type != TACS_LINEAR_STRAIN)
+        return tacs_set_error("initialize: only TACS_LINEAR_STRAIN is supported");
+      int o = el->basis->getOrder();
+      if (order == 0) order = o;
+      if (o != order) return tacs_set_error("initialize: mixed element bases are not supported");
+      TACSMaterialProperties* p = model->stiff->props;
+      if (!props0) props0 = p;
+      if (p != props0 && (p->E != props0->E || p->nu != props0->nu || p->rho != props0->rho ||
+                          p->ys != props0->ys))
+        return tacs_set_error("initialize: elements must share one material");
+      dvnum[e] = model->stiff->tNum;
+      tfixed[e] = model->stiff->t;
+    }
+    npe = order * order * order;
+    props0->evalTangentStiffness3D(mat.C);
+    mat.rho = props0->rho;
+    mat.ys = props0->ys;
+    mat.isotropic = 1;
+    mat.mu = props0->G;
+    mat.lam = mat.C[1];  // nu*E/((1+nu)(1-2nu))
   }
-  npe = order * order * order;
+  if (!props0) return tacs_set_error("initialize: no elements");
   for (int e = 0; e < numElements; e++)
     if (h_ptr[e + 1] - h_ptr[e] != npe)
       return tacs_set_error("initialize: element %d has %d nodes, expected %d", e,
                             h_ptr[e + 1] - h_ptr[e], npe);
-  props0->evalTangentStiffness3D(mat.C);
-  mat.rho = props0->rho;
-  mat.ys = props0->ys;
-  mat.isotropic = 1;
-  mat.mu = props0->G;
-  mat.lam = mat.C[1];  // nu*E/((1+nu)(1-2nu))
 
   // node -> element CSR (TACSAssembler::computeNodeToElementCSR :1791),
   // entries (element << 5 | local index), ascending element order
@@ -918,7 +991,7 @@ int TACSAssembler::initialize() {
   // diagonal kept
   h_rowp = new int[numNodes + 1];
   std::vector<int> cols;
-  cols.reserve((size_t)numNodes * (order == 2 ? 27 : 64));
+  cols.reserve((size_t)numNodes * (family == FAMILY_SHELL ? 9 : (order == 2 ? 27 : 64)));
   h_rowp[0] = 0;
   std::vector<int> tmp;
   for (int i = 0; i < numNodes; i++) {
@@ -938,7 +1011,7 @@ int TACSAssembler::initialize() {
   memcpy(h_cols, cols.data(), nnzb * sizeof(int));
   int maxrow = 0;
   for (int i = 0; i < numNodes; i++) maxrow = std::max(maxrow, h_rowp[i + 1] - h_rowp[i]);
-  int maxc = (order == 2) ? 64 : 125;
+  int maxc = (family == FAMILY_SHELL) ? (1 << 30) : ((order == 2) ? 64 : 125);
   if (maxrow > maxc)
     return tacs_set_error("initialize: a node couples to %d nodes (limit %d)", maxrow, maxc);
 
@@ -948,30 +1021,31 @@ int TACSAssembler::initialize() {
     TacsScalar* bval;
     int nbc = bcMap->getBCs(&bn, &bv, &bval);
     std::vector<int> mask(numNodes, 0), order_nodes;
-    std::vector<double> vals((size_t)numNodes * 3, 0.0);
+    const int b = varsPerNode;
+    std::vector<double> vals((size_t)numNodes * b, 0.0);
     for (int q = 0; q < nbc; q++) {
       int nd = bn[q];
       if (!mask[nd] && bv[q]) order_nodes.push_back(nd);
-      for (int k = 0; k < 3; k++)
-        if (bv[q] & (1 << k)) vals[3 * nd + k] = bval[3 * q + k];
+      for (int k = 0; k < b; k++)
+        if (bv[q] & (1 << k)) vals[(size_t)b * nd + k] = bval[(size_t)b * q + k];
       mask[nd] |= bv[q];
     }
     std::sort(order_nodes.begin(), order_nodes.end());
     nbc_nodes = (int)order_nodes.size();
     std::vector<int> masks(nbc_nodes);
-    std::vector<double> bvals((size_t)nbc_nodes * 3);
+    std::vector<double> bvals((size_t)nbc_nodes * b);
     for (int q = 0; q < nbc_nodes; q++) {
       masks[q] = mask[order_nodes[q]];
-      for (int k = 0; k < 3; k++) bvals[3 * q + k] = vals[3 * order_nodes[q] + k];
+      for (int k = 0; k < b; k++) bvals[(size_t)b * q + k] = vals[(size_t)b * order_nodes[q] + k];
     }
     d_bcvars = dalloc<int>(numNodes);
     h2d(d_bcvars, mask.data(), numNodes);
     d_bcnodes = dalloc<int>(nbc_nodes);
     d_bcmask = dalloc<int>(nbc_nodes);
-    d_bcvals = dalloc<double>((size_t)3 * nbc_nodes);
+    d_bcvals = dalloc<double>((size_t)b * nbc_nodes);
     h2d(d_bcnodes, order_nodes.data(), nbc_nodes);
     h2d(d_bcmask, masks.data(), nbc_nodes);
-    h2d(d_bcvals, bvals.data(), (size_t)3 * nbc_nodes);
+    h2d(d_bcvals, bvals.data(), (size_t)b * nbc_nodes);
     HCHECK(hipStreamSynchronize(g_stream));
   }
 
@@ -1009,9 +1083,9 @@ int TACSAssembler::initialize() {
   h2d(d_dv2e_ptr, dv2e_ptr.data(), numDesignVars + 1);
   h2d(d_dv2e, dv2e.data(), dv2e.size());
   d_X = dalloc<double>((size_t)3 * numNodes);
-  d_u = dalloc<double>((size_t)3 * numNodes);
+  d_u = dalloc<double>((size_t)varsPerNode * numNodes);
   HCHECK(hipMemsetAsync(d_X, 0, (size_t)3 * numNodes * sizeof(double), g_stream));
-  HCHECK(hipMemsetAsync(d_u, 0, (size_t)3 * numNodes * sizeof(double), g_stream));
+  HCHECK(hipMemsetAsync(d_u, 0, (size_t)varsPerNode * numNodes * sizeof(double), g_stream));
   d_status = dalloc<int>(1);
   HCHECK(hipMemsetAsync(d_status, 0, sizeof(int), g_stream));
   size_t rb = tacs_hip_red_scratch_bytes();
@@ -1050,15 +1124,15 @@ void TACSAssembler::getNodes(TACSBVec* X) {
                         hipMemcpyDeviceToDevice, g_stream));
 }
 void TACSAssembler::setVariables(TACSBVec* q, TACSBVec*, TACSBVec*) {
-  HCHECK(hipMemcpyAsync(d_u, q->devRead(), (size_t)3 * numNodes * sizeof(double),
+  HCHECK(hipMemcpyAsync(d_u, q->devRead(), (size_t)varsPerNode * numNodes * sizeof(double),
                         hipMemcpyDeviceToDevice, g_stream));
 }
 void TACSAssembler::getVariables(TACSBVec* q, TACSBVec*, TACSBVec*) {
-  HCHECK(hipMemcpyAsync(q->devWrite(), d_u, (size_t)3 * numNodes * sizeof(double),
+  HCHECK(hipMemcpyAsync(q->devWrite(), d_u, (size_t)varsPerNode * numNodes * sizeof(double),
                         hipMemcpyDeviceToDevice, g_stream));
 }
 void TACSAssembler::zeroVariables() {
-  KCHECK(tacs_hip_fill(64, (size_t)3 * numNodes, d_u, 0.0, g_stream));
+  KCHECK(tacs_hip_fill(64, (size_t)varsPerNode * numNodes, d_u, 0.0, g_stream));
 }
 void TACSAssembler::setDesignVars(TACSBVec* dvs) {
   KCHECK(tacs_hip_gather_design_vars(numElements, d_dvnum, d_tfixed,
@@ -1076,8 +1150,8 @@ void TACSAssembler::getDesignVars(TACSBVec* dvs) {
 }
 void TACSAssembler::applyBCs(TACSVec* vec) {
   TACSBVec* v = dynamic_cast<TACSBVec*>(vec);
-  KCHECK(tacs_hip_apply_bc_vec(3, nbc_nodes, d_bcnodes, d_bcmask, NULL, NULL, 1.0, v->devWrite(),
-                               g_stream));
+  KCHECK(tacs_hip_apply_bc_vec(varsPerNode, nbc_nodes, d_bcnodes, d_bcmask, NULL, NULL, 1.0,
+                               v->devWrite(), g_stream));
 }
 void TACSAssembler::applyBCs(TACSMat* m) { m->applyBCs(bcMap); }
 
@@ -1094,8 +1168,16 @@ int TACSAssembler::uploadAux() {
   std::stable_sort(idx.begin(), idx.end(),
                    [&](int a, int b) { return auxElements->nums[a] < auxElements->nums[b]; });
   std::vector<int> ae(n), af(n);
-  std::vector<double> at((size_t)3 * n);
+  std::vector<double> at((size_t)3 * n), ap(n, 0.0);
   for (int q = 0; q < n; q++) {
+    if (family == FAMILY_SHELL) {
+      TACSShellPressure* pr = dynamic_cast<TACSShellPressure*>(auxElements->elems[idx[q]]);
+      if (!pr) return tacs_set_error("aux element %d is not a TACSShellPressure", idx[q]);
+      ae[q] = auxElements->nums[idx[q]];
+      af[q] = 0;
+      ap[q] = pr->p;
+      continue;
+    }
     TACSTraction3D* tr = dynamic_cast<TACSTraction3D*>(auxElements->elems[idx[q]]);
     if (!tr) return tacs_set_error("aux element %d is not a TACSTraction3D", idx[q]);
     if (!tr->coordComponent)
@@ -1114,7 +1196,7 @@ int TACSAssembler::uploadAux() {
   std::vector<int> pos(ptr.begin(), ptr.end() - 1);
   for (int q = 0; q < n; q++)
     for (int a = 0; a < npe; a++) list[pos[h_conn[(size_t)ae[q] * npe + a]]++] = (q << 5) | a;
-  void* old[] = {d_aux_elem, d_aux_face, d_aux_trac, d_n2a_ptr, d_n2a, d_aux_ev};
+  void* old[] = {d_aux_elem, d_aux_face, d_aux_trac, d_n2a_ptr, d_n2a, d_aux_ev, d_aux_pres};
   HCHECK(hipStreamSynchronize(g_stream));
   for (void* p : old) dfree(p);
   d_aux_elem = dalloc<int>(n);
@@ -1122,7 +1204,9 @@ int TACSAssembler::uploadAux() {
   d_aux_trac = dalloc<double>((size_t)3 * n);
   d_n2a_ptr = dalloc<int>(numNodes + 1);
   d_n2a = dalloc<int>(list.size());
-  d_aux_ev = dalloc<double>((size_t)n * 3 * npe);
+  d_aux_ev = dalloc<double>((size_t)n * varsPerNode * npe);
+  d_aux_pres = dalloc<double>(n);
+  h2d(d_aux_pres, ap.data(), n);
   h2d(d_aux_elem, ae.data(), n);
   h2d(d_aux_face, af.data(), n);
   h2d(d_aux_trac, at.data(), (size_t)3 * n);
@@ -1135,8 +1219,21 @@ int TACSAssembler::uploadAux() {
 }
 
 int TACSAssembler::elementVectorsToNodes(const double* ev, double* out, int accumulate) {
+  if (family == FAMILY_SHELL)
+    return tacs_hip_gather_elem_vectors_b(6, 24, numNodes, d_n2e_ptr, d_n2e, ev, accumulate, out,
+                                          g_stream);
   return tacs_hip_gather_elem_vectors(3 * npe, numNodes, d_n2e_ptr, d_n2e, ev, accumulate, out,
                                       g_stream);
+}
+
+// Shell element kernel on the assembler's state (mode numbers of
+// tacs_hip_shell_elem_kernel)
+int TACSAssembler::shellKernel(int prec, int mode, const double* psi, double scale,
+                               double ks_weight, double ks_max, double ks_sum, double safety,
+                               void* ke, double* ev, double* es) {
+  return tacs_hip_shell_elem_kernel(prec, mode, numElements, d_conn, d_X, d_u, psi, d_telem,
+                                    &smat, scale, ks_weight, ks_max, ks_sum, safety, ke, ev, es,
+                                    g_stream);
 }
 
 int TACSAssembler::scatterElementScalarsToDVs(const double* es, double* dfdx) {
@@ -1155,19 +1252,35 @@ void TACSAssembler::assembleRes(TACSBVec* residual, const TacsScalar lambda, con
     return;
   }
   double* r = residual->devWrite();
-  ensure(d_ev, (size_t)numElements * 3 * npe);
-  KCHECK(tacs_hip_solid_elem_vectors(order, 0, numElements, d_conn, d_X, d_u, d_telem, &mat, 1.0,
-                                     0, 0, 1, 1, d_ev, g_stream));
+  ensure(d_ev, (size_t)numElements * varsPerNode * npe);
+  if (family == FAMILY_SHELL)
+    KCHECK(shellKernel(64, 1, NULL, 1.0, 0, 0, 1, 1, NULL, d_ev, NULL));
+  else
+    KCHECK(tacs_hip_solid_elem_vectors(order, 0, numElements, d_conn, d_X, d_u, d_telem, &mat,
+                                       1.0, 0, 0, 1, 1, d_ev, g_stream));
   KCHECK(elementVectorsToNodes(d_ev, r, 0));
-  if (uploadAux() == 0 && naux > 0) {
+  addAuxResidual(r, lambda);
+  if (bcs)
+    KCHECK(tacs_hip_apply_bc_vec(varsPerNode, nbc_nodes, d_bcnodes, d_bcmask, d_bcvals, d_u,
+                                 lambda, r, g_stream));
+}
+
+// Aux-element loads added to a residual (TACSAssembler.cpp:4210-4225)
+void TACSAssembler::addAuxResidual(double* r, TacsScalar lambda) {
+  if (uploadAux() != 0 || naux == 0) return;
+  if (family == FAMILY_SHELL) {
+    KCHECK(tacs_hip_shell_pressure_vectors(naux, d_aux_elem, d_aux_pres, d_conn, d_X, lambda,
+                                           d_aux_ev, g_stream));
+    KCHECK(tacs_hip_gather_elem_vectors_b(6, 24, numNodes, d_n2a_ptr, d_n2a, d_aux_ev, 1, r,
+                                          g_stream));
+    return;
+  }
+  {
     KCHECK(tacs_hip_solid_traction_vectors(order, naux, d_aux_elem, d_aux_face, d_aux_trac, d_conn,
                                            d_X, lambda, d_aux_ev, g_stream));
     KCHECK(tacs_hip_gather_elem_vectors(3 * npe, numNodes, d_n2a_ptr, d_n2a, d_aux_ev, 1, r,
                                         g_stream));
   }
-  if (bcs)
-    KCHECK(tacs_hip_apply_bc_vec(3, nbc_nodes, d_bcnodes, d_bcmask, d_bcvals, d_u, lambda, r,
-                                 g_stream));
 }
 
 // TACSAssembler::assembleJacobian src/TACSAssembler.cpp:4291.  Element
@@ -1180,14 +1293,42 @@ void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScal
     return;
   }
   TACSParallelMat* A = dynamic_cast<TACSParallelMat*>(tA);
-  if (!A || A->nrows != numNodes || A->nnzb != nnzb || A->bsize != 3) {
+  if (!A || A->nrows != numNodes || A->nnzb != nnzb || A->bsize != varsPerNode) {
     tacs_set_error("assembleJacobian: matrix does not match the assembler");
     return;
   }
-  if (residual) assembleRes(residual, lambda, bcs);
   hipEvent_t e0, e1;
   HCHECK(hipEventCreate(&e0));
   HCHECK(hipEventCreate(&e1));
+  if (family == FAMILY_SHELL) {
+    // element matrices (and residuals K u) in one pass, then the row gather
+    ensure(d_ke, (size_t)numElements * 576);
+    double* ev = NULL;
+    if (residual) ev = ensure(d_ev, (size_t)numElements * 24);
+    HCHECK(hipEventRecord(e0, g_stream));
+    KCHECK(shellKernel(64, 0, NULL, alpha, 0, 0, 1, 1, d_ke, ev, NULL));
+    KCHECK(tacs_hip_gather_elem_matrices(6, 4, 64, numNodes, d_n2e_ptr, d_n2e, d_conn, d_ke,
+                                         A->d_rowp, A->d_cols, bcs ? d_bcvars : NULL, A->d_A,
+                                         A->d_A32, g_stream));
+    HCHECK(hipEventRecord(e1, g_stream));
+    if (residual) {
+      double* r = residual->devWrite();
+      KCHECK(elementVectorsToNodes(d_ev, r, 0));
+      addAuxResidual(r, lambda);
+      if (bcs)
+        KCHECK(tacs_hip_apply_bc_vec(varsPerNode, nbc_nodes, d_bcnodes, d_bcmask, d_bcvals, d_u,
+                                     lambda, r, g_stream));
+    }
+    HCHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HCHECK(hipEventElapsedTime(&ms, e0, e1));
+    jac_ms = ms;
+    HCHECK(hipEventDestroy(e0));
+    HCHECK(hipEventDestroy(e1));
+    A->A32_current = A->d_A32 ? 1 : 0;
+    return;
+  }
+  if (residual) assembleRes(residual, lambda, bcs);
   HCHECK(hipEventRecord(e0, g_stream));
   KCHECK(tacs_hip_solid_jacobian(order, numNodes, d_n2e_ptr, d_n2e, d_conn, d_X, d_telem, &mat,
                                  alpha, gamma, A->d_rowp, A->d_cols, bcs ? d_bcvars : NULL, A->d_A,
@@ -1210,21 +1351,32 @@ void TACSAssembler::evalFunctions(int numFuncs, TACSFunction** funcs, TacsScalar
   for (int k = 0; k < numFuncs; k++) {
     vals[k] = 0.0;
     if (!funcs[k]) continue;
+    const bool sh = (family == FAMILY_SHELL);
     if (TACSCompliance* c = dynamic_cast<TACSCompliance*>(funcs[k])) {
-      KCHECK(tacs_hip_solid_elem_scalars(order, 0, numElements, d_conn, d_X, d_u, NULL, d_telem,
-                                         &mat, 1.0, 0, 0, 1, d_es, g_stream));
+      if (sh)
+        KCHECK(shellKernel(64, 2, NULL, 1.0, 0, 0, 1, 1, NULL, NULL, d_es));
+      else
+        KCHECK(tacs_hip_solid_elem_scalars(order, 0, numElements, d_conn, d_X, d_u, NULL, d_telem,
+                                           &mat, 1.0, 0, 0, 1, d_es, g_stream));
       KCHECK(tacs_hip_reduce(numElements, d_es, 0, d_red, d_scalar, g_stream));
       c->compliance = read_scalar(d_scalar);
       vals[k] = c->compliance;
     } else if (TACSKSFailure* f = dynamic_cast<TACSKSFailure*>(funcs[k])) {
-      KCHECK(tacs_hip_solid_elem_scalars(order, 1, numElements, d_conn, d_X, d_u, NULL, d_telem,
-                                         &mat, 1.0, f->ksWeight, 0, f->safetyFactor, d_es,
-                                         g_stream));
+      if (sh)
+        KCHECK(shellKernel(64, 3, NULL, 1.0, f->ksWeight, 0, 1, f->safetyFactor, NULL, NULL, d_es));
+      else
+        KCHECK(tacs_hip_solid_elem_scalars(order, 1, numElements, d_conn, d_X, d_u, NULL, d_telem,
+                                           &mat, 1.0, f->ksWeight, 0, f->safetyFactor, d_es,
+                                           g_stream));
       KCHECK(tacs_hip_reduce(numElements, d_es, 1, d_red, d_scalar, g_stream));
       f->maxFail = read_scalar(d_scalar);
-      KCHECK(tacs_hip_solid_elem_scalars(order, 2, numElements, d_conn, d_X, d_u, NULL, d_telem,
-                                         &mat, 1.0, f->ksWeight, f->maxFail, f->safetyFactor, d_es,
-                                         g_stream));
+      if (sh)
+        KCHECK(shellKernel(64, 4, NULL, 1.0, f->ksWeight, f->maxFail, 1, f->safetyFactor, NULL,
+                           NULL, d_es));
+      else
+        KCHECK(tacs_hip_solid_elem_scalars(order, 2, numElements, d_conn, d_X, d_u, NULL, d_telem,
+                                           &mat, 1.0, f->ksWeight, f->maxFail, f->safetyFactor,
+                                           d_es, g_stream));
       KCHECK(tacs_hip_reduce(numElements, d_es, 0, d_red, d_scalar, g_stream));
       f->ksFailSum = read_scalar(d_scalar);
       vals[k] = f->getFunctionValue();
@@ -1237,10 +1389,16 @@ void TACSAssembler::evalFunctions(int numFuncs, TACSFunction** funcs, TacsScalar
 // TACSAssembler::addSVSens src/TACSAssembler.cpp:4941 (no BCs applied)
 void TACSAssembler::addSVSens(TacsScalar alpha, TacsScalar beta, TacsScalar gamma, int numFuncs,
                               TACSFunction** funcs, TACSBVec** dfdu) {
-  ensure(d_ev, (size_t)numElements * 3 * npe);
+  ensure(d_ev, (size_t)numElements * varsPerNode * npe);
+  const bool sh = (family == FAMILY_SHELL);
   for (int k = 0; k < numFuncs; k++) {
     if (!funcs[k]) continue;
-    if (dynamic_cast<TACSCompliance*>(funcs[k])) {
+    if (sh && dynamic_cast<TACSCompliance*>(funcs[k])) {
+      KCHECK(shellKernel(64, 8, NULL, alpha, 0, 0, 1, 1, NULL, d_ev, NULL));
+    } else if (TACSKSFailure* f = sh ? dynamic_cast<TACSKSFailure*>(funcs[k]) : NULL) {
+      KCHECK(shellKernel(64, 5, NULL, alpha, f->ksWeight, f->maxFail, f->ksFailSum,
+                         f->safetyFactor, NULL, d_ev, NULL));
+    } else if (dynamic_cast<TACSCompliance*>(funcs[k])) {
       KCHECK(tacs_hip_solid_elem_vectors(order, 1, numElements, d_conn, d_X, d_u, d_telem, &mat,
                                          alpha, 0, 0, 1, 1, d_ev, g_stream));
     } else if (TACSKSFailure* f = dynamic_cast<TACSKSFailure*>(funcs[k])) {
@@ -1259,9 +1417,17 @@ void TACSAssembler::addSVSens(TacsScalar alpha, TacsScalar beta, TacsScalar gamm
 void TACSAssembler::addDVSens(TacsScalar coef, int numFuncs, TACSFunction** funcs,
                               TACSBVec** dfdx) {
   ensure(d_es, numElements);
+  const bool sh = (family == FAMILY_SHELL);
   for (int k = 0; k < numFuncs; k++) {
     if (!funcs[k]) continue;
-    if (dynamic_cast<TACSCompliance*>(funcs[k])) {
+    if (sh && dynamic_cast<TACSCompliance*>(funcs[k])) {
+      KCHECK(shellKernel(64, 9, NULL, coef, 0, 0, 1, 1, NULL, NULL, d_es));
+      KCHECK(scatterElementScalarsToDVs(d_es, dfdx[k]->devWrite()));
+    } else if (TACSKSFailure* f = sh ? dynamic_cast<TACSKSFailure*>(funcs[k]) : NULL) {
+      KCHECK(shellKernel(64, 6, NULL, coef, f->ksWeight, f->maxFail, f->ksFailSum,
+                         f->safetyFactor, NULL, NULL, d_es));
+      KCHECK(scatterElementScalarsToDVs(d_es, dfdx[k]->devWrite()));
+    } else if (dynamic_cast<TACSCompliance*>(funcs[k])) {
       KCHECK(tacs_hip_solid_elem_scalars(order, 3, numElements, d_conn, d_X, d_u, NULL, d_telem,
                                          &mat, coef, 0, 0, 1, d_es, g_stream));
       KCHECK(scatterElementScalarsToDVs(d_es, dfdx[k]->devWrite()));
@@ -1279,9 +1445,12 @@ void TACSAssembler::addAdjointResProducts(TacsScalar scale, int numAdjoints, TAC
                                           TACSBVec** dfdx, const TacsScalar lambda) {
   ensure(d_es, numElements);
   for (int k = 0; k < numAdjoints; k++) {
-    KCHECK(tacs_hip_solid_elem_scalars(order, 4, numElements, d_conn, d_X, d_u,
-                                       adjoint[k]->devRead(), d_telem, &mat, scale, 0, 0, 1, d_es,
-                                       g_stream));
+    if (family == FAMILY_SHELL)
+      KCHECK(shellKernel(64, 7, adjoint[k]->devRead(), scale, 0, 0, 1, 1, NULL, NULL, d_es));
+    else
+      KCHECK(tacs_hip_solid_elem_scalars(order, 4, numElements, d_conn, d_X, d_u,
+                                         adjoint[k]->devRead(), d_telem, &mat, scale, 0, 0, 1,
+                                         d_es, g_stream));
     KCHECK(scatterElementScalarsToDVs(d_es, dfdx[k]->devWrite()));
   }
 }

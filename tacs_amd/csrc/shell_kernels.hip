@@ -1,0 +1,833 @@
+// Linear MITC4 shell (TACSQuad4Shell: TACSShellElement<TACSQuadLinearQuadrature,
+// TACSShellQuadBasis<2>, TACSLinearizedRotation, TACSShellLinearModel>,
+// src/elements/shell/TACSShellElementDefs.h:18-20) with the natural transform
+// and TACSIsoShellConstitutive, for gfx950.
+//
+// Work layout.  One 32-lane team per element, four elements per 128-thread
+// workgroup.  The team stages the element geometry in LDS (node normals and
+// drill frames, tying-point frames, quadrature-point transforms), then lane c
+// (c < 24: node c/6, dof c%6) builds column c of the linear strain map
+// B_q (9 x 24 per quadrature point) -- the strain of the unit vector e_c.  The
+// map is the reference's forward strain computation (TACSShellElement::
+// addResidual, TACSShellElement.h:295-420) specialised to one nonzero
+// variable, which makes every term a rank-1 update.  The epilogue depends on
+// MODE: the element stiffness K = sum_q detXd_q B_q^T Cs B_q (lane c owns
+// column c), the residual B^T Cs B u, or the per-element function values and
+// sensitivities.  Element matrices go to an element-major buffer that
+// gather_elem_matrices sums into block-CSR rows in ascending element order
+// (deterministic, no atomics, no zero fill).
+#include "tacs_hip_common.h"
+
+namespace {
+
+constexpr int SH_TEAM = 32;
+constexpr int SH_EPB = 4;  // elements (teams) per workgroup
+constexpr int SH_NV = 24;
+
+template <typename T>
+struct ShellLDS {
+  T X[12];
+  T nXxi[4][6];   // X,xi at the nodes
+  T fn[4][3];     // unit node normals
+  T Tn[4][9];     // node transforms
+  T XTn[4][9];    // Xdn^-1 Tn
+  T tXxi[9][6];   // X,xi at the tying points
+  T tn0[9][3];    // interpolated normal at the tying points
+  T qT[4][9];     // T at the quadrature points
+  T qXT[4][9];    // Xd^-1 T
+  T qXzT[4][9];   // -Xd^-1 Xdz Xd^-1 T
+  T detw[4];      // w * det(Xd)
+  T B[4][9][SH_NV];
+  T u[SH_NV], psi[SH_NV];
+  T e[4][9], ep[4][9], de[4][9];
+  T val[4];
+};
+
+template <typename T>
+struct ShellMat {
+  T Q0, Q1, Q5;  // plane-stress Q (Q2 = Q4 = 0, Q3 = Q0)
+  T kcorr, drill_reg, ys;
+};
+
+// Gauss points of TacsGaussQuadPts2 (src/elements/basis/TACSGaussQuadrature.h:26)
+__device__ __forceinline__ double sh_gauss(int i) { return i ? 0.577350269189626 : -0.577350269189626; }
+
+// Tying points in storage order G11(2), G22(2), G12(1), G23(2), G13(2)
+// (TACSShellElementQuadBasis.h:487-560).
+__device__ __forceinline__ void sh_tying_pt(int t, double& x, double& y) {
+  x = (t == 2 || t == 5) ? -1.0 : ((t == 3 || t == 6) ? 1.0 : 0.0);
+  y = (t == 0 || t == 7) ? -1.0 : ((t == 1 || t == 8) ? 1.0 : 0.0);
+}
+
+// Shape function of node a = i + 2j and its parametric gradient at (x, y)
+template <typename T>
+__device__ __forceinline__ void sh_shape(int a, T x, T y, T& N, T& Nx, T& Ny) {
+  T na = (a & 1) ? T(0.5) * (T(1) + x) : T(0.5) * (T(1) - x);
+  T nb = (a & 2) ? T(0.5) * (T(1) + y) : T(0.5) * (T(1) - y);
+  T da = (a & 1) ? T(0.5) : T(-0.5);
+  T db = (a & 2) ? T(0.5) : T(-0.5);
+  N = na * nb;
+  Nx = da * nb;
+  Ny = na * db;
+}
+
+// X,xi of a 3-vector field at (x, y): g[2k] = d/dxi, g[2k+1] = d/deta
+template <typename T>
+__device__ __forceinline__ void sh_grad3(const T* v, T x, T y, T g[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; k++) g[k] = T(0);
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    T N, Nx, Ny;
+    sh_shape<T>(a, x, y, N, Nx, Ny);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      g[2 * k] += Nx * v[3 * a + k];
+      g[2 * k + 1] += Ny * v[3 * a + k];
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void sh_interp3(const T* v, T x, T y, T f[3]) {
+  f[0] = f[1] = f[2] = T(0);
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    T N, Nx, Ny;
+    sh_shape<T>(a, x, y, N, Nx, Ny);
+#pragma unroll
+    for (int k = 0; k < 3; k++) f[k] += N * v[3 * a + k];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void sh_cross(const T a[3], const T b[3], T c[3]) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+template <typename T>
+__device__ __forceinline__ T sh_inv3(const T A[9], T B[9]) {
+  T det = A[8] * (A[0] * A[4] - A[3] * A[1]) - A[7] * (A[0] * A[5] - A[3] * A[2]) +
+          A[6] * (A[1] * A[5] - A[2] * A[4]);
+  T r = T(1) / det;
+  B[0] = (A[4] * A[8] - A[5] * A[7]) * r;
+  B[1] = -(A[1] * A[8] - A[2] * A[7]) * r;
+  B[2] = (A[1] * A[5] - A[2] * A[4]) * r;
+  B[3] = -(A[3] * A[8] - A[5] * A[6]) * r;
+  B[4] = (A[0] * A[8] - A[2] * A[6]) * r;
+  B[5] = -(A[0] * A[5] - A[2] * A[3]) * r;
+  B[6] = (A[3] * A[7] - A[4] * A[6]) * r;
+  B[7] = -(A[0] * A[7] - A[1] * A[6]) * r;
+  B[8] = (A[0] * A[4] - A[1] * A[3]) * r;
+  return det;
+}
+
+template <typename T>
+__device__ __forceinline__ void sh_mm(const T A[9], const T B[9], T C[9]) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+
+// TACSShellNaturalTransform::computeTransform (TACSShellElementTransform.h:
+// 25-92), including its removal of the normal component from t1[0] only,
+// three times over.  Columns of T: t1, t2 = n x t1, n.
+template <typename T>
+__device__ __forceinline__ void sh_transform(const T Xxi[6], const T n0[3], T Tm[9]) {
+  T inv = T(1) / sqrt(n0[0] * n0[0] + n0[1] * n0[1] + n0[2] * n0[2]);
+  T n[3] = {n0[0] * inv, n0[1] * inv, n0[2] * inv};
+  T t1[3] = {Xxi[0], Xxi[2], Xxi[4]};
+  T d = n[0] * t1[0] + n[1] * t1[1] + n[2] * t1[2];
+  t1[0] = t1[0] - d * n[0];
+  t1[0] = t1[0] - d * n[0];
+  t1[0] = t1[0] - d * n[0];
+  inv = T(1) / sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
+  t1[0] *= inv;
+  t1[1] *= inv;
+  t1[2] *= inv;
+  T t2[3];
+  sh_cross<T>(n, t1, t2);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    Tm[3 * k] = t1[k];
+    Tm[3 * k + 1] = t2[k];
+    Tm[3 * k + 2] = n[k];
+  }
+}
+
+// Stress of TACSIsoShellConstitutive (TACSIsoShellConstitutive.cpp:153-189 with
+// tOffset = 0; computeStress TACSShellConstitutive.h:133)
+template <typename T>
+__device__ __forceinline__ void sh_stress(const ShellMat<T>& m, T t, const T e[9], T s[9]) {
+  T I = t * t * t / T(12);
+  T As = m.kcorr * t * m.Q5;
+  s[0] = t * (m.Q0 * e[0] + m.Q1 * e[1]);
+  s[1] = t * (m.Q1 * e[0] + m.Q0 * e[1]);
+  s[2] = t * (m.Q5 * e[2]);
+  s[3] = I * (m.Q0 * e[3] + m.Q1 * e[4]);
+  s[4] = I * (m.Q1 * e[3] + m.Q0 * e[4]);
+  s[5] = I * (m.Q5 * e[5]);
+  s[6] = As * e[6];
+  s[7] = As * e[7];
+  s[8] = T(0.5) * m.drill_reg * (As + As) * e[8];
+}
+
+// von Mises of a plane-stress state and its gradient (vonMisesFailure2D and
+// vonMisesFailure2DStressSens, TACSMaterialProperties.cpp:481-500)
+template <typename T>
+__device__ __forceinline__ T sh_vm2(const T s[3], T ys, T sens[3]) {
+  T f = sqrt(s[0] * s[0] + s[1] * s[1] - s[0] * s[1] + T(3) * s[2] * s[2]);
+  if (f != T(0)) {
+    sens[0] = (s[0] - T(0.5) * s[1]) / (f * ys);
+    sens[1] = (s[1] - T(0.5) * s[0]) / (f * ys);
+    sens[2] = T(3) * s[2] / (f * ys);
+  } else {
+    sens[0] = sens[1] = sens[2] = T(0);
+  }
+  return f / ys;
+}
+
+// Failure index (KS of top/bottom von Mises, weight 100;
+// TACSIsoShellConstitutive::evalFailure :254), strain gradient
+// (evalFailureStrainSens :300) and thickness derivative (addFailureDVSens :381)
+template <typename T>
+__device__ __forceinline__ T sh_failure(const ShellMat<T>& m, T t, const T e[9], T de[9], T& dt) {
+  const T ksw = T(100);
+  T ht = T(0.5) * t, hb = T(-0.5) * t;
+  T et[3], eb[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    et[k] = e[k] + ht * e[3 + k];
+    eb[k] = e[k] + hb * e[3 + k];
+  }
+  T st[3] = {m.Q0 * et[0] + m.Q1 * et[1], m.Q1 * et[0] + m.Q0 * et[1], m.Q5 * et[2]};
+  T sb[3] = {m.Q0 * eb[0] + m.Q1 * eb[1], m.Q1 * eb[0] + m.Q0 * eb[1], m.Q5 * eb[2]};
+  T pt[3], pb[3];
+  T top = sh_vm2<T>(st, m.ys, pt), bot = sh_vm2<T>(sb, m.ys, pb);
+  T mx = top > bot ? top : bot;
+  T sum = exp(ksw * (top - mx)) + exp(ksw * (bot - mx));
+  T ft = exp(ksw * (top - mx)) / sum, fb = exp(ksw * (bot - mx)) / sum;
+  T phit[3] = {m.Q0 * pt[0] + m.Q1 * pt[1], m.Q1 * pt[0] + m.Q0 * pt[1], m.Q5 * pt[2]};
+  T phib[3] = {m.Q0 * pb[0] + m.Q1 * pb[1], m.Q1 * pb[0] + m.Q0 * pb[1], m.Q5 * pb[2]};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    de[k] = ft * phit[k] + fb * phib[k];
+    de[3 + k] = ft * ht * phit[k] + fb * hb * phib[k];
+  }
+  de[6] = de[7] = de[8] = T(0);
+  dt = ft * T(0.5) * (phit[0] * e[3] + phit[1] * e[4] + phit[2] * e[5]) -
+       fb * T(0.5) * (phib[0] * e[3] + phib[1] * e[4] + phib[2] * e[5]);
+  return mx + log(sum) / ksw;
+}
+
+}  // namespace
+
+enum {
+  SH_MATRIX = 0,
+  SH_RESIDUAL = 1,
+  SH_COMPLIANCE = 2,
+  SH_KS_MAX = 3,
+  SH_KS_SUM = 4,
+  SH_KS_SV = 5,
+  SH_KS_DV = 6,
+  SH_ADJ_PRODUCT = 7,
+  SH_COMPLIANCE_SV = 8,
+  SH_COMPLIANCE_DV = 9,
+};
+
+struct ShellParams {
+  double Q0, Q1, Q5, kcorr, drill_reg, ys;
+  double scale, ks_weight, ks_max, ks_sum, safety;
+};
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
+    int nelem, const int* __restrict__ conn, const double* __restrict__ Xg,
+    const double* __restrict__ ug, const double* __restrict__ psig,
+    const double* __restrict__ telem, ShellParams P, T* __restrict__ ke,
+    double* __restrict__ ev, double* __restrict__ es) {
+  __shared__ ShellLDS<T> lds_all[SH_EPB];
+  const int team = threadIdx.x / SH_TEAM, lane = threadIdx.x % SH_TEAM;
+  const int el = blockIdx.x * SH_EPB + team;
+  const bool active = el < nelem;
+  ShellLDS<T>& L = lds_all[team];
+  const ShellMat<T> M = {T(P.Q0), T(P.Q1), T(P.Q5), T(P.kcorr), T(P.drill_reg), T(P.ys)};
+  const T t = active ? T(telem ? telem[el] : 1.0) : T(1);
+
+  // ---- load the element's nodes and variables
+  if (active) {
+    const int* c = &conn[(size_t)el * 4];
+    if (lane < 12) L.X[lane] = T(Xg[3 * (size_t)c[lane / 3] + lane % 3]);
+    if (lane < 24) {
+      L.u[lane] = ug ? T(ug[6 * (size_t)c[lane / 6] + lane % 6]) : T(0);
+      if (MODE == SH_ADJ_PRODUCT) L.psi[lane] = T(psig[6 * (size_t)c[lane / 6] + lane % 6]);
+    }
+  }
+  __syncthreads();
+
+  // ---- node normals (TacsShellComputeNodeNormals, TACSShellUtilities.h:301)
+  if (active && lane < 4) {
+    T x = (lane & 1) ? T(1) : T(-1), y = (lane & 2) ? T(1) : T(-1);
+    T g[6];
+    sh_grad3<T>(L.X, x, y, g);
+    T a[3] = {g[0], g[2], g[4]}, b[3] = {g[1], g[3], g[5]}, f[3];
+    sh_cross<T>(a, b, f);
+    T nrm = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    if (nrm != T(0)) {
+      T r = T(1) / nrm;
+      f[0] *= r;
+      f[1] *= r;
+      f[2] *= r;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) L.nXxi[lane][k] = g[k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) L.fn[lane][k] = f[k];
+  }
+  __syncthreads();
+
+  // ---- frames: node drill frames, tying points, quadrature points
+  if (active && lane < 4) {
+    T Tm[9], Xd[9], Xinv[9];
+    sh_transform<T>(L.nXxi[lane], L.fn[lane], Tm);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      Xd[3 * k] = L.nXxi[lane][2 * k];
+      Xd[3 * k + 1] = L.nXxi[lane][2 * k + 1];
+      Xd[3 * k + 2] = L.fn[lane][k];
+    }
+    sh_inv3<T>(Xd, Xinv);
+    sh_mm<T>(Xinv, Tm, L.XTn[lane]);
+#pragma unroll
+    for (int k = 0; k < 9; k++) L.Tn[lane][k] = Tm[k];
+  } else if (active && lane >= 4 && lane < 13) {
+    int ty = lane - 4;
+    double xd, yd;
+    sh_tying_pt(ty, xd, yd);
+    T g[6], n0[3];
+    sh_grad3<T>(L.X, T(xd), T(yd), g);
+    sh_interp3<T>(&L.fn[0][0], T(xd), T(yd), n0);
+#pragma unroll
+    for (int k = 0; k < 6; k++) L.tXxi[ty][k] = g[k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) L.tn0[ty][k] = n0[k];
+  } else if (active && lane >= 13 && lane < 17) {
+    int q = lane - 13;
+    T x = T(sh_gauss(q & 1)), y = T(sh_gauss(q >> 1));
+    T g[6], n0[3], nxi[6], Tm[9], Xd[9], Xdz[9], Xinv[9], XT[9], tmp[9];
+    sh_grad3<T>(L.X, x, y, g);
+    sh_interp3<T>(&L.fn[0][0], x, y, n0);
+    sh_grad3<T>(&L.fn[0][0], x, y, nxi);
+    sh_transform<T>(g, n0, Tm);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      Xd[3 * k] = g[2 * k];
+      Xd[3 * k + 1] = g[2 * k + 1];
+      Xd[3 * k + 2] = n0[k];
+      Xdz[3 * k] = nxi[2 * k];
+      Xdz[3 * k + 1] = nxi[2 * k + 1];
+      Xdz[3 * k + 2] = T(0);
+    }
+    T det = sh_inv3<T>(Xd, Xinv);
+    sh_mm<T>(Xinv, Tm, XT);
+    sh_mm<T>(Xinv, Xdz, tmp);
+#pragma unroll
+    for (int k = 0; k < 9; k++) tmp[k] = -tmp[k];
+    sh_mm<T>(tmp, XT, L.qXzT[q]);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      L.qT[q][k] = Tm[k];
+      L.qXT[q][k] = XT[k];
+    }
+    L.detw[q] = det;  // quadrature weights are 1
+  }
+  __syncthreads();
+
+  // ---- column c of B_q: the strains of the unit vector e_c
+  if (active && lane < SH_NV) {
+    const int a = lane / 6, k = lane % 6;
+    T ety[9], etn[4];
+    if (k < 3) {
+      // translation u_k of node a: Uxi[2k] = N_a,xi, Uxi[2k+1] = N_a,eta, d = 0
+#pragma unroll
+      for (int ty = 0; ty < 9; ty++) {
+        double xd, yd;
+        sh_tying_pt(ty, xd, yd);
+        T N, Nx, Ny;
+        sh_shape<T>(a, T(xd), T(yd), N, Nx, Ny);
+        const T* g = L.tXxi[ty];
+        if (ty < 2) ety[ty] = Nx * g[2 * k];
+        else if (ty < 4) ety[ty] = Ny * g[2 * k + 1];
+        else if (ty < 5) ety[ty] = T(0.5) * (Nx * g[2 * k + 1] + Ny * g[2 * k]);
+        else if (ty < 7) ety[ty] = T(0.5) * (L.tn0[ty][k] * Ny);
+        else ety[ty] = T(0.5) * (L.tn0[ty][k] * Nx);
+      }
+      // drill strain at node i: u0x = Tn^T [u0xi | 0] Xdn^-1 Tn (rank 1)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        T N, Nx, Ny;
+        sh_shape<T>(a, (i & 1) ? T(1) : T(-1), (i & 2) ? T(1) : T(-1), N, Nx, Ny);
+        const T* X = L.XTn[i];
+        T m0 = Nx * X[0] + Ny * X[3];  // row k of [u0xi|0] Xdn^-1 Tn, column 0
+        T m1 = Nx * X[1] + Ny * X[4];  // column 1
+        // u0x[1] = Tn[k][0] m1, u0x[3] = Tn[k][1] m0
+        etn[i] = T(0.5) * (L.Tn[i][3 * k + 1] * m0 - L.Tn[i][3 * k] * m1);
+      }
+    } else {
+      // rotation q_r of node a: d_a = e_r x fn_a, Uxi = 0
+      const int r = k - 3;
+      const T* f = L.fn[a];
+      T da[3];
+      da[0] = (r == 0) ? T(0) : ((r == 1) ? f[2] : -f[1]);
+      da[1] = (r == 0) ? -f[2] : ((r == 1) ? T(0) : f[0]);
+      da[2] = (r == 0) ? f[1] : ((r == 1) ? -f[0] : T(0));
+#pragma unroll
+      for (int ty = 0; ty < 9; ty++) {
+        double xd, yd;
+        sh_tying_pt(ty, xd, yd);
+        T N, Nx, Ny;
+        sh_shape<T>(a, T(xd), T(yd), N, Nx, Ny);
+        const T* g = L.tXxi[ty];
+        if (ty < 5) ety[ty] = T(0);
+        else if (ty < 7) ety[ty] = T(0.5) * (N * (g[1] * da[0] + g[3] * da[1] + g[5] * da[2]));
+        else ety[ty] = T(0.5) * (N * (g[0] * da[0] + g[2] * da[1] + g[4] * da[2]));
+      }
+      // drill strain at node a: Ct - I = Tn^T (C(q) - I) Tn with C = I - q^x
+      // (TACSLinearizedRotation::computeRotationMat, TACSDirector.h:25-34)
+      T Cm[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+      if (r == 0) { Cm[5] = T(1); Cm[7] = T(-1); }
+      if (r == 1) { Cm[2] = T(-1); Cm[6] = T(1); }
+      if (r == 2) { Cm[1] = T(1); Cm[3] = T(-1); }
+#pragma unroll
+      for (int i = 0; i < 4; i++) etn[i] = T(0);
+      const T* Tn = L.Tn[a];
+      T c10 = T(0), c01 = T(0);
+#pragma unroll
+      for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+          c10 += Tn[3 * p + 1] * Cm[3 * p + s] * Tn[3 * s];
+          c01 += Tn[3 * p] * Cm[3 * p + s] * Tn[3 * s + 1];
+        }
+      etn[a] = T(0.5) * (c10 - c01);
+    }
+
+    // quadrature points
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      T x = T(sh_gauss(q & 1)), y = T(sh_gauss(q >> 1));
+      T N, Nx, Ny;
+      sh_shape<T>(a, x, y, N, Nx, Ny);
+      const T* Tm = L.qT[q];
+      const T* XT = L.qXT[q];
+      const T* XzT = L.qXzT[q];
+      // u1x = p w^T (rank 1)
+      T p[3], w[3];
+      if (k < 3) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          p[j] = Tm[3 * k + j];
+          w[j] = Nx * XzT[j] + Ny * XzT[3 + j];
+        }
+      } else {
+        const int r = k - 3;
+        const T* f = L.fn[a];
+        T da[3];
+        da[0] = (r == 0) ? T(0) : ((r == 1) ? f[2] : -f[1]);
+        da[1] = (r == 0) ? -f[2] : ((r == 1) ? T(0) : f[0]);
+        da[2] = (r == 0) ? f[1] : ((r == 1) ? -f[0] : T(0));
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          p[j] = Tm[j] * da[0] + Tm[3 + j] * da[1] + Tm[6 + j] * da[2];
+          w[j] = Nx * XT[j] + Ny * XT[3 + j] + N * XzT[6 + j];
+        }
+      }
+      // tying strain interpolation (interpTyingStrain, :651-675)
+      T nax0 = T(0.5) * (T(1) - x), nax1 = T(0.5) * (T(1) + x);
+      T nby0 = T(0.5) * (T(1) - y), nby1 = T(0.5) * (T(1) + y);
+      T g11 = nby0 * ety[0] + nby1 * ety[1];
+      T g22 = nax0 * ety[2] + nax1 * ety[3];
+      T g12 = ety[4];
+      T g23 = nax0 * ety[5] + nax1 * ety[6];
+      T g13 = nby0 * ety[7] + nby1 * ety[8];
+      // e0ty = XT^T G XT (mat3x3SymmTransformTranspose, TACSElementAlgebra.h:1094)
+      T G[9] = {g11, g12, g13, g12, g22, g23, g13, g23, T(0)};
+      T W[9], E0[9];
+      sh_mm<T>(G, XT, W);
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          E0[3 * i + j] = XT[i] * W[j] + XT[3 + i] * W[3 + j] + XT[6 + i] * W[6 + j];
+      T et = T(0);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        T Ni, Nxi, Nyi;
+        sh_shape<T>(i, x, y, Ni, Nxi, Nyi);
+        et += Ni * etn[i];
+      }
+      L.B[q][0][lane] = E0[0];
+      L.B[q][1][lane] = E0[4];
+      L.B[q][2][lane] = T(2) * E0[1];
+      L.B[q][3][lane] = p[0] * w[0];
+      L.B[q][4][lane] = p[1] * w[1];
+      L.B[q][5][lane] = p[0] * w[1] + p[1] * w[0];
+      L.B[q][6][lane] = T(2) * E0[5];
+      L.B[q][7][lane] = T(2) * E0[2];
+      L.B[q][8][lane] = et;
+    }
+  }
+  __syncthreads();
+
+  if (MODE == SH_MATRIX) {
+    if (active && lane < SH_NV) {
+      // column c of K = sum_q detw_q B_q^T Cs B_q; residual r_c = K[:,c].u (K symmetric)
+      T s[4][9];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        T b[9];
+#pragma unroll
+        for (int kk = 0; kk < 9; kk++) b[kk] = L.B[q][kk][lane];
+        sh_stress<T>(M, t, b, s[q]);
+#pragma unroll
+        for (int kk = 0; kk < 9; kk++) s[q][kk] *= L.detw[q];
+      }
+      T racc = T(0);
+      T* kout = &ke[(size_t)el * SH_NV * SH_NV + lane];
+      for (int r = 0; r < SH_NV; r++) {
+        T acc = T(0);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+          for (int kk = 0; kk < 9; kk++) acc += L.B[q][kk][r] * s[q][kk];
+        kout[(size_t)r * SH_NV] = acc * T(P.scale);
+        racc += acc * L.u[r];
+      }
+      if (ev) ev[(size_t)el * SH_NV + lane] = double(racc);
+    }
+    return;
+  }
+
+  // ---- strains of u (and psi) at the quadrature points: lanes 0..35
+  if (active) {
+    for (int j = lane; j < 36; j += SH_TEAM) {
+      int q = j / 9, kk = j % 9;
+      T acc = T(0), accp = T(0);
+#pragma unroll
+      for (int c = 0; c < SH_NV; c++) {
+        acc += L.B[q][kk][c] * L.u[c];
+        if (MODE == SH_ADJ_PRODUCT) accp += L.B[q][kk][c] * L.psi[c];
+      }
+      // the function evaluations use e[8] = 0 (TACSShellElement::evalPointQuantity)
+      bool drop_drill = (MODE != SH_RESIDUAL && MODE != SH_ADJ_PRODUCT);
+      L.e[q][kk] = (drop_drill && kk == 8) ? T(0) : acc;
+      if (MODE == SH_ADJ_PRODUCT) L.ep[q][kk] = accp;
+    }
+  }
+  __syncthreads();
+
+  if (MODE == SH_RESIDUAL || MODE == SH_COMPLIANCE_SV) {
+    // r_c = sum_q detw_q B_q[:,c] . s_q  (s = Cs e, doubled for dC/du)
+    if (active && lane < SH_NV) {
+      T acc = T(0);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        T s[9];
+        sh_stress<T>(M, t, L.e[q], s);
+        T dot = T(0);
+#pragma unroll
+        for (int kk = 0; kk < 9; kk++) dot += L.B[q][kk][lane] * s[kk];
+        acc += L.detw[q] * dot;
+      }
+      if (MODE == SH_COMPLIANCE_SV) acc *= T(2) * T(P.scale);
+      ev[(size_t)el * SH_NV + lane] = double(acc);
+    }
+    return;
+  }
+
+  if (MODE == SH_KS_SV) {
+    if (active && lane < 4) {
+      int q = lane;
+      T de[9], dt;
+      T f = T(P.safety) * sh_failure<T>(M, t, L.e[q], de, dt);
+      T coef = exp(T(P.ks_weight) * (f - T(P.ks_max))) / T(P.ks_sum) * L.detw[q] *
+               T(P.safety) * T(P.scale);
+#pragma unroll
+      for (int kk = 0; kk < 9; kk++) L.de[q][kk] = coef * de[kk];
+    }
+    __syncthreads();
+    if (active && lane < SH_NV) {
+      T acc = T(0);
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int kk = 0; kk < 9; kk++) acc += L.B[q][kk][lane] * L.de[q][kk];
+      ev[(size_t)el * SH_NV + lane] = double(acc);
+    }
+    return;
+  }
+
+  // ---- per-element scalars: lanes 0..3 evaluate one quadrature point each
+  if (active && lane < 4) {
+    int q = lane;
+    const T* e = L.e[q];
+    T v = T(0);
+    if (MODE == SH_COMPLIANCE) {
+      T s[9];
+      sh_stress<T>(M, t, e, s);
+      T dot = T(0);
+#pragma unroll
+      for (int kk = 0; kk < 9; kk++) dot += e[kk] * s[kk];
+      v = L.detw[q] * dot;
+    } else if (MODE == SH_KS_MAX || MODE == SH_KS_SUM || MODE == SH_KS_DV) {
+      T de[9], dt;
+      T f = T(P.safety) * sh_failure<T>(M, t, e, de, dt);
+      if (MODE == SH_KS_MAX) v = f;
+      else if (MODE == SH_KS_SUM) v = L.detw[q] * exp(T(P.ks_weight) * (f - T(P.ks_max)));
+      else
+        v = T(P.scale) * exp(T(P.ks_weight) * (f - T(P.ks_max))) / T(P.ks_sum) * L.detw[q] *
+            T(P.safety) * dt;
+    } else if (MODE == SH_ADJ_PRODUCT || MODE == SH_COMPLIANCE_DV) {
+      // TACSIsoShellConstitutive::addStressDVSens (:229-251), tOffset = 0
+      const T* x = (MODE == SH_ADJ_PRODUCT) ? L.ep[q] : e;
+      T a0 = M.Q0 * x[0] + M.Q1 * x[1], a1 = M.Q1 * x[0] + M.Q0 * x[1], a2 = M.Q5 * x[2];
+      T b0 = M.Q0 * x[3] + M.Q1 * x[4], b1 = M.Q1 * x[3] + M.Q0 * x[4], b2 = M.Q5 * x[5];
+      T dI = T(0.25) * t * t;
+      v = a0 * e[0] + a1 * e[1] + a2 * e[2] + dI * (b0 * e[3] + b1 * e[4] + b2 * e[5]) +
+          M.kcorr * M.Q5 * (x[6] * e[6] + x[7] * e[7] + M.drill_reg * x[8] * e[8]);
+      v *= T(P.scale) * L.detw[q];
+    }
+    L.val[q] = v;
+  }
+  __syncthreads();
+  if (active && lane == 0) {
+    double r;
+    if (MODE == SH_KS_MAX)
+      r = fmax(fmax(double(L.val[0]), double(L.val[1])), fmax(double(L.val[2]), double(L.val[3])));
+    else
+      r = ((double(L.val[0]) + double(L.val[1])) + double(L.val[2])) + double(L.val[3]);
+    es[el] = r;
+  }
+}
+
+// Sum element matrices into BCSR rows: one wavefront per block row, values
+// of the row spread over the lanes; contributions added in ascending element
+// order (n2e entries are (element << 5 | local node)).  Dirichlet rows are
+// fused: constrained rows become zero with a unit diagonal, columns are kept
+// (TACSParallelMat::applyBCs, BCSRMat::zeroRow src/bpmat/BCSRMat.cpp:2027).
+template <int B, int NPE, typename T>
+__global__ __launch_bounds__(256) void gather_elem_matrices(
+    int nrows, const int* __restrict__ n2e_ptr, const int* __restrict__ n2e,
+    const int* __restrict__ conn, const T* __restrict__ ke, const int* __restrict__ rowp,
+    const int* __restrict__ cols, const int* __restrict__ bcvars, double* __restrict__ A,
+    float* __restrict__ A32) {
+  constexpr int NV = B * NPE, BB = B * B;
+  const int lane = threadIdx.x % TACS_WAVE;
+  const int wpb = blockDim.x / TACS_WAVE;
+  for (int row = blockIdx.x * wpb + threadIdx.x / TACS_WAVE; row < nrows; row += gridDim.x * wpb) {
+    const int r0 = rowp[row], nvals = (rowp[row + 1] - r0) * BB;
+    const int p0 = n2e_ptr[row], p1 = n2e_ptr[row + 1];
+    const int mask = bcvars ? bcvars[row] : 0;
+    for (int v = lane; v < nvals; v += TACS_WAVE) {
+      const int j = v / BB, rc = v % BB, i = rc / B, jj = rc % B;
+      const int col = cols[r0 + j];
+      double acc = 0.0;
+      for (int p = p0; p < p1; p++) {
+        const int packed = n2e[p];
+        const int e = packed >> 5, la = packed & 31;
+        const int* ce = &conn[(size_t)e * NPE];
+#pragma unroll
+        for (int lb = 0; lb < NPE; lb++)
+          if (ce[lb] == col) acc += double(ke[(size_t)e * NV * NV + (size_t)(B * la + i) * NV + B * lb + jj]);
+      }
+      if (mask & (1 << i)) acc = (col == row && i == jj) ? 1.0 : 0.0;
+      A[(size_t)r0 * BB + v] = acc;
+      if (A32) A32[(size_t)r0 * BB + v] = (float)acc;
+    }
+  }
+}
+
+// Node gather of element vectors with B values per node, ascending element order.
+template <int B, int NV>
+__global__ void gather_elem_vectors_b(int nnodes, const int* __restrict__ n2e_ptr,
+                                      const int* __restrict__ n2e, const double* __restrict__ ev,
+                                      int accumulate, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnodes) return;
+  double s[B];
+#pragma unroll
+  for (int k = 0; k < B; k++) s[k] = accumulate ? out[(size_t)B * i + k] : 0.0;
+  for (int p = n2e_ptr[i]; p < n2e_ptr[i + 1]; p++) {
+    int packed = n2e[p];
+    const double* v = &ev[(size_t)(packed >> 5) * NV + B * (packed & 31)];
+#pragma unroll
+    for (int k = 0; k < B; k++) s[k] += v[k];
+  }
+#pragma unroll
+  for (int k = 0; k < B; k++) out[(size_t)B * i + k] = s[k];
+}
+
+// Uniform pressure on shell elements (TACSShellPressure::addResidual,
+// TACSShellPressure.h:51-91): ev += N^T f, f = -p detXd n at each quadrature
+// point, n the interpolated node normal.  One thread per loaded element.
+__global__ void shell_pressure_vectors(int naux, const int* __restrict__ aux_elem,
+                                       const double* __restrict__ pres, const int* __restrict__ conn,
+                                       const double* __restrict__ Xg, double lambda,
+                                       double* __restrict__ ev) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= naux) return;
+  const int* c = &conn[(size_t)aux_elem[q] * 4];
+  double X[12], fn[12];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) X[3 * a + k] = Xg[3 * (size_t)c[a] + k];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    double g[6];
+    sh_grad3<double>(X, (i & 1) ? 1.0 : -1.0, (i & 2) ? 1.0 : -1.0, g);
+    double a[3] = {g[0], g[2], g[4]}, b[3] = {g[1], g[3], g[5]}, f[3];
+    sh_cross<double>(a, b, f);
+    double nrm = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    if (nrm != 0.0) {
+      double r = 1.0 / nrm;
+      f[0] *= r;
+      f[1] *= r;
+      f[2] *= r;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) fn[3 * i + k] = f[k];
+  }
+  double r[24];
+#pragma unroll
+  for (int k = 0; k < 24; k++) r[k] = 0.0;
+  const double p = lambda * pres[q];
+#pragma unroll
+  for (int qp = 0; qp < 4; qp++) {
+    double x = sh_gauss(qp & 1), y = sh_gauss(qp >> 1);
+    double g[6], n[3], Xd[9], Xinv[9];
+    sh_grad3<double>(X, x, y, g);
+    sh_interp3<double>(fn, x, y, n);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      Xd[3 * k] = g[2 * k];
+      Xd[3 * k + 1] = g[2 * k + 1];
+      Xd[3 * k + 2] = n[k];
+    }
+    double det = sh_inv3<double>(Xd, Xinv);
+    double f[3] = {p * -det * n[0], p * -det * n[1], p * -det * n[2]};
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      double N, Nx, Ny;
+      sh_shape<double>(a, x, y, N, Nx, Ny);
+#pragma unroll
+      for (int k = 0; k < 3; k++) r[6 * a + k] += N * f[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 24; k++) ev[(size_t)q * 24 + k] = r[k];
+}
+
+extern "C" {
+
+static ShellParams shell_params(const ShellMaterial* m, double scale, double ks_weight,
+                                double ks_max, double ks_sum, double safety) {
+  ShellParams P;
+  P.Q0 = m->Q[0];
+  P.Q1 = m->Q[1];
+  P.Q5 = m->Q[5];
+  P.kcorr = m->kcorr;
+  P.drill_reg = m->drill_reg;
+  P.ys = m->ys;
+  P.scale = scale;
+  P.ks_weight = ks_weight;
+  P.ks_max = ks_max;
+  P.ks_sum = ks_sum;
+  P.safety = safety;
+  return P;
+}
+
+#define SH_LAUNCH(T, MODE)                                                                        \
+  hipLaunchKernelGGL((shell_elem_kernel<T, MODE>), dim3(grid), dim3(SH_TEAM * SH_EPB), 0, stream, \
+                     nelem, conn, X, u, psi, telem, P, (T*)ke, ev, es)
+
+int tacs_hip_shell_elem_kernel(int prec, int mode, int nelem, const int* conn, const double* X,
+                               const double* u, const double* psi, const double* telem,
+                               const ShellMaterial* mat, double scale, double ks_weight,
+                               double ks_max, double ks_sum, double safety, void* ke, double* ev,
+                               double* es, hipStream_t stream) {
+  if (nelem <= 0) return 0;
+  if (!mat || (prec != 64 && prec != 32)) return TACS_HIP_ERR_ARG;
+  ShellParams P = shell_params(mat, scale, ks_weight, ks_max, ks_sum, safety);
+  int grid = (nelem + SH_EPB - 1) / SH_EPB;
+  if (prec == 32) {
+    if (mode != SH_MATRIX) return TACS_HIP_ERR_ARG;
+    SH_LAUNCH(float, SH_MATRIX);
+    return (int)hipGetLastError();
+  }
+  switch (mode) {
+    case SH_MATRIX: SH_LAUNCH(double, SH_MATRIX); break;
+    case SH_RESIDUAL: SH_LAUNCH(double, SH_RESIDUAL); break;
+    case SH_COMPLIANCE: SH_LAUNCH(double, SH_COMPLIANCE); break;
+    case SH_KS_MAX: SH_LAUNCH(double, SH_KS_MAX); break;
+    case SH_KS_SUM: SH_LAUNCH(double, SH_KS_SUM); break;
+    case SH_KS_SV: SH_LAUNCH(double, SH_KS_SV); break;
+    case SH_KS_DV: SH_LAUNCH(double, SH_KS_DV); break;
+    case SH_ADJ_PRODUCT: SH_LAUNCH(double, SH_ADJ_PRODUCT); break;
+    case SH_COMPLIANCE_SV: SH_LAUNCH(double, SH_COMPLIANCE_SV); break;
+    case SH_COMPLIANCE_DV: SH_LAUNCH(double, SH_COMPLIANCE_DV); break;
+    default: return TACS_HIP_ERR_ARG;
+  }
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_gather_elem_matrices(int b, int npe, int prec, int nrows, const int* n2e_ptr,
+                                  const int* n2e, const int* conn, const void* ke, const int* rowp,
+                                  const int* cols, const int* bcvars, double* A, float* A32,
+                                  hipStream_t stream) {
+  if (nrows <= 0) return 0;
+  int grid = tacs_grid_for(nrows, 4, 1 << 20);
+  if (b == 6 && npe == 4 && prec == 64)
+    hipLaunchKernelGGL((gather_elem_matrices<6, 4, double>), dim3(grid), dim3(256), 0, stream,
+                       nrows, n2e_ptr, n2e, conn, (const double*)ke, rowp, cols, bcvars, A, A32);
+  else if (b == 6 && npe == 4 && prec == 32)
+    hipLaunchKernelGGL((gather_elem_matrices<6, 4, float>), dim3(grid), dim3(256), 0, stream,
+                       nrows, n2e_ptr, n2e, conn, (const float*)ke, rowp, cols, bcvars, A, A32);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_gather_elem_vectors_b(int b, int nv, int nnodes, const int* n2e_ptr, const int* n2e,
+                                   const double* ev, int accumulate, double* out,
+                                   hipStream_t stream) {
+  if (nnodes <= 0) return 0;
+  int grid = (nnodes + 255) / 256;
+  if (b == 6 && nv == 24)
+    hipLaunchKernelGGL((gather_elem_vectors_b<6, 24>), dim3(grid), dim3(256), 0, stream, nnodes,
+                       n2e_ptr, n2e, ev, accumulate, out);
+  else if (b == 3 && nv == 24)
+    hipLaunchKernelGGL((gather_elem_vectors_b<3, 24>), dim3(grid), dim3(256), 0, stream, nnodes,
+                       n2e_ptr, n2e, ev, accumulate, out);
+  else if (b == 3 && nv == 81)
+    hipLaunchKernelGGL((gather_elem_vectors_b<3, 81>), dim3(grid), dim3(256), 0, stream, nnodes,
+                       n2e_ptr, n2e, ev, accumulate, out);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_shell_pressure_vectors(int naux, const int* aux_elem, const double* pressure,
+                                    const int* conn, const double* X, double lambda, double* ev,
+                                    hipStream_t stream) {
+  if (naux <= 0) return 0;
+  hipLaunchKernelGGL(shell_pressure_vectors, dim3((naux + 127) / 128), dim3(128), 0, stream, naux,
+                     aux_elem, pressure, conn, X, lambda, ev);
+  return (int)hipGetLastError();
+}
+
+}  // extern "C"

@@ -53,3 +53,37 @@ class Traction3D(TACSHandle):
                                        + [0.0] * max(0, 24 - len(traction))))
         super().__init__(lib().tacs_traction3d_create(varsPerNode, faceIndex, basis.ptr, arr,
                                                       1 if coordComp else 0), keep=(basis,))
+
+
+class ShellNaturalTransform(TACSHandle):
+    """TACSShellNaturalTransform (tacs/elements.pyx:646)."""
+
+    def __init__(self):
+        super().__init__(lib().tacs_shell_natural_transform_create())
+
+
+class Quad4Shell(TACSHandle):
+    """TACSQuad4Shell (tacs/elements.pyx:694): 4-node MITC shell, 6 dof per
+    node; transform None selects the natural transform."""
+
+    def __init__(self, transform, con):
+        if transform is None:
+            transform = ShellNaturalTransform()
+        self.transform = transform
+        super().__init__(lib().tacs_quad4_shell_create(transform.ptr, con.ptr),
+                         keep=(transform, con))
+
+    def getVarsPerNode(self):
+        return 6
+
+    def getNumNodes(self):
+        return 4
+
+    def createElementPressure(self, faceIndex, p):
+        """Uniform-pressure aux element (TACSShellElement::createElementPressure)."""
+        return _Aux(lib().tacs_element_create_pressure(self.ptr, faceIndex, float(p)))
+
+
+class _Aux(TACSHandle):
+    def __init__(self, ptr):
+        super().__init__(ptr)

@@ -95,3 +95,63 @@ class SolidProblem:
     def compliance(self):
         f = functions.Compliance(self.assembler)
         return self.assembler.evalFunctions([f])[0], f
+
+
+class ShellProblem:
+    """A flat square plate of Quad4 shells (6 dof per node) under uniform
+    pressure, as tests/integration_tests/test_shell_plate_quad.py builds it
+    with pyTACS (TACSQuad4Shell, TACSIsoShellConstitutive, natural transform).
+
+    bc: "ss" (simply supported with restrained in-plane motion: u = v = w = 0
+    on the perimeter, rotations free) or "clamped" (all six dofs of the
+    perimeter).  dv: None, "shared" or "element" as for SolidProblem.
+    warp: amplitude of a seeded random z perturbation of the interior nodes
+    (0 keeps the plate flat).
+    """
+
+    def __init__(self, n, L=1.0, E=70e9, nu=0.3, rho=2700.0, ys=270e6, t0=0.01, dv=None,
+                 pressure=1e4, bc="ss", warp=0.0, seed=0):
+        from .meshes import PlateMesh
+        self.mesh = m = PlateMesh(n, n, L, L)
+        if warp:
+            rng = np.random.default_rng(seed)
+            interior = np.setdiff1d(np.arange(m.nnodes), m.edge_nodes())
+            m.X[interior, 2] += rng.uniform(-warp, warp, interior.size)
+        self.props = constitutive.MaterialProperties(rho=rho, E=E, nu=nu, ys=ys)
+        self.transform = elements.ShellNaturalTransform()
+        a = self.assembler = TACS.Assembler.create(None, 6, m.nnodes, m.nelem)
+        ptr = np.arange(0, 4 * m.nelem + 1, 4, dtype=np.intc)
+        a.setElementConnectivity(ptr, m.conn.reshape(-1))
+        if dv == "element":
+            elems = [elements.Quad4Shell(self.transform,
+                                         constitutive.IsoShellConstitutive(self.props, t=t0, tNum=e))
+                     for e in range(m.nelem)]
+        else:
+            con = constitutive.IsoShellConstitutive(self.props, t=t0,
+                                                    tNum=0 if dv == "shared" else -1)
+            elems = [elements.Quad4Shell(self.transform, con)] * m.nelem
+        self.elems = elems
+        a.setElements(elems)
+        self.bc_nodes = m.edge_nodes()
+        self.bc_vars = [0, 1, 2] if bc == "ss" else [0, 1, 2, 3, 4, 5]
+        a.addBCs(self.bc_nodes, self.bc_vars)
+        a.initialize()
+        self.X = a.createNodeVec()
+        self.X.getArray()[:] = m.X.reshape(-1)
+        a.setNodes(self.X)
+        self.pressure = pressure
+        self.aux = TACS.AuxElements()
+        pr = elems[0].createElementPressure(0, pressure)
+        self.aux_elems = np.arange(m.nelem, dtype=np.int32)
+        for e in range(m.nelem):
+            self.aux.addElement(e, pr)
+        a.setAuxElements(self.aux)
+        self.res = a.createVec()
+        self.u = a.createVec()
+        self.mat = a.createMat()
+
+    set_thickness = SolidProblem.set_thickness
+    assemble = SolidProblem.assemble
+    solve_fp64 = SolidProblem.solve_fp64
+    solve_mixed = SolidProblem.solve_mixed
+    compliance = SolidProblem.compliance

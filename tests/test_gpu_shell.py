@@ -1,0 +1,185 @@
+"""GPU parity of the Quad4 shell path (6x6 block-CSR assembly, pressure loads,
+solve, compliance / KS-failure functions and their thickness adjoints)
+against the oracle's shell restatement on the same inputs."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tacs_amd import TACS, functions
+from tacs_amd.problems import ShellProblem
+
+pytestmark = pytest.mark.gpu
+
+# FP64 parity bar: values within 1e-10 relative to the largest entry;
+# sparsity pattern and BC rows bit-exact.
+RTOL = 1e-10
+E, NU, YS = 70e9, 0.3, 270e6
+
+
+def _setup(n, warp=0.02, seed=1, bc="ss", dv="element", pressure=1e4):
+    p = ShellProblem(n, warp=warp, seed=seed, bc=bc, dv=dv, pressure=pressure, ys=YS)
+    rng = np.random.default_rng(seed)
+    t = rng.uniform(0.005, 0.02, p.mesh.nelem) if dv == "element" else None
+    if t is not None:
+        p.set_thickness(t)
+    else:
+        t = np.full(p.mesh.nelem, 0.01)
+    return p, t
+
+
+def _mask(p):
+    m = 0
+    for v in p.bc_vars:
+        m |= 1 << v
+    return np.full(p.bc_nodes.size, m, np.int32)
+
+
+def _oracle(p, t, u):
+    m = p.mesh
+    X = m.X.reshape(-1)
+    rowp, cols = O.node_csr(m.nnodes, m.conn, 4)
+    A, res = O.assemble_shell(m.conn, X, u, t, E, NU, rowp, cols)
+    O.shell_add_pressure(m.conn, X, p.aux_elems, np.full(p.aux_elems.size, p.pressure), res)
+    O.apply_bc_mat(6, p.bc_nodes, _mask(p), rowp, cols, A)
+    O.apply_bc_vec(6, p.bc_nodes, _mask(p), res, u=u)
+    return rowp, cols, A, res
+
+
+def _dense(rowp, cols, A, b=6):
+    n = rowp.size - 1
+    K = np.zeros((b * n, b * n))
+    Ab = A.reshape(-1, b, b)
+    for i in range(n):
+        for j in range(rowp[i], rowp[i + 1]):
+            K[b * i:b * i + b, b * cols[j]:b * cols[j] + b] = Ab[j]
+    return K
+
+
+@pytest.mark.parametrize("n,warp", [(4, 0.02), (5, 0.0)])
+def test_shell_jacobian_and_residual_match_oracle(n, warp):
+    p, t = _setup(n, warp=warp)
+    a = p.assembler
+    rng = np.random.default_rng(3)
+    u = rng.normal(0.0, 1e-3, 6 * p.mesh.nnodes)
+    uv = a.createVec()
+    uv.getArray()[:] = u
+    a.setVariables(uv)
+    a.assembleJacobian(1.0, 0.0, 0.0, p.res, p.mat)
+    rowp, cols, A = p.mat.getArrays()
+    orp, ocols, oA, ores = _oracle(p, t, u)
+    assert np.array_equal(rowp, orp) and np.array_equal(cols, ocols)
+    assert np.abs(A.reshape(-1) - oA).max() <= RTOL * np.abs(oA).max()
+    Ab = A.reshape(-1, 6, 6)
+    for i in p.bc_nodes[:6]:
+        for j in range(rowp[i], rowp[i + 1]):
+            blk = Ab[j]
+            for r in p.bc_vars:
+                expect = np.zeros(6)
+                if cols[j] == i:
+                    expect[r] = 1.0
+                assert np.array_equal(blk[r], expect)
+    r = p.res.getArray()
+    assert np.abs(r - ores).max() <= RTOL * np.abs(ores).max()
+    # assembleRes alone (residual kernel B^T C B u, not K u) agrees too
+    r2 = a.createVec()
+    a.assembleRes(r2)
+    assert np.abs(r2.getArray() - ores).max() <= RTOL * np.abs(ores).max()
+
+
+def test_shell_spmv_matches_oracle():
+    p, t = _setup(6)
+    p.assemble()
+    rowp, cols, A = p.mat.getArrays()
+    x = p.assembler.createVec()
+    xv = np.random.default_rng(0).standard_normal(x.getSize())
+    x.getArray()[:] = xv
+    y = p.assembler.createVec()
+    p.mat.mult(x, y)
+    yo = O.bsr_mult(6, rowp, cols, A.reshape(-1), xv)
+    assert np.abs(y.getArray() - yo).max() <= 1e-13 * np.abs(yo).max() * 10
+
+
+@pytest.mark.parametrize("case,comp,ks", [
+    ("pressure", 4679.345460332448, 1.2938623156883402),
+    ("point_load", 683.8571611645635, 0.5757488025912149),
+])
+def test_shell_plate_golden(case, comp, ks):
+    """tests/integration_tests/test_shell_plate_quad.py:32-46: 10 x 10 plate,
+    t = 5 mm, ys = 464 MPa, perimeter clamped, 100 kPa pressure or a 10 kN
+    centre load; compliance and KS failure (ksWeight 10)."""
+    p = ShellProblem(10, t0=0.005, ys=464e6, bc="clamped",
+                     pressure=100e3 if case == "pressure" else 0.0)
+    a = p.assembler
+    p.assemble()
+    if case == "point_load":
+        f = a.createVec()
+        f.getArray()[:] = 0.0
+        f.getArray()[6 * p.mesh.center_node() + 2] = -1e4  # residual = K u - F
+        p.res.axpy(1.0, f)
+    flag, its, pc, ksm = p.solve_fp64(m=200, restarts=50, rtol=1e-14)
+    c = a.evalFunctions([functions.Compliance(a)])[0]
+    k = a.evalFunctions([functions.KSFailure(a, ksWeight=10.0)])[0]
+    assert abs(c - comp) <= 1e-9 * comp
+    assert abs(k - ks) <= 1e-9 * ks
+
+
+@pytest.mark.parametrize("solver", ["fp64", "mixed"])
+def test_shell_solve_functions_and_thickness_adjoint(solver):
+    p, t = _setup(8, warp=0.01, seed=4)
+    a = p.assembler
+    p.assemble()
+    if solver == "fp64":
+        flag, its, pc, ksm = p.solve_fp64(m=120, restarts=200, rtol=1e-12)
+    else:
+        flag, its, pc, ksm = p.solve_mixed(m=120, restarts=500, rtol=1e-12)
+    assert flag == 1
+    u = p.u.getArray().copy()
+    rowp, cols, A, res0 = _oracle(p, t, np.zeros_like(u))
+    K = _dense(rowp, cols, A)
+    # true residual of the GPU solution against the oracle's system
+    assert np.linalg.norm(K @ u + res0) <= 1e-10 * np.linalg.norm(res0)
+    X = p.mesh.X.reshape(-1)
+    conn = p.mesh.conn
+    # functions at the same u
+    comp = functions.Compliance(a)
+    ksf = functions.KSFailure(a, ksWeight=100.0)
+    vals = a.evalFunctions([comp, ksf])
+    oc = O.shell_compliance(conn, X, u, t, E, NU)
+    oks = O.shell_ks_failure(conn, X, u, t, E, NU, YS, 100.0)
+    assert abs(vals[0] - oc) <= RTOL * abs(oc)
+    assert abs(vals[1] - oks[0]) <= RTOL * abs(oks[0])
+    # d/du and partial d/dt
+    dfdu = [a.createVec(), a.createVec()]
+    a.addSVSens([comp, ksf], dfdu)
+    dcdu_o, dcdt_o = O.shell_compliance_sens(conn, X, u, t, E, NU)
+    dkdu_o, dkdt_o = O.shell_ks_sens(conn, X, u, t, E, NU, YS, 100.0, oks[1], oks[2])
+    g0, g1 = dfdu[0].getArray().copy(), dfdu[1].getArray().copy()
+    assert np.abs(g0 - dcdu_o).max() <= RTOL * np.abs(dcdu_o).max()
+    assert np.abs(g1 - dkdu_o).max() <= RTOL * np.abs(dkdu_o).max()
+    dfdx = [a.createDesignVec(), a.createDesignVec()]
+    a.addDVSens([comp, ksf], dfdx, 1.0)
+    assert np.abs(dfdx[0].getArray() - dcdt_o).max() <= RTOL * np.abs(dcdt_o).max()
+    assert np.abs(dfdx[1].getArray() - dkdt_o).max() <= RTOL * np.abs(dkdt_o).max()
+    # adjoint of the KS failure and the total thickness gradient
+    a.applyBCs(dfdu[1])
+    psi = a.createVec()
+    assert ksm.solve(dfdu[1], psi) == 1
+    pv = psi.getArray().copy()
+    prod = a.createDesignVec()
+    a.addAdjointResProducts([psi], [prod], -1.0)
+    oprod = O.shell_adj_res_product(conn, X, u, pv, t, E, NU, -1.0)
+    assert np.abs(prod.getArray() - oprod).max() <= RTOL * np.abs(oprod).max()
+    a.addAdjointResProducts([psi], [dfdx[1]], -1.0)
+    g = dfdx[1].getArray().copy()
+    # oracle gradient from its own dense adjoint solve
+    free = np.ones(u.size, bool)
+    for b in p.bc_nodes:
+        for v in p.bc_vars:
+            free[6 * b + v] = False
+    rhs = dkdu_o.copy()
+    rhs[~free] = 0.0
+    po = np.zeros_like(u)
+    po[free] = np.linalg.solve(K[np.ix_(free, free)].T, rhs[free])
+    go = dkdt_o + O.shell_adj_res_product(conn, X, u, po, t, E, NU, -1.0)
+    tol = 1e-6 if solver == "mixed" else 1e-8
+    assert np.abs(g - go).max() <= tol * np.abs(go).max()

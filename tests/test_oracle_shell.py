@@ -163,3 +163,27 @@ def test_shell_adjoint_product_fd():
         rm = O.assemble_shell(m.conn, X, u, tm, E, NU, rowp, cols)[1]
         fd = psi @ (rp - rm) / (2 * h)
         assert abs(fd - prod[e]) <= 1e-6 * np.abs(prod).max()
+
+
+def test_shell_compliance_sensitivities_fd():
+    rng = np.random.default_rng(9)
+    m, X, t, u = _small_shell(rng)
+    dfdu, dfdt = O.shell_compliance_sens(m.conn, X, u, t, E, NU)
+    c0 = O.shell_compliance(m.conn, X, u, t, E, NU)
+    # compliance is quadratic in u: the central difference is exact up to rounding
+    h = 1e-7
+    for i in rng.choice(u.size, 12, replace=False):
+        up, um = u.copy(), u.copy()
+        up[i] += h
+        um[i] -= h
+        fd = (O.shell_compliance(m.conn, X, up, t, E, NU)
+              - O.shell_compliance(m.conn, X, um, t, E, NU)) / (2 * h)
+        assert abs(fd - dfdu[i]) <= 1e-6 * np.abs(dfdu).max()
+    h = 1e-7
+    for e in range(m.nelem):
+        tp, tm = t.copy(), t.copy()
+        tp[e] += h
+        tm[e] -= h
+        fd = (O.shell_compliance(m.conn, X, u, tp, E, NU)
+              - O.shell_compliance(m.conn, X, u, tm, E, NU)) / (2 * h)
+        assert abs(fd - dfdt[e]) <= 1e-6 * abs(c0)
