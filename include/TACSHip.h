@@ -269,6 +269,7 @@ class TACSParallelMat : public TACSMat {
   float* getValues32(int refresh);
   int bsize, nrows;
   long nnzb;
+  long version;  // incremented whenever the values change
   int *d_rowp, *d_cols;
   double* d_A;
   float* d_A32;
@@ -339,14 +340,29 @@ class TACSMixedGMRES : public TACSKsm {
   TACSMixedGMRES(TACSParallelMat* mat, TACSBlockJacobiPc* pc, int m, int nrestart);
   ~TACSMixedGMRES();
   int solve(TACSVec* b, TACSVec* x, int zero_guess = 1);
+  int solveDeflated(TACSVec* b, TACSVec* x, int zero_guess);
   void setTolerances(double rtol, double atol);
+  // Krylov subspace recycling (GCRO-DR, Parks et al., SIAM J. Sci. Comput.
+  // 28 (2006)): each cycle keeps the k harmonic Ritz vectors of smallest
+  // harmonic Ritz value of the fixed FP32-valued operator, and the space is
+  // reused by later cycles and later solves with the same matrix
+  // (k <= 31, m <= 255); k = 0 restarts plainly.
+  int setDeflation(int k);
   double rtol, atol;
-  int msub, nrestart, outerCount;
+  int msub, nrestart, outerCount, ndefl;
+  int recycleDim;       // k of the current recycle space (0: none)
+  long recycleVersion;  // matrix version the recycle space belongs to
   TACSParallelMat* mat;
   TACSBlockJacobiPc* pc;
   float* d_W;
   float* d_work32;
   float* d_d32;
+  double* d_B;     // FP64 basis [U (k) | C (k) | V (m+1)] of the recycling solver
+  double* d_T;     // 2k temporary vectors (new U and C)
+  double* d_P;     // small device matrices (basis changes, inner products)
+  double* d_tsg;   // tall-skinny product scratch
+  double* d_x64;   // FP64 work vectors of the recycling solver
+  double* d_y64;
   double* d_r;
   double* d_scal;
   void* d_red;

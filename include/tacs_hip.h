@@ -186,6 +186,13 @@ int tacs_hip_bsr_spmv(int b, int nrows, const int* rowp, const int* cols, const 
 /* FP32 variant of tacs_hip_bsr_spmv (mixed-precision inner solve). */
 int tacs_hip_bsr_spmv_f32(int b, int nrows, const int* rowp, const int* cols, const float* A,
                           const float* x, float* y, int add, hipStream_t stream);
+/* y (+)= A32 x with FP32 matrix values and FP64 vectors and arithmetic: the
+   fixed FP32-rounded operator of the deflated mixed-precision GMRES. */
+int tacs_hip_bsr_spmv_mixed(int b, int nrows, const int* rowp, const int* cols, const float* A,
+                            const double* x, double* y, int add, hipStream_t stream);
+/* y = D32^{-1} x per block with FP32 inverse blocks and FP64 vectors. */
+int tacs_hip_bsr_diag_apply_mixed(int b, int nrows, const float* Dinv, const double* x, double* y,
+                                  hipStream_t stream);
 /* Y = A X for nrhs right-hand sides stored node-major
    X[(3*node + c)*nrhs + j]; the matrix is read once for all of them
    (the reference multiplies one vector at a time, BCSRMatVecMult3). */
@@ -233,6 +240,32 @@ int tacs_hip_givens(int i, int m, double* H, double* Qc, double* Qs, double* res
                     hipStream_t stream);
 int tacs_hip_backsolve(int niters, int m, const double* H, double* res, hipStream_t stream);
 int tacs_hip_scalar_op(const double* src, double s, double* dst, hipStream_t stream);
+
+/* out[:, j] = sum_i W[:, i] P[i, j] for j < nout (FP32 basis W, n x nin with
+   leading dimension ldw; P column-major nin x nout, FP64 on the device;
+   nout <= 32, nin <= 512): the basis change of a deflated GMRES restart.  No
+   counterpart in the reference. */
+int tacs_hip_basis_combine(size_t n, int nin, const float* W, size_t ldw, const double* P,
+                           int nout, float* out, size_t ldo, hipStream_t stream);
+
+/* FP64 basis change out[:, j] (+)= sum_i W[:, i] P[i, j] (nin <= 256,
+   nout <= 32), and the tall-skinny product out = X^T Y (n x p and n x q,
+   p*q <= 4096, p + q <= 256; scratch of tacs_hip_tsgemm_scratch_bytes()),
+   used by the recycling GMRES.  No counterpart in the reference. */
+int tacs_hip_basis_combine64(size_t n, int nin, const double* W, size_t ldw, const double* P,
+                             int nout, double* out, size_t ldo, int add, hipStream_t stream);
+size_t tacs_hip_tsgemm_scratch_bytes(void);
+int tacs_hip_tsgemm(size_t n, int p, const double* X, int q, const double* Y, double* scratch,
+                    double* out, hipStream_t stream);
+
+/* Host-side dense helper of the deflated-restart GMRES: eigenvalues of the
+   row-major n x n matrix G sorted by ascending modulus (wr, wi) and a real
+   basis V (column-major n x kout) of the eigenvectors of the k smallest; a
+   complex pair contributes its real and imaginary parts (kout <= k + 1).
+   Returns kout, or -1 when the QR iteration fails.  No counterpart in the
+   reference (its GMRES restarts without deflation). */
+int tacs_dense_smallest_eigvecs(int n, const double* G, int k, double* V, double* wr,
+                                double* wi);
 
 /* ------------------------------------------------------------------------
    2. Object entry points (mirror tacs/cpp_headers/TACS.pxd and the Cython
@@ -355,6 +388,10 @@ int tacs_gmres_set_ortho_type(tacs_obj ksm, int ortho);
 /* Mixed-precision restarted GMRES(m) (FP32 Arnoldi cycles, FP64 restart
    residual and solution); no counterpart in the reference. */
 tacs_obj tacs_mixed_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart);
+/* Deflated restarts for the mixed GMRES (GMRES-DR: keep the k harmonic Ritz
+   vectors of smallest harmonic Ritz value at each restart; 0 <= k <= 31,
+   k < m, m <= 511).  No counterpart in the reference. */
+int tacs_mixed_gmres_set_deflation(tacs_obj ksm, int k);
 int tacs_ksm_set_tolerances(tacs_obj ksm, double rtol, double atol);
 int tacs_ksm_solve(tacs_obj ksm, tacs_obj b, tacs_obj x, int zero_guess);
 int tacs_ksm_get_iter_count(tacs_obj ksm);

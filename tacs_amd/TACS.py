@@ -160,11 +160,19 @@ class MixedKSM(KSM):
     restarts from the FP64 residual b - A x; its Arnoldi steps use the FP32
     matrix, preconditioner and basis with FP64 Gram-Schmidt dot products and
     Hessenberg; the correction is accumulated in FP64.  No counterpart in the
-    reference."""
+    reference.
 
-    def __init__(self, mat, pc, m=30, restarts=1000):
+    deflation=k > 0 turns on deflated restarts (GMRES-DR): each restart keeps
+    the k harmonic Ritz vectors with the smallest harmonic Ritz values."""
+
+    def __init__(self, mat, pc, m=30, restarts=1000, deflation=0):
         TACSHandle.__init__(self, lib().tacs_mixed_gmres_create(mat.ptr, pc.ptr, m, restarts),
                             keep=(mat, pc))
+        if deflation:
+            self.setDeflation(deflation)
+
+    def setDeflation(self, k):
+        check(lib().tacs_mixed_gmres_set_deflation(self.ptr, int(k)), "MixedKSM.setDeflation")
 
     def getOuterCount(self):
         return lib().tacs_mixed_gmres_get_outer_count(self.ptr)

@@ -19,12 +19,13 @@
 // handled with all loads of the row issued before the arithmetic; the
 // values are streamed with non-temporal loads so that x stays in cache.
 // Partial sums are reduced over c and bo with cross-lane shuffles.
-// y = A x (+ y if add).
+// y = A x (+ y if add).  TA: value type of A; T: vectors (FP32 values with
+// FP64 vectors is the fixed FP32-rounded operator of the deflated solver).
 // ---------------------------------------------------------------------------
-template <typename T, typename ACC, int NT, int R>
+template <typename TA, typename T, typename ACC, int NT, int R>
 __global__ __launch_bounds__(256) void bsr3_spmv(int nrows, const int* __restrict__ rowp,
                                                  const int* __restrict__ cols,
-                                                 const T* __restrict__ A,
+                                                 const TA* __restrict__ A,
                                                  const T* __restrict__ x, T* __restrict__ y,
                                                  int add) {
   const int lane = threadIdx.x & 63;
@@ -49,7 +50,7 @@ __global__ __launch_bounds__(256) void bsr3_spmv(int nrows, const int* __restric
     ACC acc[R];
     if (fast) {
       int col[R][NT];
-      T av[R][NT];
+      TA av[R][NT];
 #pragma unroll
       for (int r = 0; r < R; r++)
 #pragma unroll
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(256) void bsr3_spmv(int nrows, const int* __restric
           int s = bo + 7 * t;
           av[r][t] = (ok && s < nb[r])
                          ? __builtin_nontemporal_load(&A[((size_t)rs[r] + s) * 9 + rc])
-                         : (T)0;
+                         : (TA)0;
         }
 #pragma unroll
       for (int r = 0; r < R; r++) {
@@ -105,10 +106,10 @@ __global__ __launch_bounds__(256) void bsr3_spmv(int nrows, const int* __restric
 // Generic b x b SpMV (b = 6 for shells): one wavefront per block row, lanes
 // stride over the row's contiguous values.
 // ---------------------------------------------------------------------------
-template <int B, typename T, typename ACC>
+template <int B, typename TA, typename T, typename ACC>
 __global__ __launch_bounds__(256) void bsr_spmv_generic(int nrows, const int* __restrict__ rowp,
                                                         const int* __restrict__ cols,
-                                                        const T* __restrict__ A,
+                                                        const TA* __restrict__ A,
                                                         const T* __restrict__ x,
                                                         T* __restrict__ y, int add) {
   constexpr int B2 = B * B;
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(256) void bsr_spmv_generic(int nrows, const int* __
     ACC acc[B];
 #pragma unroll
     for (int r = 0; r < B; r++) acc[r] = 0;
-    const T* Ar = A + (size_t)rs * B2;
+    const TA* Ar = A + (size_t)rs * B2;
     for (int k = lane; k < nval; k += 64) {
       int s = k / B2, rc = k - s * B2, r = rc / B, c = rc - r * B;
       ACC v = (ACC)Ar[k] * (ACC)x[(size_t)cols[rs + s] * B + c];
@@ -268,20 +269,20 @@ __global__ void bsr_diag_inverse(int nrows, const int* __restrict__ diag,
 }
 
 // y = D^{-1} x per block; one thread per block row
-template <int B, typename T>
-__global__ void bsr_diag_apply(int nrows, const T* __restrict__ Dinv, const T* __restrict__ x,
+template <int B, typename TD, typename T>
+__global__ void bsr_diag_apply(int nrows, const TD* __restrict__ Dinv, const T* __restrict__ x,
                                T* __restrict__ y) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrows) return;
   T xx[B];
 #pragma unroll
   for (int c = 0; c < B; c++) xx[c] = x[(size_t)B * i + c];
-  const T* d = Dinv + (size_t)B * B * i;
+  const TD* d = Dinv + (size_t)B * B * i;
 #pragma unroll
   for (int r = 0; r < B; r++) {
     T v = 0;
 #pragma unroll
-    for (int c = 0; c < B; c++) v += d[B * r + c] * xx[c];
+    for (int c = 0; c < B; c++) v += (T)d[B * r + c] * xx[c];
     y[(size_t)B * i + r] = v;
   }
 }
@@ -318,10 +319,10 @@ int tacs_hip_bsr_spmv(int b, int nrows, const int* rowp, const int* cols, const 
   if (nrows <= 0) return 0;
   int grid = tacs_grid_for(nrows, 8, 65536);
   if (b == 3)
-    hipLaunchKernelGGL((bsr3_spmv<double, double, 4, 2>), dim3(grid), dim3(256), 0, stream, nrows, rowp,
+    hipLaunchKernelGGL((bsr3_spmv<double, double, double, 4, 2>), dim3(grid), dim3(256), 0, stream, nrows, rowp,
                        cols, A, x, y, add);
   else if (b == 6)
-    hipLaunchKernelGGL((bsr_spmv_generic<6, double, double>), dim3(grid), dim3(256), 0, stream,
+    hipLaunchKernelGGL((bsr_spmv_generic<6, double, double, double>), dim3(grid), dim3(256), 0, stream,
                        nrows, rowp, cols, A, x, y, add);
   else
     return TACS_HIP_ERR_ARG;
@@ -333,11 +334,39 @@ int tacs_hip_bsr_spmv_f32(int b, int nrows, const int* rowp, const int* cols, co
   if (nrows <= 0) return 0;
   int grid = tacs_grid_for(nrows, 8, 65536);
   if (b == 3)
-    hipLaunchKernelGGL((bsr3_spmv<float, float, 4, 2>), dim3(grid), dim3(256), 0, stream, nrows, rowp,
+    hipLaunchKernelGGL((bsr3_spmv<float, float, float, 4, 2>), dim3(grid), dim3(256), 0, stream, nrows, rowp,
                        cols, A, x, y, add);
   else if (b == 6)
-    hipLaunchKernelGGL((bsr_spmv_generic<6, float, float>), dim3(grid), dim3(256), 0, stream,
+    hipLaunchKernelGGL((bsr_spmv_generic<6, float, float, float>), dim3(grid), dim3(256), 0, stream,
                        nrows, rowp, cols, A, x, y, add);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_bsr_spmv_mixed(int b, int nrows, const int* rowp, const int* cols, const float* A,
+                            const double* x, double* y, int add, hipStream_t stream) {
+  if (nrows <= 0) return 0;
+  int grid = tacs_grid_for(nrows, 8, 65536);
+  if (b == 3)
+    hipLaunchKernelGGL((bsr3_spmv<float, double, double, 4, 2>), dim3(grid), dim3(256), 0, stream,
+                       nrows, rowp, cols, A, x, y, add);
+  else if (b == 6)
+    hipLaunchKernelGGL((bsr_spmv_generic<6, float, double, double>), dim3(grid), dim3(256), 0,
+                       stream, nrows, rowp, cols, A, x, y, add);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_bsr_diag_apply_mixed(int b, int nrows, const float* Dinv, const double* x, double* y,
+                                  hipStream_t stream) {
+  if (nrows <= 0) return 0;
+  dim3 g((nrows + 255) / 256), t(256);
+  if (b == 3)
+    hipLaunchKernelGGL((bsr_diag_apply<3, float, double>), g, t, 0, stream, nrows, Dinv, x, y);
+  else if (b == 6)
+    hipLaunchKernelGGL((bsr_diag_apply<6, float, double>), g, t, 0, stream, nrows, Dinv, x, y);
   else
     return TACS_HIP_ERR_ARG;
   return (int)hipGetLastError();
@@ -378,9 +407,9 @@ int tacs_hip_bsr_diag_apply(int b, int nrows, const double* Dinv, const double* 
   if (nrows <= 0) return 0;
   dim3 g((nrows + 255) / 256), t(256);
   if (b == 3)
-    hipLaunchKernelGGL((bsr_diag_apply<3, double>), g, t, 0, stream, nrows, Dinv, x, y);
+    hipLaunchKernelGGL((bsr_diag_apply<3, double, double>), g, t, 0, stream, nrows, Dinv, x, y);
   else if (b == 6)
-    hipLaunchKernelGGL((bsr_diag_apply<6, double>), g, t, 0, stream, nrows, Dinv, x, y);
+    hipLaunchKernelGGL((bsr_diag_apply<6, double, double>), g, t, 0, stream, nrows, Dinv, x, y);
   else
     return TACS_HIP_ERR_ARG;
   return (int)hipGetLastError();
@@ -391,9 +420,9 @@ int tacs_hip_bsr_diag_apply_f32(int b, int nrows, const float* Dinv, const float
   if (nrows <= 0) return 0;
   dim3 g((nrows + 255) / 256), t(256);
   if (b == 3)
-    hipLaunchKernelGGL((bsr_diag_apply<3, float>), g, t, 0, stream, nrows, Dinv, x, y);
+    hipLaunchKernelGGL((bsr_diag_apply<3, float, float>), g, t, 0, stream, nrows, Dinv, x, y);
   else if (b == 6)
-    hipLaunchKernelGGL((bsr_diag_apply<6, float>), g, t, 0, stream, nrows, Dinv, x, y);
+    hipLaunchKernelGGL((bsr_diag_apply<6, float, float>), g, t, 0, stream, nrows, Dinv, x, y);
   else
     return TACS_HIP_ERR_ARG;
   return (int)hipGetLastError();

@@ -414,6 +414,11 @@ tacs_obj tacs_mixed_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart)
   }
   return (TACSObject*)new TACSMixedGMRES(A, p, m, nrestart);
 }
+int tacs_mixed_gmres_set_deflation(tacs_obj ksm, int k) {
+  clear_error();
+  REQUIRE(TACSMixedGMRES, g, ksm);
+  return g->setDeflation(k);
+}
 int tacs_ksm_set_tolerances(tacs_obj ksm, double rtol, double atol) {
   REQUIRE(TACSKsm, k, ksm);
   k->setTolerances(rtol, atol);

@@ -381,7 +381,7 @@ void TACSBVec::setBCs(TACSBcMap* map, TacsScalar lambda) {
 // TACSParallelMat (one rank = the [Aloc] block-CSR matrix)
 // ---------------------------------------------------------------------------
 TACSParallelMat::TACSParallelMat(int _bsize, int _nrows, const int* rowp, const int* cols)
-    : bsize(_bsize), nrows(_nrows) {
+    : bsize(_bsize), nrows(_nrows), version(0) {
   nnzb = rowp[nrows];
   h_rowp = new int[nrows + 1];
   h_cols = new int[nnzb > 0 ? nnzb : 1];
@@ -417,6 +417,7 @@ void TACSParallelMat::mult(TACSVec* tx, TACSVec* ty) {
 void TACSParallelMat::zeroEntries() {
   KCHECK(tacs_hip_fill(64, (size_t)nnzb * bsize * bsize, d_A, 0.0, g_stream));
   A32_current = 0;
+  version++;
 }
 void TACSParallelMat::applyBCs(TACSBcMap* bcmap) {
   const int *nodes, *vars;
@@ -427,6 +428,8 @@ void TACSParallelMat::applyBCs(TACSBcMap* bcmap) {
   h2d(dn, nodes, nbc);
   h2d(dv, vars, nbc);
   KCHECK(tacs_hip_bsr_zero_rows(bsize, nbc, dn, dv, d_rowp, d_cols, d_A, 1, g_stream));
+  A32_current = 0;
+  version++;
   HCHECK(hipStreamSynchronize(g_stream));
   dfree(dn);
   dfree(dv);
@@ -678,7 +681,9 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
 // bandwidth-bound work runs on FP32 data.
 // ---------------------------------------------------------------------------
 TACSMixedGMRES::TACSMixedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int _m, int _nrestart)
-    : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart), outerCount(0), mat(m), pc(p) {
+    : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart), outerCount(0), ndefl(0),
+      recycleDim(0), recycleVersion(-1), mat(m), pc(p), d_B(NULL), d_T(NULL), d_P(NULL),
+      d_tsg(NULL), d_x64(NULL), d_y64(NULL) {
   mat->incref();
   if (pc) pc->incref();
   mat->getValues32(0);  // allocate the FP32 copy now; assembly fills it
@@ -693,9 +698,11 @@ TACSMixedGMRES::TACSMixedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int _m,
   size_t rb = tacs_hip_red_scratch_bytes();
   HCHECK(hipMalloc(&d_red, rb));
   HCHECK(hipMemsetAsync(d_red, 0, rb, g_stream));
-  HCHECK(hipHostMalloc((void**)&h_pin, 64 * sizeof(double), 0));
+  HCHECK(hipHostMalloc((void**)&h_pin, (size_t)(msub + 64) * sizeof(double), 0));
 }
 TACSMixedGMRES::~TACSMixedGMRES() {
+  void* rec[] = {d_B, d_T, d_P, d_tsg, d_x64, d_y64};
+  for (void* q : rec) dfree(q);
   dfree(d_W);
   dfree(d_work32);
   dfree(d_d32);
@@ -711,7 +718,31 @@ void TACSMixedGMRES::setTolerances(double r, double a) {
   atol = a;
 }
 
+int TACSMixedGMRES::setDeflation(int k) {
+  if (k < 0 || k > 31 || (k > 0 && (k >= msub - 1 || msub + 2 * k > 250)))
+    return tacs_set_error("setDeflation: need 0 <= k <= 31, k < m - 1 and m + 2k <= 250 "
+                          "(k = %d, m = %d)", k, msub);
+  HCHECK(hipStreamSynchronize(g_stream));
+  void* rec[] = {d_B, d_T, d_P, d_tsg, d_x64, d_y64};
+  for (void* q : rec) dfree(q);
+  d_B = d_T = d_P = d_tsg = d_x64 = d_y64 = NULL;
+  ndefl = k;
+  recycleDim = 0;
+  recycleVersion = -1;
+  if (k > 0) {
+    size_t n = (size_t)mat->nrows * mat->bsize;
+    d_B = dalloc<double>((size_t)(2 * k + msub + 1) * n);
+    d_T = dalloc<double>((size_t)(2 * k) * n);
+    d_P = dalloc<double>((size_t)(2 * k + msub + 2) * 64);
+    d_tsg = dalloc<double>(tacs_hip_tsgemm_scratch_bytes() / sizeof(double));
+    d_x64 = dalloc<double>(n);
+    d_y64 = dalloc<double>(n);
+  }
+  return 0;
+}
+
 int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
+  if (ndefl > 0) return solveDeflated(tb, tx, zero_guess);
   TACSBVec* b = dynamic_cast<TACSBVec*>(tb);
   TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
   const size_t n = (size_t)mat->nrows * mat->bsize;
@@ -777,6 +808,341 @@ int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
     KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
     KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
   }
+  HCHECK(hipStreamSynchronize(g_stream));
+  return flag;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision GMRES with Krylov subspace recycling (GCRO-DR: M. L. Parks,
+// E. de Sturler, G. Mackey, D. D. Johnson and S. Maiti, SIAM J. Sci. Comput.
+// 28 (2006) 1651-1674).  The Arnoldi operator is the fixed FP32-rounded
+// operator A32 D32^{-1} applied in FP64 arithmetic to FP64 vectors (so the
+// recycled relation A32 D32^{-1} U = C stays exact across cycles, solves and
+// right-hand sides); every cycle starts from the true FP64 residual
+// r = b - A x, and the FP64 basis, Gram-Schmidt and small dense algebra
+// follow.  A cycle: x += D^{-1} U C^T r, r -= C C^T r; m - k Arnoldi steps of
+// (I - C C^T) A32 D32^{-1} from r; least squares over [U D_u, V]; then the k
+// harmonic Ritz vectors of smallest harmonic Ritz value of that projected
+// problem become the next U, C.  The least-squares problem and the small
+// eigenproblem run on the host (the Hessenberg column is read back each step).
+// ---------------------------------------------------------------------------
+namespace {
+struct HostLS {
+  int ld;                      // m + 1
+  std::vector<double> Hb, Hr;  // Hessenberg (column-major, ld x m): plain / rotated
+  std::vector<double> c, g;    // right-hand side: plain / rotated
+  std::vector<int> rp;         // rotations: rows (rp, rp+1)
+  std::vector<double> rc, rs;
+  explicit HostLS(int m) : ld(m + 1), Hb((size_t)(m + 1) * m), Hr((size_t)(m + 1) * m), c(m + 1),
+                           g(m + 1) {}
+  void reset() {
+    std::fill(Hb.begin(), Hb.end(), 0.0);
+    std::fill(Hr.begin(), Hr.end(), 0.0);
+    std::fill(c.begin(), c.end(), 0.0);
+    std::fill(g.begin(), g.end(), 0.0);
+    rp.clear();
+    rc.clear();
+    rs.clear();
+  }
+  void rotate(double* v, int p, double cs, double sn) {
+    double a = v[p], b = v[p + 1];
+    v[p] = cs * a + sn * b;
+    v[p + 1] = -sn * a + cs * b;
+  }
+  // zero Hr(row+1, col) against Hr(row, col): rotation of rows (row, row+1)
+  void new_rotation(int row, int col, int ncols_from, int ncols_to) {
+    double a = Hr[(size_t)col * ld + row], b = Hr[(size_t)col * ld + row + 1];
+    double r = std::hypot(a, b);
+    double cs = r == 0.0 ? 1.0 : a / r, sn = r == 0.0 ? 0.0 : b / r;
+    for (int j = ncols_from; j < ncols_to; j++) rotate(&Hr[(size_t)j * ld], row, cs, sn);
+    rotate(g.data(), row, cs, sn);
+    rp.push_back(row);
+    rc.push_back(cs);
+    rs.push_back(sn);
+  }
+  // triangularise the leading dense (k+1) x k block
+  void triangularize_leading(int k) {
+    for (int i = 0; i < k; i++)
+      for (int r = k; r > i; r--) new_rotation(r - 1, i, i, k);
+  }
+  // add column j (entries 0..j+1 in h); returns the residual estimate |g[j+1]|
+  double add_column(int j, const double* h) {
+    for (int i = 0; i < ld; i++) {
+      Hb[(size_t)j * ld + i] = i <= j + 1 ? h[i] : 0.0;
+      Hr[(size_t)j * ld + i] = Hb[(size_t)j * ld + i];
+    }
+    double* col = &Hr[(size_t)j * ld];
+    for (size_t q = 0; q < rp.size(); q++) rotate(col, rp[q], rc[q], rs[q]);
+    new_rotation(j, j, j, j + 1);
+    return fabs(g[j + 1]);
+  }
+  // y = R^{-1} g for the first nc columns
+  void solve(int nc, double* y) {
+    for (int i = nc - 1; i >= 0; i--) {
+      double s = g[i];
+      for (int j = i + 1; j < nc; j++) s -= Hr[(size_t)j * ld + i] * y[j];
+      y[i] = s / Hr[(size_t)i * ld + i];
+    }
+  }
+};
+
+// Solve A X = B in place (A row-major n x n, B row-major n x nrhs) by LU with
+// partial pivoting; returns 1 when A is singular.
+static int dense_solve_rowmajor(int n, const double* Ain, double* B, int nrhs) {
+  std::vector<double> M(Ain, Ain + (size_t)n * n);
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    for (int i = k + 1; i < n; i++)
+      if (fabs(M[(size_t)i * n + k]) > fabs(M[(size_t)p * n + k])) p = i;
+    if (M[(size_t)p * n + k] == 0.0) return 1;
+    if (p != k) {
+      for (int jj = 0; jj < n; jj++) std::swap(M[(size_t)k * n + jj], M[(size_t)p * n + jj]);
+      for (int jj = 0; jj < nrhs; jj++) std::swap(B[(size_t)k * nrhs + jj], B[(size_t)p * nrhs + jj]);
+    }
+    for (int i = k + 1; i < n; i++) {
+      double f = M[(size_t)i * n + k] / M[(size_t)k * n + k];
+      if (f == 0.0) continue;
+      for (int jj = k + 1; jj < n; jj++) M[(size_t)i * n + jj] -= f * M[(size_t)k * n + jj];
+      for (int jj = 0; jj < nrhs; jj++) B[(size_t)i * nrhs + jj] -= f * B[(size_t)k * nrhs + jj];
+    }
+  }
+  for (int i = n - 1; i >= 0; i--)
+    for (int jj = 0; jj < nrhs; jj++) {
+      double v = B[(size_t)i * nrhs + jj];
+      for (int q = i + 1; q < n; q++) v -= M[(size_t)i * n + q] * B[(size_t)q * nrhs + jj];
+      B[(size_t)i * nrhs + jj] = v / M[(size_t)i * n + i];
+    }
+  return 0;
+}
+}  // namespace
+
+int TACSMixedGMRES::solveDeflated(TACSVec* tb, TACSVec* tx, int zero_guess) {
+  TACSBVec* b = dynamic_cast<TACSBVec*>(tb);
+  TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
+  const size_t n = (size_t)mat->nrows * mat->bsize;
+  const int bs = mat->bsize, nr = mat->nrows, m = msub, K = ndefl;
+  float* A32 = mat->getValues32(1);
+  double* misc = d_scal;  // first 16 entries used as device scalars
+  double* dcol = d_P;     // device Hessenberg column / small vectors (2K + m + 2 entries)
+  double* dmat = d_P + (size_t)(2 * K + m + 2) * 2;  // device small matrices
+  double* U = d_B;
+  double* Cm = d_B + (size_t)K * n;
+  double* V0 = Cm;  // V follows the kd active columns of C: [C | V] is contiguous
+  auto Vv = [&](int i) { return V0 + (size_t)i * n; };
+  const double* bd = b->devRead();
+  double* xd = x->devWrite();
+  if (recycleVersion != mat->version) recycleDim = 0;
+  int kd = recycleDim;
+  iterCount = 0;
+  outerCount = 0;
+  if (zero_guess) {
+    KCHECK(tacs_hip_fill(64, n, xd, 0.0, g_stream));
+    HCHECK(hipMemcpyAsync(d_r, bd, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
+  } else {
+    KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
+    KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
+  }
+  std::vector<double> y(m + 1), hc(2 * K + m + 2), dscale(K + 1, 1.0);
+  std::vector<double> tmpv;
+  double r0 = 0.0;
+  int flag = 0;
+  for (int count = 0; count <= nrestart; count++) {
+    KCHECK(tacs_hip_dot(64, n, NULL, NULL, d_r, d_r, d_red, misc + MISC_NRM, 1, misc + MISC_INV,
+                        g_stream));
+    double rn = pin_read(h_pin, misc + MISC_NRM);
+    if (count == 0) r0 = rn;
+    resNorm = rn;
+    if (rn <= rtol * r0 || rn < atol) {
+      flag = 1;
+      break;
+    }
+    if (!std::isfinite(rn)) break;
+    if (kd > 0) {
+      // x += D^{-1} U (C^T r);  r -= C (C^T r)
+      KCHECK(tacs_hip_mdot(64, n, kd, Cm, n, d_r, d_red, dcol, 0, g_stream));
+      KCHECK(tacs_hip_lincomb(64, 64, n, kd, U, n, dcol, d_x64, 0, g_stream));
+      KCHECK(tacs_hip_bsr_diag_apply_mixed(bs, nr, pc->d_Dinv32, d_x64, d_y64, g_stream));
+      KCHECK(tacs_hip_axpby(64, 64, n, 1.0, d_y64, 1.0, xd, g_stream));
+      KCHECK(tacs_hip_cgs_update(64, n, kd, Cm, n, dcol, NULL, d_r, g_stream));
+      KCHECK(tacs_hip_dot(64, n, NULL, NULL, d_r, d_r, d_red, misc + MISC_NRM, 1,
+                          misc + MISC_INV, g_stream));
+    }
+    double rp = pin_read(h_pin, misc + MISC_NRM);
+    const double target = std::max(rtol * r0, atol);
+    if (rp <= target) {
+      // the projection alone converged: refresh the true residual
+      KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
+      KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
+      outerCount++;
+      continue;
+    }
+    V0 = Cm + (size_t)kd * n;
+    KCHECK(tacs_hip_scale(64, n, d_r, misc + MISC_INV, 0.0, 0, g_stream));
+    HCHECK(hipMemcpyAsync(Vv(0), d_r, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
+    // u-column scales D_u (columns of U are not normalised)
+    if (kd > 0) {
+      KCHECK(tacs_hip_tsgemm(n, kd, U, kd, U, d_tsg, dmat, g_stream));
+      tmpv.resize((size_t)kd * kd);
+      HCHECK(hipMemcpy(tmpv.data(), dmat, (size_t)kd * kd * sizeof(double),
+                       hipMemcpyDeviceToHost));
+      for (int i = 0; i < kd; i++) dscale[i] = 1.0 / sqrt(tmpv[(size_t)i * kd + i]);
+    }
+    // least squares over [U D_u, V]: columns 0..kd-1 are D_u (diagonal), then Arnoldi
+    HostLS ls(kd + (m - kd) + 1);
+    ls.reset();
+    ls.c[kd] = 1.0;
+    ls.g[kd] = 1.0;
+    for (int i = 0; i < kd; i++) {
+      std::fill(hc.begin(), hc.end(), 0.0);
+      hc[i] = dscale[i];
+      ls.add_column(i, hc.data());
+    }
+    const int s = m - kd;
+    const double tgt = target / rp;
+    int jend = 0;
+    for (int j = 0; j < s; j++) {
+      double* w = Vv(j + 1);
+      KCHECK(tacs_hip_bsr_diag_apply_mixed(bs, nr, pc->d_Dinv32, Vv(j), d_x64, g_stream));
+      KCHECK(tacs_hip_bsr_spmv_mixed(bs, nr, mat->d_rowp, mat->d_cols, A32, d_x64, w, 0,
+                                     g_stream));
+      // against [C, V_0..V_j] (contiguous): coefficients [B_j ; h_j]
+      cgs2(64, n, kd + j + 1, Cm, w, dcol, dcol + (2 * K + m + 2), d_red);
+      KCHECK(tacs_hip_dot(64, n, NULL, NULL, w, w, d_red, dcol + kd + j + 1, 1, misc + MISC_INV,
+                          g_stream));
+      KCHECK(tacs_hip_scale(64, n, w, misc + MISC_INV, 0.0, 0, g_stream));
+      HCHECK(hipMemcpyAsync(h_pin, dcol, (kd + j + 2) * sizeof(double), hipMemcpyDeviceToHost,
+                            g_stream));
+      HCHECK(hipStreamSynchronize(g_stream));
+      double st = ls.add_column(kd + j, h_pin);
+      jend = j + 1;
+      if (st <= tgt || !std::isfinite(st)) break;
+    }
+    iterCount += jend;
+    outerCount++;
+    const int nc = kd + jend;  // columns of the small problem
+    ls.solve(nc, y.data());
+    // x += rp D^{-1} (U D_u y_u + V y_v)
+    for (int i = 0; i < nc; i++) h_pin[i] = (i < kd ? dscale[i] : 1.0) * y[i];
+    HCHECK(hipMemcpyAsync(dcol, h_pin, nc * sizeof(double), hipMemcpyHostToDevice, g_stream));
+    KCHECK(tacs_hip_fill(64, n, d_x64, 0.0, g_stream));
+    if (kd > 0) KCHECK(tacs_hip_lincomb(64, 64, n, kd, U, n, dcol, d_x64, 1, g_stream));
+    KCHECK(tacs_hip_lincomb(64, 64, n, jend, V0, n, dcol + kd, d_x64, 1, g_stream));
+    KCHECK(tacs_hip_bsr_diag_apply_mixed(bs, nr, pc->d_Dinv32, d_x64, d_y64, g_stream));
+    KCHECK(tacs_hip_axpby(64, 64, n, rp, d_y64, 1.0, xd, g_stream));
+    KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
+    KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
+
+    // ---- new recycle space from the harmonic Ritz vectors of this cycle
+    if (nc <= K + 1) continue;  // too few columns to choose from: keep the old space
+    const int nrw = nc + 1;     // rows of Gbar: [C (kd) ; V_0..V_jend]
+    const int ld = ls.ld;
+    // Gbar^T Gbar and Gbar^T (What^T W)
+    std::vector<double> GtG((size_t)nc * nc, 0.0), WtW((size_t)nrw * nc, 0.0);
+    for (int a = 0; a < nc; a++)
+      for (int bb = 0; bb < nc; bb++) {
+        double acc = 0.0;
+        for (int r = 0; r < nrw; r++) acc += ls.Hb[(size_t)a * ld + r] * ls.Hb[(size_t)bb * ld + r];
+        GtG[(size_t)a * nc + bb] = acc;  // row-major
+      }
+    // What = [C, V_0..V_jend], W = [U D_u, V_0..V_{jend-1}]
+    for (int i = 0; i < jend; i++) WtW[(size_t)(kd + i) * nrw + (kd + i)] = 1.0;  // V^T V
+    if (kd > 0) {
+      // [C, V_0..V_jend]^T U (C and V contiguous), scaled by D_u
+      KCHECK(tacs_hip_tsgemm(n, kd + jend + 1, Cm, kd, U, d_tsg, dmat, g_stream));
+      tmpv.resize((size_t)(kd + jend + 1) * kd);
+      HCHECK(hipMemcpy(tmpv.data(), dmat, tmpv.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (int i = 0; i < kd; i++)
+        for (int r = 0; r < kd + jend + 1; r++)
+          WtW[(size_t)i * nrw + r] = tmpv[(size_t)i * (kd + jend + 1) + r] * dscale[i];
+    }
+    std::vector<double> GtW((size_t)nc * nc, 0.0);  // row-major Gbar^T (What^T W)
+    for (int a = 0; a < nc; a++)
+      for (int bb = 0; bb < nc; bb++) {
+        double acc = 0.0;
+        for (int r = 0; r < nrw; r++) acc += ls.Hb[(size_t)a * ld + r] * WtW[(size_t)bb * nrw + r];
+        GtW[(size_t)a * nc + bb] = acc;
+      }
+    // G = (Gbar^T What^T W)^{-1} Gbar^T Gbar: harmonic Ritz values theta, vectors z
+    std::vector<double> Gm = GtG;
+    if (dense_solve_rowmajor(nc, GtW.data(), Gm.data(), nc) != 0) continue;
+    std::vector<double> Z((size_t)nc * (K + 1), 0.0);
+    int kout = tacs_dense_smallest_eigvecs(nc, Gm.data(), K, Z.data(), NULL, NULL);
+    if (kout <= 0) continue;
+    if (kout > K) kout = K;
+    // Q R = Gbar P_k (nrw x kout);  C_new = What Q;  U_new = W P_k R^{-1}
+    std::vector<double> GP((size_t)nrw * kout, 0.0), R((size_t)kout * kout, 0.0);
+    for (int jj = 0; jj < kout; jj++)
+      for (int l = 0; l < nc; l++) {
+        double pv = Z[(size_t)jj * nc + l];
+        for (int r = 0; r < nrw; r++) GP[(size_t)jj * nrw + r] += ls.Hb[(size_t)l * ld + r] * pv;
+      }
+    int kk = 0;
+    std::vector<int> keep;
+    for (int jj = 0; jj < kout; jj++) {
+      // modified Gram-Schmidt (twice) of GP column jj against the kept ones
+      double* g = &GP[(size_t)jj * nrw];
+      double nrm0 = 0.0;
+      for (int r = 0; r < nrw; r++) nrm0 += g[r] * g[r];
+      nrm0 = sqrt(nrm0);
+      std::vector<double> rcol(kout, 0.0);
+      for (int pass = 0; pass < 2; pass++)
+        for (int q = 0; q < kk; q++) {
+          const double* gq = &GP[(size_t)keep[q] * nrw];
+          double d = 0.0;
+          for (int r = 0; r < nrw; r++) d += gq[r] * g[r];
+          for (int r = 0; r < nrw; r++) g[r] -= d * gq[r];
+          rcol[q] += d;
+        }
+      double nrm = 0.0;
+      for (int r = 0; r < nrw; r++) nrm += g[r] * g[r];
+      nrm = sqrt(nrm);
+      if (!(nrm > 1e-10 * nrm0)) continue;
+      for (int r = 0; r < nrw; r++) g[r] /= nrm;
+      rcol[kk] = nrm;
+      for (int q = 0; q <= kk; q++) R[(size_t)kk * kout + q] = rcol[q];  // column kk of R
+      keep.push_back(jj);
+      kk++;
+    }
+    if (kk == 0) continue;
+    // P R^{-1} (nc x kk): solve the upper-triangular system column by column
+    std::vector<double> PR((size_t)nc * kk, 0.0);
+    for (int jj = 0; jj < kk; jj++) {
+      for (int l = 0; l < nc; l++) {
+        double v = Z[(size_t)keep[jj] * nc + l];
+        for (int q = 0; q < jj; q++) v -= PR[(size_t)q * nc + l] * R[(size_t)jj * kout + q];
+        PR[(size_t)jj * nc + l] = v / R[(size_t)jj * kout + jj];
+      }
+    }
+    // upload: Q (nrw x kk) for C_new = [C, V] Q; PRu (kd x kk, scaled by D_u) and PRv (jend x kk)
+    std::vector<double> Qm((size_t)nrw * kk), PRu((size_t)std::max(kd, 1) * kk, 0.0),
+        PRv((size_t)jend * kk);
+    for (int jj = 0; jj < kk; jj++) {
+      for (int r = 0; r < nrw; r++) Qm[(size_t)jj * nrw + r] = GP[(size_t)keep[jj] * nrw + r];
+      for (int i = 0; i < kd; i++) PRu[(size_t)jj * kd + i] = PR[(size_t)jj * nc + i] * dscale[i];
+      for (int i = 0; i < jend; i++) PRv[(size_t)jj * jend + i] = PR[(size_t)jj * nc + kd + i];
+    }
+    double* Cn = d_T;
+    double* Un = d_T + (size_t)K * n;
+    HCHECK(hipMemcpy(dmat, Qm.data(), Qm.size() * sizeof(double), hipMemcpyHostToDevice));
+    KCHECK(tacs_hip_basis_combine64(n, nrw, Cm, n, dmat, kk, Cn, n, 0, g_stream));
+    HCHECK(hipMemcpyAsync(dmat, PRv.data(), PRv.size() * sizeof(double), hipMemcpyHostToDevice,
+                          g_stream));
+    KCHECK(tacs_hip_basis_combine64(n, jend, V0, n, dmat, kk, Un, n, 0, g_stream));
+    if (kd > 0) {
+      double* dm2 = dmat + PRv.size() + 64;
+      HCHECK(hipMemcpyAsync(dm2, PRu.data(), (size_t)kd * kk * sizeof(double),
+                            hipMemcpyHostToDevice, g_stream));
+      KCHECK(tacs_hip_basis_combine64(n, kd, U, n, dm2, kk, Un, n, 1, g_stream));
+    }
+    HCHECK(hipMemcpyAsync(Cm, Cn, (size_t)kk * n * sizeof(double), hipMemcpyDeviceToDevice,
+                          g_stream));
+    HCHECK(hipMemcpyAsync(U, Un, (size_t)kk * n * sizeof(double), hipMemcpyDeviceToDevice,
+                          g_stream));
+    HCHECK(hipStreamSynchronize(g_stream));
+    kd = kk;
+  }
+  recycleDim = kd;
+  recycleVersion = mat->version;
   HCHECK(hipStreamSynchronize(g_stream));
   return flag;
 }
@@ -1326,6 +1692,7 @@ void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScal
     HCHECK(hipEventDestroy(e0));
     HCHECK(hipEventDestroy(e1));
     A->A32_current = A->d_A32 ? 1 : 0;
+    A->version++;
     return;
   }
   if (residual) assembleRes(residual, lambda, bcs);
@@ -1341,6 +1708,7 @@ void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScal
   HCHECK(hipEventDestroy(e0));
   HCHECK(hipEventDestroy(e1));
   A->A32_current = A->d_A32 ? 1 : 0;
+  A->version++;
   int s = status();
   if (s) tacs_set_error("assembleJacobian: kernel status %d", s);
 }

@@ -271,6 +271,122 @@ __global__ void scalar_op(const double* __restrict__ src, double s, double* __re
   if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = s * src[0];
 }
 
+// out[:, j] = sum_i W[:, i] P[i, j], j < nout <= 32 (P column-major nin x nout,
+// staged in LDS as FP32): the basis change V_new = V P of a deflated restart.
+// One thread per row, 32 accumulators.
+__global__ __launch_bounds__(256) void basis_combine_kernel(size_t n, int nin,
+                                                            const float* __restrict__ W,
+                                                            size_t ldw, const double* __restrict__ P,
+                                                            int nout, float* __restrict__ out,
+                                                            size_t ldo) {
+  extern __shared__ float sP[];  // [nin][32]
+  for (int t = threadIdx.x; t < nin * 32; t += blockDim.x) {
+    int i = t / 32, j = t % 32;
+    sP[t] = (j < nout) ? (float)P[i + (size_t)nin * j] : 0.0f;
+  }
+  __syncthreads();
+  for (size_t row = (size_t)blockIdx.x * blockDim.x + threadIdx.x; row < n;
+       row += (size_t)gridDim.x * blockDim.x) {
+    float acc[32];
+#pragma unroll
+    for (int j = 0; j < 32; j++) acc[j] = 0.0f;
+    for (int i = 0; i < nin; i++) {
+      float w = W[(size_t)i * ldw + row];
+#pragma unroll
+      for (int j = 0; j < 32; j++) acc[j] += w * sP[i * 32 + j];
+    }
+#pragma unroll
+    for (int j = 0; j < 32; j++)
+      if (j < nout) out[(size_t)j * ldo + row] = acc[j];
+  }
+}
+
+// FP64 basis change: out[:, j] (+)= sum_i W[:, i] P[i, j], j < nout <= 32.
+__global__ __launch_bounds__(256) void basis_combine64_kernel(size_t n, int nin,
+                                                              const double* __restrict__ W,
+                                                              size_t ldw,
+                                                              const double* __restrict__ P,
+                                                              int nout, double* __restrict__ out,
+                                                              size_t ldo, int add) {
+  extern __shared__ double sP64[];  // [nin][32]
+  for (int t = threadIdx.x; t < nin * 32; t += blockDim.x) {
+    int i = t / 32, j = t % 32;
+    sP64[t] = (j < nout) ? P[i + (size_t)nin * j] : 0.0;
+  }
+  __syncthreads();
+  for (size_t row = (size_t)blockIdx.x * blockDim.x + threadIdx.x; row < n;
+       row += (size_t)gridDim.x * blockDim.x) {
+    double acc[32];
+#pragma unroll
+    for (int j = 0; j < 32; j++) acc[j] = 0.0;
+    for (int i = 0; i < nin; i++) {
+      double w = W[(size_t)i * ldw + row];
+#pragma unroll
+      for (int j = 0; j < 32; j++) acc[j] += w * sP64[i * 32 + j];
+    }
+#pragma unroll
+    for (int j = 0; j < 32; j++)
+      if (j < nout) {
+        size_t o = (size_t)j * ldo + row;
+        out[o] = add ? out[o] + acc[j] : acc[j];
+      }
+  }
+}
+
+// Tall-skinny product out = X^T Y (X: n x p, Y: n x q, column-major with
+// leading dimension n; p * q <= TSG_MAXPQ): each block stages 32-row tiles of
+// every column in LDS and accumulates its share of the p x q sums; the
+// partial sums are added in a second, deterministic launch.
+#define TSG_ROWS 32
+#define TSG_BLOCKS 512
+#define TSG_MAXPQ 4096
+__global__ __launch_bounds__(256) void tsgemm_partial(size_t n, int p, const double* __restrict__ X,
+                                                      int q, const double* __restrict__ Y,
+                                                      double* __restrict__ part) {
+  extern __shared__ double tile[];  // [(p + q)][TSG_ROWS]
+  const int pq = p * q;
+  double acc[TSG_MAXPQ / 256];
+#pragma unroll
+  for (int t = 0; t < TSG_MAXPQ / 256; t++) acc[t] = 0.0;
+  for (size_t r0 = (size_t)blockIdx.x * TSG_ROWS; r0 < n; r0 += (size_t)gridDim.x * TSG_ROWS) {
+    const int nr = (int)((n - r0) < TSG_ROWS ? (n - r0) : TSG_ROWS);
+    for (int t = threadIdx.x; t < (p + q) * TSG_ROWS; t += blockDim.x) {
+      int v = t / TSG_ROWS, r = t % TSG_ROWS;
+      double val = 0.0;
+      if (r < nr) val = (v < p) ? X[(size_t)v * n + r0 + r] : Y[(size_t)(v - p) * n + r0 + r];
+      tile[t] = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < TSG_MAXPQ / 256; t++) {
+      int e = threadIdx.x + 256 * t;
+      if (e < pq) {
+        int i = e % p, j = e / p;
+        const double* xi = &tile[i * TSG_ROWS];
+        const double* yj = &tile[(p + j) * TSG_ROWS];
+        double s = 0.0;
+        for (int r = 0; r < TSG_ROWS; r++) s += xi[r] * yj[r];
+        acc[t] += s;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int t = 0; t < TSG_MAXPQ / 256; t++) {
+    int e = threadIdx.x + 256 * t;
+    if (e < pq) part[(size_t)blockIdx.x * pq + e] = acc[t];
+  }
+}
+
+__global__ void tsgemm_finalize(int nb, int pq, const double* __restrict__ part,
+                                double* __restrict__ out) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pq) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; b++) s += part[(size_t)b * pq + e];
+  out[e] = s;
+}
+
 static inline int vgrid(size_t n) {
   size_t g = (n + 255) / 256;
   if (g > 8192) g = 8192;
@@ -414,6 +530,43 @@ int tacs_hip_givens(int i, int m, double* H, double* Qc, double* Qs, double* res
 
 int tacs_hip_backsolve(int niters, int m, const double* H, double* res, hipStream_t stream) {
   hipLaunchKernelGGL(backsolve_kernel, dim3(1), dim3(64), 0, stream, niters, m, H, res);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_basis_combine(size_t n, int nin, const float* W, size_t ldw, const double* P,
+                           int nout, float* out, size_t ldo, hipStream_t stream) {
+  if (n == 0 || nout == 0) return 0;
+  if (nout > 32 || nin < 1 || nin > 512) return TACS_HIP_ERR_ARG;
+  size_t lds = (size_t)nin * 32 * sizeof(float);
+  hipLaunchKernelGGL(basis_combine_kernel, dim3(vgrid(n)), dim3(256), lds, stream, n, nin, W, ldw,
+                     P, nout, out, ldo);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_basis_combine64(size_t n, int nin, const double* W, size_t ldw, const double* P,
+                             int nout, double* out, size_t ldo, int add, hipStream_t stream) {
+  if (n == 0 || nout == 0 || nin == 0) return 0;
+  if (nout > 32 || nin < 1 || nin > 256) return TACS_HIP_ERR_ARG;
+  size_t lds = (size_t)nin * 32 * sizeof(double);
+  hipLaunchKernelGGL(basis_combine64_kernel, dim3(vgrid(n)), dim3(256), lds, stream, n, nin, W,
+                     ldw, P, nout, out, ldo, add);
+  return (int)hipGetLastError();
+}
+
+size_t tacs_hip_tsgemm_scratch_bytes() { return (size_t)TSG_BLOCKS * TSG_MAXPQ * sizeof(double); }
+
+int tacs_hip_tsgemm(size_t n, int p, const double* X, int q, const double* Y, double* scratch,
+                    double* out, hipStream_t stream) {
+  if (p <= 0 || q <= 0) return 0;
+  if (p * q > TSG_MAXPQ || (size_t)(p + q) * TSG_ROWS * sizeof(double) > 65536)
+    return TACS_HIP_ERR_ARG;
+  size_t nb = (n + TSG_ROWS - 1) / TSG_ROWS;
+  if (nb > TSG_BLOCKS) nb = TSG_BLOCKS;
+  if (nb < 1) nb = 1;
+  size_t lds = (size_t)(p + q) * TSG_ROWS * sizeof(double);
+  hipLaunchKernelGGL(tsgemm_partial, dim3((int)nb), dim3(256), lds, stream, n, p, X, q, Y, scratch);
+  hipLaunchKernelGGL(tsgemm_finalize, dim3((p * q + 255) / 256), dim3(256), 0, stream, (int)nb,
+                     p * q, scratch, out);
   return (int)hipGetLastError();
 }
 

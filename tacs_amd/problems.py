@@ -82,10 +82,10 @@ class SolidProblem:
         self.assembler.setVariables(self.u)
         return flag, ksm.getIterCount(), pc, ksm
 
-    def solve_mixed(self, m=30, restarts=1000, rtol=1e-8):
+    def solve_mixed(self, m=30, restarts=1000, rtol=1e-8, deflation=0):
         pc = TACS.Pc(self.mat)
         pc.factor()
-        ksm = TACS.MixedKSM(self.mat, pc, m, restarts)
+        ksm = TACS.MixedKSM(self.mat, pc, m, restarts, deflation=deflation)
         ksm.setTolerances(rtol, 1e-30)
         flag = ksm.solve(self.res, self.u)
         self.u.scale(-1.0)

@@ -99,7 +99,7 @@ def test_golden_cube_compliance_and_ks():
     assert abs(vals[0] - 1.1074017881777185) <= 1e-9
 
 
-@pytest.mark.parametrize("solver", ["fp64", "fp64_cgs", "mixed"])
+@pytest.mark.parametrize("solver", ["fp64", "fp64_cgs", "mixed", "mixed_dr"])
 def test_solve_functions_and_adjoint(solver):
     n = 6
     p = SolidProblem(n, dv="element")
@@ -109,6 +109,8 @@ def test_solve_functions_and_adjoint(solver):
     p.assemble()
     if solver == "mixed":
         flag, its, pc, ksm = p.solve_mixed(rtol=1e-10)
+    elif solver == "mixed_dr":
+        flag, its, pc, ksm = p.solve_mixed(m=40, rtol=1e-10, deflation=12)
     else:
         ortho = TACS.CLASSICAL_GRAM_SCHMIDT if solver == "fp64_cgs" else TACS.MODIFIED_GRAM_SCHMIDT
         flag, its, pc, ksm = p.solve_fp64(m=30, rtol=1e-10, ortho=ortho)
