@@ -450,6 +450,9 @@ class TACSAssembler : public TACSObject {
                  TACSFunction** funcs, TACSBVec** dfdu);
   void addAdjointResProducts(TacsScalar scale, int numAdjoints, TACSBVec** adjoint,
                              TACSBVec** dfdx, const TacsScalar lambda = 1.0);
+  void addXptSens(TacsScalar coef, int numFuncs, TACSFunction** funcs, TACSBVec** dfdXpt);
+  void addAdjointResXptSensProducts(TacsScalar scale, int numAdjoints, TACSBVec** adjoint,
+                                    TACSBVec** dfdXpt, const TacsScalar lambda = 1.0);
 
   // Device view of the element-level state used by the kernels
   const double* devNodes() { return d_X; }

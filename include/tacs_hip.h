@@ -96,6 +96,27 @@ int tacs_hip_solid_traction_vectors(int order, int naux, const int* aux_elem,
                                     const int* conn, const double* X, double lambda,
                                     double* ev, hipStream_t stream);
 
+/* Node-coordinate sensitivities as element vectors out[nelem * 3*npe]:
+   mode 0 compliance dC/dX at fixed u (TACSAssembler::addXptSens
+          src/TACSAssembler.cpp:4858 -> TACSCompliance::getElementXptSens
+          src/functions/TACSCompliance.cpp:129), coef = scale,
+   mode 1 KS failure dKS/dX at fixed u (TACSKSFailure::getElementXptSens),
+   mode 2 scale * psi^T dR/dX (TACSAssembler::addAdjointResXptSensProducts
+          :5136 -> TACSElement3D::addAdjResXptProduct
+          src/elements/TACSElement3D.cpp:260). */
+int tacs_hip_solid_xpt_vectors(int order, int mode, int nelem, const int* conn, const double* X,
+                               const double* u, const double* psi, const double* telem,
+                               const SolidMaterial* mat, double scale, double ks_weight,
+                               double ks_max, double ks_sum, double safety, double* out,
+                               hipStream_t stream);
+/* scale * psi^T dR/dX of traction aux elements, exact derivative (the
+   reference differences TACSTraction3D's residual forward,
+   TACSElement::addAdjResXptProduct src/elements/TACSElement.cpp:270). */
+int tacs_hip_solid_traction_xpt_vectors(int order, int naux, const int* aux_elem,
+                                        const int* aux_face, const double* aux_trac,
+                                        const int* conn, const double* X, const double* psi,
+                                        double scale, double* out, hipStream_t stream);
+
 /* Node gather of element vectors in ascending element order: the reverse of
    TACSBVec::setValues ADD (src/bpmat/TACSBVec.cpp:757). nv = 24 or 81. */
 int tacs_hip_gather_elem_vectors(int nv, int nnodes, const int* n2e_ptr, const int* n2e,
@@ -351,6 +372,14 @@ int tacs_assembler_add_sv_sens(tacs_obj a, double alpha, double beta, double gam
                                tacs_obj* funcs, tacs_obj* dfdu);
 int tacs_assembler_add_adjoint_res_products(tacs_obj a, double scale, int n, tacs_obj* adjoint,
                                             tacs_obj* dfdx, double lambda);
+/* TACSAssembler::addXptSens (src/TACSAssembler.cpp:4858; TACS.pyx :2382) and
+   addAdjointResXptSensProducts (:5136; TACS.pyx :2454): node-coordinate
+   sensitivities (solid family). */
+int tacs_assembler_add_xpt_sens(tacs_obj assembler, double coef, int n, tacs_obj* funcs,
+                                tacs_obj* dfdX);
+int tacs_assembler_add_adjoint_res_xpt_sens_products(tacs_obj assembler, double scale, int n,
+                                                     tacs_obj* adjoint, tacs_obj* dfdX,
+                                                     double lambda);
 
 /* TACSBVec (TACS.pyx Vec :594; src/bpmat/TACSBVec.h:67) */
 int tacs_vec_get_size(tacs_obj v, int* size, int* bsize);

@@ -358,3 +358,44 @@ def shell_compliance_sens(conn, X, u, t_elem, E, nu):
     lib().orc_shell_compliance_sens(nelem, _p(conn, np.int32), _p(_dd(X)), _p(_dd(u)),
                                     _p(_dd(t_elem)), _c(E), _c(nu), _p(dfdu), _p(dfdt))
     return dfdu, dfdt
+
+
+# ---------------------------------------------------------------- node coordinates
+def compliance_xptsens(order, conn, X, u, t_elem, C21, scale=1.0):
+    conn = _ii(conn).reshape(-1)
+    X = _dd(X).reshape(-1)
+    out = np.zeros(X.size)
+    lib().orc_compliance_xptsens(order, conn.size // order ** 3, _p(conn, np.int32), _p(X),
+                                 _p(_dd(u)), _p(_dd(t_elem)), _p(_dd(C21)), _c(scale), _p(out))
+    return out
+
+
+def adj_res_xpt_product(order, conn, X, u, psi, t_elem, C21, scale=1.0):
+    conn = _ii(conn).reshape(-1)
+    X = _dd(X).reshape(-1)
+    out = np.zeros(X.size)
+    lib().orc_adj_res_xpt_product(order, conn.size // order ** 3, _p(conn, np.int32), _p(X),
+                                  _p(_dd(u)), _p(_dd(psi)), _p(_dd(t_elem)), _p(_dd(C21)),
+                                  _c(scale), _p(out))
+    return out
+
+
+def traction_xpt_product(order, conn, X, aux_elems, aux_faces, trac, psi, scale=1.0):
+    conn = _ii(conn).reshape(-1)
+    X = _dd(X).reshape(-1)
+    out = np.zeros(X.size)
+    ae = _ii(aux_elems)
+    lib().orc_traction_xpt_product(order, ae.size, _p(ae, np.int32), _p(_ii(aux_faces), np.int32),
+                                   _p(_dd(trac).reshape(-1)), _p(conn, np.int32), _p(X),
+                                   _p(_dd(psi)), _c(scale), _p(out))
+    return out
+
+
+def ks_xptsens(order, conn, X, u, C21, ys, ks_weight, max_fail, ks_sum, safety=1.0, scale=1.0):
+    conn = _ii(conn).reshape(-1)
+    X = _dd(X).reshape(-1)
+    out = np.zeros(X.size)
+    lib().orc_ks_xptsens(order, conn.size // order ** 3, _p(conn, np.int32), _p(X), _p(_dd(u)),
+                         _p(_dd(C21)), _c(ys), _c(ks_weight), _c(safety), _c(max_fail),
+                         _c(ks_sum), _c(scale), _p(out))
+    return out

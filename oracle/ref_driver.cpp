@@ -36,6 +36,54 @@ void ref_solid_element(int order, const double* X, const double* u, double E, do
   elem->decref();
 }
 
+static TACSElement3D* make_solid(int order, double E, double nu, double t) {
+  TACSMaterialProperties* props = new TACSMaterialProperties(2700.0, 921.0, E, nu, 270e6, 0.0, 0.0);
+  TACSSolidConstitutive* con = new TACSSolidConstitutive(props, t, 0);
+  TACSLinearElasticity3D* model = new TACSLinearElasticity3D(con, TACS_LINEAR_STRAIN);
+  TACSElementBasis* basis;
+  if (order == 2)
+    basis = new TACSLinearHexaBasis();
+  else
+    basis = new TACSQuadraticHexaBasis();
+  return new TACSElement3D(model, basis);
+}
+
+// psi^T dR/dX of one solid element (TACSElement3D::addAdjResXptProduct).
+void ref_solid_adj_xpt(int order, const double* X, const double* u, const double* psi, double E,
+                       double nu, double t, double* out) {
+  TACSElement3D* elem = make_solid(order, E, nu, t);
+  elem->incref();
+  int nv = elem->getNumVariables();
+  double* zero = new double[nv]();
+  for (int i = 0; i < nv; i++) out[i] = 0.0;
+  elem->addAdjResXptProduct(0, 0.0, 1.0, psi, X, u, zero, zero, out);
+  delete[] zero;
+  elem->decref();
+}
+
+// d/dX of the element compliance sum_q w detXd U0, through the element's
+// point-quantity interface as the compliance function drives it.
+void ref_solid_compliance_xpt(int order, const double* X, const double* u, double E, double nu,
+                              double t, double* out) {
+  TACSElement3D* elem = make_solid(order, E, nu, t);
+  elem->incref();
+  int nv = elem->getNumVariables();
+  double* zero = new double[nv]();
+  for (int i = 0; i < nv; i++) out[i] = 0.0;
+  for (int i = 0; i < elem->getNumQuadraturePoints(); i++) {
+    double pt[3];
+    double w = elem->getQuadraturePoint(i, pt);
+    TacsScalar U0 = 0.0, detXd = 0.0;
+    elem->evalPointQuantity(0, TACS_STRAIN_ENERGY_DENSITY, 0.0, i, pt, X, u, zero, zero, &detXd,
+                            &U0);
+    TacsScalar dfdq = w * detXd;
+    elem->addPointQuantityXptSens(0, TACS_STRAIN_ENERGY_DENSITY, 0.0, 1.0, i, pt, X, u, zero,
+                                  zero, w * U0, &dfdq, out);
+  }
+  delete[] zero;
+  elem->decref();
+}
+
 // Traction residual of one solid element face (TACSTraction3D, coordinate
 // components).
 void ref_solid_traction(int order, int face, const double* X, const double* trac, double* res) {

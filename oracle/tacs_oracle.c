@@ -955,3 +955,231 @@ void orc_ks_svsens(int order, int nelem, const int *conn, const double *X,
     }
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* Node-coordinate sensitivities                                       */
+
+/* 3x3 stress tensor of Voigt s (xx, yy, zz, yz, xz, xy) */
+static void sig_tensor(const double s[6], double S[9]) {
+  S[0] = s[0]; S[4] = s[1]; S[8] = s[2];
+  S[5] = S[7] = s[3];
+  S[2] = S[6] = s[4];
+  S[1] = S[3] = s[5];
+}
+
+/* dfdX_a += f * (W I - sum_p Ux_p^T P_p) grad_x N_a for one quadrature point
+   (the derivative of w detXd W(Ux) with Ux = Ud Xd^{-1} w.r.t. the nodes) */
+static void add_xpt_term(int npe, const double *Nx, double f, double W,
+                         int np, const double *Ux[], const double *P[],
+                         const int *c, double *dfdX) {
+  double M[9];
+  for (int i = 0; i < 3; i++) {
+    for (int l = 0; l < 3; l++) {
+      double v = (i == l) ? W : 0.0;
+      for (int p = 0; p < np; p++) {
+        for (int k = 0; k < 3; k++) v -= Ux[p][3 * k + i] * P[p][3 * k + l];
+      }
+      M[3 * i + l] = v;
+    }
+  }
+  for (int a = 0; a < npe; a++) {
+    for (int i = 0; i < 3; i++) {
+      double v = 0.0;
+      for (int l = 0; l < 3; l++) v += M[3 * i + l] * Nx[3 * a + l];
+      dfdX[3 * c[a] + i] += f * v;
+    }
+  }
+}
+
+/*
+  Partial derivative of the compliance w.r.t. the node coordinates at fixed
+  u: TACSAssembler::addXptSens (src/TACSAssembler.cpp:4858) ->
+  TACSCompliance::getElementXptSens (src/functions/TACSCompliance.cpp:129:
+  dfdq = w detXd, dfddetXd = w U0) -> TACSElement3D::addPointQuantityXptSens
+  (src/elements/TACSElement3D.cpp:618).  Per quadrature point the derivative
+  of w detXd W, W = e.(tCe), is w detXd (W I - Ux^T 2 sigma) grad_x N_a.
+*/
+void orc_compliance_xptsens(int order, int nelem, const int *conn,
+                            const double *X, const double *u,
+                            const double *t_elem, const double C21[21],
+                            double scale, double *dfdX) {
+  const int npe = order * order * order;
+  double D[36], Xe[3 * MAXN], ue[3 * MAXN];
+  unpack21(C21, D);
+  for (int el = 0; el < nelem; el++) {
+    const int *c = &conn[npe * el];
+    elem_gather(npe, c, X, u, Xe, ue);
+    for (int n = 0; n < npe; n++) {
+      double pt[3], N[MAXN], Nxi[3 * MAXN], Nx[3 * MAXN], J[9], Ux[9], e[6], s[6], S[9];
+      double w = orc_hex_quadrature(order, n, pt);
+      orc_hex_basis(order, pt, N, Nxi);
+      double det = qp_geometry(npe, Nxi, Xe, Nx, J);
+      grad_field(npe, Nx, ue, Ux);
+      strain(Ux, e);
+      matvec6(D, e, s);
+      for (int i = 0; i < 6; i++) s[i] *= t_elem[el];
+      double W = 0.0;
+      for (int i = 0; i < 6; i++) W += e[i] * s[i];
+      sig_tensor(s, S);
+      for (int i = 0; i < 9; i++) S[i] *= 2.0;
+      const double *Ups[1] = {Ux};
+      const double *Ps[1] = {S};
+      add_xpt_term(npe, Nx, scale * w * det, W, 1, Ups, Ps, c, dfdX);
+    }
+  }
+}
+
+/*
+  scale * psi^T dR/dX of the element residual: TACSAssembler::
+  addAdjointResXptSensProducts (src/TACSAssembler.cpp:5136) ->
+  TACSElement3D::addAdjResXptProduct (src/elements/TACSElement3D.cpp:260).
+  psi^T R_e = sum_q w detXd e(psi).(t C e(u)); its node derivative per
+  quadrature point is w detXd (g I - Ux_u^T sigma_psi - Ux_psi^T sigma_u)
+  grad_x N_a with g = e(psi).sigma_u.
+*/
+void orc_adj_res_xpt_product(int order, int nelem, const int *conn,
+                             const double *X, const double *u,
+                             const double *psi, const double *t_elem,
+                             const double C21[21], double scale, double *dfdX) {
+  const int npe = order * order * order;
+  double D[36], Xe[3 * MAXN], ue[3 * MAXN], pe[3 * MAXN];
+  unpack21(C21, D);
+  for (int el = 0; el < nelem; el++) {
+    const int *c = &conn[npe * el];
+    elem_gather(npe, c, X, u, Xe, ue);
+    elem_gather(npe, c, X, psi, Xe, pe);
+    for (int n = 0; n < npe; n++) {
+      double pt[3], N[MAXN], Nxi[3 * MAXN], Nx[3 * MAXN], J[9];
+      double Ux[9], Px[9], eu[6], ep[6], su[6], sp[6], Su[9], Sp[9];
+      double w = orc_hex_quadrature(order, n, pt);
+      orc_hex_basis(order, pt, N, Nxi);
+      double det = qp_geometry(npe, Nxi, Xe, Nx, J);
+      grad_field(npe, Nx, ue, Ux);
+      grad_field(npe, Nx, pe, Px);
+      strain(Ux, eu);
+      strain(Px, ep);
+      matvec6(D, eu, su);
+      matvec6(D, ep, sp);
+      for (int i = 0; i < 6; i++) {
+        su[i] *= t_elem[el];
+        sp[i] *= t_elem[el];
+      }
+      double g = 0.0;
+      for (int i = 0; i < 6; i++) g += ep[i] * su[i];
+      sig_tensor(su, Su);
+      sig_tensor(sp, Sp);
+      const double *Ups[2] = {Ux, Px};
+      const double *Ps[2] = {Sp, Su};
+      add_xpt_term(npe, Nx, scale * w * det, g, 2, Ups, Ps, c, dfdX);
+    }
+  }
+}
+
+/*
+  scale * psi^T dR/dX of coordinate-component traction aux elements.  The
+  residual is -sum_q w A N_a t with A = |t1 x t2| (TACSTraction3D::addResidual
+  src/elements/TACSTraction3D.cpp:87, TACSElementBasis::getFaceNormal
+  basis/TACSElementBasis.cpp:37).  The reference has no analytic version (it
+  differences the residual forward with step 1e-7, TACSElement::
+  addAdjResXptProduct src/elements/TACSElement.cpp:270); this is the exact
+  derivative: dA/dX_a = (t2 x n)(grad N_a . tg1) + (n x t1)(grad N_a . tg2).
+*/
+void orc_traction_xpt_product(int order, int naux, const int *aux_elem,
+                              const int *aux_face, const double *aux_trac,
+                              const int *conn, const double *X,
+                              const double *psi, double scale, double *dfdX) {
+  const int npe = order * order * order;
+  const int nfq = order * order;
+  for (int q = 0; q < naux; q++) {
+    const int *c = &conn[npe * aux_elem[q]];
+    const double *tr = &aux_trac[3 * q];
+    double Xe[3 * MAXN], pe[3 * MAXN];
+    elem_gather(npe, c, X, psi, Xe, pe);
+    for (int n = 0; n < nfq; n++) {
+      double pt[3], tg[6], N[MAXN], Nxi[3 * MAXN], Xd[9] = {0};
+      double w = orc_hex_face_quadrature(order, aux_face[q], n, pt, tg);
+      orc_hex_basis(order, pt, N, Nxi);
+      for (int a = 0; a < npe; a++) {
+        for (int i = 0; i < 3; i++) {
+          for (int k = 0; k < 3; k++) Xd[3 * i + k] += Xe[3 * a + i] * Nxi[3 * a + k];
+        }
+      }
+      double t1[3], t2[3], nv[3];
+      for (int i = 0; i < 3; i++) {
+        t1[i] = Xd[3 * i] * tg[0] + Xd[3 * i + 1] * tg[1] + Xd[3 * i + 2] * tg[2];
+        t2[i] = Xd[3 * i] * tg[3] + Xd[3 * i + 1] * tg[4] + Xd[3 * i + 2] * tg[5];
+      }
+      nv[0] = t1[1] * t2[2] - t1[2] * t2[1];
+      nv[1] = t1[2] * t2[0] - t1[0] * t2[2];
+      nv[2] = t1[0] * t2[1] - t1[1] * t2[0];
+      double A = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+      double nh[3] = {nv[0] / A, nv[1] / A, nv[2] / A};
+      double a1[3] = {t2[1] * nh[2] - t2[2] * nh[1], t2[2] * nh[0] - t2[0] * nh[2],
+                      t2[0] * nh[1] - t2[1] * nh[0]};
+      double a2[3] = {nh[1] * t1[2] - nh[2] * t1[1], nh[2] * t1[0] - nh[0] * t1[2],
+                      nh[0] * t1[1] - nh[1] * t1[0]};
+      /* psi^T R = -w A P, P = sum_a N_a psi_a . t */
+      double P = 0.0;
+      for (int a = 0; a < npe; a++) {
+        P += N[a] * (pe[3 * a] * tr[0] + pe[3 * a + 1] * tr[1] + pe[3 * a + 2] * tr[2]);
+      }
+      for (int a = 0; a < npe; a++) {
+        double g1 = Nxi[3 * a] * tg[0] + Nxi[3 * a + 1] * tg[1] + Nxi[3 * a + 2] * tg[2];
+        double g2 = Nxi[3 * a] * tg[3] + Nxi[3 * a + 1] * tg[4] + Nxi[3 * a + 2] * tg[5];
+        for (int i = 0; i < 3; i++) {
+          dfdX[3 * c[a] + i] += scale * (-w * P) * (a1[i] * g1 + a2[i] * g2);
+        }
+      }
+    }
+  }
+}
+
+/*
+  Partial derivative of the continuous KS failure w.r.t. the node coordinates
+  at fixed u (TACSKSFailure::getElementXptSens src/functions/TACSKSFailure.cpp
+  -> TACSElement3D::addPointQuantityXptSens :618): with
+  Phi_q = exp(rho (f_q - max)) / (rho S), KS = max + log(sum_q w detXd
+  exp(rho (f_q - max))) / rho has node derivative sum_q w detXd (Phi I -
+  Ux^T rho Phi df/dUx) grad_x N_a; f = safety * von Mises / ys is independent
+  of the thickness for solids.
+*/
+void orc_ks_xptsens(int order, int nelem, const int *conn, const double *X,
+                    const double *u, const double C21[21], double ys,
+                    double ksWeight, double safety, double maxFail,
+                    double ksSum, double scale, double *dfdX) {
+  const int npe = order * order * order;
+  double D[36], Xe[3 * MAXN], ue[3 * MAXN];
+  unpack21(C21, D);
+  for (int el = 0; el < nelem; el++) {
+    const int *c = &conn[npe * el];
+    elem_gather(npe, c, X, u, Xe, ue);
+    for (int n = 0; n < npe; n++) {
+      double pt[3], N[MAXN], Nxi[3 * MAXN], Nx[3 * MAXN], J[9], Ux[9], e[6], s[6];
+      double w = orc_hex_quadrature(order, n, pt);
+      orc_hex_basis(order, pt, N, Nxi);
+      double det = qp_geometry(npe, Nxi, Xe, Nx, J);
+      grad_field(npe, Nx, ue, Ux);
+      strain(Ux, e);
+      matvec6(D, e, s);
+      double vm = von_mises(s);
+      double f = safety * vm / ys;
+      double Phi = exp(ksWeight * (f - maxFail)) / (ksWeight * ksSum);
+      double ds[6] = {0}, de[6], P[9];
+      if (vm != 0.0) {
+        double fact = 0.5 / (ys * vm);
+        ds[0] = fact * (2.0 * s[0] - s[1] - s[2]);
+        ds[1] = fact * (2.0 * s[1] - s[0] - s[2]);
+        ds[2] = fact * (2.0 * s[2] - s[0] - s[1]);
+        ds[3] = 6.0 * fact * s[3];
+        ds[4] = 6.0 * fact * s[4];
+        ds[5] = 6.0 * fact * s[5];
+      }
+      matvec6(D, ds, de);
+      sig_tensor(de, P);
+      for (int i = 0; i < 9; i++) P[i] *= ksWeight * Phi * safety;
+      const double *Ups[1] = {Ux};
+      const double *Ps[1] = {P};
+      add_xpt_term(npe, Nx, scale * w * det, Phi, 1, Ups, Ps, c, dfdX);
+    }
+  }
+}

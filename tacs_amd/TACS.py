@@ -330,3 +330,13 @@ class Assembler(TACSHandle):
         check(lib().tacs_assembler_add_adjoint_res_products(
             self.ptr, alpha, len(adjlist), _ptr_array(adjlist), _ptr_array(dfdxlist), loadScale),
             "addAdjointResProducts")
+
+    def addXptSens(self, funclist, dfdXlist, alpha=1.0):
+        check(lib().tacs_assembler_add_xpt_sens(self.ptr, alpha, len(funclist),
+                                                _ptr_array(funclist), _ptr_array(dfdXlist)),
+              "addXptSens")
+
+    def addAdjointResXptSensProducts(self, adjlist, dfdXlist, alpha=1.0, loadScale=1.0):
+        check(lib().tacs_assembler_add_adjoint_res_xpt_sens_products(
+            self.ptr, alpha, len(adjlist), _ptr_array(adjlist), _ptr_array(dfdXlist), loadScale),
+            "addAdjointResXptSensProducts")

@@ -3,6 +3,8 @@
 // reference's Cython layer (tacs/cpp_headers/TACS.pxd, tacs/*.pyx).
 #include <stdio.h>
 
+#include <vector>
+
 #include "../../include/TACSHip.h"
 
 int tacs_set_error(const char* fmt, ...);
@@ -257,6 +259,35 @@ int tacs_assembler_add_adjoint_res_products(tacs_obj h, double scale, int n, tac
   }
   clear_error();
   a->addAdjointResProducts(scale, n, ad, d, lambda);
+  return last_error_code();
+}
+
+int tacs_assembler_add_xpt_sens(tacs_obj h, double coef, int n, tacs_obj* funcs,
+                                tacs_obj* dfdX) {
+  clear_error();
+  REQUIRE(TACSAssembler, a, h);
+  std::vector<TACSFunction*> f(n);
+  std::vector<TACSBVec*> v(n);
+  for (int k = 0; k < n; k++) {
+    f[k] = OBJ(TACSFunction, funcs[k]);
+    v[k] = OBJ(TACSBVec, dfdX[k]);
+    if (!f[k] || !v[k]) return tacs_set_error("addXptSens: bad handle %d", k);
+  }
+  a->addXptSens(coef, n, f.data(), v.data());
+  return last_error_code();
+}
+int tacs_assembler_add_adjoint_res_xpt_sens_products(tacs_obj h, double scale, int n,
+                                                     tacs_obj* adjoint, tacs_obj* dfdX,
+                                                     double lambda) {
+  clear_error();
+  REQUIRE(TACSAssembler, a, h);
+  std::vector<TACSBVec*> psi(n), v(n);
+  for (int k = 0; k < n; k++) {
+    psi[k] = OBJ(TACSBVec, adjoint[k]);
+    v[k] = OBJ(TACSBVec, dfdX[k]);
+    if (!psi[k] || !v[k]) return tacs_set_error("addAdjointResXptSensProducts: bad handle %d", k);
+  }
+  a->addAdjointResXptSensProducts(scale, n, psi.data(), v.data(), lambda);
   return last_error_code();
 }
 

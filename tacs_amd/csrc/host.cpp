@@ -1822,3 +1822,58 @@ void TACSAssembler::addAdjointResProducts(TacsScalar scale, int numAdjoints, TAC
     KCHECK(scatterElementScalarsToDVs(d_es, dfdx[k]->devWrite()));
   }
 }
+
+// TACSAssembler::addXptSens src/TACSAssembler.cpp:4858: partial derivatives
+// of the functions w.r.t. the node coordinates at fixed state (solid family).
+void TACSAssembler::addXptSens(TacsScalar coef, int numFuncs, TACSFunction** funcs,
+                               TACSBVec** dfdXpt) {
+  if (family != FAMILY_SOLID) {
+    tacs_set_error("addXptSens: only the solid element family is supported");
+    return;
+  }
+  ensure(d_ev, (size_t)numElements * 3 * npe);
+  for (int k = 0; k < numFuncs; k++) {
+    if (!funcs[k]) continue;
+    if (dynamic_cast<TACSCompliance*>(funcs[k])) {
+      KCHECK(tacs_hip_solid_xpt_vectors(order, 0, numElements, d_conn, d_X, d_u, NULL, d_telem,
+                                        &mat, coef, 0, 0, 1, 1, d_ev, g_stream));
+    } else if (TACSKSFailure* f = dynamic_cast<TACSKSFailure*>(funcs[k])) {
+      KCHECK(tacs_hip_solid_xpt_vectors(order, 1, numElements, d_conn, d_X, d_u, NULL, d_telem,
+                                        &mat, coef, f->ksWeight, f->maxFail, f->ksFailSum,
+                                        f->safetyFactor, d_ev, g_stream));
+    } else {
+      tacs_set_error("addXptSens: unsupported function type");
+      continue;
+    }
+    KCHECK(tacs_hip_gather_elem_vectors(3 * npe, numNodes, d_n2e_ptr, d_n2e, d_ev, 1,
+                                        dfdXpt[k]->devWrite(), g_stream));
+  }
+}
+
+// TACSAssembler::addAdjointResXptSensProducts src/TACSAssembler.cpp:5136:
+// dfdXpt += scale psi^T dR/dXpt, element residuals plus aux elements (whose
+// contribution is scaled by lambda, as there).
+void TACSAssembler::addAdjointResXptSensProducts(TacsScalar scale, int numAdjoints,
+                                                 TACSBVec** adjoint, TACSBVec** dfdXpt,
+                                                 const TacsScalar lambda) {
+  if (family != FAMILY_SOLID) {
+    tacs_set_error("addAdjointResXptSensProducts: only the solid element family is supported");
+    return;
+  }
+  ensure(d_ev, (size_t)numElements * 3 * npe);
+  for (int k = 0; k < numAdjoints; k++) {
+    const double* psi = adjoint[k]->devRead();
+    double* out = dfdXpt[k]->devWrite();
+    KCHECK(tacs_hip_solid_xpt_vectors(order, 2, numElements, d_conn, d_X, d_u, psi, d_telem, &mat,
+                                      scale, 0, 0, 1, 1, d_ev, g_stream));
+    KCHECK(tacs_hip_gather_elem_vectors(3 * npe, numNodes, d_n2e_ptr, d_n2e, d_ev, 1, out,
+                                        g_stream));
+    if (uploadAux() == 0 && naux > 0) {
+      KCHECK(tacs_hip_solid_traction_xpt_vectors(order, naux, d_aux_elem, d_aux_face, d_aux_trac,
+                                                 d_conn, d_X, psi, lambda * scale, d_aux_ev,
+                                                 g_stream));
+      KCHECK(tacs_hip_gather_elem_vectors(3 * npe, numNodes, d_n2a_ptr, d_n2a, d_aux_ev, 1, out,
+                                          g_stream));
+    }
+  }
+}

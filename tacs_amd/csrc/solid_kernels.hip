@@ -419,6 +419,229 @@ __global__ __launch_bounds__(128) void solid_elem_vectors(
   for (int k = 0; k < NV; k++) o[k] = r[k];
 }
 
+// ---------------------------------------------------------------------------
+// Node-coordinate sensitivities (one thread per element).  For an integrand
+// w detXd W(Ux), Ux = Ud Xd^{-1}, the node derivative is
+// w detXd (W I - sum_p Ux_p^T dW/dUx_p) grad_x N_a:
+//   XV_COMPLIANCE  W = e.(tCe), dW/dUx = 2 sigma  (TACSCompliance::
+//                  getElementXptSens src/functions/TACSCompliance.cpp:129 ->
+//                  TACSElement3D::addPointQuantityXptSens TACSElement3D.cpp:618)
+//   XV_KS          W = exp(rho (f - max)) / (rho S), f = safety vm / ys
+//                  (TACSKSFailure::getElementXptSens)
+//   XV_ADJ         W = e(psi).sigma(u), dW/dUx_u = sigma(psi), dW/dUx_psi =
+//                  sigma(u) (TACSElement3D::addAdjResXptProduct :260)
+// ---------------------------------------------------------------------------
+enum { XV_COMPLIANCE = 0, XV_KS = 1, XV_ADJ = 2 };
+
+__device__ __forceinline__ void sig3(const double s[6], double S[9]) {
+  S[0] = s[0];
+  S[4] = s[1];
+  S[8] = s[2];
+  S[5] = S[7] = s[3];
+  S[2] = S[6] = s[4];
+  S[1] = S[3] = s[5];
+}
+
+template <int ORD, int MODE>
+__global__ __launch_bounds__(128) void solid_xpt_vectors(
+    int nelem, const int* __restrict__ conn, const double* __restrict__ X,
+    const double* __restrict__ u, const double* __restrict__ psi,
+    const double* __restrict__ telem, SolidMaterial mat, double scale, double ks_weight,
+    double ks_max, double ks_sum, double safety, double* __restrict__ out) {
+  constexpr int NPE = ORD * ORD * ORD, NQP = NPE, NV = 3 * NPE;
+  int el = blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= nelem) return;
+  const int* c = &conn[(size_t)el * NPE];
+  double r[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) r[k] = 0.0;
+  const double t = telem ? telem[el] : 1.0;
+  for (int q = 0; q < NQP; q++) {
+    QP1D<ORD> qp;
+    qp.eval(q);
+    double J[9];
+    double det = elem_geometry<ORD>(qp, c, X, J);
+    double Ux[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Px[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < NPE; b++) {
+      double N, g0[3], g[3];
+      qp.basis(b, N, g0);
+      grad_x(J, g0, g);
+      int nb = c[b];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        double ub = u[3 * nb + j];
+        Ux[3 * j] += ub * g[0];
+        Ux[3 * j + 1] += ub * g[1];
+        Ux[3 * j + 2] += ub * g[2];
+        if (MODE == XV_ADJ) {
+          double pb = psi[3 * nb + j];
+          Px[3 * j] += pb * g[0];
+          Px[3 * j + 1] += pb * g[1];
+          Px[3 * j + 2] += pb * g[2];
+        }
+      }
+    }
+    double e[6] = {Ux[0], Ux[4], Ux[8], Ux[5] + Ux[7], Ux[2] + Ux[6], Ux[1] + Ux[3]};
+    double s[6];
+    stress6(mat, e, s);
+    double W, M[9];
+    if (MODE == XV_COMPLIANCE) {
+      W = 0.0;
+#pragma unroll
+      for (int i = 0; i < 6; i++) {
+        s[i] *= t;
+        W += e[i] * s[i];
+      }
+      double S[9];
+      sig3(s, S);
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+          double v = (i == l) ? W : 0.0;
+#pragma unroll
+          for (int k = 0; k < 3; k++) v -= 2.0 * Ux[3 * k + i] * S[3 * k + l];
+          M[3 * i + l] = v;
+        }
+    } else if (MODE == XV_KS) {
+      double vm = sqrt(0.5 * ((s[0] - s[1]) * (s[0] - s[1]) + (s[0] - s[2]) * (s[0] - s[2]) +
+                              (s[1] - s[2]) * (s[1] - s[2]) +
+                              6.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5])));
+      double fail = safety * vm / mat.ys;
+      W = exp(ks_weight * (fail - ks_max)) / (ks_weight * ks_sum);
+      double ds[6] = {0, 0, 0, 0, 0, 0}, de[6], S[9];
+      if (vm != 0.0) {
+        double fact = 0.5 / (mat.ys * vm);
+        ds[0] = fact * (2.0 * s[0] - s[1] - s[2]);
+        ds[1] = fact * (2.0 * s[1] - s[0] - s[2]);
+        ds[2] = fact * (2.0 * s[2] - s[0] - s[1]);
+        ds[3] = 6.0 * fact * s[3];
+        ds[4] = 6.0 * fact * s[4];
+        ds[5] = 6.0 * fact * s[5];
+      }
+      stress6(mat, ds, de);
+      sig3(de, S);
+      const double f = ks_weight * W * safety;
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+          double v = (i == l) ? W : 0.0;
+#pragma unroll
+          for (int k = 0; k < 3; k++) v -= f * Ux[3 * k + i] * S[3 * k + l];
+          M[3 * i + l] = v;
+        }
+    } else {
+      double ep[6] = {Px[0], Px[4], Px[8], Px[5] + Px[7], Px[2] + Px[6], Px[1] + Px[3]};
+      double sp[6];
+      stress6(mat, ep, sp);
+      W = 0.0;
+#pragma unroll
+      for (int i = 0; i < 6; i++) {
+        s[i] *= t;
+        sp[i] *= t;
+        W += ep[i] * s[i];
+      }
+      double Su[9], Sp[9];
+      sig3(s, Su);
+      sig3(sp, Sp);
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int l = 0; l < 3; l++) {
+          double v = (i == l) ? W : 0.0;
+#pragma unroll
+          for (int k = 0; k < 3; k++) v -= Ux[3 * k + i] * Sp[3 * k + l] + Px[3 * k + i] * Su[3 * k + l];
+          M[3 * i + l] = v;
+        }
+    }
+    const double f = scale * qp.w * det;
+#pragma unroll
+    for (int a = 0; a < NPE; a++) {
+      double N, g0[3], g[3];
+      qp.basis(a, N, g0);
+      grad_x(J, g0, g);
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+        r[3 * a + i] += f * (M[3 * i] * g[0] + M[3 * i + 1] * g[1] + M[3 * i + 2] * g[2]);
+    }
+  }
+  double* o = &out[(size_t)el * NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) o[k] = r[k];
+}
+
+// scale * psi^T dR/dX of coordinate-component traction aux elements: the
+// exact derivative of -sum_q w A N_a t, A = |t1 x t2|:
+// dA/dX_a = (t2 x n)(grad N_a . tg1) + (n x t1)(grad N_a . tg2).
+template <int ORD>
+__global__ void solid_traction_xpt_vectors(int naux, const int* __restrict__ aux_elem,
+                                           const int* __restrict__ aux_face,
+                                           const double* __restrict__ aux_trac,
+                                           const int* __restrict__ conn,
+                                           const double* __restrict__ X,
+                                           const double* __restrict__ psi, double scale,
+                                           double* __restrict__ out) {
+  constexpr int NPE = ORD * ORD * ORD, NV = 3 * NPE;
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= naux) return;
+  const int* c = &conn[(size_t)aux_elem[q] * NPE];
+  const int face = aux_face[q];
+  const double tr[3] = {aux_trac[3 * q], aux_trac[3 * q + 1], aux_trac[3 * q + 2]};
+  double* o = &out[(size_t)q * NV];
+  for (int k = 0; k < NV; k++) o[k] = 0.0;
+  const double tang[6][6] = {{0, 0, 1, 0, 1, 0}, {0, 1, 0, 0, 0, 1}, {1, 0, 0, 0, 0, 1},
+                             {0, 0, 1, 1, 0, 0}, {0, 1, 0, 1, 0, 0}, {1, 0, 0, 0, 1, 0}};
+  for (int n = 0; n < ORD * ORD; n++) {
+    int ia = n % ORD, ib = n / ORD;
+    double fixed = -1.0 + 2.0 * (face % 2);
+    double pt[3];
+    if (face / 2 == 0) {
+      pt[0] = fixed; pt[1] = gauss_pt<ORD>(ia); pt[2] = gauss_pt<ORD>(ib);
+    } else if (face / 2 == 1) {
+      pt[0] = gauss_pt<ORD>(ia); pt[1] = fixed; pt[2] = gauss_pt<ORD>(ib);
+    } else {
+      pt[0] = gauss_pt<ORD>(ia); pt[1] = gauss_pt<ORD>(ib); pt[2] = fixed;
+    }
+    double w = gauss_wt<ORD>(ia) * gauss_wt<ORD>(ib);
+    QP1D<ORD> qp;
+    lag1d<ORD>(pt[0], qp.n[0], qp.d[0]);
+    lag1d<ORD>(pt[1], qp.n[1], qp.d[1]);
+    lag1d<ORD>(pt[2], qp.n[2], qp.d[2]);
+    double Xd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, P = 0.0;
+    for (int b = 0; b < NPE; b++) {
+      double N, g[3];
+      qp.basis(b, N, g);
+      for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 3; k++) Xd[3 * i + k] += X[3 * c[b] + i] * g[k];
+      P += N * (psi[3 * c[b]] * tr[0] + psi[3 * c[b] + 1] * tr[1] + psi[3 * c[b] + 2] * tr[2]);
+    }
+    const double* tg = tang[face];
+    double t1[3], t2[3];
+    for (int i = 0; i < 3; i++) {
+      t1[i] = Xd[3 * i] * tg[0] + Xd[3 * i + 1] * tg[1] + Xd[3 * i + 2] * tg[2];
+      t2[i] = Xd[3 * i] * tg[3] + Xd[3 * i + 1] * tg[4] + Xd[3 * i + 2] * tg[5];
+    }
+    double nv[3] = {t1[1] * t2[2] - t1[2] * t2[1], t1[2] * t2[0] - t1[0] * t2[2],
+                    t1[0] * t2[1] - t1[1] * t2[0]};
+    double A = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    double nh[3] = {nv[0] / A, nv[1] / A, nv[2] / A};
+    double a1[3] = {t2[1] * nh[2] - t2[2] * nh[1], t2[2] * nh[0] - t2[0] * nh[2],
+                    t2[0] * nh[1] - t2[1] * nh[0]};
+    double a2[3] = {nh[1] * t1[2] - nh[2] * t1[1], nh[2] * t1[0] - nh[0] * t1[2],
+                    nh[0] * t1[1] - nh[1] * t1[0]};
+    const double f = -scale * w * P;
+    for (int a = 0; a < NPE; a++) {
+      double N, g[3];
+      qp.basis(a, N, g);
+      double g1 = g[0] * tg[0] + g[1] * tg[1] + g[2] * tg[2];
+      double g2 = g[0] * tg[3] + g[1] * tg[4] + g[2] * tg[5];
+      for (int i = 0; i < 3; i++) o[3 * a + i] += f * (a1[i] * g1 + a2[i] * g2);
+    }
+  }
+}
+
 // Traction aux elements (TACSTraction3D::addResidual, TACSTraction3D.cpp:87):
 // one thread per (aux element); element vector out[aux * NV].
 template <int ORD>
@@ -680,6 +903,48 @@ int tacs_hip_solid_traction_vectors(int order, int naux, const int* aux_elem,
   else if (order == 3)
     hipLaunchKernelGGL(solid_traction_vectors<3>, dim3(grid), dim3(64), 0, stream, naux,
                        aux_elem, aux_face, aux_trac, conn, X, lambda, ev);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_solid_xpt_vectors(int order, int mode, int nelem, const int* conn, const double* X,
+                               const double* u, const double* psi, const double* telem,
+                               const SolidMaterial* mat, double scale, double ks_weight,
+                               double ks_max, double ks_sum, double safety, double* out,
+                               hipStream_t stream) {
+  if (nelem <= 0) return 0;
+  if ((order != 2 && order != 3) || mode < 0 || mode > 2 || (mode == 2 && !psi))
+    return TACS_HIP_ERR_ARG;
+  dim3 g((nelem + 127) / 128), tb(128);
+#define XV_LAUNCH(O, M)                                                                          \
+  hipLaunchKernelGGL((solid_xpt_vectors<O, M>), g, tb, 0, stream, nelem, conn, X, u, psi, telem, \
+                     *mat, scale, ks_weight, ks_max, ks_sum, safety, out)
+  if (order == 2) {
+    if (mode == 0) XV_LAUNCH(2, XV_COMPLIANCE);
+    else if (mode == 1) XV_LAUNCH(2, XV_KS);
+    else XV_LAUNCH(2, XV_ADJ);
+  } else {
+    if (mode == 0) XV_LAUNCH(3, XV_COMPLIANCE);
+    else if (mode == 1) XV_LAUNCH(3, XV_KS);
+    else XV_LAUNCH(3, XV_ADJ);
+  }
+#undef XV_LAUNCH
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_solid_traction_xpt_vectors(int order, int naux, const int* aux_elem,
+                                        const int* aux_face, const double* aux_trac,
+                                        const int* conn, const double* X, const double* psi,
+                                        double scale, double* out, hipStream_t stream) {
+  if (naux <= 0) return 0;
+  dim3 g((naux + 127) / 128), tb(128);
+  if (order == 2)
+    hipLaunchKernelGGL(solid_traction_xpt_vectors<2>, g, tb, 0, stream, naux, aux_elem, aux_face,
+                       aux_trac, conn, X, psi, scale, out);
+  else if (order == 3)
+    hipLaunchKernelGGL(solid_traction_xpt_vectors<3>, g, tb, 0, stream, naux, aux_elem, aux_face,
+                       aux_trac, conn, X, psi, scale, out);
   else
     return TACS_HIP_ERR_ARG;
   return (int)hipGetLastError();

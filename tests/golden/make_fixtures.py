@@ -53,6 +53,20 @@ def make(seed=2024):
             for face in range(6):
                 L.ref_solid_traction(order, face, _d(X), _d(trac), _d(tr[face]))
             out[key + "_trac"], out[key + "_trac_res"] = trac, tr
+    # node-coordinate sensitivities of the solid elements (separate stream so
+    # the values above do not move)
+    rng2 = np.random.default_rng(seed + 1)
+    for order in (2, 3):
+        npe = order ** 3
+        for s in range(2):
+            key = f"hex{npe}_{s}"
+            X, u, t = out[key + "_X"], out[key + "_u"], float(out[key + "_t"][0])
+            psi = rng2.standard_normal(3 * npe)
+            adj = np.zeros(3 * npe)
+            L.ref_solid_adj_xpt(order, _d(X), _d(u), _d(psi), c(70e9), c(0.3), c(t), _d(adj))
+            cx = np.zeros(3 * npe)
+            L.ref_solid_compliance_xpt(order, _d(X), _d(u), c(70e9), c(0.3), c(t), _d(cx))
+            out[key + "_psi"], out[key + "_adjxpt"], out[key + "_compxpt"] = psi, adj, cx
     for s in range(4):
         X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], float) * [0.3, 0.2, 1.0]
         X = (X + 0.02 * rng.standard_normal((4, 3))).reshape(-1)

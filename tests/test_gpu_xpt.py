@@ -1,0 +1,102 @@
+"""GPU parity of the node-coordinate sensitivities (addXptSens,
+addAdjointResXptSensProducts) against the oracle, and the total compliance
+gradient w.r.t. the nodes through the adjoint."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tacs_amd import TACS, functions
+from tacs_amd.problems import SolidProblem
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def _distorted_problem(n, order, seed):
+    p = SolidProblem(n, order=order, dv="element")
+    rng = np.random.default_rng(seed)
+    X = p.mesh.X + 0.02 / n * rng.standard_normal(p.mesh.X.shape)
+    p.X.getArray()[:] = X.reshape(-1)
+    p.assembler.setNodes(p.X)
+    t = rng.uniform(0.3, 1.0, p.mesh.nelem)
+    p.set_thickness(t)
+    return p, X.reshape(-1), t, rng
+
+
+@pytest.mark.parametrize("n,order", [(3, 2), (2, 3)])
+def test_xpt_sens_match_oracle(n, order):
+    p, X, t, rng = _distorted_problem(n, order, 5)
+    a, m = p.assembler, p.mesh
+    C = O.iso_C21(70e9, 0.3)
+    u = 1e-4 * rng.standard_normal(3 * m.nnodes)
+    psi = rng.standard_normal(3 * m.nnodes)
+    uv, pv = a.createVec(), a.createVec()
+    uv.getArray()[:] = u
+    pv.getArray()[:] = psi
+    a.setVariables(uv)
+    comp = functions.Compliance(a)
+    ksf = functions.KSFailure(a, ksWeight=30.0)
+    a.evalFunctions([comp, ksf])
+    dX = [a.createNodeVec(), a.createNodeVec()]
+    a.addXptSens([comp, ksf], dX)
+    oc = O.compliance_xptsens(order, m.conn, X, u, t, C)
+    oks = O.ks_failure(order, m.conn, X, u, C, 270e6, 30.0)
+    ok = O.ks_xptsens(order, m.conn, X, u, C, 270e6, 30.0, oks[1], oks[2])
+    g0, g1 = dX[0].getArray(), dX[1].getArray()
+    assert np.abs(g0 - oc).max() <= RTOL * np.abs(oc).max()
+    assert np.abs(g1 - ok).max() <= RTOL * np.abs(ok).max()
+    dR = a.createNodeVec()
+    a.addAdjointResXptSensProducts([pv], [dR], -1.0)
+    trac = np.tile(p.traction, (p.aux_elems.size, 1))
+    oR = (O.adj_res_xpt_product(order, m.conn, X, u, psi, t, C, -1.0)
+          + O.traction_xpt_product(order, m.conn, X, p.aux_elems, p.aux_faces, trac, psi, -1.0))
+    assert np.abs(dR.getArray() - oR).max() <= RTOL * np.abs(oR).max()
+
+
+def test_total_compliance_node_gradient():
+    """The compliance node gradient of the cube problem (solve, adjoint solve,
+    addXptSens, addAdjointResXptSensProducts) against the oracle's dense
+    adjoint."""
+    p, X, t, rng = _distorted_problem(4, 2, 6)
+    a, m = p.assembler, p.mesh
+    p.assemble()
+    flag, its, pc, ksm = p.solve_fp64(m=100, restarts=100, rtol=1e-12)
+    assert flag == 1
+    u = p.u.getArray().copy()
+    comp = functions.Compliance(a)
+    a.evalFunctions([comp])
+    dfdu = a.createVec()
+    a.addSVSens([comp], [dfdu])
+    a.applyBCs(dfdu)
+    psi = a.createVec()
+    assert ksm.solve(dfdu, psi) == 1
+    g = a.createNodeVec()
+    a.addXptSens([comp], [g])
+    a.addAdjointResXptSensProducts([psi], [g], -1.0)
+    # oracle: dense solve and dense adjoint
+    C = O.iso_C21(70e9, 0.3)
+    rowp, cols = O.node_csr(m.nnodes, m.conn, 8)
+    A, res = O.assemble_solid(2, m.conn, X, np.zeros(3 * m.nnodes), t, C, rowp, cols)
+    trac = np.tile(p.traction, (p.aux_elems.size, 1))
+    O.add_tractions(2, m.conn, X, p.aux_elems, p.aux_faces, trac, res)
+    n = m.nnodes
+    K = np.zeros((3 * n, 3 * n))
+    Ab = A.reshape(-1, 3, 3)
+    for i in range(n):
+        for j in range(rowp[i], rowp[i + 1]):
+            K[3 * i:3 * i + 3, 3 * cols[j]:3 * cols[j] + 3] = Ab[j]
+    free = np.ones(3 * n, bool)
+    for b in p.bc_nodes:
+        free[3 * b:3 * b + 3] = False
+    uo = np.zeros(3 * n)
+    uo[free] = -np.linalg.solve(K[np.ix_(free, free)], res[free])
+    assert np.abs(u - uo).max() <= 1e-8 * np.abs(uo).max()
+    dcdu = O.compliance_svsens(2, m.conn, X, uo, t, C)
+    dcdu[~free] = 0.0
+    po = np.zeros_like(uo)
+    po[free] = np.linalg.solve(K[np.ix_(free, free)].T, dcdu[free])
+    go = (O.compliance_xptsens(2, m.conn, X, uo, t, C)
+          - O.adj_res_xpt_product(2, m.conn, X, uo, po, t, C)
+          - O.traction_xpt_product(2, m.conn, X, p.aux_elems, p.aux_faces, trac, po))
+    assert np.abs(g.getArray() - go).max() <= 1e-7 * np.abs(go).max()
