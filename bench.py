@@ -70,42 +70,43 @@ def time_spmv(L, ev, p, reps=50):
     """Average duration of the FP64 and FP32 block-SpMV kernels (HIP events)."""
     rp, cp, ap = p.mat.getDeviceArrays()
     b, nr, nnz = p.mat.getInfo()
+    bb = b * b
     x = p.assembler.createVec()
     x.set(1.0)
     y = p.assembler.createVec()
     xd, yd = x.getDevicePointer(), y.getDevicePointer()
     res = {}
     for _ in range(5):
-        L.tacs_hip_bsr_spmv(3, nr, rp, cp, ap, xd, yd, 0, None)
+        L.tacs_hip_bsr_spmv(b, nr, rp, cp, ap, xd, yd, 0, None)
     e0, e1 = ev.event(), ev.event()
     ev.record(e0)
     for _ in range(reps):
-        L.tacs_hip_bsr_spmv(3, nr, rp, cp, ap, xd, yd, 0, None)
+        L.tacs_hip_bsr_spmv(b, nr, rp, cp, ap, xd, yd, 0, None)
     ev.record(e1)
     res["fp64_ms"] = ev.elapsed_ms(e0, e1) / reps
     # FP32 copy of the values, FP32 vectors
     import tacs_amd.TACS as T  # noqa
-    n = 3 * nr
+    n = b * nr
     hip = ev.hip
     a32 = ctypes.c_void_p()
     x32 = ctypes.c_void_p()
     y32 = ctypes.c_void_p()
-    assert hip.hipMalloc(ctypes.byref(a32), ctypes.c_size_t(4 * 9 * nnz)) == 0
+    assert hip.hipMalloc(ctypes.byref(a32), ctypes.c_size_t(4 * bb * nnz)) == 0
     assert hip.hipMalloc(ctypes.byref(x32), ctypes.c_size_t(4 * n)) == 0
     assert hip.hipMalloc(ctypes.byref(y32), ctypes.c_size_t(4 * n)) == 0
-    L.tacs_hip_convert_f64_f32(9 * nnz, ap, a32, None)
+    L.tacs_hip_convert_f64_f32(bb * nnz, ap, a32, None)
     L.tacs_hip_convert_f64_f32(n, xd, x32, None)
     for _ in range(5):
-        L.tacs_hip_bsr_spmv_f32(3, nr, rp, cp, a32, x32, y32, 0, None)
+        L.tacs_hip_bsr_spmv_f32(b, nr, rp, cp, a32, x32, y32, 0, None)
     ev.record(e0)
     for _ in range(reps):
-        L.tacs_hip_bsr_spmv_f32(3, nr, rp, cp, a32, x32, y32, 0, None)
+        L.tacs_hip_bsr_spmv_f32(b, nr, rp, cp, a32, x32, y32, 0, None)
     ev.record(e1)
     res["fp32_ms"] = ev.elapsed_ms(e0, e1) / reps
     for q in (a32, x32, y32):
         hip.hipFree(q)
-    res["fp64_bytes"] = spmv_bytes(3, nr, nnz, 8)
-    res["fp32_bytes"] = spmv_bytes(3, nr, nnz, 4)
+    res["fp64_bytes"] = spmv_bytes(b, nr, nnz, 8)
+    res["fp32_bytes"] = spmv_bytes(b, nr, nnz, 4)
     return res
 
 
@@ -195,6 +196,105 @@ def cpu_baseline(p, gpu_its, budget_s=25.0):
             "step_s_estimate": step}
 
 
+def shell_step(p, pc, ksm, ks_weight=100.0, want_grad=False):
+    """One pass of the shell path: assembly, solve, KS failure, its adjoint
+    w.r.t. the element thicknesses."""
+    from tacs_amd import functions, lib
+    L = lib()
+    a = p.assembler
+    t = {}
+    t0 = time.perf_counter()
+    a.zeroVariables()
+    a.assembleJacobian(1.0, 0.0, 0.0, p.res, p.mat)
+    L.tacs_device_synchronize()
+    t["assembly_s"] = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    pc.factor()
+    flag_f = ksm.solve(p.res, p.u)
+    its_f = ksm.getIterCount()
+    p.u.scale(-1.0)
+    a.setVariables(p.u)
+    ks = functions.KSFailure(a, ksWeight=ks_weight)
+    val = a.evalFunctions([ks])[0]
+    dfdu = a.createVec()
+    a.addSVSens([ks], [dfdu])
+    a.applyBCs(dfdu)
+    psi = a.createVec()
+    flag_a = ksm.solve(dfdu, psi)
+    its_a = ksm.getIterCount()
+    dfdx = a.createDesignVec()
+    a.addDVSens([ks], [dfdx], 1.0)
+    a.addAdjointResProducts([psi], [dfdx], -1.0)
+    L.tacs_device_synchronize()
+    t["solve_adjoint_s"] = time.perf_counter() - t1
+    t["step_s"] = time.perf_counter() - t0
+    out = dict(t=t, ks=val, its=(its_f, its_a), flags=(flag_f, flag_a))
+    if want_grad:
+        out["grad"] = dfdx.getArray().copy()
+    return out
+
+
+def shell_main(args, L, ev):
+    """Quad4 shell plate (512^2 by default): assembly, mixed solve and KS
+    adjoint w.r.t. thickness; kernel timings.  Prints its own JSON line (not
+    the headline metric)."""
+    from tacs_amd.problems import ShellProblem
+    n = args.n if args.n != 128 else 512
+    t0 = time.perf_counter()
+    p = ShellProblem(n, dv="element", pressure=1e4, bc="ss", t0=0.01)
+    rng = np.random.default_rng(7)
+    p.set_thickness(rng.uniform(0.005, 0.02, p.mesh.nelem))
+    log(f"setup {time.perf_counter() - t0:.1f}s: {p.mesh.nelem} shells, {6 * p.mesh.nnodes} DOF")
+    p.assemble()
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0)
+    for _ in range(5):
+        p.assembler.assembleJacobian(1.0, 0.0, 0.0, None, p.mat)
+    ev.record(e1)
+    asm_ms = ev.elapsed_ms(e0, e1) / 5
+    sp = time_spmv(L, ev, p)
+    nelem = p.mesh.nelem
+    kern = {"assembly_ms": asm_ms, "assembly_Melem_s": nelem / (asm_ms * 1e-3) / 1e6,
+            "spmv6_fp64": {"avg_ms": sp["fp64_ms"], "bytes": sp["fp64_bytes"],
+                           "GBps": sp["fp64_bytes"] / sp["fp64_ms"] / 1e6},
+            "spmv6_fp32": {"avg_ms": sp["fp32_ms"], "bytes": sp["fp32_bytes"],
+                           "GBps": sp["fp32_bytes"] / sp["fp32_ms"] / 1e6}}
+    if args.kernels_only:
+        kern.update(elements=nelem, dof=6 * p.mesh.nnodes, nnz_blocks=p.mat.getInfo()[2])
+        print(json.dumps(kern))
+        return
+    pc, ksm = make_solver(p, "mixed", args.m, args.rtol, args.deflation)
+    for w in range(args.warmup):
+        shell_step(p, pc, ksm)
+    steps = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        steps.append(shell_step(p, pc, ksm, want_grad=(k == args.steps - 1)))
+        log(f"step {k}: {steps[-1]['t']} its={steps[-1]['its']} flags={steps[-1]['flags']}")
+    wall = time.perf_counter() - t0
+    ke_bytes = nelem * 576 * 8
+    out = {
+        "workload": f"Quad4 shell plate {n}x{n}, t ~ U[0.005, 0.02], E=70e9 nu=0.3, simply "
+                    f"supported (u=v=w=0 on the edges), pressure 1e4 Pa; KS failure (rho=100) "
+                    f"adjoint w.r.t. thickness; mixed GMRES({args.m}) recycling "
+                    f"{args.deflation}, block-Jacobi, rtol {args.rtol:g}",
+        "elements": nelem, "dof": 6 * p.mesh.nnodes, "nnz_blocks": p.mat.getInfo()[2],
+        "ms_per_step": 1e3 * wall / args.steps,
+        "Melem_per_s_step": nelem / (wall / args.steps) / 1e6,
+        "assembly_ms": asm_ms, "assembly_Melem_s": nelem / (asm_ms * 1e-3) / 1e6,
+        "assembly_bytes_elem_matrices": ke_bytes,
+        "iterations": list(steps[-1]["its"]), "flags": list(steps[-1]["flags"]),
+        "solve_adjoint_s": steps[-1]["t"]["solve_adjoint_s"], "ks": steps[-1]["ks"],
+        "spmv6_fp64": {"avg_ms": sp["fp64_ms"], "bytes": sp["fp64_bytes"],
+                       "GBps": sp["fp64_bytes"] / sp["fp64_ms"] / 1e6,
+                       "frac": sp["fp64_bytes"] / sp["fp64_ms"] / 1e6 / 8000.0},
+        "spmv6_fp32": {"avg_ms": sp["fp32_ms"], "bytes": sp["fp32_bytes"],
+                       "GBps": sp["fp32_bytes"] / sp["fp32_ms"] / 1e6,
+                       "frac": sp["fp32_bytes"] / sp["fp32_ms"] / 1e6 / 8000.0},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +312,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="assemble and time the SpMV kernels only (for profiler runs)")
+    ap.add_argument("--workload", default="hex", choices=["hex", "shell"],
+                    help="hex: the 128^3 headline line; shell: the Quad4 plate (own JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -222,6 +324,9 @@ def main():
     from tacs_amd.problems import SolidProblem
     L = lib()
     ev = HipEvents()
+    if args.workload == "shell":
+        shell_main(args, L, ev)
+        return
     t0 = time.perf_counter()
     p = SolidProblem(args.n, dv="element")
     log(f"setup {time.perf_counter() - t0:.1f}s: {p.mesh.nelem} elements, {3 * p.mesh.nnodes} DOF")
