@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include <vector>
+
 #include "tacs_hip.h"
 
 typedef double TacsScalar;
@@ -367,6 +369,59 @@ class TACSMixedGMRES : public TACSKsm {
   double* d_scal;
   void* d_red;
   double* h_pin;
+};
+
+// Batched restarted GMRES(m) for up to R right-hand sides of one 3x3-block
+// matrix (R <= 64): the R Arnoldi recurrences advance together, so every step
+// reads the matrix once for all right-hand sides (multi-RHS SpMM over
+// interleaved block vectors V[i*R + j]) and the Gram-Schmidt products of all
+// right-hand sides run as one batched multi-dot.  Every cycle restarts from
+// the true FP64 residual B - A X.  prec 64 runs the Arnoldi steps on the FP64
+// matrix and block-Jacobi inverse, prec 32 on their FP32-rounded values (FP64
+// vectors and arithmetic).  Each right-hand side stops on its own criterion
+// and keeps its own iteration count.  No counterpart in the reference (one
+// GMRES::solve per load case there, src/bpmat/KSM.cpp:785).
+class TACSBatchedGMRES : public TACSObject {
+ public:
+  TACSBatchedGMRES(TACSParallelMat* mat, TACSBlockJacobiPc* pc, int m, int nrestart, int nrhs,
+                   int prec);
+  ~TACSBatchedGMRES();
+  // x[j] = A^{-1} b[j], j < nrhs <= R; returns 1 when every system converged
+  int solve(int nrhs, TACSBVec** b, TACSBVec** x, int zero_guess = 1);
+  // the same on interleaved device block vectors B[i*nrhs + j], X[i*nrhs + j]
+  int solveBlock(int nrhs, const double* B, double* X, int zero_guess);
+  void setTolerances(double rtol, double atol);
+  // Deflate every right-hand side by the recycled space (U, C) of a
+  // recycling solver over the same matrix (A32 D32^{-1} U = C, C orthonormal):
+  // each cycle starts with x += D^{-1} U C^T r, r -= C C^T r and runs its
+  // Arnoldi steps on (I - C C^T) A M^{-1}; the space stays fixed.  Copies the
+  // space; it is dropped when the matrix values change.  Returns its
+  // dimension (0: src has none).
+  int setRecycleSpace(TACSMixedGMRES* src);
+  double rtol, atol;
+  int msub, nrestart, R, prec, cycles;
+  int kdefl;         // dimension of the recycled space (0: none)
+  long kVersion;     // matrix version of the recycled space
+  double *d_U, *d_C;
+  TACSParallelMat* mat;
+  TACSBlockJacobiPc* pc;
+  double* d_V;     // (m+1) block vectors of n*R
+  double* d_work;  // block work vector
+  double* d_Bb;    // packed right-hand sides / solutions of solve()
+  double* d_Xb;
+  double* d_scal;  // per right-hand side: H, res, Qc, Qs, tmp (stride slice())
+  double* d_vec;   // |r|^2, |r|, 1/|r|, Givens estimates (64 each)
+  int* d_cnt;
+  void* d_red;
+  double* h_pin;
+  std::vector<int> iters, flags;
+  std::vector<double> r0, rn;
+  // per right-hand side: H (m+1) x m, res m+1, Qc m, Qs m, tmp m+1, then the
+  // recycled-space coefficients Bc (KD_MAX x m), tmpC, z, projection (KD_MAX each)
+  static const int KD_MAX = 32;
+  size_t slice() const {
+    return (size_t)(msub + 1) * msub + 4 * (size_t)msub + 2 + (size_t)KD_MAX * (msub + 3);
+  }
 };
 
 class TACSAssembler;

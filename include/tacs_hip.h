@@ -279,6 +279,55 @@ size_t tacs_hip_tsgemm_scratch_bytes(void);
 int tacs_hip_tsgemm(size_t n, int p, const double* X, int q, const double* Y, double* scratch,
                     double* out, hipStream_t stream);
 
+/* Batched GMRES kernels over interleaved block vectors V[i*R + j] (dof i,
+   right-hand side j, R <= 64; the layout of tacs_hip_bsr3_spmm).  Per
+   right-hand side j the Hessenberg, rotations and residual vectors are slices
+   with strides sH, sQ, sRes.  No counterpart in the reference (one GMRES per
+   right-hand side there, KSM.cpp:785).
+   bmdot:        out[j*sR + v] (+)= W_v(:, j) . q(:, j), v < k
+   *_shared:     the same with single vectors W_v (W[v*ldw + i]) common to every
+                 right-hand side (a recycled space)
+   bsmall_gemv:  z[j*sz + v] = sum_{i < cnt[j]} B[j*sB + v + i*ldb] y[j*sy + i], v < k
+   bcgs_update:  q(:, j) -= sum_v W_v(:, j) c[j*sR + v]
+   blincomb:     y(:, j) (+)= sum_{v < cnt[j]} W_v(:, j) c[j*sR + v]
+   badd:         y[j*sy + v] += x[j*sx + v], v < k
+   bscale:       x(:, j) *= s[j];   bsqrt_inv: nrm = sqrt(sq), inv = 1/nrm
+   bgivens / bbacksolve: the GMRES Givens update / back-substitution per j
+   bdiag_apply3: 3x3 block-Jacobi (FP64 or FP32 inverse blocks)
+   bsr3_spmm_mixed: Y = A32 X with FP32 values and FP64 block vectors
+   bpack / bunpack: copy one right-hand side into / out of a block vector */
+size_t tacs_hip_batch_scratch_bytes(int R);
+int tacs_hip_bmdot(size_t n, int R, int k, const double* W, size_t ldw, const double* q,
+                   double* scratch, double* out, size_t sR, int accumulate, hipStream_t stream);
+int tacs_hip_bmdot_shared(size_t n, int R, int k, const double* W, size_t ldw, const double* q,
+                          double* scratch, double* out, size_t sR, int accumulate,
+                          hipStream_t stream);
+int tacs_hip_bcgs_update(size_t n, int R, int k, const double* W, size_t ldw, const double* c,
+                         size_t sR, double* q, hipStream_t stream);
+int tacs_hip_bcgs_update_shared(size_t n, int R, int k, const double* W, size_t ldw,
+                                const double* c, size_t sR, double* q, hipStream_t stream);
+int tacs_hip_blincomb_shared(size_t n, int R, int k, const double* W, size_t ldw, const double* c,
+                             size_t sR, double* y, int add, hipStream_t stream);
+int tacs_hip_bsmall_gemv(int R, int k, const int* cnt, const double* B, size_t sB, int ldb,
+                         const double* y, size_t sy, double* z, size_t sz, hipStream_t stream);
+int tacs_hip_blincomb(size_t n, int R, int kmax, const double* W, size_t ldw, const double* c,
+                      size_t sR, const int* cnt, double* y, int add, hipStream_t stream);
+int tacs_hip_badd(int R, int k, const double* x, size_t sx, double* y, size_t sy,
+                  hipStream_t stream);
+int tacs_hip_bscale(size_t n, int R, const double* s, double* x, hipStream_t stream);
+int tacs_hip_bsqrt_inv(int R, const double* sq, size_t sS, double* nrm, size_t sN, double* inv,
+                       hipStream_t stream);
+int tacs_hip_bgivens(int R, int i, int m, double* H, size_t sH, double* Qc, double* Qs, size_t sQ,
+                     double* res, size_t sRes, double* st, hipStream_t stream);
+int tacs_hip_bbacksolve(int R, int m, const int* cnt, const double* H, size_t sH, double* res,
+                        size_t sRes, hipStream_t stream);
+int tacs_hip_bdiag_apply3(int prec, int nrows, int R, const void* Dinv, const double* x,
+                          double* y, hipStream_t stream);
+int tacs_hip_bsr3_spmm_mixed(int nrows, int nrhs, const int* rowp, const int* cols, const float* A,
+                             const double* X, double* Y, hipStream_t stream);
+int tacs_hip_bpack(size_t n, int R, int j, const double* x, double* X, hipStream_t stream);
+int tacs_hip_bunpack(size_t n, int R, int j, const double* X, double* x, hipStream_t stream);
+
 /* Host-side dense helper of the deflated-restart GMRES: eigenvalues of the
    row-major n x n matrix G sorted by ascending modulus (wr, wi) and a real
    basis V (column-major n x kout) of the eigenvectors of the k smallest; a
@@ -421,6 +470,28 @@ tacs_obj tacs_mixed_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart)
    vectors of smallest harmonic Ritz value at each restart; 0 <= k <= 31,
    k < m, m <= 511).  No counterpart in the reference. */
 int tacs_mixed_gmres_set_deflation(tacs_obj ksm, int k);
+/* Batched restarted GMRES(m) for nrhs <= 64 right-hand sides of one 3x3
+   block matrix: one multi-RHS SpMM per step for all systems, per-system
+   Hessenberg, rotations, stopping test and iteration count; prec 64 (FP64
+   Arnoldi operator) or 32 (FP32-rounded matrix and block-Jacobi values, FP64
+   vectors), every cycle restarted from the FP64 residual.  No counterpart in
+   the reference, which solves load cases one GMRES::solve at a time
+   (src/bpmat/KSM.cpp:785).  solve_block takes device block vectors with the
+   right-hand sides interleaved, B[i*nrhs + j]; get_stats fills per-system
+   iteration counts, flags (1: converged), initial and final residual norms
+   and returns the number of cycles. */
+tacs_obj tacs_batched_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int nrhs,
+                                   int prec);
+int tacs_batched_gmres_set_tolerances(tacs_obj ksm, double rtol, double atol);
+/* Deflate the batched solves by the recycled space of a mixed GMRES with
+   deflation over the same matrix (copied; dropped when the matrix values
+   change).  Returns its dimension, 0 when src has none, -1 on error. */
+int tacs_batched_gmres_set_recycle_space(tacs_obj ksm, tacs_obj src);
+int tacs_batched_gmres_solve(tacs_obj ksm, int nrhs, tacs_obj* b, tacs_obj* x, int zero_guess);
+int tacs_batched_gmres_solve_block(tacs_obj ksm, int nrhs, const double* B, double* X,
+                                   int zero_guess);
+int tacs_batched_gmres_get_stats(tacs_obj ksm, int nrhs, int* iters, int* flags, double* r0,
+                                 double* rn);
 int tacs_ksm_set_tolerances(tacs_obj ksm, double rtol, double atol);
 int tacs_ksm_solve(tacs_obj ksm, tacs_obj b, tacs_obj x, int zero_guess);
 int tacs_ksm_get_iter_count(tacs_obj ksm);

@@ -11,7 +11,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import lib, check
+from ._lib import lib, check, TACSError
 from ._obj import TACSHandle
 
 dtype = np.float64
@@ -176,6 +176,71 @@ class MixedKSM(KSM):
 
     def getOuterCount(self):
         return lib().tacs_mixed_gmres_get_outer_count(self.ptr)
+
+
+class BatchedKSM(TACSHandle):
+    """Batched restarted GMRES(m) with block-Jacobi for up to nrhs (<= 64)
+    right-hand sides of one 3x3-block matrix: every Arnoldi step of all the
+    systems shares one multi-RHS SpMM; each system keeps its own Hessenberg,
+    stopping test and iteration count.  prec=64 runs the Arnoldi steps on the
+    FP64 matrix, prec=32 on its FP32-rounded values (FP64 vectors); every
+    cycle restarts from the FP64 residual.  No counterpart in the reference,
+    which runs one KSM.solve per load case."""
+
+    def __init__(self, mat, pc, m=30, restarts=100, nrhs=16, prec=64):
+        super().__init__(lib().tacs_batched_gmres_create(mat.ptr, pc.ptr, m, restarts, nrhs,
+                                                         prec), keep=(mat, pc))
+        self.nrhs = nrhs
+
+    def setTolerances(self, rtol, atol):
+        lib().tacs_batched_gmres_set_tolerances(self.ptr, rtol, atol)
+
+    def setRecycleSpace(self, ksm):
+        """Deflate every right-hand side by the recycled space (U, C) that the
+        MixedKSM ksm (with deflation, same matrix) holds after a solve: each
+        cycle starts with x += M^{-1} U C^T r, r -= C C^T r and its Arnoldi
+        steps run on (I - C C^T) A M^{-1}.  Returns the space's dimension."""
+        k = lib().tacs_batched_gmres_set_recycle_space(self.ptr, ksm.ptr)
+        self._raise_on_error(k, "BatchedKSM.setRecycleSpace")
+        return k
+
+    def solve(self, bs, xs, zero_guess=1):
+        """xs[j] = A^{-1} bs[j]; returns 1 when every system converged."""
+        if len(bs) != len(xs):
+            raise ValueError("BatchedKSM.solve: as many solutions as right-hand sides")
+        arr = ctypes.c_void_p * len(bs)
+        flag = lib().tacs_batched_gmres_solve(self.ptr, len(bs), arr(*[b.ptr for b in bs]),
+                                              arr(*[x.ptr for x in xs]), zero_guess)
+        self._raise_on_error(flag, "BatchedKSM.solve")
+        self._last = len(bs)
+        return flag
+
+    @staticmethod
+    def _raise_on_error(flag, what):
+        # negative: rejected arguments; a message without a negative flag: a
+        # kernel launch or HIP call failed during the solve
+        msg = lib().tacs_last_error()
+        if flag < 0 or msg:
+            raise TACSError(f"{what} failed (code {flag}): {msg.decode() if msg else ''}")
+
+    def solveBlock(self, nrhs, B_ptr, X_ptr, zero_guess=1):
+        """Solve on device block vectors (B[i*nrhs + j], X[i*nrhs + j])."""
+        flag = lib().tacs_batched_gmres_solve_block(self.ptr, nrhs, B_ptr, X_ptr, zero_guess)
+        self._raise_on_error(flag, "BatchedKSM.solveBlock")
+        self._last = nrhs
+        return flag
+
+    def getStats(self):
+        """(iterations, flags, initial residual norms, final residual norms,
+        cycles) of the last solve, per right-hand side."""
+        n = getattr(self, "_last", self.nrhs)
+        it = np.zeros(n, np.int32)
+        fl = np.zeros(n, np.int32)
+        r0 = np.zeros(n)
+        rn = np.zeros(n)
+        cyc = lib().tacs_batched_gmres_get_stats(self.ptr, n, it.ctypes.data, fl.ctypes.data,
+                                                 r0.ctypes.data, rn.ctypes.data)
+        return it, fl, r0, rn, cyc
 
 
 class AuxElements(TACSHandle):

@@ -450,6 +450,77 @@ int tacs_mixed_gmres_set_deflation(tacs_obj ksm, int k) {
   REQUIRE(TACSMixedGMRES, g, ksm);
   return g->setDeflation(k);
 }
+tacs_obj tacs_batched_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int nrhs,
+                                   int prec) {
+  clear_error();
+  REQUIRE_NULL(TACSParallelMat, A, mat);
+  REQUIRE_NULL(TACSBlockJacobiPc, p, pc);
+  if (A->bsize != 3 || m < 1 || m > 512 || nrhs < 1 || nrhs > 64 || (prec != 32 && prec != 64) ||
+      nrestart < 0) {
+    tacs_set_error("batched GMRES: need 3x3 blocks, 1 <= m <= 512, 1 <= nrhs <= 64, "
+                   "prec 32 or 64, nrestart >= 0");
+    return NULL;
+  }
+  return (TACSObject*)new TACSBatchedGMRES(A, p, m, nrestart, nrhs, prec);
+}
+int tacs_batched_gmres_set_tolerances(tacs_obj ksm, double rtol, double atol) {
+  REQUIRE(TACSBatchedGMRES, k, ksm);
+  k->setTolerances(rtol, atol);
+  return 0;
+}
+int tacs_batched_gmres_set_recycle_space(tacs_obj ksm, tacs_obj src) {
+  clear_error();
+  TACSBatchedGMRES* k = OBJ(TACSBatchedGMRES, ksm);
+  TACSMixedGMRES* g = OBJ(TACSMixedGMRES, src);
+  if (!k || !g) {
+    tacs_set_error("%s: expected a batched GMRES and a mixed GMRES handle", __func__);
+    return -1;
+  }
+  return k->setRecycleSpace(g);
+}
+int tacs_batched_gmres_solve(tacs_obj ksm, int nrhs, tacs_obj* b, tacs_obj* x, int zero_guess) {
+  clear_error();
+  TACSBatchedGMRES* k = OBJ(TACSBatchedGMRES, ksm);
+  if (!k) {
+    tacs_set_error("%s: expected a TACSBatchedGMRES handle", __func__);
+    return -1;
+  }
+  if (nrhs < 1 || nrhs > k->R) {
+    tacs_set_error("%s: nrhs = %d outside [1, %d]", __func__, nrhs, k->R);
+    return -1;
+  }
+  std::vector<TACSBVec*> bv(nrhs), xv(nrhs);
+  for (int j = 0; j < nrhs; j++) {
+    bv[j] = OBJ(TACSBVec, b[j]);
+    xv[j] = OBJ(TACSBVec, x[j]);
+    if (!bv[j] || !xv[j]) {
+      tacs_set_error("%s: vector %d is not a TACSBVec", __func__, j);
+      return -1;
+    }
+  }
+  return k->solve(nrhs, bv.data(), xv.data(), zero_guess);
+}
+int tacs_batched_gmres_solve_block(tacs_obj ksm, int nrhs, const double* B, double* X,
+                                   int zero_guess) {
+  clear_error();
+  TACSBatchedGMRES* k = OBJ(TACSBatchedGMRES, ksm);
+  if (!k) {
+    tacs_set_error("%s: expected a TACSBatchedGMRES handle", __func__);
+    return -1;
+  }
+  return k->solveBlock(nrhs, B, X, zero_guess);
+}
+int tacs_batched_gmres_get_stats(tacs_obj ksm, int nrhs, int* iters, int* flags, double* r0,
+                                 double* rn) {
+  REQUIRE(TACSBatchedGMRES, k, ksm);
+  for (int j = 0; j < nrhs && j < (int)k->iters.size(); j++) {
+    if (iters) iters[j] = k->iters[j];
+    if (flags) flags[j] = k->flags[j];
+    if (r0) r0[j] = k->r0[j];
+    if (rn) rn[j] = k->rn[j];
+  }
+  return k->cycles;
+}
 int tacs_ksm_set_tolerances(tacs_obj ksm, double rtol, double atol) {
   REQUIRE(TACSKsm, k, ksm);
   k->setTolerances(rtol, atol);

@@ -1148,6 +1148,278 @@ int TACSMixedGMRES::solveDeflated(TACSVec* tb, TACSVec* tx, int zero_guess) {
 }
 
 // ---------------------------------------------------------------------------
+// Batched GMRES: R restarted GMRES(m) recurrences (the recurrence of
+// GMRES::solve src/bpmat/KSM.cpp:785-958, classical Gram-Schmidt twice)
+// advanced together over interleaved block vectors.  Each cycle starts from
+// the true FP64 residual R = B - A X with V_0(:, j) = R(:, j) / |R(:, j)| and
+// res_j[0] = 1; a right-hand side whose Givens estimate reaches
+// max(rtol |r_j(0)|, atol) / |r_j| keeps its step count cnt_j and its later
+// columns are ignored; the cycle ends when every active right-hand side has
+// converged or m steps are done; then X(:, j) += |r_j| M^{-1} V y_j.
+// Right-hand sides that finished are masked out of later cycles (zero start
+// vector, cnt_j = 0).
+// ---------------------------------------------------------------------------
+TACSBatchedGMRES::TACSBatchedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int _m, int _nrestart,
+                                   int nrhs, int _prec)
+    : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart), R(nrhs), prec(_prec), cycles(0),
+      kdefl(0), kVersion(-1), d_U(NULL), d_C(NULL), mat(m), pc(p) {
+  mat->incref();
+  pc->incref();
+  if (prec == 32) mat->getValues32(0);
+  size_t N = (size_t)mat->nrows * mat->bsize * R;
+  d_V = dalloc<double>((size_t)(msub + 1) * N);
+  d_work = dalloc<double>(N);
+  d_Bb = dalloc<double>(N);
+  d_Xb = dalloc<double>(N);
+  d_scal = dalloc<double>(slice() * R);
+  HCHECK(hipMemsetAsync(d_scal, 0, slice() * R * sizeof(double), g_stream));
+  d_vec = dalloc<double>(4 * 64);
+  d_cnt = dalloc<int>(64);
+  size_t rb = tacs_hip_batch_scratch_bytes(R);
+  HCHECK(hipMalloc(&d_red, rb));
+  HCHECK(hipHostMalloc((void**)&h_pin, 512 * sizeof(double), 0));
+}
+TACSBatchedGMRES::~TACSBatchedGMRES() {
+  HCHECK(hipStreamSynchronize(g_stream));
+  void* bufs[] = {d_V, d_work, d_Bb, d_Xb, d_scal, d_vec, d_cnt, d_red, d_U, d_C};
+  for (void* q : bufs) dfree(q);
+  HCHECK(hipHostFree(h_pin));
+  mat->decref();
+  pc->decref();
+}
+void TACSBatchedGMRES::setTolerances(double r, double a) {
+  rtol = r;
+  atol = a;
+}
+
+int TACSBatchedGMRES::setRecycleSpace(TACSMixedGMRES* src) {
+  HCHECK(hipStreamSynchronize(g_stream));
+  dfree(d_U);
+  dfree(d_C);
+  d_U = d_C = NULL;
+  kdefl = 0;
+  kVersion = -1;
+  if (src->mat != mat) {
+    tacs_set_error("setRecycleSpace: the recycling solver belongs to another matrix");
+    return -1;
+  }
+  int k = src->recycleDim;
+  if (k <= 0 || src->recycleVersion != mat->version) return 0;
+  if (k > KD_MAX) k = KD_MAX;
+  const size_t n = (size_t)mat->nrows * mat->bsize;
+  d_U = dalloc<double>((size_t)k * n);
+  d_C = dalloc<double>((size_t)k * n);
+  // [U (ndefl) | C (kd) | V] in the source; the first k columns of each
+  HCHECK(hipMemcpyAsync(d_U, src->d_B, (size_t)k * n * sizeof(double), hipMemcpyDeviceToDevice,
+                        g_stream));
+  HCHECK(hipMemcpyAsync(d_C, src->d_B + (size_t)src->ndefl * n, (size_t)k * n * sizeof(double),
+                        hipMemcpyDeviceToDevice, g_stream));
+  HCHECK(hipStreamSynchronize(g_stream));
+  kdefl = k;
+  kVersion = mat->version;
+  return k;
+}
+
+int TACSBatchedGMRES::solve(int nrhs, TACSBVec** b, TACSBVec** x, int zero_guess) {
+  if (nrhs < 1 || nrhs > R) {
+    tacs_set_error("TACSBatchedGMRES::solve: nrhs = %d outside [1, %d]", nrhs, R);
+    return -1;
+  }
+  const size_t n = (size_t)mat->nrows * mat->bsize;
+  for (int j = 0; j < nrhs; j++) {
+    if (b[j]->n != n || x[j]->n != n) {
+      tacs_set_error("TACSBatchedGMRES::solve: vector %d has the wrong size", j);
+      return -1;
+    }
+    KCHECK(tacs_hip_bpack(n, nrhs, j, b[j]->devRead(), d_Bb, g_stream));
+    if (!zero_guess) KCHECK(tacs_hip_bpack(n, nrhs, j, x[j]->devRead(), d_Xb, g_stream));
+  }
+  int flag = solveBlock(nrhs, d_Bb, d_Xb, zero_guess);
+  for (int j = 0; j < nrhs; j++) KCHECK(tacs_hip_bunpack(n, nrhs, j, d_Xb, x[j]->devWrite(), g_stream));
+  HCHECK(hipStreamSynchronize(g_stream));
+  return flag;
+}
+
+int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_guess) {
+  if (Rn < 1 || Rn > R) {
+    tacs_set_error("TACSBatchedGMRES: nrhs = %d outside [1, %d]", Rn, R);
+    return -1;
+  }
+  const int nr = mat->nrows, m = msub;
+  const size_t n = (size_t)nr * 3, N = n * Rn, S = slice();
+  // per right-hand side slice (stride S)
+  double* H = d_scal;
+  double* res = d_scal + (size_t)(m + 1) * m;
+  double* Qc = res + (m + 1);
+  double* Qs = Qc + m;
+  double* tmp = Qs + m;
+  double* Bc = tmp + (m + 1);             // KD_MAX x m, column i at Bc + i*KD_MAX
+  double* tmpC = Bc + (size_t)KD_MAX * m;
+  double* zc = tmpC + KD_MAX;
+  double* pco = zc + KD_MAX;
+  double *nrm2 = d_vec, *nrm = d_vec + 64, *inv = d_vec + 128, *st = d_vec + 192;
+  double *h_inv = h_pin, *h_res0 = h_pin + 64, *h_read = h_pin + 256;
+  int* h_cnt = (int*)(h_pin + 128);
+  auto V = [&](int i) { return d_V + (size_t)i * N; };
+  const float* A32 = prec == 32 ? mat->getValues32(1) : NULL;
+  const void* Dinv = prec == 32 ? (const void*)pc->d_Dinv32 : (const void*)pc->d_Dinv;
+  const int kd = (kdefl > 0 && kVersion == mat->version) ? kdefl : 0;
+  auto spmm64 = [&](const double* x, double* y) {
+    KCHECK(tacs_hip_bsr3_spmm(nr, Rn, mat->d_rowp, mat->d_cols, mat->d_A, x, y, g_stream));
+  };
+  // y = A M^{-1} x with the Arnoldi operator of this precision
+  auto op = [&](const double* x, double* y) {
+    KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, x, d_work, g_stream));
+    if (prec == 32)
+      KCHECK(tacs_hip_bsr3_spmm_mixed(nr, Rn, mat->d_rowp, mat->d_cols, A32, d_work, y, g_stream));
+    else
+      spmm64(d_work, y);
+  };
+  // q(:, j) /= |q(:, j)|, with |q(:, j)| written to out[j*sOut]
+  auto normalize = [&](double* q, double* out, size_t sOut) {
+    KCHECK(tacs_hip_bmdot(n, Rn, 1, q, N, q, (double*)d_red, out, sOut, 0, g_stream));
+    KCHECK(tacs_hip_bsqrt_inv(Rn, out, sOut, out, sOut, inv, g_stream));
+    KCHECK(tacs_hip_bscale(n, Rn, inv, q, g_stream));
+  };
+  std::vector<char> active(Rn, 1), hit(Rn, 0), fail(Rn, 0), conv(Rn);
+  std::vector<double> target(Rn);
+  iters.assign(Rn, 0);
+  flags.assign(Rn, 0);
+  r0.assign(Rn, 0.0);
+  rn.assign(Rn, 0.0);
+  cycles = 0;
+  if (zero_guess) {
+    KCHECK(tacs_hip_fill(64, N, Xd, 0.0, g_stream));
+    HCHECK(hipMemcpyAsync(V(0), Bd, N * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
+  } else {
+    spmm64(Xd, V(0));
+    KCHECK(tacs_hip_axpby(64, 64, N, 1.0, Bd, -1.0, V(0), g_stream));
+  }
+  for (int count = 0;; count++) {
+    // FP64 residual norms
+    KCHECK(tacs_hip_bmdot(n, Rn, 1, V(0), N, V(0), (double*)d_red, nrm2, 1, 0, g_stream));
+    KCHECK(tacs_hip_bsqrt_inv(Rn, nrm2, 1, nrm, 1, inv, g_stream));
+    d2h(h_read, nrm, Rn);
+    int nact = 0;
+    for (int j = 0; j < Rn; j++) {
+      if (!active[j]) continue;
+      double r = h_read[j];
+      if (count == 0) r0[j] = r;
+      rn[j] = r;
+      if (!std::isfinite(r) || fail[j]) {
+        active[j] = 0;
+      } else if (r <= rtol * r0[j] || r < atol || hit[j]) {
+        active[j] = 0;
+        flags[j] = 1;
+      } else {
+        nact++;
+      }
+    }
+    if (nact == 0 || count > nrestart) break;
+    cycles++;
+    // V_0 = r / |r| for the active right-hand sides, zero for the others
+    for (int j = 0; j < Rn; j++) {
+      h_inv[j] = active[j] ? 1.0 / rn[j] : 0.0;
+      target[j] = active[j] ? std::max(rtol * r0[j], atol) / rn[j] : 0.0;
+      conv[j] = !active[j];
+      h_cnt[j] = 0;
+    }
+    h2d(inv, h_inv, Rn);
+    KCHECK(tacs_hip_bscale(n, Rn, inv, V(0), g_stream));
+    if (kd > 0) {
+      // x += |r| M^{-1} U C^T v_0;  v_0 -= C C^T v_0
+      KCHECK(tacs_hip_bmdot_shared(n, Rn, kd, d_C, n, V(0), (double*)d_red, pco, S, 0, g_stream));
+      KCHECK(tacs_hip_blincomb_shared(n, Rn, kd, d_U, n, pco, S, V(1), 0, g_stream));
+      KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, V(1), d_work, g_stream));
+      KCHECK(tacs_hip_bscale(n, Rn, nrm, d_work, g_stream));
+      KCHECK(tacs_hip_axpby(64, 64, N, 1.0, d_work, 1.0, Xd, g_stream));
+      KCHECK(tacs_hip_bcgs_update_shared(n, Rn, kd, d_C, n, pco, S, V(0), g_stream));
+    }
+    if (kd == 0) {
+      // res_j[0] = 1 (0 for the inactive right-hand sides)
+      for (int j = 0; j < Rn; j++) h_res0[j] = active[j] ? 1.0 : 0.0;
+      HCHECK(hipMemcpy2DAsync(res, S * sizeof(double), h_res0, sizeof(double), sizeof(double), Rn,
+                              hipMemcpyHostToDevice, g_stream));
+    } else {
+      // res_j[0] = |(I - C C^T) v_0(:, j)|, the deflated residual normalised
+      normalize(V(0), res, S);
+      HCHECK(hipMemcpy2DAsync(h_read, sizeof(double), res, S * sizeof(double), sizeof(double), Rn,
+                              hipMemcpyDeviceToHost, g_stream));
+      HCHECK(hipStreamSynchronize(g_stream));
+      for (int j = 0; j < Rn; j++)
+        if (!conv[j] && h_read[j] <= target[j]) conv[j] = 1;  // the projection alone converged
+    }
+    int niters = 0;
+    bool all = true;
+    for (int j = 0; j < Rn; j++) all = all && conv[j];
+    for (int i = 0; i < m && !all; i++) {
+      double* q = V(i + 1);
+      double* h = H + (size_t)i * (m + 1);
+      double* bc = Bc + (size_t)i * KD_MAX;
+      op(V(i), q);
+      // classical Gram-Schmidt, twice, against [C, V_0..V_i]: (bc, h) = [C V]^T q
+      // (both passes), q -= C bc + V h
+      for (int pass = 0; pass < 2; pass++) {
+        double* cc = pass == 0 ? bc : tmpC;
+        double* hh = pass == 0 ? h : tmp;
+        if (kd > 0)
+          KCHECK(tacs_hip_bmdot_shared(n, Rn, kd, d_C, n, q, (double*)d_red, cc, S, 0, g_stream));
+        KCHECK(tacs_hip_bmdot(n, Rn, i + 1, V(0), N, q, (double*)d_red, hh, S, 0, g_stream));
+        if (kd > 0) KCHECK(tacs_hip_bcgs_update_shared(n, Rn, kd, d_C, n, cc, S, q, g_stream));
+        KCHECK(tacs_hip_bcgs_update(n, Rn, i + 1, V(0), N, hh, S, q, g_stream));
+      }
+      if (kd > 0) KCHECK(tacs_hip_badd(Rn, kd, tmpC, S, bc, S, g_stream));
+      KCHECK(tacs_hip_badd(Rn, i + 1, tmp, S, h, S, g_stream));
+      normalize(q, h + i + 1, S);
+      KCHECK(tacs_hip_bgivens(Rn, i, m, H, S, Qc, Qs, S, res, S, st, g_stream));
+      niters = i + 1;
+      d2h(h_read, st, Rn);
+      all = true;
+      for (int j = 0; j < Rn; j++) {
+        if (conv[j]) continue;
+        double e = h_read[j];
+        if (!std::isfinite(e)) {
+          conv[j] = 1;
+          fail[j] = 1;
+          h_cnt[j] = i;
+        } else if (e <= target[j]) {
+          conv[j] = 1;
+          h_cnt[j] = i + 1;
+          // with the FP64 operator and no recycled space the estimate is the
+          // true residual (GMRES::solve stops on it)
+          if (prec == 64 && kd == 0) hit[j] = 1;
+        } else {
+          all = false;
+        }
+      }
+    }
+    for (int j = 0; j < Rn; j++) {
+      if (!conv[j]) h_cnt[j] = niters;
+      if (active[j]) iters[j] += h_cnt[j];
+    }
+    if (niters > 0) {
+      h2d(d_cnt, h_cnt, Rn);
+      // X(:, j) += |r_j| M^{-1} (V y_j - U B_j y_j);  R = B - A X
+      KCHECK(tacs_hip_bbacksolve(Rn, m, d_cnt, H, S, res, S, g_stream));
+      KCHECK(tacs_hip_blincomb(n, Rn, niters, V(0), N, res, S, d_cnt, d_work, 0, g_stream));
+      if (kd > 0) {
+        KCHECK(tacs_hip_bsmall_gemv(Rn, kd, d_cnt, Bc, S, KD_MAX, res, S, zc, S, g_stream));
+        KCHECK(tacs_hip_bcgs_update_shared(n, Rn, kd, d_U, n, zc, S, d_work, g_stream));
+      }
+      KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, d_work, V(1), g_stream));
+      KCHECK(tacs_hip_bscale(n, Rn, nrm, V(1), g_stream));
+      KCHECK(tacs_hip_axpby(64, 64, N, 1.0, V(1), 1.0, Xd, g_stream));
+    }
+    spmm64(Xd, V(0));
+    KCHECK(tacs_hip_axpby(64, 64, N, 1.0, Bd, -1.0, V(0), g_stream));
+  }
+  HCHECK(hipStreamSynchronize(g_stream));
+  int ok = 1;
+  for (int j = 0; j < Rn; j++) ok &= flags[j];
+  return ok;
+}
+
+// ---------------------------------------------------------------------------
 // Functions
 // ---------------------------------------------------------------------------
 TACSFunction::TACSFunction(TACSAssembler* a) : assembler(a) { assembler->incref(); }
