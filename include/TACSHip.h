@@ -398,11 +398,28 @@ class TACSBatchedGMRES : public TACSObject {
   // space; it is dropped when the matrix values change.  Returns its
   // dimension (0: src has none).
   int setRecycleSpace(TACSMixedGMRES* src);
+  // Krylov subspace recycling per right-hand side (GCRO-DR, Parks et al.,
+  // SIAM J. Sci. Comput. 28 (2006), as TACSMixedGMRES::setDeflation): every
+  // right-hand side keeps its own k-dimensional space (U_j, C_j), updated each
+  // cycle from its harmonic Ritz vectors; the spaces persist across solves
+  // with the same matrix values and number of right-hand sides and start from
+  // the setRecycleSpace space when there is one (0 <= k <= 31, k < m - 1,
+  // m + k <= 250; k = 0 turns recycling off).
+  int setDeflation(int k);
+  int solveBlockDeflated(int nrhs, const double* B, double* X, int zero_guess);
   double rtol, atol;
   int msub, nrestart, R, prec, cycles;
-  int kdefl;         // dimension of the recycled space (0: none)
-  long kVersion;     // matrix version of the recycled space
+  int kdefl;         // dimension of the shared recycled space (0: none)
+  long kVersion;     // matrix version of the shared recycled space
   double *d_U, *d_C;
+  int ndefl;         // per right-hand side recycling dimension (0: off)
+  long defVersion;   // matrix version of the per right-hand side spaces
+  int defR;          // number of right-hand sides they were built for
+  double *d_Ub, *d_Cb, *d_Unb, *d_Cnb;  // K block vectors each
+  double* d_prod;    // per right-hand side small matrices (products, basis changes)
+  double* h_stage;   // pinned staging of the small matrices, R x prodStride()
+  std::vector<int> kdj;      // dimension of each right-hand side's space
+  std::vector<double> dsc;   // 1 / |U_j(:, v)|, R x KD_MAX
   TACSParallelMat* mat;
   TACSBlockJacobiPc* pc;
   double* d_V;     // (m+1) block vectors of n*R
@@ -419,6 +436,8 @@ class TACSBatchedGMRES : public TACSObject {
   // per right-hand side: H (m+1) x m, res m+1, Qc m, Qs m, tmp m+1, then the
   // recycled-space coefficients Bc (KD_MAX x m), tmpC, z, projection (KD_MAX each)
   static const int KD_MAX = 32;
+  // per right-hand side: four (KD_MAX + m + 1) x KD_MAX matrices
+  size_t prodStride() const { return (size_t)4 * (KD_MAX + msub + 1) * KD_MAX; }
   size_t slice() const {
     return (size_t)(msub + 1) * msub + 4 * (size_t)msub + 2 + (size_t)KD_MAX * (msub + 3);
   }

@@ -295,6 +295,8 @@ int tacs_hip_tsgemm(size_t n, int p, const double* X, int q, const double* Y, do
    bgivens / bbacksolve: the GMRES Givens update / back-substitution per j
    bdiag_apply3: 3x3 block-Jacobi (FP64 or FP32 inverse blocks)
    bsr3_spmm_mixed: Y = A32 X with FP32 values and FP64 block vectors
+   bbasis_combine: out_c(:, j) (+)= sum_{v < nin} W_v(:, j) P[j*sP + v + c*ldp], c < nout <= 32
+   bbroadcast:   X(:, j) = x for every j
    bpack / bunpack: copy one right-hand side into / out of a block vector */
 size_t tacs_hip_batch_scratch_bytes(int R);
 int tacs_hip_bmdot(size_t n, int R, int k, const double* W, size_t ldw, const double* q,
@@ -325,6 +327,10 @@ int tacs_hip_bdiag_apply3(int prec, int nrows, int R, const void* Dinv, const do
                           double* y, hipStream_t stream);
 int tacs_hip_bsr3_spmm_mixed(int nrows, int nrhs, const int* rowp, const int* cols, const float* A,
                              const double* X, double* Y, hipStream_t stream);
+int tacs_hip_bbasis_combine(size_t n, int R, int nin, const double* W, size_t ldw, const double* P,
+                            size_t sP, int ldp, int nout, double* out, size_t ldo, int add,
+                            hipStream_t stream);
+int tacs_hip_bbroadcast(size_t n, int R, const double* x, double* X, hipStream_t stream);
 int tacs_hip_bpack(size_t n, int R, int j, const double* x, double* X, hipStream_t stream);
 int tacs_hip_bunpack(size_t n, int R, int j, const double* X, double* x, hipStream_t stream);
 
@@ -483,9 +489,15 @@ int tacs_mixed_gmres_set_deflation(tacs_obj ksm, int k);
 tacs_obj tacs_batched_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int nrhs,
                                    int prec);
 int tacs_batched_gmres_set_tolerances(tacs_obj ksm, double rtol, double atol);
+/* Krylov subspace recycling per right-hand side for the batched GMRES
+   (GCRO-DR: each system keeps a k-dimensional space updated every cycle
+   from its harmonic Ritz vectors; 0 <= k <= 31, k < m - 1, m + k <= 250;
+   0 turns it off).  No counterpart in the reference. */
+int tacs_batched_gmres_set_deflation(tacs_obj ksm, int k);
 /* Deflate the batched solves by the recycled space of a mixed GMRES with
    deflation over the same matrix (copied; dropped when the matrix values
-   change).  Returns its dimension, 0 when src has none, -1 on error. */
+   change); with per right-hand side recycling on, it seeds every system's
+   space.  Returns its dimension, 0 when src has none, -1 on error. */
 int tacs_batched_gmres_set_recycle_space(tacs_obj ksm, tacs_obj src);
 int tacs_batched_gmres_solve(tacs_obj ksm, int nrhs, tacs_obj* b, tacs_obj* x, int zero_guess);
 int tacs_batched_gmres_solve_block(tacs_obj ksm, int nrhs, const double* B, double* X,

@@ -162,8 +162,9 @@ class MixedKSM(KSM):
     Hessenberg; the correction is accumulated in FP64.  No counterpart in the
     reference.
 
-    deflation=k > 0 turns on deflated restarts (GMRES-DR): each restart keeps
-    the k harmonic Ritz vectors with the smallest harmonic Ritz values."""
+    deflation=k > 0 turns on Krylov subspace recycling (GCRO-DR): each cycle
+    keeps the k harmonic Ritz vectors with the smallest harmonic Ritz values,
+    and the space carries over to later solves with the same matrix."""
 
     def __init__(self, mat, pc, m=30, restarts=1000, deflation=0):
         TACSHandle.__init__(self, lib().tacs_mixed_gmres_create(mat.ptr, pc.ptr, m, restarts),
@@ -187,10 +188,18 @@ class BatchedKSM(TACSHandle):
     cycle restarts from the FP64 residual.  No counterpart in the reference,
     which runs one KSM.solve per load case."""
 
-    def __init__(self, mat, pc, m=30, restarts=100, nrhs=16, prec=64):
+    def __init__(self, mat, pc, m=30, restarts=100, nrhs=16, prec=64, deflation=0):
         super().__init__(lib().tacs_batched_gmres_create(mat.ptr, pc.ptr, m, restarts, nrhs,
                                                          prec), keep=(mat, pc))
         self.nrhs = nrhs
+        if deflation:
+            self.setDeflation(deflation)
+
+    def setDeflation(self, k):
+        """Krylov subspace recycling per right-hand side (GCRO-DR): each system
+        keeps a k-dimensional space, updated every cycle from its harmonic
+        Ritz vectors and kept across solves with the same matrix values."""
+        check(lib().tacs_batched_gmres_set_deflation(self.ptr, int(k)), "BatchedKSM.setDeflation")
 
     def setTolerances(self, rtol, atol):
         lib().tacs_batched_gmres_set_tolerances(self.ptr, rtol, atol)

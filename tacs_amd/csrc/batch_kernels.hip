@@ -214,45 +214,143 @@ __global__ void bdiag_apply3(int nrows, int R, const TD* __restrict__ Dinv,
                                         (double)d[3 * r + 2] * x2;
 }
 
-// Multi right-hand-side 3x3 SpMM with FP32 values and FP64 block vectors:
-// one wavefront per block row, lane = (right-hand side, row of the block).
-__global__ __launch_bounds__(256) void bsr3_spmm_mixed(int nrows, int nrhs,
+// Multi right-hand-side 3x3 SpMM, Y = A X, over interleaved block vectors
+// (X[(3*node + c)*R + j]), FP64 (TA = double) or FP32 (TA = float) values,
+// FP64 vectors and accumulation.  One wavefront per block row; the 64 lanes
+// are G = 64 / RP slots of RP >= R lanes: lane (slot g, right-hand side j)
+// takes blocks g, g + G, ... of the row, so each step of the wave reads G
+// consecutive blocks (coalesced) and, per block, the 3R contiguous entries of
+// X at its column node; the three row sums are reduced over the slots.
+template <typename TA, int RP>
+__global__ __launch_bounds__(256) void bsr3_spmm_slots(int nrows, int R,
                                                        const int* __restrict__ rowp,
                                                        const int* __restrict__ cols,
-                                                       const float* __restrict__ A,
+                                                       const TA* __restrict__ A,
                                                        const double* __restrict__ X,
                                                        double* __restrict__ Y) {
-  __shared__ float blk[4][9 * 8];
+  constexpr int G = 64 / RP;
   const int lane = threadIdx.x & 63;
+  const int j = lane % RP, g = lane / RP;
+  const bool okj = j < R;
+  // XCD-aware rows: workgroups go round-robin to the 8 XCDs, so XCD x
+  // (blockIdx % 8) takes the contiguous eighth x of the rows and its L2 holds
+  // the X entries its neighbouring rows share
   const int wid = threadIdx.x >> 6;
-  const int nw = gridDim.x * 4;
-  for (int row = blockIdx.x * 4 + wid; row < nrows; row += nw) {
-    const int rs = rowp[row];
-    const int nb = rowp[row + 1] - rs;
-    for (int j0 = 0; j0 < nrhs; j0 += 21) {
-      const int jj = lane / 3, r = lane - 3 * jj;
-      const int j = j0 + jj;
-      const bool ok = (lane < 63) && (j < nrhs);
-      double acc = 0.0;
-      for (int s0 = 0; s0 < nb; s0 += 8) {
-        const int nbs = min(8, nb - s0);
-        for (int k = lane; k < nbs * 9; k += 64) blk[wid][k] = A[(size_t)(rs + s0) * 9 + k];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (ok) {
-          for (int s = 0; s < nbs; s++) {
-            int col = cols[rs + s0 + s];
-            const float* a = &blk[wid][9 * s + 3 * r];
-            const double* xx = &X[(size_t)3 * col * nrhs + j];
-            acc += (double)a[0] * xx[0] + (double)a[1] * xx[nrhs] + (double)a[2] * xx[2 * nrhs];
-          }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+  const int xcd = blockIdx.x & 7, nbx = gridDim.x >> 3;
+  const int r0 = (int)((long)nrows * xcd / 8), r1 = (int)((long)nrows * (xcd + 1) / 8);
+  for (int row = r0 + (blockIdx.x >> 3) * 4 + wid; row < r1; row += nbx * 4) {
+    const int rs = rowp[row], re = rowp[row + 1];
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    if (okj) {
+      for (int s = rs + g; s < re; s += G) {
+        const int col = cols[s];
+        const TA* a = &A[(size_t)9 * s];
+        const double* xx = &X[(size_t)3 * col * R + j];
+        const double x0 = xx[0], x1 = xx[R], x2 = xx[2 * R];
+        y0 += (double)a[0] * x0 + (double)a[1] * x1 + (double)a[2] * x2;
+        y1 += (double)a[3] * x0 + (double)a[4] * x1 + (double)a[5] * x2;
+        y2 += (double)a[6] * x0 + (double)a[7] * x1 + (double)a[8] * x2;
       }
-      if (ok) Y[(size_t)(3 * row + r) * nrhs + j] = acc;
+    }
+#pragma unroll
+    for (int off = RP; off < 64; off <<= 1) {
+      y0 += __shfl_xor(y0, off, 64);
+      y1 += __shfl_xor(y1, off, 64);
+      y2 += __shfl_xor(y2, off, 64);
+    }
+    if (g == 0 && okj) {
+      double* y = &Y[(size_t)3 * row * R + j];
+      y[0] = y0;
+      y[R] = y1;
+      y[2 * R] = y2;
     }
   }
+}
+
+template <typename TA>
+static int spmm_slots_launch(int nrows, int R, const int* rowp, const int* cols, const TA* A,
+                             const double* X, double* Y, hipStream_t stream) {
+  if (nrows <= 0 || R <= 0) return 0;
+  if (R > 64) return TACS_HIP_ERR_ARG;
+  // a multiple of 8 workgroups (one share per XCD)
+  const int grid = 8 * tacs_grid_for((nrows + 7) / 8, 4, 8192);
+#define SPMM_CASE(RPV)                                                                          \
+  hipLaunchKernelGGL((bsr3_spmm_slots<TA, RPV>), dim3(grid), dim3(256), 0, stream, nrows, R, rowp, \
+                     cols, A, X, Y)
+  switch (pow2_ceil(R)) {
+    case 1: SPMM_CASE(1); break;
+    case 2: SPMM_CASE(2); break;
+    case 4: SPMM_CASE(4); break;
+    case 8: SPMM_CASE(8); break;
+    case 16: SPMM_CASE(16); break;
+    case 32: SPMM_CASE(32); break;
+    default: SPMM_CASE(64); break;
+  }
+#undef SPMM_CASE
+  return (int)hipGetLastError();
+}
+
+// Per right-hand side basis change: out_c(:, j) (+)= sum_{v<nin} W_v(:, j)
+// P_j[v + c*ldp], c < nout <= BBC_MAXOUT, with P_j = P + j*sP (the recycled
+// space update of the batched GCRO-DR).  A block of 1024 threads owns 1024
+// consecutive entries; the coefficients of vch (<= 8, at most 64 KB) inputs
+// for every right-hand side are staged in LDS (R x vch x nout), so each
+// multiply reads its coefficient from LDS instead of a scattered global load.
+#define BBC_MAXOUT 32
+#define BBC_THREADS 1024
+__global__ __launch_bounds__(BBC_THREADS) void bbasis_combine(size_t n, int R, int nin,
+                                                              const double* __restrict__ W,
+                                                              size_t ldw,
+                                                              const double* __restrict__ P,
+                                                              size_t sP, int ldp, int nout,
+                                                              double* __restrict__ out, size_t ldo,
+                                                              int add, int vch) {
+  extern __shared__ double Ps[];  // [R][vch][nout]
+  const size_t N = n * R;
+  for (size_t base = (size_t)blockIdx.x * BBC_THREADS; base < N;
+       base += (size_t)gridDim.x * BBC_THREADS) {
+    const size_t t = base + threadIdx.x;
+    const bool ok = t < N;
+    const int j = ok ? (int)(t % R) : 0;
+    double acc[BBC_MAXOUT];
+#pragma unroll
+    for (int c = 0; c < BBC_MAXOUT; c++) acc[c] = 0.0;
+    for (int v0 = 0; v0 < nin; v0 += vch) {
+      const int vc = min(vch, nin - v0);
+      __syncthreads();
+      for (int e = threadIdx.x; e < R * vc * nout; e += BBC_THREADS) {
+        const int jj = e / (vc * nout), r = e - jj * (vc * nout);
+        const int v = r / nout, c = r - v * nout;
+        Ps[((size_t)jj * vch + v) * nout + c] = P[(size_t)jj * sP + v0 + v + (size_t)c * ldp];
+      }
+      __syncthreads();
+      if (ok) {
+        for (int v = 0; v < vc; v++) {
+          const double w = W[(size_t)(v0 + v) * ldw + t];
+          const double* pp = &Ps[((size_t)j * vch + v) * nout];
+#pragma unroll
+          for (int c = 0; c < BBC_MAXOUT; c++)
+            if (c < nout) acc[c] += w * pp[c];
+        }
+      }
+    }
+    if (ok) {
+#pragma unroll
+      for (int c = 0; c < BBC_MAXOUT; c++)
+        if (c < nout) {
+          double* o = &out[(size_t)c * ldo + t];
+          *o = add ? *o + acc[c] : acc[c];
+        }
+    }
+  }
+}
+
+// X[i*R + j] = x[i] for every j
+__global__ void bbroadcast(size_t n, int R, const double* __restrict__ x, double* __restrict__ X) {
+  const size_t N = n * R;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < N;
+       t += (size_t)gridDim.x * blockDim.x)
+    X[t] = x[t / R];
 }
 
 // X[i*R + j] = x[i] (pack) or x[i] = X[i*R + j] (unpack)
@@ -395,10 +493,30 @@ int tacs_hip_bdiag_apply3(int prec, int nrows, int R, const void* Dinv, const do
 
 int tacs_hip_bsr3_spmm_mixed(int nrows, int nrhs, const int* rowp, const int* cols, const float* A,
                              const double* X, double* Y, hipStream_t stream) {
-  if (nrows <= 0 || nrhs <= 0) return 0;
-  int grid = tacs_grid_for(nrows, 4, 65536);
-  hipLaunchKernelGGL(bsr3_spmm_mixed, dim3(grid), dim3(256), 0, stream, nrows, nrhs, rowp, cols,
-                     A, X, Y);
+  return spmm_slots_launch<float>(nrows, nrhs, rowp, cols, A, X, Y, stream);
+}
+
+int tacs_hip_bsr3_spmm(int nrows, int nrhs, const int* rowp, const int* cols, const double* A,
+                       const double* X, double* Y, hipStream_t stream) {
+  return spmm_slots_launch<double>(nrows, nrhs, rowp, cols, A, X, Y, stream);
+}
+
+int tacs_hip_bbasis_combine(size_t n, int R, int nin, const double* W, size_t ldw, const double* P,
+                            size_t sP, int ldp, int nout, double* out, size_t ldo, int add,
+                            hipStream_t stream) {
+  if (nout < 1 || nout > BBC_MAXOUT || nin < 0 || R < 1 || R > 64) return TACS_HIP_ERR_ARG;
+  int vch = (int)(65536 / ((size_t)R * nout * sizeof(double)));  // >= 4
+  if (vch > 8) vch = 8;
+  const size_t lds = (size_t)R * vch * nout * sizeof(double);
+  size_t g = (n * R + BBC_THREADS - 1) / BBC_THREADS;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(bbasis_combine, dim3((int)g), dim3(BBC_THREADS), lds, stream, n, R, nin, W,
+                     ldw, P, sP, ldp, nout, out, ldo, add, vch);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_bbroadcast(size_t n, int R, const double* x, double* X, hipStream_t stream) {
+  hipLaunchKernelGGL(bbroadcast, dim3(bgrid(n * R)), dim3(256), 0, stream, n, R, x, X);
   return (int)hipGetLastError();
 }
 

@@ -147,49 +147,6 @@ __global__ __launch_bounds__(256) void bsr_spmv_generic(int nrows, const int* __
 }
 
 // ---------------------------------------------------------------------------
-// Multi-RHS 3x3 SpMM: Y[:, j] = A X[:, j] for nrhs right-hand sides stored
-// node-major, X[(3*node + c)*nrhs + j].  The matrix is read once for all
-// right-hand sides.  One wavefront per block row; lane = (rhs j, row r).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void bsr3_spmm(int nrows, int nrhs, const int* __restrict__ rowp,
-                                                 const int* __restrict__ cols,
-                                                 const double* __restrict__ A,
-                                                 const double* __restrict__ X,
-                                                 double* __restrict__ Y) {
-  __shared__ double blk[4][2][9 * 8];
-  const int lane = threadIdx.x & 63;
-  const int wid = threadIdx.x >> 6;
-  const int nw = gridDim.x * 4;
-  for (int row = blockIdx.x * 4 + wid; row < nrows; row += nw) {
-    const int rs = rowp[row];
-    const int nb = rowp[row + 1] - rs;
-    for (int j0 = 0; j0 < nrhs; j0 += 21) {
-      // lane -> (rhs jj < 21, r < 3)
-      const int jj = lane / 3, r = lane - 3 * jj;
-      const int j = j0 + jj;
-      const bool ok = (lane < 63) && (j < nrhs);
-      double acc = 0.0;
-      for (int s0 = 0; s0 < nb; s0 += 8) {
-        const int nbs = min(8, nb - s0);
-        // stage up to 8 blocks in LDS with coalesced loads
-        for (int k = lane; k < nbs * 9; k += 64) blk[wid][0][k] = A[(size_t)(rs + s0) * 9 + k];
-        WAVE_SYNC();
-        if (ok) {
-          for (int s = 0; s < nbs; s++) {
-            int col = cols[rs + s0 + s];
-            const double* a = &blk[wid][0][9 * s + 3 * r];
-            const double* xx = &X[(size_t)3 * col * nrhs + j];
-            acc += a[0] * xx[0] + a[1] * xx[nrhs] + a[2] * xx[2 * nrhs];
-          }
-        }
-        WAVE_SYNC();
-      }
-      if (ok) Y[(size_t)(3 * row + r) * nrhs + j] = acc;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Diagonal-block positions, block-Jacobi inverse and application.
 // ---------------------------------------------------------------------------
 __global__ void bsr_find_diag(int nrows, const int* __restrict__ rowp,
@@ -369,15 +326,6 @@ int tacs_hip_bsr_diag_apply_mixed(int b, int nrows, const float* Dinv, const dou
     hipLaunchKernelGGL((bsr_diag_apply<6, float, double>), g, t, 0, stream, nrows, Dinv, x, y);
   else
     return TACS_HIP_ERR_ARG;
-  return (int)hipGetLastError();
-}
-
-int tacs_hip_bsr3_spmm(int nrows, int nrhs, const int* rowp, const int* cols, const double* A,
-                       const double* X, double* Y, hipStream_t stream) {
-  if (nrows <= 0 || nrhs <= 0) return 0;
-  int grid = tacs_grid_for(nrows, 4, 65536);
-  hipLaunchKernelGGL(bsr3_spmm, dim3(grid), dim3(256), 0, stream, nrows, nrhs, rowp, cols, A, X,
-                     Y);
   return (int)hipGetLastError();
 }
 

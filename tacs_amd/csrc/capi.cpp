@@ -468,6 +468,11 @@ int tacs_batched_gmres_set_tolerances(tacs_obj ksm, double rtol, double atol) {
   k->setTolerances(rtol, atol);
   return 0;
 }
+int tacs_batched_gmres_set_deflation(tacs_obj ksm, int k) {
+  clear_error();
+  REQUIRE(TACSBatchedGMRES, g, ksm);
+  return g->setDeflation(k);
+}
 int tacs_batched_gmres_set_recycle_space(tacs_obj ksm, tacs_obj src) {
   clear_error();
   TACSBatchedGMRES* k = OBJ(TACSBatchedGMRES, ksm);

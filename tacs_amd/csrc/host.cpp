@@ -914,6 +914,101 @@ static int dense_solve_rowmajor(int n, const double* Ain, double* B, int nrhs) {
     }
   return 0;
 }
+
+// New recycle space of a GCRO-DR cycle from its harmonic Ritz vectors (Parks
+// et al. 2006, sec. 2.4).  ls holds the cycle's plain Hessenberg Gbar
+// (nrw = kd + jend + 1 rows [C; V_0..V_jend], nc = kd + jend columns
+// [U D_u, V_0..V_jend-1]); CVtU ((kd + jend + 1) x kd, column b) holds
+// [C, V_0..V_jend]^T U_b with the unscaled U.  The K harmonic Ritz vectors of
+// smallest harmonic Ritz value, P_k, give Q R = Gbar P_k; C_new = [C V] Q and
+// U_new = [U V] P_k R^{-1}.  Returns kk (the new dimension; 0: keep the old
+// space) with Qm (nrw x kk), PRu (kd x kk, D_u folded in) and PRv (jend x kk),
+// all column-major.
+static int recycle_update(const HostLS& ls, int kd, int jend, int K, const double* dscale,
+                          const double* CVtU, std::vector<double>& Qm, std::vector<double>& PRu,
+                          std::vector<double>& PRv) {
+  const int nc = kd + jend;
+  const int nrw = nc + 1;
+  const int ld = ls.ld;
+  // Gbar^T Gbar and What^T W with What = [C, V_0..V_jend], W = [U D_u, V_0..V_jend-1]
+  std::vector<double> GtG((size_t)nc * nc, 0.0), WtW((size_t)nrw * nc, 0.0);
+  for (int a = 0; a < nc; a++)
+    for (int bb = 0; bb < nc; bb++) {
+      double acc = 0.0;
+      for (int r = 0; r < nrw; r++) acc += ls.Hb[(size_t)a * ld + r] * ls.Hb[(size_t)bb * ld + r];
+      GtG[(size_t)a * nc + bb] = acc;  // row-major
+    }
+  for (int i = 0; i < jend; i++) WtW[(size_t)(kd + i) * nrw + (kd + i)] = 1.0;  // V^T V
+  for (int i = 0; i < kd; i++)
+    for (int r = 0; r < nrw; r++) WtW[(size_t)i * nrw + r] = CVtU[(size_t)i * nrw + r] * dscale[i];
+  std::vector<double> GtW((size_t)nc * nc, 0.0);  // row-major Gbar^T (What^T W)
+  for (int a = 0; a < nc; a++)
+    for (int bb = 0; bb < nc; bb++) {
+      double acc = 0.0;
+      for (int r = 0; r < nrw; r++) acc += ls.Hb[(size_t)a * ld + r] * WtW[(size_t)bb * nrw + r];
+      GtW[(size_t)a * nc + bb] = acc;
+    }
+  // G = (Gbar^T What^T W)^{-1} Gbar^T Gbar: harmonic Ritz values theta, vectors z
+  std::vector<double> Gm = GtG;
+  if (dense_solve_rowmajor(nc, GtW.data(), Gm.data(), nc) != 0) return 0;
+  std::vector<double> Z((size_t)nc * (K + 1), 0.0);
+  int kout = tacs_dense_smallest_eigvecs(nc, Gm.data(), K, Z.data(), NULL, NULL);
+  if (kout <= 0) return 0;
+  if (kout > K) kout = K;
+  // Q R = Gbar P_k (nrw x kout)
+  std::vector<double> GP((size_t)nrw * kout, 0.0), R((size_t)kout * kout, 0.0);
+  for (int jj = 0; jj < kout; jj++)
+    for (int l = 0; l < nc; l++) {
+      double pv = Z[(size_t)jj * nc + l];
+      for (int r = 0; r < nrw; r++) GP[(size_t)jj * nrw + r] += ls.Hb[(size_t)l * ld + r] * pv;
+    }
+  int kk = 0;
+  std::vector<int> keep;
+  for (int jj = 0; jj < kout; jj++) {
+    // modified Gram-Schmidt (twice) of GP column jj against the kept ones
+    double* g = &GP[(size_t)jj * nrw];
+    double nrm0 = 0.0;
+    for (int r = 0; r < nrw; r++) nrm0 += g[r] * g[r];
+    nrm0 = sqrt(nrm0);
+    std::vector<double> rcol(kout, 0.0);
+    for (int pass = 0; pass < 2; pass++)
+      for (int q = 0; q < kk; q++) {
+        const double* gq = &GP[(size_t)keep[q] * nrw];
+        double d = 0.0;
+        for (int r = 0; r < nrw; r++) d += gq[r] * g[r];
+        for (int r = 0; r < nrw; r++) g[r] -= d * gq[r];
+        rcol[q] += d;
+      }
+    double nrm = 0.0;
+    for (int r = 0; r < nrw; r++) nrm += g[r] * g[r];
+    nrm = sqrt(nrm);
+    if (!(nrm > 1e-10 * nrm0)) continue;
+    for (int r = 0; r < nrw; r++) g[r] /= nrm;
+    rcol[kk] = nrm;
+    for (int q = 0; q <= kk; q++) R[(size_t)kk * kout + q] = rcol[q];  // column kk of R
+    keep.push_back(jj);
+    kk++;
+  }
+  if (kk == 0) return 0;
+  // P R^{-1} (nc x kk): the upper-triangular system column by column
+  std::vector<double> PR((size_t)nc * kk, 0.0);
+  for (int jj = 0; jj < kk; jj++) {
+    for (int l = 0; l < nc; l++) {
+      double v = Z[(size_t)keep[jj] * nc + l];
+      for (int q = 0; q < jj; q++) v -= PR[(size_t)q * nc + l] * R[(size_t)jj * kout + q];
+      PR[(size_t)jj * nc + l] = v / R[(size_t)jj * kout + jj];
+    }
+  }
+  Qm.assign((size_t)nrw * kk, 0.0);
+  PRu.assign((size_t)std::max(kd, 1) * kk, 0.0);
+  PRv.assign((size_t)std::max(jend, 1) * kk, 0.0);
+  for (int jj = 0; jj < kk; jj++) {
+    for (int r = 0; r < nrw; r++) Qm[(size_t)jj * nrw + r] = GP[(size_t)keep[jj] * nrw + r];
+    for (int i = 0; i < kd; i++) PRu[(size_t)jj * kd + i] = PR[(size_t)jj * nc + i] * dscale[i];
+    for (int i = 0; i < jend; i++) PRv[(size_t)jj * jend + i] = PR[(size_t)jj * nc + kd + i];
+  }
+  return kk;
+}
 }  // namespace
 
 int TACSMixedGMRES::solveDeflated(TACSVec* tb, TACSVec* tx, int zero_guess) {
@@ -1034,93 +1129,17 @@ int TACSMixedGMRES::solveDeflated(TACSVec* tb, TACSVec* tx, int zero_guess) {
 
     // ---- new recycle space from the harmonic Ritz vectors of this cycle
     if (nc <= K + 1) continue;  // too few columns to choose from: keep the old space
-    const int nrw = nc + 1;     // rows of Gbar: [C (kd) ; V_0..V_jend]
-    const int ld = ls.ld;
-    // Gbar^T Gbar and Gbar^T (What^T W)
-    std::vector<double> GtG((size_t)nc * nc, 0.0), WtW((size_t)nrw * nc, 0.0);
-    for (int a = 0; a < nc; a++)
-      for (int bb = 0; bb < nc; bb++) {
-        double acc = 0.0;
-        for (int r = 0; r < nrw; r++) acc += ls.Hb[(size_t)a * ld + r] * ls.Hb[(size_t)bb * ld + r];
-        GtG[(size_t)a * nc + bb] = acc;  // row-major
-      }
-    // What = [C, V_0..V_jend], W = [U D_u, V_0..V_{jend-1}]
-    for (int i = 0; i < jend; i++) WtW[(size_t)(kd + i) * nrw + (kd + i)] = 1.0;  // V^T V
+    // [C, V_0..V_jend]^T U (C and V contiguous)
+    tmpv.assign((size_t)(kd + jend + 1) * std::max(kd, 1), 0.0);
     if (kd > 0) {
-      // [C, V_0..V_jend]^T U (C and V contiguous), scaled by D_u
       KCHECK(tacs_hip_tsgemm(n, kd + jend + 1, Cm, kd, U, d_tsg, dmat, g_stream));
-      tmpv.resize((size_t)(kd + jend + 1) * kd);
-      HCHECK(hipMemcpy(tmpv.data(), dmat, tmpv.size() * sizeof(double), hipMemcpyDeviceToHost));
-      for (int i = 0; i < kd; i++)
-        for (int r = 0; r < kd + jend + 1; r++)
-          WtW[(size_t)i * nrw + r] = tmpv[(size_t)i * (kd + jend + 1) + r] * dscale[i];
+      HCHECK(hipMemcpy(tmpv.data(), dmat, (size_t)(kd + jend + 1) * kd * sizeof(double),
+                       hipMemcpyDeviceToHost));
     }
-    std::vector<double> GtW((size_t)nc * nc, 0.0);  // row-major Gbar^T (What^T W)
-    for (int a = 0; a < nc; a++)
-      for (int bb = 0; bb < nc; bb++) {
-        double acc = 0.0;
-        for (int r = 0; r < nrw; r++) acc += ls.Hb[(size_t)a * ld + r] * WtW[(size_t)bb * nrw + r];
-        GtW[(size_t)a * nc + bb] = acc;
-      }
-    // G = (Gbar^T What^T W)^{-1} Gbar^T Gbar: harmonic Ritz values theta, vectors z
-    std::vector<double> Gm = GtG;
-    if (dense_solve_rowmajor(nc, GtW.data(), Gm.data(), nc) != 0) continue;
-    std::vector<double> Z((size_t)nc * (K + 1), 0.0);
-    int kout = tacs_dense_smallest_eigvecs(nc, Gm.data(), K, Z.data(), NULL, NULL);
-    if (kout <= 0) continue;
-    if (kout > K) kout = K;
-    // Q R = Gbar P_k (nrw x kout);  C_new = What Q;  U_new = W P_k R^{-1}
-    std::vector<double> GP((size_t)nrw * kout, 0.0), R((size_t)kout * kout, 0.0);
-    for (int jj = 0; jj < kout; jj++)
-      for (int l = 0; l < nc; l++) {
-        double pv = Z[(size_t)jj * nc + l];
-        for (int r = 0; r < nrw; r++) GP[(size_t)jj * nrw + r] += ls.Hb[(size_t)l * ld + r] * pv;
-      }
-    int kk = 0;
-    std::vector<int> keep;
-    for (int jj = 0; jj < kout; jj++) {
-      // modified Gram-Schmidt (twice) of GP column jj against the kept ones
-      double* g = &GP[(size_t)jj * nrw];
-      double nrm0 = 0.0;
-      for (int r = 0; r < nrw; r++) nrm0 += g[r] * g[r];
-      nrm0 = sqrt(nrm0);
-      std::vector<double> rcol(kout, 0.0);
-      for (int pass = 0; pass < 2; pass++)
-        for (int q = 0; q < kk; q++) {
-          const double* gq = &GP[(size_t)keep[q] * nrw];
-          double d = 0.0;
-          for (int r = 0; r < nrw; r++) d += gq[r] * g[r];
-          for (int r = 0; r < nrw; r++) g[r] -= d * gq[r];
-          rcol[q] += d;
-        }
-      double nrm = 0.0;
-      for (int r = 0; r < nrw; r++) nrm += g[r] * g[r];
-      nrm = sqrt(nrm);
-      if (!(nrm > 1e-10 * nrm0)) continue;
-      for (int r = 0; r < nrw; r++) g[r] /= nrm;
-      rcol[kk] = nrm;
-      for (int q = 0; q <= kk; q++) R[(size_t)kk * kout + q] = rcol[q];  // column kk of R
-      keep.push_back(jj);
-      kk++;
-    }
+    std::vector<double> Qm, PRu, PRv;
+    const int nrw = nc + 1;
+    int kk = recycle_update(ls, kd, jend, K, dscale.data(), tmpv.data(), Qm, PRu, PRv);
     if (kk == 0) continue;
-    // P R^{-1} (nc x kk): solve the upper-triangular system column by column
-    std::vector<double> PR((size_t)nc * kk, 0.0);
-    for (int jj = 0; jj < kk; jj++) {
-      for (int l = 0; l < nc; l++) {
-        double v = Z[(size_t)keep[jj] * nc + l];
-        for (int q = 0; q < jj; q++) v -= PR[(size_t)q * nc + l] * R[(size_t)jj * kout + q];
-        PR[(size_t)jj * nc + l] = v / R[(size_t)jj * kout + jj];
-      }
-    }
-    // upload: Q (nrw x kk) for C_new = [C, V] Q; PRu (kd x kk, scaled by D_u) and PRv (jend x kk)
-    std::vector<double> Qm((size_t)nrw * kk), PRu((size_t)std::max(kd, 1) * kk, 0.0),
-        PRv((size_t)jend * kk);
-    for (int jj = 0; jj < kk; jj++) {
-      for (int r = 0; r < nrw; r++) Qm[(size_t)jj * nrw + r] = GP[(size_t)keep[jj] * nrw + r];
-      for (int i = 0; i < kd; i++) PRu[(size_t)jj * kd + i] = PR[(size_t)jj * nc + i] * dscale[i];
-      for (int i = 0; i < jend; i++) PRv[(size_t)jj * jend + i] = PR[(size_t)jj * nc + kd + i];
-    }
     double* Cn = d_T;
     double* Un = d_T + (size_t)K * n;
     HCHECK(hipMemcpy(dmat, Qm.data(), Qm.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -1162,7 +1181,9 @@ int TACSMixedGMRES::solveDeflated(TACSVec* tb, TACSVec* tx, int zero_guess) {
 TACSBatchedGMRES::TACSBatchedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int _m, int _nrestart,
                                    int nrhs, int _prec)
     : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart), R(nrhs), prec(_prec), cycles(0),
-      kdefl(0), kVersion(-1), d_U(NULL), d_C(NULL), mat(m), pc(p) {
+      kdefl(0), kVersion(-1), d_U(NULL), d_C(NULL), ndefl(0), defVersion(-1), defR(0),
+      d_Ub(NULL), d_Cb(NULL), d_Unb(NULL), d_Cnb(NULL), d_prod(NULL), h_stage(NULL), mat(m),
+      pc(p) {
   mat->incref();
   pc->incref();
   if (prec == 32) mat->getValues32(0);
@@ -1181,9 +1202,11 @@ TACSBatchedGMRES::TACSBatchedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int
 }
 TACSBatchedGMRES::~TACSBatchedGMRES() {
   HCHECK(hipStreamSynchronize(g_stream));
-  void* bufs[] = {d_V, d_work, d_Bb, d_Xb, d_scal, d_vec, d_cnt, d_red, d_U, d_C};
+  void* bufs[] = {d_V,  d_work, d_Bb, d_Xb, d_scal, d_vec,  d_cnt, d_red,
+                  d_U,  d_C,    d_Ub, d_Cb, d_Unb,  d_Cnb,  d_prod};
   for (void* q : bufs) dfree(q);
   HCHECK(hipHostFree(h_pin));
+  if (h_stage) HCHECK(hipHostFree(h_stage));
   mat->decref();
   pc->decref();
 }
@@ -1220,6 +1243,35 @@ int TACSBatchedGMRES::setRecycleSpace(TACSMixedGMRES* src) {
   return k;
 }
 
+int TACSBatchedGMRES::setDeflation(int k) {
+  if (k < 0 || k > 31 || (k > 0 && (k >= msub - 1 || msub + k > 250))) {
+    tacs_set_error("setDeflation: need 0 <= k <= 31, k < m - 1 and m + k <= 250 (k = %d, m = %d)",
+                   k, msub);
+    return -1;
+  }
+  HCHECK(hipStreamSynchronize(g_stream));
+  void* bufs[] = {d_Ub, d_Cb, d_Unb, d_Cnb, d_prod};
+  for (void* q : bufs) dfree(q);
+  d_Ub = d_Cb = d_Unb = d_Cnb = d_prod = NULL;
+  if (h_stage) HCHECK(hipHostFree(h_stage));
+  h_stage = NULL;
+  ndefl = k;
+  defVersion = -1;
+  defR = 0;
+  kdj.assign(R, 0);
+  dsc.assign((size_t)R * KD_MAX, 0.0);
+  if (k > 0) {
+    size_t N = (size_t)mat->nrows * mat->bsize * R;
+    d_Ub = dalloc<double>((size_t)k * N);
+    d_Cb = dalloc<double>((size_t)k * N);
+    d_Unb = dalloc<double>((size_t)k * N);
+    d_Cnb = dalloc<double>((size_t)k * N);
+    d_prod = dalloc<double>((size_t)R * prodStride());
+    HCHECK(hipHostMalloc((void**)&h_stage, (size_t)R * prodStride() * sizeof(double), 0));
+  }
+  return 0;
+}
+
 int TACSBatchedGMRES::solve(int nrhs, TACSBVec** b, TACSBVec** x, int zero_guess) {
   if (nrhs < 1 || nrhs > R) {
     tacs_set_error("TACSBatchedGMRES::solve: nrhs = %d outside [1, %d]", nrhs, R);
@@ -1245,6 +1297,7 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
     tacs_set_error("TACSBatchedGMRES: nrhs = %d outside [1, %d]", Rn, R);
     return -1;
   }
+  if (ndefl > 0) return solveBlockDeflated(Rn, Bd, Xd, zero_guess);
   const int nr = mat->nrows, m = msub;
   const size_t n = (size_t)nr * 3, N = n * Rn, S = slice();
   // per right-hand side slice (stride S)
@@ -1409,6 +1462,343 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
       KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, d_work, V(1), g_stream));
       KCHECK(tacs_hip_bscale(n, Rn, nrm, V(1), g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, V(1), 1.0, Xd, g_stream));
+    }
+    spmm64(Xd, V(0));
+    KCHECK(tacs_hip_axpby(64, 64, N, 1.0, Bd, -1.0, V(0), g_stream));
+  }
+  HCHECK(hipStreamSynchronize(g_stream));
+  int ok = 1;
+  for (int j = 0; j < Rn; j++) ok &= flags[j];
+  return ok;
+}
+
+// ---------------------------------------------------------------------------
+// Batched GCRO-DR: the cycle of TACSMixedGMRES::solveDeflated for R
+// right-hand sides at once.  Right-hand side j keeps its own recycled space
+// (U_j, C_j) as interleaved block vectors (columns beyond its dimension k_j
+// are zero), its own least-squares problem and its own harmonic Ritz update
+// on the host; the SpMM, the Gram-Schmidt products against [C_j, V_j], the
+// corrections and the basis changes run batched over all right-hand sides.
+// The Arnoldi operator is A M^{-1} with the FP64 (prec 64) or FP32-rounded
+// (prec 32) matrix and block-Jacobi values, in FP64 arithmetic, so
+// A M^{-1} U_j = C_j holds across cycles and solves.
+// ---------------------------------------------------------------------------
+int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, int zero_guess) {
+  const int nr = mat->nrows, m = msub, K = ndefl;
+  const size_t n = (size_t)nr * 3, N = n * Rn, S = slice(), SP = prodStride();
+  double* H = d_scal;
+  double* res = d_scal + (size_t)(m + 1) * m;
+  double* tmp = res + (m + 1) + 2 * (size_t)m;
+  double* Bc = tmp + (m + 1);  // KD_MAX x m, column i at Bc + i*KD_MAX
+  double* tmpC = Bc + (size_t)KD_MAX * m;
+  double* zc = tmpC + KD_MAX;
+  double* pco = zc + KD_MAX;
+  double *nrm2 = d_vec, *nrm = d_vec + 64, *inv = d_vec + 128;
+  double* h_inv = h_pin;
+  auto V = [&](int i) { return d_V + (size_t)i * N; };
+  auto Ub = [&](int v) { return d_Ub + (size_t)v * N; };
+  auto Cb = [&](int v) { return d_Cb + (size_t)v * N; };
+  const float* A32 = prec == 32 ? mat->getValues32(1) : NULL;
+  const void* Dinv = prec == 32 ? (const void*)pc->d_Dinv32 : (const void*)pc->d_Dinv;
+  auto spmm64 = [&](const double* x, double* y) {
+    KCHECK(tacs_hip_bsr3_spmm(nr, Rn, mat->d_rowp, mat->d_cols, mat->d_A, x, y, g_stream));
+  };
+  auto op = [&](const double* x, double* y) {
+    KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, x, d_work, g_stream));
+    if (prec == 32)
+      KCHECK(tacs_hip_bsr3_spmm_mixed(nr, Rn, mat->d_rowp, mat->d_cols, A32, d_work, y, g_stream));
+    else
+      spmm64(d_work, y);
+  };
+  auto normalize = [&](double* q, double* out, size_t sOut) {
+    KCHECK(tacs_hip_bmdot(n, Rn, 1, q, N, q, (double*)d_red, out, sOut, 0, g_stream));
+    KCHECK(tacs_hip_bsqrt_inv(Rn, out, sOut, out, sOut, inv, g_stream));
+    KCHECK(tacs_hip_bscale(n, Rn, inv, q, g_stream));
+  };
+  // Rn x cols blocks between host (dense) and device (row stride ds), through
+  // the pinned staging buffer; both wait for the copy
+  auto d2h_rows = [&](double* h, int cols, const double* d, size_t ds) {
+    if (cols <= 0) return;
+    HCHECK(hipMemcpy2DAsync(h_stage, cols * sizeof(double), d, ds * sizeof(double),
+                            cols * sizeof(double), Rn, hipMemcpyDeviceToHost, g_stream));
+    HCHECK(hipStreamSynchronize(g_stream));
+    memcpy(h, h_stage, (size_t)Rn * cols * sizeof(double));
+  };
+  auto h2d_rows = [&](double* d, size_t ds, const double* h, int cols) {
+    if (cols <= 0) return;
+    memcpy(h_stage, h, (size_t)Rn * cols * sizeof(double));
+    HCHECK(hipMemcpy2DAsync(d, ds * sizeof(double), h_stage, cols * sizeof(double),
+                            cols * sizeof(double), Rn, hipMemcpyHostToDevice, g_stream));
+    HCHECK(hipStreamSynchronize(g_stream));
+  };
+  std::vector<double> stage;
+  // dsc[j][v] = 1 / |U_j(:, v)|
+  auto update_dscale = [&](int k) {
+    for (int v = 0; v < k; v++)
+      KCHECK(tacs_hip_bmdot(n, Rn, 1, Ub(v), N, Ub(v), (double*)d_red, d_prod + v, SP, 0, g_stream));
+    stage.assign((size_t)Rn * std::max(k, 1), 0.0);
+    d2h_rows(stage.data(), k, d_prod, SP);
+    for (int j = 0; j < Rn; j++)
+      for (int v = 0; v < KD_MAX; v++) {
+        double d = v < k ? stage[(size_t)j * k + v] : 0.0;
+        dsc[(size_t)j * KD_MAX + v] = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
+      }
+  };
+
+  // the spaces belong to one matrix and one number of right-hand sides
+  if (defVersion != mat->version || defR != Rn) std::fill(kdj.begin(), kdj.end(), 0);
+  defVersion = mat->version;
+  defR = Rn;
+  int Kd = 0;
+  for (int j = 0; j < Rn; j++) Kd = std::max(Kd, kdj[j]);
+  if (Kd == 0 && kdefl > 0 && kVersion == mat->version) {
+    // start every right-hand side from the shared space
+    Kd = std::min(kdefl, K);
+    for (int v = 0; v < Kd; v++) {
+      KCHECK(tacs_hip_bbroadcast(n, Rn, d_U + (size_t)v * n, Ub(v), g_stream));
+      KCHECK(tacs_hip_bbroadcast(n, Rn, d_C + (size_t)v * n, Cb(v), g_stream));
+    }
+    for (int j = 0; j < Rn; j++) kdj[j] = Kd;
+    update_dscale(Kd);
+  }
+
+  std::vector<char> active(Rn, 1), fail(Rn, 0), conv(Rn);
+  std::vector<double> target(Rn), beta(Rn), tgt(Rn), hb, hh, hc((size_t)KD_MAX + m + 2);
+  std::vector<int> jend(Rn);
+  std::vector<HostLS> ls(Rn, HostLS(m + 1));
+  iters.assign(Rn, 0);
+  flags.assign(Rn, 0);
+  r0.assign(Rn, 0.0);
+  rn.assign(Rn, 0.0);
+  cycles = 0;
+  if (zero_guess) {
+    KCHECK(tacs_hip_fill(64, N, Xd, 0.0, g_stream));
+    HCHECK(hipMemcpyAsync(V(0), Bd, N * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
+  } else {
+    spmm64(Xd, V(0));
+    KCHECK(tacs_hip_axpby(64, 64, N, 1.0, Bd, -1.0, V(0), g_stream));
+  }
+  for (int count = 0;; count++) {
+    KCHECK(tacs_hip_bmdot(n, Rn, 1, V(0), N, V(0), (double*)d_red, nrm2, 1, 0, g_stream));
+    KCHECK(tacs_hip_bsqrt_inv(Rn, nrm2, 1, nrm, 1, inv, g_stream));
+    d2h(h_pin + 256, nrm, Rn);
+    int nact = 0;
+    for (int j = 0; j < Rn; j++) {
+      if (!active[j]) continue;
+      double r = h_pin[256 + j];
+      if (count == 0) r0[j] = r;
+      rn[j] = r;
+      if (!std::isfinite(r) || fail[j]) {
+        active[j] = 0;
+      } else if (r <= rtol * r0[j] || r < atol) {
+        active[j] = 0;
+        flags[j] = 1;
+      } else {
+        nact++;
+      }
+    }
+    if (nact == 0 || count > nrestart) break;
+    cycles++;
+    for (int j = 0; j < Rn; j++) {
+      h_inv[j] = active[j] ? 1.0 / rn[j] : 0.0;
+      target[j] = active[j] ? std::max(rtol * r0[j], atol) / rn[j] : 0.0;
+      conv[j] = !active[j];
+      jend[j] = 0;
+    }
+    h2d(inv, h_inv, Rn);
+    KCHECK(tacs_hip_bscale(n, Rn, inv, V(0), g_stream));
+    double* tmp2 = d_Cnb;  // block scratch (holds the new C only at the end of a cycle)
+    if (Kd > 0) {
+      // x += |r| M^{-1} U C^T v_0;  v_0 -= C C^T v_0
+      KCHECK(tacs_hip_bmdot(n, Rn, Kd, Cb(0), N, V(0), (double*)d_red, pco, S, 0, g_stream));
+      KCHECK(tacs_hip_blincomb(n, Rn, Kd, Ub(0), N, pco, S, NULL, d_work, 0, g_stream));
+      KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, d_work, tmp2, g_stream));
+      KCHECK(tacs_hip_bscale(n, Rn, nrm, tmp2, g_stream));
+      KCHECK(tacs_hip_axpby(64, 64, N, 1.0, tmp2, 1.0, Xd, g_stream));
+      KCHECK(tacs_hip_bcgs_update(n, Rn, Kd, Cb(0), N, pco, S, V(0), g_stream));
+    }
+    // beta_j = |(I - C C^T) v_0(:, j)| into res_j[0]; v_0 normalised
+    normalize(V(0), res, S);
+    stage.assign(Rn, 0.0);
+    d2h_rows(stage.data(), 1, res, S);
+    for (int j = 0; j < Rn; j++) {
+      beta[j] = stage[j];
+      if (conv[j]) continue;
+      if (beta[j] <= target[j]) {
+        conv[j] = 1;  // the projection alone converged
+        continue;
+      }
+      // least squares over [U D_u, V]: columns 0..k_j-1 are D_u, then Arnoldi
+      const int kd = kdj[j];
+      HostLS& L = ls[j];
+      L.reset();
+      L.c[kd] = 1.0;
+      L.g[kd] = 1.0;
+      for (int i = 0; i < kd; i++) {
+        std::fill(hc.begin(), hc.end(), 0.0);
+        hc[i] = dsc[(size_t)j * KD_MAX + i];
+        L.add_column(i, hc.data());
+      }
+      tgt[j] = target[j] / beta[j];
+    }
+    const int s = m - Kd;
+    int niters = 0;
+    bool all = true;
+    for (int j = 0; j < Rn; j++) all = all && conv[j];
+    for (int i = 0; i < s && !all; i++) {
+      double* q = V(i + 1);
+      double* h = H + (size_t)i * (m + 1);
+      double* bc = Bc + (size_t)i * KD_MAX;
+      op(V(i), q);
+      // classical Gram-Schmidt, twice, against [C_j, V_0..V_i]
+      for (int pass = 0; pass < 2; pass++) {
+        double* cc = pass == 0 ? bc : tmpC;
+        double* hq = pass == 0 ? h : tmp;
+        if (Kd > 0)
+          KCHECK(tacs_hip_bmdot(n, Rn, Kd, Cb(0), N, q, (double*)d_red, cc, S, 0, g_stream));
+        KCHECK(tacs_hip_bmdot(n, Rn, i + 1, V(0), N, q, (double*)d_red, hq, S, 0, g_stream));
+        if (Kd > 0) KCHECK(tacs_hip_bcgs_update(n, Rn, Kd, Cb(0), N, cc, S, q, g_stream));
+        KCHECK(tacs_hip_bcgs_update(n, Rn, i + 1, V(0), N, hq, S, q, g_stream));
+      }
+      if (Kd > 0) KCHECK(tacs_hip_badd(Rn, Kd, tmpC, S, bc, S, g_stream));
+      KCHECK(tacs_hip_badd(Rn, i + 1, tmp, S, h, S, g_stream));
+      normalize(q, h + i + 1, S);
+      niters = i + 1;
+      hb.assign((size_t)Rn * std::max(Kd, 1), 0.0);
+      hh.assign((size_t)Rn * (i + 2), 0.0);
+      d2h_rows(hb.data(), Kd, bc, S);
+      d2h_rows(hh.data(), i + 2, h, S);
+      all = true;
+      for (int j = 0; j < Rn; j++) {
+        if (conv[j]) continue;
+        const int kd = kdj[j];
+        for (int r = 0; r < kd; r++) hc[r] = hb[(size_t)j * Kd + r];
+        for (int r = 0; r < i + 2; r++) hc[kd + r] = hh[(size_t)j * (i + 2) + r];
+        double st = ls[j].add_column(kd + i, hc.data());
+        if (!std::isfinite(st)) {
+          conv[j] = 1;
+          fail[j] = 1;
+          jend[j] = i;
+        } else if (st <= tgt[j]) {
+          conv[j] = 1;
+          jend[j] = i + 1;
+        } else {
+          all = false;
+        }
+      }
+    }
+    for (int j = 0; j < Rn; j++) {
+      if (!conv[j]) jend[j] = niters;
+      if (active[j]) iters[j] += jend[j];
+    }
+    // y_j from the least squares;  X(:, j) += |r_j| beta_j M^{-1} (U D_u y_u + V y_v)
+    if (niters > 0) {
+      std::vector<double> cu((size_t)Rn * std::max(Kd, 1), 0.0), cv((size_t)Rn * niters, 0.0);
+      std::vector<double> y(m + 1);
+      for (int j = 0; j < Rn; j++) {
+        h_inv[j] = 0.0;
+        if (!active[j] || jend[j] == 0) continue;
+        const int kd = kdj[j];
+        ls[j].solve(kd + jend[j], y.data());
+        for (int i = 0; i < kd; i++) cu[(size_t)j * Kd + i] = dsc[(size_t)j * KD_MAX + i] * y[i];
+        for (int i = 0; i < jend[j]; i++) cv[(size_t)j * niters + i] = y[kd + i];
+        h_inv[j] = rn[j] * beta[j];
+      }
+      h2d_rows(zc, S, cu.data(), Kd);
+      h2d_rows(res, S, cv.data(), niters);
+      h2d(inv, h_inv, Rn);
+      if (Kd > 0) KCHECK(tacs_hip_blincomb(n, Rn, Kd, Ub(0), N, zc, S, NULL, d_work, 0, g_stream));
+      KCHECK(tacs_hip_blincomb(n, Rn, niters, V(0), N, res, S, NULL, d_work, Kd > 0, g_stream));
+      KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, d_work, tmp2, g_stream));
+      KCHECK(tacs_hip_bscale(n, Rn, inv, tmp2, g_stream));
+      KCHECK(tacs_hip_axpby(64, 64, N, 1.0, tmp2, 1.0, Xd, g_stream));
+    }
+
+    // ---- new recycled spaces from each right-hand side's harmonic Ritz vectors
+    std::vector<char> cand(Rn, 0);
+    bool any = false;
+    for (int j = 0; j < Rn; j++) {
+      cand[j] = active[j] && !fail[j] && jend[j] > 0 && kdj[j] + jend[j] > K + 1;
+      any = any || cand[j];
+    }
+    if (any) {
+      const int jm = niters;      // V_0..V_jm hold the Arnoldi vectors
+      const int p = Kd + jm + 1;  // rows of [C, V_0..V_jm]^T U_b
+      std::vector<double> prodh((size_t)Rn * std::max(p * Kd, 1), 0.0);
+      if (Kd > 0) {
+        for (int b = 0; b < Kd; b++) {
+          KCHECK(tacs_hip_bmdot(n, Rn, Kd, Cb(0), N, Ub(b), (double*)d_red, d_prod + (size_t)b * p,
+                                SP, 0, g_stream));
+          KCHECK(tacs_hip_bmdot(n, Rn, jm + 1, V(0), N, Ub(b), (double*)d_red,
+                                d_prod + (size_t)b * p + Kd, SP, 0, g_stream));
+        }
+        d2h_rows(prodh.data(), p * Kd, d_prod, SP);
+      }
+      std::vector<int> kk(Rn, 0);
+      std::vector<std::vector<double>> Qm(Rn), PRu(Rn), PRv(Rn);
+      std::vector<double> cvtu;
+      int newKd = 0;
+      for (int j = 0; j < Rn; j++) {
+        const int kd = kdj[j];
+        if (cand[j]) {
+          const int nrw = kd + jend[j] + 1;
+          cvtu.assign((size_t)nrw * std::max(kd, 1), 0.0);
+          for (int b = 0; b < kd; b++) {
+            const double* src = &prodh[(size_t)j * p * Kd + (size_t)b * p];
+            for (int r = 0; r < kd; r++) cvtu[(size_t)b * nrw + r] = src[r];
+            for (int v = 0; v <= jend[j]; v++) cvtu[(size_t)b * nrw + kd + v] = src[Kd + v];
+          }
+          kk[j] = recycle_update(ls[j], kd, jend[j], K, &dsc[(size_t)j * KD_MAX], cvtu.data(),
+                                 Qm[j], PRu[j], PRv[j]);
+        }
+        newKd = std::max(newKd, kk[j] > 0 ? kk[j] : kd);
+      }
+      bool changed = false;
+      for (int j = 0; j < Rn; j++) changed = changed || kk[j] > 0;
+      if (changed && newKd > 0) {
+        // per right-hand side P blocks: A (Kd x newKd) and B ((jm+1) x newKd) give
+        // C_new = C A + V B;  Cm (jm x newKd) and D (Kd x newKd) give U_new = V Cm + U D
+        const size_t oB = (size_t)Kd * newKd, oC = oB + (size_t)(jm + 1) * newKd,
+                     oD = oC + (size_t)jm * newKd, tot = oD + (size_t)Kd * newKd;
+        std::vector<double> P((size_t)Rn * tot, 0.0);
+        for (int j = 0; j < Rn; j++) {
+          double* Pj = &P[(size_t)j * tot];
+          const int kd = kdj[j];
+          if (kk[j] > 0) {
+            const int nrw = kd + jend[j] + 1, jn = jend[j];
+            for (int c = 0; c < kk[j]; c++) {
+              for (int r = 0; r < kd; r++) Pj[r + (size_t)c * Kd] = Qm[j][(size_t)c * nrw + r];
+              for (int v = 0; v <= jn; v++)
+                Pj[oB + v + (size_t)c * (jm + 1)] = Qm[j][(size_t)c * nrw + kd + v];
+              for (int v = 0; v < jn; v++) Pj[oC + v + (size_t)c * jm] = PRv[j][(size_t)c * jn + v];
+              for (int r = 0; r < kd; r++) Pj[oD + r + (size_t)c * Kd] = PRu[j][(size_t)c * kd + r];
+            }
+          } else {
+            for (int r = 0; r < kd; r++) {
+              Pj[r + (size_t)r * Kd] = 1.0;
+              Pj[oD + r + (size_t)r * Kd] = 1.0;
+            }
+          }
+        }
+        h2d_rows(d_prod, SP, P.data(), (int)tot);
+        if (Kd > 0)
+          KCHECK(tacs_hip_bbasis_combine(n, Rn, Kd, Cb(0), N, d_prod, SP, Kd, newKd, d_Cnb, N, 0,
+                                         g_stream));
+        KCHECK(tacs_hip_bbasis_combine(n, Rn, jm + 1, V(0), N, d_prod + oB, SP, jm + 1, newKd,
+                                       d_Cnb, N, Kd > 0, g_stream));
+        KCHECK(tacs_hip_bbasis_combine(n, Rn, jm, V(0), N, d_prod + oC, SP, jm, newKd, d_Unb, N,
+                                       0, g_stream));
+        if (Kd > 0)
+          KCHECK(tacs_hip_bbasis_combine(n, Rn, Kd, Ub(0), N, d_prod + oD, SP, Kd, newKd, d_Unb,
+                                         N, 1, g_stream));
+        HCHECK(hipStreamSynchronize(g_stream));
+        std::swap(d_Cb, d_Cnb);
+        std::swap(d_Ub, d_Unb);
+        for (int j = 0; j < Rn; j++)
+          if (kk[j] > 0) kdj[j] = kk[j];
+        Kd = newKd;
+        update_dscale(Kd);
+      }
     }
     spmm64(Xd, V(0));
     KCHECK(tacs_hip_axpby(64, 64, N, 1.0, Bd, -1.0, V(0), g_stream));

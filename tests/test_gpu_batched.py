@@ -71,12 +71,15 @@ def test_batched_solutions_match_dense_solve(system, prec):
 
 
 def test_batched_iteration_counts_match_single_gmres(system):
+    """Without restarts (m above the step count) the batched recurrences take
+    the steps of single GMRES solves; restarted runs on this small mesh
+    stagnate and their counts follow the rounding of the products."""
     p, pc, K = system
     a = p.assembler
     rng = np.random.default_rng(12)
     bs = _load_cases(p, rng)[:3]
     xs = [a.createVec() for _ in bs]
-    m = 20
+    m = 250
     ksm = TACS.BatchedKSM(p.mat, pc, m=m, restarts=200, nrhs=3, prec=64)
     ksm.setTolerances(1e-10, 1e-30)
     assert ksm.solve(bs, xs) == 1
@@ -171,3 +174,99 @@ def test_batched_with_recycled_space(system, prec):
     it_after = ksm.getStats()[0][0]
     assert plain.solve(bs[1:2], [a.createVec()]) == 1
     assert it_after == plain.getStats()[0][0]
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_batched_gcrodr_matches_dense_and_single_recycling(system, prec):
+    """Per right-hand side recycling (batched GCRO-DR): solutions against the
+    dense solve; with the FP32-valued operator each system takes the steps of
+    one MixedKSM with the same deflation solving it alone (the same cycle,
+    batched)."""
+    p, pc, K = system
+    a = p.assembler
+    rng = np.random.default_rng(15)
+    bs = _load_cases(p, rng)
+    ksm = TACS.BatchedKSM(p.mat, pc, m=20, restarts=300, nrhs=8, prec=prec, deflation=6)
+    ksm.setTolerances(1e-10, 1e-30)
+    xs = [a.createVec() for _ in bs]
+    assert ksm.solve(bs, xs) == 1
+    it, fl, r0, rn, cyc = ksm.getStats()
+    assert np.all(fl == 1)
+    for j, (b, x) in enumerate(zip(bs, xs)):
+        bv, xv = b.getArray(), x.getArray()
+        if not bv.any():
+            assert it[j] == 0 and not xv.any()
+            continue
+        xo = np.linalg.solve(K, bv)
+        assert np.abs(xv - xo).max() <= 1e-7 * np.abs(xo).max(), j
+        assert np.linalg.norm(bv - K @ xv) <= 2e-10 * np.linalg.norm(bv), j
+    assert it[4] == it[1]
+    if prec == 32:
+        for j in (0, 1, 2):
+            mk = TACS.MixedKSM(p.mat, pc, 20, 300, deflation=6)
+            mk.setTolerances(1e-10, 1e-30)
+            assert mk.solve(bs[j], a.createVec()) == 1
+            single = mk.getIterCount()
+            assert abs(single - it[j]) <= max(3, 0.05 * single), (j, single, it[j])
+    # the spaces carry over to the next solve with the same matrix
+    xs2 = [a.createVec() for _ in bs]
+    assert ksm.solve(bs, xs2) == 1
+    it2 = ksm.getStats()[0]
+    assert it2[1] < it[1], (it, it2)
+
+
+def test_batched_gcrodr_seeded_from_shared_space(system):
+    p, pc, K = system
+    a = p.assembler
+    rng = np.random.default_rng(16)
+    bs = _load_cases(p, rng)[:3]
+    mk = TACS.MixedKSM(p.mat, pc, 30, 200, deflation=8)
+    mk.setTolerances(1e-10, 1e-30)
+    assert mk.solve(bs[0], a.createVec()) == 1
+    ksm = TACS.BatchedKSM(p.mat, pc, m=20, restarts=300, nrhs=3, prec=32, deflation=8)
+    ksm.setTolerances(1e-10, 1e-30)
+    assert ksm.setRecycleSpace(mk) == 8
+    xs = [a.createVec() for _ in bs]
+    assert ksm.solve(bs, xs) == 1
+    it = ksm.getStats()[0]
+    fresh = TACS.BatchedKSM(p.mat, pc, m=20, restarts=300, nrhs=3, prec=32, deflation=8)
+    fresh.setTolerances(1e-10, 1e-30)
+    assert fresh.solve(bs, [a.createVec() for _ in bs]) == 1
+    assert it[0] < fresh.getStats()[0][0]
+    for b, x in zip(bs, xs):
+        xo = np.linalg.solve(K, b.getArray())
+        assert np.abs(x.getArray() - xo).max() <= 1e-7 * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("R", [1, 3, 16, 33])
+def test_spmm_matches_oracle(system, R):
+    """The multi right-hand-side SpMM (FP64 values and FP32-rounded values)
+    against the oracle's block-CSR product, one right-hand side at a time."""
+    import torch
+
+    import oracle as O
+    from tacs_amd import lib
+
+    p, pc, K = system
+    rowp, cols, A = p.mat.getArrays()
+    rp, cp, ap = p.mat.getDeviceArrays()
+    b, nr, nnz = p.mat.getInfo()
+    rng = np.random.default_rng(R)
+    Xh = rng.standard_normal((3 * nr, R))
+    X = torch.tensor(Xh.reshape(-1), dtype=torch.float64, device="cuda")
+    Y = torch.zeros_like(X)
+    A32 = torch.tensor(A.reshape(-1), dtype=torch.float64, device="cuda").float()
+    torch.cuda.synchronize()
+    assert lib().tacs_hip_bsr3_spmm(nr, R, rp, cp, ap, X.data_ptr(), Y.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    Yh = Y.cpu().numpy().reshape(3 * nr, R)
+    assert lib().tacs_hip_bsr3_spmm_mixed(nr, R, rp, cp, A32.data_ptr(), X.data_ptr(),
+                                          Y.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    Y32 = Y.cpu().numpy().reshape(3 * nr, R)
+    A32h = A.reshape(-1).astype(np.float32).astype(np.float64)
+    for j in range(R):
+        yo = O.bsr_mult(3, rowp, cols, A.reshape(-1), Xh[:, j])
+        assert np.abs(Yh[:, j] - yo).max() <= 1e-13 * np.abs(yo).max()
+        yo32 = O.bsr_mult(3, rowp, cols, A32h, Xh[:, j])
+        assert np.abs(Y32[:, j] - yo32).max() <= 1e-13 * np.abs(yo32).max()
