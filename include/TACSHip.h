@@ -332,6 +332,8 @@ class GMRES : public TACSKsm {
   double* h_pin;    // pinned host scalars
 };
 
+class TACSBatchedGMRES;
+
 // Mixed-precision restarted GMRES(m): each cycle starts from the FP64
 // residual b - A x and runs its Arnoldi steps on the FP32 matrix, FP32
 // block-Jacobi and an FP32 basis (Gram-Schmidt dot products, Hessenberg and
@@ -348,7 +350,7 @@ class TACSMixedGMRES : public TACSKsm {
   // 28 (2006)): each cycle keeps the k harmonic Ritz vectors of smallest
   // harmonic Ritz value of the fixed FP32-valued operator, and the space is
   // reused by later cycles and later solves with the same matrix
-  // (k <= 31, m <= 255); k = 0 restarts plainly.
+  // (k <= 31, k < m - 1, m + k <= 250); k = 0 restarts plainly.
   int setDeflation(int k);
   double rtol, atol;
   int msub, nrestart, outerCount, ndefl;
@@ -359,12 +361,7 @@ class TACSMixedGMRES : public TACSKsm {
   float* d_W;
   float* d_work32;
   float* d_d32;
-  double* d_B;     // FP64 basis [U (k) | C (k) | V (m+1)] of the recycling solver
-  double* d_T;     // 2k temporary vectors (new U and C)
-  double* d_P;     // small device matrices (basis changes, inner products)
-  double* d_tsg;   // tall-skinny product scratch
-  double* d_x64;   // FP64 work vectors of the recycling solver
-  double* d_y64;
+  TACSBatchedGMRES* rec;  // the recycling cycle (one right-hand side), k > 0
   double* d_r;
   double* d_scal;
   void* d_red;
@@ -418,6 +415,11 @@ class TACSBatchedGMRES : public TACSObject {
   double *d_Ub, *d_Cb, *d_Unb, *d_Cnb;  // K block vectors each
   double* d_prod;    // per right-hand side small matrices (products, basis changes)
   double* h_stage;   // pinned staging of the small matrices, R x prodStride()
+  void* d_dq;        // scratch of the delayed Gram-Schmidt dot products
+  // Gram-Schmidt of the recycling (GCRO-DR) Arnoldi steps: 1 (default)
+  // delayed classical Gram-Schmidt (DCGS2, two sweeps over the basis per
+  // step), 0 classical Gram-Schmidt twice (four sweeps)
+  int delayed;
   std::vector<int> kdj;      // dimension of each right-hand side's space
   std::vector<double> dsc;   // 1 / |U_j(:, v)|, R x KD_MAX
   TACSParallelMat* mat;
@@ -434,12 +436,15 @@ class TACSBatchedGMRES : public TACSObject {
   std::vector<int> iters, flags;
   std::vector<double> r0, rn;
   // per right-hand side: H (m+1) x m, res m+1, Qc m, Qs m, tmp m+1, then the
-  // recycled-space coefficients Bc (KD_MAX x m), tmpC, z, projection (KD_MAX each)
+  // recycled-space coefficients Bc (KD_MAX x m), tmpC, z, projection (KD_MAX
+  // each), then the delayed Gram-Schmidt dot products (2 x (KD_MAX + m + 1))
+  // and update coefficients (2 (KD_MAX + m) + 2)
   static const int KD_MAX = 32;
   // per right-hand side: four (KD_MAX + m + 1) x KD_MAX matrices
   size_t prodStride() const { return (size_t)4 * (KD_MAX + msub + 1) * KD_MAX; }
   size_t slice() const {
-    return (size_t)(msub + 1) * msub + 4 * (size_t)msub + 2 + (size_t)KD_MAX * (msub + 3);
+    return (size_t)(msub + 1) * msub + 4 * (size_t)msub + 2 + (size_t)KD_MAX * (msub + 3) +
+           4 * (size_t)(KD_MAX + msub) + 4;
   }
 };
 

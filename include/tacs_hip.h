@@ -314,6 +314,20 @@ int tacs_hip_bsmall_gemv(int R, int k, const int* cnt, const double* B, size_t s
                          const double* y, size_t sy, double* z, size_t sz, hipStream_t stream);
 int tacs_hip_blincomb(size_t n, int R, int kmax, const double* W, size_t ldw, const double* c,
                       size_t sR, const int* cnt, double* y, int add, hipStream_t stream);
+/* Delayed classical Gram-Schmidt (DCGS2, Bielich et al., Parallel Comput.
+   112 (2022)) over the basis [W1 (k1 block vectors), W2 (k2)], stride ldw:
+   bdotq:  out_qi[j*sR + v] = W_v(:, j) . q_qi(:, j) for nq (1 or 2) vectors
+           (scratch: tacs_hip_bdotq_scratch_bytes);
+   bdcgs_update: per right-hand side j with cf = coef + j*sC =
+           [s (k) | t (k) | 1/omega | tau], k = k1 + k2:
+           u <- (u - W s)/omega,  z <- (z - W t - u tau)/omega (z may be NULL). */
+size_t tacs_hip_bdotq_scratch_bytes(int R);
+int tacs_hip_bdotq(size_t n, int R, int nq, int k1, const double* W1, int k2, const double* W2,
+                   size_t ldw, const double* q0, const double* q1, double* scratch, double* out0,
+                   double* out1, size_t sR, hipStream_t stream);
+int tacs_hip_bdcgs_update(size_t n, int R, int k1, const double* W1, int k2, const double* W2,
+                          size_t ldw, const double* coef, size_t sC, double* u, double* z,
+                          hipStream_t stream);
 int tacs_hip_badd(int R, int k, const double* x, size_t sx, double* y, size_t sy,
                   hipStream_t stream);
 int tacs_hip_bscale(size_t n, int R, const double* s, double* x, hipStream_t stream);
@@ -472,9 +486,11 @@ int tacs_gmres_set_ortho_type(tacs_obj ksm, int ortho);
 /* Mixed-precision restarted GMRES(m) (FP32 Arnoldi cycles, FP64 restart
    residual and solution); no counterpart in the reference. */
 tacs_obj tacs_mixed_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart);
-/* Deflated restarts for the mixed GMRES (GMRES-DR: keep the k harmonic Ritz
-   vectors of smallest harmonic Ritz value at each restart; 0 <= k <= 31,
-   k < m, m <= 511).  No counterpart in the reference. */
+/* Krylov subspace recycling for the mixed GMRES (GCRO-DR: each cycle keeps
+   the k harmonic Ritz vectors of smallest harmonic Ritz value, and the space
+   carries over to later solves with the same matrix values; delayed
+   classical Gram-Schmidt; 0 <= k <= 31, k < m - 1, m + k <= 250).  No
+   counterpart in the reference. */
 int tacs_mixed_gmres_set_deflation(tacs_obj ksm, int k);
 /* Batched restarted GMRES(m) for nrhs <= 64 right-hand sides of one 3x3
    block matrix: one multi-RHS SpMM per step for all systems, per-system
@@ -494,6 +510,9 @@ int tacs_batched_gmres_set_tolerances(tacs_obj ksm, double rtol, double atol);
    from its harmonic Ritz vectors; 0 <= k <= 31, k < m - 1, m + k <= 250;
    0 turns it off).  No counterpart in the reference. */
 int tacs_batched_gmres_set_deflation(tacs_obj ksm, int k);
+/* Gram-Schmidt of the recycling Arnoldi steps: 1 (default) delayed
+   classical Gram-Schmidt (DCGS2), 0 classical Gram-Schmidt twice. */
+int tacs_batched_gmres_set_delayed_reorth(tacs_obj ksm, int delayed);
 /* Deflate the batched solves by the recycled space of a mixed GMRES with
    deflation over the same matrix (copied; dropped when the matrix values
    change); with per right-hand side recycling on, it seeds every system's

@@ -204,6 +204,13 @@ class BatchedKSM(TACSHandle):
     def setTolerances(self, rtol, atol):
         lib().tacs_batched_gmres_set_tolerances(self.ptr, rtol, atol)
 
+    def setDelayedReorthogonalization(self, flag=True):
+        """Gram-Schmidt of the recycling Arnoldi steps: delayed classical
+        Gram-Schmidt (DCGS2, the default: two sweeps over the basis per step)
+        or, flag False, classical Gram-Schmidt twice (four sweeps)."""
+        check(lib().tacs_batched_gmres_set_delayed_reorth(self.ptr, int(bool(flag))),
+              "BatchedKSM.setDelayedReorthogonalization")
+
     def setRecycleSpace(self, ksm):
         """Deflate every right-hand side by the recycled space (U, C) that the
         MixedKSM ksm (with deflation, same matrix) holds after a solve: each

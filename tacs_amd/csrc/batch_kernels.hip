@@ -128,6 +128,168 @@ __global__ void bsmall_gemv(int R, int k, const int* __restrict__ cnt,
   z[(size_t)j * sz + v] = s;
 }
 
+// ---------------------------------------------------------------------------
+// Delayed classical Gram-Schmidt (DCGS2: D. Bielich, J. Langou, S. Thomas,
+// K. Swirydowicz, I. Yamazaki, E. G. Boman, "Low-synch Gram-Schmidt with
+// delayed reorthogonalization for Krylov solvers", Parallel Computing 112
+// (2022)) needs, per Arnoldi step, one sweep of dot products of the basis
+// against two vectors and one sweep updating two vectors.
+// ---------------------------------------------------------------------------
+#define BDQ_KMAX 32
+
+// Partial dots of the concatenated basis [W1 (k1 vectors), W2 (k2 vectors)]
+// (block vectors, stride ldw; entries v0 .. v0 + kk of the list, kk <=
+// BDQ_KMAX) against NQ vectors q_0, q_1: part[((v*NQ + qi)*R + j)*nb + blk].
+// A block is 4 wavefronts over the same rows; wave w takes the basis vectors
+// v0 + 8w .. v0 + 8w + 7 (so each thread keeps 8 NQ sums and the q entries,
+// read by all four waves, come from cache).  Within a wave, lanes are (row
+// group g, right-hand side j); the G row groups are reduced by lane shuffles.
+template <int NQ>
+__global__ __launch_bounds__(256) void bdotq_partial(size_t n, int R, int v0, int kk, int k1,
+                                                     const double* __restrict__ W1,
+                                                     const double* __restrict__ W2, size_t ldw,
+                                                     const double* __restrict__ q0,
+                                                     const double* __restrict__ q1,
+                                                     double* __restrict__ part) {
+  const int Rp = pow2_ceil(R);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int j = lane % Rp, g = lane / Rp, G = 64 / Rp;
+  const int vb = wv * 8;  // first vector (relative to v0) of this wave
+  const int nv = min(8, kk - vb);
+  if (nv <= 0) return;
+  const double* Wv[8];
+#pragma unroll
+  for (int v = 0; v < 8; v++) {
+    const int vv = v0 + vb + (v < nv ? v : 0);
+    Wv[v] = vv < k1 ? W1 + (size_t)vv * ldw : W2 + (size_t)(vv - k1) * ldw;
+  }
+  double acc[NQ][8];
+#pragma unroll
+  for (int qi = 0; qi < NQ; qi++)
+#pragma unroll
+    for (int v = 0; v < 8; v++) acc[qi][v] = 0.0;
+  if (j < R) {
+    for (size_t i = (size_t)blockIdx.x * G + g; i < n; i += (size_t)gridDim.x * G) {
+      const size_t t = i * R + j;
+      const double a0 = q0[t];
+      const double a1 = NQ > 1 ? q1[t] : 0.0;
+#pragma unroll
+      for (int v = 0; v < 8; v++) {
+        if (v < nv) {
+          const double w = Wv[v][t];
+          acc[0][v] += w * a0;
+          if (NQ > 1) acc[NQ - 1][v] += w * a1;
+        }
+      }
+    }
+  }
+  for (int off = Rp; off < 64; off <<= 1) {
+#pragma unroll
+    for (int qi = 0; qi < NQ; qi++)
+#pragma unroll
+      for (int v = 0; v < 8; v++) acc[qi][v] += __shfl_xor(acc[qi][v], off, 64);
+  }
+  if (g == 0 && j < R) {
+#pragma unroll
+    for (int qi = 0; qi < NQ; qi++)
+#pragma unroll
+      for (int v = 0; v < 8; v++)
+        if (v < nv)
+          part[((size_t)((vb + v) * NQ + qi) * R + j) * gridDim.x + blockIdx.x] = acc[qi][v];
+  }
+}
+
+// out_qi[j*sR + v0 + v] = sum_b part[((v*NQ + qi)*R + j)*nb + b]; one wave each
+__global__ void bdotq_finalize(int nb, int kk, int NQ, int R, const double* __restrict__ part,
+                               double* __restrict__ out0, double* __restrict__ out1, size_t sR,
+                               int v0) {
+  const int lane = threadIdx.x & 63;
+  const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w >= kk * NQ * R) return;
+  const int j = w % R, vq = w / R, qi = vq % NQ, v = vq / NQ;
+  const double* p = &part[(size_t)w * nb];
+  double s = 0.0;
+  for (int b = lane; b < nb; b += 64) s += p[b];
+  s = wave_sum(s);
+  if (lane == 0) (qi == 0 ? out0 : out1)[(size_t)j * sR + v0 + v] = s;
+}
+
+// The update sweep of a DCGS2 step, per right-hand side j with coefficients
+// cf_j = coef + j*sC = [s (k) | t (k) | 1/omega | tau], k = k1 + k2, over the
+// basis [W1, W2]:
+//   u <- (u - W s) / omega                      (delayed reorthogonalisation)
+//   z <- (z - W t - u_new tau) / omega          (first pass of the new vector)
+// (z NULL: the u update alone).  A thread takes BDU_E entries 256 apart,
+// which belong to the same right-hand side when R divides 256, so each
+// coefficient it loads serves BDU_E entries.
+#define BDU_E 4
+__global__ __launch_bounds__(256) void bdcgs_update(size_t n, int R, int k1,
+                                                    const double* __restrict__ W1, int k2,
+                                                    const double* __restrict__ W2, size_t ldw,
+                                                    const double* __restrict__ coef, size_t sC,
+                                                    double* __restrict__ u, double* __restrict__ z) {
+  const size_t N = n * R;
+  const int k = k1 + k2;
+  for (size_t base = (size_t)blockIdx.x * (256 * BDU_E); base < N;
+       base += (size_t)gridDim.x * (256 * BDU_E)) {
+    const size_t t0 = base + threadIdx.x;
+    const double* cf = &coef[(size_t)(t0 % R) * sC];
+    double as[BDU_E], at[BDU_E];
+    bool ok[BDU_E];
+#pragma unroll
+    for (int e = 0; e < BDU_E; e++) {
+      as[e] = 0.0;
+      at[e] = 0.0;
+      ok[e] = t0 + (size_t)e * 256 < N;
+    }
+    for (int v = 0; v < k; v++) {
+      const double* w = v < k1 ? W1 + (size_t)v * ldw : W2 + (size_t)(v - k1) * ldw;
+      const double cs = cf[v], ct = cf[k + v];
+#pragma unroll
+      for (int e = 0; e < BDU_E; e++)
+        if (ok[e]) {
+          const double x = w[t0 + (size_t)e * 256];
+          as[e] += x * cs;
+          at[e] += x * ct;
+        }
+    }
+    const double io = cf[2 * k], tau = cf[2 * k + 1];
+#pragma unroll
+    for (int e = 0; e < BDU_E; e++)
+      if (ok[e]) {
+        const size_t t = t0 + (size_t)e * 256;
+        const double un = (u[t] - as[e]) * io;
+        u[t] = un;
+        if (z) z[t] = (z[t] - at[e] - un * tau) * io;
+      }
+  }
+}
+
+// General R (not dividing 256): one entry per thread
+__global__ __launch_bounds__(256) void bdcgs_update1(size_t n, int R, int k1,
+                                                     const double* __restrict__ W1, int k2,
+                                                     const double* __restrict__ W2, size_t ldw,
+                                                     const double* __restrict__ coef, size_t sC,
+                                                     double* __restrict__ u,
+                                                     double* __restrict__ z) {
+  const size_t N = n * R;
+  const int k = k1 + k2;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < N;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const double* cf = &coef[(size_t)(t % R) * sC];
+    double as = 0.0, at = 0.0;
+    for (int v = 0; v < k; v++) {
+      const double w = v < k1 ? W1[(size_t)v * ldw + t] : W2[(size_t)(v - k1) * ldw + t];
+      as += w * cf[v];
+      at += w * cf[k + v];
+    }
+    const double io = cf[2 * k], tau = cf[2 * k + 1];
+    const double un = (u[t] - as) * io;
+    u[t] = un;
+    if (z) z[t] = (z[t] - at - un * tau) * io;
+  }
+}
+
 // y[j*sy + v] += x[j*sx + v], v < k (the second Gram-Schmidt pass into H)
 __global__ void badd(int R, int k, const double* __restrict__ x, size_t sx, double* __restrict__ y,
                      size_t sy) {
@@ -444,6 +606,50 @@ int tacs_hip_bsmall_gemv(int R, int k, const int* cnt, const double* B, size_t s
   if (R * k <= 0) return 0;
   hipLaunchKernelGGL(bsmall_gemv, dim3((R * k + 255) / 256), dim3(256), 0, stream, R, k, cnt, B,
                      sB, ldb, y, sy, z, sz);
+  return (int)hipGetLastError();
+}
+
+size_t tacs_hip_bdotq_scratch_bytes(int R) {
+  return (size_t)4096 * 2 * BDQ_KMAX * (R > 0 ? R : 1) * sizeof(double);
+}
+
+int tacs_hip_bdotq(size_t n, int R, int nq, int k1, const double* W1, int k2, const double* W2,
+                   size_t ldw, const double* q0, const double* q1, double* scratch, double* out0,
+                   double* out1, size_t sR, hipStream_t stream) {
+  if (R < 1 || R > 64 || nq < 1 || nq > 2 || k1 < 0 || k2 < 0) return TACS_HIP_ERR_ARG;
+  const int k = k1 + k2;
+  const int G = 64 / pow2_ceil(R);
+  size_t nbs = (n + (size_t)G * 8 - 1) / ((size_t)G * 8);  // >= 8 rows per lane group
+  if (nbs > 4096) nbs = 4096;
+  if (nbs < 1) nbs = 1;
+  const int nb = (int)nbs;
+  for (int v0 = 0; v0 < k; v0 += BDQ_KMAX) {
+    const int kk = k - v0 < BDQ_KMAX ? k - v0 : BDQ_KMAX;
+    if (nq == 1)
+      hipLaunchKernelGGL(bdotq_partial<1>, dim3(nb), dim3(256), 0, stream, n, R, v0, kk, k1, W1,
+                         W2, ldw, q0, q0, scratch);
+    else
+      hipLaunchKernelGGL(bdotq_partial<2>, dim3(nb), dim3(256), 0, stream, n, R, v0, kk, k1, W1,
+                         W2, ldw, q0, q1, scratch);
+    const int waves = kk * nq * R;
+    hipLaunchKernelGGL(bdotq_finalize, dim3((waves + 3) / 4), dim3(256), 0, stream, nb, kk, nq, R,
+                       scratch, out0, out1, sR, v0);
+  }
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_bdcgs_update(size_t n, int R, int k1, const double* W1, int k2, const double* W2,
+                          size_t ldw, const double* coef, size_t sC, double* u, double* z,
+                          hipStream_t stream) {
+  if (256 % R == 0) {
+    size_t g = (n * R + 256 * BDU_E - 1) / (256 * BDU_E);
+    if (g > 16384) g = 16384;
+    hipLaunchKernelGGL(bdcgs_update, dim3((int)g), dim3(256), 0, stream, n, R, k1, W1, k2, W2, ldw,
+                       coef, sC, u, z);
+  } else {
+    hipLaunchKernelGGL(bdcgs_update1, dim3(bgrid(n * R)), dim3(256), 0, stream, n, R, k1, W1, k2,
+                       W2, ldw, coef, sC, u, z);
+  }
   return (int)hipGetLastError();
 }
 

@@ -473,6 +473,11 @@ int tacs_batched_gmres_set_deflation(tacs_obj ksm, int k) {
   REQUIRE(TACSBatchedGMRES, g, ksm);
   return g->setDeflation(k);
 }
+int tacs_batched_gmres_set_delayed_reorth(tacs_obj ksm, int delayed) {
+  REQUIRE(TACSBatchedGMRES, g, ksm);
+  g->delayed = delayed ? 1 : 0;
+  return 0;
+}
 int tacs_batched_gmres_set_recycle_space(tacs_obj ksm, tacs_obj src) {
   clear_error();
   TACSBatchedGMRES* k = OBJ(TACSBatchedGMRES, ksm);

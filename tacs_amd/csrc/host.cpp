@@ -682,8 +682,7 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
 // ---------------------------------------------------------------------------
 TACSMixedGMRES::TACSMixedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int _m, int _nrestart)
     : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart), outerCount(0), ndefl(0),
-      recycleDim(0), recycleVersion(-1), mat(m), pc(p), d_B(NULL), d_T(NULL), d_P(NULL),
-      d_tsg(NULL), d_x64(NULL), d_y64(NULL) {
+      recycleDim(0), recycleVersion(-1), mat(m), pc(p), rec(NULL) {
   mat->incref();
   if (pc) pc->incref();
   mat->getValues32(0);  // allocate the FP32 copy now; assembly fills it
@@ -701,8 +700,7 @@ TACSMixedGMRES::TACSMixedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int _m,
   HCHECK(hipHostMalloc((void**)&h_pin, (size_t)(msub + 64) * sizeof(double), 0));
 }
 TACSMixedGMRES::~TACSMixedGMRES() {
-  void* rec[] = {d_B, d_T, d_P, d_tsg, d_x64, d_y64};
-  for (void* q : rec) dfree(q);
+  if (rec) rec->decref();
   dfree(d_W);
   dfree(d_work32);
   dfree(d_d32);
@@ -719,24 +717,19 @@ void TACSMixedGMRES::setTolerances(double r, double a) {
 }
 
 int TACSMixedGMRES::setDeflation(int k) {
-  if (k < 0 || k > 31 || (k > 0 && (k >= msub - 1 || msub + 2 * k > 250)))
-    return tacs_set_error("setDeflation: need 0 <= k <= 31, k < m - 1 and m + 2k <= 250 "
+  if (k < 0 || k > 31 || (k > 0 && (k >= msub - 1 || msub + k > 250)))
+    return tacs_set_error("setDeflation: need 0 <= k <= 31, k < m - 1 and m + k <= 250 "
                           "(k = %d, m = %d)", k, msub);
   HCHECK(hipStreamSynchronize(g_stream));
-  void* rec[] = {d_B, d_T, d_P, d_tsg, d_x64, d_y64};
-  for (void* q : rec) dfree(q);
-  d_B = d_T = d_P = d_tsg = d_x64 = d_y64 = NULL;
+  if (rec) rec->decref();
+  rec = NULL;
   ndefl = k;
   recycleDim = 0;
   recycleVersion = -1;
   if (k > 0) {
-    size_t n = (size_t)mat->nrows * mat->bsize;
-    d_B = dalloc<double>((size_t)(2 * k + msub + 1) * n);
-    d_T = dalloc<double>((size_t)(2 * k) * n);
-    d_P = dalloc<double>((size_t)(2 * k + msub + 2) * 64);
-    d_tsg = dalloc<double>(tacs_hip_tsgemm_scratch_bytes() / sizeof(double));
-    d_x64 = dalloc<double>(n);
-    d_y64 = dalloc<double>(n);
+    rec = new TACSBatchedGMRES(mat, pc, msub, nrestart, 1, 32);
+    rec->incref();
+    if (rec->setDeflation(k) != 0) return 1;
   }
   return 0;
 }
@@ -819,12 +812,13 @@ int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
 // operator A32 D32^{-1} applied in FP64 arithmetic to FP64 vectors (so the
 // recycled relation A32 D32^{-1} U = C stays exact across cycles, solves and
 // right-hand sides); every cycle starts from the true FP64 residual
-// r = b - A x, and the FP64 basis, Gram-Schmidt and small dense algebra
-// follow.  A cycle: x += D^{-1} U C^T r, r -= C C^T r; m - k Arnoldi steps of
-// (I - C C^T) A32 D32^{-1} from r; least squares over [U D_u, V]; then the k
-// harmonic Ritz vectors of smallest harmonic Ritz value of that projected
-// problem become the next U, C.  The least-squares problem and the small
-// eigenproblem run on the host (the Hessenberg column is read back each step).
+// r = b - A x.  A cycle: x += D^{-1} U C^T r, r -= C C^T r; m - k Arnoldi steps
+// of (I - C C^T) A32 D32^{-1} from r (delayed classical Gram-Schmidt);
+// least squares over [U D_u, V]; then the k harmonic Ritz vectors of smallest
+// harmonic Ritz value of that projected problem become the next U, C.  The
+// cycle is the one of the batched solver (TACSBatchedGMRES::
+// solveBlockDeflated) with one right-hand side, whose least-squares problem
+// and small eigenproblem run on the host.
 // ---------------------------------------------------------------------------
 namespace {
 struct HostLS {
@@ -1014,155 +1008,18 @@ static int recycle_update(const HostLS& ls, int kd, int jend, int K, const doubl
 int TACSMixedGMRES::solveDeflated(TACSVec* tb, TACSVec* tx, int zero_guess) {
   TACSBVec* b = dynamic_cast<TACSBVec*>(tb);
   TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
-  const size_t n = (size_t)mat->nrows * mat->bsize;
-  const int bs = mat->bsize, nr = mat->nrows, m = msub, K = ndefl;
-  float* A32 = mat->getValues32(1);
-  double* misc = d_scal;  // first 16 entries used as device scalars
-  double* dcol = d_P;     // device Hessenberg column / small vectors (2K + m + 2 entries)
-  double* dmat = d_P + (size_t)(2 * K + m + 2) * 2;  // device small matrices
-  double* U = d_B;
-  double* Cm = d_B + (size_t)K * n;
-  double* V0 = Cm;  // V follows the kd active columns of C: [C | V] is contiguous
-  auto Vv = [&](int i) { return V0 + (size_t)i * n; };
+  rec->rtol = rtol;
+  rec->atol = atol;
+  rec->nrestart = nrestart;
   const double* bd = b->devRead();
-  double* xd = x->devWrite();
-  if (recycleVersion != mat->version) recycleDim = 0;
-  int kd = recycleDim;
-  iterCount = 0;
-  outerCount = 0;
-  if (zero_guess) {
-    KCHECK(tacs_hip_fill(64, n, xd, 0.0, g_stream));
-    HCHECK(hipMemcpyAsync(d_r, bd, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
-  } else {
-    KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
-    KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
-  }
-  std::vector<double> y(m + 1), hc(2 * K + m + 2), dscale(K + 1, 1.0);
-  std::vector<double> tmpv;
-  double r0 = 0.0;
-  int flag = 0;
-  for (int count = 0; count <= nrestart; count++) {
-    KCHECK(tacs_hip_dot(64, n, NULL, NULL, d_r, d_r, d_red, misc + MISC_NRM, 1, misc + MISC_INV,
-                        g_stream));
-    double rn = pin_read(h_pin, misc + MISC_NRM);
-    if (count == 0) r0 = rn;
-    resNorm = rn;
-    if (rn <= rtol * r0 || rn < atol) {
-      flag = 1;
-      break;
-    }
-    if (!std::isfinite(rn)) break;
-    if (kd > 0) {
-      // x += D^{-1} U (C^T r);  r -= C (C^T r)
-      KCHECK(tacs_hip_mdot(64, n, kd, Cm, n, d_r, d_red, dcol, 0, g_stream));
-      KCHECK(tacs_hip_lincomb(64, 64, n, kd, U, n, dcol, d_x64, 0, g_stream));
-      KCHECK(tacs_hip_bsr_diag_apply_mixed(bs, nr, pc->d_Dinv32, d_x64, d_y64, g_stream));
-      KCHECK(tacs_hip_axpby(64, 64, n, 1.0, d_y64, 1.0, xd, g_stream));
-      KCHECK(tacs_hip_cgs_update(64, n, kd, Cm, n, dcol, NULL, d_r, g_stream));
-      KCHECK(tacs_hip_dot(64, n, NULL, NULL, d_r, d_r, d_red, misc + MISC_NRM, 1,
-                          misc + MISC_INV, g_stream));
-    }
-    double rp = pin_read(h_pin, misc + MISC_NRM);
-    const double target = std::max(rtol * r0, atol);
-    if (rp <= target) {
-      // the projection alone converged: refresh the true residual
-      KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
-      KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
-      outerCount++;
-      continue;
-    }
-    V0 = Cm + (size_t)kd * n;
-    KCHECK(tacs_hip_scale(64, n, d_r, misc + MISC_INV, 0.0, 0, g_stream));
-    HCHECK(hipMemcpyAsync(Vv(0), d_r, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
-    // u-column scales D_u (columns of U are not normalised)
-    if (kd > 0) {
-      KCHECK(tacs_hip_tsgemm(n, kd, U, kd, U, d_tsg, dmat, g_stream));
-      tmpv.resize((size_t)kd * kd);
-      HCHECK(hipMemcpy(tmpv.data(), dmat, (size_t)kd * kd * sizeof(double),
-                       hipMemcpyDeviceToHost));
-      for (int i = 0; i < kd; i++) dscale[i] = 1.0 / sqrt(tmpv[(size_t)i * kd + i]);
-    }
-    // least squares over [U D_u, V]: columns 0..kd-1 are D_u (diagonal), then Arnoldi
-    HostLS ls(kd + (m - kd) + 1);
-    ls.reset();
-    ls.c[kd] = 1.0;
-    ls.g[kd] = 1.0;
-    for (int i = 0; i < kd; i++) {
-      std::fill(hc.begin(), hc.end(), 0.0);
-      hc[i] = dscale[i];
-      ls.add_column(i, hc.data());
-    }
-    const int s = m - kd;
-    const double tgt = target / rp;
-    int jend = 0;
-    for (int j = 0; j < s; j++) {
-      double* w = Vv(j + 1);
-      KCHECK(tacs_hip_bsr_diag_apply_mixed(bs, nr, pc->d_Dinv32, Vv(j), d_x64, g_stream));
-      KCHECK(tacs_hip_bsr_spmv_mixed(bs, nr, mat->d_rowp, mat->d_cols, A32, d_x64, w, 0,
-                                     g_stream));
-      // against [C, V_0..V_j] (contiguous): coefficients [B_j ; h_j]
-      cgs2(64, n, kd + j + 1, Cm, w, dcol, dcol + (2 * K + m + 2), d_red);
-      KCHECK(tacs_hip_dot(64, n, NULL, NULL, w, w, d_red, dcol + kd + j + 1, 1, misc + MISC_INV,
-                          g_stream));
-      KCHECK(tacs_hip_scale(64, n, w, misc + MISC_INV, 0.0, 0, g_stream));
-      HCHECK(hipMemcpyAsync(h_pin, dcol, (kd + j + 2) * sizeof(double), hipMemcpyDeviceToHost,
-                            g_stream));
-      HCHECK(hipStreamSynchronize(g_stream));
-      double st = ls.add_column(kd + j, h_pin);
-      jend = j + 1;
-      if (st <= tgt || !std::isfinite(st)) break;
-    }
-    iterCount += jend;
-    outerCount++;
-    const int nc = kd + jend;  // columns of the small problem
-    ls.solve(nc, y.data());
-    // x += rp D^{-1} (U D_u y_u + V y_v)
-    for (int i = 0; i < nc; i++) h_pin[i] = (i < kd ? dscale[i] : 1.0) * y[i];
-    HCHECK(hipMemcpyAsync(dcol, h_pin, nc * sizeof(double), hipMemcpyHostToDevice, g_stream));
-    KCHECK(tacs_hip_fill(64, n, d_x64, 0.0, g_stream));
-    if (kd > 0) KCHECK(tacs_hip_lincomb(64, 64, n, kd, U, n, dcol, d_x64, 1, g_stream));
-    KCHECK(tacs_hip_lincomb(64, 64, n, jend, V0, n, dcol + kd, d_x64, 1, g_stream));
-    KCHECK(tacs_hip_bsr_diag_apply_mixed(bs, nr, pc->d_Dinv32, d_x64, d_y64, g_stream));
-    KCHECK(tacs_hip_axpby(64, 64, n, rp, d_y64, 1.0, xd, g_stream));
-    KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
-    KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
-
-    // ---- new recycle space from the harmonic Ritz vectors of this cycle
-    if (nc <= K + 1) continue;  // too few columns to choose from: keep the old space
-    // [C, V_0..V_jend]^T U (C and V contiguous)
-    tmpv.assign((size_t)(kd + jend + 1) * std::max(kd, 1), 0.0);
-    if (kd > 0) {
-      KCHECK(tacs_hip_tsgemm(n, kd + jend + 1, Cm, kd, U, d_tsg, dmat, g_stream));
-      HCHECK(hipMemcpy(tmpv.data(), dmat, (size_t)(kd + jend + 1) * kd * sizeof(double),
-                       hipMemcpyDeviceToHost));
-    }
-    std::vector<double> Qm, PRu, PRv;
-    const int nrw = nc + 1;
-    int kk = recycle_update(ls, kd, jend, K, dscale.data(), tmpv.data(), Qm, PRu, PRv);
-    if (kk == 0) continue;
-    double* Cn = d_T;
-    double* Un = d_T + (size_t)K * n;
-    HCHECK(hipMemcpy(dmat, Qm.data(), Qm.size() * sizeof(double), hipMemcpyHostToDevice));
-    KCHECK(tacs_hip_basis_combine64(n, nrw, Cm, n, dmat, kk, Cn, n, 0, g_stream));
-    HCHECK(hipMemcpyAsync(dmat, PRv.data(), PRv.size() * sizeof(double), hipMemcpyHostToDevice,
-                          g_stream));
-    KCHECK(tacs_hip_basis_combine64(n, jend, V0, n, dmat, kk, Un, n, 0, g_stream));
-    if (kd > 0) {
-      double* dm2 = dmat + PRv.size() + 64;
-      HCHECK(hipMemcpyAsync(dm2, PRu.data(), (size_t)kd * kk * sizeof(double),
-                            hipMemcpyHostToDevice, g_stream));
-      KCHECK(tacs_hip_basis_combine64(n, kd, U, n, dm2, kk, Un, n, 1, g_stream));
-    }
-    HCHECK(hipMemcpyAsync(Cm, Cn, (size_t)kk * n * sizeof(double), hipMemcpyDeviceToDevice,
-                          g_stream));
-    HCHECK(hipMemcpyAsync(U, Un, (size_t)kk * n * sizeof(double), hipMemcpyDeviceToDevice,
-                          g_stream));
-    HCHECK(hipStreamSynchronize(g_stream));
-    kd = kk;
-  }
-  recycleDim = kd;
-  recycleVersion = mat->version;
-  HCHECK(hipStreamSynchronize(g_stream));
+  double* xd = zero_guess ? x->devWrite() : (x->devRead(), x->devWrite());
+  int flag = rec->solveBlock(1, bd, xd, zero_guess);
+  if (flag < 0) return 0;
+  iterCount = rec->iters.empty() ? 0 : rec->iters[0];
+  resNorm = rec->rn.empty() ? 0.0 : rec->rn[0];
+  outerCount = rec->cycles;
+  recycleDim = rec->kdj.empty() ? 0 : rec->kdj[0];
+  recycleVersion = rec->defVersion;
   return flag;
 }
 
@@ -1182,8 +1039,8 @@ TACSBatchedGMRES::TACSBatchedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int
                                    int nrhs, int _prec)
     : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart), R(nrhs), prec(_prec), cycles(0),
       kdefl(0), kVersion(-1), d_U(NULL), d_C(NULL), ndefl(0), defVersion(-1), defR(0),
-      d_Ub(NULL), d_Cb(NULL), d_Unb(NULL), d_Cnb(NULL), d_prod(NULL), h_stage(NULL), mat(m),
-      pc(p) {
+      d_Ub(NULL), d_Cb(NULL), d_Unb(NULL), d_Cnb(NULL), d_prod(NULL), h_stage(NULL), d_dq(NULL),
+      delayed(1), mat(m), pc(p) {
   mat->incref();
   pc->incref();
   if (prec == 32) mat->getValues32(0);
@@ -1202,8 +1059,8 @@ TACSBatchedGMRES::TACSBatchedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int
 }
 TACSBatchedGMRES::~TACSBatchedGMRES() {
   HCHECK(hipStreamSynchronize(g_stream));
-  void* bufs[] = {d_V,  d_work, d_Bb, d_Xb, d_scal, d_vec,  d_cnt, d_red,
-                  d_U,  d_C,    d_Ub, d_Cb, d_Unb,  d_Cnb,  d_prod};
+  void* bufs[] = {d_V, d_work, d_Bb,  d_Xb,  d_scal, d_vec, d_cnt, d_red,
+                  d_U, d_C,    d_Ub,  d_Cb,  d_Unb,  d_Cnb, d_prod, d_dq};
   for (void* q : bufs) dfree(q);
   HCHECK(hipHostFree(h_pin));
   if (h_stage) HCHECK(hipHostFree(h_stage));
@@ -1227,15 +1084,15 @@ int TACSBatchedGMRES::setRecycleSpace(TACSMixedGMRES* src) {
     return -1;
   }
   int k = src->recycleDim;
-  if (k <= 0 || src->recycleVersion != mat->version) return 0;
+  if (k <= 0 || !src->rec || src->recycleVersion != mat->version) return 0;
   if (k > KD_MAX) k = KD_MAX;
   const size_t n = (size_t)mat->nrows * mat->bsize;
   d_U = dalloc<double>((size_t)k * n);
   d_C = dalloc<double>((size_t)k * n);
-  // [U (ndefl) | C (kd) | V] in the source; the first k columns of each
-  HCHECK(hipMemcpyAsync(d_U, src->d_B, (size_t)k * n * sizeof(double), hipMemcpyDeviceToDevice,
-                        g_stream));
-  HCHECK(hipMemcpyAsync(d_C, src->d_B + (size_t)src->ndefl * n, (size_t)k * n * sizeof(double),
+  // the source's one right-hand side space: k vectors of n each
+  HCHECK(hipMemcpyAsync(d_U, src->rec->d_Ub, (size_t)k * n * sizeof(double),
+                        hipMemcpyDeviceToDevice, g_stream));
+  HCHECK(hipMemcpyAsync(d_C, src->rec->d_Cb, (size_t)k * n * sizeof(double),
                         hipMemcpyDeviceToDevice, g_stream));
   HCHECK(hipStreamSynchronize(g_stream));
   kdefl = k;
@@ -1250,9 +1107,10 @@ int TACSBatchedGMRES::setDeflation(int k) {
     return -1;
   }
   HCHECK(hipStreamSynchronize(g_stream));
-  void* bufs[] = {d_Ub, d_Cb, d_Unb, d_Cnb, d_prod};
+  void* bufs[] = {d_Ub, d_Cb, d_Unb, d_Cnb, d_prod, d_dq};
   for (void* q : bufs) dfree(q);
   d_Ub = d_Cb = d_Unb = d_Cnb = d_prod = NULL;
+  d_dq = NULL;
   if (h_stage) HCHECK(hipHostFree(h_stage));
   h_stage = NULL;
   ndefl = k;
@@ -1268,6 +1126,7 @@ int TACSBatchedGMRES::setDeflation(int k) {
     d_Cnb = dalloc<double>((size_t)k * N);
     d_prod = dalloc<double>((size_t)R * prodStride());
     HCHECK(hipHostMalloc((void**)&h_stage, (size_t)R * prodStride() * sizeof(double), 0));
+    HCHECK(hipMalloc(&d_dq, tacs_hip_bdotq_scratch_bytes(R)));
   }
   return 0;
 }
@@ -1493,6 +1352,9 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
   double* tmpC = Bc + (size_t)KD_MAX * m;
   double* zc = tmpC + KD_MAX;
   double* pco = zc + KD_MAX;
+  double* dA = pco + KD_MAX;             // dot products with u (KD_MAX + m + 1)
+  double* dB = dA + (KD_MAX + m + 1);    // dot products with z (KD_MAX + m + 1)
+  double* dcf = dB + (KD_MAX + m + 1);   // update coefficients (2 (KD_MAX + m) + 2)
   double *nrm2 = d_vec, *nrm = d_vec + 64, *inv = d_vec + 128;
   double* h_inv = h_pin;
   auto V = [&](int i) { return d_V + (size_t)i * N; };
@@ -1500,12 +1362,18 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
   auto Cb = [&](int v) { return d_Cb + (size_t)v * N; };
   const float* A32 = prec == 32 ? mat->getValues32(1) : NULL;
   const void* Dinv = prec == 32 ? (const void*)pc->d_Dinv32 : (const void*)pc->d_Dinv;
+  // one right-hand side: the single-vector block-SpMV kernels
   auto spmm64 = [&](const double* x, double* y) {
-    KCHECK(tacs_hip_bsr3_spmm(nr, Rn, mat->d_rowp, mat->d_cols, mat->d_A, x, y, g_stream));
+    if (Rn == 1)
+      KCHECK(tacs_hip_bsr_spmv(3, nr, mat->d_rowp, mat->d_cols, mat->d_A, x, y, 0, g_stream));
+    else
+      KCHECK(tacs_hip_bsr3_spmm(nr, Rn, mat->d_rowp, mat->d_cols, mat->d_A, x, y, g_stream));
   };
   auto op = [&](const double* x, double* y) {
     KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, x, d_work, g_stream));
-    if (prec == 32)
+    if (prec == 32 && Rn == 1)
+      KCHECK(tacs_hip_bsr_spmv_mixed(3, nr, mat->d_rowp, mat->d_cols, A32, d_work, y, 0, g_stream));
+    else if (prec == 32)
       KCHECK(tacs_hip_bsr3_spmm_mixed(nr, Rn, mat->d_rowp, mat->d_cols, A32, d_work, y, g_stream));
     else
       spmm64(d_work, y);
@@ -1571,6 +1439,134 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
   r0.assign(Rn, 0.0);
   rn.assign(Rn, 0.0);
   cycles = 0;
+
+  // Arnoldi steps with delayed reorthogonalisation (DCGS2).  Step i applies
+  // the operator to u_i (in V(i), orthogonalised once at step i - 1); one
+  // sweep of dot products of Q_i = [C, V_0..V_{i-1}] and u_i with u_i and
+  // z = A M^{-1} u_i gives the reorthogonalisation coefficients s of u_i,
+  // |u_i|^2 and the first-pass coefficients t of z.  On the host,
+  // omega = sqrt(|u_i|^2 - |s|^2) finishes column i - 1 (its first-pass
+  // coefficients plus s, subdiagonal omega), and since
+  // A M^{-1} v_i = (z - A M^{-1} Q_i s) / omega, with A M^{-1} V s_V expanded
+  // by the finished columns (the C part of s, of rounding size, is dropped),
+  // the first-pass column i is ([t; tau] - [B; H] s_V) / omega.  One update
+  // sweep then forms v_i = (u_i - Q_i s) / omega and
+  // u_{i+1} = (z - Q_i t - v_i tau) / omega.  The stopping test of column
+  // i - 1 is made at step i.  Returns the number of finished columns.
+  auto arnoldi_delayed = [&](int s, int Kd) -> int {
+    std::vector<std::vector<double>> pend(Rn);
+    std::vector<double> Ah, Bh, cf, c;
+    std::vector<char> live(Rn);
+    bool all = true;
+    for (int j = 0; j < Rn; j++) all = all && conv[j];
+    int i = 0;
+    for (; i < s && !all; i++) {
+      const int p = Kd + i;
+      for (int j = 0; j < Rn; j++) live[j] = !conv[j];
+      op(V(i), V(i + 1));
+      KCHECK(tacs_hip_bdotq(n, Rn, 2, Kd, Cb(0), i + 1, V(0), N, V(i), V(i + 1), (double*)d_dq,
+                            dA, dB, S, g_stream));
+      Ah.assign((size_t)Rn * (p + 1), 0.0);
+      Bh.assign((size_t)Rn * (p + 1), 0.0);
+      d2h_rows(Ah.data(), p + 1, dA, S);
+      d2h_rows(Bh.data(), p + 1, dB, S);
+      cf.assign((size_t)Rn * (2 * p + 2), 0.0);
+      all = true;
+      for (int j = 0; j < Rn; j++) {
+        if (!live[j]) continue;
+        const int kd = kdj[j];
+        const double* a = &Ah[(size_t)j * (p + 1)];
+        const double* b = &Bh[(size_t)j * (p + 1)];
+        double ss = 0.0, sb = 0.0;
+        for (int r = 0; r < p; r++) {
+          ss += a[r] * a[r];
+          sb += a[r] * b[r];
+        }
+        const double w2 = a[p] - ss;
+        const double om = (w2 > 0.0 && std::isfinite(w2)) ? sqrt(w2) : 0.0;
+        // least-squares row order: C (kd rows), V_0..V_{i-1}
+        auto sL = [&](int r) { return r < kd ? a[r] : a[Kd + (r - kd)]; };
+        auto tL = [&](int r) { return r < kd ? b[r] : b[Kd + (r - kd)]; };
+        HostLS& L = ls[j];
+        if (i > 0) {
+          std::vector<double>& col = pend[j];
+          col.resize((size_t)kd + i + 1);
+          for (int r = 0; r < kd + i; r++) col[r] += sL(r);
+          col[kd + i] = om;
+          const double st = L.add_column(kd + i - 1, col.data());
+          if (!std::isfinite(st)) {
+            conv[j] = 1;
+            fail[j] = 1;
+            jend[j] = i - 1;
+            continue;
+          }
+          if (st <= tgt[j] || om == 0.0) {
+            conv[j] = 1;
+            jend[j] = i;
+          }
+        } else {
+          L.c[kd] = L.g[kd] = om;  // u_0 = C s_C + omega v_0
+        }
+        const double io = om > 0.0 ? 1.0 / om : 0.0;
+        const double tau = (b[p] - sb) * io;
+        c.assign((size_t)kd + i + 1, 0.0);
+        for (int r = 0; r < kd + i; r++) c[r] = tL(r);
+        c[kd + i] = tau;
+        for (int a2 = 0; a2 < i; a2++) {
+          const double sv = a[Kd + a2];
+          const double* colf = &L.Hb[(size_t)(kd + a2) * L.ld];
+          for (int r = 0; r <= kd + a2 + 1; r++) c[r] -= colf[r] * sv;
+        }
+        for (double& x : c) x *= io;
+        pend[j] = c;
+        double* cj = &cf[(size_t)j * (2 * p + 2)];
+        for (int r = 0; r < p; r++) {
+          cj[r] = a[r];
+          cj[p + r] = b[r];
+        }
+        cj[2 * p] = io;
+        cj[2 * p + 1] = tau;
+        if (!conv[j]) all = false;
+      }
+      h2d_rows(dcf, S, cf.data(), 2 * p + 2);
+      KCHECK(tacs_hip_bdcgs_update(n, Rn, Kd, Cb(0), i, V(0), N, dcf, S, V(i), V(i + 1), g_stream));
+    }
+    if (all) return i;
+    // m - Kd steps done: finish column s - 1 and v_s
+    const int p = Kd + s;
+    KCHECK(tacs_hip_bdotq(n, Rn, 1, Kd, Cb(0), s + 1, V(0), N, V(s), NULL, (double*)d_dq, dA, NULL,
+                          S, g_stream));
+    Ah.assign((size_t)Rn * (p + 1), 0.0);
+    d2h_rows(Ah.data(), p + 1, dA, S);
+    cf.assign((size_t)Rn * (2 * p + 2), 0.0);
+    for (int j = 0; j < Rn; j++) {
+      if (conv[j]) continue;
+      const int kd = kdj[j];
+      const double* a = &Ah[(size_t)j * (p + 1)];
+      double ss = 0.0;
+      for (int r = 0; r < p; r++) ss += a[r] * a[r];
+      const double w2 = a[p] - ss;
+      const double om = (w2 > 0.0 && std::isfinite(w2)) ? sqrt(w2) : 0.0;
+      std::vector<double>& col = pend[j];
+      col.resize((size_t)kd + s + 1);
+      for (int r = 0; r < kd + s; r++) col[r] += r < kd ? a[r] : a[Kd + (r - kd)];
+      col[kd + s] = om;
+      const double st = ls[j].add_column(kd + s - 1, col.data());
+      if (!std::isfinite(st)) {
+        conv[j] = 1;
+        fail[j] = 1;
+        jend[j] = s - 1;
+        continue;
+      }
+      double* cj = &cf[(size_t)j * (2 * p + 2)];
+      for (int r = 0; r < p; r++) cj[r] = a[r];
+      cj[2 * p] = om > 0.0 ? 1.0 / om : 0.0;
+    }
+    h2d_rows(dcf, S, cf.data(), 2 * p + 2);
+    KCHECK(tacs_hip_bdcgs_update(n, Rn, Kd, Cb(0), s, V(0), N, dcf, S, V(s), NULL, g_stream));
+    return s;
+  };
+
   if (zero_guess) {
     KCHECK(tacs_hip_fill(64, N, Xd, 0.0, g_stream));
     HCHECK(hipMemcpyAsync(V(0), Bd, N * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
@@ -1645,6 +1641,10 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
     int niters = 0;
     bool all = true;
     for (int j = 0; j < Rn; j++) all = all && conv[j];
+    if (delayed) {
+      niters = arnoldi_delayed(s, Kd);
+      all = true;  // skips the classical loop below
+    }
     for (int i = 0; i < s && !all; i++) {
       double* q = V(i + 1);
       double* h = H + (size_t)i * (m + 1);

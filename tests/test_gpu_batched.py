@@ -176,8 +176,8 @@ def test_batched_with_recycled_space(system, prec):
     assert it_after == plain.getStats()[0][0]
 
 
-@pytest.mark.parametrize("prec", [32, 64])
-def test_batched_gcrodr_matches_dense_and_single_recycling(system, prec):
+@pytest.mark.parametrize("prec,delayed", [(32, True), (64, True), (32, False)])
+def test_batched_gcrodr_matches_dense_and_single_recycling(system, prec, delayed):
     """Per right-hand side recycling (batched GCRO-DR): solutions against the
     dense solve; with the FP32-valued operator each system takes the steps of
     one MixedKSM with the same deflation solving it alone (the same cycle,
@@ -187,6 +187,7 @@ def test_batched_gcrodr_matches_dense_and_single_recycling(system, prec):
     rng = np.random.default_rng(15)
     bs = _load_cases(p, rng)
     ksm = TACS.BatchedKSM(p.mat, pc, m=20, restarts=300, nrhs=8, prec=prec, deflation=6)
+    ksm.setDelayedReorthogonalization(delayed)
     ksm.setTolerances(1e-10, 1e-30)
     xs = [a.createVec() for _ in bs]
     assert ksm.solve(bs, xs) == 1
