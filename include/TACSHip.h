@@ -415,7 +415,7 @@ class TACSBatchedGMRES : public TACSObject {
   double *d_Ub, *d_Cb, *d_Unb, *d_Cnb;  // K block vectors each
   double* d_prod;    // per right-hand side small matrices (products, basis changes)
   double* h_stage;   // pinned staging of the small matrices, R x prodStride()
-  void* d_dq;        // scratch of the delayed Gram-Schmidt dot products
+  void* d_dq;        // scratch of the delayed Gram-Schmidt dots and the products
   // Gram-Schmidt of the recycling (GCRO-DR) Arnoldi steps: 1 (default)
   // delayed classical Gram-Schmidt (DCGS2, two sweeps over the basis per
   // step), 0 classical Gram-Schmidt twice (four sweeps)

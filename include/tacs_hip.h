@@ -345,6 +345,12 @@ int tacs_hip_bbasis_combine(size_t n, int R, int nin, const double* W, size_t ld
                             size_t sP, int ldp, int nout, double* out, size_t ldo, int add,
                             hipStream_t stream);
 int tacs_hip_bbroadcast(size_t n, int R, const double* x, double* X, hipStream_t stream);
+/* Per right-hand side tall-skinny products of block vectors:
+   out[j*sR + b*ldo + v] = X_v(:, j) . Y_b(:, j), v < p, b < q (X_v, Y_b at
+   stride ldw); scratch: tacs_hip_btsgemm_scratch_bytes. */
+size_t tacs_hip_btsgemm_scratch_bytes(int R);
+int tacs_hip_btsgemm(size_t n, int R, int p, const double* X, int q, const double* Y, size_t ldw,
+                     double* scratch, double* out, size_t sR, int ldo, hipStream_t stream);
 int tacs_hip_bpack(size_t n, int R, int j, const double* x, double* X, hipStream_t stream);
 int tacs_hip_bunpack(size_t n, int R, int j, const double* X, double* x, hipStream_t stream);
 

@@ -507,6 +507,91 @@ __global__ __launch_bounds__(BBC_THREADS) void bbasis_combine(size_t n, int R, i
   }
 }
 
+// Per right-hand side tall-skinny products out_j = X_j^T Y_j (X_j: pc block
+// vectors, Y_j: qc block vectors, interleaved), the chunk (v0.., b0..) of a
+// p x q product.  A block stages BTS_TR rows of all pc + qc vectors for every
+// right-hand side in LDS ([vector][row][j]); thread tiles are 4 x 4 entries
+// (v, b) of one right-hand side, at most BTS_MAXT tiles per thread, so each
+// LDS read serves 4 multiply-adds.  part[(blk * R + j) * pc * qc + e] holds a
+// block's sums (e = b * pc + v).
+#define BTS_TR 4
+#define BTS_MAXT 2
+#define BTS_BLOCKS 512
+__global__ __launch_bounds__(256) void btsgemm_partial(size_t n, int R, int pc, const double* __restrict__ X,
+                                                       int qc, const double* __restrict__ Y,
+                                                       size_t ldw, double* __restrict__ part) {
+  extern __shared__ double tile[];  // [(pc + qc)][BTS_TR][R]
+  const int P4 = (pc + 3) / 4, Q4 = (qc + 3) / 4;
+  const int ntiles = R * P4 * Q4;
+  double acc[BTS_MAXT][16];
+#pragma unroll
+  for (int t = 0; t < BTS_MAXT; t++)
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[t][e] = 0.0;
+  const int nv = pc + qc;
+  for (size_t r0 = (size_t)blockIdx.x * BTS_TR; r0 < n; r0 += (size_t)gridDim.x * BTS_TR) {
+    const int nr = (int)((n - r0) < BTS_TR ? (n - r0) : BTS_TR);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nv * BTS_TR * R; t += 256) {
+      const int v = t / (BTS_TR * R), rr = (t / R) % BTS_TR, j = t % R;
+      double val = 0.0;
+      if (rr < nr) {
+        const size_t o = (r0 + rr) * R + j;
+        val = v < pc ? X[(size_t)v * ldw + o] : Y[(size_t)(v - pc) * ldw + o];
+      }
+      tile[t] = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tt = 0; tt < BTS_MAXT; tt++) {
+      const int e = threadIdx.x + 256 * tt;
+      if (e < ntiles) {
+        const int j = e % R, v4 = (e / R) % P4, b4 = e / (R * P4);
+        for (int rr = 0; rr < BTS_TR; rr++) {
+          double xv[4], yv[4];
+#pragma unroll
+          for (int a = 0; a < 4; a++) {
+            const int v = min(4 * v4 + a, pc - 1), b = min(4 * b4 + a, qc - 1);
+            xv[a] = tile[((size_t)v * BTS_TR + rr) * R + j];
+            yv[a] = tile[((size_t)(pc + b) * BTS_TR + rr) * R + j];
+          }
+#pragma unroll
+          for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[tt][a * 4 + c] += xv[a] * yv[c];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int tt = 0; tt < BTS_MAXT; tt++) {
+    const int e = threadIdx.x + 256 * tt;
+    if (e < ntiles) {
+      const int j = e % R, v4 = (e / R) % P4, b4 = e / (R * P4);
+      double* o = &part[((size_t)blockIdx.x * R + j) * pc * qc];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const int v = 4 * v4 + a, b = 4 * b4 + c;
+          if (v < pc && b < qc) o[(size_t)b * pc + v] = acc[tt][a * 4 + c];
+        }
+    }
+  }
+}
+
+// out[j*sR + (b0 + b)*ldo + v0 + v] = sum over blocks
+__global__ void btsgemm_finalize(int nb, int R, int pc, int qc, const double* __restrict__ part,
+                                 double* __restrict__ out, size_t sR, int ldo, int v0, int b0) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = pc * qc;
+  if (t >= R * per) return;
+  const int j = t / per, e = t % per, v = e % pc, b = e / pc;
+  double s = 0.0;
+  for (int blk = 0; blk < nb; blk++) s += part[((size_t)blk * R + j) * per + e];
+  out[(size_t)j * sR + (size_t)(b0 + b) * ldo + v0 + v] = s;
+}
+
 // X[i*R + j] = x[i] for every j
 __global__ void bbroadcast(size_t n, int R, const double* __restrict__ x, double* __restrict__ X) {
   const size_t N = n * R;
@@ -723,6 +808,45 @@ int tacs_hip_bbasis_combine(size_t n, int R, int nin, const double* W, size_t ld
 
 int tacs_hip_bbroadcast(size_t n, int R, const double* x, double* X, hipStream_t stream) {
   hipLaunchKernelGGL(bbroadcast, dim3(bgrid(n * R)), dim3(256), 0, stream, n, R, x, X);
+  return (int)hipGetLastError();
+}
+
+size_t tacs_hip_btsgemm_scratch_bytes(int R) {
+  (void)R;
+  return (size_t)BTS_BLOCKS * 256 * BTS_MAXT * 16 * sizeof(double);
+}
+
+int tacs_hip_btsgemm(size_t n, int R, int p, const double* X, int q, const double* Y, size_t ldw,
+                     double* scratch, double* out, size_t sR, int ldo, hipStream_t stream) {
+  if (R < 1 || R > 64 || p < 0 || q < 0) return TACS_HIP_ERR_ARG;
+  if (p == 0 || q == 0) return 0;
+  // chunks of pc x qc with R * ceil(pc/4) * ceil(qc/4) <= 256 * BTS_MAXT tiles
+  // and (pc + qc) BTS_TR R doubles of LDS <= 64 KB
+  const int maxt = 256 * BTS_MAXT;
+  int Q4 = (q + 3) / 4, P4 = (p + 3) / 4;
+  int q4c = Q4, p4c = P4;
+  while ((long)R * p4c * q4c > maxt || (long)(4 * (p4c + q4c)) * BTS_TR * R * 8 > 65536) {
+    if (p4c >= q4c && p4c > 1)
+      p4c = (p4c + 1) / 2;
+    else if (q4c > 1)
+      q4c = (q4c + 1) / 2;
+    else
+      return TACS_HIP_ERR_ARG;
+  }
+  size_t nb = (n + BTS_TR - 1) / BTS_TR;
+  if (nb > BTS_BLOCKS) nb = BTS_BLOCKS;
+  for (int b0 = 0; b0 < q; b0 += 4 * q4c) {
+    const int qc = q - b0 < 4 * q4c ? q - b0 : 4 * q4c;
+    for (int v0 = 0; v0 < p; v0 += 4 * p4c) {
+      const int pc = p - v0 < 4 * p4c ? p - v0 : 4 * p4c;
+      const size_t lds = (size_t)(pc + qc) * BTS_TR * R * sizeof(double);
+      hipLaunchKernelGGL(btsgemm_partial, dim3((int)nb), dim3(256), lds, stream, n, R, pc,
+                         X + (size_t)v0 * ldw, qc, Y + (size_t)b0 * ldw, ldw, scratch);
+      const int tot = R * pc * qc;
+      hipLaunchKernelGGL(btsgemm_finalize, dim3((tot + 255) / 256), dim3(256), 0, stream, (int)nb,
+                         R, pc, qc, scratch, out, sR, ldo, v0, b0);
+    }
+  }
   return (int)hipGetLastError();
 }
 

@@ -34,21 +34,26 @@ __global__ __launch_bounds__(256) void bsr3_spmv(int nrows, const int* __restric
   const int rc = lane - 9 * bo;
   const int c = rc % 3;
   const bool ok = lane < 63;
-  const int nw = gridDim.x * 4;
+  // XCD-aware rows: workgroups go round-robin to the 8 XCDs, so XCD x
+  // (blockIdx % 8) takes the contiguous eighth x of the rows, and the x
+  // entries that neighbouring rows share stay in that XCD's L2
+  const int xcd = blockIdx.x & 7, nbx = gridDim.x >> 3;
+  const int rbeg = (int)((long)nrows * xcd / 8), rend = (int)((long)nrows * (xcd + 1) / 8);
+  const int nw = nbx * 4;
   // each wave handles R consecutive rows per pass, with every load of the R
   // rows in flight together
-  for (int row0 = (blockIdx.x * 4 + wid) * R; row0 < nrows; row0 += nw * R) {
+  for (int row0 = rbeg + ((blockIdx.x >> 3) * 4 + wid) * R; row0 < rend; row0 += nw * R) {
     int rs[R], nb[R];
     bool fast = true;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       int row = row0 + r;
-      rs[r] = row < nrows ? rowp[row] : 0;
-      nb[r] = row < nrows ? rowp[row + 1] - rs[r] : 0;
+      rs[r] = row < rend ? rowp[row] : 0;
+      nb[r] = row < rend ? rowp[row + 1] - rs[r] : 0;
       fast = fast && nb[r] <= 7 * NT;
     }
     ACC acc[R];
-    if (fast) {
+    if (fast && sizeof(TA) == 8) {
       int col[R][NT];
       TA av[R][NT];
 #pragma unroll
@@ -74,6 +79,37 @@ __global__ __launch_bounds__(256) void bsr3_spmv(int nrows, const int* __restric
         for (int t = 0; t < NT; t++)
           if (col[r][t] >= 0) acc[r] += (ACC)av[r][t] * (ACC)x[3 * col[r][t] + c];
       }
+    } else if (fast) {
+      // FP32 values: every load is unconditional (an absent block reads
+      // entry 0 and is masked in the sum), so the compiler issues all of them
+      // before the first wait instead of one branch-and-wait per block (with
+      // the conditional form each FP32 load was followed by its wait and
+      // conversion)
+      int col[R][NT];
+      TA av[R][NT];
+      T xv[R][NT];
+      bool use[R][NT];
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          const int s = bo + 7 * t;
+          use[r][t] = ok && s < nb[r];
+          const int e = use[r][t] ? rs[r] + s : 0;
+          col[r][t] = cols[e];
+          av[r][t] = __builtin_nontemporal_load(&A[(size_t)e * 9 + (use[r][t] ? rc : 0)]);
+        }
+#pragma unroll
+      for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) xv[r][t] = x[3 * col[r][t] + c];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        acc[r] = 0;
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+          acc[r] += use[r][t] ? (ACC)av[r][t] * (ACC)xv[r][t] : (ACC)0;
+      }
     } else {
 #pragma unroll
       for (int r = 0; r < R; r++) {
@@ -94,7 +130,7 @@ __global__ __launch_bounds__(256) void bsr3_spmv(int nrows, const int* __restric
       ACC tot = 0;
 #pragma unroll
       for (int b = 0; b < 7; b++) tot += __shfl(s3, (9 * b + 3 * lane) & 63, 64);
-      if (lane < 3 && row0 + r < nrows) {
+      if (lane < 3 && row0 + r < rend) {
         size_t o = 3 * (size_t)(row0 + r) + lane;
         y[o] = add ? (T)((ACC)y[o] + tot) : (T)tot;
       }
@@ -274,7 +310,7 @@ extern "C" {
 int tacs_hip_bsr_spmv(int b, int nrows, const int* rowp, const int* cols, const double* A,
                       const double* x, double* y, int add, hipStream_t stream) {
   if (nrows <= 0) return 0;
-  int grid = tacs_grid_for(nrows, 8, 65536);
+  int grid = 8 * tacs_grid_for((nrows + 7) / 8, 8, 8192);  // a multiple of 8 (XCD shares)
   if (b == 3)
     hipLaunchKernelGGL((bsr3_spmv<double, double, double, 4, 2>), dim3(grid), dim3(256), 0, stream, nrows, rowp,
                        cols, A, x, y, add);
@@ -289,7 +325,7 @@ int tacs_hip_bsr_spmv(int b, int nrows, const int* rowp, const int* cols, const 
 int tacs_hip_bsr_spmv_f32(int b, int nrows, const int* rowp, const int* cols, const float* A,
                           const float* x, float* y, int add, hipStream_t stream) {
   if (nrows <= 0) return 0;
-  int grid = tacs_grid_for(nrows, 8, 65536);
+  int grid = 8 * tacs_grid_for((nrows + 7) / 8, 8, 8192);  // a multiple of 8 (XCD shares)
   if (b == 3)
     hipLaunchKernelGGL((bsr3_spmv<float, float, float, 4, 2>), dim3(grid), dim3(256), 0, stream, nrows, rowp,
                        cols, A, x, y, add);
@@ -304,7 +340,7 @@ int tacs_hip_bsr_spmv_f32(int b, int nrows, const int* rowp, const int* cols, co
 int tacs_hip_bsr_spmv_mixed(int b, int nrows, const int* rowp, const int* cols, const float* A,
                             const double* x, double* y, int add, hipStream_t stream) {
   if (nrows <= 0) return 0;
-  int grid = tacs_grid_for(nrows, 8, 65536);
+  int grid = 8 * tacs_grid_for((nrows + 7) / 8, 8, 8192);  // a multiple of 8 (XCD shares)
   if (b == 3)
     hipLaunchKernelGGL((bsr3_spmv<float, double, double, 4, 2>), dim3(grid), dim3(256), 0, stream,
                        nrows, rowp, cols, A, x, y, add);

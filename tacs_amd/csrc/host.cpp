@@ -1126,7 +1126,8 @@ int TACSBatchedGMRES::setDeflation(int k) {
     d_Cnb = dalloc<double>((size_t)k * N);
     d_prod = dalloc<double>((size_t)R * prodStride());
     HCHECK(hipHostMalloc((void**)&h_stage, (size_t)R * prodStride() * sizeof(double), 0));
-    HCHECK(hipMalloc(&d_dq, tacs_hip_bdotq_scratch_bytes(R)));
+    HCHECK(hipMalloc(&d_dq, std::max(tacs_hip_bdotq_scratch_bytes(R),
+                                     tacs_hip_btsgemm_scratch_bytes(R))));
   }
   return 0;
 }
@@ -1726,12 +1727,11 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       const int p = Kd + jm + 1;  // rows of [C, V_0..V_jm]^T U_b
       std::vector<double> prodh((size_t)Rn * std::max(p * Kd, 1), 0.0);
       if (Kd > 0) {
-        for (int b = 0; b < Kd; b++) {
-          KCHECK(tacs_hip_bmdot(n, Rn, Kd, Cb(0), N, Ub(b), (double*)d_red, d_prod + (size_t)b * p,
-                                SP, 0, g_stream));
-          KCHECK(tacs_hip_bmdot(n, Rn, jm + 1, V(0), N, Ub(b), (double*)d_red,
-                                d_prod + (size_t)b * p + Kd, SP, 0, g_stream));
-        }
+        // column b of [C, V_0..V_jm]^T U_b at d_prod + b p (per right-hand side)
+        KCHECK(tacs_hip_btsgemm(n, Rn, Kd, Cb(0), Kd, Ub(0), N, (double*)d_dq, d_prod, SP, p,
+                                g_stream));
+        KCHECK(tacs_hip_btsgemm(n, Rn, jm + 1, V(0), Kd, Ub(0), N, (double*)d_dq, d_prod + Kd, SP,
+                                p, g_stream));
         d2h_rows(prodh.data(), p * Kd, d_prod, SP);
       }
       std::vector<int> kk(Rn, 0);

@@ -271,3 +271,51 @@ def test_spmm_matches_oracle(system, R):
         assert np.abs(Yh[:, j] - yo).max() <= 1e-13 * np.abs(yo).max()
         yo32 = O.bsr_mult(3, rowp, cols, A32h, Xh[:, j])
         assert np.abs(Y32[:, j] - yo32).max() <= 1e-13 * np.abs(yo32).max()
+
+
+@pytest.mark.parametrize("R,p,q", [(1, 9, 5), (3, 5, 7), (16, 61, 20), (64, 40, 32)])
+def test_batched_products_and_basis_change(R, p, q):
+    """Per right-hand side products X_j^T Y_j and basis changes W_j P_j of
+    interleaved block vectors against numpy, incl. the chunked launch
+    shapes (large R x p x q)."""
+    import torch
+
+    from tacs_amd import lib
+
+    n = 3 * 517
+    rng = np.random.default_rng(R + p + q)
+    X = rng.standard_normal((p, n, R))
+    Y = rng.standard_normal((q, n, R))
+    Xd = torch.tensor(X.reshape(-1), device="cuda")
+    Yd = torch.tensor(Y.reshape(-1), device="cuda")
+    ldo, sR = p + 3, (p + 3) * q + 5
+    out = torch.zeros(R * sR, dtype=torch.float64, device="cuda")
+    scratch = torch.empty(lib().tacs_hip_btsgemm_scratch_bytes(R) // 8 + 1, dtype=torch.float64,
+                          device="cuda")
+    torch.cuda.synchronize()
+    assert lib().tacs_hip_btsgemm(n, R, p, Xd.data_ptr(), q, Yd.data_ptr(), n * R,
+                                  scratch.data_ptr(), out.data_ptr(), sR, ldo, None) == 0
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    for j in range(R):
+        ref = X[:, :, j] @ Y[:, :, j].T  # p x q
+        got = np.array([[o[j * sR + b * ldo + v] for b in range(q)] for v in range(p)])
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    # basis change: out_c(:, j) = sum_v X_v(:, j) P_j[v + c*p], c < min(q, 32)
+    nout = min(q, 32)
+    P = rng.standard_normal((R, nout, p))  # P[j][c][v]
+    sP = p * nout + 7
+    Pd = torch.zeros(R * sP, dtype=torch.float64, device="cuda")
+    Ph = np.zeros(R * sP)
+    for j in range(R):
+        Ph[j * sP:j * sP + p * nout] = P[j].reshape(-1)  # v fastest within column c
+    Pd.copy_(torch.tensor(Ph))
+    Od = torch.full((nout * n * R,), 2.0, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    assert lib().tacs_hip_bbasis_combine(n, R, p, Xd.data_ptr(), n * R, Pd.data_ptr(), sP, p, nout,
+                                         Od.data_ptr(), n * R, 1, None) == 0
+    torch.cuda.synchronize()
+    Oh = Od.cpu().numpy().reshape(nout, n, R)
+    for j in range(R):
+        ref = 2.0 + np.einsum("vn,cv->cn", X[:, :, j], P[j])
+        assert np.abs(Oh[:, :, j] - ref).max() <= 1e-12 * np.abs(ref).max()
