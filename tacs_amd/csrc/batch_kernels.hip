@@ -429,6 +429,71 @@ __global__ __launch_bounds__(256) void bsr3_spmm_slots(int nrows, int R,
   }
 }
 
+// The same product for R divisible by 4: each lane takes four consecutive
+// right-hand sides (16-byte loads of X, 12 sums), so NL = R/4 lanes cover a
+// block and a wave works on G = 64/NL blocks at once.
+typedef double spmm_dbl2 __attribute__((ext_vector_type(2)));
+template <typename TA, int NL>
+__global__ __launch_bounds__(256) void bsr3_spmm_quad(int nrows, int R,
+                                                      const int* __restrict__ rowp,
+                                                      const int* __restrict__ cols,
+                                                      const TA* __restrict__ A,
+                                                      const double* __restrict__ X,
+                                                      double* __restrict__ Y) {
+  constexpr int G = 64 / NL;
+  const int lane = threadIdx.x & 63;
+  const int jq = lane % NL, g = lane / NL;  // right-hand sides 4 jq .. 4 jq + 3
+  const bool okq = 4 * jq < R;               // NL is R/4 rounded up to a power of two
+  const int wid = threadIdx.x >> 6;
+  const int xcd = blockIdx.x & 7, nbx = gridDim.x >> 3;
+  const int r0 = (int)((long)nrows * xcd / 8), r1 = (int)((long)nrows * (xcd + 1) / 8);
+  for (int row = r0 + (blockIdx.x >> 3) * 4 + wid; row < r1; row += nbx * 4) {
+    const int rs = rowp[row], re = rowp[row + 1];
+    double y[3][4];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) y[r][q] = 0.0;
+    for (int s = rs + g; okq && s < re; s += G) {
+      const int col = cols[s];
+      const TA* a = &A[(size_t)9 * s];
+      double av[9];
+#pragma unroll
+      for (int e = 0; e < 9; e++) av[e] = (double)a[e];
+      const double* xb = &X[(size_t)3 * col * R + 4 * jq];
+      double xv[3][4];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const spmm_dbl2 lo = *(const spmm_dbl2*)(xb + (size_t)c * R);
+        const spmm_dbl2 hi = *(const spmm_dbl2*)(xb + (size_t)c * R + 2);
+        xv[c][0] = lo.x;
+        xv[c][1] = lo.y;
+        xv[c][2] = hi.x;
+        xv[c][3] = hi.y;
+      }
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          y[r][q] += av[3 * r] * xv[0][q] + av[3 * r + 1] * xv[1][q] + av[3 * r + 2] * xv[2][q];
+    }
+#pragma unroll
+    for (int off = NL; off < 64; off <<= 1)
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) y[r][q] += __shfl_xor(y[r][q], off, 64);
+    if (g == 0 && okq) {
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        double* yo = &Y[((size_t)3 * row + r) * R + 4 * jq];
+        *(spmm_dbl2*)yo = spmm_dbl2{y[r][0], y[r][1]};
+        *(spmm_dbl2*)(yo + 2) = spmm_dbl2{y[r][2], y[r][3]};
+      }
+    }
+  }
+}
+
 template <typename TA>
 static int spmm_slots_launch(int nrows, int R, const int* rowp, const int* cols, const TA* A,
                              const double* X, double* Y, hipStream_t stream) {
@@ -436,6 +501,21 @@ static int spmm_slots_launch(int nrows, int R, const int* rowp, const int* cols,
   if (R > 64) return TACS_HIP_ERR_ARG;
   // a multiple of 8 workgroups (one share per XCD)
   const int grid = 8 * tacs_grid_for((nrows + 7) / 8, 4, 8192);
+  if (R % 4 == 0) {
+    // 16-byte aligned X rows: four right-hand sides per lane
+#define SPMM_QUAD(NLV)                                                                          \
+  hipLaunchKernelGGL((bsr3_spmm_quad<TA, NLV>), dim3(grid), dim3(256), 0, stream, nrows, R, rowp, \
+                     cols, A, X, Y)
+    switch (pow2_ceil(R / 4)) {
+      case 1: SPMM_QUAD(1); break;
+      case 2: SPMM_QUAD(2); break;
+      case 4: SPMM_QUAD(4); break;
+      case 8: SPMM_QUAD(8); break;
+      default: SPMM_QUAD(16); break;
+    }
+#undef SPMM_QUAD
+    return (int)hipGetLastError();
+  }
 #define SPMM_CASE(RPV)                                                                          \
   hipLaunchKernelGGL((bsr3_spmm_slots<TA, RPV>), dim3(grid), dim3(256), 0, stream, nrows, R, rowp, \
                      cols, A, X, Y)
@@ -467,7 +547,8 @@ __global__ __launch_bounds__(BBC_THREADS) void bbasis_combine(size_t n, int R, i
                                                               size_t sP, int ldp, int nout,
                                                               double* __restrict__ out, size_t ldo,
                                                               int add, int vch) {
-  extern __shared__ double Ps[];  // [R][vch][nout]
+  extern __shared__ double Ps[];  // [R][vch][nout], right-hand side stride js (odd: no bank conflicts)
+  const int js = vch * nout + 1;
   const size_t N = n * R;
   for (size_t base = (size_t)blockIdx.x * BBC_THREADS; base < N;
        base += (size_t)gridDim.x * BBC_THREADS) {
@@ -483,13 +564,13 @@ __global__ __launch_bounds__(BBC_THREADS) void bbasis_combine(size_t n, int R, i
       for (int e = threadIdx.x; e < R * vc * nout; e += BBC_THREADS) {
         const int jj = e / (vc * nout), r = e - jj * (vc * nout);
         const int v = r / nout, c = r - v * nout;
-        Ps[((size_t)jj * vch + v) * nout + c] = P[(size_t)jj * sP + v0 + v + (size_t)c * ldp];
+        Ps[(size_t)jj * js + v * nout + c] = P[(size_t)jj * sP + v0 + v + (size_t)c * ldp];
       }
       __syncthreads();
       if (ok) {
         for (int v = 0; v < vc; v++) {
           const double w = W[(size_t)(v0 + v) * ldw + t];
-          const double* pp = &Ps[((size_t)j * vch + v) * nout];
+          const double* pp = &Ps[(size_t)j * js + v * nout];
 #pragma unroll
           for (int c = 0; c < BBC_MAXOUT; c++)
             if (c < nout) acc[c] += w * pp[c];
@@ -796,9 +877,9 @@ int tacs_hip_bbasis_combine(size_t n, int R, int nin, const double* W, size_t ld
                             size_t sP, int ldp, int nout, double* out, size_t ldo, int add,
                             hipStream_t stream) {
   if (nout < 1 || nout > BBC_MAXOUT || nin < 0 || R < 1 || R > 64) return TACS_HIP_ERR_ARG;
-  int vch = (int)(65536 / ((size_t)R * nout * sizeof(double)));  // >= 4
+  int vch = (int)((65536 / ((size_t)R * sizeof(double)) - 1) / nout);  // >= 3
   if (vch > 8) vch = 8;
-  const size_t lds = (size_t)R * vch * nout * sizeof(double);
+  const size_t lds = (size_t)R * (vch * nout + 1) * sizeof(double);
   size_t g = (n * R + BBC_THREADS - 1) / BBC_THREADS;
   if (g > 4096) g = 4096;
   hipLaunchKernelGGL(bbasis_combine, dim3((int)g), dim3(BBC_THREADS), lds, stream, n, R, nin, W,

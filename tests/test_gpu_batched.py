@@ -239,7 +239,7 @@ def test_batched_gcrodr_seeded_from_shared_space(system):
         assert np.abs(x.getArray() - xo).max() <= 1e-7 * np.abs(xo).max()
 
 
-@pytest.mark.parametrize("R", [1, 3, 16, 33])
+@pytest.mark.parametrize("R", [1, 3, 4, 12, 16, 33, 64])
 def test_spmm_matches_oracle(system, R):
     """The multi right-hand-side SpMM (FP64 values and FP32-rounded values)
     against the oracle's block-CSR product, one right-hand side at a time."""
