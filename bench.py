@@ -159,23 +159,22 @@ def run_step(p, pc, ksm, want_grad=False):
     return out
 
 
-def cpu_baseline(p, gpu_its, budget_s=25.0):
+def cpu_baseline(p, gpu_its, order=2, nrhs=1, name="128^3"):
     """The oracle (CPU restatement of the reference path, one core) timed on a
-    bounded sample of the same 128^3 workload: the assembly of a slice of the
+    bounded sample of the same workload: the assembly of a slice of the
     elements and the first Arnoldi steps of GMRES(30) on the full matrix;
-    scaled to the whole step with the GPU FP64 run's iteration count."""
+    scaled to the whole step with the GPU FP64 run's iteration count (per
+    load case, nrhs load cases)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     m = p.mesh
     C = O.iso_C21(70e9, 0.3)
-    t0 = time.perf_counter()
     rowp, cols, A = p.mat.getArrays()
-    t_setup = time.perf_counter() - t0
-    nsample = max(1, m.nelem // 64)
+    nsample = max(1, m.nelem // (64 if order == 2 else 256))
     conn_s = np.ascontiguousarray(m.conn[:nsample])
     Xs = m.X.reshape(-1)
     t0 = time.perf_counter()
-    O.assemble_solid(2, conn_s, Xs, np.zeros(3 * m.nnodes), np.ones(nsample), C, rowp, cols)
+    O.assemble_solid(order, conn_s, Xs, np.zeros(3 * m.nnodes), np.ones(nsample), C, rowp, cols)
     t_asm = (time.perf_counter() - t0) * (m.nelem / nsample)
     Aflat = A.reshape(-1)
     D = O.bsr_diag_inv(3, rowp, cols, Aflat)
@@ -185,15 +184,21 @@ def cpu_baseline(p, gpu_its, budget_s=25.0):
     O.gmres(3, rowp, cols, Aflat, rhs, D, m=30, nrestart=0, rtol=1e-30, max_its=kits)
     t_it = (time.perf_counter() - t0) / kits
     its = sum(gpu_its)
-    step = t_asm + its * t_it
-    return {"value": m.nelem / step / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle (C restatement, 1 core) on the 128^3 workload: assembly of "
-                       f"{nsample} of {m.nelem} elements ({t_asm:.1f} s scaled), first {kits} "
-                       f"Arnoldi steps of FP64 GMRES(30)+block-Jacobi on the full matrix "
-                       f"({t_it:.2f} s/step), scaled to the {its} steps the GPU FP64 path took "
-                       f"(later steps orthogonalise against more vectors, so this "
-                       f"underestimates the CPU time)"),
-            "step_s_estimate": step}
+    step = t_asm + nrhs * its * t_it
+    out = {"cores": 1, "kind": "port",
+           "sample": (f"oracle (C restatement, 1 core) on the {name} workload: assembly of "
+                      f"{nsample} of {m.nelem} elements ({t_asm:.1f} s scaled), first {kits} "
+                      f"Arnoldi steps of FP64 GMRES(30)+block-Jacobi on the full matrix "
+                      f"({t_it:.2f} s/step), scaled to the {its} steps per load case the GPU "
+                      f"FP64 path took" + (f", {nrhs} load cases" if nrhs > 1 else "") +
+                      " (later steps orthogonalise against more vectors, so this "
+                      "underestimates the CPU time)"),
+           "step_s_estimate": step}
+    if nrhs > 1:
+        out.update(value=nrhs / step, unit="load cases/s")
+    else:
+        out.update(value=m.nelem / step / 1e6, unit="Melem/s")
+    return out
 
 
 def shell_step(p, pc, ksm, ks_weight=100.0, want_grad=False):
@@ -295,6 +300,212 @@ def shell_main(args, L, ev):
     print(json.dumps(out))
 
 
+def batched_loads(p, nrhs, seed=5):
+    """Load cases on the x = 1 face: at every face node a seeded normal
+    traction (mean 0, std 1e6 Pa per component), lumped to a nodal force
+    (traction x face area / number of face nodes); zero on the clamped face."""
+    a = p.assembler
+    rng = np.random.default_rng(seed)
+    face = p.mesh.nodes_on_x(1)
+    vecs = []
+    for _ in range(nrhs):
+        v = a.createVec()
+        arr = v.getArray()
+        arr[:] = 0.0
+        arr.reshape(-1, 3)[face] = 1e6 * rng.standard_normal((face.size, 3)) / face.size
+        a.applyBCs(v)
+        vecs.append(v)
+    return vecs
+
+
+def batched_step(p, pc, ksm, bs, xs):
+    """Assembly, the batched solve of every load case and, per load case, the
+    compliance and its gradient w.r.t. the element thicknesses (compliance is
+    self-adjoint: psi = 2 u, so no adjoint solve)."""
+    from tacs_amd import functions, lib
+    L = lib()
+    a = p.assembler
+    t = {}
+    t0 = time.perf_counter()
+    a.zeroVariables()
+    a.assembleJacobian(1.0, 0.0, 0.0, p.res, p.mat)
+    L.tacs_device_synchronize()
+    t["assembly_s"] = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    pc.factor()
+    flag = ksm.solve(bs, xs)
+    L.tacs_device_synchronize()
+    t["solve_s"] = time.perf_counter() - t1
+    it = ksm.getStats()[0]
+    comp = functions.Compliance(a)
+    vals, grads = [], []
+    psi = a.createVec()
+    for x in xs:
+        a.setVariables(x)
+        vals.append(a.evalFunctions([comp])[0])
+        g = a.createDesignVec()
+        a.addDVSens([comp], [g], 1.0)
+        psi.copyValues(x)
+        psi.scale(2.0)
+        a.addAdjointResProducts([psi], [g], -1.0)
+        grads.append(g)
+    L.tacs_device_synchronize()
+    t["step_s"] = time.perf_counter() - t0
+    return dict(t=t, flag=flag, its=[int(v) for v in it], compliance=vals,
+                grads=[g.getArray().copy() for g in grads])
+
+
+def batched_main(args, L, ev):
+    """27-node hex cube (40^3 by default) with nrhs load cases solved as one
+    block of right-hand sides (batched mixed-precision GMRES with Krylov
+    subspace recycling per load case); the plain FP64 GMRES(30) one load case
+    at a time and the single right-hand-side recycling solver one load case
+    at a time are timed beside it.  Prints its own JSON line."""
+    import torch
+    from tacs_amd import TACS
+    from tacs_amd.problems import SolidProblem
+    n = args.n if args.n != 128 else 40
+    R = args.nrhs
+    t0 = time.perf_counter()
+    p = SolidProblem(n, order=3, dv="element")
+    log(f"setup {time.perf_counter() - t0:.1f}s: {p.mesh.nelem} hex27, {3 * p.mesh.nnodes} DOF")
+    p.assemble()
+    bs = batched_loads(p, R)
+    xs = [p.assembler.createVec() for _ in range(R)]
+    pc = TACS.Pc(p.mat)
+    ksm = TACS.BatchedKSM(p.mat, pc, m=args.m, restarts=100000, nrhs=R, prec=32,
+                          deflation=args.deflation)
+    ksm.setTolerances(args.rtol, 1e-30)
+    for w in range(args.warmup):
+        r = batched_step(p, pc, ksm, bs, xs)
+        log(f"warmup {w}: {r['t']} flag={r['flag']} its={r['its']}")
+    L.tacs_device_synchronize()
+    t0 = time.perf_counter()
+    steps = []
+    for k in range(args.steps):
+        steps.append(batched_step(p, pc, ksm, bs, xs))
+        log(f"step {k}: {steps[-1]['t']} flag={steps[-1]['flag']} its={steps[-1]['its']}")
+    L.tacs_device_synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    last = steps[-1]
+    u_b = [x.getArray().copy() for x in xs[:2]]
+
+    # kernels in isolation (HIP events): the multi-RHS SpMM and the DCGS2 sweeps
+    rp, cp, ap = p.mat.getDeviceArrays()
+    b, nr, nnz = p.mat.getInfo()
+    nd = 3 * nr
+    N = nd * R
+    A32 = torch.empty(nnz * 9, dtype=torch.float32, device="cuda")
+    L.tacs_hip_convert_f64_f32(nnz * 9, ap, A32.data_ptr(), None)
+    kb = 40  # basis vectors of a typical step (20 recycled + 20 Arnoldi)
+    W = torch.rand((kb + 2) * N, dtype=torch.float64, device="cuda")
+    coef = torch.rand(R * (2 * kb + 2), dtype=torch.float64, device="cuda") * 1e-3
+    dots = torch.zeros(2 * R * (kb + 1), dtype=torch.float64, device="cuda")
+    scratch = torch.empty(L.tacs_hip_bdotq_scratch_bytes(R) // 8, dtype=torch.float64,
+                          device="cuda")
+    torch.cuda.synchronize()
+
+    def timed(fn, reps=20):
+        for _ in range(3):
+            fn()
+        e0, e1 = ev.event(), ev.event()
+        ev.record(e0)
+        for _ in range(reps):
+            fn()
+        ev.record(e1)
+        return ev.elapsed_ms(e0, e1) / reps
+
+    Wp, X, Y = W.data_ptr(), W.data_ptr() + kb * N * 8, W.data_ptr() + (kb + 1) * N * 8
+    spmm_ms = timed(lambda: L.tacs_hip_bsr3_spmm_mixed(nr, R, rp, cp, A32.data_ptr(), X, Y, None))
+    upd_ms = timed(lambda: L.tacs_hip_bdcgs_update(nd, R, 0, Wp, kb, Wp, N, coef.data_ptr(),
+                                                   2 * kb + 2, X, Y, None))
+    dot_ms = timed(lambda: L.tacs_hip_bdotq(nd, R, 2, 0, Wp, kb, Wp, N, X, Y, scratch.data_ptr(),
+                                            dots.data_ptr(), dots.data_ptr() + R * (kb + 1) * 8,
+                                            kb + 1, None))
+    spmm_bytes = 36 * nnz + 4 * nnz + 4 * (nr + 1) + 2 * 8 * N
+    upd_bytes = (kb + 4) * N * 8
+    dot_bytes = (kb + 2) * N * 8
+    del W, A32
+    torch.cuda.empty_cache()
+
+    # one load case at a time: plain FP64 GMRES(30) (2 cases timed) and the
+    # single right-hand-side recycling solver (4 cases timed)
+    seq = {}
+    if not args.no_fp64:
+        pc64 = TACS.Pc(p.mat)
+        pc64.factor()
+        k64 = TACS.KSM(p.mat, pc64, args.m_fp64, 100000)
+        k64.setTolerances(args.rtol, 1e-30)
+        x = p.assembler.createVec()
+        tt, its64, u64 = 0.0, [], []
+        for j in range(2):
+            t1 = time.perf_counter()
+            k64.solve(bs[j], x)
+            L.tacs_device_synchronize()
+            tt += time.perf_counter() - t1
+            its64.append(k64.getIterCount())
+            u64.append(x.getArray().copy())
+        seq["fp64_gmres_s_per_case"] = tt / 2
+        seq["fp64_iterations"] = its64
+        seq["fp64_solve_s_16_cases_estimate"] = tt / 2 * R
+        seq["u_rel_err_vs_fp64"] = max(float(np.abs(u_b[j] - u64[j]).max() / np.abs(u64[j]).max())
+                                       for j in range(2))
+        log(f"fp64 sequential: {tt / 2:.2f} s per case, its {its64}")
+    mk = TACS.MixedKSM(p.mat, pc, args.m, 100000, deflation=args.deflation)
+    mk.setTolerances(args.rtol, 1e-30)
+    x = p.assembler.createVec()
+    t1 = time.perf_counter()
+    its1 = []
+    for j in range(4):
+        mk.solve(bs[j], x)
+        its1.append(mk.getIterCount())
+    L.tacs_device_synchronize()
+    seq["single_recycling_s_per_case"] = (time.perf_counter() - t1) / 4
+    seq["single_recycling_iterations"] = its1
+
+    solve_s = last["t"]["solve_s"]
+    out = {
+        "metric": f"batched solve: load cases/s (hex27, {R} load cases, {args.rtol:g} relative residual)",
+        "value": R / (ms_per_step * 1e-3),
+        "unit": "load cases/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 assembly + f32-valued operator / f64 vectors solve",
+        "data": "synthetic structured mesh and seeded load cases (no external data)",
+        "config": {"workload": f"27-node hex cube {n}^3, E=70e9 nu=0.3, x=0 clamped, {R} load "
+                               f"cases (seeded normal nodal traction, std 1e6 Pa, on x=1); "
+                               f"assembly + batched mixed GMRES({args.m}) with per-load-case "
+                               f"recycling of {args.deflation} vectors, block-Jacobi, to "
+                               f"{args.rtol:g} relative residual + per-case compliance and "
+                               f"thickness gradient",
+                   "elements": p.mesh.nelem, "dof": nd, "nnz_blocks": nnz, "load_cases": R},
+        "assembly_s": last["t"]["assembly_s"], "solve_s_batched": solve_s,
+        "iterations_batched": last["its"], "flag": last["flag"],
+        "roofline": {"bound": "hbm", "achieved": upd_bytes / upd_ms / 1e6, "peak": 8000.0,
+                     "unit": "GB/s", "frac": upd_bytes / upd_ms / 1e6 / 8000.0, "traffic": None,
+                     "kernel": f"bdcgs_update (DCGS2 update sweep, {kb} basis vectors, {R} RHS)",
+                     "avg_ms": upd_ms, "algorithmic_bytes": upd_bytes},
+        "roofline_dcgs_dots": {"bound": "hbm", "achieved": dot_bytes / dot_ms / 1e6,
+                               "peak": 8000.0, "unit": "GB/s",
+                               "frac": dot_bytes / dot_ms / 1e6 / 8000.0, "avg_ms": dot_ms,
+                               "algorithmic_bytes": dot_bytes},
+        "roofline_spmm": {"bound": "hbm", "achieved": spmm_bytes / spmm_ms / 1e6, "peak": 8000.0,
+                          "unit": "GB/s", "frac": spmm_bytes / spmm_ms / 1e6 / 8000.0,
+                          "avg_ms": spmm_ms, "algorithmic_bytes": spmm_bytes,
+                          "kernel": "bsr3_spmm_quad<float> (FP32 values, FP64 block vectors)"},
+        "one_case_at_a_time": seq,
+    }
+    if "fp64_solve_s_16_cases_estimate" in seq:
+        out["speedup_batched_vs_fp64_sequential"] = seq["fp64_solve_s_16_cases_estimate"] / solve_s
+    out["speedup_batched_vs_single_recycling"] = seq["single_recycling_s_per_case"] * R / solve_s
+    if not args.no_cpu:
+        its = seq.get("fp64_iterations", last["its"])
+        out["cpu_baseline"] = cpu_baseline(p, [int(np.mean(its))], order=3, nrhs=R,
+                                           name=f"hex27 {n}^3 x {R}")
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -312,8 +523,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="assemble and time the SpMV kernels only (for profiler runs)")
-    ap.add_argument("--workload", default="hex", choices=["hex", "shell"],
-                    help="hex: the 128^3 headline line; shell: the Quad4 plate (own JSON line)")
+    ap.add_argument("--workload", default="hex", choices=["hex", "shell", "batched"],
+                    help="hex: the 128^3 headline line; shell: the Quad4 plate; batched: the "
+                         "hex27 cube with a block of load cases (own JSON lines)")
+    ap.add_argument("--nrhs", type=int, default=16, help="load cases of the batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -326,6 +539,9 @@ def main():
     ev = HipEvents()
     if args.workload == "shell":
         shell_main(args, L, ev)
+        return
+    if args.workload == "batched":
+        batched_main(args, L, ev)
         return
     t0 = time.perf_counter()
     p = SolidProblem(args.n, dv="element")
