@@ -433,7 +433,7 @@ __global__ __launch_bounds__(256) void bsr3_spmm_slots(int nrows, int R,
 // right-hand sides (16-byte loads of X, 12 sums), so NL = R/4 lanes cover a
 // block and a wave works on G = 64/NL blocks at once.
 typedef double spmm_dbl2 __attribute__((ext_vector_type(2)));
-template <typename TA, int NL>
+template <typename TA, int NL, int SPU = 1, int NTA = 1>
 __global__ __launch_bounds__(256) void bsr3_spmm_quad(int nrows, int R,
                                                       const int* __restrict__ rowp,
                                                       const int* __restrict__ cols,
@@ -454,28 +454,49 @@ __global__ __launch_bounds__(256) void bsr3_spmm_quad(int nrows, int R,
     for (int r = 0; r < 3; r++)
 #pragma unroll
       for (int q = 0; q < 4; q++) y[r][q] = 0.0;
-    for (int s = rs + g; okq && s < re; s += G) {
-      const int col = cols[s];
-      const TA* a = &A[(size_t)9 * s];
-      double av[9];
+    // SPU blocks per lane per trip, all loads issued before the arithmetic
+    // (a block past the row end re-reads a valid block and is masked)
+    for (int s0 = rs + g; okq && s0 < re; s0 += SPU * G) {
+      int sb[SPU];
+      bool use[SPU];
+      int col[SPU];
 #pragma unroll
-      for (int e = 0; e < 9; e++) av[e] = (double)a[e];
-      const double* xb = &X[(size_t)3 * col * R + 4 * jq];
-      double xv[3][4];
+      for (int u = 0; u < SPU; u++) {
+        use[u] = s0 + u * G < re;
+        sb[u] = use[u] ? s0 + u * G : s0;
+        col[u] = cols[sb[u]];
+      }
+      TA av[SPU][9];
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        const spmm_dbl2 lo = *(const spmm_dbl2*)(xb + (size_t)c * R);
-        const spmm_dbl2 hi = *(const spmm_dbl2*)(xb + (size_t)c * R + 2);
-        xv[c][0] = lo.x;
-        xv[c][1] = lo.y;
-        xv[c][2] = hi.x;
-        xv[c][3] = hi.y;
+      for (int u = 0; u < SPU; u++)
+#pragma unroll
+        for (int e = 0; e < 9; e++)
+          av[u][e] = NTA ? __builtin_nontemporal_load(&A[(size_t)9 * sb[u] + e])
+                         : A[(size_t)9 * sb[u] + e];
+      double xv[SPU][3][4];
+#pragma unroll
+      for (int u = 0; u < SPU; u++) {
+        const double* xb = &X[(size_t)3 * col[u] * R + 4 * jq];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const spmm_dbl2 lo = *(const spmm_dbl2*)(xb + (size_t)c * R);
+          const spmm_dbl2 hi = *(const spmm_dbl2*)(xb + (size_t)c * R + 2);
+          xv[u][c][0] = lo.x;
+          xv[u][c][1] = lo.y;
+          xv[u][c][2] = hi.x;
+          xv[u][c][3] = hi.y;
+        }
       }
 #pragma unroll
-      for (int r = 0; r < 3; r++)
+      for (int u = 0; u < SPU; u++) {
+        if (!use[u]) continue;
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-          y[r][q] += av[3 * r] * xv[0][q] + av[3 * r + 1] * xv[1][q] + av[3 * r + 2] * xv[2][q];
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            y[r][q] += (double)av[u][3 * r] * xv[u][0][q] + (double)av[u][3 * r + 1] * xv[u][1][q] +
+                       (double)av[u][3 * r + 2] * xv[u][2][q];
+      }
     }
 #pragma unroll
     for (int off = NL; off < 64; off <<= 1)
