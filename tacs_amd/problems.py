@@ -155,3 +155,100 @@ class ShellProblem:
     solve_fp64 = SolidProblem.solve_fp64
     solve_mixed = SolidProblem.solve_mixed
     compliance = SolidProblem.compliance
+
+
+class MultiCubeProblem:
+    """Many independent 8-node hex cubes in one assembler: the disjoint union
+    of the meshes (node and element ids offset cube by cube), so one
+    assembly launch covers every cube and the block-diagonal stiffness is one
+    BCSR matrix.  Cube c has sizes[c] elements per edge, unit edge length,
+    x = 0 clamped and the traction `traction` on its x = 1 face (the load of
+    the 128^3 workload).  dv: None, "shared" or "element" as for SolidProblem.
+
+    node_off / elem_off give each cube's first node and element; dof(c) the
+    slice of its dofs in assembler vectors.
+    """
+
+    def __init__(self, sizes, E=70e9, nu=0.3, rho=2700.0, ys=270e6,
+                 traction=(0.0, 0.0, 1e6), t0=1.0, dv=None):
+        self.sizes = [int(n) for n in sizes]
+        self.meshes = [HexMesh(n, n, n) for n in self.sizes]
+        nn = np.array([m.nnodes for m in self.meshes])
+        ne = np.array([m.nelem for m in self.meshes])
+        self.node_off = np.concatenate([[0], np.cumsum(nn)]).astype(np.int64)
+        self.elem_off = np.concatenate([[0], np.cumsum(ne)]).astype(np.int64)
+        nnodes, nelem = int(self.node_off[-1]), int(self.elem_off[-1])
+        self.nnodes, self.nelem = nnodes, nelem
+        X = np.concatenate([m.X + np.array([2.0 * c, 0.0, 0.0])
+                            for c, m in enumerate(self.meshes)])
+        conn = np.concatenate([m.conn + self.node_off[c] for c, m in enumerate(self.meshes)])
+        self.X_all, self.conn = X, conn.astype(np.int32)
+        self.props = constitutive.MaterialProperties(rho=rho, E=E, nu=nu, ys=ys)
+        self.basis = elements.LinearHexaBasis()
+        a = self.assembler = TACS.Assembler.create(None, 3, nnodes, nelem)
+        ptr = np.arange(0, 8 * nelem + 1, 8, dtype=np.intc)
+        a.setElementConnectivity(ptr, self.conn.reshape(-1))
+        if dv == "element":
+            elems = []
+            for e in range(nelem):
+                con = constitutive.SolidConstitutive(self.props, t=t0, tNum=e)
+                elems.append(elements.Element3D(elements.LinearElasticity3D(con), self.basis))
+        else:
+            con = constitutive.SolidConstitutive(self.props, t=t0,
+                                                 tNum=0 if dv == "shared" else -1)
+            elems = [elements.Element3D(elements.LinearElasticity3D(con), self.basis)] * nelem
+        a.setElements(elems)
+        self.bc_nodes = np.concatenate([m.nodes_on_x(0) + self.node_off[c]
+                                        for c, m in enumerate(self.meshes)]).astype(np.int32)
+        a.addBCs(self.bc_nodes)
+        a.initialize()
+        self.X = a.createNodeVec()
+        self.X.getArray()[:] = X.reshape(-1)
+        a.setNodes(self.X)
+        self.traction = np.asarray(traction, dtype=float)
+        ae = [m.elems_on_x(1)[0] + self.elem_off[c] for c, m in enumerate(self.meshes)]
+        self.aux_elems = np.concatenate(ae).astype(np.int32)
+        self.aux_faces = np.ones(self.aux_elems.size, dtype=np.int32)
+        self.aux = TACS.AuxElements()
+        tr = elements.Traction3D(3, 1, self.basis, list(self.traction))
+        for e in self.aux_elems:
+            self.aux.addElement(int(e), tr)
+        a.setAuxElements(self.aux)
+        self.res = a.createVec()
+        self.u = a.createVec()
+        self.mat = a.createMat()
+
+    def dof(self, c):
+        """Slice of cube c's dofs in an assembler vector."""
+        return slice(3 * int(self.node_off[c]), 3 * int(self.node_off[c + 1]))
+
+    def elems(self, c):
+        return slice(int(self.elem_off[c]), int(self.elem_off[c + 1]))
+
+    def scale_cubes(self, b, out):
+        """out = b with each cube's part scaled to unit norm (zero parts kept);
+        the scales are kept for unscale_cubes."""
+        bv = b.getArray()
+        ov = out.getArray()
+        self._scales = []
+        for c in range(len(self.sizes)):
+            sl = self.dof(c)
+            nrm = np.linalg.norm(bv[sl])
+            s = 1.0 / nrm if nrm > 0 else 1.0
+            ov[sl] = s * bv[sl]
+            self._scales.append(s)
+
+    def unscale_cubes(self, x):
+        """Undo scale_cubes on a solution of the scaled system."""
+        xv = x.getArray()
+        for c, s in enumerate(self._scales):
+            xv[self.dof(c)] /= s
+
+    def cube_atol(self, b, rtol):
+        """Absolute tolerance on the union residual that brings every cube to
+        |r_c| <= rtol |b_c| (since |r_c| <= |r|)."""
+        bv = b.getArray()
+        return rtol * min(np.linalg.norm(bv[self.dof(c)]) for c in range(len(self.sizes)))
+
+    set_thickness = SolidProblem.set_thickness
+    assemble = SolidProblem.assemble
