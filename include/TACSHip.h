@@ -269,9 +269,26 @@ class TACSParallelMat : public TACSMat {
                  TacsScalar** A);
   // FP32 shadow of the values for the mixed-precision solver
   float* getValues32(int refresh);
+  // y = A x on device vectors: FP64 values and vectors; FP32-rounded values
+  // with FP64 vectors (the recycling solver's operator); FP32 values and
+  // vectors.  3x3 blocks go through a sliced ELL copy of the values (slices of
+  // 64 rows, one lane per row, refreshed when the values change); other block
+  // sizes through the BCSR.
+  void mult64(const double* x, double* y);
+  void multMixed(const double* x, double* y);
+  void mult32(const float* x, float* y);
   int bsize, nrows;
   long nnzb;
   long version;  // incremented whenever the values change
+  // sliced ELL copy (3x3 blocks)
+  void sellUpdate(int prec);
+  int* d_sw;
+  long* d_soff;
+  int* d_scols;
+  double* d_s64;
+  float* d_s32;
+  long sellVer64, sellVer32;
+  size_t nslots;
   int *d_rowp, *d_cols;
   double* d_A;
   float* d_A32;

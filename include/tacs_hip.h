@@ -219,6 +219,23 @@ int tacs_hip_bsr_diag_apply_mixed(int b, int nrows, const float* Dinv, const dou
    (the reference multiplies one vector at a time, BCSRMatVecMult3). */
 int tacs_hip_bsr3_spmm(int nrows, int nrhs, const int* rowp, const int* cols, const double* A,
                        const double* X, double* Y, hipStream_t stream);
+/* Sliced ELL copy of a 3x3 BCSR matrix for the product y = A x (no
+   counterpart in the reference; BCSRMat::mult src/bpmat/BCSRMat.cpp:1226 is
+   what it computes).  Slices of 64 rows; slice s is width[s] blocks wide and
+   starts at slot coff[s] (coff has nslices + 1 entries); slot t of row
+   64 s + l holds its column in cs[coff[s] + 64 t + l] and value q of the block
+   in vs[9 coff[s] + 64 (9 t + q) + l]; padding slots are zero blocks in the
+   row's own column.
+   sell3_slots:  host: widths and offsets from the host row pointer (either
+                 output may be NULL); returns the number of slots;
+   sell3_fill:   values (FP64 or FP32, prec 64/32) and, if cs != NULL, columns
+                 from the BCSR arrays;
+   sell3_spmv:   values prec_vals (64/32), vectors prec_vecs (64/32). */
+size_t tacs_hip_sell3_slots(int nrows, const int* rowp_host, int* width, long* coff);
+int tacs_hip_sell3_fill(int prec, int nrows, const int* rowp, const int* cols, const int* width,
+                        const long* coff, const double* A, void* vs, int* cs, hipStream_t stream);
+int tacs_hip_sell3_spmv(int prec_vals, int prec_vecs, int nrows, const int* width, const long* coff,
+                        const int* cs, const void* vs, const void* x, void* y, hipStream_t stream);
 /* Diagonal block positions; inverse diagonal blocks (BCSRMat::factorDiag
    src/bpmat/BCSRMat.cpp:1527); y = D^{-1} x. */
 int tacs_hip_bsr_find_diag(int nrows, const int* rowp, const int* cols, int* diag,
@@ -476,12 +493,19 @@ int tacs_vec_set_rand(tacs_obj v, double lower, double upper);
 
 /* TACSParallelMat (TACS.pyx Mat :960) */
 int tacs_mat_mult(tacs_obj m, tacs_obj x, tacs_obj y);
+/* y = A x on device vectors in the precisions the solvers use (values
+   prec_vals, vectors prec_vecs: 64/64, 32/64 or 32/32), as
+   TACSParallelMat::mult src/bpmat/TACSParallelMat.cpp:248 with FP64
+   throughout; FP32 values are the FP64 values rounded to nearest. */
+int tacs_mat_mult_prec(tacs_obj m, int prec_vals, int prec_vecs, const void* x, void* y);
 int tacs_mat_zero_entries(tacs_obj m);
 int tacs_mat_get_info(tacs_obj m, int* bsize, int* nrows, long* nnzb);
 /* Host copies of the block-CSR arrays (BCSRMat::getArrays BCSRMat.h:82). */
 int tacs_mat_get_arrays(tacs_obj m, const int** rowp, const int** cols, const double** A);
-/* Device arrays of the matrix (valid until the matrix is destroyed). */
-int tacs_mat_device_arrays(tacs_obj m, const int** rowp, const int** cols, double** A);
+/* Device arrays of the matrix (valid until the matrix is destroyed).  The
+   values are read-only here: the FP32 and sliced ELL copies the products use
+   are refreshed when assembly, zeroEntries or applyBCs change them. */
+int tacs_mat_device_arrays(tacs_obj m, const int** rowp, const int** cols, const double** A);
 
 /* Preconditioner / Krylov (TACS.pyx Pc :1106, KSM :1256) */
 tacs_obj tacs_block_jacobi_pc_create(tacs_obj mat);

@@ -3,6 +3,8 @@
 // preconditioner (inverse diagonal blocks, as BCSRMat::factorDiag
 // src/bpmat/BCSRMat.cpp:1527 computes them), Dirichlet rows
 // (BCSRMat::zeroRow :2027) and precision conversion.
+#include <algorithm>
+
 #include "tacs_hip_common.h"
 
 #define WAVE_SYNC()                                        \
@@ -41,17 +43,25 @@ __global__ __launch_bounds__(256) void bsr3_spmv(int nrows, const int* __restric
   const int rbeg = (int)((long)nrows * xcd / 8), rend = (int)((long)nrows * (xcd + 1) / 8);
   const int nw = nbx * 4;
   // each wave handles R consecutive rows per pass, with every load of the R
-  // rows in flight together
-  for (int row0 = rbeg + ((blockIdx.x >> 3) * 4 + wid) * R; row0 < rend; row0 += nw * R) {
+  // rows in flight together; the row pointers of the next pass are loaded
+  // during this one, so a pass waits on one memory latency, not two
+  int row0 = rbeg + ((blockIdx.x >> 3) * 4 + wid) * R;
+  int pre[R + 1];
+#pragma unroll
+  for (int r = 0; r <= R; r++) pre[r] = row0 + r <= rend ? rowp[row0 + r] : 0;
+  for (; row0 < rend; row0 += nw * R) {
     int rs[R], nb[R];
     bool fast = true;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       int row = row0 + r;
-      rs[r] = row < rend ? rowp[row] : 0;
-      nb[r] = row < rend ? rowp[row + 1] - rs[r] : 0;
+      rs[r] = row < rend ? pre[r] : 0;
+      nb[r] = row < rend ? pre[r + 1] - pre[r] : 0;
       fast = fast && nb[r] <= 7 * NT;
     }
+    const int nxt = row0 + nw * R;
+#pragma unroll
+    for (int r = 0; r <= R; r++) pre[r] = nxt + r <= rend ? rowp[nxt + r] : 0;
     ACC acc[R];
     if (fast && sizeof(TA) == 8) {
       int col[R][NT];
@@ -299,6 +309,74 @@ __global__ void bsr_zero_rows(int b, int nbc, const int* __restrict__ nodes,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Sliced ELL copy of a 3x3 BCSR matrix for the product (no counterpart in the
+// reference, whose BCSRMat is the only layout; the BCSR stays the matrix of
+// record).  Rows are cut into slices of 64 (one wavefront); slice s has width
+// w_s = the longest row of the slice in blocks; slot t of row 64 s + lane has
+// its column at cs[coff[s] + 64 t + lane] and value q of the block at
+// vs[9 coff[s] + 64 (9 t + q) + lane].  Padding slots point at the row itself
+// with zero values.  With lane = row every load of a wavefront is 64
+// consecutive entries and no cross-lane reduction is needed.
+// ---------------------------------------------------------------------------
+template <typename TA, typename T>
+__global__ __launch_bounds__(256) void sell3_spmv(int nslices, int nrows,
+                                                  const int* __restrict__ width,
+                                                  const long* __restrict__ coff,
+                                                  const int* __restrict__ cs,
+                                                  const TA* __restrict__ vs,
+                                                  const T* __restrict__ x, T* __restrict__ y) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int s = blockIdx.x * 4 + wid; s < nslices; s += gridDim.x * 4) {
+    const int w = width[s];
+    const long co = coff[s], vo = 9 * co;
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    for (int t = 0; t < w; t++) {
+      const int col = cs[co + 64 * t + lane];
+      TA a[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) a[q] = __builtin_nontemporal_load(&vs[vo + 64 * (9 * t + q) + lane]);
+      const double x0 = (double)x[3 * col], x1 = (double)x[3 * col + 1],
+                   x2 = (double)x[3 * col + 2];
+      y0 += (double)a[0] * x0 + (double)a[1] * x1 + (double)a[2] * x2;
+      y1 += (double)a[3] * x0 + (double)a[4] * x1 + (double)a[5] * x2;
+      y2 += (double)a[6] * x0 + (double)a[7] * x1 + (double)a[8] * x2;
+    }
+    const int row = 64 * s + lane;
+    if (row < nrows) {
+      y[3 * (size_t)row] = (T)y0;
+      y[3 * (size_t)row + 1] = (T)y1;
+      y[3 * (size_t)row + 2] = (T)y2;
+    }
+  }
+}
+
+// Fill the sliced ELL values (and, when cs is not NULL, the columns) from the
+// BCSR arrays: one lane per row, slot by slot.
+template <typename TA>
+__global__ __launch_bounds__(256) void sell3_fill(int nslices, int nrows, const int* __restrict__ rowp,
+                                                  const int* __restrict__ cols,
+                                                  const int* __restrict__ width,
+                                                  const long* __restrict__ coff,
+                                                  const double* __restrict__ A, TA* __restrict__ vs,
+                                                  int* __restrict__ cs) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int s = blockIdx.x * 4 + wid; s < nslices; s += gridDim.x * 4) {
+    const int w = width[s];
+    const long co = coff[s], vo = 9 * co;
+    const int row = 64 * s + lane;
+    const int rs = row < nrows ? rowp[row] : 0;
+    const int nb = row < nrows ? rowp[row + 1] - rs : 0;
+    for (int t = 0; t < w; t++) {
+      const bool use = t < nb;
+      if (cs) cs[co + 64 * t + lane] = use ? cols[rs + t] : (row < nrows ? row : 0);
+#pragma unroll
+      for (int q = 0; q < 9; q++)
+        vs[vo + 64 * (9 * t + q) + lane] = use ? (TA)A[9 * ((size_t)rs + t) + q] : (TA)0;
+    }
+  }
+}
+
 __global__ void convert_f64_to_f32(size_t n, const double* __restrict__ x, float* __restrict__ y) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -360,6 +438,53 @@ int tacs_hip_bsr_diag_apply_mixed(int b, int nrows, const float* Dinv, const dou
     hipLaunchKernelGGL((bsr_diag_apply<3, float, double>), g, t, 0, stream, nrows, Dinv, x, y);
   else if (b == 6)
     hipLaunchKernelGGL((bsr_diag_apply<6, float, double>), g, t, 0, stream, nrows, Dinv, x, y);
+  else
+    return TACS_HIP_ERR_ARG;
+  return (int)hipGetLastError();
+}
+
+size_t tacs_hip_sell3_slots(int nrows, const int* rowp_host, int* width, long* coff) {
+  const int ns = (nrows + 63) / 64;
+  long off = 0;
+  for (int s = 0; s < ns; s++) {
+    int w = 0;
+    for (int r = 64 * s; r < 64 * s + 64 && r < nrows; r++) w = std::max(w, rowp_host[r + 1] - rowp_host[r]);
+    if (width) width[s] = w;
+    if (coff) coff[s] = off;
+    off += 64L * w;
+  }
+  if (coff) coff[ns] = off;
+  return (size_t)off;
+}
+
+int tacs_hip_sell3_fill(int prec, int nrows, const int* rowp, const int* cols, const int* width,
+                        const long* coff, const double* A, void* vs, int* cs, hipStream_t stream) {
+  const int ns = (nrows + 63) / 64;
+  if (ns <= 0) return 0;
+  const int grid = tacs_grid_for(ns, 4, 65536);
+  if (prec == 64)
+    hipLaunchKernelGGL((sell3_fill<double>), dim3(grid), dim3(256), 0, stream, ns, nrows, rowp, cols,
+                       width, coff, A, (double*)vs, cs);
+  else
+    hipLaunchKernelGGL((sell3_fill<float>), dim3(grid), dim3(256), 0, stream, ns, nrows, rowp, cols,
+                       width, coff, A, (float*)vs, cs);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_sell3_spmv(int prec_vals, int prec_vecs, int nrows, const int* width, const long* coff,
+                        const int* cs, const void* vs, const void* x, void* y, hipStream_t stream) {
+  const int ns = (nrows + 63) / 64;
+  if (ns <= 0) return 0;
+  const int grid = tacs_grid_for(ns, 4, 65536);
+  if (prec_vals == 64 && prec_vecs == 64)
+    hipLaunchKernelGGL((sell3_spmv<double, double>), dim3(grid), dim3(256), 0, stream, ns, nrows,
+                       width, coff, cs, (const double*)vs, (const double*)x, (double*)y);
+  else if (prec_vals == 32 && prec_vecs == 64)
+    hipLaunchKernelGGL((sell3_spmv<float, double>), dim3(grid), dim3(256), 0, stream, ns, nrows,
+                       width, coff, cs, (const float*)vs, (const double*)x, (double*)y);
+  else if (prec_vals == 32 && prec_vecs == 32)
+    hipLaunchKernelGGL((sell3_spmv<float, float>), dim3(grid), dim3(256), 0, stream, ns, nrows,
+                       width, coff, cs, (const float*)vs, (const float*)x, (float*)y);
   else
     return TACS_HIP_ERR_ARG;
   return (int)hipGetLastError();

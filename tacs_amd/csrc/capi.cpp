@@ -378,6 +378,19 @@ int tacs_mat_mult(tacs_obj m, tacs_obj x, tacs_obj y) {
   A->mult(xv, yv);
   return last_error_code();
 }
+int tacs_mat_mult_prec(tacs_obj m, int prec_vals, int prec_vecs, const void* x, void* y) {
+  REQUIRE(TACSParallelMat, A, m);
+  clear_error();
+  if (prec_vals == 64 && prec_vecs == 64)
+    A->mult64((const double*)x, (double*)y);
+  else if (prec_vals == 32 && prec_vecs == 64)
+    A->multMixed((const double*)x, (double*)y);
+  else if (prec_vals == 32 && prec_vecs == 32)
+    A->mult32((const float*)x, (float*)y);
+  else
+    return tacs_set_error("tacs_mat_mult_prec: precisions %d/%d", prec_vals, prec_vecs);
+  return last_error_code();
+}
 int tacs_mat_zero_entries(tacs_obj m) {
   REQUIRE(TACSParallelMat, A, m);
   A->zeroEntries();
@@ -397,7 +410,7 @@ int tacs_mat_get_arrays(tacs_obj m, const int** rowp, const int** cols, const do
   if (Av) *Av = h;
   return 0;
 }
-int tacs_mat_device_arrays(tacs_obj m, const int** rowp, const int** cols, double** Av) {
+int tacs_mat_device_arrays(tacs_obj m, const int** rowp, const int** cols, const double** Av) {
   REQUIRE(TACSParallelMat, A, m);
   if (rowp) *rowp = A->d_rowp;
   if (cols) *cols = A->d_cols;

@@ -397,12 +397,21 @@ TACSParallelMat::TACSParallelMat(int _bsize, int _nrows, const int* rowp, const 
   d_A32 = NULL;
   A32_current = 0;
   h_A = NULL;
+  d_sw = NULL;
+  d_soff = NULL;
+  d_scols = NULL;
+  d_s64 = NULL;
+  d_s32 = NULL;
+  sellVer64 = sellVer32 = -1;
+  nslots = 0;
 }
 TACSParallelMat::~TACSParallelMat() {
   dfree(d_rowp);
   dfree(d_cols);
   dfree(d_A);
   dfree(d_A32);
+  void* sell[] = {d_sw, d_soff, d_scols, d_s64, d_s32};
+  for (void* q : sell) dfree(q);
   delete[] h_rowp;
   delete[] h_cols;
   if (h_A) HCHECK(hipHostFree(h_A));
@@ -412,7 +421,63 @@ void TACSParallelMat::mult(TACSVec* tx, TACSVec* ty) {
   TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
   TACSBVec* y = dynamic_cast<TACSBVec*>(ty);
   const double* xd = x->devRead();
-  KCHECK(tacs_hip_bsr_spmv(bsize, nrows, d_rowp, d_cols, d_A, xd, y->devWrite(), 0, g_stream));
+  mult64(xd, y->devWrite());
+}
+// Sliced ELL pattern on first use (host widths and offsets from the row
+// pointer, columns with the first fill); values refreshed when the matrix
+// version moved.
+void TACSParallelMat::sellUpdate(int prec) {
+  const int ns = (nrows + 63) / 64;
+  int* cs_fill = NULL;
+  if (!d_sw) {
+    std::vector<int> w(ns > 0 ? ns : 1);
+    std::vector<long> off((size_t)ns + 1);
+    nslots = tacs_hip_sell3_slots(nrows, h_rowp, w.data(), off.data());
+    d_sw = dalloc<int>(ns);
+    d_soff = dalloc<long>((size_t)ns + 1);
+    d_scols = dalloc<int>(nslots);
+    h2d(d_sw, w.data(), ns);
+    h2d(d_soff, off.data(), (size_t)ns + 1);
+    HCHECK(hipStreamSynchronize(g_stream));
+    cs_fill = d_scols;
+  }
+  if (prec == 64) {
+    if (!d_s64) d_s64 = dalloc<double>(9 * nslots);
+    KCHECK(tacs_hip_sell3_fill(64, nrows, d_rowp, d_cols, d_sw, d_soff, d_A, d_s64, cs_fill,
+                               g_stream));
+    sellVer64 = version;
+  } else {
+    if (!d_s32) d_s32 = dalloc<float>(9 * nslots);
+    KCHECK(tacs_hip_sell3_fill(32, nrows, d_rowp, d_cols, d_sw, d_soff, d_A, d_s32, cs_fill,
+                               g_stream));
+    sellVer32 = version;
+  }
+}
+void TACSParallelMat::mult64(const double* x, double* y) {
+  if (bsize != 3) {
+    KCHECK(tacs_hip_bsr_spmv(bsize, nrows, d_rowp, d_cols, d_A, x, y, 0, g_stream));
+    return;
+  }
+  if (sellVer64 != version) sellUpdate(64);
+  KCHECK(tacs_hip_sell3_spmv(64, 64, nrows, d_sw, d_soff, d_scols, d_s64, x, y, g_stream));
+}
+void TACSParallelMat::multMixed(const double* x, double* y) {
+  if (bsize != 3) {
+    KCHECK(tacs_hip_bsr_spmv_mixed(bsize, nrows, d_rowp, d_cols, getValues32(1), x, y, 0,
+                                   g_stream));
+    return;
+  }
+  if (sellVer32 != version) sellUpdate(32);
+  KCHECK(tacs_hip_sell3_spmv(32, 64, nrows, d_sw, d_soff, d_scols, d_s32, x, y, g_stream));
+}
+void TACSParallelMat::mult32(const float* x, float* y) {
+  if (bsize != 3) {
+    KCHECK(tacs_hip_bsr_spmv_f32(bsize, nrows, d_rowp, d_cols, getValues32(1), x, y, 0,
+                                 g_stream));
+    return;
+  }
+  if (sellVer32 != version) sellUpdate(32);
+  KCHECK(tacs_hip_sell3_spmv(32, 32, nrows, d_sw, d_soff, d_scols, d_s32, x, y, g_stream));
 }
 void TACSParallelMat::zeroEntries() {
   KCHECK(tacs_hip_fill(64, (size_t)nnzb * bsize * bsize, d_A, 0.0, g_stream));
@@ -608,7 +673,7 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
       KCHECK(tacs_hip_dot(64, n, NULL, NULL, W(0), W(0), d_red, res, 1, misc + MISC_INV, g_stream));
       KCHECK(tacs_hip_scale(64, n, W(0), misc + MISC_INV, 0.0, 0, g_stream));
     } else {
-      KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, W(0), 0, g_stream));
+      mat->mult64(xd, W(0));
       KCHECK(tacs_hip_axpby(64, 64, n, -1.0, bd, 1.0, W(0), g_stream));
       KCHECK(tacs_hip_dot(64, n, NULL, NULL, W(0), W(0), d_red, res, 1, misc + MISC_INV, g_stream));
       KCHECK(tacs_hip_scalar_op(misc + MISC_INV, -1.0, misc + MISC_NINV, g_stream));
@@ -628,9 +693,9 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
       double* q = W(i + 1);
       if (pc) {
         KCHECK(tacs_hip_bsr_diag_apply(bs, nr, pc->d_Dinv, W(i), d_work, g_stream));
-        KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, d_work, q, 0, g_stream));
+        mat->mult64(d_work, q);
       } else {
-        KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, W(i), q, 0, g_stream));
+        mat->mult64(W(i), q);
       }
       double* h = H + (size_t)i * (msub + 1);
       if (ortho == MODIFIED_GRAM_SCHMIDT) {
@@ -757,7 +822,7 @@ int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
     KCHECK(tacs_hip_fill(64, n, xd, 0.0, g_stream));
     HCHECK(hipMemcpyAsync(d_r, bd, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
   } else {
-    KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
+    mat->mult64(xd, d_r);
     KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
   }
   double r0 = 0.0;
@@ -781,7 +846,7 @@ int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
     for (int i = 0; i < msub; i++) {
       float* q = W(i + 1);
       KCHECK(tacs_hip_bsr_diag_apply_f32(bs, nr, pc->d_Dinv32, W(i), d_work32, g_stream));
-      KCHECK(tacs_hip_bsr_spmv_f32(bs, nr, mat->d_rowp, mat->d_cols, A32, d_work32, q, 0, g_stream));
+      mat->mult32(d_work32, q);
       double* h = H + (size_t)i * (msub + 1);
       cgs2(32, n, i + 1, d_W, q, h, tmp, d_red);
       KCHECK(tacs_hip_dot(32, n, NULL, NULL, q, q, d_red, &h[i + 1], 1, misc + MISC_INV, g_stream));
@@ -798,7 +863,7 @@ int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
     KCHECK(tacs_hip_lincomb(32, 32, n, niters, d_W, n, res, d_work32, 0, g_stream));
     KCHECK(tacs_hip_bsr_diag_apply_f32(bs, nr, pc->d_Dinv32, d_work32, d_d32, g_stream));
     KCHECK(tacs_hip_axpby(32, 64, n, rn, d_d32, 1.0, xd, g_stream));
-    KCHECK(tacs_hip_bsr_spmv(bs, nr, mat->d_rowp, mat->d_cols, mat->d_A, xd, d_r, 0, g_stream));
+    mat->mult64(xd, d_r);
     KCHECK(tacs_hip_axpby(64, 64, n, 1.0, bd, -1.0, d_r, g_stream));
   }
   HCHECK(hipStreamSynchronize(g_stream));
@@ -1366,14 +1431,14 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
   // one right-hand side: the single-vector block-SpMV kernels
   auto spmm64 = [&](const double* x, double* y) {
     if (Rn == 1)
-      KCHECK(tacs_hip_bsr_spmv(3, nr, mat->d_rowp, mat->d_cols, mat->d_A, x, y, 0, g_stream));
+      mat->mult64(x, y);
     else
       KCHECK(tacs_hip_bsr3_spmm(nr, Rn, mat->d_rowp, mat->d_cols, mat->d_A, x, y, g_stream));
   };
   auto op = [&](const double* x, double* y) {
     KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, x, d_work, g_stream));
     if (prec == 32 && Rn == 1)
-      KCHECK(tacs_hip_bsr_spmv_mixed(3, nr, mat->d_rowp, mat->d_cols, A32, d_work, y, 0, g_stream));
+      mat->multMixed(d_work, y);
     else if (prec == 32)
       KCHECK(tacs_hip_bsr3_spmm_mixed(nr, Rn, mat->d_rowp, mat->d_cols, A32, d_work, y, g_stream));
     else

@@ -74,6 +74,126 @@ def test_spmv_matches_oracle():
     assert np.abs(y.getArray() - yo).max() <= 1e-13 * np.abs(A).max() * np.abs(xv).sum() / xv.size * 30
 
 
+def _ragged_pattern(nrows, seed):
+    """Random sorted-unique block-CSR pattern with empty rows, rows up to 40
+    blocks long and a row count that is not a multiple of 64."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 41, nrows)
+    lens[rng.random(nrows) < 0.1] = 0
+    rows = [np.sort(rng.choice(nrows, size=min(k, nrows), replace=False)) for k in lens]
+    rowp = np.zeros(nrows + 1, np.int32)
+    rowp[1:] = np.cumsum([r.size for r in rows])
+    cols = np.concatenate(rows).astype(np.int32)
+    A = rng.standard_normal(9 * cols.size)
+    return rowp, cols, A
+
+
+@pytest.mark.parametrize("nrows", [1, 63, 1000, 4161])
+def test_sliced_ell_spmv_matches_oracle(nrows):
+    """The sliced ELL copy of a 3x3 BCSR matrix (slices of 64 rows, zero
+    padding blocks) and its products with FP64 values and vectors, FP32
+    values and FP64 vectors, FP32 values and vectors, against the oracle's
+    block-CSR product (BCSRMat::mult) on a ragged pattern."""
+    import ctypes
+
+    import torch
+
+    from tacs_amd import lib
+
+    rowp, cols, A = _ragged_pattern(nrows, nrows)
+    ns = (nrows + 63) // 64
+    width = np.zeros(ns, np.int32)
+    coff = np.zeros(ns + 1, np.int64)
+    slots = lib().tacs_hip_sell3_slots(nrows, rowp.ctypes.data, width.ctypes.data, coff.ctypes.data)
+    lens = np.diff(rowp)
+    for s in range(ns):
+        assert width[s] == lens[64 * s:64 * s + 64].max()
+    assert coff[-1] == slots == 64 * width.sum()
+    assert lib().tacs_hip_sell3_slots(nrows, rowp.ctypes.data, None, None) == slots
+    dev = lambda a: torch.from_numpy(a).to("cuda")
+    rp, cp, Ad, wd, od = dev(rowp), dev(cols), dev(A), dev(width), dev(coff)
+    cs = torch.full((max(slots, 1),), -1, dtype=torch.int32, device="cuda")
+    v64 = torch.full((9 * max(slots, 1),), np.nan, dtype=torch.float64, device="cuda")
+    v32 = torch.full((9 * max(slots, 1),), np.nan, dtype=torch.float32, device="cuda")
+    rng = np.random.default_rng(7)
+    xh = rng.standard_normal(3 * nrows)
+    x64, y64 = dev(xh), torch.zeros(3 * nrows, dtype=torch.float64, device="cuda")
+    x32, y32 = x64.float(), torch.zeros(3 * nrows, dtype=torch.float32, device="cuda")
+    ym = torch.zeros_like(y64)
+    torch.cuda.synchronize()
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    assert lib().tacs_hip_sell3_fill(64, nrows, P(rp), P(cp), P(wd), P(od), P(Ad), P(v64), P(cs),
+                                     None) == 0
+    assert lib().tacs_hip_sell3_fill(32, nrows, P(rp), P(cp), P(wd), P(od), P(Ad), P(v32), None,
+                                     None) == 0
+    assert lib().tacs_hip_sell3_spmv(64, 64, nrows, P(wd), P(od), P(cs), P(v64), P(x64), P(y64),
+                                     None) == 0
+    assert lib().tacs_hip_sell3_spmv(32, 64, nrows, P(wd), P(od), P(cs), P(v32), P(x64), P(ym),
+                                     None) == 0
+    assert lib().tacs_hip_sell3_spmv(32, 32, nrows, P(wd), P(od), P(cs), P(v32), P(x32), P(y32),
+                                     None) == 0
+    assert lib().tacs_hip_sell3_spmv(64, 32, nrows, P(wd), P(od), P(cs), P(v64), P(x64), P(y64),
+                                     None) != 0
+    torch.cuda.synchronize()
+    # every slot written: columns in range, padding blocks zero in the row's own column
+    csh = cs.cpu().numpy()[:slots]
+    vh = v64.cpu().numpy()[:9 * slots]
+    assert csh.min() >= 0 and csh.max() < nrows and not np.isnan(vh).any()
+    for s in range(ns):
+        for t in range(width[s]):
+            for l in range(64):
+                r = 64 * s + l
+                if r >= nrows or t >= lens[r]:
+                    assert vh[9 * coff[s] + 64 * 9 * t + l::64][:9].tolist() == [0.0] * 9
+                    assert csh[coff[s] + 64 * t + l] == (r if r < nrows else 0)
+    yo = O.bsr_mult(3, rowp, cols, A, xh)
+    scale = np.abs(A).max() * np.abs(xh).max() * 40
+    assert np.abs(y64.cpu().numpy() - yo).max() <= 1e-14 * scale
+    A32 = A.astype(np.float32).astype(np.float64)
+    yo32 = O.bsr_mult(3, rowp, cols, A32, xh)
+    assert np.abs(ym.cpu().numpy() - yo32).max() <= 1e-14 * scale
+    yo3232 = O.bsr_mult(3, rowp, cols, A32, xh.astype(np.float32).astype(np.float64))
+    assert np.abs(y32.cpu().numpy() - yo3232).max() <= 1e-6 * scale
+
+
+def test_products_follow_reassembly():
+    """The matrix's product copies (FP32 values, sliced ELL) are refreshed
+    when assembly changes the values: y = A x after a second assembly with
+    other thicknesses matches the oracle's product with the new matrix, in
+    the FP64, FP32-valued and FP32 products."""
+    import torch
+
+    from tacs_amd import lib
+
+    p = SolidProblem(5, dv="element")
+    rng = np.random.default_rng(3)
+    x = p.assembler.createVec()
+    xv = rng.standard_normal(x.getSize())
+    x.getArray()[:] = xv
+    y = p.assembler.createVec()
+    for k in range(2):
+        p.set_thickness(rng.uniform(0.2, 1.0, p.mesh.nelem))
+        p.assemble()
+        p.mat.mult(x, y)
+        rowp, cols, A = p.mat.getArrays()
+        yo = O.bsr_mult(3, rowp, cols, A.reshape(-1), xv)
+        assert np.abs(y.getArray() - yo).max() <= 1e-13 * np.abs(yo).max()
+    xd = torch.tensor(xv, device="cuda")
+    ym = torch.zeros_like(xd)
+    x32 = xd.float()
+    y32 = torch.zeros(xv.size, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    assert lib().tacs_mat_mult_prec(p.mat.ptr, 32, 64, xd.data_ptr(), ym.data_ptr()) == 0
+    assert lib().tacs_mat_mult_prec(p.mat.ptr, 32, 32, x32.data_ptr(), y32.data_ptr()) == 0
+    assert lib().tacs_mat_mult_prec(p.mat.ptr, 64, 32, x32.data_ptr(), y32.data_ptr()) != 0
+    torch.cuda.synchronize()
+    A32 = A.reshape(-1).astype(np.float32).astype(np.float64)
+    yo32 = O.bsr_mult(3, rowp, cols, A32, xv)
+    assert np.abs(ym.cpu().numpy() - yo32).max() <= 1e-13 * np.abs(yo32).max()
+    yo3232 = O.bsr_mult(3, rowp, cols, A32, xv.astype(np.float32).astype(np.float64))
+    assert np.abs(y32.cpu().numpy() - yo3232).max() <= 1e-6 * np.abs(yo3232).max()
+
+
 def test_golden_cube_compliance_and_ks():
     """tests/integration_tests/test_elast_linhexa_element_3d.py:11-27: 3x3x3
     cube of side 10, rho=2570, E=70e9, nu=0.3, ys=350e6, x=0 clamped,
