@@ -271,16 +271,16 @@ class TACSParallelMat : public TACSMat {
   float* getValues32(int refresh);
   // y = A x on device vectors: FP64 values and vectors; FP32-rounded values
   // with FP64 vectors (the recycling solver's operator); FP32 values and
-  // vectors.  3x3 blocks go through a sliced ELL copy of the values (slices of
-  // 64 rows, one lane per row, refreshed when the values change); other block
-  // sizes through the BCSR.
+  // vectors.  The products read a sliced ELL copy of the values (3x3 or 6x6
+  // blocks; slices of 64 rows, one lane per row), refreshed when the values
+  // change.
   void mult64(const double* x, double* y);
   void multMixed(const double* x, double* y);
   void mult32(const float* x, float* y);
   int bsize, nrows;
   long nnzb;
   long version;  // incremented whenever the values change
-  // sliced ELL copy (3x3 blocks)
+  // sliced ELL copy
   void sellUpdate(int prec);
   int* d_sw;
   long* d_soff;

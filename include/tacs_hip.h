@@ -219,23 +219,25 @@ int tacs_hip_bsr_diag_apply_mixed(int b, int nrows, const float* Dinv, const dou
    (the reference multiplies one vector at a time, BCSRMatVecMult3). */
 int tacs_hip_bsr3_spmm(int nrows, int nrhs, const int* rowp, const int* cols, const double* A,
                        const double* X, double* Y, hipStream_t stream);
-/* Sliced ELL copy of a 3x3 BCSR matrix for the product y = A x (no
+/* Sliced ELL copy of a 3x3 or 6x6 BCSR matrix for the product y = A x (no
    counterpart in the reference; BCSRMat::mult src/bpmat/BCSRMat.cpp:1226 is
    what it computes).  Slices of 64 rows; slice s is width[s] blocks wide and
    starts at slot coff[s] (coff has nslices + 1 entries); slot t of row
    64 s + l holds its column in cs[coff[s] + 64 t + l] and value q of the block
-   in vs[9 coff[s] + 64 (9 t + q) + l]; padding slots are zero blocks in the
-   row's own column.
-   sell3_slots:  host: widths and offsets from the host row pointer (either
-                 output may be NULL); returns the number of slots;
-   sell3_fill:   values (FP64 or FP32, prec 64/32) and, if cs != NULL, columns
-                 from the BCSR arrays;
-   sell3_spmv:   values prec_vals (64/32), vectors prec_vecs (64/32). */
-size_t tacs_hip_sell3_slots(int nrows, const int* rowp_host, int* width, long* coff);
-int tacs_hip_sell3_fill(int prec, int nrows, const int* rowp, const int* cols, const int* width,
-                        const long* coff, const double* A, void* vs, int* cs, hipStream_t stream);
-int tacs_hip_sell3_spmv(int prec_vals, int prec_vecs, int nrows, const int* width, const long* coff,
-                        const int* cs, const void* vs, const void* x, void* y, hipStream_t stream);
+   (row-major) in vs[b^2 coff[s] + 64 (b^2 t + q) + l]; padding slots are zero
+   blocks in the row's own column.
+   sell_slots:  host: widths and offsets from the host row pointer (either
+                output may be NULL); returns the number of slots;
+   sell_fill:   values (FP64 or FP32, prec 64/32) and, if cs != NULL, columns
+                from the BCSR arrays;
+   sell_spmv:   values prec_vals (64/32), vectors prec_vecs (64/32). */
+size_t tacs_hip_sell_slots(int nrows, const int* rowp_host, int* width, long* coff);
+int tacs_hip_sell_fill(int b, int prec, int nrows, const int* rowp, const int* cols,
+                       const int* width, const long* coff, const double* A, void* vs, int* cs,
+                       hipStream_t stream);
+int tacs_hip_sell_spmv(int b, int prec_vals, int prec_vecs, int nrows, const int* width,
+                       const long* coff, const int* cs, const void* vs, const void* x, void* y,
+                       hipStream_t stream);
 /* Diagonal block positions; inverse diagonal blocks (BCSRMat::factorDiag
    src/bpmat/BCSRMat.cpp:1527); y = D^{-1} x. */
 int tacs_hip_bsr_find_diag(int nrows, const int* rowp, const int* cols, int* diag,
@@ -310,7 +312,7 @@ int tacs_hip_tsgemm(size_t n, int p, const double* X, int q, const double* Y, do
    badd:         y[j*sy + v] += x[j*sx + v], v < k
    bscale:       x(:, j) *= s[j];   bsqrt_inv: nrm = sqrt(sq), inv = 1/nrm
    bgivens / bbacksolve: the GMRES Givens update / back-substitution per j
-   bdiag_apply3: 3x3 block-Jacobi (FP64 or FP32 inverse blocks)
+   bdiag_apply:  b x b block-Jacobi, b = 3 or 6 (FP64 or FP32 inverse blocks)
    bsr3_spmm_mixed: Y = A32 X with FP32 values and FP64 block vectors
    bbasis_combine: out_c(:, j) (+)= sum_{v < nin} W_v(:, j) P[j*sP + v + c*ldp], c < nout <= 32
    bbroadcast:   X(:, j) = x for every j
@@ -354,8 +356,8 @@ int tacs_hip_bgivens(int R, int i, int m, double* H, size_t sH, double* Qc, doub
                      double* res, size_t sRes, double* st, hipStream_t stream);
 int tacs_hip_bbacksolve(int R, int m, const int* cnt, const double* H, size_t sH, double* res,
                         size_t sRes, hipStream_t stream);
-int tacs_hip_bdiag_apply3(int prec, int nrows, int R, const void* Dinv, const double* x,
-                          double* y, hipStream_t stream);
+int tacs_hip_bdiag_apply(int b, int prec, int nrows, int R, const void* Dinv, const double* x,
+                         double* y, hipStream_t stream);
 int tacs_hip_bsr3_spmm_mixed(int nrows, int nrhs, const int* rowp, const int* cols, const float* A,
                              const double* X, double* Y, hipStream_t stream);
 int tacs_hip_bbasis_combine(size_t n, int R, int nin, const double* W, size_t ldw, const double* P,

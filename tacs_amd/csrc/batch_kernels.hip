@@ -360,20 +360,25 @@ __global__ void bbacksolve(int R, int m, const int* __restrict__ cnt, const doub
   }
 }
 
-// y = D^{-1} x per 3x3 block for interleaved block vectors (block-Jacobi)
-template <typename TD>
-__global__ void bdiag_apply3(int nrows, int R, const TD* __restrict__ Dinv,
-                             const double* __restrict__ x, double* __restrict__ y) {
+// y = D^{-1} x per b x b block (b = 3 or 6) for interleaved block vectors
+// (block-Jacobi)
+template <int B, typename TD>
+__global__ void bdiag_apply(int nrows, int R, const TD* __restrict__ Dinv,
+                            const double* __restrict__ x, double* __restrict__ y) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)nrows * R) return;
   int node = (int)(t / R), j = (int)(t % R);
-  const TD* d = &Dinv[(size_t)9 * node];
-  double x0 = x[((size_t)3 * node) * R + j], x1 = x[((size_t)3 * node + 1) * R + j],
-         x2 = x[((size_t)3 * node + 2) * R + j];
+  const TD* d = &Dinv[(size_t)B * B * node];
+  double xc[B];
 #pragma unroll
-  for (int r = 0; r < 3; r++)
-    y[((size_t)3 * node + r) * R + j] = (double)d[3 * r] * x0 + (double)d[3 * r + 1] * x1 +
-                                        (double)d[3 * r + 2] * x2;
+  for (int c = 0; c < B; c++) xc[c] = x[((size_t)B * node + c) * R + j];
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    double v = (double)d[B * r] * xc[0];
+#pragma unroll
+    for (int c = 1; c < B; c++) v += (double)d[B * r + c] * xc[c];
+    y[((size_t)B * node + r) * R + j] = v;
+  }
 }
 
 // Multi right-hand-side 3x3 SpMM, Y = A X, over interleaved block vectors
@@ -871,16 +876,20 @@ int tacs_hip_bbacksolve(int R, int m, const int* cnt, const double* H, size_t sH
   return (int)hipGetLastError();
 }
 
-int tacs_hip_bdiag_apply3(int prec, int nrows, int R, const void* Dinv, const double* x,
-                          double* y, hipStream_t stream) {
+int tacs_hip_bdiag_apply(int b, int prec, int nrows, int R, const void* Dinv, const double* x,
+                         double* y, hipStream_t stream) {
+  if ((b != 3 && b != 6) || (prec != 32 && prec != 64)) return TACS_HIP_ERR_ARG;
   size_t N = (size_t)nrows * R;
   if (N == 0) return 0;
-  if (prec == 64)
-    hipLaunchKernelGGL((bdiag_apply3<double>), dim3((N + 255) / 256), dim3(256), 0, stream, nrows,
-                       R, (const double*)Dinv, x, y);
+  dim3 g((unsigned)((N + 255) / 256)), t(256);
+  if (b == 3 && prec == 64)
+    hipLaunchKernelGGL((bdiag_apply<3, double>), g, t, 0, stream, nrows, R, (const double*)Dinv, x, y);
+  else if (b == 3)
+    hipLaunchKernelGGL((bdiag_apply<3, float>), g, t, 0, stream, nrows, R, (const float*)Dinv, x, y);
+  else if (prec == 64)
+    hipLaunchKernelGGL((bdiag_apply<6, double>), g, t, 0, stream, nrows, R, (const double*)Dinv, x, y);
   else
-    hipLaunchKernelGGL((bdiag_apply3<float>), dim3((N + 255) / 256), dim3(256), 0, stream, nrows,
-                       R, (const float*)Dinv, x, y);
+    hipLaunchKernelGGL((bdiag_apply<6, float>), g, t, 0, stream, nrows, R, (const float*)Dinv, x, y);
   return (int)hipGetLastError();
 }
 

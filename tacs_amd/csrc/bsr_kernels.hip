@@ -310,60 +310,104 @@ __global__ void bsr_zero_rows(int b, int nbc, const int* __restrict__ nodes,
 }
 
 // ---------------------------------------------------------------------------
-// Sliced ELL copy of a 3x3 BCSR matrix for the product (no counterpart in the
-// reference, whose BCSRMat is the only layout; the BCSR stays the matrix of
-// record).  Rows are cut into slices of 64 (one wavefront); slice s has width
-// w_s = the longest row of the slice in blocks; slot t of row 64 s + lane has
-// its column at cs[coff[s] + 64 t + lane] and value q of the block at
-// vs[9 coff[s] + 64 (9 t + q) + lane].  Padding slots point at the row itself
-// with zero values.  With lane = row every load of a wavefront is 64
-// consecutive entries and no cross-lane reduction is needed.
+// Sliced ELL copy of a b x b BCSR matrix (b = 3 or 6) for the product (no
+// counterpart in the reference, whose BCSRMat is the only layout; the BCSR
+// stays the matrix of record).  Rows are cut into slices of 64 (one
+// wavefront); slice s has width w_s = the longest row of the slice in blocks;
+// slot t of row 64 s + lane has its column at cs[coff[s] + 64 t + lane] and
+// value q (row-major in the block) at vs[b^2 coff[s] + 64 (b^2 t + q) + lane].
+// Padding slots point at the row itself with zero values.  With lane = row
+// every load of a wavefront is 64 consecutive entries, no cross-lane
+// reduction is needed, and each row adds its blocks in column order with the
+// reference's in-block order (BCSRMatVecMult3 src/bpmat/BCSRMatMult3.cpp:28).
 // ---------------------------------------------------------------------------
-template <typename TA, typename T>
-__global__ __launch_bounds__(256) void sell3_spmv(int nslices, int nrows,
-                                                  const int* __restrict__ width,
-                                                  const long* __restrict__ coff,
-                                                  const int* __restrict__ cs,
-                                                  const TA* __restrict__ vs,
-                                                  const T* __restrict__ x, T* __restrict__ y) {
+template <int B, typename TA, typename T>
+__device__ inline void sell_slot(const TA* __restrict__ vs, long v, const T* __restrict__ x, int col,
+                                 double* yr) {
+  TA a[B * B];
+#pragma unroll
+  for (int q = 0; q < B * B; q++) a[q] = __builtin_nontemporal_load(&vs[v + 64 * q]);
+  double xc[B];
+#pragma unroll
+  for (int c = 0; c < B; c++) xc[c] = (double)x[(size_t)B * col + c];
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    double t = (double)a[B * r] * xc[0];
+#pragma unroll
+    for (int c = 1; c < B; c++) t += (double)a[B * r + c] * xc[c];
+    yr[r] += t;
+  }
+}
+
+// U slots per iteration: every load of the U slots is issued before the
+// first block is multiplied (U = 2 for 3x3 blocks: about 2 % faster for FP64
+// values and 5 % for FP32 at 128^3 than one slot at a time); the blocks are
+// still added in slot order.
+template <int B, int U, typename TA, typename T>
+__global__ __launch_bounds__(256) void sell_spmv(int nslices, int nrows,
+                                                 const int* __restrict__ width,
+                                                 const long* __restrict__ coff,
+                                                 const int* __restrict__ cs,
+                                                 const TA* __restrict__ vs,
+                                                 const T* __restrict__ x, T* __restrict__ y) {
+  constexpr int B2 = B * B;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int s = blockIdx.x * 4 + wid; s < nslices; s += gridDim.x * 4) {
     const int w = width[s];
-    const long co = coff[s], vo = 9 * co;
-    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-    for (int t = 0; t < w; t++) {
-      const int col = cs[co + 64 * t + lane];
-      TA a[9];
+    const long co = coff[s] + lane, vo = B2 * coff[s] + lane;
+    double yr[B];
 #pragma unroll
-      for (int q = 0; q < 9; q++) a[q] = __builtin_nontemporal_load(&vs[vo + 64 * (9 * t + q) + lane]);
-      const double x0 = (double)x[3 * col], x1 = (double)x[3 * col + 1],
-                   x2 = (double)x[3 * col + 2];
-      y0 += (double)a[0] * x0 + (double)a[1] * x1 + (double)a[2] * x2;
-      y1 += (double)a[3] * x0 + (double)a[4] * x1 + (double)a[5] * x2;
-      y2 += (double)a[6] * x0 + (double)a[7] * x1 + (double)a[8] * x2;
+    for (int r = 0; r < B; r++) yr[r] = 0.0;
+    int t = 0;
+    if (U > 1) {
+      for (; t + U <= w; t += U) {
+        int cu[U];
+        TA a[U][B2];
+#pragma unroll
+        for (int u = 0; u < U; u++) cu[u] = cs[co + 64 * (t + u)];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+          for (int q = 0; q < B2; q++)
+            a[u][q] = __builtin_nontemporal_load(&vs[vo + 64 * (B2 * (t + u) + q)]);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          double xc[B];
+#pragma unroll
+          for (int c = 0; c < B; c++) xc[c] = (double)x[(size_t)B * cu[u] + c];
+#pragma unroll
+          for (int r = 0; r < B; r++) {
+            double v = (double)a[u][B * r] * xc[0];
+#pragma unroll
+            for (int c = 1; c < B; c++) v += (double)a[u][B * r + c] * xc[c];
+            yr[r] += v;
+          }
+        }
+      }
     }
+    for (; t < w; t++) sell_slot<B, TA, T>(vs, vo + 64 * B2 * t, x, cs[co + 64 * t], yr);
     const int row = 64 * s + lane;
     if (row < nrows) {
-      y[3 * (size_t)row] = (T)y0;
-      y[3 * (size_t)row + 1] = (T)y1;
-      y[3 * (size_t)row + 2] = (T)y2;
+#pragma unroll
+      for (int r = 0; r < B; r++) y[(size_t)B * row + r] = (T)yr[r];
     }
   }
 }
 
 // Fill the sliced ELL values (and, when cs is not NULL, the columns) from the
 // BCSR arrays: one lane per row, slot by slot.
-template <typename TA>
-__global__ __launch_bounds__(256) void sell3_fill(int nslices, int nrows, const int* __restrict__ rowp,
-                                                  const int* __restrict__ cols,
-                                                  const int* __restrict__ width,
-                                                  const long* __restrict__ coff,
-                                                  const double* __restrict__ A, TA* __restrict__ vs,
-                                                  int* __restrict__ cs) {
+template <int B, typename TA>
+__global__ __launch_bounds__(256) void sell_fill(int nslices, int nrows, const int* __restrict__ rowp,
+                                                 const int* __restrict__ cols,
+                                                 const int* __restrict__ width,
+                                                 const long* __restrict__ coff,
+                                                 const double* __restrict__ A, TA* __restrict__ vs,
+                                                 int* __restrict__ cs) {
+  constexpr int B2 = B * B;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int s = blockIdx.x * 4 + wid; s < nslices; s += gridDim.x * 4) {
     const int w = width[s];
-    const long co = coff[s], vo = 9 * co;
+    const long co = coff[s], vo = B2 * co;
     const int row = 64 * s + lane;
     const int rs = row < nrows ? rowp[row] : 0;
     const int nb = row < nrows ? rowp[row + 1] - rs : 0;
@@ -371,8 +415,8 @@ __global__ __launch_bounds__(256) void sell3_fill(int nslices, int nrows, const 
       const bool use = t < nb;
       if (cs) cs[co + 64 * t + lane] = use ? cols[rs + t] : (row < nrows ? row : 0);
 #pragma unroll
-      for (int q = 0; q < 9; q++)
-        vs[vo + 64 * (9 * t + q) + lane] = use ? (TA)A[9 * ((size_t)rs + t) + q] : (TA)0;
+      for (int q = 0; q < B2; q++)
+        vs[vo + 64 * (B2 * t + q) + lane] = use ? (TA)A[B2 * ((size_t)rs + t) + q] : (TA)0;
     }
   }
 }
@@ -443,7 +487,7 @@ int tacs_hip_bsr_diag_apply_mixed(int b, int nrows, const float* Dinv, const dou
   return (int)hipGetLastError();
 }
 
-size_t tacs_hip_sell3_slots(int nrows, const int* rowp_host, int* width, long* coff) {
+size_t tacs_hip_sell_slots(int nrows, const int* rowp_host, int* width, long* coff) {
   const int ns = (nrows + 63) / 64;
   long off = 0;
   for (int s = 0; s < ns; s++) {
@@ -457,36 +501,58 @@ size_t tacs_hip_sell3_slots(int nrows, const int* rowp_host, int* width, long* c
   return (size_t)off;
 }
 
-int tacs_hip_sell3_fill(int prec, int nrows, const int* rowp, const int* cols, const int* width,
-                        const long* coff, const double* A, void* vs, int* cs, hipStream_t stream) {
+int tacs_hip_sell_fill(int b, int prec, int nrows, const int* rowp, const int* cols,
+                       const int* width, const long* coff, const double* A, void* vs, int* cs,
+                       hipStream_t stream) {
   const int ns = (nrows + 63) / 64;
+  if ((b != 3 && b != 6) || (prec != 64 && prec != 32)) return TACS_HIP_ERR_ARG;
   if (ns <= 0) return 0;
   const int grid = tacs_grid_for(ns, 4, 65536);
-  if (prec == 64)
-    hipLaunchKernelGGL((sell3_fill<double>), dim3(grid), dim3(256), 0, stream, ns, nrows, rowp, cols,
-                       width, coff, A, (double*)vs, cs);
+#define SELL_FILL(BB, TA)                                                                     \
+  hipLaunchKernelGGL((sell_fill<BB, TA>), dim3(grid), dim3(256), 0, stream, ns, nrows, rowp, cols, \
+                     width, coff, A, (TA*)vs, cs)
+  if (b == 3 && prec == 64)
+    SELL_FILL(3, double);
+  else if (b == 3)
+    SELL_FILL(3, float);
+  else if (prec == 64)
+    SELL_FILL(6, double);
   else
-    hipLaunchKernelGGL((sell3_fill<float>), dim3(grid), dim3(256), 0, stream, ns, nrows, rowp, cols,
-                       width, coff, A, (float*)vs, cs);
+    SELL_FILL(6, float);
+#undef SELL_FILL
   return (int)hipGetLastError();
 }
 
-int tacs_hip_sell3_spmv(int prec_vals, int prec_vecs, int nrows, const int* width, const long* coff,
-                        const int* cs, const void* vs, const void* x, void* y, hipStream_t stream) {
+int tacs_hip_sell_spmv(int b, int prec_vals, int prec_vecs, int nrows, const int* width,
+                       const long* coff, const int* cs, const void* vs, const void* x, void* y,
+                       hipStream_t stream) {
   const int ns = (nrows + 63) / 64;
+  const int kind = (prec_vals == 64 && prec_vecs == 64)   ? 0
+                   : (prec_vals == 32 && prec_vecs == 64) ? 1
+                   : (prec_vals == 32 && prec_vecs == 32) ? 2
+                                                          : -1;
+  if ((b != 3 && b != 6) || kind < 0) return TACS_HIP_ERR_ARG;
   if (ns <= 0) return 0;
   const int grid = tacs_grid_for(ns, 4, 65536);
-  if (prec_vals == 64 && prec_vecs == 64)
-    hipLaunchKernelGGL((sell3_spmv<double, double>), dim3(grid), dim3(256), 0, stream, ns, nrows,
-                       width, coff, cs, (const double*)vs, (const double*)x, (double*)y);
-  else if (prec_vals == 32 && prec_vecs == 64)
-    hipLaunchKernelGGL((sell3_spmv<float, double>), dim3(grid), dim3(256), 0, stream, ns, nrows,
-                       width, coff, cs, (const float*)vs, (const double*)x, (double*)y);
-  else if (prec_vals == 32 && prec_vecs == 32)
-    hipLaunchKernelGGL((sell3_spmv<float, float>), dim3(grid), dim3(256), 0, stream, ns, nrows,
-                       width, coff, cs, (const float*)vs, (const float*)x, (float*)y);
-  else
-    return TACS_HIP_ERR_ARG;
+#define SELL_SPMV(BB, U, TA, T)                                                                  \
+  hipLaunchKernelGGL((sell_spmv<BB, U, TA, T>), dim3(grid), dim3(256), 0, stream, ns, nrows, width, \
+                     coff, cs, (const TA*)vs, (const T*)x, (T*)y)
+  if (b == 3) {
+    if (kind == 0)
+      SELL_SPMV(3, 2, double, double);
+    else if (kind == 1)
+      SELL_SPMV(3, 2, float, double);
+    else
+      SELL_SPMV(3, 2, float, float);
+  } else {
+    if (kind == 0)
+      SELL_SPMV(6, 1, double, double);
+    else if (kind == 1)
+      SELL_SPMV(6, 1, float, double);
+    else
+      SELL_SPMV(6, 1, float, float);
+  }
+#undef SELL_SPMV
   return (int)hipGetLastError();
 }
 

@@ -428,11 +428,12 @@ void TACSParallelMat::mult(TACSVec* tx, TACSVec* ty) {
 // version moved.
 void TACSParallelMat::sellUpdate(int prec) {
   const int ns = (nrows + 63) / 64;
+  const size_t b2 = (size_t)bsize * bsize;
   int* cs_fill = NULL;
   if (!d_sw) {
     std::vector<int> w(ns > 0 ? ns : 1);
     std::vector<long> off((size_t)ns + 1);
-    nslots = tacs_hip_sell3_slots(nrows, h_rowp, w.data(), off.data());
+    nslots = tacs_hip_sell_slots(nrows, h_rowp, w.data(), off.data());
     d_sw = dalloc<int>(ns);
     d_soff = dalloc<long>((size_t)ns + 1);
     d_scols = dalloc<int>(nslots);
@@ -442,42 +443,29 @@ void TACSParallelMat::sellUpdate(int prec) {
     cs_fill = d_scols;
   }
   if (prec == 64) {
-    if (!d_s64) d_s64 = dalloc<double>(9 * nslots);
-    KCHECK(tacs_hip_sell3_fill(64, nrows, d_rowp, d_cols, d_sw, d_soff, d_A, d_s64, cs_fill,
-                               g_stream));
+    if (!d_s64) d_s64 = dalloc<double>(b2 * nslots);
+    KCHECK(tacs_hip_sell_fill(bsize, 64, nrows, d_rowp, d_cols, d_sw, d_soff, d_A, d_s64, cs_fill,
+                              g_stream));
     sellVer64 = version;
   } else {
-    if (!d_s32) d_s32 = dalloc<float>(9 * nslots);
-    KCHECK(tacs_hip_sell3_fill(32, nrows, d_rowp, d_cols, d_sw, d_soff, d_A, d_s32, cs_fill,
-                               g_stream));
+    if (!d_s32) d_s32 = dalloc<float>(b2 * nslots);
+    KCHECK(tacs_hip_sell_fill(bsize, 32, nrows, d_rowp, d_cols, d_sw, d_soff, d_A, d_s32, cs_fill,
+                              g_stream));
     sellVer32 = version;
   }
 }
+// The products run over the sliced ELL copies (3x3 and 6x6 blocks).
 void TACSParallelMat::mult64(const double* x, double* y) {
-  if (bsize != 3) {
-    KCHECK(tacs_hip_bsr_spmv(bsize, nrows, d_rowp, d_cols, d_A, x, y, 0, g_stream));
-    return;
-  }
   if (sellVer64 != version) sellUpdate(64);
-  KCHECK(tacs_hip_sell3_spmv(64, 64, nrows, d_sw, d_soff, d_scols, d_s64, x, y, g_stream));
+  KCHECK(tacs_hip_sell_spmv(bsize, 64, 64, nrows, d_sw, d_soff, d_scols, d_s64, x, y, g_stream));
 }
 void TACSParallelMat::multMixed(const double* x, double* y) {
-  if (bsize != 3) {
-    KCHECK(tacs_hip_bsr_spmv_mixed(bsize, nrows, d_rowp, d_cols, getValues32(1), x, y, 0,
-                                   g_stream));
-    return;
-  }
   if (sellVer32 != version) sellUpdate(32);
-  KCHECK(tacs_hip_sell3_spmv(32, 64, nrows, d_sw, d_soff, d_scols, d_s32, x, y, g_stream));
+  KCHECK(tacs_hip_sell_spmv(bsize, 32, 64, nrows, d_sw, d_soff, d_scols, d_s32, x, y, g_stream));
 }
 void TACSParallelMat::mult32(const float* x, float* y) {
-  if (bsize != 3) {
-    KCHECK(tacs_hip_bsr_spmv_f32(bsize, nrows, d_rowp, d_cols, getValues32(1), x, y, 0,
-                                 g_stream));
-    return;
-  }
   if (sellVer32 != version) sellUpdate(32);
-  KCHECK(tacs_hip_sell3_spmv(32, 32, nrows, d_sw, d_soff, d_scols, d_s32, x, y, g_stream));
+  KCHECK(tacs_hip_sell_spmv(bsize, 32, 32, nrows, d_sw, d_soff, d_scols, d_s32, x, y, g_stream));
 }
 void TACSParallelMat::zeroEntries() {
   KCHECK(tacs_hip_fill(64, (size_t)nnzb * bsize * bsize, d_A, 0.0, g_stream));
@@ -1222,9 +1210,16 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
     tacs_set_error("TACSBatchedGMRES: nrhs = %d outside [1, %d]", Rn, R);
     return -1;
   }
+  // several right-hand sides share the 3x3 multi-RHS SpMM; one right-hand
+  // side runs the single-vector products of any block size
+  if (Rn > 1 && mat->bsize != 3) {
+    tacs_set_error("TACSBatchedGMRES: %d right-hand sides need 3x3 blocks (b = %d)", Rn,
+                   mat->bsize);
+    return -1;
+  }
   if (ndefl > 0) return solveBlockDeflated(Rn, Bd, Xd, zero_guess);
   const int nr = mat->nrows, m = msub;
-  const size_t n = (size_t)nr * 3, N = n * Rn, S = slice();
+  const size_t n = (size_t)nr * mat->bsize, N = n * Rn, S = slice();
   // per right-hand side slice (stride S)
   double* H = d_scal;
   double* res = d_scal + (size_t)(m + 1) * m;
@@ -1242,13 +1237,19 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
   const float* A32 = prec == 32 ? mat->getValues32(1) : NULL;
   const void* Dinv = prec == 32 ? (const void*)pc->d_Dinv32 : (const void*)pc->d_Dinv;
   const int kd = (kdefl > 0 && kVersion == mat->version) ? kdefl : 0;
+  // one right-hand side: the single-vector products
   auto spmm64 = [&](const double* x, double* y) {
-    KCHECK(tacs_hip_bsr3_spmm(nr, Rn, mat->d_rowp, mat->d_cols, mat->d_A, x, y, g_stream));
+    if (Rn == 1)
+      mat->mult64(x, y);
+    else
+      KCHECK(tacs_hip_bsr3_spmm(nr, Rn, mat->d_rowp, mat->d_cols, mat->d_A, x, y, g_stream));
   };
   // y = A M^{-1} x with the Arnoldi operator of this precision
   auto op = [&](const double* x, double* y) {
-    KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, x, d_work, g_stream));
-    if (prec == 32)
+    KCHECK(tacs_hip_bdiag_apply(mat->bsize, prec, nr, Rn, Dinv, x, d_work, g_stream));
+    if (prec == 32 && Rn == 1)
+      mat->multMixed(d_work, y);
+    else if (prec == 32)
       KCHECK(tacs_hip_bsr3_spmm_mixed(nr, Rn, mat->d_rowp, mat->d_cols, A32, d_work, y, g_stream));
     else
       spmm64(d_work, y);
@@ -1308,7 +1309,7 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
       // x += |r| M^{-1} U C^T v_0;  v_0 -= C C^T v_0
       KCHECK(tacs_hip_bmdot_shared(n, Rn, kd, d_C, n, V(0), (double*)d_red, pco, S, 0, g_stream));
       KCHECK(tacs_hip_blincomb_shared(n, Rn, kd, d_U, n, pco, S, V(1), 0, g_stream));
-      KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, V(1), d_work, g_stream));
+      KCHECK(tacs_hip_bdiag_apply(mat->bsize, prec, nr, Rn, Dinv, V(1), d_work, g_stream));
       KCHECK(tacs_hip_bscale(n, Rn, nrm, d_work, g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, d_work, 1.0, Xd, g_stream));
       KCHECK(tacs_hip_bcgs_update_shared(n, Rn, kd, d_C, n, pco, S, V(0), g_stream));
@@ -1384,7 +1385,7 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
         KCHECK(tacs_hip_bsmall_gemv(Rn, kd, d_cnt, Bc, S, KD_MAX, res, S, zc, S, g_stream));
         KCHECK(tacs_hip_bcgs_update_shared(n, Rn, kd, d_U, n, zc, S, d_work, g_stream));
       }
-      KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, d_work, V(1), g_stream));
+      KCHECK(tacs_hip_bdiag_apply(mat->bsize, prec, nr, Rn, Dinv, d_work, V(1), g_stream));
       KCHECK(tacs_hip_bscale(n, Rn, nrm, V(1), g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, V(1), 1.0, Xd, g_stream));
     }
@@ -1410,7 +1411,7 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
 // ---------------------------------------------------------------------------
 int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, int zero_guess) {
   const int nr = mat->nrows, m = msub, K = ndefl;
-  const size_t n = (size_t)nr * 3, N = n * Rn, S = slice(), SP = prodStride();
+  const size_t n = (size_t)nr * mat->bsize, N = n * Rn, S = slice(), SP = prodStride();
   double* H = d_scal;
   double* res = d_scal + (size_t)(m + 1) * m;
   double* tmp = res + (m + 1) + 2 * (size_t)m;
@@ -1436,7 +1437,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       KCHECK(tacs_hip_bsr3_spmm(nr, Rn, mat->d_rowp, mat->d_cols, mat->d_A, x, y, g_stream));
   };
   auto op = [&](const double* x, double* y) {
-    KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, x, d_work, g_stream));
+    KCHECK(tacs_hip_bdiag_apply(mat->bsize, prec, nr, Rn, Dinv, x, d_work, g_stream));
     if (prec == 32 && Rn == 1)
       mat->multMixed(d_work, y);
     else if (prec == 32)
@@ -1674,7 +1675,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       // x += |r| M^{-1} U C^T v_0;  v_0 -= C C^T v_0
       KCHECK(tacs_hip_bmdot(n, Rn, Kd, Cb(0), N, V(0), (double*)d_red, pco, S, 0, g_stream));
       KCHECK(tacs_hip_blincomb(n, Rn, Kd, Ub(0), N, pco, S, NULL, d_work, 0, g_stream));
-      KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, d_work, tmp2, g_stream));
+      KCHECK(tacs_hip_bdiag_apply(mat->bsize, prec, nr, Rn, Dinv, d_work, tmp2, g_stream));
       KCHECK(tacs_hip_bscale(n, Rn, nrm, tmp2, g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, tmp2, 1.0, Xd, g_stream));
       KCHECK(tacs_hip_bcgs_update(n, Rn, Kd, Cb(0), N, pco, S, V(0), g_stream));
@@ -1775,7 +1776,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       h2d(inv, h_inv, Rn);
       if (Kd > 0) KCHECK(tacs_hip_blincomb(n, Rn, Kd, Ub(0), N, zc, S, NULL, d_work, 0, g_stream));
       KCHECK(tacs_hip_blincomb(n, Rn, niters, V(0), N, res, S, NULL, d_work, Kd > 0, g_stream));
-      KCHECK(tacs_hip_bdiag_apply3(prec, nr, Rn, Dinv, d_work, tmp2, g_stream));
+      KCHECK(tacs_hip_bdiag_apply(mat->bsize, prec, nr, Rn, Dinv, d_work, tmp2, g_stream));
       KCHECK(tacs_hip_bscale(n, Rn, inv, tmp2, g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, tmp2, 1.0, Xd, g_stream));
     }

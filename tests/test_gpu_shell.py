@@ -123,13 +123,17 @@ def test_shell_plate_golden(case, comp, ks):
     assert abs(k - ks) <= 1e-9 * ks
 
 
-@pytest.mark.parametrize("solver", ["fp64", "mixed"])
+@pytest.mark.parametrize("solver", ["fp64", "mixed", "mixed_dr"])
 def test_shell_solve_functions_and_thickness_adjoint(solver):
+    """mixed_dr: the recycling solver (6x6 block-Jacobi and products over
+    6-dof nodes, recycled space reused by the adjoint solve)."""
     p, t = _setup(8, warp=0.01, seed=4)
     a = p.assembler
     p.assemble()
     if solver == "fp64":
         flag, its, pc, ksm = p.solve_fp64(m=120, restarts=200, rtol=1e-12)
+    elif solver == "mixed_dr":
+        flag, its, pc, ksm = p.solve_mixed(m=40, restarts=500, rtol=1e-12, deflation=12)
     else:
         flag, its, pc, ksm = p.solve_mixed(m=120, restarts=500, rtol=1e-12)
     assert flag == 1

@@ -74,7 +74,7 @@ def test_spmv_matches_oracle():
     assert np.abs(y.getArray() - yo).max() <= 1e-13 * np.abs(A).max() * np.abs(xv).sum() / xv.size * 30
 
 
-def _ragged_pattern(nrows, seed):
+def _ragged_pattern(nrows, seed, b=3):
     """Random sorted-unique block-CSR pattern with empty rows, rows up to 40
     blocks long and a row count that is not a multiple of 64."""
     rng = np.random.default_rng(seed)
@@ -84,14 +84,15 @@ def _ragged_pattern(nrows, seed):
     rowp = np.zeros(nrows + 1, np.int32)
     rowp[1:] = np.cumsum([r.size for r in rows])
     cols = np.concatenate(rows).astype(np.int32)
-    A = rng.standard_normal(9 * cols.size)
+    A = rng.standard_normal(b * b * cols.size)
     return rowp, cols, A
 
 
-@pytest.mark.parametrize("nrows", [1, 63, 1000, 4161])
-def test_sliced_ell_spmv_matches_oracle(nrows):
-    """The sliced ELL copy of a 3x3 BCSR matrix (slices of 64 rows, zero
-    padding blocks) and its products with FP64 values and vectors, FP32
+@pytest.mark.parametrize("b,nrows", [(3, 1), (3, 63), (3, 1000), (3, 4161), (6, 1), (6, 1000),
+                                     (6, 2113)])
+def test_sliced_ell_spmv_matches_oracle(b, nrows):
+    """The sliced ELL copy of a 3x3 or 6x6 BCSR matrix (slices of 64 rows,
+    zero padding blocks) and its products with FP64 values and vectors, FP32
     values and FP64 vectors, FP32 values and vectors, against the oracle's
     block-CSR product (BCSRMat::mult) on a ragged pattern."""
     import ctypes
@@ -100,59 +101,66 @@ def test_sliced_ell_spmv_matches_oracle(nrows):
 
     from tacs_amd import lib
 
-    rowp, cols, A = _ragged_pattern(nrows, nrows)
+    b2 = b * b
+    rowp, cols, A = _ragged_pattern(nrows, nrows + b, b)
     ns = (nrows + 63) // 64
     width = np.zeros(ns, np.int32)
     coff = np.zeros(ns + 1, np.int64)
-    slots = lib().tacs_hip_sell3_slots(nrows, rowp.ctypes.data, width.ctypes.data, coff.ctypes.data)
+    slots = lib().tacs_hip_sell_slots(nrows, rowp.ctypes.data, width.ctypes.data, coff.ctypes.data)
     lens = np.diff(rowp)
     for s in range(ns):
         assert width[s] == lens[64 * s:64 * s + 64].max()
     assert coff[-1] == slots == 64 * width.sum()
-    assert lib().tacs_hip_sell3_slots(nrows, rowp.ctypes.data, None, None) == slots
+    assert lib().tacs_hip_sell_slots(nrows, rowp.ctypes.data, None, None) == slots
     dev = lambda a: torch.from_numpy(a).to("cuda")
     rp, cp, Ad, wd, od = dev(rowp), dev(cols), dev(A), dev(width), dev(coff)
     cs = torch.full((max(slots, 1),), -1, dtype=torch.int32, device="cuda")
-    v64 = torch.full((9 * max(slots, 1),), np.nan, dtype=torch.float64, device="cuda")
-    v32 = torch.full((9 * max(slots, 1),), np.nan, dtype=torch.float32, device="cuda")
+    v64 = torch.full((b2 * max(slots, 1),), np.nan, dtype=torch.float64, device="cuda")
+    v32 = torch.full((b2 * max(slots, 1),), np.nan, dtype=torch.float32, device="cuda")
     rng = np.random.default_rng(7)
-    xh = rng.standard_normal(3 * nrows)
-    x64, y64 = dev(xh), torch.zeros(3 * nrows, dtype=torch.float64, device="cuda")
-    x32, y32 = x64.float(), torch.zeros(3 * nrows, dtype=torch.float32, device="cuda")
+    xh = rng.standard_normal(b * nrows)
+    x64, y64 = dev(xh), torch.zeros(b * nrows, dtype=torch.float64, device="cuda")
+    x32, y32 = x64.float(), torch.zeros(b * nrows, dtype=torch.float32, device="cuda")
     ym = torch.zeros_like(y64)
     torch.cuda.synchronize()
     P = lambda t: ctypes.c_void_p(t.data_ptr())
-    assert lib().tacs_hip_sell3_fill(64, nrows, P(rp), P(cp), P(wd), P(od), P(Ad), P(v64), P(cs),
-                                     None) == 0
-    assert lib().tacs_hip_sell3_fill(32, nrows, P(rp), P(cp), P(wd), P(od), P(Ad), P(v32), None,
-                                     None) == 0
-    assert lib().tacs_hip_sell3_spmv(64, 64, nrows, P(wd), P(od), P(cs), P(v64), P(x64), P(y64),
-                                     None) == 0
-    assert lib().tacs_hip_sell3_spmv(32, 64, nrows, P(wd), P(od), P(cs), P(v32), P(x64), P(ym),
-                                     None) == 0
-    assert lib().tacs_hip_sell3_spmv(32, 32, nrows, P(wd), P(od), P(cs), P(v32), P(x32), P(y32),
-                                     None) == 0
-    assert lib().tacs_hip_sell3_spmv(64, 32, nrows, P(wd), P(od), P(cs), P(v64), P(x64), P(y64),
-                                     None) != 0
+    L = lib()
+    assert L.tacs_hip_sell_fill(b, 64, nrows, P(rp), P(cp), P(wd), P(od), P(Ad), P(v64), P(cs),
+                                None) == 0
+    assert L.tacs_hip_sell_fill(b, 32, nrows, P(rp), P(cp), P(wd), P(od), P(Ad), P(v32), None,
+                                None) == 0
+    assert L.tacs_hip_sell_spmv(b, 64, 64, nrows, P(wd), P(od), P(cs), P(v64), P(x64), P(y64),
+                                None) == 0
+    assert L.tacs_hip_sell_spmv(b, 32, 64, nrows, P(wd), P(od), P(cs), P(v32), P(x64), P(ym),
+                                None) == 0
+    assert L.tacs_hip_sell_spmv(b, 32, 32, nrows, P(wd), P(od), P(cs), P(v32), P(x32), P(y32),
+                                None) == 0
+    # rejected before any launch: FP64 values with FP32 vectors, other block sizes
+    assert L.tacs_hip_sell_spmv(b, 64, 32, nrows, P(wd), P(od), P(cs), P(v64), P(x64), P(y64),
+                                None) != 0
+    assert L.tacs_hip_sell_spmv(4, 64, 64, nrows, P(wd), P(od), P(cs), P(v64), P(x64), P(y64),
+                                None) != 0
+    assert L.tacs_hip_sell_fill(5, 64, nrows, P(rp), P(cp), P(wd), P(od), P(Ad), P(v64), None,
+                                None) != 0
     torch.cuda.synchronize()
     # every slot written: columns in range, padding blocks zero in the row's own column
     csh = cs.cpu().numpy()[:slots]
-    vh = v64.cpu().numpy()[:9 * slots]
+    vh = v64.cpu().numpy()[:b2 * slots]
     assert csh.min() >= 0 and csh.max() < nrows and not np.isnan(vh).any()
     for s in range(ns):
         for t in range(width[s]):
             for l in range(64):
                 r = 64 * s + l
                 if r >= nrows or t >= lens[r]:
-                    assert vh[9 * coff[s] + 64 * 9 * t + l::64][:9].tolist() == [0.0] * 9
+                    assert vh[b2 * coff[s] + 64 * b2 * t + l::64][:b2].tolist() == [0.0] * b2
                     assert csh[coff[s] + 64 * t + l] == (r if r < nrows else 0)
-    yo = O.bsr_mult(3, rowp, cols, A, xh)
-    scale = np.abs(A).max() * np.abs(xh).max() * 40
+    yo = O.bsr_mult(b, rowp, cols, A, xh)
+    scale = np.abs(A).max() * np.abs(xh).max() * 40 * b
     assert np.abs(y64.cpu().numpy() - yo).max() <= 1e-14 * scale
     A32 = A.astype(np.float32).astype(np.float64)
-    yo32 = O.bsr_mult(3, rowp, cols, A32, xh)
+    yo32 = O.bsr_mult(b, rowp, cols, A32, xh)
     assert np.abs(ym.cpu().numpy() - yo32).max() <= 1e-14 * scale
-    yo3232 = O.bsr_mult(3, rowp, cols, A32, xh.astype(np.float32).astype(np.float64))
+    yo3232 = O.bsr_mult(b, rowp, cols, A32, xh.astype(np.float32).astype(np.float64))
     assert np.abs(y32.cpu().numpy() - yo3232).max() <= 1e-6 * scale
 
 
