@@ -432,6 +432,10 @@ class TACSBatchedGMRES : public TACSObject {
   double *d_Ub, *d_Cb, *d_Unb, *d_Cnb;  // K block vectors each
   double* d_prod;    // per right-hand side small matrices (products, basis changes)
   double* h_stage;   // pinned staging of the small matrices, R x prodStride()
+  double* h_up;      // pinned staging of uploads (same size); an upload is not
+                     // waited for: the next one waits on ev_up before reusing it
+  hipEvent_t ev_up;
+  int up_pending;
   void* d_dq;        // scratch of the delayed Gram-Schmidt dots and the products
   // Gram-Schmidt of the recycling (GCRO-DR) Arnoldi steps: 1 (default)
   // delayed classical Gram-Schmidt (DCGS2, two sweeps over the basis per

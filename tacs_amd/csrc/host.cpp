@@ -1092,7 +1092,8 @@ TACSBatchedGMRES::TACSBatchedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int
                                    int nrhs, int _prec)
     : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart), R(nrhs), prec(_prec), cycles(0),
       kdefl(0), kVersion(-1), d_U(NULL), d_C(NULL), ndefl(0), defVersion(-1), defR(0),
-      d_Ub(NULL), d_Cb(NULL), d_Unb(NULL), d_Cnb(NULL), d_prod(NULL), h_stage(NULL), d_dq(NULL),
+      d_Ub(NULL), d_Cb(NULL), d_Unb(NULL), d_Cnb(NULL), d_prod(NULL), h_stage(NULL), h_up(NULL),
+      ev_up(NULL), up_pending(0), d_dq(NULL),
       delayed(1), mat(m), pc(p) {
   mat->incref();
   pc->incref();
@@ -1117,6 +1118,8 @@ TACSBatchedGMRES::~TACSBatchedGMRES() {
   for (void* q : bufs) dfree(q);
   HCHECK(hipHostFree(h_pin));
   if (h_stage) HCHECK(hipHostFree(h_stage));
+  if (h_up) HCHECK(hipHostFree(h_up));
+  if (ev_up) HCHECK(hipEventDestroy(ev_up));
   mat->decref();
   pc->decref();
 }
@@ -1165,7 +1168,9 @@ int TACSBatchedGMRES::setDeflation(int k) {
   d_Ub = d_Cb = d_Unb = d_Cnb = d_prod = NULL;
   d_dq = NULL;
   if (h_stage) HCHECK(hipHostFree(h_stage));
-  h_stage = NULL;
+  if (h_up) HCHECK(hipHostFree(h_up));
+  h_stage = h_up = NULL;
+  up_pending = 0;
   ndefl = k;
   defVersion = -1;
   defR = 0;
@@ -1179,6 +1184,8 @@ int TACSBatchedGMRES::setDeflation(int k) {
     d_Cnb = dalloc<double>((size_t)k * N);
     d_prod = dalloc<double>((size_t)R * prodStride());
     HCHECK(hipHostMalloc((void**)&h_stage, (size_t)R * prodStride() * sizeof(double), 0));
+    HCHECK(hipHostMalloc((void**)&h_up, (size_t)R * prodStride() * sizeof(double), 0));
+    if (!ev_up) HCHECK(hipEventCreateWithFlags(&ev_up, hipEventDisableTiming));
     HCHECK(hipMalloc(&d_dq, std::max(tacs_hip_bdotq_scratch_bytes(R),
                                      tacs_hip_btsgemm_scratch_bytes(R))));
   }
@@ -1459,12 +1466,16 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
     HCHECK(hipStreamSynchronize(g_stream));
     memcpy(h, h_stage, (size_t)Rn * cols * sizeof(double));
   };
+  // uploads are not waited for (the kernels that read them follow in stream
+  // order); the staging buffer is reused once the previous upload is done
   auto h2d_rows = [&](double* d, size_t ds, const double* h, int cols) {
     if (cols <= 0) return;
-    memcpy(h_stage, h, (size_t)Rn * cols * sizeof(double));
-    HCHECK(hipMemcpy2DAsync(d, ds * sizeof(double), h_stage, cols * sizeof(double),
+    if (up_pending) HCHECK(hipEventSynchronize(ev_up));
+    memcpy(h_up, h, (size_t)Rn * cols * sizeof(double));
+    HCHECK(hipMemcpy2DAsync(d, ds * sizeof(double), h_up, cols * sizeof(double),
                             cols * sizeof(double), Rn, hipMemcpyHostToDevice, g_stream));
-    HCHECK(hipStreamSynchronize(g_stream));
+    HCHECK(hipEventRecord(ev_up, g_stream));
+    up_pending = 1;
   };
   std::vector<double> stage;
   // dsc[j][v] = 1 / |U_j(:, v)|
@@ -1522,7 +1533,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
   // i - 1 is made at step i.  Returns the number of finished columns.
   auto arnoldi_delayed = [&](int s, int Kd) -> int {
     std::vector<std::vector<double>> pend(Rn);
-    std::vector<double> Ah, Bh, cf, c;
+    std::vector<double> Ah, Bh, AB, cf, c;
     std::vector<char> live(Rn);
     bool all = true;
     for (int j = 0; j < Rn; j++) all = all && conv[j];
@@ -1533,10 +1544,17 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       op(V(i), V(i + 1));
       KCHECK(tacs_hip_bdotq(n, Rn, 2, Kd, Cb(0), i + 1, V(0), N, V(i), V(i + 1), (double*)d_dq,
                             dA, dB, S, g_stream));
-      Ah.assign((size_t)Rn * (p + 1), 0.0);
-      Bh.assign((size_t)Rn * (p + 1), 0.0);
-      d2h_rows(Ah.data(), p + 1, dA, S);
-      d2h_rows(Bh.data(), p + 1, dB, S);
+      // one download of both dot lists (dB follows dA in each slice)
+      const int wAB = (int)(dB - dA) + p + 1;
+      AB.assign((size_t)Rn * wAB, 0.0);
+      d2h_rows(AB.data(), wAB, dA, S);
+      Ah.resize((size_t)Rn * (p + 1));
+      Bh.resize((size_t)Rn * (p + 1));
+      for (int j = 0; j < Rn; j++) {
+        const double* src = &AB[(size_t)j * wAB];
+        std::copy(src, src + p + 1, &Ah[(size_t)j * (p + 1)]);
+        std::copy(src + (dB - dA), src + (dB - dA) + p + 1, &Bh[(size_t)j * (p + 1)]);
+      }
       cf.assign((size_t)Rn * (2 * p + 2), 0.0);
       all = true;
       for (int j = 0; j < Rn; j++) {
