@@ -436,6 +436,16 @@ class TACSBatchedGMRES : public TACSObject {
                      // waited for: the next one waits on ev_up before reusing it
   hipEvent_t ev_up;
   int up_pending;
+  // device-side DCGS2 steps (tacs_hip_bdcgs_coeffs): least-squares state,
+  // 4 ints per right-hand side, recycled dimensions and stopping targets;
+  // host-visible flags of the last DCGS_LAG + 1 steps and their events
+  double* d_lsst;
+  int* d_ist;
+  int* d_kdv;
+  double* d_tgtv;
+  int* h_flags;
+  int* d_flags;
+  hipEvent_t ev_step[3];
   void* d_dq;        // scratch of the delayed Gram-Schmidt dots and the products
   // Gram-Schmidt of the recycling (GCRO-DR) Arnoldi steps: 1 (default)
   // delayed classical Gram-Schmidt (DCGS2, two sweeps over the basis per
@@ -461,6 +471,8 @@ class TACSBatchedGMRES : public TACSObject {
   // each), then the delayed Gram-Schmidt dot products (2 x (KD_MAX + m + 1))
   // and update coefficients (2 (KD_MAX + m) + 2)
   static const int KD_MAX = 32;
+  // steps queued ahead of the host's look at a step's stopping flags
+  static const int DCGS_LAG = 2;
   // per right-hand side: four (KD_MAX + m + 1) x KD_MAX matrices
   size_t prodStride() const { return (size_t)4 * (KD_MAX + msub + 1) * KD_MAX; }
   size_t slice() const {

@@ -347,6 +347,22 @@ int tacs_hip_bdotq(size_t n, int R, int nq, int k1, const double* W1, int k2, co
 int tacs_hip_bdcgs_update(size_t n, int R, int k1, const double* W1, int k2, const double* W2,
                           size_t ldw, const double* coef, size_t sC, double* u, double* z,
                           hipStream_t stream);
+/* The per-step small computation of the recycling solver's DCGS2 Arnoldi
+   step on the device (one wavefront per right-hand side; no counterpart in
+   the reference, whose GMRES::solve src/bpmat/KSM.cpp:785 updates its
+   Hessenberg and Givens rotations on the host).  From the dot lists dA, dB
+   (slices of stride S) of step i it finishes least-squares column kd_j + i - 1
+   (Givens update, stopping test against tgt[j]), forms the first-pass
+   coefficients of column kd_j + i and the update coefficients cf = [s | t |
+   1/omega | tau] (stride sC).  ls: per right-hand side least-squares state
+   (stride sLS >= tacs_hip_bdcgs_state_stride(ld): Hb, Hr (ld x (ld - 1)), g,
+   c, rotation cosines, sines, pending column (ld each)); ist: 4 ints per
+   right-hand side (converged, failed, finished columns, columns); flags: a
+   host-visible copy of ist after the step (may be NULL).  ld <= 512. */
+size_t tacs_hip_bdcgs_state_stride(int ld);
+int tacs_hip_bdcgs_coeffs(int R, int i, int Kd, int ld, const int* kd, const double* tgt,
+                          const double* dA, const double* dB, size_t S, double* ls, size_t sLS,
+                          int* ist, double* cf, size_t sC, int* flags, hipStream_t stream);
 int tacs_hip_badd(int R, int k, const double* x, size_t sx, double* y, size_t sy,
                   hipStream_t stream);
 int tacs_hip_bscale(size_t n, int R, const double* s, double* x, hipStream_t stream);

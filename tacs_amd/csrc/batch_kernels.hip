@@ -307,6 +307,136 @@ __global__ __launch_bounds__(256) void bdcgs_update1(size_t n, int R, int k1,
   }
 }
 
+// ---------------------------------------------------------------------------
+// The small per-step computation of a DCGS2 Arnoldi step of the recycling
+// (GCRO-DR) cycle, on the device so that the host does not wait for each
+// step.  One wavefront per right-hand side j.  Inputs: the dot lists of the
+// step, a = Q^T u_i (p = Kd + i entries, then |u_i|^2) and b = Q^T z (then
+// u_i^T z), Q = [C (Kd vectors), V_0..V_{i-1}].  State of j (ls + j*sLS,
+// ld = rows of the least-squares matrix, nc = ld - 1 columns):
+//   Hb, Hr [ld x nc column-major]: the least-squares matrix, plain / rotated
+//   g, c [ld]: its right-hand side, rotated / plain
+//   rc, rs [ld]: rotation q acts on rows (q, q + 1)
+//   pend [ld]: first-pass coefficients of the column being finished
+// and ist[4j..4j+3] = converged, failed, finished columns (jend), columns in
+// the least-squares matrix.  Per step:
+//   omega = sqrt(|u_i|^2 - |s|^2);
+//   i > 0: column kd + i - 1 = pend + s (least-squares row order: the
+//          kd_j recycled rows, then V), subdiagonal omega; Givens update;
+//          converged when |g| <= tgt_j or omega = 0, failed when not finite;
+//   tau = (u_i^T z - s . t) / omega;
+//   pend = ([t; tau] - sum_{a < i} Hb(:, kd + a) s_V[a]) / omega;
+//   cf = [s | t | 1/omega | tau] for the update sweep (zeros once j stopped).
+// flags (host-visible, may be NULL) receives ist after the step.
+// ---------------------------------------------------------------------------
+#define BDC_LDMAX 512
+__global__ __launch_bounds__(64) void bdcgs_coeffs(int i, int Kd, int ld, const int* __restrict__ kdv,
+                                                   const double* __restrict__ tgtv,
+                                                   const double* __restrict__ dA,
+                                                   const double* __restrict__ dB, size_t S,
+                                                   double* __restrict__ ls, size_t sLS,
+                                                   int* __restrict__ ist, double* __restrict__ cf,
+                                                   size_t sC, int* __restrict__ flags) {
+  __shared__ double col[BDC_LDMAX];
+  __shared__ int stop;
+  const int j = blockIdx.x, lane = threadIdx.x;
+  const int p = Kd + i, nc = ld - 1;
+  const double* a = dA + (size_t)j * S;
+  const double* b = dB + (size_t)j * S;
+  double* cj = cf + (size_t)j * sC;
+  int* st = ist + 4 * j;
+  double* Hb = ls + (size_t)j * sLS;
+  double* Hr = Hb + (size_t)ld * nc;
+  double* g = Hr + (size_t)ld * nc;
+  double* cc = g + ld;
+  double* rc = cc + ld;
+  double* rs = rc + ld;
+  double* pend = rs + ld;
+  bool done = st[0] != 0;
+  if (!done) {
+    double ss = 0.0, sb = 0.0;
+    for (int r = lane; r < p; r += 64) {
+      ss += a[r] * a[r];
+      sb += a[r] * b[r];
+    }
+    ss = wave_sum(ss);
+    sb = wave_sum(sb);
+    const double w2 = a[p] - ss;
+    const double om = (w2 > 0.0 && isfinite(w2)) ? sqrt(w2) : 0.0;
+    const int kd = kdv[j];
+    if (i > 0) {
+      const int jj = kd + i - 1;  // the column finished at this step
+      for (int r = lane; r < ld; r += 64) {
+        double v = 0.0;
+        if (r < kd + i)
+          v = pend[r] + (r < kd ? a[r] : a[Kd + (r - kd)]);
+        else if (r == kd + i)
+          v = om;
+        Hb[(size_t)jj * ld + r] = v;
+        col[r] = v;
+      }
+      __syncthreads();
+      if (lane == 0) {
+        for (int q = 0; q < jj; q++) {
+          const double x = col[q], y = col[q + 1];
+          col[q] = rc[q] * x + rs[q] * y;
+          col[q + 1] = -rs[q] * x + rc[q] * y;
+        }
+        const double x = col[jj], y = col[jj + 1];
+        const double h = hypot(x, y);
+        const double cs = h == 0.0 ? 1.0 : x / h, sn = h == 0.0 ? 0.0 : y / h;
+        col[jj] = cs * x + sn * y;
+        col[jj + 1] = -sn * x + cs * y;
+        const double g0 = g[jj], g1 = g[jj + 1];
+        g[jj] = cs * g0 + sn * g1;
+        g[jj + 1] = -sn * g0 + cs * g1;
+        rc[jj] = cs;
+        rs[jj] = sn;
+        st[3] = jj + 1;
+        const double est = fabs(g[jj + 1]);
+        stop = 0;
+        if (!isfinite(est)) {
+          st[0] = 1;
+          st[1] = 1;
+          st[2] = i - 1;
+          stop = 1;
+        } else if (est <= tgtv[j] || om == 0.0) {
+          st[0] = 1;
+          st[2] = i;
+        }
+      }
+      __syncthreads();
+      for (int r = lane; r < ld; r += 64) Hr[(size_t)jj * ld + r] = col[r];
+      done = stop != 0;
+    } else if (lane == 0) {
+      cc[kd] = om;  // u_0 = C s_C + omega v_0
+      g[kd] = om;
+    }
+    if (!done) {
+      const double io = om > 0.0 ? 1.0 / om : 0.0;
+      const double tau = (b[p] - sb) * io;
+      for (int r = lane; r <= kd + i; r += 64) {
+        double v = r < kd + i ? (r < kd ? b[r] : b[Kd + (r - kd)]) : tau;
+        for (int q = 0; q < i; q++)
+          if (r <= kd + q + 1) v -= Hb[(size_t)(kd + q) * ld + r] * a[Kd + q];
+        pend[r] = v * io;
+      }
+      for (int r = lane; r < p; r += 64) {
+        cj[r] = a[r];
+        cj[p + r] = b[r];
+      }
+      if (lane == 0) {
+        cj[2 * p] = io;
+        cj[2 * p + 1] = tau;
+      }
+    }
+  }
+  if (done)
+    for (int r = lane; r < 2 * p + 2; r += 64) cj[r] = 0.0;
+  __syncthreads();
+  if (flags && lane < 4) flags[4 * j + lane] = st[lane];
+}
+
 // y[j*sy + v] += x[j*sx + v], v < k (the second Gram-Schmidt pass into H)
 __global__ void badd(int R, int k, const double* __restrict__ x, size_t sx, double* __restrict__ y,
                      size_t sy) {
@@ -859,6 +989,19 @@ int tacs_hip_bdcgs_update(size_t n, int R, int k1, const double* W1, int k2, con
     hipLaunchKernelGGL(bdcgs_update1, dim3(bgrid(n * R)), dim3(256), 0, stream, n, R, k1, W1, k2,
                        W2, ldw, coef, sC, u, z);
   }
+  return (int)hipGetLastError();
+}
+
+size_t tacs_hip_bdcgs_state_stride(int ld) { return (size_t)2 * ld * (ld - 1) + 5 * (size_t)ld; }
+
+int tacs_hip_bdcgs_coeffs(int R, int i, int Kd, int ld, const int* kd, const double* tgt,
+                          const double* dA, const double* dB, size_t S, double* ls, size_t sLS,
+                          int* ist, double* cf, size_t sC, int* flags, hipStream_t stream) {
+  if (R < 1 || R > 64 || i < 0 || Kd < 0 || ld > BDC_LDMAX || Kd + i + 2 > ld ||
+      sLS < tacs_hip_bdcgs_state_stride(ld))
+    return TACS_HIP_ERR_ARG;
+  hipLaunchKernelGGL(bdcgs_coeffs, dim3(R), dim3(64), 0, stream, i, Kd, ld, kd, tgt, dA, dB, S, ls,
+                     sLS, ist, cf, sC, flags);
   return (int)hipGetLastError();
 }
 

@@ -1093,7 +1093,8 @@ TACSBatchedGMRES::TACSBatchedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int
     : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart), R(nrhs), prec(_prec), cycles(0),
       kdefl(0), kVersion(-1), d_U(NULL), d_C(NULL), ndefl(0), defVersion(-1), defR(0),
       d_Ub(NULL), d_Cb(NULL), d_Unb(NULL), d_Cnb(NULL), d_prod(NULL), h_stage(NULL), h_up(NULL),
-      ev_up(NULL), up_pending(0), d_dq(NULL),
+      ev_up(NULL), up_pending(0), d_lsst(NULL), d_ist(NULL), d_kdv(NULL), d_tgtv(NULL),
+      h_flags(NULL), d_flags(NULL), ev_step{NULL, NULL, NULL}, d_dq(NULL),
       delayed(1), mat(m), pc(p) {
   mat->incref();
   pc->incref();
@@ -1120,6 +1121,11 @@ TACSBatchedGMRES::~TACSBatchedGMRES() {
   if (h_stage) HCHECK(hipHostFree(h_stage));
   if (h_up) HCHECK(hipHostFree(h_up));
   if (ev_up) HCHECK(hipEventDestroy(ev_up));
+  void* st[] = {d_lsst, d_ist, d_kdv, d_tgtv};
+  for (void* q : st) dfree(q);
+  if (h_flags) HCHECK(hipHostFree(h_flags));
+  for (hipEvent_t e : ev_step)
+    if (e) HCHECK(hipEventDestroy(e));
   mat->decref();
   pc->decref();
 }
@@ -1171,6 +1177,12 @@ int TACSBatchedGMRES::setDeflation(int k) {
   if (h_up) HCHECK(hipHostFree(h_up));
   h_stage = h_up = NULL;
   up_pending = 0;
+  void* st[] = {d_lsst, d_ist, d_kdv, d_tgtv};
+  for (void* q : st) dfree(q);
+  d_lsst = d_tgtv = NULL;
+  d_ist = d_kdv = NULL;
+  if (h_flags) HCHECK(hipHostFree(h_flags));
+  h_flags = d_flags = NULL;
   ndefl = k;
   defVersion = -1;
   defR = 0;
@@ -1186,6 +1198,16 @@ int TACSBatchedGMRES::setDeflation(int k) {
     HCHECK(hipHostMalloc((void**)&h_stage, (size_t)R * prodStride() * sizeof(double), 0));
     HCHECK(hipHostMalloc((void**)&h_up, (size_t)R * prodStride() * sizeof(double), 0));
     if (!ev_up) HCHECK(hipEventCreateWithFlags(&ev_up, hipEventDisableTiming));
+    const int ld = msub + 2;
+    d_lsst = dalloc<double>((size_t)R * tacs_hip_bdcgs_state_stride(ld));
+    d_ist = dalloc<int>((size_t)4 * R);
+    d_kdv = dalloc<int>(R);
+    d_tgtv = dalloc<double>(R);
+    HCHECK(hipHostMalloc((void**)&h_flags, (size_t)(DCGS_LAG + 1) * 4 * R * sizeof(int), 0));
+    memset(h_flags, 0, (size_t)(DCGS_LAG + 1) * 4 * R * sizeof(int));
+    HCHECK(hipHostGetDevicePointer((void**)&d_flags, h_flags, 0));
+    for (hipEvent_t& e : ev_step)
+      if (!e) HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HCHECK(hipMalloc(&d_dq, std::max(tacs_hip_bdotq_scratch_bytes(R),
                                      tacs_hip_btsgemm_scratch_bytes(R))));
   }
@@ -1531,92 +1553,96 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
   // sweep then forms v_i = (u_i - Q_i s) / omega and
   // u_{i+1} = (z - Q_i t - v_i tau) / omega.  The stopping test of column
   // i - 1 is made at step i.  Returns the number of finished columns.
+  static_assert(DCGS_LAG + 1 == sizeof(ev_step) / sizeof(ev_step[0]), "one event per queued step");
   auto arnoldi_delayed = [&](int s, int Kd) -> int {
     std::vector<std::vector<double>> pend(Rn);
-    std::vector<double> Ah, Bh, AB, cf, c;
-    std::vector<char> live(Rn);
+    std::vector<double> Ah, cf;
     bool all = true;
     for (int j = 0; j < Rn; j++) all = all && conv[j];
+    if (all) return 0;
+    // the least-squares state of every right-hand side goes to the device,
+    // which runs the per-step small computation (tacs_hip_bdcgs_coeffs), so
+    // the host only queues kernels; it reads each step's stopping flags
+    // DCGS_LAG steps later and stops queueing once every system stopped
+    const int ld = m + 2;
+    const size_t sLS = tacs_hip_bdcgs_state_stride(ld);
+    const size_t HH = (size_t)ld * (ld - 1);
+    std::vector<double> hls((size_t)Rn * sLS, 0.0);
+    std::vector<int> hist((size_t)4 * Rn, 0), hkd(Rn);
+    for (int j = 0; j < Rn; j++) {
+      const HostLS& L = ls[j];
+      double* q = &hls[(size_t)j * sLS];
+      std::copy(L.Hb.begin(), L.Hb.end(), q);
+      std::copy(L.Hr.begin(), L.Hr.end(), q + HH);
+      double* g = q + 2 * HH;
+      std::copy(L.g.begin(), L.g.end(), g);
+      std::copy(L.c.begin(), L.c.end(), g + ld);
+      std::copy(L.rc.begin(), L.rc.end(), g + 2 * ld);
+      std::copy(L.rs.begin(), L.rs.end(), g + 3 * ld);
+      hist[4 * j] = conv[j];
+      hist[4 * j + 1] = fail[j];
+      hist[4 * j + 2] = jend[j];
+      hist[4 * j + 3] = (int)L.rc.size();
+      hkd[j] = kdj[j];
+    }
+    HCHECK(hipMemcpyAsync(d_lsst, hls.data(), hls.size() * sizeof(double), hipMemcpyHostToDevice,
+                          g_stream));
+    HCHECK(hipMemcpyAsync(d_ist, hist.data(), hist.size() * sizeof(int), hipMemcpyHostToDevice,
+                          g_stream));
+    HCHECK(hipMemcpyAsync(d_kdv, hkd.data(), Rn * sizeof(int), hipMemcpyHostToDevice, g_stream));
+    HCHECK(hipMemcpyAsync(d_tgtv, tgt.data(), Rn * sizeof(double), hipMemcpyHostToDevice,
+                          g_stream));
     int i = 0;
-    for (; i < s && !all; i++) {
-      const int p = Kd + i;
-      for (int j = 0; j < Rn; j++) live[j] = !conv[j];
+    for (; i < s; i++) {
+      if (i >= DCGS_LAG) {
+        const int q = (i - DCGS_LAG) % (DCGS_LAG + 1);
+        HCHECK(hipEventSynchronize(ev_step[q]));
+        const volatile int* f = h_flags + (size_t)q * 4 * R;
+        bool alld = true;
+        for (int j = 0; j < Rn; j++) alld = alld && f[4 * j] != 0;
+        if (alld) break;
+      }
       op(V(i), V(i + 1));
       KCHECK(tacs_hip_bdotq(n, Rn, 2, Kd, Cb(0), i + 1, V(0), N, V(i), V(i + 1), (double*)d_dq,
                             dA, dB, S, g_stream));
-      // one download of both dot lists (dB follows dA in each slice)
-      const int wAB = (int)(dB - dA) + p + 1;
-      AB.assign((size_t)Rn * wAB, 0.0);
-      d2h_rows(AB.data(), wAB, dA, S);
-      Ah.resize((size_t)Rn * (p + 1));
-      Bh.resize((size_t)Rn * (p + 1));
-      for (int j = 0; j < Rn; j++) {
-        const double* src = &AB[(size_t)j * wAB];
-        std::copy(src, src + p + 1, &Ah[(size_t)j * (p + 1)]);
-        std::copy(src + (dB - dA), src + (dB - dA) + p + 1, &Bh[(size_t)j * (p + 1)]);
-      }
-      cf.assign((size_t)Rn * (2 * p + 2), 0.0);
-      all = true;
-      for (int j = 0; j < Rn; j++) {
-        if (!live[j]) continue;
-        const int kd = kdj[j];
-        const double* a = &Ah[(size_t)j * (p + 1)];
-        const double* b = &Bh[(size_t)j * (p + 1)];
-        double ss = 0.0, sb = 0.0;
-        for (int r = 0; r < p; r++) {
-          ss += a[r] * a[r];
-          sb += a[r] * b[r];
-        }
-        const double w2 = a[p] - ss;
-        const double om = (w2 > 0.0 && std::isfinite(w2)) ? sqrt(w2) : 0.0;
-        // least-squares row order: C (kd rows), V_0..V_{i-1}
-        auto sL = [&](int r) { return r < kd ? a[r] : a[Kd + (r - kd)]; };
-        auto tL = [&](int r) { return r < kd ? b[r] : b[Kd + (r - kd)]; };
-        HostLS& L = ls[j];
-        if (i > 0) {
-          std::vector<double>& col = pend[j];
-          col.resize((size_t)kd + i + 1);
-          for (int r = 0; r < kd + i; r++) col[r] += sL(r);
-          col[kd + i] = om;
-          const double st = L.add_column(kd + i - 1, col.data());
-          if (!std::isfinite(st)) {
-            conv[j] = 1;
-            fail[j] = 1;
-            jend[j] = i - 1;
-            continue;
-          }
-          if (st <= tgt[j] || om == 0.0) {
-            conv[j] = 1;
-            jend[j] = i;
-          }
-        } else {
-          L.c[kd] = L.g[kd] = om;  // u_0 = C s_C + omega v_0
-        }
-        const double io = om > 0.0 ? 1.0 / om : 0.0;
-        const double tau = (b[p] - sb) * io;
-        c.assign((size_t)kd + i + 1, 0.0);
-        for (int r = 0; r < kd + i; r++) c[r] = tL(r);
-        c[kd + i] = tau;
-        for (int a2 = 0; a2 < i; a2++) {
-          const double sv = a[Kd + a2];
-          const double* colf = &L.Hb[(size_t)(kd + a2) * L.ld];
-          for (int r = 0; r <= kd + a2 + 1; r++) c[r] -= colf[r] * sv;
-        }
-        for (double& x : c) x *= io;
-        pend[j] = c;
-        double* cj = &cf[(size_t)j * (2 * p + 2)];
-        for (int r = 0; r < p; r++) {
-          cj[r] = a[r];
-          cj[p + r] = b[r];
-        }
-        cj[2 * p] = io;
-        cj[2 * p + 1] = tau;
-        if (!conv[j]) all = false;
-      }
-      h2d_rows(dcf, S, cf.data(), 2 * p + 2);
+      KCHECK(tacs_hip_bdcgs_coeffs(Rn, i, Kd, ld, d_kdv, d_tgtv, dA, dB, S, d_lsst, sLS, d_ist,
+                                   dcf, S, d_flags + (size_t)(i % (DCGS_LAG + 1)) * 4 * R,
+                                   g_stream));
+      HCHECK(hipEventRecord(ev_step[i % (DCGS_LAG + 1)], g_stream));
       KCHECK(tacs_hip_bdcgs_update(n, Rn, Kd, Cb(0), i, V(0), N, dcf, S, V(i), V(i + 1), g_stream));
     }
-    if (all) return i;
+    // the state back to the host
+    HCHECK(hipMemcpyAsync(hls.data(), d_lsst, hls.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          g_stream));
+    HCHECK(hipMemcpyAsync(hist.data(), d_ist, hist.size() * sizeof(int), hipMemcpyDeviceToHost,
+                          g_stream));
+    HCHECK(hipStreamSynchronize(g_stream));
+    all = true;
+    int last = -1;  // the step at which the last system stopped
+    for (int j = 0; j < Rn; j++) {
+      HostLS& L = ls[j];
+      const double* q = &hls[(size_t)j * sLS];
+      std::copy(q, q + HH, L.Hb.begin());
+      std::copy(q + HH, q + 2 * HH, L.Hr.begin());
+      const double* g = q + 2 * HH;
+      std::copy(g, g + ld, L.g.begin());
+      std::copy(g + ld, g + 2 * ld, L.c.begin());
+      const int ncol = hist[4 * j + 3];
+      L.rp.resize(ncol);
+      for (int r = 0; r < ncol; r++) L.rp[r] = r;
+      L.rc.assign(g + 2 * ld, g + 2 * ld + ncol);
+      L.rs.assign(g + 3 * ld, g + 3 * ld + ncol);
+      pend[j].assign(g + 4 * ld, g + 5 * ld);
+      conv[j] = (char)hist[4 * j];
+      fail[j] = (char)hist[4 * j + 1];
+      if (conv[j]) {
+        jend[j] = hist[4 * j + 2];
+        last = std::max(last, fail[j] ? jend[j] + 1 : jend[j]);
+      } else {
+        all = false;
+      }
+    }
+    if (all) return last + 1;
     // m - Kd steps done: finish column s - 1 and v_s
     const int p = Kd + s;
     KCHECK(tacs_hip_bdotq(n, Rn, 1, Kd, Cb(0), s + 1, V(0), N, V(s), NULL, (double*)d_dq, dA, NULL,
