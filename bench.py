@@ -103,6 +103,9 @@ def time_spmv(L, ev, p, reps=50):
     return res
 
 
+MONITOR = {"file": None, "freq": 1000}
+
+
 def make_solver(p, solver, m, rtol, deflation=0):
     from tacs_amd import TACS
     pc = TACS.Pc(p.mat)
@@ -111,6 +114,11 @@ def make_solver(p, solver, m, rtol, deflation=0):
     else:
         ksm = TACS.KSM(p.mat, pc, m, 100000)
     ksm.setTolerances(rtol, 1e-30)
+    if MONITOR["file"]:
+        # residual lines (every cycle start of the recycling solver, every
+        # restart and freq-th step of GMRES) to a file, for long solves
+        ksm.setMonitor(descript=f"{solver}", freq=MONITOR["freq"],
+                       filename=f"{MONITOR['file']}.{solver}")
     return pc, ksm
 
 
@@ -346,6 +354,96 @@ def batched_step(p, pc, ksm, bs, xs):
     t["step_s"] = time.perf_counter() - t0
     return dict(t=t, flag=flag, its=[int(v) for v in it], compliance=vals,
                 grads=[g.getArray().copy() for g in grads])
+
+
+def simp_main(args, L, ev):
+    """SIMP compliance on the n^3 hex cube (96^3 by default): one density DV
+    per element, x ~ U[0.1, 1] (PCG64 seed 0), t = Emin + x^p (E0 - Emin)
+    (p = 3, E0 = 1, Emin = 1e-9, nu = 0.3), x = 0 clamped, unit traction in z
+    on the centred 8 x 8 element patch of the x = 1 face.  A step is the
+    forward (assembly, recycling mixed solve, compliance) and backward
+    (adjoint solve, addDVSens - addAdjointResProducts, chain rule) of the
+    torch.autograd.Function.  The plain FP64 GMRES(30) path runs the same
+    step once for the gradient comparison.  --fd K adds central differences
+    (h = 1e-6, both sides solved to 1e-14) on K seeded DVs.  Prints its own
+    JSON line."""
+    import torch
+    from tacs_amd.autograd import SolidComplianceProblem, simp_compliance
+    from tacs_amd.problems import SolidProblem
+    n = args.n if args.n != 128 else 96
+    t0 = time.perf_counter()
+    prob = SolidProblem(n, E=1.0, nu=0.3, traction=(0.0, 0.0, 1.0), dv="element",
+                        traction_patch=8)
+    rng = np.random.default_rng(0)
+    x0 = rng.uniform(0.1, 1.0, prob.mesh.nelem)
+    log(f"setup {time.perf_counter() - t0:.1f}s: {prob.mesh.nelem} elements, "
+        f"{3 * prob.mesh.nnodes} DOF")
+    sp = SolidComplianceProblem(prob, p=3.0, E0=1.0, Emin=1e-9, rtol=args.rtol, m=args.m,
+                                deflation=args.deflation)
+    if MONITOR["file"]:
+        sp.ksm.setMonitor(descript="simp", freq=MONITOR["freq"], filename=MONITOR["file"] + ".simp")
+
+    def step(spx):
+        x = torch.tensor(x0, dtype=torch.float64, device="cuda", requires_grad=True)
+        t = time.perf_counter()
+        c = simp_compliance(x, spx)
+        c.backward()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, c.item(), x.grad.detach().cpu().numpy()
+
+    for w in range(args.warmup):
+        dt, c, g = step(sp)
+        log(f"warmup {w}: {dt:.3f}s its={sp.iters[-2:]}")
+    times = []
+    for k in range(args.steps):
+        dt, c, g = step(sp)
+        times.append(dt)
+        log(f"step {k}: {dt:.3f}s its={sp.iters[-2:]}")
+    its = sp.iters[-2:]
+    out = {
+        "metric": "SIMP compliance gradient (forward + adjoint) per second",
+        "value": 1.0 / (sum(times) / len(times)), "unit": "gradients/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * sum(times) / len(times),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 assembly + f32/f64 mixed solve",
+        "data": "synthetic structured mesh, seeded densities (no external data)",
+        "config": {"workload": f"SIMP hex cube {n}^3, x ~ U[0.1,1] seed 0, p=3 E0=1 Emin=1e-9 "
+                               f"nu=0.3, x=0 clamped, unit z traction on the 8x8 patch at x=1; "
+                               f"mixed GMRES({args.m}) recycling {args.deflation} to "
+                               f"{args.rtol:g}; plain path FP64 GMRES({args.m_fp64})+MGS",
+                   "elements": prob.mesh.nelem, "dof": 3 * prob.mesh.nnodes},
+        "compliance": c, "iterations_mixed": its,
+    }
+    if not args.no_fp64:
+        sp64 = SolidComplianceProblem(prob, p=3.0, E0=1.0, Emin=1e-9, rtol=args.rtol,
+                                      m=args.m_fp64, mixed=False)
+        dt64, c64, g64 = step(sp64)
+        log(f"fp64 step: {dt64:.3f}s its={sp64.iters[-2:]}")
+        out.update(step_s_fp64=dt64, iterations_fp64=sp64.iters[-2:],
+                   speedup_mixed_vs_fp64=dt64 / (sum(times) / len(times)),
+                   grad_rel_err_vs_fp64=float(np.abs(g - g64).max() / np.abs(g64).max()),
+                   compliance_rel_err_vs_fp64=abs(c - c64) / abs(c64))
+        del sp64
+    if args.fd:
+        sp.ksm.setTolerances(1e-14, 1e-30)
+        idx = np.random.default_rng(1).choice(prob.mesh.nelem, args.fd, replace=False)
+        h = 1e-6
+        fd = []
+        for q, e in enumerate(idx):
+            vals = []
+            for sgn in (1.0, -1.0):
+                xp = x0.copy()
+                xp[e] += sgn * h
+                vals.append(sp.forward(torch.tensor(xp, dtype=torch.float64, device="cuda")))
+            fd.append((vals[0] - vals[1]) / (2 * h))
+            log(f"fd {q}: element {e} fd {fd[-1]:.6e} adjoint {g[e]:.6e}")
+        fd = np.array(fd)
+        ge = g[idx]
+        out["fd_check"] = {
+            "h": h, "dvs": [int(e) for e in idx], "fd": fd.tolist(), "adjoint": ge.tolist(),
+            "max_rel_err": float(np.max(np.abs(fd - ge) / np.abs(ge))),
+            "max_err_over_max_grad": float(np.max(np.abs(fd - ge)) / np.abs(ge).max())}
+    print(json.dumps(out))
 
 
 def batched_main(args, L, ev):
@@ -645,13 +743,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="assemble and time the SpMV kernels only (for profiler runs)")
-    ap.add_argument("--workload", default="hex", choices=["hex", "shell", "batched", "multi"],
+    ap.add_argument("--workload", default="hex",
+                    choices=["hex", "shell", "batched", "multi", "simp"],
                     help="hex: the 128^3 headline line; shell: the Quad4 plate; batched: the "
                          "hex27 cube with a block of load cases (own JSON lines)")
     ap.add_argument("--nrhs", type=int, default=16, help="load cases of the batched workload")
     ap.add_argument("--ncubes", type=int, default=32, help="cubes of the multi workload")
     ap.add_argument("--seed", type=int, default=11, help="seed of the multi workload's sizes")
+    ap.add_argument("--fd", type=int, default=0,
+                    help="simp workload: central-difference check on this many DVs")
+    ap.add_argument("--monitor", default=None,
+                    help="write the solvers' residual monitor lines to FILE.<solver>")
     args = ap.parse_args()
+    MONITOR["file"] = args.monitor
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
               "described in DESIGN.md and not built", file=sys.stderr)
@@ -669,6 +773,9 @@ def main():
         return
     if args.workload == "multi":
         multi_main(args, L, ev)
+        return
+    if args.workload == "simp":
+        simp_main(args, L, ev)
         return
     t0 = time.perf_counter()
     p = SolidProblem(args.n, dv="element")

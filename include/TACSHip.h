@@ -16,7 +16,9 @@
 
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <stdio.h>
 
+#include <string>
 #include <vector>
 
 #include "tacs_hip.h"
@@ -321,14 +323,53 @@ class TACSBlockJacobiPc : public TACSPc {
 };
 
 // src/bpmat/KSM.h:200 (TACSKsm), :266 (GMRES)
+// Residual monitors (KSMPrint src/bpmat/KSM.h:229, KSMPrintStdout :293,
+// KSMPrintFile :310): printResidual(iter, res) prints "descript[iter]: res"
+// when iter == 0 or iter is a multiple of freq, on rank 0; the file variant
+// flushes each line.
+class KSMPrint : public TACSObject {
+ public:
+  virtual ~KSMPrint() {}
+  virtual void printResidual(int iter, TacsScalar res) = 0;
+  virtual void print(const char* cstr) = 0;
+};
+class KSMPrintStdout : public KSMPrint {
+ public:
+  KSMPrintStdout(const char* descript, int rank, int freq);
+  void printResidual(int iter, TacsScalar res);
+  void print(const char* cstr);
+  std::string descript;
+  int rank, freq;
+};
+class KSMPrintFile : public KSMPrint {
+ public:
+  KSMPrintFile(const char* filename, const char* descript, int rank, int freq);
+  ~KSMPrintFile();
+  void printResidual(int iter, TacsScalar res);
+  void print(const char* cstr);
+  FILE* fp;
+  std::string descript;
+  int rank, freq;
+};
+
 class TACSKsm : public TACSObject {
  public:
+  virtual ~TACSKsm() {
+    if (monitor) monitor->decref();
+  }
   virtual int solve(TACSVec* b, TACSVec* x, int zero_guess = 1) = 0;
   virtual void setTolerances(double rtol, double atol) = 0;
+  // GMRES::setMonitor src/bpmat/KSM.cpp:739 (the monitor is reference counted)
+  virtual void setMonitor(KSMPrint* m) {
+    if (m) m->incref();
+    if (monitor) monitor->decref();
+    monitor = m;
+  }
   int getIterCount() { return iterCount; }
   TacsScalar getResidualNorm() { return resNorm; }
   int iterCount = 0;
   TacsScalar resNorm = 0.0;
+  KSMPrint* monitor = NULL;
 };
 
 class GMRES : public TACSKsm {
@@ -402,6 +443,14 @@ class TACSBatchedGMRES : public TACSObject {
   ~TACSBatchedGMRES();
   // x[j] = A^{-1} b[j], j < nrhs <= R; returns 1 when every system converged
   int solve(int nrhs, TACSBVec** b, TACSBVec** x, int zero_guess = 1);
+  // residual monitor: at the start of each cycle, the true residual norm of
+  // every system still iterating, printResidual(0, |r_j|)
+  void setMonitor(KSMPrint* m) {
+    if (m) m->incref();
+    if (monitor) monitor->decref();
+    monitor = m;
+  }
+  KSMPrint* monitor = NULL;
   // the same on interleaved device block vectors B[i*nrhs + j], X[i*nrhs + j]
   int solveBlock(int nrhs, const double* B, double* X, int zero_guess);
   void setTolerances(double rtol, double atol);

@@ -530,6 +530,16 @@ tacs_obj tacs_block_jacobi_pc_create(tacs_obj mat);
 int tacs_pc_factor(tacs_obj pc);
 int tacs_pc_apply(tacs_obj pc, tacs_obj x, tacs_obj y);
 tacs_obj tacs_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int is_flexible);
+/* Residual monitors (KSMPrintStdout src/bpmat/KSM.h:293, KSMPrintFile :310;
+   GMRES::setMonitor KSM.cpp:739, TACS.pyx KSM.setMonitor :1326): GMRES and
+   the mixed GMRES print residual norms per step (0 at each restart); the
+   recycling and batched solvers print the true residual norm of each
+   system at the start of every cycle.  set_monitor takes any of the three
+   solvers; monitor NULL removes it. */
+tacs_obj tacs_ksm_print_stdout_create(const char* descript, int rank, int freq);
+tacs_obj tacs_ksm_print_file_create(const char* filename, const char* descript, int rank,
+                                    int freq);
+int tacs_ksm_set_monitor(tacs_obj ksm, tacs_obj monitor);
 int tacs_gmres_set_ortho_type(tacs_obj ksm, int ortho);
 /* Mixed-precision restarted GMRES(m) (FP32 Arnoldi cycles, FP64 restart
    residual and solution); no counterpart in the reference. */

@@ -148,6 +148,20 @@ class KSM(TACSHandle):
     def setOrthoType(self, ortho):
         check(lib().tacs_gmres_set_ortho_type(self.ptr, ortho), "KSM.setOrthoType")
 
+    def setMonitor(self, comm=None, descript="GMRES", freq=10, filename=None):
+        """Residual monitor (KSM.setMonitor tacs/TACS.pyx:1326, KSMPrintStdout
+        src/bpmat/KSM.cpp:261): prints "descript[iter]: residual" at every
+        restart and every freq-th step, on rank 0 of comm (None: rank 0).
+        filename: write the lines to that file instead (KSMPrintFile)."""
+        rank = comm.rank if comm is not None else 0
+        d = descript.encode()
+        if filename is None:
+            mon = TACSHandle(lib().tacs_ksm_print_stdout_create(d, rank, int(freq)))
+        else:
+            mon = TACSHandle(lib().tacs_ksm_print_file_create(str(filename).encode(), d, rank,
+                                                              int(freq)))
+        check(lib().tacs_ksm_set_monitor(self.ptr, mon.ptr), "setMonitor")
+
     def getIterCount(self):
         return lib().tacs_ksm_get_iter_count(self.ptr)
 
@@ -203,6 +217,21 @@ class BatchedKSM(TACSHandle):
 
     def setTolerances(self, rtol, atol):
         lib().tacs_batched_gmres_set_tolerances(self.ptr, rtol, atol)
+
+    def setMonitor(self, comm=None, descript="GMRES", freq=10, filename=None):
+        """Residual monitor (KSM.setMonitor tacs/TACS.pyx:1326; here the true residual of each
+        system at the start of every cycle, KSMPrintStdout
+        src/bpmat/KSM.cpp:261): prints "descript[iter]: residual" at every
+        restart and every freq-th step, on rank 0 of comm (None: rank 0).
+        filename: write the lines to that file instead (KSMPrintFile)."""
+        rank = comm.rank if comm is not None else 0
+        d = descript.encode()
+        if filename is None:
+            mon = TACSHandle(lib().tacs_ksm_print_stdout_create(d, rank, int(freq)))
+        else:
+            mon = TACSHandle(lib().tacs_ksm_print_file_create(str(filename).encode(), d, rank,
+                                                              int(freq)))
+        check(lib().tacs_ksm_set_monitor(self.ptr, mon.ptr), "setMonitor")
 
     def setDelayedReorthogonalization(self, flag=True):
         """Gram-Schmidt of the recycling Arnoldi steps: delayed classical

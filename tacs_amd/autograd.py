@@ -24,7 +24,8 @@ class SolidComplianceProblem:
     """Holds a SolidProblem (one DV per element, tNum = element index) and its
     solver for repeated forward / backward passes."""
 
-    def __init__(self, problem, p=3.0, E0=1.0, Emin=1e-9, rtol=1e-8, m=120, mixed=True):
+    def __init__(self, problem, p=3.0, E0=1.0, Emin=1e-9, rtol=1e-8, m=120, mixed=True,
+                 deflation=0):
         self.prob = problem
         self.p, self.E0, self.Emin, self.rtol = p, E0, Emin, rtol
         a = problem.assembler
@@ -32,7 +33,9 @@ class SolidComplianceProblem:
         self.dfdt = a.createDesignVec()
         self.pc = TACS.Pc(problem.mat)
         if mixed:
-            self.ksm = TACS.MixedKSM(problem.mat, self.pc, m, 100000)
+            # deflation > 0: the recycling solver (its space carries over to
+            # the adjoint solve with the same matrix)
+            self.ksm = TACS.MixedKSM(problem.mat, self.pc, m, 100000, deflation=deflation)
         else:
             self.ksm = TACS.KSM(problem.mat, self.pc, min(m, 512), 100000)
         self.ksm.setTolerances(rtol, 1e-30)

@@ -418,6 +418,34 @@ int tacs_mat_device_arrays(tacs_obj m, const int** rowp, const int** cols, const
   return 0;
 }
 
+tacs_obj tacs_ksm_print_stdout_create(const char* descript, int rank, int freq) {
+  return (TACSObject*)new KSMPrintStdout(descript, rank, freq);
+}
+tacs_obj tacs_ksm_print_file_create(const char* filename, const char* descript, int rank,
+                                    int freq) {
+  clear_error();
+  KSMPrintFile* f = new KSMPrintFile(filename, descript, rank, freq);
+  if (rank == 0 && !f->fp) {
+    delete f;
+    return NULL;
+  }
+  return (TACSObject*)f;
+}
+int tacs_ksm_set_monitor(tacs_obj ksm, tacs_obj monitor) {
+  KSMPrint* m = NULL;
+  if (monitor) {
+    m = OBJ(KSMPrint, monitor);
+    if (!m) return tacs_set_error("%s: expected a KSMPrint handle", __func__);
+  }
+  if (TACSKsm* k = OBJ(TACSKsm, ksm)) {
+    k->setMonitor(m);
+    return 0;
+  }
+  REQUIRE(TACSBatchedGMRES, b, ksm);
+  b->setMonitor(m);
+  return 0;
+}
+
 tacs_obj tacs_block_jacobi_pc_create(tacs_obj mat) {
   REQUIRE_NULL(TACSParallelMat, A, mat);
   return (TACSObject*)new TACSBlockJacobiPc(A);

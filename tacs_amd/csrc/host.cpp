@@ -573,6 +573,44 @@ struct GmresScal {
 };
 enum { MISC_ST = 0, MISC_NRM = 1, MISC_INV = 2, MISC_NINV = 3, MISC_ONE = 4 };
 
+// KSMPrintStdout / KSMPrintFile (src/bpmat/KSM.cpp:239, :288)
+KSMPrintStdout::KSMPrintStdout(const char* d, int r, int f)
+    : descript(d ? d : ""), rank(r), freq(f < 1 ? 1 : f) {}
+void KSMPrintStdout::printResidual(int iter, TacsScalar res) {
+  if ((iter == 0 || iter % freq == 0) && rank == 0) {
+    printf("%s[%3d]: %15.8e\n", descript.c_str(), iter, res);
+    fflush(stdout);
+  }
+}
+void KSMPrintStdout::print(const char* cstr) {
+  if (rank == 0) {
+    printf("%s", cstr);
+    fflush(stdout);
+  }
+}
+KSMPrintFile::KSMPrintFile(const char* filename, const char* d, int r, int f)
+    : fp(NULL), descript(d ? d : ""), rank(r), freq(f < 1 ? 1 : f) {
+  if (rank == 0 && filename) {
+    fp = fopen(filename, "w");
+    if (!fp) tacs_set_error("KSMPrintFile: cannot open %s", filename);
+  }
+}
+KSMPrintFile::~KSMPrintFile() {
+  if (fp) fclose(fp);
+}
+void KSMPrintFile::printResidual(int iter, TacsScalar res) {
+  if ((iter == 0 || iter % freq == 0) && fp) {
+    fprintf(fp, "%s[%3d]: %15.8e\n", descript.c_str(), iter, res);
+    fflush(fp);
+  }
+}
+void KSMPrintFile::print(const char* cstr) {
+  if (fp) {
+    fprintf(fp, "%s", cstr);
+    fflush(fp);
+  }
+}
+
 GMRES::GMRES(TACSMat* m, TACSPc* p, int _m, int _nrestart, int isFlexible)
     : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart >= 0 ? _nrestart : 0),
       ortho(MODIFIED_GRAM_SCHMIDT) {
@@ -668,6 +706,7 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
       KCHECK(tacs_hip_scale(64, n, W(0), misc + MISC_NINV, 0.0, 0, g_stream));
     }
     double r0 = pin_read(h_pin, res);
+    if (monitor) monitor->printResidual(0, fabs(r0));
     if (count == 0) {
       rhs_norm = r0;
       resNorm = rhs_norm;
@@ -702,6 +741,7 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
       KCHECK(tacs_hip_givens(i, msub, H, Qc, Qs, res, misc + MISC_ST, g_stream));
       niters++;
       resNorm = pin_read(h_pin, misc + MISC_ST);
+      if (monitor) monitor->printResidual(i + 1, resNorm);
       if (resNorm < atol || resNorm < rtol * rhs_norm || !std::isfinite(resNorm)) {
         solve_flag = std::isfinite(resNorm) ? 1 : 0;
         break;
@@ -820,6 +860,7 @@ int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
     KCHECK(tacs_hip_dot(64, n, NULL, NULL, d_r, d_r, d_red, misc + MISC_NRM, 1, misc + MISC_INV,
                         g_stream));
     double rn = pin_read(h_pin, misc + MISC_NRM);
+    if (monitor) monitor->printResidual(0, rn);
     if (count == 0) r0 = rn;
     resNorm = rn;
     if (rn <= rtol * r0 || rn < atol) {
@@ -842,6 +883,7 @@ int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
       KCHECK(tacs_hip_givens(i, msub, H, Qc, Qs, res, misc + MISC_ST, g_stream));
       niters++;
       double st = pin_read(h_pin, misc + MISC_ST);
+      if (monitor) monitor->printResidual(i + 1, st * rn);
       if (st <= target || !std::isfinite(st)) break;
     }
     iterCount += niters;
@@ -1064,6 +1106,7 @@ int TACSMixedGMRES::solveDeflated(TACSVec* tb, TACSVec* tx, int zero_guess) {
   rec->rtol = rtol;
   rec->atol = atol;
   rec->nrestart = nrestart;
+  rec->setMonitor(monitor);
   const double* bd = b->devRead();
   double* xd = zero_guess ? x->devWrite() : (x->devRead(), x->devWrite());
   int flag = rec->solveBlock(1, bd, xd, zero_guess);
@@ -1126,6 +1169,7 @@ TACSBatchedGMRES::~TACSBatchedGMRES() {
   if (h_flags) HCHECK(hipHostFree(h_flags));
   for (hipEvent_t e : ev_step)
     if (e) HCHECK(hipEventDestroy(e));
+  if (monitor) monitor->decref();
   mat->decref();
   pc->decref();
 }
@@ -1312,6 +1356,7 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
     for (int j = 0; j < Rn; j++) {
       if (!active[j]) continue;
       double r = h_read[j];
+      if (monitor) monitor->printResidual(0, r);
       if (count == 0) r0[j] = r;
       rn[j] = r;
       if (!std::isfinite(r) || fail[j]) {
@@ -1693,6 +1738,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
     for (int j = 0; j < Rn; j++) {
       if (!active[j]) continue;
       double r = h_pin[256 + j];
+      if (monitor) monitor->printResidual(0, r);
       if (count == 0) r0[j] = r;
       rn[j] = r;
       if (!std::isfinite(r) || fail[j]) {

@@ -292,3 +292,45 @@ def test_solve_functions_and_adjoint(solver):
     # closed form: dC/dt_e = -u_e^T K0_e u_e  (< 0)
     closed = -O.compliance_dvsens(2, p.mesh.conn, p.mesh.X, u_ref, C)
     assert np.abs(grad - closed).max() <= 1e-6 * np.abs(closed).max()
+
+
+def test_solver_residual_monitors(tmp_path):
+    """KSM.setMonitor (KSMPrintFile): GMRES prints |r| at every restart and
+    every freq-th step (GMRES::solve src/bpmat/KSM.cpp:815, :893); the
+    recycling solver prints the true residual at each cycle start."""
+    import re
+
+    p = SolidProblem(5)
+    p.assemble()
+    pc = TACS.Pc(p.mat)
+    pc.factor()
+    ksm = TACS.KSM(p.mat, pc, 20, 50)
+    ksm.setTolerances(1e-10, 1e-30)
+    f1 = tmp_path / "gmres.txt"
+    ksm.setMonitor(descript="GMRES", freq=5, filename=f1)
+    x = p.assembler.createVec()
+    assert ksm.solve(p.res, x) == 1
+    its = ksm.getIterCount()
+    del ksm  # closes the file
+    lines = f1.read_text().splitlines()
+    pat = re.compile(r"GMRES\[\s*(\d+)\]:\s+(\S+)$")
+    recs = [pat.match(l) for l in lines]
+    assert all(recs)
+    steps = [int(r.group(1)) for r in recs]
+    vals = [float(r.group(2)) for r in recs]
+    assert steps[0] == 0 and abs(vals[0] - np.linalg.norm(p.res.getArray())) <= 1e-12 * vals[0]
+    assert all(k == 0 or k % 5 == 0 for k in steps)
+    nrestart = (its + 19) // 20
+    assert steps.count(0) == nrestart and len(steps) == nrestart + sum(
+        min(20, its - 20 * c) // 5 for c in range(nrestart))
+    mk = TACS.MixedKSM(p.mat, pc, 30, 100, deflation=8)
+    mk.setTolerances(1e-10, 1e-30)
+    f2 = tmp_path / "rec.txt"
+    mk.setMonitor(descript="rec", freq=1, filename=f2)
+    assert mk.solve(p.res, p.assembler.createVec()) == 1
+    cycles = mk.getOuterCount()
+    del mk
+    lines = f2.read_text().splitlines()
+    assert len(lines) == cycles + 1 and all(l.startswith("rec[  0]: ") for l in lines)
+    r = [float(l.split()[-1]) for l in lines]
+    assert r[-1] <= 1e-10 * r[0] < r[-2]

@@ -60,6 +60,26 @@ def test_assembler_argument_errors():
     assert a.getNumElements() == m.nelem and a.getVarsPerNode() == 3
 
 
+def test_ksm_monitor_objects(tmp_path):
+    """KSMPrintStdout / KSMPrintFile (src/bpmat/KSM.cpp:239-330): reference
+    counted handles; a file that cannot be opened is an error; set_monitor
+    rejects a handle that is not a Krylov solver or not a monitor."""
+    L = lib()
+    m = L.tacs_ksm_print_stdout_create(b"GMRES", 0, 10)
+    assert m and L.tacs_refcount(m) == 0
+    L.tacs_incref(m)
+    assert L.tacs_refcount(m) == 1
+    f = L.tacs_ksm_print_file_create(str(tmp_path / "mon.txt").encode(), b"x", 0, 5)
+    assert f and (tmp_path / "mon.txt").exists()
+    L.tacs_incref(f)
+    assert not L.tacs_ksm_print_file_create(str(tmp_path / "no" / "dir.txt").encode(), b"x", 0, 5)
+    assert b"cannot open" in L.tacs_last_error()
+    props = constitutive.MaterialProperties(E=70e9, nu=0.3)
+    assert L.tacs_ksm_set_monitor(props.ptr, m) != 0
+    L.tacs_decref(f)
+    L.tacs_decref(m)
+
+
 def test_header_parser_types():
     d = parse_header()
     res, args = d["tacs_hip_bsr_spmv"]
