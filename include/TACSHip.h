@@ -133,8 +133,27 @@ class TACSElement3D : public TACSElement {
   ~TACSElement3D();
   int getVarsPerNode() { return model->getVarsPerNode(); }
   int getNumNodes() { return basis->getNumNodes(); }
+  // TACSElement3D::createElementTraction / createElementPressure
+  // src/elements/TACSElement3D.cpp:48, :54 (new objects, reference count 0)
+  TACSElement* createElementTraction(int faceIndex, const TacsScalar t[]);
+  TACSElement* createElementPressure(int faceIndex, TacsScalar p);
   TACSElementModel* model;
   TACSElementBasis* basis;
+};
+
+// src/elements/TACSPressure3D.h: constant pressure p on face faceIndex of a
+// solid element; the residual adds -p n (n the outward unit normal of the
+// face) integrated against the basis (TACSPressure3D::addResidual
+// src/elements/TACSPressure3D.cpp:60).
+class TACSPressure3D : public TACSElement {
+ public:
+  TACSPressure3D(int varsPerNode, int faceIndex, TACSElementBasis* basis, TacsScalar p);
+  ~TACSPressure3D();
+  int getVarsPerNode() { return varsPerNode; }
+  int getNumNodes() { return basis->getNumNodes(); }
+  int varsPerNode, faceIndex;
+  TACSElementBasis* basis;
+  TacsScalar p;
 };
 
 // src/elements/TACSTraction3D.h:22 (constant traction, coordinate components)
@@ -654,6 +673,7 @@ class TACSAssembler : public TACSObject {
   int *d_aux_elem, *d_aux_face, *d_n2a_ptr, *d_n2a;
   double *d_aux_trac, *d_aux_ev, *d_aux_pres;
   int naux;
+  int naux_pressure;  // TACSPressure3D aux elements among them (solids)
   int* d_status;
   void* d_red;
   double* d_scalar;

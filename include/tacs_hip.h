@@ -89,12 +89,16 @@ int tacs_hip_solid_elem_vectors(int order, int mode, int nelem, const int* conn,
                                 double ks_max, double ks_sum, double safety, double* ev,
                                 hipStream_t stream);
 
-/* Traction aux-element vectors (TACSTraction3D::addResidual
-   src/elements/TACSTraction3D.cpp:87, coordinate-component tractions). */
+/* Traction and pressure aux-element vectors: aux element q adds the traction
+   aux_trac[3q..3q+2] (TACSTraction3D::addResidual
+   src/elements/TACSTraction3D.cpp:87, coordinate components) and the
+   pressure aux_pres[q] (TACSPressure3D::addResidual
+   src/elements/TACSPressure3D.cpp:60: traction -p n, n the outward unit
+   normal of face aux_face[q]); aux_pres may be NULL. */
 int tacs_hip_solid_traction_vectors(int order, int naux, const int* aux_elem,
                                     const int* aux_face, const double* aux_trac,
-                                    const int* conn, const double* X, double lambda,
-                                    double* ev, hipStream_t stream);
+                                    const double* aux_pres, const int* conn, const double* X,
+                                    double lambda, double* ev, hipStream_t stream);
 
 /* Node-coordinate sensitivities as element vectors out[nelem * 3*npe]:
    mode 0 compliance dC/dX at fixed u (TACSAssembler::addXptSens
@@ -427,6 +431,10 @@ tacs_obj tacs_linear_elasticity3d_create(tacs_obj stiff, int strain_type);
 tacs_obj tacs_element3d_create(tacs_obj model, tacs_obj basis);
 tacs_obj tacs_traction3d_create(int vars_per_node, int face, tacs_obj basis,
                                 const double* trac, int coord_component);
+/* TACSPressure3D (src/elements/TACSPressure3D.h; TACSElement3D::createElementPressure
+   src/elements/TACSElement3D.cpp:54): constant pressure p on face (0..5) of a
+   solid element, traction -p n with n the outward unit face normal. */
+tacs_obj tacs_pressure3d_create(int vars_per_node, int face, tacs_obj basis, double p);
 /* TACSIsoShellConstitutive(props, t, tNum, tlb, tub, tOffset, kcorr)
    (src/constitutive/TACSIsoShellConstitutive.h:33; tacs/constitutive.pyx:1109) */
 tacs_obj tacs_iso_shell_constitutive_create(tacs_obj props, double t, int tNum, double tlb,

@@ -132,6 +132,21 @@ def add_tractions(order, conn, X, aux_elem, aux_face, aux_trac, res):
     return res
 
 
+def pressure_face(order, face, Xe, p):
+    """Residual of one TACSPressure3D face (element vector, 3 * npe)."""
+    res = np.zeros(3 * order ** 3)
+    lib().orc_pressure_face(order, face, _p(_dd(Xe)), ctypes.c_double(p), _p(res))
+    return res
+
+
+def add_pressures(order, conn, X, aux_elem, aux_face, aux_p, res):
+    aux_elem = _ii(aux_elem)
+    lib().orc_add_pressures(order, aux_elem.size, _p(aux_elem, np.int32),
+                            _p(_ii(aux_face), np.int32), _p(_dd(aux_p)),
+                            _p(_ii(conn).reshape(-1), np.int32), _p(_dd(X)), _p(res))
+    return res
+
+
 def apply_bc_mat(b, bc_nodes, bc_vars, rowp, cols, A):
     bc_nodes = _ii(bc_nodes)
     lib().orc_apply_bc_mat(b, bc_nodes.size, _p(bc_nodes, np.int32),

@@ -10,6 +10,7 @@
 #include "TACSShellElementDefs.h"
 #include "TACSSolidConstitutive.h"
 #include "TACSTraction3D.h"
+#include "TACSPressure3D.h"
 
 extern "C" {
 
@@ -101,6 +102,25 @@ void ref_solid_traction(int order, int face, const double* X, const double* trac
   tr->addResidual(0, 0.0, X, zero, zero, zero, res);
   delete[] zero;
   tr->decref();
+  basis->decref();
+}
+
+// Pressure residual of one solid element face (TACSPressure3D).
+void ref_solid_pressure(int order, int face, const double* X, double p, double* res) {
+  TACSElementBasis* basis;
+  if (order == 2)
+    basis = new TACSLinearHexaBasis();
+  else
+    basis = new TACSQuadraticHexaBasis();
+  basis->incref();
+  TACSPressure3D* pr = new TACSPressure3D(3, face, basis, p);
+  pr->incref();
+  int nv = 3 * basis->getNumNodes();
+  double* zero = new double[nv]();
+  for (int i = 0; i < nv; i++) res[i] = 0.0;
+  pr->addResidual(0, 0.0, X, zero, zero, zero, res);
+  delete[] zero;
+  pr->decref();
   basis->decref();
 }
 

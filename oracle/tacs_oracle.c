@@ -314,6 +314,60 @@ void orc_traction_face(int order, int face, const double *X,
   }
 }
 
+/*
+  TACSPressure3D::addResidual (src/elements/TACSPressure3D.cpp:60): at each
+  face quadrature point the unit normal n = (t1 x t2)/|t1 x t2| with the face
+  tangents of TACSHexaBasis (getFaceNormal basis/TACSElementBasis.cpp:37),
+  DUt = -p n, res += w |t1 x t2| N DUt.
+*/
+void orc_pressure_face(int order, int face, const double *X, double p, double *res) {
+  const int npe = order * order * order;
+  const int nfq = order * order;
+  for (int n = 0; n < nfq; n++) {
+    double pt[3], tg[6], N[MAXN], Nxi[3 * MAXN], Xd[9] = {0};
+    double w = orc_hex_face_quadrature(order, face, n, pt, tg);
+    orc_hex_basis(order, pt, N, Nxi);
+    for (int a = 0; a < npe; a++) {
+      for (int i = 0; i < 3; i++) {
+        for (int k = 0; k < 3; k++) Xd[3 * i + k] += X[3 * a + i] * Nxi[3 * a + k];
+      }
+    }
+    double t1[3], t2[3], nrm[3];
+    for (int i = 0; i < 3; i++) {
+      t1[i] = Xd[3 * i] * tg[0] + Xd[3 * i + 1] * tg[1] + Xd[3 * i + 2] * tg[2];
+      t2[i] = Xd[3 * i] * tg[3] + Xd[3 * i + 1] * tg[4] + Xd[3 * i + 2] * tg[5];
+    }
+    nrm[0] = t1[1] * t2[2] - t1[2] * t2[1];
+    nrm[1] = t1[2] * t2[0] - t1[0] * t2[2];
+    nrm[2] = t1[0] * t2[1] - t1[1] * t2[0];
+    double A = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    double DUt[3];
+    for (int k = 0; k < 3; k++) DUt[k] = -p * nrm[k] / A;
+    double wa = w * A;
+    for (int a = 0; a < npe; a++) {
+      for (int k = 0; k < 3; k++) res[3 * a + k] += wa * N[a] * DUt[k];
+    }
+  }
+}
+
+/* Pressure aux elements (TACSPressure3D), summed into the node residual. */
+void orc_add_pressures(int order, int naux, const int *aux_elem, const int *aux_face,
+                       const double *aux_p, const int *conn, const double *X, double *res) {
+  const int npe = order * order * order;
+  double Xe[3 * MAXN], re[3 * MAXN];
+  for (int q = 0; q < naux; q++) {
+    const int *c = &conn[npe * aux_elem[q]];
+    for (int a = 0; a < npe; a++) {
+      for (int k = 0; k < 3; k++) Xe[3 * a + k] = X[3 * c[a] + k];
+    }
+    memset(re, 0, sizeof(re));
+    orc_pressure_face(order, aux_face[q], Xe, aux_p[q], re);
+    for (int a = 0; a < npe; a++) {
+      for (int k = 0; k < 3; k++) res[3 * c[a] + k] += re[3 * a + k];
+    }
+  }
+}
+
 static int cmp_int(const void *a, const void *b) {
   int x = *(const int *)a, y = *(const int *)b;
   return (x > y) - (x < y);

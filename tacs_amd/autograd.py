@@ -25,7 +25,7 @@ class SolidComplianceProblem:
     solver for repeated forward / backward passes."""
 
     def __init__(self, problem, p=3.0, E0=1.0, Emin=1e-9, rtol=1e-8, m=120, mixed=True,
-                 deflation=0):
+                 deflation=0, restarts=100000):
         self.prob = problem
         self.p, self.E0, self.Emin, self.rtol = p, E0, Emin, rtol
         a = problem.assembler
@@ -35,9 +35,9 @@ class SolidComplianceProblem:
         if mixed:
             # deflation > 0: the recycling solver (its space carries over to
             # the adjoint solve with the same matrix)
-            self.ksm = TACS.MixedKSM(problem.mat, self.pc, m, 100000, deflation=deflation)
+            self.ksm = TACS.MixedKSM(problem.mat, self.pc, m, restarts, deflation=deflation)
         else:
-            self.ksm = TACS.KSM(problem.mat, self.pc, min(m, 512), 100000)
+            self.ksm = TACS.KSM(problem.mat, self.pc, min(m, 512), restarts)
         self.ksm.setTolerances(rtol, 1e-30)
         self.comp = functions.Compliance(a)
         self.iters = []

@@ -74,6 +74,14 @@ tacs_obj tacs_traction3d_create(int vpn, int face, tacs_obj basis, const double*
   }
   return (TACSObject*)new TACSTraction3D(vpn, face, b, trac, coord);
 }
+tacs_obj tacs_pressure3d_create(int vpn, int face, tacs_obj basis, double p) {
+  REQUIRE_NULL(TACSElementBasis, b, basis);
+  if (face < 0 || face > 5) {
+    tacs_set_error("tacs_pressure3d_create: face %d out of range", face);
+    return NULL;
+  }
+  return (TACSObject*)new TACSPressure3D(vpn, face, b, p);
+}
 tacs_obj tacs_iso_shell_constitutive_create(tacs_obj props, double t, int tNum, double tlb,
                                             double tub, double tOffset, double kcorr) {
   REQUIRE_NULL(TACSMaterialProperties, p, props);

@@ -162,6 +162,17 @@ TACSTraction3D::TACSTraction3D(int vpn, int face, TACSElementBasis* b, const Tac
   for (int i = 0; i < 24; i++) trac[i] = (i < n) ? t[i] : 0.0;
 }
 TACSTraction3D::~TACSTraction3D() { basis->decref(); }
+TACSPressure3D::TACSPressure3D(int vpn, int face, TACSElementBasis* b, TacsScalar _p)
+    : varsPerNode(vpn), faceIndex(face), basis(b), p(_p) {
+  basis->incref();
+}
+TACSPressure3D::~TACSPressure3D() { basis->decref(); }
+TACSElement* TACSElement3D::createElementTraction(int face, const TacsScalar t[]) {
+  return new TACSTraction3D(getVarsPerNode(), face, basis, t, 1);
+}
+TACSElement* TACSElement3D::createElementPressure(int face, TacsScalar p) {
+  return new TACSPressure3D(getVarsPerNode(), face, basis, p);
+}
 
 TACSAuxElements::TACSAuxElements(int size) : n(0), cap(size > 0 ? size : 16), version(0) {
   nums = new int[cap];
@@ -1984,7 +1995,7 @@ TacsScalar TACSKSFailure::getFunctionValue() {
 TACSAssembler::TACSAssembler(TACSComm comm, int vpn, int nowned, int nelems, int ndep)
     : varsPerNode(vpn), numNodes(nowned), numElements(nelems), npe(0), order(0),
       initialized(0), numDesignVars(0), h_ptr(NULL), h_conn(NULL), elements(NULL),
-      auxElements(NULL), aux_version(-1), naux(0), nnzb(0), jac_ms(0.0) {
+      auxElements(NULL), aux_version(-1), naux(0), naux_pressure(0), nnzb(0), jac_ms(0.0) {
   if (ndep != 0) tacs_set_error("TACSAssembler: dependent nodes are not supported");
   bcMap = new TACSBcMap(varsPerNode, 64);
   bcMap->incref();
@@ -2341,7 +2352,7 @@ void TACSAssembler::applyBCs(TACSMat* m) { m->applyBCs(bcMap); }
 
 int TACSAssembler::uploadAux() {
   if (!auxElements) {
-    naux = 0;
+    naux = naux_pressure = 0;
     return 0;
   }
   if (aux_version == auxElements->version) return 0;
@@ -2352,7 +2363,8 @@ int TACSAssembler::uploadAux() {
   std::stable_sort(idx.begin(), idx.end(),
                    [&](int a, int b) { return auxElements->nums[a] < auxElements->nums[b]; });
   std::vector<int> ae(n), af(n);
-  std::vector<double> at((size_t)3 * n), ap(n, 0.0);
+  std::vector<double> at((size_t)3 * n, 0.0), ap(n, 0.0);
+  int npres = 0;
   for (int q = 0; q < n; q++) {
     if (family == FAMILY_SHELL) {
       TACSShellPressure* pr = dynamic_cast<TACSShellPressure*>(auxElements->elems[idx[q]]);
@@ -2362,8 +2374,18 @@ int TACSAssembler::uploadAux() {
       ap[q] = pr->p;
       continue;
     }
+    if (TACSPressure3D* pr = dynamic_cast<TACSPressure3D*>(auxElements->elems[idx[q]])) {
+      if (pr->basis->getOrder() != order) return tacs_set_error("aux element basis mismatch");
+      if (pr->faceIndex < 0 || pr->faceIndex > 5)
+        return tacs_set_error("aux element %d: face %d out of range", idx[q], pr->faceIndex);
+      ae[q] = auxElements->nums[idx[q]];
+      af[q] = pr->faceIndex;
+      ap[q] = pr->p;
+      npres++;
+      continue;
+    }
     TACSTraction3D* tr = dynamic_cast<TACSTraction3D*>(auxElements->elems[idx[q]]);
-    if (!tr) return tacs_set_error("aux element %d is not a TACSTraction3D", idx[q]);
+    if (!tr) return tacs_set_error("aux element %d is not a TACSTraction3D or TACSPressure3D", idx[q]);
     if (!tr->coordComponent)
       return tacs_set_error("aux element %d: only coordinate-component tractions supported", idx[q]);
     if (tr->basis->getOrder() != order) return tacs_set_error("aux element basis mismatch");
@@ -2398,6 +2420,7 @@ int TACSAssembler::uploadAux() {
   h2d(d_n2a, list.data(), list.size());
   HCHECK(hipStreamSynchronize(g_stream));
   naux = n;
+  naux_pressure = family == FAMILY_SHELL ? 0 : npres;
   aux_version = auxElements->version;
   return 0;
 }
@@ -2460,8 +2483,9 @@ void TACSAssembler::addAuxResidual(double* r, TacsScalar lambda) {
     return;
   }
   {
-    KCHECK(tacs_hip_solid_traction_vectors(order, naux, d_aux_elem, d_aux_face, d_aux_trac, d_conn,
-                                           d_X, lambda, d_aux_ev, g_stream));
+    KCHECK(tacs_hip_solid_traction_vectors(order, naux, d_aux_elem, d_aux_face, d_aux_trac,
+                                           naux_pressure ? d_aux_pres : NULL, d_conn, d_X, lambda,
+                                           d_aux_ev, g_stream));
     KCHECK(tacs_hip_gather_elem_vectors(3 * npe, numNodes, d_n2a_ptr, d_n2a, d_aux_ev, 1, r,
                                         g_stream));
   }
@@ -2687,6 +2711,10 @@ void TACSAssembler::addAdjointResXptSensProducts(TacsScalar scale, int numAdjoin
     KCHECK(tacs_hip_gather_elem_vectors(3 * npe, numNodes, d_n2e_ptr, d_n2e, d_ev, 1, out,
                                         g_stream));
     if (uploadAux() == 0 && naux > 0) {
+      if (naux_pressure) {
+        tacs_set_error("addAdjointResXptSensProducts: pressure aux elements are not supported");
+        return;
+      }
       KCHECK(tacs_hip_solid_traction_xpt_vectors(order, naux, d_aux_elem, d_aux_face, d_aux_trac,
                                                  d_conn, d_X, psi, lambda * scale, d_aux_ev,
                                                  g_stream));

@@ -642,12 +642,15 @@ __global__ void solid_traction_xpt_vectors(int naux, const int* __restrict__ aux
   }
 }
 
-// Traction aux elements (TACSTraction3D::addResidual, TACSTraction3D.cpp:87):
-// one thread per (aux element); element vector out[aux * NV].
+// Traction aux elements (TACSTraction3D::addResidual, TACSTraction3D.cpp:87)
+// and pressure aux elements (TACSPressure3D::addResidual, TACSPressure3D.cpp:60,
+// traction -p n with n the outward unit normal): one thread per aux element;
+// element vector out[aux * NV].
 template <int ORD>
 __global__ void solid_traction_vectors(int naux, const int* __restrict__ aux_elem,
                                        const int* __restrict__ aux_face,
                                        const double* __restrict__ aux_trac,
+                                       const double* __restrict__ aux_pres,
                                        const int* __restrict__ conn,
                                        const double* __restrict__ X, double lambda,
                                        double* __restrict__ out) {
@@ -656,7 +659,8 @@ __global__ void solid_traction_vectors(int naux, const int* __restrict__ aux_ele
   if (q >= naux) return;
   const int* c = &conn[(size_t)aux_elem[q] * NPE];
   const int face = aux_face[q];
-  const double tr[3] = {aux_trac[3 * q], aux_trac[3 * q + 1], aux_trac[3 * q + 2]};
+  const double tr0[3] = {aux_trac[3 * q], aux_trac[3 * q + 1], aux_trac[3 * q + 2]};
+  const double pres = aux_pres ? aux_pres[q] : 0.0;
   double* o = &out[(size_t)q * NV];
   for (int k = 0; k < NV; k++) o[k] = 0.0;
   // tangents ordered so that t1 x t2 is the outward normal
@@ -694,7 +698,11 @@ __global__ void solid_traction_vectors(int naux, const int* __restrict__ aux_ele
     double nx = t1[1] * t2[2] - t1[2] * t2[1];
     double ny = t1[2] * t2[0] - t1[0] * t2[2];
     double nz = t1[0] * t2[1] - t1[1] * t2[0];
-    double wa = lambda * w * sqrt(nx * nx + ny * ny + nz * nz);
+    const double area = sqrt(nx * nx + ny * ny + nz * nz);
+    const double wa = lambda * w * area;
+    // traction plus pressure: t - (-p n) enters the residual as -(t + p n)
+    const double pa = area > 0.0 ? pres / area : 0.0;
+    const double tr[3] = {tr0[0] + pa * nx, tr0[1] + pa * ny, tr0[2] + pa * nz};
     for (int a = 0; a < NPE; a++) {
       double N, g[3];
       qp.basis(a, N, g);
@@ -893,16 +901,16 @@ int tacs_hip_solid_elem_vectors(int order, int mode, int nelem, const int* conn,
 
 int tacs_hip_solid_traction_vectors(int order, int naux, const int* aux_elem,
                                     const int* aux_face, const double* aux_trac,
-                                    const int* conn, const double* X, double lambda,
-                                    double* ev, hipStream_t stream) {
+                                    const double* aux_pres, const int* conn, const double* X,
+                                    double lambda, double* ev, hipStream_t stream) {
   if (naux <= 0) return 0;
   int grid = (naux + 63) / 64;
   if (order == 2)
     hipLaunchKernelGGL(solid_traction_vectors<2>, dim3(grid), dim3(64), 0, stream, naux,
-                       aux_elem, aux_face, aux_trac, conn, X, lambda, ev);
+                       aux_elem, aux_face, aux_trac, aux_pres, conn, X, lambda, ev);
   else if (order == 3)
     hipLaunchKernelGGL(solid_traction_vectors<3>, dim3(grid), dim3(64), 0, stream, naux,
-                       aux_elem, aux_face, aux_trac, conn, X, lambda, ev);
+                       aux_elem, aux_face, aux_trac, aux_pres, conn, X, lambda, ev);
   else
     return TACS_HIP_ERR_ARG;
   return (int)hipGetLastError();

@@ -46,6 +46,10 @@ class Element3D(TACSHandle):
     def createElementTraction(self, faceIndex, trac):
         return Traction3D(3, faceIndex, self.basis, trac)
 
+    def createElementPressure(self, faceIndex, p):
+        """TACSElement3D::createElementPressure (src/elements/TACSElement3D.cpp:54)."""
+        return Pressure3D(3, faceIndex, self.basis, p)
+
 
 class Traction3D(TACSHandle):
     def __init__(self, varsPerNode, faceIndex, basis, traction, coordComp=True):
@@ -53,6 +57,15 @@ class Traction3D(TACSHandle):
                                        + [0.0] * max(0, 24 - len(traction))))
         super().__init__(lib().tacs_traction3d_create(varsPerNode, faceIndex, basis.ptr, arr,
                                                       1 if coordComp else 0), keep=(basis,))
+
+
+class Pressure3D(TACSHandle):
+    """TACSPressure3D (src/elements/TACSPressure3D.h): constant pressure p on
+    one face of a solid element (traction -p n, n the outward unit normal)."""
+
+    def __init__(self, varsPerNode, faceIndex, basis, p):
+        super().__init__(lib().tacs_pressure3d_create(varsPerNode, faceIndex, basis.ptr,
+                                                      float(p)), keep=(basis,))
 
 
 class ShellNaturalTransform(TACSHandle):

@@ -91,6 +91,18 @@ def make(seed=2024):
         out[key + "_t"], out[key + "_res"], out[key + "_mat"] = np.array([t]), res, mat
         out[key + "_adj"], out[key + "_fail"], out[key + "_det"] = np.array([adj]), fail, det
         out[key + "_pres"] = pres
+    # solid face pressures (TACSPressure3D), separate stream
+    rng3 = np.random.default_rng(seed + 2)
+    for order in (2, 3):
+        npe = order ** 3
+        for s in range(2):
+            key = f"hex{npe}_{s}"
+            X = out[key + "_X"]
+            pv = float(rng3.standard_normal())
+            pr = np.zeros((6, 3 * npe))
+            for face in range(6):
+                L.ref_solid_pressure(order, face, _d(X), c(pv), _d(pr[face]))
+            out[key + "_p"], out[key + "_p_res"] = np.array([pv]), pr
     np.savez_compressed(os.path.join(HERE, "elements.npz"), **out)
     return out
 

@@ -202,6 +202,38 @@ def test_products_follow_reassembly():
     assert np.abs(y32.cpu().numpy() - yo3232).max() <= 1e-6 * np.abs(yo3232).max()
 
 
+@pytest.mark.parametrize("order", [2, 3])
+def test_pressure_aux_elements_match_oracle(order):
+    """TACSPressure3D aux elements (several per element, every face, with a
+    traction on the x = 1 face as well) in the assembled residual, against the
+    oracle's restatement of TACSPressure3D::addResidual."""
+    from tacs_amd import elements
+
+    p = SolidProblem(3, order=order)
+    m = p.mesh
+    rng = np.random.default_rng(order)
+    q = 12
+    elems = rng.choice(m.nelem, q)
+    faces = rng.integers(0, 6, q)
+    pv = rng.standard_normal(q) * 3e7
+    aux = TACS.AuxElements()
+    for e in p.aux_elems:
+        aux.addElement(int(e), elements.Traction3D(3, 1, p.basis, list(p.traction)))
+    for e, f, v in zip(elems, faces, pv):
+        aux.addElement(int(e), elements.Pressure3D(3, int(f), p.basis, float(v)))
+    p.assembler.setAuxElements(aux)
+    p.assemble()
+    r = p.res.getArray().copy()
+    C, t, rowp, cols, A, res = _oracle_system(p)
+    pres = np.zeros(3 * m.nnodes)
+    O.add_pressures(order, m.conn, m.X, elems, faces, pv, pres)
+    bcv = np.full(p.bc_nodes.size, 7, np.int32)
+    O.apply_bc_vec(3, p.bc_nodes, bcv, pres)
+    expect = res + pres
+    assert np.abs(r - expect).max() <= 1e-12 * np.abs(expect).max()
+    assert np.abs(pres).max() > 0.1 * np.abs(expect).max()
+
+
 def test_golden_cube_compliance_and_ks():
     """tests/integration_tests/test_elast_linhexa_element_3d.py:11-27: 3x3x3
     cube of side 10, rho=2570, E=70e9, nu=0.3, ys=350e6, x=0 clamped,

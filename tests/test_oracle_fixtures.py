@@ -32,6 +32,33 @@ def test_solid_element_matches_reference(key):
         assert np.abs(ro - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("key", ["hex8_0", "hex8_1", "hex27_0", "hex27_1"])
+def test_solid_pressure_matches_reference(key):
+    """TACSPressure3D face residuals of the reference build on distorted
+    hexes (every face), and the closed form on a box: the face residual sums
+    to -p n A, i.e. a force p n A along the outward normal n (positive
+    pressure pulls; pyTACS flips the sign of Nastran PLOAD4 on solids)."""
+    order = 2 if key.startswith("hex8") else 3
+    X, pv = FIX[key + "_X"], float(FIX[key + "_p"][0])
+    for face in range(6):
+        ref = FIX[key + "_p_res"][face]
+        ro = O.pressure_face(order, face, X, pv)
+        assert np.abs(ro - ref).max() <= 1e-13 * np.abs(ref).max()
+    k = [-1.0, 1.0] if order == 2 else [-1.0, 0.0, 1.0]
+    box = np.array([[x, y, z] for z in k for y in k for x in k]) * [0.5, 0.75, 1.25] + 2.0
+    dims = np.array([1.0, 1.5, 2.5])
+    for face in range(6):
+        ax, sgn = face // 2, (1.0 if face % 2 else -1.0)
+        n = np.zeros(3)
+        n[ax] = sgn
+        area = np.prod(np.delete(dims, ax))
+        r = O.pressure_face(order, face, box.reshape(-1), 3.0).reshape(-1, 3)
+        assert np.allclose(r.sum(0), -3.0 * n * area, rtol=1e-13, atol=1e-13)
+        # only the nodes of the face carry load
+        on = np.isclose(box[:, ax], 2.0 + sgn * dims[ax] / 2)
+        assert np.abs(r[~on]).max() == 0.0
+
+
 @pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built (no reference tree)")
 def test_solid_element_fresh_inputs_against_reference_build():
     R = ctypes.CDLL(REF, mode=ctypes.RTLD_GLOBAL)
