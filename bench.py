@@ -749,10 +749,12 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--m", type=int, default=60,
-                    help="restart length of the mixed-precision GMRES")
-    ap.add_argument("--deflation", type=int, default=20,
-                    help="recycled harmonic Ritz vectors of the mixed-precision GMRES")
+    ap.add_argument("--m", type=int, default=None,
+                    help="restart length of the mixed-precision GMRES (default 60; 120 for "
+                         "the shell workload)")
+    ap.add_argument("--deflation", type=int, default=None,
+                    help="recycled harmonic Ritz vectors of the mixed-precision GMRES (default "
+                         "20; 30 for the shell workload)")
     ap.add_argument("--m-fp64", type=int, default=30,
                     help="restart length of the plain FP64 GMRES (the reference default MGS)")
     ap.add_argument("--rtol", type=float, default=1e-8)
@@ -772,6 +774,13 @@ def main():
     ap.add_argument("--monitor", default=None,
                     help="write the solvers' residual monitor lines to FILE.<solver>")
     args = ap.parse_args()
+    # the thin plate's block-Jacobi spectrum wants a longer cycle and a larger
+    # recycled space (512^2: 67k + 42k steps with 120 / 30)
+    shell = args.workload == "shell"
+    if args.m is None:
+        args.m = 120 if shell else 60
+    if args.deflation is None:
+        args.deflation = 30 if shell else 20
     MONITOR["file"] = args.monitor
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "

@@ -1512,6 +1512,10 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
   auto V = [&](int i) { return d_V + (size_t)i * N; };
   auto Ub = [&](int v) { return d_Ub + (size_t)v * N; };
   auto Cb = [&](int v) { return d_Cb + (size_t)v * N; };
+  // pr: precision of the operator's values; a solve that stalls with the
+  // FP32-valued operator continues with the FP64 one (see the cycle loop)
+  int pr = prec;
+  bool fell_back = false;
   const float* A32 = prec == 32 ? mat->getValues32(1) : NULL;
   const void* Dinv = prec == 32 ? (const void*)pc->d_Dinv32 : (const void*)pc->d_Dinv;
   // one right-hand side: the single-vector block-SpMV kernels
@@ -1522,10 +1526,10 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       KCHECK(tacs_hip_bsr3_spmm(nr, Rn, mat->d_rowp, mat->d_cols, mat->d_A, x, y, g_stream));
   };
   auto op = [&](const double* x, double* y) {
-    KCHECK(tacs_hip_bdiag_apply(mat->bsize, prec, nr, Rn, Dinv, x, d_work, g_stream));
-    if (prec == 32 && Rn == 1)
+    KCHECK(tacs_hip_bdiag_apply(mat->bsize, pr, nr, Rn, Dinv, x, d_work, g_stream));
+    if (pr == 32 && Rn == 1)
       mat->multMixed(d_work, y);
-    else if (prec == 32)
+    else if (pr == 32)
       KCHECK(tacs_hip_bsr3_spmm_mixed(nr, Rn, mat->d_rowp, mat->d_cols, A32, d_work, y, g_stream));
     else
       spmm64(d_work, y);
@@ -1569,6 +1573,68 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       }
   };
 
+  // Refresh of the recycled spaces: C_j = A M^-1 U_j recomputed with the
+  // current operator, then C_j = Q R by Cholesky QR, twice, and U_j <- U_j R^-1,
+  // so A M^-1 U_j = C_j with C_j orthonormal to rounding again.  The cycle-end
+  // updates keep the relation only in exact arithmetic; over thousands of
+  // cycles the drift makes the projection's residual estimate optimistic and
+  // the true residual stalls.
+  auto refresh_space = [&](int Kd) {
+    if (Kd <= 0) return;
+    for (int v = 0; v < Kd; v++) op(Ub(v), Cb(v));
+    std::vector<double> G((size_t)Rn * Kd * Kd), P((size_t)Rn * Kd * Kd);
+    for (int pass = 0; pass < 2; pass++) {
+      KCHECK(tacs_hip_btsgemm(n, Rn, Kd, Cb(0), Kd, Cb(0), N, (double*)d_dq, d_prod, SP, Kd,
+                              g_stream));
+      d2h_rows(G.data(), Kd * Kd, d_prod, SP);
+      std::fill(P.begin(), P.end(), 0.0);
+      for (int j = 0; j < Rn; j++) {
+        const int kd = kdj[j];
+        const double* g = &G[(size_t)j * Kd * Kd];  // g[a + b Kd] = C_a . C_b
+        double* pj = &P[(size_t)j * Kd * Kd];        // pj[v + c Kd] = (R^-1)_{vc}
+        std::vector<double> L((size_t)kd * kd, 0.0);  // G = L L^T, row-major
+        bool ok = true;
+        for (int r = 0; r < kd && ok; r++)
+          for (int c = 0; c <= r; c++) {
+            double acc = g[r + (size_t)c * Kd];
+            for (int q = 0; q < c; q++) acc -= L[(size_t)r * kd + q] * L[(size_t)c * kd + q];
+            if (r == c) {
+              if (!(acc > 0.0)) {
+                ok = false;
+                break;
+              }
+              L[(size_t)r * kd + r] = sqrt(acc);
+            } else {
+              L[(size_t)r * kd + c] = acc / L[(size_t)c * kd + c];
+            }
+          }
+        if (!ok) {
+          kdj[j] = 0;  // a space that lost its rank is dropped
+          continue;
+        }
+        // R = L^T; R^-1 column by column (upper triangular)
+        for (int c = 0; c < kd; c++)
+          for (int v = c; v >= 0; v--) {
+            double acc = v == c ? 1.0 : 0.0;
+            for (int q = v + 1; q <= c; q++) acc -= L[(size_t)q * kd + v] * pj[q + (size_t)c * Kd];
+            pj[v + (size_t)c * Kd] = acc / L[(size_t)v * kd + v];
+          }
+        for (int c = kd; c < Kd; c++) pj[c + (size_t)c * Kd] = kd == 0 ? 0.0 : 1.0;
+      }
+      h2d_rows(d_prod, SP, P.data(), Kd * Kd);
+      KCHECK(tacs_hip_bbasis_combine(n, Rn, Kd, Cb(0), N, d_prod, SP, Kd, Kd, d_Cnb, N, 0,
+                                     g_stream));
+      KCHECK(tacs_hip_bbasis_combine(n, Rn, Kd, Ub(0), N, d_prod, SP, Kd, Kd, d_Unb, N, 0,
+                                     g_stream));
+      HCHECK(hipStreamSynchronize(g_stream));
+      std::swap(d_Cb, d_Cnb);
+      std::swap(d_Ub, d_Unb);
+    }
+    update_dscale(Kd);
+  };
+  int refreshes = 0;
+  bool nodefl = false;  // deflation switched off after repeated stalls
+
   // the spaces belong to one matrix and one number of right-hand sides
   if (defVersion != mat->version || defR != Rn) std::fill(kdj.begin(), kdj.end(), 0);
   defVersion = mat->version;
@@ -1588,6 +1654,8 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
 
   std::vector<char> active(Rn, 1), fail(Rn, 0), conv(Rn);
   std::vector<double> target(Rn), beta(Rn), tgt(Rn), hb, hh, hc((size_t)KD_MAX + m + 2);
+  std::vector<int> stall(Rn, 0);
+  std::vector<char> est_hit(Rn, 0);
   std::vector<int> jend(Rn);
   std::vector<HostLS> ls(Rn, HostLS(m + 1));
   iters.assign(Rn, 0);
@@ -1762,6 +1830,46 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       }
     }
     if (nact == 0 || count > nrestart) break;
+    // The cycle's least-squares estimate and the true FP64 residual part when
+    // rounding in the operator (about 6e-8 |A| |A^-1| for FP32 values) or in
+    // the recycled relation A M^-1 U = C matters at the requested tolerance.
+    // A stall: 5 cycles in a row whose estimate reached the target while the
+    // true residual did not; it is answered in stages below.
+    bool stalled = false;
+    for (int j = 0; j < Rn; j++) {
+      if (!active[j]) continue;
+      stall[j] = est_hit[j] ? stall[j] + 1 : 0;
+      if (stall[j] >= 5) stalled = true;
+    }
+    if (stalled) {
+      // first refresh the spaces; a second stall with the FP32-valued
+      // operator continues with the FP64 one (spaces refreshed for it); a
+      // further stall stops deflating for the rest of the solve: a recycled
+      // vector of a small eigenvalue of A M^-1 carries its rounding times
+      // |U| ~ 1/lambda into the true residual, which bounds what deflated
+      // cycles can reach on very ill-conditioned systems
+      if (Kd > 0 && refreshes > 0 && pr == 64) {
+        Kd = 0;
+        std::fill(kdj.begin(), kdj.end(), 0);
+        nodefl = true;
+        fell_back = true;
+        if (monitor)
+          monitor->print("recycling GMRES: stalled, continuing without deflation, classical "
+                         "Gram-Schmidt\n");
+      } else {
+        if (refreshes > 0 && pr == 32) {
+          pr = 64;
+          Dinv = pc->d_Dinv;
+          fell_back = true;
+          if (monitor) monitor->print("recycling GMRES: stalled, continuing with the FP64 operator\n");
+        } else if (monitor) {
+          monitor->print("recycling GMRES: stalled, recycled spaces refreshed\n");
+        }
+        refresh_space(Kd);
+        refreshes++;
+      }
+      std::fill(stall.begin(), stall.end(), 0);
+    }
     cycles++;
     for (int j = 0; j < Rn; j++) {
       h_inv[j] = active[j] ? 1.0 / rn[j] : 0.0;
@@ -1776,7 +1884,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       // x += |r| M^{-1} U C^T v_0;  v_0 -= C C^T v_0
       KCHECK(tacs_hip_bmdot(n, Rn, Kd, Cb(0), N, V(0), (double*)d_red, pco, S, 0, g_stream));
       KCHECK(tacs_hip_blincomb(n, Rn, Kd, Ub(0), N, pco, S, NULL, d_work, 0, g_stream));
-      KCHECK(tacs_hip_bdiag_apply(mat->bsize, prec, nr, Rn, Dinv, d_work, tmp2, g_stream));
+      KCHECK(tacs_hip_bdiag_apply(mat->bsize, pr, nr, Rn, Dinv, d_work, tmp2, g_stream));
       KCHECK(tacs_hip_bscale(n, Rn, nrm, tmp2, g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, tmp2, 1.0, Xd, g_stream));
       KCHECK(tacs_hip_bcgs_update(n, Rn, Kd, Cb(0), N, pco, S, V(0), g_stream));
@@ -1809,7 +1917,9 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
     int niters = 0;
     bool all = true;
     for (int j = 0; j < Rn; j++) all = all && conv[j];
-    if (delayed) {
+    // after repeated stalls the cycles use classical Gram-Schmidt twice with
+    // each column finished in its own step (no lagged normalisation)
+    if (delayed && !nodefl) {
       niters = arnoldi_delayed(s, Kd);
       all = true;  // skips the classical loop below
     }
@@ -1856,6 +1966,10 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       }
     }
     for (int j = 0; j < Rn; j++) {
+      // the cycle ended on the least-squares estimate (or the projection)
+      // reaching the target; whether the true residual agrees is checked at
+      // the next cycle start
+      est_hit[j] = active[j] && conv[j] && !fail[j];
       if (!conv[j]) jend[j] = niters;
       if (active[j]) iters[j] += jend[j];
     }
@@ -1877,7 +1991,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       h2d(inv, h_inv, Rn);
       if (Kd > 0) KCHECK(tacs_hip_blincomb(n, Rn, Kd, Ub(0), N, zc, S, NULL, d_work, 0, g_stream));
       KCHECK(tacs_hip_blincomb(n, Rn, niters, V(0), N, res, S, NULL, d_work, Kd > 0, g_stream));
-      KCHECK(tacs_hip_bdiag_apply(mat->bsize, prec, nr, Rn, Dinv, d_work, tmp2, g_stream));
+      KCHECK(tacs_hip_bdiag_apply(mat->bsize, pr, nr, Rn, Dinv, d_work, tmp2, g_stream));
       KCHECK(tacs_hip_bscale(n, Rn, inv, tmp2, g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, tmp2, 1.0, Xd, g_stream));
     }
@@ -1886,7 +2000,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
     std::vector<char> cand(Rn, 0);
     bool any = false;
     for (int j = 0; j < Rn; j++) {
-      cand[j] = active[j] && !fail[j] && jend[j] > 0 && kdj[j] + jend[j] > K + 1;
+      cand[j] = !nodefl && active[j] && !fail[j] && jend[j] > 0 && kdj[j] + jend[j] > K + 1;
       any = any || cand[j];
     }
     if (any) {
@@ -1971,6 +2085,8 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
     KCHECK(tacs_hip_axpby(64, 64, N, 1.0, Bd, -1.0, V(0), g_stream));
   }
   HCHECK(hipStreamSynchronize(g_stream));
+  // spaces built with the FP64 operator do not satisfy A32 D32^-1 U = C
+  if (fell_back && prec == 32) defVersion = -1;
   int ok = 1;
   for (int j = 0; j < Rn; j++) ok &= flags[j];
   return ok;
