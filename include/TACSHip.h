@@ -662,6 +662,7 @@ class TACSAssembler : public TACSObject {
   SolidMaterial mat;
   ShellMaterial smat;
   double* d_ke;  // shell element matrices [nelem][24][24]
+  double* d_qgeo;  // solid quadrature geometry [nelem][npe][10] (Jacobian scratch)
   // device data
   int *d_conn, *d_n2e_ptr, *d_n2e, *d_bcvars, *d_bcnodes, *d_bcmask;
   double* d_bcvals;

@@ -70,12 +70,14 @@ typedef struct ShellMaterial {
    (src/bpmat/BCSRMat.cpp:1800) and TACSParallelMat::applyBCs
    (src/bpmat/TACSParallelMat.cpp:343).
    order: 2 (8-node) or 3 (27-node) hexahedra; n2e: node->element CSR with
-   entries (element << 5 | local node); A (FP64) and/or A32 (FP32) outputs. */
-int tacs_hip_solid_jacobian(int order, int nnodes, const int* n2e_ptr, const int* n2e,
-                            const int* conn, const double* X, const double* telem,
-                            const SolidMaterial* mat, double alpha, double gamma,
-                            const int* rowp, const int* cols, const int* bcvars,
-                            double* A, float* A32, int* status, hipStream_t stream);
+   entries (element << 5 | local node); A (FP64) and/or A32 (FP32) outputs;
+   G: scratch of 10 * order^3 * nelem doubles (the per-quadrature-point
+   inverse Jacobians and weights, computed once per call). */
+int tacs_hip_solid_jacobian(int order, int nnodes, int nelem, const int* n2e_ptr,
+                            const int* n2e, const int* conn, const double* X,
+                            const double* telem, const SolidMaterial* mat, double alpha,
+                            double gamma, const int* rowp, const int* cols, const int* bcvars,
+                            double* G, double* A, float* A32, int* status, hipStream_t stream);
 
 /* Element vectors ev[nelem * 3*npe]:
    mode 0 residual (TACSElement3D::addResidual src/elements/TACSElement3D.cpp:139),

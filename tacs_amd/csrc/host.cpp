@@ -2119,6 +2119,7 @@ TACSAssembler::TACSAssembler(TACSComm comm, int vpn, int nowned, int nelems, int
   memset(&smat, 0, sizeof(smat));
   family = FAMILY_SOLID;
   d_ke = NULL;
+  d_qgeo = NULL;
   d_aux_pres = NULL;
   d_conn = d_n2e_ptr = d_n2e = d_bcvars = d_bcnodes = d_bcmask = NULL;
   d_bcvals = NULL;
@@ -2148,7 +2149,7 @@ TACSAssembler::~TACSAssembler() {
   void* ds[] = {d_conn, d_n2e_ptr, d_n2e, d_bcvars, d_bcnodes, d_bcmask, d_bcvals, d_dvnum,
                 d_dv2e_ptr, d_dv2e, d_tfixed, d_telem, d_X, d_u, d_ev, d_es, d_aux_elem,
                 d_aux_face, d_n2a_ptr, d_n2a, d_aux_trac, d_aux_ev, d_status, d_red, d_scalar,
-                d_ke, d_aux_pres};
+                d_ke, d_aux_pres, d_qgeo};
   for (void* p : ds) dfree(p);
 }
 
@@ -2655,9 +2656,11 @@ void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScal
   }
   if (residual) assembleRes(residual, lambda, bcs);
   HCHECK(hipEventRecord(e0, g_stream));
-  KCHECK(tacs_hip_solid_jacobian(order, numNodes, d_n2e_ptr, d_n2e, d_conn, d_X, d_telem, &mat,
-                                 alpha, gamma, A->d_rowp, A->d_cols, bcs ? d_bcvars : NULL, A->d_A,
-                                 A->d_A32, d_status, g_stream));
+  ensure(d_qgeo, (size_t)numElements * npe * 10);
+  KCHECK(tacs_hip_solid_jacobian(order, numNodes, numElements, d_n2e_ptr, d_n2e, d_conn, d_X,
+                                 d_telem, &mat, alpha, gamma, A->d_rowp, A->d_cols,
+                                 bcs ? d_bcvars : NULL, d_qgeo, A->d_A, A->d_A32, d_status,
+                                 g_stream));
   HCHECK(hipEventRecord(e1, g_stream));
   HCHECK(hipEventSynchronize(e1));
   float ms = 0.f;

@@ -56,7 +56,12 @@ __device__ __forceinline__ void lag1d(double x, double* n, double* d) {
 // dynamic indexing (which would place the array in private memory).
 template <int ORD>
 __device__ __forceinline__ double pick(const double* v, int i) {
-  if (ORD == 2) return i ? v[1] : v[0];
+  // both values are loaded and one is selected: written as i ? v[1] : v[0]
+  // the compiler indexed the array with i and kept it in scratch memory
+  if (ORD == 2) {
+    const double v0 = v[0], v1 = v[1];
+    return (i == 1) ? v1 : v0;
+  }
   return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
 }
 
@@ -166,6 +171,31 @@ __device__ __forceinline__ void grad_x(const double J[9], const double g[3], dou
 }
 
 // ---------------------------------------------------------------------------
+// Geometry of every (element, quadrature point): G[(e NQP + q) 10 + k] =
+// J^-1 (k < 9, J = dX/dxi) and w det(J) t_e (k = 9), computed once per
+// assembly; the row kernel copies the entries of a row's elements instead of
+// recomputing them for each of the element's nodes.
+// ---------------------------------------------------------------------------
+template <int ORD>
+__global__ __launch_bounds__(256) void solid_qp_geometry(int nelem, const int* __restrict__ conn,
+                                                         const double* __restrict__ X,
+                                                         const double* __restrict__ telem,
+                                                         double* __restrict__ G) {
+  constexpr int NPE = ORD * ORD * ORD, NQP = NPE;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nelem * NQP) return;
+  const int elem = (int)(t / NQP), q = (int)(t - (size_t)elem * NQP);
+  QP1D<ORD> qp;
+  qp.eval(q);
+  double J[9];
+  const double det = elem_geometry<ORD>(qp, &conn[(size_t)elem * NPE], X, J);
+  double* g = &G[t * 10];
+#pragma unroll
+  for (int k = 0; k < 9; k++) g[k] = J[k];
+  g[9] = qp.w * det * telem[elem];
+}
+
+// ---------------------------------------------------------------------------
 // Jacobian: one wavefront per block row.
 // ---------------------------------------------------------------------------
 template <int ORD>
@@ -179,11 +209,12 @@ struct JacCfg<3> {
   static constexpr int MAXC = 125, WPB = 2;
 };
 
-template <int ORD>
+// ISO: isotropic tangent (lambda, mu closed form) or the general C[21]; two
+// instantiations keep the general path's registers out of the isotropic one.
+template <int ORD, bool ISO>
 __global__ __launch_bounds__(64 * JacCfg<ORD>::WPB) void solid_jacobian_rows(
     int nnodes, const int* __restrict__ n2e_ptr, const int* __restrict__ n2e,
-    const int* __restrict__ conn, const double* __restrict__ X,
-    const double* __restrict__ telem, SolidMaterial mat, double alpha,
+    const int* __restrict__ conn, const double* __restrict__ Gq, SolidMaterial mat, double alpha,
     double gamma, const int* __restrict__ rowp, const int* __restrict__ cols,
     const int* __restrict__ bcvars, double* __restrict__ A,
     float* __restrict__ A32, int* __restrict__ status) {
@@ -215,17 +246,13 @@ __global__ __launch_bounds__(64 * JacCfg<ORD>::WPB) void solid_jacobian_rows(
 
     for (int c0 = 0; c0 < ne; c0 += ECH) {
       const int nce = min(ECH, ne - c0);
-      // Geometry of every (element, quadrature point) pair of the chunk.
-      for (int p = lane; p < nce * NQP; p += 64) {
-        int e = p / NQP, q = p - e * NQP;
+      // Geometry of every (element, quadrature point) pair of the chunk,
+      // copied from the per-assembly table (10 values each).
+      for (int p = lane; p < nce * NQP * 10; p += 64) {
+        int eq = p / 10, k = p - 10 * eq;
+        int e = eq / NQP, q = eq - e * NQP;
         int elem = n2e[es + c0 + e] >> 5;
-        QP1D<ORD> qp;
-        qp.eval(q);
-        double J[9];
-        double det = elem_geometry<ORD>(qp, &conn[(size_t)elem * NPE], X, J);
-#pragma unroll
-        for (int k = 0; k < 9; k++) G[p][k] = J[k];
-        G[p][9] = qp.w * det * telem[elem];
+        G[eq][k] = Gq[((size_t)elem * NQP + q) * 10 + k];
       }
       WAVE_SYNC();
       // (element, column node) pairs: accumulate the 3x3 block over qp.
@@ -265,7 +292,7 @@ __global__ __launch_bounds__(64 * JacCfg<ORD>::WPB) void solid_jacobian_rows(
             grad_x(J, ga0, ga);
             grad_x(J, gb0, gb);
             double f = alpha * wdt;
-            if (mat.isotropic) {
+            if (ISO) {
               double gab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
 #pragma unroll
               for (int j = 0; j < 3; j++) {
@@ -850,26 +877,35 @@ __global__ void apply_bc_vec(int b, int nbc, const int* __restrict__ nodes,
 // ---------------------------------------------------------------------------
 extern "C" {
 
-int tacs_hip_solid_jacobian(int order, int nnodes, const int* n2e_ptr, const int* n2e,
-                            const int* conn, const double* X, const double* telem,
-                            const SolidMaterial* mat, double alpha, double gamma,
-                            const int* rowp, const int* cols, const int* bcvars,
-                            double* A, float* A32, int* status, hipStream_t stream) {
-  if (order == 2) {
-    constexpr int WPB = JacCfg<2>::WPB;
-    int grid = tacs_grid_for(nnodes, WPB, 1 << 20);
-    hipLaunchKernelGGL(solid_jacobian_rows<2>, dim3(grid), dim3(64 * WPB), 0, stream, nnodes,
-                       n2e_ptr, n2e, conn, X, telem, *mat, alpha, gamma, rowp, cols, bcvars, A,
-                       A32, status);
-  } else if (order == 3) {
-    constexpr int WPB = JacCfg<3>::WPB;
-    int grid = tacs_grid_for(nnodes, WPB, 1 << 20);
-    hipLaunchKernelGGL(solid_jacobian_rows<3>, dim3(grid), dim3(64 * WPB), 0, stream, nnodes,
-                       n2e_ptr, n2e, conn, X, telem, *mat, alpha, gamma, rowp, cols, bcvars, A,
-                       A32, status);
-  } else {
-    return TACS_HIP_ERR_ARG;
+int tacs_hip_solid_jacobian(int order, int nnodes, int nelem, const int* n2e_ptr,
+                            const int* n2e, const int* conn, const double* X,
+                            const double* telem, const SolidMaterial* mat, double alpha,
+                            double gamma, const int* rowp, const int* cols, const int* bcvars,
+                            double* G, double* A, float* A32, int* status, hipStream_t stream) {
+  if (order != 2 && order != 3) return TACS_HIP_ERR_ARG;
+  const int nqp = order * order * order;
+  const size_t nq = (size_t)nelem * nqp;
+  if (nq > 0) {
+    if (order == 2)
+      hipLaunchKernelGGL(solid_qp_geometry<2>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
+                         stream, nelem, conn, X, telem, G);
+    else
+      hipLaunchKernelGGL(solid_qp_geometry<3>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
+                         stream, nelem, conn, X, telem, G);
   }
+#define JAC_LAUNCH(O, I)                                                                        \
+  hipLaunchKernelGGL((solid_jacobian_rows<O, I>), dim3(tacs_grid_for(nnodes, JacCfg<O>::WPB, 1 << 20)), \
+                     dim3(64 * JacCfg<O>::WPB), 0, stream, nnodes, n2e_ptr, n2e, conn, G, *mat,  \
+                     alpha, gamma, rowp, cols, bcvars, A, A32, status)
+  if (order == 2 && mat->isotropic)
+    JAC_LAUNCH(2, true);
+  else if (order == 2)
+    JAC_LAUNCH(2, false);
+  else if (mat->isotropic)
+    JAC_LAUNCH(3, true);
+  else
+    JAC_LAUNCH(3, false);
+#undef JAC_LAUNCH
   return (int)hipGetLastError();
 }
 
