@@ -743,6 +743,101 @@ def multi_main(args, L, ev):
     print(json.dumps(out))
 
 
+def matfree_main(args, L, ev):
+    """Matrix-free Jacobian products (TACSMatrixFreeMat: per quadrature point
+    J^-1 and w det t, element products gathered per node) against the FP64
+    product over the assembled sliced ELL copy, on the same cube (hex27 40^3
+    by default, --order 2 for hex8), and FP64 GMRES(m_fp64) + MGS solves with
+    either operator and the same block-Jacobi preconditioner.  A step is one
+    operator product; prints its own JSON line."""
+    from tacs_amd import TACS
+    from tacs_amd.problems import SolidProblem
+    order = args.order
+    n = args.n if not (order == 3 and args.n == 128) else 40
+    p = SolidProblem(n, order=order, dv="element")
+    m = p.mesh
+    log(f"matfree: order {order}, {n}^3, {m.nelem} elements, {3 * m.nnodes} DOF")
+    p.assemble()
+    sp = time_spmv(L, ev, p)
+    a = p.assembler
+    mf = TACS.MatrixFreeMat(a)
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0)
+    mf.assembleMatrixFreeData(TACS.JACOBIAN_MATRIX, 1.0, 0.0, 0.0)
+    ev.record(e1)
+    data_ms = ev.elapsed_ms(e0, e1)
+    x, y, y2 = a.createVec(), a.createVec(), a.createVec()
+    rng = np.random.default_rng(0)
+    xv = rng.standard_normal(x.getSize()).reshape(-1, 3)
+    xv[p.bc_nodes] = 0.0
+    x.getArray()[:] = xv.reshape(-1)
+    xd, yd = x.getDevicePointer(), y.getDevicePointer()
+    for _ in range(args.warmup + 4):
+        assert L.tacs_matfree_mult_device(mf.ptr, xd, yd) == 0
+    e0, e1 = ev.event(), ev.event()
+    reps = 50
+    ev.record(e0)
+    for _ in range(reps):
+        L.tacs_matfree_mult_device(mf.ptr, xd, yd)
+    ev.record(e1)
+    mf_ms = ev.elapsed_ms(e0, e1) / reps
+    # parity with the assembled operator (Dirichlet rows of the product zeroed)
+    mf.mult(x, y)
+    p.mat.mult(x, y2)
+    ym = y2.getArray().reshape(-1, 3).copy()
+    ym[p.bc_nodes] = 0.0
+    par = float(np.abs(y.getArray() - ym.reshape(-1)).max() / np.abs(ym).max())
+    npe = order ** 3
+    N = 3 * m.nnodes
+    table = 8 * (4 + 10 * npe * m.nelem)
+    alg = table + 4 * npe * m.nelem + 2 * 8 * N
+    ev_bytes = 2 * 8 * 3 * npe * m.nelem + 4 * npe * m.nelem + 4 * (m.nnodes + 1)
+    out = {
+        "metric": "matrix-free Jacobian product time vs the assembled FP64 product",
+        "value": mf_ms, "unit": "ms per product", "n_gpus": 1, "steps": reps, "warmup": 5,
+        "ms_per_step": mf_ms, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic structured mesh (no external data)",
+        "config": {"workload": f"{'hex27' if order == 3 else 'hex8'} cube {n}^3, E=70e9 nu=0.3, "
+                               f"x=0 clamped; TACSMatrixFreeMat (TACS_JACOBIAN_MATRIX) vs the "
+                               f"assembled 3x3 BCSR product",
+                   "elements": m.nelem, "dof": N},
+        "matfree_ms": mf_ms, "assembled_fp64_ms": sp["fp64_ms"],
+        "speedup_vs_assembled_fp64": sp["fp64_ms"] / mf_ms,
+        "assembled_fp32_values_fp64_vectors_ms": sp["mixed_ms"],
+        "matfree_data_ms": data_ms, "matfree_data_bytes": table,
+        "assembled_fp64_bytes": sp["fp64_bytes"],
+        "roofline": {"bound": "hbm", "achieved": alg / mf_ms / 1e6, "peak": 8000.0,
+                     "unit": "GB/s", "frac": alg / mf_ms / 1e6 / 8000.0, "traffic": None,
+                     "kernel": f"solid_matfree_ev<{order}> + gather_elem_vectors<{3 * npe}>",
+                     "algorithmic_bytes": alg,
+                     "note": "table + connectivity + x + y; the element-vector round trip "
+                             f"adds {ev_bytes} B"},
+        "max_rel_diff_vs_assembled": par,
+    }
+    if not args.no_fp64:
+        pc = TACS.Pc(p.mat)
+        pc.factor()
+        res = {}
+        for key, op in (("assembled", p.mat), ("matfree", mf)):
+            ksm = TACS.KSM(op, pc, args.m_fp64, 100000)
+            ksm.setTolerances(args.rtol, 1e-30)
+            u = a.createVec()
+            L.tacs_device_synchronize()
+            t0 = time.perf_counter()
+            flag = ksm.solve(p.res, u)
+            L.tacs_device_synchronize()
+            res[key] = (time.perf_counter() - t0, ksm.getIterCount(), flag, u.getArray().copy())
+            log(f"{key} GMRES: {res[key][0]:.2f}s its={res[key][1]} flag={flag}")
+        ua, uf = res["assembled"][3], res["matfree"][3]
+        out.update(gmres_assembled_s=res["assembled"][0], gmres_matfree_s=res["matfree"][0],
+                   gmres_iterations=[res["assembled"][1], res["matfree"][1]],
+                   gmres_flags=[res["assembled"][2], res["matfree"][2]],
+                   gmres_speedup=res["assembled"][0] / res["matfree"][0],
+                   u_rel_diff=float(np.linalg.norm(uf - ua) / np.linalg.norm(ua)),
+                   gmres=f"FP64 GMRES({args.m_fp64}) + MGS + block-Jacobi to {args.rtol:g}")
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -763,10 +858,12 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="assemble and time the SpMV kernels only (for profiler runs)")
     ap.add_argument("--workload", default="hex",
-                    choices=["hex", "shell", "batched", "multi", "simp"],
+                    choices=["hex", "shell", "batched", "multi", "simp", "matfree"],
                     help="hex: the 128^3 headline line; shell: the Quad4 plate; batched: the "
                          "hex27 cube with a block of load cases (own JSON lines)")
     ap.add_argument("--nrhs", type=int, default=16, help="load cases of the batched workload")
+    ap.add_argument("--order", type=int, default=3, choices=[2, 3],
+                    help="matfree workload: 3 (hex27, 40^3 unless --n) or 2 (hex8)")
     ap.add_argument("--ncubes", type=int, default=32, help="cubes of the multi workload")
     ap.add_argument("--seed", type=int, default=11, help="seed of the multi workload's sizes")
     ap.add_argument("--fd", type=int, default=0,
@@ -802,6 +899,9 @@ def main():
         return
     if args.workload == "simp":
         simp_main(args, L, ev)
+        return
+    if args.workload == "matfree":
+        matfree_main(args, L, ev)
         return
     t0 = time.perf_counter()
     p = SolidProblem(args.n, dv="element")

@@ -30,6 +30,14 @@ typedef struct tacs_comm_opaque* TACSComm;
 enum MatrixOrientation { TACS_MAT_NORMAL, TACS_MAT_TRANSPOSE };
 enum ElementStrainType { TACS_LINEAR_STRAIN, TACS_NONLINEAR_STRAIN };
 enum OrthoType { CLASSICAL_GRAM_SCHMIDT, MODIFIED_GRAM_SCHMIDT };
+// src/elements/TACSElementTypes.h:113
+enum ElementMatrixType {
+  TACS_JACOBIAN_MATRIX,
+  TACS_STIFFNESS_MATRIX,
+  TACS_MASS_MATRIX,
+  TACS_GEOMETRIC_STIFFNESS_MATRIX,
+  TACS_STIFFNESS_PRODUCT_DERIVATIVE
+};
 
 // src/TACSObject.h:112-140
 class TACSObject {
@@ -391,6 +399,8 @@ class TACSKsm : public TACSObject {
   KSMPrint* monitor = NULL;
 };
 
+class TACSMatrixFreeMat;
+
 class GMRES : public TACSKsm {
  public:
   GMRES(TACSMat* mat, TACSPc* pc, int m, int nrestart, int isFlexible = 0);
@@ -400,7 +410,11 @@ class GMRES : public TACSKsm {
   void setOrthoType(enum OrthoType otype);
   double rtol, atol;
   int msub, nrestart, ortho;
+  // the operator: an assembled matrix or a matrix-free one (the other NULL)
   TACSParallelMat* mat;
+  TACSMatrixFreeMat* mfree;
+  size_t n;
+  void opMult(const double* x, double* y);
   TACSBlockJacobiPc* pc;
   double* d_W;      // (msub+1) basis vectors, contiguous
   double* d_work;
@@ -634,6 +648,22 @@ class TACSAssembler : public TACSObject {
   void addAdjointResXptSensProducts(TacsScalar scale, int numAdjoints, TACSBVec** adjoint,
                                     TACSBVec** dfdXpt, const TacsScalar lambda = 1.0);
 
+  // Matrix-free products (src/TACSAssembler.cpp:5502-5717).  data and temp
+  // are device arrays of getMatrixFreeDataSize's sizes (doubles): data holds
+  // per quadrature point the inverse Jacobian and w det(J) t behind the
+  // coefficients of the stiffness and mass terms; temp receives the element
+  // products before their node sums.
+  void getMatrixFreeDataSize(ElementMatrixType matType, int* data_size, int* temp_size);
+  void assembleMatrixFreeData(ElementMatrixType matType, TacsScalar alpha, TacsScalar beta,
+                              TacsScalar gamma, TacsScalar data[]);
+  void addMatrixFreeVecProduct(ElementMatrixType matType, const TacsScalar data[],
+                               TacsScalar temp[], TACSBVec* x, TACSBVec* y,
+                               MatrixOrientation matOr = TACS_MAT_NORMAL,
+                               const TacsScalar lambda = 1.0, const bool applyBCs = true);
+  // y (+)= the product on device vectors; zeroes the Dirichlet rows of y when bcs
+  int matFreeProduct(const double* data, double* temp, const double* x, double* y,
+                     int accumulate, int bcs);
+
   // Device view of the element-level state used by the kernels
   const double* devNodes() { return d_X; }
   const double* devVars() { return d_u; }
@@ -681,6 +711,34 @@ class TACSAssembler : public TACSObject {
   int *h_rowp, *h_cols;
   long nnzb;
   double jac_ms;
+};
+
+// src/TACSMatrixFreeMat.h: the Jacobian (or stiffness / mass matrix) of the
+// assembler's solid elements applied element by element from quadrature-
+// point data, without an assembled matrix.  mult gives K x with the
+// Dirichlet rows of the result zeroed (TACSAssembler::
+// addMatrixFreeVecProduct with applyBCs).  Shell elements contribute
+// nothing, as TACSElement's default getMatVecProductData.
+class TACSMatrixFreeMat : public TACSMat {
+ public:
+  TACSMatrixFreeMat(TACSAssembler* assembler);
+  ~TACSMatrixFreeMat();
+  void assembleMatrixFreeData(ElementMatrixType matType, double alpha, double beta,
+                              double gamma);
+  TACSVec* createVec();
+  void mult(TACSVec* x, TACSVec* y);
+  void multTranspose(TACSVec* x, TACSVec* y);
+  void zeroEntries() {}
+  void applyBCs(TACSBcMap* bcmap) {}
+  const char* getObjectName() { return "TACSMatrixFreeMat"; }
+  // y = K x on device vectors of length length()
+  void mult64(const double* x, double* y);
+  size_t length();
+  TACSAssembler* assembler;
+  ElementMatrixType matType;
+  int data_size, temp_size;
+  double* data;
+  double* temp;
 };
 
 #endif

@@ -79,6 +79,25 @@ int tacs_hip_solid_jacobian(int order, int nnodes, int nelem, const int* n2e_ptr
                             double gamma, const int* rowp, const int* cols, const int* bcvars,
                             double* G, double* A, float* A32, int* status, hipStream_t stream);
 
+/* Matrix-free Jacobian data (replaces TACSAssembler::assembleMatrixFreeData
+   src/TACSAssembler.cpp:5563 over TACSElement3D::getMatVecProductData
+   src/elements/TACSElement3D.cpp:379): data[0..3] = [kc, mc, 0, 0], then per
+   (element, quadrature point) J^-1 (9 values) and w det(J) t_e; in all
+   4 + 10 * order^3 * nelem doubles.  kc multiplies the stiffness, mc the
+   consistent mass (alpha and gamma for TACS_JACOBIAN_MATRIX). */
+int tacs_hip_solid_matfree_data(int order, int nelem, const int* conn, const double* X,
+                                const double* telem, double kc, double mc, double* data,
+                                hipStream_t stream);
+
+/* Element part of the matrix-free product (TACSAssembler::
+   addMatrixFreeVecProduct src/TACSAssembler.cpp:5645 over
+   TACSElementBasis::addMatVecProduct basis/TACSElementBasis.cpp:940):
+   ev[nelem * 3*npe] = per-element products of the data above with x; the
+   node sums follow with tacs_hip_gather_elem_vectors. */
+int tacs_hip_solid_matfree_ev(int order, int nelem, const int* conn, const double* data,
+                              const SolidMaterial* mat, const double* x, double* ev,
+                              hipStream_t stream);
+
 /* Element vectors ev[nelem * 3*npe]:
    mode 0 residual (TACSElement3D::addResidual src/elements/TACSElement3D.cpp:139),
    mode 1 compliance d/du (TACSCompliance::getElementSVSens
@@ -521,6 +540,33 @@ int tacs_vec_set_rand(tacs_obj v, double lower, double upper);
 
 /* TACSParallelMat (TACS.pyx Mat :960) */
 int tacs_mat_mult(tacs_obj m, tacs_obj x, tacs_obj y);
+
+/* Matrix-free Jacobian products.
+   tacs_matfree_create: TACSMatrixFreeMat(assembler) src/TACSMatrixFreeMat.cpp:17;
+   tacs_matfree_assemble_data: TACSMatrixFreeMat::assembleMatrixFreeData
+     src/TACSMatrixFreeMat.cpp:37 (matType: ElementMatrixType of
+     src/elements/TACSElementTypes.h:113; JACOBIAN, STIFFNESS and MASS);
+   tacs_mat_mult accepts the matrix-free matrix too (TACSMatrixFreeMat::mult
+     :55, Dirichlet rows of y zeroed); tacs_matfree_mult_transpose :66;
+   tacs_matfree_mult_device: the same product on device arrays;
+   tacs_assembler_get_matrix_free_data_size / _assemble_matrix_free_data /
+   _add_matrix_free_vec_product: TACSAssembler::getMatrixFreeDataSize,
+     assembleMatrixFreeData, addMatrixFreeVecProduct src/TACSAssembler.cpp:
+     5502, 5563, 5645 with device arrays data and temp of the returned sizes
+     (doubles).  A TACSMatrixFreeMat is also a GMRES operator
+     (tacs_gmres_create). */
+tacs_obj tacs_matfree_create(tacs_obj assembler);
+int tacs_matfree_assemble_data(tacs_obj m, int matType, double alpha, double beta,
+                               double gamma);
+int tacs_matfree_mult_transpose(tacs_obj m, tacs_obj x, tacs_obj y);
+int tacs_matfree_mult_device(tacs_obj m, const double* x, double* y);
+int tacs_assembler_get_matrix_free_data_size(tacs_obj a, int matType, int* data_size,
+                                             int* temp_size);
+int tacs_assembler_assemble_matrix_free_data(tacs_obj a, int matType, double alpha, double beta,
+                                             double gamma, double* data);
+int tacs_assembler_add_matrix_free_vec_product(tacs_obj a, int matType, const double* data,
+                                               double* temp, tacs_obj x, tacs_obj y,
+                                               int matOr, double lambda, int applyBCs);
 /* y = A x on device vectors in the precisions the solvers use (values
    prec_vals, vectors prec_vecs: 64/64, 32/64 or 32/32), as
    TACSParallelMat::mult src/bpmat/TACSParallelMat.cpp:248 with FP64

@@ -92,6 +92,26 @@ def solid_element(order, X, u, C21, t=1.0, rho=0.0, alpha=1.0, gamma=0.0,
     return res, mat
 
 
+def solid_matvec_element(order, Xe, xe, C21, t=1.0, rho=0.0, kc=1.0, mc=0.0):
+    """Element matrix-free product (TACSElement3D::addMatVecProduct)."""
+    c = ctypes.c_double
+    y = np.zeros(3 * order ** 3)
+    lib().orc_solid_matvec_element(order, _p(_dd(Xe)), _p(_dd(xe)), _p(_dd(C21)), c(t), c(rho),
+                                   c(kc), c(mc), _p(y))
+    return y
+
+
+def matfree_product(order, conn, X, t_elem, C21, x, rho=0.0, kc=1.0, mc=0.0):
+    """sum_e P_e^T K_e P_e x (TACSAssembler::addMatrixFreeVecProduct without BCs)."""
+    c = ctypes.c_double
+    conn = _ii(conn)
+    x = _dd(x)
+    y = np.zeros_like(x)
+    lib().orc_matfree_product(order, conn.shape[0], _p(conn, np.int32), _p(_dd(X)),
+                              _p(_dd(t_elem)), _p(_dd(C21)), c(rho), c(kc), c(mc), _p(x), _p(y))
+    return y
+
+
 def node_csr(nnodes, conn, npe):
     conn = _ii(conn).reshape(-1)
     nelem = conn.size // npe

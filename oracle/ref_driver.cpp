@@ -37,6 +37,38 @@ void ref_solid_element(int order, const double* X, const double* u, double E, do
   elem->decref();
 }
 
+// Matrix-free product of one 8- or 27-node solid element
+// (TACSElement3D::getMatVecProductData + addMatVecProduct) for matType
+// 0 (Jacobian), 1 (stiffness) or 2 (mass) with coefficients alpha, gamma.
+void ref_solid_matvec(int order, int matType, const double* X, const double* x, double E,
+                      double nu, double rho, double t, double alpha, double gamma, double* y) {
+  TACSMaterialProperties* props = new TACSMaterialProperties(rho, 921.0, E, nu, 270e6, 0.0, 0.0);
+  TACSSolidConstitutive* con = new TACSSolidConstitutive(props, t, 0);
+  TACSLinearElasticity3D* model = new TACSLinearElasticity3D(con, TACS_LINEAR_STRAIN);
+  TACSElementBasis* basis;
+  if (order == 2)
+    basis = new TACSLinearHexaBasis();
+  else
+    basis = new TACSQuadraticHexaBasis();
+  TACSElement3D* elem = new TACSElement3D(model, basis);
+  elem->incref();
+  ElementMatrixType mt = matType == 0 ? TACS_JACOBIAN_MATRIX
+                                      : (matType == 1 ? TACS_STIFFNESS_MATRIX : TACS_MASS_MATRIX);
+  int nv = elem->getNumVariables();
+  int dsize = 0, tsize = 0;
+  elem->getMatVecDataSizes(mt, 0, &dsize, &tsize);
+  double* data = new double[dsize]();
+  double* temp = new double[tsize]();
+  double* zero = new double[nv]();
+  elem->getMatVecProductData(mt, 0, 0.0, alpha, 0.0, gamma, X, zero, zero, zero, data);
+  for (int i = 0; i < nv; i++) y[i] = 0.0;
+  elem->addMatVecProduct(mt, 0, data, temp, x, y);
+  delete[] data;
+  delete[] temp;
+  delete[] zero;
+  elem->decref();
+}
+
 static TACSElement3D* make_solid(int order, double E, double nu, double t) {
   TACSMaterialProperties* props = new TACSMaterialProperties(2700.0, 921.0, E, nu, 270e6, 0.0, 0.0);
   TACSSolidConstitutive* con = new TACSSolidConstitutive(props, t, 0);

@@ -13,6 +13,9 @@
       exact integration of polynomials, literal Gauss constants
       src/elements/basis/TACSGaussQuadrature.h:26,29).
     - orc_iso_C21: closed form of isotropic Hooke's law; Voigt round trip.
+    - orc_solid_matvec_element / orc_matfree_product: the reference's own
+      TACSElement3D matrix-free product (oracle/_ref) and K x from the
+      assembled element matrices of orc_solid_element (mass and stiffness).
     - orc_solid_element: symmetric Ke, rigid-body null space, patch test,
       residual = Ke*u, finite differences of the residual, golden compliance
       of tests/integration_tests/test_elast_linhexa_element_3d.py:11-27.
@@ -275,6 +278,91 @@ void orc_solid_element(int order, const double *X, const double *u,
           }
         }
       }
+    }
+  }
+}
+
+/*
+  Matrix-free element product y += K_e x_e with K_e = kc * stiffness +
+  mc * consistent mass, restating TACSElement3D::getMatVecProductData
+  (src/elements/TACSElement3D.cpp:379-425: per quadrature point J = Xd^-1 and
+  the weak-form Jacobian entries of TACSLinearElasticity3D::evalWeakMatrix
+  scaled by TACSElementBasis::scaleWeakMatrix, basis/TACSElementBasis.cpp:840)
+  and TACSElementBasis::addMatVecProduct (basis/TACSElementBasis.cpp:940-
+  1050): interpolate U and dU/dxi of x, map the gradient with J^T, apply the
+  entries (stress t C e, mass rho t U), map DUx back with J and add the
+  transposed interpolation.
+*/
+void orc_solid_matvec_element(int order, const double *X, const double *x,
+                              const double C21[21], double t, double rho,
+                              double kc, double mc, double *y) {
+  const int npe = order * order * order;
+  double D[36];
+  unpack21(C21, D);
+  for (int n = 0; n < npe; n++) {
+    double pt[3], N[MAXN], Nxi[3 * MAXN], Nx[3 * MAXN], J[9];
+    double w = orc_hex_quadrature(order, n, pt);
+    orc_hex_basis(order, pt, N, Nxi);
+    double det = qp_geometry(npe, Nxi, X, Nx, J);
+    double U[3] = {0, 0, 0}, Ud[9] = {0}, Ux[9];
+    for (int a = 0; a < npe; a++) {
+      for (int c = 0; c < 3; c++) {
+        U[c] += N[a] * x[3 * a + c];
+        for (int k = 0; k < 3; k++) Ud[3 * c + k] += x[3 * a + c] * Nxi[3 * a + k];
+      }
+    }
+    /* mat3x3MultTrans(J, Ud): Ux[c][d] = sum_k J[3k + d] Ud[c][k] */
+    for (int c = 0; c < 3; c++) {
+      for (int d = 0; d < 3; d++) {
+        Ux[3 * c + d] = J[d] * Ud[3 * c] + J[3 + d] * Ud[3 * c + 1] + J[6 + d] * Ud[3 * c + 2];
+      }
+    }
+    double e[6], sg[6];
+    strain(Ux, e);
+    matvec6(D, e, sg);
+    const double wd = w * det;
+    double DUx[9], DU[3], ud[9];
+    for (int c = 0; c < 3; c++) {
+      DU[c] = mc * wd * rho * t * U[c];
+      for (int d = 0; d < 3; d++) DUx[3 * c + d] = kc * wd * t * sg[voigt(c, d)];
+    }
+    /* mat3x3Mult(J, DUx): ud[c][k] = sum_d J[3k + d] DUx[c][d] */
+    for (int c = 0; c < 3; c++) {
+      for (int k = 0; k < 3; k++) {
+        ud[3 * c + k] = J[3 * k] * DUx[3 * c] + J[3 * k + 1] * DUx[3 * c + 1] +
+                        J[3 * k + 2] * DUx[3 * c + 2];
+      }
+    }
+    for (int a = 0; a < npe; a++) {
+      for (int c = 0; c < 3; c++) {
+        double v = N[a] * DU[c];
+        for (int k = 0; k < 3; k++) v += Nxi[3 * a + k] * ud[3 * c + k];
+        y[3 * a + c] += v;
+      }
+    }
+  }
+}
+
+/* Mesh-level matrix-free product y += sum_e P_e^T K_e P_e x, element by
+   element in ascending order (TACSAssembler::addMatrixFreeVecProduct
+   src/TACSAssembler.cpp:5645-5717, before its applyBCs). */
+void orc_matfree_product(int order, int nelem, const int *conn, const double *X,
+                         const double *t_elem, const double C21[21], double rho,
+                         double kc, double mc, const double *x, double *y) {
+  const int npe = order * order * order;
+  double Xe[3 * MAXN], xe[3 * MAXN], ye[3 * MAXN];
+  for (int e = 0; e < nelem; e++) {
+    const int *c = &conn[npe * e];
+    for (int a = 0; a < npe; a++) {
+      for (int k = 0; k < 3; k++) {
+        Xe[3 * a + k] = X[3 * c[a] + k];
+        xe[3 * a + k] = x[3 * c[a] + k];
+        ye[3 * a + k] = 0.0;
+      }
+    }
+    orc_solid_matvec_element(order, Xe, xe, C21, t_elem[e], rho, kc, mc, ye);
+    for (int a = 0; a < npe; a++) {
+      for (int k = 0; k < 3; k++) y[3 * c[a] + k] += ye[3 * a + k];
     }
   }
 }

@@ -19,6 +19,11 @@ NORMAL = 0
 TRANSPOSE = 1
 CLASSICAL_GRAM_SCHMIDT = 0
 MODIFIED_GRAM_SCHMIDT = 1
+# ElementMatrixType (src/elements/TACSElementTypes.h:113, tacs/TACS.pyx:55-57)
+JACOBIAN_MATRIX = 0
+STIFFNESS_MATRIX = 1
+MASS_MATRIX = 2
+GEOMETRIC_STIFFNESS_MATRIX = 3
 
 
 def _ptr_array(objs):
@@ -116,6 +121,32 @@ class Mat(TACSHandle):
         lib().tacs_mat_device_arrays(self.ptr, ctypes.byref(rp), ctypes.byref(cp),
                                      ctypes.byref(ap))
         return rp.value, cp.value, ap.value
+
+
+class MatrixFreeMat(TACSHandle):
+    """TACSMatrixFreeMat (src/TACSMatrixFreeMat.h): the solid elements'
+    Jacobian (or stiffness / mass matrix) applied element by element from
+    quadrature-point data, with no assembled matrix.  ``mult`` returns K x
+    with the Dirichlet rows zeroed; it is also a KSM operator."""
+
+    def __init__(self, assembler):
+        self.assembler = assembler
+        super().__init__(lib().tacs_matfree_create(assembler.ptr), keep=(assembler,))
+
+    def assembleMatrixFreeData(self, matType=JACOBIAN_MATRIX, alpha=1.0, beta=0.0, gamma=0.0):
+        check(lib().tacs_matfree_assemble_data(self.ptr, int(matType), alpha, beta, gamma),
+              "MatrixFreeMat.assembleMatrixFreeData")
+
+    def mult(self, x, y):
+        check(lib().tacs_mat_mult(self.ptr, x.ptr, y.ptr), "MatrixFreeMat.mult")
+
+    def multTranspose(self, x, y):
+        check(lib().tacs_matfree_mult_transpose(self.ptr, x.ptr, y.ptr),
+              "MatrixFreeMat.multTranspose")
+
+    def multDevice(self, x_ptr, y_ptr):
+        """y = K x on device arrays (raw pointers) of the assembler's length."""
+        check(lib().tacs_matfree_mult_device(self.ptr, x_ptr, y_ptr), "MatrixFreeMat.multDevice")
 
 
 class Pc(TACSHandle):
@@ -419,6 +450,27 @@ class Assembler(TACSHandle):
         check(lib().tacs_assembler_assemble_jacobian(
             self.ptr, alpha, beta, gamma, residual.ptr if residual is not None else None, A.ptr,
             matOr, loadScale, 1 if applyBCs else 0), "assembleJacobian")
+
+    def getMatrixFreeDataSize(self, matType=JACOBIAN_MATRIX):
+        """(data_size, temp_size) in doubles (TACSAssembler::getMatrixFreeDataSize)."""
+        d, t = ctypes.c_int(), ctypes.c_int()
+        check(lib().tacs_assembler_get_matrix_free_data_size(self.ptr, int(matType),
+                                                             ctypes.byref(d), ctypes.byref(t)),
+              "getMatrixFreeDataSize")
+        return d.value, t.value
+
+    def assembleMatrixFreeData(self, matType, alpha, beta, gamma, data_ptr):
+        """Fill the device array at data_ptr (TACSAssembler::assembleMatrixFreeData)."""
+        check(lib().tacs_assembler_assemble_matrix_free_data(self.ptr, int(matType), alpha, beta,
+                                                             gamma, data_ptr),
+              "assembleMatrixFreeData")
+
+    def addMatrixFreeVecProduct(self, matType, data_ptr, temp_ptr, x, y, matOr=NORMAL,
+                                loadScale=1.0, applyBCs=True):
+        """y += K x from the device data (TACSAssembler::addMatrixFreeVecProduct)."""
+        check(lib().tacs_assembler_add_matrix_free_vec_product(
+            self.ptr, int(matType), data_ptr, temp_ptr, x.ptr, y.ptr, matOr, loadScale,
+            1 if applyBCs else 0), "addMatrixFreeVecProduct")
 
     def evalFunctions(self, funclist):
         vals = np.zeros(len(funclist))

@@ -379,11 +379,61 @@ int tacs_vec_set_rand(tacs_obj v, double lo, double hi) {
 }
 
 int tacs_mat_mult(tacs_obj m, tacs_obj x, tacs_obj y) {
-  REQUIRE(TACSParallelMat, A, m);
+  REQUIRE(TACSMat, A, m);
   REQUIRE(TACSBVec, xv, x);
   REQUIRE(TACSBVec, yv, y);
   clear_error();
   A->mult(xv, yv);
+  return last_error_code();
+}
+tacs_obj tacs_matfree_create(tacs_obj assembler) {
+  REQUIRE_NULL(TACSAssembler, a, assembler);
+  return (TACSObject*)new TACSMatrixFreeMat(a);
+}
+int tacs_matfree_assemble_data(tacs_obj m, int matType, double alpha, double beta,
+                               double gamma) {
+  REQUIRE(TACSMatrixFreeMat, F, m);
+  clear_error();
+  F->assembleMatrixFreeData((ElementMatrixType)matType, alpha, beta, gamma);
+  return last_error_code();
+}
+int tacs_matfree_mult_transpose(tacs_obj m, tacs_obj x, tacs_obj y) {
+  REQUIRE(TACSMatrixFreeMat, F, m);
+  REQUIRE(TACSBVec, xv, x);
+  REQUIRE(TACSBVec, yv, y);
+  clear_error();
+  F->multTranspose(xv, yv);
+  return last_error_code();
+}
+int tacs_matfree_mult_device(tacs_obj m, const double* x, double* y) {
+  REQUIRE(TACSMatrixFreeMat, F, m);
+  clear_error();
+  F->mult64(x, y);
+  return last_error_code();
+}
+int tacs_assembler_get_matrix_free_data_size(tacs_obj h, int matType, int* data_size,
+                                             int* temp_size) {
+  REQUIRE(TACSAssembler, a, h);
+  clear_error();
+  a->getMatrixFreeDataSize((ElementMatrixType)matType, data_size, temp_size);
+  return last_error_code();
+}
+int tacs_assembler_assemble_matrix_free_data(tacs_obj h, int matType, double alpha, double beta,
+                                             double gamma, double* data) {
+  REQUIRE(TACSAssembler, a, h);
+  clear_error();
+  a->assembleMatrixFreeData((ElementMatrixType)matType, alpha, beta, gamma, data);
+  return last_error_code();
+}
+int tacs_assembler_add_matrix_free_vec_product(tacs_obj h, int matType, const double* data,
+                                               double* temp, tacs_obj x, tacs_obj y,
+                                               int matOr, double lambda, int applyBCs) {
+  REQUIRE(TACSAssembler, a, h);
+  REQUIRE(TACSBVec, xv, x);
+  REQUIRE(TACSBVec, yv, y);
+  clear_error();
+  a->addMatrixFreeVecProduct((ElementMatrixType)matType, data, temp, xv, yv,
+                             (MatrixOrientation)matOr, lambda, applyBCs != 0);
   return last_error_code();
 }
 int tacs_mat_mult_prec(tacs_obj m, int prec_vals, int prec_vecs, const void* x, void* y) {
@@ -472,13 +522,25 @@ int tacs_pc_apply(tacs_obj pc, tacs_obj x, tacs_obj y) {
   return 0;
 }
 tacs_obj tacs_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int flexible) {
-  REQUIRE_NULL(TACSParallelMat, A, mat);
+  // the operator: an assembled TACSParallelMat or a TACSMatrixFreeMat
+  TACSParallelMat* A = OBJ(TACSParallelMat, mat);
+  TACSMatrixFreeMat* F = A ? NULL : OBJ(TACSMatrixFreeMat, mat);
+  if (!A && !F) {
+    tacs_set_error("GMRES: expected a TACSParallelMat or TACSMatrixFreeMat operator");
+    return NULL;
+  }
   TACSPc* p = pc ? OBJ(TACSPc, pc) : NULL;
   if (m < 1 || m > 512) {
     tacs_set_error("GMRES: subspace size must be in [1, 512]");
     return NULL;
   }
-  return (TACSObject*)new GMRES(A, p, m, nrestart, flexible);
+  TACSBlockJacobiPc* bj = p ? dynamic_cast<TACSBlockJacobiPc*>(p) : NULL;
+  const size_t n = A ? (size_t)A->nrows * A->bsize : F->length();
+  if (bj && (size_t)bj->mat->nrows * bj->mat->bsize != n) {
+    tacs_set_error("GMRES: preconditioner size does not match the operator");
+    return NULL;
+  }
+  return (TACSObject*)new GMRES(A ? (TACSMat*)A : (TACSMat*)F, p, m, nrestart, flexible);
 }
 int tacs_gmres_set_ortho_type(tacs_obj ksm, int ortho) {
   REQUIRE(GMRES, g, ksm);

@@ -626,10 +626,12 @@ GMRES::GMRES(TACSMat* m, TACSPc* p, int _m, int _nrestart, int isFlexible)
     : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart >= 0 ? _nrestart : 0),
       ortho(MODIFIED_GRAM_SCHMIDT) {
   mat = dynamic_cast<TACSParallelMat*>(m);
+  mfree = mat ? NULL : dynamic_cast<TACSMatrixFreeMat*>(m);
   pc = p ? dynamic_cast<TACSBlockJacobiPc*>(p) : NULL;
-  mat->incref();
+  // the C ABI checks the operator type and the preconditioner's size
+  m->incref();
   if (pc) pc->incref();
-  size_t n = (size_t)mat->nrows * mat->bsize;
+  n = mat ? (size_t)mat->nrows * mat->bsize : mfree->length();
   d_W = dalloc<double>((size_t)(msub + 1) * n);
   d_work = dalloc<double>(n);
   GmresScal L(msub);
@@ -646,8 +648,17 @@ GMRES::~GMRES() {
   dfree(d_scal);
   dfree(d_red);
   HCHECK(hipHostFree(h_pin));
-  mat->decref();
+  if (mat)
+    mat->decref();
+  else
+    mfree->decref();
   if (pc) pc->decref();
+}
+void GMRES::opMult(const double* x, double* y) {
+  if (mat)
+    mat->mult64(x, y);
+  else
+    mfree->mult64(x, y);
 }
 void GMRES::setTolerances(double r, double a) {
   rtol = r;
@@ -687,8 +698,7 @@ static double pin_read(double* h_pin, const double* d) {
 int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
   TACSBVec* b = dynamic_cast<TACSBVec*>(tb);
   TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
-  const size_t n = (size_t)mat->nrows * mat->bsize;
-  const int bs = mat->bsize, nr = mat->nrows;
+  const int bs = pc ? pc->mat->bsize : 0, nr = pc ? pc->mat->nrows : 0;
   GmresScal L(msub);
   double* H = d_scal + L.H;
   double* res = d_scal + L.res;
@@ -710,7 +720,7 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
       KCHECK(tacs_hip_dot(64, n, NULL, NULL, W(0), W(0), d_red, res, 1, misc + MISC_INV, g_stream));
       KCHECK(tacs_hip_scale(64, n, W(0), misc + MISC_INV, 0.0, 0, g_stream));
     } else {
-      mat->mult64(xd, W(0));
+      opMult(xd, W(0));
       KCHECK(tacs_hip_axpby(64, 64, n, -1.0, bd, 1.0, W(0), g_stream));
       KCHECK(tacs_hip_dot(64, n, NULL, NULL, W(0), W(0), d_red, res, 1, misc + MISC_INV, g_stream));
       KCHECK(tacs_hip_scalar_op(misc + MISC_INV, -1.0, misc + MISC_NINV, g_stream));
@@ -731,9 +741,9 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
       double* q = W(i + 1);
       if (pc) {
         KCHECK(tacs_hip_bsr_diag_apply(bs, nr, pc->d_Dinv, W(i), d_work, g_stream));
-        mat->mult64(d_work, q);
+        opMult(d_work, q);
       } else {
-        mat->mult64(W(i), q);
+        opMult(W(i), q);
       }
       double* h = H + (size_t)i * (msub + 1);
       if (ortho == MODIFIED_GRAM_SCHMIDT) {
@@ -2675,6 +2685,159 @@ void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScal
 }
 
 // TACSAssembler::evalFunctions src/TACSAssembler.cpp:4617
+// ---------------------------------------------------------------------------
+// Matrix-free products, TACSAssembler.cpp:5502-5717 and
+// src/TACSMatrixFreeMat.cpp.  The reference stores per quadrature point the
+// inverse Jacobian and the 84 weak-form Jacobian entries of
+// TACSLinearElasticity3D (TACSElement3D.cpp:362-425); here the material
+// enters the kernel as (lambda, mu) or C[21], so a point keeps the inverse
+// Jacobian and w det(J) t (10 values), and the coefficients of the stiffness
+// and mass terms head the array.
+// ---------------------------------------------------------------------------
+static int matfree_coefs(ElementMatrixType matType, double alpha, double gamma, double* kc,
+                         double* mc) {
+  // TACSLinearElasticity3D::getWeakMatrixNonzeros (TACSLinearElasticity.cpp:1137)
+  // with TACSElementBasis::scaleWeakMatrix (basis/TACSElementBasis.cpp:840):
+  // stiffness pairs scale by alpha, acceleration pairs by gamma
+  switch (matType) {
+    case TACS_JACOBIAN_MATRIX:
+      *kc = alpha;
+      *mc = gamma;
+      return 0;
+    case TACS_STIFFNESS_MATRIX:
+      *kc = alpha;
+      *mc = 0.0;
+      return 0;
+    case TACS_MASS_MATRIX:
+      *kc = 0.0;
+      *mc = gamma;
+      return 0;
+    default:
+      return 1;
+  }
+}
+
+void TACSAssembler::getMatrixFreeDataSize(ElementMatrixType matType, int* ds, int* ts) {
+  long d = 0, t = 0;
+  if (family == FAMILY_SOLID && initialized) {
+    d = 4 + 10L * npe * numElements;
+    t = 3L * npe * numElements;
+  }
+  if (d > 2147483647L) {
+    tacs_set_error("getMatrixFreeDataSize: %ld values exceed the int size of the interface", d);
+    d = t = 0;
+  }
+  if (ds) *ds = (int)d;
+  if (ts) *ts = (int)t;
+}
+
+void TACSAssembler::assembleMatrixFreeData(ElementMatrixType matType, TacsScalar alpha,
+                                           TacsScalar beta, TacsScalar gamma, TacsScalar data[]) {
+  if (!initialized) {
+    tacs_set_error("assembleMatrixFreeData before initialize()");
+    return;
+  }
+  double kc, mc;
+  if (matfree_coefs(matType, alpha, gamma, &kc, &mc)) {
+    tacs_set_error("assembleMatrixFreeData: matrix type %d is not supported", (int)matType);
+    return;
+  }
+  if (family != FAMILY_SOLID) return;  // shells: no matrix-free data (TACSElement defaults)
+  if (!data || ((uintptr_t)data & 15) != 0) {
+    tacs_set_error("assembleMatrixFreeData: expected a 16-byte aligned device data array");
+    return;
+  }
+  KCHECK(tacs_hip_solid_matfree_data(order, numElements, d_conn, d_X, d_telem, kc, mc, data,
+                                     g_stream));
+}
+
+int TACSAssembler::matFreeProduct(const double* data, double* temp, const double* x, double* y,
+                                  int accumulate, int bcs) {
+  const size_t n = (size_t)numNodes * varsPerNode;
+  if (family == FAMILY_SOLID) {
+    if (!data || !temp) return tacs_set_error("matrix-free product: no data (assemble it first)");
+    if (((uintptr_t)data & 15) != 0)
+      return tacs_set_error("matrix-free product: the data array must be 16-byte aligned");
+    KCHECK(tacs_hip_solid_matfree_ev(order, numElements, d_conn, data, &mat, x, temp, g_stream));
+    KCHECK(tacs_hip_gather_elem_vectors(3 * npe, numNodes, d_n2e_ptr, d_n2e, temp, accumulate, y,
+                                        g_stream));
+  } else if (!accumulate) {
+    KCHECK(tacs_hip_fill(64, n, y, 0.0, g_stream));
+  }
+  if (bcs)
+    KCHECK(tacs_hip_apply_bc_vec(varsPerNode, nbc_nodes, d_bcnodes, d_bcmask, NULL, NULL, 1.0, y,
+                                 g_stream));
+  return 0;
+}
+
+void TACSAssembler::addMatrixFreeVecProduct(ElementMatrixType matType, const TacsScalar data[],
+                                            TacsScalar temp[], TACSBVec* x, TACSBVec* y,
+                                            MatrixOrientation matOr, const TacsScalar lambda,
+                                            const bool bcs) {
+  // element matrices of linear elasticity are symmetric: both orientations
+  // give the same product; aux elements (tractions, pressures) carry no
+  // matrix-free data, so lambda scales nothing
+  if (!initialized) {
+    tacs_set_error("addMatrixFreeVecProduct before initialize()");
+    return;
+  }
+  const double* xd = x->devRead();
+  double* yd = (x == y) ? NULL : y->devWrite();
+  if (!yd) {
+    tacs_set_error("addMatrixFreeVecProduct: x and y must differ");
+    return;
+  }
+  matFreeProduct(data, temp, xd, yd, 1, bcs ? 1 : 0);
+}
+
+TACSMatrixFreeMat::TACSMatrixFreeMat(TACSAssembler* a)
+    : assembler(a), matType(TACS_JACOBIAN_MATRIX), data_size(0), temp_size(0), data(NULL),
+      temp(NULL) {
+  assembler->incref();
+}
+TACSMatrixFreeMat::~TACSMatrixFreeMat() {
+  dfree(data);
+  dfree(temp);
+  assembler->decref();
+}
+// TACSMatrixFreeMat::assembleMatrixFreeData src/TACSMatrixFreeMat.cpp:37
+void TACSMatrixFreeMat::assembleMatrixFreeData(ElementMatrixType mt, double alpha, double beta,
+                                               double gamma) {
+  int ds = 0, ts = 0;
+  assembler->getMatrixFreeDataSize(mt, &ds, &ts);
+  if (ds != data_size) {
+    dfree(data);
+    data = ds > 0 ? dalloc<double>((size_t)ds) : NULL;
+    data_size = ds;
+  }
+  if (ts != temp_size) {
+    dfree(temp);
+    temp = ts > 0 ? dalloc<double>((size_t)ts) : NULL;
+    temp_size = ts;
+  }
+  matType = mt;
+  assembler->assembleMatrixFreeData(mt, alpha, beta, gamma, data);
+}
+TACSVec* TACSMatrixFreeMat::createVec() { return assembler->createVec(); }
+size_t TACSMatrixFreeMat::length() {
+  return (size_t)assembler->getNumNodes() * assembler->getVarsPerNode();
+}
+void TACSMatrixFreeMat::mult64(const double* x, double* y) {
+  assembler->matFreeProduct(data, temp, x, y, 0, 1);
+}
+// TACSMatrixFreeMat::mult src/TACSMatrixFreeMat.cpp:55: y = 0, then the
+// assembler's product with the Dirichlet rows zeroed
+void TACSMatrixFreeMat::mult(TACSVec* tx, TACSVec* ty) {
+  TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
+  TACSBVec* y = dynamic_cast<TACSBVec*>(ty);
+  if (!x || !y || x == y) {
+    tacs_set_error("TACSMatrixFreeMat::mult: expected two distinct TACSBVec");
+    return;
+  }
+  mult64(x->devRead(), y->devWrite());
+}
+void TACSMatrixFreeMat::multTranspose(TACSVec* x, TACSVec* y) { mult(x, y); }
+
 void TACSAssembler::evalFunctions(int numFuncs, TACSFunction** funcs, TacsScalar* vals) {
   ensure(d_es, numElements);
   for (int k = 0; k < numFuncs; k++) {

@@ -738,6 +738,160 @@ __global__ void solid_traction_vectors(int naux, const int* __restrict__ aux_ele
   }
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-free Jacobian-vector product, element part (TACSElement3D::
+// addMatVecProduct src/elements/TACSElement3D.cpp:428 through
+// TACSElementBasis::addMatVecProduct basis/TACSElementBasis.cpp:940):
+//   ev[e] = sum_q  Bq^T (kc w det t C) Bq x_e  +  mc rho w det t Nq^T Nq x_e
+// from the quadrature-point geometry table of solid_qp_geometry (J^-1 and
+// w det t, 10 values per point) behind a 4-value header [kc, mc, 0, 0].
+// A group of L lanes owns one element: lane q interpolates the displacement
+// gradient at quadrature point q and turns it into the stress, pulled back
+// to reference coordinates, in LDS; lane a then contracts it with the basis
+// gradients of node a.  No element matrix is formed.
+// ---------------------------------------------------------------------------
+template <int ORD>
+struct MFCfg;
+template <>
+struct MFCfg<2> {
+  static constexpr int L = 8;
+};
+template <>
+struct MFCfg<3> {
+  static constexpr int L = 32;
+};
+
+template <int ORD, bool ISO>
+__global__ __launch_bounds__(256) void solid_matfree_ev(int nelem, const int* __restrict__ conn,
+                                                        const double* __restrict__ data,
+                                                        SolidMaterial mat,
+                                                        const double* __restrict__ x,
+                                                        double* __restrict__ ev) {
+  constexpr int NPE = ORD * ORD * ORD, NQP = NPE;
+  constexpr int L = MFCfg<ORD>::L, EPB = 256 / L;
+  constexpr int UK = ORD == 2 ? 8 : 1;  // unrolling of the quadrature / node loops
+  __shared__ double xs[EPB][NPE * 3];
+  __shared__ double ss[EPB][NQP][12];
+  const int g = threadIdx.x / L;
+  const int l = threadIdx.x - g * L;
+  const double kc = data[0], mc = data[1];
+  const double* __restrict__ Gq = data + 4;
+  for (int e0 = blockIdx.x * EPB; e0 < nelem; e0 += gridDim.x * EPB) {
+    const int elem = e0 + g;
+    const bool ok = elem < nelem;
+    if (ok && l < NPE) {
+      const int n = conn[(size_t)elem * NPE + l];
+      xs[g][3 * l] = x[3 * (size_t)n];
+      xs[g][3 * l + 1] = x[3 * (size_t)n + 1];
+      xs[g][3 * l + 2] = x[3 * (size_t)n + 2];
+    }
+    WAVE_SYNC();
+    if (ok && l < NQP) {
+      QP1D<ORD> qp;
+      qp.eval(l);
+      // U and dU/dxi by sum factorisation over the tensor-product nodes:
+      // for each (ja, ka) line, t0 = sum_ia x n_ia(xi), t1 = sum_ia x n'_ia(xi)
+      double U[3] = {0, 0, 0}, Ud[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll UK
+      for (int ka = 0; ka < ORD; ka++) {
+        const double nz = pick<ORD>(qp.n[2], ka), dz = pick<ORD>(qp.d[2], ka);
+#pragma unroll
+        for (int ja = 0; ja < ORD; ja++) {
+          double t0[3] = {0, 0, 0}, t1[3] = {0, 0, 0};
+#pragma unroll
+          for (int ia = 0; ia < ORD; ia++) {
+            const double* xa = &xs[g][3 * (ia + ORD * (ja + ORD * ka))];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              t0[c] += qp.n[0][ia] * xa[c];
+              t1[c] += qp.d[0][ia] * xa[c];
+            }
+          }
+          const double w = qp.n[1][ja] * nz, wy = qp.d[1][ja] * nz, wz = qp.n[1][ja] * dz;
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            U[c] += w * t0[c];
+            Ud[3 * c] += w * t1[c];
+            Ud[3 * c + 1] += wy * t0[c];
+            Ud[3 * c + 2] += wz * t0[c];
+          }
+        }
+      }
+      const double2* G2 = reinterpret_cast<const double2*>(Gq + ((size_t)elem * NQP + l) * 10);
+      double J[10];
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        const double2 v = G2[k];
+        J[2 * k] = v.x;
+        J[2 * k + 1] = v.y;
+      }
+      const double wdt = J[9];
+      double Ux[9];
+#pragma unroll
+      for (int c = 0; c < 3; c++) grad_x(J, &Ud[3 * c], &Ux[3 * c]);
+      const double e[6] = {Ux[0], Ux[4], Ux[8], Ux[5] + Ux[7], Ux[2] + Ux[6], Ux[1] + Ux[3]};
+      double s[6];
+      if (ISO) {
+        const double tr = e[0] + e[1] + e[2];
+        s[0] = mat.lam * tr + 2.0 * mat.mu * e[0];
+        s[1] = mat.lam * tr + 2.0 * mat.mu * e[1];
+        s[2] = mat.lam * tr + 2.0 * mat.mu * e[2];
+        s[3] = mat.mu * e[3];
+        s[4] = mat.mu * e[4];
+        s[5] = mat.mu * e[5];
+      } else {
+        stress6(mat, e, s);
+      }
+      const double f = kc * wdt;
+      // physical stress tensor rows [s0 s5 s4; s5 s1 s3; s4 s3 s2], scaled
+      const double P[9] = {f * s[0], f * s[5], f * s[4], f * s[5], f * s[1],
+                           f * s[3], f * s[4], f * s[3], f * s[2]};
+      // reference-coordinate pull-back: S[c][k] = sum_d P[c][d] J[3k + d]
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+          ss[g][l][3 * c + k] =
+              P[3 * c] * J[3 * k] + P[3 * c + 1] * J[3 * k + 1] + P[3 * c + 2] * J[3 * k + 2];
+      }
+      const double m = mc * mat.rho * wdt;
+      ss[g][l][9] = m * U[0];
+      ss[g][l][10] = m * U[1];
+      ss[g][l][11] = m * U[2];
+    }
+    WAVE_SYNC();
+    if (ok && l < NPE) {
+      double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+#pragma unroll UK
+      for (int q = 0; q < NQP; q++) {
+        QP1D<ORD> qp;
+        qp.eval(q);
+        double N, g0[3];
+        qp.basis(l, N, g0);
+        const double* S = ss[g][q];
+        o0 += g0[0] * S[0] + g0[1] * S[1] + g0[2] * S[2] + N * S[9];
+        o1 += g0[0] * S[3] + g0[1] * S[4] + g0[2] * S[5] + N * S[10];
+        o2 += g0[0] * S[6] + g0[1] * S[7] + g0[2] * S[8] + N * S[11];
+      }
+      double* o = &ev[((size_t)elem * NPE + l) * 3];
+      o[0] = o0;
+      o[1] = o1;
+      o[2] = o2;
+    }
+    WAVE_SYNC();
+  }
+}
+
+// Header of the matrix-free data: [kc, mc, 0, 0].
+__global__ void matfree_header(double* data, double kc, double mc) {
+  if (threadIdx.x == 0) {
+    data[0] = kc;
+    data[1] = mc;
+    data[2] = 0.0;
+    data[3] = 0.0;
+  }
+}
+
 // Node gather of element vectors in ascending element order:
 // out[3*i + j] (+)= sum_{(e,a) in n2e(i)} ev[e*NV + 3a + j].
 template <int NV>
@@ -906,6 +1060,45 @@ int tacs_hip_solid_jacobian(int order, int nnodes, int nelem, const int* n2e_ptr
   else
     JAC_LAUNCH(3, false);
 #undef JAC_LAUNCH
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_solid_matfree_data(int order, int nelem, const int* conn, const double* X,
+                                const double* telem, double kc, double mc, double* data,
+                                hipStream_t stream) {
+  if (order != 2 && order != 3) return TACS_HIP_ERR_ARG;
+  hipLaunchKernelGGL(matfree_header, dim3(1), dim3(64), 0, stream, data, kc, mc);
+  const size_t nq = (size_t)nelem * order * order * order;
+  if (nq > 0) {
+    if (order == 2)
+      hipLaunchKernelGGL(solid_qp_geometry<2>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
+                         stream, nelem, conn, X, telem, data + 4);
+    else
+      hipLaunchKernelGGL(solid_qp_geometry<3>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
+                         stream, nelem, conn, X, telem, data + 4);
+  }
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_solid_matfree_ev(int order, int nelem, const int* conn, const double* data,
+                              const SolidMaterial* mat, const double* x, double* ev,
+                              hipStream_t stream) {
+  if (order != 2 && order != 3) return TACS_HIP_ERR_ARG;
+  if (nelem <= 0) return 0;
+  const int epb = 256 / (order == 2 ? MFCfg<2>::L : MFCfg<3>::L);
+  const int grid = tacs_grid_for(nelem, epb, 1 << 20);
+#define MF_LAUNCH(O, I)                                                                     \
+  hipLaunchKernelGGL((solid_matfree_ev<O, I>), dim3(grid), dim3(256), 0, stream, nelem, conn, \
+                     data, *mat, x, ev)
+  if (order == 2 && mat->isotropic)
+    MF_LAUNCH(2, true);
+  else if (order == 2)
+    MF_LAUNCH(2, false);
+  else if (mat->isotropic)
+    MF_LAUNCH(3, true);
+  else
+    MF_LAUNCH(3, false);
+#undef MF_LAUNCH
   return (int)hipGetLastError();
 }
 

@@ -103,6 +103,19 @@ def make(seed=2024):
             for face in range(6):
                 L.ref_solid_pressure(order, face, _d(X), c(pv), _d(pr[face]))
             out[key + "_p"], out[key + "_p_res"] = np.array([pv]), pr
+    # matrix-free element products (Jacobian, stiffness, mass), separate stream
+    rng4 = np.random.default_rng(seed + 3)
+    for order in (2, 3):
+        npe = order ** 3
+        for s in range(2):
+            key = f"hex{npe}_{s}"
+            X, t = out[key + "_X"], float(out[key + "_t"][0])
+            x = rng4.standard_normal(3 * npe)
+            mv = np.zeros((3, 3 * npe))
+            for mt in range(3):
+                L.ref_solid_matvec(order, mt, _d(X), _d(x), c(70e9), c(0.3), c(2700.0), c(t),
+                                   c(0.8), c(0.3), _d(mv[mt]))
+            out[key + "_mvx"], out[key + "_mv"] = x, mv
     np.savez_compressed(os.path.join(HERE, "elements.npz"), **out)
     return out
 

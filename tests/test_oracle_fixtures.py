@@ -59,6 +59,33 @@ def test_solid_pressure_matches_reference(key):
         assert np.abs(r[~on]).max() == 0.0
 
 
+@pytest.mark.parametrize("key", ["hex8_0", "hex8_1", "hex27_0", "hex27_1"])
+def test_solid_matvec_matches_reference(key):
+    """Matrix-free element products (TACSElement3D::getMatVecProductData +
+    addMatVecProduct) of the reference build for the Jacobian (alpha 0.8,
+    gamma 0.3), stiffness and mass types, and the same products from the
+    oracle's element matrices (K x with K = alpha Ks + gamma M)."""
+    order = 2 if key.startswith("hex8") else 3
+    X, t, x = FIX[key + "_X"], float(FIX[key + "_t"][0]), FIX[key + "_mvx"]
+    C = O.iso_C21(70e9, 0.3)
+    coefs = [(0.8, 0.3), (0.8, 0.0), (0.0, 0.3)]
+    for mt, (kc, mc) in enumerate(coefs):
+        ref = FIX[key + "_mv"][mt]
+        y = O.solid_matvec_element(order, X, x, C, t=t, rho=2700.0, kc=kc, mc=mc)
+        assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
+        _, K = O.solid_element(order, X, np.zeros_like(x), C, t=t, rho=2700.0, alpha=kc, gamma=mc,
+                               want_res=False)
+        assert np.abs(K @ x - y).max() <= 1e-13 * np.abs(y).max()
+    # the mass product of a constant field integrates rho t over the element
+    ones = np.tile([1.0, 0.0, 0.0], order ** 3)
+    m = O.solid_matvec_element(order, X, ones, C, t=t, rho=2700.0, kc=0.0, mc=1.0)
+    vol = sum(O.hex_quadrature(order, q)[0] * abs(np.linalg.det(
+        O.hex_basis(order, O.hex_quadrature(order, q)[1])[1].reshape(-1, 3).T
+        @ X.reshape(-1, 3))) for q in range(order ** 3))
+    assert np.isclose(m[0::3].sum(), 2700.0 * t * vol, rtol=1e-12)
+    assert np.abs(m[1::3]).max() == 0.0 and np.abs(m[2::3]).max() == 0.0
+
+
 @pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built (no reference tree)")
 def test_solid_element_fresh_inputs_against_reference_build():
     R = ctypes.CDLL(REF, mode=ctypes.RTLD_GLOBAL)
