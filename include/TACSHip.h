@@ -638,6 +638,14 @@ class TACSAssembler : public TACSObject {
                         TACSBVec* residual, TACSMat* A,
                         MatrixOrientation matOr = TACS_MAT_NORMAL,
                         const TacsScalar lambda = 1.0, const bool applyBCs = true);
+  // Stiffness, mass and geometric stiffness (at the current state) matrices
+  // and their combinations (src/TACSAssembler.cpp:4418, :4519)
+  void assembleMatType(ElementMatrixType matType, TACSMat* A,
+                       MatrixOrientation matOr = TACS_MAT_NORMAL, const TacsScalar lambda = 1.0,
+                       const bool applyBCs = true);
+  void assembleMatCombo(ElementMatrixType matTypes[], TacsScalar scale[], int nmats, TACSMat* A,
+                        MatrixOrientation matOr = TACS_MAT_NORMAL, const TacsScalar lambda = 1.0,
+                        const bool applyBCs = true);
   void evalFunctions(int numFuncs, TACSFunction** funcs, TacsScalar* funcVals);
   void addDVSens(TacsScalar coef, int numFuncs, TACSFunction** funcs, TACSBVec** dfdx);
   void addSVSens(TacsScalar alpha, TacsScalar beta, TacsScalar gamma, int numFuncs,
@@ -693,6 +701,7 @@ class TACSAssembler : public TACSObject {
   ShellMaterial smat;
   double* d_ke;  // shell element matrices [nelem][24][24]
   double* d_qgeo;  // solid quadrature geometry [nelem][npe][10] (Jacobian scratch)
+  double* d_qstress;  // solid quadrature stresses [nelem][npe][6] (geometric stiffness)
   // device data
   int *d_conn, *d_n2e_ptr, *d_n2e, *d_bcvars, *d_bcnodes, *d_bcmask;
   double* d_bcvals;

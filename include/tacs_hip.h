@@ -79,6 +79,21 @@ int tacs_hip_solid_jacobian(int order, int nnodes, int nelem, const int* n2e_ptr
                             double gamma, const int* rowp, const int* cols, const int* bcvars,
                             double* G, double* A, float* A32, int* status, hipStream_t stream);
 
+/* Element-matrix combinations of the solid family straight into block-CSR:
+   kc K + mc M + gc G(u) with K the stiffness, M the consistent mass and G
+   the geometric stiffness at the state u (replaces the element loops of
+   TACSAssembler::assembleMatType src/TACSAssembler.cpp:4418 and
+   assembleMatCombo :4519 over TACSElement3D::getMatType
+   src/elements/TACSElement3D.cpp:316).  Arguments as
+   tacs_hip_solid_jacobian; S: scratch of 6 * order^3 * nelem doubles (the
+   per-point stresses), needed with u when gc != 0. */
+int tacs_hip_solid_mat_type(int order, int nnodes, int nelem, const int* n2e_ptr,
+                            const int* n2e, const int* conn, const double* X, const double* u,
+                            const double* telem, const SolidMaterial* mat, double kc, double mc,
+                            double gc, const int* rowp, const int* cols, const int* bcvars,
+                            double* G, double* S, double* A, float* A32, int* status,
+                            hipStream_t stream);
+
 /* Matrix-free Jacobian data (replaces TACSAssembler::assembleMatrixFreeData
    src/TACSAssembler.cpp:5563 over TACSElement3D::getMatVecProductData
    src/elements/TACSElement3D.cpp:379): data[0..3] = [kc, mc, 0, 0], then per
@@ -540,6 +555,16 @@ int tacs_vec_set_rand(tacs_obj v, double lower, double upper);
 
 /* TACSParallelMat (TACS.pyx Mat :960) */
 int tacs_mat_mult(tacs_obj m, tacs_obj x, tacs_obj y);
+
+/* TACSAssembler::assembleMatType src/TACSAssembler.cpp:4418 (TACS.pyx
+   Assembler.assembleMatType :2228) and assembleMatCombo :4519 (TACS.pyx
+   :2256, there with two types): matType 1 stiffness, 2 mass, 3 geometric
+   stiffness at the current variables (solids); shells: stiffness only. */
+int tacs_assembler_assemble_mat_type(tacs_obj a, int matType, tacs_obj mat, int matOr,
+                                     double loadScale, int applyBCs);
+int tacs_assembler_assemble_mat_combo(tacs_obj a, int nmats, const int* matTypes,
+                                      const double* scale, tacs_obj mat, int matOr,
+                                      double loadScale, int applyBCs);
 
 /* Matrix-free Jacobian products.
    tacs_matfree_create: TACSMatrixFreeMat(assembler) src/TACSMatrixFreeMat.cpp:17;

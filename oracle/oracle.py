@@ -92,6 +92,24 @@ def solid_element(order, X, u, C21, t=1.0, rho=0.0, alpha=1.0, gamma=0.0,
     return res, mat
 
 
+def solid_geo_element(order, Xe, ue, C21, t=1.0):
+    """Geometric stiffness element matrix at the state ue
+    (TACSElement3D::getMatType, TACS_GEOMETRIC_STIFFNESS_MATRIX)."""
+    nv = 3 * order ** 3
+    mat = np.zeros((nv, nv))
+    lib().orc_solid_geo_element(order, _p(_dd(Xe)), _p(_dd(ue)), _p(_dd(C21)),
+                                ctypes.c_double(t), _p(mat))
+    return mat
+
+
+def assemble_geo(order, conn, X, u, t_elem, C21, rowp, cols, A, gc=1.0):
+    """A += gc * geometric stiffness at u (in place); returns missing slots."""
+    conn = _ii(conn)
+    return lib().orc_assemble_geo(order, conn.shape[0], _p(conn, np.int32), _p(_dd(X)),
+                                  _p(_dd(u)), _p(_dd(t_elem)), _p(_dd(C21)), ctypes.c_double(gc),
+                                  _p(_ii(rowp), np.int32), _p(_ii(cols), np.int32), _p(A))
+
+
 def solid_matvec_element(order, Xe, xe, C21, t=1.0, rho=0.0, kc=1.0, mc=0.0):
     """Element matrix-free product (TACSElement3D::addMatVecProduct)."""
     c = ctypes.c_double

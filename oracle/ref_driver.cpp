@@ -69,6 +69,27 @@ void ref_solid_matvec(int order, int matType, const double* X, const double* x, 
   elem->decref();
 }
 
+// Element matrix of one 8- or 27-node solid element for matType 1 (stiffness),
+// 2 (mass) or 3 (geometric stiffness at u) (TACSElement3D::getMatType).
+void ref_solid_mat_type(int order, int matType, const double* X, const double* u, double E,
+                        double nu, double rho, double t, double* mat) {
+  TACSMaterialProperties* props = new TACSMaterialProperties(rho, 921.0, E, nu, 270e6, 0.0, 0.0);
+  TACSSolidConstitutive* con = new TACSSolidConstitutive(props, t, 0);
+  TACSLinearElasticity3D* model = new TACSLinearElasticity3D(con, TACS_LINEAR_STRAIN);
+  TACSElementBasis* basis;
+  if (order == 2)
+    basis = new TACSLinearHexaBasis();
+  else
+    basis = new TACSQuadraticHexaBasis();
+  TACSElement3D* elem = new TACSElement3D(model, basis);
+  elem->incref();
+  ElementMatrixType mt = matType == 1 ? TACS_STIFFNESS_MATRIX
+                                      : (matType == 2 ? TACS_MASS_MATRIX
+                                                      : TACS_GEOMETRIC_STIFFNESS_MATRIX);
+  elem->getMatType(mt, 0, 0.0, X, u, mat);
+  elem->decref();
+}
+
 static TACSElement3D* make_solid(int order, double E, double nu, double t) {
   TACSMaterialProperties* props = new TACSMaterialProperties(2700.0, 921.0, E, nu, 270e6, 0.0, 0.0);
   TACSSolidConstitutive* con = new TACSSolidConstitutive(props, t, 0);

@@ -13,6 +13,10 @@
       exact integration of polynomials, literal Gauss constants
       src/elements/basis/TACSGaussQuadrature.h:26,29).
     - orc_iso_C21: closed form of isotropic Hooke's law; Voigt round trip.
+    - orc_solid_geo_element / orc_assemble_geo: the reference's own
+      TACSElement3D::getMatType geometric stiffness (oracle/_ref); symmetry;
+      the closed form G_ab = (grad N_a . sigma . grad N_b) I for a uniform
+      stress state.
     - orc_solid_matvec_element / orc_matfree_product: the reference's own
       TACSElement3D matrix-free product (oracle/_ref) and K x from the
       assembled element matrices of orc_solid_element (mass and stiffness).
@@ -280,6 +284,82 @@ void orc_solid_element(int order, const double *X, const double *u,
       }
     }
   }
+}
+
+/*
+  Geometric stiffness of one element at the state u, restating
+  TACSElement3D::getMatType (src/elements/TACSElement3D.cpp:316-360) for
+  TACS_GEOMETRIC_STIFFNESS_MATRIX: TACSLinearElasticity3D::evalWeakMatrix
+  (TACSLinearElasticity.cpp:1704-) gives the coefficient sigma_de = (t C e)_de
+  of du_j/dx_d x du_j/dx_e (same component j only), scaled by w detJ.
+  mat (nv x nv, node-major dofs) is added to.
+*/
+void orc_solid_geo_element(int order, const double *X, const double *u,
+                           const double C21[21], double t, double *mat) {
+  const int npe = order * order * order;
+  const int nv = 3 * npe;
+  double D[36];
+  unpack21(C21, D);
+  for (int n = 0; n < npe; n++) {
+    double pt[3], N[MAXN], Nxi[3 * MAXN], Nx[3 * MAXN], J[9], Ux[9], e[6], sg[6];
+    double w = orc_hex_quadrature(order, n, pt);
+    orc_hex_basis(order, pt, N, Nxi);
+    double det = qp_geometry(npe, Nxi, X, Nx, J);
+    grad_field(npe, Nx, u, Ux);
+    strain(Ux, e);
+    matvec6(D, e, sg);
+    for (int i = 0; i < 6; i++) sg[i] *= t;
+    for (int a = 0; a < npe; a++) {
+      for (int b = 0; b < npe; b++) {
+        double v = 0.0;
+        for (int d = 0; d < 3; d++) {
+          for (int f = 0; f < 3; f++) v += Nx[3 * a + d] * sg[voigt(d, f)] * Nx[3 * b + f];
+        }
+        v *= w * det;
+        for (int j = 0; j < 3; j++) mat[(3 * a + j) * nv + 3 * b + j] += v;
+      }
+    }
+  }
+}
+
+static long find_slot(const int *rowp, const int *cols, int row, int col);
+
+/* A += gc * (geometric stiffness at u) over the mesh, element by element
+   (the element loop of TACSAssembler::assembleMatType src/TACSAssembler.cpp:
+   4469-4494 for this type). */
+long orc_assemble_geo(int order, int nelem, const int *conn, const double *X,
+                      const double *u, const double *t_elem, const double C21[21],
+                      double gc, const int *rowp, const int *cols, double *A) {
+  const int npe = order * order * order;
+  const int nv = 3 * npe;
+  double Xe[3 * MAXN], ue[3 * MAXN];
+  double *ke = (double *)malloc((size_t)nv * nv * sizeof(double));
+  long missing = 0;
+  for (int e = 0; e < nelem; e++) {
+    const int *c = &conn[npe * e];
+    for (int a = 0; a < npe; a++) {
+      for (int k = 0; k < 3; k++) {
+        Xe[3 * a + k] = X[3 * c[a] + k];
+        ue[3 * a + k] = u[3 * c[a] + k];
+      }
+    }
+    memset(ke, 0, (size_t)nv * nv * sizeof(double));
+    orc_solid_geo_element(order, Xe, ue, C21, t_elem[e], ke);
+    for (int a = 0; a < npe; a++) {
+      for (int b = 0; b < npe; b++) {
+        long s = find_slot(rowp, cols, c[a], c[b]);
+        if (s < 0) {
+          missing++;
+          continue;
+        }
+        for (int r = 0; r < 3; r++) {
+          for (int q = 0; q < 3; q++) A[9 * s + 3 * r + q] += gc * ke[(3 * a + r) * nv + 3 * b + q];
+        }
+      }
+    }
+  }
+  free(ke);
+  return missing;
 }
 
 /*

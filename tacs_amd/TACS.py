@@ -451,6 +451,24 @@ class Assembler(TACSHandle):
             self.ptr, alpha, beta, gamma, residual.ptr if residual is not None else None, A.ptr,
             matOr, loadScale, 1 if applyBCs else 0), "assembleJacobian")
 
+    def assembleMatType(self, matType, A, matOr=NORMAL, loadScale=1.0, applyBCs=True):
+        """Stiffness, mass or geometric stiffness matrix (Assembler.assembleMatType
+        tacs/TACS.pyx:2228); Dirichlet rows zeroed with a unit diagonal."""
+        check(lib().tacs_assembler_assemble_mat_type(self.ptr, int(matType), A.ptr, matOr,
+                                                      loadScale, 1 if applyBCs else 0),
+              "assembleMatType")
+
+    def assembleMatCombo(self, matType1, scale1, matType2, scale2, A, matOr=NORMAL,
+                         loadScale=1.0, applyBCs=True):
+        """A = scale1 matType1 + scale2 matType2 (Assembler.assembleMatCombo
+        tacs/TACS.pyx:2256)."""
+        types = np.array([matType1, matType2], dtype=np.intc)
+        scales = np.array([scale1, scale2], dtype=np.float64)
+        check(lib().tacs_assembler_assemble_mat_combo(self.ptr, 2, types.ctypes.data,
+                                                       scales.ctypes.data, A.ptr, matOr,
+                                                       loadScale, 1 if applyBCs else 0),
+              "assembleMatCombo")
+
     def getMatrixFreeDataSize(self, matType=JACOBIAN_MATRIX):
         """(data_size, temp_size) in doubles (TACSAssembler::getMatrixFreeDataSize)."""
         d, t = ctypes.c_int(), ctypes.c_int()

@@ -386,6 +386,32 @@ int tacs_mat_mult(tacs_obj m, tacs_obj x, tacs_obj y) {
   A->mult(xv, yv);
   return last_error_code();
 }
+int tacs_assembler_assemble_mat_type(tacs_obj h, int matType, tacs_obj mat, int matOr,
+                                     double loadScale, int applyBCs) {
+  REQUIRE(TACSAssembler, a, h);
+  REQUIRE(TACSMat, A, mat);
+  clear_error();
+  a->assembleMatType((ElementMatrixType)matType, A, (MatrixOrientation)matOr, loadScale,
+                     applyBCs != 0);
+  return last_error_code();
+}
+int tacs_assembler_assemble_mat_combo(tacs_obj h, int nmats, const int* matTypes,
+                                      const double* scale, tacs_obj mat, int matOr,
+                                      double loadScale, int applyBCs) {
+  REQUIRE(TACSAssembler, a, h);
+  REQUIRE(TACSMat, A, mat);
+  if (nmats < 1 || nmats > 8 || !matTypes || !scale)
+    return tacs_set_error("assembleMatCombo: 1 to 8 matrix types with their scales");
+  ElementMatrixType t[8];
+  TacsScalar sc[8];
+  for (int i = 0; i < nmats; i++) {
+    t[i] = (ElementMatrixType)matTypes[i];
+    sc[i] = scale[i];
+  }
+  clear_error();
+  a->assembleMatCombo(t, sc, nmats, A, (MatrixOrientation)matOr, loadScale, applyBCs != 0);
+  return last_error_code();
+}
 tacs_obj tacs_matfree_create(tacs_obj assembler) {
   REQUIRE_NULL(TACSAssembler, a, assembler);
   return (TACSObject*)new TACSMatrixFreeMat(a);

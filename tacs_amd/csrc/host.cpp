@@ -2130,6 +2130,7 @@ TACSAssembler::TACSAssembler(TACSComm comm, int vpn, int nowned, int nelems, int
   family = FAMILY_SOLID;
   d_ke = NULL;
   d_qgeo = NULL;
+  d_qstress = NULL;
   d_aux_pres = NULL;
   d_conn = d_n2e_ptr = d_n2e = d_bcvars = d_bcnodes = d_bcmask = NULL;
   d_bcvals = NULL;
@@ -2159,7 +2160,7 @@ TACSAssembler::~TACSAssembler() {
   void* ds[] = {d_conn, d_n2e_ptr, d_n2e, d_bcvars, d_bcnodes, d_bcmask, d_bcvals, d_dvnum,
                 d_dv2e_ptr, d_dv2e, d_tfixed, d_telem, d_X, d_u, d_ev, d_es, d_aux_elem,
                 d_aux_face, d_n2a_ptr, d_n2a, d_aux_trac, d_aux_ev, d_status, d_red, d_scalar,
-                d_ke, d_aux_pres, d_qgeo};
+                d_ke, d_aux_pres, d_qgeo, d_qstress};
   for (void* p : ds) dfree(p);
 }
 
@@ -2684,7 +2685,72 @@ void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScal
   if (s) tacs_set_error("assembleJacobian: kernel status %d", s);
 }
 
-// TACSAssembler::evalFunctions src/TACSAssembler.cpp:4617
+// TACSAssembler::assembleMatType src/TACSAssembler.cpp:4418 and
+// assembleMatCombo :4519.  Solids: one pass of the row kernel with the summed
+// coefficients of the stiffness, the consistent mass and the geometric
+// stiffness at the current state; aux elements (tractions, pressures) add no
+// matrix terms (TACSElement::getMatType src/elements/TACSElement.cpp:334 over
+// their zero Jacobians).  Shells: the stiffness only.
+void TACSAssembler::assembleMatCombo(ElementMatrixType matTypes[], TacsScalar scale[], int nmats,
+                                     TACSMat* tA, MatrixOrientation matOr, const TacsScalar lambda,
+                                     const bool bcs) {
+  if (!initialized) {
+    tacs_set_error("assembleMatCombo before initialize()");
+    return;
+  }
+  TACSParallelMat* A = dynamic_cast<TACSParallelMat*>(tA);
+  if (!A || A->nrows != numNodes || A->nnzb != nnzb || A->bsize != varsPerNode) {
+    tacs_set_error("assembleMatType: matrix does not match the assembler");
+    return;
+  }
+  double kc = 0.0, mc = 0.0, gc = 0.0;
+  for (int i = 0; i < nmats; i++) {
+    switch (matTypes[i]) {
+      case TACS_STIFFNESS_MATRIX:
+        kc += scale[i];
+        break;
+      case TACS_MASS_MATRIX:
+        mc += scale[i];
+        break;
+      case TACS_GEOMETRIC_STIFFNESS_MATRIX:
+        gc += scale[i];
+        break;
+      default:
+        tacs_set_error("assembleMatType: matrix type %d is not supported", (int)matTypes[i]);
+        return;
+    }
+  }
+  if (family == FAMILY_SHELL) {
+    if (mc != 0.0 || gc != 0.0) {
+      tacs_set_error("assembleMatType: shell mass and geometric stiffness are not built");
+      return;
+    }
+    ensure(d_ke, (size_t)numElements * 576);
+    KCHECK(shellKernel(64, 0, NULL, kc, 0, 0, 1, 1, d_ke, NULL, NULL));
+    KCHECK(tacs_hip_gather_elem_matrices(6, 4, 64, numNodes, d_n2e_ptr, d_n2e, d_conn, d_ke,
+                                         A->d_rowp, A->d_cols, bcs ? d_bcvars : NULL, A->d_A,
+                                         A->d_A32, g_stream));
+  } else {
+    ensure(d_qgeo, (size_t)numElements * npe * 10);
+    if (gc != 0.0) ensure(d_qstress, (size_t)numElements * npe * 6);
+    KCHECK(tacs_hip_solid_mat_type(order, numNodes, numElements, d_n2e_ptr, d_n2e, d_conn, d_X,
+                                   d_u, d_telem, &mat, kc, mc, gc, A->d_rowp, A->d_cols,
+                                   bcs ? d_bcvars : NULL, d_qgeo, gc != 0.0 ? d_qstress : NULL,
+                                   A->d_A, A->d_A32, d_status, g_stream));
+  }
+  A->A32_current = A->d_A32 ? 1 : 0;
+  A->version++;
+  int s = status();
+  if (s) tacs_set_error("assembleMatType: kernel status %d", s);
+}
+void TACSAssembler::assembleMatType(ElementMatrixType matType, TACSMat* A,
+                                    MatrixOrientation matOr, const TacsScalar lambda,
+                                    const bool bcs) {
+  ElementMatrixType t[1] = {matType};
+  TacsScalar sc[1] = {1.0};
+  assembleMatCombo(t, sc, 1, A, matOr, lambda, bcs);
+}
+
 // ---------------------------------------------------------------------------
 // Matrix-free products, TACSAssembler.cpp:5502-5717 and
 // src/TACSMatrixFreeMat.cpp.  The reference stores per quadrature point the
@@ -2838,6 +2904,7 @@ void TACSMatrixFreeMat::mult(TACSVec* tx, TACSVec* ty) {
 }
 void TACSMatrixFreeMat::multTranspose(TACSVec* x, TACSVec* y) { mult(x, y); }
 
+// TACSAssembler::evalFunctions src/TACSAssembler.cpp:4617
 void TACSAssembler::evalFunctions(int numFuncs, TACSFunction** funcs, TacsScalar* vals) {
   ensure(d_es, numElements);
   for (int k = 0; k < numFuncs; k++) {

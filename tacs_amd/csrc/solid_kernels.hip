@@ -195,6 +195,46 @@ __global__ __launch_bounds__(256) void solid_qp_geometry(int nelem, const int* _
   g[9] = qp.w * det * telem[elem];
 }
 
+// Stress C e(u) (no thickness factor; the table's w det t carries it) at
+// every (element, quadrature point), for the geometric stiffness:
+// S[(e NQP + q) 6 + k], Voigt order of TACSSolidConstitutive::evalStress.
+template <int ORD>
+__global__ __launch_bounds__(256) void solid_qp_stress(int nelem, const int* __restrict__ conn,
+                                                       const double* __restrict__ u,
+                                                       const double* __restrict__ G,
+                                                       SolidMaterial mat,
+                                                       double* __restrict__ S) {
+  constexpr int NPE = ORD * ORD * ORD, NQP = NPE;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nelem * NQP) return;
+  const int elem = (int)(t / NQP), q = (int)(t - (size_t)elem * NQP);
+  QP1D<ORD> qp;
+  qp.eval(q);
+  double J[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) J[k] = G[t * 10 + k];
+  double Ux[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int* c = &conn[(size_t)elem * NPE];
+  for (int a = 0; a < NPE; a++) {
+    double N, g0[3], g[3];
+    qp.basis(a, N, g0);
+    grad_x(J, g0, g);
+    const int na = c[a];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const double uj = u[3 * (size_t)na + j];
+      Ux[3 * j] += uj * g[0];
+      Ux[3 * j + 1] += uj * g[1];
+      Ux[3 * j + 2] += uj * g[2];
+    }
+  }
+  const double e[6] = {Ux[0], Ux[4], Ux[8], Ux[5] + Ux[7], Ux[2] + Ux[6], Ux[1] + Ux[3]};
+  double s[6];
+  stress6(mat, e, s);
+#pragma unroll
+  for (int k = 0; k < 6; k++) S[t * 6 + k] = s[k];
+}
+
 // ---------------------------------------------------------------------------
 // Jacobian: one wavefront per block row.
 // ---------------------------------------------------------------------------
@@ -211,23 +251,30 @@ struct JacCfg<3> {
 
 // ISO: isotropic tangent (lambda, mu closed form) or the general C[21]; two
 // instantiations keep the general path's registers out of the isotropic one.
-template <int ORD, bool ISO>
+// GEO: adds geo times the geometric stiffness from the per-point stresses Sq
+// (TACSLinearElasticity3D::evalWeakMatrix, TACS_GEOMETRIC_STIFFNESS_MATRIX,
+// src/elements/TACSLinearElasticity.cpp:1704: coefficient sigma_de of
+// du_j/dx_d du_j/dx_e).
+template <int ORD, bool ISO, bool GEO>
 __global__ __launch_bounds__(64 * JacCfg<ORD>::WPB) void solid_jacobian_rows(
     int nnodes, const int* __restrict__ n2e_ptr, const int* __restrict__ n2e,
-    const int* __restrict__ conn, const double* __restrict__ Gq, SolidMaterial mat, double alpha,
-    double gamma, const int* __restrict__ rowp, const int* __restrict__ cols,
+    const int* __restrict__ conn, const double* __restrict__ Gq, const double* __restrict__ Sq,
+    SolidMaterial mat, double alpha, double gamma, double geo, const int* __restrict__ rowp,
+    const int* __restrict__ cols,
     const int* __restrict__ bcvars, double* __restrict__ A,
     float* __restrict__ A32, int* __restrict__ status) {
   constexpr int NPE = ORD * ORD * ORD, NQP = NPE;
   constexpr int MAXC = JacCfg<ORD>::MAXC, WPB = JacCfg<ORD>::WPB;
   constexpr int ECH = 8;  // elements per chunk
-  __shared__ double geo[WPB][ECH * NQP][10];
+  __shared__ double geom[WPB][ECH * NQP][10];
+  __shared__ double sgeo[WPB][GEO ? ECH * NQP : 1][6];
   __shared__ double rowbuf[WPB][MAXC * 9];
   __shared__ int colbuf[WPB][MAXC];
 
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
-  double (*G)[10] = geo[wid];
+  double (*G)[10] = geom[wid];
+  double (*SG)[6] = sgeo[wid];
   double* R = rowbuf[wid];
   int* CB = colbuf[wid];
 
@@ -253,6 +300,14 @@ __global__ __launch_bounds__(64 * JacCfg<ORD>::WPB) void solid_jacobian_rows(
         int e = eq / NQP, q = eq - e * NQP;
         int elem = n2e[es + c0 + e] >> 5;
         G[eq][k] = Gq[((size_t)elem * NQP + q) * 10 + k];
+      }
+      if (GEO) {
+        for (int p = lane; p < nce * NQP * 6; p += 64) {
+          int eq = p / 6, k = p - 6 * eq;
+          int e = eq / NQP, q = eq - e * NQP;
+          int elem = n2e[es + c0 + e] >> 5;
+          SG[eq][k] = Sq[((size_t)elem * NQP + q) * 6 + k];
+        }
       }
       WAVE_SYNC();
       // (element, column node) pairs: accumulate the 3x3 block over qp.
@@ -324,6 +379,17 @@ __global__ __launch_bounds__(64 * JacCfg<ORD>::WPB) void solid_jacobian_rows(
               acc[0] += m;
               acc[4] += m;
               acc[8] += m;
+            }
+            if (GEO) {
+              // ga . sigma . gb, sigma = [s0 s5 s4; s5 s1 s3; s4 s3 s2]
+              const double* sq = SG[e * NQP + q];
+              const double t0 = sq[0] * gb[0] + sq[5] * gb[1] + sq[4] * gb[2];
+              const double t1 = sq[5] * gb[0] + sq[1] * gb[1] + sq[3] * gb[2];
+              const double t2 = sq[4] * gb[0] + sq[3] * gb[1] + sq[2] * gb[2];
+              const double gg = geo * wdt * (ga[0] * t0 + ga[1] * t1 + ga[2] * t2);
+              acc[0] += gg;
+              acc[4] += gg;
+              acc[8] += gg;
             }
           }
           if (slot < 0) atomicExch(status, TACS_HIP_ERR_MISSING_ENTRY);
@@ -861,17 +927,38 @@ __global__ __launch_bounds__(256) void solid_matfree_ev(int nelem, const int* __
     }
     WAVE_SYNC();
     if (ok && l < NPE) {
+      // node l's 1D Lagrange values and slopes at the Gauss coordinates,
+      // evaluated once; the quadrature loop then only multiplies them
+      const int ia = l % ORD, ja = (l / ORD) % ORD, ka = l / (ORD * ORD);
+      double nI[ORD], dI[ORD], nJ[ORD], dJ[ORD], nK[ORD], dK[ORD];
+#pragma unroll
+      for (int r = 0; r < ORD; r++) {
+        double n1[ORD], d1[ORD];
+        lag1d<ORD>(gauss_pt<ORD>(r), n1, d1);
+        nI[r] = pick<ORD>(n1, ia);
+        dI[r] = pick<ORD>(d1, ia);
+        nJ[r] = pick<ORD>(n1, ja);
+        dJ[r] = pick<ORD>(d1, ja);
+        nK[r] = pick<ORD>(n1, ka);
+        dK[r] = pick<ORD>(d1, ka);
+      }
       double o0 = 0.0, o1 = 0.0, o2 = 0.0;
 #pragma unroll UK
-      for (int q = 0; q < NQP; q++) {
-        QP1D<ORD> qp;
-        qp.eval(q);
-        double N, g0[3];
-        qp.basis(l, N, g0);
-        const double* S = ss[g][q];
-        o0 += g0[0] * S[0] + g0[1] * S[1] + g0[2] * S[2] + N * S[9];
-        o1 += g0[0] * S[3] + g0[1] * S[4] + g0[2] * S[5] + N * S[10];
-        o2 += g0[0] * S[6] + g0[1] * S[7] + g0[2] * S[8] + N * S[11];
+      for (int kq = 0; kq < ORD; kq++) {
+        const double nk = pick<ORD>(nK, kq), dk = pick<ORD>(dK, kq);
+#pragma unroll
+        for (int jq = 0; jq < ORD; jq++) {
+          const double njk = nJ[jq] * nk, djk = dJ[jq] * nk, nzk = nJ[jq] * dk;
+#pragma unroll
+          for (int iq = 0; iq < ORD; iq++) {
+            const double* S = ss[g][iq + ORD * (jq + ORD * kq)];
+            const double g0 = dI[iq] * njk, g1 = nI[iq] * djk, g2 = nI[iq] * nzk;
+            const double N = nI[iq] * njk;
+            o0 += g0 * S[0] + g1 * S[1] + g2 * S[2] + N * S[9];
+            o1 += g0 * S[3] + g1 * S[4] + g2 * S[5] + N * S[10];
+            o2 += g0 * S[6] + g1 * S[7] + g2 * S[8] + N * S[11];
+          }
+        }
       }
       double* o = &ev[((size_t)elem * NPE + l) * 3];
       o[0] = o0;
@@ -1031,36 +1118,70 @@ __global__ void apply_bc_vec(int b, int nbc, const int* __restrict__ nodes,
 // ---------------------------------------------------------------------------
 extern "C" {
 
+int tacs_hip_solid_mat_type(int order, int nnodes, int nelem, const int* n2e_ptr,
+                            const int* n2e, const int* conn, const double* X, const double* u,
+                            const double* telem, const SolidMaterial* mat, double kc, double mc,
+                            double gc, const int* rowp, const int* cols, const int* bcvars,
+                            double* G, double* S, double* A, float* A32, int* status,
+                            hipStream_t stream) {
+  if (order != 2 && order != 3) return TACS_HIP_ERR_ARG;
+  if (gc != 0.0 && (!u || !S)) return TACS_HIP_ERR_ARG;
+  const int nqp = order * order * order;
+  const size_t nq = (size_t)nelem * nqp;
+  const dim3 qgrid((unsigned)((nq + 255) / 256));
+  if (nq > 0) {
+    if (order == 2)
+      hipLaunchKernelGGL(solid_qp_geometry<2>, qgrid, dim3(256), 0, stream, nelem, conn, X, telem,
+                         G);
+    else
+      hipLaunchKernelGGL(solid_qp_geometry<3>, qgrid, dim3(256), 0, stream, nelem, conn, X, telem,
+                         G);
+    if (gc != 0.0) {
+      if (order == 2)
+        hipLaunchKernelGGL(solid_qp_stress<2>, qgrid, dim3(256), 0, stream, nelem, conn, u, G,
+                           *mat, S);
+      else
+        hipLaunchKernelGGL(solid_qp_stress<3>, qgrid, dim3(256), 0, stream, nelem, conn, u, G,
+                           *mat, S);
+    }
+  }
+#define JAC_LAUNCH(O, I, GE)                                                                   \
+  hipLaunchKernelGGL((solid_jacobian_rows<O, I, GE>),                                          \
+                     dim3(tacs_grid_for(nnodes, JacCfg<O>::WPB, 1 << 20)),                     \
+                     dim3(64 * JacCfg<O>::WPB), 0, stream, nnodes, n2e_ptr, n2e, conn, G, S,   \
+                     *mat, kc, mc, gc, rowp, cols, bcvars, A, A32, status)
+#define JAC_ISO(O, GE)            \
+  if (mat->isotropic)             \
+    JAC_LAUNCH(O, true, GE);      \
+  else                            \
+    JAC_LAUNCH(O, false, GE)
+  const bool ge = gc != 0.0;
+  if (order == 2) {
+    if (ge) {
+      JAC_ISO(2, true);
+    } else {
+      JAC_ISO(2, false);
+    }
+  } else {
+    if (ge) {
+      JAC_ISO(3, true);
+    } else {
+      JAC_ISO(3, false);
+    }
+  }
+#undef JAC_ISO
+#undef JAC_LAUNCH
+  return (int)hipGetLastError();
+}
+
 int tacs_hip_solid_jacobian(int order, int nnodes, int nelem, const int* n2e_ptr,
                             const int* n2e, const int* conn, const double* X,
                             const double* telem, const SolidMaterial* mat, double alpha,
                             double gamma, const int* rowp, const int* cols, const int* bcvars,
                             double* G, double* A, float* A32, int* status, hipStream_t stream) {
-  if (order != 2 && order != 3) return TACS_HIP_ERR_ARG;
-  const int nqp = order * order * order;
-  const size_t nq = (size_t)nelem * nqp;
-  if (nq > 0) {
-    if (order == 2)
-      hipLaunchKernelGGL(solid_qp_geometry<2>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
-                         stream, nelem, conn, X, telem, G);
-    else
-      hipLaunchKernelGGL(solid_qp_geometry<3>, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
-                         stream, nelem, conn, X, telem, G);
-  }
-#define JAC_LAUNCH(O, I)                                                                        \
-  hipLaunchKernelGGL((solid_jacobian_rows<O, I>), dim3(tacs_grid_for(nnodes, JacCfg<O>::WPB, 1 << 20)), \
-                     dim3(64 * JacCfg<O>::WPB), 0, stream, nnodes, n2e_ptr, n2e, conn, G, *mat,  \
-                     alpha, gamma, rowp, cols, bcvars, A, A32, status)
-  if (order == 2 && mat->isotropic)
-    JAC_LAUNCH(2, true);
-  else if (order == 2)
-    JAC_LAUNCH(2, false);
-  else if (mat->isotropic)
-    JAC_LAUNCH(3, true);
-  else
-    JAC_LAUNCH(3, false);
-#undef JAC_LAUNCH
-  return (int)hipGetLastError();
+  return tacs_hip_solid_mat_type(order, nnodes, nelem, n2e_ptr, n2e, conn, X, NULL, telem, mat,
+                                 alpha, gamma, 0.0, rowp, cols, bcvars, G, NULL, A, A32, status,
+                                 stream);
 }
 
 int tacs_hip_solid_matfree_data(int order, int nelem, const int* conn, const double* X,

@@ -116,6 +116,17 @@ def make(seed=2024):
                 L.ref_solid_matvec(order, mt, _d(X), _d(x), c(70e9), c(0.3), c(2700.0), c(t),
                                    c(0.8), c(0.3), _d(mv[mt]))
             out[key + "_mvx"], out[key + "_mv"] = x, mv
+    # element matrices by type (stiffness, mass, geometric stiffness at u)
+    for order in (2, 3):
+        npe = order ** 3
+        for s in range(2):
+            key = f"hex{npe}_{s}"
+            X, u, t = out[key + "_X"], out[key + "_u"], float(out[key + "_t"][0])
+            mt = np.zeros((3, 9 * npe * npe))
+            for k in range(3):
+                L.ref_solid_mat_type(order, k + 1, _d(X), _d(u), c(70e9), c(0.3), c(2700.0), c(t),
+                                     _d(mt[k]))
+            out[key + "_mattype"] = mt
     np.savez_compressed(os.path.join(HERE, "elements.npz"), **out)
     return out
 

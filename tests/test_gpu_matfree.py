@@ -146,3 +146,63 @@ def test_gmres_with_matrix_free_operator(order, n):
     p.mat.mult(u_free, r)
     r.axpy(-1.0, p.res)
     assert r.norm() <= 1e-9 * p.res.norm()
+
+
+def _oracle_mat(p, X, t, u, kc, mc, gc):
+    m = p.mesh
+    C = O.iso_C21(70e9, 0.3)
+    rowp, cols = O.node_csr(m.nnodes, m.conn, m.npe)
+    A, _ = O.assemble_solid(p.order, m.conn, X, np.zeros(3 * m.nnodes), t, C, rowp, cols,
+                            rho=RHO, alpha=kc, gamma=mc)
+    if gc != 0.0:
+        assert O.assemble_geo(p.order, m.conn, X, u, t, C, rowp, cols, A, gc=gc) == 0
+    O.apply_bc_mat(3, p.bc_nodes, np.full(p.bc_nodes.size, 7, np.int32), rowp, cols, A)
+    return rowp, cols, A
+
+
+@pytest.mark.parametrize("order,shape", [(2, (4, 3, 3)), (3, (2, 2, 2))])
+def test_assemble_mat_type_matches_oracle(order, shape):
+    """assembleMatType (stiffness, mass, geometric stiffness at a random
+    state) and assembleMatCombo against the oracle's element-by-element
+    assembly (TACSAssembler::assembleMatType / assembleMatCombo), Dirichlet
+    rows zeroed with a unit diagonal."""
+    p, X, t = _distorted(order, shape, 31 + order)
+    a = p.assembler
+    rng = np.random.default_rng(17)
+    u = 1e-3 * rng.standard_normal(3 * p.mesh.nnodes)
+    uv = a.createVec()
+    uv.getArray()[:] = u
+    a.setVariables(uv)
+    for types, coefs in (((TACS.STIFFNESS_MATRIX,), (1.0, 0.0, 0.0)),
+                         ((TACS.MASS_MATRIX,), (0.0, 1.0, 0.0)),
+                         ((TACS.GEOMETRIC_STIFFNESS_MATRIX,), (0.0, 0.0, 1.0)),
+                         ((TACS.STIFFNESS_MATRIX, TACS.GEOMETRIC_STIFFNESS_MATRIX),
+                          (1.5, 0.0, -0.75))):
+        if len(types) == 1:
+            a.assembleMatType(types[0], p.mat)
+        else:
+            a.assembleMatCombo(types[0], coefs[0], types[1], coefs[2], p.mat)
+        rowp, cols, A = p.mat.getArrays()
+        orp, ocols, oA = _oracle_mat(p, X, t, u, *coefs)
+        assert np.array_equal(rowp, orp) and np.array_equal(cols, ocols)
+        # normwise over the non-identity rows
+        scale = np.abs(oA).max() if coefs != (0.0, 0.0, 1.0) else np.abs(oA[np.abs(oA) != 1.0]).max()
+        assert np.abs(A.reshape(-1) - oA).max() <= 1e-10 * scale, types
+
+
+def test_assemble_mat_type_shells():
+    """Shell stiffness through assembleMatType equals the Jacobian with
+    alpha = 1; the shell mass and geometric stiffness are reported as not
+    built."""
+    from tacs_amd.problems import ShellProblem
+
+    p = ShellProblem(8)
+    a = p.assembler
+    a.zeroVariables()
+    a.assembleJacobian(1.0, 0.0, 0.0, None, p.mat)
+    _, _, Aj = p.mat.getArrays()
+    a.assembleMatType(TACS.STIFFNESS_MATRIX, p.mat)
+    _, _, As = p.mat.getArrays()
+    assert np.array_equal(Aj, As)
+    with pytest.raises(Exception):
+        a.assembleMatType(TACS.MASS_MATRIX, p.mat)

@@ -86,6 +86,35 @@ def test_solid_matvec_matches_reference(key):
     assert np.abs(m[1::3]).max() == 0.0 and np.abs(m[2::3]).max() == 0.0
 
 
+@pytest.mark.parametrize("key", ["hex8_0", "hex8_1", "hex27_0", "hex27_1"])
+def test_solid_mat_types_match_reference(key):
+    """Stiffness, mass and geometric-stiffness element matrices
+    (TACSElement3D::getMatType) of the reference build against the oracle;
+    the geometric stiffness is symmetric, acts per displacement component
+    and annihilates rigid translations (sum_b grad N_b = 0)."""
+    order = 2 if key.startswith("hex8") else 3
+    X, u, t = FIX[key + "_X"], FIX[key + "_u"], float(FIX[key + "_t"][0])
+    ref = FIX[key + "_mattype"]
+    C = O.iso_C21(70e9, 0.3)
+    z = np.zeros_like(u)
+    _, K = O.solid_element(order, X, z, C, t=t, rho=2700.0, alpha=1.0, gamma=0.0, want_res=False)
+    _, M = O.solid_element(order, X, z, C, t=t, rho=2700.0, alpha=0.0, gamma=1.0, want_res=False)
+    G = O.solid_geo_element(order, X, u, C, t=t)
+    for k, mine in enumerate((K, M, G)):
+        assert np.abs(mine.reshape(-1) - ref[k]).max() <= 1e-13 * np.abs(ref[k]).max()
+    assert np.abs(G - G.T).max() <= 1e-14 * np.abs(G).max()
+    for j in range(3):
+        e = np.zeros(3)
+        e[j] = 1.0
+        assert np.abs(G @ np.tile(e, order ** 3)).max() <= 1e-12 * np.abs(G).max()
+    # no coupling between displacement components
+    Gb = G.reshape(order ** 3, 3, order ** 3, 3)
+    assert np.abs(Gb[:, 0, :, 1]).max() == 0.0 and np.abs(Gb[:, 1, :, 2]).max() == 0.0
+    # linear in the state
+    assert np.allclose(O.solid_geo_element(order, X, 2.5 * u, C, t=t), 2.5 * G, rtol=1e-13,
+                       atol=1e-13 * np.abs(G).max())
+
+
 @pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built (no reference tree)")
 def test_solid_element_fresh_inputs_against_reference_build():
     R = ctypes.CDLL(REF, mode=ctypes.RTLD_GLOBAL)
