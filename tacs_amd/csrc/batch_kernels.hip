@@ -259,6 +259,9 @@ __global__ __launch_bounds__(256) void bdcgs_update(size_t n, int R, int k1,
       at[e] = 0.0;
       ok[e] = t0 + (size_t)e * 256 < N;
     }
+    // four basis vectors per pass keep 16 loads per thread in flight (small
+    // systems have too few waves to cover the memory latency otherwise)
+#pragma unroll 4
     for (int v = 0; v < k; v++) {
       const double* w = v < k1 ? W1 + (size_t)v * ldw : W2 + (size_t)(v - k1) * ldw;
       const double cs = cf[v], ct = cf[k + v];

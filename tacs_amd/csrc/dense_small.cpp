@@ -33,9 +33,14 @@ inline void house(int len, const double* x, double* v, double& beta) {
   for (int i = 0; i < len; i++) v[i] /= v0;
 }
 
-// Reduce A (row-major n x n) to upper Hessenberg form in place by similarity.
-void hessenberg(int n, double* A) {
+// Reduce A (row-major n x n) to upper Hessenberg form in place by similarity;
+// when Q is not NULL it receives the orthogonal Q (row-major) with
+// A_in = Q A_out Q^T.
+void hessenberg(int n, double* A, double* Q = NULL) {
   std::vector<double> x(n), v(n);
+  if (Q)
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) Q[i * n + j] = (i == j) ? 1.0 : 0.0;
   for (int k = 0; k + 2 < n; k++) {
     int len = n - k - 1;
     for (int i = 0; i < len; i++) x[i] = A[(k + 1 + i) * n + k];
@@ -57,6 +62,15 @@ void hessenberg(int n, double* A) {
       for (int j = 0; j < len; j++) A[i * n + k + 1 + j] -= s * v[j];
     }
     for (int i = k + 2; i < n; i++) A[i * n + k] = 0.0;
+    if (Q) {
+      // Q <- Q (I - beta v v^T) on columns k+1..n-1
+      for (int i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int j = 0; j < len; j++) s += Q[i * n + k + 1 + j] * v[j];
+        s *= beta;
+        for (int j = 0; j < len; j++) Q[i * n + k + 1 + j] -= s * v[j];
+      }
+    }
   }
 }
 
@@ -168,37 +182,50 @@ int hqr_eigvals(int n, double* H, double* wr, double* wi) {
   return 0;
 }
 
-// Inverse iteration for the eigenvector of G (row-major n x n) at lambda
-// (complex): LU with partial pivoting of G - lambda' I, lambda' slightly
-// perturbed, four solves from a fixed start vector.
-void inverse_iteration(int n, const double* G, cplx lambda, double gnorm, std::vector<cplx>& v) {
+// Inverse iteration on an upper Hessenberg H (row-major n x n) at lambda
+// (complex): LU of H - lambda' I with partial pivoting between neighbouring
+// rows (the only rows with entries in a pivot column), O(n^2) per vector
+// instead of the O(n^3) of a dense factorisation; lambda' = lambda slightly
+// perturbed, four solves from a fixed start vector s of the original basis
+// (Q^T s here, so the iterates are those of the dense iteration in that basis).
+void inverse_iteration_hess(int n, const double* Hh, const double* Q, cplx lambda, double gnorm,
+                            std::vector<cplx>& v) {
   std::vector<cplx> M((size_t)n * n);
   cplx lam = lambda + cplx(1e-10 * gnorm, 0.0);
   for (int i = 0; i < n; i++)
-    for (int j = 0; j < n; j++) M[(size_t)i * n + j] = G[(size_t)i * n + j] - (i == j ? lam : 0.0);
-  std::vector<int> piv(n);
-  for (int k = 0; k < n; k++) {
-    int p = k;
-    for (int i = k + 1; i < n; i++)
-      if (std::abs(M[(size_t)i * n + k]) > std::abs(M[(size_t)p * n + k])) p = i;
-    piv[k] = p;
-    if (p != k)
-      for (int j = 0; j < n; j++) std::swap(M[(size_t)k * n + j], M[(size_t)p * n + j]);
-    cplx d = M[(size_t)k * n + k];
-    if (std::abs(d) == 0.0) d = M[(size_t)k * n + k] = cplx(1e-14 * gnorm, 0.0);
-    for (int i = k + 1; i < n; i++) {
-      cplx f = M[(size_t)i * n + k] / d;
-      M[(size_t)i * n + k] = f;
-      for (int j = k + 1; j < n; j++) M[(size_t)i * n + j] -= f * M[(size_t)k * n + j];
+    for (int j = 0; j < n; j++)
+      M[(size_t)i * n + j] = (j >= i - 1) ? Hh[(size_t)i * n + j] - (i == j ? lam : 0.0) : 0.0;
+  std::vector<cplx> fac(n, cplx(0.0, 0.0));
+  std::vector<char> swp(n, 0);
+  for (int k = 0; k + 1 < n; k++) {
+    cplx* rk = &M[(size_t)k * n];
+    cplx* r1 = &M[(size_t)(k + 1) * n];
+    if (std::abs(r1[k]) > std::abs(rk[k])) {
+      for (int j = k; j < n; j++) std::swap(rk[j], r1[j]);
+      swp[k] = 1;
     }
+    if (std::abs(rk[k]) == 0.0) rk[k] = cplx(1e-14 * gnorm, 0.0);
+    cplx f = r1[k] / rk[k];
+    fac[k] = f;
+    r1[k] = 0.0;
+    for (int j = k + 1; j < n; j++) r1[j] -= f * rk[j];
   }
+  if (std::abs(M[(size_t)(n - 1) * n + n - 1]) == 0.0)
+    M[(size_t)(n - 1) * n + n - 1] = cplx(1e-14 * gnorm, 0.0);
+  // the fixed start vector s of the original basis, as Q^T s
+  std::vector<cplx> s0(n);
+  for (int i = 0; i < n; i++) s0[i] = cplx(1.0 + 0.1 * ((i * 7) % 11), 0.05 * (i % 3));
   v.assign(n, cplx(0.0, 0.0));
-  for (int i = 0; i < n; i++) v[i] = cplx(1.0 + 0.1 * ((i * 7) % 11), 0.05 * (i % 3));
+  for (int c = 0; c < n; c++) {
+    cplx acc(0.0, 0.0);
+    for (int r = 0; r < n; r++) acc += Q[(size_t)r * n + c] * s0[r];
+    v[c] = acc;
+  }
   for (int it = 0; it < 4; it++) {
-    // rows were swapped whole during the factorisation: permute, then solve L
-    for (int k = 0; k < n; k++) std::swap(v[k], v[piv[k]]);
-    for (int k = 0; k < n; k++)
-      for (int i = k + 1; i < n; i++) v[i] -= M[(size_t)i * n + k] * v[k];
+    for (int k = 0; k + 1 < n; k++) {
+      if (swp[k]) std::swap(v[k], v[k + 1]);
+      v[k + 1] -= fac[k] * v[k];
+    }
     for (int i = n - 1; i >= 0; i--) {
       cplx s = v[i];
       for (int j = i + 1; j < n; j++) s -= M[(size_t)i * n + j] * v[j];
@@ -223,8 +250,9 @@ extern "C" {
 int tacs_dense_smallest_eigvecs(int n, const double* G, int k, double* V, double* wr,
                                 double* wi) {
   if (n <= 0) return 0;
-  std::vector<double> H(G, G + (size_t)n * n);
-  hessenberg(n, H.data());
+  std::vector<double> H(G, G + (size_t)n * n), Q((size_t)n * n);
+  hessenberg(n, H.data(), Q.data());
+  const std::vector<double> Hh(H);  // the QR iteration overwrites H
   std::vector<double> er(n), ei(n);
   if (hqr_eigvals(n, H.data(), er.data(), ei.data())) return -1;
   std::vector<int> idx(n);
@@ -243,11 +271,18 @@ int tacs_dense_smallest_eigvecs(int n, const double* G, int k, double* V, double
   for (size_t i = 0; i < (size_t)n * n; i++) gnorm = std::max(gnorm, fabs(G[i]));
   if (gnorm == 0.0) gnorm = 1.0;
   int kout = 0;
-  std::vector<cplx> v;
+  std::vector<cplx> v, vh;
   for (int s = 0; s < n && kout < k; s++) {
     int i = idx[s];
     if (ei[i] < 0.0) continue;  // the conjugate of a pair already taken
-    inverse_iteration(n, G, cplx(er[i], ei[i]), gnorm, v);
+    // eigenvector of the Hessenberg form, then G's: v = Q v_h
+    inverse_iteration_hess(n, Hh.data(), Q.data(), cplx(er[i], ei[i]), gnorm, vh);
+    v.assign(n, cplx(0.0, 0.0));
+    for (int r = 0; r < n; r++) {
+      cplx acc(0.0, 0.0);
+      for (int c = 0; c < n; c++) acc += Q[(size_t)r * n + c] * vh[c];
+      v[r] = acc;
+    }
     if (ei[i] == 0.0) {
       // real eigenvector: rotate the phase so the largest entry is real
       int p = 0;

@@ -107,3 +107,33 @@ def test_mesh_numbering_matches_reference_test_convention():
     xyz[..., 0], xyz[..., 1], xyz[..., 2] = np.meshgrid(x, x, x, indexing="ij")
     assert np.array_equal(m.X, xyz.reshape(-1, 3))
     assert np.array_equal(m.nodes_on_x(0), np.sort(ids[0].reshape(-1)))
+
+
+@pytest.mark.parametrize("n,k", [(12, 4), (81, 20), (151, 30)])
+def test_dense_smallest_eigvecs(n, k):
+    """Host eigen-solver of the recycling GMRES (Hessenberg reduction,
+    Francis QR, inverse iteration on the Hessenberg form): the returned
+    columns span an invariant subspace whose eigenvalues are the k of
+    smallest modulus (numpy's), complex pairs kept whole."""
+    import ctypes
+
+    from tacs_amd import lib
+
+    rng = np.random.default_rng(n)
+    G = rng.standard_normal((n, n)) + 0.3 * n * np.eye(n)
+    V = np.zeros((k + 1) * n)
+    wr, wi = np.zeros(n), np.zeros(n)
+    kout = lib().tacs_dense_smallest_eigvecs(n, G.ctypes.data, k, V.ctypes.data, wr.ctypes.data,
+                                             wi.ctypes.data)
+    assert k <= kout <= k + 1
+    Vm = V[:kout * n].reshape(kout, n).T
+    T = np.linalg.lstsq(Vm, G @ Vm, rcond=None)[0]
+    assert np.linalg.norm(G @ Vm - Vm @ T) <= 1e-12 * np.linalg.norm(G @ Vm)
+    ev = np.linalg.eigvals(G)
+    ev = ev[np.argsort(np.abs(ev), kind="stable")]
+    got = np.sort_complex(np.linalg.eigvals(T))
+    assert np.abs(got - np.sort_complex(ev[:kout])).max() <= 1e-10 * np.abs(ev).max()
+    # all eigenvalues, ascending modulus
+    w = wr + 1j * wi
+    assert np.all(np.diff(np.abs(w)) >= -1e-12 * np.abs(w).max())
+    assert np.abs(np.sort_complex(w) - np.sort_complex(ev)).max() <= 1e-10 * np.abs(ev).max()
