@@ -1,4 +1,9 @@
-"""Compliance of a solid SIMP problem as a differentiable torch function.
+"""Structural functions of the GPU path as differentiable torch functions.
+
+SIMPCompliance: the compliance of a solid SIMP problem in the element
+densities.  ThicknessFunction: the compliance or KS failure of a solid or shell
+problem in its per-element thickness variables (the shells' thickness DVs of
+TACSIsoShellConstitutive, the solids' t of TACSSolidConstitutive).
 
 Forward: t = (Emin + x^p (E0 - Emin)) / E0 (the reference's solid
 constitutive model is linear in its thickness variable t,
@@ -99,3 +104,85 @@ class SIMPCompliance(torch.autograd.Function):
 
 def simp_compliance(x, prob):
     return SIMPCompliance.apply(x, prob)
+
+
+class ThicknessFunctionProblem:
+    """f(t) and df/dt for one DV per element (tNum = element index) of a
+    SolidProblem or ShellProblem: f is the compliance or the KS failure
+    (KS_CONTINUOUS, ksWeight).  Forward: setDesignVars, assembleJacobian, the
+    solve, evalFunctions.  Backward: addSVSens, the adjoint solve with the
+    same matrix, addDVSens - addAdjointResProducts (TACSAssembler.cpp:4767,
+    :5033), the element strains recomputed in the kernels from u and psi."""
+
+    def __init__(self, problem, func="ks", ks_weight=100.0, rtol=1e-8, m=60, mixed=True,
+                 deflation=20, restarts=100000):
+        self.prob = problem
+        a = problem.assembler
+        self.dv = a.createDesignVec()
+        self.dfdt = a.createDesignVec()
+        self.pc = TACS.Pc(problem.mat)
+        if mixed:
+            self.ksm = TACS.MixedKSM(problem.mat, self.pc, m, restarts, deflation=deflation)
+        else:
+            self.ksm = TACS.KSM(problem.mat, self.pc, min(m, 512), restarts)
+        self.ksm.setTolerances(rtol, 1e-30)
+        if func == "ks":
+            self.func = functions.KSFailure(a, ksWeight=ks_weight)
+        elif func == "compliance":
+            self.func = functions.Compliance(a)
+        else:
+            raise ValueError(f"unknown function {func!r}")
+        self.iters = []
+
+    def forward(self, t):
+        a = self.prob.assembler
+        t = t.to(torch.float64).contiguous()
+        _sync()
+        self.dv.copyFromDevice(t.data_ptr(), t.numel())
+        a.setDesignVars(self.dv)
+        a.zeroVariables()
+        a.assembleJacobian(1.0, 0.0, 0.0, self.prob.res, self.prob.mat)
+        self.pc.factor()
+        if self.ksm.solve(self.prob.res, self.prob.u) != 1:
+            raise RuntimeError("forward solve did not converge")
+        self.iters.append(self.ksm.getIterCount())
+        self.prob.u.scale(-1.0)
+        a.setVariables(self.prob.u)
+        return a.evalFunctions([self.func])[0]
+
+    def gradient_t(self):
+        """df/dt on the device (torch tensor) at the current state."""
+        a = self.prob.assembler
+        dfdu = a.createVec()
+        a.addSVSens([self.func], [dfdu])
+        a.applyBCs(dfdu)
+        psi = a.createVec()
+        if self.ksm.solve(dfdu, psi) != 1:
+            raise RuntimeError("adjoint solve did not converge")
+        self.iters.append(self.ksm.getIterCount())
+        self.dfdt.zeroEntries()
+        a.addDVSens([self.func], [self.dfdt], 1.0)
+        a.addAdjointResProducts([psi], [self.dfdt], -1.0)
+        n = self.dfdt.getSize()
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+        self.dfdt.copyToDevice(out.data_ptr(), n)
+        _sync()
+        return out
+
+
+class ThicknessFunction(torch.autograd.Function):
+    """f(t) for per-element thicknesses t (1-D float64 CUDA tensor)."""
+
+    @staticmethod
+    def forward(ctx, t, prob):
+        f = prob.forward(t)
+        ctx.prob = prob
+        return torch.tensor(f, dtype=torch.float64, device=t.device)
+
+    @staticmethod
+    def backward(ctx, grad):
+        return grad * ctx.prob.gradient_t(), None
+
+
+def thickness_function(t, prob):
+    return ThicknessFunction.apply(t, prob)
