@@ -253,11 +253,15 @@ __global__ __launch_bounds__(256) void bdcgs_update(size_t n, int R, int k1,
     const double* cf = &coef[(size_t)(t0 % R) * sC];
     double as[BDU_E], at[BDU_E];
     bool ok[BDU_E];
+    size_t te[BDU_E];
 #pragma unroll
     for (int e = 0; e < BDU_E; e++) {
       as[e] = 0.0;
       at[e] = 0.0;
       ok[e] = t0 + (size_t)e * 256 < N;
+      // entries past the end re-read the last one and are not written: the
+      // loads stay unconditional, so the unrolled passes issue them together
+      te[e] = ok[e] ? t0 + (size_t)e * 256 : N - 1;
     }
     // four basis vectors per pass keep 16 loads per thread in flight (small
     // systems have too few waves to cover the memory latency otherwise)
@@ -266,12 +270,11 @@ __global__ __launch_bounds__(256) void bdcgs_update(size_t n, int R, int k1,
       const double* w = v < k1 ? W1 + (size_t)v * ldw : W2 + (size_t)(v - k1) * ldw;
       const double cs = cf[v], ct = cf[k + v];
 #pragma unroll
-      for (int e = 0; e < BDU_E; e++)
-        if (ok[e]) {
-          const double x = w[t0 + (size_t)e * 256];
-          as[e] += x * cs;
-          at[e] += x * ct;
-        }
+      for (int e = 0; e < BDU_E; e++) {
+        const double x = w[te[e]];
+        as[e] += x * cs;
+        at[e] += x * ct;
+      }
     }
     const double io = cf[2 * k], tau = cf[2 * k + 1];
 #pragma unroll
