@@ -305,6 +305,20 @@ int tacs_hip_dot(int prec, size_t n, const void* wprev, const double* hprev, voi
                  hipStream_t stream);
 int tacs_hip_mdot(int prec, size_t n, int k, const void* W, size_t ldw, const void* q,
                   void* scratch, double* h, int accumulate, hipStream_t stream);
+/* tacs_hip_mdot without the scratch argument, for callers that cannot
+   allocate (code under HIP graph capture, serving loops, harnesses that forbid
+   hipMalloc).  No counterpart in the reference.  It allocates nothing: the
+   block partials live in a static device array of the library (512 KiB on
+   every GPU it runs on, held until the process ends), whose address the first
+   call made with a GPU current resolves, so make that first call outside any
+   capture.  Limit: 1 <= k <= 32 vectors (n is unbounded); other k, or prec
+   other than 64 / 32, return TACS_HIP_ERR_ARG with a message in
+   tacs_last_error().  Every call on one GPU shares that array: calls that
+   overlap on one GPU (several streams or threads at once) race and give wrong
+   dots; calls ordered on one stream are safe.  Nothing in the array needs to
+   start at zero (each call writes every partial it reads). */
+int tacs_hip_mdot_noalloc(int prec, size_t n, int k, const void* W, size_t ldw, const void* q,
+                          double* h, int accumulate, hipStream_t stream);
 int tacs_hip_cgs_update(int prec, size_t n, int k, const void* W, size_t ldw, const double* c,
                         const double* c2, void* q, hipStream_t stream);
 int tacs_hip_lincomb(int prec_w, int prec_y, size_t n, int k, const void* W, size_t ldw,

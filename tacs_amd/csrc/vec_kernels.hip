@@ -3,6 +3,8 @@
 // (so Arnoldi steps never wait on the host), the fused modified-Gram-Schmidt step, the
 // classical-Gram-Schmidt multi-dot / update pair, and the Givens update of
 // the Hessenberg column (GMRES::solve, src/bpmat/KSM.cpp:785-958).
+#include <mutex>
+
 #include "tacs_hip_common.h"
 
 #define RED_BLOCKS 2048
@@ -394,6 +396,70 @@ static inline int vgrid(size_t n) {
   return (int)g;
 }
 
+// Reduction partials of the allocation-free multi-dot (tacs_hip_mdot_noalloc):
+// RED_BLOCKS x RED_MAXK doubles (512 KiB), one copy on every GPU the library
+// runs on, held until the process ends.  Host code never passes this name to
+// a kernel: the device address of each GPU's copy is resolved once, with that
+// GPU current, by scratch_ptr.
+static __device__ __align__(256) double scratch[RED_BLOCKS * RED_MAXK];
+static_assert(sizeof(scratch) == sizeof(RedScratch), "scratch holds one RedScratch");
+
+int tacs_set_error(const char* fmt, ...);
+
+#define SCRATCH_MAX_DEVICES 64
+static void* g_scratch_dev[SCRATCH_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// Device address of the current GPU's copy of scratch (resolved on the first
+// call made with that GPU current, then cached).
+static int scratch_ptr(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    tacs_set_error("tacs_hip_mdot_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    return (int)e;
+  }
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+    tacs_set_error("tacs_hip_mdot_noalloc: device %d beyond the %d this library tracks", dev,
+                   SCRATCH_MAX_DEVICES);
+    return TACS_HIP_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = NULL;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      tacs_set_error("tacs_hip_mdot_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                     hipGetErrorString(e));
+      return e != hipSuccess ? (int)e : TACS_HIP_ERR_ARG;
+    }
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+static int mdot_launch(int prec, size_t n, int k, const void* W, size_t ldw, const void* q,
+                       RedScratch* rs, double* h, int accumulate, hipStream_t stream) {
+  const int nb = red_blocks(n);
+#define MD_LAUNCH(T, KM)                                                                       \
+  hipLaunchKernelGGL((mdot_partial<T, KM>), dim3(nb), dim3(RED_THREADS), 0, stream, n, k,      \
+                     (const T*)W, ldw, (const T*)q, rs)
+  if (prec == 64) {
+    if (k <= 8) MD_LAUNCH(double, 8);
+    else if (k <= 16) MD_LAUNCH(double, 16);
+    else MD_LAUNCH(double, 32);
+  } else {
+    if (k <= 8) MD_LAUNCH(float, 8);
+    else if (k <= 16) MD_LAUNCH(float, 16);
+    else MD_LAUNCH(float, 32);
+  }
+#undef MD_LAUNCH
+  hipLaunchKernelGGL(mdot_finalize, dim3(1), dim3(RED_THREADS), 0, stream, nb, k, rs, h,
+                     accumulate);
+  return (int)hipGetLastError();
+}
+
 extern "C" {
 
 size_t tacs_hip_red_scratch_bytes() { return sizeof(RedScratch); }
@@ -418,24 +484,27 @@ int tacs_hip_dot(int prec, size_t n, const void* wprev, const double* hprev, voi
 int tacs_hip_mdot(int prec, size_t n, int k, const void* W, size_t ldw, const void* q,
                   void* scratch, double* h, int accumulate, hipStream_t stream) {
   if (k > RED_MAXK || k < 1) return TACS_HIP_ERR_ARG;
-  RedScratch* rs = (RedScratch*)scratch;
-  const int nb = red_blocks(n);
-#define MD_LAUNCH(T, KM)                                                                       \
-  hipLaunchKernelGGL((mdot_partial<T, KM>), dim3(nb), dim3(RED_THREADS), 0, stream, n, k,      \
-                     (const T*)W, ldw, (const T*)q, rs)
-  if (prec == 64) {
-    if (k <= 8) MD_LAUNCH(double, 8);
-    else if (k <= 16) MD_LAUNCH(double, 16);
-    else MD_LAUNCH(double, 32);
-  } else {
-    if (k <= 8) MD_LAUNCH(float, 8);
-    else if (k <= 16) MD_LAUNCH(float, 16);
-    else MD_LAUNCH(float, 32);
+  return mdot_launch(prec, n, k, W, ldw, q, (RedScratch*)scratch, h, accumulate, stream);
+}
+
+// The same multi-dot with its partials in this file's static device array:
+// allocates nothing, for callers that cannot allocate (graph capture,
+// serving loops).
+int tacs_hip_mdot_noalloc(int prec, size_t n, int k, const void* W, size_t ldw, const void* q,
+                          double* h, int accumulate, hipStream_t stream) {
+  if (k > RED_MAXK || k < 1) {
+    tacs_set_error("tacs_hip_mdot_noalloc: k = %d vectors; its static scratch holds 1 to %d", k,
+                   RED_MAXK);
+    return TACS_HIP_ERR_ARG;
   }
-#undef MD_LAUNCH
-  hipLaunchKernelGGL(mdot_finalize, dim3(1), dim3(RED_THREADS), 0, stream, nb, k, rs, h,
-                     accumulate);
-  return (int)hipGetLastError();
+  if (prec != 64 && prec != 32) {
+    tacs_set_error("tacs_hip_mdot_noalloc: prec %d (64 or 32)", prec);
+    return TACS_HIP_ERR_ARG;
+  }
+  void* rs = NULL;
+  int e = scratch_ptr(&rs);
+  if (e) return e;
+  return mdot_launch(prec, n, k, W, ldw, q, (RedScratch*)rs, h, accumulate, stream);
 }
 
 int tacs_hip_cgs_update(int prec, size_t n, int k, const void* W, size_t ldw, const double* c,
