@@ -672,6 +672,19 @@ class TACSAssembler : public TACSObject {
   int matFreeProduct(const double* data, double* temp, const double* x, double* y,
                      int accumulate, int bcs);
 
+  // Arithmetic of the element kernels that build the Jacobian of
+  // assembleJacobian: 64 (default) or 32.  With 32 the element
+  // matrices are computed in FP32 (shells: TACSQuad4Shell's MITC algebra;
+  // solids: the isotropic hexahedra) and summed into the block-CSR values in
+  // FP64; residuals, functions and sensitivities stay FP64.  No counterpart
+  // in the reference (its TacsScalar is double throughout).
+  int setElementPrecision(int prec);
+  int getElementPrecision() { return elemPrec; }
+  // Solid Jacobian kernel: 0 picks by a measured table, 1 the node-centric
+  // row kernel, 2 the element blocks + row gather (isotropic material only).
+  int setJacobianKernel(int kernel);
+  int jacobianKernelInUse();  // 1 or 2: what assembleJacobian launches now
+
   // Device view of the element-level state used by the kernels
   const double* devNodes() { return d_X; }
   const double* devVars() { return d_u; }
@@ -720,6 +733,12 @@ class TACSAssembler : public TACSObject {
   int *h_rowp, *h_cols;
   long nnzb;
   double jac_ms;
+  int elemPrec, jacKernel;
+  // element-block path: per matrix slot the list of element blocks
+  // (e NPE^2 + a NPE + b) summed into it, ascending element; the blocks
+  int *d_cptr, *d_clist;
+  void* d_keb;
+  int buildBlockLists();
 };
 
 // src/TACSMatrixFreeMat.h: the Jacobian (or stiffness / mass matrix) of the

@@ -79,6 +79,22 @@ int tacs_hip_solid_jacobian(int order, int nnodes, int nelem, const int* n2e_ptr
                             double gamma, const int* rowp, const int* cols, const int* bcvars,
                             double* G, double* A, float* A32, int* status, hipStream_t stream);
 
+/* Element-centric Jacobian of the isotropic solid family, the default path
+   of TACSAssembler::assembleJacobian (same reference loop as
+   tacs_hip_solid_jacobian).  tacs_hip_solid_elem_blocks writes every
+   element's 3x3 blocks K_ab = alpha K + gamma M (local nodes a, b; blocks
+   row-major) to ke[e][a*npe + b][9], computed in FP64 (prec 64, ke double)
+   or FP32 (prec 32, ke float) arithmetic; tacs_hip_gather_solid_blocks sums
+   them into the block-CSR values in FP64 (A and/or its FP32 copy A32), slot
+   s taking ke[clist[c]] for c in [cptr[s], cptr[s+1]) in that order, with
+   the Dirichlet rows of bcvars applied.  order 2 or 3; isotropic material. */
+int tacs_hip_solid_elem_blocks(int order, int prec, int nelem, const int* conn, const double* X,
+                               const double* telem, const SolidMaterial* mat, double alpha,
+                               double gamma, void* ke, hipStream_t stream);
+int tacs_hip_gather_solid_blocks(int prec, int nrows, const int* rowp, const int* cols,
+                                 const int* cptr, const int* clist, const void* ke,
+                                 const int* bcvars, double* A, float* A32, hipStream_t stream);
+
 /* Element-matrix combinations of the solid family straight into block-CSR:
    kc K + mc M + gc G(u) with K the stiffness, M the consistent mass and G
    the geometric stiffness at the state u (replaces the element loops of
@@ -509,6 +525,13 @@ int tacs_assembler_set_elements(tacs_obj a, tacs_obj* elements);
 int tacs_assembler_add_bcs(tacs_obj a, int nnodes, const int* nodes, int nbcs, const int* vars,
                            const double* vals);
 int tacs_assembler_initialize(tacs_obj a);
+/* TACSAssembler::setElementPrecision (64 or 32) and setJacobianKernel (0
+   measured choice, 1 node-centric rows, 2 element blocks + gather); no
+   counterpart in the reference.  tacs_assembler_jacobian_kernel returns the
+   solid Jacobian kernel assembleJacobian launches (1 or 2; 0 for shells). */
+int tacs_assembler_set_element_precision(tacs_obj a, int prec);
+int tacs_assembler_set_jacobian_kernel(tacs_obj a, int kernel);
+int tacs_assembler_jacobian_kernel(tacs_obj a);
 int tacs_assembler_set_aux_elements(tacs_obj a, tacs_obj aux);
 int tacs_assembler_get_info(tacs_obj a, int* vars_per_node, int* num_nodes,
                             int* num_elements, int* num_design_vars);

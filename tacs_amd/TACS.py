@@ -43,12 +43,17 @@ class Vec(TACSHandle):
         return b.value
 
     def getArray(self):
-        """Host view of the vector; modifications are uploaded before the next
-        device operation (TACSBVec::getArray semantics)."""
+        """Host view of the vector (TACSBVec::getArray).  Writes through the
+        view reach the device before every later device read, until a device
+        operation writes the vector (solve output, axpy, scale, zeroEntries,
+        assembly into it); after such a write the view is stale and getArray
+        must be called again to see the device values."""
         n = self.getSize()
         p = lib().tacs_vec_get_array(self.ptr)
         if n == 0:
             return np.zeros(0)
+        if not p:
+            check(1, "Vec.getArray")
         return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_double)), shape=(n,))
 
     def getDevicePointer(self):
@@ -373,6 +378,23 @@ class Assembler(TACSHandle):
 
     def initialize(self):
         check(lib().tacs_assembler_initialize(self.ptr), "initialize")
+
+    def setElementPrecision(self, prec):
+        """Arithmetic of the element kernels of assembleJacobian: 64 or 32
+        (FP32 element matrices summed into the FP64 block-CSR values;
+        residuals, functions and sensitivities stay FP64).  No counterpart in
+        the reference."""
+        check(lib().tacs_assembler_set_element_precision(self.ptr, int(prec)),
+              "setElementPrecision")
+
+    def setJacobianKernel(self, kernel):
+        """Solid Jacobian kernel: 0 measured choice, 1 node-centric rows, 2
+        element blocks + row gather."""
+        check(lib().tacs_assembler_set_jacobian_kernel(self.ptr, int(kernel)), "setJacobianKernel")
+
+    def getJacobianKernel(self):
+        """The solid Jacobian kernel assembleJacobian launches (1 or 2)."""
+        return lib().tacs_assembler_jacobian_kernel(self.ptr)
 
     def setAuxElements(self, elems=None):
         self._aux = elems

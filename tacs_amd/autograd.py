@@ -13,6 +13,12 @@ solve K psi = dC/du (addSVSens + applyBCs), then
 dC/dt = addDVSens - addAdjointResProducts (TACSAssembler.cpp:4767, :5033) in
 the GPU kernels, which recompute the element strains from u and psi instead
 of storing them, and the chain rule dt/dx = p x^(p-1) (E0 - Emin)/E0.
+
+The problem objects hold one state (design variables, assembled matrix,
+displacements).  Every forward stamps it with a new version; a backward whose
+forward is no longer the latest (f(x1) + f(x2), a line search, a retained
+graph) first runs its own forward again at the saved point, so the gradient
+is always taken at the point its value was.
 """
 import torch
 
@@ -46,9 +52,11 @@ class SolidComplianceProblem:
         self.ksm.setTolerances(rtol, 1e-30)
         self.comp = functions.Compliance(a)
         self.iters = []
+        self.version = 0
 
     def forward(self, x):
         a = self.prob.assembler
+        self.version += 1
         t = simp_thickness(x, self.p, self.E0, self.Emin).to(torch.float64).contiguous()
         _sync()
         self.dv.copyFromDevice(t.data_ptr(), t.numel())
@@ -90,6 +98,7 @@ class SIMPCompliance(torch.autograd.Function):
     def forward(ctx, x, prob):
         c = prob.forward(x)
         ctx.prob = prob
+        ctx.version = prob.version
         ctx.save_for_backward(x)
         return torch.tensor(c, dtype=torch.float64, device=x.device)
 
@@ -97,6 +106,9 @@ class SIMPCompliance(torch.autograd.Function):
     def backward(ctx, grad):
         (x,) = ctx.saved_tensors
         prob = ctx.prob
+        if prob.version != ctx.version:  # another forward moved the state
+            prob.forward(x.detach())
+            ctx.version = prob.version
         dfdt = prob.gradient_t()
         dtdx = simp_dthickness(x, prob.p, prob.E0, prob.Emin)
         return grad * dfdt * dtdx, None
@@ -133,9 +145,11 @@ class ThicknessFunctionProblem:
         else:
             raise ValueError(f"unknown function {func!r}")
         self.iters = []
+        self.version = 0
 
     def forward(self, t):
         a = self.prob.assembler
+        self.version += 1
         t = t.to(torch.float64).contiguous()
         _sync()
         self.dv.copyFromDevice(t.data_ptr(), t.numel())
@@ -177,11 +191,18 @@ class ThicknessFunction(torch.autograd.Function):
     def forward(ctx, t, prob):
         f = prob.forward(t)
         ctx.prob = prob
+        ctx.version = prob.version
+        ctx.save_for_backward(t)
         return torch.tensor(f, dtype=torch.float64, device=t.device)
 
     @staticmethod
     def backward(ctx, grad):
-        return grad * ctx.prob.gradient_t(), None
+        (t,) = ctx.saved_tensors
+        prob = ctx.prob
+        if prob.version != ctx.version:  # another forward moved the state
+            prob.forward(t.detach())
+            ctx.version = prob.version
+        return grad * prob.gradient_t(), None
 
 
 def thickness_function(t, prob):

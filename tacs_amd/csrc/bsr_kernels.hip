@@ -340,9 +340,10 @@ __device__ inline void sell_slot(const TA* __restrict__ vs, long v, const T* __r
 }
 
 // U slots per iteration: every load of the U slots is issued before the
-// first block is multiplied (U = 2 for 3x3 blocks: about 2 % faster for FP64
-// values and 5 % for FP32 at 128^3 than one slot at a time); the blocks are
-// still added in slot order.
+// first block is multiplied (3x3 blocks: U = 2 for FP64 values, about 2 %
+// faster than one slot at a time at 128^3; U = 4 for FP32 values, 0.7-0.9 %
+// faster than U = 2; 6x6 blocks: U = 1); the blocks are still added in slot
+// order.
 template <int B, int U, typename TA, typename T>
 __global__ __launch_bounds__(256) void sell_spmv(int nslices, int nrows,
                                                  const int* __restrict__ width,

@@ -12,15 +12,29 @@ const char* tacs_get_error();
 void tacs_set_stream_impl(hipStream_t s);
 hipStream_t tacs_stream();
 
+// Every entry point starts with the error (and the halt of launches that a
+// failed allocation, copy or launch sets) cleared: tacs_last_error()
+// describes the most recent failing call.
 #define OBJ(T, h) (dynamic_cast<T*>((TACSObject*)(h)))
 #define REQUIRE(T, v, h)                                           \
+  tacs_clear_error();                                              \
   T* v = OBJ(T, h);                                                \
   if (!v) return tacs_set_error("%s: expected a %s handle", __func__, #T)
 #define REQUIRE_NULL(T, v, h)                                      \
+  tacs_clear_error();                                              \
   T* v = OBJ(T, h);                                                \
   if (!v) {                                                        \
     tacs_set_error("%s: expected a %s handle", __func__, #T);      \
     return NULL;                                                   \
+  }
+// device work needs the arrays initialize() uploads
+#define REQUIRE_INIT(a)                                                              \
+  if (!(a)->initialized)                                                             \
+  return tacs_set_error("%s: the assembler is not initialized", __func__)
+#define REQUIRE_INIT_NULL(a)                                                         \
+  if (!(a)->initialized) {                                                           \
+    tacs_set_error("%s: the assembler is not initialized", __func__);                \
+    return NULL;                                                                     \
   }
 
 static int last_error_code() {
@@ -28,7 +42,19 @@ static int last_error_code() {
   return (e && e[0]) ? 1 : 0;
 }
 void tacs_clear_error();
+int tacs_halted();
 static void clear_error() { tacs_clear_error(); }
+// An object whose construction failed (a device allocation or upload) is
+// destroyed and NULL returned, so no half-built object reaches a caller.
+template <class T>
+static tacs_obj built(T* o) {
+  if (o && (tacs_halted() || last_error_code())) {
+    o->incref();
+    o->decref();
+    return NULL;
+  }
+  return (TACSObject*)o;
+}
 
 extern "C" {
 
@@ -140,6 +166,18 @@ int tacs_assembler_initialize(tacs_obj h) {
   clear_error();
   return a->initialize();
 }
+int tacs_assembler_set_element_precision(tacs_obj h, int prec) {
+  REQUIRE(TACSAssembler, a, h);
+  return a->setElementPrecision(prec);
+}
+int tacs_assembler_set_jacobian_kernel(tacs_obj h, int kernel) {
+  REQUIRE(TACSAssembler, a, h);
+  return a->setJacobianKernel(kernel);
+}
+int tacs_assembler_jacobian_kernel(tacs_obj h) {
+  REQUIRE(TACSAssembler, a, h);
+  return a->jacobianKernelInUse();
+}
 int tacs_assembler_set_aux_elements(tacs_obj h, tacs_obj aux) {
   REQUIRE(TACSAssembler, a, h);
   a->setAuxElements(aux ? OBJ(TACSAuxElements, aux) : NULL);
@@ -155,24 +193,25 @@ int tacs_assembler_get_info(tacs_obj h, int* vpn, int* nn, int* ne, int* ndv) {
 }
 tacs_obj tacs_assembler_create_vec(tacs_obj h) {
   REQUIRE_NULL(TACSAssembler, a, h);
-  return (TACSObject*)a->createVec();
+  return built(a->createVec());
 }
 tacs_obj tacs_assembler_create_node_vec(tacs_obj h) {
   REQUIRE_NULL(TACSAssembler, a, h);
-  return (TACSObject*)a->createNodeVec();
+  return built(a->createNodeVec());
 }
 tacs_obj tacs_assembler_create_design_vec(tacs_obj h) {
   REQUIRE_NULL(TACSAssembler, a, h);
-  return (TACSObject*)a->createDesignVec();
+  return built(a->createDesignVec());
 }
 tacs_obj tacs_assembler_create_mat(tacs_obj h) {
   REQUIRE_NULL(TACSAssembler, a, h);
-  return (TACSObject*)a->createMat();
+  return built(a->createMat());
 }
 
 #define ASM_VEC_OP(name, call)                      \
   int name(tacs_obj h, tacs_obj v) {                \
     REQUIRE(TACSAssembler, a, h);                   \
+    REQUIRE_INIT(a);                                \
     REQUIRE(TACSBVec, x, v);                        \
     clear_error();                                  \
     a->call(x);                                     \
@@ -187,23 +226,26 @@ ASM_VEC_OP(tacs_assembler_get_design_vars, getDesignVars)
 
 int tacs_assembler_zero_variables(tacs_obj h) {
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   a->zeroVariables();
-  return 0;
+  return last_error_code();
 }
 int tacs_assembler_apply_bcs_vec(tacs_obj h, tacs_obj v) {
   REQUIRE(TACSAssembler, a, h);
   REQUIRE(TACSBVec, x, v);
   a->applyBCs((TACSVec*)x);
-  return 0;
+  return last_error_code();
 }
 int tacs_assembler_apply_bcs_mat(tacs_obj h, tacs_obj m) {
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   REQUIRE(TACSParallelMat, x, m);
   a->applyBCs((TACSMat*)x);
-  return 0;
+  return last_error_code();
 }
 int tacs_assembler_assemble_res(tacs_obj h, tacs_obj res, double lambda, int bcs) {
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   REQUIRE(TACSBVec, r, res);
   clear_error();
   a->assembleRes(r, lambda, bcs != 0);
@@ -213,6 +255,7 @@ int tacs_assembler_assemble_jacobian(tacs_obj h, double alpha, double beta, doub
                                      tacs_obj res, tacs_obj mat, int mat_or, double lambda,
                                      int bcs) {
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   REQUIRE(TACSParallelMat, m, mat);
   TACSBVec* r = res ? OBJ(TACSBVec, res) : NULL;
   clear_error();
@@ -221,6 +264,7 @@ int tacs_assembler_assemble_jacobian(tacs_obj h, double alpha, double beta, doub
 }
 int tacs_assembler_eval_functions(tacs_obj h, int n, tacs_obj* funcs, double* vals) {
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   TACSFunction* f[64];
   if (n > 64) return tacs_set_error("evalFunctions: at most 64 functions");
   for (int i = 0; i < n; i++) f[i] = OBJ(TACSFunction, funcs[i]);
@@ -230,6 +274,7 @@ int tacs_assembler_eval_functions(tacs_obj h, int n, tacs_obj* funcs, double* va
 }
 int tacs_assembler_add_dv_sens(tacs_obj h, double coef, int n, tacs_obj* funcs, tacs_obj* dfdx) {
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   TACSFunction* f[64];
   TACSBVec* d[64];
   if (n > 64) return tacs_set_error("addDVSens: at most 64 functions");
@@ -244,6 +289,7 @@ int tacs_assembler_add_dv_sens(tacs_obj h, double coef, int n, tacs_obj* funcs, 
 int tacs_assembler_add_sv_sens(tacs_obj h, double alpha, double beta, double gamma, int n,
                                tacs_obj* funcs, tacs_obj* dfdu) {
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   TACSFunction* f[64];
   TACSBVec* d[64];
   if (n > 64) return tacs_set_error("addSVSens: at most 64 functions");
@@ -258,6 +304,7 @@ int tacs_assembler_add_sv_sens(tacs_obj h, double alpha, double beta, double gam
 int tacs_assembler_add_adjoint_res_products(tacs_obj h, double scale, int n, tacs_obj* adjoint,
                                             tacs_obj* dfdx, double lambda) {
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   TACSBVec* ad[64];
   TACSBVec* d[64];
   if (n > 64) return tacs_set_error("addAdjointResProducts: at most 64 adjoints");
@@ -274,6 +321,7 @@ int tacs_assembler_add_xpt_sens(tacs_obj h, double coef, int n, tacs_obj* funcs,
                                 tacs_obj* dfdX) {
   clear_error();
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   std::vector<TACSFunction*> f(n);
   std::vector<TACSBVec*> v(n);
   for (int k = 0; k < n; k++) {
@@ -289,6 +337,7 @@ int tacs_assembler_add_adjoint_res_xpt_sens_products(tacs_obj h, double scale, i
                                                      double lambda) {
   clear_error();
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   std::vector<TACSBVec*> psi(n), v(n);
   for (int k = 0; k < n; k++) {
     psi[k] = OBJ(TACSBVec, adjoint[k]);
@@ -330,10 +379,12 @@ int tacs_vec_copy_to_device(tacs_obj v, double* dst, size_t n) {
   return e == hipSuccess ? 0 : tacs_set_error("copy_to_device: %s", hipGetErrorString(e));
 }
 double tacs_vec_norm(tacs_obj v) {
+  clear_error();
   TACSBVec* x = OBJ(TACSBVec, v);
   return x ? x->norm() : 0.0;
 }
 double tacs_vec_dot(tacs_obj v, tacs_obj w) {
+  clear_error();
   TACSBVec* x = OBJ(TACSBVec, v);
   TACSBVec* y = OBJ(TACSBVec, w);
   return (x && y) ? x->dot(y) : 0.0;
@@ -342,40 +393,40 @@ int tacs_vec_axpy(tacs_obj v, double alpha, tacs_obj w) {
   REQUIRE(TACSBVec, y, v);
   REQUIRE(TACSBVec, x, w);
   y->axpy(alpha, x);
-  return 0;
+  return last_error_code();
 }
 int tacs_vec_axpby(tacs_obj v, double alpha, double beta, tacs_obj w) {
   REQUIRE(TACSBVec, y, v);
   REQUIRE(TACSBVec, x, w);
   y->axpby(alpha, beta, x);
-  return 0;
+  return last_error_code();
 }
 int tacs_vec_scale(tacs_obj v, double alpha) {
   REQUIRE(TACSBVec, y, v);
   y->scale(alpha);
-  return 0;
+  return last_error_code();
 }
 int tacs_vec_copy_values(tacs_obj v, tacs_obj w) {
   REQUIRE(TACSBVec, y, v);
   REQUIRE(TACSBVec, x, w);
   if (x->n != y->n) return tacs_set_error("copyValues: size mismatch");
   y->copyValues(x);
-  return 0;
+  return last_error_code();
 }
 int tacs_vec_zero_entries(tacs_obj v) {
   REQUIRE(TACSBVec, y, v);
   y->zeroEntries();
-  return 0;
+  return last_error_code();
 }
 int tacs_vec_set(tacs_obj v, double val) {
   REQUIRE(TACSBVec, y, v);
   y->set(val);
-  return 0;
+  return last_error_code();
 }
 int tacs_vec_set_rand(tacs_obj v, double lo, double hi) {
   REQUIRE(TACSBVec, y, v);
   y->setRand(lo, hi);
-  return 0;
+  return last_error_code();
 }
 
 int tacs_mat_mult(tacs_obj m, tacs_obj x, tacs_obj y) {
@@ -389,6 +440,7 @@ int tacs_mat_mult(tacs_obj m, tacs_obj x, tacs_obj y) {
 int tacs_assembler_assemble_mat_type(tacs_obj h, int matType, tacs_obj mat, int matOr,
                                      double loadScale, int applyBCs) {
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   REQUIRE(TACSMat, A, mat);
   clear_error();
   a->assembleMatType((ElementMatrixType)matType, A, (MatrixOrientation)matOr, loadScale,
@@ -399,6 +451,7 @@ int tacs_assembler_assemble_mat_combo(tacs_obj h, int nmats, const int* matTypes
                                       const double* scale, tacs_obj mat, int matOr,
                                       double loadScale, int applyBCs) {
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   REQUIRE(TACSMat, A, mat);
   if (nmats < 1 || nmats > 8 || !matTypes || !scale)
     return tacs_set_error("assembleMatCombo: 1 to 8 matrix types with their scales");
@@ -414,7 +467,8 @@ int tacs_assembler_assemble_mat_combo(tacs_obj h, int nmats, const int* matTypes
 }
 tacs_obj tacs_matfree_create(tacs_obj assembler) {
   REQUIRE_NULL(TACSAssembler, a, assembler);
-  return (TACSObject*)new TACSMatrixFreeMat(a);
+  REQUIRE_INIT_NULL(a);
+  return built(new TACSMatrixFreeMat(a));
 }
 int tacs_matfree_assemble_data(tacs_obj m, int matType, double alpha, double beta,
                                double gamma) {
@@ -440,6 +494,7 @@ int tacs_matfree_mult_device(tacs_obj m, const double* x, double* y) {
 int tacs_assembler_get_matrix_free_data_size(tacs_obj h, int matType, int* data_size,
                                              int* temp_size) {
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   clear_error();
   a->getMatrixFreeDataSize((ElementMatrixType)matType, data_size, temp_size);
   return last_error_code();
@@ -447,6 +502,7 @@ int tacs_assembler_get_matrix_free_data_size(tacs_obj h, int matType, int* data_
 int tacs_assembler_assemble_matrix_free_data(tacs_obj h, int matType, double alpha, double beta,
                                              double gamma, double* data) {
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   clear_error();
   a->assembleMatrixFreeData((ElementMatrixType)matType, alpha, beta, gamma, data);
   return last_error_code();
@@ -455,6 +511,7 @@ int tacs_assembler_add_matrix_free_vec_product(tacs_obj h, int matType, const do
                                                double* temp, tacs_obj x, tacs_obj y,
                                                int matOr, double lambda, int applyBCs) {
   REQUIRE(TACSAssembler, a, h);
+  REQUIRE_INIT(a);
   REQUIRE(TACSBVec, xv, x);
   REQUIRE(TACSBVec, yv, y);
   clear_error();
@@ -478,7 +535,7 @@ int tacs_mat_mult_prec(tacs_obj m, int prec_vals, int prec_vecs, const void* x, 
 int tacs_mat_zero_entries(tacs_obj m) {
   REQUIRE(TACSParallelMat, A, m);
   A->zeroEntries();
-  return 0;
+  return last_error_code();
 }
 int tacs_mat_get_info(tacs_obj m, int* bsize, int* nrows, long* nnzb) {
   REQUIRE(TACSParallelMat, A, m);
@@ -492,7 +549,7 @@ int tacs_mat_get_arrays(tacs_obj m, const int** rowp, const int** cols, const do
   double* h = NULL;
   A->getArrays(NULL, NULL, rowp, cols, &h);
   if (Av) *Av = h;
-  return 0;
+  return last_error_code();
 }
 int tacs_mat_device_arrays(tacs_obj m, const int** rowp, const int** cols, const double** Av) {
   REQUIRE(TACSParallelMat, A, m);
@@ -532,7 +589,7 @@ int tacs_ksm_set_monitor(tacs_obj ksm, tacs_obj monitor) {
 
 tacs_obj tacs_block_jacobi_pc_create(tacs_obj mat) {
   REQUIRE_NULL(TACSParallelMat, A, mat);
-  return (TACSObject*)new TACSBlockJacobiPc(A);
+  return built(new TACSBlockJacobiPc(A));
 }
 int tacs_pc_factor(tacs_obj pc) {
   REQUIRE(TACSPc, p, pc);
@@ -545,9 +602,10 @@ int tacs_pc_apply(tacs_obj pc, tacs_obj x, tacs_obj y) {
   REQUIRE(TACSBVec, xv, x);
   REQUIRE(TACSBVec, yv, y);
   p->applyFactor(xv, yv);
-  return 0;
+  return last_error_code();
 }
 tacs_obj tacs_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int flexible) {
+  clear_error();
   // the operator: an assembled TACSParallelMat or a TACSMatrixFreeMat
   TACSParallelMat* A = OBJ(TACSParallelMat, mat);
   TACSMatrixFreeMat* F = A ? NULL : OBJ(TACSMatrixFreeMat, mat);
@@ -566,7 +624,7 @@ tacs_obj tacs_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int f
     tacs_set_error("GMRES: preconditioner size does not match the operator");
     return NULL;
   }
-  return (TACSObject*)new GMRES(A ? (TACSMat*)A : (TACSMat*)F, p, m, nrestart, flexible);
+  return built(new GMRES(A ? (TACSMat*)A : (TACSMat*)F, p, m, nrestart, flexible));
 }
 int tacs_gmres_set_ortho_type(tacs_obj ksm, int ortho) {
   REQUIRE(GMRES, g, ksm);
@@ -580,7 +638,7 @@ tacs_obj tacs_mixed_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart)
     tacs_set_error("mixed GMRES: subspace size must be in [1, 512]");
     return NULL;
   }
-  return (TACSObject*)new TACSMixedGMRES(A, p, m, nrestart);
+  return built(new TACSMixedGMRES(A, p, m, nrestart));
 }
 int tacs_mixed_gmres_set_deflation(tacs_obj ksm, int k) {
   clear_error();
@@ -598,7 +656,7 @@ tacs_obj tacs_batched_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestar
                    "prec 32 or 64, nrestart >= 0");
     return NULL;
   }
-  return (TACSObject*)new TACSBatchedGMRES(A, p, m, nrestart, nrhs, prec);
+  return built(new TACSBatchedGMRES(A, p, m, nrestart, nrhs, prec));
 }
 int tacs_batched_gmres_set_tolerances(tacs_obj ksm, double rtol, double atol) {
   REQUIRE(TACSBatchedGMRES, k, ksm);
@@ -674,6 +732,7 @@ int tacs_ksm_set_tolerances(tacs_obj ksm, double rtol, double atol) {
   return 0;
 }
 int tacs_ksm_solve(tacs_obj ksm, tacs_obj b, tacs_obj x, int zero_guess) {
+  clear_error();
   TACSKsm* k = OBJ(TACSKsm, ksm);
   TACSBVec* bv = OBJ(TACSBVec, b);
   TACSBVec* xv = OBJ(TACSBVec, x);

@@ -21,6 +21,12 @@
 // ---------------------------------------------------------------------------
 static hipStream_t g_stream = 0;
 static thread_local std::string g_err;
+// Set when an allocation, copy or kernel launch failed: every later copy and
+// launch of this thread is skipped (and the solvers leave their loops) until
+// the error is cleared, so a failed hipMalloc never turns into a kernel
+// working through a NULL buffer.  The C ABI clears it at the start of each
+// call; C++ callers clear it with tacs_clear_error().
+static thread_local int g_halt = 0;
 
 int tacs_set_error(const char* fmt, ...) {
   char buf[1024];
@@ -33,32 +39,50 @@ int tacs_set_error(const char* fmt, ...) {
   return 1;
 }
 const char* tacs_get_error() { return g_err.c_str(); }
-void tacs_clear_error() { g_err.clear(); }
+void tacs_clear_error() {
+  g_err.clear();
+  g_halt = 0;
+}
+int tacs_halted() { return g_halt; }
 void tacs_set_stream_impl(hipStream_t s) { g_stream = s; }
 hipStream_t tacs_stream() { return g_stream; }
 
 #define HCHECK(expr)                                                                     \
   do {                                                                                   \
+    if (g_halt) break;                                                                   \
     hipError_t _e = (expr);                                                              \
-    if (_e != hipSuccess)                                                                \
+    if (_e != hipSuccess) {                                                              \
       tacs_set_error("%s failed at %s:%d: %s", #expr, __FILE__, __LINE__,                \
                      hipGetErrorString(_e));                                             \
+      g_halt = 1;                                                                        \
+    }                                                                                    \
   } while (0)
 #define KCHECK(expr)                                                                     \
   do {                                                                                   \
+    if (g_halt) break;                                                                   \
     int _e = (expr);                                                                     \
-    if (_e != 0) tacs_set_error("%s failed at %s:%d: code %d", #expr, __FILE__, __LINE__, _e); \
+    if (_e != 0) {                                                                       \
+      tacs_set_error("%s failed at %s:%d: code %d", #expr, __FILE__, __LINE__, _e);      \
+      g_halt = 1;                                                                        \
+    }                                                                                    \
   } while (0)
 
+// Device allocation; NULL (with the error set and later launches halted)
+// when it fails.
 template <typename T>
 static T* dalloc(size_t n) {
   T* p = NULL;
   if (n == 0) n = 1;
   HCHECK(hipMalloc((void**)&p, n * sizeof(T)));
+  if (g_halt) return NULL;
   return p;
 }
+// Frees whatever it is given, halted or not.
 static void dfree(void* p) {
-  if (p) HCHECK(hipFree(p));
+  if (p) (void)hipFree(p);
+}
+static void hfree(void* p) {
+  if (p) (void)hipHostFree(p);
 }
 template <typename T>
 static void h2d(T* d, const T* h, size_t n) {
@@ -272,11 +296,18 @@ enum { VEC_DEVICE = 0, VEC_BOTH = 1, VEC_HOST = 2 };
 static void* g_vec_scratch = NULL;  // reduction scratch shared by vector ops
 static double* g_vec_scalar = NULL;
 static void* vec_scratch() {
-  if (!g_vec_scratch) {
+  if (!g_vec_scratch || !g_vec_scalar) {
     size_t b = tacs_hip_red_scratch_bytes();
-    HCHECK(hipMalloc(&g_vec_scratch, b));
-    HCHECK(hipMemset(g_vec_scratch, 0, b));
-    g_vec_scalar = dalloc<double>(8);
+    void* rs = dalloc<char>(b);
+    double* sc = dalloc<double>(8);
+    HCHECK(hipMemset(rs, 0, b));
+    if (g_halt) {  // both or neither
+      dfree(rs);
+      dfree(sc);
+      return NULL;
+    }
+    g_vec_scratch = rs;
+    g_vec_scalar = sc;
   }
   return g_vec_scratch;
 }
@@ -290,7 +321,7 @@ TACSBVec::TACSBVec(int _nnodes, int _bsize) : nnodes(_nnodes), bsize(_bsize) {
 }
 TACSBVec::~TACSBVec() {
   dfree(d_x);
-  if (h_x) HCHECK(hipHostFree(h_x));
+  hfree(h_x);
 }
 void TACSBVec::getSize(int* size) { *size = (int)n; }
 int TACSBVec::getArray(TacsScalar** array) {
@@ -303,15 +334,20 @@ int TACSBVec::getArray(TacsScalar** array) {
   if (array) *array = h_x;
   return (int)n;
 }
+// A host mirror handed out by getArray stays writable (the reference's
+// getArray returns the vector's own array), so while the device copy has not
+// been written since, every device read uploads the mirror: writes through a
+// held array reach the device however many reads came between them.
 const double* TACSBVec::devRead() {
-  if (state == VEC_HOST) {
+  if (state == VEC_HOST || state == VEC_BOTH) {
     h2d(d_x, h_x, n);
     state = VEC_BOTH;
   }
   return d_x;
 }
+// in-place device updates (axpy, scale) read the values too
 double* TACSBVec::devWrite() {
-  if (state == VEC_HOST) h2d(d_x, h_x, n);
+  if (state == VEC_HOST || state == VEC_BOTH) h2d(d_x, h_x, n);
   state = VEC_DEVICE;
   return d_x;
 }
@@ -425,7 +461,7 @@ TACSParallelMat::~TACSParallelMat() {
   for (void* q : sell) dfree(q);
   delete[] h_rowp;
   delete[] h_cols;
-  if (h_A) HCHECK(hipHostFree(h_A));
+  hfree(h_A);
 }
 TACSVec* TACSParallelMat::createVec() { return new TACSBVec(nrows, bsize); }
 void TACSParallelMat::mult(TACSVec* tx, TACSVec* ty) {
@@ -445,12 +481,21 @@ void TACSParallelMat::sellUpdate(int prec) {
     std::vector<int> w(ns > 0 ? ns : 1);
     std::vector<long> off((size_t)ns + 1);
     nslots = tacs_hip_sell_slots(nrows, h_rowp, w.data(), off.data());
-    d_sw = dalloc<int>(ns);
-    d_soff = dalloc<long>((size_t)ns + 1);
-    d_scols = dalloc<int>(nslots);
-    h2d(d_sw, w.data(), ns);
-    h2d(d_soff, off.data(), (size_t)ns + 1);
+    int* sw = dalloc<int>(ns);
+    long* soff = dalloc<long>((size_t)ns + 1);
+    int* scols = dalloc<int>(nslots);
+    h2d(sw, w.data(), ns);
+    h2d(soff, off.data(), (size_t)ns + 1);
     HCHECK(hipStreamSynchronize(g_stream));
+    if (g_halt) {  // the pattern arrays exist together or not at all
+      dfree(sw);
+      dfree(soff);
+      dfree(scols);
+      return;
+    }
+    d_sw = sw;
+    d_soff = soff;
+    d_scols = scols;
     cs_fill = d_scols;
   }
   if (prec == 64) {
@@ -647,7 +692,7 @@ GMRES::~GMRES() {
   dfree(d_work);
   dfree(d_scal);
   dfree(d_red);
-  HCHECK(hipHostFree(h_pin));
+  hfree(h_pin);
   if (mat)
     mat->decref();
   else
@@ -714,6 +759,7 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
   iterCount = 0;
 
   for (int count = 0; count < nrestart + 1; count++) {
+    if (g_halt) return 0;  // a copy or launch failed: no further work
     if (zero_guess && count == 0) {
       KCHECK(tacs_hip_fill(64, n, xd, 0.0, g_stream));
       HCHECK(hipMemcpyAsync(W(0), bd, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
@@ -738,6 +784,7 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
       break;
     }
     for (int i = 0; i < msub; i++) {
+      if (g_halt) return 0;  // a copy or launch failed: no further work
       double* q = W(i + 1);
       if (pc) {
         KCHECK(tacs_hip_bsr_diag_apply(bs, nr, pc->d_Dinv, W(i), d_work, g_stream));
@@ -821,7 +868,7 @@ TACSMixedGMRES::~TACSMixedGMRES() {
   dfree(d_r);
   dfree(d_scal);
   dfree(d_red);
-  HCHECK(hipHostFree(h_pin));
+  hfree(h_pin);
   mat->decref();
   if (pc) pc->decref();
 }
@@ -877,6 +924,7 @@ int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
   double r0 = 0.0;
   int flag = 0;
   for (int count = 0; count <= nrestart; count++) {
+    if (g_halt) return 0;  // a copy or launch failed: no further work
     // FP64 residual norm; W0 = r/|r| in FP32 (res[0] = 1)
     KCHECK(tacs_hip_dot(64, n, NULL, NULL, d_r, d_r, d_red, misc + MISC_NRM, 1, misc + MISC_INV,
                         g_stream));
@@ -894,6 +942,7 @@ int TACSMixedGMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
     const double target = std::max(rtol * r0, atol) / rn;  // in units of |r|
     int niters = 0;
     for (int i = 0; i < msub; i++) {
+      if (g_halt) return 0;  // a copy or launch failed: no further work
       float* q = W(i + 1);
       KCHECK(tacs_hip_bsr_diag_apply_f32(bs, nr, pc->d_Dinv32, W(i), d_work32, g_stream));
       mat->mult32(d_work32, q);
@@ -1181,13 +1230,13 @@ TACSBatchedGMRES::~TACSBatchedGMRES() {
   void* bufs[] = {d_V, d_work, d_Bb,  d_Xb,  d_scal, d_vec, d_cnt, d_red,
                   d_U, d_C,    d_Ub,  d_Cb,  d_Unb,  d_Cnb, d_prod, d_dq};
   for (void* q : bufs) dfree(q);
-  HCHECK(hipHostFree(h_pin));
-  if (h_stage) HCHECK(hipHostFree(h_stage));
-  if (h_up) HCHECK(hipHostFree(h_up));
+  hfree(h_pin);
+  if (h_stage) hfree(h_stage);
+  if (h_up) hfree(h_up);
   if (ev_up) HCHECK(hipEventDestroy(ev_up));
   void* st[] = {d_lsst, d_ist, d_kdv, d_tgtv};
   for (void* q : st) dfree(q);
-  if (h_flags) HCHECK(hipHostFree(h_flags));
+  if (h_flags) hfree(h_flags);
   for (hipEvent_t e : ev_step)
     if (e) HCHECK(hipEventDestroy(e));
   if (monitor) monitor->decref();
@@ -1222,6 +1271,12 @@ int TACSBatchedGMRES::setRecycleSpace(TACSMixedGMRES* src) {
   HCHECK(hipMemcpyAsync(d_C, src->rec->d_Cb, (size_t)k * n * sizeof(double),
                         hipMemcpyDeviceToDevice, g_stream));
   HCHECK(hipStreamSynchronize(g_stream));
+  if (g_halt) {
+    dfree(d_U);
+    dfree(d_C);
+    d_U = d_C = NULL;
+    return -1;
+  }
   kdefl = k;
   kVersion = mat->version;
   return k;
@@ -1234,20 +1289,23 @@ int TACSBatchedGMRES::setDeflation(int k) {
     return -1;
   }
   HCHECK(hipStreamSynchronize(g_stream));
-  void* bufs[] = {d_Ub, d_Cb, d_Unb, d_Cnb, d_prod, d_dq};
-  for (void* q : bufs) dfree(q);
-  d_Ub = d_Cb = d_Unb = d_Cnb = d_prod = NULL;
-  d_dq = NULL;
-  if (h_stage) HCHECK(hipHostFree(h_stage));
-  if (h_up) HCHECK(hipHostFree(h_up));
-  h_stage = h_up = NULL;
-  up_pending = 0;
-  void* st[] = {d_lsst, d_ist, d_kdv, d_tgtv};
-  for (void* q : st) dfree(q);
-  d_lsst = d_tgtv = NULL;
-  d_ist = d_kdv = NULL;
-  if (h_flags) HCHECK(hipHostFree(h_flags));
-  h_flags = d_flags = NULL;
+  auto release = [&]() {
+    void* bufs[] = {d_Ub, d_Cb, d_Unb, d_Cnb, d_prod, d_dq};
+    for (void* q : bufs) dfree(q);
+    d_Ub = d_Cb = d_Unb = d_Cnb = d_prod = NULL;
+    d_dq = NULL;
+    hfree(h_stage);
+    hfree(h_up);
+    h_stage = h_up = NULL;
+    up_pending = 0;
+    void* st[] = {d_lsst, d_ist, d_kdv, d_tgtv};
+    for (void* q : st) dfree(q);
+    d_lsst = d_tgtv = NULL;
+    d_ist = d_kdv = NULL;
+    hfree(h_flags);
+    h_flags = d_flags = NULL;
+  };
+  release();
   ndefl = k;
   defVersion = -1;
   defR = 0;
@@ -1269,12 +1327,17 @@ int TACSBatchedGMRES::setDeflation(int k) {
     d_kdv = dalloc<int>(R);
     d_tgtv = dalloc<double>(R);
     HCHECK(hipHostMalloc((void**)&h_flags, (size_t)(DCGS_LAG + 1) * 4 * R * sizeof(int), 0));
-    memset(h_flags, 0, (size_t)(DCGS_LAG + 1) * 4 * R * sizeof(int));
+    if (h_flags) memset(h_flags, 0, (size_t)(DCGS_LAG + 1) * 4 * R * sizeof(int));
     HCHECK(hipHostGetDevicePointer((void**)&d_flags, h_flags, 0));
     for (hipEvent_t& e : ev_step)
       if (!e) HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    HCHECK(hipMalloc(&d_dq, std::max(tacs_hip_bdotq_scratch_bytes(R),
-                                     tacs_hip_btsgemm_scratch_bytes(R))));
+    d_dq = dalloc<char>(std::max(tacs_hip_bdotq_scratch_bytes(R), tacs_hip_btsgemm_scratch_bytes(R)));
+    if (g_halt) {  // all of the recycling workspace, or none of it
+      release();
+      ndefl = 0;
+      kdj.assign(R, 0);
+      return -1;
+    }
   }
   return 0;
 }
@@ -1369,6 +1432,7 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
     KCHECK(tacs_hip_axpby(64, 64, N, 1.0, Bd, -1.0, V(0), g_stream));
   }
   for (int count = 0;; count++) {
+    if (g_halt) return 0;  // a copy or launch failed: no further work
     // FP64 residual norms
     KCHECK(tacs_hip_bmdot(n, Rn, 1, V(0), N, V(0), (double*)d_red, nrm2, 1, 0, g_stream));
     KCHECK(tacs_hip_bsqrt_inv(Rn, nrm2, 1, nrm, 1, inv, g_stream));
@@ -1820,6 +1884,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
     KCHECK(tacs_hip_axpby(64, 64, N, 1.0, Bd, -1.0, V(0), g_stream));
   }
   for (int count = 0;; count++) {
+    if (g_halt) return 0;  // a copy or launch failed: no further work
     KCHECK(tacs_hip_bmdot(n, Rn, 1, V(0), N, V(0), (double*)d_red, nrm2, 1, 0, g_stream));
     KCHECK(tacs_hip_bsqrt_inv(Rn, nrm2, 1, nrm, 1, inv, g_stream));
     d2h(h_pin + 256, nrm, Rn);
@@ -2143,6 +2208,10 @@ TACSAssembler::TACSAssembler(TACSComm comm, int vpn, int nowned, int nelems, int
   d_red = NULL;
   d_scalar = NULL;
   h_rowp = h_cols = NULL;
+  elemPrec = 64;
+  jacKernel = 0;
+  d_cptr = d_clist = NULL;
+  d_keb = NULL;
 }
 
 TACSAssembler::~TACSAssembler() {
@@ -2160,12 +2229,100 @@ TACSAssembler::~TACSAssembler() {
   void* ds[] = {d_conn, d_n2e_ptr, d_n2e, d_bcvars, d_bcnodes, d_bcmask, d_bcvals, d_dvnum,
                 d_dv2e_ptr, d_dv2e, d_tfixed, d_telem, d_X, d_u, d_ev, d_es, d_aux_elem,
                 d_aux_face, d_n2a_ptr, d_n2a, d_aux_trac, d_aux_ev, d_status, d_red, d_scalar,
-                d_ke, d_aux_pres, d_qgeo, d_qstress};
+                d_ke, d_aux_pres, d_qgeo, d_qstress, d_cptr, d_clist, d_keb};
   for (void* p : ds) dfree(p);
+}
+
+int TACSAssembler::setElementPrecision(int prec) {
+  if (prec != 64 && prec != 32) return tacs_set_error("setElementPrecision: 64 or 32, not %d", prec);
+  elemPrec = prec;
+  return 0;
+}
+int TACSAssembler::setJacobianKernel(int kernel) {
+  if (kernel < 0 || kernel > 2) return tacs_set_error("setJacobianKernel: 0, 1 or 2, not %d", kernel);
+  jacKernel = kernel;
+  return 0;
+}
+// The element-block path needs the isotropic tangent and block indices that
+// fit int32; elsewhere (and when asked) the row kernel runs.  jacKernel = 0
+// follows the timings measured on the MI355X: 27-node hexahedra (40^3) 5.5 ms
+// with element blocks against 27.8 ms with rows; 8-node (128^3) 11.5 ms
+// against 8.2 ms, so FP64 8-node elements keep the rows.  FP32 element
+// arithmetic exists only as element blocks.
+int TACSAssembler::jacobianKernelInUse() {
+  if (family != FAMILY_SOLID) return 0;
+  const bool fits = mat.isotropic && (double)numElements * npe * npe < 2147483647.0;
+  if (jacKernel == 1 || !fits) return 1;
+  if (jacKernel == 2) return 2;
+  return (order == 3 || elemPrec == 32) ? 2 : 1;
+}
+
+// Per slot of the block-CSR pattern, the element blocks that sum into it:
+// for row i, incident element e (ascending) with local node a, and local
+// column node b, the block e NPE^2 + a NPE + b goes to the slot of column
+// conn[e][b].  Built once, on the host, from the connectivity.
+int TACSAssembler::buildBlockLists() {
+  if (d_cptr) return 0;
+  const long NB = (long)npe * npe;
+  std::vector<int> n2e_ptr(numNodes + 1, 0);
+  for (size_t i = 0; i < (size_t)numElements * npe; i++) n2e_ptr[h_conn[i] + 1]++;
+  for (int i = 0; i < numNodes; i++) n2e_ptr[i + 1] += n2e_ptr[i];
+  std::vector<int> n2e(n2e_ptr[numNodes]);
+  {
+    std::vector<int> pos(n2e_ptr.begin(), n2e_ptr.end() - 1);
+    for (int e = 0; e < numElements; e++)
+      for (int a = 0; a < npe; a++) n2e[pos[h_conn[(size_t)e * npe + a]]++] = (e << 5) | a;
+  }
+  std::vector<int> cptr((size_t)nnzb + 1, 0);
+  std::vector<int> clist((size_t)numElements * NB);
+  std::vector<int> slot(npe * 8), cnt;
+  for (int i = 0; i < numNodes; i++) {
+    const int rs = h_rowp[i], nc = h_rowp[i + 1] - rs;
+    const int* cs = h_cols + rs;
+    cnt.assign(nc + 1, 0);
+    const int p0 = n2e_ptr[i], ne = n2e_ptr[i + 1] - p0;
+    if ((int)slot.size() < ne * npe) slot.resize((size_t)ne * npe);
+    for (int p = 0; p < ne; p++) {
+      const int e = n2e[p0 + p] >> 5;
+      for (int b = 0; b < npe; b++) {
+        const int col = h_conn[(size_t)e * npe + b];
+        const int s = (int)(std::lower_bound(cs, cs + nc, col) - cs);
+        if (s >= nc || cs[s] != col) return tacs_set_error("buildBlockLists: column %d missing", col);
+        slot[(size_t)p * npe + b] = s;
+        cnt[s + 1]++;
+      }
+    }
+    for (int s = 0; s < nc; s++) cnt[s + 1] += cnt[s];
+    for (int s = 0; s < nc; s++) cptr[rs + s + 1] = cptr[rs] + cnt[s + 1];
+    for (int p = 0; p < ne; p++) {
+      const int e = n2e[p0 + p] >> 5, a = n2e[p0 + p] & 31;
+      for (int b = 0; b < npe; b++) {
+        const int s = slot[(size_t)p * npe + b];
+        clist[(size_t)cptr[rs] + cnt[s]++] = (int)(e * NB + (long)a * npe + b);
+      }
+    }
+  }
+  int* cp = dalloc<int>(cptr.size());
+  int* cl = dalloc<int>(clist.size());
+  h2d(cp, cptr.data(), cptr.size());
+  h2d(cl, clist.data(), clist.size());
+  HCHECK(hipStreamSynchronize(g_stream));
+  if (g_halt) {
+    dfree(cp);
+    dfree(cl);
+    return 1;
+  }
+  d_cptr = cp;
+  d_clist = cl;
+  return 0;
 }
 
 int TACSAssembler::setElementConnectivity(const int* ptr, const int* conn) {
   if (initialized) return tacs_set_error("setElementConnectivity after initialize()");
+  if (numElements < 0 || numElements >= (1 << 26))
+    return tacs_set_error("setElementConnectivity: %d elements; the node-to-element packing holds "
+                          "fewer than 2^26 = 67108864",
+                          numElements);
   delete[] h_ptr;
   delete[] h_conn;
   h_ptr = new int[numElements + 1];
@@ -2299,16 +2456,25 @@ int TACSAssembler::initialize() {
                             h_ptr[e + 1] - h_ptr[e], npe);
 
   // node -> element CSR (TACSAssembler::computeNodeToElementCSR :1791),
-  // entries (element << 5 | local index), ascending element order
+  // entries (element << 5 | local index), ascending element order: the
+  // packing holds element indices below 2^26 (67,108,864 elements) and
+  // int32 offsets hold at most 2^31 - 1 (element, node) incidences
+  if (numElements >= (1 << 26))
+    return tacs_set_error("initialize: %d elements; the node-to-element packing holds fewer than "
+                          "2^26 = 67108864",
+                          numElements);
+  if ((long)numElements * npe > 0x7fffffffL)
+    return tacs_set_error("initialize: %ld element-node incidences exceed int32 offsets",
+                          (long)numElements * npe);
   std::vector<int> n2e_ptr(numNodes + 1, 0);
   for (int e = 0; e < numElements; e++)
-    for (int a = 0; a < npe; a++) n2e_ptr[h_conn[e * npe + a] + 1]++;
+    for (int a = 0; a < npe; a++) n2e_ptr[h_conn[(size_t)e * npe + a] + 1]++;
   for (int i = 0; i < numNodes; i++) n2e_ptr[i + 1] += n2e_ptr[i];
   std::vector<int> n2e(n2e_ptr[numNodes]);
   {
     std::vector<int> pos(n2e_ptr.begin(), n2e_ptr.end() - 1);
     for (int e = 0; e < numElements; e++)
-      for (int a = 0; a < npe; a++) n2e[pos[h_conn[e * npe + a]]++] = (e << 5) | a;
+      for (int a = 0; a < npe; a++) n2e[pos[h_conn[(size_t)e * npe + a]]++] = (e << 5) | a;
   }
   // node -> node CSR (computeLocalNodeToNodeCSR :1899): sorted, unique,
   // diagonal kept
@@ -2321,7 +2487,7 @@ int TACSAssembler::initialize() {
     tmp.clear();
     for (int p = n2e_ptr[i]; p < n2e_ptr[i + 1]; p++) {
       int e = n2e[p] >> 5;
-      for (int a = 0; a < npe; a++) tmp.push_back(h_conn[e * npe + a]);
+      for (int a = 0; a < npe; a++) tmp.push_back(h_conn[(size_t)e * npe + a]);
     }
     std::sort(tmp.begin(), tmp.end());
     tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
@@ -2412,10 +2578,11 @@ int TACSAssembler::initialize() {
   d_status = dalloc<int>(1);
   HCHECK(hipMemsetAsync(d_status, 0, sizeof(int), g_stream));
   size_t rb = tacs_hip_red_scratch_bytes();
-  HCHECK(hipMalloc(&d_red, rb));
+  d_red = dalloc<char>(rb);
   HCHECK(hipMemsetAsync(d_red, 0, rb, g_stream));
   d_scalar = dalloc<double>(16);
   HCHECK(hipStreamSynchronize(g_stream));
+  if (g_halt) return 1;  // an upload failed: stays uninitialised (error set)
   initialized = 1;
   return 0;
 }
@@ -2547,6 +2714,15 @@ int TACSAssembler::uploadAux() {
   h2d(d_n2a_ptr, ptr.data(), numNodes + 1);
   h2d(d_n2a, list.data(), list.size());
   HCHECK(hipStreamSynchronize(g_stream));
+  if (g_halt) {  // no partial aux state survives a failed upload
+    void* fresh[] = {d_aux_elem, d_aux_face, d_aux_trac, d_n2a_ptr, d_n2a, d_aux_ev, d_aux_pres};
+    for (void* p : fresh) dfree(p);
+    d_aux_elem = d_aux_face = d_n2a_ptr = d_n2a = NULL;
+    d_aux_trac = d_aux_ev = d_aux_pres = NULL;
+    naux = naux_pressure = 0;
+    aux_version = -1;
+    return 1;
+  }
   naux = n;
   naux_pressure = family == FAMILY_SHELL ? 0 : npres;
   aux_version = auxElements->version;
@@ -2638,12 +2814,16 @@ void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScal
   HCHECK(hipEventCreate(&e1));
   if (family == FAMILY_SHELL) {
     // element matrices (and residuals K u) in one pass, then the row gather
+    // (FP32 element matrices: the FP64 residual is a pass of its own)
     ensure(d_ke, (size_t)numElements * 576);
     double* ev = NULL;
     if (residual) ev = ensure(d_ev, (size_t)numElements * 24);
+    if (residual && elemPrec == 32)
+      KCHECK(shellKernel(64, 1, NULL, 1.0, 0, 0, 1, 1, NULL, ev, NULL));
     HCHECK(hipEventRecord(e0, g_stream));
-    KCHECK(shellKernel(64, 0, NULL, alpha, 0, 0, 1, 1, d_ke, ev, NULL));
-    KCHECK(tacs_hip_gather_elem_matrices(6, 4, 64, numNodes, d_n2e_ptr, d_n2e, d_conn, d_ke,
+    KCHECK(shellKernel(elemPrec, 0, NULL, alpha, 0, 0, 1, 1, d_ke, elemPrec == 64 ? ev : NULL,
+                       NULL));
+    KCHECK(tacs_hip_gather_elem_matrices(6, 4, elemPrec, numNodes, d_n2e_ptr, d_n2e, d_conn, d_ke,
                                          A->d_rowp, A->d_cols, bcs ? d_bcvars : NULL, A->d_A,
                                          A->d_A32, g_stream));
     HCHECK(hipEventRecord(e1, g_stream));
@@ -2666,12 +2846,36 @@ void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScal
     return;
   }
   if (residual) assembleRes(residual, lambda, bcs);
-  HCHECK(hipEventRecord(e0, g_stream));
-  ensure(d_qgeo, (size_t)numElements * npe * 10);
-  KCHECK(tacs_hip_solid_jacobian(order, numNodes, numElements, d_n2e_ptr, d_n2e, d_conn, d_X,
-                                 d_telem, &mat, alpha, gamma, A->d_rowp, A->d_cols,
-                                 bcs ? d_bcvars : NULL, d_qgeo, A->d_A, A->d_A32, d_status,
-                                 g_stream));
+  const int kern = jacobianKernelInUse();
+  if (elemPrec == 32 && kern != 2) {
+    HCHECK(hipEventDestroy(e0));
+    HCHECK(hipEventDestroy(e1));
+    tacs_set_error("assembleJacobian: FP32 element kernels need the element-block kernel "
+                   "(isotropic material)");
+    return;
+  }
+  if (kern == 2) {
+    // element blocks in the element precision, then the row gather
+    if (buildBlockLists() != 0) {
+      HCHECK(hipEventDestroy(e0));
+      HCHECK(hipEventDestroy(e1));
+      return;
+    }
+    if (!d_keb) d_keb = dalloc<char>((size_t)numElements * npe * npe * 9 * sizeof(double));
+    HCHECK(hipEventRecord(e0, g_stream));
+    KCHECK(tacs_hip_solid_elem_blocks(order, elemPrec, numElements, d_conn, d_X, d_telem, &mat,
+                                      alpha, gamma, d_keb, g_stream));
+    KCHECK(tacs_hip_gather_solid_blocks(elemPrec, numNodes, A->d_rowp, A->d_cols, d_cptr, d_clist,
+                                        d_keb, bcs ? d_bcvars : NULL, A->d_A, A->d_A32,
+                                        g_stream));
+  } else {
+    HCHECK(hipEventRecord(e0, g_stream));
+    ensure(d_qgeo, (size_t)numElements * npe * 10);
+    KCHECK(tacs_hip_solid_jacobian(order, numNodes, numElements, d_n2e_ptr, d_n2e, d_conn, d_X,
+                                   d_telem, &mat, alpha, gamma, A->d_rowp, A->d_cols,
+                                   bcs ? d_bcvars : NULL, d_qgeo, A->d_A, A->d_A32, d_status,
+                                   g_stream));
+  }
   HCHECK(hipEventRecord(e1, g_stream));
   HCHECK(hipEventSynchronize(e1));
   float ms = 0.f;

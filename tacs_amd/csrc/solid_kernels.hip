@@ -1116,6 +1116,226 @@ __global__ void apply_bc_vec(int b, int nbc, const int* __restrict__ nodes,
 
 // Host-side launchers (C ABI, raw device pointers, explicit stream).
 // ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
+// Element-centric Jacobian.  A team of threads owns one element: it stages
+// the element's nodes, the inverse Jacobian and w det(J) t of every
+// quadrature point, and the physical gradients (and values) of every basis
+// function at every point in LDS, then each thread accumulates whole 3x3
+// blocks K_ab (local nodes a, b) over the points in registers.  T is the
+// arithmetic type of the element (double, or float for the FP32 element
+// kernels).  The blocks go to ke[e][a NPE + b][9] (row-major blocks); the
+// row gather below sums them into block-CSR in a fixed order.  Isotropic
+// tangent (lambda, mu) plus the consistent mass; the general C[21] and the
+// geometric stiffness stay with the row kernel above.
+// ---------------------------------------------------------------------------
+template <int ORD>
+struct EBCfg;
+template <>
+struct EBCfg<2> {  // 64 blocks: one per thread of a wavefront, 4 elements per group
+  static constexpr int TEAM = 64, EPB = 4;
+};
+template <>
+struct EBCfg<3> {  // 729 blocks over 256 threads (2 or 3 each), one element per group
+  static constexpr int TEAM = 256, EPB = 1;
+};
+
+template <int ORD, typename T>
+struct EBLds {
+  static constexpr int NPE = ORD * ORD * ORD, NQP = NPE;
+  T X[NPE][3];
+  T J[NQP][9];
+  T wdt[NQP];
+  T g[NQP][3][NPE];  // component k of the gradient of basis function a at point q
+  T N[NQP][NPE];
+};
+
+template <int ORD, typename T>
+__device__ __forceinline__ void lag1d_t(T x, T* n, T* d) {
+  if (ORD == 2) {
+    n[0] = T(0.5) * (T(1) - x);
+    n[1] = T(0.5) * (T(1) + x);
+    d[0] = T(-0.5);
+    d[1] = T(0.5);
+  } else {
+    n[0] = T(-0.5) * x * (T(1) - x);
+    n[1] = (T(1) - x) * (T(1) + x);
+    n[2] = T(0.5) * (T(1) + x) * x;
+    d[0] = T(-0.5) + x;
+    d[1] = T(-2) * x;
+    d[2] = T(0.5) + x;
+  }
+}
+
+// value and reference gradient of basis function a at point q
+template <int ORD, typename T>
+__device__ __forceinline__ void basis_t(int q, int a, T& N, T g[3]) {
+  T n[3][ORD], d[3][ORD];
+  lag1d_t<ORD, T>(T(gauss_pt<ORD>(q % ORD)), n[0], d[0]);
+  lag1d_t<ORD, T>(T(gauss_pt<ORD>((q / ORD) % ORD)), n[1], d[1]);
+  lag1d_t<ORD, T>(T(gauss_pt<ORD>(q / (ORD * ORD))), n[2], d[2]);
+  const int ia = a % ORD, ja = (a / ORD) % ORD, ka = a / (ORD * ORD);
+  T n0 = n[0][0], n1 = n[1][0], n2 = n[2][0], d0 = d[0][0], d1 = d[1][0], d2 = d[2][0];
+#pragma unroll
+  for (int i = 1; i < ORD; i++) {
+    if (ia == i) { n0 = n[0][i]; d0 = d[0][i]; }
+    if (ja == i) { n1 = n[1][i]; d1 = d[1][i]; }
+    if (ka == i) { n2 = n[2][i]; d2 = d[2][i]; }
+  }
+  N = n0 * n1 * n2;
+  g[0] = d0 * n1 * n2;
+  g[1] = n0 * d1 * n2;
+  g[2] = n0 * n1 * d2;
+}
+
+template <typename T>
+__device__ __forceinline__ T inv3_t(const T A[9], T B[9]) {
+  T det = A[8] * (A[0] * A[4] - A[3] * A[1]) - A[7] * (A[0] * A[5] - A[3] * A[2]) +
+          A[6] * (A[1] * A[5] - A[2] * A[4]);
+  T r = T(1) / det;
+  B[0] = (A[4] * A[8] - A[5] * A[7]) * r;
+  B[1] = -(A[1] * A[8] - A[2] * A[7]) * r;
+  B[2] = (A[1] * A[5] - A[2] * A[4]) * r;
+  B[3] = -(A[3] * A[8] - A[5] * A[6]) * r;
+  B[4] = (A[0] * A[8] - A[2] * A[6]) * r;
+  B[5] = -(A[0] * A[5] - A[2] * A[3]) * r;
+  B[6] = (A[3] * A[7] - A[4] * A[6]) * r;
+  B[7] = -(A[0] * A[7] - A[1] * A[6]) * r;
+  B[8] = (A[0] * A[4] - A[1] * A[3]) * r;
+  return det;
+}
+
+template <int ORD, typename T>
+__global__ __launch_bounds__(EBCfg<ORD>::TEAM* EBCfg<ORD>::EPB) void solid_elem_blocks(
+    int nelem, const int* __restrict__ conn, const double* __restrict__ Xg,
+    const double* __restrict__ telem, SolidMaterial mat, double alpha, double gamma,
+    T* __restrict__ ke) {
+  constexpr int NPE = ORD * ORD * ORD, NQP = NPE, NB = NPE * NPE;
+  constexpr int TEAM = EBCfg<ORD>::TEAM, EPB = EBCfg<ORD>::EPB;
+  constexpr int BPT = (NB + TEAM - 1) / TEAM;
+  __shared__ EBLds<ORD, T> lds[EPB];
+  const int team = threadIdx.x / TEAM, tid = threadIdx.x % TEAM;
+  const int el = blockIdx.x * EPB + team;
+  const bool active = el < nelem;
+  EBLds<ORD, T>& L = lds[team];
+  const int* c = &conn[(size_t)(active ? el : 0) * NPE];
+
+  if (active)
+    for (int k = tid; k < NPE * 3; k += TEAM) L.X[k / 3][k % 3] = T(Xg[3 * (size_t)c[k / 3] + k % 3]);
+  __syncthreads();
+  // inverse Jacobian and w det(J) t at every point
+  if (active) {
+    const T te = T(telem[el]);
+    for (int q = tid; q < NQP; q += TEAM) {
+      T Xd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int b = 0; b < NPE; b++) {
+        T N, g[3];
+        basis_t<ORD, T>(q, b, N, g);
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          const T x = L.X[b][r];
+          Xd[3 * r] += x * g[0];
+          Xd[3 * r + 1] += x * g[1];
+          Xd[3 * r + 2] += x * g[2];
+        }
+      }
+      T Ji[9];
+      const T det = inv3_t<T>(Xd, Ji);
+#pragma unroll
+      for (int k = 0; k < 9; k++) L.J[q][k] = Ji[k];
+      const T w = T(gauss_wt<ORD>(q % ORD) * gauss_wt<ORD>((q / ORD) % ORD) *
+                    gauss_wt<ORD>(q / (ORD * ORD)));
+      L.wdt[q] = w * det * te;
+    }
+  }
+  __syncthreads();
+  // physical gradients g = J^-T dN/dxi and values of every basis function
+  if (active) {
+    for (int p = tid; p < NQP * NPE; p += TEAM) {
+      const int q = p / NPE, a = p - q * NPE;
+      T N, g0[3];
+      basis_t<ORD, T>(q, a, N, g0);
+      const T* J = L.J[q];
+      L.g[q][0][a] = g0[0] * J[0] + g0[1] * J[3] + g0[2] * J[6];
+      L.g[q][1][a] = g0[0] * J[1] + g0[1] * J[4] + g0[2] * J[7];
+      L.g[q][2][a] = g0[0] * J[2] + g0[1] * J[5] + g0[2] * J[8];
+      L.N[q][a] = N;
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  // K_ab = sum_q alpha w det t [lam g_a g_b^T + mu g_b g_a^T + mu (g_a.g_b) I]
+  //        + gamma rho w det t N_a N_b I
+  // (TACSLinearElasticity3D::evalWeakMatrix, TACSLinearElasticity.cpp:1158)
+  T acc[BPT][9];
+#pragma unroll
+  for (int r = 0; r < BPT; r++)
+#pragma unroll
+    for (int k = 0; k < 9; k++) acc[r][k] = T(0);
+  const T lam = T(mat.lam), mu = T(mat.mu), al = T(alpha), gr = T(gamma * mat.rho);
+  const bool mass = gamma != 0.0;
+  for (int q = 0; q < NQP; q++) {
+    const T f = al * L.wdt[q];
+#pragma unroll
+    for (int r = 0; r < BPT; r++) {
+      const int p = tid + r * TEAM;
+      if (BPT * TEAM == NB || p < NB) {
+        const int a = p / NPE, b = p - a * NPE;
+        const T ga[3] = {f * L.g[q][0][a], f * L.g[q][1][a], f * L.g[q][2][a]};
+        const T gb[3] = {L.g[q][0][b], L.g[q][1][b], L.g[q][2][b]};
+        const T la[3] = {lam * ga[0], lam * ga[1], lam * ga[2]};
+        const T mb[3] = {mu * gb[0], mu * gb[1], mu * gb[2]};
+        T d = mu * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+        if (mass) d += gr * L.wdt[q] * L.N[q][a] * L.N[q][b];
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+          for (int k = 0; k < 3; k++) acc[r][3 * j + k] += la[j] * gb[k] + mb[j] * ga[k];
+        acc[r][0] += d;
+        acc[r][4] += d;
+        acc[r][8] += d;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < BPT; r++) {
+    const int p = tid + r * TEAM;
+    if (BPT * TEAM == NB || p < NB) {
+      T* o = &ke[((size_t)el * NB + p) * 9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) o[k] = acc[r][k];
+    }
+  }
+}
+
+// Block-CSR rows from element blocks: value k of slot s of a row is the sum,
+// in the order of the slot's list (ascending element, as the reference's
+// element loop adds them, src/TACSAssembler.cpp:4352-4383), of value k of
+// the listed element blocks ke[clist[c]]; accumulated in FP64.  One
+// wavefront per row; Dirichlet rows fused (BCSRMat::zeroRow ident = 1,
+// src/bpmat/BCSRMat.cpp:2027).
+template <typename T>
+__global__ __launch_bounds__(256) void gather_solid_blocks(
+    int nrows, const int* __restrict__ rowp, const int* __restrict__ cols,
+    const int* __restrict__ cptr, const int* __restrict__ clist, const T* __restrict__ ke,
+    const int* __restrict__ bcvars, double* __restrict__ A, float* __restrict__ A32) {
+  const int lane = threadIdx.x % TACS_WAVE;
+  const int wpb = blockDim.x / TACS_WAVE;
+  for (int row = blockIdx.x * wpb + threadIdx.x / TACS_WAVE; row < nrows; row += gridDim.x * wpb) {
+    const int r0 = rowp[row], nvals = (rowp[row + 1] - r0) * 9;
+    const int mask = bcvars ? bcvars[row] : 0;
+    for (int v = lane; v < nvals; v += TACS_WAVE) {
+      const int s = r0 + v / 9, k = v % 9;
+      double acc = 0.0;
+      const int c1 = cptr[s + 1];
+      for (int cc = cptr[s]; cc < c1; cc++) acc += (double)ke[(size_t)clist[cc] * 9 + k];
+      if (mask & (1 << (k / 3))) acc = (cols[s] == row && k / 3 == k % 3) ? 1.0 : 0.0;
+      if (A) A[(size_t)s * 9 + k] = acc;
+      if (A32) A32[(size_t)s * 9 + k] = (float)acc;
+    }
+  }
+}
+
 extern "C" {
 
 int tacs_hip_solid_mat_type(int order, int nnodes, int nelem, const int* n2e_ptr,
@@ -1171,6 +1391,44 @@ int tacs_hip_solid_mat_type(int order, int nnodes, int nelem, const int* n2e_ptr
   }
 #undef JAC_ISO
 #undef JAC_LAUNCH
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_solid_elem_blocks(int order, int prec, int nelem, const int* conn, const double* X,
+                               const double* telem, const SolidMaterial* mat, double alpha,
+                               double gamma, void* ke, hipStream_t stream) {
+  if ((order != 2 && order != 3) || (prec != 64 && prec != 32) || !mat || !mat->isotropic)
+    return TACS_HIP_ERR_ARG;
+  if (nelem <= 0) return 0;
+#define EB_LAUNCH(O, T)                                                                         \
+  hipLaunchKernelGGL((solid_elem_blocks<O, T>),                                                 \
+                     dim3((nelem + EBCfg<O>::EPB - 1) / EBCfg<O>::EPB),                         \
+                     dim3(EBCfg<O>::TEAM * EBCfg<O>::EPB), 0, stream, nelem, conn, X, telem, *mat, \
+                     alpha, gamma, (T*)ke)
+  if (order == 2 && prec == 64)
+    EB_LAUNCH(2, double);
+  else if (order == 2)
+    EB_LAUNCH(2, float);
+  else if (prec == 64)
+    EB_LAUNCH(3, double);
+  else
+    EB_LAUNCH(3, float);
+#undef EB_LAUNCH
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_gather_solid_blocks(int prec, int nrows, const int* rowp, const int* cols,
+                                 const int* cptr, const int* clist, const void* ke,
+                                 const int* bcvars, double* A, float* A32, hipStream_t stream) {
+  if (prec != 64 && prec != 32) return TACS_HIP_ERR_ARG;
+  if (nrows <= 0) return 0;
+  const int grid = tacs_grid_for(nrows, 4, 1 << 20);
+  if (prec == 64)
+    hipLaunchKernelGGL(gather_solid_blocks<double>, dim3(grid), dim3(256), 0, stream, nrows, rowp,
+                       cols, cptr, clist, (const double*)ke, bcvars, A, A32);
+  else
+    hipLaunchKernelGGL(gather_solid_blocks<float>, dim3(grid), dim3(256), 0, stream, nrows, rowp,
+                       cols, cptr, clist, (const float*)ke, bcvars, A, A32);
   return (int)hipGetLastError();
 }
 

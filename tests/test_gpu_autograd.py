@@ -64,3 +64,44 @@ def test_simp_compliance_gradient():
             vals.append(sp.forward(torch.tensor(xp, dtype=torch.float64, device="cuda")))
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(fd - g[e]) <= 1e-4 * abs(g[e]) + 1e-8 * abs(c_ref)
+
+
+def test_backward_uses_its_own_forward_state():
+    """forward(x1), forward(x2), then the backward of the first node: the
+    gradient is the one at x1 (the problem re-solves at x1), equal to a
+    fresh forward/backward at x1 and to central differences there."""
+    import torch
+    from tacs_amd.autograd import SolidComplianceProblem, simp_compliance
+    from tacs_amd.problems import SolidProblem
+
+    prob = SolidProblem(4, E=1.0, nu=0.3, traction=(0.0, 0.0, 1.0), dv="element",
+                        traction_patch=2)
+    rng = np.random.default_rng(5)
+    x1 = rng.uniform(0.1, 1.0, prob.mesh.nelem)
+    x2 = rng.uniform(0.1, 1.0, prob.mesh.nelem)
+    sp = SolidComplianceProblem(prob, p=3.0, E0=1.0, Emin=1e-9, rtol=1e-12, m=60, mixed=False,
+                                restarts=500)
+    xa = torch.tensor(x1, dtype=torch.float64, device="cuda", requires_grad=True)
+    xb = torch.tensor(x2, dtype=torch.float64, device="cuda", requires_grad=True)
+    ca = simp_compliance(xa, sp)
+    cb = simp_compliance(xb, sp)
+    ca.backward()
+    ga = xa.grad.detach().cpu().numpy()
+    cb.backward()
+    gb = xb.grad.detach().cpu().numpy()
+    # fresh passes at each point
+    for x0, g in ((x1, ga), (x2, gb)):
+        xf = torch.tensor(x0, dtype=torch.float64, device="cuda", requires_grad=True)
+        simp_compliance(xf, sp).backward()
+        gf = xf.grad.detach().cpu().numpy()
+        assert np.abs(g - gf).max() <= 1e-9 * np.abs(gf).max()
+    assert np.abs(ga - gb).max() > 1e-3 * np.abs(ga).max()  # the points differ
+    e = int(np.argmax(np.abs(ga)))
+    h = 1e-6
+    vals = []
+    for s in (1.0, -1.0):
+        xp = x1.copy()
+        xp[e] += s * h
+        vals.append(sp.forward(torch.tensor(xp, dtype=torch.float64, device="cuda")))
+    fd = (vals[0] - vals[1]) / (2 * h)
+    assert abs(fd - ga[e]) <= 1e-4 * abs(ga[e])

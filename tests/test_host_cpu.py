@@ -137,3 +137,51 @@ def test_dense_smallest_eigvecs(n, k):
     w = wr + 1j * wi
     assert np.all(np.diff(np.abs(w)) >= -1e-12 * np.abs(w).max())
     assert np.abs(np.sort_complex(w) - np.sort_complex(ev)).max() <= 1e-10 * np.abs(ev).max()
+
+
+def _has_gpu():
+    import torch
+
+    return torch.cuda.device_count() > 0
+
+
+def test_failed_device_allocation_is_an_error_with_no_launch():
+    """With no GPU every hipMalloc fails: initialize() reports it, leaves the
+    assembler uninitialised, creators return NULL instead of half-built
+    objects, and device work on the assembler is refused before any launch."""
+    if _has_gpu():
+        pytest.skip("allocations succeed on a GPU (tests/test_gpu_errors.py covers this there)")
+    L = lib()
+    m = HexMesh(2, 2, 2)
+    a = TACS.Assembler.create(None, 3, m.nnodes, m.nelem)
+    a.setElementConnectivity(np.arange(0, 8 * m.nelem + 1, 8), m.conn.reshape(-1))
+    props = constitutive.MaterialProperties(E=70e9, nu=0.3)
+    el = elements.Element3D(elements.LinearElasticity3D(constitutive.SolidConstitutive(props)),
+                            elements.LinearHexaBasis())
+    a.setElements([el] * m.nelem)
+    with pytest.raises(TACSError, match="hipMalloc"):
+        a.initialize()
+    assert not L.tacs_assembler_create_vec(a.ptr)
+    assert b"hipMalloc" in L.tacs_last_error()
+    assert not L.tacs_assembler_create_mat(a.ptr)
+    assert b"initialize" in L.tacs_last_error()
+    assert L.tacs_assembler_assemble_jacobian(a.ptr, 1.0, 0.0, 0.0, None, None, 0, 1.0, 1) != 0
+    assert L.tacs_assembler_zero_variables(a.ptr) != 0
+    assert b"not initialized" in L.tacs_last_error()
+    assert not L.tacs_matfree_create(a.ptr)
+
+
+def test_element_index_packing_limit():
+    """The node-to-element CSR packs (element << 5 | local node) in int32:
+    2^26 elements or more are rejected before anything is read or built."""
+    L = lib()
+    a = L.tacs_assembler_create(3, 8, 1 << 26, 0)
+    L.tacs_incref(a)
+    ptr = np.arange(0, 9, 8, dtype=np.intc)
+    conn = np.arange(8, dtype=np.intc)
+    assert L.tacs_assembler_set_element_connectivity(a, ptr.ctypes.data, conn.ctypes.data) != 0
+    assert b"2^26" in L.tacs_last_error()
+    L.tacs_decref(a)
+    b = L.tacs_assembler_create(3, 8, (1 << 26) - 1, 0)
+    L.tacs_incref(b)
+    L.tacs_decref(b)
