@@ -324,6 +324,31 @@ class BatchedKSM(TACSHandle):
         return it, fl, r0, rn, cyc
 
 
+class RecyclingKSM:
+    """Krylov subspace recycling (GCRO-DR with delayed classical Gram-Schmidt)
+    for one right-hand side through the KSM interface (solve, setTolerances,
+    getIterCount): the batched solver with one right-hand side, on the FP64
+    operator (prec=64) or its FP32-rounded values with FP64 vectors
+    (prec=32).  The recycled space carries over to later solves with the same
+    matrix values (the adjoint solve).  No counterpart in the reference."""
+
+    def __init__(self, mat, pc, m=60, restarts=100000, deflation=20, prec=64):
+        self.batched = BatchedKSM(mat, pc, m=m, restarts=restarts, nrhs=1, prec=prec,
+                                  deflation=deflation)
+
+    def setTolerances(self, rtol, atol):
+        self.batched.setTolerances(rtol, atol)
+
+    def setMonitor(self, *args, **kw):
+        self.batched.setMonitor(*args, **kw)
+
+    def solve(self, b, x, zero_guess=1):
+        return self.batched.solve([b], [x], zero_guess)
+
+    def getIterCount(self):
+        return int(self.batched.getStats()[0][0])
+
+
 class AuxElements(TACSHandle):
     def __init__(self, size=100):
         super().__init__(lib().tacs_aux_elements_create(size))

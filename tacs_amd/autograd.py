@@ -47,6 +47,9 @@ class SolidComplianceProblem:
             # deflation > 0: the recycling solver (its space carries over to
             # the adjoint solve with the same matrix)
             self.ksm = TACS.MixedKSM(problem.mat, self.pc, m, restarts, deflation=deflation)
+        elif deflation:
+            # the same recycling on the FP64 operator
+            self.ksm = TACS.RecyclingKSM(problem.mat, self.pc, m, restarts, deflation, prec=64)
         else:
             self.ksm = TACS.KSM(problem.mat, self.pc, min(m, 512), restarts)
         self.ksm.setTolerances(rtol, 1e-30)
@@ -135,6 +138,8 @@ class ThicknessFunctionProblem:
         self.pc = TACS.Pc(problem.mat)
         if mixed:
             self.ksm = TACS.MixedKSM(problem.mat, self.pc, m, restarts, deflation=deflation)
+        elif deflation:
+            self.ksm = TACS.RecyclingKSM(problem.mat, self.pc, m, restarts, deflation, prec=64)
         else:
             self.ksm = TACS.KSM(problem.mat, self.pc, min(m, 512), restarts)
         self.ksm.setTolerances(rtol, 1e-30)

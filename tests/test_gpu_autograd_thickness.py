@@ -35,7 +35,7 @@ def test_shell_thickness_gradient_central_differences(func):
     p = ShellProblem(8, warp=0.01, seed=4, dv="element", ys=270e6)
     rng = np.random.default_rng(4)
     t0 = rng.uniform(0.005, 0.02, p.mesh.nelem)
-    prob = ThicknessFunctionProblem(p, func=func, ks_weight=100.0, rtol=1e-12, m=120, mixed=False,
+    prob = ThicknessFunctionProblem(p, func=func, ks_weight=100.0, rtol=1e-12, m=120, mixed=False, deflation=0,
                                     restarts=500)
     idx = rng.choice(p.mesh.nelem, 6, replace=False)
     f64, g64 = _fd_check(prob, t0, idx, 1e-5, 2e-5)
@@ -54,6 +54,6 @@ def test_solid_ks_thickness_gradient_central_differences():
     p = SolidProblem(4, dv="element")
     rng = np.random.default_rng(9)
     t0 = rng.uniform(0.5, 1.5, p.mesh.nelem)
-    prob = ThicknessFunctionProblem(p, func="ks", ks_weight=50.0, rtol=1e-12, m=60, mixed=False,
+    prob = ThicknessFunctionProblem(p, func="ks", ks_weight=50.0, rtol=1e-12, m=60, mixed=False, deflation=0,
                                     restarts=500)
     _fd_check(prob, t0, rng.choice(p.mesh.nelem, 6, replace=False), 1e-5, 2e-5)
