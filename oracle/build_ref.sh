@@ -19,7 +19,8 @@ SRCS="src/TACSObject.cpp src/TacsUtilities.cpp
  src/elements/basis/TACSTensorProductBasisImpl.cpp
  src/constitutive/TACSConstitutive.cpp src/constitutive/TACSMaterialProperties.cpp
  src/constitutive/TACSSolidConstitutive.cpp src/constitutive/TACSShellConstitutive.cpp
- src/constitutive/TACSIsoShellConstitutive.cpp src/constitutive/TACSBeamConstitutive.cpp"
+ src/constitutive/TACSIsoShellConstitutive.cpp src/constitutive/TACSBeamConstitutive.cpp
+ src/bpmat/BCSRMatMult.cpp src/bpmat/BCSRMatMult3.cpp src/bpmat/BCSRMatMult6.cpp"
 OBJS=""
 for s in $SRCS; do
   o=$OUT/$(basename "${s%.cpp}").o

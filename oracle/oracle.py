@@ -41,6 +41,11 @@ def lib():
         L.orc_shell_failure.restype = ctypes.c_double
         L.orc_shell_compliance.restype = ctypes.c_double
         L.orc_assemble_shell.restype = ctypes.c_long
+        L.orc_par_assemble_solid.restype = ctypes.c_long
+        L.orc_par_gmres.restype = ctypes.c_int
+        L.orc_color_elements.restype = ctypes.c_int
+        L.orc_par_threads.restype = ctypes.c_int
+        L.orc_hex_quadrature.argtypes = None
         _lib = L
     return _lib
 
@@ -452,3 +457,62 @@ def ks_xptsens(order, conn, X, u, C21, ys, ks_weight, max_fail, ks_sum, safety=1
                          _p(_dd(C21)), _c(ys), _c(ks_weight), _c(safety), _c(max_fail),
                          _c(ks_sum), _c(scale), _p(out))
     return out
+
+
+# ---------------------------------------------------------------- multi-core baseline
+# cpu_baseline.c: the same algorithms over all host cores (OpenMP), for the
+# cpu_baseline leg of bench.py; equal to the serial functions up to summation
+# order.
+def par_threads():
+    return lib().orc_par_threads()
+
+
+def color_elements(nnodes, conn, npe):
+    """Greedy element colouring (no two elements of a colour share a node):
+    (color_ptr, color_elems)."""
+    conn = _ii(conn).reshape(-1)
+    nelem = conn.size // npe
+    cptr = np.zeros(65, np.int32)
+    celems = np.zeros(max(nelem, 1), np.int32)
+    nc = lib().orc_color_elements(nelem, npe, nnodes, _p(conn, np.int32), _p(cptr, np.int32),
+                                  _p(celems, np.int32))
+    assert nc > 0
+    return cptr[:nc + 1].copy(), celems[:nelem]
+
+
+def par_assemble_solid(order, colors, conn, X, u, t_elem, C21, rowp, cols, rho=0.0,
+                       alpha=1.0, gamma=0.0, want_mat=True, want_res=True):
+    cptr, celems = colors
+    nnodes = rowp.size - 1
+    A = np.zeros(cols.size * 9) if want_mat else None
+    res = np.zeros(3 * nnodes) if want_res else None
+    missing = lib().orc_par_assemble_solid(
+        order, cptr.size - 1, _p(_ii(cptr), np.int32), _p(_ii(celems), np.int32),
+        _p(_ii(conn).reshape(-1), np.int32), _p(_dd(X)),
+        _p(_dd(u)) if u is not None else None, _p(_dd(t_elem)), _p(_dd(C21)),
+        ctypes.c_double(rho), ctypes.c_double(alpha), ctypes.c_double(gamma),
+        _p(_ii(rowp), np.int32), _p(_ii(cols), np.int32), _p(A), _p(res))
+    assert missing == 0
+    return A, res
+
+
+def par_bsr_mult(b, rowp, cols, A, x):
+    n = rowp.size - 1
+    y = np.zeros(b * n)
+    lib().orc_par_bsr_mult(b, n, _p(_ii(rowp), np.int32), _p(_ii(cols), np.int32),
+                           _p(_dd(A)), _p(_dd(x)), _p(y))
+    return y
+
+
+def par_gmres(b, rowp, cols, A, rhs, Dinv=None, m=30, nrestart=10, rtol=1e-8,
+              atol=1e-30, x0=None, max_its=0):
+    n = rowp.size - 1
+    x = np.zeros(b * n) if x0 is None else _dd(x0).copy()
+    iters = ctypes.c_int(0)
+    rn = ctypes.c_double(0.0)
+    flag = lib().orc_par_gmres(b, n, _p(_ii(rowp), np.int32), _p(_ii(cols), np.int32),
+                               _p(_dd(A)), _p(_dd(Dinv)) if Dinv is not None else None,
+                               _p(_dd(rhs)), _p(x), m, nrestart, ctypes.c_double(rtol),
+                               ctypes.c_double(atol), 1 if x0 is None else 0, max_its,
+                               ctypes.byref(iters), ctypes.byref(rn))
+    return x, flag, iters.value, rn.value

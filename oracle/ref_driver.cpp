@@ -11,6 +11,7 @@
 #include "TACSSolidConstitutive.h"
 #include "TACSTraction3D.h"
 #include "TACSPressure3D.h"
+#include "BCSRMatImpl.h"
 
 extern "C" {
 
@@ -236,4 +237,47 @@ void ref_quad4_pressure(const double* X, double p, double* res) {
   pr->decref();
   elem->decref();
 }
+
+// Speed of the reference's hot loops, for the oracle's speed ratio against
+// them on identical inputs: reps products y = A x through BCSRMatVecMult3 /
+// BCSRMatVecMult6 (src/bpmat/BCSRMatMult3.cpp:28, BCSRMatMult6.cpp:82) over
+// the caller's arrays (borrowed: handed back before the data object goes).
+void ref_bsr_mult(int b, int nrows, int* rowp, int* cols, double* A, double* x, double* y,
+                  int reps) {
+  BCSRMatData* d = new BCSRMatData(b, nrows, nrows);
+  d->incref();
+  d->rowp = rowp;
+  d->cols = cols;
+  d->A = A;
+  for (int r = 0; r < reps; r++) {
+    if (b == 3)
+      BCSRMatVecMult3(d, x, y);
+    else
+      BCSRMatVecMult6(d, x, y);
+  }
+  d->rowp = NULL;
+  d->cols = NULL;
+  d->A = NULL;
+  d->decref();
 }
+
+// Jacobians of nelem solid elements through one TACSElement3D (the body of
+// the reference's element loop, TACSAssembler.cpp:4352-4383 ->
+// TACSElement3D::addJacobian): Xe holds 3 npe coordinates per element, mat
+// receives the last element's matrix (earlier ones are overwritten).
+void ref_solid_jacobian_loop(int order, int nelem, const double* Xe, double E, double nu,
+                             double t, double* res, double* mat) {
+  TACSElement3D* elem = make_solid(order, E, nu, t);
+  elem->incref();
+  const int nv = elem->getNumVariables(), npe = nv / 3;
+  double* zero = new double[nv]();
+  for (int e = 0; e < nelem; e++) {
+    for (int i = 0; i < nv; i++) res[i] = 0.0;
+    for (int i = 0; i < nv * nv; i++) mat[i] = 0.0;
+    elem->addJacobian(0, 0.0, 1.0, 0.0, 0.0, &Xe[(size_t)3 * npe * e], zero, zero, zero, res,
+                      mat);
+  }
+  delete[] zero;
+  elem->decref();
+}
+}  // extern "C"

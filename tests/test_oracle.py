@@ -329,3 +329,48 @@ def test_traction_total_force():
         res = np.zeros(3 * m.nnodes)
         O.add_tractions(order, m.conn, m.X, ae, af, np.tile([1.0, -2.0, 3.0], (ae.size, 1)), res)
         assert np.allclose(res.reshape(-1, 3).sum(0), -np.array([1.0, -2.0, 3.0]) * 1.0, rtol=1e-13)
+
+
+def test_multicore_baseline_matches_serial_oracle():
+    """The OpenMP baseline functions (cpu_baseline.c) compute what the serial
+    oracle computes: a valid colouring, the same matrix and residual up to
+    summation order, the same product, and the same GMRES steps."""
+    from tacs_amd.meshes import HexMesh
+
+    for order, n in ((2, 6), (3, 3)):
+        m = HexMesh(n, n, n, order=order)
+        npe = order ** 3
+        cptr, celems = O.color_elements(m.nnodes, m.conn, npe)
+        assert np.array_equal(np.sort(celems), np.arange(m.nelem))
+        for c in range(cptr.size - 1):
+            els = celems[cptr[c]:cptr[c + 1]]
+            nodes = m.conn[els].reshape(-1)
+            assert np.unique(nodes).size == nodes.size  # no shared node within a colour
+        if order == 2:
+            assert cptr.size - 1 == 8  # structured hexahedra: 2 x 2 x 2 colours
+        rng = np.random.default_rng(order)
+        X = m.X.reshape(-1) + 0.02 / n * rng.standard_normal(3 * m.nnodes)
+        u = rng.standard_normal(3 * m.nnodes)
+        t = rng.uniform(0.5, 1.5, m.nelem)
+        C = O.iso_C21(70e9, 0.3)
+        rowp, cols = O.node_csr(m.nnodes, m.conn, npe)
+        A, r = O.assemble_solid(order, m.conn, X, u, t, C, rowp, cols)
+        Ap, rp = O.par_assemble_solid(order, (cptr, celems), m.conn, X, u, t, C, rowp, cols)
+        assert np.abs(Ap - A).max() <= 1e-12 * np.abs(A).max()
+        assert np.abs(rp - r).max() <= 1e-12 * np.abs(r).max()
+        # matrix only: the baseline's isotropic element kernel
+        Ai, _ = O.par_assemble_solid(order, (cptr, celems), m.conn, X, None, t, C, rowp, cols,
+                                     want_res=False)
+        assert np.abs(Ai - A).max() <= 1e-12 * np.abs(A).max()
+        x = rng.standard_normal(3 * m.nnodes)
+        assert np.array_equal(O.par_bsr_mult(3, rowp, cols, A, x), O.bsr_mult(3, rowp, cols, A, x))
+        bcn = m.nodes_on_x(0)
+        O.apply_bc_mat(3, bcn, np.full(bcn.size, 7, np.int32), rowp, cols, A)
+        D = O.bsr_diag_inv(3, rowp, cols, A)
+        rhs = rng.standard_normal(3 * m.nnodes)
+        O.apply_bc_vec(3, bcn, np.full(bcn.size, 7, np.int32), rhs)
+        xs, fs, its, rns = O.gmres(3, rowp, cols, A, rhs, D, m=20, nrestart=200, rtol=1e-10)
+        xp, fp, itp, rnp = O.par_gmres(3, rowp, cols, A, rhs, D, m=20, nrestart=200, rtol=1e-10)
+        assert fs == fp == 1
+        assert abs(its - itp) <= 2
+        assert np.abs(xp - xs).max() <= 1e-8 * np.abs(xs).max()
