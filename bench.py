@@ -60,6 +60,17 @@ class HipEvents:
         return ms.value
 
 
+def spmv_traffic(kernel):
+    """HBM bytes per launch of a product kernel at 128^3 from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE runs, scaled by a
+    copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "spmv_traffic_hex8_128cube.json")
+    try:
+        return json.load(open(path))["kernels"][kernel]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def spmv_bytes(b, nrows, nnzb, word):
     """Algorithmic bytes of y = A x: values, column indices, row pointers, x
     and y once each."""
@@ -107,10 +118,14 @@ MONITOR = {"file": None, "freq": 1000}
 
 
 def make_solver(p, solver, m, rtol, deflation=0):
+    """mixed: the recycling solver on the FP32-valued operator; recycle64: the
+    same recycling on the FP64 operator; fp64: GMRES(m) + MGS."""
     from tacs_amd import TACS
     pc = TACS.Pc(p.mat)
     if solver == "mixed":
         ksm = TACS.MixedKSM(p.mat, pc, m, 100000, deflation=deflation)
+    elif solver == "recycle64":
+        ksm = TACS.RecyclingKSM(p.mat, pc, m, 100000, deflation, prec=64)
     else:
         ksm = TACS.KSM(p.mat, pc, m, 100000)
     ksm.setTolerances(rtol, 1e-30)
@@ -122,11 +137,21 @@ def make_solver(p, solver, m, rtol, deflation=0):
     return pc, ksm
 
 
-def run_step(p, pc, ksm, want_grad=False):
+def step_work(p):
+    """Vectors and the function object of run_step, allocated once before
+    the timed region."""
+    from tacs_amd import functions
+    a = p.assembler
+    return dict(comp=functions.Compliance(a), dfdu=a.createVec(), psi=a.createVec(),
+                dfdx=a.createDesignVec())
+
+
+def run_step(p, pc, ksm, work, want_grad=False):
     """One pass of the hot path; returns timings and results."""
-    from tacs_amd import functions, lib
+    from tacs_amd import lib
     L = lib()
     a = p.assembler
+    comp, dfdu, psi, dfdx = work["comp"], work["dfdu"], work["psi"], work["dfdx"]
     t = {}
     t0 = time.perf_counter()
     a.zeroVariables()
@@ -139,15 +164,13 @@ def run_step(p, pc, ksm, want_grad=False):
     its_f = ksm.getIterCount()
     p.u.scale(-1.0)
     a.setVariables(p.u)
-    comp = functions.Compliance(a)
     c = a.evalFunctions([comp])[0]
-    dfdu = a.createVec()
+    dfdu.zeroEntries()
     a.addSVSens([comp], [dfdu])
     a.applyBCs(dfdu)
-    psi = a.createVec()
     flag_a = ksm.solve(dfdu, psi)
     its_a = ksm.getIterCount()
-    dfdx = a.createDesignVec()
+    dfdx.zeroEntries()
     a.addDVSens([comp], [dfdx], 1.0)
     a.addAdjointResProducts([psi], [dfdx], -1.0)
     L.tacs_device_synchronize()
@@ -160,46 +183,237 @@ def run_step(p, pc, ksm, want_grad=False):
     return out
 
 
-def cpu_baseline(p, gpu_its, order=2, nrhs=1, name="128^3"):
-    """The oracle (CPU restatement of the reference path, one core) timed on a
-    bounded sample of the same workload: the assembly of a slice of the
-    elements and the first Arnoldi steps of GMRES(30) on the full matrix;
-    scaled to the whole step with the GPU FP64 run's iteration count (per
-    load case, nrhs load cases)."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    return O
+
+
+def cpu_env():
+    """Host CPU model and the threads the OpenMP baseline runs on."""
+    O = _oracle()
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, O.par_threads()
+
+
+def _timed(fn):
+    t0 = time.perf_counter()
+    out = fn()
+    return time.perf_counter() - t0, out
+
+
+def cpu_solid_fraction(p, t_elem, fp64_its, name, E=70e9, nu=0.3, gmres_steps=120, nrhs=1):
+    """Measured multi-core CPU baseline on the same workload: the oracle's
+    OpenMP path (oracle/cpu_baseline.c) runs, on all host cores, the whole
+    assembly (coloured element loop, block-CSR scatter, traction loads, BC
+    rows), the block-Jacobi factorisation and gmres_steps real Arnoldi steps
+    of FP64 GMRES(30) + MGS on the full matrix (whole restart cycles, so the
+    per-step cost averages the cycle's growing Gram-Schmidt).  The step
+    estimate adds the measured per-step time times the FP64 GPU path's step
+    count (forward + adjoint, per load case)."""
+    O = _oracle()
+    model, threads = cpu_env()
     m = p.mesh
-    C = O.iso_C21(70e9, 0.3)
-    rowp, cols, A = p.mat.getArrays()
-    nsample = max(1, m.nelem // (64 if order == 2 else 256))
-    conn_s = np.ascontiguousarray(m.conn[:nsample])
-    Xs = m.X.reshape(-1)
-    t0 = time.perf_counter()
-    O.assemble_solid(order, conn_s, Xs, np.zeros(3 * m.nnodes), np.ones(nsample), C, rowp, cols)
-    t_asm = (time.perf_counter() - t0) * (m.nelem / nsample)
-    Aflat = A.reshape(-1)
-    D = O.bsr_diag_inv(3, rowp, cols, Aflat)
-    rhs = -p.res.getArray().copy()
-    kits = 8
-    t0 = time.perf_counter()
-    O.gmres(3, rowp, cols, Aflat, rhs, D, m=30, nrestart=0, rtol=1e-30, max_its=kits)
-    t_it = (time.perf_counter() - t0) / kits
-    its = sum(gpu_its)
-    step = t_asm + nrhs * its * t_it
-    out = {"cores": 1, "kind": "port",
-           "sample": (f"oracle (C restatement, 1 core) on the {name} workload: assembly of "
-                      f"{nsample} of {m.nelem} elements ({t_asm:.1f} s scaled), first {kits} "
-                      f"Arnoldi steps of FP64 GMRES(30)+block-Jacobi on the full matrix "
-                      f"({t_it:.2f} s/step), scaled to the {its} steps per load case the GPU "
-                      f"FP64 path took" + (f", {nrhs} load cases" if nrhs > 1 else "") +
-                      " (later steps orthogonalise against more vectors, so this "
-                      "underestimates the CPU time)"),
-           "step_s_estimate": step}
+    X = m.X.reshape(-1)
+    C = O.iso_C21(E, nu)
+    rowp, cols = O.node_csr(m.nnodes, m.conn, m.npe)
+    t_col, colors = _timed(lambda: O.color_elements(m.nnodes, m.conn, m.npe))
+    t_asm, (A, _) = _timed(lambda: O.par_assemble_solid(m.order, colors, m.conn, X, None, t_elem,
+                                                        C, rowp, cols, want_res=False))
+    res = np.zeros(3 * m.nnodes)
+    trac = np.tile(p.traction, (p.aux_elems.size, 1))
+    t_tr, _ = _timed(lambda: O.add_tractions(m.order, m.conn, X, p.aux_elems, p.aux_faces, trac,
+                                             res))
+    bcv = np.full(p.bc_nodes.size, 7, np.int32)
+    t_bc, _ = _timed(lambda: (O.apply_bc_mat(3, p.bc_nodes, bcv, rowp, cols, A),
+                              O.apply_bc_vec(3, p.bc_nodes, bcv, res)))
+    t_pc, D = _timed(lambda: O.bsr_diag_inv(3, rowp, cols, A))
+    steps = max(30, 30 * (gmres_steps // 30))
+    t_gm, out = _timed(lambda: O.par_gmres(3, rowp, cols, A, -res, D, m=30, nrestart=100000,
+                                           rtol=1e-30, max_its=steps))
+    t_step = t_gm / out[2]
+    its = int(sum(fp64_its))
+    asm_s = t_col + t_asm + t_tr + t_bc
+    step_s = asm_s + t_pc + nrhs * its * t_step
+    measured_s = asm_s + t_pc + t_gm
+    res_d = {
+        "cores": threads, "kind": "port", "cpu_model": model,
+        "sample": (f"oracle OpenMP path (oracle/cpu_baseline.c) on {threads} threads of {model}, "
+                   f"the {name} workload: the whole assembly ({asm_s:.2f} s incl. colouring, "
+                   f"tractions, BC rows) and block-Jacobi ({t_pc:.2f} s) run in full, then "
+                   f"{out[2]} real FP64 GMRES(30)+MGS steps on the full matrix "
+                   f"({1e3 * t_step:.1f} ms/step); the step is that plus the measured per-step "
+                   f"time x the {its} steps" + (f" per load case x {nrhs} load cases" if nrhs > 1
+                                                else "") +
+                   f" the GPU FP64 path took ({measured_s:.1f} s of CPU work measured)"),
+        "assembly_s": asm_s, "pc_factor_s": t_pc, "gmres_step_s": t_step,
+        "gmres_steps_measured": out[2], "step_s_estimate": step_s,
+    }
+    del A
     if nrhs > 1:
-        out.update(value=nrhs / step, unit="load cases/s")
+        res_d.update(value=nrhs / step_s, unit="load cases/s")
     else:
-        out.update(value=m.nelem / step / 1e6, unit="Melem/s")
-    return out
+        res_d.update(value=m.nelem / step_s / 1e6, unit="Melem/s")
+    return res_d
+
+
+def cpu_shell_fraction(p, t_elem, its, name, E=70e9, nu=0.3, gmres_steps=120):
+    """Measured multi-core CPU baseline of the shell workload: the whole
+    assembly (coloured Quad4 element loop, 6x6 block scatter, pressure loads,
+    BC rows) and block-Jacobi on all host cores, then gmres_steps real FP64
+    GMRES(30)+MGS steps on the full matrix; the step adds the measured
+    per-step time x the GPU step count `its` (a lower bound for GMRES(30),
+    which needs more steps than the recycling solver)."""
+    O = _oracle()
+    model, threads = cpu_env()
+    m = p.mesh
+    X = m.X.reshape(-1)
+    rowp, cols = O.node_csr(m.nnodes, m.conn, 4)
+    colors = O.color_elements(m.nnodes, m.conn, 4)
+    t0 = time.perf_counter()
+    A, res = O.par_assemble_shell(colors, m.conn, X, None, t_elem, E, nu, rowp, cols)
+    O.shell_add_pressure(m.conn, X, p.aux_elems, np.full(p.aux_elems.size, p.pressure), res)
+    mask = 0
+    for v in p.bc_vars:
+        mask |= 1 << v
+    bm = np.full(p.bc_nodes.size, mask, np.int32)
+    O.apply_bc_mat(6, p.bc_nodes, bm, rowp, cols, A)
+    O.apply_bc_vec(6, p.bc_nodes, bm, res)
+    asm_s = time.perf_counter() - t0
+    t_pc, D = _timed(lambda: O.bsr_diag_inv(6, rowp, cols, A))
+    steps = max(30, 30 * (gmres_steps // 30))
+    t_gm, out = _timed(lambda: O.par_gmres(6, rowp, cols, A, -res, D, m=30, nrestart=100000,
+                                           rtol=1e-30, max_its=steps))
+    t_step = t_gm / out[2]
+    total = int(sum(its))
+    step_s = asm_s + t_pc + total * t_step
+    return {"cores": threads, "kind": "port", "cpu_model": model,
+            "sample": (f"oracle OpenMP path on {threads} threads of {model}, the {name} "
+                       f"workload: the whole assembly ({asm_s:.2f} s) and block-Jacobi "
+                       f"({t_pc:.2f} s) in full, {out[2]} real FP64 GMRES(30)+MGS steps "
+                       f"({1e3 * t_step:.1f} ms/step); the step adds the per-step time x the "
+                       f"{total} steps of the GPU recycling solver (GMRES(30) needs more, so "
+                       f"this is a lower bound of the CPU time)"),
+            "assembly_s": asm_s, "pc_factor_s": t_pc, "gmres_step_s": t_step,
+            "step_s_estimate": step_s, "value": m.nelem / step_s / 1e6, "unit": "Melem/s"}
+
+
+def cpu_multi(p, fp64_its, gmres_steps=30):
+    """Measured multi-core CPU baseline of the many-cube workload, cube by
+    cube as the reference runs them (one assembler and one GMRES per mesh):
+    every cube's whole assembly and block-Jacobi on all host cores, and
+    gmres_steps real FP64 GMRES(30)+MGS steps on its matrix; each cube adds
+    its measured per-step time x its FP64 GPU step count (forward solve)."""
+    O = _oracle()
+    model, threads = cpu_env()
+    C = O.iso_C21(70e9, 0.3)
+    total, measured = 0.0, 0.0
+    for c, m in enumerate(p.meshes):
+        X = m.X.reshape(-1)
+        rowp, cols = O.node_csr(m.nnodes, m.conn, 8)
+        colors = O.color_elements(m.nnodes, m.conn, 8)
+        t0 = time.perf_counter()
+        A, _ = O.par_assemble_solid(2, colors, m.conn, X, None, np.ones(m.nelem), C, rowp, cols,
+                                    want_res=False)
+        res = np.zeros(3 * m.nnodes)
+        ae, af = m.elems_on_x(1)
+        O.add_tractions(2, m.conn, X, ae, af, np.tile(p.traction, (ae.size, 1)), res)
+        bn = m.nodes_on_x(0)
+        bv = np.full(bn.size, 7, np.int32)
+        O.apply_bc_mat(3, bn, bv, rowp, cols, A)
+        O.apply_bc_vec(3, bn, bv, res)
+        D = O.bsr_diag_inv(3, rowp, cols, A)
+        t_asm = time.perf_counter() - t0
+        t_gm, out = _timed(lambda: O.par_gmres(3, rowp, cols, A, -res, D, m=30, nrestart=100000,
+                                               rtol=1e-30, max_its=gmres_steps))
+        total += t_asm + fp64_its[c] * t_gm / out[2]
+        measured += t_asm + t_gm
+    return {"cores": threads, "kind": "port", "cpu_model": model,
+            "sample": (f"oracle OpenMP path on {threads} threads of {model}, cube by cube: every "
+                       f"cube's whole assembly and block-Jacobi plus {gmres_steps} real FP64 "
+                       f"GMRES(30)+MGS steps ({measured:.1f} s measured); each cube adds its "
+                       f"per-step time x its GPU FP64 GMRES(30) step count"),
+            "step_s_estimate": total, "value": len(p.meshes) / total, "unit": "cubes/s"}
+
+
+def cfg1_cube20(L, rtol=1e-8):
+    """The 20^3 cube of the reference's CPU-runnable case, run in full on the
+    GPU (FP64 GMRES(30) + MGS, the reference's settings) and by the oracle's
+    OpenMP path on all host cores (no extrapolation): assembly, solve,
+    compliance, adjoint solve, node gradient."""
+    from tacs_amd import functions
+    from tacs_amd.problems import SolidProblem
+    O = _oracle()
+    model, threads = cpu_env()
+    p = SolidProblem(20)
+    a, m = p.assembler, p.mesh
+    comp = functions.Compliance(a)
+    dfdu, psi, g = a.createVec(), a.createVec(), a.createNodeVec()
+
+    def gpu_step():
+        t0 = time.perf_counter()
+        a.zeroVariables()
+        a.assembleJacobian(1.0, 0.0, 0.0, p.res, p.mat)
+        flag, its, pc, ksm = p.solve_fp64(m=30, restarts=100000, rtol=rtol)
+        c = a.evalFunctions([comp])[0]
+        dfdu.zeroEntries()
+        a.addSVSens([comp], [dfdu])
+        a.applyBCs(dfdu)
+        flag2 = ksm.solve(dfdu, psi)
+        its2 = ksm.getIterCount()
+        g.zeroEntries()
+        a.addXptSens([comp], [g])
+        a.addAdjointResXptSensProducts([psi], [g], -1.0)
+        L.tacs_device_synchronize()
+        return time.perf_counter() - t0, c, (its, its2), (flag, flag2)
+
+    gpu_step()
+    t_gpu, c_gpu, its_gpu, flags = gpu_step()
+    g_gpu = g.getArray().copy()
+    # the CPU: the same step by the oracle's OpenMP path
+    X = m.X.reshape(-1)
+    t = np.ones(m.nelem)
+    C = O.iso_C21(70e9, 0.3)
+    # the pattern and the colouring are set-up, as the GPU's initialize()
+    rowp, cols = O.node_csr(m.nnodes, m.conn, 8)
+    colors = O.color_elements(m.nnodes, m.conn, 8)
+    t0 = time.perf_counter()
+    A, _ = O.par_assemble_solid(2, colors, m.conn, X, None, t, C, rowp, cols, want_res=False)
+    res = np.zeros(3 * m.nnodes)
+    trac = np.tile(p.traction, (p.aux_elems.size, 1))
+    O.add_tractions(2, m.conn, X, p.aux_elems, p.aux_faces, trac, res)
+    bcv = np.full(p.bc_nodes.size, 7, np.int32)
+    O.apply_bc_mat(3, p.bc_nodes, bcv, rowp, cols, A)
+    O.apply_bc_vec(3, p.bc_nodes, bcv, res)
+    D = O.bsr_diag_inv(3, rowp, cols, A)
+    t_asm = time.perf_counter() - t0
+    u, f1, it1, _ = O.par_gmres(3, rowp, cols, A, -res, D, m=30, nrestart=100000, rtol=rtol)
+    c_cpu = O.compliance(2, m.conn, X, u, t, C)
+    dc = O.compliance_svsens(2, m.conn, X, u, t, C)
+    O.apply_bc_vec(3, p.bc_nodes, bcv, dc)
+    ps, f2, it2, _ = O.par_gmres(3, rowp, cols, A, dc, D, m=30, nrestart=100000, rtol=rtol)
+    g_cpu = (O.compliance_xptsens(2, m.conn, X, u, t, C)
+             - O.adj_res_xpt_product(2, m.conn, X, u, ps, t, C)
+             - O.traction_xpt_product(2, m.conn, X, p.aux_elems, p.aux_faces, trac, ps))
+    t_cpu = time.perf_counter() - t0
+    return {
+        "workload": "linear hex cube 20^3 (8,000 elements, 27,783 DOF), E=70e9 nu=0.3, x=0 "
+                    "clamped, traction 1e6 Pa on x=1: assembly, FP64 GMRES(30)+MGS+block-Jacobi "
+                    f"to {rtol:g}, compliance, adjoint solve, node gradient",
+        "gpu_step_s": t_gpu, "gpu_iterations": list(its_gpu), "gpu_flags": list(flags),
+        "cpu_step_s": t_cpu, "cpu_assembly_s": t_asm, "cpu_iterations": [it1, it2],
+        "cpu_flags": [f1, f2], "cpu_cores": threads, "cpu_model": model,
+        "gpu_vs_cpu": t_cpu / t_gpu,
+        "compliance_rel_diff": abs(c_gpu - c_cpu) / abs(c_cpu),
+        "node_gradient_rel_diff": float(np.abs(g_gpu - g_cpu).max() / np.abs(g_cpu).max()),
+    }
 
 
 def shell_step(p, pc, ksm, ks_weight=100.0, want_grad=False):
@@ -249,7 +463,8 @@ def shell_main(args, L, ev):
     t0 = time.perf_counter()
     p = ShellProblem(n, dv="element", pressure=1e4, bc="ss", t0=0.01)
     rng = np.random.default_rng(7)
-    p.set_thickness(rng.uniform(0.005, 0.02, p.mesh.nelem))
+    t_elem = rng.uniform(0.005, 0.02, p.mesh.nelem)
+    p.set_thickness(t_elem)
     log(f"setup {time.perf_counter() - t0:.1f}s: {p.mesh.nelem} shells, {6 * p.mesh.nnodes} DOF")
     p.assemble()
     e0, e1 = ev.event(), ev.event()
@@ -258,9 +473,20 @@ def shell_main(args, L, ev):
         p.assembler.assembleJacobian(1.0, 0.0, 0.0, None, p.mat)
     ev.record(e1)
     asm_ms = ev.elapsed_ms(e0, e1) / 5
+    # the same assembly with FP32 element matrices (summed in FP64)
+    p.assembler.setElementPrecision(32)
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0)
+    for _ in range(5):
+        p.assembler.assembleJacobian(1.0, 0.0, 0.0, None, p.mat)
+    ev.record(e1)
+    asm32_ms = ev.elapsed_ms(e0, e1) / 5
+    p.assembler.setElementPrecision(64)
+    p.assemble()
     sp = time_spmv(L, ev, p)
     nelem = p.mesh.nelem
     kern = {"assembly_ms": asm_ms, "assembly_Melem_s": nelem / (asm_ms * 1e-3) / 1e6,
+            "assembly_fp32_elements_ms": asm32_ms,
             "spmv6_fp64": {"avg_ms": sp["fp64_ms"], "bytes": sp["fp64_bytes"],
                            "GBps": sp["fp64_bytes"] / sp["fp64_ms"] / 1e6},
             "spmv6_fp32": {"avg_ms": sp["fp32_ms"], "bytes": sp["fp32_bytes"],
@@ -278,6 +504,15 @@ def shell_main(args, L, ev):
         steps.append(shell_step(p, pc, ksm, want_grad=(k == args.steps - 1)))
         log(f"step {k}: {steps[-1]['t']} its={steps[-1]['its']} flags={steps[-1]['flags']}")
     wall = time.perf_counter() - t0
+    # the same step with FP32 element matrices, against the FP64-element step
+    fp32e = None
+    if not args.no_fp64:
+        p.assembler.setElementPrecision(32)
+        pc32, ksm32 = make_solver(p, "mixed", args.m, args.rtol, args.deflation)
+        fp32e = shell_step(p, pc32, ksm32, want_grad=True)
+        p.assembler.setElementPrecision(64)
+        log(f"fp32-element step: {fp32e['t']} its={fp32e['its']} flags={fp32e['flags']}")
+        del pc32, ksm32
     ke_bytes = nelem * 576 * 8
     out = {
         "workload": f"Quad4 shell plate {n}x{n}, t ~ U[0.005, 0.02], E=70e9 nu=0.3, simply "
@@ -297,7 +532,22 @@ def shell_main(args, L, ev):
         "spmv6_fp32": {"avg_ms": sp["fp32_ms"], "bytes": sp["fp32_bytes"],
                        "GBps": sp["fp32_bytes"] / sp["fp32_ms"] / 1e6,
                        "frac": sp["fp32_bytes"] / sp["fp32_ms"] / 1e6 / 8000.0},
+        "assembly_fp32_elements_ms": asm32_ms,
+        "assembly_fp32_elements_Melem_s": nelem / (asm32_ms * 1e-3) / 1e6,
     }
+    if fp32e is not None:
+        g64 = steps[-1]["grad"]
+        out["fp32_elements"] = {
+            "solve_adjoint_s": fp32e["t"]["solve_adjoint_s"], "iterations": list(fp32e["its"]),
+            "flags": list(fp32e["flags"]),
+            "ks_rel_err_vs_fp64_elements": abs(fp32e["ks"] - steps[-1]["ks"]) / abs(steps[-1]["ks"]),
+            "ks_grad_rel_err_vs_fp64_elements": float(np.abs(fp32e["grad"] - g64).max()
+                                                      / np.abs(g64).max()),
+            "note": "the same step with the element matrices computed in FP32 (summed into "
+                    "FP64 block-CSR values); the errors are against the FP64-element step, "
+                    "whose elements match the FP64 oracle to 1e-10"}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_shell_fraction(p, t_elem, steps[-1]["its"], f"{n}^2 Quad4")
     print(json.dumps(out))
 
 
@@ -424,6 +674,13 @@ def simp_main(args, L, ev):
                    grad_rel_err_vs_fp64=float(np.abs(g - g64).max() / np.abs(g64).max()),
                    compliance_rel_err_vs_fp64=abs(c - c64) / abs(c64))
         del sp64
+    if not args.no_cpu:
+        from tacs_amd.meshes import simp_thickness
+        its64 = out.get("iterations_fp64", its)
+        cb = cpu_solid_fraction(prob, simp_thickness(x0), its64, f"SIMP {n}^3 hex8", E=1.0,
+                                nu=0.3)
+        cb.update(value=1.0 / cb["step_s_estimate"], unit="gradients/s")
+        out["cpu_baseline"] = cb
     if args.fd:
         # The solves reach about 1e-13 relative residual (the FP64 rounding
         # floor of this stiffness contrast), so the compliance is accurate to
@@ -609,8 +866,8 @@ def batched_main(args, L, ev):
     out["speedup_batched_vs_single_recycling"] = seq["single_recycling_s_per_case"] * R / solve_s
     if not args.no_cpu:
         its = seq.get("fp64_iterations", last["its"])
-        out["cpu_baseline"] = cpu_baseline(p, [int(np.mean(its))], order=3, nrhs=R,
-                                           name=f"hex27 {n}^3 x {R}")
+        out["cpu_baseline"] = cpu_solid_fraction(p, np.ones(p.mesh.nelem), [int(np.mean(its))],
+                                                 f"hex27 {n}^3 x {R}", nrhs=R)
     print(json.dumps(out))
 
 
@@ -740,6 +997,8 @@ def multi_main(args, L, ev):
         out["cube_by_cube_fp64_gmres_solve_s"] = t_fp64
         out["cube_by_cube_fp64_iterations"] = its_fp64
         out["speedup_vs_fp64_cube_by_cube_solves"] = t_fp64 / (ms * 1e-3)
+        if not args.no_cpu:
+            out["cpu_baseline"] = cpu_multi(p, its_fp64)
     print(json.dumps(out))
 
 
@@ -935,40 +1194,64 @@ def main():
         print(json.dumps(sp))
         return
     pc, ksm = make_solver(p, "mixed", args.m, args.rtol, args.deflation)
+    work = step_work(p)
     for w in range(args.warmup):
-        r = run_step(p, pc, ksm)
+        r = run_step(p, pc, ksm, work)
         log(f"warmup {w}: {r['t']} its={r['its']}")
     L.tacs_device_synchronize()
     t0 = time.perf_counter()
     steps = []
     for k in range(args.steps):
-        steps.append(run_step(p, pc, ksm, want_grad=(k == args.steps - 1)))
+        steps.append(run_step(p, pc, ksm, work, want_grad=(k == args.steps - 1)))
         log(f"step {k}: {steps[-1]['t']} its={steps[-1]['its']} flags={steps[-1]['flags']}")
     L.tacs_device_synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / args.steps
     mixed = steps[-1]
 
-    # assembly kernel timing and SpMV roofline (HIP events)
-    e0, e1 = ev.event(), ev.event()
-    ev.record(e0)
-    for _ in range(3):
-        p.assembler.assembleJacobian(1.0, 0.0, 0.0, None, p.mat)
-    ev.record(e1)
-    asm_ms = ev.elapsed_ms(e0, e1) / 3
+    # kernel timings (HIP events on the library's stream): the assembly as the
+    # step runs it (Jacobian with its FP32 copy, residual), the Jacobian alone
+    # (the roofline_assembly kernel), and the operator build of a step
+    # (assembly + the FP64 and FP32 sliced ELL copies the products read)
+    a = p.assembler
+
+    def ev_ms(fn, reps=3):
+        e0, e1 = ev.event(), ev.event()
+        ev.record(e0)
+        for _ in range(reps):
+            fn()
+        ev.record(e1)
+        return ev.elapsed_ms(e0, e1) / reps
+
+    asm_ms = ev_ms(lambda: a.assembleJacobian(1.0, 0.0, 0.0, p.res, p.mat))
+    jac_ms = ev_ms(lambda: a.assembleJacobian(1.0, 0.0, 0.0, None, p.mat))
+
+    def build():
+        a.assembleJacobian(1.0, 0.0, 0.0, p.res, p.mat)
+        p.mat.prepareProducts(64)
+        p.mat.prepareProducts(32)
+
+    build_ms = ev_ms(build)
     sp = time_spmv(L, ev, p)
     peak_bw = 8000.0
     bw64 = sp["fp64_bytes"] / sp["fp64_ms"] / 1e6
     bw32 = sp["fp32_bytes"] / sp["fp32_ms"] / 1e6
     bwm = sp["mixed_bytes"] / sp["mixed_ms"] / 1e6
     jac_flops = 69120.0 * p.mesh.nelem  # dense B^T C B count per 8-node element
-    jac_tf = jac_flops / (asm_ms * 1e-3) / 1e12
+    jac_tf = jac_flops / (jac_ms * 1e-3) / 1e12
 
-    fp64 = None
+    fp64 = rec64 = None
     if not args.no_fp64:
         del pc, ksm
+        # the same recycling algorithm on the FP64 operator: separates the
+        # algorithm's share of the speed-up from the precision's
+        pcr, ksmr = make_solver(p, "recycle64", args.m, args.rtol, args.deflation)
+        run_step(p, pcr, ksmr, work)
+        rec64 = run_step(p, pcr, ksmr, work, want_grad=True)
+        log(f"fp64 recycling step: {rec64['t']} its={rec64['its']} flags={rec64['flags']}")
+        del pcr, ksmr
         pc64, ksm64 = make_solver(p, "fp64", args.m_fp64, args.rtol)
-        fp64 = run_step(p, pc64, ksm64, want_grad=True)
+        fp64 = run_step(p, pc64, ksm64, work, want_grad=True)
         log(f"fp64 step: {fp64['t']} its={fp64['its']} flags={fp64['flags']}")
 
     out = {
@@ -994,13 +1277,22 @@ def main():
                    "nnz_blocks": p.mat.getInfo()[2]},
         "assembly_Melem_s": p.mesh.nelem / (asm_ms * 1e-3) / 1e6,
         "assembly_ms": asm_ms,
+        "assembly_note": "assembleJacobian as the step runs it: Jacobian (FP64 values and "
+                         "the FP32 copy) and residual",
+        "jacobian_ms": jac_ms,
+        "operator_build_ms": build_ms,
+        "operator_build_Melem_s": p.mesh.nelem / (build_ms * 1e-3) / 1e6,
+        "operator_build_note": "assembly + the FP64 and FP32 sliced ELL copies the products "
+                               "read (rebuilt every step, the matrix is reassembled)",
         "solve_adjoint_s_mixed": mixed["t"]["solve_adjoint_s"],
         "iterations_mixed": list(mixed["its"]),
         "roofline": {"bound": "hbm", "achieved": bw64, "peak": peak_bw, "unit": "GB/s",
-                     "frac": bw64 / peak_bw, "traffic": None,
+                     "frac": bw64 / peak_bw, "traffic": spmv_traffic("sell_spmv<3,2,double,double>"),
                      "kernel": "sell3_spmv<double,double> (FP64 3x3 block SpMV over the "
                                "sliced ELL copy, plain path)",
-                     "avg_ms": sp["fp64_ms"], "algorithmic_bytes": sp["fp64_bytes"]},
+                     "avg_ms": sp["fp64_ms"], "algorithmic_bytes": sp["fp64_bytes"],
+                     "traffic_source": "profiles/spmv_traffic_hex8_128cube.json (rocprofv3 "
+                                       "FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)"},
         "roofline_mixed_spmv": {"bound": "hbm", "achieved": bwm, "peak": peak_bw,
                                 "unit": "GB/s", "frac": bwm / peak_bw, "avg_ms": sp["mixed_ms"],
                                 "algorithmic_bytes": sp["mixed_bytes"],
@@ -1008,9 +1300,12 @@ def main():
         "roofline_fp32_spmv": {"bound": "hbm", "achieved": bw32, "peak": peak_bw,
                                "unit": "GB/s", "frac": bw32 / peak_bw, "avg_ms": sp["fp32_ms"],
                                "algorithmic_bytes": sp["fp32_bytes"]},
-        "roofline_assembly": {"bound": "mfma", "achieved": jac_tf, "peak": 78.6,
-                              "unit": "TFLOP/s", "frac": jac_tf / 78.6,
-                              "note": "69,120 FLOP/element dense B^T C B count; FP64 vector peak"},
+        "roofline_assembly": {"bound": "fp64_valu", "achieved": jac_tf, "peak": 78.6,
+                              "unit": "TFLOP/s", "frac": jac_tf / 78.6, "avg_ms": jac_ms,
+                              "kernel": "solid_jacobian_rows<2,true,false> (no MFMA: FP64 "
+                                        "vector ALU)",
+                              "note": "69,120 FLOP/element dense B^T C B count over the "
+                                      "Jacobian alone; MI355X FP64 peak"},
     }
     if fp64 is not None:
         g, g64 = mixed["grad"], fp64["grad"]
@@ -1022,9 +1317,28 @@ def main():
                                          / np.abs(fp64["u"]).max())
         out["compliance_rel_err_vs_fp64"] = abs(mixed["compliance"] - fp64["compliance"]) / abs(
             fp64["compliance"])
+    if rec64 is not None:
+        t_mix = mixed["t"]["solve_adjoint_s"]
+        t_rec = rec64["t"]["solve_adjoint_s"]
+        t_64 = fp64["t"]["solve_adjoint_s"]
+        out["solve_adjoint_s_fp64_recycling"] = t_rec
+        out["iterations_fp64_recycling"] = list(rec64["its"])
+        out["speedup_split"] = {
+            "algorithm": t_64 / t_rec,
+            "precision": t_rec / t_mix,
+            "total": t_64 / t_mix,
+            "note": f"algorithm: FP64 GMRES({args.m_fp64})+MGS -> FP64 GCRO-DR({args.m}, "
+                    f"{args.deflation}) with delayed CGS2; precision: that -> the same solver "
+                    "on the FP32-valued operator",
+        }
+        g64r = rec64["grad"]
+        out["grad_rel_err_mixed_vs_fp64_recycling"] = float(
+            np.abs(mixed["grad"] - g64r).max() / np.abs(g64r).max())
     if not args.no_cpu:
         its = fp64["its"] if fp64 is not None else mixed["its"]
-        out["cpu_baseline"] = cpu_baseline(p, its)
+        out["cpu_baseline"] = cpu_solid_fraction(p, np.ones(p.mesh.nelem), its,
+                                                 f"{args.n}^3 hex8")
+        out["cfg1_cube20"] = cfg1_cube20(L, args.rtol)
     print(json.dumps(out))
 
 

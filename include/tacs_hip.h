@@ -286,11 +286,16 @@ int tacs_hip_bsr3_spmm(int nrows, int nrhs, const int* rowp, const int* cols, co
                 output may be NULL); returns the number of slots;
    sell_fill:   values (FP64 or FP32, prec 64/32) and, if cs != NULL, columns
                 from the BCSR arrays;
+   sell_fill2:  the FP64 copy v64 and/or the FP32 copy v32 (either may be
+                NULL) in one pass over the BCSR values;
    sell_spmv:   values prec_vals (64/32), vectors prec_vecs (64/32). */
 size_t tacs_hip_sell_slots(int nrows, const int* rowp_host, int* width, long* coff);
 int tacs_hip_sell_fill(int b, int prec, int nrows, const int* rowp, const int* cols,
                        const int* width, const long* coff, const double* A, void* vs, int* cs,
                        hipStream_t stream);
+int tacs_hip_sell_fill2(int b, int nrows, const int* rowp, const int* cols, const int* width,
+                        const long* coff, const double* A, double* v64, float* v32, int* cs,
+                        hipStream_t stream);
 int tacs_hip_sell_spmv(int b, int prec_vals, int prec_vecs, int nrows, const int* width,
                        const long* coff, const int* cs, const void* vs, const void* x, void* y,
                        hipStream_t stream);
@@ -635,6 +640,11 @@ int tacs_assembler_add_matrix_free_vec_product(tacs_obj a, int matType, const do
    throughout; FP32 values are the FP64 values rounded to nearest. */
 int tacs_mat_mult_prec(tacs_obj m, int prec_vals, int prec_vecs, const void* x, void* y);
 int tacs_mat_zero_entries(tacs_obj m);
+/* Brings the product layouts of an assembled matrix up to date with its
+   values now (the sliced ELL copy in FP64, prec 64, or FP32, prec 32), as
+   the first product after a change of the values otherwise does; no
+   counterpart in the reference. */
+int tacs_mat_prepare_products(tacs_obj m, int prec);
 int tacs_mat_get_info(tacs_obj m, int* bsize, int* nrows, long* nnzb);
 /* Host copies of the block-CSR arrays (BCSRMat::getArrays BCSRMat.h:82). */
 int tacs_mat_get_arrays(tacs_obj m, const int** rowp, const int** cols, const double** A);

@@ -103,6 +103,11 @@ class Mat(TACSHandle):
     def zeroEntries(self):
         lib().tacs_mat_zero_entries(self.ptr)
 
+    def prepareProducts(self, prec=64):
+        """Build the products' sliced ELL copy of the current values (FP64 or
+        FP32) now instead of at the first product."""
+        check(lib().tacs_mat_prepare_products(self.ptr, int(prec)), "Mat.prepareProducts")
+
     def getInfo(self):
         b, n, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
         lib().tacs_mat_get_info(self.ptr, ctypes.byref(b), ctypes.byref(n), ctypes.byref(nnz))

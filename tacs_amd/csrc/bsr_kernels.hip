@@ -395,16 +395,60 @@ __global__ __launch_bounds__(256) void sell_spmv(int nslices, int nrows,
   }
 }
 
-// Fill the sliced ELL values (and, when cs is not NULL, the columns) from the
-// BCSR arrays: one lane per row, slot by slot.
-template <int B, typename TA>
+// Sliced ELL copy of the block-CSR values (and, with cs, the columns): one
+// wavefront per slice of 64 rows, T slots at a time.  The wave reads the T
+// consecutive blocks of each of the 64 rows as one contiguous run (lanes over
+// the run's values: coalesced) into LDS, then writes slot t, value q of the
+// 64 rows as 64 consecutive entries (coalesced).  Padding slots (t beyond a
+// row's length) are zero blocks in the row's own column.  v64 and/or v32
+// receive the FP64 and FP32 copies in the same pass.
+// Sliced ELL copy of the block-CSR values (and, with cs, the columns): one
+// lane per row of a 64-row slice, T slots per pass.  A lane issues the loads
+// of its row's T consecutive blocks (one contiguous run of T b^2 values) back
+// to back, so every cache line of the run is fetched once and consumed by the
+// burst, then writes slot t, value q: the wave's 64 lanes store 64
+// consecutive entries.  Padding slots (t beyond a row's length) are zero
+// blocks in the row's own column.  v64 and/or v32 receive the FP64 and FP32
+// copies in the same pass.
+template <int B>
+struct SellFillCfg {
+  static constexpr int T = B == 3 ? 4 : 1;  // slots per pass (8: 1.3x fetch, but occupancy 2 and slower)
+};
+template <int B, int TW>
+__device__ __forceinline__ void sell_fill_pass(int lane, int t0, int rs, int nb, int rowc,
+                                               long vo, long co, const int* __restrict__ cols,
+                                               const double* __restrict__ A,
+                                               double* __restrict__ v64, float* __restrict__ v32,
+                                               int* __restrict__ cs) {
+  constexpr int B2 = B * B;
+  const double* a = A + (size_t)B2 * (rs + t0);
+  double v[TW * B2];
+#pragma unroll
+  for (int t = 0; t < TW; t++) {
+    const bool use = t0 + t < nb;
+#pragma unroll
+    for (int q = 0; q < B2; q++) v[t * B2 + q] = use ? a[t * B2 + q] : 0.0;
+  }
+#pragma unroll
+  for (int t = 0; t < TW; t++) {
+#pragma unroll
+    for (int q = 0; q < B2; q++) {
+      const size_t o = vo + 64 * ((size_t)B2 * (t0 + t) + q) + lane;
+      if (v64) v64[o] = v[t * B2 + q];
+      if (v32) v32[o] = (float)v[t * B2 + q];
+    }
+    if (cs) cs[co + 64 * (t0 + t) + lane] = t0 + t < nb ? cols[rs + t0 + t] : rowc;
+  }
+}
+template <int B>
 __global__ __launch_bounds__(256) void sell_fill(int nslices, int nrows, const int* __restrict__ rowp,
                                                  const int* __restrict__ cols,
                                                  const int* __restrict__ width,
                                                  const long* __restrict__ coff,
-                                                 const double* __restrict__ A, TA* __restrict__ vs,
+                                                 const double* __restrict__ A,
+                                                 double* __restrict__ v64, float* __restrict__ v32,
                                                  int* __restrict__ cs) {
-  constexpr int B2 = B * B;
+  constexpr int B2 = B * B, T = SellFillCfg<B>::T;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int s = blockIdx.x * 4 + wid; s < nslices; s += gridDim.x * 4) {
     const int w = width[s];
@@ -412,13 +456,12 @@ __global__ __launch_bounds__(256) void sell_fill(int nslices, int nrows, const i
     const int row = 64 * s + lane;
     const int rs = row < nrows ? rowp[row] : 0;
     const int nb = row < nrows ? rowp[row + 1] - rs : 0;
-    for (int t = 0; t < w; t++) {
-      const bool use = t < nb;
-      if (cs) cs[co + 64 * t + lane] = use ? cols[rs + t] : (row < nrows ? row : 0);
-#pragma unroll
-      for (int q = 0; q < B2; q++)
-        vs[vo + 64 * (B2 * t + q) + lane] = use ? (TA)A[B2 * ((size_t)rs + t) + q] : (TA)0;
-    }
+    const int rowc = row < nrows ? row : 0;
+    int t0 = 0;
+    for (; t0 + T <= w; t0 += T)
+      sell_fill_pass<B, T>(lane, t0, rs, nb, rowc, vo, co, cols, A, v64, v32, cs);
+    for (; t0 < w; t0++)
+      sell_fill_pass<B, 1>(lane, t0, rs, nb, rowc, vo, co, cols, A, v64, v32, cs);
   }
 }
 
@@ -502,26 +545,29 @@ size_t tacs_hip_sell_slots(int nrows, const int* rowp_host, int* width, long* co
   return (size_t)off;
 }
 
+int tacs_hip_sell_fill2(int b, int nrows, const int* rowp, const int* cols, const int* width,
+                        const long* coff, const double* A, double* v64, float* v32, int* cs,
+                        hipStream_t stream) {
+  const int ns = (nrows + 63) / 64;
+  if ((b != 3 && b != 6) || (!v64 && !v32)) return TACS_HIP_ERR_ARG;
+  if (ns <= 0) return 0;
+  const int grid = tacs_grid_for(ns, 4, 65536);
+  if (b == 3)
+    hipLaunchKernelGGL(sell_fill<3>, dim3(grid), dim3(256), 0, stream, ns, nrows, rowp, cols,
+                       width, coff, A, v64, v32, cs);
+  else
+    hipLaunchKernelGGL(sell_fill<6>, dim3(grid), dim3(256), 0, stream, ns, nrows, rowp, cols,
+                       width, coff, A, v64, v32, cs);
+  return (int)hipGetLastError();
+}
+
 int tacs_hip_sell_fill(int b, int prec, int nrows, const int* rowp, const int* cols,
                        const int* width, const long* coff, const double* A, void* vs, int* cs,
                        hipStream_t stream) {
-  const int ns = (nrows + 63) / 64;
-  if ((b != 3 && b != 6) || (prec != 64 && prec != 32)) return TACS_HIP_ERR_ARG;
-  if (ns <= 0) return 0;
-  const int grid = tacs_grid_for(ns, 4, 65536);
-#define SELL_FILL(BB, TA)                                                                     \
-  hipLaunchKernelGGL((sell_fill<BB, TA>), dim3(grid), dim3(256), 0, stream, ns, nrows, rowp, cols, \
-                     width, coff, A, (TA*)vs, cs)
-  if (b == 3 && prec == 64)
-    SELL_FILL(3, double);
-  else if (b == 3)
-    SELL_FILL(3, float);
-  else if (prec == 64)
-    SELL_FILL(6, double);
-  else
-    SELL_FILL(6, float);
-#undef SELL_FILL
-  return (int)hipGetLastError();
+  if (prec != 64 && prec != 32) return TACS_HIP_ERR_ARG;
+  return tacs_hip_sell_fill2(b, nrows, rowp, cols, width, coff, A,
+                             prec == 64 ? (double*)vs : NULL, prec == 32 ? (float*)vs : NULL, cs,
+                             stream);
 }
 
 int tacs_hip_sell_spmv(int b, int prec_vals, int prec_vecs, int nrows, const int* width,

@@ -532,6 +532,12 @@ int tacs_mat_mult_prec(tacs_obj m, int prec_vals, int prec_vecs, const void* x, 
     return tacs_set_error("tacs_mat_mult_prec: precisions %d/%d", prec_vals, prec_vecs);
   return last_error_code();
 }
+int tacs_mat_prepare_products(tacs_obj m, int prec) {
+  REQUIRE(TACSParallelMat, A, m);
+  if (prec != 64 && prec != 32) return tacs_set_error("%s: prec 64 or 32", __func__);
+  if ((prec == 64 ? A->sellVer64 : A->sellVer32) != A->version) A->sellUpdate(prec);
+  return last_error_code();
+}
 int tacs_mat_zero_entries(tacs_obj m) {
   REQUIRE(TACSParallelMat, A, m);
   A->zeroEntries();

@@ -498,17 +498,18 @@ void TACSParallelMat::sellUpdate(int prec) {
     d_scols = scols;
     cs_fill = d_scols;
   }
-  if (prec == 64) {
-    if (!d_s64) d_s64 = dalloc<double>(b2 * nslots);
-    KCHECK(tacs_hip_sell_fill(bsize, 64, nrows, d_rowp, d_cols, d_sw, d_soff, d_A, d_s64, cs_fill,
-                              g_stream));
-    sellVer64 = version;
-  } else {
-    if (!d_s32) d_s32 = dalloc<float>(b2 * nslots);
-    KCHECK(tacs_hip_sell_fill(bsize, 32, nrows, d_rowp, d_cols, d_sw, d_soff, d_A, d_s32, cs_fill,
-                              g_stream));
-    sellVer32 = version;
-  }
+  // the requested copy, and the other precision's copy too when it exists
+  // and is stale (the recycling solver reads both after every assembly):
+  // one pass over the BCSR values writes both
+  if (prec == 64 && !d_s64) d_s64 = dalloc<double>(b2 * nslots);
+  if (prec == 32 && !d_s32) d_s32 = dalloc<float>(b2 * nslots);
+  if (g_halt) return;
+  double* v64 = (d_s64 && sellVer64 != version) ? d_s64 : NULL;
+  float* v32 = (d_s32 && sellVer32 != version) ? d_s32 : NULL;
+  KCHECK(tacs_hip_sell_fill2(bsize, nrows, d_rowp, d_cols, d_sw, d_soff, d_A, v64, v32, cs_fill,
+                             g_stream));
+  if (v64) sellVer64 = version;
+  if (v32) sellVer32 = version;
 }
 // The products run over the sliced ELL copies (3x3 and 6x6 blocks).
 void TACSParallelMat::mult64(const double* x, double* y) {
