@@ -8,8 +8,11 @@
 
   Everything lives on one GPU: vectors and matrices are device resident;
   TACSBVec::getArray returns a host mirror that is synchronised on demand.
-  The communicator argument of the reference constructors is kept in place as
-  an opaque TACSComm (this build is serial; pass TACS_COMM_SELF).
+  The constructors take the reference's MPI_Comm (src/TACSAssembler.h:83);
+  this build runs one rank on one GPU, so a communicator of more than one
+  process is refused by TACSAssembler::initialize (the library calls no MPI
+  routine itself and does not link MPI; it asks the process's MPI, when one
+  is loaded and initialised, for the communicator's size).
 */
 #ifndef TACS_HIP_CXX_H
 #define TACS_HIP_CXX_H
@@ -21,11 +24,15 @@
 #include <string>
 #include <vector>
 
+#define MPICH_SKIP_MPICXX
+#define OMPI_SKIP_MPICXX
+#include <mpi.h>
+
 #include "tacs_hip.h"
 
 typedef double TacsScalar;
-typedef struct tacs_comm_opaque* TACSComm;
-#define TACS_COMM_SELF ((TACSComm)0)
+typedef MPI_Comm TACSComm;
+#define TACS_COMM_SELF MPI_COMM_SELF
 
 enum MatrixOrientation { TACS_MAT_NORMAL, TACS_MAT_TRANSPOSE };
 enum ElementStrainType { TACS_LINEAR_STRAIN, TACS_NONLINEAR_STRAIN };
@@ -401,6 +408,9 @@ class TACSKsm : public TACSObject {
 
 class TACSMatrixFreeMat;
 
+// GMRES::solve src/bpmat/KSM.cpp:785: right-preconditioned; with isFlexible
+// the preconditioned vectors Z_i = M^{-1} v_i are kept and the update is
+// x += Z y (FGMRES), otherwise x += M^{-1} (V y).
 class GMRES : public TACSKsm {
  public:
   GMRES(TACSMat* mat, TACSPc* pc, int m, int nrestart, int isFlexible = 0);
@@ -416,7 +426,9 @@ class GMRES : public TACSKsm {
   size_t n;
   void opMult(const double* x, double* y);
   TACSBlockJacobiPc* pc;
+  int flexible;
   double* d_W;      // (msub+1) basis vectors, contiguous
+  double* d_Z;      // msub preconditioned vectors (flexible only)
   double* d_work;
   double* d_scal;   // H, res, Qc, Qs, status scalars
   void* d_red;
@@ -616,6 +628,7 @@ class TACSAssembler : public TACSObject {
   int getNumDesignVars() { return numDesignVars; }
   TACSBcMap* getBcMap() { return bcMap; }
   void setNumThreads(int t) {}
+  TACSComm getMPIComm() { return comm; }
 
   TACSBVec* createVec();
   TACSBVec* createNodeVec();
@@ -733,6 +746,7 @@ class TACSAssembler : public TACSObject {
   int *h_rowp, *h_cols;
   long nnzb;
   double jac_ms;
+  TACSComm comm;
   int elemPrec, jacKernel;
   // element-block path: per matrix slot the list of element blocks
   // (e NPE^2 + a NPE + b) summed into it, ascending element; the blocks

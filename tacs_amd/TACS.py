@@ -160,9 +160,25 @@ class MatrixFreeMat(TACSHandle):
 
 
 class Pc(TACSHandle):
-    """Block-Jacobi preconditioner of an assembled matrix."""
+    """Preconditioner of an assembled matrix: block-Jacobi (the inverse
+    diagonal blocks, BCSRMat::factorDiag src/bpmat/BCSRMat.cpp:1527).
 
-    def __init__(self, mat):
+    The reference's TACS.Pc (tacs/TACS.pyx:1106) builds a Schur-complement
+    (TACSSchurPc) or additive-Schwarz ILU(k) factorisation from lev_fill,
+    fill and reorder_schur; those factorisations are not in this build, so
+    its arguments are accepted and a warning says that block-Jacobi runs
+    (stronger preconditioners change the iteration counts, not the
+    solutions, at a given tolerance)."""
+
+    _warned = False
+
+    def __init__(self, mat, lev_fill=None, fill=None, reorder_schur=None, **kw):
+        if not Pc._warned and (lev_fill is not None or fill is not None or kw):
+            import warnings
+
+            warnings.warn("TACS.Pc: block-Jacobi preconditioner (the reference's Schur / "
+                          "ILU(k) factorisations are not part of this build)", stacklevel=2)
+            Pc._warned = True
         self.mat = mat
         super().__init__(lib().tacs_block_jacobi_pc_create(mat.ptr), keep=(mat,))
 

@@ -18,6 +18,10 @@ FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
          # packed FP32 instructions are left to no code path of this library
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I", os.path.join(HERE, "..", "include"), "-Wno-unused-result"]
+# MPI_Comm of the C++ API: MPICH's mpi.h (the library links no MPI)
+MPI_INC = os.environ.get("TACS_MPI_INCLUDE", "/opt/conda/include")
+if os.path.exists(os.path.join(MPI_INC, "mpi.h")):
+    FLAGS += ["-idirafter", MPI_INC]
 
 
 def _compile(src):

@@ -3,6 +3,7 @@
 // src/bpmat/TACSBVec.h, src/bpmat/TACSParallelMat.h, src/bpmat/KSM.h,
 // src/functions/*.h) while all data lives on the GPU and every loop runs as a
 // kernel of tacs_amd/csrc/*.hip.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -670,7 +671,7 @@ void KSMPrintFile::print(const char* cstr) {
 
 GMRES::GMRES(TACSMat* m, TACSPc* p, int _m, int _nrestart, int isFlexible)
     : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart >= 0 ? _nrestart : 0),
-      ortho(MODIFIED_GRAM_SCHMIDT) {
+      ortho(MODIFIED_GRAM_SCHMIDT), d_Z(NULL) {
   mat = dynamic_cast<TACSParallelMat*>(m);
   mfree = mat ? NULL : dynamic_cast<TACSMatrixFreeMat*>(m);
   pc = p ? dynamic_cast<TACSBlockJacobiPc*>(p) : NULL;
@@ -678,6 +679,8 @@ GMRES::GMRES(TACSMat* m, TACSPc* p, int _m, int _nrestart, int isFlexible)
   m->incref();
   if (pc) pc->incref();
   n = mat ? (size_t)mat->nrows * mat->bsize : mfree->length();
+  flexible = (isFlexible && pc) ? 1 : 0;
+  if (flexible) d_Z = dalloc<double>((size_t)msub * n);
   d_W = dalloc<double>((size_t)(msub + 1) * n);
   d_work = dalloc<double>(n);
   GmresScal L(msub);
@@ -689,6 +692,7 @@ GMRES::GMRES(TACSMat* m, TACSPc* p, int _m, int _nrestart, int isFlexible)
   HCHECK(hipHostMalloc((void**)&h_pin, 64 * sizeof(double), 0));
 }
 GMRES::~GMRES() {
+  dfree(d_Z);
   dfree(d_W);
   dfree(d_work);
   dfree(d_scal);
@@ -788,8 +792,10 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
       if (g_halt) return 0;  // a copy or launch failed: no further work
       double* q = W(i + 1);
       if (pc) {
-        KCHECK(tacs_hip_bsr_diag_apply(bs, nr, pc->d_Dinv, W(i), d_work, g_stream));
-        opMult(d_work, q);
+        // flexible: keep z_i = M^{-1} v_i for the update (FGMRES)
+        double* z = flexible ? d_Z + (size_t)i * n : d_work;
+        KCHECK(tacs_hip_bsr_diag_apply(bs, nr, pc->d_Dinv, W(i), z, g_stream));
+        opMult(z, q);
       } else {
         opMult(W(i), q);
       }
@@ -819,7 +825,9 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
     iterCount += niters;
     if (niters > 0) {
       KCHECK(tacs_hip_backsolve(niters, msub, H, res, g_stream));
-      if (pc) {
+      if (pc && flexible) {
+        KCHECK(tacs_hip_lincomb(64, 64, n, niters, d_Z, n, res, xd, 1, g_stream));
+      } else if (pc) {
         KCHECK(tacs_hip_lincomb(64, 64, n, niters, d_W, n, res, d_work, 0, g_stream));
         KCHECK(tacs_hip_bsr_diag_apply(bs, nr, pc->d_Dinv, d_work, W(0), g_stream));
         KCHECK(tacs_hip_axpby(64, 64, n, 1.0, W(0), 1.0, xd, g_stream));
@@ -2184,7 +2192,21 @@ TacsScalar TACSKSFailure::getFunctionValue() {
 // ---------------------------------------------------------------------------
 // TACSAssembler
 // ---------------------------------------------------------------------------
-TACSAssembler::TACSAssembler(TACSComm comm, int vpn, int nowned, int nelems, int ndep)
+// Number of processes of comm, asked of the MPI the process has loaded and
+// initialised (the driver's); 1 when there is none.  The library links no
+// MPI and calls nothing else of it.
+static int comm_size(TACSComm c) {
+  typedef int (*initialized_fn)(int*);
+  typedef int (*size_fn)(MPI_Comm, int*);
+  initialized_fn mi = (initialized_fn)dlsym(RTLD_DEFAULT, "MPI_Initialized");
+  size_fn ms = (size_fn)dlsym(RTLD_DEFAULT, "MPI_Comm_size");
+  int flag = 0, n = 1;
+  if (!mi || !ms || mi(&flag) != 0 || !flag) return 1;
+  if (ms(c, &n) != 0) return 1;
+  return n;
+}
+
+TACSAssembler::TACSAssembler(TACSComm _comm, int vpn, int nowned, int nelems, int ndep)
     : varsPerNode(vpn), numNodes(nowned), numElements(nelems), npe(0), order(0),
       initialized(0), numDesignVars(0), h_ptr(NULL), h_conn(NULL), elements(NULL),
       auxElements(NULL), aux_version(-1), naux(0), naux_pressure(0), nnzb(0), jac_ms(0.0) {
@@ -2209,6 +2231,7 @@ TACSAssembler::TACSAssembler(TACSComm comm, int vpn, int nowned, int nelems, int
   d_red = NULL;
   d_scalar = NULL;
   h_rowp = h_cols = NULL;
+  comm = _comm;
   elemPrec = 64;
   jacKernel = 0;
   d_cptr = d_clist = NULL;
@@ -2373,6 +2396,11 @@ void TACSAssembler::setAuxElements(TACSAuxElements* aux) {
 
 int TACSAssembler::initialize() {
   if (initialized) return 0;
+  const int nranks = comm_size(comm);
+  if (nranks > 1)
+    return tacs_set_error("initialize: the communicator has %d processes; this build runs one "
+                          "rank on one GPU",
+                          nranks);
   if (!h_ptr || !h_conn) return tacs_set_error("initialize: connectivity not set");
   if (!elements) return tacs_set_error("initialize: elements not set");
   std::vector<int> dvnum(numElements);
