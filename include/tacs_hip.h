@@ -669,6 +669,10 @@ tacs_obj tacs_ksm_print_file_create(const char* filename, const char* descript, 
                                     int freq);
 int tacs_ksm_set_monitor(tacs_obj ksm, tacs_obj monitor);
 int tacs_gmres_set_ortho_type(tacs_obj ksm, int ortho);
+/* GMRES::setCycleGraph: each restart cycle's Arnoldi steps as one captured
+   HIP graph (assembled operator; same iterates as the step-by-step loop).  No
+   counterpart in the reference. */
+int tacs_gmres_set_cycle_graph(tacs_obj ksm, int on);
 /* Mixed-precision restarted GMRES(m) (FP32 Arnoldi cycles, FP64 restart
    residual and solution); no counterpart in the reference. */
 tacs_obj tacs_mixed_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart);

@@ -202,6 +202,12 @@ class KSM(TACSHandle):
     def setTolerances(self, rtol, atol):
         lib().tacs_ksm_set_tolerances(self.ptr, rtol, atol)
 
+    def setCycleGraph(self, flag=True):
+        """Run each restart cycle's Arnoldi steps as one captured HIP graph
+        (assembled operator): no host round trip per step, the same iterates
+        (the first step that meets the tolerance is taken after the cycle)."""
+        check(lib().tacs_gmres_set_cycle_graph(self.ptr, int(bool(flag))), "KSM.setCycleGraph")
+
     def setOrthoType(self, ortho):
         check(lib().tacs_gmres_set_ortho_type(self.ptr, ortho), "KSM.setOrthoType")
 

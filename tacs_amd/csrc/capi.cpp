@@ -632,6 +632,11 @@ tacs_obj tacs_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int f
   }
   return built(new GMRES(A ? (TACSMat*)A : (TACSMat*)F, p, m, nrestart, flexible));
 }
+int tacs_gmres_set_cycle_graph(tacs_obj ksm, int on) {
+  REQUIRE(GMRES, g, ksm);
+  g->setCycleGraph(on);
+  return 0;
+}
 int tacs_gmres_set_ortho_type(tacs_obj ksm, int ortho) {
   REQUIRE(GMRES, g, ksm);
   g->setOrthoType((enum OrthoType)ortho);

@@ -671,7 +671,8 @@ void KSMPrintFile::print(const char* cstr) {
 
 GMRES::GMRES(TACSMat* m, TACSPc* p, int _m, int _nrestart, int isFlexible)
     : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart >= 0 ? _nrestart : 0),
-      ortho(MODIFIED_GRAM_SCHMIDT), d_Z(NULL) {
+      ortho(MODIFIED_GRAM_SCHMIDT), useGraph(1), gstream(NULL), gexec(NULL), gexec_ortho(-1),
+      d_hist(NULL), h_hist(NULL), d_Z(NULL) {
   mat = dynamic_cast<TACSParallelMat*>(m);
   mfree = mat ? NULL : dynamic_cast<TACSMatrixFreeMat*>(m);
   pc = p ? dynamic_cast<TACSBlockJacobiPc*>(p) : NULL;
@@ -692,6 +693,10 @@ GMRES::GMRES(TACSMat* m, TACSPc* p, int _m, int _nrestart, int isFlexible)
   HCHECK(hipHostMalloc((void**)&h_pin, 64 * sizeof(double), 0));
 }
 GMRES::~GMRES() {
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (gstream) (void)hipStreamDestroy(gstream);
+  dfree(d_hist);
+  hfree(h_hist);
   dfree(d_Z);
   dfree(d_W);
   dfree(d_work);
@@ -715,6 +720,86 @@ void GMRES::setTolerances(double r, double a) {
   atol = a;
 }
 void GMRES::setOrthoType(enum OrthoType o) { ortho = o; }
+void GMRES::setCycleGraph(int on) { useGraph = on ? 1 : 0; }
+
+static void cgs2(int prec, size_t n, int k, const void* W, void* q, double* h, double* tmp,
+                 void* red);
+
+// One Arnoldi step of GMRES::solve (src/bpmat/KSM.cpp:850-900): w = A M^{-1}
+// v_i, Gram-Schmidt against v_0..v_i, h_{i+1,i} = |w|, v_{i+1} = w / |w|, the
+// Givens update of column i; |residual estimate| goes to st.
+void GMRES::arnoldiStep(int i, double* st) {
+  GmresScal L(msub);
+  double* H = d_scal + L.H;
+  double* res = d_scal + L.res;
+  double* Qc = d_scal + L.Qc;
+  double* Qs = d_scal + L.Qs;
+  double* misc = d_scal + L.misc;
+  double* tmp = d_scal + L.tmp;
+  const int bs = pc ? pc->mat->bsize : 0, nr = pc ? pc->mat->nrows : 0;
+  auto W = [&](int k) { return d_W + (size_t)k * n; };
+  double* q = W(i + 1);
+  if (pc) {
+    // flexible: keep z_i = M^{-1} v_i for the update (FGMRES)
+    double* z = flexible ? d_Z + (size_t)i * n : d_work;
+    KCHECK(tacs_hip_bsr_diag_apply(bs, nr, pc->d_Dinv, W(i), z, g_stream));
+    opMult(z, q);
+  } else {
+    opMult(W(i), q);
+  }
+  double* h = H + (size_t)i * (msub + 1);
+  if (ortho == MODIFIED_GRAM_SCHMIDT) {
+    // h_j = W_j . q, q -= h_j W_j for j = 0..i (ModifiedGramSchmidt
+    // KSM.cpp:526); each subtraction is fused into the next dot.
+    KCHECK(tacs_hip_dot(64, n, NULL, NULL, q, W(0), d_red, &h[0], 0, NULL, g_stream));
+    for (int j = 1; j <= i; j++)
+      KCHECK(tacs_hip_dot(64, n, W(j - 1), &h[j - 1], q, W(j), d_red, &h[j], 0, NULL, g_stream));
+    KCHECK(tacs_hip_dot(64, n, W(i), &h[i], q, q, d_red, &h[i + 1], 1, misc + MISC_INV, g_stream));
+  } else {
+    // classical Gram-Schmidt, twice (ClassicalGramSchmidt KSM.cpp:515)
+    cgs2(64, n, i + 1, d_W, q, h, tmp, d_red);
+    KCHECK(tacs_hip_dot(64, n, NULL, NULL, q, q, d_red, &h[i + 1], 1, misc + MISC_INV, g_stream));
+  }
+  KCHECK(tacs_hip_scale(64, n, q, misc + MISC_INV, 0.0, 0, g_stream));
+  KCHECK(tacs_hip_givens(i, msub, H, Qc, Qs, res, st, g_stream));
+}
+
+// Captures the msub Arnoldi steps of a cycle as one graph on the solver's own
+// stream (the library stream is switched to it for the capture); step i's
+// estimate goes to d_hist[i].  The graph reads the matrix's sliced ELL copy,
+// the block-Jacobi inverse and the solver's own arrays, whose addresses stay
+// fixed for the solver's life.  Returns 0 on success.
+int GMRES::captureCycle() {
+  if (!gstream) HCHECK(hipStreamCreateWithFlags(&gstream, hipStreamNonBlocking));
+  if (!d_hist) d_hist = dalloc<double>(msub);
+  if (!h_hist) HCHECK(hipHostMalloc((void**)&h_hist, msub * sizeof(double), 0));
+  if (g_halt) return 1;
+  if (gexec) {
+    (void)hipGraphExecDestroy(gexec);
+    gexec = NULL;
+  }
+  HCHECK(hipStreamSynchronize(g_stream));
+  hipStream_t saved = g_stream;
+  g_stream = gstream;
+  hipGraph_t graph = NULL;
+  hipError_t e = hipStreamBeginCapture(gstream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (int i = 0; i < msub; i++) arnoldiStep(i, d_hist + i);
+    e = hipStreamEndCapture(gstream, &graph);
+  }
+  g_stream = saved;
+  if (e == hipSuccess && !g_halt) e = hipGraphInstantiate(&gexec, graph, NULL, NULL, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (e != hipSuccess || g_halt) {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = NULL;
+    tacs_set_error("GMRES: capture of the Arnoldi cycle failed: %s", hipGetErrorString(e));
+    g_halt = 1;
+    return 1;
+  }
+  gexec_ortho = ortho;
+  return 0;
+}
 
 
 // Classical Gram-Schmidt, twice, of q against the k vectors of W (stride n),
@@ -762,6 +847,13 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
   double rhs_norm = 0.0;
   int solve_flag = 0;
   iterCount = 0;
+  // the captured cycle: assembled operator only, product layout current
+  // before the capture and before every launch
+  const bool graph = useGraph && mat != NULL;
+  if (graph) {
+    if (mat->sellVer64 != mat->version) mat->sellUpdate(64);
+    if ((!gexec || gexec_ortho != ortho) && captureCycle() != 0) return 0;
+  }
 
   for (int count = 0; count < nrestart + 1; count++) {
     if (g_halt) return 0;  // a copy or launch failed: no further work
@@ -788,38 +880,31 @@ int GMRES::solve(TACSVec* tb, TACSVec* tx, int zero_guess) {
       solve_flag = 1;
       break;
     }
-    for (int i = 0; i < msub; i++) {
-      if (g_halt) return 0;  // a copy or launch failed: no further work
-      double* q = W(i + 1);
-      if (pc) {
-        // flexible: keep z_i = M^{-1} v_i for the update (FGMRES)
-        double* z = flexible ? d_Z + (size_t)i * n : d_work;
-        KCHECK(tacs_hip_bsr_diag_apply(bs, nr, pc->d_Dinv, W(i), z, g_stream));
-        opMult(z, q);
-      } else {
-        opMult(W(i), q);
+    if (graph) {
+      // the whole cycle as one graph launch, then the first step that stops
+      HCHECK(hipGraphLaunch(gexec, g_stream));
+      d2h(h_hist, d_hist, msub);
+      if (g_halt) return 0;
+      for (int i = 0; i < msub; i++) {
+        niters++;
+        resNorm = h_hist[i];
+        if (monitor) monitor->printResidual(i + 1, resNorm);
+        if (resNorm < atol || resNorm < rtol * rhs_norm || !std::isfinite(resNorm)) {
+          solve_flag = std::isfinite(resNorm) ? 1 : 0;
+          break;
+        }
       }
-      double* h = H + (size_t)i * (msub + 1);
-      if (ortho == MODIFIED_GRAM_SCHMIDT) {
-        // h_j = W_j . q, q -= h_j W_j for j = 0..i (ModifiedGramSchmidt
-        // KSM.cpp:526); each subtraction is fused into the next dot.
-        KCHECK(tacs_hip_dot(64, n, NULL, NULL, q, W(0), d_red, &h[0], 0, NULL, g_stream));
-        for (int j = 1; j <= i; j++)
-          KCHECK(tacs_hip_dot(64, n, W(j - 1), &h[j - 1], q, W(j), d_red, &h[j], 0, NULL, g_stream));
-        KCHECK(tacs_hip_dot(64, n, W(i), &h[i], q, q, d_red, &h[i + 1], 1, misc + MISC_INV, g_stream));
-      } else {
-        // classical Gram-Schmidt, twice (ClassicalGramSchmidt KSM.cpp:515)
-        cgs2(64, n, i + 1, d_W, q, h, tmp, d_red);
-        KCHECK(tacs_hip_dot(64, n, NULL, NULL, q, q, d_red, &h[i + 1], 1, misc + MISC_INV, g_stream));
-      }
-      KCHECK(tacs_hip_scale(64, n, q, misc + MISC_INV, 0.0, 0, g_stream));
-      KCHECK(tacs_hip_givens(i, msub, H, Qc, Qs, res, misc + MISC_ST, g_stream));
-      niters++;
-      resNorm = pin_read(h_pin, misc + MISC_ST);
-      if (monitor) monitor->printResidual(i + 1, resNorm);
-      if (resNorm < atol || resNorm < rtol * rhs_norm || !std::isfinite(resNorm)) {
-        solve_flag = std::isfinite(resNorm) ? 1 : 0;
-        break;
+    } else {
+      for (int i = 0; i < msub; i++) {
+        if (g_halt) return 0;  // a copy or launch failed: no further work
+        arnoldiStep(i, misc + MISC_ST);
+        niters++;
+        resNorm = pin_read(h_pin, misc + MISC_ST);
+        if (monitor) monitor->printResidual(i + 1, resNorm);
+        if (resNorm < atol || resNorm < rtol * rhs_norm || !std::isfinite(resNorm)) {
+          solve_flag = std::isfinite(resNorm) ? 1 : 0;
+          break;
+        }
       }
     }
     iterCount += niters;
