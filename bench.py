@@ -1097,6 +1097,51 @@ def matfree_main(args, L, ev):
     print(json.dumps(out))
 
 
+def gmres_graph_main(args, L, ev):
+    """FP64 GMRES(30) + MGS + block-Jacobi (the reference's solver) on the
+    n^3 hex8 cubes n = 8, 12, 20, 40, step by step and with each restart
+    cycle replayed as one captured graph: time per Arnoldi step.  Prints its
+    own JSON line."""
+    from tacs_amd import TACS
+    from tacs_amd.problems import SolidProblem
+    rows = {}
+    for n in (8, 12, 20, 40):
+        p = SolidProblem(n)
+        p.assemble()
+        pc = TACS.Pc(p.mat)
+        pc.factor()
+        x = p.assembler.createVec()
+        for graph in (False, True):
+            ksm = TACS.KSM(p.mat, pc, 30, 100000)
+            ksm.setTolerances(args.rtol, 1e-30)
+            ksm.setCycleGraph(graph)
+            ksm.solve(p.res, x)
+            L.tacs_device_synchronize()
+            t0 = time.perf_counter()
+            reps = 3
+            for _ in range(reps):
+                ksm.solve(p.res, x)
+            L.tacs_device_synchronize()
+            dt = (time.perf_counter() - t0) / reps
+            its = ksm.getIterCount()
+            rows[f"{n}^3 {'graph' if graph else 'step_by_step'}"] = {
+                "solve_s": dt, "iterations": its, "us_per_step": 1e6 * dt / its,
+                "dof": 3 * p.mesh.nnodes}
+            log(f"{n}^3 graph={graph}: {dt:.4f} s, {its} steps")
+    g20 = rows["20^3 graph"]["us_per_step"]
+    s20 = rows["20^3 step_by_step"]["us_per_step"]
+    out = {"metric": "FP64 GMRES(30) time per Arnoldi step, captured cycles vs step by step",
+           "value": g20, "unit": "us per step (20^3 cube, captured)", "n_gpus": 1,
+           "steps": 3, "warmup": 1, "ms_per_step": g20 / 1e3, "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic structured meshes (no external data)",
+           "config": {"workload": f"hex8 cubes 8^3..40^3, E=70e9 nu=0.3, x=0 clamped, traction "
+                                  f"1e6 Pa on x=1; FP64 GMRES(30)+MGS+block-Jacobi to "
+                                  f"{args.rtol:g}"},
+           "speedup_20cube": s20 / g20, "rows": rows}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -1117,7 +1162,8 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="assemble and time the SpMV kernels only (for profiler runs)")
     ap.add_argument("--workload", default="hex",
-                    choices=["hex", "shell", "batched", "multi", "simp", "matfree"],
+                    choices=["hex", "shell", "batched", "multi", "simp", "matfree",
+                             "gmres_graph"],
                     help="hex: the 128^3 headline line; shell: the Quad4 plate; batched: the "
                          "hex27 cube with a block of load cases (own JSON lines)")
     ap.add_argument("--nrhs", type=int, default=16, help="load cases of the batched workload")
@@ -1161,6 +1207,9 @@ def main():
         return
     if args.workload == "matfree":
         matfree_main(args, L, ev)
+        return
+    if args.workload == "gmres_graph":
+        gmres_graph_main(args, L, ev)
         return
     t0 = time.perf_counter()
     p = SolidProblem(args.n, dv="element")

@@ -363,3 +363,58 @@ void orc_solid_jacobian_loop(int order, int nelem, const double *Xe, const doubl
     orc_solid_element(order, &Xe[(size_t)3 * npe * e], u, C21, t, 0.0, 1.0, 0.0, res, mat);
   }
 }
+
+void orc_quad4_element(const double X[12], const double v[24], double E, double nu, double t,
+                       double alpha, double *res, double *mat);
+
+/* The oracle's Quad4 shell element Jacobian over nelem elements (12
+   coordinates each), for the speed ratio against the reference's loop. */
+void orc_quad4_jacobian_loop(int nelem, const double *X, double E, double nu, double t,
+                             double *res, double *mat) {
+  double v[24] = {0};
+  for (int e = 0; e < nelem; e++) {
+    memset(res, 0, 24 * sizeof(double));
+    memset(mat, 0, 576 * sizeof(double));
+    orc_quad4_element(&X[(size_t)12 * e], v, E, nu, t, 1.0, res, mat);
+  }
+}
+
+/* orc_assemble_shell (TACSAssembler::assembleJacobian's element loop over
+   TACSQuad4Shell, 6x6 blocks) over the colours, elements of a colour spread
+   over the threads; A and res accumulated. */
+long orc_par_assemble_shell(int ncolors, const int *color_ptr, const int *color_elems,
+                            const int *conn, const double *X, const double *u,
+                            const double *t_elem, double E, double nu, const int *rowp,
+                            const int *cols, double *A, double *res) {
+  long missing = 0;
+  for (int c = 0; c < ncolors; c++) {
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : missing)
+    for (int k = color_ptr[c]; k < color_ptr[c + 1]; k++) {
+      const int e = color_elems[k];
+      const int *cn = &conn[4 * (size_t)e];
+      double Xe[12], ue[24], re[24] = {0}, ke[576];
+      for (int a = 0; a < 4; a++) {
+        for (int d = 0; d < 3; d++) Xe[3 * a + d] = X[3 * (size_t)cn[a] + d];
+        for (int d = 0; d < 6; d++) ue[6 * a + d] = u ? u[6 * (size_t)cn[a] + d] : 0.0;
+      }
+      if (A) memset(ke, 0, sizeof(ke));
+      orc_quad4_element(Xe, ue, E, nu, t_elem[e], 1.0, res ? re : NULL, A ? ke : NULL);
+      if (res)
+        for (int a = 0; a < 4; a++)
+          for (int d = 0; d < 6; d++) res[6 * (size_t)cn[a] + d] += re[6 * a + d];
+      if (A)
+        for (int a = 0; a < 4; a++)
+          for (int b = 0; b < 4; b++) {
+            long s = find_slot_par(rowp, cols, cn[a], cn[b]);
+            if (s < 0) {
+              missing++;
+              continue;
+            }
+            double *blk = &A[36 * s];
+            for (int r = 0; r < 6; r++)
+              for (int q = 0; q < 6; q++) blk[6 * r + q] += ke[(6 * a + r) * 24 + 6 * b + q];
+          }
+    }
+  }
+  return missing;
+}

@@ -42,6 +42,7 @@ def lib():
         L.orc_shell_compliance.restype = ctypes.c_double
         L.orc_assemble_shell.restype = ctypes.c_long
         L.orc_par_assemble_solid.restype = ctypes.c_long
+        L.orc_par_assemble_shell.restype = ctypes.c_long
         L.orc_par_gmres.restype = ctypes.c_int
         L.orc_color_elements.restype = ctypes.c_int
         L.orc_par_threads.restype = ctypes.c_int
@@ -516,3 +517,18 @@ def par_gmres(b, rowp, cols, A, rhs, Dinv=None, m=30, nrestart=10, rtol=1e-8,
                                ctypes.c_double(atol), 1 if x0 is None else 0, max_its,
                                ctypes.byref(iters), ctypes.byref(rn))
     return x, flag, iters.value, rn.value
+
+
+def par_assemble_shell(colors, conn, X, u, t_elem, E, nu, rowp, cols, want_mat=True,
+                       want_res=True):
+    cptr, celems = colors
+    nnodes = rowp.size - 1
+    A = np.zeros(cols.size * 36) if want_mat else None
+    res = np.zeros(6 * nnodes) if want_res else None
+    missing = lib().orc_par_assemble_shell(
+        cptr.size - 1, _p(_ii(cptr), np.int32), _p(_ii(celems), np.int32),
+        _p(_ii(conn).reshape(-1), np.int32), _p(_dd(X)), _p(_dd(u)) if u is not None else None,
+        _p(_dd(t_elem)), ctypes.c_double(E), ctypes.c_double(nu), _p(_ii(rowp), np.int32),
+        _p(_ii(cols), np.int32), _p(A), _p(res))
+    assert missing == 0
+    return A, res

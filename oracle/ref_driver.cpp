@@ -280,4 +280,18 @@ void ref_solid_jacobian_loop(int order, int nelem, const double* Xe, double E, d
   delete[] zero;
   elem->decref();
 }
+// Jacobians of nelem TACSQuad4Shell elements through one element object
+// (X: 12 coordinates per element); mat receives the last element's matrix.
+void ref_quad4_jacobian_loop(int nelem, const double* X, double E, double nu, double t,
+                             double* res, double* mat) {
+  TACSElement* elem = make_quad4(E, nu, 270e6, t);
+  elem->incref();
+  double zero[24] = {0};
+  for (int e = 0; e < nelem; e++) {
+    for (int i = 0; i < 24; i++) res[i] = 0.0;
+    for (int i = 0; i < 576; i++) mat[i] = 0.0;
+    elem->addJacobian(0, 0.0, 1.0, 0.0, 0.0, &X[(size_t)12 * e], zero, zero, zero, res, mat);
+  }
+  elem->decref();
+}
 }  // extern "C"
