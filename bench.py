@@ -735,6 +735,25 @@ def batched_main(args, L, ev):
     p = SolidProblem(n, order=3, dv="element")
     log(f"setup {time.perf_counter() - t0:.1f}s: {p.mesh.nelem} hex27, {3 * p.mesh.nnodes} DOF")
     p.assemble()
+    if args.kernels_only:
+        # the hex27 Jacobian kernels in isolation (profiler runs): both solid
+        # Jacobian forms, three assemblies each, FP64 element arithmetic
+        a = p.assembler
+        kj = {}
+        for kern in (1, 2):
+            a.setJacobianKernel(kern)
+            a.assembleJacobian(1.0, 0.0, 0.0, None, p.mat)
+            e0, e1 = ev.event(), ev.event()
+            ev.record(e0)
+            for _ in range(3):
+                a.assembleJacobian(1.0, 0.0, 0.0, None, p.mat)
+            ev.record(e1)
+            kj["rows" if kern == 1 else "element_blocks"] = ev.elapsed_ms(e0, e1) / 3
+        a.setJacobianKernel(0)
+        kj.update(elements=p.mesh.nelem, dof=3 * p.mesh.nnodes, nnz_blocks=p.mat.getInfo()[2],
+                  flop_per_element=2283228, unit="ms per assembleJacobian (no residual)")
+        print(json.dumps(kj))
+        return
     bs = batched_loads(p, R)
     xs = [p.assembler.createVec() for _ in range(R)]
     pc = TACS.Pc(p.mat)
@@ -1094,7 +1113,37 @@ def matfree_main(args, L, ev):
                    gmres_speedup=res["assembled"][0] / res["matfree"][0],
                    u_rel_diff=float(np.linalg.norm(uf - ua) / np.linalg.norm(ua)),
                    gmres=f"FP64 GMRES({args.m_fp64}) + MGS + block-Jacobi to {args.rtol:g}")
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_matfree(p)
     print(json.dumps(out))
+
+
+def cpu_matfree(p, reps=20):
+    """Measured multi-core CPU baseline of one operator product on the same
+    mesh: the oracle's threaded 3x3 block-CSR product (oracle/cpu_baseline.c,
+    the reference's BCSRMatVecMult3 loop over rows) on the assembled
+    matrix, all host cores, best of reps products.  The reference has no
+    element-level matrix-free product for solids beyond the same per-element
+    loop the assembly runs, so the assembled product is its fastest form."""
+    O = _oracle()
+    model, threads = cpu_env()
+    m = p.mesh
+    rowp, cols = O.node_csr(m.nnodes, m.conn, m.npe)
+    colors = O.color_elements(m.nnodes, m.conn, m.npe)
+    A, _ = O.par_assemble_solid(m.order, colors, m.conn, m.X.reshape(-1), None,
+                                np.ones(m.nelem), O.iso_C21(70e9, 0.3), rowp, cols,
+                                want_res=False)
+    x = np.random.default_rng(3).standard_normal(3 * m.nnodes)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.par_bsr_mult(3, rowp, cols, A, x)
+        ts.append(time.perf_counter() - t0)
+    t = min(ts)
+    return {"cores": threads, "kind": "port", "cpu_model": model,
+            "sample": (f"oracle OpenMP 3x3 block-CSR product on {threads} threads of {model}, "
+                       f"the same assembled matrix ({cols.size} blocks), best of {reps}"),
+            "value": 1e3 * t, "unit": "ms per product"}
 
 
 def gmres_graph_main(args, L, ev):

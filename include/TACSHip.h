@@ -332,6 +332,19 @@ class TACSParallelMat : public TACSMat {
   int* h_rowp;
   int* h_cols;
   double* h_A;
+  // launch variant of the sliced ELL product per kind (0: FP64, 1: FP32
+  // values with FP64 vectors, 2: FP32), -1 for the tuning table's; every
+  // variant gives the same sums.  autotune() times each variant of each kind
+  // on this matrix (reps products each) and keeps the fastest; ms receives
+  // the average times [kind][variant] (room for 3 x maxv); returns the
+  // number of variants per kind.  No counterpart in the reference.
+  int sellVar[3];
+  int autotune(int reps, double* ms, int maxv);
+  // times every launch variant of the nrhs right-hand-side block product
+  // (3x3 blocks; prec 64: FP64 values, 32: FP32 values) on this matrix and
+  // enters the fastest in the tuning table for this size class (the batched
+  // solvers launch the table's variant); returns the number of variants.
+  int autotuneSpMM(int nrhs, int prec, int reps, double* ms, int maxv);
 };
 
 // src/bpmat/KSM.h:147 (TACSPc)
@@ -767,11 +780,20 @@ class TACSAssembler : public TACSObject {
   double jac_ms;
   TACSComm comm;
   int elemPrec, jacKernel;
-  // element-block path: per matrix slot the list of element blocks
-  // (e NPE^2 + a NPE + b) summed into it, ascending element; the blocks
-  int *d_cptr, *d_clist;
+  // element-block path: every element's npe x npe 3x3 blocks, gathered per
+  // row through the node -> element lists
   void* d_keb;
-  int buildBlockLists();
+  // Jacobian launch choice for the isotropic solid family: rows-kernel
+  // variants 0 .. nv-1 (nv = tacs_hip_solid_jacobian_nvariants(order)), nv
+  // for the element-block form; -1 for the tuning table's.  autotune()
+  // times every rows variant and (Jacobian kernel setting 0) the element-
+  // block form on mat (reps assemblies each), keeps the fastest and returns
+  // the number of choices timed (ms: their average times).  Rows variants
+  // give identical values; the element-block form agrees to rounding.  No
+  // counterpart in the reference.
+  int jacVar;
+  int autotune(TACSParallelMat* mat, int reps, double* ms, int maxv);
+  int jacobianVariant();
 };
 
 // src/TACSMatrixFreeMat.h: the Jacobian (or stiffness / mass matrix) of the

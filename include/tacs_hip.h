@@ -78,6 +78,19 @@ int tacs_hip_solid_jacobian(int order, int nnodes, int nelem, const int* n2e_ptr
                             const double* telem, const SolidMaterial* mat, double alpha,
                             double gamma, const int* rowp, const int* cols, const int* bcvars,
                             double* G, double* A, float* A32, int* status, hipStream_t stream);
+/* Launch variants of the rows kernel above for the isotropic stiffness /
+   mass (wavefronts per workgroup, elements per geometry chunk), all with the
+   same sums: 8-node elements 0 (4, 8), 1 (2, 8), 2 (8, 4), 3 (4, 4),
+   4 (4, 16); 27-node elements 0 (2, 8), 1 (1, 8), 2 (2, 4), 3 (1, 4).
+   tacs_hip_solid_jacobian launches the variant of the tuning table. */
+int tacs_hip_solid_jacobian_nvariants(int order);
+int tacs_hip_solid_jacobian_variant(int variant, int order, int nnodes, int nelem,
+                                    const int* n2e_ptr, const int* n2e, const int* conn,
+                                    const double* X, const double* telem,
+                                    const SolidMaterial* mat, double alpha, double gamma,
+                                    const int* rowp, const int* cols, const int* bcvars,
+                                    double* G, double* A, float* A32, int* status,
+                                    hipStream_t stream);
 
 /* Element-centric Jacobian of the isotropic solid family, the default path
    of TACSAssembler::assembleJacobian (same reference loop as
@@ -85,15 +98,20 @@ int tacs_hip_solid_jacobian(int order, int nnodes, int nelem, const int* n2e_ptr
    element's 3x3 blocks K_ab = alpha K + gamma M (local nodes a, b; blocks
    row-major) to ke[e][a*npe + b][9], computed in FP64 (prec 64, ke double)
    or FP32 (prec 32, ke float) arithmetic; tacs_hip_gather_solid_blocks sums
-   them into the block-CSR values in FP64 (A and/or its FP32 copy A32), slot
-   s taking ke[clist[c]] for c in [cptr[s], cptr[s+1]) in that order, with
-   the Dirichlet rows of bcvars applied.  order 2 or 3; isotropic material. */
+   them into the block-CSR values in FP64 (A and/or its FP32 copy A32): row
+   (node) i adds, over its incident elements in the order of the node ->
+   element list n2e[n2e_ptr[i] .. n2e_ptr[i+1]) (entries (e << 5) | a,
+   ascending e), block b of the element's block row a into the slot of column
+   conn[e][b], with the Dirichlet rows of bcvars applied.  order 2 or 3;
+   isotropic material; status receives TACS_HIP_ERR_ROW_TOO_LONG (more than
+   64 / 125 column blocks) or TACS_HIP_ERR_MISSING_ENTRY. */
 int tacs_hip_solid_elem_blocks(int order, int prec, int nelem, const int* conn, const double* X,
                                const double* telem, const SolidMaterial* mat, double alpha,
                                double gamma, void* ke, hipStream_t stream);
-int tacs_hip_gather_solid_blocks(int prec, int nrows, const int* rowp, const int* cols,
-                                 const int* cptr, const int* clist, const void* ke,
-                                 const int* bcvars, double* A, float* A32, hipStream_t stream);
+int tacs_hip_gather_solid_blocks(int order, int prec, int nrows, const int* rowp,
+                                 const int* cols, const int* n2e_ptr, const int* n2e,
+                                 const int* conn, const void* ke, const int* bcvars, double* A,
+                                 float* A32, int* status, hipStream_t stream);
 
 /* Element-matrix combinations of the solid family straight into block-CSR:
    kc K + mc M + gc G(u) with K the stiffness, M the consistent mass and G
@@ -299,6 +317,37 @@ int tacs_hip_sell_fill2(int b, int nrows, const int* rowp, const int* cols, cons
 int tacs_hip_sell_spmv(int b, int prec_vals, int prec_vecs, int nrows, const int* width,
                        const long* coff, const int* cs, const void* vs, const void* x, void* y,
                        hipStream_t stream);
+/* Launch variants of the sliced ELL product, all with the same sums (slots
+   per unrolled pass U, non-temporal value loads on or off): 3x3 blocks U in
+   {1, 2, 4, 8} with (variants 0-3) or without (4-7) non-temporal loads, 6x6
+   blocks U in {1, 2} with (0, 1) or without (2, 3).  tacs_hip_sell_spmv
+   launches the variant of the tuning table. */
+int tacs_hip_sell_spmv_nvariants(int b);
+int tacs_hip_sell_spmv_variant(int variant, int b, int prec_vals, int prec_vecs, int nrows,
+                               const int* width, const long* coff, const int* cs, const void* vs,
+                               const void* x, void* y, hipStream_t stream);
+
+/* Tuning table (no counterpart in the reference beyond the block-size choice
+   of BCSRMat::initBlockImpl, src/bpmat/BCSRMat.cpp:1028): (kernel, shape,
+   size class) -> launch variant, -1 for the built-in default.  Every variant
+   of a kernel gives the same results; tables are filled by measurement
+   (tacs_amd/autotune.py, or the per-object tacs_mat_autotune /
+   tacs_assembler_autotune / tacs_batched_autotune_spmm).
+   kernel TACS_TUNE_SELL_SPMV: shape 10 b + kind (kind 0: FP64 values and
+     vectors, 1: FP32 values with FP64 vectors, 2: FP32 values and vectors);
+   kernel TACS_TUNE_SPMM (tacs_hip_bsr3_spmm*): shape R (FP64 values) or
+     100 + R (FP32 values);
+   kernel TACS_TUNE_JACOBIAN (solid Jacobian rows kernel): shape = order.
+   Size class from the number of block rows: 0 below 2^15, 1 below 2^19,
+   2 from 2^19.  Returns 0 or TACS_HIP_ERR_ARG. */
+#define TACS_TUNE_SELL_SPMV 0
+#define TACS_TUNE_SPMM 1
+#define TACS_TUNE_JACOBIAN 2
+int tacs_tune_set(int kernel, int shape, int size_class, int variant);
+int tacs_tune_get(int kernel, int shape, int size_class);
+int tacs_tune_size_class(long rows);
+int tacs_tune_variant(int kernel, int shape, long rows);
+void tacs_tune_reset(void);
 /* Diagonal block positions; inverse diagonal blocks (BCSRMat::factorDiag
    src/bpmat/BCSRMat.cpp:1527); y = D^{-1} x. */
 int tacs_hip_bsr_find_diag(int nrows, const int* rowp, const int* cols, int* diag,
@@ -451,6 +500,15 @@ int tacs_hip_bdiag_apply(int b, int prec, int nrows, int R, const void* Dinv, co
                          double* y, hipStream_t stream);
 int tacs_hip_bsr3_spmm_mixed(int nrows, int nrhs, const int* rowp, const int* cols, const float* A,
                              const double* X, double* Y, hipStream_t stream);
+/* Launch variants of tacs_hip_bsr3_spmm / _mixed (prec_vals 64 or 32), all
+   with the same sums: for R a multiple of 4, blocks per lane per trip and
+   non-temporal value loads, variants 0 (1, on), 1 (2, on), 2 (1, off),
+   3 (2, off), 4 (4, on); one variant otherwise.  The two entries above
+   launch the variant of the tuning table. */
+int tacs_hip_bsr3_spmm_nvariants(int nrhs);
+int tacs_hip_bsr3_spmm_variant(int variant, int prec_vals, int nrows, int nrhs, const int* rowp,
+                               const int* cols, const void* A, const double* X, double* Y,
+                               hipStream_t stream);
 int tacs_hip_bbasis_combine(size_t n, int R, int nin, const double* W, size_t ldw, const double* P,
                             size_t sP, int ldp, int nout, double* out, size_t ldo, int add,
                             hipStream_t stream);
@@ -645,6 +703,31 @@ int tacs_mat_zero_entries(tacs_obj m);
    the first product after a change of the values otherwise does; no
    counterpart in the reference. */
 int tacs_mat_prepare_products(tacs_obj m, int prec);
+
+/* Measured launch choices (no counterpart in the reference; see the tuning
+   table above).  tacs_mat_autotune times every sliced ELL product variant
+   of the three kinds on this matrix (reps launches each, after one untimed)
+   and keeps the fastest per kind for the matrix; ms[kind * maxv + v]
+   receives the average times, *nvariants the variants per kind.  The
+   get/set pair reads or forces a kind's variant (-1: the table's).
+   tacs_mat_autotune_spmm times the variants of the nrhs right-hand-side
+   product (prec 64 or 32 values) and enters the fastest in the tuning table
+   for this matrix's size class.  tacs_assembler_autotune times the solid
+   Jacobian's rows-kernel variants and, with Jacobian kernel setting 0, the
+   element-block form (variant nv), assembling alpha = 1 Jacobians into mat
+   (reps each), and keeps the fastest for the assembler; the get/set pair
+   reads or forces the choice (-1: the table's).  Every variant of a kernel
+   gives identical values; the element-block form agrees with the rows to
+   rounding. */
+int tacs_mat_autotune(tacs_obj mat, int reps, double* ms, int maxv, int* nvariants);
+int tacs_mat_get_sell_variant(tacs_obj mat, int kind, int* variant);
+int tacs_mat_set_sell_variant(tacs_obj mat, int kind, int variant);
+int tacs_mat_autotune_spmm(tacs_obj mat, int nrhs, int prec, int reps, double* ms, int maxv,
+                           int* nvariants);
+int tacs_assembler_autotune(tacs_obj assembler, tacs_obj mat, int reps, double* ms, int maxv,
+                            int* nchoices);
+int tacs_assembler_get_jacobian_variant(tacs_obj assembler, int* variant);
+int tacs_assembler_set_jacobian_variant(tacs_obj assembler, int variant);
 int tacs_mat_get_info(tacs_obj m, int* bsize, int* nrows, long* nnzb);
 /* Host copies of the block-CSR arrays (BCSRMat::getArrays BCSRMat.h:82). */
 int tacs_mat_get_arrays(tacs_obj m, const int** rowp, const int** cols, const double** A);

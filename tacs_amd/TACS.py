@@ -8,6 +8,8 @@ Schur-complement direct factorisation), and ``Assembler.create`` ignores its
 communicator argument (single rank).
 """
 import ctypes
+import json
+import os
 
 import numpy as np
 
@@ -107,6 +109,42 @@ class Mat(TACSHandle):
         """Build the products' sliced ELL copy of the current values (FP64 or
         FP32) now instead of at the first product."""
         check(lib().tacs_mat_prepare_products(self.ptr, int(prec)), "Mat.prepareProducts")
+
+    def autotune(self, reps=20):
+        """Times every launch variant of the sliced ELL product of each kind
+        (0: FP64, 1: FP32 values with FP64 vectors, 2: FP32) on this matrix
+        and keeps the fastest per kind (all variants give identical
+        products).  Returns {"ms": [[per variant] per kind], "chosen": [...]}.
+        No counterpart in the reference."""
+        maxv = 8
+        ms = np.zeros(3 * maxv)
+        nv = ctypes.c_int()
+        check(lib().tacs_mat_autotune(self.ptr, int(reps), ms.ctypes.data, maxv, ctypes.byref(nv)),
+              "Mat.autotune")
+        return {"ms": ms.reshape(3, maxv)[:, :nv.value].tolist(),
+                "chosen": [self.getSellVariant(k) for k in range(3)]}
+
+    def getSellVariant(self, kind):
+        v = ctypes.c_int()
+        check(lib().tacs_mat_get_sell_variant(self.ptr, int(kind), ctypes.byref(v)),
+              "Mat.getSellVariant")
+        return v.value
+
+    def setSellVariant(self, kind, variant):
+        """Force the product variant of a kind (-1: the tuning table's)."""
+        check(lib().tacs_mat_set_sell_variant(self.ptr, int(kind), int(variant)),
+              "Mat.setSellVariant")
+
+    def autotuneSpMM(self, nrhs, prec=32, reps=10):
+        """Times the variants of the nrhs right-hand-side block product and
+        enters the fastest in the tuning table (size class of this matrix);
+        returns the times per variant."""
+        ms = np.zeros(8)
+        nv = ctypes.c_int()
+        check(lib().tacs_mat_autotune_spmm(self.ptr, int(nrhs), int(prec), int(reps),
+                                           ms.ctypes.data, 8, ctypes.byref(nv)),
+              "Mat.autotuneSpMM")
+        return ms[:nv.value].tolist()
 
     def getInfo(self):
         b, n, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
@@ -448,6 +486,28 @@ class Assembler(TACSHandle):
         """The solid Jacobian kernel assembleJacobian launches (1 or 2)."""
         return lib().tacs_assembler_jacobian_kernel(self.ptr)
 
+    def autotune(self, mat, reps=3):
+        """Times the solid Jacobian's launch choices (rows-kernel variants,
+        and the element-block form when the Jacobian kernel setting is 0) by
+        assembling alpha = 1 Jacobians into mat, and keeps the fastest.
+        Returns the times per choice.  No counterpart in the reference."""
+        ms = np.zeros(8)
+        nc = ctypes.c_int()
+        check(lib().tacs_assembler_autotune(self.ptr, mat.ptr, int(reps), ms.ctypes.data, 8,
+                                            ctypes.byref(nc)), "Assembler.autotune")
+        return ms[:nc.value].tolist()
+
+    def getJacobianVariant(self):
+        v = ctypes.c_int()
+        check(lib().tacs_assembler_get_jacobian_variant(self.ptr, ctypes.byref(v)),
+              "getJacobianVariant")
+        return v.value
+
+    def setJacobianVariant(self, variant):
+        """Force the Jacobian launch choice (-1: the tuning table's)."""
+        check(lib().tacs_assembler_set_jacobian_variant(self.ptr, int(variant)),
+              "setJacobianVariant")
+
     def setAuxElements(self, elems=None):
         self._aux = elems
         check(lib().tacs_assembler_set_aux_elements(self.ptr, elems.ptr if elems else None),
@@ -594,3 +654,39 @@ class Assembler(TACSHandle):
         check(lib().tacs_assembler_add_adjoint_res_xpt_sens_products(
             self.ptr, alpha, len(adjlist), _ptr_array(adjlist), _ptr_array(dfdXlist), loadScale),
             "addAdjointResXptSensProducts")
+
+
+# ---------------------------------------------------------------------------
+# Tuning table: measured launch variants per (kernel, shape, size class)
+# ---------------------------------------------------------------------------
+TUNE_SELL_SPMV, TUNE_SPMM, TUNE_JACOBIAN = 0, 1, 2
+TUNING_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_table.json")
+
+
+def setTuning(kernel, shape, size_class, variant):
+    """Enter a launch variant in the library's tuning table (-1 restores the
+    built-in default)."""
+    check(lib().tacs_tune_set(int(kernel), int(shape), int(size_class), int(variant)),
+          "setTuning")
+
+
+def getTuning(kernel, shape, size_class):
+    return lib().tacs_tune_get(int(kernel), int(shape), int(size_class))
+
+
+def loadTuningTable(path=TUNING_TABLE):
+    """Enter every entry of a table written by tacs_amd/autotune.py; returns
+    the number of entries (0 when the file does not exist)."""
+    if not os.path.exists(path):
+        return 0
+    with open(path) as f:
+        table = json.load(f)
+    n = 0
+    for e in table.get("entries", []):
+        setTuning(e["kernel_id"], e["shape"], e["size_class"], e["variant"])
+        n += 1
+    return n
+
+
+def resetTuning():
+    lib().tacs_tune_reset()

@@ -6,3 +6,6 @@ from ._lib import lib, LIB_PATH, TACSError  # noqa: F401
 from . import TACS, elements, constitutive, functions, meshes  # noqa: F401
 
 lib()  # fail loudly at import when the native library is missing
+# measured launch variants (tacs_amd/tuning_table.json, written by
+# tacs_amd/autotune.py): every variant gives the same results
+TACS.loadTuningTable()

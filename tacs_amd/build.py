@@ -10,7 +10,7 @@ SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT, "libtacs_hip.so")
 
-SOURCES = ["solid_kernels.hip", "shell_kernels.hip", "batch_kernels.hip", "dense_small.cpp", "bsr_kernels.hip", "vec_kernels.hip", "host.cpp", "capi.cpp"]
+SOURCES = ["solid_kernels.hip", "shell_kernels.hip", "batch_kernels.hip", "dense_small.cpp", "bsr_kernels.hip", "vec_kernels.hip", "tune.cpp", "host.cpp", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TACS_HIP_ARCH", "gfx950")

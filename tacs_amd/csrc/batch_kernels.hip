@@ -673,26 +673,47 @@ __global__ __launch_bounds__(256) void bsr3_spmm_quad(int nrows, int R,
   }
 }
 
+// Launch variants of the four-right-hand-sides-per-lane product (R a
+// multiple of 4), all with the same sums (each lane still adds its blocks
+// g, g + G, ... in order): blocks per lane per trip SPU and non-temporal
+// value loads NTA: 0 (1, on), 1 (2, on), 2 (1, off), 3 (2, off), 4 (4, on).
+// Other R: one variant.
+static constexpr int kSpmmQuadVariants = 5;
+
+template <typename TA, int NLV>
+static void spmm_quad_launch(int variant, int grid, int nrows, int R, const int* rowp,
+                             const int* cols, const TA* A, const double* X, double* Y,
+                             hipStream_t stream) {
+#define SPMM_QUAD(SPU, NTA)                                                                      \
+  hipLaunchKernelGGL((bsr3_spmm_quad<TA, NLV, SPU, NTA>), dim3(grid), dim3(256), 0, stream, nrows, \
+                     R, rowp, cols, A, X, Y)
+  switch (variant) {
+    case 1: SPMM_QUAD(2, 1); break;
+    case 2: SPMM_QUAD(1, 0); break;
+    case 3: SPMM_QUAD(2, 0); break;
+    case 4: SPMM_QUAD(4, 1); break;
+    default: SPMM_QUAD(1, 1); break;
+  }
+#undef SPMM_QUAD
+}
+
 template <typename TA>
-static int spmm_slots_launch(int nrows, int R, const int* rowp, const int* cols, const TA* A,
-                             const double* X, double* Y, hipStream_t stream) {
-  if (nrows <= 0 || R <= 0) return 0;
+static int spmm_slots_launch(int variant, int nrows, int R, const int* rowp, const int* cols,
+                             const TA* A, const double* X, double* Y, hipStream_t stream) {
   if (R > 64) return TACS_HIP_ERR_ARG;
+  if (nrows <= 0 || R <= 0) return 0;
+  if (variant < 0 || variant >= (R % 4 == 0 ? kSpmmQuadVariants : 1)) return TACS_HIP_ERR_ARG;
   // a multiple of 8 workgroups (one share per XCD)
   const int grid = 8 * tacs_grid_for((nrows + 7) / 8, 4, 8192);
   if (R % 4 == 0) {
     // 16-byte aligned X rows: four right-hand sides per lane
-#define SPMM_QUAD(NLV)                                                                          \
-  hipLaunchKernelGGL((bsr3_spmm_quad<TA, NLV>), dim3(grid), dim3(256), 0, stream, nrows, R, rowp, \
-                     cols, A, X, Y)
     switch (pow2_ceil(R / 4)) {
-      case 1: SPMM_QUAD(1); break;
-      case 2: SPMM_QUAD(2); break;
-      case 4: SPMM_QUAD(4); break;
-      case 8: SPMM_QUAD(8); break;
-      default: SPMM_QUAD(16); break;
+      case 1: spmm_quad_launch<TA, 1>(variant, grid, nrows, R, rowp, cols, A, X, Y, stream); break;
+      case 2: spmm_quad_launch<TA, 2>(variant, grid, nrows, R, rowp, cols, A, X, Y, stream); break;
+      case 4: spmm_quad_launch<TA, 4>(variant, grid, nrows, R, rowp, cols, A, X, Y, stream); break;
+      case 8: spmm_quad_launch<TA, 8>(variant, grid, nrows, R, rowp, cols, A, X, Y, stream); break;
+      default: spmm_quad_launch<TA, 16>(variant, grid, nrows, R, rowp, cols, A, X, Y, stream); break;
     }
-#undef SPMM_QUAD
     return (int)hipGetLastError();
   }
 #define SPMM_CASE(RPV)                                                                          \
@@ -1061,12 +1082,28 @@ int tacs_hip_bdiag_apply(int b, int prec, int nrows, int R, const void* Dinv, co
 
 int tacs_hip_bsr3_spmm_mixed(int nrows, int nrhs, const int* rowp, const int* cols, const float* A,
                              const double* X, double* Y, hipStream_t stream) {
-  return spmm_slots_launch<float>(nrows, nrhs, rowp, cols, A, X, Y, stream);
+  return spmm_slots_launch<float>(tacs_tune_variant(TACS_TUNE_SPMM, 100 + nrhs, nrows), nrows,
+                                  nrhs, rowp, cols, A, X, Y, stream);
 }
 
 int tacs_hip_bsr3_spmm(int nrows, int nrhs, const int* rowp, const int* cols, const double* A,
                        const double* X, double* Y, hipStream_t stream) {
-  return spmm_slots_launch<double>(nrows, nrhs, rowp, cols, A, X, Y, stream);
+  return spmm_slots_launch<double>(tacs_tune_variant(TACS_TUNE_SPMM, nrhs, nrows), nrows, nrhs,
+                                   rowp, cols, A, X, Y, stream);
+}
+
+int tacs_hip_bsr3_spmm_nvariants(int nrhs) { return nrhs % 4 == 0 ? kSpmmQuadVariants : 1; }
+
+int tacs_hip_bsr3_spmm_variant(int variant, int prec_vals, int nrows, int nrhs, const int* rowp,
+                               const int* cols, const void* A, const double* X, double* Y,
+                               hipStream_t stream) {
+  if (prec_vals == 64)
+    return spmm_slots_launch<double>(variant, nrows, nrhs, rowp, cols, (const double*)A, X, Y,
+                                     stream);
+  if (prec_vals == 32)
+    return spmm_slots_launch<float>(variant, nrows, nrhs, rowp, cols, (const float*)A, X, Y,
+                                    stream);
+  return TACS_HIP_ERR_ARG;
 }
 
 int tacs_hip_bbasis_combine(size_t n, int R, int nin, const double* W, size_t ldw, const double* P,

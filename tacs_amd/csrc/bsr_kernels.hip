@@ -321,12 +321,13 @@ __global__ void bsr_zero_rows(int b, int nbc, const int* __restrict__ nodes,
 // reduction is needed, and each row adds its blocks in column order with the
 // reference's in-block order (BCSRMatVecMult3 src/bpmat/BCSRMatMult3.cpp:28).
 // ---------------------------------------------------------------------------
-template <int B, typename TA, typename T>
+template <int B, typename TA, typename T, bool NT = true>
 __device__ inline void sell_slot(const TA* __restrict__ vs, long v, const T* __restrict__ x, int col,
                                  double* yr) {
   TA a[B * B];
 #pragma unroll
-  for (int q = 0; q < B * B; q++) a[q] = __builtin_nontemporal_load(&vs[v + 64 * q]);
+  for (int q = 0; q < B * B; q++)
+    a[q] = NT ? __builtin_nontemporal_load(&vs[v + 64 * q]) : vs[v + 64 * q];
   double xc[B];
 #pragma unroll
   for (int c = 0; c < B; c++) xc[c] = (double)x[(size_t)B * col + c];
@@ -340,11 +341,11 @@ __device__ inline void sell_slot(const TA* __restrict__ vs, long v, const T* __r
 }
 
 // U slots per iteration: every load of the U slots is issued before the
-// first block is multiplied (3x3 blocks: U = 2 for FP64 values, about 2 %
-// faster than one slot at a time at 128^3; U = 4 for FP32 values, 0.7-0.9 %
-// faster than U = 2; 6x6 blocks: U = 1); the blocks are still added in slot
-// order.
-template <int B, int U, typename TA, typename T>
+// first block is multiplied; the blocks are still added in slot order, so
+// every U gives the same sums.  NT: the values are read with non-temporal
+// loads (streamed past the caches, which keep x).  The launch takes U and
+// NT from the tuning table (sell_variant).
+template <int B, int U, typename TA, typename T, bool NT = true>
 __global__ __launch_bounds__(256) void sell_spmv(int nslices, int nrows,
                                                  const int* __restrict__ width,
                                                  const long* __restrict__ coff,
@@ -370,7 +371,8 @@ __global__ __launch_bounds__(256) void sell_spmv(int nslices, int nrows,
         for (int u = 0; u < U; u++)
 #pragma unroll
           for (int q = 0; q < B2; q++)
-            a[u][q] = __builtin_nontemporal_load(&vs[vo + 64 * (B2 * (t + u) + q)]);
+            a[u][q] = NT ? __builtin_nontemporal_load(&vs[vo + 64 * (B2 * (t + u) + q)])
+                         : vs[vo + 64 * (B2 * (t + u) + q)];
 #pragma unroll
         for (int u = 0; u < U; u++) {
           double xc[B];
@@ -386,7 +388,7 @@ __global__ __launch_bounds__(256) void sell_spmv(int nslices, int nrows,
         }
       }
     }
-    for (; t < w; t++) sell_slot<B, TA, T>(vs, vo + 64 * B2 * t, x, cs[co + 64 * t], yr);
+    for (; t < w; t++) sell_slot<B, TA, T, NT>(vs, vo + 64 * B2 * t, x, cs[co + 64 * t], yr);
     const int row = 64 * s + lane;
     if (row < nrows) {
 #pragma unroll
@@ -570,39 +572,68 @@ int tacs_hip_sell_fill(int b, int prec, int nrows, const int* rowp, const int* c
                              stream);
 }
 
-int tacs_hip_sell_spmv(int b, int prec_vals, int prec_vecs, int nrows, const int* width,
-                       const long* coff, const int* cs, const void* vs, const void* x, void* y,
-                       hipStream_t stream) {
+// Variants of the sliced ELL product (all give the same sums): 3x3 blocks
+// U in {1, 2, 4, 8} with (variants 0-3) or without (4-7) non-temporal value
+// loads; 6x6 blocks U in {1, 2}, with (0, 1) or without (2, 3).
+int tacs_hip_sell_spmv_nvariants(int b) { return b == 3 ? 8 : b == 6 ? 4 : 0; }
+
+int tacs_hip_sell_spmv_variant(int variant, int b, int prec_vals, int prec_vecs, int nrows,
+                               const int* width, const long* coff, const int* cs, const void* vs,
+                               const void* x, void* y, hipStream_t stream) {
   const int ns = (nrows + 63) / 64;
   const int kind = (prec_vals == 64 && prec_vecs == 64)   ? 0
                    : (prec_vals == 32 && prec_vecs == 64) ? 1
                    : (prec_vals == 32 && prec_vecs == 32) ? 2
                                                           : -1;
   if ((b != 3 && b != 6) || kind < 0) return TACS_HIP_ERR_ARG;
+  if (variant < 0 || variant >= tacs_hip_sell_spmv_nvariants(b)) return TACS_HIP_ERR_ARG;
   if (ns <= 0) return 0;
   const int grid = tacs_grid_for(ns, 4, 65536);
-#define SELL_SPMV(BB, U, TA, T)                                                                  \
-  hipLaunchKernelGGL((sell_spmv<BB, U, TA, T>), dim3(grid), dim3(256), 0, stream, ns, nrows, width, \
-                     coff, cs, (const TA*)vs, (const T*)x, (T*)y)
+#define SELL_SPMV(BB, U, NT)                                                                    \
+  do {                                                                                          \
+    if (kind == 0)                                                                              \
+      hipLaunchKernelGGL((sell_spmv<BB, U, double, double, NT>), dim3(grid), dim3(256), 0, stream, \
+                         ns, nrows, width, coff, cs, (const double*)vs, (const double*)x,      \
+                         (double*)y);                                                           \
+    else if (kind == 1)                                                                         \
+      hipLaunchKernelGGL((sell_spmv<BB, U, float, double, NT>), dim3(grid), dim3(256), 0, stream, \
+                         ns, nrows, width, coff, cs, (const float*)vs, (const double*)x,       \
+                         (double*)y);                                                           \
+    else                                                                                        \
+      hipLaunchKernelGGL((sell_spmv<BB, U, float, float, NT>), dim3(grid), dim3(256), 0, stream, \
+                         ns, nrows, width, coff, cs, (const float*)vs, (const float*)x,        \
+                         (float*)y);                                                            \
+  } while (0)
   if (b == 3) {
-    // slots per unrolled pass: 2 for FP64 values (3 measured slower), 4 for
-    // FP32 values (0.7-0.9 % faster than 2 at 128^3)
-    if (kind == 0)
-      SELL_SPMV(3, 2, double, double);
-    else if (kind == 1)
-      SELL_SPMV(3, 4, float, double);
-    else
-      SELL_SPMV(3, 4, float, float);
+    switch (variant) {
+      case 0: SELL_SPMV(3, 1, true); break;
+      case 1: SELL_SPMV(3, 2, true); break;
+      case 2: SELL_SPMV(3, 4, true); break;
+      case 3: SELL_SPMV(3, 8, true); break;
+      case 4: SELL_SPMV(3, 1, false); break;
+      case 5: SELL_SPMV(3, 2, false); break;
+      case 6: SELL_SPMV(3, 4, false); break;
+      default: SELL_SPMV(3, 8, false); break;
+    }
   } else {
-    if (kind == 0)
-      SELL_SPMV(6, 1, double, double);
-    else if (kind == 1)
-      SELL_SPMV(6, 1, float, double);
-    else
-      SELL_SPMV(6, 1, float, float);
+    switch (variant) {
+      case 0: SELL_SPMV(6, 1, true); break;
+      case 1: SELL_SPMV(6, 2, true); break;
+      case 2: SELL_SPMV(6, 1, false); break;
+      default: SELL_SPMV(6, 2, false); break;
+    }
   }
 #undef SELL_SPMV
   return (int)hipGetLastError();
+}
+
+int tacs_hip_sell_spmv(int b, int prec_vals, int prec_vecs, int nrows, const int* width,
+                       const long* coff, const int* cs, const void* vs, const void* x, void* y,
+                       hipStream_t stream) {
+  const int kind = (prec_vals == 64 && prec_vecs == 64) ? 0 : prec_vecs == 64 ? 1 : 2;
+  return tacs_hip_sell_spmv_variant(tacs_tune_variant(TACS_TUNE_SELL_SPMV, 10 * b + kind, nrows),
+                                    b, prec_vals, prec_vecs, nrows, width, coff, cs, vs, x, y,
+                                    stream);
 }
 
 int tacs_hip_bsr_find_diag(int nrows, const int* rowp, const int* cols, int* diag,

@@ -178,6 +178,28 @@ int tacs_assembler_jacobian_kernel(tacs_obj h) {
   REQUIRE(TACSAssembler, a, h);
   return a->jacobianKernelInUse();
 }
+int tacs_assembler_autotune(tacs_obj h, tacs_obj mat, int reps, double* ms, int maxv,
+                            int* nchoices) {
+  REQUIRE(TACSAssembler, a, h);
+  REQUIRE(TACSParallelMat, A, mat);
+  const int nc = a->autotune(A, reps, ms, maxv);
+  if (nchoices) *nchoices = last_error_code() ? 0 : nc;
+  return last_error_code();
+}
+int tacs_assembler_get_jacobian_variant(tacs_obj h, int* variant) {
+  REQUIRE(TACSAssembler, a, h);
+  if (!variant) return tacs_set_error("%s: variant pointer", __func__);
+  *variant = a->jacobianVariant();
+  return 0;
+}
+int tacs_assembler_set_jacobian_variant(tacs_obj h, int variant) {
+  REQUIRE(TACSAssembler, a, h);
+  const int nv = tacs_hip_solid_jacobian_nvariants(a->order);
+  if (variant < -1 || variant > nv)
+    return tacs_set_error("%s: variant -1..%d (%d: element blocks)", __func__, nv, nv);
+  a->jacVar = variant;
+  return 0;
+}
 int tacs_assembler_set_aux_elements(tacs_obj h, tacs_obj aux) {
   REQUIRE(TACSAssembler, a, h);
   a->setAuxElements(aux ? OBJ(TACSAuxElements, aux) : NULL);
@@ -536,6 +558,35 @@ int tacs_mat_prepare_products(tacs_obj m, int prec) {
   REQUIRE(TACSParallelMat, A, m);
   if (prec != 64 && prec != 32) return tacs_set_error("%s: prec 64 or 32", __func__);
   if ((prec == 64 ? A->sellVer64 : A->sellVer32) != A->version) A->sellUpdate(prec);
+  return last_error_code();
+}
+int tacs_mat_autotune(tacs_obj m, int reps, double* ms, int maxv, int* nvariants) {
+  REQUIRE(TACSParallelMat, A, m);
+  const int nv = A->autotune(reps, ms, maxv);
+  if (nvariants) *nvariants = last_error_code() ? 0 : nv;
+  return last_error_code();
+}
+int tacs_mat_get_sell_variant(tacs_obj m, int kind, int* variant) {
+  REQUIRE(TACSParallelMat, A, m);
+  if (kind < 0 || kind > 2 || !variant) return tacs_set_error("%s: kind 0, 1 or 2", __func__);
+  *variant = A->sellVar[kind] >= 0 ? A->sellVar[kind]
+                                   : tacs_tune_variant(TACS_TUNE_SELL_SPMV, 10 * A->bsize + kind,
+                                                       A->nrows);
+  return 0;
+}
+int tacs_mat_set_sell_variant(tacs_obj m, int kind, int variant) {
+  REQUIRE(TACSParallelMat, A, m);
+  if (kind < 0 || kind > 2 || variant < -1 || variant >= tacs_hip_sell_spmv_nvariants(A->bsize))
+    return tacs_set_error("%s: kind 0..2, variant -1..%d", __func__,
+                          tacs_hip_sell_spmv_nvariants(A->bsize) - 1);
+  A->sellVar[kind] = variant;
+  return 0;
+}
+int tacs_mat_autotune_spmm(tacs_obj m, int nrhs, int prec, int reps, double* ms, int maxv,
+                           int* nvariants) {
+  REQUIRE(TACSParallelMat, A, m);
+  const int nv = A->autotuneSpMM(nrhs, prec, reps, ms, maxv);
+  if (nvariants) *nvariants = last_error_code() ? 0 : nv;
   return last_error_code();
 }
 int tacs_mat_zero_entries(tacs_obj m) {

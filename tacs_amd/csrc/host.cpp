@@ -452,6 +452,7 @@ TACSParallelMat::TACSParallelMat(int _bsize, int _nrows, const int* rowp, const 
   d_s32 = NULL;
   sellVer64 = sellVer32 = -1;
   nslots = 0;
+  sellVar[0] = sellVar[1] = sellVar[2] = -1;
 }
 TACSParallelMat::~TACSParallelMat() {
   dfree(d_rowp);
@@ -512,18 +513,112 @@ void TACSParallelMat::sellUpdate(int prec) {
   if (v64) sellVer64 = version;
   if (v32) sellVer32 = version;
 }
-// The products run over the sliced ELL copies (3x3 and 6x6 blocks).
+// The products run over the sliced ELL copies (3x3 and 6x6 blocks), in the
+// matrix's launch variant (autotune) or the tuning table's.
+static int sell_kind_variant(const TACSParallelMat* m, int kind) {
+  return m->sellVar[kind] >= 0 ? m->sellVar[kind]
+                               : tacs_tune_variant(TACS_TUNE_SELL_SPMV, 10 * m->bsize + kind,
+                                                   m->nrows);
+}
 void TACSParallelMat::mult64(const double* x, double* y) {
   if (sellVer64 != version) sellUpdate(64);
-  KCHECK(tacs_hip_sell_spmv(bsize, 64, 64, nrows, d_sw, d_soff, d_scols, d_s64, x, y, g_stream));
+  KCHECK(tacs_hip_sell_spmv_variant(sell_kind_variant(this, 0), bsize, 64, 64, nrows, d_sw, d_soff,
+                                    d_scols, d_s64, x, y, g_stream));
 }
 void TACSParallelMat::multMixed(const double* x, double* y) {
   if (sellVer32 != version) sellUpdate(32);
-  KCHECK(tacs_hip_sell_spmv(bsize, 32, 64, nrows, d_sw, d_soff, d_scols, d_s32, x, y, g_stream));
+  KCHECK(tacs_hip_sell_spmv_variant(sell_kind_variant(this, 1), bsize, 32, 64, nrows, d_sw, d_soff,
+                                    d_scols, d_s32, x, y, g_stream));
 }
 void TACSParallelMat::mult32(const float* x, float* y) {
   if (sellVer32 != version) sellUpdate(32);
-  KCHECK(tacs_hip_sell_spmv(bsize, 32, 32, nrows, d_sw, d_soff, d_scols, d_s32, x, y, g_stream));
+  KCHECK(tacs_hip_sell_spmv_variant(sell_kind_variant(this, 2), bsize, 32, 32, nrows, d_sw, d_soff,
+                                    d_scols, d_s32, x, y, g_stream));
+}
+// Average time of reps launches of fn on the library stream (one untimed
+// launch first), in ms; negative after a failure.
+template <typename F>
+static double time_launches(int reps, F fn) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return -1.0;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return -1.0;
+  }
+  int err = fn();
+  HCHECK(hipEventRecord(e0, g_stream));
+  for (int r = 0; r < reps && !err; r++) err = fn();
+  HCHECK(hipEventRecord(e1, g_stream));
+  HCHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HCHECK(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (err) {
+    tacs_set_error("autotune: launch failed (%d)", err);
+    return -1.0;
+  }
+  return g_halt ? -1.0 : (double)ms / reps;
+}
+int TACSParallelMat::autotune(int reps, double* ms, int maxv) {
+  const int nv = tacs_hip_sell_spmv_nvariants(bsize);
+  if (reps < 1 || !ms || maxv < nv) return tacs_set_error("autotune: reps >= 1 and room for %d variants", nv);
+  if (sellVer64 != version) sellUpdate(64);
+  if (!d_s32) sellUpdate(32);
+  else if (sellVer32 != version) sellUpdate(32);
+  const size_t n = (size_t)nrows * bsize;
+  double* x = dalloc<double>(n);
+  double* y = dalloc<double>(n);
+  float* x32 = dalloc<float>(n);
+  float* y32 = dalloc<float>(n);
+  if (!g_halt) {
+    KCHECK(tacs_hip_fill(64, n, x, 1.0, g_stream));
+    KCHECK(tacs_hip_fill(32, n, x32, 1.0, g_stream));
+  }
+  for (int kind = 0; kind < 3 && !g_halt; kind++) {
+    int best = 0;
+    for (int v = 0; v < nv && !g_halt; v++) {
+      const int pv = kind == 0 ? 64 : 32, pw = kind == 2 ? 32 : 64;
+      const void* vs = kind == 0 ? (const void*)d_s64 : (const void*)d_s32;
+      const void* xa = kind == 2 ? (const void*)x32 : (const void*)x;
+      void* ya = kind == 2 ? (void*)y32 : (void*)y;
+      ms[kind * maxv + v] = time_launches(reps, [&]() {
+        return tacs_hip_sell_spmv_variant(v, bsize, pv, pw, nrows, d_sw, d_soff, d_scols, vs, xa,
+                                          ya, g_stream);
+      });
+      if (ms[kind * maxv + v] < ms[kind * maxv + best]) best = v;
+    }
+    if (!g_halt) sellVar[kind] = best;
+  }
+  dfree(x);
+  dfree(y);
+  dfree(x32);
+  dfree(y32);
+  return g_halt ? 1 : nv;
+}
+int TACSParallelMat::autotuneSpMM(int nrhs, int prec, int reps, double* ms, int maxv) {
+  if (bsize != 3) return tacs_set_error("autotuneSpMM: 3x3 blocks only");
+  if (nrhs < 1 || nrhs > 64 || (prec != 64 && prec != 32))
+    return tacs_set_error("autotuneSpMM: 1 <= nrhs <= 64, prec 64 or 32");
+  const int nv = tacs_hip_bsr3_spmm_nvariants(nrhs);
+  if (reps < 1 || !ms || maxv < nv) return tacs_set_error("autotuneSpMM: reps >= 1 and room for %d variants", nv);
+  const void* vals = prec == 64 ? (const void*)d_A : (const void*)getValues32(1);
+  const size_t n = (size_t)nrows * 3 * nrhs;
+  double* X = dalloc<double>(n);
+  double* Y = dalloc<double>(n);
+  if (!g_halt) KCHECK(tacs_hip_fill(64, n, X, 1.0, g_stream));
+  int best = 0;
+  for (int v = 0; v < nv && !g_halt; v++) {
+    ms[v] = time_launches(reps, [&]() {
+      return tacs_hip_bsr3_spmm_variant(v, prec, nrows, nrhs, d_rowp, d_cols, vals, X, Y, g_stream);
+    });
+    if (ms[v] < ms[best]) best = v;
+  }
+  dfree(X);
+  dfree(Y);
+  if (g_halt) return 1;
+  tacs_tune_set(TACS_TUNE_SPMM, (prec == 32 ? 100 : 0) + nrhs, tacs_tune_size_class(nrows), best);
+  return nv;
 }
 void TACSParallelMat::zeroEntries() {
   KCHECK(tacs_hip_fill(64, (size_t)nnzb * bsize * bsize, d_A, 0.0, g_stream));
@@ -2319,7 +2414,7 @@ TACSAssembler::TACSAssembler(TACSComm _comm, int vpn, int nowned, int nelems, in
   comm = _comm;
   elemPrec = 64;
   jacKernel = 0;
-  d_cptr = d_clist = NULL;
+  jacVar = -1;
   d_keb = NULL;
 }
 
@@ -2338,7 +2433,7 @@ TACSAssembler::~TACSAssembler() {
   void* ds[] = {d_conn, d_n2e_ptr, d_n2e, d_bcvars, d_bcnodes, d_bcmask, d_bcvals, d_dvnum,
                 d_dv2e_ptr, d_dv2e, d_tfixed, d_telem, d_X, d_u, d_ev, d_es, d_aux_elem,
                 d_aux_face, d_n2a_ptr, d_n2a, d_aux_trac, d_aux_ev, d_status, d_red, d_scalar,
-                d_ke, d_aux_pres, d_qgeo, d_qstress, d_cptr, d_clist, d_keb};
+                d_ke, d_aux_pres, d_qgeo, d_qstress, d_keb};
   for (void* p : ds) dfree(p);
 }
 
@@ -2354,76 +2449,48 @@ int TACSAssembler::setJacobianKernel(int kernel) {
 }
 // The element-block path needs the isotropic tangent and block indices that
 // fit int32; elsewhere (and when asked) the row kernel runs.  jacKernel = 0
-// follows the timings measured on the MI355X: 27-node hexahedra (40^3) 5.5 ms
+// follows the assembler's autotune choice or the tuning table, whose
+// built-in defaults are the MI355X timings: 27-node hexahedra (40^3) 5.0 ms
 // with element blocks against 27.8 ms with rows; 8-node (128^3) 11.5 ms
 // against 8.2 ms, so FP64 8-node elements keep the rows.  FP32 element
 // arithmetic exists only as element blocks.
+int TACSAssembler::jacobianVariant() {
+  const int nv = tacs_hip_solid_jacobian_nvariants(order);
+  const int v = jacVar >= 0 ? jacVar : tacs_tune_variant(TACS_TUNE_JACOBIAN, order, numNodes);
+  return v > nv ? nv : v;
+}
 int TACSAssembler::jacobianKernelInUse() {
   if (family != FAMILY_SOLID) return 0;
   const bool fits = mat.isotropic && (double)numElements * npe * npe < 2147483647.0;
   if (jacKernel == 1 || !fits) return 1;
-  if (jacKernel == 2) return 2;
-  return (order == 3 || elemPrec == 32) ? 2 : 1;
+  if (jacKernel == 2 || elemPrec == 32) return 2;
+  return jacobianVariant() == tacs_hip_solid_jacobian_nvariants(order) ? 2 : 1;
 }
-
-// Per slot of the block-CSR pattern, the element blocks that sum into it:
-// for row i, incident element e (ascending) with local node a, and local
-// column node b, the block e NPE^2 + a NPE + b goes to the slot of column
-// conn[e][b].  Built once, on the host, from the connectivity.
-int TACSAssembler::buildBlockLists() {
-  if (d_cptr) return 0;
-  const long NB = (long)npe * npe;
-  std::vector<int> n2e_ptr(numNodes + 1, 0);
-  for (size_t i = 0; i < (size_t)numElements * npe; i++) n2e_ptr[h_conn[i] + 1]++;
-  for (int i = 0; i < numNodes; i++) n2e_ptr[i + 1] += n2e_ptr[i];
-  std::vector<int> n2e(n2e_ptr[numNodes]);
-  {
-    std::vector<int> pos(n2e_ptr.begin(), n2e_ptr.end() - 1);
-    for (int e = 0; e < numElements; e++)
-      for (int a = 0; a < npe; a++) n2e[pos[h_conn[(size_t)e * npe + a]]++] = (e << 5) | a;
-  }
-  std::vector<int> cptr((size_t)nnzb + 1, 0);
-  std::vector<int> clist((size_t)numElements * NB);
-  std::vector<int> slot(npe * 8), cnt;
-  for (int i = 0; i < numNodes; i++) {
-    const int rs = h_rowp[i], nc = h_rowp[i + 1] - rs;
-    const int* cs = h_cols + rs;
-    cnt.assign(nc + 1, 0);
-    const int p0 = n2e_ptr[i], ne = n2e_ptr[i + 1] - p0;
-    if ((int)slot.size() < ne * npe) slot.resize((size_t)ne * npe);
-    for (int p = 0; p < ne; p++) {
-      const int e = n2e[p0 + p] >> 5;
-      for (int b = 0; b < npe; b++) {
-        const int col = h_conn[(size_t)e * npe + b];
-        const int s = (int)(std::lower_bound(cs, cs + nc, col) - cs);
-        if (s >= nc || cs[s] != col) return tacs_set_error("buildBlockLists: column %d missing", col);
-        slot[(size_t)p * npe + b] = s;
-        cnt[s + 1]++;
-      }
+int TACSAssembler::autotune(TACSParallelMat* A, int reps, double* ms, int maxv) {
+  if (!initialized || family != FAMILY_SOLID || !mat.isotropic || !A)
+    return tacs_set_error("autotune: an initialised assembler of isotropic solid elements and a "
+                          "matrix");
+  if (jacKernel == 2 || elemPrec == 32) return 0;  // the element-block form only: nothing to time
+  const int nv = tacs_hip_solid_jacobian_nvariants(order);
+  const bool fits = (double)numElements * npe * npe < 2147483647.0;
+  const int nc = nv + (jacKernel == 0 && fits ? 1 : 0);
+  if (reps < 1 || !ms || maxv < nc)
+    return tacs_set_error("autotune: reps >= 1 and room for %d choices", nc);
+  const int saved = jacVar;
+  int best = 0;
+  for (int c = 0; c < nc && !g_halt; c++) {
+    jacVar = c;
+    assembleJacobian(1.0, 0.0, 0.0, NULL, A, TACS_MAT_NORMAL, 1.0, 1);
+    double t = 0.0;
+    for (int r = 0; r < reps && !g_halt; r++) {
+      assembleJacobian(1.0, 0.0, 0.0, NULL, A, TACS_MAT_NORMAL, 1.0, 1);
+      t += jac_ms;
     }
-    for (int s = 0; s < nc; s++) cnt[s + 1] += cnt[s];
-    for (int s = 0; s < nc; s++) cptr[rs + s + 1] = cptr[rs] + cnt[s + 1];
-    for (int p = 0; p < ne; p++) {
-      const int e = n2e[p0 + p] >> 5, a = n2e[p0 + p] & 31;
-      for (int b = 0; b < npe; b++) {
-        const int s = slot[(size_t)p * npe + b];
-        clist[(size_t)cptr[rs] + cnt[s]++] = (int)(e * NB + (long)a * npe + b);
-      }
-    }
+    ms[c] = t / reps;
+    if (ms[c] < ms[best]) best = c;
   }
-  int* cp = dalloc<int>(cptr.size());
-  int* cl = dalloc<int>(clist.size());
-  h2d(cp, cptr.data(), cptr.size());
-  h2d(cl, clist.data(), clist.size());
-  HCHECK(hipStreamSynchronize(g_stream));
-  if (g_halt) {
-    dfree(cp);
-    dfree(cl);
-    return 1;
-  }
-  d_cptr = cp;
-  d_clist = cl;
-  return 0;
+  jacVar = g_halt ? saved : best;
+  return g_halt ? 1 : nc;
 }
 
 int TACSAssembler::setElementConnectivity(const int* ptr, const int* conn) {
@@ -2970,25 +3037,22 @@ void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScal
   }
   if (kern == 2) {
     // element blocks in the element precision, then the row gather
-    if (buildBlockLists() != 0) {
-      HCHECK(hipEventDestroy(e0));
-      HCHECK(hipEventDestroy(e1));
-      return;
-    }
     if (!d_keb) d_keb = dalloc<char>((size_t)numElements * npe * npe * 9 * sizeof(double));
     HCHECK(hipEventRecord(e0, g_stream));
     KCHECK(tacs_hip_solid_elem_blocks(order, elemPrec, numElements, d_conn, d_X, d_telem, &mat,
                                       alpha, gamma, d_keb, g_stream));
-    KCHECK(tacs_hip_gather_solid_blocks(elemPrec, numNodes, A->d_rowp, A->d_cols, d_cptr, d_clist,
-                                        d_keb, bcs ? d_bcvars : NULL, A->d_A, A->d_A32,
-                                        g_stream));
+    KCHECK(tacs_hip_gather_solid_blocks(order, elemPrec, numNodes, A->d_rowp, A->d_cols,
+                                        d_n2e_ptr, d_n2e, d_conn, d_keb, bcs ? d_bcvars : NULL,
+                                        A->d_A, A->d_A32, d_status, g_stream));
   } else {
     HCHECK(hipEventRecord(e0, g_stream));
     ensure(d_qgeo, (size_t)numElements * npe * 10);
-    KCHECK(tacs_hip_solid_jacobian(order, numNodes, numElements, d_n2e_ptr, d_n2e, d_conn, d_X,
-                                   d_telem, &mat, alpha, gamma, A->d_rowp, A->d_cols,
-                                   bcs ? d_bcvars : NULL, d_qgeo, A->d_A, A->d_A32, d_status,
-                                   g_stream));
+    int v = jacobianVariant();
+    if (v >= tacs_hip_solid_jacobian_nvariants(order)) v = 0;
+    KCHECK(tacs_hip_solid_jacobian_variant(v, order, numNodes, numElements, d_n2e_ptr, d_n2e,
+                                           d_conn, d_X, d_telem, &mat, alpha, gamma, A->d_rowp,
+                                           A->d_cols, bcs ? d_bcvars : NULL, d_qgeo, A->d_A,
+                                           A->d_A32, d_status, g_stream));
   }
   HCHECK(hipEventRecord(e1, g_stream));
   HCHECK(hipEventSynchronize(e1));

@@ -255,8 +255,10 @@ struct JacCfg<3> {
 // (TACSLinearElasticity3D::evalWeakMatrix, TACS_GEOMETRIC_STIFFNESS_MATRIX,
 // src/elements/TACSLinearElasticity.cpp:1704: coefficient sigma_de of
 // du_j/dx_d du_j/dx_e).
-template <int ORD, bool ISO, bool GEO>
-__global__ __launch_bounds__(64 * JacCfg<ORD>::WPB) void solid_jacobian_rows(
+// WPB wavefronts (rows) per workgroup and ECH elements per LDS geometry
+// chunk are launch variants: the sums do not depend on them.
+template <int ORD, bool ISO, bool GEO, int WPB = JacCfg<ORD>::WPB, int ECH = 8>
+__global__ __launch_bounds__(64 * WPB) void solid_jacobian_rows(
     int nnodes, const int* __restrict__ n2e_ptr, const int* __restrict__ n2e,
     const int* __restrict__ conn, const double* __restrict__ Gq, const double* __restrict__ Sq,
     SolidMaterial mat, double alpha, double gamma, double geo, const int* __restrict__ rowp,
@@ -264,8 +266,7 @@ __global__ __launch_bounds__(64 * JacCfg<ORD>::WPB) void solid_jacobian_rows(
     const int* __restrict__ bcvars, double* __restrict__ A,
     float* __restrict__ A32, int* __restrict__ status) {
   constexpr int NPE = ORD * ORD * ORD, NQP = NPE;
-  constexpr int MAXC = JacCfg<ORD>::MAXC, WPB = JacCfg<ORD>::WPB;
-  constexpr int ECH = 8;  // elements per chunk
+  constexpr int MAXC = JacCfg<ORD>::MAXC;
   __shared__ double geom[WPB][ECH * NQP][10];
   __shared__ double sgeo[WPB][GEO ? ECH * NQP : 1][6];
   __shared__ double rowbuf[WPB][MAXC * 9];
@@ -1308,42 +1309,150 @@ __global__ __launch_bounds__(EBCfg<ORD>::TEAM* EBCfg<ORD>::EPB) void solid_elem_
   }
 }
 
-// Block-CSR rows from element blocks: value k of slot s of a row is the sum,
-// in the order of the slot's list (ascending element, as the reference's
-// element loop adds them, src/TACSAssembler.cpp:4352-4383), of value k of
-// the listed element blocks ke[clist[c]]; accumulated in FP64.  One
-// wavefront per row; Dirichlet rows fused (BCSRMat::zeroRow ident = 1,
-// src/bpmat/BCSRMat.cpp:2027).
-template <typename T>
-__global__ __launch_bounds__(256) void gather_solid_blocks(
+// Block-CSR rows from element blocks: one wavefront per row (node a).  The
+// row's incident (element e, local node a) pairs are taken in ascending
+// element order (the order the reference's element loop adds them,
+// src/TACSAssembler.cpp:4352-4383), CH pairs at a time: the wave loads the
+// CH element block rows ke[e][a npe .. a npe + npe) (npe contiguous 3x3
+// blocks each: coalesced runs, all loads in flight together), finds the
+// row's slot of every column conn[e][b], then adds the pairs one after the
+// other into the row accumulated in FP64 in LDS.  Dirichlet rows fused
+// (BCSRMat::zeroRow ident = 1, src/bpmat/BCSRMat.cpp:2027); the row is
+// written once, coalesced.
+template <int NPE>
+struct GatherCfg {
+  static constexpr int MAXC = NPE == 8 ? 64 : 125, WPB = 4, CH = 8;
+  static constexpr int NV = NPE * 9, PER = (NV + 63) / 64;  // values of a block row, per lane
+};
+template <int NPE, typename T>
+__global__ __launch_bounds__(64 * GatherCfg<NPE>::WPB) void gather_solid_rows(
     int nrows, const int* __restrict__ rowp, const int* __restrict__ cols,
-    const int* __restrict__ cptr, const int* __restrict__ clist, const T* __restrict__ ke,
-    const int* __restrict__ bcvars, double* __restrict__ A, float* __restrict__ A32) {
-  const int lane = threadIdx.x % TACS_WAVE;
-  const int wpb = blockDim.x / TACS_WAVE;
-  for (int row = blockIdx.x * wpb + threadIdx.x / TACS_WAVE; row < nrows; row += gridDim.x * wpb) {
-    const int r0 = rowp[row], nvals = (rowp[row + 1] - r0) * 9;
-    const int mask = bcvars ? bcvars[row] : 0;
-    for (int v = lane; v < nvals; v += TACS_WAVE) {
-      const int s = r0 + v / 9, k = v % 9;
-      double acc = 0.0;
-      const int c1 = cptr[s + 1];
-      for (int cc = cptr[s]; cc < c1; cc++) acc += (double)ke[(size_t)clist[cc] * 9 + k];
-      if (mask & (1 << (k / 3))) acc = (cols[s] == row && k / 3 == k % 3) ? 1.0 : 0.0;
-      if (A) A[(size_t)s * 9 + k] = acc;
-      if (A32) A32[(size_t)s * 9 + k] = (float)acc;
+    const int* __restrict__ n2e_ptr, const int* __restrict__ n2e, const int* __restrict__ conn,
+    const T* __restrict__ ke, const int* __restrict__ bcvars, double* __restrict__ A,
+    float* __restrict__ A32, int* __restrict__ status) {
+  constexpr int MAXC = GatherCfg<NPE>::MAXC, WPB = GatherCfg<NPE>::WPB, CH = GatherCfg<NPE>::CH;
+  constexpr int NV = GatherCfg<NPE>::NV, PER = GatherCfg<NPE>::PER;
+  __shared__ double rowbuf[WPB][MAXC * 9];
+  __shared__ int colbuf[WPB][MAXC];
+  __shared__ short smap[WPB][CH * NPE];  // slots < MAXC <= 125
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double* R = rowbuf[wid];
+  int* CB = colbuf[wid];
+  short* SM = smap[wid];
+  for (int row = blockIdx.x * WPB + wid; row < nrows; row += gridDim.x * WPB) {
+    const int rs = rowp[row];
+    const int nc = rowp[row + 1] - rs;
+    if (nc > MAXC) {
+      if (lane == 0) atomicExch(status, TACS_HIP_ERR_ROW_TOO_LONG);
+      continue;
     }
+    for (int k = lane; k < nc; k += 64) CB[k] = cols[rs + k];
+    for (int k = lane; k < nc * 9; k += 64) R[k] = 0.0;
+    const int es = n2e_ptr[row];
+    const int ne = n2e_ptr[row + 1] - es;
+    WAVE_SYNC();
+    for (int p0 = 0; p0 < ne; p0 += CH) {
+      const int np = min(CH, ne - p0);
+      // the chunk's block rows, every load issued before the first use
+      T v[CH][PER];
+#pragma unroll
+      for (int c = 0; c < CH; c++) {
+        const int packed = c < np ? n2e[es + p0 + c] : 0;
+        const T* src = ke + ((size_t)(packed >> 5) * NPE + (packed & 31)) * NPE * 9;
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+          const int q = lane + 64 * r;
+          v[c][r] = (c < np && (NV % 64 == 0 || q < NV)) ? src[q] : T(0);
+        }
+      }
+      // slot of column conn[e][b] for (pair c, block b) = t: t = c npe + b
+      for (int t = lane; t < np * NPE; t += 64) {
+        const int c = t / NPE, b = t - c * NPE;
+        const int cn = conn[(size_t)(n2e[es + p0 + c] >> 5) * NPE + b];
+        int lo = 0, hi = nc - 1, sl = -1;
+        while (lo <= hi) {
+          const int mid = (lo + hi) >> 1, cv = CB[mid];
+          if (cv == cn) {
+            sl = mid;
+            break;
+          }
+          if (cv < cn) lo = mid + 1;
+          else hi = mid - 1;
+        }
+        if (sl < 0) atomicExch(status, TACS_HIP_ERR_MISSING_ENTRY);
+        SM[t] = (short)sl;
+      }
+      WAVE_SYNC();
+#pragma unroll
+      for (int c = 0; c < CH; c++) {
+        if (c < np) {
+#pragma unroll
+          for (int r = 0; r < PER; r++) {
+            const int q = lane + 64 * r;
+            if (NV % 64 == 0 || q < NV) {
+              const int b = q / 9, k = q - 9 * b;
+              const int sl = SM[c * NPE + b];
+              if (sl >= 0) R[sl * 9 + k] += (double)v[c][r];
+            }
+          }
+        }
+        WAVE_SYNC();
+      }
+    }
+    const int bc = bcvars ? bcvars[row] : 0;
+    for (int k = lane; k < nc * 9; k += 64) {
+      const int s = k / 9, rc = k - 9 * s, r = rc / 3, cc = rc - 3 * r;
+      double val = R[k];
+      if (bc & (1 << r)) val = (CB[s] == row && r == cc) ? 1.0 : 0.0;
+      if (A) A[(size_t)rs * 9 + k] = val;
+      if (A32) A32[(size_t)rs * 9 + k] = (float)val;
+    }
+    WAVE_SYNC();
   }
 }
 
-extern "C" {
+// Launch variants (WPB, ECH) of the isotropic stiffness / mass rows kernel:
+// 8-node elements 0 (4, 8), 1 (2, 8), 2 (8, 4), 3 (4, 4), 4 (4, 16);
+// 27-node elements 0 (2, 8), 1 (1, 8), 2 (2, 4), 3 (1, 4).
+static int jac_rows_nvariants(int order) { return order == 2 ? 5 : order == 3 ? 4 : 0; }
 
-int tacs_hip_solid_mat_type(int order, int nnodes, int nelem, const int* n2e_ptr,
-                            const int* n2e, const int* conn, const double* X, const double* u,
-                            const double* telem, const SolidMaterial* mat, double kc, double mc,
-                            double gc, const int* rowp, const int* cols, const int* bcvars,
-                            double* G, double* S, double* A, float* A32, int* status,
-                            hipStream_t stream) {
+template <int O>
+static void jac_rows_iso_launch(int variant, int nnodes, const int* n2e_ptr, const int* n2e,
+                                const int* conn, const double* G, const SolidMaterial* mat,
+                                double kc, double mc, const int* rowp, const int* cols,
+                                const int* bcvars, double* A, float* A32, int* status,
+                                hipStream_t stream) {
+#define JAC_V(W, E)                                                                              \
+  hipLaunchKernelGGL((solid_jacobian_rows<O, true, false, W, E>),                                \
+                     dim3(tacs_grid_for(nnodes, W, 1 << 20)), dim3(64 * W), 0, stream, nnodes,   \
+                     n2e_ptr, n2e, conn, G, (const double*)NULL, *mat, kc, mc, 0.0, rowp, cols,  \
+                     bcvars, A, A32, status)
+  if constexpr (O == 2) {
+    switch (variant) {
+      case 1: JAC_V(2, 8); break;
+      case 2: JAC_V(8, 4); break;
+      case 3: JAC_V(4, 4); break;
+      case 4: JAC_V(4, 16); break;
+      default: JAC_V(4, 8); break;
+    }
+  } else {
+    switch (variant) {
+      case 1: JAC_V(1, 8); break;
+      case 2: JAC_V(2, 4); break;
+      case 3: JAC_V(1, 4); break;
+      default: JAC_V(2, 8); break;
+    }
+  }
+#undef JAC_V
+}
+
+static int solid_mat_type_impl(int rows_variant, int order, int nnodes, int nelem,
+                               const int* n2e_ptr, const int* n2e, const int* conn,
+                               const double* X, const double* u, const double* telem,
+                               const SolidMaterial* mat, double kc, double mc, double gc,
+                               const int* rowp, const int* cols, const int* bcvars, double* G,
+                               double* S, double* A, float* A32, int* status,
+                               hipStream_t stream) {
   if (order != 2 && order != 3) return TACS_HIP_ERR_ARG;
   if (gc != 0.0 && (!u || !S)) return TACS_HIP_ERR_ARG;
   const int nqp = order * order * order;
@@ -1376,6 +1485,15 @@ int tacs_hip_solid_mat_type(int order, int nnodes, int nelem, const int* n2e_ptr
   else                            \
     JAC_LAUNCH(O, false, GE)
   const bool ge = gc != 0.0;
+  if (!ge && mat->isotropic) {
+    if (order == 2)
+      jac_rows_iso_launch<2>(rows_variant, nnodes, n2e_ptr, n2e, conn, G, mat, kc, mc, rowp, cols,
+                             bcvars, A, A32, status, stream);
+    else
+      jac_rows_iso_launch<3>(rows_variant, nnodes, n2e_ptr, n2e, conn, G, mat, kc, mc, rowp, cols,
+                             bcvars, A, A32, status, stream);
+    return (int)hipGetLastError();
+  }
   if (order == 2) {
     if (ge) {
       JAC_ISO(2, true);
@@ -1392,6 +1510,35 @@ int tacs_hip_solid_mat_type(int order, int nnodes, int nelem, const int* n2e_ptr
 #undef JAC_ISO
 #undef JAC_LAUNCH
   return (int)hipGetLastError();
+}
+
+extern "C" {
+
+int tacs_hip_solid_mat_type(int order, int nnodes, int nelem, const int* n2e_ptr,
+                            const int* n2e, const int* conn, const double* X, const double* u,
+                            const double* telem, const SolidMaterial* mat, double kc, double mc,
+                            double gc, const int* rowp, const int* cols, const int* bcvars,
+                            double* G, double* S, double* A, float* A32, int* status,
+                            hipStream_t stream) {
+  int v = tacs_tune_variant(TACS_TUNE_JACOBIAN, order, nnodes);
+  if (v >= jac_rows_nvariants(order)) v = 0;  // the element-block form: rows default
+  return solid_mat_type_impl(v, order, nnodes, nelem, n2e_ptr, n2e, conn, X, u, telem, mat, kc,
+                             mc, gc, rowp, cols, bcvars, G, S, A, A32, status, stream);
+}
+
+int tacs_hip_solid_jacobian_nvariants(int order) { return jac_rows_nvariants(order); }
+
+int tacs_hip_solid_jacobian_variant(int variant, int order, int nnodes, int nelem,
+                                    const int* n2e_ptr, const int* n2e, const int* conn,
+                                    const double* X, const double* telem,
+                                    const SolidMaterial* mat, double alpha, double gamma,
+                                    const int* rowp, const int* cols, const int* bcvars,
+                                    double* G, double* A, float* A32, int* status,
+                                    hipStream_t stream) {
+  if (variant < 0 || variant >= jac_rows_nvariants(order)) return TACS_HIP_ERR_ARG;
+  return solid_mat_type_impl(variant, order, nnodes, nelem, n2e_ptr, n2e, conn, X, NULL, telem,
+                             mat, alpha, gamma, 0.0, rowp, cols, bcvars, G, NULL, A, A32, status,
+                             stream);
 }
 
 int tacs_hip_solid_elem_blocks(int order, int prec, int nelem, const int* conn, const double* X,
@@ -1417,18 +1564,26 @@ int tacs_hip_solid_elem_blocks(int order, int prec, int nelem, const int* conn, 
   return (int)hipGetLastError();
 }
 
-int tacs_hip_gather_solid_blocks(int prec, int nrows, const int* rowp, const int* cols,
-                                 const int* cptr, const int* clist, const void* ke,
-                                 const int* bcvars, double* A, float* A32, hipStream_t stream) {
-  if (prec != 64 && prec != 32) return TACS_HIP_ERR_ARG;
+int tacs_hip_gather_solid_blocks(int order, int prec, int nrows, const int* rowp,
+                                 const int* cols, const int* n2e_ptr, const int* n2e,
+                                 const int* conn, const void* ke, const int* bcvars, double* A,
+                                 float* A32, int* status, hipStream_t stream) {
+  if ((order != 2 && order != 3) || (prec != 64 && prec != 32)) return TACS_HIP_ERR_ARG;
   if (nrows <= 0) return 0;
-  const int grid = tacs_grid_for(nrows, 4, 1 << 20);
-  if (prec == 64)
-    hipLaunchKernelGGL(gather_solid_blocks<double>, dim3(grid), dim3(256), 0, stream, nrows, rowp,
-                       cols, cptr, clist, (const double*)ke, bcvars, A, A32);
+#define GS_LAUNCH(NPE, T)                                                                       \
+  hipLaunchKernelGGL((gather_solid_rows<NPE, T>),                                               \
+                     dim3(tacs_grid_for(nrows, GatherCfg<NPE>::WPB, 1 << 20)),                  \
+                     dim3(64 * GatherCfg<NPE>::WPB), 0, stream, nrows, rowp, cols, n2e_ptr, n2e, \
+                     conn, (const T*)ke, bcvars, A, A32, status)
+  if (order == 2 && prec == 64)
+    GS_LAUNCH(8, double);
+  else if (order == 2)
+    GS_LAUNCH(8, float);
+  else if (prec == 64)
+    GS_LAUNCH(27, double);
   else
-    hipLaunchKernelGGL(gather_solid_blocks<float>, dim3(grid), dim3(256), 0, stream, nrows, rowp,
-                       cols, cptr, clist, (const float*)ke, bcvars, A, A32);
+    GS_LAUNCH(27, float);
+#undef GS_LAUNCH
   return (int)hipGetLastError();
 }
 

@@ -3,6 +3,7 @@ library loads and exports every symbol include/tacs_hip.h declares, handles
 are reference counted, and argument errors are reported the reference's way
 (non-zero status + message) without touching a GPU."""
 import ctypes
+import os
 import subprocess
 
 import numpy as np
@@ -204,3 +205,65 @@ def test_cpp_driver_builds_against_the_reference_names():
     r = subprocess.run([drv, "2"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert "set-up failed" in r.stdout and "hipMalloc" in r.stderr
+
+
+def test_tuning_table_host_semantics(tmp_path):
+    """The tuning table is host state: set / get / reset, size classes,
+    argument checks and the JSON loader (no GPU involved)."""
+    import json
+
+    from tacs_amd import TACS
+
+    L = TACS.lib()
+    assert [L.tacs_tune_size_class(r) for r in (1, 32767, 32768, 524287, 524288)] == [0, 0, 1, 1, 2]
+    # built-in defaults: 3x3 FP64 U = 2 (variant 1), FP32 U = 4 (variant 2),
+    # 6x6 U = 1 (0); 27-node Jacobian the element-block form (4)
+    saved = [(k, sh, c, TACS.getTuning(k, sh, c)) for k, sh in ((0, 30), (0, 31), (2, 3))
+             for c in range(3)]
+    TACS.resetTuning()
+    assert L.tacs_tune_variant(0, 30, 10) == 1
+    assert L.tacs_tune_variant(0, 31, 10**6) == 2
+    assert L.tacs_tune_variant(0, 60, 10) == 0
+    assert L.tacs_tune_variant(2, 3, 10) == 4
+    assert L.tacs_tune_variant(2, 2, 10) == 0
+    TACS.setTuning(0, 30, 2, 5)
+    assert TACS.getTuning(0, 30, 2) == 5 and TACS.getTuning(0, 30, 1) == -1
+    assert L.tacs_tune_variant(0, 30, 10**6) == 5 and L.tacs_tune_variant(0, 30, 10) == 1
+    with pytest.raises(TACS.TACSError):
+        TACS.setTuning(0, 30, 3, 1)
+    with pytest.raises(TACS.TACSError):
+        TACS.setTuning(7, 30, 0, 1)
+    path = tmp_path / "table.json"
+    path.write_text(json.dumps({"entries": [
+        {"kernel_id": 0, "shape": 31, "size_class": 1, "variant": 6},
+        {"kernel_id": 1, "shape": 116, "size_class": 2, "variant": 3}]}))
+    assert TACS.loadTuningTable(str(path)) == 2
+    assert TACS.getTuning(0, 31, 1) == 6 and TACS.getTuning(1, 116, 2) == 3
+    assert TACS.loadTuningTable(str(tmp_path / "missing.json")) == 0
+    TACS.resetTuning()
+    assert TACS.getTuning(0, 31, 1) == -1
+    for k, sh, c, v in saved:
+        TACS.setTuning(k, sh, c, v)
+
+
+def test_committed_tuning_table_is_well_formed():
+    """Every entry of the committed table names a valid variant of its
+    kernel (variant counts as the launchers define them)."""
+    import json
+
+    from tacs_amd import TACS
+
+    if not os.path.exists(TACS.TUNING_TABLE):
+        pytest.skip("no tuning table committed")
+    table = json.load(open(TACS.TUNING_TABLE))
+    L = TACS.lib()
+    for e in table["entries"]:
+        k, sh, v = e["kernel_id"], e["shape"], e["variant"]
+        if k == TACS.TUNE_SELL_SPMV:
+            nv = L.tacs_hip_sell_spmv_nvariants(sh // 10)
+        elif k == TACS.TUNE_SPMM:
+            nv = L.tacs_hip_bsr3_spmm_nvariants(sh % 100)
+        else:
+            nv = L.tacs_hip_solid_jacobian_nvariants(sh) + 1
+        assert 0 <= v < nv and 0 <= e["size_class"] < 3, e
+        assert len(e["ms"]) == nv and min(range(nv), key=lambda i: e["ms"][i]) == v, e
