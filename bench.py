@@ -909,10 +909,12 @@ def multi_main(args, L, ev):
         f"{3 * p.nnodes} DOF")
     a = p.assembler
     pc = TACS.Pc(p.mat)
-    ksm = TACS.MixedKSM(p.mat, pc, args.m, 100000, deflation=args.deflation)
+    union = TACS.MixedKSM(p.mat, pc, args.m, 100000, deflation=args.deflation)
+    seg = TACS.SegmentedKSM(p.mat, pc, p.node_off, m=args.m, restarts=100000, prec=32,
+                            deflation=args.deflation)
     rhs = a.createVec()
 
-    def step():
+    def step(kind):
         t = {}
         t0 = time.perf_counter()
         a.zeroVariables()
@@ -920,14 +922,21 @@ def multi_main(args, L, ev):
         L.tacs_device_synchronize()
         t["assembly_s"] = time.perf_counter() - t0
         pc.factor()
-        # each cube's right-hand side scaled to unit norm (the cubes are decoupled,
-        # so cube c's solution scales with it): |r'| <= rtol then gives every
-        # cube |r_c| <= rtol |b_c|
-        p.scale_cubes(p.res, rhs)
-        ksm.setTolerances(1e-30, args.rtol)
-        flag = ksm.solve(rhs, p.u)
-        its = ksm.getIterCount()
-        p.unscale_cubes(p.u)
+        if kind == "segmented":
+            # every cube its own system: own Krylov and recycled space, own
+            # stopping test |r_c| <= rtol |b_c|
+            seg.setTolerances(args.rtol, 1e-30)
+            flag = seg.solve(p.res, p.u)
+            its = [int(v) for v in seg.getStats()[0]]
+        else:
+            # each cube's right-hand side scaled to unit norm (the cubes are
+            # decoupled, so cube c's solution scales with it): |r'| <= rtol then
+            # gives every cube |r_c| <= rtol |b_c|
+            p.scale_cubes(p.res, rhs)
+            union.setTolerances(1e-30, args.rtol)
+            flag = union.solve(rhs, p.u)
+            its = union.getIterCount()
+            p.unscale_cubes(p.u)
         p.u.scale(-1.0)
         a.setVariables(p.u)
         comp = functions.Compliance(a)
@@ -942,24 +951,34 @@ def multi_main(args, L, ev):
         t["step_s"] = time.perf_counter() - t0
         return t, flag, its
 
-    for _ in range(args.warmup):
-        step()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        t, flag, its = step()
-        log(f"step {k}: {t} flag={flag} its={its}")
-    wall = time.perf_counter() - t0
-    ms = 1e3 * wall / args.steps
-    # every cube's true relative residual (host, from the assembled matrix)
-    rowp, cols, A = p.mat.getArrays()
-    u = p.u.getArray().copy()
-    b = -p.res.getArray()
-    Ku = np.zeros_like(u)
-    Ab = A.reshape(-1, 3, 3)
-    rows = np.repeat(np.arange(rowp.size - 1), np.diff(rowp))
-    np.add.at(Ku.reshape(-1, 3), rows, np.einsum("kij,kj->ki", Ab, u.reshape(-1, 3)[cols]))
-    rel = [float(np.linalg.norm((Ku - b)[p.dof(c)]) / np.linalg.norm(b[p.dof(c)]))
-           for c in range(len(sizes))]
+    def cube_residuals():
+        # every cube's true relative residual (host, from the assembled matrix)
+        rowp, cols, A = p.mat.getArrays()
+        u = p.u.getArray().copy()
+        b = -p.res.getArray()
+        Ku = np.zeros_like(u)
+        Ab = A.reshape(-1, 3, 3)
+        rows = np.repeat(np.arange(rowp.size - 1), np.diff(rowp))
+        np.add.at(Ku.reshape(-1, 3), rows, np.einsum("kij,kj->ki", Ab, u.reshape(-1, 3)[cols]))
+        return [float(np.linalg.norm((Ku - b)[p.dof(c)]) / np.linalg.norm(b[p.dof(c)]))
+                for c in range(len(sizes))]
+
+    legs = {}
+    for kind in ("segmented", "union"):
+        for _ in range(args.warmup):
+            step(kind)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            t, flag, its = step(kind)
+            log(f"{kind} step {k}: {t} flag={flag} its={its}")
+        wall = time.perf_counter() - t0
+        legs[kind] = dict(ms=1e3 * wall / args.steps, t=t, flag=flag, its=its,
+                          rel=cube_residuals(), u=p.u.getArray().copy())
+    main_leg = legs[args.multi_solver]
+    ms, t, its, rel = main_leg["ms"], main_leg["t"], main_leg["its"], main_leg["rel"]
+    other = "union" if args.multi_solver == "segmented" else "segmented"
+    u_diff = float(np.abs(legs["segmented"]["u"] - legs["union"]["u"]).max()
+                   / np.abs(legs["union"]["u"]).max())
 
     # cube by cube: one assembler each, the same recycling solver; FP64 GMRES(30)
     t_seq, t_fp64, its_seq, its_fp64 = 0.0, 0.0, [], []
@@ -1002,13 +1021,25 @@ def multi_main(args, L, ev):
         "config": {"workload": f"{len(sizes)} independent 8-node hex unit cubes, edge lengths "
                                f"{sizes} elements, E=70e9 nu=0.3, x=0 clamped, traction 1e6 Pa "
                                f"on x=1; one assembler (block-diagonal union): one assembly "
-                               f"launch, mixed GMRES({args.m}) recycling {args.deflation} on "
-                               f"the union with per-cube unit-norm right-hand sides, "
-                               f"every cube to {args.rtol:g} relative residual, compliance and "
-                               f"thickness gradient",
-                   "elements": p.nelem, "dof": 3 * p.nnodes, "cubes": len(sizes)},
-        "assembly_s": t["assembly_s"], "iterations_union": its,
+                               f"launch, then " + (
+                                   f"every cube its own system (own Krylov and recycled space, "
+                                   f"own stopping test) in one batched mixed GMRES({args.m}) "
+                                   f"recycling {args.deflation} per cube"
+                                   if args.multi_solver == "segmented" else
+                                   f"mixed GMRES({args.m}) recycling {args.deflation} on the "
+                                   f"union with per-cube unit-norm right-hand sides") +
+                               f", every cube to {args.rtol:g} relative residual, compliance "
+                               f"and thickness gradient",
+                   "elements": p.nelem, "dof": 3 * p.nnodes, "cubes": len(sizes),
+                   "solver": args.multi_solver},
+        "assembly_s": t["assembly_s"],
+        "iterations_segmented_per_cube": legs["segmented"]["its"],
+        "iterations_union": legs["union"]["its"],
         "max_cube_rel_residual": max(rel),
+        f"ms_per_step_{other}": legs[other]["ms"],
+        f"max_cube_rel_residual_{other}": max(legs[other]["rel"]),
+        "segmented_vs_union_u_rel_diff": u_diff,
+        "speedup_segmented_vs_union": legs["union"]["ms"] / legs["segmented"]["ms"],
         "cube_by_cube_same_solver_s": t_seq, "cube_by_cube_iterations": its_seq,
         "speedup_vs_cube_by_cube": t_seq / (ms * 1e-3),
     }
@@ -1216,6 +1247,9 @@ def main():
                     help="hex: the 128^3 headline line; shell: the Quad4 plate; batched: the "
                          "hex27 cube with a block of load cases (own JSON lines)")
     ap.add_argument("--nrhs", type=int, default=16, help="load cases of the batched workload")
+    ap.add_argument("--multi-solver", default="segmented", choices=["segmented", "union"],
+                    help="multi workload: every cube its own system (segmented) or one union "
+                         "system")
     ap.add_argument("--order", type=int, default=3, choices=[2, 3],
                     help="matfree workload: 3 (hex27, 40^3 unless --n) or 2 (hex8)")
     ap.add_argument("--ncubes", type=int, default=32, help="cubes of the multi workload")

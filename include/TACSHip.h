@@ -517,7 +517,29 @@ class TACSBatchedGMRES : public TACSObject {
  public:
   TACSBatchedGMRES(TACSParallelMat* mat, TACSBlockJacobiPc* pc, int m, int nrestart, int nrhs,
                    int prec);
+  // Segmented systems: mat is block diagonal with nseg <= 64 diagonal
+  // blocks, block j the node range [node_off[j], node_off[j+1]) (the meshes
+  // of a multi-mesh assembler); each block is solved as its own system, with
+  // its own Krylov space (and recycled space, setDeflation), Hessenberg
+  // matrix and stopping test, all advancing together (solveSegments).  A
+  // block vector holds local node i of system j at (3 i + c) nseg + j, sized
+  // by the largest block.  Rows that couple two blocks are refused.
+  TACSBatchedGMRES(TACSParallelMat* mat, TACSBlockJacobiPc* pc, int m, int nrestart, int nseg,
+                   const int* node_off, int prec);
   ~TACSBatchedGMRES();
+  // x = A^{-1} b for segmented systems (b, x: whole-matrix vectors); each
+  // block stops when its own residual meets rtol |b_j| (or atol); returns 1
+  // when every block converged; iters, flags, r0 and rn are per block
+  int solveSegments(TACSBVec* b, TACSBVec* x, int zero_guess = 1);
+  int nseg = 0;           // segmented systems (0: right-hand sides of one matrix)
+  int segMaxNodes = 0;    // nodes of the largest block
+  int* d_segoff = NULL;   // node_off on the device
+  double *d_ux = NULL, *d_uy = NULL;  // whole-matrix work vectors of the products
+  std::vector<int> segOff;
+  int sysRows() const { return nseg > 0 ? segMaxNodes : mat->nrows; }
+  void allocate();
+  void bdiag(int pr, int Rn, const void* Dinv, const double* x, double* y);
+  void bmult(int pr, int Rn, const float* A32, const double* x, double* y);
   // x[j] = A^{-1} b[j], j < nrhs <= R; returns 1 when every system converged
   int solve(int nrhs, TACSBVec** b, TACSBVec** x, int zero_guess = 1);
   // residual monitor: at the start of each cycle, the true residual norm of

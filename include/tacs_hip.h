@@ -498,6 +498,34 @@ int tacs_hip_bbacksolve(int R, int m, const int* cnt, const double* H, size_t sH
                         size_t sRes, hipStream_t stream);
 int tacs_hip_bdiag_apply(int b, int prec, int nrows, int R, const void* Dinv, const double* x,
                          double* y, hipStream_t stream);
+
+/* Segmented systems (the R <= 64 diagonal blocks of a block-diagonal 3x3
+   BCSR matrix, block j = node range [off[j], off[j+1]), solved as R
+   independent systems): a segmented block vector keeps local node i of
+   system j at X[(3 i + c) R + j], i < maxn (the largest block); entries
+   beyond a system's size are zero.  No counterpart in the reference (one
+   assembler and one GMRES per mesh there).
+   seg_pack:    X = the segmented form of the union vector x;
+   seg_unpack:  x (nnodes = off[R] nodes) from X;
+   seg_bdiag_apply: y = D^{-1} x per local node, D^{-1} the union's inverse
+                diagonal blocks (prec 64 / 32);
+   seg_spmm3:   Y = A X with the union's rows and values (prec 64 / 32),
+                every column of a row of block j inside block j. */
+int tacs_hip_seg_pack(int maxn, int R, const int* off, const double* x, double* X,
+                      hipStream_t stream);
+int tacs_hip_seg_unpack(int nnodes, int R, const int* off, const double* X, double* x,
+                        hipStream_t stream);
+int tacs_hip_seg_bdiag_apply(int prec, int maxn, int R, const int* off, const void* Dinv,
+                             const double* x, double* y, hipStream_t stream);
+/* seg_to_union / seg_from_union: the same conversions as unpack / pack,
+   through LDS tiles of 32 local nodes (coalesced on both sides). */
+int tacs_hip_seg_to_union(int maxn, int R, const int* off, const double* X, double* x,
+                          hipStream_t stream);
+int tacs_hip_seg_from_union(int maxn, int R, const int* off, const double* x, double* X,
+                            hipStream_t stream);
+int tacs_hip_seg_spmm3(int prec, int maxn, int R, const int* off, const int* rowp,
+                       const int* cols, const void* A, const double* X, double* Y,
+                       hipStream_t stream);
 int tacs_hip_bsr3_spmm_mixed(int nrows, int nrhs, const int* rowp, const int* cols, const float* A,
                              const double* X, double* Y, hipStream_t stream);
 /* Launch variants of tacs_hip_bsr3_spmm / _mixed (prec_vals 64 or 32), all
@@ -796,6 +824,18 @@ int tacs_batched_gmres_solve_block(tacs_obj ksm, int nrhs, const double* B, doub
                                    int zero_guess);
 int tacs_batched_gmres_get_stats(tacs_obj ksm, int nrhs, int* iters, int* flags, double* r0,
                                  double* rn);
+/* Segmented systems (no counterpart in the reference, which builds one
+   assembler and one GMRES per mesh): mat block diagonal with nseg <= 64
+   diagonal blocks, block j the node range [node_off[j], node_off[j+1])
+   (node_off[0] = 0, node_off[nseg] = rows; a row coupling two blocks is
+   refused).  Each block is its own system with its own Krylov space,
+   recycled space (set_deflation), Hessenberg matrix and stopping test
+   |r_j| <= rtol |r_j(0)| (or atol); all advance together and a converged
+   block stops counting steps.  solve_segments takes whole-matrix vectors;
+   get_stats reports per block.  Returns 1 when every block converged. */
+tacs_obj tacs_batched_gmres_create_segments(tacs_obj mat, tacs_obj pc, int m, int nrestart,
+                                            int nseg, const int* node_off, int prec);
+int tacs_batched_gmres_solve_segments(tacs_obj ksm, tacs_obj b, tacs_obj x, int zero_guess);
 int tacs_ksm_set_tolerances(tacs_obj ksm, double rtol, double atol);
 int tacs_ksm_solve(tacs_obj ksm, tacs_obj b, tacs_obj x, int zero_guess);
 int tacs_ksm_get_iter_count(tacs_obj ksm);

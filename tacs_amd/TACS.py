@@ -389,6 +389,33 @@ class BatchedKSM(TACSHandle):
         return it, fl, r0, rn, cyc
 
 
+class SegmentedKSM(BatchedKSM):
+    """Independent systems on the diagonal blocks of one block-diagonal 3x3
+    matrix (the meshes of a multi-mesh assembler): block j is the node range
+    [node_off[j], node_off[j+1]) and is solved as its own system, with its
+    own Krylov space, recycled space (deflation > 0, GCRO-DR), Hessenberg
+    matrix and stopping test, all advancing together (one batched step
+    serves every block; a converged block stops counting steps).  At most 64
+    blocks; rows coupling two blocks are refused.  No counterpart in the
+    reference, which builds one assembler and one GMRES per mesh."""
+
+    def __init__(self, mat, pc, node_off, m=60, restarts=100000, prec=32, deflation=20):
+        off = np.ascontiguousarray(node_off, dtype=np.int32)
+        TACSHandle.__init__(self, lib().tacs_batched_gmres_create_segments(
+            mat.ptr, pc.ptr, m, restarts, off.size - 1, off.ctypes.data, prec), keep=(mat, pc))
+        self.nrhs = off.size - 1
+        if deflation:
+            self.setDeflation(deflation)
+
+    def solve(self, b, x, zero_guess=1):
+        """x = A^{-1} b block by block; returns 1 when every block converged
+        (getStats: per block)."""
+        flag = lib().tacs_batched_gmres_solve_segments(self.ptr, b.ptr, x.ptr, zero_guess)
+        self._raise_on_error(flag, "SegmentedKSM.solve")
+        self._last = self.nrhs
+        return flag
+
+
 class RecyclingKSM:
     """Krylov subspace recycling (GCRO-DR with delayed classical Gram-Schmidt)
     for one right-hand side through the KSM interface (solve, setTolerances,

@@ -900,6 +900,141 @@ static inline int bgrid(size_t n) {
   return (int)g;
 }
 
+// ---------------------------------------------------------------------------
+// Segmented systems: the R diagonal blocks of a block-diagonal 3x3-block
+// matrix, block j being the node range [off[j], off[j+1]) of one BCSR matrix
+// (the meshes of a multi-mesh assembler, no coupling between them), solved as
+// R independent systems by the batched solver.  A segmented block vector
+// keeps local node i of system j at X[(3 i + c) R + j], i < maxn = the
+// largest system; entries of local nodes beyond a system's size are zero.
+// Every column of a row of block j must lie in block j (the host checks).
+// ---------------------------------------------------------------------------
+__global__ void seg_pack(int maxn, int R, const int* __restrict__ off, const double* __restrict__ x,
+                         double* __restrict__ X) {
+  const size_t total = (size_t)3 * maxn * R;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(t % R);
+    const size_t ic = t / R;
+    const int i = (int)(ic / 3), c = (int)(ic - 3 * (size_t)i);
+    const int nj = off[j + 1] - off[j];
+    X[t] = i < nj ? x[3 * ((size_t)off[j] + i) + c] : 0.0;
+  }
+}
+
+__global__ void seg_unpack(int nnodes, int R, const int* __restrict__ off,
+                           const double* __restrict__ X, double* __restrict__ x) {
+  const size_t total = (size_t)3 * nnodes;
+  for (size_t d = (size_t)blockIdx.x * blockDim.x + threadIdx.x; d < total;
+       d += (size_t)gridDim.x * blockDim.x) {
+    const int node = (int)(d / 3), c = (int)(d - 3 * (size_t)node);
+    int lo = 0, hi = R - 1;  // the system whose range holds node
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= node) lo = mid;
+      else hi = mid - 1;
+    }
+    x[d] = X[((size_t)3 * (node - off[lo]) + c) * R + lo];
+  }
+}
+
+// Segmented block vector <-> whole-matrix vector through an LDS tile of SEG_T
+// local nodes x R systems: the segmented side is read / written as one
+// contiguous run of 3 SEG_T R values, the whole-matrix side as R runs of
+// 3 SEG_T values (one per system).  to_union: x[3 (off[j] + i) + c] =
+// X[(3 i + c) R + j]; from_union: X = the segmented form of x (zeros beyond
+// a system's size).
+#define SEG_T 32
+template <bool TO_UNION>
+__global__ __launch_bounds__(256) void seg_transpose(int maxn, int R, const int* __restrict__ off,
+                                                      const double* __restrict__ src,
+                                                      double* __restrict__ dst) {
+  __shared__ double tile[3 * SEG_T * 64];
+  const int i0 = blockIdx.x * SEG_T;
+  const int ni = min(SEG_T, maxn - i0);
+  const int nv = 3 * ni * R;  // values of the tile
+  const size_t base = (size_t)3 * i0 * R;
+  if (TO_UNION) {
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) tile[t] = src[base + t];
+    __syncthreads();
+    // system j's run: local values 3 i + c, i < min(ni, n_j - i0)
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) {
+      const int j = t / (3 * ni), q = t - j * 3 * ni;  // q = 3 (i - i0) + c
+      const int nj = off[j + 1] - off[j];
+      if (i0 + q / 3 < nj) dst[3 * ((size_t)off[j] + i0) + q] = tile[q * R + j];
+    }
+  } else {
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) {
+      const int j = t / (3 * ni), q = t - j * 3 * ni;
+      const int nj = off[j + 1] - off[j];
+      tile[q * R + j] = i0 + q / 3 < nj ? src[3 * ((size_t)off[j] + i0) + q] : 0.0;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) dst[base + t] = tile[t];
+  }
+}
+
+// y = D^{-1} x per local node (i, j): the inverse diagonal block of union
+// node off[j] + i; zero beyond a system's size.
+template <typename TD>
+__global__ void seg_bdiag_apply(int maxn, int R, const int* __restrict__ off,
+                                const TD* __restrict__ Dinv, const double* __restrict__ x,
+                                double* __restrict__ y) {
+  const size_t total = (size_t)maxn * R;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(t % R), i = (int)(t / R);
+    const int nj = off[j + 1] - off[j];
+    double* yo = &y[(size_t)3 * i * R + j];
+    if (i >= nj) {
+      yo[0] = 0.0;
+      yo[R] = 0.0;
+      yo[2 * R] = 0.0;
+      continue;
+    }
+    const TD* d = &Dinv[9 * ((size_t)off[j] + i)];
+    const double* xi = &x[(size_t)3 * i * R + j];
+    const double x0 = xi[0], x1 = xi[R], x2 = xi[2 * R];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      yo[(size_t)r * R] = (double)d[3 * r] * x0 + (double)d[3 * r + 1] * x1 + (double)d[3 * r + 2] * x2;
+  }
+}
+
+// Y = A X per local row (i, j): the blocks of union row off[j] + i in slot
+// order (FP64 sums; TA double or float values), columns taken relative to
+// off[j]; zero beyond a system's size.  Threads take j fastest, so the
+// writes of the R systems of a local node are contiguous.
+template <typename TA>
+__global__ void seg_spmm3(int maxn, int R, const int* __restrict__ off,
+                          const int* __restrict__ rowp, const int* __restrict__ cols,
+                          const TA* __restrict__ A, const double* __restrict__ X,
+                          double* __restrict__ Y) {
+  const size_t total = (size_t)maxn * R;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(t % R), i = (int)(t / R);
+    const int o = off[j], nj = off[j + 1] - o;
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    if (i < nj) {
+      const int row = o + i;
+      for (int s = rowp[row]; s < rowp[row + 1]; s++) {
+        const int cl = cols[s] - o;
+        const TA* a = &A[(size_t)9 * s];
+        const double* xc = &X[(size_t)3 * cl * R + j];
+        const double x0 = xc[0], x1 = xc[R], x2 = xc[2 * R];
+        y0 += (double)a[0] * x0 + (double)a[1] * x1 + (double)a[2] * x2;
+        y1 += (double)a[3] * x0 + (double)a[4] * x1 + (double)a[5] * x2;
+        y2 += (double)a[6] * x0 + (double)a[7] * x1 + (double)a[8] * x2;
+      }
+    }
+    double* yo = &Y[(size_t)3 * i * R + j];
+    yo[0] = y0;
+    yo[R] = y1;
+    yo[2 * R] = y2;
+  }
+}
+
 extern "C" {
 
 size_t tacs_hip_batch_scratch_bytes(int R) {
@@ -1171,6 +1306,71 @@ int tacs_hip_bpack(size_t n, int R, int j, const double* x, double* X, hipStream
 
 int tacs_hip_bunpack(size_t n, int R, int j, const double* X, double* x, hipStream_t stream) {
   hipLaunchKernelGGL(bpack, dim3(bgrid(n)), dim3(256), 0, stream, n, R, j, X, x, 1);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_seg_pack(int maxn, int R, const int* off, const double* x, double* X,
+                      hipStream_t stream) {
+  if (R < 1 || R > 64 || maxn < 0) return TACS_HIP_ERR_ARG;
+  const size_t total = (size_t)3 * maxn * R;
+  if (total == 0) return 0;
+  hipLaunchKernelGGL(seg_pack, dim3(bgrid(total)), dim3(256), 0, stream, maxn, R, off, x, X);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_seg_unpack(int nnodes, int R, const int* off, const double* X, double* x,
+                        hipStream_t stream) {
+  if (R < 1 || R > 64 || nnodes < 0) return TACS_HIP_ERR_ARG;
+  if (nnodes == 0) return 0;
+  hipLaunchKernelGGL(seg_unpack, dim3(bgrid((size_t)3 * nnodes)), dim3(256), 0, stream, nnodes, R,
+                     off, X, x);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_seg_to_union(int maxn, int R, const int* off, const double* X, double* x,
+                          hipStream_t stream) {
+  if (R < 1 || R > 64 || maxn < 0) return TACS_HIP_ERR_ARG;
+  if (maxn == 0) return 0;
+  hipLaunchKernelGGL(seg_transpose<true>, dim3((maxn + SEG_T - 1) / SEG_T), dim3(256), 0, stream,
+                     maxn, R, off, X, x);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_seg_from_union(int maxn, int R, const int* off, const double* x, double* X,
+                            hipStream_t stream) {
+  if (R < 1 || R > 64 || maxn < 0) return TACS_HIP_ERR_ARG;
+  if (maxn == 0) return 0;
+  hipLaunchKernelGGL(seg_transpose<false>, dim3((maxn + SEG_T - 1) / SEG_T), dim3(256), 0, stream,
+                     maxn, R, off, x, X);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_seg_bdiag_apply(int prec, int maxn, int R, const int* off, const void* Dinv,
+                             const double* x, double* y, hipStream_t stream) {
+  if (R < 1 || R > 64 || maxn < 0 || (prec != 64 && prec != 32)) return TACS_HIP_ERR_ARG;
+  const size_t total = (size_t)maxn * R;
+  if (total == 0) return 0;
+  if (prec == 64)
+    hipLaunchKernelGGL(seg_bdiag_apply<double>, dim3(bgrid(total)), dim3(256), 0, stream, maxn, R,
+                       off, (const double*)Dinv, x, y);
+  else
+    hipLaunchKernelGGL(seg_bdiag_apply<float>, dim3(bgrid(total)), dim3(256), 0, stream, maxn, R,
+                       off, (const float*)Dinv, x, y);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_seg_spmm3(int prec, int maxn, int R, const int* off, const int* rowp,
+                       const int* cols, const void* A, const double* X, double* Y,
+                       hipStream_t stream) {
+  if (R < 1 || R > 64 || maxn < 0 || (prec != 64 && prec != 32)) return TACS_HIP_ERR_ARG;
+  const size_t total = (size_t)maxn * R;
+  if (total == 0) return 0;
+  if (prec == 64)
+    hipLaunchKernelGGL(seg_spmm3<double>, dim3(bgrid(total)), dim3(256), 0, stream, maxn, R, off,
+                       rowp, cols, (const double*)A, X, Y);
+  else
+    hipLaunchKernelGGL(seg_spmm3<float>, dim3(bgrid(total)), dim3(256), 0, stream, maxn, R, off,
+                       rowp, cols, (const float*)A, X, Y);
   return (int)hipGetLastError();
 }
 

@@ -720,6 +720,30 @@ tacs_obj tacs_batched_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestar
   }
   return built(new TACSBatchedGMRES(A, p, m, nrestart, nrhs, prec));
 }
+tacs_obj tacs_batched_gmres_create_segments(tacs_obj mat, tacs_obj pc, int m, int nrestart,
+                                            int nseg, const int* node_off, int prec) {
+  clear_error();
+  REQUIRE_NULL(TACSParallelMat, A, mat);
+  REQUIRE_NULL(TACSBlockJacobiPc, p, pc);
+  if (A->bsize != 3 || m < 1 || m > 512 || nseg < 1 || nseg > 64 || !node_off ||
+      (prec != 32 && prec != 64) || nrestart < 0) {
+    tacs_set_error("segmented batched GMRES: need 3x3 blocks, 1 <= m <= 512, 1 <= nseg <= 64, "
+                   "node offsets, prec 32 or 64, nrestart >= 0");
+    return NULL;
+  }
+  return built(new TACSBatchedGMRES(A, p, m, nrestart, nseg, node_off, prec));
+}
+int tacs_batched_gmres_solve_segments(tacs_obj ksm, tacs_obj b, tacs_obj x, int zero_guess) {
+  clear_error();
+  TACSBatchedGMRES* k = OBJ(TACSBatchedGMRES, ksm);
+  TACSBVec* bv = OBJ(TACSBVec, b);
+  TACSBVec* xv = OBJ(TACSBVec, x);
+  if (!k || !bv || !xv) {
+    tacs_set_error("%s: expected a TACSBatchedGMRES handle and two TACSBVec", __func__);
+    return -1;
+  }
+  return k->solveSegments(bv, xv, zero_guess);
+}
 int tacs_batched_gmres_set_tolerances(tacs_obj ksm, double rtol, double atol) {
   REQUIRE(TACSBatchedGMRES, k, ksm);
   k->setTolerances(rtol, atol);

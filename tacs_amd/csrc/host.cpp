@@ -11,8 +11,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/TACSHip.h"
@@ -28,6 +30,17 @@ static thread_local std::string g_err;
 // working through a NULL buffer.  The C ABI clears it at the start of each
 // call; C++ callers clear it with tacs_clear_error().
 static thread_local int g_halt = 0;
+
+// Host threads for independent per-system work (at most 16, the share of a
+// GPU's host cores; TACS_HOST_THREADS overrides).
+static int host_threads() {
+  static int n = [] {
+    const char* e = getenv("TACS_HOST_THREADS");
+    int v = e ? atoi(e) : (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(v, 16));
+  }();
+  return n;
+}
 
 int tacs_set_error(const char* fmt, ...) {
   char buf[1024];
@@ -1398,10 +1411,63 @@ TACSBatchedGMRES::TACSBatchedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int
       ev_up(NULL), up_pending(0), d_lsst(NULL), d_ist(NULL), d_kdv(NULL), d_tgtv(NULL),
       h_flags(NULL), d_flags(NULL), ev_step{NULL, NULL, NULL}, d_dq(NULL),
       delayed(1), mat(m), pc(p) {
+  allocate();
+}
+// Segmented systems: the nseg diagonal blocks (node ranges [node_off[j],
+// node_off[j+1])) of the block-diagonal matrix m, each solved as its own
+// system with its own Krylov space and stopping test.
+TACSBatchedGMRES::TACSBatchedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int _m, int _nrestart,
+                                   int _nseg, const int* node_off, int _prec)
+    : rtol(1e-8), atol(1e-30), msub(_m), nrestart(_nrestart), R(_nseg), prec(_prec), cycles(0),
+      kdefl(0), kVersion(-1), d_U(NULL), d_C(NULL), ndefl(0), defVersion(-1), defR(0),
+      d_Ub(NULL), d_Cb(NULL), d_Unb(NULL), d_Cnb(NULL), d_prod(NULL), h_stage(NULL), h_up(NULL),
+      ev_up(NULL), up_pending(0), d_lsst(NULL), d_ist(NULL), d_kdv(NULL), d_tgtv(NULL),
+      h_flags(NULL), d_flags(NULL), ev_step{NULL, NULL, NULL}, d_dq(NULL),
+      delayed(1), mat(m), pc(p) {
+  // every column of a row of system j inside system j: the segmented
+  // kernels index x relative to the system's first node
+  if (R < 1 || R > 64 || !node_off || node_off[0] != 0 || node_off[R] != m->nrows ||
+      m->bsize != 3) {
+    tacs_set_error("TACSBatchedGMRES: segments need 1 <= nseg <= 64 ranges covering the %d rows "
+                   "of a 3x3-block matrix",
+                   m->nrows);
+    R = 0;
+  } else {
+    for (int j = 0; j < R && R > 0; j++) {
+      if (node_off[j + 1] < node_off[j]) {
+        tacs_set_error("TACSBatchedGMRES: segment %d has a negative size", j);
+        R = 0;
+        break;
+      }
+      for (int r = node_off[j]; r < node_off[j + 1] && R > 0; r++)
+        for (int s = m->h_rowp[r]; s < m->h_rowp[r + 1]; s++)
+          if (m->h_cols[s] < node_off[j] || m->h_cols[s] >= node_off[j + 1]) {
+            tacs_set_error("TACSBatchedGMRES: row %d of segment %d couples to node %d outside it",
+                           r, j, m->h_cols[s]);
+            R = 0;
+            break;
+          }
+    }
+  }
+  if (R > 0) {
+    nseg = R;
+    segOff.assign(node_off, node_off + R + 1);
+    segMaxNodes = 0;
+    for (int j = 0; j < R; j++) segMaxNodes = std::max(segMaxNodes, node_off[j + 1] - node_off[j]);
+    d_segoff = dalloc<int>(R + 1);
+    h2d(d_segoff, node_off, R + 1);
+    d_ux = dalloc<double>((size_t)m->nrows * 3);
+    d_uy = dalloc<double>((size_t)m->nrows * 3);
+  } else {
+    R = 1;  // a valid (unused) size: the object is destroyed by the C ABI
+  }
+  allocate();
+}
+void TACSBatchedGMRES::allocate() {
   mat->incref();
   pc->incref();
   if (prec == 32) mat->getValues32(0);
-  size_t N = (size_t)mat->nrows * mat->bsize * R;
+  size_t N = (size_t)sysRows() * mat->bsize * R;
   d_V = dalloc<double>((size_t)(msub + 1) * N);
   d_work = dalloc<double>(N);
   d_Bb = dalloc<double>(N);
@@ -1417,7 +1483,7 @@ TACSBatchedGMRES::TACSBatchedGMRES(TACSParallelMat* m, TACSBlockJacobiPc* p, int
 TACSBatchedGMRES::~TACSBatchedGMRES() {
   HCHECK(hipStreamSynchronize(g_stream));
   void* bufs[] = {d_V, d_work, d_Bb,  d_Xb,  d_scal, d_vec, d_cnt, d_red,
-                  d_U, d_C,    d_Ub,  d_Cb,  d_Unb,  d_Cnb, d_prod, d_dq};
+                  d_U, d_C,    d_Ub,  d_Cb,  d_Unb,  d_Cnb, d_prod, d_dq, d_segoff, d_ux, d_uy};
   for (void* q : bufs) dfree(q);
   hfree(h_pin);
   if (h_stage) hfree(h_stage);
@@ -1438,6 +1504,10 @@ void TACSBatchedGMRES::setTolerances(double r, double a) {
 }
 
 int TACSBatchedGMRES::setRecycleSpace(TACSMixedGMRES* src) {
+  if (nseg > 0) {
+    tacs_set_error("setRecycleSpace: not for segmented systems");
+    return -1;
+  }
   HCHECK(hipStreamSynchronize(g_stream));
   dfree(d_U);
   dfree(d_C);
@@ -1501,7 +1571,7 @@ int TACSBatchedGMRES::setDeflation(int k) {
   kdj.assign(R, 0);
   dsc.assign((size_t)R * KD_MAX, 0.0);
   if (k > 0) {
-    size_t N = (size_t)mat->nrows * mat->bsize * R;
+    size_t N = (size_t)sysRows() * mat->bsize * R;
     d_Ub = dalloc<double>((size_t)k * N);
     d_Cb = dalloc<double>((size_t)k * N);
     d_Unb = dalloc<double>((size_t)k * N);
@@ -1531,7 +1601,54 @@ int TACSBatchedGMRES::setDeflation(int k) {
   return 0;
 }
 
+int TACSBatchedGMRES::solveSegments(TACSBVec* b, TACSBVec* x, int zero_guess) {
+  if (nseg <= 0) {
+    tacs_set_error("solveSegments: the solver was not built for segmented systems");
+    return -1;
+  }
+  const size_t n = (size_t)mat->nrows * 3;
+  if (b->n != n || x->n != n) {
+    tacs_set_error("solveSegments: vectors of %zu entries expected", n);
+    return -1;
+  }
+  KCHECK(tacs_hip_seg_pack(segMaxNodes, R, d_segoff, b->devRead(), d_Bb, g_stream));
+  if (!zero_guess) KCHECK(tacs_hip_seg_pack(segMaxNodes, R, d_segoff, x->devRead(), d_Xb, g_stream));
+  int flag = solveBlock(R, d_Bb, d_Xb, zero_guess);
+  KCHECK(tacs_hip_seg_unpack(mat->nrows, R, d_segoff, d_Xb, x->devWrite(), g_stream));
+  HCHECK(hipStreamSynchronize(g_stream));
+  return flag;
+}
+// Block-Jacobi and products of the batched cycles: the whole matrix for every
+// right-hand side, or the segmented systems' own blocks
+void TACSBatchedGMRES::bdiag(int pr, int Rn, const void* Dinv, const double* x, double* y) {
+  if (nseg > 0)
+    KCHECK(tacs_hip_seg_bdiag_apply(pr, segMaxNodes, Rn, d_segoff, Dinv, x, y, g_stream));
+  else
+    KCHECK(tacs_hip_bdiag_apply(mat->bsize, pr, mat->nrows, Rn, Dinv, x, y, g_stream));
+}
+void TACSBatchedGMRES::bmult(int pr, int Rn, const float* A32, const double* x, double* y) {
+  if (nseg > 0) {
+    // through a whole-matrix vector: the matrix's own sliced ELL product
+    KCHECK(tacs_hip_seg_to_union(segMaxNodes, Rn, d_segoff, x, d_ux, g_stream));
+    if (pr == 32)
+      mat->multMixed(d_ux, d_uy);
+    else
+      mat->mult64(d_ux, d_uy);
+    KCHECK(tacs_hip_seg_from_union(segMaxNodes, Rn, d_segoff, d_uy, y, g_stream));
+  } else if (Rn == 1 && pr == 32)
+    mat->multMixed(x, y);
+  else if (Rn == 1)
+    mat->mult64(x, y);
+  else if (pr == 32)
+    KCHECK(tacs_hip_bsr3_spmm_mixed(mat->nrows, Rn, mat->d_rowp, mat->d_cols, A32, x, y, g_stream));
+  else
+    KCHECK(tacs_hip_bsr3_spmm(mat->nrows, Rn, mat->d_rowp, mat->d_cols, mat->d_A, x, y, g_stream));
+}
 int TACSBatchedGMRES::solve(int nrhs, TACSBVec** b, TACSBVec** x, int zero_guess) {
+  if (nseg > 0) {
+    tacs_set_error("TACSBatchedGMRES::solve: segmented systems are solved with solveSegments");
+    return -1;
+  }
   if (nrhs < 1 || nrhs > R) {
     tacs_set_error("TACSBatchedGMRES::solve: nrhs = %d outside [1, %d]", nrhs, R);
     return -1;
@@ -1563,8 +1680,12 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
                    mat->bsize);
     return -1;
   }
+  if (nseg > 0 && Rn != nseg) {
+    tacs_set_error("TACSBatchedGMRES: %d segmented systems, not %d", nseg, Rn);
+    return -1;
+  }
   if (ndefl > 0) return solveBlockDeflated(Rn, Bd, Xd, zero_guess);
-  const int nr = mat->nrows, m = msub;
+  const int nr = sysRows(), m = msub;
   const size_t n = (size_t)nr * mat->bsize, N = n * Rn, S = slice();
   // per right-hand side slice (stride S)
   double* H = d_scal;
@@ -1583,22 +1704,11 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
   const float* A32 = prec == 32 ? mat->getValues32(1) : NULL;
   const void* Dinv = prec == 32 ? (const void*)pc->d_Dinv32 : (const void*)pc->d_Dinv;
   const int kd = (kdefl > 0 && kVersion == mat->version) ? kdefl : 0;
-  // one right-hand side: the single-vector products
-  auto spmm64 = [&](const double* x, double* y) {
-    if (Rn == 1)
-      mat->mult64(x, y);
-    else
-      KCHECK(tacs_hip_bsr3_spmm(nr, Rn, mat->d_rowp, mat->d_cols, mat->d_A, x, y, g_stream));
-  };
+  auto spmm64 = [&](const double* x, double* y) { bmult(64, Rn, NULL, x, y); };
   // y = A M^{-1} x with the Arnoldi operator of this precision
   auto op = [&](const double* x, double* y) {
-    KCHECK(tacs_hip_bdiag_apply(mat->bsize, prec, nr, Rn, Dinv, x, d_work, g_stream));
-    if (prec == 32 && Rn == 1)
-      mat->multMixed(d_work, y);
-    else if (prec == 32)
-      KCHECK(tacs_hip_bsr3_spmm_mixed(nr, Rn, mat->d_rowp, mat->d_cols, A32, d_work, y, g_stream));
-    else
-      spmm64(d_work, y);
+    bdiag(prec, Rn, Dinv, x, d_work);
+    bmult(prec, Rn, A32, d_work, y);
   };
   // q(:, j) /= |q(:, j)|, with |q(:, j)| written to out[j*sOut]
   auto normalize = [&](double* q, double* out, size_t sOut) {
@@ -1657,7 +1767,7 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
       // x += |r| M^{-1} U C^T v_0;  v_0 -= C C^T v_0
       KCHECK(tacs_hip_bmdot_shared(n, Rn, kd, d_C, n, V(0), (double*)d_red, pco, S, 0, g_stream));
       KCHECK(tacs_hip_blincomb_shared(n, Rn, kd, d_U, n, pco, S, V(1), 0, g_stream));
-      KCHECK(tacs_hip_bdiag_apply(mat->bsize, prec, nr, Rn, Dinv, V(1), d_work, g_stream));
+      bdiag(prec, Rn, Dinv, V(1), d_work);
       KCHECK(tacs_hip_bscale(n, Rn, nrm, d_work, g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, d_work, 1.0, Xd, g_stream));
       KCHECK(tacs_hip_bcgs_update_shared(n, Rn, kd, d_C, n, pco, S, V(0), g_stream));
@@ -1733,7 +1843,7 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
         KCHECK(tacs_hip_bsmall_gemv(Rn, kd, d_cnt, Bc, S, KD_MAX, res, S, zc, S, g_stream));
         KCHECK(tacs_hip_bcgs_update_shared(n, Rn, kd, d_U, n, zc, S, d_work, g_stream));
       }
-      KCHECK(tacs_hip_bdiag_apply(mat->bsize, prec, nr, Rn, Dinv, d_work, V(1), g_stream));
+      bdiag(prec, Rn, Dinv, d_work, V(1));
       KCHECK(tacs_hip_bscale(n, Rn, nrm, V(1), g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, V(1), 1.0, Xd, g_stream));
     }
@@ -1758,7 +1868,7 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
 // A M^{-1} U_j = C_j holds across cycles and solves.
 // ---------------------------------------------------------------------------
 int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, int zero_guess) {
-  const int nr = mat->nrows, m = msub, K = ndefl;
+  const int nr = sysRows(), m = msub, K = ndefl;
   const size_t n = (size_t)nr * mat->bsize, N = n * Rn, S = slice(), SP = prodStride();
   double* H = d_scal;
   double* res = d_scal + (size_t)(m + 1) * m;
@@ -1781,21 +1891,10 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
   bool fell_back = false;
   const float* A32 = prec == 32 ? mat->getValues32(1) : NULL;
   const void* Dinv = prec == 32 ? (const void*)pc->d_Dinv32 : (const void*)pc->d_Dinv;
-  // one right-hand side: the single-vector block-SpMV kernels
-  auto spmm64 = [&](const double* x, double* y) {
-    if (Rn == 1)
-      mat->mult64(x, y);
-    else
-      KCHECK(tacs_hip_bsr3_spmm(nr, Rn, mat->d_rowp, mat->d_cols, mat->d_A, x, y, g_stream));
-  };
+  auto spmm64 = [&](const double* x, double* y) { bmult(64, Rn, NULL, x, y); };
   auto op = [&](const double* x, double* y) {
-    KCHECK(tacs_hip_bdiag_apply(mat->bsize, pr, nr, Rn, Dinv, x, d_work, g_stream));
-    if (pr == 32 && Rn == 1)
-      mat->multMixed(d_work, y);
-    else if (pr == 32)
-      KCHECK(tacs_hip_bsr3_spmm_mixed(nr, Rn, mat->d_rowp, mat->d_cols, A32, d_work, y, g_stream));
-    else
-      spmm64(d_work, y);
+    bdiag(pr, Rn, Dinv, x, d_work);
+    bmult(pr, Rn, A32, d_work, y);
   };
   auto normalize = [&](double* q, double* out, size_t sOut) {
     KCHECK(tacs_hip_bmdot(n, Rn, 1, q, N, q, (double*)d_red, out, sOut, 0, g_stream));
@@ -2148,7 +2247,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       // x += |r| M^{-1} U C^T v_0;  v_0 -= C C^T v_0
       KCHECK(tacs_hip_bmdot(n, Rn, Kd, Cb(0), N, V(0), (double*)d_red, pco, S, 0, g_stream));
       KCHECK(tacs_hip_blincomb(n, Rn, Kd, Ub(0), N, pco, S, NULL, d_work, 0, g_stream));
-      KCHECK(tacs_hip_bdiag_apply(mat->bsize, pr, nr, Rn, Dinv, d_work, tmp2, g_stream));
+      bdiag(pr, Rn, Dinv, d_work, tmp2);
       KCHECK(tacs_hip_bscale(n, Rn, nrm, tmp2, g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, tmp2, 1.0, Xd, g_stream));
       KCHECK(tacs_hip_bcgs_update(n, Rn, Kd, Cb(0), N, pco, S, V(0), g_stream));
@@ -2255,7 +2354,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       h2d(inv, h_inv, Rn);
       if (Kd > 0) KCHECK(tacs_hip_blincomb(n, Rn, Kd, Ub(0), N, zc, S, NULL, d_work, 0, g_stream));
       KCHECK(tacs_hip_blincomb(n, Rn, niters, V(0), N, res, S, NULL, d_work, Kd > 0, g_stream));
-      KCHECK(tacs_hip_bdiag_apply(mat->bsize, pr, nr, Rn, Dinv, d_work, tmp2, g_stream));
+      bdiag(pr, Rn, Dinv, d_work, tmp2);
       KCHECK(tacs_hip_bscale(n, Rn, inv, tmp2, g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, tmp2, 1.0, Xd, g_stream));
     }
@@ -2281,23 +2380,39 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       }
       std::vector<int> kk(Rn, 0);
       std::vector<std::vector<double>> Qm(Rn), PRu(Rn), PRv(Rn);
-      std::vector<double> cvtu;
-      int newKd = 0;
-      for (int j = 0; j < Rn; j++) {
+      // the systems' harmonic Ritz updates are independent: host threads
+      // take them in turn
+      auto update_one = [&](int j, std::vector<double>& cvtu) {
         const int kd = kdj[j];
-        if (cand[j]) {
-          const int nrw = kd + jend[j] + 1;
-          cvtu.assign((size_t)nrw * std::max(kd, 1), 0.0);
-          for (int b = 0; b < kd; b++) {
-            const double* src = &prodh[(size_t)j * p * Kd + (size_t)b * p];
-            for (int r = 0; r < kd; r++) cvtu[(size_t)b * nrw + r] = src[r];
-            for (int v = 0; v <= jend[j]; v++) cvtu[(size_t)b * nrw + kd + v] = src[Kd + v];
-          }
-          kk[j] = recycle_update(ls[j], kd, jend[j], K, &dsc[(size_t)j * KD_MAX], cvtu.data(),
-                                 Qm[j], PRu[j], PRv[j]);
+        const int nrw = kd + jend[j] + 1;
+        cvtu.assign((size_t)nrw * std::max(kd, 1), 0.0);
+        for (int b = 0; b < kd; b++) {
+          const double* src = &prodh[(size_t)j * p * Kd + (size_t)b * p];
+          for (int r = 0; r < kd; r++) cvtu[(size_t)b * nrw + r] = src[r];
+          for (int v = 0; v <= jend[j]; v++) cvtu[(size_t)b * nrw + kd + v] = src[Kd + v];
         }
-        newKd = std::max(newKd, kk[j] > 0 ? kk[j] : kd);
+        kk[j] = recycle_update(ls[j], kd, jend[j], K, &dsc[(size_t)j * KD_MAX], cvtu.data(),
+                               Qm[j], PRu[j], PRv[j]);
+      };
+      std::vector<int> todo;
+      for (int j = 0; j < Rn; j++)
+        if (cand[j]) todo.push_back(j);
+      const int nth = std::min<int>((int)todo.size(), host_threads());
+      if (nth <= 1) {
+        std::vector<double> cvtu;
+        for (int j : todo) update_one(j, cvtu);
+      } else {
+        std::atomic<int> next(0);
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nth; t++)
+          pool.emplace_back([&]() {
+            std::vector<double> cvtu;
+            for (int q = next++; q < (int)todo.size(); q = next++) update_one(todo[q], cvtu);
+          });
+        for (std::thread& th : pool) th.join();
       }
+      int newKd = 0;
+      for (int j = 0; j < Rn; j++) newKd = std::max(newKd, kk[j] > 0 ? kk[j] : kdj[j]);
       bool changed = false;
       for (int j = 0; j < Rn; j++) changed = changed || kk[j] > 0;
       if (changed && newKd > 0) {

@@ -74,3 +74,67 @@ def test_union_solve_reaches_each_cube_tolerance(deflation):
         uo[free] = -np.linalg.solve(K[np.ix_(free, free)], ores[free])
         assert not uc[~free].any()
         assert np.abs(uc - uo).max() <= 1e-6 * np.abs(uo).max(), c
+
+
+@pytest.mark.parametrize("prec,deflation", [(64, 0), (32, 0), (32, 8), (64, 8)])
+def test_segmented_solve_per_cube_against_oracle(prec, deflation):
+    """Per-cube Krylov spaces (SegmentedKSM): every cube converges on its own
+    test and matches the oracle's dense solve of that cube."""
+    p = MultiCubeProblem(SIZES)
+    p.assemble()
+    pc = TACS.Pc(p.mat)
+    pc.factor()
+    rtol = 1e-10
+    ksm = TACS.SegmentedKSM(p.mat, pc, p.node_off, m=20, restarts=1000, prec=prec,
+                            deflation=deflation)
+    ksm.setTolerances(rtol, 1e-30)
+    assert ksm.solve(p.res, p.u) == 1
+    its, flags, r0, rn, cycles = ksm.getStats()
+    assert list(flags) == [1] * len(SIZES)
+    assert np.all(rn <= 1.0001 * rtol * r0)
+    # the cubes stop on their own: the 2^3 cube needs fewer steps than the 4^3
+    assert its[SIZES.index(2)] < its[SIZES.index(4)]
+    u = -p.u.getArray()
+    for c, n in enumerate(SIZES):
+        orp, ocols, oA, ores = _oracle_cube(n, p.traction)
+        nn = orp.size - 1
+        K = np.zeros((3 * nn, 3 * nn))
+        Ab = oA.reshape(-1, 3, 3)
+        for i in range(nn):
+            for s in range(orp[i], orp[i + 1]):
+                K[3 * i:3 * i + 3, 3 * ocols[s]:3 * ocols[s] + 3] = Ab[s]
+        uc = u[p.dof(c)]
+        assert np.linalg.norm(K @ uc + ores) <= 1.5 * rtol * np.linalg.norm(ores), c
+        # dense solve on the free dofs (the identity BC rows would spoil its scaling)
+        free = np.ones(ores.size, bool)
+        for b in p.meshes[c].nodes_on_x(0):
+            free[3 * b:3 * b + 3] = False
+        uo = np.zeros_like(uc)
+        uo[free] = -np.linalg.solve(K[np.ix_(free, free)], ores[free])
+        assert not uc[~free].any()
+        assert np.abs(uc - uo).max() <= 1e-6 * np.abs(uo).max(), c
+
+
+def test_segmented_matches_union_and_refuses_coupled_rows():
+    p = MultiCubeProblem(SIZES)
+    p.assemble()
+    pc = TACS.Pc(p.mat)
+    pc.factor()
+    seg = TACS.SegmentedKSM(p.mat, pc, p.node_off, m=30, prec=64, deflation=0)
+    seg.setTolerances(1e-11, 1e-30)
+    x1 = p.assembler.createVec()
+    assert seg.solve(p.res, x1) == 1
+    ksm = TACS.KSM(p.mat, pc, 60, 1000)
+    ksm.setTolerances(1e-30, p.cube_atol(p.res, 1e-11))
+    x2 = p.assembler.createVec()
+    assert ksm.solve(p.res, x2) == 1
+    a, b = x1.getArray(), x2.getArray()
+    assert np.abs(a - b).max() <= 1e-8 * np.abs(b).max()
+    # offsets that split a cube: its rows couple the two ranges
+    bad = np.array(p.node_off).copy()
+    bad[1] -= 1
+    with pytest.raises(TACS.TACSError):
+        TACS.SegmentedKSM(p.mat, pc, bad, m=20, prec=64, deflation=0)
+    # offsets that do not cover the matrix
+    with pytest.raises(TACS.TACSError):
+        TACS.SegmentedKSM(p.mat, pc, np.array(p.node_off[:-1]), m=20, prec=64, deflation=0)
