@@ -752,6 +752,30 @@ def batched_main(args, L, ev):
         a.setJacobianKernel(0)
         kj.update(elements=p.mesh.nelem, dof=3 * p.mesh.nnodes, nnz_blocks=p.mat.getInfo()[2],
                   flop_per_element=2283228, unit="ms per assembleJacobian (no residual)")
+        # the R right-hand-side product of the batched solver (tuned variant),
+        # FP32 and FP64 values, 20 launches each
+        import torch
+        a.assembleJacobian(1.0, 0.0, 0.0, None, p.mat)
+        rp, cp, ap = p.mat.getDeviceArrays()
+        b, nr, nnz = p.mat.getInfo()
+        A32 = torch.empty(nnz * 9, dtype=torch.float32, device="cuda")
+        L.tacs_hip_convert_f64_f32(nnz * 9, ap, A32.data_ptr(), None)
+        X = torch.rand(3 * nr * R, dtype=torch.float64, device="cuda")
+        Y = torch.empty_like(X)
+        torch.cuda.synchronize()
+        for key, fn in (("spmm_fp32_values_ms", lambda: L.tacs_hip_bsr3_spmm_mixed(
+                             nr, R, rp, cp, A32.data_ptr(), X.data_ptr(), Y.data_ptr(), None)),
+                        ("spmm_fp64_values_ms", lambda: L.tacs_hip_bsr3_spmm(
+                             nr, R, rp, cp, ap, X.data_ptr(), Y.data_ptr(), None))):
+            fn()
+            e0, e1 = ev.event(), ev.event()
+            ev.record(e0)
+            for _ in range(20):
+                fn()
+            ev.record(e1)
+            kj[key] = ev.elapsed_ms(e0, e1) / 20
+        kj["spmm_nrhs"] = R
+        kj["spmm_fp32_algorithmic_bytes"] = 36 * nnz + 4 * nnz + 4 * (nr + 1) + 2 * 8 * 3 * nr * R
         print(json.dumps(kj))
         return
     bs = batched_loads(p, R)

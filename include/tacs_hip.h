@@ -529,10 +529,12 @@ int tacs_hip_seg_spmm3(int prec, int maxn, int R, const int* off, const int* row
 int tacs_hip_bsr3_spmm_mixed(int nrows, int nrhs, const int* rowp, const int* cols, const float* A,
                              const double* X, double* Y, hipStream_t stream);
 /* Launch variants of tacs_hip_bsr3_spmm / _mixed (prec_vals 64 or 32), all
-   with the same sums: for R a multiple of 4, blocks per lane per trip and
-   non-temporal value loads, variants 0 (1, on), 1 (2, on), 2 (1, off),
-   3 (2, off), 4 (4, on); one variant otherwise.  The two entries above
-   launch the variant of the tuning table. */
+   with the same sums.  nrhs even and at most 12 (two right-hand sides per
+   lane): blocks per lane per trip 1 (variant 0) or 2 (1).  nrhs a multiple
+   of 4 from 16 (four per lane): blocks per lane per trip and non-temporal
+   value loads, variants 0 (1, on), 1 (2, on), 2 (1, off), 3 (2, off),
+   4 (4, on).  Other nrhs: one variant.  The two entries above launch the
+   variant of the tuning table. */
 int tacs_hip_bsr3_spmm_nvariants(int nrhs);
 int tacs_hip_bsr3_spmm_variant(int variant, int prec_vals, int nrows, int nrhs, const int* rowp,
                                const int* cols, const void* A, const double* X, double* Y,

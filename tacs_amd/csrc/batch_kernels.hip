@@ -673,12 +673,114 @@ __global__ __launch_bounds__(256) void bsr3_spmm_quad(int nrows, int R,
   }
 }
 
-// Launch variants of the four-right-hand-sides-per-lane product (R a
-// multiple of 4), all with the same sums (each lane still adds its blocks
-// g, g + G, ... in order): blocks per lane per trip SPU and non-temporal
-// value loads NTA: 0 (1, on), 1 (2, on), 2 (1, off), 3 (2, off), 4 (4, on).
-// Other R: one variant.
-static constexpr int kSpmmQuadVariants = 5;
+// Forms and launch variants of the R right-hand-side product.  R even and
+// at most 12: two right-hand sides per lane (bsr3_spmm_pair; hex27 40^3,
+// FP32 values: 4 RHS 0.46 ms against 0.77 ms four per lane, 8 RHS 0.62
+// against 0.80), variants 0 (1 block per lane per trip) and 1 (2).  R a
+// multiple of 4 from 16: four per lane (bsr3_spmm_quad), variants by blocks
+// per lane per trip SPU and non-temporal value loads NTA: 0 (1, on), 1 (2,
+// on), 2 (1, off), 3 (2, off), 4 (4, on).  Other R: one lane per right-hand
+// side (bsr3_spmm_slots), one variant.  The variants of a form give the same
+// sums (each lane still adds its blocks g, g + G, ... in order).
+static constexpr int kSpmmQuadVariants = 5, kSpmmPairVariants = 2;
+static inline bool spmm_pair_form(int R) { return R % 2 == 0 && R <= 12; }
+static inline bool spmm_quad_form(int R) { return R % 4 == 0 && R >= 16; }
+static inline int spmm_nvariants(int R) {
+  return spmm_pair_form(R) ? kSpmmPairVariants : spmm_quad_form(R) ? kSpmmQuadVariants : 1;
+}
+template <typename TA>
+static void spmm_pair_launch(int spu, int grid, int nrows, int R, const int* rowp, const int* cols,
+                             const TA* A, const double* X, double* Y, hipStream_t stream);
+
+// The same product with two right-hand sides per lane and NL = R / 2 lanes
+// per block (R even): each component's R values of X at a column node are
+// one R x 8-byte run read by the NL lanes of a block with 16-byte loads (a
+// 128-byte line for R = 16), and a block's 9 values come in with three
+// vector loads (two of four values, one of one) that its NL lanes share, so
+// a wave step of G = 64 / NL blocks touches about 4 cache lines per block
+// where the four-per-lane form touches about 9.  SPU blocks per lane per
+// trip (loads first; a block past the row end is masked).
+typedef float spmm_flt4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef double spmm_dbl2u __attribute__((ext_vector_type(2), aligned(8)));
+template <typename TA>
+__device__ __forceinline__ void spmm_load_block(const TA* __restrict__ a, TA v[9]) {
+  if constexpr (sizeof(TA) == 4) {
+    const spmm_flt4u p0 = *(const spmm_flt4u*)a;
+    const spmm_flt4u p1 = *(const spmm_flt4u*)(a + 4);
+    v[0] = p0.x, v[1] = p0.y, v[2] = p0.z, v[3] = p0.w;
+    v[4] = p1.x, v[5] = p1.y, v[6] = p1.z, v[7] = p1.w;
+    v[8] = a[8];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const spmm_dbl2u p = *(const spmm_dbl2u*)(a + 2 * q);
+      v[2 * q] = p.x;
+      v[2 * q + 1] = p.y;
+    }
+    v[8] = a[8];
+  }
+}
+template <typename TA, int NL, int SPU>
+__global__ __launch_bounds__(256) void bsr3_spmm_pair(int nrows, int R,
+                                                      const int* __restrict__ rowp,
+                                                      const int* __restrict__ cols,
+                                                      const TA* __restrict__ A,
+                                                      const double* __restrict__ X,
+                                                      double* __restrict__ Y) {
+  constexpr int G = 64 / NL;
+  const int lane = threadIdx.x & 63;
+  const int jp = lane % NL, g = lane / NL;  // right-hand sides 2 jp, 2 jp + 1
+  const bool okp = 2 * jp < R;
+  const int wid = threadIdx.x >> 6;
+  const int xcd = blockIdx.x & 7, nbx = gridDim.x >> 3;
+  const int r0 = (int)((long)nrows * xcd / 8), r1 = (int)((long)nrows * (xcd + 1) / 8);
+  for (int row = r0 + (blockIdx.x >> 3) * 4 + wid; row < r1; row += nbx * 4) {
+    const int rs = rowp[row], re = rowp[row + 1];
+    double y[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (int s0 = rs + g; okp && s0 < re; s0 += SPU * G) {
+      int sb[SPU];
+      bool use[SPU];
+#pragma unroll
+      for (int u = 0; u < SPU; u++) {
+        use[u] = s0 + u * G < re;
+        sb[u] = use[u] ? s0 + u * G : s0;
+      }
+      TA av[SPU][9];
+      spmm_dbl2 xv[SPU][3];
+#pragma unroll
+      for (int u = 0; u < SPU; u++) {
+        const int col = cols[sb[u]];
+        spmm_load_block<TA>(&A[(size_t)9 * sb[u]], av[u]);
+        const double* xb = &X[(size_t)3 * col * R + 2 * jp];
+#pragma unroll
+        for (int c = 0; c < 3; c++) xv[u][c] = *(const spmm_dbl2*)(xb + (size_t)c * R);
+      }
+#pragma unroll
+      for (int u = 0; u < SPU; u++) {
+        if (!use[u]) continue;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          y[r][0] += (double)av[u][3 * r] * xv[u][0].x + (double)av[u][3 * r + 1] * xv[u][1].x +
+                     (double)av[u][3 * r + 2] * xv[u][2].x;
+          y[r][1] += (double)av[u][3 * r] * xv[u][0].y + (double)av[u][3 * r + 1] * xv[u][1].y +
+                     (double)av[u][3 * r + 2] * xv[u][2].y;
+        }
+      }
+    }
+#pragma unroll
+    for (int off = NL; off < 64; off <<= 1)
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        y[r][0] += __shfl_xor(y[r][0], off, 64);
+        y[r][1] += __shfl_xor(y[r][1], off, 64);
+      }
+    if (g == 0 && okp) {
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+        *(spmm_dbl2*)&Y[((size_t)3 * row + r) * R + 2 * jp] = spmm_dbl2{y[r][0], y[r][1]};
+    }
+  }
+}
 
 template <typename TA, int NLV>
 static void spmm_quad_launch(int variant, int grid, int nrows, int R, const int* rowp,
@@ -698,14 +800,39 @@ static void spmm_quad_launch(int variant, int grid, int nrows, int R, const int*
 }
 
 template <typename TA>
+static void spmm_pair_launch(int spu, int grid, int nrows, int R, const int* rowp, const int* cols,
+                             const TA* A, const double* X, double* Y, hipStream_t stream) {
+#define SPMM_PAIR(NLV)                                                                          \
+  do {                                                                                          \
+    if (spu == 1)                                                                               \
+      hipLaunchKernelGGL((bsr3_spmm_pair<TA, NLV, 1>), dim3(grid), dim3(256), 0, stream, nrows, \
+                         R, rowp, cols, A, X, Y);                                               \
+    else                                                                                        \
+      hipLaunchKernelGGL((bsr3_spmm_pair<TA, NLV, 2>), dim3(grid), dim3(256), 0, stream, nrows, \
+                         R, rowp, cols, A, X, Y);                                               \
+  } while (0)
+  switch (pow2_ceil(R / 2)) {
+    case 1: SPMM_PAIR(1); break;
+    case 2: SPMM_PAIR(2); break;
+    case 4: SPMM_PAIR(4); break;
+    default: SPMM_PAIR(8); break;
+  }
+#undef SPMM_PAIR
+}
+
+template <typename TA>
 static int spmm_slots_launch(int variant, int nrows, int R, const int* rowp, const int* cols,
                              const TA* A, const double* X, double* Y, hipStream_t stream) {
   if (R > 64) return TACS_HIP_ERR_ARG;
   if (nrows <= 0 || R <= 0) return 0;
-  if (variant < 0 || variant >= (R % 4 == 0 ? kSpmmQuadVariants : 1)) return TACS_HIP_ERR_ARG;
+  if (variant < 0 || variant >= spmm_nvariants(R)) return TACS_HIP_ERR_ARG;
   // a multiple of 8 workgroups (one share per XCD)
   const int grid = 8 * tacs_grid_for((nrows + 7) / 8, 4, 8192);
-  if (R % 4 == 0) {
+  if (spmm_pair_form(R)) {
+    spmm_pair_launch<TA>(variant + 1, grid, nrows, R, rowp, cols, A, X, Y, stream);
+    return (int)hipGetLastError();
+  }
+  if (spmm_quad_form(R)) {
     // 16-byte aligned X rows: four right-hand sides per lane
     switch (pow2_ceil(R / 4)) {
       case 1: spmm_quad_launch<TA, 1>(variant, grid, nrows, R, rowp, cols, A, X, Y, stream); break;
@@ -1227,7 +1354,7 @@ int tacs_hip_bsr3_spmm(int nrows, int nrhs, const int* rowp, const int* cols, co
                                    rowp, cols, A, X, Y, stream);
 }
 
-int tacs_hip_bsr3_spmm_nvariants(int nrhs) { return nrhs % 4 == 0 ? kSpmmQuadVariants : 1; }
+int tacs_hip_bsr3_spmm_nvariants(int nrhs) { return spmm_nvariants(nrhs); }
 
 int tacs_hip_bsr3_spmm_variant(int variant, int prec_vals, int nrows, int nrhs, const int* rowp,
                                const int* cols, const void* A, const double* X, double* Y,

@@ -65,7 +65,7 @@ def test_sell_variants_identical(kind_of_mesh):
         p.mat.setSellVariant(0, nv)
 
 
-@pytest.mark.parametrize("nrhs", [4, 8, 12, 6])
+@pytest.mark.parametrize("nrhs", [4, 8, 12, 6, 16, 5])
 @pytest.mark.parametrize("prec", [64, 32])
 def test_spmm_variants_identical(nrhs, prec):
     import torch
@@ -81,7 +81,8 @@ def test_spmm_variants_identical(nrhs, prec):
     g = torch.Generator(device="cpu").manual_seed(nrhs)
     X = torch.randn(3 * nr * nrhs, generator=g, dtype=torch.float64).cuda()
     nv = L.tacs_hip_bsr3_spmm_nvariants(nrhs)
-    assert nv == (5 if nrhs % 4 == 0 else 1)
+    # two right-hand sides per lane (even R <= 12), four per lane (R = 16, 20, ...)
+    assert nv == {4: 2, 8: 2, 12: 2, 6: 2, 16: 5, 5: 1}[nrhs]
     outs = []
     for v in range(nv):
         Y = torch.zeros_like(X)
