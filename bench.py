@@ -117,11 +117,18 @@ def time_spmv(L, ev, p, reps=50):
 MONITOR = {"file": None, "freq": 1000}
 
 
+CHEB = {"degree": 1, "ratio": 300.0}
+
+
 def make_solver(p, solver, m, rtol, deflation=0):
     """mixed: the recycling solver on the FP32-valued operator; recycle64: the
-    same recycling on the FP64 operator; fp64: GMRES(m) + MGS."""
+    same recycling on the FP64 operator; fp64: GMRES(m) + MGS.  The recycling
+    solvers take the Chebyshev preconditioner of --cheb-degree (1: plain
+    block-Jacobi, as GMRES)."""
     from tacs_amd import TACS
     pc = TACS.Pc(p.mat)
+    if solver in ("mixed", "recycle64") and CHEB["degree"] > 1:
+        pc.setChebyshev(CHEB["degree"], CHEB["ratio"])
     if solver == "mixed":
         ksm = TACS.MixedKSM(p.mat, pc, m, 100000, deflation=deflation)
     elif solver == "recycle64":
@@ -1271,6 +1278,10 @@ def main():
                     help="hex: the 128^3 headline line; shell: the Quad4 plate; batched: the "
                          "hex27 cube with a block of load cases (own JSON lines)")
     ap.add_argument("--nrhs", type=int, default=16, help="load cases of the batched workload")
+    ap.add_argument("--cheb-degree", type=int, default=None,
+                    help="Chebyshev preconditioner degree of the recycling solvers (1: plain "
+                         "block-Jacobi; default 10 for solids, 1 for shells)")
+    ap.add_argument("--cheb-ratio", type=float, default=300.0)
     ap.add_argument("--multi-solver", default="segmented", choices=["segmented", "union"],
                     help="multi workload: every cube its own system (segmented) or one union "
                          "system")
@@ -1291,6 +1302,9 @@ def main():
     if args.deflation is None:
         args.deflation = 30 if shell else 20
     MONITOR["file"] = args.monitor
+    if args.cheb_degree is None:
+        args.cheb_degree = 1 if shell else 10
+    CHEB.update(degree=args.cheb_degree, ratio=args.cheb_ratio)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
               "described in DESIGN.md and not built", file=sys.stderr)
@@ -1426,7 +1440,10 @@ def main():
         "config": {"workload": f"linear hex cube {args.n}^3, E=70e9 nu=0.3, x=0 clamped, "
                                f"traction 1e6 Pa on x=1, per-element thickness DVs; assembly + "
                                f"mixed GMRES({args.m}) recycling {args.deflation} vectors, "
-                               f"block-Jacobi, solve + compliance adjoint "
+                               + (f"Chebyshev degree {args.cheb_degree} on [lmax/"
+                                  f"{args.cheb_ratio:g}, lmax] of block-Jacobi"
+                                  if args.cheb_degree > 1 else "block-Jacobi") +
+                               f", solve + compliance adjoint "
                                f"to {args.rtol:g} relative residual; plain path FP64 "
                                f"GMRES({args.m_fp64})+MGS",
                    "elements": p.mesh.nelem, "dof": 3 * p.mesh.nnodes,
@@ -1483,9 +1500,11 @@ def main():
             "algorithm": t_64 / t_rec,
             "precision": t_rec / t_mix,
             "total": t_64 / t_mix,
-            "note": f"algorithm: FP64 GMRES({args.m_fp64})+MGS -> FP64 GCRO-DR({args.m}, "
-                    f"{args.deflation}) with delayed CGS2; precision: that -> the same solver "
-                    "on the FP32-valued operator",
+            "note": f"algorithm: FP64 GMRES({args.m_fp64})+MGS with block-Jacobi -> FP64 "
+                    f"GCRO-DR({args.m}, {args.deflation}) with delayed CGS2" +
+                    (f" and the degree-{args.cheb_degree} Chebyshev preconditioner"
+                     if args.cheb_degree > 1 else "") +
+                    "; precision: that -> the same solver on the FP32-valued operator",
         }
         g64r = rec64["grad"]
         out["grad_rel_err_mixed_vs_fp64_recycling"] = float(

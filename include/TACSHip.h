@@ -367,6 +367,17 @@ class TACSBlockJacobiPc : public TACSPc {
   int* d_diag;
   double* d_Dinv;
   float* d_Dinv32;
+  // Chebyshev polynomial of the block-Jacobi iteration (no counterpart in
+  // the reference): with degree > 1 the recycling and batched solvers apply
+  // M^{-1} x = z_degree of the Chebyshev iteration for A z = x (degree - 1
+  // products with the solver's operator values), on the interval
+  // [lmax / ratio, lmax] of the eigenvalues of D^{-1} A; factor() estimates
+  // lmax by power iteration (FP32-rounded values, 12 steps, x 1.1).  The
+  // smallest eigenvalues are left to the recycled space.  degree 1 (the
+  // default) is plain block-Jacobi.  Not applied to segmented systems.
+  int setChebyshev(int degree, double ratio);
+  int chebDegree = 1;
+  double chebRatio = 20.0, chebMax = 0.0;
 };
 
 // src/bpmat/KSM.h:200 (TACSKsm), :266 (GMRES)
@@ -539,6 +550,9 @@ class TACSBatchedGMRES : public TACSObject {
   int sysRows() const { return nseg > 0 ? segMaxNodes : mat->nrows; }
   void allocate();
   void bdiag(int pr, int Rn, const void* Dinv, const double* x, double* y);
+  // y = M^{-1} x: block-Jacobi, or the preconditioner's Chebyshev polynomial
+  void precond(int pr, int Rn, const void* Dinv, const float* A32, const double* x, double* y);
+  double *d_cd = NULL, *d_caz = NULL;  // Chebyshev work vectors
   void bmult(int pr, int Rn, const float* A32, const double* x, double* y);
   // x[j] = A^{-1} b[j], j < nrhs <= R; returns 1 when every system converged
   int solve(int nrhs, TACSBVec** b, TACSBVec** x, int zero_guess = 1);

@@ -398,6 +398,9 @@ int tacs_hip_scale(int prec, size_t n, void* x, const double* sp, double sv, int
 int tacs_hip_axpby(int px, int py, size_t n, double a, const void* x, double b, void* y,
                    hipStream_t stream);
 int tacs_hip_fill(int prec, size_t n, void* x, double v, hipStream_t stream);
+/* x[i] in [-1, 1) from a 64-bit hash of (seed, i): a reproducible start
+   vector (power iteration). */
+int tacs_hip_fill_random(size_t n, double* x, unsigned long long seed, hipStream_t stream);
 int tacs_hip_reduce(size_t n, const double* x, int op, void* scratch, double* out,
                     hipStream_t stream);
 int tacs_hip_givens(int i, int m, double* H, double* Qc, double* Qs, double* res, double* st,
@@ -498,6 +501,15 @@ int tacs_hip_bbacksolve(int R, int m, const int* cnt, const double* H, size_t sH
                         size_t sRes, hipStream_t stream);
 int tacs_hip_bdiag_apply(int b, int prec, int nrows, int R, const void* Dinv, const double* x,
                          double* y, hipStream_t stream);
+
+/* One Chebyshev step for A z = x with the block-Jacobi inverse D^{-1}
+   (prec 64 / 32 blocks, b = 3 or 6) over interleaved block vectors:
+   w = D^{-1} (x - Az) (first: D^{-1} x), d = c1 d + c2 w (first: c2 w),
+   z += d (first: z = d).  Saad, Iterative Methods for Sparse Linear
+   Systems, 2nd ed., Alg. 12.1. */
+int tacs_hip_bcheb_step(int b, int prec, int nrows, int R, const void* Dinv, const double* x,
+                        const double* Az, double* d, double* z, double c1, double c2, int first,
+                        hipStream_t stream);
 
 /* Segmented systems (the R <= 64 diagonal blocks of a block-diagonal 3x3
    BCSR matrix, block j = node range [off[j], off[j+1]), solved as R
@@ -770,6 +782,15 @@ int tacs_mat_device_arrays(tacs_obj m, const int** rowp, const int** cols, const
 tacs_obj tacs_block_jacobi_pc_create(tacs_obj mat);
 int tacs_pc_factor(tacs_obj pc);
 int tacs_pc_apply(tacs_obj pc, tacs_obj x, tacs_obj y);
+/* Chebyshev polynomial of the block-Jacobi iteration as the preconditioner
+   of the recycling and batched solvers (degree 1..16, 1 = plain
+   block-Jacobi; interval [lmax / ratio, lmax] of D^{-1} A, lmax estimated by
+   the next factor()); GMRES, pc_apply and segmented systems keep plain
+   block-Jacobi.  No counterpart in the reference. */
+int tacs_pc_set_chebyshev(tacs_obj pc, int degree, double ratio);
+/* the settings and the lmax the last factor() estimated (1.1 x the power
+   iteration's value; 0 before a factor() with degree > 1) */
+int tacs_pc_get_chebyshev(tacs_obj pc, int* degree, double* ratio, double* lmax);
 tacs_obj tacs_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int is_flexible);
 /* Residual monitors (KSMPrintStdout src/bpmat/KSM.h:293, KSMPrintFile :310;
    GMRES::setMonitor KSM.cpp:739, TACS.pyx KSM.setMonitor :1326): GMRES and

@@ -226,6 +226,21 @@ class Pc(TACSHandle):
     def applyFactor(self, x, y):
         lib().tacs_pc_apply(self.ptr, x.ptr, y.ptr)
 
+    def setChebyshev(self, degree, ratio=20.0):
+        """Chebyshev polynomial (degree - 1 operator products per
+        application) of the block-Jacobi iteration as the preconditioner of
+        the recycling and batched solvers, on [lmax / ratio, lmax] of the
+        eigenvalues of D^{-1} A (lmax estimated at the next factor()).
+        degree 1: plain block-Jacobi.  No counterpart in the reference."""
+        check(lib().tacs_pc_set_chebyshev(self.ptr, int(degree), float(ratio)), "Pc.setChebyshev")
+
+    def getChebyshev(self):
+        """(degree, ratio, lmax): lmax as the last factor() estimated it."""
+        d, r, l = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        check(lib().tacs_pc_get_chebyshev(self.ptr, ctypes.byref(d), ctypes.byref(r),
+                                          ctypes.byref(l)), "Pc.getChebyshev")
+        return d.value, r.value, l.value
+
 
 class KSM(TACSHandle):
     """GMRES(m) with right preconditioning (GMRES::solve src/bpmat/KSM.cpp:785)."""

@@ -18,6 +18,7 @@ _SCALARS = {
     "float": ctypes.c_float,
     "size_t": ctypes.c_size_t,
     "long": ctypes.c_long,
+    "unsigned long long": ctypes.c_ulonglong,
     "void": None,
     "hipStream_t": ctypes.c_void_p,
     "tacs_obj": ctypes.c_void_p,

@@ -534,6 +534,37 @@ __global__ void bdiag_apply(int nrows, int R, const TD* __restrict__ Dinv,
   }
 }
 
+// One step of the Chebyshev iteration for A z = x preconditioned by the
+// block-Jacobi inverse (Saad, Iterative Methods for Sparse Linear Systems,
+// 2nd ed., Alg. 12.1), over interleaved block vectors: with w = D^{-1}
+// (x - A z) (first step: w = D^{-1} x, z = 0), d = c1 d + c2 w and z += d.
+template <int B, typename TD>
+__global__ void bcheb_step(int nrows, int R, const TD* __restrict__ Dinv,
+                           const double* __restrict__ x, const double* __restrict__ Az,
+                           double* __restrict__ d, double* __restrict__ z, double c1, double c2,
+                           int first) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nrows * R) return;
+  const int node = (int)(t / R), j = (int)(t % R);
+  const TD* D = &Dinv[(size_t)B * B * node];
+  double rc[B];
+#pragma unroll
+  for (int c = 0; c < B; c++) {
+    const size_t o = ((size_t)B * node + c) * R + j;
+    rc[c] = first ? x[o] : x[o] - Az[o];
+  }
+#pragma unroll
+  for (int r = 0; r < B; r++) {
+    double w = (double)D[B * r] * rc[0];
+#pragma unroll
+    for (int c = 1; c < B; c++) w += (double)D[B * r + c] * rc[c];
+    const size_t o = ((size_t)B * node + r) * R + j;
+    const double dn = first ? c2 * w : c1 * d[o] + c2 * w;
+    d[o] = dn;
+    z[o] = first ? dn : z[o] + dn;
+  }
+}
+
 // Multi right-hand-side 3x3 SpMM, Y = A X, over interleaved block vectors
 // (X[(3*node + c)*R + j]), FP64 (TA = double) or FP32 (TA = float) values,
 // FP64 vectors and accumulation.  One wavefront per block row; the 64 lanes
@@ -1433,6 +1464,28 @@ int tacs_hip_bpack(size_t n, int R, int j, const double* x, double* X, hipStream
 
 int tacs_hip_bunpack(size_t n, int R, int j, const double* X, double* x, hipStream_t stream) {
   hipLaunchKernelGGL(bpack, dim3(bgrid(n)), dim3(256), 0, stream, n, R, j, X, x, 1);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_bcheb_step(int b, int prec, int nrows, int R, const void* Dinv, const double* x,
+                        const double* Az, double* d, double* z, double c1, double c2, int first,
+                        hipStream_t stream) {
+  if ((b != 3 && b != 6) || (prec != 32 && prec != 64) || R < 1) return TACS_HIP_ERR_ARG;
+  const size_t N = (size_t)nrows * R;
+  if (N == 0) return 0;
+  dim3 g((unsigned)((N + 255) / 256)), t(256);
+#define CHEB(BB, TD)                                                                             \
+  hipLaunchKernelGGL((bcheb_step<BB, TD>), g, t, 0, stream, nrows, R, (const TD*)Dinv, x, Az, d, \
+                     z, c1, c2, first)
+  if (b == 3 && prec == 64)
+    CHEB(3, double);
+  else if (b == 3)
+    CHEB(3, float);
+  else if (prec == 64)
+    CHEB(6, double);
+  else
+    CHEB(6, float);
+#undef CHEB
   return (int)hipGetLastError();
 }
 

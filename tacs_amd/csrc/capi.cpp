@@ -654,6 +654,17 @@ int tacs_pc_factor(tacs_obj pc) {
   p->factor();
   return last_error_code();
 }
+int tacs_pc_set_chebyshev(tacs_obj pc, int degree, double ratio) {
+  REQUIRE(TACSBlockJacobiPc, p, pc);
+  return p->setChebyshev(degree, ratio);
+}
+int tacs_pc_get_chebyshev(tacs_obj pc, int* degree, double* ratio, double* lmax) {
+  REQUIRE(TACSBlockJacobiPc, p, pc);
+  if (degree) *degree = p->chebDegree;
+  if (ratio) *ratio = p->chebRatio;
+  if (lmax) *lmax = p->chebMax;
+  return 0;
+}
 int tacs_pc_apply(tacs_obj pc, tacs_obj x, tacs_obj y) {
   REQUIRE(TACSPc, p, pc);
   REQUIRE(TACSBVec, xv, x);

@@ -709,6 +709,42 @@ void TACSBlockJacobiPc::factor() {
   d2h(&hs, st, 1);
   dfree(st);
   if (hs) tacs_set_error("TACSBlockJacobiPc::factor: singular or missing diagonal block (%d)", hs);
+  if (chebDegree > 1 && !g_halt) {
+    // largest eigenvalue of D^{-1} A (FP32-rounded values, as the solvers
+    // apply the polynomial) by power iteration from a hashed start vector
+    const size_t n = (size_t)mat->nrows * mat->bsize;
+    double* v = dalloc<double>(n);
+    double* w = dalloc<double>(n);
+    double* nr = dalloc<double>(1);
+    double lam = 0.0;
+    if (!g_halt) {
+      KCHECK(tacs_hip_fill_random(n, v, 12345, g_stream));
+      for (int it = 0; it < 12 && !g_halt; it++) {
+        mat->multMixed(v, w);
+        KCHECK(tacs_hip_bdiag_apply(mat->bsize, 32, mat->nrows, 1, d_Dinv32, w, v, g_stream));
+        KCHECK(tacs_hip_mdot_noalloc(64, n, 1, v, n, v, nr, 0, g_stream));
+        double h = 0.0;
+        d2h(&h, nr, 1);
+        const double nv = std::sqrt(h);
+        if (!(nv > 0.0)) break;
+        if (it > 0) lam = nv;  // |D^{-1} A v| for the unit vector v of the last step
+        KCHECK(tacs_hip_scale(64, n, v, NULL, 1.0 / nv, 0, g_stream));
+      }
+    }
+    dfree(v);
+    dfree(w);
+    dfree(nr);
+    chebMax = 1.1 * lam;
+    if (!(chebMax > 0.0)) tacs_set_error("TACSBlockJacobiPc::factor: no eigenvalue estimate");
+  }
+}
+int TACSBlockJacobiPc::setChebyshev(int degree, double ratio) {
+  if (degree < 1 || degree > 16 || !(ratio > 1.0))
+    return tacs_set_error("setChebyshev: 1 <= degree <= 16 and ratio > 1");
+  chebDegree = degree;
+  chebRatio = ratio;
+  chebMax = 0.0;
+  return 0;
 }
 void TACSBlockJacobiPc::applyFactor(TACSVec* tx, TACSVec* ty) {
   TACSBVec* x = dynamic_cast<TACSBVec*>(tx);
@@ -1483,7 +1519,8 @@ void TACSBatchedGMRES::allocate() {
 TACSBatchedGMRES::~TACSBatchedGMRES() {
   HCHECK(hipStreamSynchronize(g_stream));
   void* bufs[] = {d_V, d_work, d_Bb,  d_Xb,  d_scal, d_vec, d_cnt, d_red,
-                  d_U, d_C,    d_Ub,  d_Cb,  d_Unb,  d_Cnb, d_prod, d_dq, d_segoff, d_ux, d_uy};
+                  d_U, d_C,    d_Ub,  d_Cb,  d_Unb,  d_Cnb, d_prod, d_dq, d_segoff, d_ux, d_uy,
+                  d_cd, d_caz};
   for (void* q : bufs) dfree(q);
   hfree(h_pin);
   if (h_stage) hfree(h_stage);
@@ -1626,6 +1663,34 @@ void TACSBatchedGMRES::bdiag(int pr, int Rn, const void* Dinv, const double* x, 
   else
     KCHECK(tacs_hip_bdiag_apply(mat->bsize, pr, mat->nrows, Rn, Dinv, x, y, g_stream));
 }
+void TACSBatchedGMRES::precond(int pr, int Rn, const void* Dinv, const float* A32,
+                               const double* x, double* y) {
+  const int deg = pc->chebDegree;
+  if (deg <= 1 || nseg > 0) {
+    bdiag(pr, Rn, Dinv, x, y);
+    return;
+  }
+  // Chebyshev iteration for A z = x on [lmax / ratio, lmax] (Saad, Alg. 12.1),
+  // the iterate z kept in y (y and x are distinct buffers at every call)
+  if (!d_cd) {
+    const size_t Nmax = (size_t)mat->nrows * mat->bsize * R;
+    d_cd = dalloc<double>(Nmax);
+    d_caz = dalloc<double>(Nmax);
+    if (g_halt) return;
+  }
+  const double b = pc->chebMax, a = b / pc->chebRatio;
+  const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, Rn, Dinv, x, NULL, d_cd, y, 0.0,
+                             1.0 / theta, 1, g_stream));
+  for (int k = 1; k < deg; k++) {
+    bmult(pr, Rn, A32, y, d_caz);
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, Rn, Dinv, x, d_caz, d_cd, y, rn * rho,
+                               2.0 * rn / delta, 0, g_stream));
+    rho = rn;
+  }
+}
 void TACSBatchedGMRES::bmult(int pr, int Rn, const float* A32, const double* x, double* y) {
   if (nseg > 0) {
     // through a whole-matrix vector: the matrix's own sliced ELL product
@@ -1707,7 +1772,7 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
   auto spmm64 = [&](const double* x, double* y) { bmult(64, Rn, NULL, x, y); };
   // y = A M^{-1} x with the Arnoldi operator of this precision
   auto op = [&](const double* x, double* y) {
-    bdiag(prec, Rn, Dinv, x, d_work);
+    precond(prec, Rn, Dinv, A32, x, d_work);
     bmult(prec, Rn, A32, d_work, y);
   };
   // q(:, j) /= |q(:, j)|, with |q(:, j)| written to out[j*sOut]
@@ -1767,7 +1832,7 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
       // x += |r| M^{-1} U C^T v_0;  v_0 -= C C^T v_0
       KCHECK(tacs_hip_bmdot_shared(n, Rn, kd, d_C, n, V(0), (double*)d_red, pco, S, 0, g_stream));
       KCHECK(tacs_hip_blincomb_shared(n, Rn, kd, d_U, n, pco, S, V(1), 0, g_stream));
-      bdiag(prec, Rn, Dinv, V(1), d_work);
+      precond(prec, Rn, Dinv, A32, V(1), d_work);
       KCHECK(tacs_hip_bscale(n, Rn, nrm, d_work, g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, d_work, 1.0, Xd, g_stream));
       KCHECK(tacs_hip_bcgs_update_shared(n, Rn, kd, d_C, n, pco, S, V(0), g_stream));
@@ -1843,7 +1908,7 @@ int TACSBatchedGMRES::solveBlock(int Rn, const double* Bd, double* Xd, int zero_
         KCHECK(tacs_hip_bsmall_gemv(Rn, kd, d_cnt, Bc, S, KD_MAX, res, S, zc, S, g_stream));
         KCHECK(tacs_hip_bcgs_update_shared(n, Rn, kd, d_U, n, zc, S, d_work, g_stream));
       }
-      bdiag(prec, Rn, Dinv, d_work, V(1));
+      precond(prec, Rn, Dinv, A32, d_work, V(1));
       KCHECK(tacs_hip_bscale(n, Rn, nrm, V(1), g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, V(1), 1.0, Xd, g_stream));
     }
@@ -1893,7 +1958,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
   const void* Dinv = prec == 32 ? (const void*)pc->d_Dinv32 : (const void*)pc->d_Dinv;
   auto spmm64 = [&](const double* x, double* y) { bmult(64, Rn, NULL, x, y); };
   auto op = [&](const double* x, double* y) {
-    bdiag(pr, Rn, Dinv, x, d_work);
+    precond(pr, Rn, Dinv, A32, x, d_work);
     bmult(pr, Rn, A32, d_work, y);
   };
   auto normalize = [&](double* q, double* out, size_t sOut) {
@@ -2247,7 +2312,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       // x += |r| M^{-1} U C^T v_0;  v_0 -= C C^T v_0
       KCHECK(tacs_hip_bmdot(n, Rn, Kd, Cb(0), N, V(0), (double*)d_red, pco, S, 0, g_stream));
       KCHECK(tacs_hip_blincomb(n, Rn, Kd, Ub(0), N, pco, S, NULL, d_work, 0, g_stream));
-      bdiag(pr, Rn, Dinv, d_work, tmp2);
+      precond(pr, Rn, Dinv, A32, d_work, tmp2);
       KCHECK(tacs_hip_bscale(n, Rn, nrm, tmp2, g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, tmp2, 1.0, Xd, g_stream));
       KCHECK(tacs_hip_bcgs_update(n, Rn, Kd, Cb(0), N, pco, S, V(0), g_stream));
@@ -2354,7 +2419,7 @@ int TACSBatchedGMRES::solveBlockDeflated(int Rn, const double* Bd, double* Xd, i
       h2d(inv, h_inv, Rn);
       if (Kd > 0) KCHECK(tacs_hip_blincomb(n, Rn, Kd, Ub(0), N, zc, S, NULL, d_work, 0, g_stream));
       KCHECK(tacs_hip_blincomb(n, Rn, niters, V(0), N, res, S, NULL, d_work, Kd > 0, g_stream));
-      bdiag(pr, Rn, Dinv, d_work, tmp2);
+      precond(pr, Rn, Dinv, A32, d_work, tmp2);
       KCHECK(tacs_hip_bscale(n, Rn, inv, tmp2, g_stream));
       KCHECK(tacs_hip_axpby(64, 64, N, 1.0, tmp2, 1.0, Xd, g_stream));
     }

@@ -460,6 +460,19 @@ static int mdot_launch(int prec, size_t n, int k, const void* W, size_t ldw, con
   return (int)hipGetLastError();
 }
 
+// x[i] = a value in [-1, 1) from a 64-bit mix of (seed, i) (splitmix64 finaliser):
+// a reproducible start vector with every frequency present.
+__global__ void fill_hash_kernel(size_t n, double* __restrict__ x, unsigned long long seed) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    x[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
 extern "C" {
 
 size_t tacs_hip_red_scratch_bytes() { return sizeof(RedScratch); }
@@ -641,6 +654,14 @@ int tacs_hip_tsgemm(size_t n, int p, const double* X, int q, const double* Y, do
 
 int tacs_hip_scalar_op(const double* src, double s, double* dst, hipStream_t stream) {
   hipLaunchKernelGGL(scalar_op, dim3(1), dim3(64), 0, stream, src, s, dst);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_fill_random(size_t n, double* x, unsigned long long seed, hipStream_t stream) {
+  if (n == 0) return 0;
+  size_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(fill_hash_kernel, dim3((unsigned)g), dim3(256), 0, stream, n, x, seed);
   return (int)hipGetLastError();
 }
 

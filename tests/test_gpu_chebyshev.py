@@ -1,0 +1,130 @@
+"""Chebyshev polynomial preconditioner of the recycling and batched solvers:
+the power-iteration estimate of lmax(D^{-1} A) against the dense eigenvalues
+of the oracle's matrix, and solves with it against the oracle's dense
+solution (solid cube, several load cases, a shell plate)."""
+import numpy as np
+import pytest
+
+from tacs_amd import TACS
+from tacs_amd.problems import ShellProblem, SolidProblem
+
+pytestmark = pytest.mark.gpu
+
+
+def _dense(p):
+    rowp, cols, A = p.mat.getArrays()
+    b = A.shape[1]
+    n = rowp.size - 1
+    K = np.zeros((b * n, b * n))
+    for i in range(n):
+        for s in range(rowp[i], rowp[i + 1]):
+            K[b * i:b * i + b, b * cols[s]:b * cols[s] + b] = A[s]
+    return K, rowp, cols, A
+
+
+def _dinv_a_lmax(K, b):
+    n = K.shape[0] // b
+    D = np.zeros_like(K)
+    for i in range(n):
+        sl = slice(b * i, b * i + b)
+        D[sl, sl] = np.linalg.inv(K[sl, sl])
+    return np.max(np.abs(np.linalg.eigvals(D @ K)))
+
+
+def test_power_iteration_estimate():
+    p = SolidProblem(5, dv="element")
+    p.set_thickness(np.random.default_rng(2).uniform(0.5, 1.5, p.mesh.nelem))
+    p.assemble()
+    pc = TACS.Pc(p.mat)
+    assert pc.getChebyshev() == (1, 20.0, 0.0)
+    pc.setChebyshev(4, 30.0)
+    pc.factor()
+    deg, ratio, lmax = pc.getChebyshev()
+    assert (deg, ratio) == (4, 30.0)
+    K, *_ = _dense(p)
+    true = _dinv_a_lmax(K, 3)
+    # 1.1 x a power-iteration value from below: within [1.0, 1.1] x lmax for
+    # a converged iteration, never far below it
+    assert 0.95 * true <= lmax <= 1.1 * true * 1.0001, (lmax, true)
+    with pytest.raises(TACS.TACSError):
+        pc.setChebyshev(0)
+    with pytest.raises(TACS.TACSError):
+        pc.setChebyshev(4, 1.0)
+
+
+@pytest.mark.parametrize("deg,ratio", [(2, 20.0), (6, 60.0), (10, 300.0)])
+def test_recycling_solve_with_chebyshev_matches_dense(deg, ratio):
+    p = SolidProblem(5, dv="element")
+    p.set_thickness(np.random.default_rng(3).uniform(0.5, 1.5, p.mesh.nelem))
+    p.assemble()
+    pc = TACS.Pc(p.mat)
+    pc.setChebyshev(deg, ratio)
+    pc.factor()
+    ksm = TACS.MixedKSM(p.mat, pc, 30, 1000, deflation=8)
+    ksm.setTolerances(1e-11, 1e-30)
+    x = p.assembler.createVec()
+    assert ksm.solve(p.res, x) == 1
+    its1 = ksm.getIterCount()
+    assert ksm.solve(p.res, x) == 1  # again, with the recycled space
+    K, *_ = _dense(p)
+    r = p.res.getArray()
+    free = np.ones(r.size, bool)
+    for bn in p.bc_nodes:
+        free[3 * bn:3 * bn + 3] = False
+    uo = np.zeros_like(r)
+    uo[free] = np.linalg.solve(K[np.ix_(free, free)], r[free])
+    u = x.getArray()
+    assert np.linalg.norm(K @ u - r) <= 2e-11 * np.linalg.norm(r)
+    assert np.abs(u - uo).max() <= 1e-8 * np.abs(uo).max()
+    # fewer outer steps than block-Jacobi alone
+    pc1 = TACS.Pc(p.mat)
+    pc1.factor()
+    k1 = TACS.MixedKSM(p.mat, pc1, 30, 1000, deflation=8)
+    k1.setTolerances(1e-11, 1e-30)
+    assert k1.solve(p.res, x) == 1
+    assert its1 < k1.getIterCount()
+
+
+def test_batched_solve_with_chebyshev_matches_dense():
+    p = SolidProblem(3, order=3)
+    p.assemble()
+    pc = TACS.Pc(p.mat)
+    pc.setChebyshev(5, 50.0)
+    pc.factor()
+    R = 4
+    rng = np.random.default_rng(5)
+    a = p.assembler
+    bs = [a.createVec() for _ in range(R)]
+    xs = [a.createVec() for _ in range(R)]
+    K, *_ = _dense(p)
+    free = np.ones(K.shape[0], bool)
+    for bn in p.bc_nodes:
+        free[3 * bn:3 * bn + 3] = False
+    for b in bs:
+        v = rng.standard_normal(K.shape[0])
+        v[~free] = 0.0
+        b.getArray()[:] = v
+    ksm = TACS.BatchedKSM(p.mat, pc, m=30, restarts=1000, nrhs=R, prec=32, deflation=8)
+    ksm.setTolerances(1e-10, 1e-30)
+    assert ksm.solve(bs, xs) == 1
+    for b, x in zip(bs, xs):
+        r = b.getArray()
+        uo = np.zeros_like(r)
+        uo[free] = np.linalg.solve(K[np.ix_(free, free)], r[free])
+        assert np.abs(x.getArray() - uo).max() <= 1e-7 * np.abs(uo).max()
+
+
+def test_shell_recycling_solve_with_chebyshev():
+    p = ShellProblem(8, dv="element", pressure=1e4, bc="ss", t0=0.01)
+    p.set_thickness(np.random.default_rng(4).uniform(0.005, 0.02, p.mesh.nelem))
+    p.assemble()
+    pc = TACS.Pc(p.mat)
+    pc.setChebyshev(4, 50.0)
+    pc.factor()
+    ksm = TACS.MixedKSM(p.mat, pc, 60, 1000, deflation=10)
+    ksm.setTolerances(1e-9, 1e-30)
+    x = p.assembler.createVec()
+    assert ksm.solve(p.res, x) == 1
+    K, *_ = _dense(p)
+    r = p.res.getArray()
+    assert np.linalg.norm(K @ x.getArray() - r) <= 2e-9 * np.linalg.norm(r)
