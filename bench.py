@@ -636,7 +636,9 @@ def simp_main(args, L, ev):
     log(f"setup {time.perf_counter() - t0:.1f}s: {prob.mesh.nelem} elements, "
         f"{3 * prob.mesh.nnodes} DOF")
     sp = SolidComplianceProblem(prob, p=3.0, E0=1.0, Emin=1e-9, rtol=args.rtol, m=args.m,
-                                deflation=args.deflation, restarts=2000)
+                                deflation=args.deflation, restarts=2000,
+                                chebyshev=(CHEB["degree"], CHEB["ratio"])
+                                if CHEB["degree"] > 1 else None)
     if MONITOR["file"]:
         sp.ksm.setMonitor(descript="simp", freq=MONITOR["freq"], filename=MONITOR["file"] + ".simp")
 
@@ -788,6 +790,8 @@ def batched_main(args, L, ev):
     bs = batched_loads(p, R)
     xs = [p.assembler.createVec() for _ in range(R)]
     pc = TACS.Pc(p.mat)
+    if CHEB["degree"] > 1:
+        pc.setChebyshev(CHEB["degree"], CHEB["ratio"])
     ksm = TACS.BatchedKSM(p.mat, pc, m=args.m, restarts=100000, nrhs=R, prec=32,
                           deflation=args.deflation)
     ksm.setTolerances(args.rtol, 1e-30)
@@ -940,6 +944,8 @@ def multi_main(args, L, ev):
         f"{3 * p.nnodes} DOF")
     a = p.assembler
     pc = TACS.Pc(p.mat)
+    if CHEB["degree"] > 1:
+        pc.setChebyshev(CHEB["degree"], CHEB["ratio"])
     union = TACS.MixedKSM(p.mat, pc, args.m, 100000, deflation=args.deflation)
     seg = TACS.SegmentedKSM(p.mat, pc, p.node_off, m=args.m, restarts=100000, prec=32,
                             deflation=args.deflation)
@@ -1280,8 +1286,10 @@ def main():
     ap.add_argument("--nrhs", type=int, default=16, help="load cases of the batched workload")
     ap.add_argument("--cheb-degree", type=int, default=None,
                     help="Chebyshev preconditioner degree of the recycling solvers (1: plain "
-                         "block-Jacobi; default 10 for solids, 1 for shells)")
-    ap.add_argument("--cheb-ratio", type=float, default=300.0)
+                         "block-Jacobi; default 10 for solids, 24 for shells)")
+    ap.add_argument("--cheb-ratio", type=float, default=None,
+                    help="interval [lmax / ratio, lmax] of the Chebyshev polynomial (default 300 "
+                         "for solids, 3000 for shells)")
     ap.add_argument("--multi-solver", default="segmented", choices=["segmented", "union"],
                     help="multi workload: every cube its own system (segmented) or one union "
                          "system")
@@ -1303,7 +1311,9 @@ def main():
         args.deflation = 30 if shell else 20
     MONITOR["file"] = args.monitor
     if args.cheb_degree is None:
-        args.cheb_degree = 1 if shell else 10
+        args.cheb_degree = 24 if shell else 10
+    if args.cheb_ratio is None:
+        args.cheb_ratio = 3000.0 if shell else 300.0
     CHEB.update(degree=args.cheb_degree, ratio=args.cheb_ratio)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "

@@ -372,10 +372,11 @@ class TACSBlockJacobiPc : public TACSPc {
   // M^{-1} x = z_degree of the Chebyshev iteration for A z = x (degree - 1
   // products with the solver's operator values), on the interval
   // [lmax / ratio, lmax] of the eigenvalues of D^{-1} A; factor() estimates
-  // lmax by power iteration (FP32-rounded values, 12 steps, x 1.1).  The
+  // lmax by 30 Lanczos steps (FP32-rounded values, x 1.05).  The
   // smallest eigenvalues are left to the recycled space.  degree 1 (the
-  // default) is plain block-Jacobi.  Not applied to segmented systems.
+  // default) is plain block-Jacobi.
   int setChebyshev(int degree, double ratio);
+  double lanczosMax(int steps);
   int chebDegree = 1;
   double chebRatio = 20.0, chebMax = 0.0;
 };

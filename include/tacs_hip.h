@@ -350,6 +350,10 @@ int tacs_tune_variant(int kernel, int shape, long rows);
 void tacs_tune_reset(void);
 /* Diagonal block positions; inverse diagonal blocks (BCSRMat::factorDiag
    src/bpmat/BCSRMat.cpp:1527); y = D^{-1} x. */
+/* x[b i + r] = 0 where row r of block row i is an identity row (the
+   Dirichlet rows of applyBCs: diagonal value 1, every other value 0). */
+int tacs_hip_bsr_zero_identity_rows(int b, int nrows, const int* rowp, const int* cols,
+                                    const double* A, double* x, hipStream_t stream);
 int tacs_hip_bsr_find_diag(int nrows, const int* rowp, const int* cols, int* diag,
                            hipStream_t stream);
 int tacs_hip_bsr_diag_inverse(int b, int nrows, const int* diag, const double* A, double* Dinv,
@@ -783,13 +787,14 @@ tacs_obj tacs_block_jacobi_pc_create(tacs_obj mat);
 int tacs_pc_factor(tacs_obj pc);
 int tacs_pc_apply(tacs_obj pc, tacs_obj x, tacs_obj y);
 /* Chebyshev polynomial of the block-Jacobi iteration as the preconditioner
-   of the recycling and batched solvers (degree 1..16, 1 = plain
+   of the recycling and batched solvers (degree 1..32, 1 = plain
    block-Jacobi; interval [lmax / ratio, lmax] of D^{-1} A, lmax estimated by
-   the next factor()); GMRES, pc_apply and segmented systems keep plain
-   block-Jacobi.  No counterpart in the reference. */
+   the next factor()); GMRES and pc_apply keep plain block-Jacobi.  No
+   counterpart in the reference. */
 int tacs_pc_set_chebyshev(tacs_obj pc, int degree, double ratio);
-/* the settings and the lmax the last factor() estimated (1.1 x the power
-   iteration's value; 0 before a factor() with degree > 1) */
+/* the settings and the lmax the last factor() estimated (1.05 x the
+   largest Ritz value of 30 Lanczos steps; 0 before a factor() with
+   degree > 1) */
 int tacs_pc_get_chebyshev(tacs_obj pc, int* degree, double* ratio, double* lmax);
 tacs_obj tacs_gmres_create(tacs_obj mat, tacs_obj pc, int m, int nrestart, int is_flexible);
 /* Residual monitors (KSMPrintStdout src/bpmat/KSM.h:293, KSMPrintFile :310;

@@ -230,7 +230,8 @@ class Pc(TACSHandle):
         """Chebyshev polynomial (degree - 1 operator products per
         application) of the block-Jacobi iteration as the preconditioner of
         the recycling and batched solvers, on [lmax / ratio, lmax] of the
-        eigenvalues of D^{-1} A (lmax estimated at the next factor()).
+        eigenvalues of D^{-1} A (lmax estimated at the next factor(): 1.05 x
+        the largest Ritz value of 30 Lanczos steps).
         degree 1: plain block-Jacobi.  No counterpart in the reference."""
         check(lib().tacs_pc_set_chebyshev(self.ptr, int(degree), float(ratio)), "Pc.setChebyshev")
 

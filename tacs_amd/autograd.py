@@ -36,13 +36,16 @@ class SolidComplianceProblem:
     solver for repeated forward / backward passes."""
 
     def __init__(self, problem, p=3.0, E0=1.0, Emin=1e-9, rtol=1e-8, m=120, mixed=True,
-                 deflation=0, restarts=100000):
+                 deflation=0, restarts=100000, chebyshev=None):
         self.prob = problem
         self.p, self.E0, self.Emin, self.rtol = p, E0, Emin, rtol
         a = problem.assembler
         self.dv = a.createDesignVec()
         self.dfdt = a.createDesignVec()
         self.pc = TACS.Pc(problem.mat)
+        if chebyshev and deflation:
+            # (degree, ratio): the recycling solvers' Chebyshev preconditioner
+            self.pc.setChebyshev(*chebyshev)
         if mixed:
             # deflation > 0: the recycling solver (its space carries over to
             # the adjoint solve with the same matrix)
@@ -130,12 +133,14 @@ class ThicknessFunctionProblem:
     :5033), the element strains recomputed in the kernels from u and psi."""
 
     def __init__(self, problem, func="ks", ks_weight=100.0, rtol=1e-8, m=60, mixed=True,
-                 deflation=20, restarts=100000):
+                 deflation=20, restarts=100000, chebyshev=None):
         self.prob = problem
         a = problem.assembler
         self.dv = a.createDesignVec()
         self.dfdt = a.createDesignVec()
         self.pc = TACS.Pc(problem.mat)
+        if chebyshev and deflation:
+            self.pc.setChebyshev(*chebyshev)
         if mixed:
             self.ksm = TACS.MixedKSM(problem.mat, self.pc, m, restarts, deflation=deflation)
         elif deflation:

@@ -473,6 +473,25 @@ __global__ void convert_f64_to_f32(size_t n, const double* __restrict__ x, float
   for (; i < n; i += stride) y[i] = (float)x[i];
 }
 
+// x[b i + r] = 0 where row r of block row i is an identity row (Dirichlet
+// row: diagonal value 1, every other value of the row 0).
+__global__ void bsr_zero_identity_rows(int b, int nrows, const int* __restrict__ rowp,
+                                       const int* __restrict__ cols,
+                                       const double* __restrict__ A, double* __restrict__ x) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nrows * b) return;
+  const int node = (int)(t / b), r = (int)(t % b);
+  bool ident = true;
+  for (int s = rowp[node]; s < rowp[node + 1] && ident; s++) {
+    const double* a = &A[((size_t)s * b + r) * b];
+    for (int c = 0; c < b; c++) {
+      const double want = (cols[s] == node && c == r) ? 1.0 : 0.0;
+      if (a[c] != want) ident = false;
+    }
+  }
+  if (ident) x[t] = 0.0;
+}
+
 extern "C" {
 
 int tacs_hip_bsr_spmv(int b, int nrows, const int* rowp, const int* cols, const double* A,
@@ -695,6 +714,16 @@ int tacs_hip_convert_f64_f32(size_t n, const double* x, float* y, hipStream_t st
   if (n == 0) return 0;
   int grid = (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
   hipLaunchKernelGGL(convert_f64_to_f32, dim3(grid), dim3(256), 0, stream, n, x, y);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_bsr_zero_identity_rows(int b, int nrows, const int* rowp, const int* cols,
+                                    const double* A, double* x, hipStream_t stream) {
+  if (b < 1 || b > 8) return TACS_HIP_ERR_ARG;
+  const size_t n = (size_t)nrows * b;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(bsr_zero_identity_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     stream, b, nrows, rowp, cols, A, x);
   return (int)hipGetLastError();
 }
 

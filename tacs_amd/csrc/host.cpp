@@ -709,38 +709,101 @@ void TACSBlockJacobiPc::factor() {
   d2h(&hs, st, 1);
   dfree(st);
   if (hs) tacs_set_error("TACSBlockJacobiPc::factor: singular or missing diagonal block (%d)", hs);
-  if (chebDegree > 1 && !g_halt) {
-    // largest eigenvalue of D^{-1} A (FP32-rounded values, as the solvers
-    // apply the polynomial) by power iteration from a hashed start vector
-    const size_t n = (size_t)mat->nrows * mat->bsize;
-    double* v = dalloc<double>(n);
-    double* w = dalloc<double>(n);
-    double* nr = dalloc<double>(1);
-    double lam = 0.0;
-    if (!g_halt) {
-      KCHECK(tacs_hip_fill_random(n, v, 12345, g_stream));
-      for (int it = 0; it < 12 && !g_halt; it++) {
-        mat->multMixed(v, w);
-        KCHECK(tacs_hip_bdiag_apply(mat->bsize, 32, mat->nrows, 1, d_Dinv32, w, v, g_stream));
-        KCHECK(tacs_hip_mdot_noalloc(64, n, 1, v, n, v, nr, 0, g_stream));
-        double h = 0.0;
-        d2h(&h, nr, 1);
-        const double nv = std::sqrt(h);
-        if (!(nv > 0.0)) break;
-        if (it > 0) lam = nv;  // |D^{-1} A v| for the unit vector v of the last step
-        KCHECK(tacs_hip_scale(64, n, v, NULL, 1.0 / nv, 0, g_stream));
-      }
-    }
-    dfree(v);
-    dfree(w);
-    dfree(nr);
-    chebMax = 1.1 * lam;
-    if (!(chebMax > 0.0)) tacs_set_error("TACSBlockJacobiPc::factor: no eigenvalue estimate");
+  if (chebDegree > 1 && !g_halt) chebMax = 1.05 * lanczosMax(30);
+}
+// Largest eigenvalue of D^{-1} A (FP32-rounded values, as the solvers apply
+// the polynomial): D^{-1} A is self-adjoint in the A inner product, so
+// Lanczos in that inner product (A v carried along: one product and one
+// block-Jacobi application per step) gives Ritz values from below that reach
+// the top of the spectrum in a few tens of steps (Golub & Van Loan, Matrix
+// Computations, 4th ed., sec. 10.1); the largest eigenvalue of the
+// tridiagonal matrix comes from Sturm-sequence bisection.  A hashed start
+// vector makes it reproducible.
+double TACSBlockJacobiPc::lanczosMax(int steps) {
+  const size_t n = (size_t)mat->nrows * mat->bsize;
+  double* buf = dalloc<double>(6 * n + 2);
+  if (g_halt) {
+    dfree(buf);
+    return 0.0;
   }
+  double *v = buf, *u = buf + n, *vp = buf + 2 * n, *up = buf + 3 * n, *w = buf + 4 * n,
+         *aw = buf + 5 * n, *dot = buf + 6 * n;
+  auto dotp = [&](const double* x, const double* y) {
+    KCHECK(tacs_hip_mdot_noalloc(64, n, 1, x, n, y, dot, 0, g_stream));
+    double h = 0.0;
+    d2h(&h, dot, 1);
+    return h;
+  };
+  // the start vector vanishes on the Dirichlet (identity) rows: the
+  // iteration then stays in the free dofs, where A is symmetric positive
+  // definite (the identity rows only add the eigenvalue 1)
+  KCHECK(tacs_hip_fill_random(n, v, 12345, g_stream));
+  KCHECK(tacs_hip_bsr_zero_identity_rows(mat->bsize, mat->nrows, mat->d_rowp, mat->d_cols,
+                                         mat->d_A, v, g_stream));
+  mat->multMixed(v, u);
+  double g2 = dotp(v, u);
+  std::vector<double> al, be;  // T: diagonal al, off-diagonal be
+  if (g2 > 0.0) {
+    const double g = 1.0 / std::sqrt(g2);
+    KCHECK(tacs_hip_scale(64, n, v, NULL, g, 0, g_stream));
+    KCHECK(tacs_hip_scale(64, n, u, NULL, g, 0, g_stream));
+    KCHECK(tacs_hip_fill(64, n, vp, 0.0, g_stream));
+    KCHECK(tacs_hip_fill(64, n, up, 0.0, g_stream));
+    double beta = 0.0;
+    for (int k = 0; k < steps && !g_halt; k++) {
+      KCHECK(tacs_hip_bdiag_apply(mat->bsize, 32, mat->nrows, 1, d_Dinv32, u, w, g_stream));
+      const double alpha = dotp(u, w);
+      mat->multMixed(w, aw);
+      // w -= alpha v + beta v_prev, and A w alike
+      KCHECK(tacs_hip_axpby(64, 64, n, -alpha, v, 1.0, w, g_stream));
+      KCHECK(tacs_hip_axpby(64, 64, n, -beta, vp, 1.0, w, g_stream));
+      KCHECK(tacs_hip_axpby(64, 64, n, -alpha, u, 1.0, aw, g_stream));
+      KCHECK(tacs_hip_axpby(64, 64, n, -beta, up, 1.0, aw, g_stream));
+      al.push_back(alpha);
+      const double b2 = dotp(w, aw);
+      if (!(b2 > 1e-30 * alpha * alpha) || k == steps - 1) break;
+      beta = std::sqrt(b2);
+      be.push_back(beta);
+      std::swap(vp, v);
+      std::swap(up, u);
+      // v = w / beta, u = A w / beta (into the old v_prev, u_prev storage)
+      HCHECK(hipMemcpyAsync(v, w, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
+      HCHECK(hipMemcpyAsync(u, aw, n * sizeof(double), hipMemcpyDeviceToDevice, g_stream));
+      KCHECK(tacs_hip_scale(64, n, v, NULL, 1.0 / beta, 0, g_stream));
+      KCHECK(tacs_hip_scale(64, n, u, NULL, 1.0 / beta, 0, g_stream));
+    }
+  }
+  dfree(buf);
+  const int m = (int)al.size();
+  if (m == 0 || g_halt) return 0.0;
+  // largest eigenvalue of T by bisection on the Sturm count
+  double lo = al[0], hi = al[0];
+  for (int i = 0; i < m; i++) {
+    const double r = (i > 0 ? std::fabs(be[i - 1]) : 0.0) + (i < m - 1 ? std::fabs(be[i]) : 0.0);
+    lo = std::min(lo, al[i] - r);
+    hi = std::max(hi, al[i] + r);
+  }
+  auto count_below = [&](double x) {  // eigenvalues of T less than x
+    int c = 0;
+    double q = al[0] - x;
+    if (q < 0) c++;
+    for (int i = 1; i < m; i++) {
+      if (q == 0.0) q = 1e-300;
+      q = al[i] - x - be[i - 1] * be[i - 1] / q;
+      if (q < 0) c++;
+    }
+    return c;
+  };
+  for (int it = 0; it < 200 && hi - lo > 1e-12 * std::fabs(hi); it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (count_below(mid) >= m) hi = mid;  // every eigenvalue below mid
+    else lo = mid;
+  }
+  return hi;
 }
 int TACSBlockJacobiPc::setChebyshev(int degree, double ratio) {
-  if (degree < 1 || degree > 16 || !(ratio > 1.0))
-    return tacs_set_error("setChebyshev: 1 <= degree <= 16 and ratio > 1");
+  if (degree < 1 || degree > 32 || !(ratio > 1.0))
+    return tacs_set_error("setChebyshev: 1 <= degree <= 32 and ratio > 1");
   chebDegree = degree;
   chebRatio = ratio;
   chebMax = 0.0;
@@ -1666,30 +1729,49 @@ void TACSBatchedGMRES::bdiag(int pr, int Rn, const void* Dinv, const double* x, 
 void TACSBatchedGMRES::precond(int pr, int Rn, const void* Dinv, const float* A32,
                                const double* x, double* y) {
   const int deg = pc->chebDegree;
-  if (deg <= 1 || nseg > 0) {
+  if (deg <= 1) {
     bdiag(pr, Rn, Dinv, x, y);
     return;
   }
   // Chebyshev iteration for A z = x on [lmax / ratio, lmax] (Saad, Alg. 12.1),
-  // the iterate z kept in y (y and x are distinct buffers at every call)
+  // the iterate z kept in the output (distinct from the input at every call);
+  // segmented systems run it on whole-matrix vectors (their blocks are the
+  // diagonal blocks of the matrix, so the polynomial acts on each alone)
   if (!d_cd) {
-    const size_t Nmax = (size_t)mat->nrows * mat->bsize * R;
+    const size_t Nmax = (size_t)mat->nrows * mat->bsize * (nseg > 0 ? 1 : R);
     d_cd = dalloc<double>(Nmax);
     d_caz = dalloc<double>(Nmax);
     if (g_halt) return;
   }
   const double b = pc->chebMax, a = b / pc->chebRatio;
   const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
+  const int rc = nseg > 0 ? 1 : Rn;
+  const double* xc = x;
+  double* zc = y;
+  if (nseg > 0) {
+    KCHECK(tacs_hip_seg_to_union(segMaxNodes, Rn, d_segoff, x, d_ux, g_stream));
+    xc = d_ux;
+    zc = d_uy;
+  }
+  auto mult = [&](const double* in, double* out) {
+    if (nseg == 0)
+      bmult(pr, Rn, A32, in, out);
+    else if (pr == 32)
+      mat->multMixed(in, out);
+    else
+      mat->mult64(in, out);
+  };
   double rho = 1.0 / sigma;
-  KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, Rn, Dinv, x, NULL, d_cd, y, 0.0,
+  KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, rc, Dinv, xc, NULL, d_cd, zc, 0.0,
                              1.0 / theta, 1, g_stream));
   for (int k = 1; k < deg; k++) {
-    bmult(pr, Rn, A32, y, d_caz);
+    mult(zc, d_caz);
     const double rn = 1.0 / (2.0 * sigma - rho);
-    KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, Rn, Dinv, x, d_caz, d_cd, y, rn * rho,
+    KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, rc, Dinv, xc, d_caz, d_cd, zc, rn * rho,
                                2.0 * rn / delta, 0, g_stream));
     rho = rn;
   }
+  if (nseg > 0) KCHECK(tacs_hip_seg_from_union(segMaxNodes, Rn, d_segoff, zc, y, g_stream));
 }
 void TACSBatchedGMRES::bmult(int pr, int Rn, const float* A32, const double* x, double* y) {
   if (nseg > 0) {

@@ -76,13 +76,17 @@ def test_union_solve_reaches_each_cube_tolerance(deflation):
         assert np.abs(uc - uo).max() <= 1e-6 * np.abs(uo).max(), c
 
 
-@pytest.mark.parametrize("prec,deflation", [(64, 0), (32, 0), (32, 8), (64, 8)])
-def test_segmented_solve_per_cube_against_oracle(prec, deflation):
+@pytest.mark.parametrize("prec,deflation,cheb", [(64, 0, 1), (32, 0, 1), (32, 8, 1), (64, 8, 1),
+                                                (32, 8, 6)])
+def test_segmented_solve_per_cube_against_oracle(prec, deflation, cheb):
     """Per-cube Krylov spaces (SegmentedKSM): every cube converges on its own
-    test and matches the oracle's dense solve of that cube."""
+    test and matches the oracle's dense solve of that cube (with block-Jacobi
+    or its Chebyshev polynomial)."""
     p = MultiCubeProblem(SIZES)
     p.assemble()
     pc = TACS.Pc(p.mat)
+    if cheb > 1:
+        pc.setChebyshev(cheb, 60.0)
     pc.factor()
     rtol = 1e-10
     ksm = TACS.SegmentedKSM(p.mat, pc, p.node_off, m=20, restarts=1000, prec=prec,
