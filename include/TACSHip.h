@@ -313,6 +313,11 @@ class TACSParallelMat : public TACSMat {
   void mult64(const double* x, double* y);
   void multMixed(const double* x, double* y);
   void mult32(const float* x, float* y);
+  // one Chebyshev step for A z = rhs fused into the product (values of
+  // precision pr, Dinv in that precision): d = c1 d + c2 D^{-1} (rhs - A z),
+  // zout = z + d
+  void chebStep(int pr, const void* Dinv, const double* z, double* zout, const double* rhs,
+                double* d, double c1, double c2);
   int bsize, nrows;
   long nnzb;
   long version;  // incremented whenever the values change

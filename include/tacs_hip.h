@@ -323,6 +323,15 @@ int tacs_hip_sell_spmv(int b, int prec_vals, int prec_vecs, int nrows, const int
    blocks U in {1, 2} with (0, 1) or without (2, 3).  tacs_hip_sell_spmv
    launches the variant of the tuning table. */
 int tacs_hip_sell_spmv_nvariants(int b);
+/* One Chebyshev step for A z = rhs fused into the sliced ELL product (FP64
+   vectors; prec_vals 64 or 32, Dinv the b x b inverse diagonal blocks in the
+   same precision): per row, a = (A z)_row, w = D^{-1} (rhs - a),
+   d = c1 d + c2 w, zout = z + d.  zout must differ from z.  variant as for
+   tacs_hip_sell_spmv_variant. */
+int tacs_hip_sell_cheb_step(int variant, int b, int prec_vals, int nrows, const int* width,
+                            const long* coff, const int* cs, const void* vs, const double* z,
+                            double* zout, const double* rhs, const void* Dinv, double* d,
+                            double c1, double c2, hipStream_t stream);
 int tacs_hip_sell_spmv_variant(int variant, int b, int prec_vals, int prec_vecs, int nrows,
                                const int* width, const long* coff, const int* cs, const void* vs,
                                const void* x, void* y, hipStream_t stream);

@@ -345,13 +345,25 @@ __device__ inline void sell_slot(const TA* __restrict__ vs, long v, const T* __r
 // every U gives the same sums.  NT: the values are read with non-temporal
 // loads (streamed past the caches, which keep x).  The launch takes U and
 // NT from the tuning table (sell_variant).
-template <int B, int U, typename TA, typename T, bool NT = true>
+// Chebyshev epilogue of the product (CHEB): with the product a = A z of
+// the row, w = D^{-1} (rhs - a), d = c1 d + c2 w and zout = z + d for the
+// row (one step of the Chebyshev iteration for A z = rhs, Saad, Alg. 12.1),
+// instead of writing a.  zout must not be z (other rows still read z).
+struct SellCheb {
+  const void* Dinv;  // b x b inverse diagonal blocks, the value type of the matrix
+  const double* rhs;
+  double* d;
+  double* zout;
+  double c1, c2;
+};
+template <int B, int U, typename TA, typename T, bool NT = true, bool CHEB = false>
 __global__ __launch_bounds__(256) void sell_spmv(int nslices, int nrows,
                                                  const int* __restrict__ width,
                                                  const long* __restrict__ coff,
                                                  const int* __restrict__ cs,
                                                  const TA* __restrict__ vs,
-                                                 const T* __restrict__ x, T* __restrict__ y) {
+                                                 const T* __restrict__ x, T* __restrict__ y,
+                                                 SellCheb ch = SellCheb{}) {
   constexpr int B2 = B * B;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int s = blockIdx.x * 4 + wid; s < nslices; s += gridDim.x * 4) {
@@ -391,8 +403,25 @@ __global__ __launch_bounds__(256) void sell_spmv(int nslices, int nrows,
     for (; t < w; t++) sell_slot<B, TA, T, NT>(vs, vo + 64 * B2 * t, x, cs[co + 64 * t], yr);
     const int row = 64 * s + lane;
     if (row < nrows) {
+      if constexpr (CHEB) {
+        const TA* D = (const TA*)ch.Dinv + (size_t)B2 * row;
+        double rr[B];
 #pragma unroll
-      for (int r = 0; r < B; r++) y[(size_t)B * row + r] = (T)yr[r];
+        for (int r = 0; r < B; r++) rr[r] = ch.rhs[(size_t)B * row + r] - yr[r];
+#pragma unroll
+        for (int r = 0; r < B; r++) {
+          double wv = (double)D[B * r] * rr[0];
+#pragma unroll
+          for (int c = 1; c < B; c++) wv += (double)D[B * r + c] * rr[c];
+          const size_t o = (size_t)B * row + r;
+          const double dn = ch.c1 * ch.d[o] + ch.c2 * wv;
+          ch.d[o] = dn;
+          ch.zout[o] = (double)x[o] + dn;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < B; r++) y[(size_t)B * row + r] = (T)yr[r];
+      }
     }
   }
 }
@@ -643,6 +672,50 @@ int tacs_hip_sell_spmv_variant(int variant, int b, int prec_vals, int prec_vecs,
     }
   }
 #undef SELL_SPMV
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_sell_cheb_step(int variant, int b, int prec_vals, int nrows, const int* width,
+                            const long* coff, const int* cs, const void* vs, const double* z,
+                            double* zout, const double* rhs, const void* Dinv, double* d,
+                            double c1, double c2, hipStream_t stream) {
+  const int ns = (nrows + 63) / 64;
+  if ((b != 3 && b != 6) || (prec_vals != 64 && prec_vals != 32) || z == zout) return TACS_HIP_ERR_ARG;
+  if (variant < 0 || variant >= tacs_hip_sell_spmv_nvariants(b)) return TACS_HIP_ERR_ARG;
+  if (ns <= 0) return 0;
+  const int grid = tacs_grid_for(ns, 4, 65536);
+  const SellCheb ch{Dinv, rhs, d, zout, c1, c2};
+#define SELL_CHEB(BB, U, NT)                                                                     \
+  do {                                                                                           \
+    if (prec_vals == 64)                                                                         \
+      hipLaunchKernelGGL((sell_spmv<BB, U, double, double, NT, true>), dim3(grid), dim3(256), 0, \
+                         stream, ns, nrows, width, coff, cs, (const double*)vs, z,              \
+                         (double*)NULL, ch);                                                     \
+    else                                                                                         \
+      hipLaunchKernelGGL((sell_spmv<BB, U, float, double, NT, true>), dim3(grid), dim3(256), 0,  \
+                         stream, ns, nrows, width, coff, cs, (const float*)vs, z,               \
+                         (double*)NULL, ch);                                                     \
+  } while (0)
+  if (b == 3) {
+    switch (variant) {
+      case 0: SELL_CHEB(3, 1, true); break;
+      case 1: SELL_CHEB(3, 2, true); break;
+      case 2: SELL_CHEB(3, 4, true); break;
+      case 3: SELL_CHEB(3, 8, true); break;
+      case 4: SELL_CHEB(3, 1, false); break;
+      case 5: SELL_CHEB(3, 2, false); break;
+      case 6: SELL_CHEB(3, 4, false); break;
+      default: SELL_CHEB(3, 8, false); break;
+    }
+  } else {
+    switch (variant) {
+      case 0: SELL_CHEB(6, 1, true); break;
+      case 1: SELL_CHEB(6, 2, true); break;
+      case 2: SELL_CHEB(6, 1, false); break;
+      default: SELL_CHEB(6, 2, false); break;
+    }
+  }
+#undef SELL_CHEB
   return (int)hipGetLastError();
 }
 

@@ -548,6 +548,17 @@ void TACSParallelMat::mult32(const float* x, float* y) {
   KCHECK(tacs_hip_sell_spmv_variant(sell_kind_variant(this, 2), bsize, 32, 32, nrows, d_sw, d_soff,
                                     d_scols, d_s32, x, y, g_stream));
 }
+// One Chebyshev step fused into the product with the values of precision pr
+// (zout = z + d after d = c1 d + c2 D^{-1} (rhs - A z)).
+void TACSParallelMat::chebStep(int pr, const void* Dinv, const double* z, double* zout,
+                               const double* rhs, double* d, double c1, double c2) {
+  const int kind = pr == 64 ? 0 : 1;
+  if (pr == 64 && sellVer64 != version) sellUpdate(64);
+  if (pr == 32 && sellVer32 != version) sellUpdate(32);
+  KCHECK(tacs_hip_sell_cheb_step(sell_kind_variant(this, kind), bsize, pr, nrows, d_sw, d_soff,
+                                 d_scols, pr == 64 ? (const void*)d_s64 : (const void*)d_s32, z,
+                                 zout, rhs, Dinv, d, c1, c2, g_stream));
+}
 // Average time of reps launches of fn on the library stream (one untimed
 // launch first), in ms; negative after a failure.
 template <typename F>
@@ -1762,14 +1773,29 @@ void TACSBatchedGMRES::precond(int pr, int Rn, const void* Dinv, const float* A3
       mat->mult64(in, out);
   };
   double rho = 1.0 / sigma;
-  KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, rc, Dinv, xc, NULL, d_cd, zc, 0.0,
-                             1.0 / theta, 1, g_stream));
-  for (int k = 1; k < deg; k++) {
-    mult(zc, d_caz);
-    const double rn = 1.0 / (2.0 * sigma - rho);
-    KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, rc, Dinv, xc, d_caz, d_cd, zc, rn * rho,
-                               2.0 * rn / delta, 0, g_stream));
-    rho = rn;
+  if (rc == 1) {
+    // one system: the step fused into the product, iterates alternating
+    // between two buffers so that the last lands in zc
+    double* buf[2];
+    buf[(deg - 1) % 2] = zc;
+    buf[deg % 2] = d_caz;
+    KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, 1, Dinv, xc, NULL, d_cd, buf[0], 0.0,
+                               1.0 / theta, 1, g_stream));
+    for (int k = 1; k < deg; k++) {
+      const double rn = 1.0 / (2.0 * sigma - rho);
+      mat->chebStep(pr, Dinv, buf[(k - 1) % 2], buf[k % 2], xc, d_cd, rn * rho, 2.0 * rn / delta);
+      rho = rn;
+    }
+  } else {
+    KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, rc, Dinv, xc, NULL, d_cd, zc, 0.0,
+                               1.0 / theta, 1, g_stream));
+    for (int k = 1; k < deg; k++) {
+      mult(zc, d_caz);
+      const double rn = 1.0 / (2.0 * sigma - rho);
+      KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, rc, Dinv, xc, d_caz, d_cd, zc,
+                                 rn * rho, 2.0 * rn / delta, 0, g_stream));
+      rho = rn;
+    }
   }
   if (nseg > 0) KCHECK(tacs_hip_seg_from_union(segMaxNodes, Rn, d_segoff, zc, y, g_stream));
 }
