@@ -114,10 +114,34 @@ def time_spmv(L, ev, p, reps=50):
     return res
 
 
+def time_cheb(L, ev, p, pc, reps=50, values=16):
+    """Average duration of the fused Chebyshev product of the recycling
+    solver's preconditioner (values in bfloat16 or FP32, FP64 vectors; HIP
+    events on the library's stream) and its algorithmic bytes: the values
+    and column indices once, row pointers, z gathered once, rhs, d read and
+    written, zout written, the FP32 inverse diagonal blocks."""
+    b, nr, nnz = p.mat.getInfo()
+    vecs = [p.assembler.createVec() for _ in range(4)]
+    for k, v in enumerate(vecs):
+        v.set(1.0 + 0.1 * k)
+    z, zo, rhs, d = [v.getDevicePointer() for v in vecs]
+    for _ in range(5):
+        assert L.tacs_mat_cheb_step(p.mat.ptr, pc.ptr, values, z, zo, rhs, d, 0.5, 0.1) == 0
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0)
+    for _ in range(reps):
+        L.tacs_mat_cheb_step(p.mat.ptr, pc.ptr, values, z, zo, rhs, d, 0.5, 0.1)
+    ev.record(e1)
+    ms = ev.elapsed_ms(e0, e1) / reps
+    word = 2 if values == 16 else 4
+    nbytes = word * b * b * nnz + 4 * nnz + 4 * (nr + 1) + 8 * b * nr * 5 + 4 * b * b * nr
+    return ms, nbytes
+
+
 MONITOR = {"file": None, "freq": 1000}
 
 
-CHEB = {"degree": 1, "ratio": 300.0}
+CHEB = {"degree": 1, "ratio": 300.0, "values": 16}
 
 
 def make_solver(p, solver, m, rtol, deflation=0):
@@ -128,7 +152,7 @@ def make_solver(p, solver, m, rtol, deflation=0):
     from tacs_amd import TACS
     pc = TACS.Pc(p.mat)
     if solver in ("mixed", "recycle64") and CHEB["degree"] > 1:
-        pc.setChebyshev(CHEB["degree"], CHEB["ratio"])
+        pc.setChebyshev(CHEB["degree"], CHEB["ratio"], CHEB["values"])
     if solver == "mixed":
         ksm = TACS.MixedKSM(p.mat, pc, m, 100000, deflation=deflation)
     elif solver == "recycle64":
@@ -637,7 +661,7 @@ def simp_main(args, L, ev):
         f"{3 * prob.mesh.nnodes} DOF")
     sp = SolidComplianceProblem(prob, p=3.0, E0=1.0, Emin=1e-9, rtol=args.rtol, m=args.m,
                                 deflation=args.deflation, restarts=2000,
-                                chebyshev=(CHEB["degree"], CHEB["ratio"])
+                                chebyshev=(CHEB["degree"], CHEB["ratio"], CHEB["values"])
                                 if CHEB["degree"] > 1 else None)
     if MONITOR["file"]:
         sp.ksm.setMonitor(descript="simp", freq=MONITOR["freq"], filename=MONITOR["file"] + ".simp")
@@ -791,7 +815,7 @@ def batched_main(args, L, ev):
     xs = [p.assembler.createVec() for _ in range(R)]
     pc = TACS.Pc(p.mat)
     if CHEB["degree"] > 1:
-        pc.setChebyshev(CHEB["degree"], CHEB["ratio"])
+        pc.setChebyshev(CHEB["degree"], CHEB["ratio"], CHEB["values"])
     ksm = TACS.BatchedKSM(p.mat, pc, m=args.m, restarts=100000, nrhs=R, prec=32,
                           deflation=args.deflation)
     ksm.setTolerances(args.rtol, 1e-30)
@@ -945,7 +969,7 @@ def multi_main(args, L, ev):
     a = p.assembler
     pc = TACS.Pc(p.mat)
     if CHEB["degree"] > 1:
-        pc.setChebyshev(CHEB["degree"], CHEB["ratio"])
+        pc.setChebyshev(CHEB["degree"], CHEB["ratio"], CHEB["values"])
     union = TACS.MixedKSM(p.mat, pc, args.m, 100000, deflation=args.deflation)
     seg = TACS.SegmentedKSM(p.mat, pc, p.node_off, m=args.m, restarts=100000, prec=32,
                             deflation=args.deflation)
@@ -1287,6 +1311,9 @@ def main():
     ap.add_argument("--cheb-degree", type=int, default=None,
                     help="Chebyshev preconditioner degree of the recycling solvers (1: plain "
                          "block-Jacobi; default 10 for solids, 24 for shells)")
+    ap.add_argument("--cheb-values", type=int, default=16, choices=[32, 16],
+                    help="precision of the matrix values the Chebyshev polynomial's products read "
+                         "(16: bfloat16 copy)")
     ap.add_argument("--cheb-ratio", type=float, default=None,
                     help="interval [lmax / ratio, lmax] of the Chebyshev polynomial (default 300 "
                          "for solids, 3000 for shells)")
@@ -1314,7 +1341,7 @@ def main():
         args.cheb_degree = 24 if shell else 10
     if args.cheb_ratio is None:
         args.cheb_ratio = 3000.0 if shell else 300.0
-    CHEB.update(degree=args.cheb_degree, ratio=args.cheb_ratio)
+    CHEB.update(degree=args.cheb_degree, ratio=args.cheb_ratio, values=args.cheb_values)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
               "described in DESIGN.md and not built", file=sys.stderr)
@@ -1371,6 +1398,13 @@ def main():
         sp["fp64_GBps"] = sp["fp64_bytes"] / sp["fp64_ms"] / 1e6
         sp["fp32_GBps"] = sp["fp32_bytes"] / sp["fp32_ms"] / 1e6
         sp["mixed_GBps"] = sp["mixed_bytes"] / sp["mixed_ms"] / 1e6
+        # the fused Chebyshev product of the preconditioner (bfloat16 values)
+        from tacs_amd import TACS
+        pck = TACS.Pc(p.mat)
+        pck.setChebyshev(10, 300.0, 16)
+        pck.factor()
+        cms, cbytes = time_cheb(L, ev, p, pck)
+        sp.update(cheb_bf16_ms=cms, cheb_bf16_bytes=cbytes, cheb_bf16_GBps=cbytes / cms / 1e6)
         print(json.dumps(sp))
         return
     pc, ksm = make_solver(p, "mixed", args.m, args.rtol, args.deflation)
@@ -1415,6 +1449,7 @@ def main():
     sp = time_spmv(L, ev, p)
     peak_bw = 8000.0
     bw64 = sp["fp64_bytes"] / sp["fp64_ms"] / 1e6
+    cheb = time_cheb(L, ev, p, pc, values=CHEB["values"]) if CHEB["degree"] > 1 else None
     bw32 = sp["fp32_bytes"] / sp["fp32_ms"] / 1e6
     bwm = sp["mixed_bytes"] / sp["mixed_ms"] / 1e6
     jac_flops = 69120.0 * p.mesh.nelem  # dense B^T C B count per 8-node element
@@ -1451,7 +1486,9 @@ def main():
                                f"traction 1e6 Pa on x=1, per-element thickness DVs; assembly + "
                                f"mixed GMRES({args.m}) recycling {args.deflation} vectors, "
                                + (f"Chebyshev degree {args.cheb_degree} on [lmax/"
-                                  f"{args.cheb_ratio:g}, lmax] of block-Jacobi"
+                                  f"{args.cheb_ratio:g}, lmax] of block-Jacobi (products inside "
+                                  f"the polynomial on {'bfloat16' if args.cheb_values == 16 else 'FP32'}"
+                                  f" values)"
                                   if args.cheb_degree > 1 else "block-Jacobi") +
                                f", solve + compliance adjoint "
                                f"to {args.rtol:g} relative residual; plain path FP64 "
@@ -1469,14 +1506,17 @@ def main():
                                "read (rebuilt every step, the matrix is reassembled)",
         "solve_adjoint_s_mixed": mixed["t"]["solve_adjoint_s"],
         "iterations_mixed": list(mixed["its"]),
-        "roofline": {"bound": "hbm", "achieved": bw64, "peak": peak_bw, "unit": "GB/s",
-                     "frac": bw64 / peak_bw, "traffic": spmv_traffic("sell_spmv<3,2,double,double>"),
-                     "kernel": "sell3_spmv<double,double> (FP64 3x3 block SpMV over the "
-                               "sliced ELL copy, plain path)",
-                     "avg_ms": sp["fp64_ms"], "algorithmic_bytes": sp["fp64_bytes"],
-                     "traffic_source": "profiles/spmv_traffic_hex8_128cube.json (rocprofv3 "
-                                       "FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)"},
+        "roofline_fp64_spmv": {"bound": "hbm", "achieved": bw64, "peak": peak_bw,
+                               "unit": "GB/s", "frac": bw64 / peak_bw,
+                               "traffic": spmv_traffic("sell_spmv<3,2,double,double>"),
+                               "kernel": "sell3_spmv<double,double> (FP64 3x3 block SpMV over "
+                                         "the sliced ELL copy, plain path)",
+                               "avg_ms": sp["fp64_ms"], "algorithmic_bytes": sp["fp64_bytes"],
+                               "traffic_source": "profiles/spmv_traffic_hex8_128cube.json "
+                                                 "(rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                                 "copy-kernel scaled)"},
         "roofline_mixed_spmv": {"bound": "hbm", "achieved": bwm, "peak": peak_bw,
+                                "traffic": spmv_traffic("sell_spmv<3,4,float,double>"),
                                 "unit": "GB/s", "frac": bwm / peak_bw, "avg_ms": sp["mixed_ms"],
                                 "algorithmic_bytes": sp["mixed_bytes"],
                                 "kernel": "sell3_spmv<float,double> (recycling solver operator)"},
@@ -1490,6 +1530,25 @@ def main():
                               "note": "69,120 FLOP/element dense B^T C B count over the "
                                       "Jacobian alone; MI355X FP64 peak"},
     }
+    if cheb is not None:
+        # the step's dominant kernel: the fused Chebyshev product of the
+        # preconditioner (about half of the step's kernel time at 128^3)
+        cms, cbytes = cheb
+        ckey = ("sell_spmv<3,4,unsigned short,double,true,true>" if CHEB["values"] == 16
+                else "sell_spmv<3,4,float,double,true,true>")
+        out["roofline"] = {
+            "bound": "hbm", "achieved": cbytes / cms / 1e6, "peak": peak_bw, "unit": "GB/s",
+            "frac": cbytes / cms / 1e6 / peak_bw, "traffic": spmv_traffic(ckey),
+            "kernel": f"{ckey} (one Chebyshev step fused into the 3x3 block product over the "
+                      f"sliced ELL copy, {'bfloat16' if CHEB['values'] == 16 else 'FP32'} "
+                      f"values, FP64 vectors)",
+            "avg_ms": cms, "algorithmic_bytes": cbytes,
+            "traffic_source": "profiles/spmv_traffic_hex8_128cube.json (rocprofv3 FETCH_SIZE + "
+                              "WRITE_SIZE, copy-kernel scaled)",
+            "note": "values and column indices once, row pointers, z gathered once, rhs, d read "
+                    "and written, zout written, FP32 inverse diagonal blocks"}
+    else:
+        out["roofline"] = out["roofline_fp64_spmv"]
     if fp64 is not None:
         g, g64 = mixed["grad"], fp64["grad"]
         out["solve_adjoint_s_fp64"] = fp64["t"]["solve_adjoint_s"]

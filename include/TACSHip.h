@@ -328,7 +328,8 @@ class TACSParallelMat : public TACSMat {
   int* d_scols;
   double* d_s64;
   float* d_s32;
-  long sellVer64, sellVer32;
+  unsigned short* d_s16;  // bfloat16 copy (Chebyshev polynomial products)
+  long sellVer64, sellVer32, sellVer16;
   size_t nslots;
   int *d_rowp, *d_cols;
   double* d_A;
@@ -380,9 +381,11 @@ class TACSBlockJacobiPc : public TACSPc {
   // lmax by 30 Lanczos steps (FP32-rounded values, x 1.05).  The
   // smallest eigenvalues are left to the recycled space.  degree 1 (the
   // default) is plain block-Jacobi.
-  int setChebyshev(int degree, double ratio);
+  // values 16: the products inside the polynomial read a bfloat16 copy of
+  // the values (single systems; the outer operator keeps its values)
+  int setChebyshev(int degree, double ratio, int values = 32);
   double lanczosMax(int steps);
-  int chebDegree = 1;
+  int chebDegree = 1, chebValues = 32;
   double chebRatio = 20.0, chebMax = 0.0;
 };
 

@@ -314,6 +314,11 @@ int tacs_hip_sell_fill(int b, int prec, int nrows, const int* rowp, const int* c
 int tacs_hip_sell_fill2(int b, int nrows, const int* rowp, const int* cols, const int* width,
                         const long* coff, const double* A, double* v64, float* v32, int* cs,
                         hipStream_t stream);
+/* sell_fill3: sell_fill2 plus a bfloat16 copy v16 (the upper 16 bits of the
+   FP32 value, rounded to nearest even), any of the three may be NULL. */
+int tacs_hip_sell_fill3(int b, int nrows, const int* rowp, const int* cols, const int* width,
+                        const long* coff, const double* A, double* v64, float* v32,
+                        unsigned short* v16, int* cs, hipStream_t stream);
 int tacs_hip_sell_spmv(int b, int prec_vals, int prec_vecs, int nrows, const int* width,
                        const long* coff, const int* cs, const void* vs, const void* x, void* y,
                        hipStream_t stream);
@@ -324,8 +329,9 @@ int tacs_hip_sell_spmv(int b, int prec_vals, int prec_vecs, int nrows, const int
    launches the variant of the tuning table. */
 int tacs_hip_sell_spmv_nvariants(int b);
 /* One Chebyshev step for A z = rhs fused into the sliced ELL product (FP64
-   vectors; prec_vals 64 or 32, Dinv the b x b inverse diagonal blocks in the
-   same precision): per row, a = (A z)_row, w = D^{-1} (rhs - a),
+   vectors; prec_vals 64, 32 or 16 (bfloat16 values), Dinv the b x b inverse
+   diagonal blocks in the same precision, FP32 for bfloat16 values): per
+   row, a = (A z)_row, w = D^{-1} (rhs - a),
    d = c1 d + c2 w, zout = z + d.  zout must differ from z.  variant as for
    tacs_hip_sell_spmv_variant. */
 int tacs_hip_sell_cheb_step(int variant, int b, int prec_vals, int nrows, const int* width,
@@ -775,6 +781,13 @@ int tacs_mat_prepare_products(tacs_obj m, int prec);
    gives identical values; the element-block form agrees with the rows to
    rounding. */
 int tacs_mat_autotune(tacs_obj mat, int reps, double* ms, int maxv, int* nvariants);
+/* One Chebyshev step of the recycling solvers' preconditioner on device
+   vectors (tacs_hip_sell_cheb_step over the matrix's sliced ELL copy in
+   precision prec_vals 64, 32 or 16 (bfloat16), with pc's inverse diagonal
+   blocks): d = c1 d + c2 D^{-1} (rhs - A z), zout = z + d.  No counterpart
+   in the reference. */
+int tacs_mat_cheb_step(tacs_obj mat, tacs_obj pc, int prec_vals, const double* z, double* zout,
+                       const double* rhs, double* d, double c1, double c2);
 int tacs_mat_get_sell_variant(tacs_obj mat, int kind, int* variant);
 int tacs_mat_set_sell_variant(tacs_obj mat, int kind, int variant);
 int tacs_mat_autotune_spmm(tacs_obj mat, int nrhs, int prec, int reps, double* ms, int maxv,
@@ -798,9 +811,11 @@ int tacs_pc_apply(tacs_obj pc, tacs_obj x, tacs_obj y);
 /* Chebyshev polynomial of the block-Jacobi iteration as the preconditioner
    of the recycling and batched solvers (degree 1..32, 1 = plain
    block-Jacobi; interval [lmax / ratio, lmax] of D^{-1} A, lmax estimated by
-   the next factor()); GMRES and pc_apply keep plain block-Jacobi.  No
-   counterpart in the reference. */
-int tacs_pc_set_chebyshev(tacs_obj pc, int degree, double ratio);
+   the next factor()); values 32, or 16 for products inside the polynomial
+   that read a bfloat16 copy of the values (single systems with FP32-valued
+   operators).  GMRES and pc_apply keep plain block-Jacobi.  No counterpart
+   in the reference. */
+int tacs_pc_set_chebyshev(tacs_obj pc, int degree, double ratio, int values);
 /* the settings and the lmax the last factor() estimated (1.05 x the
    largest Ritz value of 30 Lanczos steps; 0 before a factor() with
    degree > 1) */

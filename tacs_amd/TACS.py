@@ -226,14 +226,17 @@ class Pc(TACSHandle):
     def applyFactor(self, x, y):
         lib().tacs_pc_apply(self.ptr, x.ptr, y.ptr)
 
-    def setChebyshev(self, degree, ratio=20.0):
+    def setChebyshev(self, degree, ratio=20.0, values=32):
         """Chebyshev polynomial (degree - 1 operator products per
         application) of the block-Jacobi iteration as the preconditioner of
         the recycling and batched solvers, on [lmax / ratio, lmax] of the
         eigenvalues of D^{-1} A (lmax estimated at the next factor(): 1.05 x
         the largest Ritz value of 30 Lanczos steps).
-        degree 1: plain block-Jacobi.  No counterpart in the reference."""
-        check(lib().tacs_pc_set_chebyshev(self.ptr, int(degree), float(ratio)), "Pc.setChebyshev")
+        degree 1: plain block-Jacobi.  values=16: the products inside the
+        polynomial read a bfloat16 copy of the matrix values (single systems
+        whose operator has FP32 values).  No counterpart in the reference."""
+        check(lib().tacs_pc_set_chebyshev(self.ptr, int(degree), float(ratio), int(values)),
+              "Pc.setChebyshev")
 
     def getChebyshev(self):
         """(degree, ratio, lmax): lmax as the last factor() estimated it."""

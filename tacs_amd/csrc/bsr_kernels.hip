@@ -321,6 +321,29 @@ __global__ void bsr_zero_rows(int b, int nbc, const int* __restrict__ nodes,
 // reduction is needed, and each row adds its blocks in column order with the
 // reference's in-block order (BCSRMatVecMult3 src/bpmat/BCSRMatMult3.cpp:28).
 // ---------------------------------------------------------------------------
+// Matrix values of the sliced ELL copies: double, float, or bfloat16 kept
+// as its 16 bits (unsigned short; the upper half of a float).
+__device__ __forceinline__ double sell_val(double a) { return a; }
+__device__ __forceinline__ double sell_val(float a) { return (double)a; }
+__device__ __forceinline__ double sell_val(unsigned short h) {
+  return (double)__uint_as_float((unsigned int)h << 16);
+}
+// float -> bfloat16, rounded to nearest even (NaN kept a NaN)
+__device__ __forceinline__ unsigned short f32_to_bf16(float f) {
+  unsigned int u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+template <typename TA>
+struct SellDinv {  // type of the inverse diagonal blocks that go with TA values
+  typedef TA type;
+};
+template <>
+struct SellDinv<unsigned short> {
+  typedef float type;
+};
+
 template <int B, typename TA, typename T, bool NT = true>
 __device__ inline void sell_slot(const TA* __restrict__ vs, long v, const T* __restrict__ x, int col,
                                  double* yr) {
@@ -333,9 +356,9 @@ __device__ inline void sell_slot(const TA* __restrict__ vs, long v, const T* __r
   for (int c = 0; c < B; c++) xc[c] = (double)x[(size_t)B * col + c];
 #pragma unroll
   for (int r = 0; r < B; r++) {
-    double t = (double)a[B * r] * xc[0];
+    double t = sell_val(a[B * r]) * xc[0];
 #pragma unroll
-    for (int c = 1; c < B; c++) t += (double)a[B * r + c] * xc[c];
+    for (int c = 1; c < B; c++) t += sell_val(a[B * r + c]) * xc[c];
     yr[r] += t;
   }
 }
@@ -392,9 +415,9 @@ __global__ __launch_bounds__(256) void sell_spmv(int nslices, int nrows,
           for (int c = 0; c < B; c++) xc[c] = (double)x[(size_t)B * cu[u] + c];
 #pragma unroll
           for (int r = 0; r < B; r++) {
-            double v = (double)a[u][B * r] * xc[0];
+            double v = sell_val(a[u][B * r]) * xc[0];
 #pragma unroll
-            for (int c = 1; c < B; c++) v += (double)a[u][B * r + c] * xc[c];
+            for (int c = 1; c < B; c++) v += sell_val(a[u][B * r + c]) * xc[c];
             yr[r] += v;
           }
         }
@@ -404,7 +427,8 @@ __global__ __launch_bounds__(256) void sell_spmv(int nslices, int nrows,
     const int row = 64 * s + lane;
     if (row < nrows) {
       if constexpr (CHEB) {
-        const TA* D = (const TA*)ch.Dinv + (size_t)B2 * row;
+        typedef typename SellDinv<TA>::type TD;
+        const TD* D = (const TD*)ch.Dinv + (size_t)B2 * row;
         double rr[B];
 #pragma unroll
         for (int r = 0; r < B; r++) rr[r] = ch.rhs[(size_t)B * row + r] - yr[r];
@@ -450,6 +474,7 @@ __device__ __forceinline__ void sell_fill_pass(int lane, int t0, int rs, int nb,
                                                long vo, long co, const int* __restrict__ cols,
                                                const double* __restrict__ A,
                                                double* __restrict__ v64, float* __restrict__ v32,
+                                               unsigned short* __restrict__ v16,
                                                int* __restrict__ cs) {
   constexpr int B2 = B * B;
   const double* a = A + (size_t)B2 * (rs + t0);
@@ -467,6 +492,7 @@ __device__ __forceinline__ void sell_fill_pass(int lane, int t0, int rs, int nb,
       const size_t o = vo + 64 * ((size_t)B2 * (t0 + t) + q) + lane;
       if (v64) v64[o] = v[t * B2 + q];
       if (v32) v32[o] = (float)v[t * B2 + q];
+      if (v16) v16[o] = f32_to_bf16((float)v[t * B2 + q]);
     }
     if (cs) cs[co + 64 * (t0 + t) + lane] = t0 + t < nb ? cols[rs + t0 + t] : rowc;
   }
@@ -478,6 +504,7 @@ __global__ __launch_bounds__(256) void sell_fill(int nslices, int nrows, const i
                                                  const long* __restrict__ coff,
                                                  const double* __restrict__ A,
                                                  double* __restrict__ v64, float* __restrict__ v32,
+                                                 unsigned short* __restrict__ v16,
                                                  int* __restrict__ cs) {
   constexpr int B2 = B * B, T = SellFillCfg<B>::T;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -490,9 +517,9 @@ __global__ __launch_bounds__(256) void sell_fill(int nslices, int nrows, const i
     const int rowc = row < nrows ? row : 0;
     int t0 = 0;
     for (; t0 + T <= w; t0 += T)
-      sell_fill_pass<B, T>(lane, t0, rs, nb, rowc, vo, co, cols, A, v64, v32, cs);
+      sell_fill_pass<B, T>(lane, t0, rs, nb, rowc, vo, co, cols, A, v64, v32, v16, cs);
     for (; t0 < w; t0++)
-      sell_fill_pass<B, 1>(lane, t0, rs, nb, rowc, vo, co, cols, A, v64, v32, cs);
+      sell_fill_pass<B, 1>(lane, t0, rs, nb, rowc, vo, co, cols, A, v64, v32, v16, cs);
   }
 }
 
@@ -595,20 +622,27 @@ size_t tacs_hip_sell_slots(int nrows, const int* rowp_host, int* width, long* co
   return (size_t)off;
 }
 
-int tacs_hip_sell_fill2(int b, int nrows, const int* rowp, const int* cols, const int* width,
-                        const long* coff, const double* A, double* v64, float* v32, int* cs,
-                        hipStream_t stream) {
+int tacs_hip_sell_fill3(int b, int nrows, const int* rowp, const int* cols, const int* width,
+                        const long* coff, const double* A, double* v64, float* v32,
+                        unsigned short* v16, int* cs, hipStream_t stream) {
   const int ns = (nrows + 63) / 64;
-  if ((b != 3 && b != 6) || (!v64 && !v32)) return TACS_HIP_ERR_ARG;
+  if ((b != 3 && b != 6) || (!v64 && !v32 && !v16)) return TACS_HIP_ERR_ARG;
   if (ns <= 0) return 0;
   const int grid = tacs_grid_for(ns, 4, 65536);
   if (b == 3)
     hipLaunchKernelGGL(sell_fill<3>, dim3(grid), dim3(256), 0, stream, ns, nrows, rowp, cols,
-                       width, coff, A, v64, v32, cs);
+                       width, coff, A, v64, v32, v16, cs);
   else
     hipLaunchKernelGGL(sell_fill<6>, dim3(grid), dim3(256), 0, stream, ns, nrows, rowp, cols,
-                       width, coff, A, v64, v32, cs);
+                       width, coff, A, v64, v32, v16, cs);
   return (int)hipGetLastError();
+}
+
+int tacs_hip_sell_fill2(int b, int nrows, const int* rowp, const int* cols, const int* width,
+                        const long* coff, const double* A, double* v64, float* v32, int* cs,
+                        hipStream_t stream) {
+  if (!v64 && !v32) return TACS_HIP_ERR_ARG;
+  return tacs_hip_sell_fill3(b, nrows, rowp, cols, width, coff, A, v64, v32, NULL, cs, stream);
 }
 
 int tacs_hip_sell_fill(int b, int prec, int nrows, const int* rowp, const int* cols,
@@ -680,7 +714,8 @@ int tacs_hip_sell_cheb_step(int variant, int b, int prec_vals, int nrows, const 
                             double* zout, const double* rhs, const void* Dinv, double* d,
                             double c1, double c2, hipStream_t stream) {
   const int ns = (nrows + 63) / 64;
-  if ((b != 3 && b != 6) || (prec_vals != 64 && prec_vals != 32) || z == zout) return TACS_HIP_ERR_ARG;
+  if ((b != 3 && b != 6) || (prec_vals != 64 && prec_vals != 32 && prec_vals != 16) || z == zout)
+    return TACS_HIP_ERR_ARG;
   if (variant < 0 || variant >= tacs_hip_sell_spmv_nvariants(b)) return TACS_HIP_ERR_ARG;
   if (ns <= 0) return 0;
   const int grid = tacs_grid_for(ns, 4, 65536);
@@ -691,10 +726,14 @@ int tacs_hip_sell_cheb_step(int variant, int b, int prec_vals, int nrows, const 
       hipLaunchKernelGGL((sell_spmv<BB, U, double, double, NT, true>), dim3(grid), dim3(256), 0, \
                          stream, ns, nrows, width, coff, cs, (const double*)vs, z,              \
                          (double*)NULL, ch);                                                     \
-    else                                                                                         \
+    else if (prec_vals == 32)                                                                    \
       hipLaunchKernelGGL((sell_spmv<BB, U, float, double, NT, true>), dim3(grid), dim3(256), 0,  \
                          stream, ns, nrows, width, coff, cs, (const float*)vs, z,               \
                          (double*)NULL, ch);                                                     \
+    else                                                                                         \
+      hipLaunchKernelGGL((sell_spmv<BB, U, unsigned short, double, NT, true>), dim3(grid),      \
+                         dim3(256), 0, stream, ns, nrows, width, coff, cs,                      \
+                         (const unsigned short*)vs, z, (double*)NULL, ch);                      \
   } while (0)
   if (b == 3) {
     switch (variant) {

@@ -560,6 +560,17 @@ int tacs_mat_prepare_products(tacs_obj m, int prec) {
   if ((prec == 64 ? A->sellVer64 : A->sellVer32) != A->version) A->sellUpdate(prec);
   return last_error_code();
 }
+int tacs_mat_cheb_step(tacs_obj m, tacs_obj pc, int prec_vals, const double* z, double* zout,
+                       const double* rhs, double* d, double c1, double c2) {
+  REQUIRE(TACSParallelMat, A, m);
+  REQUIRE(TACSBlockJacobiPc, p, pc);
+  if ((prec_vals != 64 && prec_vals != 32 && prec_vals != 16) || z == zout || p->mat != A)
+    return tacs_set_error("%s: prec 64, 32 or 16, zout != z, the matrix's own preconditioner",
+                          __func__);
+  A->chebStep(prec_vals, prec_vals == 64 ? (const void*)p->d_Dinv : (const void*)p->d_Dinv32, z,
+              zout, rhs, d, c1, c2);
+  return last_error_code();
+}
 int tacs_mat_autotune(tacs_obj m, int reps, double* ms, int maxv, int* nvariants) {
   REQUIRE(TACSParallelMat, A, m);
   const int nv = A->autotune(reps, ms, maxv);
@@ -654,9 +665,9 @@ int tacs_pc_factor(tacs_obj pc) {
   p->factor();
   return last_error_code();
 }
-int tacs_pc_set_chebyshev(tacs_obj pc, int degree, double ratio) {
+int tacs_pc_set_chebyshev(tacs_obj pc, int degree, double ratio, int values) {
   REQUIRE(TACSBlockJacobiPc, p, pc);
-  return p->setChebyshev(degree, ratio);
+  return p->setChebyshev(degree, ratio, values);
 }
 int tacs_pc_get_chebyshev(tacs_obj pc, int* degree, double* ratio, double* lmax) {
   REQUIRE(TACSBlockJacobiPc, p, pc);

@@ -463,7 +463,8 @@ TACSParallelMat::TACSParallelMat(int _bsize, int _nrows, const int* rowp, const 
   d_scols = NULL;
   d_s64 = NULL;
   d_s32 = NULL;
-  sellVer64 = sellVer32 = -1;
+  d_s16 = NULL;
+  sellVer64 = sellVer32 = sellVer16 = -1;
   nslots = 0;
   sellVar[0] = sellVar[1] = sellVar[2] = -1;
 }
@@ -472,7 +473,7 @@ TACSParallelMat::~TACSParallelMat() {
   dfree(d_cols);
   dfree(d_A);
   dfree(d_A32);
-  void* sell[] = {d_sw, d_soff, d_scols, d_s64, d_s32};
+  void* sell[] = {d_sw, d_soff, d_scols, d_s64, d_s32, d_s16};
   for (void* q : sell) dfree(q);
   delete[] h_rowp;
   delete[] h_cols;
@@ -518,13 +519,16 @@ void TACSParallelMat::sellUpdate(int prec) {
   // one pass over the BCSR values writes both
   if (prec == 64 && !d_s64) d_s64 = dalloc<double>(b2 * nslots);
   if (prec == 32 && !d_s32) d_s32 = dalloc<float>(b2 * nslots);
+  if (prec == 16 && !d_s16) d_s16 = dalloc<unsigned short>(b2 * nslots);
   if (g_halt) return;
   double* v64 = (d_s64 && sellVer64 != version) ? d_s64 : NULL;
   float* v32 = (d_s32 && sellVer32 != version) ? d_s32 : NULL;
-  KCHECK(tacs_hip_sell_fill2(bsize, nrows, d_rowp, d_cols, d_sw, d_soff, d_A, v64, v32, cs_fill,
-                             g_stream));
+  unsigned short* v16 = (d_s16 && sellVer16 != version) ? d_s16 : NULL;
+  KCHECK(tacs_hip_sell_fill3(bsize, nrows, d_rowp, d_cols, d_sw, d_soff, d_A, v64, v32, v16,
+                             cs_fill, g_stream));
   if (v64) sellVer64 = version;
   if (v32) sellVer32 = version;
+  if (v16) sellVer16 = version;
 }
 // The products run over the sliced ELL copies (3x3 and 6x6 blocks), in the
 // matrix's launch variant (autotune) or the tuning table's.
@@ -555,9 +559,10 @@ void TACSParallelMat::chebStep(int pr, const void* Dinv, const double* z, double
   const int kind = pr == 64 ? 0 : 1;
   if (pr == 64 && sellVer64 != version) sellUpdate(64);
   if (pr == 32 && sellVer32 != version) sellUpdate(32);
+  if (pr == 16 && sellVer16 != version) sellUpdate(16);
+  const void* vs = pr == 64 ? (const void*)d_s64 : pr == 32 ? (const void*)d_s32 : (const void*)d_s16;
   KCHECK(tacs_hip_sell_cheb_step(sell_kind_variant(this, kind), bsize, pr, nrows, d_sw, d_soff,
-                                 d_scols, pr == 64 ? (const void*)d_s64 : (const void*)d_s32, z,
-                                 zout, rhs, Dinv, d, c1, c2, g_stream));
+                                 d_scols, vs, z, zout, rhs, Dinv, d, c1, c2, g_stream));
 }
 // Average time of reps launches of fn on the library stream (one untimed
 // launch first), in ms; negative after a failure.
@@ -812,11 +817,12 @@ double TACSBlockJacobiPc::lanczosMax(int steps) {
   }
   return hi;
 }
-int TACSBlockJacobiPc::setChebyshev(int degree, double ratio) {
-  if (degree < 1 || degree > 32 || !(ratio > 1.0))
-    return tacs_set_error("setChebyshev: 1 <= degree <= 32 and ratio > 1");
+int TACSBlockJacobiPc::setChebyshev(int degree, double ratio, int values) {
+  if (degree < 1 || degree > 32 || !(ratio > 1.0) || (values != 32 && values != 16))
+    return tacs_set_error("setChebyshev: 1 <= degree <= 32, ratio > 1, values 32 or 16");
   chebDegree = degree;
   chebRatio = ratio;
+  chebValues = values;
   chebMax = 0.0;
   return 0;
 }
@@ -1775,7 +1781,10 @@ void TACSBatchedGMRES::precond(int pr, int Rn, const void* Dinv, const float* A3
   double rho = 1.0 / sigma;
   if (rc == 1) {
     // one system: the step fused into the product, iterates alternating
-    // between two buffers so that the last lands in zc
+    // between two buffers so that the last lands in zc; the products inside
+    // the polynomial may read bfloat16 values (a fixed linear operator
+    // still: FP64 vectors and sums)
+    const int pin = (pr == 32 && pc->chebValues == 16) ? 16 : pr;
     double* buf[2];
     buf[(deg - 1) % 2] = zc;
     buf[deg % 2] = d_caz;
@@ -1783,7 +1792,8 @@ void TACSBatchedGMRES::precond(int pr, int Rn, const void* Dinv, const float* A3
                                1.0 / theta, 1, g_stream));
     for (int k = 1; k < deg; k++) {
       const double rn = 1.0 / (2.0 * sigma - rho);
-      mat->chebStep(pr, Dinv, buf[(k - 1) % 2], buf[k % 2], xc, d_cd, rn * rho, 2.0 * rn / delta);
+      mat->chebStep(pin, Dinv, buf[(k - 1) % 2], buf[k % 2], xc, d_cd, rn * rho,
+                    2.0 * rn / delta);
       rho = rn;
     }
   } else {

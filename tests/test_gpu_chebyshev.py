@@ -53,13 +53,14 @@ def test_lanczos_estimate(order, n):
         pc.setChebyshev(4, 1.0)
 
 
-@pytest.mark.parametrize("deg,ratio", [(2, 20.0), (6, 60.0), (10, 300.0)])
-def test_recycling_solve_with_chebyshev_matches_dense(deg, ratio):
+@pytest.mark.parametrize("deg,ratio,values", [(2, 20.0, 32), (6, 60.0, 32), (10, 300.0, 32),
+                                              (10, 300.0, 16)])
+def test_recycling_solve_with_chebyshev_matches_dense(deg, ratio, values):
     p = SolidProblem(5, dv="element")
     p.set_thickness(np.random.default_rng(3).uniform(0.5, 1.5, p.mesh.nelem))
     p.assemble()
     pc = TACS.Pc(p.mat)
-    pc.setChebyshev(deg, ratio)
+    pc.setChebyshev(deg, ratio, values)
     pc.factor()
     ksm = TACS.MixedKSM(p.mat, pc, 30, 1000, deflation=8)
     ksm.setTolerances(1e-11, 1e-30)
