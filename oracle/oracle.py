@@ -41,6 +41,7 @@ def lib():
         L.orc_shell_failure.restype = ctypes.c_double
         L.orc_shell_compliance.restype = ctypes.c_double
         L.orc_assemble_shell.restype = ctypes.c_long
+        L.orc_assemble_shell_mat.restype = ctypes.c_long
         L.orc_par_assemble_solid.restype = ctypes.c_long
         L.orc_par_assemble_shell.restype = ctypes.c_long
         L.orc_par_gmres.restype = ctypes.c_int
@@ -317,6 +318,12 @@ def quad4_element(X, v, E, nu, t, alpha=1.0):
     return res, mat.reshape(24, 24)
 
 
+def quad4_mass(X, rho, t, alpha=1.0):
+    mat = np.zeros(576)
+    lib().orc_quad4_mass(_p(_dd(X)), _c(rho), _c(t), _c(alpha), _p(mat))
+    return mat.reshape(24, 24)
+
+
 def quad4_adj_product(X, v, psi, E, nu, t):
     return lib().orc_quad4_adj_product(_p(_dd(X)), _p(_dd(v)), _p(_dd(psi)), _c(E),
                                        _c(nu), _c(t))
@@ -361,6 +368,18 @@ def assemble_shell(conn, X, u, t_elem, E, nu, rowp, cols):
                                     _p(A), _p(res))
     assert miss == 0
     return A, res
+
+
+def assemble_shell_mat(conn, X, t_elem, E, nu, rho, kc, mc, rowp, cols):
+    """kc K + mc M of Quad4 shells assembled into 6x6 block rows."""
+    conn = _ii(conn).reshape(-1)
+    A = np.zeros(36 * cols.size)
+    miss = lib().orc_assemble_shell_mat(conn.size // 4, _p(conn, np.int32), _p(_dd(X)),
+                                        _p(_dd(t_elem)), _c(E), _c(nu), _c(rho), _c(kc),
+                                        _c(mc), _p(_ii(rowp), np.int32),
+                                        _p(_ii(cols), np.int32), _p(A))
+    assert miss == 0
+    return A
 
 
 def shell_add_pressure(conn, X, aux_elems, pressures, res):

@@ -198,6 +198,16 @@ void ref_quad4_shell(const double* X, const double* vars, double E, double nu, d
   elem->decref();
 }
 
+// Element matrix of one TACSQuad4Shell by type (TACSElement::getMatType):
+// 1 (stiffness) or 2 (mass) at the state vars.
+void ref_quad4_mat_type(int matType, const double* X, const double* vars, double E, double nu,
+                        double t, double* mat) {
+  TACSElement* elem = make_quad4(E, nu, 270e6, t);
+  elem->incref();
+  elem->getMatType((ElementMatrixType)matType, 0, 0.0, X, vars, mat);
+  elem->decref();
+}
+
 // psi^T dR/dt of one TACSQuad4Shell (thickness DV, number 0).
 double ref_quad4_adj_product(const double* X, const double* vars, const double* psi, double E,
                              double nu, double ys, double t) {

@@ -20,8 +20,9 @@
   which is the reference's element Jacobian / residual for a static state
   (the inertial terms multiply the zero accelerations).
 
-  Pins (tests/test_oracle_shell.py): residual, Jacobian, thickness adjoint
-  product, failure index, detXd and pressure load against values written by
+  Pins (tests/test_oracle_shell.py): residual, Jacobian, stiffness and mass
+  matrices, thickness adjoint product, failure index, detXd and pressure
+  load against values written by
   the reference's own element code (tests/golden/elements.npz); mesh-level
   compliance and KS failure of the 10x10 clamped plate against the golden
   values of tests/integration_tests/test_shell_plate_quad.py:32-46; rigid
@@ -429,6 +430,55 @@ void orc_quad4_element(const double X[12], const double v[24], double E,
   }
 }
 
+/*
+  Consistent mass of one element, added as alpha * M: the kinetic energy
+  T = 1/2 sum_q w_q detXd_q (m0 |u0dot|^2 + m2 |d0dot|^2) of
+  TACSShellElement::computeEnergies (TACSShellElement.h:272-284) with the mass
+  moments of TACSIsoShellConstitutive::evalMassMoments
+  (TACSIsoShellConstitutive.cpp:120-129) at tOffset = 0 (m0 = rho t, m1 = 0,
+  m2 = rho t^3 / 12).  u0 interpolates the node displacements, d0 the node
+  directors d_a = q_a x fn_a (TACSLinearizedRotation::computeDirectorRates,
+  TACSDirector.h:210-227); both are linear in the element variables, so
+  M = sum_q w_q detXd_q (m0 Nu_q^T Nu_q + m2 Nd_q^T Nd_q) with the 3 x 24
+  maps Nu_q, Nd_q built column by column from unit variable vectors.  This is
+  the matrix TACSShellElement::getMatType(TACS_MASS_MATRIX) (:644) assembles
+  through addJacobian's gamma terms (:605-617) and
+  addDirectorJacobian (TACSDirector.h:369-429).
+*/
+void orc_quad4_mass(const double X[12], double rho, double t, double alpha,
+                    double *mat) {
+  double fn[12], zero[SH_NV] = {0}, e[4][9], detw[4];
+  sh_node_normals(X, fn, NULL);
+  orc_quad4_strains(X, zero, 1, e, detw);
+  const double m0 = rho * t, m2 = rho * t * t * t / 12.0;
+  for (int q = 0; q < 4; q++) {
+    double pt[2] = {sh_gp[q % 2], sh_gp[q / 2]};
+    double Nu[3][SH_NV], Nd[3][SH_NV];
+    for (int c = 0; c < SH_NV; c++) {
+      double v[SH_NV] = {0}, dn[12];
+      v[c] = 1.0;
+      for (int a = 0; a < SH_NN; a++) {
+        const double *qa = &v[6 * a + 3];
+        sh_cross(qa, &fn[3 * a], &dn[3 * a]);
+      }
+      double u0[6], d0[3];
+      sh_interp(6, 3, pt, v, u0);
+      sh_interp(3, 3, pt, dn, d0);
+      for (int k = 0; k < 3; k++) {
+        Nu[k][c] = u0[k];
+        Nd[k][c] = d0[k];
+      }
+    }
+    for (int r = 0; r < SH_NV; r++) {
+      for (int c = 0; c < SH_NV; c++) {
+        double acc = 0.0;
+        for (int k = 0; k < 3; k++) acc += m0 * Nu[k][r] * Nu[k][c] + m2 * Nd[k][r] * Nd[k][c];
+        mat[SH_NV * r + c] += alpha * detw[q] * acc;
+      }
+    }
+  }
+}
+
 /* psi^T dR/dt of one element: TACSShellElement::addAdjResProduct (:917)
    -> TACSIsoShellConstitutive::addStressDVSens
    (TACSIsoShellConstitutive.cpp:229-251) with e = e(v), psi-strain = e(psi). */
@@ -607,6 +657,41 @@ long orc_assemble_shell(int nelem, const int *conn, const double *X,
           for (int r = 0; r < 6; r++) {
             for (int q = 0; q < 6; q++) blk[6 * r + q] += ke[(6 * a + r) * SH_NV + 6 * b + q];
           }
+        }
+      }
+    }
+  }
+  return missing;
+}
+
+/*
+  kc K + mc M assembled element by element (TACSAssembler::assembleMatType /
+  assembleMatCombo, src/TACSAssembler.cpp:4418-4560, over
+  TACSShellElement::getMatType) into 6x6 BCSR; returns the number of missing
+  pattern entries.
+*/
+long orc_assemble_shell_mat(int nelem, const int *conn, const double *X,
+                            const double *t_elem, double E, double nu,
+                            double rho, double kc, double mc, const int *rowp,
+                            const int *cols, double *A) {
+  long missing = 0;
+  for (int el = 0; el < nelem; el++) {
+    const int *c = &conn[4 * el];
+    double Xe[12], ke[SH_NV * SH_NV], zero[SH_NV] = {0};
+    sh_gather(c, X, NULL, Xe, NULL);
+    memset(ke, 0, sizeof(ke));
+    orc_quad4_element(Xe, zero, E, nu, t_elem[el], kc, NULL, ke);
+    orc_quad4_mass(Xe, rho, t_elem[el], mc, ke);
+    for (int a = 0; a < SH_NN; a++) {
+      for (int b = 0; b < SH_NN; b++) {
+        long s = sh_find_slot(rowp, cols, c[a], c[b]);
+        if (s < 0) {
+          missing++;
+          continue;
+        }
+        double *blk = &A[36 * s];
+        for (int r = 0; r < 6; r++) {
+          for (int q = 0; q < 6; q++) blk[6 * r + q] += ke[(6 * a + r) * SH_NV + 6 * b + q];
         }
       }
     }

@@ -242,6 +242,7 @@ enum {
 struct ShellParams {
   double Q0, Q1, Q5, kcorr, drill_reg, ys;
   double scale, ks_weight, ks_max, ks_sum, safety;
+  double mass, rho;  // coefficient of the consistent mass (matrix mode), density
 };
 
 template <typename T, int MODE>
@@ -496,6 +497,31 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
 #pragma unroll
         for (int kk = 0; kk < 9; kk++) s[q][kk] *= L.detw[q];
       }
+      // consistent mass (TACSShellElement::addJacobian, TACSShellElement.h:
+      // 605-617, and TACSLinearizedRotation::addDirectorJacobian,
+      // TACSDirector.h:381-429) with tOffset = 0: moments m0 = rho t,
+      // m1 = 0, m2 = rho t^3 / 12 (TACSIsoShellConstitutive.cpp:120-129).
+      // Node pair (b row, a column), cN = sum_q detw_q N_b N_a:
+      //   u-u block  m0 cN I,
+      //   q-q block  -m2 cN fn_b^x fn_a^x = m2 cN ((fn_b . fn_a) I - fn_a fn_b^T).
+      const int ca = lane / 6, ck = lane % 6;
+      T cN[4] = {T(0), T(0), T(0), T(0)};
+      const bool with_mass = P.mass != 0.0;
+      if (with_mass) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          T x = T(sh_gauss(q & 1)), y = T(sh_gauss(q >> 1));
+          T Na, Nx, Ny;
+          sh_shape<T>(ca, x, y, Na, Nx, Ny);
+#pragma unroll
+          for (int b = 0; b < 4; b++) {
+            T Nb;
+            sh_shape<T>(b, x, y, Nb, Nx, Ny);
+            cN[b] += L.detw[q] * Nb * Na;
+          }
+        }
+      }
+      const T m0 = T(P.rho) * t, m2 = T(P.rho) * t * t * t / T(12);
       T racc = T(0);
       T* kout = &ke[(size_t)el * SH_NV * SH_NV + lane];
       for (int r = 0; r < SH_NV; r++) {
@@ -504,7 +530,21 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
         for (int q = 0; q < 4; q++)
 #pragma unroll
           for (int kk = 0; kk < 9; kk++) acc += L.B[q][kk][r] * s[q][kk];
-        kout[(size_t)r * SH_NV] = acc * T(P.scale);
+        T kv = acc * T(P.scale);
+        if (with_mass) {
+          const int rb = r / 6, rk = r % 6;
+          T mv = T(0);
+          if (rk < 3 && ck < 3) {
+            if (rk == ck) mv = m0 * cN[rb];
+          } else if (rk >= 3 && ck >= 3) {
+            const T* fb = L.fn[rb];
+            const T* fa = L.fn[ca];
+            T dot = fb[0] * fa[0] + fb[1] * fa[1] + fb[2] * fa[2];
+            mv = m2 * cN[rb] * ((rk == ck ? dot : T(0)) - fa[rk - 3] * fb[ck - 3]);
+          }
+          kv += T(P.mass) * mv;
+        }
+        kout[(size_t)r * SH_NV] = kv;
         racc += acc * L.u[r];
       }
       if (ev) ev[(size_t)el * SH_NV + lane] = double(racc);
@@ -748,6 +788,8 @@ static ShellParams shell_params(const ShellMaterial* m, double scale, double ks_
   P.ks_max = ks_max;
   P.ks_sum = ks_sum;
   P.safety = safety;
+  P.mass = 0.0;
+  P.rho = m->rho;
   return P;
 }
 
@@ -782,6 +824,24 @@ int tacs_hip_shell_elem_kernel(int prec, int mode, int nelem, const int* conn, c
     case SH_COMPLIANCE_DV: SH_LAUNCH(double, SH_COMPLIANCE_DV); break;
     default: return TACS_HIP_ERR_ARG;
   }
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_shell_mat_type(int prec, int nelem, const int* conn, const double* X,
+                            const double* u, const double* telem, const ShellMaterial* mat,
+                            double kscale, double mscale, void* ke, double* ev,
+                            hipStream_t stream) {
+  if (nelem <= 0) return 0;
+  if (!mat || (prec != 64 && prec != 32) || (prec == 32 && ev)) return TACS_HIP_ERR_ARG;
+  ShellParams P = shell_params(mat, kscale, 0.0, 0.0, 1.0, 1.0);
+  P.mass = mscale;
+  int grid = (nelem + SH_EPB - 1) / SH_EPB;
+  const double* psi = NULL;
+  double* es = NULL;
+  if (prec == 32)
+    SH_LAUNCH(float, SH_MATRIX);
+  else
+    SH_LAUNCH(double, SH_MATRIX);
   return (int)hipGetLastError();
 }
 

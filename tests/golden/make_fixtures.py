@@ -127,6 +127,14 @@ def make(seed=2024):
                 L.ref_solid_mat_type(order, k + 1, _d(X), _d(u), c(70e9), c(0.3), c(2700.0), c(t),
                                      _d(mt[k]))
             out[key + "_mattype"] = mt
+    # Quad4 shell element matrices by type (stiffness, mass)
+    for s in range(4):
+        key = f"quad4_{s}"
+        X, v, t = out[key + "_X"], out[key + "_v"], float(out[key + "_t"][0])
+        mt = np.zeros((2, 576))
+        for k in range(2):
+            L.ref_quad4_mat_type(k + 1, _d(X), _d(v), c(70e9), c(0.3), c(t), _d(mt[k]))
+        out[key + "_mattype"] = mt
     np.savez_compressed(os.path.join(HERE, "elements.npz"), **out)
     return out
 

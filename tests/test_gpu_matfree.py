@@ -192,8 +192,7 @@ def test_assemble_mat_type_matches_oracle(order, shape):
 
 def test_assemble_mat_type_shells():
     """Shell stiffness through assembleMatType equals the Jacobian with
-    alpha = 1; the shell mass and geometric stiffness are reported as not
-    built."""
+    alpha = 1; the shell geometric stiffness is reported as not built."""
     from tacs_amd.problems import ShellProblem
 
     p = ShellProblem(8)
@@ -205,4 +204,4 @@ def test_assemble_mat_type_shells():
     _, _, As = p.mat.getArrays()
     assert np.array_equal(Aj, As)
     with pytest.raises(Exception):
-        a.assembleMatType(TACS.MASS_MATRIX, p.mat)
+        a.assembleMatType(TACS.GEOMETRIC_STIFFNESS_MATRIX, p.mat)

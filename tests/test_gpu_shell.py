@@ -86,6 +86,52 @@ def test_shell_jacobian_and_residual_match_oracle(n, warp):
     assert np.abs(r2.getArray() - ores).max() <= RTOL * np.abs(ores).max()
 
 
+def _blockwise_close(A, oA, what):
+    """Translational (rows and columns 0-2) and rotational (3-5) entries,
+    each within RTOL of its own largest entry: the rotary inertia is about
+    t^2 / 12 of the translational mass."""
+    A, oA = A.reshape(-1, 6, 6), oA.reshape(-1, 6, 6)
+    for sl in (slice(0, 3), slice(3, 6)):
+        a, o = A[:, sl, sl], oA[:, sl, sl]
+        assert np.abs(a - o).max() <= RTOL * np.abs(o).max(), what
+    assert np.abs(A - oA).max() <= RTOL * np.abs(oA).max(), what
+
+
+@pytest.mark.parametrize("bc", ["ss", "clamped"])
+def test_shell_mass_and_mat_combo_match_oracle(bc):
+    """assembleMatType(TACS_MASS_MATRIX), assembleMatCombo(stiffness, mass)
+    and assembleJacobian's gamma term against the oracle's element-by-element
+    assembly (TACSShellElement::getMatType / addJacobian), Dirichlet rows
+    zeroed with a unit diagonal."""
+    p, t = _setup(4, warp=0.03, seed=5, bc=bc)
+    a, m = p.assembler, p.mesh
+    X = m.X.reshape(-1)
+    rowp, cols = O.node_csr(m.nnodes, m.conn, 4)
+    rng = np.random.default_rng(9)
+    u = rng.normal(0.0, 1e-3, 6 * m.nnodes)
+    uv = a.createVec()
+    uv.getArray()[:] = u
+    a.setVariables(uv)
+
+    def oracle(kc, mc):
+        oA = O.assemble_shell_mat(m.conn, X, t, E, NU, 2700.0, kc, mc, rowp, cols)
+        O.apply_bc_mat(6, p.bc_nodes, _mask(p), rowp, cols, oA)
+        return oA
+
+    a.assembleMatType(TACS.MASS_MATRIX, p.mat)
+    grp, gcols, A = p.mat.getArrays()
+    assert np.array_equal(grp, rowp) and np.array_equal(gcols, cols)
+    _blockwise_close(A, oracle(0.0, 1.0), "mass")
+    a.assembleMatCombo(TACS.STIFFNESS_MATRIX, 1.5, TACS.MASS_MATRIX, 2e9, p.mat)
+    _blockwise_close(p.mat.getArrays()[2], oracle(1.5, 2e9), "combo")
+    # alpha K + gamma M with the residual K u of the static state
+    a.assembleJacobian(0.7, 0.0, 3e8, p.res, p.mat)
+    _blockwise_close(p.mat.getArrays()[2], oracle(0.7, 3e8), "jacobian")
+    _, _, _, ores = _oracle(p, t, u)
+    got = p.res.getArray()
+    assert np.abs(got - ores).max() <= RTOL * np.abs(ores).max()
+
+
 def test_shell_spmv_matches_oracle():
     p, t = _setup(6)
     p.assemble()

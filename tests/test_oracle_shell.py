@@ -1,8 +1,12 @@
 """Pins of the oracle's Quad4 shell restatement (oracle/shell_oracle.c).
 
-- element residual, Jacobian, thickness adjoint product, failure index,
-  detXd and pressure load against values the reference's own element code
-  wrote (tests/golden/elements.npz, made by tests/golden/make_fixtures.py);
+- element residual, Jacobian, stiffness and consistent mass matrices,
+  thickness adjoint product, failure index, detXd and pressure load against
+  values the reference's own element code wrote (tests/golden/elements.npz,
+  made by tests/golden/make_fixtures.py);
+- the mass of a flat rectangle in closed form: total mass rho t a b for a
+  rigid translation and the rotary inertia rho t^3 a b / 12 for a uniform
+  in-plane rotation, zero for the drilling rotation;
 - the 10x10 clamped plate of tests/integration_tests/test_shell_plate_quad.py
   (pressure 100 kPa and a 10 kN centre load): compliance and KS failure
   (ksWeight 10) against the golden values held in that test (:32-46);
@@ -45,6 +49,42 @@ def test_quad4_against_reference_element(fixtures, case):
         assert abs(det - fixtures[k + "_det"][n]) <= 1e-14 * abs(fixtures[k + "_det"][n])
     pres = O.quad4_pressure(X, 1e4)
     assert np.abs(pres - fixtures[k + "_pres"]).max() <= 1e-14 * np.abs(fixtures[k + "_pres"]).max()
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_quad4_mat_types_against_reference_element(fixtures, case):
+    """getMatType(TACS_STIFFNESS_MATRIX) and getMatType(TACS_MASS_MATRIX) of
+    the reference element (density 2700) against the oracle."""
+    k = f"quad4_{case}"
+    X = fixtures[k + "_X"]
+    t = float(fixtures[k + "_t"].reshape(-1)[0])
+    ref = fixtures[k + "_mattype"]
+    _, K = O.quad4_element(X, np.zeros(24), E, NU, t)
+    assert np.abs(K.reshape(-1) - ref[0]).max() <= 1e-14 * np.abs(ref[0]).max()
+    M = O.quad4_mass(X, 2700.0, t)
+    assert np.abs(M.reshape(-1) - ref[1]).max() <= 1e-14 * np.abs(ref[1]).max()
+    # alpha scales the whole matrix
+    assert np.array_equal(O.quad4_mass(X, 2700.0, t, 2.0), 2.0 * M)
+
+
+def test_quad4_mass_closed_form():
+    """Flat a x b rectangle in the x-y plane: v^T M v equals rho t a b for a
+    rigid translation, rho t^3 a b / 12 for a unit rotation about x or y and
+    0 about z (the director q x n vanishes); M is symmetric and positive
+    semi-definite."""
+    a, b, rho, t = 0.7, 0.4, 2700.0, 0.02
+    X = np.array([[0, 0, 0], [a, 0, 0], [0, b, 0], [a, b, 0]], float).reshape(-1)
+    M = O.quad4_mass(X, rho, t)
+    assert np.abs(M - M.T).max() <= 1e-15 * np.abs(M).max()
+    for k in range(3):
+        v = np.zeros(24)
+        v[k::6] = 1.0
+        assert abs(v @ M @ v - rho * t * a * b) <= 1e-13 * rho * t * a * b
+    for k, expect in ((3, rho * t ** 3 * a * b / 12), (4, rho * t ** 3 * a * b / 12), (5, 0.0)):
+        v = np.zeros(24)
+        v[k::6] = 1.0
+        assert abs(v @ M @ v - expect) <= 1e-13 * rho * t ** 3 * a * b / 12
+    assert np.linalg.eigvalsh(M).min() >= -1e-12 * np.abs(M).max()
 
 
 def _warped_quad(rng):
