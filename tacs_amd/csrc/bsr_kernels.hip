@@ -451,20 +451,13 @@ __global__ __launch_bounds__(256) void sell_spmv(int nslices, int nrows,
 }
 
 // Sliced ELL copy of the block-CSR values (and, with cs, the columns): one
-// wavefront per slice of 64 rows, T slots at a time.  The wave reads the T
-// consecutive blocks of each of the 64 rows as one contiguous run (lanes over
-// the run's values: coalesced) into LDS, then writes slot t, value q of the
-// 64 rows as 64 consecutive entries (coalesced).  Padding slots (t beyond a
-// row's length) are zero blocks in the row's own column.  v64 and/or v32
-// receive the FP64 and FP32 copies in the same pass.
-// Sliced ELL copy of the block-CSR values (and, with cs, the columns): one
 // lane per row of a 64-row slice, T slots per pass.  A lane issues the loads
 // of its row's T consecutive blocks (one contiguous run of T b^2 values) back
 // to back, so every cache line of the run is fetched once and consumed by the
 // burst, then writes slot t, value q: the wave's 64 lanes store 64
 // consecutive entries.  Padding slots (t beyond a row's length) are zero
-// blocks in the row's own column.  v64 and/or v32 receive the FP64 and FP32
-// copies in the same pass.
+// blocks in the row's own column.  v64, v32 and v16 receive the FP64, FP32
+// and bfloat16 copies in the same pass.
 template <int B>
 struct SellFillCfg {
   static constexpr int T = B == 3 ? 4 : 1;  // slots per pass (8: 1.3x fetch, but occupancy 2 and slower)

@@ -130,3 +130,69 @@ def test_shell_recycling_solve_with_chebyshev():
     K, *_ = _dense(p)
     r = p.res.getArray()
     assert np.linalg.norm(K @ x.getArray() - r) <= 2e-9 * np.linalg.norm(r)
+
+
+def _bf16(a):
+    """float64 -> float32 -> bfloat16 (round to nearest even) -> float64."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("kind", ["solid", "shell"])
+@pytest.mark.parametrize("values", [64, 32, 16])
+def test_fused_chebyshev_step_matches_numpy(kind, values):
+    """One fused Chebyshev step (tacs_mat_cheb_step: a = A z, w = D^-1 (rhs - a),
+    d = c1 d + c2 w, zout = z + d) against numpy with the matrix values and
+    inverse diagonal blocks rounded as the kernel holds them (FP64; FP32;
+    bfloat16 values with FP32 inverse blocks).  The FP64 arithmetic after
+    the rounding is the same, so the bar is 1e-10 normwise."""
+    from tacs_amd._lib import lib
+    if kind == "solid":
+        p = SolidProblem(6, dv="element")
+        p.set_thickness(np.random.default_rng(5).uniform(0.5, 1.5, p.mesh.nelem))
+    else:
+        p = ShellProblem(9, warp=0.02, seed=4)
+    p.assemble()
+    a = p.assembler
+    K, rowp, cols, A = _dense(p)
+    b = A.shape[1]
+    n = rowp.size - 1
+    pc = TACS.Pc(p.mat)
+    pc.factor()
+    Dinv = np.zeros((n, b, b))
+    xv, yv = a.createVec(), a.createVec()
+    for c in range(b):
+        e = np.zeros((n, b))
+        e[:, c] = 1.0
+        xv.getArray()[:] = e.reshape(-1)
+        pc.applyFactor(xv, yv)
+        Dinv[:, :, c] = yv.getArray().reshape(n, b)
+    if values == 64:
+        Kr, Dr = K, Dinv
+    elif values == 32:
+        Kr = K.astype(np.float32).astype(np.float64)
+        Dr = Dinv.astype(np.float32).astype(np.float64)
+    else:
+        Kr = _bf16(K)
+        Dr = Dinv.astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(values + b)
+    z = rng.standard_normal(b * n)
+    rhs = Kr @ rng.standard_normal(b * n)
+    d0 = rng.standard_normal(b * n)
+    c1, c2 = 0.37, 1.9
+    vecs = [a.createVec() for _ in range(4)]
+    for v, arr in zip(vecs, (z, np.zeros_like(z), rhs, d0)):
+        v.getArray()[:] = arr
+    ptrs = [v.getDevicePointer() for v in vecs]
+    rc = lib().tacs_mat_cheb_step(p.mat.ptr, pc.ptr, values, ptrs[0], ptrs[1], ptrs[2], ptrs[3],
+                                  c1, c2)
+    assert rc == 0
+    w = np.einsum("nij,nj->ni", Dr, (rhs - Kr @ z).reshape(n, b)).reshape(-1)
+    d_exp = c1 * d0 + c2 * w
+    z_exp = z + d_exp
+    d_got, z_got = vecs[3].getArray().copy(), vecs[1].getArray().copy()
+    assert np.abs(d_got - d_exp).max() <= 1e-10 * np.abs(d_exp).max()
+    assert np.abs(z_got - z_exp).max() <= 1e-10 * np.abs(z_exp).max()
+    # z and rhs unchanged
+    assert np.array_equal(vecs[0].getArray(), z) and np.array_equal(vecs[2].getArray(), rhs)
