@@ -267,6 +267,29 @@ int tacs_hip_shell_mat_type(int prec, int nelem, const int* conn, const double* 
                             double kscale, double mscale, void* ke, double* ev,
                             hipStream_t stream);
 
+/* Node-coordinate derivatives of the Quad4 shell family, exact (forward-mode
+   dual numbers): es[12 e + k] = d/dX_k of element e's term (coordinate k =
+   3 node + component), for node gathering (tacs_hip_gather_elem_vectors_b
+   with b = 3, nv = 12).  mode 0: scale psi^T R_e at the state u
+   (TACSAssembler::addAdjointResXptSensProducts src/TACSAssembler.cpp:5136
+   over TACSElement::addAdjResXptProduct src/elements/TACSElement.cpp:270);
+   1: the compliance term scale sum_q detXd e.s (TACSCompliance::
+   getElementXptSens src/functions/TACSCompliance.cpp:129); 2: the
+   continuous KS failure term with the aggregate's ks_max and ks_sum
+   (TACSKSFailure::getElementXptSens src/functions/TACSKSFailure.cpp:289,
+   through addPointQuantityXptSens src/elements/TACSElement.cpp:746). */
+int tacs_hip_shell_xpt_kernel(int mode, int nelem, const int* conn, const double* X,
+                              const double* u, const double* psi, const double* telem,
+                              const ShellMaterial* mat, double scale, double ks_weight,
+                              double ks_max, double ks_sum, double safety, double* es,
+                              hipStream_t stream);
+/* out[12 q + k] = scale d/dX_k (psi . F_q) for the pressure loads F_q of the
+   listed shell elements (TACSShellPressure through
+   TACSElement::addAdjResXptProduct). */
+int tacs_hip_shell_pressure_xpt(int naux, const int* aux_elem, const double* pressure,
+                                const int* conn, const double* X, const double* psi, double scale,
+                                double* out, hipStream_t stream);
+
 /* Sum element matrices ke (FP64 when prec = 64, FP32 when 32) into b x b
    block-CSR rows in ascending element order, Dirichlet rows fused
    (TACSAssembler::addMatValues src/TACSAssembler.h:545 -> BCSRMat::addRowValues
@@ -279,7 +302,7 @@ int tacs_hip_gather_elem_matrices(int b, int npe, int prec, int nrows, const int
 
 /* Node gather of element vectors with b values per node (the reverse of
    TACSBVec::setValues ADD src/bpmat/TACSBVec.cpp:757): (b, nv) = (6, 24),
-   (3, 24), (3, 81). */
+   (3, 24), (3, 81), (3, 12). */
 int tacs_hip_gather_elem_vectors_b(int b, int nv, int nnodes, const int* n2e_ptr, const int* n2e,
                                    const double* ev, int accumulate, double* out,
                                    hipStream_t stream);

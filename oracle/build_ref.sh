@@ -33,3 +33,21 @@ g++ -O2 -fPIC -std=c++11 -w $INC -c "$HERE/ref_driver.cpp" -o "$OUT/ref_driver.o
 g++ -shared -o "$OUT/libtacs_ref.so" $OBJS "$OUT/ref_driver.o" \
   -L/opt/conda/lib -Wl,-rpath,/opt/conda/lib -lmpi -lmkl_rt
 echo "built $OUT/libtacs_ref.so"
+
+# The same sources with TACS_USE_COMPLEX (complex-step element derivatives)
+# plus oracle/ref_driver_cplx.cpp -> oracle/_ref/cplx/libtacs_ref_cplx.so
+OUTC=$OUT/cplx
+mkdir -p "$OUTC"
+OBJSC=""
+for s in $SRCS; do
+  o=$OUTC/$(basename "${s%.cpp}").o
+  if [ ! -f "$o" ] || [ "$REF/$s" -nt "$o" ]; then
+    g++ -O2 -fPIC -std=c++11 -w -DTACS_USE_COMPLEX $INC -c "$REF/$s" -o "$o"
+  fi
+  OBJSC="$OBJSC $o"
+done
+g++ -O2 -fPIC -std=c++11 -w -DTACS_USE_COMPLEX $INC -c "$HERE/ref_driver_cplx.cpp" \
+  -o "$OUTC/ref_driver_cplx.o"
+g++ -shared -o "$OUTC/libtacs_ref_cplx.so" $OBJSC "$OUTC/ref_driver_cplx.o" \
+  -L/opt/conda/lib -Wl,-rpath,/opt/conda/lib -lmpi -lmkl_rt
+echo "built $OUTC/libtacs_ref_cplx.so"

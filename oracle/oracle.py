@@ -439,6 +439,51 @@ def shell_compliance_sens(conn, X, u, t_elem, E, nu):
 
 
 # ---------------------------------------------------------------- node coordinates
+def quad4_adj_xpt(X, v, psi, E, nu, t, scale=1.0):
+    out = np.zeros(12)
+    lib().orc_quad4_adj_xpt(_p(_dd(X)), _p(_dd(v)), _p(_dd(psi)), _c(E), _c(nu), _c(t),
+                            _c(scale), _p(out))
+    return out
+
+
+def quad4_pressure_xpt(X, p, psi, scale=1.0):
+    out = np.zeros(12)
+    lib().orc_quad4_pressure_xpt(_p(_dd(X)), _c(p), _p(_dd(psi)), _c(scale), _p(out))
+    return out
+
+
+def quad4_point_xpt(quantity, X, v, E, nu, ys, t, dfddet, dfdq):
+    """d/dX sum_q (dfddet_q detXd_q + dfdq_q q_q); quantity 2: strain energy
+    density, 1: failure index."""
+    out = np.zeros(12)
+    lib().orc_quad4_point_xpt(int(quantity), _p(_dd(X)), _p(_dd(v)), _c(E), _c(nu), _c(ys),
+                              _c(t), _p(_dd(dfddet)), _p(_dd(dfdq)), _p(out))
+    return out
+
+
+def shell_xpt_sens(mode, conn, X, u, t_elem, E, nu, psi=None, ys=270e6, ks_weight=0.0,
+                   max_fail=0.0, ks_sum=1.0, safety=1.0, scale=1.0):
+    """Node sensitivities of the shell mesh: mode 0 scale psi^T dR/dX, 1 the
+    compliance, 2 the continuous KS failure (aggregate max and sum given)."""
+    conn = _ii(conn).reshape(-1)
+    out = np.zeros(np.asarray(X).size)
+    lib().orc_shell_xpt_sens(int(mode), conn.size // 4, _p(conn, np.int32), _p(_dd(X)),
+                             _p(_dd(u)), _p(_dd(psi)) if psi is not None else None,
+                             _p(_dd(t_elem)), _c(E), _c(nu), _c(ys), _c(ks_weight),
+                             _c(max_fail), _c(ks_sum), _c(safety), _c(scale), _p(out))
+    return out
+
+
+def shell_pressure_xpt(conn, X, aux_elems, pressures, psi, scale=1.0):
+    conn = _ii(conn).reshape(-1)
+    aux = _ii(aux_elems)
+    out = np.zeros(np.asarray(X).size)
+    lib().orc_shell_pressure_xpt(aux.size, _p(aux, np.int32), _p(_dd(pressures)),
+                                 _p(conn, np.int32), _p(_dd(X)), _p(_dd(psi)), _c(scale),
+                                 _p(out))
+    return out
+
+
 def compliance_xptsens(order, conn, X, u, t_elem, C21, scale=1.0):
     conn = _ii(conn).reshape(-1)
     X = _dd(X).reshape(-1)

@@ -845,3 +845,173 @@ void orc_shell_compliance_sens(int nelem, const int *conn, const double *X,
     }
   }
 }
+
+/* ------------------------------------------------------------------------
+   Node-coordinate sensitivities.  The reference's TACSShellElement has no
+   analytic node derivatives: it inherits TACSElement::addAdjResXptProduct
+   and addPointQuantityXptSens (src/elements/TACSElement.cpp:270, :746), which
+   difference the residual / point quantity in each coordinate (forward steps
+   of 1e-7 in the real build, complex steps of 1e-30 in the complex build).
+   The restatement differences the same functions with a sixth-order central
+   stencil (steps of 1e-3 times the element size; truncation and rounding
+   both near 1e-12 relative), pinned against the complex-step values of the
+   reference's complex build (tests/golden/elements.npz).
+   ------------------------------------------------------------------------ */
+typedef double (*sh_xfun)(const double X[12], const void *ctx);
+
+static void sh_xgrad(sh_xfun f, const void *ctx, const double X[12], double g[12]) {
+  static const double cw[3] = {45.0, -9.0, 1.0};
+  double L = 0.0;
+  for (int a = 0; a < SH_NN; a++) {
+    for (int b = a + 1; b < SH_NN; b++) {
+      double d2 = 0.0;
+      for (int k = 0; k < 3; k++) d2 += (X[3 * a + k] - X[3 * b + k]) * (X[3 * a + k] - X[3 * b + k]);
+      if (d2 > L) L = d2;
+    }
+  }
+  const double h = 1e-3 * sqrt(L);
+  for (int k = 0; k < 12; k++) {
+    double Xp[12], acc = 0.0;
+    memcpy(Xp, X, sizeof(Xp));
+    for (int j = 0; j < 3; j++) {
+      Xp[k] = X[k] + (j + 1) * h;
+      double fp = f(Xp, ctx);
+      Xp[k] = X[k] - (j + 1) * h;
+      double fm = f(Xp, ctx);
+      acc += cw[j] * (fp - fm);
+    }
+    g[k] = acc / (60.0 * h);
+  }
+}
+
+typedef struct {
+  const double *v, *psi;
+  double E, nu, ys, t, p;
+  const double *dfddet, *dfdq;
+  int quantity;
+} sh_xctx;
+
+static double sh_x_adj(const double X[12], const void *c) {
+  const sh_xctx *x = (const sh_xctx *)c;
+  double res[SH_NV] = {0}, r = 0.0;
+  orc_quad4_element(X, x->v, x->E, x->nu, x->t, 1.0, res, NULL);
+  for (int i = 0; i < SH_NV; i++) r += x->psi[i] * res[i];
+  return r;
+}
+
+static double sh_x_pres(const double X[12], const void *c) {
+  const sh_xctx *x = (const sh_xctx *)c;
+  double res[SH_NV] = {0}, r = 0.0;
+  orc_quad4_pressure(X, x->p, res);
+  for (int i = 0; i < SH_NV; i++) r += x->psi[i] * res[i];
+  return r;
+}
+
+/* sum_q dfddet_q detXd_q + dfdq_q q_q with q the strain energy density e.s
+   (quantity 2, TACS_STRAIN_ENERGY_DENSITY) or the failure index (quantity 1),
+   both with the drill strain dropped (TACSShellElement::evalPointQuantity) */
+static double sh_x_point(const double X[12], const void *c) {
+  const sh_xctx *x = (const sh_xctx *)c;
+  double e[4][9], detw[4], r = 0.0;
+  orc_quad4_strains(X, x->v, 0, e, detw);
+  for (int q = 0; q < 4; q++) {
+    double val;
+    if (x->quantity == 1) {
+      val = orc_shell_failure(x->E, x->nu, x->ys, x->t, e[q], NULL, NULL);
+    } else {
+      double s[9];
+      orc_shell_stress(x->E, x->nu, x->t, e[q], s);
+      val = 0.0;
+      for (int k = 0; k < 9; k++) val += e[q][k] * s[k];
+    }
+    r += x->dfddet[q] * detw[q] + x->dfdq[q] * val;
+  }
+  return r;
+}
+
+/* scale psi^T dR/dX of one element (TACSElement::addAdjResXptProduct) */
+void orc_quad4_adj_xpt(const double X[12], const double v[24], const double psi[24],
+                       double E, double nu, double t, double scale, double out[12]) {
+  sh_xctx c = {v, psi, E, nu, 0.0, t, 0.0, NULL, NULL, 0};
+  sh_xgrad(sh_x_adj, &c, X, out);
+  for (int k = 0; k < 12; k++) out[k] *= scale;
+}
+
+/* scale psi^T dF/dX of the element's pressure load (TACSShellPressure) */
+void orc_quad4_pressure_xpt(const double X[12], double p, const double psi[24], double scale,
+                            double out[12]) {
+  sh_xctx c = {NULL, psi, 0.0, 0.0, 0.0, 0.0, p, NULL, NULL, 0};
+  sh_xgrad(sh_x_pres, &c, X, out);
+  for (int k = 0; k < 12; k++) out[k] *= scale;
+}
+
+/* d/dX sum_q (dfddet_q detXd_q + dfdq_q q_q) (TACSElement::addPointQuantityXptSens
+   with scale 1, summed over the quadrature points) */
+void orc_quad4_point_xpt(int quantity, const double X[12], const double v[24], double E,
+                         double nu, double ys, double t, const double dfddet[4],
+                         const double dfdq[4], double out[12]) {
+  sh_xctx c = {v, NULL, E, nu, ys, t, 0.0, dfddet, dfdq, quantity};
+  sh_xgrad(sh_x_point, &c, X, out);
+}
+
+static void sh_scatter_x(const int *c, const double g[12], double *out) {
+  for (int a = 0; a < SH_NN; a++) {
+    for (int k = 0; k < 3; k++) out[3 * c[a] + k] += g[3 * a + k];
+  }
+}
+
+/*
+  Mesh sums (TACSAssembler::addAdjointResXptSensProducts src/TACSAssembler.cpp:
+  5136 and addXptSens :4858 over the element routines above):
+    mode 0: scale psi^T dR/dX (element residuals),
+    mode 1: compliance, sum_q w detXd e.s (TACSCompliance::getElementXptSens),
+    mode 2: continuous KS failure with the aggregate's max and sum
+            (TACSKSFailure::getElementXptSens, KS_CONTINUOUS: dfddetXd =
+            w exp(k (f - max)) / (k sum), dfdq = safety w detXd exp(..) / sum).
+  out (3 per node) is added to.
+*/
+void orc_shell_xpt_sens(int mode, int nelem, const int *conn, const double *X,
+                        const double *u, const double *psi, const double *t_elem, double E,
+                        double nu, double ys, double ksWeight, double maxFail, double ksSum,
+                        double safety, double scale, double *out) {
+  for (int el = 0; el < nelem; el++) {
+    const int *c = &conn[4 * el];
+    double Xe[12], ue[24], pe[24], g[12];
+    sh_gather(c, X, u, Xe, ue);
+    if (mode == 0) {
+      sh_gather(c, X, psi, Xe, pe);
+      orc_quad4_adj_xpt(Xe, ue, pe, E, nu, t_elem[el], scale, g);
+    } else {
+      double e[4][9], detw[4], dfddet[4], dfdq[4];
+      orc_quad4_strains(Xe, ue, 0, e, detw);
+      for (int q = 0; q < 4; q++) {
+        if (mode == 1) {
+          double s[9], U = 0.0;
+          orc_shell_stress(E, nu, t_elem[el], e[q], s);
+          for (int k = 0; k < 9; k++) U += e[q][k] * s[k];
+          dfddet[q] = scale * U;
+          dfdq[q] = scale * detw[q];
+        } else {
+          double f = safety * orc_shell_failure(E, nu, ys, t_elem[el], e[q], NULL, NULL);
+          double ef = exp(ksWeight * (f - maxFail)) / ksSum;
+          dfddet[q] = scale * ef / ksWeight;
+          dfdq[q] = scale * safety * detw[q] * ef;
+        }
+      }
+      orc_quad4_point_xpt(mode == 1 ? 2 : 1, Xe, ue, E, nu, ys, t_elem[el], dfddet, dfdq, g);
+    }
+    sh_scatter_x(c, g, out);
+  }
+}
+
+/* scale psi^T dF/dX of pressure aux elements (one pressure each) */
+void orc_shell_pressure_xpt(int naux, const int *aux_elem, const double *p, const int *conn,
+                            const double *X, const double *psi, double scale, double *out) {
+  for (int i = 0; i < naux; i++) {
+    const int *c = &conn[4 * aux_elem[i]];
+    double Xe[12], pe[24], g[12];
+    sh_gather(c, X, psi, Xe, pe);
+    orc_quad4_pressure_xpt(Xe, p[i], pe, scale, g);
+    sh_scatter_x(c, g, out);
+  }
+}

@@ -3699,8 +3699,25 @@ void TACSAssembler::addAdjointResProducts(TacsScalar scale, int numAdjoints, TAC
 // of the functions w.r.t. the node coordinates at fixed state (solid family).
 void TACSAssembler::addXptSens(TacsScalar coef, int numFuncs, TACSFunction** funcs,
                                TACSBVec** dfdXpt) {
-  if (family != FAMILY_SOLID) {
-    tacs_set_error("addXptSens: only the solid element family is supported");
+  if (family == FAMILY_SHELL) {
+    // exact node derivatives of the element terms, 12 per element
+    ensure(d_ev, (size_t)numElements * 24);
+    for (int k = 0; k < numFuncs; k++) {
+      if (!funcs[k]) continue;
+      if (dynamic_cast<TACSCompliance*>(funcs[k])) {
+        KCHECK(tacs_hip_shell_xpt_kernel(1, numElements, d_conn, d_X, d_u, NULL, d_telem, &smat,
+                                         coef, 0, 0, 1, 1, d_ev, g_stream));
+      } else if (TACSKSFailure* f = dynamic_cast<TACSKSFailure*>(funcs[k])) {
+        KCHECK(tacs_hip_shell_xpt_kernel(2, numElements, d_conn, d_X, d_u, NULL, d_telem, &smat,
+                                         coef, f->ksWeight, f->maxFail, f->ksFailSum,
+                                         f->safetyFactor, d_ev, g_stream));
+      } else {
+        tacs_set_error("addXptSens: unsupported function type");
+        continue;
+      }
+      KCHECK(tacs_hip_gather_elem_vectors_b(3, 12, numNodes, d_n2e_ptr, d_n2e, d_ev, 1,
+                                            dfdXpt[k]->devWrite(), g_stream));
+    }
     return;
   }
   ensure(d_ev, (size_t)numElements * 3 * npe);
@@ -3728,8 +3745,23 @@ void TACSAssembler::addXptSens(TacsScalar coef, int numFuncs, TACSFunction** fun
 void TACSAssembler::addAdjointResXptSensProducts(TacsScalar scale, int numAdjoints,
                                                  TACSBVec** adjoint, TACSBVec** dfdXpt,
                                                  const TacsScalar lambda) {
-  if (family != FAMILY_SOLID) {
-    tacs_set_error("addAdjointResXptSensProducts: only the solid element family is supported");
+  if (family == FAMILY_SHELL) {
+    ensure(d_ev, (size_t)numElements * 24);
+    for (int k = 0; k < numAdjoints; k++) {
+      const double* psi = adjoint[k]->devRead();
+      double* out = dfdXpt[k]->devWrite();
+      KCHECK(tacs_hip_shell_xpt_kernel(0, numElements, d_conn, d_X, d_u, psi, d_telem, &smat,
+                                       scale, 0, 0, 1, 1, d_ev, g_stream));
+      KCHECK(tacs_hip_gather_elem_vectors_b(3, 12, numNodes, d_n2e_ptr, d_n2e, d_ev, 1, out,
+                                            g_stream));
+      if (uploadAux() == 0 && naux > 0) {
+        // pressure loads (aux residual scaled by lambda, TACSAssembler.cpp:4210-4225)
+        KCHECK(tacs_hip_shell_pressure_xpt(naux, d_aux_elem, d_aux_pres, d_conn, d_X, psi,
+                                           lambda * scale, d_aux_ev, g_stream));
+        KCHECK(tacs_hip_gather_elem_vectors_b(3, 12, numNodes, d_n2a_ptr, d_n2a, d_aux_ev, 1, out,
+                                              g_stream));
+      }
+    }
     return;
   }
   ensure(d_ev, (size_t)numElements * 3 * npe);

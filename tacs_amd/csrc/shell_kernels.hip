@@ -24,6 +24,62 @@ constexpr int SH_TEAM = 32;
 constexpr int SH_EPB = 4;  // elements (teams) per workgroup
 constexpr int SH_NV = 24;
 
+// Forward-mode dual number: value and derivative along one direction.  The
+// element routines instantiated with it give the exact directional
+// derivatives in the node coordinates, which the reference's shell element
+// approximates by differencing its residual and point quantities in each
+// coordinate (TACSElement::addAdjResXptProduct / addPointQuantityXptSens,
+// src/elements/TACSElement.cpp:270, :746; exact to rounding in its complex
+// build).
+struct ShDual {
+  double v, d;
+  ShDual() = default;
+  __host__ __device__ constexpr ShDual(double a) : v(a), d(0.0) {}
+  __host__ __device__ constexpr ShDual(double a, double b) : v(a), d(b) {}
+  __device__ explicit operator double() const { return v; }
+  // hidden friends: found by argument-dependent lookup only, so they never
+  // capture float or double arguments
+  friend __device__ ShDual operator+(ShDual a, ShDual b) { return ShDual(a.v + b.v, a.d + b.d); }
+  friend __device__ ShDual operator-(ShDual a, ShDual b) { return ShDual(a.v - b.v, a.d - b.d); }
+  friend __device__ ShDual operator-(ShDual a) { return ShDual(-a.v, -a.d); }
+  friend __device__ ShDual operator*(ShDual a, ShDual b) {
+    return ShDual(a.v * b.v, a.d * b.v + a.v * b.d);
+  }
+  friend __device__ ShDual operator/(ShDual a, ShDual b) {
+    const double q = a.v / b.v;
+    return ShDual(q, (a.d - q * b.d) / b.v);
+  }
+  friend __device__ ShDual& operator+=(ShDual& a, ShDual b) { return a = a + b; }
+  friend __device__ ShDual& operator-=(ShDual& a, ShDual b) { return a = a - b; }
+  friend __device__ ShDual& operator*=(ShDual& a, ShDual b) { return a = a * b; }
+  friend __device__ ShDual& operator/=(ShDual& a, ShDual b) { return a = a / b; }
+  friend __device__ bool operator<(ShDual a, ShDual b) { return a.v < b.v; }
+  friend __device__ bool operator>(ShDual a, ShDual b) { return a.v > b.v; }
+  friend __device__ bool operator==(ShDual a, ShDual b) { return a.v == b.v; }
+  friend __device__ bool operator!=(ShDual a, ShDual b) { return a.v != b.v; }
+  friend __device__ ShDual sqrt(ShDual a) {
+    const double r = ::sqrt(a.v);
+    return ShDual(r, r > 0.0 ? 0.5 * a.d / r : 0.0);
+  }
+  friend __device__ ShDual exp(ShDual a) {
+    const double e = ::exp(a.v);
+    return ShDual(e, e * a.d);
+  }
+  friend __device__ ShDual log(ShDual a) { return ShDual(::log(a.v), a.d / a.v); }
+};
+// node coordinate x, seeded with derivative 1 when it is the direction
+template <typename T>
+__device__ __forceinline__ T sh_seed(double x, bool dir) {
+  return T(x);
+}
+template <>
+__device__ __forceinline__ ShDual sh_seed<ShDual>(double x, bool dir) {
+  return ShDual(x, dir ? 1.0 : 0.0);
+}
+__device__ __forceinline__ double sh_dpart(double) { return 0.0; }
+__device__ __forceinline__ double sh_dpart(float) { return 0.0; }
+__device__ __forceinline__ double sh_dpart(ShDual a) { return a.d; }
+
 template <typename T>
 struct ShellLDS {
   T X[12];
@@ -41,6 +97,7 @@ struct ShellLDS {
   T u[SH_NV], psi[SH_NV];
   T e[4][9], ep[4][9], de[4][9];
   T val[4];
+  T red[SH_NV];
 };
 
 template <typename T>
@@ -237,6 +294,11 @@ enum {
   SH_ADJ_PRODUCT = 7,
   SH_COMPLIANCE_SV = 8,
   SH_COMPLIANCE_DV = 9,
+  // node-coordinate derivatives (T = ShDual, one team per element and
+  // coordinate direction; es[12 el + k] receives d/dX_k):
+  SH_XPT_ADJ = 10,         // scale psi^T R
+  SH_XPT_COMPLIANCE = 11,  // scale sum_q detXd_q e.s
+  SH_XPT_KS = 12,          // scale sum_q detXd_q exp(w (f_q - max)) / (w sum)
 };
 
 struct ShellParams {
@@ -253,7 +315,10 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
     double* __restrict__ ev, double* __restrict__ es) {
   __shared__ ShellLDS<T> lds_all[SH_EPB];
   const int team = threadIdx.x / SH_TEAM, lane = threadIdx.x % SH_TEAM;
-  const int el = blockIdx.x * SH_EPB + team;
+  constexpr bool XPT = MODE >= SH_XPT_ADJ;
+  const int gt = blockIdx.x * SH_EPB + team;
+  const int el = XPT ? gt / 12 : gt;
+  const int dir = XPT ? gt % 12 : -1;
   const bool active = el < nelem;
   ShellLDS<T>& L = lds_all[team];
   const ShellMat<T> M = {T(P.Q0), T(P.Q1), T(P.Q5), T(P.kcorr), T(P.drill_reg), T(P.ys)};
@@ -262,10 +327,11 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
   // ---- load the element's nodes and variables
   if (active) {
     const int* c = &conn[(size_t)el * 4];
-    if (lane < 12) L.X[lane] = T(Xg[3 * (size_t)c[lane / 3] + lane % 3]);
+    if (lane < 12) L.X[lane] = sh_seed<T>(Xg[3 * (size_t)c[lane / 3] + lane % 3], lane == dir);
     if (lane < 24) {
       L.u[lane] = ug ? T(ug[6 * (size_t)c[lane / 6] + lane % 6]) : T(0);
-      if (MODE == SH_ADJ_PRODUCT) L.psi[lane] = T(psig[6 * (size_t)c[lane / 6] + lane % 6]);
+      if (MODE == SH_ADJ_PRODUCT || MODE == SH_XPT_ADJ)
+        L.psi[lane] = T(psig[6 * (size_t)c[lane / 6] + lane % 6]);
     }
   }
   __syncthreads();
@@ -563,12 +629,36 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
         if (MODE == SH_ADJ_PRODUCT) accp += L.B[q][kk][c] * L.psi[c];
       }
       // the function evaluations use e[8] = 0 (TACSShellElement::evalPointQuantity)
-      bool drop_drill = (MODE != SH_RESIDUAL && MODE != SH_ADJ_PRODUCT);
+      bool drop_drill = (MODE != SH_RESIDUAL && MODE != SH_ADJ_PRODUCT && MODE != SH_XPT_ADJ);
       L.e[q][kk] = (drop_drill && kk == 8) ? T(0) : acc;
       if (MODE == SH_ADJ_PRODUCT) L.ep[q][kk] = accp;
     }
   }
   __syncthreads();
+
+  if (MODE == SH_XPT_ADJ) {
+    // psi . r with r_c = sum_q detw_q B_q[:,c] . s_q, summed in lane order
+    if (active && lane < SH_NV) {
+      T acc = T(0);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        T s[9];
+        sh_stress<T>(M, t, L.e[q], s);
+        T dot = T(0);
+#pragma unroll
+        for (int kk = 0; kk < 9; kk++) dot += L.B[q][kk][lane] * s[kk];
+        acc += L.detw[q] * dot;
+      }
+      L.red[lane] = acc * L.psi[lane];
+    }
+    __syncthreads();
+    if (active && lane == 0) {
+      T sum = T(0);
+      for (int c = 0; c < SH_NV; c++) sum += L.red[c];
+      es[(size_t)el * 12 + dir] = P.scale * sh_dpart(sum);
+    }
+    return;
+  }
 
   if (MODE == SH_RESIDUAL || MODE == SH_COMPLIANCE_SV) {
     // r_c = sum_q detw_q B_q[:,c] . s_q  (s = Cs e, doubled for dC/du)
@@ -616,18 +706,19 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
     int q = lane;
     const T* e = L.e[q];
     T v = T(0);
-    if (MODE == SH_COMPLIANCE) {
+    if (MODE == SH_COMPLIANCE || MODE == SH_XPT_COMPLIANCE) {
       T s[9];
       sh_stress<T>(M, t, e, s);
       T dot = T(0);
 #pragma unroll
       for (int kk = 0; kk < 9; kk++) dot += e[kk] * s[kk];
       v = L.detw[q] * dot;
-    } else if (MODE == SH_KS_MAX || MODE == SH_KS_SUM || MODE == SH_KS_DV) {
+    } else if (MODE == SH_KS_MAX || MODE == SH_KS_SUM || MODE == SH_KS_DV || MODE == SH_XPT_KS) {
       T de[9], dt;
       T f = T(P.safety) * sh_failure<T>(M, t, e, de, dt);
       if (MODE == SH_KS_MAX) v = f;
-      else if (MODE == SH_KS_SUM) v = L.detw[q] * exp(T(P.ks_weight) * (f - T(P.ks_max)));
+      else if (MODE == SH_KS_SUM || MODE == SH_XPT_KS)
+        v = L.detw[q] * exp(T(P.ks_weight) * (f - T(P.ks_max)));
       else
         v = T(P.scale) * exp(T(P.ks_weight) * (f - T(P.ks_max))) / T(P.ks_sum) * L.detw[q] *
             T(P.safety) * dt;
@@ -644,6 +735,15 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
     L.val[q] = v;
   }
   __syncthreads();
+  if (XPT) {
+    if (active && lane == 0) {
+      const T sum = ((L.val[0] + L.val[1]) + L.val[2]) + L.val[3];
+      double dv = sh_dpart(sum);
+      if (MODE == SH_XPT_KS) dv /= P.ks_weight * P.ks_sum;
+      es[(size_t)el * 12 + dir] = P.scale * dv;
+    }
+    return;
+  }
   if (active && lane == 0) {
     double r;
     if (MODE == SH_KS_MAX)
@@ -714,6 +814,51 @@ __global__ void gather_elem_vectors_b(int nnodes, const int* __restrict__ n2e_pt
 // Uniform pressure on shell elements (TACSShellPressure::addResidual,
 // TACSShellPressure.h:51-91): ev += N^T f, f = -p detXd n at each quadrature
 // point, n the interpolated node normal.  One thread per loaded element.
+template <typename T>
+__device__ __forceinline__ void sh_pressure_elem(const T X[12], T p, T r[24]) {
+  T fn[12];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    T g[6];
+    sh_grad3<T>(X, (i & 1) ? T(1) : T(-1), (i & 2) ? T(1) : T(-1), g);
+    T a[3] = {g[0], g[2], g[4]}, b[3] = {g[1], g[3], g[5]}, f[3];
+    sh_cross<T>(a, b, f);
+    T nrm = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    if (nrm != T(0)) {
+      T rn = T(1) / nrm;
+      f[0] *= rn;
+      f[1] *= rn;
+      f[2] *= rn;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) fn[3 * i + k] = f[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 24; k++) r[k] = T(0);
+#pragma unroll
+  for (int qp = 0; qp < 4; qp++) {
+    T x = T(sh_gauss(qp & 1)), y = T(sh_gauss(qp >> 1));
+    T g[6], n[3], Xd[9], Xinv[9];
+    sh_grad3<T>(X, x, y, g);
+    sh_interp3<T>(fn, x, y, n);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      Xd[3 * k] = g[2 * k];
+      Xd[3 * k + 1] = g[2 * k + 1];
+      Xd[3 * k + 2] = n[k];
+    }
+    T det = sh_inv3<T>(Xd, Xinv);
+    T f[3] = {p * -det * n[0], p * -det * n[1], p * -det * n[2]};
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      T N, Nx, Ny;
+      sh_shape<T>(a, x, y, N, Nx, Ny);
+#pragma unroll
+      for (int k = 0; k < 3; k++) r[6 * a + k] += N * f[k];
+    }
+  }
+}
+
 __global__ void shell_pressure_vectors(int naux, const int* __restrict__ aux_elem,
                                        const double* __restrict__ pres, const int* __restrict__ conn,
                                        const double* __restrict__ Xg, double lambda,
@@ -721,55 +866,38 @@ __global__ void shell_pressure_vectors(int naux, const int* __restrict__ aux_ele
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= naux) return;
   const int* c = &conn[(size_t)aux_elem[q] * 4];
-  double X[12], fn[12];
+  double X[12], r[24];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int k = 0; k < 3; k++) X[3 * a + k] = Xg[3 * (size_t)c[a] + k];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    double g[6];
-    sh_grad3<double>(X, (i & 1) ? 1.0 : -1.0, (i & 2) ? 1.0 : -1.0, g);
-    double a[3] = {g[0], g[2], g[4]}, b[3] = {g[1], g[3], g[5]}, f[3];
-    sh_cross<double>(a, b, f);
-    double nrm = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
-    if (nrm != 0.0) {
-      double r = 1.0 / nrm;
-      f[0] *= r;
-      f[1] *= r;
-      f[2] *= r;
-    }
-#pragma unroll
-    for (int k = 0; k < 3; k++) fn[3 * i + k] = f[k];
-  }
-  double r[24];
-#pragma unroll
-  for (int k = 0; k < 24; k++) r[k] = 0.0;
-  const double p = lambda * pres[q];
-#pragma unroll
-  for (int qp = 0; qp < 4; qp++) {
-    double x = sh_gauss(qp & 1), y = sh_gauss(qp >> 1);
-    double g[6], n[3], Xd[9], Xinv[9];
-    sh_grad3<double>(X, x, y, g);
-    sh_interp3<double>(fn, x, y, n);
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      Xd[3 * k] = g[2 * k];
-      Xd[3 * k + 1] = g[2 * k + 1];
-      Xd[3 * k + 2] = n[k];
-    }
-    double det = sh_inv3<double>(Xd, Xinv);
-    double f[3] = {p * -det * n[0], p * -det * n[1], p * -det * n[2]};
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-      double N, Nx, Ny;
-      sh_shape<double>(a, x, y, N, Nx, Ny);
-#pragma unroll
-      for (int k = 0; k < 3; k++) r[6 * a + k] += N * f[k];
-    }
-  }
+  sh_pressure_elem<double>(X, lambda * pres[q], r);
 #pragma unroll
   for (int k = 0; k < 24; k++) ev[(size_t)q * 24 + k] = r[k];
+}
+
+// Node-coordinate derivative of the pressure load: one thread per (loaded
+// element, coordinate k), out[12 q + k] = scale d/dX_k (psi . r).
+__global__ void shell_pressure_xpt(int naux, const int* __restrict__ aux_elem,
+                                   const double* __restrict__ pres, const int* __restrict__ conn,
+                                   const double* __restrict__ Xg, const double* __restrict__ psi,
+                                   double scale, double* __restrict__ out) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = gt / 12, dir = gt % 12;
+  if (q >= naux) return;
+  const int* c = &conn[(size_t)aux_elem[q] * 4];
+  ShDual X[12], r[24];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) X[3 * a + k] = sh_seed<ShDual>(Xg[3 * (size_t)c[a] + k], 3 * a + k == dir);
+  sh_pressure_elem<ShDual>(X, ShDual(pres[q]), r);
+  double acc = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) acc += psi[6 * (size_t)c[a] + k] * r[6 * a + k].d;
+  out[(size_t)q * 12 + dir] = scale * acc;
 }
 
 extern "C" {
@@ -827,6 +955,40 @@ int tacs_hip_shell_elem_kernel(int prec, int mode, int nelem, const int* conn, c
   return (int)hipGetLastError();
 }
 
+int tacs_hip_shell_xpt_kernel(int mode, int nelem, const int* conn, const double* X,
+                              const double* u, const double* psi, const double* telem,
+                              const ShellMaterial* mat, double scale, double ks_weight,
+                              double ks_max, double ks_sum, double safety, double* es,
+                              hipStream_t stream) {
+  if (nelem <= 0) return 0;
+  if (!mat || !es || mode < 0 || mode > 2 || (mode == 0 && !psi)) return TACS_HIP_ERR_ARG;
+  ShellParams P = shell_params(mat, scale, ks_weight, ks_max, ks_sum, safety);
+  const int grid = (int)(((long)nelem * 12 + SH_EPB - 1) / SH_EPB);
+  ShDual* ke = NULL;
+  double* ev = NULL;
+  if (mode == 0)
+    hipLaunchKernelGGL((shell_elem_kernel<ShDual, SH_XPT_ADJ>), dim3(grid), dim3(SH_TEAM * SH_EPB),
+                       0, stream, nelem, conn, X, u, psi, telem, P, ke, ev, es);
+  else if (mode == 1)
+    hipLaunchKernelGGL((shell_elem_kernel<ShDual, SH_XPT_COMPLIANCE>), dim3(grid),
+                       dim3(SH_TEAM * SH_EPB), 0, stream, nelem, conn, X, u, psi, telem, P, ke, ev,
+                       es);
+  else
+    hipLaunchKernelGGL((shell_elem_kernel<ShDual, SH_XPT_KS>), dim3(grid), dim3(SH_TEAM * SH_EPB),
+                       0, stream, nelem, conn, X, u, psi, telem, P, ke, ev, es);
+  return (int)hipGetLastError();
+}
+
+int tacs_hip_shell_pressure_xpt(int naux, const int* aux_elem, const double* pressure,
+                                const int* conn, const double* X, const double* psi, double scale,
+                                double* out, hipStream_t stream) {
+  if (naux <= 0) return 0;
+  const long nt = (long)naux * 12;
+  hipLaunchKernelGGL(shell_pressure_xpt, dim3((unsigned)((nt + 127) / 128)), dim3(128), 0, stream,
+                     naux, aux_elem, pressure, conn, X, psi, scale, out);
+  return (int)hipGetLastError();
+}
+
 int tacs_hip_shell_mat_type(int prec, int nelem, const int* conn, const double* X,
                             const double* u, const double* telem, const ShellMaterial* mat,
                             double kscale, double mscale, void* ke, double* ev,
@@ -872,6 +1034,9 @@ int tacs_hip_gather_elem_vectors_b(int b, int nv, int nnodes, const int* n2e_ptr
                        n2e_ptr, n2e, ev, accumulate, out);
   else if (b == 3 && nv == 24)
     hipLaunchKernelGGL((gather_elem_vectors_b<3, 24>), dim3(grid), dim3(256), 0, stream, nnodes,
+                       n2e_ptr, n2e, ev, accumulate, out);
+  else if (b == 3 && nv == 12)
+    hipLaunchKernelGGL((gather_elem_vectors_b<3, 12>), dim3(grid), dim3(256), 0, stream, nnodes,
                        n2e_ptr, n2e, ev, accumulate, out);
   else if (b == 3 && nv == 81)
     hipLaunchKernelGGL((gather_elem_vectors_b<3, 81>), dim3(grid), dim3(256), 0, stream, nnodes,

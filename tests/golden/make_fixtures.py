@@ -2,9 +2,17 @@
 the reference's own sources (oracle/_ref/libtacs_ref.so, built by
 oracle/build_ref.sh from /root/reference) on seeded inputs.  Run in a
 container that has the reference tree:  python tests/golden/make_fixtures.py
+
+The Quad4 shell node-coordinate sensitivities come from the complex build of
+the same sources (oracle/_ref/cplx/libtacs_ref_cplx.so: the reference's
+default element derivatives take complex steps there), run in a child
+process because the two builds define the same C++ symbols.
 """
 import ctypes
 import os
+import subprocess
+import sys
+import tempfile
 
 import numpy as np
 
@@ -135,10 +143,66 @@ def make(seed=2024):
         for k in range(2):
             L.ref_quad4_mat_type(k + 1, _d(X), _d(v), c(70e9), c(0.3), c(t), _d(mt[k]))
         out[key + "_mattype"] = mt
+    # Quad4 shell node-coordinate sensitivities (complex build, child process)
+    with tempfile.TemporaryDirectory() as tmp:
+        src = os.path.join(tmp, "in.npz")
+        dst = os.path.join(tmp, "out.npz")
+        np.savez(src, **{k: v for k, v in out.items() if k.startswith("quad4_")})
+        subprocess.run([sys.executable, os.path.abspath(__file__), "--cplx", src, dst], check=True)
+        with np.load(dst) as f:
+            out.update({k: f[k] for k in f.files})
     np.savez_compressed(os.path.join(HERE, "elements.npz"), **out)
     return out
 
 
+KS_WEIGHT = 30.0
+
+
+def make_cplx(src, dst):
+    """Element node sensitivities of the four Quad4 cases: psi^T dR/dX, the
+    pressure load's psi^T dF/dX (p = 1e4), the compliance element term
+    (TACSCompliance::getElementXptSens: dfdq = w detXd, dfddetXd = w U) and
+    the KS failure element term of a one-element KS_CONTINUOUS aggregate
+    (ksWeight 30; TACSKSFailure::getElementXptSens)."""
+    C = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "cplx", "libtacs_ref_cplx.so"))
+    C.refc_quad4_point.restype = ctypes.c_double
+    c = ctypes.c_double
+    res = {}
+    with np.load(src) as f:
+        for s in range(4):
+            key = f"quad4_{s}"
+            X, v, psi = f[key + "_X"].copy(), f[key + "_v"].copy(), f[key + "_psi"].copy()
+            t = float(f[key + "_t"][0])
+            adj = np.zeros(12)
+            C.refc_quad4_adj_xpt(_d(X), _d(v), _d(psi), c(70e9), c(0.3), c(t), c(1.0), _d(adj))
+            pres = np.zeros(12)
+            C.refc_quad4_pressure_xpt(_d(X), c(1e4), _d(psi), c(1.0), _d(pres))
+            U, F, det = np.zeros(4), np.zeros(4), np.zeros(4)
+            for n in range(4):
+                dd = ctypes.c_double()
+                U[n] = C.refc_quad4_point(2, _d(X), _d(v), c(70e9), c(0.3), c(270e6), c(t), n,
+                                          ctypes.byref(dd))
+                det[n] = dd.value
+                F[n] = C.refc_quad4_point(1, _d(X), _d(v), c(70e9), c(0.3), c(270e6), c(t), n,
+                                          ctypes.byref(dd))
+            comp = np.zeros(12)
+            C.refc_quad4_point_xpt(2, _d(X), _d(v), c(70e9), c(0.3), c(270e6), c(t), _d(U),
+                                   _d(det.copy()), _d(comp))
+            fmax = F.max()
+            ksum = np.sum(det * np.exp(KS_WEIGHT * (F - fmax)))
+            expf = np.exp(KS_WEIGHT * (F - fmax)) / ksum
+            ks = np.zeros(12)
+            C.refc_quad4_point_xpt(1, _d(X), _d(v), c(70e9), c(0.3), c(270e6), c(t),
+                                   _d(expf / KS_WEIGHT), _d(det * expf), _d(ks))
+            res[key + "_adjxpt"], res[key + "_presxpt"] = adj, pres
+            res[key + "_compxpt"], res[key + "_ksxpt"] = comp, ks
+            res[key + "_sed"] = U
+    np.savez(dst, **res)
+
+
 if __name__ == "__main__":
-    make()
-    print("wrote", os.path.join(HERE, "elements.npz"))
+    if len(sys.argv) == 4 and sys.argv[1] == "--cplx":
+        make_cplx(sys.argv[2], sys.argv[3])
+    else:
+        make()
+        print("wrote", os.path.join(HERE, "elements.npz"))

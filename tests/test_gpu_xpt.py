@@ -100,3 +100,92 @@ def test_total_compliance_node_gradient():
           - O.adj_res_xpt_product(2, m.conn, X, uo, po, t, C)
           - O.traction_xpt_product(2, m.conn, X, p.aux_elems, p.aux_faces, trac, po))
     assert np.abs(g.getArray() - go).max() <= 1e-7 * np.abs(go).max()
+
+
+@pytest.mark.parametrize("n,warp", [(4, 0.02), (5, 0.0)])
+def test_shell_xpt_sens_match_oracle(n, warp):
+    """Quad4 shells: addXptSens (compliance, continuous KS failure) and
+    addAdjointResXptSensProducts (element residuals plus the pressure loads)
+    against the oracle, whose element derivatives are pinned to the
+    complex-step values of the reference's complex build."""
+    from tacs_amd.problems import ShellProblem
+
+    p = ShellProblem(n, warp=warp, seed=3, dv="element", ys=270e6)
+    a, m = p.assembler, p.mesh
+    rng = np.random.default_rng(21)
+    t = rng.uniform(0.005, 0.02, m.nelem)
+    p.set_thickness(t)
+    X = m.X.reshape(-1)
+    u = 1e-3 * rng.standard_normal(6 * m.nnodes)
+    psi = rng.standard_normal(6 * m.nnodes)
+    uv, pv = a.createVec(), a.createVec()
+    uv.getArray()[:] = u
+    pv.getArray()[:] = psi
+    a.setVariables(uv)
+    comp = functions.Compliance(a)
+    ksf = functions.KSFailure(a, ksWeight=30.0)
+    a.evalFunctions([comp, ksf])
+    dX = [a.createNodeVec(), a.createNodeVec()]
+    a.addXptSens([comp, ksf], dX)
+    E, NU = 70e9, 0.3
+    oc = O.shell_xpt_sens(1, m.conn, X, u, t, E, NU)
+    ks, fmax, ksum = O.shell_ks_failure(m.conn, X, u, t, E, NU, 270e6, 30.0)
+    ok = O.shell_xpt_sens(2, m.conn, X, u, t, E, NU, ys=270e6, ks_weight=30.0,
+                          max_fail=fmax, ks_sum=ksum)
+    g0, g1 = dX[0].getArray(), dX[1].getArray()
+    assert np.abs(g0 - oc).max() <= RTOL * np.abs(oc).max()
+    assert np.abs(g1 - ok).max() <= RTOL * np.abs(ok).max()
+    dR = a.createNodeVec()
+    a.addAdjointResXptSensProducts([pv], [dR], -1.0)
+    oR = (O.shell_xpt_sens(0, m.conn, X, u, t, E, NU, psi=psi, scale=-1.0)
+          + O.shell_pressure_xpt(m.conn, X, p.aux_elems, np.full(p.aux_elems.size, p.pressure),
+                                 psi, -1.0))
+    assert np.abs(dR.getArray() - oR).max() <= RTOL * np.abs(oR).max()
+
+
+def test_shell_total_compliance_node_gradient_fd():
+    """Total compliance node gradient of the pressure-loaded plate (solve,
+    adjoint, addXptSens, addAdjointResXptSensProducts) against central
+    differences of the solved compliance in a few coordinates."""
+    from tacs_amd.problems import ShellProblem
+
+    def solve_compliance(p):
+        p.assemble()
+        flag, *_ = p.solve_fp64(m=200, restarts=200, rtol=1e-13)
+        assert flag == 1
+        comp = functions.Compliance(p.assembler)
+        return p.assembler.evalFunctions([comp])[0], comp
+
+    p = ShellProblem(4, warp=0.02, seed=7)
+    a, m = p.assembler, p.mesh
+    c0, comp = solve_compliance(p)
+    pc = TACS.Pc(p.mat)
+    pc.factor()
+    ksm = TACS.KSM(p.mat, pc, 200, 200)
+    ksm.setTolerances(1e-13, 1e-30)
+    dfdu = a.createVec()
+    a.addSVSens([comp], [dfdu])
+    a.applyBCs(dfdu)
+    psi = a.createVec()
+    assert ksm.solve(dfdu, psi) == 1
+    g = a.createNodeVec()
+    a.addXptSens([comp], [g])
+    a.addAdjointResXptSensProducts([psi], [g], -1.0)
+    grad = g.getArray().copy()
+    X0 = m.X.reshape(-1).copy()
+    rng = np.random.default_rng(2)
+    interior = np.setdiff1d(np.arange(m.nnodes), p.bc_nodes)
+    for node in rng.choice(interior, 3, replace=False):
+        i = 3 * node + 2
+        h = 1e-6
+        vals = []
+        for sgn in (1, -1):
+            Xs = X0.copy()
+            Xs[i] += sgn * h
+            p.X.getArray()[:] = Xs
+            a.setNodes(p.X)
+            vals.append(solve_compliance(p)[0])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(fd - grad[i]) <= 1e-5 * np.abs(grad).max(), (i, fd, grad[i])
+    p.X.getArray()[:] = X0
+    a.setNodes(p.X)

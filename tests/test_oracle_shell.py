@@ -227,3 +227,69 @@ def test_shell_compliance_sensitivities_fd():
         fd = (O.shell_compliance(m.conn, X, u, tp, E, NU)
               - O.shell_compliance(m.conn, X, u, tm, E, NU)) / (2 * h)
         assert abs(fd - dfdt[e]) <= 1e-6 * abs(c0)
+
+
+KS_WEIGHT = 30.0
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_quad4_xpt_against_complex_step_reference(fixtures, case):
+    """Element node sensitivities (psi^T dR/dX, pressure psi^T dF/dX, the
+    compliance and one-element continuous KS terms) against the complex-step
+    values of the reference's complex build."""
+    k = f"quad4_{case}"
+    X, v, psi = fixtures[k + "_X"], fixtures[k + "_v"], fixtures[k + "_psi"]
+    t = float(fixtures[k + "_t"].reshape(-1)[0])
+
+    def close(a, b):
+        return np.abs(a - b).max() <= 1e-11 * np.abs(b).max()
+
+    assert close(O.quad4_adj_xpt(X, v, psi, E, NU, t), fixtures[k + "_adjxpt"])
+    assert close(O.quad4_pressure_xpt(X, 1e4, psi), fixtures[k + "_presxpt"])
+    U = fixtures[k + "_sed"]
+    det = fixtures[k + "_det"]
+    assert close(O.quad4_point_xpt(2, X, v, E, NU, YS, t, U, det), fixtures[k + "_compxpt"])
+    F = fixtures[k + "_fail"]
+    ksum = np.sum(det * np.exp(KS_WEIGHT * (F - F.max())))
+    ef = np.exp(KS_WEIGHT * (F - F.max())) / ksum
+    assert close(O.quad4_point_xpt(1, X, v, E, NU, YS, t, ef / KS_WEIGHT, det * ef),
+                 fixtures[k + "_ksxpt"])
+    # the strain energy density the compliance term uses
+    e, detw = O.quad4_strains(X, v, drill=False)
+    sed = np.array([e[q] @ O.shell_stress(E, NU, t, e[q]) for q in range(4)])
+    assert np.abs(sed - U).max() <= 1e-13 * np.abs(U).max()
+
+
+def test_shell_xpt_mesh_translation_invariance_and_total_derivative():
+    """Mesh sums of the node sensitivities: a rigid translation of every node
+    changes neither the residual nor the functions (at fixed u), so each
+    coordinate direction sums to zero over the nodes; the compliance and KS
+    gradients match central differences of the oracle's mesh totals."""
+    m = PlateMesh(3, 3, 0.3, 0.3)
+    rng = np.random.default_rng(8)
+    X = m.X.copy()
+    X[:, 2] += 0.01 * rng.standard_normal(m.nnodes)
+    X = X.reshape(-1)
+    t = rng.uniform(0.005, 0.02, m.nelem)
+    u = 1e-3 * rng.standard_normal(6 * m.nnodes)
+    psi = rng.standard_normal(6 * m.nnodes)
+    gA = O.shell_xpt_sens(0, m.conn, X, u, t, E, NU, psi=psi)
+    gC = O.shell_xpt_sens(1, m.conn, X, u, t, E, NU)
+    ks, fmax, ksum = O.shell_ks_failure(m.conn, X, u, t, E, NU, YS, KS_WEIGHT)
+    gK = O.shell_xpt_sens(2, m.conn, X, u, t, E, NU, ys=YS, ks_weight=KS_WEIGHT,
+                          max_fail=fmax, ks_sum=ksum)
+    gP = O.shell_pressure_xpt(m.conn, X, np.arange(m.nelem), np.full(m.nelem, 1e4), psi)
+    for g in (gA, gC, gK, gP):
+        s = g.reshape(-1, 3).sum(axis=0)
+        assert np.abs(s).max() <= 1e-9 * np.abs(g).max()
+    for i in rng.choice(X.size, 6, replace=False):
+        h = 1e-6
+        Xp, Xm = X.copy(), X.copy()
+        Xp[i] += h
+        Xm[i] -= h
+        dC = (O.shell_compliance(m.conn, Xp, u, t, E, NU)
+              - O.shell_compliance(m.conn, Xm, u, t, E, NU)) / (2 * h)
+        assert abs(dC - gC[i]) <= 1e-6 * np.abs(gC).max()
+        dK = (O.shell_ks_failure(m.conn, Xp, u, t, E, NU, YS, KS_WEIGHT)[0]
+              - O.shell_ks_failure(m.conn, Xm, u, t, E, NU, YS, KS_WEIGHT)[0]) / (2 * h)
+        assert abs(dK - gK[i]) <= 1e-6 * np.abs(gK).max()
