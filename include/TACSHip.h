@@ -187,6 +187,14 @@ class TACSTraction3D : public TACSElement {
 // src/elements/shell/TACSShellElementTransform.h:8-92
 class TACSShellTransform : public TACSObject {};
 class TACSShellNaturalTransform : public TACSShellTransform {};
+// src/elements/shell/TACSShellElementTransform.h:95: frame from the reference
+// axis projected on the tangent plane
+class TACSShellRefAxisTransform : public TACSShellTransform {
+ public:
+  TACSShellRefAxisTransform(const TacsScalar _axis[]);
+  void getRefAxis(TacsScalar _axis[]);
+  TacsScalar axis[3];
+};
 
 // TACSQuad4Shell: src/elements/shell/TACSShellElementDefs.h:18-20 over
 // TACSShellElement (src/elements/shell/TACSShellElement.h:39); linear model,

@@ -48,13 +48,17 @@ typedef struct SolidMaterial {
    (TACSMaterialProperties::evalTangentStiffness2D
    src/constitutive/TACSMaterialProperties.cpp:323), shear correction kcorr,
    drilling regularization (TACSShellConstitutive.cpp:61), yield stress and
-   density.  tOffset = 0. */
+   density.  tOffset = 0.  The element frame: ref_axis = 0 for
+   TACSShellNaturalTransform, 1 for TACSShellRefAxisTransform with the unit
+   axis (src/elements/shell/TACSShellElementTransform.h:95). */
 typedef struct ShellMaterial {
   double Q[6];
   double kcorr;
   double drill_reg;
   double ys;
   double rho;
+  double axis[3];
+  int ref_axis;
 } ShellMaterial;
 
 /* ------------------------------------------------------------------------
@@ -671,6 +675,9 @@ tacs_obj tacs_iso_shell_constitutive_create(tacs_obj props, double t, int tNum, 
                                             double tub, double tOffset, double kcorr);
 /* TACSShellNaturalTransform (src/elements/shell/TACSShellElementTransform.h:21) */
 tacs_obj tacs_shell_natural_transform_create(void);
+/* TACSShellRefAxisTransform(axis) (TACSShellElementTransform.h:95); the axis
+   is normalised as there. */
+tacs_obj tacs_shell_ref_axis_transform_create(const double* axis);
 /* TACSQuad4Shell(transform, con) (src/elements/shell/TACSShellElementDefs.h:18;
    tacs/elements.pyx:694) */
 tacs_obj tacs_quad4_shell_create(tacs_obj transform, tacs_obj con);

@@ -318,6 +318,16 @@ def quad4_element(X, v, E, nu, t, alpha=1.0):
     return res, mat.reshape(24, 24)
 
 
+def shell_set_ref_axis(axis):
+    """Frame of the oracle's shell functions: TACSShellRefAxisTransform(axis),
+    or the natural transform for None."""
+    if axis is None:
+        lib().orc_shell_set_ref_axis(None)
+    else:
+        a = np.ascontiguousarray(axis, dtype=np.float64)
+        lib().orc_shell_set_ref_axis(_p(a))
+
+
 def quad4_mass(X, rho, t, alpha=1.0):
     mat = np.zeros(576)
     lib().orc_quad4_mass(_p(_dd(X)), _c(rho), _c(t), _c(alpha), _p(mat))

@@ -208,6 +208,28 @@ void ref_quad4_mat_type(int matType, const double* X, const double* vars, double
   elem->decref();
 }
 
+// Residual, Jacobian and failure index (point n) of one TACSQuad4Shell with
+// the reference-axis transform (TACSShellRefAxisTransform).
+void ref_quad4_shell_axis(const double* X, const double* vars, double E, double nu, double ys,
+                          double t, const double* axis, double* res, double* mat, double* fail) {
+  TACSMaterialProperties* props = new TACSMaterialProperties(2700.0, 921.0, E, nu, ys, 0.0, 0.0);
+  TACSIsoShellConstitutive* con = new TACSIsoShellConstitutive(props, t, 0);
+  TACSShellTransform* transform = new TACSShellRefAxisTransform(axis);
+  TACSElement* elem = new TACSQuad4Shell(transform, con);
+  elem->incref();
+  double zero[24] = {0};
+  for (int i = 0; i < 24; i++) res[i] = 0.0;
+  for (int i = 0; i < 576; i++) mat[i] = 0.0;
+  elem->addJacobian(0, 0.0, 1.0, 0.0, 0.0, X, vars, zero, zero, res, mat);
+  for (int n = 0; n < 4; n++) {
+    double pt[3], det = 0.0;
+    elem->getQuadraturePoint(n, pt);
+    elem->evalPointQuantity(0, TACS_FAILURE_INDEX, 0.0, n, pt, X, vars, zero, zero, &det,
+                            &fail[n]);
+  }
+  elem->decref();
+}
+
 // psi^T dR/dt of one TACSQuad4Shell (thickness DV, number 0).
 double ref_quad4_adj_product(const double* X, const double* vars, const double* psi, double E,
                              double nu, double ys, double t) {

@@ -137,6 +137,22 @@ static void sh_frame(const double Xxi[6], const double *n, double Xd[9]) {
   }
 }
 
+/* The element frame in use: the natural transform, or
+   TACSShellRefAxisTransform (TACSShellElementTransform.h:95-160) after
+   orc_shell_set_ref_axis(axis) (NULL returns to the natural transform).  The
+   setting is global to the oracle's shell functions. */
+static int sh_ref_axis = 0;
+static double sh_axis[3] = {0.0, 0.0, 0.0};
+
+void orc_shell_set_ref_axis(const double *axis) {
+  sh_ref_axis = axis != NULL;
+  if (axis) {
+    double nrm = sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    double inv = nrm != 0.0 ? 1.0 / nrm : 0.0;
+    for (int k = 0; k < 3; k++) sh_axis[k] = inv * axis[k];
+  }
+}
+
 /* TACSShellNaturalTransform::computeTransform
    (TACSShellElementTransform.h:25-92).  The reference removes the normal
    component from the first entry of t1 only, three times over; that is what
@@ -148,10 +164,16 @@ static void sh_transform(const double Xxi[6], const double n0[3], double T[9]) {
   n[1] *= inv;
   n[2] *= inv;
   double t1[3] = {Xxi[0], Xxi[2], Xxi[4]};
-  double d = n[0] * t1[0] + n[1] * t1[1] + n[2] * t1[2];
-  t1[0] = t1[0] - d * n[0];
-  t1[0] = t1[0] - d * n[0];
-  t1[0] = t1[0] - d * n[0];
+  if (sh_ref_axis) {
+    /* TACSShellRefAxisTransform::computeTransform (:116-160) */
+    double an = sh_axis[0] * n[0] + sh_axis[1] * n[1] + sh_axis[2] * n[2];
+    for (int k = 0; k < 3; k++) t1[k] = sh_axis[k] - an * n[k];
+  } else {
+    double d = n[0] * t1[0] + n[1] * t1[1] + n[2] * t1[2];
+    t1[0] = t1[0] - d * n[0];
+    t1[0] = t1[0] - d * n[0];
+    t1[0] = t1[0] - d * n[0];
+  }
   inv = 1.0 / sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
   t1[0] *= inv;
   t1[1] *= inv;

@@ -116,6 +116,13 @@ tacs_obj tacs_iso_shell_constitutive_create(tacs_obj props, double t, int tNum, 
 tacs_obj tacs_shell_natural_transform_create(void) {
   return (TACSObject*)new TACSShellNaturalTransform();
 }
+tacs_obj tacs_shell_ref_axis_transform_create(const double* axis) {
+  if (!axis) {
+    tacs_set_error("%s: axis is NULL", __func__);
+    return NULL;
+  }
+  return (TACSObject*)new TACSShellRefAxisTransform(axis);
+}
 tacs_obj tacs_quad4_shell_create(tacs_obj transform, tacs_obj con) {
   REQUIRE_NULL(TACSShellTransform, tr, transform);
   REQUIRE_NULL(TACSShellConstitutive, c, con);

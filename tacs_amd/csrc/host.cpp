@@ -179,6 +179,14 @@ TACSIsoShellConstitutive::~TACSIsoShellConstitutive() {
   if (props) props->decref();
 }
 
+TACSShellRefAxisTransform::TACSShellRefAxisTransform(const TacsScalar _axis[]) {
+  const double nrm = sqrt(_axis[0] * _axis[0] + _axis[1] * _axis[1] + _axis[2] * _axis[2]);
+  const double inv = nrm != 0.0 ? 1.0 / nrm : 0.0;
+  for (int k = 0; k < 3; k++) axis[k] = inv * _axis[k];
+}
+void TACSShellRefAxisTransform::getRefAxis(TacsScalar _axis[]) {
+  for (int k = 0; k < 3; k++) _axis[k] = axis[k];
+}
 TACSQuad4Shell::TACSQuad4Shell(TACSShellTransform* tr, TACSShellConstitutive* c)
     : transform(tr), con(c) {
   transform->incref();
@@ -2862,11 +2870,24 @@ int TACSAssembler::initialize() {
     family = FAMILY_SHELL;
     if (varsPerNode != 6) return tacs_set_error("initialize: shells need 6 variables per node");
     double kcorr = 0.0;
+    TACSShellTransform* tr0 = NULL;
     for (int e = 0; e < numElements; e++) {
       TACSQuad4Shell* el = dynamic_cast<TACSQuad4Shell*>(elements[e]);
       if (!el) return tacs_set_error("initialize: element %d is not a TACSQuad4Shell", e);
-      if (!dynamic_cast<TACSShellNaturalTransform*>(el->transform))
-        return tacs_set_error("initialize: only TACSShellNaturalTransform is supported");
+      TACSShellRefAxisTransform* ra = dynamic_cast<TACSShellRefAxisTransform*>(el->transform);
+      if (!dynamic_cast<TACSShellNaturalTransform*>(el->transform) && !ra)
+        return tacs_set_error("initialize: the shell transform must be TACSShellNaturalTransform "
+                              "or TACSShellRefAxisTransform");
+      // one frame per assembler: one transform object, or transforms of one kind
+      // and axis
+      if (!tr0) tr0 = el->transform;
+      if (el->transform != tr0) {
+        TACSShellRefAxisTransform* ra0 = dynamic_cast<TACSShellRefAxisTransform*>(tr0);
+        const bool same = (ra && ra0) ? (ra->axis[0] == ra0->axis[0] && ra->axis[1] == ra0->axis[1] &&
+                                         ra->axis[2] == ra0->axis[2])
+                                      : (!ra && !ra0);
+        if (!same) return tacs_set_error("initialize: elements must share one shell transform");
+      }
       TACSIsoShellConstitutive* con = dynamic_cast<TACSIsoShellConstitutive*>(el->con);
       if (!con) return tacs_set_error("initialize: element %d constitutive is not isotropic", e);
       if (con->tOffset != 0.0) return tacs_set_error("initialize: tOffset != 0 is not supported");
@@ -2896,6 +2917,9 @@ int TACSAssembler::initialize() {
     smat.drill_reg = 0.1;  // TACSShellConstitutive::DRILLING_REGULARIZATION
     smat.ys = props0->ys;
     smat.rho = props0->rho;
+    TACSShellRefAxisTransform* ra0 = dynamic_cast<TACSShellRefAxisTransform*>(tr0);
+    smat.ref_axis = ra0 ? 1 : 0;
+    for (int k = 0; k < 3; k++) smat.axis[k] = ra0 ? ra0->axis[k] : 0.0;
   } else {
     // Solid family: TACSElement3D + TACSLinearElasticity3D +
     // TACSSolidConstitutive on one hexahedral basis and one material.

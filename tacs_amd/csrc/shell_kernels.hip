@@ -193,15 +193,35 @@ __device__ __forceinline__ void sh_mm(const T A[9], const T B[9], T C[9]) {
 // TACSShellNaturalTransform::computeTransform (TACSShellElementTransform.h:
 // 25-92), including its removal of the normal component from t1[0] only,
 // three times over.  Columns of T: t1, t2 = n x t1, n.
+// Element frame T = [t1 t2 n].  Natural transform
+// (TACSShellNaturalTransform::computeTransform, TACSShellElementTransform.h:
+// 25-60): t1 from X,xi with the reference's projection step, which subtracts
+// d n[0] from t1[0] three times and leaves t1[1], t1[2] as they are, so t1
+// is not orthogonal to n on a warped element; kept as it is, since it
+// decides the element matrices.  Reference-axis transform
+// (TACSShellRefAxisTransform::computeTransform :116-160): t1 = the unit
+// axis minus its normal component, normalised.  t2 = n x t1.
 template <typename T>
-__device__ __forceinline__ void sh_transform(const T Xxi[6], const T n0[3], T Tm[9]) {
+__device__ __forceinline__ void sh_transform(const T Xxi[6], const T n0[3], T Tm[9],
+                                             int ref_axis = 0, const double* axis = nullptr) {
   T inv = T(1) / sqrt(n0[0] * n0[0] + n0[1] * n0[1] + n0[2] * n0[2]);
   T n[3] = {n0[0] * inv, n0[1] * inv, n0[2] * inv};
-  T t1[3] = {Xxi[0], Xxi[2], Xxi[4]};
-  T d = n[0] * t1[0] + n[1] * t1[1] + n[2] * t1[2];
-  t1[0] = t1[0] - d * n[0];
-  t1[0] = t1[0] - d * n[0];
-  t1[0] = t1[0] - d * n[0];
+  T t1[3];
+  if (ref_axis) {
+    const T a[3] = {T(axis[0]), T(axis[1]), T(axis[2])};
+    T an = a[0] * n[0] + a[1] * n[1] + a[2] * n[2];
+    t1[0] = a[0] - an * n[0];
+    t1[1] = a[1] - an * n[1];
+    t1[2] = a[2] - an * n[2];
+  } else {
+    t1[0] = Xxi[0];
+    t1[1] = Xxi[2];
+    t1[2] = Xxi[4];
+    T d = n[0] * t1[0] + n[1] * t1[1] + n[2] * t1[2];
+    t1[0] = t1[0] - d * n[0];
+    t1[0] = t1[0] - d * n[0];
+    t1[0] = t1[0] - d * n[0];
+  }
   inv = T(1) / sqrt(t1[0] * t1[0] + t1[1] * t1[1] + t1[2] * t1[2]);
   t1[0] *= inv;
   t1[1] *= inv;
@@ -305,6 +325,8 @@ struct ShellParams {
   double Q0, Q1, Q5, kcorr, drill_reg, ys;
   double scale, ks_weight, ks_max, ks_sum, safety;
   double mass, rho;  // coefficient of the consistent mass (matrix mode), density
+  double axis[3];    // reference axis of TACSShellRefAxisTransform (ref_axis = 1)
+  int ref_axis;
 };
 
 template <typename T, int MODE>
@@ -360,7 +382,7 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
   // ---- frames: node drill frames, tying points, quadrature points
   if (active && lane < 4) {
     T Tm[9], Xd[9], Xinv[9];
-    sh_transform<T>(L.nXxi[lane], L.fn[lane], Tm);
+    sh_transform<T>(L.nXxi[lane], L.fn[lane], Tm, P.ref_axis, P.axis);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       Xd[3 * k] = L.nXxi[lane][2 * k];
@@ -389,7 +411,7 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
     sh_grad3<T>(L.X, x, y, g);
     sh_interp3<T>(&L.fn[0][0], x, y, n0);
     sh_grad3<T>(&L.fn[0][0], x, y, nxi);
-    sh_transform<T>(g, n0, Tm);
+    sh_transform<T>(g, n0, Tm, P.ref_axis, P.axis);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       Xd[3 * k] = g[2 * k];
@@ -918,6 +940,8 @@ static ShellParams shell_params(const ShellMaterial* m, double scale, double ks_
   P.safety = safety;
   P.mass = 0.0;
   P.rho = m->rho;
+  P.ref_axis = m->ref_axis;
+  for (int k = 0; k < 3; k++) P.axis[k] = m->axis[k];
   return P;
 }
 

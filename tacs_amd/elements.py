@@ -3,6 +3,8 @@ LinearHexaBasis :297, QuadraticHexaBasis :308, LinearElasticity3D :498,
 Element3D :564, Traction3D :597)."""
 import ctypes
 
+import numpy as np
+
 from ._lib import lib
 from ._obj import TACSHandle
 
@@ -73,6 +75,19 @@ class ShellNaturalTransform(TACSHandle):
 
     def __init__(self):
         super().__init__(lib().tacs_shell_natural_transform_create())
+
+
+class ShellRefAxisTransform(TACSHandle):
+    """TACSShellRefAxisTransform(axis) (tacs/elements.pyx:659): the local 1-axis
+    is the reference axis projected on the tangent plane."""
+
+    def __init__(self, axis):
+        a = np.ascontiguousarray(axis, dtype=np.float64).reshape(3)
+        super().__init__(lib().tacs_shell_ref_axis_transform_create(a.ctypes.data))
+        self.axis = a / np.linalg.norm(a) if np.linalg.norm(a) != 0.0 else a
+
+    def getRefAxis(self):
+        return self.axis.copy()
 
 
 class Quad4Shell(TACSHandle):

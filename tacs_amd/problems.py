@@ -106,11 +106,12 @@ class ShellProblem:
     on the perimeter, rotations free) or "clamped" (all six dofs of the
     perimeter).  dv: None, "shared" or "element" as for SolidProblem.
     warp: amplitude of a seeded random z perturbation of the interior nodes
-    (0 keeps the plate flat).
+    (0 keeps the plate flat).  axis: reference axis of a
+    TACSShellRefAxisTransform (None: the natural transform).
     """
 
     def __init__(self, n, L=1.0, E=70e9, nu=0.3, rho=2700.0, ys=270e6, t0=0.01, dv=None,
-                 pressure=1e4, bc="ss", warp=0.0, seed=0):
+                 pressure=1e4, bc="ss", warp=0.0, seed=0, axis=None):
         from .meshes import PlateMesh
         self.mesh = m = PlateMesh(n, n, L, L)
         if warp:
@@ -118,7 +119,8 @@ class ShellProblem:
             interior = np.setdiff1d(np.arange(m.nnodes), m.edge_nodes())
             m.X[interior, 2] += rng.uniform(-warp, warp, interior.size)
         self.props = constitutive.MaterialProperties(rho=rho, E=E, nu=nu, ys=ys)
-        self.transform = elements.ShellNaturalTransform()
+        self.transform = (elements.ShellNaturalTransform() if axis is None
+                          else elements.ShellRefAxisTransform(axis))
         a = self.assembler = TACS.Assembler.create(None, 6, m.nnodes, m.nelem)
         ptr = np.arange(0, 4 * m.nelem + 1, 4, dtype=np.intc)
         a.setElementConnectivity(ptr, m.conn.reshape(-1))

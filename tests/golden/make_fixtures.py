@@ -143,6 +143,16 @@ def make(seed=2024):
         for k in range(2):
             L.ref_quad4_mat_type(k + 1, _d(X), _d(v), c(70e9), c(0.3), c(t), _d(mt[k]))
         out[key + "_mattype"] = mt
+    # Quad4 shells with TACSShellRefAxisTransform (residual, Jacobian, failure)
+    axis = np.array([0.3, 1.0, 0.2])
+    out["quad4_axis"] = axis
+    for s in range(4):
+        key = f"quad4_{s}"
+        X, v, t = out[key + "_X"], out[key + "_v"], float(out[key + "_t"][0])
+        res, mat, fail = np.zeros(24), np.zeros(576), np.zeros(4)
+        L.ref_quad4_shell_axis(_d(X), _d(v), c(70e9), c(0.3), c(270e6), c(t), _d(axis), _d(res),
+                               _d(mat), _d(fail))
+        out[key + "_axis_res"], out[key + "_axis_mat"], out[key + "_axis_fail"] = res, mat, fail
     # Quad4 shell node-coordinate sensitivities (complex build, child process)
     with tempfile.TemporaryDirectory() as tmp:
         src = os.path.join(tmp, "in.npz")

@@ -233,3 +233,39 @@ def test_shell_solve_functions_and_thickness_adjoint(solver):
     go = dkdt_o + O.shell_adj_res_product(conn, X, u, po, t, E, NU, -1.0)
     tol = 1e-6 if solver == "mixed" else 1e-8
     assert np.abs(g - go).max() <= tol * np.abs(go).max()
+
+
+@pytest.mark.parametrize("axis", [(0.3, 1.0, 0.2), (1.0, 0.0, 0.0)])
+def test_shell_ref_axis_transform_matches_oracle(axis):
+    """TACSShellRefAxisTransform elements: Jacobian, residual, compliance and
+    KS failure against the oracle in the same frame; the axis does not change
+    the results (isotropic material)."""
+    p = ShellProblem(4, warp=0.03, seed=2, dv="element", ys=YS, axis=axis)
+    a, m = p.assembler, p.mesh
+    rng = np.random.default_rng(6)
+    t = rng.uniform(0.005, 0.02, m.nelem)
+    p.set_thickness(t)
+    u = rng.normal(0.0, 1e-3, 6 * m.nnodes)
+    uv = a.createVec()
+    uv.getArray()[:] = u
+    a.setVariables(uv)
+    a.assembleJacobian(1.0, 0.0, 0.0, p.res, p.mat)
+    _, _, A = p.mat.getArrays()
+    comp = functions.Compliance(a)
+    ksf = functions.KSFailure(a, ksWeight=30.0)
+    vals = a.evalFunctions([comp, ksf])
+    X = m.X.reshape(-1)
+    try:
+        O.shell_set_ref_axis(np.asarray(axis, float))
+        rowp, cols, oA, ores = _oracle(p, t, u)
+        oc = O.shell_compliance(m.conn, X, u, t, E, NU)
+        oks = O.shell_ks_failure(m.conn, X, u, t, E, NU, YS, 30.0)[0]
+        O.shell_set_ref_axis(None)
+        _, _, nA, _ = _oracle(p, t, u)
+    finally:
+        O.shell_set_ref_axis(None)
+    assert np.abs(A.reshape(-1) - oA).max() <= RTOL * np.abs(oA).max()
+    assert np.abs(p.res.getArray() - ores).max() <= RTOL * np.abs(ores).max()
+    assert abs(vals[0] - oc) <= RTOL * abs(oc) and abs(vals[1] - oks) <= RTOL * abs(oks)
+    # a different frame from the natural transform on this warped plate
+    assert np.abs(A.reshape(-1) - nA).max() >= 1e-6 * np.abs(nA).max()

@@ -293,3 +293,33 @@ def test_shell_xpt_mesh_translation_invariance_and_total_derivative():
         dK = (O.shell_ks_failure(m.conn, Xp, u, t, E, NU, YS, KS_WEIGHT)[0]
               - O.shell_ks_failure(m.conn, Xm, u, t, E, NU, YS, KS_WEIGHT)[0]) / (2 * h)
         assert abs(dK - gK[i]) <= 1e-6 * np.abs(gK).max()
+
+
+def test_quad4_ref_axis_transform_against_reference(fixtures):
+    """TACSShellRefAxisTransform: residual, Jacobian and failure index of the
+    reference build against the oracle's frame; the results do not depend
+    on the axis (isotropic material, orthonormal frame), while the natural
+    transform's frame, whose t1 keeps part of the normal on a warped
+    element, gives different matrices."""
+    axis = fixtures["quad4_axis"]
+    try:
+        for case in range(4):
+            k = f"quad4_{case}"
+            X, v = fixtures[k + "_X"], fixtures[k + "_v"]
+            t = float(fixtures[k + "_t"].reshape(-1)[0])
+            O.shell_set_ref_axis(axis)
+            res, mat = O.quad4_element(X, v, E, NU, t)
+            R, M = fixtures[k + "_axis_res"], fixtures[k + "_axis_mat"].reshape(24, 24)
+            assert np.abs(res - R).max() <= 1e-14 * np.abs(R).max()
+            assert np.abs(mat - M).max() <= 1e-14 * np.abs(M).max()
+            for n in range(4):
+                f, _ = O.quad4_failure(X, v, E, NU, YS, t, n)
+                assert abs(f - fixtures[k + "_axis_fail"][n]) <= 1e-13 * abs(f)
+            O.shell_set_ref_axis([0.0, 0.2, -1.0] if case % 2 else [1.0, 0.0, 0.0])
+            _, mat2 = O.quad4_element(X, v, E, NU, t)
+            assert np.abs(mat2 - M).max() <= 1e-12 * np.abs(M).max()
+            O.shell_set_ref_axis(None)
+            _, mat3 = O.quad4_element(X, v, E, NU, t)
+            assert np.abs(mat3 - M).max() >= 1e-6 * np.abs(M).max()
+    finally:
+        O.shell_set_ref_axis(None)
