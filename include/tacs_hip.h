@@ -48,9 +48,12 @@ typedef struct SolidMaterial {
    (TACSMaterialProperties::evalTangentStiffness2D
    src/constitutive/TACSMaterialProperties.cpp:323), shear correction kcorr,
    drilling regularization (TACSShellConstitutive.cpp:61), yield stress and
-   density.  tOffset = 0.  The element frame: ref_axis = 0 for
-   TACSShellNaturalTransform, 1 for TACSShellRefAxisTransform with the unit
-   axis (src/elements/shell/TACSShellElementTransform.h:95). */
+   density, and the thickness offset toff (tOffset: B = -toff t^2 Q,
+   D = t^3 (1/12 + toff^2) Q, faces at t (1/2 - toff) and -t (1/2 + toff),
+   mass moments rho t, -rho t^2 toff, rho t^3 (toff^2 + 1/12)).  The element
+   frame: ref_axis = 0 for TACSShellNaturalTransform, 1 for
+   TACSShellRefAxisTransform with the unit axis
+   (src/elements/shell/TACSShellElementTransform.h:95). */
 typedef struct ShellMaterial {
   double Q[6];
   double kcorr;
@@ -59,6 +62,7 @@ typedef struct ShellMaterial {
   double rho;
   double axis[3];
   int ref_axis;
+  double toff;
 } ShellMaterial;
 
 /* ------------------------------------------------------------------------

@@ -328,6 +328,11 @@ def shell_set_ref_axis(axis):
         lib().orc_shell_set_ref_axis(_p(a))
 
 
+def shell_set_offset(toff):
+    """tOffset of the oracle's shell constitutive model (0: mid-surface)."""
+    lib().orc_shell_set_offset(_c(toff))
+
+
 def quad4_mass(X, rho, t, alpha=1.0):
     mat = np.zeros(576)
     lib().orc_quad4_mass(_p(_dd(X)), _c(rho), _c(t), _c(alpha), _p(mat))

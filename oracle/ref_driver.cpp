@@ -230,6 +230,36 @@ void ref_quad4_shell_axis(const double* X, const double* vars, double E, double 
   elem->decref();
 }
 
+// One TACSQuad4Shell whose TACSIsoShellConstitutive has the offset toff:
+// residual and Jacobian, mass matrix (getMatType), psi^T dR/dt, and per
+// quadrature point the failure index and its thickness derivative.
+void ref_quad4_shell_offset(const double* X, const double* vars, const double* psi, double E,
+                            double nu, double ys, double t, double toff, double* res,
+                            double* mat, double* mass, double* adj, double* fail,
+                            double* dfail) {
+  TACSMaterialProperties* props = new TACSMaterialProperties(2700.0, 921.0, E, nu, ys, 0.0, 0.0);
+  TACSIsoShellConstitutive* con = new TACSIsoShellConstitutive(props, t, 0, 0.0, 1e20, toff);
+  TACSElement* elem = new TACSQuad4Shell(new TACSShellNaturalTransform(), con);
+  elem->incref();
+  double zero[24] = {0};
+  for (int i = 0; i < 24; i++) res[i] = 0.0;
+  for (int i = 0; i < 576; i++) mat[i] = 0.0;
+  elem->addJacobian(0, 0.0, 1.0, 0.0, 0.0, X, vars, zero, zero, res, mat);
+  elem->getMatType(TACS_MASS_MATRIX, 0, 0.0, X, vars, mass);
+  adj[0] = 0.0;
+  elem->addAdjResProduct(0, 0.0, 1.0, psi, X, vars, zero, zero, 1, adj);
+  for (int n = 0; n < 4; n++) {
+    double pt[3], det = 0.0, one = 1.0;
+    elem->getQuadraturePoint(n, pt);
+    elem->evalPointQuantity(0, TACS_FAILURE_INDEX, 0.0, n, pt, X, vars, zero, zero, &det,
+                            &fail[n]);
+    dfail[n] = 0.0;
+    elem->addPointQuantityDVSens(0, TACS_FAILURE_INDEX, 0.0, 1.0, n, pt, X, vars, zero, zero,
+                                 &one, 1, &dfail[n]);
+  }
+  elem->decref();
+}
+
 // psi^T dR/dt of one TACSQuad4Shell (thickness DV, number 0).
 double ref_quad4_adj_product(const double* X, const double* vars, const double* psi, double E,
                              double nu, double ys, double t) {

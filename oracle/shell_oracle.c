@@ -396,6 +396,14 @@ static const double sh_drill_reg = 0.1; /* TACSShellConstitutive.cpp:61 */
    (TACSIsoShellConstitutive.cpp:153-189, tOffset = 0) through
    computeStress (TACSShellConstitutive.h:133): A = tQ, B = 0, D = t^3/12 Q,
    As = kcorr t G, drill = 0.5 * 0.1 * (As0 + As2). */
+/* Thickness offset of TACSIsoShellConstitutive (tOffset, the mid-surface
+   at t (1/2 - tOffset) below the top face), global to the oracle's shell
+   functions; 0 unless orc_shell_set_offset says otherwise. */
+static double sh_toff = 0.0;
+void orc_shell_set_offset(double toff) { sh_toff = toff; }
+
+/* computeStress with A = t Q, B = -tOffset t^2 Q, D = t^3 (1/12 + tOffset^2) Q
+   (TACSIsoShellConstitutive::evalStress :153-189) */
 void orc_shell_stress(double E, double nu, double t, const double e[9], double s[9]) {
   double Q[6], a[3], b[3];
   sh_q(E, nu, Q);
@@ -403,9 +411,10 @@ void orc_shell_stress(double E, double nu, double t, const double e[9], double s
   sh_symm3(Q, &e[3], b);
   double I = t * t * t / 12.0;
   double As = sh_kcorr * t * Q[5];
+  double Bc = -sh_toff * t * t, Dc = sh_toff * sh_toff * t * t * t;
   for (int k = 0; k < 3; k++) {
-    s[k] = t * a[k];
-    s[3 + k] = I * b[k];
+    s[k] = t * a[k] + Bc * b[k];
+    s[3 + k] = I * b[k] + Dc * b[k] + Bc * a[k];
   }
   s[6] = As * e[6];
   s[7] = As * e[7];
@@ -454,14 +463,15 @@ void orc_quad4_element(const double X[12], const double v[24], double E,
 
 /*
   Consistent mass of one element, added as alpha * M: the kinetic energy
-  T = 1/2 sum_q w_q detXd_q (m0 |u0dot|^2 + m2 |d0dot|^2) of
-  TACSShellElement::computeEnergies (TACSShellElement.h:272-284) with the mass
-  moments of TACSIsoShellConstitutive::evalMassMoments
-  (TACSIsoShellConstitutive.cpp:120-129) at tOffset = 0 (m0 = rho t, m1 = 0,
-  m2 = rho t^3 / 12).  u0 interpolates the node displacements, d0 the node
+  T = 1/2 sum_q w_q detXd_q (m0 |u0dot|^2 + 2 m1 u0dot.d0dot + m2 |d0dot|^2)
+  of TACSShellElement::computeEnergies (TACSShellElement.h:272-284) with the
+  mass moments of TACSIsoShellConstitutive::evalMassMoments
+  (TACSIsoShellConstitutive.cpp:120-129): m0 = rho t, m1 = -rho t^2 tOffset,
+  m2 = rho t^3 (tOffset^2 + 1/12).  u0 interpolates the node displacements, d0 the node
   directors d_a = q_a x fn_a (TACSLinearizedRotation::computeDirectorRates,
   TACSDirector.h:210-227); both are linear in the element variables, so
-  M = sum_q w_q detXd_q (m0 Nu_q^T Nu_q + m2 Nd_q^T Nd_q) with the 3 x 24
+  M = sum_q w_q detXd_q (m0 Nu^T Nu + m1 (Nu^T Nd + Nd^T Nu) + m2 Nd^T Nd)
+  (per quadrature point q) with the 3 x 24
   maps Nu_q, Nd_q built column by column from unit variable vectors.  This is
   the matrix TACSShellElement::getMatType(TACS_MASS_MATRIX) (:644) assembles
   through addJacobian's gamma terms (:605-617) and
@@ -472,7 +482,8 @@ void orc_quad4_mass(const double X[12], double rho, double t, double alpha,
   double fn[12], zero[SH_NV] = {0}, e[4][9], detw[4];
   sh_node_normals(X, fn, NULL);
   orc_quad4_strains(X, zero, 1, e, detw);
-  const double m0 = rho * t, m2 = rho * t * t * t / 12.0;
+  const double m0 = rho * t, m1 = -rho * t * t * sh_toff,
+               m2 = rho * t * t * t * (sh_toff * sh_toff + 1.0 / 12.0);
   for (int q = 0; q < 4; q++) {
     double pt[2] = {sh_gp[q % 2], sh_gp[q / 2]};
     double Nu[3][SH_NV], Nd[3][SH_NV];
@@ -494,7 +505,9 @@ void orc_quad4_mass(const double X[12], double rho, double t, double alpha,
     for (int r = 0; r < SH_NV; r++) {
       for (int c = 0; c < SH_NV; c++) {
         double acc = 0.0;
-        for (int k = 0; k < 3; k++) acc += m0 * Nu[k][r] * Nu[k][c] + m2 * Nd[k][r] * Nd[k][c];
+        for (int k = 0; k < 3; k++)
+          acc += m0 * Nu[k][r] * Nu[k][c] + m2 * Nd[k][r] * Nd[k][c] +
+                 m1 * (Nu[k][r] * Nd[k][c] + Nd[k][r] * Nu[k][c]);
         mat[SH_NV * r + c] += alpha * detw[q] * acc;
       }
     }
@@ -511,13 +524,16 @@ double orc_quad4_adj_product(const double X[12], const double v[24],
   orc_quad4_strains(X, v, 1, eu, detw);
   orc_quad4_strains(X, psi, 1, ep, detw);
   sh_q(E, nu, Q);
-  double dI = 0.25 * t * t, out = 0.0;
+  /* addStressDVSens (:229-251): d/dt of A, B, D */
+  double dI = (3.0 * sh_toff * sh_toff + 0.25) * t * t, dB = -2.0 * sh_toff * t, out = 0.0;
   for (int q = 0; q < 4; q++) {
     double a[3], b[3];
     sh_symm3(Q, ep[q], a);
     sh_symm3(Q, &ep[q][3], b);
     double val = a[0] * eu[q][0] + a[1] * eu[q][1] + a[2] * eu[q][2] +
                  dI * (b[0] * eu[q][3] + b[1] * eu[q][4] + b[2] * eu[q][5]) +
+                 dB * (a[0] * eu[q][3] + a[1] * eu[q][4] + a[2] * eu[q][5] +
+                       b[0] * eu[q][0] + b[1] * eu[q][1] + b[2] * eu[q][2]) +
                  sh_kcorr * Q[5] *
                      (ep[q][6] * eu[q][6] + ep[q][7] * eu[q][7] +
                       sh_drill_reg * ep[q][8] * eu[q][8]);
@@ -553,7 +569,7 @@ double orc_shell_failure(double E, double nu, double ys, double t,
   const double ksw = 100.0;
   double Q[6], et[3], eb[3], st[3], sb[3], pt[3], pb[3];
   sh_q(E, nu, Q);
-  double ht = 0.5 * t, hb = -0.5 * t;
+  double ht = (0.5 - sh_toff) * t, hb = -(0.5 + sh_toff) * t;
   for (int k = 0; k < 3; k++) {
     et[k] = e[k] + ht * e[3 + k];
     eb[k] = e[k] + hb * e[3 + k];
@@ -577,8 +593,8 @@ double orc_shell_failure(double E, double nu, double ys, double t,
       dfde[6] = dfde[7] = dfde[8] = 0.0;
     }
     if (dfdt) {
-      *dfdt = ft * 0.5 * (phit[0] * e[3] + phit[1] * e[4] + phit[2] * e[5]) -
-              fb * 0.5 * (phib[0] * e[3] + phib[1] * e[4] + phib[2] * e[5]);
+      *dfdt = ft * (0.5 - sh_toff) * (phit[0] * e[3] + phit[1] * e[4] + phit[2] * e[5]) -
+              fb * (0.5 + sh_toff) * (phib[0] * e[3] + phib[1] * e[4] + phib[2] * e[5]);
     }
   }
   return val;
@@ -860,8 +876,11 @@ void orc_shell_compliance_sens(int nelem, const int *conn, const double *X,
         double a3[3], b3[3];
         sh_symm3(Q, e[q], a3);
         sh_symm3(Q, &e[q][3], b3);
+        const double dI = (3.0 * sh_toff * sh_toff + 0.25) * t * t;
         dfdt[el] += detw[q] * (a3[0] * e[q][0] + a3[1] * e[q][1] + a3[2] * e[q][2] +
-                               0.25 * t * t * (b3[0] * e[q][3] + b3[1] * e[q][4] + b3[2] * e[q][5]) +
+                               dI * (b3[0] * e[q][3] + b3[1] * e[q][4] + b3[2] * e[q][5]) -
+                               4.0 * sh_toff * t *
+                                   (a3[0] * e[q][3] + a3[1] * e[q][4] + a3[2] * e[q][5]) +
                                sh_kcorr * Q[5] * (e[q][6] * e[q][6] + e[q][7] * e[q][7]));
       }
     }

@@ -2865,11 +2865,12 @@ int TACSAssembler::initialize() {
   std::vector<double> tfixed(numElements);
   TACSMaterialProperties* props0 = NULL;
   if (numElements > 0 && dynamic_cast<TACSQuad4Shell*>(elements[0])) {
-    // Shell family: TACSQuad4Shell + TACSShellNaturalTransform +
-    // TACSIsoShellConstitutive (tOffset = 0), one material.
+    // Shell family: TACSQuad4Shell + TACSShellNaturalTransform or
+    // TACSShellRefAxisTransform + TACSIsoShellConstitutive, one material,
+    // kcorr and tOffset.
     family = FAMILY_SHELL;
     if (varsPerNode != 6) return tacs_set_error("initialize: shells need 6 variables per node");
-    double kcorr = 0.0;
+    double kcorr = 0.0, toff = 0.0;
     TACSShellTransform* tr0 = NULL;
     for (int e = 0; e < numElements; e++) {
       TACSQuad4Shell* el = dynamic_cast<TACSQuad4Shell*>(elements[e]);
@@ -2890,12 +2891,13 @@ int TACSAssembler::initialize() {
       }
       TACSIsoShellConstitutive* con = dynamic_cast<TACSIsoShellConstitutive*>(el->con);
       if (!con) return tacs_set_error("initialize: element %d constitutive is not isotropic", e);
-      if (con->tOffset != 0.0) return tacs_set_error("initialize: tOffset != 0 is not supported");
       TACSMaterialProperties* p = con->props;
       if (!props0) {
         props0 = p;
         kcorr = con->kcorr;
+        toff = con->tOffset;
       }
+      if (con->tOffset != toff) return tacs_set_error("initialize: elements must share tOffset");
       if (p != props0 && (p->E != props0->E || p->nu != props0->nu || p->rho != props0->rho ||
                           p->ys != props0->ys))
         return tacs_set_error("initialize: elements must share one material");
@@ -2919,6 +2921,7 @@ int TACSAssembler::initialize() {
     smat.rho = props0->rho;
     TACSShellRefAxisTransform* ra0 = dynamic_cast<TACSShellRefAxisTransform*>(tr0);
     smat.ref_axis = ra0 ? 1 : 0;
+    smat.toff = toff;
     for (int k = 0; k < 3; k++) smat.axis[k] = ra0 ? ra0->axis[k] : 0.0;
   } else {
     // Solid family: TACSElement3D + TACSLinearElasticity3D +

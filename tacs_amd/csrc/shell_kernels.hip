@@ -104,7 +104,18 @@ template <typename T>
 struct ShellMat {
   T Q0, Q1, Q5;  // plane-stress Q (Q2 = Q4 = 0, Q3 = Q0)
   T kcorr, drill_reg, ys;
+  T toff;  // tOffset: the mid-surface sits t (1/2 - toff) below the top face
 };
+
+// Entry (p, q) of the cross-product matrix a^x (a^x v = a x v)
+template <typename T>
+__device__ __forceinline__ T sh_skew(const T* a, int p, int q) {
+  if (p == q) return T(0);
+  const int r = 3 - p - q;  // the third index
+  const T v = a[r];
+  // (p, q) in cyclic order (0,1), (1,2), (2,0) -> -a_r, otherwise +a_r
+  return ((p + 1) % 3 == q) ? -v : v;
+}
 
 // Gauss points of TacsGaussQuadPts2 (src/elements/basis/TACSGaussQuadrature.h:26)
 __device__ __forceinline__ double sh_gauss(int i) { return i ? 0.577350269189626 : -0.577350269189626; }
@@ -240,14 +251,26 @@ __device__ __forceinline__ void sh_transform(const T Xxi[6], const T n0[3], T Tm
 // tOffset = 0; computeStress TACSShellConstitutive.h:133)
 template <typename T>
 __device__ __forceinline__ void sh_stress(const ShellMat<T>& m, T t, const T e[9], T s[9]) {
+  // A = t Q, B = -toff t^2 Q, D = t^3 (1/12 + toff^2) Q
   T I = t * t * t / T(12);
   T As = m.kcorr * t * m.Q5;
-  s[0] = t * (m.Q0 * e[0] + m.Q1 * e[1]);
-  s[1] = t * (m.Q1 * e[0] + m.Q0 * e[1]);
-  s[2] = t * (m.Q5 * e[2]);
-  s[3] = I * (m.Q0 * e[3] + m.Q1 * e[4]);
-  s[4] = I * (m.Q1 * e[3] + m.Q0 * e[4]);
-  s[5] = I * (m.Q5 * e[5]);
+  T a0 = m.Q0 * e[0] + m.Q1 * e[1], a1 = m.Q1 * e[0] + m.Q0 * e[1], a2 = m.Q5 * e[2];
+  T b0 = m.Q0 * e[3] + m.Q1 * e[4], b1 = m.Q1 * e[3] + m.Q0 * e[4], b2 = m.Q5 * e[5];
+  s[0] = t * a0;
+  s[1] = t * a1;
+  s[2] = t * a2;
+  s[3] = I * b0;
+  s[4] = I * b1;
+  s[5] = I * b2;
+  if (m.toff != T(0)) {
+    const T Bc = -m.toff * t * t, Dc = m.toff * m.toff * t * t * t;
+    s[0] += Bc * b0;
+    s[1] += Bc * b1;
+    s[2] += Bc * b2;
+    s[3] += Dc * b0 + Bc * a0;
+    s[4] += Dc * b1 + Bc * a1;
+    s[5] += Dc * b2 + Bc * a2;
+  }
   s[6] = As * e[6];
   s[7] = As * e[7];
   s[8] = T(0.5) * m.drill_reg * (As + As) * e[8];
@@ -274,7 +297,7 @@ __device__ __forceinline__ T sh_vm2(const T s[3], T ys, T sens[3]) {
 template <typename T>
 __device__ __forceinline__ T sh_failure(const ShellMat<T>& m, T t, const T e[9], T de[9], T& dt) {
   const T ksw = T(100);
-  T ht = T(0.5) * t, hb = T(-0.5) * t;
+  T ht = (T(0.5) - m.toff) * t, hb = -(T(0.5) + m.toff) * t;
   T et[3], eb[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
@@ -296,8 +319,8 @@ __device__ __forceinline__ T sh_failure(const ShellMat<T>& m, T t, const T e[9],
     de[3 + k] = ft * ht * phit[k] + fb * hb * phib[k];
   }
   de[6] = de[7] = de[8] = T(0);
-  dt = ft * T(0.5) * (phit[0] * e[3] + phit[1] * e[4] + phit[2] * e[5]) -
-       fb * T(0.5) * (phib[0] * e[3] + phib[1] * e[4] + phib[2] * e[5]);
+  dt = ft * (T(0.5) - m.toff) * (phit[0] * e[3] + phit[1] * e[4] + phit[2] * e[5]) -
+       fb * (T(0.5) + m.toff) * (phib[0] * e[3] + phib[1] * e[4] + phib[2] * e[5]);
   return mx + log(sum) / ksw;
 }
 
@@ -327,6 +350,7 @@ struct ShellParams {
   double mass, rho;  // coefficient of the consistent mass (matrix mode), density
   double axis[3];    // reference axis of TACSShellRefAxisTransform (ref_axis = 1)
   int ref_axis;
+  double toff;
 };
 
 template <typename T, int MODE>
@@ -343,7 +367,8 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
   const int dir = XPT ? gt % 12 : -1;
   const bool active = el < nelem;
   ShellLDS<T>& L = lds_all[team];
-  const ShellMat<T> M = {T(P.Q0), T(P.Q1), T(P.Q5), T(P.kcorr), T(P.drill_reg), T(P.ys)};
+  const ShellMat<T> M = {T(P.Q0), T(P.Q1), T(P.Q5), T(P.kcorr), T(P.drill_reg), T(P.ys),
+                         T(P.toff)};
   const T t = active ? T(telem ? telem[el] : 1.0) : T(1);
 
   // ---- load the element's nodes and variables
@@ -587,11 +612,12 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
       }
       // consistent mass (TACSShellElement::addJacobian, TACSShellElement.h:
       // 605-617, and TACSLinearizedRotation::addDirectorJacobian,
-      // TACSDirector.h:381-429) with tOffset = 0: moments m0 = rho t,
-      // m1 = 0, m2 = rho t^3 / 12 (TACSIsoShellConstitutive.cpp:120-129).
+      // TACSDirector.h:381-467): moments m0 = rho t, m1 = -rho t^2 toff,
+      // m2 = rho t^3 (toff^2 + 1/12) (TACSIsoShellConstitutive.cpp:120-129).
       // Node pair (b row, a column), cN = sum_q detw_q N_b N_a:
       //   u-u block  m0 cN I,
-      //   q-q block  -m2 cN fn_b^x fn_a^x = m2 cN ((fn_b . fn_a) I - fn_a fn_b^T).
+      //   q-q block  -m2 cN fn_b^x fn_a^x = m2 cN ((fn_b . fn_a) I - fn_a fn_b^T),
+      //   q-u block  m1 cN fn_b^x, u-q block its transpose.
       const int ca = lane / 6, ck = lane % 6;
       T cN[4] = {T(0), T(0), T(0), T(0)};
       const bool with_mass = P.mass != 0.0;
@@ -609,7 +635,8 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
           }
         }
       }
-      const T m0 = T(P.rho) * t, m2 = T(P.rho) * t * t * t / T(12);
+      const T m0 = T(P.rho) * t, m1 = -T(P.rho) * t * t * M.toff,
+              m2 = T(P.rho) * t * t * t * (M.toff * M.toff + T(1) / T(12));
       T racc = T(0);
       T* kout = &ke[(size_t)el * SH_NV * SH_NV + lane];
       for (int r = 0; r < SH_NV; r++) {
@@ -629,6 +656,13 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
             const T* fa = L.fn[ca];
             T dot = fb[0] * fa[0] + fb[1] * fa[1] + fb[2] * fa[2];
             mv = m2 * cN[rb] * ((rk == ck ? dot : T(0)) - fa[rk - 3] * fb[ck - 3]);
+          } else if (rk >= 3) {
+            // rotation row, displacement column: m1 cN (fn_b^x)[rk-3][ck]
+            // (the d2Tdotu terms of addDirectorJacobian, TACSDirector.h:431-467)
+            mv = m1 * cN[rb] * sh_skew(L.fn[rb], rk - 3, ck);
+          } else {
+            // displacement row, rotation column: the transposed entry
+            mv = m1 * cN[rb] * sh_skew(L.fn[ca], ck - 3, rk);
           }
           kv += T(P.mass) * mv;
         }
@@ -745,13 +779,16 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
         v = T(P.scale) * exp(T(P.ks_weight) * (f - T(P.ks_max))) / T(P.ks_sum) * L.detw[q] *
             T(P.safety) * dt;
     } else if (MODE == SH_ADJ_PRODUCT || MODE == SH_COMPLIANCE_DV) {
-      // TACSIsoShellConstitutive::addStressDVSens (:229-251), tOffset = 0
+      // TACSIsoShellConstitutive::addStressDVSens (:229-251): d/dt of A, B, D
       const T* x = (MODE == SH_ADJ_PRODUCT) ? L.ep[q] : e;
       T a0 = M.Q0 * x[0] + M.Q1 * x[1], a1 = M.Q1 * x[0] + M.Q0 * x[1], a2 = M.Q5 * x[2];
       T b0 = M.Q0 * x[3] + M.Q1 * x[4], b1 = M.Q1 * x[3] + M.Q0 * x[4], b2 = M.Q5 * x[5];
-      T dI = T(0.25) * t * t;
+      T dI = (T(3) * M.toff * M.toff + T(0.25)) * t * t;
       v = a0 * e[0] + a1 * e[1] + a2 * e[2] + dI * (b0 * e[3] + b1 * e[4] + b2 * e[5]) +
           M.kcorr * M.Q5 * (x[6] * e[6] + x[7] * e[7] + M.drill_reg * x[8] * e[8]);
+      if (M.toff != T(0))
+        v += T(-2) * M.toff * t *
+             (a0 * e[3] + a1 * e[4] + a2 * e[5] + b0 * e[0] + b1 * e[1] + b2 * e[2]);
       v *= T(P.scale) * L.detw[q];
     }
     L.val[q] = v;
@@ -941,6 +978,7 @@ static ShellParams shell_params(const ShellMaterial* m, double scale, double ks_
   P.mass = 0.0;
   P.rho = m->rho;
   P.ref_axis = m->ref_axis;
+  P.toff = m->toff;
   for (int k = 0; k < 3; k++) P.axis[k] = m->axis[k];
   return P;
 }

@@ -107,11 +107,12 @@ class ShellProblem:
     perimeter).  dv: None, "shared" or "element" as for SolidProblem.
     warp: amplitude of a seeded random z perturbation of the interior nodes
     (0 keeps the plate flat).  axis: reference axis of a
-    TACSShellRefAxisTransform (None: the natural transform).
+    TACSShellRefAxisTransform (None: the natural transform).  toff: the
+    constitutive model's thickness offset (tOffset).
     """
 
     def __init__(self, n, L=1.0, E=70e9, nu=0.3, rho=2700.0, ys=270e6, t0=0.01, dv=None,
-                 pressure=1e4, bc="ss", warp=0.0, seed=0, axis=None):
+                 pressure=1e4, bc="ss", warp=0.0, seed=0, axis=None, toff=0.0):
         from .meshes import PlateMesh
         self.mesh = m = PlateMesh(n, n, L, L)
         if warp:
@@ -126,11 +127,13 @@ class ShellProblem:
         a.setElementConnectivity(ptr, m.conn.reshape(-1))
         if dv == "element":
             elems = [elements.Quad4Shell(self.transform,
-                                         constitutive.IsoShellConstitutive(self.props, t=t0, tNum=e))
+                                         constitutive.IsoShellConstitutive(self.props, t=t0, tNum=e,
+                                                                           tOffset=toff))
                      for e in range(m.nelem)]
         else:
             con = constitutive.IsoShellConstitutive(self.props, t=t0,
-                                                    tNum=0 if dv == "shared" else -1)
+                                                    tNum=0 if dv == "shared" else -1,
+                                                    tOffset=toff)
             elems = [elements.Quad4Shell(self.transform, con)] * m.nelem
         self.elems = elems
         a.setElements(elems)

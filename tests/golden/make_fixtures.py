@@ -153,6 +153,20 @@ def make(seed=2024):
         L.ref_quad4_shell_axis(_d(X), _d(v), c(70e9), c(0.3), c(270e6), c(t), _d(axis), _d(res),
                                _d(mat), _d(fail))
         out[key + "_axis_res"], out[key + "_axis_mat"], out[key + "_axis_fail"] = res, mat, fail
+    # Quad4 shells with a thickness offset (TACSIsoShellConstitutive tOffset)
+    for s in range(4):
+        key = f"quad4_{s}"
+        X, v, psi = out[key + "_X"], out[key + "_v"], out[key + "_psi"]
+        t = float(out[key + "_t"][0])
+        toff = (-0.3, 0.5, 0.15, -0.5)[s]
+        res, mat, mass = np.zeros(24), np.zeros(576), np.zeros(576)
+        adj, fail, dfail = np.zeros(1), np.zeros(4), np.zeros(4)
+        L.ref_quad4_shell_offset(_d(X), _d(v), _d(psi), c(70e9), c(0.3), c(270e6), c(t),
+                                 c(toff), _d(res), _d(mat), _d(mass), _d(adj), _d(fail),
+                                 _d(dfail))
+        out[key + "_toff"] = np.array([toff])
+        out[key + "_off_res"], out[key + "_off_mat"], out[key + "_off_mass"] = res, mat, mass
+        out[key + "_off_adj"], out[key + "_off_fail"], out[key + "_off_dfail"] = adj, fail, dfail
     # Quad4 shell node-coordinate sensitivities (complex build, child process)
     with tempfile.TemporaryDirectory() as tmp:
         src = os.path.join(tmp, "in.npz")

@@ -269,3 +269,60 @@ def test_shell_ref_axis_transform_matches_oracle(axis):
     assert abs(vals[0] - oc) <= RTOL * abs(oc) and abs(vals[1] - oks) <= RTOL * abs(oks)
     # a different frame from the natural transform on this warped plate
     assert np.abs(A.reshape(-1) - nA).max() >= 1e-6 * np.abs(nA).max()
+
+
+@pytest.mark.parametrize("toff", [0.3, -0.5])
+def test_shell_thickness_offset_matches_oracle(toff):
+    """TACSIsoShellConstitutive with tOffset != 0: Jacobian, residual, mass
+    matrix, compliance and KS failure with their state, thickness and node
+    sensitivities, and the thickness adjoint product against the oracle
+    (pinned to the reference build with offsets -0.5 .. 0.5)."""
+    p = ShellProblem(5, warp=0.02, seed=8, dv="element", ys=YS, toff=toff)
+    a, m = p.assembler, p.mesh
+    rng = np.random.default_rng(12)
+    t = rng.uniform(0.005, 0.02, m.nelem)
+    p.set_thickness(t)
+    u = rng.normal(0.0, 1e-3, 6 * m.nnodes)
+    psi = rng.standard_normal(6 * m.nnodes)
+    uv, pv = a.createVec(), a.createVec()
+    uv.getArray()[:] = u
+    pv.getArray()[:] = psi
+    a.setVariables(uv)
+    X, conn = m.X.reshape(-1), m.conn
+    try:
+        O.shell_set_offset(toff)
+        a.assembleJacobian(1.0, 0.0, 0.0, p.res, p.mat)
+        rowp, cols, oA, ores = _oracle(p, t, u)
+        _, _, A = p.mat.getArrays()
+        assert np.abs(A.reshape(-1) - oA).max() <= RTOL * np.abs(oA).max()
+        assert np.abs(p.res.getArray() - ores).max() <= RTOL * np.abs(ores).max()
+        a.assembleMatType(TACS.MASS_MATRIX, p.mat)
+        oM = O.assemble_shell_mat(conn, X, t, E, NU, 2700.0, 0.0, 1.0, rowp, cols)
+        O.apply_bc_mat(6, p.bc_nodes, _mask(p), rowp, cols, oM)
+        _blockwise_close(p.mat.getArrays()[2], oM, "mass")
+        comp = functions.Compliance(a)
+        ksf = functions.KSFailure(a, ksWeight=50.0)
+        vals = a.evalFunctions([comp, ksf])
+        oc = O.shell_compliance(conn, X, u, t, E, NU)
+        oks = O.shell_ks_failure(conn, X, u, t, E, NU, YS, 50.0)
+        assert abs(vals[0] - oc) <= RTOL * abs(oc) and abs(vals[1] - oks[0]) <= RTOL * abs(oks[0])
+        dfdu = [a.createVec(), a.createVec()]
+        a.addSVSens([comp, ksf], dfdu)
+        dcdu_o, dcdt_o = O.shell_compliance_sens(conn, X, u, t, E, NU)
+        dkdu_o, dkdt_o = O.shell_ks_sens(conn, X, u, t, E, NU, YS, 50.0, oks[1], oks[2])
+        for g, o in ((dfdu[0], dcdu_o), (dfdu[1], dkdu_o)):
+            assert np.abs(g.getArray() - o).max() <= RTOL * np.abs(o).max()
+        dfdx = [a.createDesignVec(), a.createDesignVec()]
+        a.addDVSens([comp, ksf], dfdx, 1.0)
+        for g, o in ((dfdx[0], dcdt_o), (dfdx[1], dkdt_o)):
+            assert np.abs(g.getArray() - o).max() <= RTOL * np.abs(o).max()
+        prod = a.createDesignVec()
+        a.addAdjointResProducts([pv], [prod], -1.0)
+        oprod = O.shell_adj_res_product(conn, X, u, psi, t, E, NU, -1.0)
+        assert np.abs(prod.getArray() - oprod).max() <= RTOL * np.abs(oprod).max()
+        dX = a.createNodeVec()
+        a.addXptSens([comp], [dX])
+        oX = O.shell_xpt_sens(1, conn, X, u, t, E, NU)
+        assert np.abs(dX.getArray() - oX).max() <= RTOL * np.abs(oX).max()
+    finally:
+        O.shell_set_offset(0.0)

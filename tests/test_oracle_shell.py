@@ -323,3 +323,39 @@ def test_quad4_ref_axis_transform_against_reference(fixtures):
             assert np.abs(mat3 - M).max() >= 1e-6 * np.abs(M).max()
     finally:
         O.shell_set_ref_axis(None)
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_quad4_thickness_offset_against_reference(fixtures, case):
+    """TACSIsoShellConstitutive with tOffset != 0 (membrane-bending coupling,
+    shifted top and bottom faces, mass moment m1): residual, Jacobian, mass
+    matrix, thickness adjoint product, failure index and its thickness
+    derivative against the reference build."""
+    k = f"quad4_{case}"
+    X, v, psi = fixtures[k + "_X"], fixtures[k + "_v"], fixtures[k + "_psi"]
+    t = float(fixtures[k + "_t"].reshape(-1)[0])
+    toff = float(fixtures[k + "_toff"][0])
+    try:
+        O.shell_set_offset(toff)
+        res, mat = O.quad4_element(X, v, E, NU, t)
+        R, K = fixtures[k + "_off_res"], fixtures[k + "_off_mat"].reshape(24, 24)
+        assert np.abs(res - R).max() <= 1e-14 * np.abs(R).max()
+        assert np.abs(mat - K).max() <= 1e-14 * np.abs(K).max()
+        M = O.quad4_mass(X, 2700.0, t)
+        Mr = fixtures[k + "_off_mass"].reshape(24, 24)
+        assert np.abs(M - Mr).max() <= 1e-14 * np.abs(Mr).max()
+        adj = O.quad4_adj_product(X, v, psi, E, NU, t)
+        ref_adj = float(fixtures[k + "_off_adj"][0])
+        assert abs(adj - ref_adj) <= 1e-13 * abs(ref_adj)
+        e, _ = O.quad4_strains(X, v, drill=False)
+        for n in range(4):
+            f, _, dfdt = O.shell_failure(E, NU, YS, t, e[n])
+            assert abs(f - fixtures[k + "_off_fail"][n]) <= 1e-13 * abs(f)
+            ref_dt = fixtures[k + "_off_dfail"][n]
+            assert abs(dfdt - ref_dt) <= 1e-12 * abs(ref_dt)
+        # the offset couples membrane and bending: K differs from tOffset = 0
+        O.shell_set_offset(0.0)
+        _, K0 = O.quad4_element(X, v, E, NU, t)
+        assert np.abs(K0 - K).max() >= 1e-4 * np.abs(K).max()
+    finally:
+        O.shell_set_offset(0.0)
