@@ -819,6 +819,8 @@ class TACSAssembler : public TACSObject {
   int nbc_nodes;
   int *d_dvnum, *d_dv2e_ptr, *d_dv2e;
   double *d_tfixed, *d_telem;
+  void* d_mtab;  // per-element material table (when elements differ), or NULL
+  int* d_emat;
   double *d_X, *d_u;
   double *d_ev, *d_es;
   int *d_aux_elem, *d_aux_face, *d_n2a_ptr, *d_n2a;

@@ -53,7 +53,11 @@ typedef struct SolidMaterial {
    mass moments rho t, -rho t^2 toff, rho t^3 (toff^2 + 1/12)).  The element
    frame: ref_axis = 0 for TACSShellNaturalTransform, 1 for
    TACSShellRefAxisTransform with the unit axis
-   (src/elements/shell/TACSShellElementTransform.h:95). */
+   (src/elements/shell/TACSShellElementTransform.h:95).  Per-element
+   materials: when elem_mat != NULL (device arrays), element e uses
+   table[elem_mat[e]] (Q, kcorr, drill_reg, ys, rho, toff; the frame stays
+   this struct's), as TACSAssembler::setElements allows one constitutive
+   object per element (src/TACSAssembler.cpp:623). */
 typedef struct ShellMaterial {
   double Q[6];
   double kcorr;
@@ -63,6 +67,8 @@ typedef struct ShellMaterial {
   double axis[3];
   int ref_axis;
   double toff;
+  const struct ShellMaterial* table;
+  const int* elem_mat;
 } ShellMaterial;
 
 /* ------------------------------------------------------------------------

@@ -2711,6 +2711,8 @@ TACSAssembler::TACSAssembler(TACSComm _comm, int vpn, int nowned, int nelems, in
   nbc_nodes = 0;
   d_dvnum = d_dv2e_ptr = d_dv2e = NULL;
   d_tfixed = d_telem = d_X = d_u = d_ev = d_es = NULL;
+  d_mtab = NULL;
+  d_emat = NULL;
   d_aux_elem = d_aux_face = d_n2a_ptr = d_n2a = NULL;
   d_aux_trac = d_aux_ev = NULL;
   d_status = NULL;
@@ -2739,7 +2741,7 @@ TACSAssembler::~TACSAssembler() {
   void* ds[] = {d_conn, d_n2e_ptr, d_n2e, d_bcvars, d_bcnodes, d_bcmask, d_bcvals, d_dvnum,
                 d_dv2e_ptr, d_dv2e, d_tfixed, d_telem, d_X, d_u, d_ev, d_es, d_aux_elem,
                 d_aux_face, d_n2a_ptr, d_n2a, d_aux_trac, d_aux_ev, d_status, d_red, d_scalar,
-                d_ke, d_aux_pres, d_qgeo, d_qstress, d_keb};
+                d_ke, d_aux_pres, d_qgeo, d_qstress, d_keb, d_mtab, d_emat};
   for (void* p : ds) dfree(p);
 }
 
@@ -2863,6 +2865,8 @@ int TACSAssembler::initialize() {
   if (!elements) return tacs_set_error("initialize: elements not set");
   std::vector<int> dvnum(numElements);
   std::vector<double> tfixed(numElements);
+  std::vector<int> matIndex(numElements, 0);  // per-element materials
+  std::vector<ShellMaterial> shellTab;
   TACSMaterialProperties* props0 = NULL;
   if (numElements > 0 && dynamic_cast<TACSQuad4Shell*>(elements[0])) {
     // Shell family: TACSQuad4Shell + TACSShellNaturalTransform or
@@ -2870,7 +2874,6 @@ int TACSAssembler::initialize() {
     // kcorr and tOffset.
     family = FAMILY_SHELL;
     if (varsPerNode != 6) return tacs_set_error("initialize: shells need 6 variables per node");
-    double kcorr = 0.0, toff = 0.0;
     TACSShellTransform* tr0 = NULL;
     for (int e = 0; e < numElements; e++) {
       TACSQuad4Shell* el = dynamic_cast<TACSQuad4Shell*>(elements[e]);
@@ -2892,36 +2895,38 @@ int TACSAssembler::initialize() {
       TACSIsoShellConstitutive* con = dynamic_cast<TACSIsoShellConstitutive*>(el->con);
       if (!con) return tacs_set_error("initialize: element %d constitutive is not isotropic", e);
       TACSMaterialProperties* p = con->props;
-      if (!props0) {
-        props0 = p;
-        kcorr = con->kcorr;
-        toff = con->tOffset;
+      if (!props0) props0 = p;
+      // the element's material: plane-stress Q (TACSMaterialProperties::
+      // evalTangentStiffness2D :323), kcorr, drilling regularization
+      // (TACSShellConstitutive::DRILLING_REGULARIZATION), ys, rho, tOffset
+      ShellMaterial me = {};
+      const double D = p->E / (1.0 - p->nu * p->nu);
+      me.Q[0] = D;
+      me.Q[1] = p->nu * D;
+      me.Q[3] = D;
+      me.Q[5] = p->G;
+      me.kcorr = con->kcorr;
+      me.drill_reg = 0.1;
+      me.ys = p->ys;
+      me.rho = p->rho;
+      me.toff = con->tOffset;
+      size_t k = 0;
+      for (; k < shellTab.size(); k++) {
+        const ShellMaterial& o = shellTab[k];
+        if (o.Q[0] == me.Q[0] && o.Q[1] == me.Q[1] && o.Q[5] == me.Q[5] && o.kcorr == me.kcorr &&
+            o.ys == me.ys && o.rho == me.rho && o.toff == me.toff)
+          break;
       }
-      if (con->tOffset != toff) return tacs_set_error("initialize: elements must share tOffset");
-      if (p != props0 && (p->E != props0->E || p->nu != props0->nu || p->rho != props0->rho ||
-                          p->ys != props0->ys))
-        return tacs_set_error("initialize: elements must share one material");
-      if (con->kcorr != kcorr) return tacs_set_error("initialize: elements must share kcorr");
+      if (k == shellTab.size()) shellTab.push_back(me);
+      matIndex[e] = (int)k;
       dvnum[e] = con->tNum;
       tfixed[e] = con->t;
     }
     order = 2;
     npe = 4;
-    // plane-stress Q (TACSMaterialProperties::evalTangentStiffness2D :323)
-    double D = props0->E / (1.0 - props0->nu * props0->nu);
-    smat.Q[0] = D;
-    smat.Q[1] = props0->nu * D;
-    smat.Q[2] = 0.0;
-    smat.Q[3] = D;
-    smat.Q[4] = 0.0;
-    smat.Q[5] = props0->G;
-    smat.kcorr = kcorr;
-    smat.drill_reg = 0.1;  // TACSShellConstitutive::DRILLING_REGULARIZATION
-    smat.ys = props0->ys;
-    smat.rho = props0->rho;
+    smat = shellTab[0];
     TACSShellRefAxisTransform* ra0 = dynamic_cast<TACSShellRefAxisTransform*>(tr0);
     smat.ref_axis = ra0 ? 1 : 0;
-    smat.toff = toff;
     for (int k = 0; k < 3; k++) smat.axis[k] = ra0 ? ra0->axis[k] : 0.0;
   } else {
     // Solid family: TACSElement3D + TACSLinearElasticity3D +
@@ -3073,6 +3078,15 @@ int TACSAssembler::initialize() {
   h2d(d_dvnum, dvnum.data(), numElements);
   h2d(d_tfixed, tfixed.data(), numElements);
   h2d(d_telem, tfixed.data(), numElements);
+  if (family == FAMILY_SHELL && shellTab.size() > 1) {
+    ShellMaterial* tab = dalloc<ShellMaterial>(shellTab.size());
+    d_emat = dalloc<int>(numElements);
+    h2d(tab, shellTab.data(), shellTab.size());
+    h2d(d_emat, matIndex.data(), numElements);
+    d_mtab = tab;
+    smat.table = tab;
+    smat.elem_mat = d_emat;
+  }
   d_dv2e_ptr = dalloc<int>(numDesignVars + 1);
   d_dv2e = dalloc<int>(dv2e.size());
   h2d(d_dv2e_ptr, dv2e_ptr.data(), numDesignVars + 1);

@@ -351,6 +351,8 @@ struct ShellParams {
   double axis[3];    // reference axis of TACSShellRefAxisTransform (ref_axis = 1)
   int ref_axis;
   double toff;
+  const ShellMaterial* mtab;  // per-element materials (device), or NULL
+  const int* emat;
 };
 
 template <typename T, int MODE>
@@ -367,8 +369,14 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
   const int dir = XPT ? gt % 12 : -1;
   const bool active = el < nelem;
   ShellLDS<T>& L = lds_all[team];
-  const ShellMat<T> M = {T(P.Q0), T(P.Q1), T(P.Q5), T(P.kcorr), T(P.drill_reg), T(P.ys),
-                         T(P.toff)};
+  ShellMat<T> M = {T(P.Q0), T(P.Q1), T(P.Q5), T(P.kcorr), T(P.drill_reg), T(P.ys), T(P.toff)};
+  double rho = P.rho;
+  if (P.emat && active) {
+    // the element's own material (per-element constitutive objects)
+    const ShellMaterial& me = P.mtab[P.emat[el]];
+    M = {T(me.Q[0]), T(me.Q[1]), T(me.Q[5]), T(me.kcorr), T(me.drill_reg), T(me.ys), T(me.toff)};
+    rho = me.rho;
+  }
   const T t = active ? T(telem ? telem[el] : 1.0) : T(1);
 
   // ---- load the element's nodes and variables
@@ -635,8 +643,8 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
           }
         }
       }
-      const T m0 = T(P.rho) * t, m1 = -T(P.rho) * t * t * M.toff,
-              m2 = T(P.rho) * t * t * t * (M.toff * M.toff + T(1) / T(12));
+      const T m0 = T(rho) * t, m1 = -T(rho) * t * t * M.toff,
+              m2 = T(rho) * t * t * t * (M.toff * M.toff + T(1) / T(12));
       T racc = T(0);
       T* kout = &ke[(size_t)el * SH_NV * SH_NV + lane];
       for (int r = 0; r < SH_NV; r++) {
@@ -979,6 +987,8 @@ static ShellParams shell_params(const ShellMaterial* m, double scale, double ks_
   P.rho = m->rho;
   P.ref_axis = m->ref_axis;
   P.toff = m->toff;
+  P.mtab = m->table;
+  P.emat = m->elem_mat;
   for (int k = 0; k < 3; k++) P.axis[k] = m->axis[k];
   return P;
 }

@@ -108,11 +108,13 @@ class ShellProblem:
     warp: amplitude of a seeded random z perturbation of the interior nodes
     (0 keeps the plate flat).  axis: reference axis of a
     TACSShellRefAxisTransform (None: the natural transform).  toff: the
-    constitutive model's thickness offset (tOffset).
+    constitutive model's thickness offset (tOffset).  materials: a list of
+    dicts (E, nu, rho, ys, toff); element e takes materials[e % len] (one
+    constitutive object per element, dv as above).
     """
 
     def __init__(self, n, L=1.0, E=70e9, nu=0.3, rho=2700.0, ys=270e6, t0=0.01, dv=None,
-                 pressure=1e4, bc="ss", warp=0.0, seed=0, axis=None, toff=0.0):
+                 pressure=1e4, bc="ss", warp=0.0, seed=0, axis=None, toff=0.0, materials=None):
         from .meshes import PlateMesh
         self.mesh = m = PlateMesh(n, n, L, L)
         if warp:
@@ -125,7 +127,18 @@ class ShellProblem:
         a = self.assembler = TACS.Assembler.create(None, 6, m.nnodes, m.nelem)
         ptr = np.arange(0, 4 * m.nelem + 1, 4, dtype=np.intc)
         a.setElementConnectivity(ptr, m.conn.reshape(-1))
-        if dv == "element":
+        if materials is not None:
+            self.mat_props = [(constitutive.MaterialProperties(rho=d["rho"], E=d["E"], nu=d["nu"],
+                                                               ys=d["ys"]), d.get("toff", 0.0))
+                              for d in materials]
+            elems = []
+            for e in range(m.nelem):
+                pm, to = self.mat_props[e % len(materials)]
+                tnum = e if dv == "element" else (0 if dv == "shared" else -1)
+                elems.append(elements.Quad4Shell(
+                    self.transform, constitutive.IsoShellConstitutive(pm, t=t0, tNum=tnum,
+                                                                      tOffset=to)))
+        elif dv == "element":
             elems = [elements.Quad4Shell(self.transform,
                                          constitutive.IsoShellConstitutive(self.props, t=t0, tNum=e,
                                                                            tOffset=toff))

@@ -326,3 +326,77 @@ def test_shell_thickness_offset_matches_oracle(toff):
         assert np.abs(dX.getArray() - oX).max() <= RTOL * np.abs(oX).max()
     finally:
         O.shell_set_offset(0.0)
+
+
+MATS = [dict(E=70e9, nu=0.3, rho=2700.0, ys=270e6, toff=0.0),
+        dict(E=200e9, nu=0.27, rho=7800.0, ys=400e6, toff=0.2),
+        dict(E=110e9, nu=0.34, rho=4500.0, ys=800e6, toff=-0.1)]
+
+
+def test_shell_per_element_materials_match_oracle():
+    """One constitutive object per element with three materials (E, nu, rho,
+    ys, tOffset differ): Jacobian, residual, mass, compliance, KS failure and
+    the thickness sensitivities against the oracle summed over the material
+    groups (element assembly is additive; the KS aggregate combines the
+    groups' maxima and sums)."""
+    p = ShellProblem(5, warp=0.02, seed=9, dv="element", materials=MATS)
+    a, m = p.assembler, p.mesh
+    rng = np.random.default_rng(13)
+    t = rng.uniform(0.005, 0.02, m.nelem)
+    p.set_thickness(t)
+    u = rng.normal(0.0, 1e-3, 6 * m.nnodes)
+    uv = a.createVec()
+    uv.getArray()[:] = u
+    a.setVariables(uv)
+    a.assembleJacobian(1.0, 0.0, 0.0, p.res, p.mat)
+    _, _, A = p.mat.getArrays()
+    res = p.res.getArray().copy()
+    comp = functions.Compliance(a)
+    ksf = functions.KSFailure(a, ksWeight=40.0)
+    vals = a.evalFunctions([comp, ksf])
+    dfdx = [a.createDesignVec(), a.createDesignVec()]
+    a.addDVSens([comp, ksf], dfdx, 1.0)
+    a.assembleMatType(TACS.MASS_MATRIX, p.mat)
+    _, _, Mg = p.mat.getArrays()
+    X, conn = m.X.reshape(-1), m.conn
+    rowp, cols = O.node_csr(m.nnodes, conn, 4)
+    oA = np.zeros(36 * cols.size)
+    oM = np.zeros(36 * cols.size)
+    ores = np.zeros(6 * m.nnodes)
+    oc, ks_parts = 0.0, []
+    groups = [np.arange(g, m.nelem, len(MATS)) for g in range(len(MATS))]
+    try:
+        for g, idx in enumerate(groups):
+            d = MATS[g]
+            O.shell_set_offset(d["toff"])
+            Ag, rg = O.assemble_shell(conn[idx], X, u, t[idx], d["E"], d["nu"], rowp, cols)
+            oA += Ag
+            ores += rg
+            oM += O.assemble_shell_mat(conn[idx], X, t[idx], d["E"], d["nu"], d["rho"], 0.0, 1.0,
+                                       rowp, cols)
+            oc += O.shell_compliance(conn[idx], X, u, t[idx], d["E"], d["nu"])
+            ks_parts.append(O.shell_ks_failure(conn[idx], X, u, t[idx], d["E"], d["nu"], d["ys"],
+                                               40.0))
+        fmax = max(k[1] for k in ks_parts)
+        ksum = sum(k[2] * np.exp(40.0 * (k[1] - fmax)) for k in ks_parts)
+        oks = fmax + np.log(ksum) / 40.0
+        dct = np.zeros(m.nelem)
+        dkt = np.zeros(m.nelem)
+        for g, idx in enumerate(groups):
+            d = MATS[g]
+            O.shell_set_offset(d["toff"])
+            dct[idx] = O.shell_compliance_sens(conn[idx], X, u, t[idx], d["E"], d["nu"])[1]
+            dkt[idx] = O.shell_ks_sens(conn[idx], X, u, t[idx], d["E"], d["nu"], d["ys"], 40.0,
+                                       fmax, ksum)[1]
+    finally:
+        O.shell_set_offset(0.0)
+    O.shell_add_pressure(conn, X, p.aux_elems, np.full(p.aux_elems.size, p.pressure), ores)
+    O.apply_bc_mat(6, p.bc_nodes, _mask(p), rowp, cols, oA)
+    O.apply_bc_vec(6, p.bc_nodes, _mask(p), ores, u=u)
+    O.apply_bc_mat(6, p.bc_nodes, _mask(p), rowp, cols, oM)
+    assert np.abs(A.reshape(-1) - oA).max() <= RTOL * np.abs(oA).max()
+    assert np.abs(res - ores).max() <= RTOL * np.abs(ores).max()
+    _blockwise_close(Mg, oM, "mass")
+    assert abs(vals[0] - oc) <= RTOL * abs(oc) and abs(vals[1] - oks) <= RTOL * abs(oks)
+    assert np.abs(dfdx[0].getArray() - dct).max() <= RTOL * np.abs(dct).max()
+    assert np.abs(dfdx[1].getArray() - dkt).max() <= RTOL * np.abs(dkt).max()
