@@ -34,13 +34,18 @@ extern "C" {
 /* Packed solid material: the reference's C[21] tangent stiffness
    (TACSMaterialProperties::evalTangentStiffness3D
    src/constitutive/TACSMaterialProperties.cpp:270), density and yield
-   stress, plus the Lame constants used when the material is isotropic. */
+   stress, plus the Lame constants used when the material is isotropic.
+   Per-element isotropic materials (TACSAssembler::setElements with one
+   constitutive object per element, src/TACSAssembler.cpp:623): when
+   elem_props != NULL (device array, 4 per element: lam, mu, rho, ys),
+   element e uses those constants instead of lam, mu, rho, ys. */
 typedef struct SolidMaterial {
   double C[21];
   double rho;
   double ys;
   int isotropic;
   double lam, mu;
+  const double* elem_props;
 } SolidMaterial;
 
 /* Isotropic shell material of TACSIsoShellConstitutive

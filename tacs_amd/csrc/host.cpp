@@ -2867,6 +2867,8 @@ int TACSAssembler::initialize() {
   std::vector<double> tfixed(numElements);
   std::vector<int> matIndex(numElements, 0);  // per-element materials
   std::vector<ShellMaterial> shellTab;
+  std::vector<double> solidProps;
+  bool solidMixed = false;
   TACSMaterialProperties* props0 = NULL;
   if (numElements > 0 && dynamic_cast<TACSQuad4Shell*>(elements[0])) {
     // Shell family: TACSQuad4Shell + TACSShellNaturalTransform or
@@ -2933,6 +2935,7 @@ int TACSAssembler::initialize() {
     // TACSSolidConstitutive on one hexahedral basis and one material.
     family = FAMILY_SOLID;
     if (varsPerNode != 3) return tacs_set_error("initialize: solids need 3 variables per node");
+    solidProps.resize(4 * (size_t)numElements);
     for (int e = 0; e < numElements; e++) {
       TACSElement3D* el = dynamic_cast<TACSElement3D*>(elements[e]);
       if (!el) return tacs_set_error("initialize: element %d is not a TACSElement3D", e);
@@ -2946,9 +2949,16 @@ int TACSAssembler::initialize() {
       if (o != order) return tacs_set_error("initialize: mixed element bases are not supported");
       TACSMaterialProperties* p = model->stiff->props;
       if (!props0) props0 = p;
+      // the element's isotropic constants (lam, mu, rho, ys)
+      double C[21];
+      p->evalTangentStiffness3D(C);
+      solidProps[4 * (size_t)e] = C[1];
+      solidProps[4 * (size_t)e + 1] = p->G;
+      solidProps[4 * (size_t)e + 2] = p->rho;
+      solidProps[4 * (size_t)e + 3] = p->ys;
       if (p != props0 && (p->E != props0->E || p->nu != props0->nu || p->rho != props0->rho ||
                           p->ys != props0->ys))
-        return tacs_set_error("initialize: elements must share one material");
+        solidMixed = true;
       dvnum[e] = model->stiff->tNum;
       tfixed[e] = model->stiff->t;
     }
@@ -3078,6 +3088,12 @@ int TACSAssembler::initialize() {
   h2d(d_dvnum, dvnum.data(), numElements);
   h2d(d_tfixed, tfixed.data(), numElements);
   h2d(d_telem, tfixed.data(), numElements);
+  if (family == FAMILY_SOLID && solidMixed) {
+    double* tab = dalloc<double>(solidProps.size());
+    h2d(tab, solidProps.data(), solidProps.size());
+    d_mtab = tab;
+    mat.elem_props = tab;
+  }
   if (family == FAMILY_SHELL && shellTab.size() > 1) {
     ShellMaterial* tab = dalloc<ShellMaterial>(shellTab.size());
     d_emat = dalloc<int>(numElements);

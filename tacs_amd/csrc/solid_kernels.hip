@@ -120,6 +120,22 @@ __device__ __forceinline__ double c21(const SolidMaterial& m, int i, int j) {
   return m.C[off[i] + (j - i)];
 }
 
+// Element e's material: mat, with (lam, mu, rho, ys) from the per-element
+// table when the elements carry different isotropic materials
+// (TACSAssembler::setElements allows one constitutive object per element,
+// src/TACSAssembler.cpp:623).
+__device__ __forceinline__ SolidMaterial elem_mat(const SolidMaterial& mat, int e) {
+  SolidMaterial m = mat;
+  if (mat.elem_props) {
+    const double* q = mat.elem_props + 4 * (size_t)e;
+    m.lam = q[0];
+    m.mu = q[1];
+    m.rho = q[2];
+    m.ys = q[3];
+  }
+  return m;
+}
+
 // s = C e (no thickness scaling), TACSSolidConstitutive::evalStress :142.
 __device__ __forceinline__ void stress6(const SolidMaterial& m, const double e[6],
                                         double s[6]) {
@@ -208,6 +224,7 @@ __global__ __launch_bounds__(256) void solid_qp_stress(int nelem, const int* __r
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)nelem * NQP) return;
   const int elem = (int)(t / NQP), q = (int)(t - (size_t)elem * NQP);
+  const SolidMaterial me = elem_mat(mat, elem);
   QP1D<ORD> qp;
   qp.eval(q);
   double J[9];
@@ -230,7 +247,7 @@ __global__ __launch_bounds__(256) void solid_qp_stress(int nelem, const int* __r
   }
   const double e[6] = {Ux[0], Ux[4], Ux[8], Ux[5] + Ux[7], Ux[2] + Ux[6], Ux[1] + Ux[3]};
   double s[6];
-  stress6(mat, e, s);
+  stress6(me, e, s);
 #pragma unroll
   for (int k = 0; k < 6; k++) S[t * 6 + k] = s[k];
 }
@@ -319,9 +336,16 @@ __global__ __launch_bounds__(64 * WPB) void solid_jacobian_rows(
         int b = valid ? p - e * NPE : 0;
         double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         int slot = -1;
+        double lam = mat.lam, mu = mat.mu, rho = mat.rho;
         if (valid) {
           int packed = n2e[es + c0 + e];
           int elem = packed >> 5, a = packed & 31;
+          if (mat.elem_props) {
+            const double* mp = mat.elem_props + 4 * (size_t)elem;
+            lam = mp[0];
+            mu = mp[1];
+            rho = mp[2];
+          }
           int cn = conn[(size_t)elem * NPE + b];
           int lo = 0, hi = nc - 1;
           while (lo <= hi) {
@@ -354,8 +378,8 @@ __global__ __launch_bounds__(64 * WPB) void solid_jacobian_rows(
               for (int j = 0; j < 3; j++) {
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
-                  double v = mat.lam * ga[j] * gb[k] + mat.mu * gb[j] * ga[k];
-                  if (j == k) v += mat.mu * gab;
+                  double v = lam * ga[j] * gb[k] + mu * gb[j] * ga[k];
+                  if (j == k) v += mu * gab;
                   acc[3 * j + k] += f * v;
                 }
               }
@@ -376,7 +400,7 @@ __global__ __launch_bounds__(64 * WPB) void solid_jacobian_rows(
               }
             }
             if (gamma != 0.0) {
-              double m = gamma * mat.rho * wdt * Na * Nb;
+              double m = gamma * rho * wdt * Na * Nb;
               acc[0] += m;
               acc[4] += m;
               acc[8] += m;
@@ -438,6 +462,7 @@ __global__ __launch_bounds__(128) void solid_elem_vectors(
   constexpr int NPE = ORD * ORD * ORD, NQP = NPE, NV = 3 * NPE;
   int el = blockIdx.x * blockDim.x + threadIdx.x;
   if (el >= nelem) return;
+  const SolidMaterial me = elem_mat(mat, el);
   const int* c = &conn[(size_t)el * NPE];
   double* o = &out[(size_t)el * NV];
   double r[NV];
@@ -466,7 +491,7 @@ __global__ __launch_bounds__(128) void solid_elem_vectors(
     }
     double e[6] = {Ux[0], Ux[4], Ux[8], Ux[5] + Ux[7], Ux[2] + Ux[6], Ux[1] + Ux[3]};
     double s[6];
-    stress6(mat, e, s);
+    stress6(me, e, s);
     double f;
     double sv[6];
     if (MODE == EV_RESIDUAL) {
@@ -484,11 +509,11 @@ __global__ __launch_bounds__(128) void solid_elem_vectors(
       double vm = sqrt(0.5 * ((s[0] - s[1]) * (s[0] - s[1]) + (s[0] - s[2]) * (s[0] - s[2]) +
                               (s[1] - s[2]) * (s[1] - s[2]) +
                               6.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5])));
-      double fail = safety * vm / mat.ys;
+      double fail = safety * vm / me.ys;
       double dfdq = exp(ks_weight * (fail - ks_max)) / ks_sum * qp.w * det * safety;
       double ds[6] = {0, 0, 0, 0, 0, 0};
       if (vm != 0.0) {
-        double fact = 0.5 / (mat.ys * vm);
+        double fact = 0.5 / (me.ys * vm);
         ds[0] = fact * (2.0 * s[0] - s[1] - s[2]);
         ds[1] = fact * (2.0 * s[1] - s[0] - s[2]);
         ds[2] = fact * (2.0 * s[2] - s[0] - s[1]);
@@ -496,7 +521,7 @@ __global__ __launch_bounds__(128) void solid_elem_vectors(
         ds[4] = 6.0 * fact * s[4];
         ds[5] = 6.0 * fact * s[5];
       }
-      stress6(mat, ds, sv);
+      stress6(me, ds, sv);
       f = coef * dfdq;
     }
 #pragma unroll
@@ -545,6 +570,7 @@ __global__ __launch_bounds__(128) void solid_xpt_vectors(
   constexpr int NPE = ORD * ORD * ORD, NQP = NPE, NV = 3 * NPE;
   int el = blockIdx.x * blockDim.x + threadIdx.x;
   if (el >= nelem) return;
+  const SolidMaterial me = elem_mat(mat, el);
   const int* c = &conn[(size_t)el * NPE];
   double r[NV];
 #pragma unroll
@@ -578,7 +604,7 @@ __global__ __launch_bounds__(128) void solid_xpt_vectors(
     }
     double e[6] = {Ux[0], Ux[4], Ux[8], Ux[5] + Ux[7], Ux[2] + Ux[6], Ux[1] + Ux[3]};
     double s[6];
-    stress6(mat, e, s);
+    stress6(me, e, s);
     double W, M[9];
     if (MODE == XV_COMPLIANCE) {
       W = 0.0;
@@ -602,11 +628,11 @@ __global__ __launch_bounds__(128) void solid_xpt_vectors(
       double vm = sqrt(0.5 * ((s[0] - s[1]) * (s[0] - s[1]) + (s[0] - s[2]) * (s[0] - s[2]) +
                               (s[1] - s[2]) * (s[1] - s[2]) +
                               6.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5])));
-      double fail = safety * vm / mat.ys;
+      double fail = safety * vm / me.ys;
       W = exp(ks_weight * (fail - ks_max)) / (ks_weight * ks_sum);
       double ds[6] = {0, 0, 0, 0, 0, 0}, de[6], S[9];
       if (vm != 0.0) {
-        double fact = 0.5 / (mat.ys * vm);
+        double fact = 0.5 / (me.ys * vm);
         ds[0] = fact * (2.0 * s[0] - s[1] - s[2]);
         ds[1] = fact * (2.0 * s[1] - s[0] - s[2]);
         ds[2] = fact * (2.0 * s[2] - s[0] - s[1]);
@@ -614,7 +640,7 @@ __global__ __launch_bounds__(128) void solid_xpt_vectors(
         ds[4] = 6.0 * fact * s[4];
         ds[5] = 6.0 * fact * s[5];
       }
-      stress6(mat, ds, de);
+      stress6(me, ds, de);
       sig3(de, S);
       const double f = ks_weight * W * safety;
 #pragma unroll
@@ -629,7 +655,7 @@ __global__ __launch_bounds__(128) void solid_xpt_vectors(
     } else {
       double ep[6] = {Px[0], Px[4], Px[8], Px[5] + Px[7], Px[2] + Px[6], Px[1] + Px[3]};
       double sp[6];
-      stress6(mat, ep, sp);
+      stress6(me, ep, sp);
       W = 0.0;
 #pragma unroll
       for (int i = 0; i < 6; i++) {
@@ -846,6 +872,7 @@ __global__ __launch_bounds__(256) void solid_matfree_ev(int nelem, const int* __
   for (int e0 = blockIdx.x * EPB; e0 < nelem; e0 += gridDim.x * EPB) {
     const int elem = e0 + g;
     const bool ok = elem < nelem;
+    const SolidMaterial me = elem_mat(mat, ok ? elem : 0);
     if (ok && l < NPE) {
       const int n = conn[(size_t)elem * NPE + l];
       xs[g][3 * l] = x[3 * (size_t)n];
@@ -900,14 +927,14 @@ __global__ __launch_bounds__(256) void solid_matfree_ev(int nelem, const int* __
       double s[6];
       if (ISO) {
         const double tr = e[0] + e[1] + e[2];
-        s[0] = mat.lam * tr + 2.0 * mat.mu * e[0];
-        s[1] = mat.lam * tr + 2.0 * mat.mu * e[1];
-        s[2] = mat.lam * tr + 2.0 * mat.mu * e[2];
-        s[3] = mat.mu * e[3];
-        s[4] = mat.mu * e[4];
-        s[5] = mat.mu * e[5];
+        s[0] = me.lam * tr + 2.0 * me.mu * e[0];
+        s[1] = me.lam * tr + 2.0 * me.mu * e[1];
+        s[2] = me.lam * tr + 2.0 * me.mu * e[2];
+        s[3] = me.mu * e[3];
+        s[4] = me.mu * e[4];
+        s[5] = me.mu * e[5];
       } else {
-        stress6(mat, e, s);
+        stress6(me, e, s);
       }
       const double f = kc * wdt;
       // physical stress tensor rows [s0 s5 s4; s5 s1 s3; s4 s3 s2], scaled
@@ -921,7 +948,7 @@ __global__ __launch_bounds__(256) void solid_matfree_ev(int nelem, const int* __
           ss[g][l][3 * c + k] =
               P[3 * c] * J[3 * k] + P[3 * c + 1] * J[3 * k + 1] + P[3 * c + 2] * J[3 * k + 2];
       }
-      const double m = mc * mat.rho * wdt;
+      const double m = mc * me.rho * wdt;
       ss[g][l][9] = m * U[0];
       ss[g][l][10] = m * U[1];
       ss[g][l][11] = m * U[2];
@@ -1022,6 +1049,7 @@ __global__ __launch_bounds__(128) void solid_elem_scalars(
   constexpr int NPE = ORD * ORD * ORD, NQP = NPE;
   int el = blockIdx.x * blockDim.x + threadIdx.x;
   if (el >= nelem) return;
+  const SolidMaterial me = elem_mat(mat, el);
   const int* c = &conn[(size_t)el * NPE];
   const double t = telem ? telem[el] : 1.0;
   double acc = (MODE == ES_KS_MAX) ? -1e20 : 0.0;
@@ -1054,7 +1082,7 @@ __global__ __launch_bounds__(128) void solid_elem_scalars(
     }
     double e[6] = {Ux[0], Ux[4], Ux[8], Ux[5] + Ux[7], Ux[2] + Ux[6], Ux[1] + Ux[3]};
     double s[6];
-    stress6(mat, e, s);
+    stress6(me, e, s);
     double wd = qp.w * det;
     if (MODE == ES_COMPLIANCE) {
       acc += wd * t * (e[0] * s[0] + e[1] * s[1] + e[2] * s[2] + e[3] * s[3] + e[4] * s[4] + e[5] * s[5]);
@@ -1067,7 +1095,7 @@ __global__ __launch_bounds__(128) void solid_elem_scalars(
       double vm = sqrt(0.5 * ((s[0] - s[1]) * (s[0] - s[1]) + (s[0] - s[2]) * (s[0] - s[2]) +
                               (s[1] - s[2]) * (s[1] - s[2]) +
                               6.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5])));
-      double fail = safety * vm / mat.ys;
+      double fail = safety * vm / me.ys;
       if (MODE == ES_KS_MAX) acc = fmax(acc, fail);
       else acc += wd * exp(ks_weight * (fail - ks_max));
     }
@@ -1218,6 +1246,7 @@ __global__ __launch_bounds__(EBCfg<ORD>::TEAM* EBCfg<ORD>::EPB) void solid_elem_
   const int team = threadIdx.x / TEAM, tid = threadIdx.x % TEAM;
   const int el = blockIdx.x * EPB + team;
   const bool active = el < nelem;
+  const SolidMaterial me = elem_mat(mat, active ? el : 0);
   EBLds<ORD, T>& L = lds[team];
   const int* c = &conn[(size_t)(active ? el : 0) * NPE];
 
@@ -1273,7 +1302,7 @@ __global__ __launch_bounds__(EBCfg<ORD>::TEAM* EBCfg<ORD>::EPB) void solid_elem_
   for (int r = 0; r < BPT; r++)
 #pragma unroll
     for (int k = 0; k < 9; k++) acc[r][k] = T(0);
-  const T lam = T(mat.lam), mu = T(mat.mu), al = T(alpha), gr = T(gamma * mat.rho);
+  const T lam = T(me.lam), mu = T(me.mu), al = T(alpha), gr = T(gamma * me.rho);
   const bool mass = gamma != 0.0;
   for (int q = 0; q < NQP; q++) {
     const T f = al * L.wdt[q];

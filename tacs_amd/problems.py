@@ -14,10 +14,13 @@ class SolidProblem:
     dv: None (fixed t), "shared" (one DV for every element, tNum = 0) or
     "element" (one DV per element, tNum = element index).
     traction_patch: None (whole x = L face) or n (the centred n x n patch).
+    materials: a list of dicts (E, nu, rho, ys); element e takes
+    materials[e % len] (one constitutive object per element).
     """
 
     def __init__(self, n, order=2, L=1.0, E=70e9, nu=0.3, rho=2700.0, ys=270e6,
-                 traction=(0.0, 0.0, 1e6), t0=1.0, dv=None, traction_patch=None, shape=None):
+                 traction=(0.0, 0.0, 1e6), t0=1.0, dv=None, traction_patch=None, shape=None,
+                 materials=None):
         nx, ny, nz = shape if shape is not None else (n, n, n)
         self.mesh = m = HexMesh(nx, ny, nz, order=order, L=(L * nx / nz, L * ny / nz, L)
                                 if shape is not None else (L, L, L))
@@ -27,7 +30,16 @@ class SolidProblem:
         a = self.assembler = TACS.Assembler.create(None, 3, m.nnodes, m.nelem)
         ptr = np.arange(0, m.npe * m.nelem + 1, m.npe, dtype=np.intc)
         a.setElementConnectivity(ptr, m.conn.reshape(-1))
-        if dv == "element":
+        if materials is not None:
+            self.mat_props = [constitutive.MaterialProperties(rho=d["rho"], E=d["E"], nu=d["nu"],
+                                                              ys=d["ys"]) for d in materials]
+            elems = []
+            for e in range(m.nelem):
+                tnum = e if dv == "element" else (0 if dv == "shared" else -1)
+                con = constitutive.SolidConstitutive(self.mat_props[e % len(materials)], t=t0,
+                                                     tNum=tnum)
+                elems.append(elements.Element3D(elements.LinearElasticity3D(con), self.basis))
+        elif dv == "element":
             elems = []
             for e in range(m.nelem):
                 con = constitutive.SolidConstitutive(self.props, t=t0, tNum=e)
