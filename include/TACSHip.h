@@ -570,6 +570,7 @@ class TACSBatchedGMRES : public TACSObject {
   // y = M^{-1} x: block-Jacobi, or the preconditioner's Chebyshev polynomial
   void precond(int pr, int Rn, const void* Dinv, const float* A32, const double* x, double* y);
   double *d_cd = NULL, *d_caz = NULL;  // Chebyshev work vectors
+  float* d_cz32 = NULL;                // FP32 copies of the iterate (two buffers)
   void bmult(int pr, int Rn, const float* A32, const double* x, double* y);
   // x[j] = A^{-1} b[j], j < nrhs <= R; returns 1 when every system converged
   int solve(int nrhs, TACSBVec** b, TACSBVec** x, int zero_guess = 1);

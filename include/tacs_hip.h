@@ -583,11 +583,18 @@ int tacs_hip_bdiag_apply(int b, int prec, int nrows, int R, const void* Dinv, co
 /* One Chebyshev step for A z = x with the block-Jacobi inverse D^{-1}
    (prec 64 / 32 blocks, b = 3 or 6) over interleaved block vectors:
    w = D^{-1} (x - Az) (first: D^{-1} x), d = c1 d + c2 w (first: c2 w),
-   z += d (first: z = d).  Saad, Iterative Methods for Sparse Linear
-   Systems, 2nd ed., Alg. 12.1. */
+   z += d (first: z = d), and z32 = (float) z when z32 != NULL.  Saad,
+   Iterative Methods for Sparse Linear Systems, 2nd ed., Alg. 12.1. */
 int tacs_hip_bcheb_step(int b, int prec, int nrows, int R, const void* Dinv, const double* x,
                         const double* Az, double* d, double* z, double c1, double c2, int first,
-                        hipStream_t stream);
+                        float* z32, hipStream_t stream);
+/* The same step fused into the 3x3 product over R = 16, 20, ... (R % 4 == 0)
+   interleaved right-hand sides with FP32 values A: a = A X (X the FP32 copy
+   of z), then w = D^{-1} (x - a) (Dinv FP32), d = c1 d + c2 w, z += d (FP64)
+   and Xout = (float) z (Xout != X). */
+int tacs_hip_bsr3_spmm_cheb(int nrows, int R, const int* rowp, const int* cols, const float* A,
+                            const float* X, const double* x, const float* Dinv, double* d,
+                            double* z, float* Xout, double c1, double c2, hipStream_t stream);
 
 /* Segmented systems (the R <= 64 diagonal blocks of a block-diagonal 3x3
    BCSR matrix, block j = node range [off[j], off[j+1]), solved as R

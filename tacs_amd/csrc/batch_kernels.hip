@@ -542,7 +542,7 @@ template <int B, typename TD>
 __global__ void bcheb_step(int nrows, int R, const TD* __restrict__ Dinv,
                            const double* __restrict__ x, const double* __restrict__ Az,
                            double* __restrict__ d, double* __restrict__ z, double c1, double c2,
-                           int first) {
+                           int first, float* __restrict__ z32) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)nrows * R) return;
   const int node = (int)(t / R), j = (int)(t % R);
@@ -561,7 +561,9 @@ __global__ void bcheb_step(int nrows, int R, const TD* __restrict__ Dinv,
     const size_t o = ((size_t)B * node + r) * R + j;
     const double dn = first ? c2 * w : c1 * d[o] + c2 * w;
     d[o] = dn;
-    z[o] = first ? dn : z[o] + dn;
+    const double zn = first ? dn : z[o] + dn;
+    z[o] = zn;
+    if (z32) z32[o] = (float)zn;
   }
 }
 
@@ -622,6 +624,7 @@ __global__ __launch_bounds__(256) void bsr3_spmm_slots(int nrows, int R,
 // right-hand sides (16-byte loads of X, 12 sums), so NL = R/4 lanes cover a
 // block and a wave works on G = 64/NL blocks at once.
 typedef double spmm_dbl2 __attribute__((ext_vector_type(2)));
+typedef float spmm_flt4 __attribute__((ext_vector_type(4)));
 template <typename TA, int NL, int SPU = 1, int NTA = 1>
 __global__ __launch_bounds__(256) void bsr3_spmm_quad(int nrows, int R,
                                                       const int* __restrict__ rowp,
@@ -809,6 +812,104 @@ __global__ __launch_bounds__(256) void bsr3_spmm_pair(int nrows, int R,
 #pragma unroll
       for (int r = 0; r < 3; r++)
         *(spmm_dbl2*)&Y[((size_t)3 * row + r) * R + 2 * jp] = spmm_dbl2{y[r][0], y[r][1]};
+    }
+  }
+}
+
+// One Chebyshev step of the batched preconditioner fused into the product:
+// the quad-form product a = A z32 (FP32 z), then per row and right-hand side
+// w = D^{-1} (x - a), d = c1 d + c2 w, z = z + d (FP64) and z32out = (float) z.
+// z32 is a copy of z that the products of the polynomial read (half the
+// gather traffic of FP64 z); z32out must differ from z32in (other rows still
+// read it).  R = 4 NL, 16 <= R; Dinv in FP32 (the values' precision).
+template <typename TA, int NL, int SPU = 1, int NTA = 1>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void bsr3_spmm_quad_cheb(
+    int nrows, int R, const int* __restrict__ rowp, const int* __restrict__ cols,
+    const TA* __restrict__ A, const float* __restrict__ X, const double* __restrict__ xr,
+    const float* __restrict__ Dinv, double* __restrict__ d, double* __restrict__ z,
+    float* __restrict__ Xout, double c1, double c2) {
+  constexpr int G = 64 / NL;
+  const int lane = threadIdx.x & 63;
+  const int jq = lane % NL, g = lane / NL;
+  const bool okq = 4 * jq < R;
+  const int wid = threadIdx.x >> 6;
+  const int xcd = blockIdx.x & 7, nbx = gridDim.x >> 3;
+  const int r0 = (int)((long)nrows * xcd / 8), r1 = (int)((long)nrows * (xcd + 1) / 8);
+  for (int row = r0 + (blockIdx.x >> 3) * 4 + wid; row < r1; row += nbx * 4) {
+    const int rs = rowp[row], re = rowp[row + 1];
+    double y[3][4];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) y[r][q] = 0.0;
+    // software pipeline over the wave's steps of G blocks: the next step's
+    // column and values are loaded after this step's X gathers, so a step
+    // waits on one memory latency (the gathers), not on the column load
+    // and then the gathers
+    if (okq && rs + g < re) {
+      int colN = cols[rs + g];
+      TA avN[9];
+#pragma unroll
+      for (int e = 0; e < 9; e++)
+        avN[e] = NTA ? __builtin_nontemporal_load(&A[(size_t)9 * (rs + g) + e])
+                     : A[(size_t)9 * (rs + g) + e];
+      for (int s0 = rs + g; s0 < re; s0 += G) {
+        const int col = colN;
+        TA av[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) av[e] = avN[e];
+        double xv[3][4];
+        const float* xb = &X[(size_t)3 * col * R + 4 * jq];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const spmm_flt4 v = *(const spmm_flt4*)(xb + (size_t)c * R);
+          xv[c][0] = v.x;
+          xv[c][1] = v.y;
+          xv[c][2] = v.z;
+          xv[c][3] = v.w;
+        }
+        const int s1 = s0 + G;
+        if (s1 < re) {
+          colN = cols[s1];
+#pragma unroll
+          for (int e = 0; e < 9; e++)
+            avN[e] = NTA ? __builtin_nontemporal_load(&A[(size_t)9 * s1 + e])
+                         : A[(size_t)9 * s1 + e];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            y[r][q] += (double)av[3 * r] * xv[0][q] + (double)av[3 * r + 1] * xv[1][q] +
+                       (double)av[3 * r + 2] * xv[2][q];
+      }
+    }
+    for (int off = NL; off < 64; off <<= 1)
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) y[r][q] += __shfl_xor(y[r][q], off, 64);
+    // every lane holds the row sums of its right-hand sides; the G lanes of
+    // each jq share its 12 (block row r, right-hand side q) outputs
+    for (int k = g; okq && k < 12; k += G) {
+      const int r = k >> 2, q = k & 3;
+      double yq[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        yq[c] = q == 0 ? y[c][0] : (q == 1 ? y[c][1] : (q == 2 ? y[c][2] : y[c][3]));
+      const size_t base = (size_t)3 * row * R + 4 * jq + q;
+      const double rc0 = xr[base] - yq[0], rc1 = xr[base + R] - yq[1],
+                   rc2 = xr[base + 2 * (size_t)R] - yq[2];
+      const float* D = &Dinv[(size_t)9 * row + 3 * r];
+      double w = (double)D[0] * rc0;
+      w += (double)D[1] * rc1;
+      w += (double)D[2] * rc2;
+      const size_t o = base + (size_t)r * R;
+      const double dn = c1 * d[o] + c2 * w;
+      d[o] = dn;
+      const double zn = z[o] + dn;
+      z[o] = zn;
+      Xout[o] = (float)zn;
     }
   }
 }
@@ -1387,6 +1488,24 @@ int tacs_hip_bsr3_spmm(int nrows, int nrhs, const int* rowp, const int* cols, co
 
 int tacs_hip_bsr3_spmm_nvariants(int nrhs) { return spmm_nvariants(nrhs); }
 
+int tacs_hip_bsr3_spmm_cheb(int nrows, int R, const int* rowp, const int* cols, const float* A,
+                            const float* X, const double* x, const float* Dinv, double* d,
+                            double* z, float* Xout, double c1, double c2, hipStream_t stream) {
+  if (!spmm_quad_form(R) || R > 64 || X == Xout) return TACS_HIP_ERR_ARG;
+  if (nrows <= 0) return 0;
+  const int grid = 8 * tacs_grid_for((nrows + 7) / 8, 4, 8192);
+#define SPMM_CHEB(NLV)                                                                            \
+  hipLaunchKernelGGL((bsr3_spmm_quad_cheb<float, NLV, 1, 0>), dim3(grid), dim3(256), 0, stream,  \
+                     nrows, R, rowp, cols, A, X, x, Dinv, d, z, Xout, c1, c2)
+  switch (pow2_ceil(R / 4)) {
+    case 4: SPMM_CHEB(4); break;
+    case 8: SPMM_CHEB(8); break;
+    default: SPMM_CHEB(16); break;
+  }
+#undef SPMM_CHEB
+  return (int)hipGetLastError();
+}
+
 int tacs_hip_bsr3_spmm_variant(int variant, int prec_vals, int nrows, int nrhs, const int* rowp,
                                const int* cols, const void* A, const double* X, double* Y,
                                hipStream_t stream) {
@@ -1469,14 +1588,14 @@ int tacs_hip_bunpack(size_t n, int R, int j, const double* X, double* x, hipStre
 
 int tacs_hip_bcheb_step(int b, int prec, int nrows, int R, const void* Dinv, const double* x,
                         const double* Az, double* d, double* z, double c1, double c2, int first,
-                        hipStream_t stream) {
+                        float* z32, hipStream_t stream) {
   if ((b != 3 && b != 6) || (prec != 32 && prec != 64) || R < 1) return TACS_HIP_ERR_ARG;
   const size_t N = (size_t)nrows * R;
   if (N == 0) return 0;
   dim3 g((unsigned)((N + 255) / 256)), t(256);
 #define CHEB(BB, TD)                                                                             \
   hipLaunchKernelGGL((bcheb_step<BB, TD>), g, t, 0, stream, nrows, R, (const TD*)Dinv, x, Az, d, \
-                     z, c1, c2, first)
+                     z, c1, c2, first, z32)
   if (b == 3 && prec == 64)
     CHEB(3, double);
   else if (b == 3)

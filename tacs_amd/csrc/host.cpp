@@ -1608,7 +1608,7 @@ TACSBatchedGMRES::~TACSBatchedGMRES() {
   HCHECK(hipStreamSynchronize(g_stream));
   void* bufs[] = {d_V, d_work, d_Bb,  d_Xb,  d_scal, d_vec, d_cnt, d_red,
                   d_U, d_C,    d_Ub,  d_Cb,  d_Unb,  d_Cnb, d_prod, d_dq, d_segoff, d_ux, d_uy,
-                  d_cd, d_caz};
+                  d_cd, d_caz, d_cz32};
   for (void* q : bufs) dfree(q);
   hfree(h_pin);
   if (h_stage) hfree(h_stage);
@@ -1797,21 +1797,41 @@ void TACSBatchedGMRES::precond(int pr, int Rn, const void* Dinv, const float* A3
     buf[(deg - 1) % 2] = zc;
     buf[deg % 2] = d_caz;
     KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, 1, Dinv, xc, NULL, d_cd, buf[0], 0.0,
-                               1.0 / theta, 1, g_stream));
+                               1.0 / theta, 1, NULL, g_stream));
     for (int k = 1; k < deg; k++) {
       const double rn = 1.0 / (2.0 * sigma - rho);
       mat->chebStep(pin, Dinv, buf[(k - 1) % 2], buf[k % 2], xc, d_cd, rn * rho,
                     2.0 * rn / delta);
       rho = rn;
     }
+  } else if (pr == 32 && pc->chebValues == 16 && mat->bsize == 3 && rc % 4 == 0 && rc >= 16) {
+    // R right-hand sides, reduced precision inside the polynomial: the
+    // products read an FP32 copy of the iterate (half the gather traffic),
+    // each step fused into its product; the iterate and d stay FP64
+    if (!d_cz32) {
+      d_cz32 = dalloc<float>(2 * (size_t)mat->nrows * 3 * R);
+      if (g_halt) return;
+    }
+    float* z32[2] = {d_cz32, d_cz32 + (size_t)mat->nrows * 3 * R};
+    const int* rowp = mat->d_rowp;
+    const int* cols = mat->d_cols;
+    KCHECK(tacs_hip_bcheb_step(3, 32, mat->nrows, rc, Dinv, xc, NULL, d_cd, zc, 0.0, 1.0 / theta,
+                               1, z32[0], g_stream));
+    for (int k = 1; k < deg; k++) {
+      const double rn = 1.0 / (2.0 * sigma - rho);
+      KCHECK(tacs_hip_bsr3_spmm_cheb(mat->nrows, rc, rowp, cols, A32, z32[(k - 1) % 2],
+                                     xc, (const float*)Dinv, d_cd, zc, z32[k % 2], rn * rho,
+                                     2.0 * rn / delta, g_stream));
+      rho = rn;
+    }
   } else {
     KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, rc, Dinv, xc, NULL, d_cd, zc, 0.0,
-                               1.0 / theta, 1, g_stream));
+                               1.0 / theta, 1, NULL, g_stream));
     for (int k = 1; k < deg; k++) {
       mult(zc, d_caz);
       const double rn = 1.0 / (2.0 * sigma - rho);
       KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, rc, Dinv, xc, d_caz, d_cd, zc,
-                                 rn * rho, 2.0 * rn / delta, 0, g_stream));
+                                 rn * rho, 2.0 * rn / delta, 0, NULL, g_stream));
       rho = rn;
     }
   }

@@ -87,13 +87,15 @@ def test_recycling_solve_with_chebyshev_matches_dense(deg, ratio, values):
     assert its1 < k1.getIterCount()
 
 
-def test_batched_solve_with_chebyshev_matches_dense():
+@pytest.mark.parametrize("R,values", [(4, 32), (16, 16), (16, 32), (32, 16)])
+def test_batched_solve_with_chebyshev_matches_dense(R, values):
+    """values 16 with R = 16, 32: the polynomial's products read an FP32 copy
+    of the iterate, each step fused into its product (bsr3_spmm_quad_cheb)."""
     p = SolidProblem(3, order=3)
     p.assemble()
     pc = TACS.Pc(p.mat)
-    pc.setChebyshev(5, 50.0)
+    pc.setChebyshev(5, 50.0, values=values)
     pc.factor()
-    R = 4
     rng = np.random.default_rng(5)
     a = p.assembler
     bs = [a.createVec() for _ in range(R)]
