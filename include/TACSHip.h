@@ -390,7 +390,9 @@ class TACSBlockJacobiPc : public TACSPc {
   // smallest eigenvalues are left to the recycled space.  degree 1 (the
   // default) is plain block-Jacobi.
   // values 16: the products inside the polynomial read a bfloat16 copy of
-  // the values (single systems; the outer operator keeps its values)
+  // the values (single systems; the outer operator keeps its values) or,
+  // in the batched solver with 16 or more right-hand sides, an FP32 copy of
+  // the iterate
   int setChebyshev(int degree, double ratio, int values = 32);
   double lanczosMax(int steps);
   int chebDegree = 1, chebValues = 32;

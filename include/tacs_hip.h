@@ -882,8 +882,9 @@ int tacs_pc_apply(tacs_obj pc, tacs_obj x, tacs_obj y);
    block-Jacobi; interval [lmax / ratio, lmax] of D^{-1} A, lmax estimated by
    the next factor()); values 32, or 16 for products inside the polynomial
    that read a bfloat16 copy of the values (single systems with FP32-valued
-   operators).  GMRES and pc_apply keep plain block-Jacobi.  No counterpart
-   in the reference. */
+   operators) or an FP32 copy of the iterate (batched solves of 16 or more
+   right-hand sides, 3x3 blocks).  GMRES and pc_apply keep plain
+   block-Jacobi.  No counterpart in the reference. */
 int tacs_pc_set_chebyshev(tacs_obj pc, int degree, double ratio, int values);
 /* the settings and the lmax the last factor() estimated (1.05 x the
    largest Ritz value of 30 Lanczos steps; 0 before a factor() with

@@ -232,9 +232,12 @@ class Pc(TACSHandle):
         the recycling and batched solvers, on [lmax / ratio, lmax] of the
         eigenvalues of D^{-1} A (lmax estimated at the next factor(): 1.05 x
         the largest Ritz value of 30 Lanczos steps).
-        degree 1: plain block-Jacobi.  values=16: the products inside the
-        polynomial read a bfloat16 copy of the matrix values (single systems
-        whose operator has FP32 values).  No counterpart in the reference."""
+        degree 1: plain block-Jacobi.  values=16: reduced precision inside
+        the polynomial of FP32-operator solvers: single systems read a
+        bfloat16 copy of the matrix values (the polynomial stays linear);
+        the batched solver with 16 or more right-hand sides (3x3 blocks)
+        reads an FP32 copy of the iterate (not exactly linear; the solver's
+        FP64 restarts absorb it).  No counterpart in the reference."""
         check(lib().tacs_pc_set_chebyshev(self.ptr, int(degree), float(ratio), int(values)),
               "Pc.setChebyshev")
 
