@@ -269,21 +269,23 @@ int tacs_hip_shell_elem_kernel(int prec, int mode, int nelem, const int* conn, c
                                double ks_max, double ks_sum, double safety, void* ke, double* ev,
                                double* es, hipStream_t stream);
 
-/* Element matrices ke = kscale K + mscale M of the Quad4 shell family
-   (24 x 24 per element, FP64 when prec = 64, FP32 when 32), with K the
-   tangent stiffness and M the consistent mass at tOffset = 0, and the
-   FP64 element residuals ev = K u when ev != NULL (prec 64 only; u = NULL
-   is the zero state).  Replaces TACSShellElement::getMatType
-   (src/elements/shell/TACSShellElement.h:644: stiffness alpha = 1, mass
-   gamma = 1 through addJacobian :447 with the mass moments of
+/* Element matrices ke = kscale K + mscale M + gscale G of the Quad4 shell
+   family (24 x 24 per element, FP64 when prec = 64, FP32 when 32): K the
+   tangent stiffness, M the consistent mass (mass moments of
    TACSIsoShellConstitutive::evalMassMoments
-   src/constitutive/TACSIsoShellConstitutive.cpp:120) and the gamma term of
-   TACSShellElement::addJacobian :605-617 with
+   src/constitutive/TACSIsoShellConstitutive.cpp:120), G the geometric
+   stiffness along the state u (FP64 only, u required: the derivative along
+   u of TACSShellNonlinearModel's tangent stiffness at the zero state), and
+   the FP64 element residuals ev = K u when ev != NULL (prec 64 only; u =
+   NULL is the zero state).  Replaces TACSShellElement::getMatType
+   (src/elements/shell/TACSShellElement.h:644: stiffness alpha = 1, mass
+   gamma = 1 through addJacobian :447, geometric stiffness :661-908) and the
+   gamma term of TACSShellElement::addJacobian :605-617 with
    TACSLinearizedRotation::addDirectorJacobian
    src/elements/shell/TACSDirector.h:369. */
 int tacs_hip_shell_mat_type(int prec, int nelem, const int* conn, const double* X,
                             const double* u, const double* telem, const ShellMaterial* mat,
-                            double kscale, double mscale, void* ke, double* ev,
+                            double kscale, double mscale, double gscale, void* ke, double* ev,
                             hipStream_t stream);
 
 /* Node-coordinate derivatives of the Quad4 shell family, exact (forward-mode

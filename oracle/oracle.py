@@ -385,16 +385,33 @@ def assemble_shell(conn, X, u, t_elem, E, nu, rowp, cols):
     return A, res
 
 
-def assemble_shell_mat(conn, X, t_elem, E, nu, rho, kc, mc, rowp, cols):
-    """kc K + mc M of Quad4 shells assembled into 6x6 block rows."""
+def assemble_shell_mat(conn, X, t_elem, E, nu, rho, kc, mc, rowp, cols, gc=0.0, u=None):
+    """kc K + mc M + gc G(u) of Quad4 shells assembled into 6x6 block rows."""
     conn = _ii(conn).reshape(-1)
     A = np.zeros(36 * cols.size)
     miss = lib().orc_assemble_shell_mat(conn.size // 4, _p(conn, np.int32), _p(_dd(X)),
+                                        _p(_dd(u)) if u is not None else None,
                                         _p(_dd(t_elem)), _c(E), _c(nu), _c(rho), _c(kc),
-                                        _c(mc), _p(_ii(rowp), np.int32),
+                                        _c(mc), _c(gc), _p(_ii(rowp), np.int32),
                                         _p(_ii(cols), np.int32), _p(A))
     assert miss == 0
     return A
+
+
+def quad4_nl_element(X, u, E, nu, t):
+    """Residual and tangent stiffness of the nonlinear shell model at u."""
+    res = np.zeros(24)
+    mat = np.zeros(576)
+    lib().orc_quad4_nl_element(_p(_dd(X)), _p(_dd(u)), _c(E), _c(nu), _c(t), _c(1.0),
+                               _p(res), _p(mat))
+    return res, mat.reshape(24, 24)
+
+
+def quad4_geo(X, p, E, nu, t):
+    """Geometric stiffness of the linear shell element along the path p."""
+    mat = np.zeros(576)
+    lib().orc_quad4_geo(_p(_dd(X)), _p(_dd(p)), _c(E), _c(nu), _c(t), _c(1.0), _p(mat))
+    return mat.reshape(24, 24)
 
 
 def shell_add_pressure(conn, X, aux_elems, pressures, res):

@@ -260,6 +260,21 @@ void ref_quad4_shell_offset(const double* X, const double* vars, const double* p
   elem->decref();
 }
 
+// Jacobian (alpha = 1) of one TACSQuad4NonlinearShell at the state vars
+// (TACSShellNonlinearModel), with the residual.
+void ref_quad4_nonlinear_shell(const double* X, const double* vars, double E, double nu,
+                               double t, double* res, double* mat) {
+  TACSMaterialProperties* props = new TACSMaterialProperties(2700.0, 921.0, E, nu, 270e6, 0.0, 0.0);
+  TACSIsoShellConstitutive* con = new TACSIsoShellConstitutive(props, t, 0);
+  TACSElement* elem = new TACSQuad4NonlinearShell(new TACSShellNaturalTransform(), con);
+  elem->incref();
+  double zero[24] = {0};
+  for (int i = 0; i < 24; i++) res[i] = 0.0;
+  for (int i = 0; i < 576; i++) mat[i] = 0.0;
+  elem->addJacobian(0, 0.0, 1.0, 0.0, 0.0, X, vars, zero, zero, res, mat);
+  elem->decref();
+}
+
 // psi^T dR/dt of one TACSQuad4Shell (thickness DV, number 0).
 double ref_quad4_adj_product(const double* X, const double* vars, const double* psi, double E,
                              double nu, double ys, double t) {

@@ -237,7 +237,26 @@ static void sh_node_normals(const double X[12], double fn[12], double Xdn[36]) {
     - e = [e0ty0, e0ty3, 2 e0ty1, u1x0, u1x4, u1x1 + u1x3, 2 e0ty4,
       2 e0ty2, et] (TACSShellLinearModel::evalStrain :717-731).
 */
+static void sh_strains(const double X[12], const double v[24], int drill, int nonlinear,
+                       double e[4][9], double detw[4]);
+
 void orc_quad4_strains(const double X[12], const double v[24], int drill,
+                       double e[4][9], double detw[4]) {
+  sh_strains(X, v, drill, 0, e, detw);
+}
+
+/* Strains of TACSShellNonlinearModel (TACSShellElementModel.h:1067): the
+   tying strains with their quadratic terms (computeTyingStrain :1088-1140:
+   g11 += Uxi_1.Uxi_1 / 2, g22 += Uxi_2.Uxi_2 / 2, g12 += Uxi_1.Uxi_2 / 2,
+   g13 += d0.Uxi_1 / 2, g23 += d0.Uxi_2 / 2) and the bending strains with
+   u0x^T u1x (evalStrain :1996-2012); the drill strain as in the linear
+   model. */
+void orc_quad4_strains_nl(const double X[12], const double v[24], int drill,
+                          double e[4][9], double detw[4]) {
+  sh_strains(X, v, drill, 1, e, detw);
+}
+
+static void sh_strains(const double X[12], const double v[24], int drill, int nonlinear,
                        double e[4][9], double detw[4]) {
   double fn[12], Xdn[36];
   sh_node_normals(X, fn, Xdn);
@@ -277,23 +296,29 @@ void orc_quad4_strains(const double X[12], const double v[24], int drill,
     double Uxi[6], Xxi[6];
     sh_grad(3, 3, pt, X, Xxi);
     sh_grad(6, 3, pt, v, Uxi);
+    const double nl = nonlinear ? 1.0 : 0.0;
     if (t < 2) {
-      ety[t] = Uxi[0] * Xxi[0] + Uxi[2] * Xxi[2] + Uxi[4] * Xxi[4];
+      ety[t] = Uxi[0] * Xxi[0] + Uxi[2] * Xxi[2] + Uxi[4] * Xxi[4] +
+               nl * 0.5 * (Uxi[0] * Uxi[0] + Uxi[2] * Uxi[2] + Uxi[4] * Uxi[4]);
     } else if (t < 4) {
-      ety[t] = Uxi[1] * Xxi[1] + Uxi[3] * Xxi[3] + Uxi[5] * Xxi[5];
+      ety[t] = Uxi[1] * Xxi[1] + Uxi[3] * Xxi[3] + Uxi[5] * Xxi[5] +
+               nl * 0.5 * (Uxi[1] * Uxi[1] + Uxi[3] * Uxi[3] + Uxi[5] * Uxi[5]);
     } else if (t < 5) {
       ety[t] = 0.5 * (Uxi[0] * Xxi[1] + Uxi[2] * Xxi[3] + Uxi[4] * Xxi[5] +
-                      Uxi[1] * Xxi[0] + Uxi[3] * Xxi[2] + Uxi[5] * Xxi[4]);
+                      Uxi[1] * Xxi[0] + Uxi[3] * Xxi[2] + Uxi[5] * Xxi[4] +
+                      nl * (Uxi[0] * Uxi[1] + Uxi[2] * Uxi[3] + Uxi[4] * Uxi[5]));
     } else {
       double d0[3], n0[3];
       sh_interp(3, 3, pt, d, d0);
       sh_interp(3, 3, pt, fn, n0);
       if (t < 7) { /* g23 */
         ety[t] = 0.5 * (Xxi[1] * d0[0] + Xxi[3] * d0[1] + Xxi[5] * d0[2] +
-                        n0[0] * Uxi[1] + n0[1] * Uxi[3] + n0[2] * Uxi[5]);
+                        (n0[0] + nl * d0[0]) * Uxi[1] + (n0[1] + nl * d0[1]) * Uxi[3] +
+                        (n0[2] + nl * d0[2]) * Uxi[5]);
       } else { /* g13 */
         ety[t] = 0.5 * (Xxi[0] * d0[0] + Xxi[2] * d0[1] + Xxi[4] * d0[2] +
-                        n0[0] * Uxi[0] + n0[1] * Uxi[2] + n0[2] * Uxi[4]);
+                        (n0[0] + nl * d0[0]) * Uxi[0] + (n0[1] + nl * d0[1]) * Uxi[2] +
+                        (n0[2] + nl * d0[2]) * Uxi[4]);
       }
     }
   }
@@ -350,6 +375,17 @@ void orc_quad4_strains(const double X[12], const double v[24], int drill,
     eq[3] = u1x[0];
     eq[4] = u1x[4];
     eq[5] = u1x[1] + u1x[3];
+    if (nonlinear) {
+      /* u0x = T^T [u0,xi | d0] Xdinv T (TacsShellComputeDispGrad,
+         TACSShellUtilities.h:405-418) */
+      double u0x[9];
+      sh_mm(u0d, XdinvT, tmp);
+      sh_mtm(T, tmp, u0x);
+      eq[3] += u0x[0] * u1x[0] + u0x[3] * u1x[3] + u0x[6] * u1x[6];
+      eq[4] += u0x[1] * u1x[1] + u0x[4] * u1x[4] + u0x[7] * u1x[7];
+      eq[5] += u0x[0] * u1x[1] + u0x[3] * u1x[4] + u0x[6] * u1x[7] + u1x[0] * u0x[1] +
+               u1x[3] * u0x[4] + u1x[6] * u0x[7];
+    }
     eq[6] = 2.0 * E0[5];
     eq[7] = 2.0 * E0[2];
     eq[8] = drill ? et : 0.0;
@@ -508,6 +544,137 @@ void orc_quad4_mass(const double X[12], double rho, double t, double alpha,
         for (int k = 0; k < 3; k++)
           acc += m0 * Nu[k][r] * Nu[k][c] + m2 * Nd[k][r] * Nd[k][c] +
                  m1 * (Nu[k][r] * Nd[k][c] + Nd[k][r] * Nu[k][c]);
+        mat[SH_NV * r + c] += alpha * detw[q] * acc;
+      }
+    }
+  }
+}
+
+/*
+  TACSShellNonlinearModel element (TACSQuad4NonlinearShell,
+  TACSShellElementDefs.h:58-60) and the geometric stiffness of the linear
+  element (TACSShellElement::getMatType(TACS_GEOMETRIC_STIFFNESS_MATRIX),
+  TACSShellElement.h:661-908: the directional derivative, along the path
+  p = the element variables, of the nonlinear model's tangent stiffness at the
+  zero state).  With e(u) = L u + Q(u, u) the nonlinear strain (quadratic in
+  u) at a quadrature point, s = C e and U = 1/2 sum_q w detXd e.s:
+    B(u)[:, c] = de/du_c = L e_c + 2 Q(u, e_c),
+    K_T(u)[r][c] = sum_q w detXd (B(u)[:, r] . C B(u)[:, c] + s(u) . H[:, r, c]),
+    H[:, r, c] = 2 Q(e_r, e_c),
+    G = dK_T(eps p)/deps at 0
+      = sum_q w detXd (B1[:, r] . C L e_c + L e_r . C B1[:, c] + (C L p) . H[:, r, c]),
+    B1[:, c] = 2 Q(p, e_c).
+  The bilinear terms come from strain evaluations by polarisation,
+  2 Q(a, b) = e(a + b) - e(a) - e(b), exact for the quadratic strain.
+*/
+static void sh_nl_bmat(const double X[12], const double u[SH_NV], double B[4][9][SH_NV]) {
+  double eu[4][9], ec[4][9], euc[4][9], detw[4];
+  double v[SH_NV];
+  sh_strains(X, u, 1, 1, eu, detw);
+  for (int c = 0; c < SH_NV; c++) {
+    for (int k = 0; k < SH_NV; k++) v[k] = 0.0;
+    v[c] = 1.0;
+    sh_strains(X, v, 1, 1, ec, detw);
+    double el[4][9];
+    sh_strains(X, v, 1, 0, el, detw);
+    for (int k = 0; k < SH_NV; k++) v[k] = u[k] + (k == c ? 1.0 : 0.0);
+    sh_strains(X, v, 1, 1, euc, detw);
+    for (int q = 0; q < 4; q++) {
+      for (int k = 0; k < 9; k++) B[q][k][c] = el[q][k] + (euc[q][k] - eu[q][k] - ec[q][k]);
+    }
+  }
+}
+
+/* 2 Q(e_r, e_c) at the four points for all pairs: H[q][k][r][c] */
+static void sh_nl_hess(const double X[12], double (*H)[9][SH_NV][SH_NV]) {
+  static double ec[SH_NV][4][9];
+  double detw[4], v[SH_NV] = {0}, erc[4][9];
+  for (int c = 0; c < SH_NV; c++) {
+    v[c] = 1.0;
+    sh_strains(X, v, 1, 1, ec[c], detw);
+    v[c] = 0.0;
+  }
+  for (int r = 0; r < SH_NV; r++) {
+    for (int c = r; c < SH_NV; c++) {
+      v[r] += 1.0;
+      v[c] += 1.0;
+      sh_strains(X, v, 1, 1, erc, detw);
+      v[r] = 0.0;
+      v[c] = 0.0;
+      for (int q = 0; q < 4; q++) {
+        for (int k = 0; k < 9; k++) {
+          double h = erc[q][k] - ec[r][q][k] - ec[c][q][k];
+          if (r == c) h = erc[q][k] - 2.0 * ec[r][q][k];
+          H[q][k][r][c] = h;
+          H[q][k][c][r] = h;
+        }
+      }
+    }
+  }
+}
+
+/* Residual and tangent stiffness (added, scaled by alpha) of the nonlinear
+   model at the state u */
+void orc_quad4_nl_element(const double X[12], const double u[24], double E, double nu,
+                          double t, double alpha, double *res, double *mat) {
+  static double B[4][9][SH_NV], H[4][9][SH_NV][SH_NV];
+  double eu[4][9], detw[4];
+  sh_strains(X, u, 1, 1, eu, detw);
+  sh_nl_bmat(X, u, B);
+  if (mat) sh_nl_hess(X, H);
+  for (int q = 0; q < 4; q++) {
+    double s[9], CB[9][SH_NV];
+    orc_shell_stress(E, nu, t, eu[q], s);
+    for (int c = 0; c < SH_NV; c++) {
+      double b[9], sb[9];
+      for (int k = 0; k < 9; k++) b[k] = B[q][k][c];
+      orc_shell_stress(E, nu, t, b, sb);
+      for (int k = 0; k < 9; k++) CB[k][c] = sb[k];
+    }
+    if (res) {
+      for (int c = 0; c < SH_NV; c++) {
+        double acc = 0.0;
+        for (int k = 0; k < 9; k++) acc += B[q][k][c] * s[k];
+        res[c] += detw[q] * acc;
+      }
+    }
+    if (mat) {
+      for (int r = 0; r < SH_NV; r++) {
+        for (int c = 0; c < SH_NV; c++) {
+          double acc = 0.0;
+          for (int k = 0; k < 9; k++) acc += B[q][k][r] * CB[k][c] + s[k] * H[q][k][r][c];
+          mat[SH_NV * r + c] += alpha * detw[q] * acc;
+        }
+      }
+    }
+  }
+}
+
+/* Geometric stiffness along the path p (added, scaled by alpha) */
+void orc_quad4_geo(const double X[12], const double p[24], double E, double nu, double t,
+                   double alpha, double *mat) {
+  static double B0[4][9][SH_NV], Bp[4][9][SH_NV], H[4][9][SH_NV][SH_NV];
+  double el[4][9], detw[4], zero[SH_NV] = {0};
+  sh_nl_bmat(X, zero, B0);  /* L e_c */
+  sh_nl_bmat(X, p, Bp);     /* L e_c + 2 Q(p, e_c) */
+  sh_nl_hess(X, H);
+  sh_strains(X, p, 1, 0, el, detw);  /* L p */
+  for (int q = 0; q < 4; q++) {
+    double sp[9], CB0[9][SH_NV];
+    orc_shell_stress(E, nu, t, el[q], sp);
+    for (int c = 0; c < SH_NV; c++) {
+      double b[9], sb[9];
+      for (int k = 0; k < 9; k++) b[k] = B0[q][k][c];
+      orc_shell_stress(E, nu, t, b, sb);
+      for (int k = 0; k < 9; k++) CB0[k][c] = sb[k];
+    }
+    for (int r = 0; r < SH_NV; r++) {
+      for (int c = 0; c < SH_NV; c++) {
+        double acc = 0.0;
+        for (int k = 0; k < 9; k++) {
+          const double b1r = Bp[q][k][r] - B0[q][k][r], b1c = Bp[q][k][c] - B0[q][k][c];
+          acc += b1r * CB0[k][c] + CB0[k][r] * b1c + sp[k] * H[q][k][r][c];
+        }
         mat[SH_NV * r + c] += alpha * detw[q] * acc;
       }
     }
@@ -703,23 +870,25 @@ long orc_assemble_shell(int nelem, const int *conn, const double *X,
 }
 
 /*
-  kc K + mc M assembled element by element (TACSAssembler::assembleMatType /
+  kc K + mc M + gc G (G: geometric stiffness along the state u, NULL = 0)
+  assembled element by element (TACSAssembler::assembleMatType /
   assembleMatCombo, src/TACSAssembler.cpp:4418-4560, over
   TACSShellElement::getMatType) into 6x6 BCSR; returns the number of missing
   pattern entries.
 */
 long orc_assemble_shell_mat(int nelem, const int *conn, const double *X,
-                            const double *t_elem, double E, double nu,
-                            double rho, double kc, double mc, const int *rowp,
+                            const double *u, const double *t_elem, double E, double nu,
+                            double rho, double kc, double mc, double gc, const int *rowp,
                             const int *cols, double *A) {
   long missing = 0;
   for (int el = 0; el < nelem; el++) {
     const int *c = &conn[4 * el];
-    double Xe[12], ke[SH_NV * SH_NV], zero[SH_NV] = {0};
-    sh_gather(c, X, NULL, Xe, NULL);
+    double Xe[12], ue[24], ke[SH_NV * SH_NV], zero[SH_NV] = {0};
+    sh_gather(c, X, u, Xe, ue);
     memset(ke, 0, sizeof(ke));
     orc_quad4_element(Xe, zero, E, nu, t_elem[el], kc, NULL, ke);
     orc_quad4_mass(Xe, rho, t_elem[el], mc, ke);
+    if (gc != 0.0) orc_quad4_geo(Xe, ue, E, nu, t_elem[el], gc, ke);
     for (int a = 0; a < SH_NN; a++) {
       for (int b = 0; b < SH_NN; b++) {
         long s = sh_find_slot(rowp, cols, c[a], c[b]);

@@ -3378,7 +3378,7 @@ void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScal
       KCHECK(shellKernel(64, 1, NULL, 1.0, 0, 0, 1, 1, NULL, ev, NULL));
     HCHECK(hipEventRecord(e0, g_stream));
     KCHECK(tacs_hip_shell_mat_type(elemPrec, numElements, d_conn, d_X, d_u, d_telem, &smat, alpha,
-                                   gamma, d_ke, elemPrec == 64 ? ev : NULL, g_stream));
+                                   gamma, 0.0, d_ke, elemPrec == 64 ? ev : NULL, g_stream));
     KCHECK(tacs_hip_gather_elem_matrices(6, 4, elemPrec, numNodes, d_n2e_ptr, d_n2e, d_conn, d_ke,
                                          A->d_rowp, A->d_cols, bcs ? d_bcvars : NULL, A->d_A,
                                          A->d_A32, g_stream));
@@ -3447,7 +3447,8 @@ void TACSAssembler::assembleJacobian(TacsScalar alpha, TacsScalar beta, TacsScal
 // coefficients of the stiffness, the consistent mass and the geometric
 // stiffness at the current state; aux elements (tractions, pressures) add no
 // matrix terms (TACSElement::getMatType src/elements/TACSElement.cpp:334 over
-// their zero Jacobians).  Shells: the stiffness and the consistent mass.
+// their zero Jacobians).  Shells: the stiffness, the consistent mass and the
+// geometric stiffness along the current state.
 void TACSAssembler::assembleMatCombo(ElementMatrixType matTypes[], TacsScalar scale[], int nmats,
                                      TACSMat* tA, MatrixOrientation matOr, const TacsScalar lambda,
                                      const bool bcs) {
@@ -3478,13 +3479,9 @@ void TACSAssembler::assembleMatCombo(ElementMatrixType matTypes[], TacsScalar sc
     }
   }
   if (family == FAMILY_SHELL) {
-    if (gc != 0.0) {
-      tacs_set_error("assembleMatType: the shell geometric stiffness is not built");
-      return;
-    }
     ensure(d_ke, (size_t)numElements * 576);
-    KCHECK(tacs_hip_shell_mat_type(64, numElements, d_conn, d_X, NULL, d_telem, &smat, kc, mc,
-                                   d_ke, NULL, g_stream));
+    KCHECK(tacs_hip_shell_mat_type(64, numElements, d_conn, d_X, gc != 0.0 ? d_u : NULL, d_telem,
+                                   &smat, kc, mc, gc, d_ke, NULL, g_stream));
     KCHECK(tacs_hip_gather_elem_matrices(6, 4, 64, numNodes, d_n2e_ptr, d_n2e, d_conn, d_ke,
                                          A->d_rowp, A->d_cols, bcs ? d_bcvars : NULL, A->d_A,
                                          A->d_A32, g_stream));

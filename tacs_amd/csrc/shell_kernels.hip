@@ -342,6 +342,19 @@ enum {
   SH_XPT_ADJ = 10,         // scale psi^T R
   SH_XPT_COMPLIANCE = 11,  // scale sum_q detXd_q e.s
   SH_XPT_KS = 12,          // scale sum_q detXd_q exp(w (f_q - max)) / (w sum)
+  SH_GEO = 13,             // ke += scale G (geometric stiffness along the state u)
+};
+
+// Work arrays of the geometric stiffness (SH_GEO), per element
+struct ShellGeoLDS {
+  double sp[4][9];      // stress of the linear strain of the path at the points
+  double W[9];          // tying-point weights of sp (see SH_GEO)
+  double PU[9][6];      // path: dU/dxi at the tying points
+  double PD[9][3];      // path: director at the tying points
+  double PU0[4][9];     // path: u0x at the points
+  double PU1[4][9];     // path: u1x at the points
+  double B1[4][9][24];  // 2 Q(path, e_c): the path derivative of the strain map
+  double PW[4][24][9];  // unit column c at point q: p (3), w0 (3), w1 (3)
 };
 
 struct ShellParams {
@@ -363,7 +376,7 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
     double* __restrict__ ev, double* __restrict__ es) {
   __shared__ ShellLDS<T> lds_all[SH_EPB];
   const int team = threadIdx.x / SH_TEAM, lane = threadIdx.x % SH_TEAM;
-  constexpr bool XPT = MODE >= SH_XPT_ADJ;
+  constexpr bool XPT = MODE == SH_XPT_ADJ || MODE == SH_XPT_COMPLIANCE || MODE == SH_XPT_KS;
   const int gt = blockIdx.x * SH_EPB + team;
   const int el = XPT ? gt / 12 : gt;
   const int dir = XPT ? gt % 12 : -1;
@@ -604,6 +617,359 @@ __global__ __launch_bounds__(SH_TEAM* SH_EPB) void shell_elem_kernel(
     }
   }
   __syncthreads();
+
+  if constexpr (MODE == SH_GEO) {
+    // Geometric stiffness of TACSShellElement::getMatType(TACS_GEOMETRIC_
+    // STIFFNESS_MATRIX) (TACSShellElement.h:661-908): the derivative, along
+    // the path p = the element state, of the tangent stiffness of
+    // TACSShellNonlinearModel at the zero state.  The nonlinear strain is
+    // e(u) = L u + Q(u, u) (L: the linear map above, B[q][k][c] = (L e_c)_k;
+    // Q: the quadratic terms of computeTyingStrain and evalStrain,
+    // TACSShellElementModel.h:1088-1140, :1996-2012), so with s_p = C L p
+    //   G[r][c] = sum_q detw_q (B1[:, r] . C B[:, c] + B[:, r] . C B1[:, c]
+    //                           + s_p . 2 Q(e_r, e_c)),   B1[:, c] = 2 Q(p, e_c).
+    // The tying part of s_p . Q(e_r, e_c) is sum_t W_t g_t(e_r, e_c) over the
+    // nine tying-point bilinear values: the interpolation and the transform
+    // E0 = XT^T G XT are linear, so s_p contracts with them once per element
+    // (M_q = XT S XT^T, S the symmetric matrix of s_p's in-plane and shear
+    // parts).
+    __shared__ ShellGeoLDS geo_all[SH_EPB];
+    ShellGeoLDS& Gs = geo_all[team];
+    // (1) s_p at the points (lanes 0-3) and the path's tying-point fields
+    //     (lanes 4-12) and point gradients (lanes 13-16)
+    if (active && lane < 4) {
+      const int q = lane;
+      T ep[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        T acc = T(0);
+        for (int c = 0; c < SH_NV; c++) acc += L.B[q][k][c] * L.u[c];
+        ep[k] = acc;
+      }
+      T sq[9];
+      sh_stress<T>(M, t, ep, sq);
+#pragma unroll
+      for (int k = 0; k < 9; k++) Gs.sp[q][k] = double(sq[k]);
+    } else if (active && lane >= 4 && lane < 13) {
+      const int ty = lane - 4;
+      double xd, yd;
+      sh_tying_pt(ty, xd, yd);
+      T U[6] = {T(0), T(0), T(0), T(0), T(0), T(0)}, D[3] = {T(0), T(0), T(0)};
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        T N, Nx, Ny;
+        sh_shape<T>(a, T(xd), T(yd), N, Nx, Ny);
+        const T* ua = &L.u[6 * a];
+        const T* f = L.fn[a];
+        // director of node a: q_a x fn_a
+        const T da[3] = {ua[4] * f[2] - ua[5] * f[1], ua[5] * f[0] - ua[3] * f[2],
+                         ua[3] * f[1] - ua[4] * f[0]};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          U[2 * k] += Nx * ua[k];
+          U[2 * k + 1] += Ny * ua[k];
+          D[k] += N * da[k];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 6; k++) Gs.PU[ty][k] = double(U[k]);
+#pragma unroll
+      for (int k = 0; k < 3; k++) Gs.PD[ty][k] = double(D[k]);
+    } else if (active && lane >= 13 && lane < 17) {
+      const int q = lane - 13;
+      const T x = T(sh_gauss(q & 1)), y = T(sh_gauss(q >> 1));
+      T ux[6] = {T(0), T(0), T(0), T(0), T(0), T(0)}, d0[3] = {T(0), T(0), T(0)},
+        dx[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        T N, Nx, Ny;
+        sh_shape<T>(a, x, y, N, Nx, Ny);
+        const T* ua = &L.u[6 * a];
+        const T* f = L.fn[a];
+        const T da[3] = {ua[4] * f[2] - ua[5] * f[1], ua[5] * f[0] - ua[3] * f[2],
+                         ua[3] * f[1] - ua[4] * f[0]};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          ux[2 * k] += Nx * ua[k];
+          ux[2 * k + 1] += Ny * ua[k];
+          d0[k] += N * da[k];
+          dx[2 * k] += Nx * da[k];
+          dx[2 * k + 1] += Ny * da[k];
+        }
+      }
+      // u0d = [u0,xi | d0], d0d = [d0,xi | 0] (rows: components)
+      T u0d[9], d0d[9];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        u0d[3 * k] = ux[2 * k];
+        u0d[3 * k + 1] = ux[2 * k + 1];
+        u0d[3 * k + 2] = d0[k];
+        d0d[3 * k] = dx[2 * k];
+        d0d[3 * k + 1] = dx[2 * k + 1];
+        d0d[3 * k + 2] = T(0);
+      }
+      const T* Tm = L.qT[q];
+      T a1[9], a2[9], u0x[9], u1x[9];
+      sh_mm<T>(u0d, L.qXT[q], a1);  // u0x = T^T u0d XT
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          u0x[3 * i + j] = Tm[i] * a1[j] + Tm[3 + i] * a1[3 + j] + Tm[6 + i] * a1[6 + j];
+      sh_mm<T>(d0d, L.qXT[q], a1);  // u1x = T^T (d0d XT + u0d XzT)
+      sh_mm<T>(u0d, L.qXzT[q], a2);
+#pragma unroll
+      for (int k = 0; k < 9; k++) a1[k] += a2[k];
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          u1x[3 * i + j] = Tm[i] * a1[j] + Tm[3 + i] * a1[3 + j] + Tm[6 + i] * a1[6 + j];
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        Gs.PU0[q][k] = double(u0x[k]);
+        Gs.PU1[q][k] = double(u1x[k]);
+      }
+    }
+    // (1b) unit columns: p, w0, w1 at every point (u0x = p w0^T, u1x = p w1^T)
+    if (active && lane < SH_NV) {
+      const int a = lane / 6, k = lane % 6;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const T x = T(sh_gauss(q & 1)), y = T(sh_gauss(q >> 1));
+        T N, Nx, Ny;
+        sh_shape<T>(a, x, y, N, Nx, Ny);
+        const T* Tm = L.qT[q];
+        const T* XT = L.qXT[q];
+        const T* XzT = L.qXzT[q];
+        T pv[3], w0[3], w1[3];
+        if (k < 3) {
+#pragma unroll
+          for (int j = 0; j < 3; j++) {
+            pv[j] = Tm[3 * k + j];
+            w0[j] = Nx * XT[j] + Ny * XT[3 + j];
+            w1[j] = Nx * XzT[j] + Ny * XzT[3 + j];
+          }
+        } else {
+          const int r = k - 3;
+          const T* f = L.fn[a];
+          T da[3];
+          da[0] = (r == 0) ? T(0) : ((r == 1) ? f[2] : -f[1]);
+          da[1] = (r == 0) ? -f[2] : ((r == 1) ? T(0) : f[0]);
+          da[2] = (r == 0) ? f[1] : ((r == 1) ? -f[0] : T(0));
+#pragma unroll
+          for (int j = 0; j < 3; j++) {
+            pv[j] = Tm[j] * da[0] + Tm[3 + j] * da[1] + Tm[6 + j] * da[2];
+            w0[j] = N * XT[6 + j];
+            w1[j] = Nx * XT[j] + Ny * XT[3 + j] + N * XzT[6 + j];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          Gs.PW[q][lane][j] = double(pv[j]);
+          Gs.PW[q][lane][3 + j] = double(w0[j]);
+          Gs.PW[q][lane][6 + j] = double(w1[j]);
+        }
+      }
+    }
+    __syncthreads();
+    // (2) tying weights W_t of s_p (lane 0)
+    if (active && lane == 0) {
+      double W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int q = 0; q < 4; q++) {
+        const double* sq = Gs.sp[q];
+        const double S[9] = {sq[0], sq[2], sq[7], sq[2], sq[1], sq[6], sq[7], sq[6], 0.0};
+        double XT[9], A1[9], Mq[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) XT[k] = double(L.qXT[q][k]);
+        // M = XT S XT^T
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) A1[3 * i + j] = XT[3 * i] * S[j] + XT[3 * i + 1] * S[3 + j] + XT[3 * i + 2] * S[6 + j];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            Mq[3 * i + j] = A1[3 * i] * XT[3 * j] + A1[3 * i + 1] * XT[3 * j + 1] + A1[3 * i + 2] * XT[3 * j + 2];
+        const double x = sh_gauss(q & 1), y = sh_gauss(q >> 1);
+        const double na0 = 0.5 * (1.0 - x), na1 = 0.5 * (1.0 + x);
+        const double nb0 = 0.5 * (1.0 - y), nb1 = 0.5 * (1.0 + y);
+        const double dw = double(L.detw[q]);
+        W[0] += dw * Mq[0] * nb0;
+        W[1] += dw * Mq[0] * nb1;
+        W[2] += dw * Mq[4] * na0;
+        W[3] += dw * Mq[4] * na1;
+        W[4] += dw * 2.0 * Mq[1];
+        W[5] += dw * 2.0 * Mq[5] * na0;
+        W[6] += dw * 2.0 * Mq[5] * na1;
+        W[7] += dw * 2.0 * Mq[2] * nb0;
+        W[8] += dw * 2.0 * Mq[2] * nb1;
+      }
+#pragma unroll
+      for (int k = 0; k < 9; k++) Gs.W[k] = W[k];
+    }
+    // (3) B1[:, c] = 2 Q(p, e_c) (lanes 0-23)
+    if (active && lane < SH_NV) {
+      const int c = lane, a = c / 6, k = c % 6;
+      double gq[9];  // 2 g_t(p, e_c)
+#pragma unroll
+      for (int ty = 0; ty < 9; ty++) {
+        double xd, yd;
+        sh_tying_pt(ty, xd, yd);
+        double N, Nx, Ny;
+        sh_shape<double>(a, xd, yd, N, Nx, Ny);
+        double Uc[6] = {0, 0, 0, 0, 0, 0}, Dc[3] = {0, 0, 0};
+        if (k < 3) {
+          Uc[2 * k] = Nx;
+          Uc[2 * k + 1] = Ny;
+        } else {
+          const int r = k - 3;
+          const double f0 = double(L.fn[a][0]), f1 = double(L.fn[a][1]), f2 = double(L.fn[a][2]);
+          Dc[0] = N * ((r == 0) ? 0.0 : ((r == 1) ? f2 : -f1));
+          Dc[1] = N * ((r == 0) ? -f2 : ((r == 1) ? 0.0 : f0));
+          Dc[2] = N * ((r == 0) ? f1 : ((r == 1) ? -f0 : 0.0));
+        }
+        const double* Up = Gs.PU[ty];
+        const double* Dp = Gs.PD[ty];
+        double g;
+        if (ty < 2) g = 0.5 * (Up[0] * Uc[0] + Up[2] * Uc[2] + Up[4] * Uc[4]);
+        else if (ty < 4) g = 0.5 * (Up[1] * Uc[1] + Up[3] * Uc[3] + Up[5] * Uc[5]);
+        else if (ty < 5)
+          g = 0.25 * (Up[0] * Uc[1] + Up[2] * Uc[3] + Up[4] * Uc[5] + Uc[0] * Up[1] + Uc[2] * Up[3] +
+                      Uc[4] * Up[5]);
+        else if (ty < 7)
+          g = 0.25 * (Dp[0] * Uc[1] + Dp[1] * Uc[3] + Dp[2] * Uc[5] + Dc[0] * Up[1] + Dc[1] * Up[3] +
+                      Dc[2] * Up[5]);
+        else
+          g = 0.25 * (Dp[0] * Uc[0] + Dp[1] * Uc[2] + Dp[2] * Uc[4] + Dc[0] * Up[0] + Dc[1] * Up[2] +
+                      Dc[2] * Up[4]);
+        gq[ty] = 2.0 * g;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const double x = sh_gauss(q & 1), y = sh_gauss(q >> 1);
+        const double na0 = 0.5 * (1.0 - x), na1 = 0.5 * (1.0 + x);
+        const double nb0 = 0.5 * (1.0 - y), nb1 = 0.5 * (1.0 + y);
+        const double g11 = nb0 * gq[0] + nb1 * gq[1], g22 = na0 * gq[2] + na1 * gq[3];
+        const double g12 = gq[4], g23 = na0 * gq[5] + na1 * gq[6], g13 = nb0 * gq[7] + nb1 * gq[8];
+        const double Gm[9] = {g11, g12, g13, g12, g22, g23, g13, g23, 0.0};
+        double XT[9], Wm[9], E0[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) XT[i] = double(L.qXT[q][i]);
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            Wm[3 * i + j] = Gm[3 * i] * XT[j] + Gm[3 * i + 1] * XT[3 + j] + Gm[3 * i + 2] * XT[6 + j];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++)
+            E0[3 * i + j] = XT[i] * Wm[j] + XT[3 + i] * Wm[3 + j] + XT[6 + i] * Wm[6 + j];
+        // bending: u0x_c = pc w0c^T, u1x_c = pc w1c^T; path u0x, u1x full
+        const double* pw = Gs.PW[q][c];
+        const double* P0 = Gs.PU0[q];
+        const double* P1 = Gs.PU1[q];
+        double u0c[9], u1c[9];
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) {
+            u0c[3 * i + j] = pw[i] * pw[3 + j];
+            u1c[3 * i + j] = pw[i] * pw[6 + j];
+          }
+        double b3 = 0.0, b4 = 0.0, b5 = 0.0;
+        for (int i = 0; i < 3; i++) {
+          b3 += P0[3 * i] * u1c[3 * i] + u0c[3 * i] * P1[3 * i];
+          b4 += P0[3 * i + 1] * u1c[3 * i + 1] + u0c[3 * i + 1] * P1[3 * i + 1];
+          b5 += P0[3 * i] * u1c[3 * i + 1] + u0c[3 * i] * P1[3 * i + 1] + P1[3 * i] * u0c[3 * i + 1] +
+                u1c[3 * i] * P0[3 * i + 1];
+        }
+        // 2 Q = the symmetric sums above (each the sum of both orders)
+        double* b1 = &Gs.B1[q][0][0];
+        b1[0 * 24 + c] = E0[0];
+        b1[1 * 24 + c] = E0[4];
+        b1[2 * 24 + c] = 2.0 * E0[1];
+        b1[3 * 24 + c] = b3;
+        b1[4 * 24 + c] = b4;
+        b1[5 * 24 + c] = b5;
+        b1[6 * 24 + c] = 2.0 * E0[5];
+        b1[7 * 24 + c] = 2.0 * E0[2];
+        b1[8 * 24 + c] = 0.0;
+      }
+    }
+    __syncthreads();
+    // (4) column c of G (lanes 0-23)
+    if (active && lane < SH_NV) {
+      const int c = lane, ac = c / 6, kc = c % 6;
+      double CB0[4][9], CB1[4][9];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        T b[9], sb[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) b[k] = L.B[q][k][c];
+        sh_stress<T>(M, t, b, sb);
+#pragma unroll
+        for (int k = 0; k < 9; k++) CB0[q][k] = double(sb[k]) * double(L.detw[q]);
+#pragma unroll
+        for (int k = 0; k < 9; k++) b[k] = T(Gs.B1[q][k][c]);
+        sh_stress<T>(M, t, b, sb);
+#pragma unroll
+        for (int k = 0; k < 9; k++) CB1[q][k] = double(sb[k]) * double(L.detw[q]);
+      }
+      double fc[3] = {double(L.fn[ac][0]), double(L.fn[ac][1]), double(L.fn[ac][2])};
+      T* kout = &ke[(size_t)el * SH_NV * SH_NV + c];
+      for (int r = 0; r < SH_NV; r++) {
+        const int ar = r / 6, kr = r % 6;
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+          for (int k = 0; k < 9; k++)
+            acc += Gs.B1[q][k][r] * CB0[q][k] + double(L.B[q][k][r]) * CB1[q][k];
+        // tying part of s_p . 2 Q(e_r, e_c)
+        double h = 0.0;
+        const bool tr = kr < 3, tc = kc < 3;
+        if ((tr && tc && kr == kc) || (tr != tc)) {
+#pragma unroll
+          for (int ty = 0; ty < 9; ty++) {
+            double xd, yd;
+            sh_tying_pt(ty, xd, yd);
+            double Nr, Nxr, Nyr, Nc, Nxc, Nyc;
+            sh_shape<double>(ar, xd, yd, Nr, Nxr, Nyr);
+            sh_shape<double>(ac, xd, yd, Nc, Nxc, Nyc);
+            double g = 0.0;
+            if (tr && tc) {
+              if (ty < 2) g = 0.5 * Nxr * Nxc;
+              else if (ty < 4) g = 0.5 * Nyr * Nyc;
+              else if (ty < 5) g = 0.25 * (Nxr * Nyc + Nxc * Nyr);
+            } else if (ty >= 5) {
+              // one translation (node at, component kt) and one rotation
+              // (node ad, axis rd): 0.25 N_ad (e_rd x fn_ad)_kt dN_at
+              const int at = tr ? ar : ac, kt = tr ? kr : kc;
+              const int ad = tr ? ac : ar, rd = (tr ? kc : kr) - 3;
+              const double Nd = tr ? Nc : Nr;
+              const double Ntx = tr ? Nxr : Nxc, Nty = tr ? Nyr : Nyc;
+              const double f0 = double(L.fn[ad][0]), f1 = double(L.fn[ad][1]), f2 = double(L.fn[ad][2]);
+              const double dk = (kt == 0) ? ((rd == 0) ? 0.0 : ((rd == 1) ? f2 : -f1))
+                                          : ((kt == 1) ? ((rd == 0) ? -f2 : ((rd == 1) ? 0.0 : f0))
+                                                       : ((rd == 0) ? f1 : ((rd == 1) ? -f0 : 0.0)));
+              (void)at;
+              g = 0.25 * Nd * dk * (ty < 7 ? Nty : Ntx);
+            }
+            h += Gs.W[ty] * 2.0 * g;
+          }
+        }
+        // bending part: 2 Q(e_r, e_c) at the points, weighted by detw s_p
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const double* pr = Gs.PW[q][r];
+          const double* pc = Gs.PW[q][c];
+          const double pp = pr[0] * pc[0] + pr[1] * pc[1] + pr[2] * pc[2];
+          if (pp != 0.0) {
+            const double q3 = pp * (pr[3] * pc[6] + pc[3] * pr[6]);
+            const double q4 = pp * (pr[4] * pc[7] + pc[4] * pr[7]);
+            const double q5 = pp * (pr[3] * pc[7] + pc[3] * pr[7] + pr[6] * pc[4] + pc[6] * pr[4]);
+            const double* sq = Gs.sp[q];
+            h += double(L.detw[q]) * (sq[3] * q3 + sq[4] * q4 + sq[5] * q5);
+          }
+        }
+        kout[(size_t)r * SH_NV] += T(P.scale * (acc + h));
+      }
+    }
+    return;
+  }
 
   if (MODE == SH_MATRIX) {
     if (active && lane < SH_NV) {
@@ -1063,8 +1429,9 @@ int tacs_hip_shell_pressure_xpt(int naux, const int* aux_elem, const double* pre
 
 int tacs_hip_shell_mat_type(int prec, int nelem, const int* conn, const double* X,
                             const double* u, const double* telem, const ShellMaterial* mat,
-                            double kscale, double mscale, void* ke, double* ev,
+                            double kscale, double mscale, double gscale, void* ke, double* ev,
                             hipStream_t stream) {
+  if (gscale != 0.0 && (prec != 64 || !u)) return TACS_HIP_ERR_ARG;
   if (nelem <= 0) return 0;
   if (!mat || (prec != 64 && prec != 32) || (prec == 32 && ev)) return TACS_HIP_ERR_ARG;
   ShellParams P = shell_params(mat, kscale, 0.0, 0.0, 1.0, 1.0);
@@ -1076,6 +1443,12 @@ int tacs_hip_shell_mat_type(int prec, int nelem, const int* conn, const double* 
     SH_LAUNCH(float, SH_MATRIX);
   else
     SH_LAUNCH(double, SH_MATRIX);
+  if (gscale != 0.0) {
+    // + gscale G: the geometric stiffness along the state u, added to ke
+    P.scale = gscale;
+    ev = NULL;
+    SH_LAUNCH(double, SH_GEO);
+  }
   return (int)hipGetLastError();
 }
 

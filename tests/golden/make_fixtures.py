@@ -135,14 +135,20 @@ def make(seed=2024):
                 L.ref_solid_mat_type(order, k + 1, _d(X), _d(u), c(70e9), c(0.3), c(2700.0), c(t),
                                      _d(mt[k]))
             out[key + "_mattype"] = mt
-    # Quad4 shell element matrices by type (stiffness, mass)
+    # Quad4 shell element matrices by type (stiffness, mass, geometric
+    # stiffness along the state v)
     for s in range(4):
         key = f"quad4_{s}"
         X, v, t = out[key + "_X"], out[key + "_v"], float(out[key + "_t"][0])
-        mt = np.zeros((2, 576))
-        for k in range(2):
+        mt = np.zeros((3, 576))
+        for k in range(3):
             L.ref_quad4_mat_type(k + 1, _d(X), _d(v), c(70e9), c(0.3), c(t), _d(mt[k]))
         out[key + "_mattype"] = mt
+        # TACSQuad4NonlinearShell at the state 30 v (residual, Jacobian)
+        vn = 30.0 * v
+        res, mat = np.zeros(24), np.zeros(576)
+        L.ref_quad4_nonlinear_shell(_d(X), _d(vn), c(70e9), c(0.3), c(t), _d(res), _d(mat))
+        out[key + "_nl_res"], out[key + "_nl_mat"] = res, mat
     # Quad4 shells with TACSShellRefAxisTransform (residual, Jacobian, failure)
     axis = np.array([0.3, 1.0, 0.2])
     out["quad4_axis"] = axis

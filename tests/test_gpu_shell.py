@@ -99,8 +99,9 @@ def _blockwise_close(A, oA, what):
 
 @pytest.mark.parametrize("bc", ["ss", "clamped"])
 def test_shell_mass_and_mat_combo_match_oracle(bc):
-    """assembleMatType(TACS_MASS_MATRIX), assembleMatCombo(stiffness, mass)
-    and assembleJacobian's gamma term against the oracle's element-by-element
+    """assembleMatType(TACS_MASS_MATRIX), assembleMatType(TACS_GEOMETRIC_
+    STIFFNESS_MATRIX) at a random state, assembleMatCombo and
+    assembleJacobian's gamma term against the oracle's element-by-element
     assembly (TACSShellElement::getMatType / addJacobian), Dirichlet rows
     zeroed with a unit diagonal."""
     p, t = _setup(4, warp=0.03, seed=5, bc=bc)
@@ -124,6 +125,20 @@ def test_shell_mass_and_mat_combo_match_oracle(bc):
     _blockwise_close(A, oracle(0.0, 1.0), "mass")
     a.assembleMatCombo(TACS.STIFFNESS_MATRIX, 1.5, TACS.MASS_MATRIX, 2e9, p.mat)
     _blockwise_close(p.mat.getArrays()[2], oracle(1.5, 2e9), "combo")
+    # geometric stiffness along the state u, alone and with the stiffness
+    def oracle_geo(kc, gc):
+        oA = O.assemble_shell_mat(m.conn, X, t, E, NU, 2700.0, kc, 0.0, rowp, cols, gc=gc, u=u)
+        O.apply_bc_mat(6, p.bc_nodes, _mask(p), rowp, cols, oA)
+        return oA
+
+    a.assembleMatType(TACS.GEOMETRIC_STIFFNESS_MATRIX, p.mat)
+    Ag = p.mat.getArrays()[2].reshape(-1)
+    oG = oracle_geo(0.0, 1.0)
+    off = np.abs(oG) != 1.0
+    assert np.abs(Ag[off] - oG[off]).max() <= RTOL * np.abs(oG[off]).max()
+    assert np.array_equal(Ag[~off], oG[~off])
+    a.assembleMatCombo(TACS.STIFFNESS_MATRIX, 1.5, TACS.GEOMETRIC_STIFFNESS_MATRIX, -0.75, p.mat)
+    _blockwise_close(p.mat.getArrays()[2], oracle_geo(1.5, -0.75), "stiffness + geometric")
     # alpha K + gamma M with the residual K u of the static state
     a.assembleJacobian(0.7, 0.0, 3e8, p.res, p.mat)
     _blockwise_close(p.mat.getArrays()[2], oracle(0.7, 3e8), "jacobian")

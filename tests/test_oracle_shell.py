@@ -1,6 +1,7 @@
 """Pins of the oracle's Quad4 shell restatement (oracle/shell_oracle.c).
 
-- element residual, Jacobian, stiffness and consistent mass matrices,
+- element residual, Jacobian, stiffness, consistent mass and geometric
+  stiffness matrices, the nonlinear model's residual and Jacobian,
   thickness adjoint product, failure index, detXd and pressure load against
   values the reference's own element code wrote (tests/golden/elements.npz,
   made by tests/golden/make_fixtures.py);
@@ -65,6 +66,31 @@ def test_quad4_mat_types_against_reference_element(fixtures, case):
     assert np.abs(M.reshape(-1) - ref[1]).max() <= 1e-14 * np.abs(ref[1]).max()
     # alpha scales the whole matrix
     assert np.array_equal(O.quad4_mass(X, 2700.0, t, 2.0), 2.0 * M)
+    # geometric stiffness along the state v (TACS_GEOMETRIC_STIFFNESS_MATRIX)
+    G = O.quad4_geo(X, fixtures[k + "_v"], E, NU, t)
+    assert np.abs(G.reshape(-1) - ref[2]).max() <= 1e-12 * np.abs(ref[2]).max()
+    # the nonlinear model (TACSQuad4NonlinearShell) at the state 30 v
+    r, K = O.quad4_nl_element(X, 30.0 * fixtures[k + "_v"], E, NU, t)
+    Rr, Kr = fixtures[k + "_nl_res"], fixtures[k + "_nl_mat"]
+    assert np.abs(r - Rr).max() <= 1e-13 * np.abs(Rr).max()
+    assert np.abs(K.reshape(-1) - Kr).max() <= 1e-13 * np.abs(Kr).max()
+
+
+def test_quad4_geometric_stiffness_properties():
+    """G is symmetric, linear in the path, zero for a zero path, and the
+    tangent of the nonlinear model's stiffness: (K_T(h p) - K_T(-h p)) / 2h."""
+    rng = np.random.default_rng(21)
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], float) * [0.3, 0.2, 1.0]
+    X = (X + 0.02 * rng.standard_normal((4, 3))).reshape(-1)
+    p = 1e-3 * rng.standard_normal(24)
+    t = 0.01
+    G = O.quad4_geo(X, p, E, NU, t)
+    assert np.abs(G - G.T).max() <= 1e-12 * np.abs(G).max()
+    assert np.abs(O.quad4_geo(X, 2.5 * p, E, NU, t) - 2.5 * G).max() <= 1e-12 * np.abs(G).max()
+    assert np.abs(O.quad4_geo(X, np.zeros(24), E, NU, t)).max() == 0.0
+    h = 1e-2
+    fd = (O.quad4_nl_element(X, h * p, E, NU, t)[1] - O.quad4_nl_element(X, -h * p, E, NU, t)[1])
+    assert np.abs(fd / (2 * h) - G).max() <= 1e-9 * np.abs(G).max()
 
 
 def test_quad4_mass_closed_form():
