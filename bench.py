@@ -1310,14 +1310,15 @@ def main():
     ap.add_argument("--nrhs", type=int, default=16, help="load cases of the batched workload")
     ap.add_argument("--cheb-degree", type=int, default=None,
                     help="Chebyshev preconditioner degree of the recycling solvers (1: plain "
-                         "block-Jacobi; default 10 for solids, 14 for the batched workload, "
-                         "24 for shells)")
+                         "block-Jacobi; default 14 for the hex and batched workloads, 10 for "
+                         "the other solid workloads, 24 for shells)")
     ap.add_argument("--cheb-values", type=int, default=16, choices=[32, 16],
                     help="precision of the matrix values the Chebyshev polynomial's products read "
                          "(16: bfloat16 copy)")
     ap.add_argument("--cheb-ratio", type=float, default=None,
-                    help="interval [lmax / ratio, lmax] of the Chebyshev polynomial (default 300 "
-                         "for solids, 3000 for shells)")
+                    help="interval [lmax / ratio, lmax] of the Chebyshev polynomial (default "
+                         "1000 for the hex workload, 300 for the other solid workloads, 3000 for "
+                         "shells)")
     ap.add_argument("--multi-solver", default="segmented", choices=["segmented", "union"],
                     help="multi workload: every cube its own system (segmented) or one union "
                          "system")
@@ -1339,12 +1340,16 @@ def main():
         args.deflation = 30 if shell else 20
     MONITOR["file"] = args.monitor
     if args.cheb_degree is None:
-        # measured optima: single solid systems 10, the batched block of load
-        # cases 14 (its polynomial steps are cheaper per Arnoldi step than the
-        # DCGS2 sweeps of 16 right-hand sides), shells 24
-        args.cheb_degree = 24 if shell else (14 if args.workload == "batched" else 10)
+        # measured optima: the 128^3 hex8 cube 14 on [lmax/1000, lmax] (solve +
+        # adjoint 0.85 s against 0.90 s with 10 on [lmax/300, lmax]), the other
+        # solid systems 10, the batched block of load cases 14 (its polynomial
+        # steps are cheaper per Arnoldi step than the DCGS2 sweeps of 16
+        # right-hand sides), shells 24
+        args.cheb_degree = (24 if shell else
+                            14 if args.workload in ("batched", "hex") else 10)
     if args.cheb_ratio is None:
-        args.cheb_ratio = 3000.0 if shell else 300.0
+        args.cheb_ratio = (3000.0 if shell else
+                           1000.0 if args.workload == "hex" else 300.0)
     CHEB.update(degree=args.cheb_degree, ratio=args.cheb_ratio, values=args.cheb_values)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
