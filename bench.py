@@ -138,6 +138,50 @@ def time_cheb(L, ev, p, pc, reps=50, values=16):
     return ms, nbytes
 
 
+def time_fill(L, ev, p, reps=5):
+    """Average duration of the sliced ELL fill of the matrix writing all
+    three copies (FP64, FP32, bfloat16) and the columns in one launch (HIP
+    events on the library's stream), and its algorithmic bytes: the BCSR
+    values and columns read once, two row pointers per row, every slot of
+    the three copies and the columns written once."""
+    b, nr, nnz = p.mat.getInfo()
+    rpp, cpp, app = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    assert L.tacs_mat_get_arrays(p.mat.ptr, ctypes.byref(rpp), ctypes.byref(cpp),
+                                 ctypes.byref(app)) == 0
+    rowp = np.ctypeslib.as_array(ctypes.cast(rpp, ctypes.POINTER(ctypes.c_int)),
+                                 shape=(nr + 1,)).copy()
+    ns = (nr + 63) // 64
+    width = np.zeros(ns, np.int32)
+    coff = np.zeros(ns + 1, np.int64)
+    slots = L.tacs_hip_sell_slots(nr, rowp.ctypes.data, width.ctypes.data, coff.ctypes.data)
+    b2 = b * b
+    sizes = {"w": 4 * ns, "o": 8 * (ns + 1), "cs": 4 * slots, "v64": 8 * b2 * slots,
+             "v32": 4 * b2 * slots, "v16": 2 * b2 * slots}
+    d = {}
+    for k, n in sizes.items():
+        d[k] = ctypes.c_void_p()
+        assert ev.hip.hipMalloc(ctypes.byref(d[k]), ctypes.c_size_t(n)) == 0
+    ev.hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert ev.hip.hipMemcpy(d["w"], width.ctypes.data, sizes["w"], 1) == 0
+    assert ev.hip.hipMemcpy(d["o"], coff.ctypes.data, sizes["o"], 1) == 0
+    rp, cp, ap = p.mat.getDeviceArrays()
+    fill = lambda: L.tacs_hip_sell_fill3(b, nr, rp, cp, d["w"], d["o"], ap, d["v64"], d["v32"],
+                                         d["v16"], d["cs"], None)
+    for _ in range(2):
+        assert fill() == 0
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0)
+    for _ in range(reps):
+        fill()
+    ev.record(e1)
+    ms = ev.elapsed_ms(e0, e1) / reps
+    for k in d:
+        ev.hip.hipFree(d[k])
+    nbytes = (8 * b2 + 4) * nnz + 8 * nr + (14 * b2 + 4) * slots
+    return {"fill_ms": ms, "fill_bytes": nbytes, "fill_GBps": nbytes / ms / 1e6,
+            "fill_slots": slots}
+
+
 MONITOR = {"file": None, "freq": 1000}
 
 
@@ -1398,6 +1442,9 @@ def main():
         L.tacs_device_synchronize()
         ev.hip.hipFree(a32)
         sp["copy_kernel_bytes"] = {"read": 8 * nv, "written": 4 * nv}
+        # the sliced ELL fill as a reassembled step runs it: the FP64, FP32
+        # and bfloat16 copies and the columns in one pass
+        sp.update(time_fill(L, ev, p))
         e0, e1 = ev.event(), ev.event()
         ev.record(e0)
         for _ in range(3):

@@ -380,6 +380,16 @@ int tacs_hip_sell_fill2(int b, int nrows, const int* rowp, const int* cols, cons
 int tacs_hip_sell_fill3(int b, int nrows, const int* rowp, const int* cols, const int* width,
                         const long* coff, const double* A, double* v64, float* v32,
                         unsigned short* v16, int* cs, hipStream_t stream);
+/* Launch variants of sell_fill3, all writing the same arrays: 3x3 blocks
+   0 (one lane per row) and 1-3 (one wave per slice staging 4, 7 or 7
+   slots of all 64 rows in LDS: contiguous reads of the BCSR runs, 64-entry
+   stores; 16, 32 or 32 rows of loads in flight; non-temporal value loads
+   in 1 and 2, non-temporal stores in 2 and 3); 6x6 blocks variant 0 only.
+   sell_fill3 launches variant 3 for 3x3 blocks.  No counterpart in the reference. */
+int tacs_hip_sell_fill_nvariants(int b);
+int tacs_hip_sell_fill_variant(int variant, int b, int nrows, const int* rowp, const int* cols,
+                               const int* width, const long* coff, const double* A, double* v64,
+                               float* v32, unsigned short* v16, int* cs, hipStream_t stream);
 int tacs_hip_sell_spmv(int b, int prec_vals, int prec_vecs, int nrows, const int* width,
                        const long* coff, const int* cs, const void* vs, const void* x, void* y,
                        hipStream_t stream);

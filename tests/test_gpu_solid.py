@@ -88,6 +88,59 @@ def _ragged_pattern(nrows, seed, b=3):
     return rowp, cols, A
 
 
+@pytest.mark.parametrize("nrows", [1, 63, 65, 1000, 4161])
+def test_sliced_ell_fill_variants_identical(nrows):
+    """Every launch variant of the sliced ELL fill (one lane per row; one wave
+    per slice staging 4 or 7 slots in LDS) writes bit-identical FP64,
+    FP32 and bfloat16 copies and columns on a ragged 3x3 pattern, and leaves
+    the entries beyond the last slot untouched."""
+    import ctypes
+
+    import torch
+
+    from tacs_amd import lib
+
+    b, b2 = 3, 9
+    rowp, cols, A = _ragged_pattern(nrows, nrows + b, b)
+    ns = (nrows + 63) // 64
+    width = np.zeros(ns, np.int32)
+    coff = np.zeros(ns + 1, np.int64)
+    slots = lib().tacs_hip_sell_slots(nrows, rowp.ctypes.data, width.ctypes.data, coff.ctypes.data)
+    dev = lambda a: torch.from_numpy(a).to("cuda")
+    rp, cp, Ad, wd, od = dev(rowp), dev(cols), dev(A), dev(width), dev(coff)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())
+    L = lib()
+    nv = L.tacs_hip_sell_fill_nvariants(b)
+    assert nv == 4 and L.tacs_hip_sell_fill_nvariants(6) == 1
+    outs = []
+    for v in range(nv):
+        cs = torch.full((slots + 64,), -7, dtype=torch.int32, device="cuda")
+        v64 = torch.full((b2 * slots + 64,), np.nan, dtype=torch.float64, device="cuda")
+        v32 = torch.full((b2 * slots + 64,), np.nan, dtype=torch.float32, device="cuda")
+        v16 = torch.full((b2 * slots + 64,), 12345, dtype=torch.int16, device="cuda")
+        torch.cuda.synchronize()
+        assert L.tacs_hip_sell_fill_variant(v, b, nrows, P(rp), P(cp), P(wd), P(od), P(Ad),
+                                            P(v64), P(v32), P(v16), P(cs), None) == 0
+        torch.cuda.synchronize()
+        outs.append([t.cpu().numpy() for t in (v64, v32, v16, cs)])
+    for o in outs:
+        for a0, a1 in zip(outs[0], o):
+            assert np.array_equal(a0.view(np.uint8), a1.view(np.uint8))
+    v64h, v32h, v16h, csh = outs[0]
+    assert not np.isnan(v64h[:b2 * slots]).any() and csh[:slots].min() >= 0
+    assert np.isnan(v64h[b2 * slots:]).all() and (csh[slots:] == -7).all()
+    assert (v16h[b2 * slots:] == 12345).all()
+    # the copies agree with the FP64 values: FP32 rounding, bfloat16 of that
+    assert np.array_equal(v32h[:b2 * slots], v64h[:b2 * slots].astype(np.float32))
+    u = v32h[:b2 * slots].view(np.uint32).astype(np.uint64)
+    bf = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    assert np.array_equal(v16h[:b2 * slots].view(np.uint16), bf)
+    assert L.tacs_hip_sell_fill_variant(nv, b, nrows, P(rp), P(cp), P(wd), P(od), P(Ad), P(v64),
+                                        None, None, None, None) != 0
+    assert L.tacs_hip_sell_fill_variant(1, 6, nrows, P(rp), P(cp), P(wd), P(od), P(Ad), P(v64),
+                                        None, None, None, None) != 0
+
+
 @pytest.mark.parametrize("b,nrows", [(3, 1), (3, 63), (3, 1000), (3, 4161), (6, 1), (6, 1000),
                                      (6, 2113)])
 def test_sliced_ell_spmv_matches_oracle(b, nrows):
