@@ -116,25 +116,26 @@ def time_spmv(L, ev, p, reps=50):
 
 def time_cheb(L, ev, p, pc, reps=50, values=16):
     """Average duration of the fused Chebyshev product of the recycling
-    solver's preconditioner (values in bfloat16 or FP32, FP64 vectors; HIP
+    solver's preconditioner in the three-term form the polynomial's steps
+    after the second run (values in bfloat16 or FP32, FP64 vectors; HIP
     events on the library's stream) and its algorithmic bytes: the values
-    and column indices once, row pointers, z gathered once, rhs, d read and
-    written, zout written, the FP32 inverse diagonal blocks."""
+    and column indices once, row pointers, z gathered once, rhs and zprev
+    read, zout written (over zprev), the FP32 inverse diagonal blocks."""
     b, nr, nnz = p.mat.getInfo()
-    vecs = [p.assembler.createVec() for _ in range(4)]
+    vecs = [p.assembler.createVec() for _ in range(3)]
     for k, v in enumerate(vecs):
         v.set(1.0 + 0.1 * k)
-    z, zo, rhs, d = [v.getDevicePointer() for v in vecs]
+    z, zp, rhs = [v.getDevicePointer() for v in vecs]
     for _ in range(5):
-        assert L.tacs_mat_cheb_step(p.mat.ptr, pc.ptr, values, z, zo, rhs, d, 0.5, 0.1) == 0
+        assert L.tacs_mat_cheb_step3(p.mat.ptr, pc.ptr, values, z, zp, rhs, 0.5, 0.1) == 0
     e0, e1 = ev.event(), ev.event()
     ev.record(e0)
     for _ in range(reps):
-        L.tacs_mat_cheb_step(p.mat.ptr, pc.ptr, values, z, zo, rhs, d, 0.5, 0.1)
+        L.tacs_mat_cheb_step3(p.mat.ptr, pc.ptr, values, z, zp, rhs, 0.5, 0.1)
     ev.record(e1)
     ms = ev.elapsed_ms(e0, e1) / reps
     word = 2 if values == 16 else 4
-    nbytes = word * b * b * nnz + 4 * nnz + 4 * (nr + 1) + 8 * b * nr * 5 + 4 * b * b * nr
+    nbytes = word * b * b * nnz + 4 * nnz + 4 * (nr + 1) + 8 * b * nr * 4 + 4 * b * b * nr
     return ms, nbytes
 
 
@@ -1601,8 +1602,8 @@ def main():
             "avg_ms": cms, "algorithmic_bytes": cbytes,
             "traffic_source": "profiles/spmv_traffic_hex8_128cube.json (rocprofv3 FETCH_SIZE + "
                               "WRITE_SIZE, copy-kernel scaled)",
-            "note": "values and column indices once, row pointers, z gathered once, rhs, d read "
-                    "and written, zout written, FP32 inverse diagonal blocks"}
+            "note": "three-term form: values and column indices once, row pointers, z gathered "
+                    "once, rhs and zprev read, zout written, FP32 inverse diagonal blocks"}
     else:
         out["roofline"] = out["roofline_fp64_spmv"]
     if fp64 is not None:

@@ -326,6 +326,10 @@ class TACSParallelMat : public TACSMat {
   // zout = z + d
   void chebStep(int pr, const void* Dinv, const double* z, double* zout, const double* rhs,
                 double* d, double c1, double c2);
+  // the same step in three-term form: zprev <- (1 + c1) z - c1 zprev
+  // + c2 D^{-1} (rhs - A z)
+  void chebStep3(int pr, const void* Dinv, const double* z, double* zprev, const double* rhs,
+                 double c1, double c2);
   int bsize, nrows;
   long nnzb;
   long version;  // incremented whenever the values change

@@ -409,6 +409,14 @@ int tacs_hip_sell_cheb_step(int variant, int b, int prec_vals, int nrows, const 
                             const long* coff, const int* cs, const void* vs, const double* z,
                             double* zout, const double* rhs, const void* Dinv, double* d,
                             double c1, double c2, hipStream_t stream);
+/* The same step in three-term form: zprev holds the iterate before z on
+   entry and receives zout = (1 + c1) z - c1 zprev + c2 D^{-1} (rhs - A z)
+   (the two-term step with d = z - zprev; no d vector is read or written).
+   zprev must differ from z.  No counterpart in the reference. */
+int tacs_hip_sell_cheb_step3(int variant, int b, int prec_vals, int nrows, const int* width,
+                             const long* coff, const int* cs, const void* vs, const double* z,
+                             double* zprev, const double* rhs, const void* Dinv, double c1,
+                             double c2, hipStream_t stream);
 int tacs_hip_sell_spmv_variant(int variant, int b, int prec_vals, int prec_vecs, int nrows,
                                const int* width, const long* coff, const int* cs, const void* vs,
                                const void* x, void* y, hipStream_t stream);
@@ -869,6 +877,11 @@ int tacs_mat_autotune(tacs_obj mat, int reps, double* ms, int maxv, int* nvarian
    in the reference. */
 int tacs_mat_cheb_step(tacs_obj mat, tacs_obj pc, int prec_vals, const double* z, double* zout,
                        const double* rhs, double* d, double c1, double c2);
+/* tacs_hip_sell_cheb_step3 over the matrix's sliced ELL copy with pc's
+   inverse diagonal blocks: zprev <- (1 + c1) z - c1 zprev
+   + c2 D^{-1} (rhs - A z).  No counterpart in the reference. */
+int tacs_mat_cheb_step3(tacs_obj mat, tacs_obj pc, int prec_vals, const double* z, double* zprev,
+                        const double* rhs, double c1, double c2);
 int tacs_mat_get_sell_variant(tacs_obj mat, int kind, int* variant);
 int tacs_mat_set_sell_variant(tacs_obj mat, int kind, int variant);
 int tacs_mat_autotune_spmm(tacs_obj mat, int nrhs, int prec, int reps, double* ms, int maxv,

@@ -372,12 +372,17 @@ __device__ inline void sell_slot(const TA* __restrict__ vs, long v, const T* __r
 // the row, w = D^{-1} (rhs - a), d = c1 d + c2 w and zout = z + d for the
 // row (one step of the Chebyshev iteration for A z = rhs, Saad, Alg. 12.1),
 // instead of writing a.  zout must not be z (other rows still read z).
+// Three-term form (zprev not NULL; d unused): with d = z - zprev the same
+// step is zout = (1 + c1) z - c1 zprev + c2 w, which reads zprev instead of
+// reading and writing d; zout may be zprev (each row reads its own entries
+// before writing them).
 struct SellCheb {
   const void* Dinv;  // b x b inverse diagonal blocks, the value type of the matrix
   const double* rhs;
   double* d;
   double* zout;
   double c1, c2;
+  const double* zprev;
 };
 template <int B, int U, typename TA, typename T, bool NT = true, bool CHEB = false>
 __global__ __launch_bounds__(256) void sell_spmv(int nslices, int nrows,
@@ -438,9 +443,14 @@ __global__ __launch_bounds__(256) void sell_spmv(int nslices, int nrows,
 #pragma unroll
           for (int c = 1; c < B; c++) wv += (double)D[B * r + c] * rr[c];
           const size_t o = (size_t)B * row + r;
-          const double dn = ch.c1 * ch.d[o] + ch.c2 * wv;
-          ch.d[o] = dn;
-          ch.zout[o] = (double)x[o] + dn;
+          if (ch.zprev) {
+            const double zr = (double)x[o];
+            ch.zout[o] = (1.0 + ch.c1) * zr - ch.c1 * ch.zprev[o] + ch.c2 * wv;
+          } else {
+            const double dn = ch.c1 * ch.d[o] + ch.c2 * wv;
+            ch.d[o] = dn;
+            ch.zout[o] = (double)x[o] + dn;
+          }
         }
       } else {
 #pragma unroll
@@ -808,17 +818,37 @@ int tacs_hip_sell_spmv_variant(int variant, int b, int prec_vals, int prec_vecs,
   return (int)hipGetLastError();
 }
 
+static int sell_cheb_launch(int variant, int b, int prec_vals, int nrows, const int* width,
+                            const long* coff, const int* cs, const void* vs, const double* z,
+                            const SellCheb ch, hipStream_t stream);
+
 int tacs_hip_sell_cheb_step(int variant, int b, int prec_vals, int nrows, const int* width,
                             const long* coff, const int* cs, const void* vs, const double* z,
                             double* zout, const double* rhs, const void* Dinv, double* d,
                             double c1, double c2, hipStream_t stream) {
+  if (z == zout) return TACS_HIP_ERR_ARG;
+  return sell_cheb_launch(variant, b, prec_vals, nrows, width, coff, cs, vs, z,
+                          SellCheb{Dinv, rhs, d, zout, c1, c2, NULL}, stream);
+}
+
+int tacs_hip_sell_cheb_step3(int variant, int b, int prec_vals, int nrows, const int* width,
+                             const long* coff, const int* cs, const void* vs, const double* z,
+                             double* zprev, const double* rhs, const void* Dinv, double c1,
+                             double c2, hipStream_t stream) {
+  if (!zprev || z == zprev) return TACS_HIP_ERR_ARG;
+  return sell_cheb_launch(variant, b, prec_vals, nrows, width, coff, cs, vs, z,
+                          SellCheb{Dinv, rhs, NULL, zprev, c1, c2, zprev}, stream);
+}
+
+static int sell_cheb_launch(int variant, int b, int prec_vals, int nrows, const int* width,
+                            const long* coff, const int* cs, const void* vs, const double* z,
+                            const SellCheb ch, hipStream_t stream) {
   const int ns = (nrows + 63) / 64;
-  if ((b != 3 && b != 6) || (prec_vals != 64 && prec_vals != 32 && prec_vals != 16) || z == zout)
+  if ((b != 3 && b != 6) || (prec_vals != 64 && prec_vals != 32 && prec_vals != 16))
     return TACS_HIP_ERR_ARG;
   if (variant < 0 || variant >= tacs_hip_sell_spmv_nvariants(b)) return TACS_HIP_ERR_ARG;
   if (ns <= 0) return 0;
   const int grid = tacs_grid_for(ns, 4, 65536);
-  const SellCheb ch{Dinv, rhs, d, zout, c1, c2};
 #define SELL_CHEB(BB, U, NT)                                                                     \
   do {                                                                                           \
     if (prec_vals == 64)                                                                         \

@@ -578,6 +578,18 @@ int tacs_mat_cheb_step(tacs_obj m, tacs_obj pc, int prec_vals, const double* z, 
               zout, rhs, d, c1, c2);
   return last_error_code();
 }
+int tacs_mat_cheb_step3(tacs_obj m, tacs_obj pc, int prec_vals, const double* z, double* zprev,
+                        const double* rhs, double c1, double c2) {
+  REQUIRE(TACSParallelMat, A, m);
+  REQUIRE(TACSBlockJacobiPc, p, pc);
+  if ((prec_vals != 64 && prec_vals != 32 && prec_vals != 16) || !z || !zprev || z == zprev ||
+      p->mat != A)
+    return tacs_set_error("%s: prec 64, 32 or 16, zprev != z, the matrix's own preconditioner",
+                          __func__);
+  A->chebStep3(prec_vals, prec_vals == 64 ? (const void*)p->d_Dinv : (const void*)p->d_Dinv32, z,
+               zprev, rhs, c1, c2);
+  return last_error_code();
+}
 int tacs_mat_autotune(tacs_obj m, int reps, double* ms, int maxv, int* nvariants) {
   REQUIRE(TACSParallelMat, A, m);
   const int nv = A->autotune(reps, ms, maxv);

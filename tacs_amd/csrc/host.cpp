@@ -572,6 +572,18 @@ void TACSParallelMat::chebStep(int pr, const void* Dinv, const double* z, double
   KCHECK(tacs_hip_sell_cheb_step(sell_kind_variant(this, kind), bsize, pr, nrows, d_sw, d_soff,
                                  d_scols, vs, z, zout, rhs, Dinv, d, c1, c2, g_stream));
 }
+// The same step in three-term form: zprev <- (1 + c1) z - c1 zprev
+// + c2 D^{-1} (rhs - A z).
+void TACSParallelMat::chebStep3(int pr, const void* Dinv, const double* z, double* zprev,
+                                const double* rhs, double c1, double c2) {
+  const int kind = pr == 64 ? 0 : 1;
+  if (pr == 64 && sellVer64 != version) sellUpdate(64);
+  if (pr == 32 && sellVer32 != version) sellUpdate(32);
+  if (pr == 16 && sellVer16 != version) sellUpdate(16);
+  const void* vs = pr == 64 ? (const void*)d_s64 : pr == 32 ? (const void*)d_s32 : (const void*)d_s16;
+  KCHECK(tacs_hip_sell_cheb_step3(sell_kind_variant(this, kind), bsize, pr, nrows, d_sw, d_soff,
+                                  d_scols, vs, z, zprev, rhs, Dinv, c1, c2, g_stream));
+}
 // Average time of reps launches of fn on the library stream (one untimed
 // launch first), in ms; negative after a failure.
 template <typename F>
@@ -1798,10 +1810,14 @@ void TACSBatchedGMRES::precond(int pr, int Rn, const void* Dinv, const float* A3
     buf[deg % 2] = d_caz;
     KCHECK(tacs_hip_bcheb_step(mat->bsize, pr, mat->nrows, 1, Dinv, xc, NULL, d_cd, buf[0], 0.0,
                                1.0 / theta, 1, NULL, g_stream));
+    // from the third iterate on, the three-term form (the iterate two steps
+    // back is in the output buffer; no d vector moves)
     for (int k = 1; k < deg; k++) {
       const double rn = 1.0 / (2.0 * sigma - rho);
-      mat->chebStep(pin, Dinv, buf[(k - 1) % 2], buf[k % 2], xc, d_cd, rn * rho,
-                    2.0 * rn / delta);
+      if (k == 1)
+        mat->chebStep(pin, Dinv, buf[0], buf[1], xc, d_cd, rn * rho, 2.0 * rn / delta);
+      else
+        mat->chebStep3(pin, Dinv, buf[(k - 1) % 2], buf[k % 2], xc, rn * rho, 2.0 * rn / delta);
       rho = rn;
     }
   } else if (pr == 32 && pc->chebValues == 16 && mat->bsize == 3 && rc % 4 == 0 && rc >= 16) {

@@ -198,3 +198,59 @@ def test_fused_chebyshev_step_matches_numpy(kind, values):
     assert np.abs(z_got - z_exp).max() <= 1e-10 * np.abs(z_exp).max()
     # z and rhs unchanged
     assert np.array_equal(vecs[0].getArray(), z) and np.array_equal(vecs[2].getArray(), rhs)
+
+
+@pytest.mark.parametrize("kind", ["solid", "shell"])
+@pytest.mark.parametrize("values", [64, 32, 16])
+def test_fused_chebyshev_step3_matches_numpy(kind, values):
+    """The three-term fused step (tacs_mat_cheb_step3: zprev <- (1 + c1) z
+    - c1 zprev + c2 D^-1 (rhs - A z), the two-term step with d = z - zprev)
+    against numpy with the values and inverse blocks rounded as the kernel
+    holds them; 1e-10 normwise.  Two steps in a row also match two-term
+    steps from the same start."""
+    from tacs_amd._lib import lib
+    if kind == "solid":
+        p = SolidProblem(6, dv="element")
+        p.set_thickness(np.random.default_rng(5).uniform(0.5, 1.5, p.mesh.nelem))
+    else:
+        p = ShellProblem(9, warp=0.02, seed=4)
+    p.assemble()
+    a = p.assembler
+    K, rowp, cols, A = _dense(p)
+    b = A.shape[1]
+    n = rowp.size - 1
+    pc = TACS.Pc(p.mat)
+    pc.factor()
+    Dinv = np.zeros((n, b, b))
+    xv, yv = a.createVec(), a.createVec()
+    for c in range(b):
+        e = np.zeros((n, b))
+        e[:, c] = 1.0
+        xv.getArray()[:] = e.reshape(-1)
+        pc.applyFactor(xv, yv)
+        Dinv[:, :, c] = yv.getArray().reshape(n, b)
+    if values == 64:
+        Kr, Dr = K, Dinv
+    elif values == 32:
+        Kr = K.astype(np.float32).astype(np.float64)
+        Dr = Dinv.astype(np.float32).astype(np.float64)
+    else:
+        Kr = _bf16(K)
+        Dr = Dinv.astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(3 * values + b)
+    z = rng.standard_normal(b * n)
+    zp = rng.standard_normal(b * n)
+    rhs = Kr @ rng.standard_normal(b * n)
+    c1, c2 = 0.37, 1.9
+    vz, vp, vr = a.createVec(), a.createVec(), a.createVec()
+    for v, arr in zip((vz, vp, vr), (z, zp, rhs)):
+        v.getArray()[:] = arr
+    pz, pp, pr = vz.getDevicePointer(), vp.getDevicePointer(), vr.getDevicePointer()
+    assert lib().tacs_mat_cheb_step3(p.mat.ptr, pc.ptr, values, pz, pp, pr, c1, c2) == 0
+    w = np.einsum("nij,nj->ni", Dr, (rhs - Kr @ z).reshape(n, b)).reshape(-1)
+    z_exp = (1.0 + c1) * z - c1 * zp + c2 * w
+    z_got = vp.getArray().copy()
+    assert np.abs(z_got - z_exp).max() <= 1e-10 * np.abs(z_exp).max()
+    assert np.array_equal(vz.getArray(), z) and np.array_equal(vr.getArray(), rhs)
+    # z == zprev is refused
+    assert lib().tacs_mat_cheb_step3(p.mat.ptr, pc.ptr, values, pz, pz, pr, c1, c2) != 0
