@@ -615,6 +615,15 @@ int tacs_hip_bcheb_step(int b, int prec, int nrows, int R, const void* Dinv, con
 int tacs_hip_bsr3_spmm_cheb(int nrows, int R, const int* rowp, const int* cols, const float* A,
                             const float* X, const double* x, const float* Dinv, double* d,
                             double* z, float* Xout, double c1, double c2, hipStream_t stream);
+/* The same fused step in the forms a polynomial runs after its first steps
+   (both read z and d and write d, no counterpart in the reference):
+   mode 1, three-term: d holds the iterate before z and receives
+   (1 + c1) z - c1 d + c2 w; mode 2, into that form: d holds the last d and
+   receives z + c1 d + c2 w.  Mode 0 is tacs_hip_bsr3_spmm_cheb.  d != z. */
+int tacs_hip_bsr3_spmm_cheb_mode(int mode, int nrows, int R, const int* rowp, const int* cols,
+                                 const float* A, const float* X, const double* x,
+                                 const float* Dinv, double* d, double* z, float* Xout, double c1,
+                                 double c2, hipStream_t stream);
 
 /* Segmented systems (the R <= 64 diagonal blocks of a block-diagonal 3x3
    BCSR matrix, block j = node range [off[j], off[j+1]), solved as R

@@ -822,12 +822,16 @@ __global__ __launch_bounds__(256) void bsr3_spmm_pair(int nrows, int R,
 // z32 is a copy of z that the products of the polynomial read (half the
 // gather traffic of FP64 z); z32out must differ from z32in (other rows still
 // read it).  R = 4 NL, 16 <= R; Dinv in FP32 (the values' precision).
+// mode 1 (three-term form): d holds the iterate before z and receives
+// (1 + c1) z - c1 d + c2 w, z is only read; mode 2 (the step into that
+// form): d holds the last d and receives z + c1 d + c2 w.  Both read z and
+// d and write d: one FP64 access per entry fewer than mode 0.
 template <typename TA, int NL, int SPU = 1, int NTA = 1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void bsr3_spmm_quad_cheb(
     int nrows, int R, const int* __restrict__ rowp, const int* __restrict__ cols,
     const TA* __restrict__ A, const float* __restrict__ X, const double* __restrict__ xr,
     const float* __restrict__ Dinv, double* __restrict__ d, double* __restrict__ z,
-    float* __restrict__ Xout, double c1, double c2) {
+    float* __restrict__ Xout, double c1, double c2, int mode) {
   constexpr int G = 64 / NL;
   const int lane = threadIdx.x & 63;
   const int jq = lane % NL, g = lane / NL;
@@ -905,10 +909,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void b
       w += (double)D[1] * rc1;
       w += (double)D[2] * rc2;
       const size_t o = base + (size_t)r * R;
-      const double dn = c1 * d[o] + c2 * w;
-      d[o] = dn;
-      const double zn = z[o] + dn;
-      z[o] = zn;
+      double zn;
+      if (mode == 0) {
+        const double dn = c1 * d[o] + c2 * w;
+        d[o] = dn;
+        zn = z[o] + dn;
+        z[o] = zn;
+      } else {
+        const double zo = z[o], dp = d[o];
+        zn = mode == 1 ? (1.0 + c1) * zo - c1 * dp + c2 * w : zo + (c1 * dp + c2 * w);
+        d[o] = zn;
+      }
       Xout[o] = (float)zn;
     }
   }
@@ -1491,12 +1502,21 @@ int tacs_hip_bsr3_spmm_nvariants(int nrhs) { return spmm_nvariants(nrhs); }
 int tacs_hip_bsr3_spmm_cheb(int nrows, int R, const int* rowp, const int* cols, const float* A,
                             const float* X, const double* x, const float* Dinv, double* d,
                             double* z, float* Xout, double c1, double c2, hipStream_t stream) {
-  if (!spmm_quad_form(R) || R > 64 || X == Xout) return TACS_HIP_ERR_ARG;
+  return tacs_hip_bsr3_spmm_cheb_mode(0, nrows, R, rowp, cols, A, X, x, Dinv, d, z, Xout, c1, c2,
+                                      stream);
+}
+
+int tacs_hip_bsr3_spmm_cheb_mode(int mode, int nrows, int R, const int* rowp, const int* cols,
+                                 const float* A, const float* X, const double* x,
+                                 const float* Dinv, double* d, double* z, float* Xout, double c1,
+                                 double c2, hipStream_t stream) {
+  if (!spmm_quad_form(R) || R > 64 || X == Xout || mode < 0 || mode > 2 || d == z)
+    return TACS_HIP_ERR_ARG;
   if (nrows <= 0) return 0;
   const int grid = 8 * tacs_grid_for((nrows + 7) / 8, 4, 8192);
 #define SPMM_CHEB(NLV)                                                                            \
   hipLaunchKernelGGL((bsr3_spmm_quad_cheb<float, NLV, 1, 0>), dim3(grid), dim3(256), 0, stream,  \
-                     nrows, R, rowp, cols, A, X, x, Dinv, d, z, Xout, c1, c2)
+                     nrows, R, rowp, cols, A, X, x, Dinv, d, z, Xout, c1, c2, mode)
   switch (pow2_ceil(R / 4)) {
     case 4: SPMM_CHEB(4); break;
     case 8: SPMM_CHEB(8); break;

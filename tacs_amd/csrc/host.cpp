@@ -1833,11 +1833,19 @@ void TACSBatchedGMRES::precond(int pr, int Rn, const void* Dinv, const float* A3
     const int* cols = mat->d_cols;
     KCHECK(tacs_hip_bcheb_step(3, 32, mat->nrows, rc, Dinv, xc, NULL, d_cd, zc, 0.0, 1.0 / theta,
                                1, z32[0], g_stream));
+    // steps in place (z += d) until step t, which writes the next iterate
+    // over d; from then on the three-term form alternates the iterate
+    // between zc and d_cd (t of the degree's parity, so the last lands in zc)
+    const int t = deg % 2 == 0 ? 2 : 3;
+    double* cur = zc;
+    double* prev = d_cd;
     for (int k = 1; k < deg; k++) {
       const double rn = 1.0 / (2.0 * sigma - rho);
-      KCHECK(tacs_hip_bsr3_spmm_cheb(mat->nrows, rc, rowp, cols, A32, z32[(k - 1) % 2],
-                                     xc, (const float*)Dinv, d_cd, zc, z32[k % 2], rn * rho,
-                                     2.0 * rn / delta, g_stream));
+      const int mode = (deg < t + 2 || k < t) ? 0 : k == t ? 2 : 1;
+      KCHECK(tacs_hip_bsr3_spmm_cheb_mode(mode, mat->nrows, rc, rowp, cols, A32,
+                                          z32[(k - 1) % 2], xc, (const float*)Dinv, prev, cur,
+                                          z32[k % 2], rn * rho, 2.0 * rn / delta, g_stream));
+      if (mode != 0) std::swap(cur, prev);
       rho = rn;
     }
   } else {
