@@ -87,14 +87,18 @@ def test_recycling_solve_with_chebyshev_matches_dense(deg, ratio, values):
     assert its1 < k1.getIterCount()
 
 
-@pytest.mark.parametrize("R,values", [(4, 32), (16, 16), (16, 32), (32, 16)])
-def test_batched_solve_with_chebyshev_matches_dense(R, values):
+@pytest.mark.parametrize("R,values,deg", [(4, 32, 5), (16, 16, 5), (16, 32, 5), (32, 16, 5),
+                                          (16, 16, 3), (16, 16, 4), (16, 16, 6), (16, 16, 7)])
+def test_batched_solve_with_chebyshev_matches_dense(R, values, deg):
     """values 16 with R = 16, 32: the polynomial's products read an FP32 copy
-    of the iterate, each step fused into its product (bsr3_spmm_quad_cheb)."""
+    of the iterate, each step fused into its product (bsr3_spmm_quad_cheb);
+    degrees 3-7 take every order of its in-place, transition and three-term
+    steps (the last iterate must land in the output for even and odd
+    degrees)."""
     p = SolidProblem(3, order=3)
     p.assemble()
     pc = TACS.Pc(p.mat)
-    pc.setChebyshev(5, 50.0, values=values)
+    pc.setChebyshev(deg, 50.0, values=values)
     pc.factor()
     rng = np.random.default_rng(5)
     a = p.assembler
